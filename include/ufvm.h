@@ -1,0 +1,171 @@
+/*
+ * ufvm.h -- C ABI of the MI355X (gfx950) uFVM pressure-solve path.
+ *
+ * The reference (en9apr/Matlab_uFVM) is pure MATLAB with no FFI of its own;
+ * its "API" for this path is a set of MATLAB functions operating on
+ * `global Domain`.  Each entry point below replaces one of those functions
+ * (cited as uFVM/<file>:<line>); a MEX gateway or a ctypes host marshals the
+ * Domain pieces into these plain arrays.
+ *
+ * Conventions
+ *  - Every function returns 0 (UFVM_OK) or a UFVM_ERR_* status; on error the
+ *    message is available from ufvm_last_error(ctx).  Nothing throws across
+ *    the ABI.
+ *  - Indices are 0-based int32.  A level's coefficients use the reference's
+ *    LDU-in-cell-lists layout (uFVM/src/fvm/setup/cfdSetupCoefficients.m:7-39)
+ *    flattened to CSR: row i holds cconn{i} (reference order) at
+ *    col[ptr[i] .. ptr[i+1]) and anb{i} at the same positions; ac is the
+ *    diagonal, bc the right-hand side, dphi the unknown.
+ *  - Array arguments are HOST pointers unless the name ends in `_dev`.
+ *    State (matrices, hierarchy, vectors) stays resident on the GPU between
+ *    calls; only scalars cross the boundary during a solve.
+ *  - A context owns one HIP stream; calls on one context must come from one
+ *    host thread at a time.
+ */
+#ifndef UFVM_H
+#define UFVM_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define UFVM_OK 0
+#define UFVM_ERR_ARG 1         /* bad argument / size */
+#define UFVM_ERR_HIP 2         /* HIP runtime error */
+#define UFVM_ERR_STATE 3       /* call out of order (e.g. no hierarchy yet) */
+#define UFVM_ERR_NUMERIC 4     /* NaN / zero diagonal / breakdown */
+#define UFVM_ERR_UNSUPPORTED 5 /* option not supported (e.g. W-cycle with < 4 levels) */
+
+#define UFVM_SMOOTHER_ILU 0    /* 'ILU' (DILU) -- cfdSolveAlgebraicSystem.m:29-46 */
+#define UFVM_SMOOTHER_SOR 1    /* 'SOR' (symmetric GS) -- cfdSolveAlgebraicSystem.m:47-62 */
+
+#define UFVM_CYCLE_V 0
+#define UFVM_CYCLE_W 1
+#define UFVM_CYCLE_F 2
+
+typedef struct ufvm_ctx ufvm_ctx;
+
+/* ---------------------------------------------------------------- context */
+/* No reference counterpart (MATLAB holds state in `global Domain`,
+ * uFVM/src/fvm/setup/cfdSetupDomain.m:11-63). */
+int ufvm_ctx_create(int32_t device, ufvm_ctx **out);
+int ufvm_ctx_destroy(ufvm_ctx *ctx);
+const char *ufvm_last_error(const ufvm_ctx *ctx);
+/* the context's hipStream_t (for event timing by a host that shares it) */
+void *ufvm_ctx_stream(ufvm_ctx *ctx);
+int ufvm_ctx_synchronize(ufvm_ctx *ctx);
+
+/* ---------------------------------------------------------- linear system */
+/* Create the level-1 system of one equation from the mesh connectivity
+ * (theMesh.cconn / theMesh.csize, uFVM/src/fvm/assemble/cfdAssembleEquation.m:18-20
+ * via cfdSetupCoefficients).  Coefficients start at zero. */
+int ufvm_system_create(ufvm_ctx *ctx, int32_t n, int32_t nnz, const int32_t *ptr,
+                       const int32_t *col, int32_t *sys_out);
+int ufvm_system_destroy(ufvm_ctx *ctx, int32_t sys);
+
+/* Domain.coefficients{1} = {ac, anb, bc} (cfdSetCoefficients.m:3-9).  Any of
+ * the pointers may be NULL to leave that array unchanged.  Setting level-1
+ * coefficients zeroes dphi, as cfdSetupCoefficients.m:17 does. */
+int ufvm_system_set_coefficients(ufvm_ctx *ctx, int32_t sys, const double *ac,
+                                 const double *anb, const double *bc);
+/* same, from device memory (e.g. a torch tensor on the context's device) */
+int ufvm_system_set_coefficients_dev(ufvm_ctx *ctx, int32_t sys, const double *ac_dev,
+                                     const double *anb_dev, const double *bc_dev);
+/* Domain.coefficients{level}.dphi read/write (cfdGetCoefficients.m:6-12) */
+int ufvm_system_set_dphi(ufvm_ctx *ctx, int32_t sys, int32_t level, const double *dphi);
+int ufvm_system_get_dphi(ufvm_ctx *ctx, int32_t sys, int32_t level, double *dphi);
+/* Domain.coefficients{level}.bc read/write */
+int ufvm_system_set_bc(ufvm_ctx *ctx, int32_t sys, int32_t level, const double *bc);
+int ufvm_system_get_bc(ufvm_ctx *ctx, int32_t sys, int32_t level, double *bc);
+
+/* r = cfdComputeResidualsArray(Domain.coefficients{level})
+ * (uFVM/src/fvm/compute/cfdComputeResidualsArray.m:11-17): bit-exact, rows
+ * summed in cconn order.  r_out (host, may be NULL) receives r; l1_out (may
+ * be NULL) receives sum(abs(r)). */
+int ufvm_compute_residuals(ufvm_ctx *ctx, int32_t sys, int32_t level, double *r_out,
+                           double *l1_out);
+
+/* ----------------------------------------------------- AMG (agglomeration) */
+/* cfdAgglomerate(maxCoarseLevels) (uFVM/src/fvm/solve/AMG/cfdAgglomerate.m:7-17):
+ * builds the coarse levels from the current level-1 coefficients with the
+ * reference's greedy agglomeration (cfdAgglomerateLevel.m:3-104) and Galerkin
+ * coarse operators (assembleAgglomeratedLHS.m:16-30).  Parents and coarse
+ * cconn are bit-exact with the reference; coarse coefficients are summed in
+ * the reference order on the GPU.  *nlevels receives maxLevels. */
+int ufvm_amg_agglomerate(ufvm_ctx *ctx, int32_t sys, int32_t max_coarse_levels,
+                         int32_t *nlevels);
+int ufvm_amg_num_levels(ufvm_ctx *ctx, int32_t sys, int32_t *nlevels);
+/* size of Domain.coefficients{level} (0-based level index) */
+int ufvm_amg_level_info(ufvm_ctx *ctx, int32_t sys, int32_t level, int32_t *n,
+                        int32_t *nnz);
+/* copy out level data; any pointer may be NULL.  parents (length n) maps the
+ * level to level+1 and is only defined below the coarsest level. */
+int ufvm_amg_level_get(ufvm_ctx *ctx, int32_t sys, int32_t level, int32_t *parents,
+                       int32_t *ptr, int32_t *col, double *ac, double *anb);
+
+/* cfdSolveAlgebraicSystem(level, smoother, nIter, rrf)
+ * (uFVM/src/fvm/solve/cfdSolveAlgebraicSystem.m:11-66) on the GPU with the
+ * reference's lexicographic SOR / DILU orderings (level-scheduled, bit-exact
+ * per sweep) and its mean-|r| early exit. */
+int ufvm_solve_algebraic_system(ufvm_ctx *ctx, int32_t sys, int32_t level, int32_t smoother,
+                                int32_t n_iter, double rrf, double *r0, double *r1);
+
+/* Settings of theEquation.multigrid / theSolutionMethods
+ * (uFVM/src/fvm/top/cfdDefineEquation.m:118-136, cfdSetupAMG.m:6-18). */
+typedef struct {
+  int32_t cycle;             /* UFVM_CYCLE_V / _W / _F */
+  int32_t max_cycles;        /* maxCycles (30) */
+  double termination;        /* termination (0.1) */
+  int32_t max_coarse_levels; /* maxCoarseLevels (10) */
+  int32_t pre_sweep;         /* preSweep (1) */
+  int32_t post_sweep;        /* postSweep (3) */
+  int32_t smoother;          /* UFVM_SMOOTHER_ILU / _SOR */
+  int32_t rebuild;           /* 1: re-agglomerate on every call as cfdApplyAMG.m:19 does */
+} ufvm_amg_cfg;
+
+/* [r0, r1] = cfdApplyAMG(theEquation) (uFVM/src/fvm/solve/AMG/cfdApplyAMG.m:9-60):
+ * stationary V/W/F cycles on the level-1 system with the reference's L1
+ * termination; *ncycles receives the reference's nCycle counter. */
+int ufvm_apply_amg(ufvm_ctx *ctx, int32_t sys, const ufvm_amg_cfg *cfg, double *r0,
+                   double *r1, int32_t *ncycles);
+
+/* ------------------------------------------------ precision-form AMG-PCG */
+/* No reference counterpart: flexible PCG (FP64) preconditioned by one
+ * AMG V-cycle over the reference's hierarchy, with multicolour Gauss-Seidel
+ * smoothing and an FP64 dense solve on the first level with at most
+ * max_coarse_cells cells. */
+typedef struct {
+  int32_t precision;         /* 32: FP32 V-cycle, 64: FP64 V-cycle */
+  int32_t pre_sweep;         /* symmetric MCGS sweeps before restriction */
+  int32_t post_sweep;        /* symmetric MCGS sweeps after prolongation */
+  int32_t max_coarse_cells;  /* dense-solve threshold (<= 4096) */
+  int32_t nullspace;         /* -1 auto-detect (zero row sums), 0 no, 1 constant nullspace */
+  int32_t max_iter;
+  double tol;                /* stop when ||r||_2 <= tol * ||b||_2 */
+} ufvm_pcg_cfg;
+
+/* Build the solve-side data of the current hierarchy (colourings, permuted
+ * FP32/FP64 level copies, coarsest inverse).  Requires ufvm_amg_agglomerate. */
+int ufvm_pcg_setup(ufvm_ctx *ctx, int32_t sys, const ufvm_pcg_cfg *cfg);
+/* Solve A dphi = bc for level 1 from dphi = 0.  On return dphi holds the
+ * solution (device; read with ufvm_system_get_dphi).  */
+int ufvm_pcg_solve(ufvm_ctx *ctx, int32_t sys, int32_t *iters, double *relres);
+/* Apply one preconditioner V-cycle: z = M^-1 r (host arrays, level-1 order). */
+int ufvm_pcg_apply_vcycle(ufvm_ctx *ctx, int32_t sys, const double *r, double *z);
+/* number of colours of a level's multicolour ordering (after ufvm_pcg_setup) */
+int ufvm_pcg_level_colours(ufvm_ctx *ctx, int32_t sys, int32_t level, int32_t *ncolours,
+                           int32_t *colour);
+
+/* ------------------------------------------------------------- profiling */
+/* Per-kernel-class HIP-event timing on the context stream.  kind names:
+ * "mcgs0" (level-0 smoother colour passes), "spmv0" (level-0 FP64 SpMV). */
+int ufvm_prof_enable(ufvm_ctx *ctx, int32_t on);
+int ufvm_prof_read(ufvm_ctx *ctx, const char *kind, double *total_ms, int64_t *launches,
+                   double *bytes);
+int ufvm_prof_reset(ufvm_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

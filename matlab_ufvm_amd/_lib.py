@@ -1,0 +1,265 @@
+"""ctypes binding of libufvm.so (the C ABI declared in include/ufvm.h).
+
+The product path has no CPU fallback: if the HIP library is missing or no GPU
+is visible, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libufvm.so")
+
+UFVM_OK = 0
+ERRORS = {1: "UFVM_ERR_ARG", 2: "UFVM_ERR_HIP", 3: "UFVM_ERR_STATE", 4: "UFVM_ERR_NUMERIC",
+          5: "UFVM_ERR_UNSUPPORTED"}
+SMOOTHER = {"ILU": 0, "SOR": 1}
+CYCLE = {"V-Cycle": 0, "W-Cycle": 1, "F-Cycle": 2}
+
+i32p = C.POINTER(C.c_int32)
+f64p = C.POINTER(C.c_double)
+
+
+class AmgCfg(C.Structure):
+    _fields_ = [("cycle", C.c_int32), ("max_cycles", C.c_int32), ("termination", C.c_double),
+                ("max_coarse_levels", C.c_int32), ("pre_sweep", C.c_int32),
+                ("post_sweep", C.c_int32), ("smoother", C.c_int32), ("rebuild", C.c_int32)]
+
+
+class PcgCfg(C.Structure):
+    _fields_ = [("precision", C.c_int32), ("pre_sweep", C.c_int32), ("post_sweep", C.c_int32),
+                ("max_coarse_cells", C.c_int32), ("nullspace", C.c_int32),
+                ("max_iter", C.c_int32), ("tol", C.c_double)]
+
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "ufvm_ctx_create": (C.c_int, [C.c_int32, C.POINTER(C.c_void_p)]),
+    "ufvm_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "ufvm_last_error": (C.c_char_p, [C.c_void_p]),
+    "ufvm_ctx_stream": (C.c_void_p, [C.c_void_p]),
+    "ufvm_ctx_synchronize": (C.c_int, [C.c_void_p]),
+    "ufvm_system_create": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, i32p, i32p, i32p]),
+    "ufvm_system_destroy": (C.c_int, [C.c_void_p, C.c_int32]),
+    "ufvm_system_set_coefficients": (C.c_int, [C.c_void_p, C.c_int32, f64p, f64p, f64p]),
+    "ufvm_system_set_coefficients_dev": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p,
+                                                   C.c_void_p, C.c_void_p]),
+    "ufvm_system_set_dphi": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f64p]),
+    "ufvm_system_get_dphi": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f64p]),
+    "ufvm_system_set_bc": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f64p]),
+    "ufvm_system_get_bc": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f64p]),
+    "ufvm_compute_residuals": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f64p, f64p]),
+    "ufvm_amg_agglomerate": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, i32p]),
+    "ufvm_amg_num_levels": (C.c_int, [C.c_void_p, C.c_int32, i32p]),
+    "ufvm_amg_level_info": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, i32p, i32p]),
+    "ufvm_amg_level_get": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, i32p, i32p, i32p, f64p,
+                                     f64p]),
+    "ufvm_solve_algebraic_system": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32,
+                                              C.c_int32, C.c_double, f64p, f64p]),
+    "ufvm_apply_amg": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(AmgCfg), f64p, f64p, i32p]),
+    "ufvm_pcg_setup": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(PcgCfg)]),
+    "ufvm_pcg_solve": (C.c_int, [C.c_void_p, C.c_int32, i32p, f64p]),
+    "ufvm_pcg_apply_vcycle": (C.c_int, [C.c_void_p, C.c_int32, f64p, f64p]),
+    "ufvm_pcg_level_colours": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, i32p, i32p]),
+    "ufvm_prof_enable": (C.c_int, [C.c_void_p, C.c_int32]),
+    "ufvm_prof_read": (C.c_int, [C.c_void_p, C.c_char_p, f64p, C.POINTER(C.c_int64), f64p]),
+    "ufvm_prof_reset": (C.c_int, [C.c_void_p]),
+}
+
+EXPORTED = tuple(_SIGS)
+_lib = None
+
+
+def load(path: str = LIB_PATH) -> C.CDLL:
+    """Load libufvm.so; raises if it is absent (there is no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"libufvm.so not built ({path}); run __graft_entry__.build()")
+    lib = C.CDLL(path)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+class UfvmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else None
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+class Context:
+    """One GPU context (HIP stream + device-resident systems)."""
+
+    def __init__(self, device: int = 0):
+        self.lib = load()
+        h = C.c_void_p()
+        rc = self.lib.ufvm_ctx_create(device, C.byref(h))
+        if rc != UFVM_OK:
+            raise UfvmError(rc, "ufvm_ctx_create failed (no GPU visible?)")
+        self.h = h
+
+    def check(self, rc):
+        if rc != UFVM_OK:
+            raise UfvmError(rc, self.lib.ufvm_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.ufvm_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self) -> int:
+        return int(self.lib.ufvm_ctx_stream(self.h) or 0)
+
+    def synchronize(self):
+        self.check(self.lib.ufvm_ctx_synchronize(self.h))
+
+    # -------------------------------------------------------------- systems
+    def system_create(self, ptr, col) -> int:
+        ptr, col = _i32(ptr), _i32(col)
+        sid = C.c_int32()
+        self.check(self.lib.ufvm_system_create(self.h, ptr.size - 1, col.size, _p(ptr, i32p),
+                                               _p(col, i32p), C.byref(sid)))
+        return sid.value
+
+    def system_destroy(self, sid):
+        self.check(self.lib.ufvm_system_destroy(self.h, sid))
+
+    def set_coefficients(self, sid, ac=None, anb=None, bc=None):
+        ac = _f64(ac) if ac is not None else None
+        anb = _f64(anb) if anb is not None else None
+        bc = _f64(bc) if bc is not None else None
+        self.check(self.lib.ufvm_system_set_coefficients(self.h, sid, _p(ac, f64p),
+                                                         _p(anb, f64p), _p(bc, f64p)))
+
+    def level_info(self, sid, level):
+        n, nnz = C.c_int32(), C.c_int32()
+        self.check(self.lib.ufvm_amg_level_info(self.h, sid, level, C.byref(n), C.byref(nnz)))
+        return n.value, nnz.value
+
+    def set_dphi(self, sid, level, x):
+        x = _f64(x)
+        self.check(self.lib.ufvm_system_set_dphi(self.h, sid, level, _p(x, f64p)))
+
+    def get_dphi(self, sid, level=0):
+        n, _ = self.level_info(sid, level)
+        x = np.empty(n)
+        self.check(self.lib.ufvm_system_get_dphi(self.h, sid, level, _p(x, f64p)))
+        return x
+
+    def set_bc(self, sid, level, b):
+        b = _f64(b)
+        self.check(self.lib.ufvm_system_set_bc(self.h, sid, level, _p(b, f64p)))
+
+    def get_bc(self, sid, level=0):
+        n, _ = self.level_info(sid, level)
+        b = np.empty(n)
+        self.check(self.lib.ufvm_system_get_bc(self.h, sid, level, _p(b, f64p)))
+        return b
+
+    def residual(self, sid, level=0):
+        n, _ = self.level_info(sid, level)
+        r = np.empty(n)
+        l1 = C.c_double()
+        self.check(self.lib.ufvm_compute_residuals(self.h, sid, level, _p(r, f64p), C.byref(l1)))
+        return r, l1.value
+
+    def agglomerate(self, sid, max_coarse_levels=10) -> int:
+        nl = C.c_int32()
+        self.check(self.lib.ufvm_amg_agglomerate(self.h, sid, max_coarse_levels, C.byref(nl)))
+        return nl.value
+
+    def num_levels(self, sid) -> int:
+        nl = C.c_int32()
+        self.check(self.lib.ufvm_amg_num_levels(self.h, sid, C.byref(nl)))
+        return nl.value
+
+    def level_get(self, sid, level):
+        n, nnz = self.level_info(sid, level)
+        nl = self.num_levels(sid)
+        par = np.empty(n, np.int32) if level < nl - 1 else None
+        ptr = np.empty(n + 1, np.int32)
+        col = np.empty(max(nnz, 1), np.int32)
+        ac = np.empty(n)
+        anb = np.empty(max(nnz, 1))
+        self.check(self.lib.ufvm_amg_level_get(self.h, sid, level, _p(par, i32p), _p(ptr, i32p),
+                                               _p(col, i32p), _p(ac, f64p), _p(anb, f64p)))
+        return dict(parents=par, ptr=ptr, col=col[:nnz], ac=ac, anb=anb[:nnz])
+
+    def solve_algebraic_system(self, sid, level, smoother, n_iter, rrf=0.1):
+        r0, r1 = C.c_double(), C.c_double()
+        self.check(self.lib.ufvm_solve_algebraic_system(self.h, sid, level, SMOOTHER[smoother],
+                                                        n_iter, rrf, C.byref(r0), C.byref(r1)))
+        return r0.value, r1.value
+
+    def apply_amg(self, sid, cycle="V-Cycle", max_cycles=30, termination=0.1,
+                  max_coarse_levels=10, pre_sweep=1, post_sweep=3, smoother="ILU", rebuild=1):
+        cfg = AmgCfg(CYCLE[cycle], max_cycles, termination, max_coarse_levels, pre_sweep,
+                     post_sweep, SMOOTHER[smoother], rebuild)
+        r0, r1, nc = C.c_double(), C.c_double(), C.c_int32()
+        self.check(self.lib.ufvm_apply_amg(self.h, sid, C.byref(cfg), C.byref(r0), C.byref(r1),
+                                           C.byref(nc)))
+        return r0.value, r1.value, nc.value
+
+    def pcg_setup(self, sid, precision=32, pre_sweep=1, post_sweep=1, max_coarse_cells=2000,
+                  nullspace=-1, max_iter=500, tol=1e-8):
+        cfg = PcgCfg(precision, pre_sweep, post_sweep, max_coarse_cells, nullspace, max_iter, tol)
+        self.check(self.lib.ufvm_pcg_setup(self.h, sid, C.byref(cfg)))
+
+    def pcg_solve(self, sid):
+        it, rr = C.c_int32(), C.c_double()
+        self.check(self.lib.ufvm_pcg_solve(self.h, sid, C.byref(it), C.byref(rr)))
+        return it.value, rr.value
+
+    def pcg_apply_vcycle(self, sid, r):
+        r = _f64(r)
+        z = np.empty_like(r)
+        self.check(self.lib.ufvm_pcg_apply_vcycle(self.h, sid, _p(r, f64p), _p(z, f64p)))
+        return z
+
+    def pcg_level_colours(self, sid, level):
+        n, _ = self.level_info(sid, level)
+        nc = C.c_int32()
+        col = np.empty(n, np.int32)
+        self.check(self.lib.ufvm_pcg_level_colours(self.h, sid, level, C.byref(nc),
+                                                   _p(col, i32p)))
+        return nc.value, col
+
+    # ------------------------------------------------------------ profiling
+    def prof_enable(self, on=True):
+        self.check(self.lib.ufvm_prof_enable(self.h, 1 if on else 0))
+
+    def prof_reset(self):
+        self.check(self.lib.ufvm_prof_reset(self.h))
+
+    def prof_read(self, kind):
+        ms, n, by = C.c_double(), C.c_int64(), C.c_double()
+        self.check(self.lib.ufvm_prof_read(self.h, kind.encode(), C.byref(ms), C.byref(n),
+                                           C.byref(by)))
+        return ms.value, n.value, by.value

@@ -1,0 +1,50 @@
+// hostalg.h -- host-side graph algorithms of the pressure-solve path:
+// the reference's greedy agglomeration, multicolour orderings and the level
+// schedules of the lexicographic smoothers.  Integer/sequential work that
+// runs once per hierarchy; its outputs are uploaded to HBM.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace ufvm {
+
+struct HostCSR {
+  int32_t n = 0;
+  std::vector<int32_t> ptr, col;  // reference (cconn) order
+};
+
+struct AggResult {
+  int32_t nc = 0;
+  std::vector<int32_t> parents;     // fine -> coarse (0-based)
+  HostCSR coarse;                   // coarse cconn, first-encounter order
+  std::vector<int32_t> child_ptr;   // coarse row -> children, increasing fine index
+  std::vector<int32_t> child;
+  std::vector<int32_t> entry_slot;  // fine nnz -> coarse nnz slot, -1 if same parent
+};
+
+// cfdAgglomerateLevel.m:3-104 on one level (anb values in cconn order).
+void agglomerate_level(const HostCSR &g, const double *anb, AggResult &out);
+
+// greedy first-fit colouring in index order; returns number of colours
+int32_t greedy_colour(const HostCSR &g, std::vector<int32_t> &colour);
+
+// Level schedule of a lexicographic sweep.  forward: row i waits for its
+// neighbours j < i; backward: for neighbours j > i.  rows grouped by level,
+// increasing row index inside a level.
+struct Schedule {
+  std::vector<int32_t> level_ptr;  // [nlevels+1]
+  std::vector<int32_t> rows;       // [n]
+};
+void lex_schedule(const HostCSR &g, bool forward, Schedule &s);
+
+// For DILU: for every row j, its lower neighbours i < j sorted by i, with the
+// position of i in row j (a_ji) and of j in row i (a_ij).
+struct LowerLists {
+  std::vector<int32_t> ptr;       // [n+1]
+  std::vector<int32_t> nb;        // i
+  std::vector<int32_t> pos_ji;    // CSR slot of a_ji in row j
+  std::vector<int32_t> pos_ij;    // CSR slot of a_ij in row i
+};
+bool lower_lists(const HostCSR &g, LowerLists &L);
+
+}  // namespace ufvm

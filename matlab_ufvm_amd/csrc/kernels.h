@@ -1,0 +1,101 @@
+// kernels.h -- launchers of the gfx950 kernels of the pressure-solve path.
+// All launches go on the given stream; all indices are 0-based int32.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ufvm {
+namespace k {
+
+constexpr int kBlock = 256;
+inline int nblocks(int64_t n, int b = kBlock) { return (int)((n + b - 1) / b); }
+
+// ---- reference-order FP64 kernels (bit-exact with the reference arithmetic)
+// r = bc - ac*x - sum_k anb_k x_col(k) (cconn order); partial[blk] = sum |r|
+void residual_ref(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col,
+                  const double *ac, const double *anb, const double *bc, const double *x,
+                  double *r, double *partial);
+// out[0] = sum(partial[0..np)) in a fixed order
+void reduce_final(hipStream_t s, const double *partial, int32_t np, double *out);
+// Gauss-Seidel update of the listed rows (one level of a lexicographic sweep)
+void gs_rows(hipStream_t s, const int32_t *rows, int32_t count, const int32_t *ptr,
+             const int32_t *col, const double *ac, const double *anb, const double *b,
+             double *x);
+// DILU factorisation of the listed rows (pull form of cfdFactorizeILU)
+void dilu_factor_rows(hipStream_t s, const int32_t *rows, int32_t count, const double *ac,
+                      const double *anb, const int32_t *lptr, const int32_t *lnb,
+                      const int32_t *lpji, const int32_t *lpij, int32_t skip_row, double *dinv);
+void dilu_fwd_rows(hipStream_t s, const int32_t *rows, int32_t count, const double *anb,
+                   const int32_t *lptr, const int32_t *lnb, const int32_t *lpji,
+                   const double *dinv, double *rc);
+void dilu_bwd_rows(hipStream_t s, const int32_t *rows, int32_t count, int32_t n,
+                   const int32_t *ptr, const int32_t *col, const double *anb, const double *dinv,
+                   double *rc, double *x);
+// bc_c[p] = sum over children (increasing fine index) of r
+void restrict_ref(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *child,
+                  const double *r, double *bc_c);
+// x[i] += xc[parent[i]]
+void prolong_ref(hipStream_t s, int32_t n, const int32_t *parent, const double *xc, double *x);
+// Galerkin coarse operator in the reference accumulation order
+void coarse_lhs(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *child,
+                const int32_t *fptr, const double *fac, const double *fanb, const int32_t *slot,
+                const int32_t *cptr_c, double *cac, double *canb);
+
+// ---- multicolour / precision-form kernels (V = float or double)
+template <typename V>
+void mcgs_rows(hipStream_t s, int32_t r0, int32_t r1, const int32_t *ptr, const int32_t *col,
+               const V *ac, const V *anb, const V *b, V *x, bool x_was_zero);
+template <typename V>
+void residual_fast(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col, const V *ac,
+                   const V *anb, const V *b, const V *x, V *r);
+template <typename V>
+void restrict_fast(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *child,
+                   const V *r, V *bc);
+template <typename V>
+void prolong_fast(hipStream_t s, int32_t n, const int32_t *parent, const V *xc, V *x);
+template <typename V>
+void dense_solve(hipStream_t s, int32_t n, const double *ainv, const V *b, V *x);
+template <typename V>
+void fill(hipStream_t s, int64_t n, V *x, V v);
+template <typename A, typename B>
+void convert(hipStream_t s, int64_t n, const A *a, B *b);
+void permute_gather(hipStream_t s, int32_t n, const int32_t *idx, const double *src, double *dst);
+void permute_scatter(hipStream_t s, int32_t n, const int32_t *idx, const double *src,
+                     double *dst);
+
+// dense Gauss-Jordan inverse in place (row-major n x n), no pivoting
+void gj_inverse(hipStream_t s, int32_t n, double *a, double *colbuf);
+// a = csr (+ shift*11^T) as dense row-major
+void csr_to_dense(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col,
+                  const double *ac, const double *anb, double shift, double *a);
+
+// ---- FPCG (FP64, level-1 order of the multicolour permutation)
+// w = A p ; partial[blk] = p.w
+void spmv_dot(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col, const double *ac,
+              const double *anb, const double *p, double *w, double *partial);
+// sums of z, z.w, z.r, w, r  -> partial[5*blk + {0..4}]
+template <typename V>
+void fcg_dots(hipStream_t s, int32_t n, const V *z, const double *w, const double *r,
+              double *partial);
+// p = (z - m) - beta p  with m = zsum/n (singular) or 0; beta = (zw - m wsum)/delta
+template <typename V>
+void fcg_update_p(hipStream_t s, int32_t n, const V *z, const double *scal, double *p,
+                  int first);
+// x += alpha p ; r -= alpha w ; partial = r.r ; rv = (V) r
+template <typename V>
+void fcg_update_xr(hipStream_t s, int32_t n, const double *scal, const double *p,
+                   const double *w, double *x, double *r, V *rv, double *partial);
+// reduce nparts partial groups of width `width` into out[0..width)
+void reduce_groups(hipStream_t s, const double *partial, int32_t nblk, int32_t width,
+                   double *out);
+// scalar bookkeeping kernels of FPCG (all on device)
+void fcg_scalars_after_dots(hipStream_t s, double *scal, int32_t n, int singular, int first);
+void fcg_scalars_after_spmv(hipStream_t s, double *scal);
+// r.r and |b| helpers
+void dot_partial(hipStream_t s, int32_t n, const double *a, const double *b, double *partial);
+void sum_partial(hipStream_t s, int32_t n, const double *a, double *partial);
+void axpy_const(hipStream_t s, int32_t n, const double *scal_slot, double mul, double *x);
+
+}  // namespace k
+}  // namespace ufvm

@@ -1,0 +1,627 @@
+// kernels.hip -- gfx950 kernels of the uFVM pressure-solve path.
+//
+// All of this work is sparse and HBM-bound: gathers over an unstructured
+// cell graph, no dense contraction (MFMA is not used).  Two families:
+//
+//  * reference-order FP64 kernels.  They sum every row in the reference's
+//    cconn order with separately rounded multiplies and adds (the library is
+//    built with -ffp-contract=off), so residuals, restriction, prolongation,
+//    Galerkin coarse operators and the level-scheduled lexicographic SOR /
+//    DILU sweeps reproduce the reference arithmetic bit for bit.
+//  * multicolour / precision-form kernels, templated on the value type, used
+//    by the FP32-or-FP64 V-cycle preconditioner and the FP64 flexible CG.
+//
+// Reductions are deterministic: per-block partial sums in a fixed tree order,
+// then one block sums the partials in a fixed order.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace ufvm {
+namespace k {
+
+constexpr int kWave = 64;
+
+template <typename T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
+  return v;
+}
+
+// sum over a 256-thread block; result valid in thread 0
+template <typename T>
+__device__ inline T block_sum(T v, T *lds) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  T r = 0;
+  if (threadIdx.x == 0) {
+    for (int w = 0; w < kBlock / kWave; ++w) r += lds[w];
+  }
+  __syncthreads();
+  return r;
+}
+
+// ------------------------------------------------------------ reference mode
+
+__global__ void __launch_bounds__(kBlock)
+k_residual_ref(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+               const double *__restrict__ ac, const double *__restrict__ anb,
+               const double *__restrict__ bc, const double *__restrict__ x,
+               double *__restrict__ r, double *__restrict__ partial) {
+  __shared__ double lds[kBlock / kWave];
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  double a = 0.0;
+  if (i < n) {
+    double s = bc[i] - ac[i] * x[i];
+    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) s = s - anb[k] * x[col[k]];
+    r[i] = s;
+    a = fabs(s);
+  }
+  a = block_sum(a, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = a;
+}
+
+void residual_ref(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col,
+                  const double *ac, const double *anb, const double *bc, const double *x,
+                  double *r, double *partial) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_residual_ref, dim3(nblocks(n)), dim3(kBlock), 0, s, n, ptr, col, ac, anb,
+                     bc, x, r, partial);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_reduce_groups(const double *__restrict__ partial, int32_t nblk, int32_t width,
+                double *__restrict__ out) {
+  __shared__ double lds[kBlock / kWave];
+  // one block per output component; each thread sums a strided subset
+  const int c = blockIdx.x;
+  double v = 0.0;
+  for (int32_t b = threadIdx.x; b < nblk; b += kBlock) v += partial[(int64_t)b * width + c];
+  v = block_sum(v, lds);
+  if (threadIdx.x == 0) out[c] = v;
+}
+
+void reduce_final(hipStream_t s, const double *partial, int32_t np, double *out) {
+  hipLaunchKernelGGL(k_reduce_groups, dim3(1), dim3(kBlock), 0, s, partial, np, 1, out);
+}
+
+void reduce_groups(hipStream_t s, const double *partial, int32_t nblk, int32_t width,
+                   double *out) {
+  hipLaunchKernelGGL(k_reduce_groups, dim3(width), dim3(kBlock), 0, s, partial, nblk, width,
+                     out);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_gs_rows(const int32_t *__restrict__ rows, int32_t count, const int32_t *__restrict__ ptr,
+          const int32_t *__restrict__ col, const double *__restrict__ ac,
+          const double *__restrict__ anb, const double *__restrict__ b, double *x) {
+  const int32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= count) return;
+  const int32_t i = rows[t];
+  double v = b[i];
+  for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) v = v - anb[k] * x[col[k]];
+  x[i] = v / ac[i];
+}
+
+void gs_rows(hipStream_t s, const int32_t *rows, int32_t count, const int32_t *ptr,
+             const int32_t *col, const double *ac, const double *anb, const double *b,
+             double *x) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_gs_rows, dim3(nblocks(count)), dim3(kBlock), 0, s, rows, count, ptr, col,
+                     ac, anb, b, x);
+}
+
+// d_j = ac_j - sum_{i<j, increasing i, i != skip} (a_ji * dinv_i) * a_ij ; dinv_j = 1/d_j
+__global__ void __launch_bounds__(kBlock)
+k_dilu_factor(const int32_t *__restrict__ rows, int32_t count, const double *__restrict__ ac,
+              const double *__restrict__ anb, const int32_t *__restrict__ lptr,
+              const int32_t *__restrict__ lnb, const int32_t *__restrict__ lpji,
+              const int32_t *__restrict__ lpij, int32_t skip_row, double *dinv) {
+  const int32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= count) return;
+  const int32_t j = rows[t];
+  double d = ac[j];
+  for (int32_t e = lptr[j]; e < lptr[j + 1]; ++e) {
+    const int32_t i = lnb[e];
+    if (i == skip_row) continue;
+    d = d - (anb[lpji[e]] * dinv[i]) * anb[lpij[e]];
+  }
+  dinv[j] = 1.0 / d;
+}
+
+void dilu_factor_rows(hipStream_t s, const int32_t *rows, int32_t count, const double *ac,
+                      const double *anb, const int32_t *lptr, const int32_t *lnb,
+                      const int32_t *lpji, const int32_t *lpij, int32_t skip_row, double *dinv) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_dilu_factor, dim3(nblocks(count)), dim3(kBlock), 0, s, rows, count, ac, anb,
+                     lptr, lnb, lpji, lpij, skip_row, dinv);
+}
+
+// rc_j -= a_ji * (dinv_i * rc_i) for lower i in increasing order
+__global__ void __launch_bounds__(kBlock)
+k_dilu_fwd(const int32_t *__restrict__ rows, int32_t count, const double *__restrict__ anb,
+           const int32_t *__restrict__ lptr, const int32_t *__restrict__ lnb,
+           const int32_t *__restrict__ lpji, const double *__restrict__ dinv, double *rc) {
+  const int32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= count) return;
+  const int32_t j = rows[t];
+  double v = rc[j];
+  for (int32_t e = lptr[j]; e < lptr[j + 1]; ++e) {
+    const int32_t i = lnb[e];
+    const double mat1 = dinv[i] * rc[i];
+    v = v - anb[lpji[e]] * mat1;
+  }
+  rc[j] = v;
+}
+
+void dilu_fwd_rows(hipStream_t s, const int32_t *rows, int32_t count, const double *anb,
+                   const int32_t *lptr, const int32_t *lnb, const int32_t *lpji,
+                   const double *dinv, double *rc) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_dilu_fwd, dim3(nblocks(count)), dim3(kBlock), 0, s, rows, count, anb, lptr,
+                     lnb, lpji, dinv, rc);
+}
+
+// rc_i -= a_ij rc_j (j > i, cconn order) ; z = dinv_i rc_i ; rc_i = z ; x_i += z
+__global__ void __launch_bounds__(kBlock)
+k_dilu_bwd(const int32_t *__restrict__ rows, int32_t count, int32_t n,
+           const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+           const double *__restrict__ anb, const double *__restrict__ dinv, double *rc,
+           double *x) {
+  const int32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= count) return;
+  const int32_t i = rows[t];
+  double v = rc[i];
+  if (i < n - 1) {
+    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+      const int32_t j = col[k];
+      if (j > i) v = v - anb[k] * rc[j];
+    }
+  }
+  const double z = dinv[i] * v;
+  rc[i] = z;
+  x[i] = x[i] + z;
+}
+
+void dilu_bwd_rows(hipStream_t s, const int32_t *rows, int32_t count, int32_t n,
+                   const int32_t *ptr, const int32_t *col, const double *anb, const double *dinv,
+                   double *rc, double *x) {
+  if (count <= 0) return;
+  hipLaunchKernelGGL(k_dilu_bwd, dim3(nblocks(count)), dim3(kBlock), 0, s, rows, count, n, ptr,
+                     col, anb, dinv, rc, x);
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kBlock)
+k_restrict(int32_t nc, const int32_t *__restrict__ cptr, const int32_t *__restrict__ child,
+           const V *__restrict__ r, V *__restrict__ bc) {
+  const int32_t p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= nc) return;
+  V s = 0;
+  for (int32_t c = cptr[p]; c < cptr[p + 1]; ++c) s = s + r[child[c]];
+  bc[p] = s;
+}
+
+void restrict_ref(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *child,
+                  const double *r, double *bc_c) {
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(k_restrict<double>, dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, cptr, child,
+                     r, bc_c);
+}
+
+template <typename V>
+void restrict_fast(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *child,
+                   const V *r, V *bc) {
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(k_restrict<V>, dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, cptr, child, r,
+                     bc);
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kBlock)
+k_prolong(int32_t n, const int32_t *__restrict__ parent, const V *__restrict__ xc,
+          V *__restrict__ x) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  x[i] = x[i] + xc[parent[i]];
+}
+
+void prolong_ref(hipStream_t s, int32_t n, const int32_t *parent, const double *xc, double *x) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_prolong<double>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, parent, xc, x);
+}
+
+template <typename V>
+void prolong_fast(hipStream_t s, int32_t n, const int32_t *parent, const V *xc, V *x) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_prolong<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, parent, xc, x);
+}
+
+// One thread per coarse row p.  For its children i (increasing fine index):
+// AC += ac_i; then per entry k of row i (cconn order): slot < 0 (same parent)
+// -> AC += anb_k, else ANB[slot] += anb_k.  Row p's ANB entries are written by
+// this thread only, in the reference's accumulation order.
+__global__ void __launch_bounds__(kBlock)
+k_coarse_lhs(int32_t nc, const int32_t *__restrict__ cptr, const int32_t *__restrict__ child,
+             const int32_t *__restrict__ fptr, const double *__restrict__ fac,
+             const double *__restrict__ fanb, const int32_t *__restrict__ slot,
+             const int32_t *__restrict__ cptr_c, double *__restrict__ cac,
+             double *__restrict__ canb) {
+  const int32_t p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= nc) return;
+  for (int32_t t = cptr_c[p]; t < cptr_c[p + 1]; ++t) canb[t] = 0.0;
+  double a = 0.0;
+  for (int32_t c = cptr[p]; c < cptr[p + 1]; ++c) {
+    const int32_t i = child[c];
+    a = a + fac[i];
+    for (int32_t k = fptr[i]; k < fptr[i + 1]; ++k) {
+      const int32_t sl = slot[k];
+      if (sl < 0) a = a + fanb[k];
+      else canb[sl] = canb[sl] + fanb[k];
+    }
+  }
+  cac[p] = a;
+}
+
+void coarse_lhs(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *child,
+                const int32_t *fptr, const double *fac, const double *fanb, const int32_t *slot,
+                const int32_t *cptr_c, double *cac, double *canb) {
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(k_coarse_lhs, dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, cptr, child, fptr,
+                     fac, fanb, slot, cptr_c, cac, canb);
+}
+
+// --------------------------------------------------------- multicolour mode
+
+// Gauss-Seidel update of the contiguous rows [r0, r1) of one colour.  Rows
+// of one colour are mutually independent, so every read of x[col] sees the
+// value from the previous colour pass.  When the whole vector was zero on
+// entry and this is the first colour, the sum over neighbours vanishes.
+template <typename V>
+__global__ void __launch_bounds__(kBlock)
+k_mcgs(int32_t r0, int32_t r1, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+       const V *__restrict__ ac, const V *__restrict__ anb, const V *__restrict__ b, V *x,
+       int zero) {
+  const int32_t i = r0 + blockIdx.x * kBlock + threadIdx.x;
+  if (i >= r1) return;
+  V v = b[i];
+  if (!zero) {
+    const int32_t k0 = ptr[i], k1 = ptr[i + 1];
+    for (int32_t k = k0; k < k1; ++k) v = v - anb[k] * x[col[k]];
+  }
+  x[i] = v / ac[i];
+}
+
+template <typename V>
+void mcgs_rows(hipStream_t s, int32_t r0, int32_t r1, const int32_t *ptr, const int32_t *col,
+               const V *ac, const V *anb, const V *b, V *x, bool x_was_zero) {
+  if (r1 <= r0) return;
+  hipLaunchKernelGGL(k_mcgs<V>, dim3(nblocks(r1 - r0)), dim3(kBlock), 0, s, r0, r1, ptr, col, ac,
+                     anb, b, x, x_was_zero ? 1 : 0);
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kBlock)
+k_residual_fast(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+                const V *__restrict__ ac, const V *__restrict__ anb, const V *__restrict__ b,
+                const V *__restrict__ x, V *__restrict__ r) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  V v = b[i] - ac[i] * x[i];
+  for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) v = v - anb[k] * x[col[k]];
+  r[i] = v;
+}
+
+template <typename V>
+void residual_fast(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col, const V *ac,
+                   const V *anb, const V *b, const V *x, V *r) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_residual_fast<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, ptr, col, ac,
+                     anb, b, x, r);
+}
+
+// x = Ainv * b, one wavefront per row, FP64 accumulation
+template <typename V>
+__global__ void __launch_bounds__(kBlock)
+k_dense_gemv(int32_t n, const double *__restrict__ ainv, const V *__restrict__ b,
+             V *__restrict__ x) {
+  const int32_t row = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (row >= n) return;
+  const double *a = ainv + (int64_t)row * n;
+  double s = 0.0;
+  for (int32_t j = lane; j < n; j += kWave) s += a[j] * (double)b[j];
+  s = wave_sum(s);
+  if (lane == 0) x[row] = (V)s;
+}
+
+template <typename V>
+void dense_solve(hipStream_t s, int32_t n, const double *ainv, const V *b, V *x) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_dense_gemv<V>, dim3((n + kBlock / kWave - 1) / (kBlock / kWave)),
+                     dim3(kBlock), 0, s, n, ainv, b, x);
+}
+
+template <typename V>
+__global__ void k_fill(int64_t n, V *x, V v) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+
+template <typename V>
+void fill(hipStream_t s, int64_t n, V *x, V v) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_fill<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, x, v);
+}
+
+template <typename A, typename B>
+__global__ void k_convert(int64_t n, const A *__restrict__ a, B *__restrict__ b) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) b[i] = (B)a[i];
+}
+
+template <typename A, typename B>
+void convert(hipStream_t s, int64_t n, const A *a, B *b) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL((k_convert<A, B>), dim3(nblocks(n)), dim3(kBlock), 0, s, n, a, b);
+}
+
+__global__ void k_gather(int32_t n, const int32_t *__restrict__ idx, const double *__restrict__ src,
+                         double *__restrict__ dst) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+__global__ void k_scatter(int32_t n, const int32_t *__restrict__ idx,
+                          const double *__restrict__ src, double *__restrict__ dst) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) dst[idx[i]] = src[i];
+}
+void permute_gather(hipStream_t s, int32_t n, const int32_t *idx, const double *src, double *dst) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gather, dim3(nblocks(n)), dim3(kBlock), 0, s, n, idx, src, dst);
+}
+void permute_scatter(hipStream_t s, int32_t n, const int32_t *idx, const double *src,
+                     double *dst) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scatter, dim3(nblocks(n)), dim3(kBlock), 0, s, n, idx, src, dst);
+}
+
+// ---------------------------------------------------------- dense inverse
+
+__global__ void k_csr_to_dense(int32_t n, const int32_t *__restrict__ ptr,
+                               const int32_t *__restrict__ col, const double *__restrict__ ac,
+                               const double *__restrict__ anb, double shift, double *a) {
+  const int32_t i = blockIdx.x;
+  double *row = a + (int64_t)i * n;
+  for (int32_t j = threadIdx.x; j < n; j += blockDim.x) row[j] = shift;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    row[i] += ac[i];
+    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) row[col[k]] += anb[k];
+  }
+}
+
+void csr_to_dense(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col,
+                  const double *ac, const double *anb, double shift, double *a) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_csr_to_dense, dim3(n), dim3(kBlock), 0, s, n, ptr, col, ac, anb, shift, a);
+}
+
+// step k, part 1: save column k, scale pivot row (pivot entry becomes 1/p)
+__global__ void k_gj_pivot(int32_t n, int32_t kk, double *a, double *colbuf) {
+  const double p = a[(int64_t)kk * n + kk];
+  for (int32_t i = threadIdx.x; i < n; i += blockDim.x) colbuf[i] = a[(int64_t)i * n + kk];
+  __syncthreads();
+  const double inv = 1.0 / p;
+  for (int32_t j = threadIdx.x; j < n; j += blockDim.x) {
+    double v = (j == kk) ? 1.0 : a[(int64_t)kk * n + j];
+    a[(int64_t)kk * n + j] = v * inv;
+  }
+}
+
+// step k, part 2: eliminate column k from every other row
+__global__ void k_gj_elim(int32_t n, int32_t kk, double *a, const double *colbuf) {
+  const int32_t i = blockIdx.y;
+  const int32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (i == kk || j >= n) return;
+  const double f = colbuf[i];
+  const double base = (j == kk) ? 0.0 : a[(int64_t)i * n + j];
+  a[(int64_t)i * n + j] = base - f * a[(int64_t)kk * n + j];
+}
+
+void gj_inverse(hipStream_t s, int32_t n, double *a, double *colbuf) {
+  for (int32_t kk = 0; kk < n; ++kk) {
+    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(kBlock), 0, s, n, kk, a, colbuf);
+    hipLaunchKernelGGL(k_gj_elim, dim3(nblocks(n), n), dim3(kBlock), 0, s, n, kk, a, colbuf);
+  }
+}
+
+// ------------------------------------------------------------------- FPCG
+
+__global__ void __launch_bounds__(kBlock)
+k_spmv_dot(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+           const double *__restrict__ ac, const double *__restrict__ anb,
+           const double *__restrict__ p, double *__restrict__ w, double *__restrict__ partial) {
+  __shared__ double lds[kBlock / kWave];
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  double d = 0.0;
+  if (i < n) {
+    const double pi = p[i];
+    double v = ac[i] * pi;
+    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) v = v + anb[k] * p[col[k]];
+    w[i] = v;
+    d = pi * v;
+  }
+  d = block_sum(d, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = d;
+}
+
+void spmv_dot(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col, const double *ac,
+              const double *anb, const double *p, double *w, double *partial) {
+  hipLaunchKernelGGL(k_spmv_dot, dim3(nblocks(n)), dim3(kBlock), 0, s, n, ptr, col, ac, anb, p, w,
+                     partial);
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kBlock)
+k_fcg_dots(int32_t n, const V *__restrict__ z, const double *__restrict__ w,
+           const double *__restrict__ r, double *__restrict__ partial) {
+  __shared__ double lds[kBlock / kWave];
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  double zs = 0, zw = 0, zr = 0, ws = 0, rs = 0;
+  if (i < n) {
+    const double zi = (double)z[i], wi = w[i], ri = r[i];
+    zs = zi;
+    zw = zi * wi;
+    zr = zi * ri;
+    ws = wi;
+    rs = ri;
+  }
+  zs = block_sum(zs, lds);
+  zw = block_sum(zw, lds);
+  zr = block_sum(zr, lds);
+  ws = block_sum(ws, lds);
+  rs = block_sum(rs, lds);
+  if (threadIdx.x == 0) {
+    double *o = partial + (int64_t)blockIdx.x * 5;
+    o[0] = zs; o[1] = zw; o[2] = zr; o[3] = ws; o[4] = rs;
+  }
+}
+
+template <typename V>
+void fcg_dots(hipStream_t s, int32_t n, const V *z, const double *w, const double *r,
+              double *partial) {
+  hipLaunchKernelGGL(k_fcg_dots<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, z, w, r, partial);
+}
+
+// scal layout: 0 zsum 1 zw 2 zr 3 wsum 4 rsum 5 delta 6 beta 7 alpha 8 rho 9 m 10 rr 11 spare
+__global__ void k_fcg_after_dots(double *scal, int32_t n, int singular, int first) {
+  if (threadIdx.x != 0) return;
+  const double m = singular ? scal[0] / (double)n : 0.0;
+  scal[9] = m;
+  scal[8] = scal[2] - m * scal[4];
+  scal[6] = first ? 0.0 : (scal[1] - m * scal[3]) / scal[5];
+}
+
+void fcg_scalars_after_dots(hipStream_t s, double *scal, int32_t n, int singular, int first) {
+  hipLaunchKernelGGL(k_fcg_after_dots, dim3(1), dim3(64), 0, s, scal, n, singular, first);
+}
+
+__global__ void k_fcg_after_spmv(double *scal) {
+  if (threadIdx.x != 0) return;
+  scal[7] = scal[8] / scal[5];
+}
+
+void fcg_scalars_after_spmv(hipStream_t s, double *scal) {
+  hipLaunchKernelGGL(k_fcg_after_spmv, dim3(1), dim3(64), 0, s, scal);
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kBlock)
+k_fcg_update_p(int32_t n, const V *__restrict__ z, const double *__restrict__ scal,
+               double *__restrict__ p, int first) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double m = scal[9];
+  const double zi = (double)z[i] - m;
+  p[i] = first ? zi : zi - scal[6] * p[i];
+}
+
+template <typename V>
+void fcg_update_p(hipStream_t s, int32_t n, const V *z, const double *scal, double *p,
+                  int first) {
+  hipLaunchKernelGGL(k_fcg_update_p<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, z, scal, p,
+                     first);
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kBlock)
+k_fcg_update_xr(int32_t n, const double *__restrict__ scal, const double *__restrict__ p,
+                const double *__restrict__ w, double *__restrict__ x, double *__restrict__ r,
+                V *__restrict__ rv, double *__restrict__ partial) {
+  __shared__ double lds[kBlock / kWave];
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  double rr = 0.0;
+  if (i < n) {
+    const double a = scal[7];
+    x[i] = x[i] + a * p[i];
+    const double ri = r[i] - a * w[i];
+    r[i] = ri;
+    rv[i] = (V)ri;
+    rr = ri * ri;
+  }
+  rr = block_sum(rr, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = rr;
+}
+
+template <typename V>
+void fcg_update_xr(hipStream_t s, int32_t n, const double *scal, const double *p,
+                   const double *w, double *x, double *r, V *rv, double *partial) {
+  hipLaunchKernelGGL(k_fcg_update_xr<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, scal, p, w, x,
+                     r, rv, partial);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_dot_partial(int32_t n, const double *__restrict__ a, const double *__restrict__ b,
+              double *__restrict__ partial) {
+  __shared__ double lds[kBlock / kWave];
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  double v = (i < n) ? a[i] * b[i] : 0.0;
+  v = block_sum(v, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+void dot_partial(hipStream_t s, int32_t n, const double *a, const double *b, double *partial) {
+  hipLaunchKernelGGL(k_dot_partial, dim3(nblocks(n)), dim3(kBlock), 0, s, n, a, b, partial);
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_sum_partial(int32_t n, const double *__restrict__ a, double *__restrict__ partial) {
+  __shared__ double lds[kBlock / kWave];
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  double v = (i < n) ? a[i] : 0.0;
+  v = block_sum(v, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+void sum_partial(hipStream_t s, int32_t n, const double *a, double *partial) {
+  hipLaunchKernelGGL(k_sum_partial, dim3(nblocks(n)), dim3(kBlock), 0, s, n, a, partial);
+}
+
+__global__ void k_axpy_const(int32_t n, const double *__restrict__ slot, double mul,
+                             double *__restrict__ x) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) x[i] = x[i] + mul * slot[0];
+}
+
+void axpy_const(hipStream_t s, int32_t n, const double *scal_slot, double mul, double *x) {
+  hipLaunchKernelGGL(k_axpy_const, dim3(nblocks(n)), dim3(kBlock), 0, s, n, scal_slot, mul, x);
+}
+
+// explicit instantiations
+#define UFVM_INST(V)                                                                            \
+  template void mcgs_rows<V>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,   \
+                             const V *, const V *, const V *, V *, bool);                       \
+  template void residual_fast<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,        \
+                                 const V *, const V *, const V *, const V *, V *);              \
+  template void restrict_fast<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,        \
+                                 const V *, V *);                                               \
+  template void prolong_fast<V>(hipStream_t, int32_t, const int32_t *, const V *, V *);         \
+  template void dense_solve<V>(hipStream_t, int32_t, const double *, const V *, V *);           \
+  template void fill<V>(hipStream_t, int64_t, V *, V);                                          \
+  template void fcg_dots<V>(hipStream_t, int32_t, const V *, const double *, const double *,    \
+                            double *);                                                          \
+  template void fcg_update_p<V>(hipStream_t, int32_t, const V *, const double *, double *, int); \
+  template void fcg_update_xr<V>(hipStream_t, int32_t, const double *, const double *,          \
+                                 const double *, double *, double *, V *, double *);
+UFVM_INST(float)
+UFVM_INST(double)
+template void convert<double, float>(hipStream_t, int64_t, const double *, float *);
+template void convert<float, double>(hipStream_t, int64_t, const float *, double *);
+template void convert<double, double>(hipStream_t, int64_t, const double *, double *);
+
+}  // namespace k
+}  // namespace ufvm

@@ -1,0 +1,135 @@
+// state.h -- device-resident state behind the C ABI: contexts, systems,
+// hierarchy levels, and the precision-form (multicolour) level copies.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "hostalg.h"
+#include "ufvm.h"
+
+namespace ufvm {
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define UFVM_HIP(call)                                                                    \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      throw ::ufvm::Error(UFVM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  void alloc(size_t count) {
+    if (count == n && p) return;
+    release();
+    if (count == 0) count = 1;
+    UFVM_HIP(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+  }
+  // synchronous with respect to the host buffer: returns after the copy is done
+  void upload(const T *h, size_t count, hipStream_t s) {
+    alloc(count);
+    if (count) {
+      UFVM_HIP(hipMemcpyAsync(p, h, count * sizeof(T), hipMemcpyHostToDevice, s));
+      UFVM_HIP(hipStreamSynchronize(s));
+    }
+  }
+  void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
+  void download(T *h, size_t count, hipStream_t s) const {
+    if (count) UFVM_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
+  }
+};
+
+// One level of the reference hierarchy, in reference numbering and cconn order.
+struct RefLevel {
+  int32_t n = 0, nnz = 0;
+  HostCSR g;
+  DBuf<int32_t> ptr, col;
+  DBuf<double> ac, anb, bc, dphi, r, rc, dinv;
+  // map to the next coarser level (defined below the coarsest)
+  std::vector<int32_t> h_parents;
+  DBuf<int32_t> parents, child_ptr, child, slot;
+  // lexicographic schedules (SOR / DILU), built on first use
+  bool sched = false;
+  Schedule fwd, bwd;
+  DBuf<int32_t> fwd_rows, bwd_rows;
+  bool low_ok = false;
+  LowerLists low;
+  DBuf<int32_t> lptr, lnb, lpji, lpij;
+};
+
+// One level of the precision-form V-cycle: unknowns renumbered so that each
+// colour class is a contiguous row range, CSR kept in cconn order per row.
+struct FastLevel {
+  int32_t n = 0, nnz = 0, ncol = 0;
+  std::vector<int32_t> col_off;   // colour c rows: [col_off[c], col_off[c+1])
+  std::vector<int32_t> colour;    // reference numbering
+  std::vector<int32_t> perm;      // perm[new] = reference index
+  std::vector<int32_t> h_ptr;     // permuted CSR row pointer (host copy)
+  DBuf<int32_t> ptr, col, perm_d;
+  DBuf<float> ac32, anb32, b32, x32, r32;
+  DBuf<double> ac64, anb64, b64, x64, r64;
+  DBuf<int32_t> parent, child_ptr, child;   // to / from the next level (new numbering)
+};
+
+struct PcgState {
+  bool ready = false;
+  ufvm_pcg_cfg cfg{};
+  int32_t nlev = 0;          // levels used; nlev-1 solved densely
+  int32_t singular = 0;
+  std::vector<std::unique_ptr<FastLevel>> lev;
+  DBuf<double> ainv, colbuf;
+  // outer FPCG vectors (level-1, multicolour numbering)
+  DBuf<double> x, r, p, w;
+};
+
+struct System {
+  int32_t n = 0, nnz = 0;
+  std::vector<std::unique_ptr<RefLevel>> lev;
+  int32_t nlevels = 1;
+  bool hier_valid = false;
+  PcgState pcg;
+};
+
+struct ProfRec {
+  std::string kind;
+  hipEvent_t a, b;
+  double bytes;
+};
+
+}  // namespace ufvm
+
+struct ufvm_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::map<int32_t, std::unique_ptr<ufvm::System>> systems;
+  int32_t next_id = 1;
+  ufvm::DBuf<double> partial;   // per-block partial sums
+  ufvm::DBuf<double> scal;      // device scalars
+  double *hscal = nullptr;      // pinned host mirror of scal
+  bool prof_on = false;
+  std::vector<ufvm::ProfRec> prof;
+  std::vector<hipEvent_t> event_pool;
+};

@@ -1,0 +1,617 @@
+/*
+ * ufvm_oracle.c -- CPU restatement of the uFVM AMG pressure-solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY: the checker for the HIP implementation and the
+ * CPU baseline of bench.py.  Nothing in the product library links or calls it.
+ *
+ * Sequential FP64, reference orderings.  Build with -ffp-contract=off so that
+ * every a*b+c rounds twice, as the interpreted reference does.  Each function
+ * cites the reference file:line it restates (paths relative to uFVM/).
+ */
+#include "ufvm_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+void orc_level_alloc(orc_level *L, int32_t n, int32_t nnz) {
+  L->n = n;
+  L->nnz = nnz;
+  L->ptr = (int32_t *)calloc((size_t)n + 1, sizeof(int32_t));
+  L->col = (int32_t *)calloc(nnz > 0 ? (size_t)nnz : 1, sizeof(int32_t));
+  L->ac = (double *)calloc((size_t)n, sizeof(double));
+  L->anb = (double *)calloc(nnz > 0 ? (size_t)nnz : 1, sizeof(double));
+  L->bc = (double *)calloc((size_t)n, sizeof(double));
+  L->dphi = (double *)calloc((size_t)n, sizeof(double));
+  L->parents = NULL;
+}
+
+void orc_level_free(orc_level *L) {
+  free(L->ptr); free(L->col); free(L->ac); free(L->anb); free(L->bc); free(L->dphi);
+  free(L->parents);
+  memset(L, 0, sizeof(*L));
+}
+
+void orc_hier_free(orc_hier *H) {
+  /* level 0 is owned by the caller; coarse levels by the hierarchy */
+  for (int l = 1; l < H->nlevels; ++l) orc_level_free(&H->lev[l]);
+  free(H->lev[0].parents);
+  H->lev[0].parents = NULL;
+  H->nlevels = 1;
+}
+
+/* cfdComputeResidualsArray.m:11-17: r_i = bc_i - ac_i*phi_i, then minus
+ * anb_i[k]*phi[cconn_i[k]] in cconn order. */
+void orc_residual(const orc_level *L, double *r) {
+  for (int32_t i = 0; i < L->n; ++i) {
+    double s = L->bc[i] - L->ac[i] * L->dphi[i];
+    for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) s = s - L->anb[k] * L->dphi[L->col[k]];
+    r[i] = s;
+  }
+}
+
+/* --------------------------- agglomeration ------------------------------ */
+
+/* cfdAgglomerateLevel.m:3-104.  Greedy seed agglomeration:
+ *  maxAnb_i = max(-anb_i)                                            (:13-15)
+ *  seeds in index order take unassigned cconn neighbours with
+ *  -anb/maxAnb_seed > 0.5; first-generation children then take their own
+ *  unassigned neighbours with -anb/maxAnb_child > 0.5               (:20-46)
+ *  a seed that took nobody is released                               (:48-52)
+ *  orphans join the assigned neighbour maximising -anb_io/maxAnb_nb (strict >,
+ *  first wins), else become a new parent                             (:57-75)
+ *  coarse cconn in first-encounter order over fine rows/cconn        (:83-98) */
+int orc_agglomerate_level(orc_level *F, orc_level *C) {
+  const int32_t n = F->n;
+  int32_t *parents = (int32_t *)calloc((size_t)n, sizeof(int32_t)); /* 1-based, 0 = none */
+  double *maxAnb = (double *)malloc((size_t)n * sizeof(double));
+  int32_t *children = (int32_t *)malloc((size_t)(n + 1) * sizeof(int32_t));
+  for (int32_t i = 0; i < n; ++i) {
+    double m = -INFINITY;
+    for (int32_t k = F->ptr[i]; k < F->ptr[i + 1]; ++k) {
+      double v = -F->anb[k];
+      if (v > m) m = v;
+    }
+    maxAnb[i] = m;
+  }
+  int32_t iParent = 1;
+  for (int32_t s = 0; s < n; ++s) {
+    if (parents[s] != 0) continue;
+    parents[s] = iParent;
+    int32_t nch = 0;
+    children[nch++] = s;
+    for (int32_t k = F->ptr[s]; k < F->ptr[s + 1]; ++k) {
+      int32_t j = F->col[k];
+      if (parents[j] == 0 && (-F->anb[k] / maxAnb[s]) > 0.5) {
+        parents[j] = iParent;
+        children[nch++] = j;
+      }
+    }
+    int32_t n2 = 0;
+    for (int32_t c = 1; c < nch; ++c) {
+      int32_t ch = children[c];
+      for (int32_t k = F->ptr[ch]; k < F->ptr[ch + 1]; ++k) {
+        int32_t j = F->col[k];
+        if (parents[j] == 0 && (-F->anb[k] / maxAnb[ch]) > 0.5) {
+          parents[j] = iParent;
+          ++n2;
+        }
+      }
+    }
+    if (nch + n2 == 1) parents[s] = 0;
+    else ++iParent;
+  }
+  for (int32_t o = 0; o < n; ++o) {
+    if (parents[o] == 0) {
+      double strength = 0.0;
+      for (int32_t k = F->ptr[o]; k < F->ptr[o + 1]; ++k) {
+        int32_t j = F->col[k];
+        if (parents[j] != 0) {
+          double v = -F->anb[k] / maxAnb[j];
+          if (strength < v) {
+            strength = v;
+            parents[o] = parents[j];
+          }
+        }
+      }
+    }
+    if (parents[o] == 0) parents[o] = iParent++;
+  }
+  const int32_t nc = iParent - 1;
+  /* coarse cconn, first-encounter order (:85-94) */
+  int32_t *cnt = (int32_t *)calloc((size_t)nc, sizeof(int32_t));
+  int32_t *cap = (int32_t *)calloc((size_t)nc, sizeof(int32_t));
+  int32_t **lst = (int32_t **)calloc((size_t)nc, sizeof(int32_t *));
+  for (int32_t i = 0; i < n; ++i) {
+    int32_t p = parents[i] - 1;
+    for (int32_t k = F->ptr[i]; k < F->ptr[i + 1]; ++k) {
+      int32_t q = parents[F->col[k]] - 1;
+      if (p == q) continue;
+      int found = 0;
+      for (int32_t t = 0; t < cnt[p]; ++t)
+        if (lst[p][t] == q) { found = 1; break; }
+      if (!found) {
+        if (cnt[p] == cap[p]) {
+          cap[p] = cap[p] ? 2 * cap[p] : 8;
+          lst[p] = (int32_t *)realloc(lst[p], (size_t)cap[p] * sizeof(int32_t));
+        }
+        lst[p][cnt[p]++] = q;
+      }
+    }
+  }
+  int64_t nnz = 0;
+  for (int32_t p = 0; p < nc; ++p) nnz += cnt[p];
+  orc_level_alloc(C, nc, (int32_t)nnz);
+  C->ptr[0] = 0;
+  for (int32_t p = 0; p < nc; ++p) {
+    C->ptr[p + 1] = C->ptr[p] + cnt[p];
+    memcpy(C->col + C->ptr[p], lst[p], (size_t)cnt[p] * sizeof(int32_t));
+    free(lst[p]);
+  }
+  free(lst); free(cnt); free(cap);
+  free(F->parents);
+  F->parents = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+  for (int32_t i = 0; i < n; ++i) F->parents[i] = parents[i] - 1;
+  free(parents); free(maxAnb); free(children);
+  return nc;
+}
+
+/* assembleAgglomeratedLHS.m:16-30: for fine i in order, AC[p] += ac_i; for
+ * each neighbour in cconn order, same parent -> AC[p] += anb, else the coarse
+ * entry of parent(nb) in CCONN{p} (first match) += anb. */
+void orc_agglomerated_lhs(const orc_level *F, orc_level *C) {
+  memset(C->ac, 0, (size_t)C->n * sizeof(double));
+  memset(C->anb, 0, (size_t)(C->nnz > 0 ? C->nnz : 1) * sizeof(double));
+  for (int32_t i = 0; i < F->n; ++i) {
+    int32_t p = F->parents[i];
+    C->ac[p] = C->ac[p] + F->ac[i];
+    for (int32_t k = F->ptr[i]; k < F->ptr[i + 1]; ++k) {
+      int32_t q = F->parents[F->col[k]];
+      if (q == p) {
+        C->ac[p] = C->ac[p] + F->anb[k];
+      } else {
+        for (int32_t t = C->ptr[p]; t < C->ptr[p + 1]; ++t)
+          if (C->col[t] == q) { C->anb[t] = C->anb[t] + F->anb[k]; break; }
+      }
+    }
+  }
+}
+
+/* cfdAgglomerate.m:7-17: agglomerate level after level until a level has at
+ * most 5 parents or maxCoarseLevels coarse levels exist.  Returns the level
+ * count (the reference's maxLevels). */
+int orc_agglomerate(orc_hier *H, int maxCoarseLevels) {
+  int iLevel = 1;  /* 1-based as in the reference */
+  while (iLevel <= maxCoarseLevels && iLevel < 16) {
+    iLevel = iLevel + 1;
+    int np = orc_agglomerate_level(&H->lev[iLevel - 2], &H->lev[iLevel - 1]);
+    orc_agglomerated_lhs(&H->lev[iLevel - 2], &H->lev[iLevel - 1]);
+    H->nlevels = iLevel;
+    if (np <= 5) break;
+  }
+  H->nlevels = iLevel;
+  return iLevel;
+}
+
+/* cfdUpdateRHS.m:10-17: BC = zeros; BC(parent(i)) += r(i) for fine i in order */
+void orc_update_rhs(const orc_level *F, const double *r, orc_level *C) {
+  memset(C->bc, 0, (size_t)C->n * sizeof(double));
+  for (int32_t i = 0; i < F->n; ++i) C->bc[F->parents[i]] = C->bc[F->parents[i]] + r[i];
+}
+
+/* cfdCorrectFinerLevelSolution.m:11-14 */
+void orc_correct_finer(orc_level *F, const orc_level *C) {
+  for (int32_t i = 0; i < F->n; ++i) F->dphi[i] = F->dphi[i] + C->dphi[F->parents[i]];
+}
+
+/* ------------------------------ smoothers ------------------------------- */
+
+static void gs_row(const orc_level *L, const double *b, double *x, int32_t i) {
+  double s = b[i];
+  for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) s = s - L->anb[k] * x[L->col[k]];
+  x[i] = s / L->ac[i];
+}
+
+/* cfdSolveSOR.m:13-29: forward then backward lexicographic Gauss-Seidel */
+void orc_sor(const orc_level *L, double *dphi) {
+  for (int32_t i = 0; i < L->n; ++i) gs_row(L, L->bc, dphi, i);
+  for (int32_t i = L->n - 1; i >= 0; --i) gs_row(L, L->bc, dphi, i);
+}
+
+static int32_t find_in_row(const orc_level *L, int32_t j, int32_t i) {
+  for (int32_t t = L->ptr[j]; t < L->ptr[j + 1]; ++t)
+    if (L->col[t] == i) return t;
+  return -1;
+}
+
+/* cfdFactorizeILU.m:10-47 (DILU).  The update loop is skipped for the row
+ * whose 1-based index is numberOfElements-1 (:21), kept as is. */
+void orc_factorize_ilu(const orc_level *L, double *dc, double *rc) {
+  const int32_t n = L->n;
+  for (int32_t i = 0; i < n; ++i) dc[i] = L->ac[i];
+  for (int32_t i = 0; i < n; ++i) {
+    dc[i] = 1.0 / dc[i];
+    rc[i] = L->bc[i];
+    if (i + 1 != n - 1) {
+      for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) {
+        int32_t j = L->col[k];
+        if (j > i) {
+          int32_t t = find_in_row(L, j, i);
+          if (t >= 0) dc[j] = dc[j] - (L->anb[t] * dc[i]) * L->anb[k];
+          else fprintf(stderr, "the index for i in j is not found\n");
+        }
+      }
+    }
+  }
+}
+
+/* cfdSolveILU.m:14-93: residual, forward DILU substitution, backward
+ * substitution with dphi += D*rc. */
+void orc_solve_ilu(const orc_level *L, const double *dc, double *rc, double *dphi) {
+  const int32_t n = L->n;
+  for (int32_t i = 0; i < n; ++i) {
+    double s = -L->ac[i] * dphi[i] + L->bc[i];
+    for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) s = s - L->anb[k] * dphi[L->col[k]];
+    rc[i] = s;
+  }
+  for (int32_t i = 0; i < n; ++i) {
+    double mat1 = dc[i] * rc[i];
+    for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) {
+      int32_t j = L->col[k];
+      if (j > i) {
+        int32_t t = find_in_row(L, j, i);
+        if (t >= 0) rc[j] = rc[j] - L->anb[t] * mat1;
+        else fprintf(stderr, "ILU Solver Error The index for i in element j is not found\n");
+      }
+    }
+  }
+  for (int32_t i = n - 1; i >= 0; --i) {
+    if (i < n - 1) {
+      for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) {
+        int32_t j = L->col[k];
+        if (j > i) rc[i] = rc[i] - L->anb[k] * rc[j];
+      }
+    }
+    double mat1 = dc[i] * rc[i];
+    rc[i] = mat1;
+    dphi[i] = dphi[i] + mat1;
+  }
+}
+
+static double l1(const double *r, int32_t n) {
+  double s = 0.0;
+  for (int32_t i = 0; i < n; ++i) s = s + fabs(r[i]);
+  return s;
+}
+
+/* cfdSolveAlgebraicSystem.m:11-66 */
+void orc_solve_algebraic_system(orc_level *L, int smoother, int nIter, double rrf, double *r0,
+                                double *r1) {
+  const int32_t n = L->n;
+  double *r = (double *)malloc((size_t)n * sizeof(double));
+  orc_residual(L, r);
+  double initial = l1(r, n) / n;
+  double final_ = initial;
+  if (smoother == 0) {
+    double *dc = (double *)malloc((size_t)n * sizeof(double));
+    double *rc = (double *)malloc((size_t)n * sizeof(double));
+    orc_factorize_ilu(L, dc, rc);
+    int iter = 1;
+    while (iter <= nIter && final_ > rrf * initial) {
+      orc_solve_ilu(L, dc, rc, L->dphi);
+      orc_residual(L, r);
+      final_ = l1(r, n) / n;
+      ++iter;
+    }
+    free(dc); free(rc);
+  } else {
+    int iter = 1;
+    while (iter <= nIter && final_ > rrf * initial) {
+      orc_sor(L, L->dphi);
+      orc_residual(L, r);
+      final_ = l1(r, n) / n;
+      ++iter;
+    }
+  }
+  free(r);
+  if (r0) *r0 = initial;
+  if (r1) *r1 = final_;
+}
+
+/* -------------------------------- cycles -------------------------------- */
+
+typedef struct {
+  orc_hier *H;
+  int smoother, pre, post;
+  double rrf;
+} cyc_t;
+
+/* cfdRestrict.m:5-12 (level is 1-based as in the reference) */
+static void restrict_(cyc_t *c, int level, int presmooth) {
+  orc_level *L = &c->H->lev[level - 1];
+  if (presmooth) orc_solve_algebraic_system(L, c->smoother, c->pre, 0.1, NULL, NULL);
+  double *r = (double *)malloc((size_t)L->n * sizeof(double));
+  orc_residual(L, r);
+  orc_update_rhs(L, r, &c->H->lev[level]);
+  free(r);
+}
+
+/* cfdProlongate.m:5-6 */
+static void prolongate_(cyc_t *c, int level) {
+  orc_correct_finer(&c->H->lev[level - 2], &c->H->lev[level - 1]);
+  orc_solve_algebraic_system(&c->H->lev[level - 2], c->smoother, c->post, c->rrf, NULL, NULL);
+}
+
+static void coarsest_(cyc_t *c, int level) {
+  orc_solve_algebraic_system(&c->H->lev[level - 1], c->smoother, c->post, 0.1, NULL, NULL);
+}
+
+static double final_res(cyc_t *c) {
+  orc_level *L = &c->H->lev[0];
+  double *r = (double *)malloc((size_t)L->n * sizeof(double));
+  orc_residual(L, r);
+  double s = l1(r, L->n);
+  free(r);
+  return s;
+}
+
+/* cfdApplyVCycle.m:5-25 */
+static double vcycle_(cyc_t *c, int G) {
+  int g = 1;
+  while (g < G) { restrict_(c, g, 1); ++g; }
+  coarsest_(c, g);
+  while (g > 1) { prolongate_(c, g); --g; }
+  return final_res(c);
+}
+
+/* cfdApplyWCycle.m:3-83 */
+static double wcycle_(cyc_t *c, int G) {
+  int g = 1;
+  while (g < G) { restrict_(c, g, 1); ++g; }
+  coarsest_(c, g);
+  prolongate_(c, g);
+  restrict_(c, g - 1, 0);
+  coarsest_(c, g);
+  prolongate_(c, g);
+  prolongate_(c, g - 1);
+  restrict_(c, g - 2, 0);
+  restrict_(c, g - 1, 1);
+  coarsest_(c, g);
+  prolongate_(c, g);
+  prolongate_(c, g - 1);
+  prolongate_(c, g - 2);
+  restrict_(c, g - 3, 0);
+  restrict_(c, g - 2, 1);
+  restrict_(c, g - 1, 1);
+  coarsest_(c, g);
+  prolongate_(c, g);
+  prolongate_(c, g - 1);
+  restrict_(c, g - 2, 0);
+  restrict_(c, g - 1, 1);
+  coarsest_(c, g);
+  prolongate_(c, g);
+  restrict_(c, g - 1, 0);
+  coarsest_(c, g);
+  while (g > 1) { prolongate_(c, g); --g; }
+  return final_res(c);
+}
+
+/* cfdApplyFCycle.m:5-63 */
+static double fcycle_(cyc_t *c, int G) {
+  int g = 1;
+  while (g < G) { restrict_(c, g, 1); ++g; }
+  coarsest_(c, g);
+  prolongate_(c, g);
+  restrict_(c, g - 1, 0);
+  coarsest_(c, g);
+  prolongate_(c, g);
+  prolongate_(c, g - 1);
+  restrict_(c, g - 2, 0);
+  restrict_(c, g - 1, 1);
+  coarsest_(c, g);
+  prolongate_(c, g);
+  prolongate_(c, g - 1);
+  prolongate_(c, g - 2);
+  restrict_(c, g - 3, 0);
+  restrict_(c, g - 2, 1);
+  restrict_(c, g - 1, 1);
+  coarsest_(c, g);
+  while (g > 1) { prolongate_(c, g); --g; }
+  return final_res(c);
+}
+
+/* cfdApplyAMG.m:23-60 (the hierarchy is built beforehand by orc_agglomerate,
+ * which is what cfdApplyAMG.m:19 calls).  *ncycles receives the reference's
+ * nCycle counter, i.e. cycles performed + 1 (:37,59). */
+void orc_apply_amg(orc_hier *H, int cycle, int maxCycles, double rrf, int pre, int post,
+                   int smoother, double *r0, double *r1, int *ncycles) {
+  cyc_t c = {H, smoother, pre, post, rrf};
+  if (cycle != 0 && H->nlevels < 4) {
+    /* the hand-unrolled W/F patterns address level maxLevels-3 (1-based) */
+    if (ncycles) *ncycles = -1;
+    return;
+  }
+  double initial = final_res(&c);
+  double final_ = initial;
+  int nCycle = 1;
+  const int G = H->nlevels;
+  while (nCycle <= maxCycles && final_ > rrf * initial) {
+    if (cycle == 0) final_ = vcycle_(&c, G);
+    else if (cycle == 1) final_ = wcycle_(&c, G);
+    else final_ = fcycle_(&c, G);
+    ++nCycle;
+  }
+  if (r0) *r0 = initial;
+  if (r1) *r1 = final_;
+  if (ncycles) *ncycles = nCycle;
+}
+
+/* ------------------------- build-only: FPCG path ------------------------- */
+
+int orc_colour(int32_t n, const int32_t *ptr, const int32_t *col, int32_t *colour) {
+  int32_t ncol = 0;
+  int32_t *mark = (int32_t *)malloc(sizeof(int32_t) * 1024);
+  for (int t = 0; t < 1024; ++t) mark[t] = -1;
+  for (int32_t i = 0; i < n; ++i) colour[i] = -1;
+  for (int32_t i = 0; i < n; ++i) {
+    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+      int32_t c = colour[col[k]];
+      if (c >= 0 && c < 1024) mark[c] = i;
+    }
+    int32_t c = 0;
+    while (c < 1024 && mark[c] == i) ++c;
+    colour[i] = c;
+    if (c + 1 > ncol) ncol = c + 1;
+  }
+  free(mark);
+  return ncol;
+}
+
+void orc_mcgs(const orc_level *L, const int32_t *colour, int ncolours, const double *b,
+              double *x) {
+  for (int c = 0; c < ncolours; ++c)
+    for (int32_t i = 0; i < L->n; ++i)
+      if (colour[i] == c) gs_row(L, b, x, i);
+  for (int c = ncolours - 1; c >= 0; --c)
+    for (int32_t i = 0; i < L->n; ++i)
+      if (colour[i] == c) gs_row(L, b, x, i);
+}
+
+/* dense LU with partial pivoting, in place on a (row-major n x n) */
+static void dense_solve(int32_t n, double *a, double *b) {
+  int32_t *piv = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+  for (int32_t k = 0; k < n; ++k) {
+    int32_t p = k;
+    for (int32_t i = k + 1; i < n; ++i)
+      if (fabs(a[(size_t)i * n + k]) > fabs(a[(size_t)p * n + k])) p = i;
+    piv[k] = p;
+    if (p != k)
+      for (int32_t j = 0; j < n; ++j) {
+        double t = a[(size_t)k * n + j];
+        a[(size_t)k * n + j] = a[(size_t)p * n + j];
+        a[(size_t)p * n + j] = t;
+      }
+    double t = b[k]; b[k] = b[p]; b[p] = t;
+    for (int32_t i = k + 1; i < n; ++i) {
+      double l = a[(size_t)i * n + k] / a[(size_t)k * n + k];
+      for (int32_t j = k + 1; j < n; ++j) a[(size_t)i * n + j] -= l * a[(size_t)k * n + j];
+      b[i] -= l * b[k];
+    }
+  }
+  for (int32_t i = n - 1; i >= 0; --i) {
+    double s = b[i];
+    for (int32_t j = i + 1; j < n; ++j) s -= a[(size_t)i * n + j] * b[j];
+    b[i] = s / a[(size_t)i * n + i];
+  }
+  free(piv);
+}
+
+static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol,
+                     const orc_vcfg *cfg, int l, const double *b, double *x) {
+  const orc_level *L = &H->lev[l];
+  const int32_t n = L->n;
+  if (l == cfg->nlev - 1) {
+    double *a = (double *)calloc((size_t)n * n, sizeof(double));
+    double shift = 0.0;
+    if (cfg->singular) {
+      for (int32_t i = 0; i < n; ++i) shift += L->ac[i];
+      shift = shift / ((double)n * n);
+    }
+    for (int32_t i = 0; i < n; ++i) {
+      for (int32_t j = 0; j < n; ++j) a[(size_t)i * n + j] = shift;
+      a[(size_t)i * n + i] += L->ac[i];
+      for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) a[(size_t)i * n + L->col[k]] += L->anb[k];
+    }
+    memcpy(x, b, (size_t)n * sizeof(double));
+    dense_solve(n, a, x);
+    free(a);
+    return;
+  }
+  memset(x, 0, (size_t)n * sizeof(double));
+  for (int s = 0; s < cfg->pre; ++s) orc_mcgs(L, colours[l], ncol[l], b, x);
+  double *r = (double *)malloc((size_t)n * sizeof(double));
+  for (int32_t i = 0; i < n; ++i) {
+    double s = b[i] - L->ac[i] * x[i];
+    for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) s = s - L->anb[k] * x[L->col[k]];
+    r[i] = s;
+  }
+  const orc_level *C = &H->lev[l + 1];
+  double *bc = (double *)calloc((size_t)C->n, sizeof(double));
+  double *xc = (double *)calloc((size_t)C->n, sizeof(double));
+  for (int32_t i = 0; i < n; ++i) bc[L->parents[i]] = bc[L->parents[i]] + r[i];
+  vcyc_rec(H, colours, ncol, cfg, l + 1, bc, xc);
+  for (int32_t i = 0; i < n; ++i) x[i] = x[i] + xc[L->parents[i]];
+  for (int s = 0; s < cfg->post; ++s) orc_mcgs(L, colours[l], ncol[l], b, x);
+  free(r); free(bc); free(xc);
+}
+
+void orc_vcycle(const orc_hier *H, int32_t *const *colours, const int *ncolours,
+                const orc_vcfg *cfg, const double *b, double *x) {
+  vcyc_rec(H, colours, ncolours, cfg, 0, b, x);
+}
+
+static double dotv(const double *a, const double *b, int32_t n) {
+  double s = 0.0;
+  for (int32_t i = 0; i < n; ++i) s += a[i] * b[i];
+  return s;
+}
+
+static void project(double *v, int32_t n) {
+  double m = 0.0;
+  for (int32_t i = 0; i < n; ++i) m += v[i];
+  m /= n;
+  for (int32_t i = 0; i < n; ++i) v[i] -= m;
+}
+
+static void spmv(const orc_level *L, const double *x, double *y) {
+  for (int32_t i = 0; i < L->n; ++i) {
+    double s = L->ac[i] * x[i];
+    for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) s = s + L->anb[k] * x[L->col[k]];
+    y[i] = s;
+  }
+}
+
+int orc_fpcg(const orc_hier *H, int32_t *const *colours, const int *ncolours,
+             const orc_vcfg *cfg, const double *b, double tol, int maxit, double *x,
+             double *relres) {
+  const orc_level *L = &H->lev[0];
+  const int32_t n = L->n;
+  double *r = (double *)malloc((size_t)n * sizeof(double));
+  double *z = (double *)malloc((size_t)n * sizeof(double));
+  double *p = (double *)malloc((size_t)n * sizeof(double));
+  double *w = (double *)malloc((size_t)n * sizeof(double));
+  memcpy(r, b, (size_t)n * sizeof(double));
+  if (cfg->singular) project(r, n);
+  memset(x, 0, (size_t)n * sizeof(double));
+  double nb = sqrt(dotv(r, r, n));
+  int it = 0;
+  double rr = nb;
+  if (nb == 0.0) goto done;
+  double delta = 1.0;
+  while (it < maxit) {
+    orc_vcycle(H, colours, ncolours, cfg, r, z);
+    if (cfg->singular) project(z, n);
+    if (it == 0) {
+      memcpy(p, z, (size_t)n * sizeof(double));
+    } else {
+      double beta = dotv(z, w, n) / delta;
+      for (int32_t i = 0; i < n; ++i) p[i] = z[i] - beta * p[i];
+    }
+    double rho = dotv(z, r, n);
+    spmv(L, p, w);
+    delta = dotv(p, w, n);
+    double alpha = rho / delta;
+    for (int32_t i = 0; i < n; ++i) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * w[i];
+    }
+    ++it;
+    rr = sqrt(dotv(r, r, n));
+    if (rr <= tol * nb) break;
+  }
+done:
+  if (relres) *relres = nb > 0 ? rr / nb : 0.0;
+  free(r); free(z); free(p); free(w);
+  return it;
+}
