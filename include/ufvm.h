@@ -54,6 +54,7 @@ int ufvm_ctx_destroy(ufvm_ctx *ctx);
 const char *ufvm_last_error(const ufvm_ctx *ctx);
 /* the context's hipStream_t (for event timing by a host that shares it) */
 void *ufvm_ctx_stream(ufvm_ctx *ctx);
+/* wait for the context stream and for the whole device */
 int ufvm_ctx_synchronize(ufvm_ctx *ctx);
 
 /* ---------------------------------------------------------- linear system */

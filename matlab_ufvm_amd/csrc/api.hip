@@ -637,7 +637,10 @@ const char *ufvm_last_error(const ufvm_ctx *ctx) { return ctx ? ctx->err.c_str()
 void *ufvm_ctx_stream(ufvm_ctx *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
 int ufvm_ctx_synchronize(ufvm_ctx *ctx) {
-  return guard(ctx, [&] { UFVM_HIP(hipStreamSynchronize(ctx->stream)); });
+  return guard(ctx, [&] {
+    UFVM_HIP(hipStreamSynchronize(ctx->stream));
+    UFVM_HIP(hipDeviceSynchronize());
+  });
 }
 
 int ufvm_system_create(ufvm_ctx *ctx, int32_t n, int32_t nnz, const int32_t *ptr,

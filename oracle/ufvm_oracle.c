@@ -478,54 +478,81 @@ void orc_mcgs(const orc_level *L, const int32_t *colour, int ncolours, const dou
       if (colour[i] == c) gs_row(L, b, x, i);
 }
 
-/* dense LU with partial pivoting, in place on a (row-major n x n) */
-static void dense_solve(int32_t n, double *a, double *b) {
-  int32_t *piv = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+/* dense LU with partial pivoting of a (row-major n x n), in place */
+typedef struct {
+  int32_t n;
+  double *a;
+  int32_t *piv;
+} orc_lu;
+
+static void lu_factor(orc_lu *F) {
+  const int32_t n = F->n;
+  double *a = F->a;
   for (int32_t k = 0; k < n; ++k) {
     int32_t p = k;
     for (int32_t i = k + 1; i < n; ++i)
       if (fabs(a[(size_t)i * n + k]) > fabs(a[(size_t)p * n + k])) p = i;
-    piv[k] = p;
+    F->piv[k] = p;
     if (p != k)
       for (int32_t j = 0; j < n; ++j) {
         double t = a[(size_t)k * n + j];
         a[(size_t)k * n + j] = a[(size_t)p * n + j];
         a[(size_t)p * n + j] = t;
       }
-    double t = b[k]; b[k] = b[p]; b[p] = t;
     for (int32_t i = k + 1; i < n; ++i) {
       double l = a[(size_t)i * n + k] / a[(size_t)k * n + k];
+      a[(size_t)i * n + k] = l;
       for (int32_t j = k + 1; j < n; ++j) a[(size_t)i * n + j] -= l * a[(size_t)k * n + j];
-      b[i] -= l * b[k];
     }
+  }
+}
+
+static void lu_solve(const orc_lu *F, double *b) {
+  const int32_t n = F->n;
+  const double *a = F->a;
+  for (int32_t k = 0; k < n; ++k) {
+    int32_t p = F->piv[k];
+    double t = b[k]; b[k] = b[p]; b[p] = t;
+    for (int32_t i = k + 1; i < n; ++i) b[i] -= a[(size_t)i * n + k] * b[k];
   }
   for (int32_t i = n - 1; i >= 0; --i) {
     double s = b[i];
     for (int32_t j = i + 1; j < n; ++j) s -= a[(size_t)i * n + j] * b[j];
     b[i] = s / a[(size_t)i * n + i];
   }
-  free(piv);
+}
+
+/* coarsest operator (+ mean(diag)/n * 11^T when singular), factored */
+static void lu_build(const orc_level *L, int singular, orc_lu *F) {
+  const int32_t n = L->n;
+  F->n = n;
+  F->a = (double *)calloc((size_t)n * n, sizeof(double));
+  F->piv = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+  double shift = 0.0;
+  if (singular) {
+    for (int32_t i = 0; i < n; ++i) shift += L->ac[i];
+    shift = shift / ((double)n * n);
+  }
+  for (int32_t i = 0; i < n; ++i) {
+    for (int32_t j = 0; j < n; ++j) F->a[(size_t)i * n + j] = shift;
+    F->a[(size_t)i * n + i] += L->ac[i];
+    for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) F->a[(size_t)i * n + L->col[k]] += L->anb[k];
+  }
+  lu_factor(F);
+}
+
+static void lu_free(orc_lu *F) {
+  free(F->a);
+  free(F->piv);
 }
 
 static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol,
-                     const orc_vcfg *cfg, int l, const double *b, double *x) {
+                     const orc_vcfg *cfg, const orc_lu *lu, int l, const double *b, double *x) {
   const orc_level *L = &H->lev[l];
   const int32_t n = L->n;
   if (l == cfg->nlev - 1) {
-    double *a = (double *)calloc((size_t)n * n, sizeof(double));
-    double shift = 0.0;
-    if (cfg->singular) {
-      for (int32_t i = 0; i < n; ++i) shift += L->ac[i];
-      shift = shift / ((double)n * n);
-    }
-    for (int32_t i = 0; i < n; ++i) {
-      for (int32_t j = 0; j < n; ++j) a[(size_t)i * n + j] = shift;
-      a[(size_t)i * n + i] += L->ac[i];
-      for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) a[(size_t)i * n + L->col[k]] += L->anb[k];
-    }
     memcpy(x, b, (size_t)n * sizeof(double));
-    dense_solve(n, a, x);
-    free(a);
+    lu_solve(lu, x);
     return;
   }
   memset(x, 0, (size_t)n * sizeof(double));
@@ -540,7 +567,7 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
   double *bc = (double *)calloc((size_t)C->n, sizeof(double));
   double *xc = (double *)calloc((size_t)C->n, sizeof(double));
   for (int32_t i = 0; i < n; ++i) bc[L->parents[i]] = bc[L->parents[i]] + r[i];
-  vcyc_rec(H, colours, ncol, cfg, l + 1, bc, xc);
+  vcyc_rec(H, colours, ncol, cfg, lu, l + 1, bc, xc);
   for (int32_t i = 0; i < n; ++i) x[i] = x[i] + xc[L->parents[i]];
   for (int s = 0; s < cfg->post; ++s) orc_mcgs(L, colours[l], ncol[l], b, x);
   free(r); free(bc); free(xc);
@@ -548,7 +575,10 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
 
 void orc_vcycle(const orc_hier *H, int32_t *const *colours, const int *ncolours,
                 const orc_vcfg *cfg, const double *b, double *x) {
-  vcyc_rec(H, colours, ncolours, cfg, 0, b, x);
+  orc_lu lu;
+  lu_build(&H->lev[cfg->nlev - 1], cfg->singular, &lu);
+  vcyc_rec(H, colours, ncolours, cfg, &lu, 0, b, x);
+  lu_free(&lu);
 }
 
 static double dotv(const double *a, const double *b, int32_t n) {
@@ -589,8 +619,10 @@ int orc_fpcg(const orc_hier *H, int32_t *const *colours, const int *ncolours,
   double rr = nb;
   if (nb == 0.0) goto done;
   double delta = 1.0;
+  orc_lu lu;
+  lu_build(&H->lev[cfg->nlev - 1], cfg->singular, &lu);
   while (it < maxit) {
-    orc_vcycle(H, colours, ncolours, cfg, r, z);
+    vcyc_rec(H, colours, ncolours, cfg, &lu, 0, r, z);
     if (cfg->singular) project(z, n);
     if (it == 0) {
       memcpy(p, z, (size_t)n * sizeof(double));
@@ -610,6 +642,7 @@ int orc_fpcg(const orc_hier *H, int32_t *const *colours, const int *ncolours,
     rr = sqrt(dotv(r, r, n));
     if (rr <= tol * nb) break;
   }
+  lu_free(&lu);
 done:
   if (relres) *relres = nb > 0 ? rr / nb : 0.0;
   free(r); free(z); free(p); free(w);
