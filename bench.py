@@ -66,7 +66,7 @@ def timed_solves(ctx, sid, steps, warmup, prof=False):
     return total, np.array(lat), its, rr
 
 
-def cpu_baseline(ptr, col, ac, anb, b, max_coarse):
+def cpu_baseline(ptr, col, ac, anb, b, max_coarse, coarse_smoother):
     """The oracle (sequential C, FP64 V-cycle FPCG, same algorithm) on one core:
     one full solve of the same 2.1M-cell system; hierarchy set-up not timed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -76,7 +76,8 @@ def cpu_baseline(ptr, col, ac, anb, b, max_coarse):
     S.agglomerate(10)
     t_setup = time.time() - t
     t = time.time()
-    x, it, rr = S.fpcg(b, tol=1e-8, max_coarse_cells=max_coarse, singular=1)
+    x, it, rr = S.fpcg(b, tol=1e-8, max_coarse_cells=max_coarse, singular=1,
+                       coarse_smoother=coarse_smoother)
     dt = time.time() - t
     log(f"cpu oracle: setup {t_setup:.1f}s, solve {dt:.1f}s, {it} iterations, relres {rr:.1e}")
     return {"value": 1.0 / dt, "unit": "solves/s", "cores": 1, "kind": "port",
@@ -93,6 +94,8 @@ def main():
     ap.add_argument("--pre", type=int, default=1)
     ap.add_argument("--post", type=int, default=1)
     ap.add_argument("--max-coarse", type=int, default=2000)
+    ap.add_argument("--coarse-smoother", type=int, default=1,
+                    help="levels below the first: 0 block-local MCGS, 1 damped Jacobi")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     args = ap.parse_args()
@@ -116,7 +119,8 @@ def main():
     for prec in ([32] if args.no_fp64 else [32, 64]):
         t = time.time()
         ctx.pcg_setup(sid, precision=prec, pre_sweep=args.pre, post_sweep=args.post,
-                      max_coarse_cells=args.max_coarse, tol=1e-8, max_iter=500)
+                      max_coarse_cells=args.max_coarse, tol=1e-8, max_iter=500,
+                      coarse_smoother=args.coarse_smoother)
         t_setup = time.time() - t
         total, lat, its, rr = timed_solves(ctx, sid, args.steps, args.warmup, prof=(prec == 32))
         x = ctx.get_dphi(sid, 0)
@@ -132,7 +136,8 @@ def main():
     # field error of each solve against a tight FP64 solve (shifted to p'(0) = 0 as
     # the pressure correction is, cfdCorrectPPField.m:44-50)
     ctx.pcg_setup(sid, precision=64, pre_sweep=args.pre, post_sweep=args.post,
-                  max_coarse_cells=args.max_coarse, tol=1e-12, max_iter=1000)
+                  max_coarse_cells=args.max_coarse, tol=1e-12, max_iter=1000,
+                  coarse_smoother=args.coarse_smoother)
     ctx.pcg_solve(sid)
     x_tight = ctx.get_dphi(sid, 0)
     d_ref = x_tight - x_tight[0]
@@ -158,6 +163,7 @@ def main():
                    "cells": int(m.n_cells), "faces": int(m.n_faces), "nnz_offdiag": int(col.size),
                    "tol": 1e-8, "amg_levels": sizes, "pre_sweep": args.pre,
                    "post_sweep": args.post, "max_coarse_cells": args.max_coarse,
+                   "coarse_smoother": ["block-MCGS", "jacobi"][args.coarse_smoother],
                    "hierarchy_cached": True},
         "latency_ms": {"p50": round(float(np.median(r32["lat"])) * 1e3, 3),
                        "p99": round(float(np.percentile(r32["lat"], 99)) * 1e3, 3)},
@@ -180,7 +186,7 @@ def main():
                            "field_rel_l2_err": ferr(r64["x"])}
         out["speedup_fp32_vs_fp64_pcg"] = round(value / v64, 3)
     if not args.no_cpu:
-        cb, x_cpu = cpu_baseline(ptr, col, ac, anb, b, args.max_coarse)
+        cb, x_cpu = cpu_baseline(ptr, col, ac, anb, b, args.max_coarse, args.coarse_smoother)
         cb["field_rel_l2_err_vs_gpu_tight"] = ferr(x_cpu)
         out["cpu_baseline"] = cb
     else:

@@ -138,8 +138,11 @@ int ufvm_apply_amg(ufvm_ctx *ctx, int32_t sys, const ufvm_amg_cfg *cfg, double *
  * max_coarse_cells cells. */
 typedef struct {
   int32_t precision;         /* 32: FP32 V-cycle, 64: FP64 V-cycle */
-  int32_t pre_sweep;         /* symmetric MCGS sweeps before restriction */
-  int32_t post_sweep;        /* symmetric MCGS sweeps after prolongation */
+  int32_t pre_sweep;         /* smoothing sweeps before restriction */
+  int32_t post_sweep;        /* smoothing sweeps after prolongation */
+  int32_t coarse_smoother;   /* levels below the first: 0 block-local multicolour GS,
+                                1 damped Jacobi (omega = 2/3); level 1 always uses
+                                symmetric multicolour GS */
   int32_t max_coarse_cells;  /* dense-solve threshold (<= 4096) */
   int32_t nullspace;         /* -1 auto-detect (zero row sums), 0 no, 1 constant nullspace */
   int32_t max_iter;
@@ -157,6 +160,79 @@ int ufvm_pcg_apply_vcycle(ufvm_ctx *ctx, int32_t sys, const double *r, double *z
 /* number of colours of a level's multicolour ordering (after ufvm_pcg_setup) */
 int ufvm_pcg_level_colours(ufvm_ctx *ctx, int32_t sys, int32_t level, int32_t *ncolours,
                            int32_t *colour);
+
+
+/* ------------------------------------------------- SIMPLE outer iteration */
+/* Processed mesh (the reference's theMesh after cfdProcessOpenFoamMesh,
+ * uFVM/src/utilities/IO/cfdProcessOpenFoamMesh.m:15-241), 0-based, vectors as
+ * [face][3] / [cell][3].  Interior faces come first and are owner < neighbour;
+ * boundary faces follow, grouped by patch. */
+#define UFVM_PATCH_WALL 0
+#define UFVM_PATCH_EMPTY 1
+#define UFVM_BC_FIXED_VALUE 0
+#define UFVM_BC_ZERO_GRADIENT 1
+#define UFVM_BC_EMPTY 2
+typedef struct {
+  int32_t n_cells, n_faces, n_interior;
+  const int32_t *owner, *neighbour;
+  const double *Sf, *area, *CN, *eCN, *gDiff, *T, *gf, *face_centroid;
+  const double *volume, *centroid;
+  const int32_t *cconn_ptr, *cconn;     /* theMesh.cconn as CSR, reference order */
+  const int32_t *own_nnz, *nei_nnz;     /* iOwnerNeighbourCoef / iNeighbourOwnerCoef as CSR slots */
+  int32_t n_patches;
+  const int32_t *patch_start, *patch_nfaces, *patch_type;   /* UFVM_PATCH_* */
+} ufvm_mesh_desc;
+
+/* Steady incompressible SIMPLE settings: theEquation fields of U and p
+ * (uFVM/src/fvm/top/cfdDefineEquation.m:84-136), fvSolution SIMPLE block and
+ * the per-patch boundary conditions (U: fixedValue with a value, or empty;
+ * p: zeroGradient or empty).  Pressure-correction solver: 0 = cfdApplyAMG
+ * (stationary cycles, hierarchy rebuilt every solve as the reference does),
+ * 1 = AMG-PCG with the hierarchy rebuilt every solve, 2 = AMG-PCG with the
+ * hierarchy of the first solve kept. */
+typedef struct {
+  double nu, urf_u, urf_p;
+  int32_t n_correctors;
+  int32_t p_ref_cell;                     /* 0-based, -1 none (cfdReadSystem.m:171-180) */
+  int32_t u_sweeps;                       /* ILU sweeps of each momentum solve (10) */
+  const int32_t *u_bc;                    /* [n_patches] UFVM_BC_* */
+  const double *u_value;                  /* [n_patches][3] */
+  const int32_t *p_bc;                    /* [n_patches] UFVM_BC_* */
+  int32_t p_solver;
+  ufvm_amg_cfg p_amg;
+  ufvm_pcg_cfg p_pcg;
+} ufvm_simple_cfg;
+
+/* Upload a mesh and create its fields (uniform zero initial conditions,
+ * boundary values and mdot_f initialised as cfdInitializeCase.m:17-83). */
+int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *mesh, const ufvm_simple_cfg *cfg,
+                     int32_t *case_out);
+int ufvm_case_destroy(ufvm_ctx *ctx, int32_t cas);
+/* Field access (cfdGetMeshField / cfdSetMeshField.m): "U", "p", "PP", "DU",
+ * "DUT", "mdot_f", "gradU", "gradp", "gradPP", "gamma_f".  count must equal
+ * the field size. */
+int ufvm_case_set_field(ufvm_ctx *ctx, int32_t cas, const char *name, const double *data,
+                        int64_t count);
+int ufvm_case_get_field(ufvm_ctx *ctx, int32_t cas, const char *name, double *data,
+                        int64_t count);
+/* the linear system behind equation "U" or "p" (Domain.coefficients{1}) */
+int ufvm_case_system(ufvm_ctx *ctx, int32_t cas, const char *eq, int32_t *sys_out);
+/* cfdUpdateFields (uFVM/src/fvm/update/cfdUpdateEquations.m:14-24) */
+int ufvm_update_fields(ufvm_ctx *ctx, int32_t cas);
+/* [rms, max] = cfdAssembleEquation(eq, comp) (cfdAssembleEquation.m:1-27);
+ * comp is 0-based; "p" assembles the Rhie-Chow fluxes and the
+ * pressure-correction matrix (cfdAssembleMdotTerm.m:38-62). */
+int ufvm_assemble_equation(ufvm_ctx *ctx, int32_t cas, const char *eq, int32_t comp,
+                           double *rms, double *max);
+/* [r0, r1] = cfdSolveEquation(eq) (cfdSolveEquation.m:11-17); *ncycles gets
+ * the AMG cycle counter (or the PCG iteration count). */
+int ufvm_solve_equation(ufvm_ctx *ctx, int32_t cas, const char *eq, double *r0, double *r1,
+                        int32_t *ncycles);
+/* cfdCorrectEquation(eq, comp) (cfdCorrectEquation.m:17-51) */
+int ufvm_correct_equation(ufvm_ctx *ctx, int32_t cas, const char *eq, int32_t comp);
+/* n steady outer iterations of cfdRunCase (cfdRunCase.m:92-142); rms (may be
+ * NULL) receives 4 values per iteration: RMS residuals of Ux, Uy, Uz, p. */
+int ufvm_simple_iterate(ufvm_ctx *ctx, int32_t cas, int32_t n, double *rms);
 
 /* ------------------------------------------------------------- profiling */
 /* Per-kernel-class HIP-event timing on the context stream.  kind names:

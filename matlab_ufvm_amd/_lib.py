@@ -31,8 +31,27 @@ class AmgCfg(C.Structure):
 
 class PcgCfg(C.Structure):
     _fields_ = [("precision", C.c_int32), ("pre_sweep", C.c_int32), ("post_sweep", C.c_int32),
+                ("coarse_smoother", C.c_int32),
                 ("max_coarse_cells", C.c_int32), ("nullspace", C.c_int32),
                 ("max_iter", C.c_int32), ("tol", C.c_double)]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("n_cells", C.c_int32), ("n_faces", C.c_int32), ("n_interior", C.c_int32),
+                ("owner", i32p), ("neighbour", i32p),
+                ("Sf", f64p), ("area", f64p), ("CN", f64p), ("eCN", f64p), ("gDiff", f64p),
+                ("T", f64p), ("gf", f64p), ("face_centroid", f64p),
+                ("volume", f64p), ("centroid", f64p),
+                ("cconn_ptr", i32p), ("cconn", i32p), ("own_nnz", i32p), ("nei_nnz", i32p),
+                ("n_patches", C.c_int32), ("patch_start", i32p), ("patch_nfaces", i32p),
+                ("patch_type", i32p)]
+
+
+class SimpleCfg(C.Structure):
+    _fields_ = [("nu", C.c_double), ("urf_u", C.c_double), ("urf_p", C.c_double),
+                ("n_correctors", C.c_int32), ("p_ref_cell", C.c_int32), ("u_sweeps", C.c_int32),
+                ("u_bc", i32p), ("u_value", f64p), ("p_bc", i32p), ("p_solver", C.c_int32),
+                ("p_amg", AmgCfg), ("p_pcg", PcgCfg)]
 
 
 # name -> (restype, argtypes)
@@ -64,6 +83,17 @@ _SIGS = {
     "ufvm_pcg_solve": (C.c_int, [C.c_void_p, C.c_int32, i32p, f64p]),
     "ufvm_pcg_apply_vcycle": (C.c_int, [C.c_void_p, C.c_int32, f64p, f64p]),
     "ufvm_pcg_level_colours": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, i32p, i32p]),
+    "ufvm_case_create": (C.c_int, [C.c_void_p, C.POINTER(MeshDesc), C.POINTER(SimpleCfg), i32p]),
+    "ufvm_case_destroy": (C.c_int, [C.c_void_p, C.c_int32]),
+    "ufvm_case_set_field": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, f64p, C.c_int64]),
+    "ufvm_case_get_field": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, f64p, C.c_int64]),
+    "ufvm_case_system": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, i32p]),
+    "ufvm_update_fields": (C.c_int, [C.c_void_p, C.c_int32]),
+    "ufvm_assemble_equation": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32, f64p,
+                                         f64p]),
+    "ufvm_solve_equation": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, f64p, f64p, i32p]),
+    "ufvm_correct_equation": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32]),
+    "ufvm_simple_iterate": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f64p]),
     "ufvm_prof_enable": (C.c_int, [C.c_void_p, C.c_int32]),
     "ufvm_prof_read": (C.c_int, [C.c_void_p, C.c_char_p, f64p, C.POINTER(C.c_int64), f64p]),
     "ufvm_prof_reset": (C.c_int, [C.c_void_p]),
@@ -228,8 +258,9 @@ class Context:
         return r0.value, r1.value, nc.value
 
     def pcg_setup(self, sid, precision=32, pre_sweep=1, post_sweep=1, max_coarse_cells=2000,
-                  nullspace=-1, max_iter=500, tol=1e-8):
-        cfg = PcgCfg(precision, pre_sweep, post_sweep, max_coarse_cells, nullspace, max_iter, tol)
+                  nullspace=-1, max_iter=500, tol=1e-8, coarse_smoother=1):
+        cfg = PcgCfg(precision, pre_sweep, post_sweep, coarse_smoother, max_coarse_cells,
+                     nullspace, max_iter, tol)
         self.check(self.lib.ufvm_pcg_setup(self.h, sid, C.byref(cfg)))
 
     def pcg_solve(self, sid):

@@ -7,35 +7,15 @@
 #include <cstring>
 #include <numeric>
 
+#include "internal.h"
 #include "kernels.h"
 #include "state.h"
 #include "ufvm.h"
 
 using namespace ufvm;
 
-namespace {
-
-constexpr int kScal = 32;
-
-template <typename F>
-int guard(ufvm_ctx *ctx, F &&f) {
-  if (!ctx) return UFVM_ERR_ARG;
-  try {
-    UFVM_HIP(hipSetDevice(ctx->device));
-    f();
-    ctx->err.clear();
-    return UFVM_OK;
-  } catch (const Error &e) {
-    ctx->err = e.what();
-    return e.code;
-  } catch (const std::bad_alloc &) {
-    ctx->err = "host out of memory";
-    return UFVM_ERR_ARG;
-  } catch (const std::exception &e) {
-    ctx->err = e.what();
-    return UFVM_ERR_ARG;
-  }
-}
+namespace ufvm {
+namespace detail {
 
 System &sys_of(ufvm_ctx *ctx, int32_t id) {
   auto it = ctx->systems.find(id);
@@ -324,7 +304,13 @@ double fcycle(Cyc &c, int G) {  // cfdApplyFCycle.m:5-63
 
 // ------------------------------------------------------- precision-form AMG
 
-void build_fast_level(ufvm_ctx *ctx, System &S, int l, bool dense_level) {
+// Precision-form copy of hierarchy level l.  kind 0 (level 1): unknowns
+// renumbered so every colour class of a global greedy colouring is one
+// contiguous row range.  kind 1 (coarse levels): blocks of kBRows consecutive
+// reference rows, each sorted by a colouring of the block's own subgraph.
+// kind 2: the coarsest level, kept in reference order for the dense solve.
+// Rows keep their cconn order in every kind.
+void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
   hipStream_t s = ctx->stream;
   RefLevel &R = *S.lev[l];
   auto &FLp = S.pcg.lev[l];
@@ -333,25 +319,47 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, bool dense_level) {
   const int32_t n = R.n;
   FL.n = n;
   FL.nnz = R.nnz;
+  FL.kind = kind;
   std::vector<double> ac(n), anb(std::max(R.nnz, 1));
   R.ac.download(ac.data(), n, s);
   R.anb.download(anb.data(), R.nnz, s);
   UFVM_HIP(hipStreamSynchronize(s));
-  if (dense_level) {
-    FL.ncol = 1;
-    FL.colour.assign(n, 0);
-  } else {
-    FL.ncol = greedy_colour(R.g, FL.colour);
-  }
-  FL.col_off.assign(FL.ncol + 1, 0);
-  for (int32_t i = 0; i < n; ++i) ++FL.col_off[FL.colour[i] + 1];
-  for (int c = 0; c < FL.ncol; ++c) FL.col_off[c + 1] += FL.col_off[c];
   FL.perm.resize(n);
-  std::vector<int32_t> iperm(n);
-  {
+  if (kind == 0) {
+    FL.ncol = greedy_colour(R.g, FL.colour);
+    FL.col_off.assign(FL.ncol + 1, 0);
+    for (int32_t i = 0; i < n; ++i) ++FL.col_off[FL.colour[i] + 1];
+    for (int c = 0; c < FL.ncol; ++c) FL.col_off[c + 1] += FL.col_off[c];
     std::vector<int32_t> fillp(FL.col_off.begin(), FL.col_off.end() - 1);
     for (int32_t i = 0; i < n; ++i) FL.perm[fillp[FL.colour[i]]++] = i;
+  } else if (kind == 1) {
+    const int32_t B = k::kBRows;
+    FL.ncol = greedy_colour_blocks(R.g, B, FL.colour);
+    FL.bncol = FL.ncol;
+    FL.nblk = (n + B - 1) / B;
+    FL.h_bptr.assign(FL.nblk + 1, 0);
+    FL.h_boff.assign((size_t)FL.nblk * (FL.bncol + 1), 0);
+    for (int32_t q = 0; q < FL.nblk; ++q) {
+      const int32_t lo = q * B, hi = std::min(n, lo + B);
+      FL.h_bptr[q] = lo;
+      int32_t *off = &FL.h_boff[(size_t)q * (FL.bncol + 1)];
+      std::vector<int32_t> cnt(FL.bncol + 1, 0);
+      for (int32_t i = lo; i < hi; ++i) ++cnt[FL.colour[i] + 1];
+      off[0] = lo;
+      for (int c = 0; c < FL.bncol; ++c) off[c + 1] = off[c] + cnt[c + 1];
+      std::vector<int32_t> fillp(off, off + FL.bncol);
+      for (int32_t i = lo; i < hi; ++i) FL.perm[fillp[FL.colour[i]]++] = i;
+    }
+    FL.h_bptr[FL.nblk] = n;
+    FL.bptr.upload(FL.h_bptr, s);
+    FL.boff.upload(FL.h_boff, s);
+  } else {
+    // kind 2 (dense coarsest) and kind 3 (Jacobi-smoothed): reference order
+    FL.ncol = 1;
+    FL.colour.assign(n, 0);
+    for (int32_t i = 0; i < n; ++i) FL.perm[i] = i;
   }
+  std::vector<int32_t> iperm(n);
   for (int32_t t = 0; t < n; ++t) iperm[FL.perm[t]] = t;
   FL.h_ptr.assign(n + 1, 0);
   std::vector<int32_t> col(std::max(R.nnz, 1));
@@ -374,8 +382,8 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, bool dense_level) {
   FL.anb64.upload(a64, s);
   FL.ac32.upload(d32, s);
   FL.anb32.upload(a32, s);
-  FL.b32.alloc(n); FL.x32.alloc(n); FL.r32.alloc(n);
-  FL.b64.alloc(n); FL.x64.alloc(n); FL.r64.alloc(n);
+  FL.b32.alloc(n); FL.x32.alloc(n); FL.r32.alloc(n); FL.y32.alloc(n);
+  FL.b64.alloc(n); FL.x64.alloc(n); FL.r64.alloc(n); FL.y64.alloc(n);
   UFVM_HIP(hipStreamSynchronize(s));
 }
 
@@ -409,6 +417,7 @@ template <>
 struct VecSel<float> {
   static float *b(FastLevel &L) { return L.b32.p; }
   static float *x(FastLevel &L) { return L.x32.p; }
+  static float *y(FastLevel &L) { return L.y32.p; }
   static float *r(FastLevel &L) { return L.r32.p; }
   static const float *ac(FastLevel &L) { return L.ac32.p; }
   static const float *anb(FastLevel &L) { return L.anb32.p; }
@@ -417,49 +426,111 @@ template <>
 struct VecSel<double> {
   static double *b(FastLevel &L) { return L.b64.p; }
   static double *x(FastLevel &L) { return L.x64.p; }
+  static double *y(FastLevel &L) { return L.y64.p; }
   static double *r(FastLevel &L) { return L.r64.p; }
   static const double *ac(FastLevel &L) { return L.ac64.p; }
   static const double *anb(FastLevel &L) { return L.anb64.p; }
 };
 
+// Symmetric multicolour Gauss-Seidel sweeps on level 1 (kind 0): colours
+// upward then downward; the downward pass skips the last colour and every
+// sweep after the first skips colour 0, because re-solving the colour just
+// solved repeats identical arithmetic.  from_zero: x holds no data yet.
 template <typename V>
-void mcgs_sweep(ufvm_ctx *ctx, FastLevel &L, int l, bool first_zero) {
+void mcgs_sweeps(ufvm_ctx *ctx, FastLevel &L, int nsweep, bool from_zero) {
   using Sel = VecSel<V>;
   const double w = sizeof(V);
-  auto pass = [&](int c, bool zero) {
+  auto pass = [&](int c, int32_t zero_above) {
     const int32_t r0 = L.col_off[c], r1 = L.col_off[c + 1];
     double bytes = 0.0;
-    if (l == 0 && ctx->prof_on) {
+    if (ctx->prof_on) {
       const double R = r1 - r0, Z = L.h_ptr[r1] - L.h_ptr[r0];
       bytes = 4.0 * (R + 1) + (4.0 + w) * Z + 3.0 * w * R + w * (L.n - R);
     }
-    ProfScope ps(ctx, l == 0 ? "mcgs0" : "mcgs", bytes);
+    ProfScope ps(ctx, "mcgs0", bytes);
     k::mcgs_rows<V>(ctx->stream, r0, r1, L.ptr.p, L.col.p, Sel::ac(L), Sel::anb(L), Sel::b(L),
-                    Sel::x(L), zero);
+                    Sel::x(L), zero_above);
   };
-  for (int c = 0; c < L.ncol; ++c) pass(c, first_zero && c == 0);
-  for (int c = L.ncol - 1; c >= 0; --c) pass(c, false);
+  for (int sw = 0; sw < nsweep; ++sw) {
+    for (int c = (sw == 0 ? 0 : 1); c < L.ncol; ++c)
+      pass(c, (sw == 0 && from_zero) ? L.col_off[c + 1] : L.n);
+    for (int c = L.ncol - 2; c >= 0; --c) pass(c, L.n);
+  }
 }
 
+// One V-cycle from level l down; returns the level's solution vector.
 template <typename V>
-void vcycle_fast(ufvm_ctx *ctx, System &S, int l) {
+V *vcycle_fast(ufvm_ctx *ctx, System &S, int l) {
   using Sel = VecSel<V>;
   PcgState &P = S.pcg;
   FastLevel &L = *P.lev[l];
   hipStream_t s = ctx->stream;
+  const int pre = P.cfg.pre_sweep, post = P.cfg.post_sweep;
   if (l == P.nlev - 1) {
     k::dense_solve<V>(s, L.n, P.ainv.p, Sel::b(L), Sel::x(L));
-    return;
+    return Sel::x(L);
   }
-  k::fill<V>(s, L.n, Sel::x(L), V(0));
-  for (int it = 0; it < P.cfg.pre_sweep; ++it) mcgs_sweep<V>(ctx, L, l, it == 0);
-  k::residual_fast<V>(s, L.n, L.ptr.p, L.col.p, Sel::ac(L), Sel::anb(L), Sel::b(L), Sel::x(L),
-                      Sel::r(L));
   FastLevel &C = *P.lev[l + 1];
-  k::restrict_fast<V>(s, C.n, L.child_ptr.p, L.child.p, Sel::r(L), Sel::b(C));
-  vcycle_fast<V>(ctx, S, l + 1);
-  k::prolong_fast<V>(s, L.n, L.parent.p, Sel::x(C), Sel::x(L));
-  for (int it = 0; it < P.cfg.post_sweep; ++it) mcgs_sweep<V>(ctx, L, l, false);
+  V *x = Sel::x(L);
+  const V omega = V(2.0 / 3.0);
+  if (L.kind == 3) {
+    // x already holds one Jacobi sweep from zero (fused into the restriction
+    // that produced b); further pre-sweeps ping-pong between x and y
+    for (int it = 1; it < pre; ++it) {
+      k::jacobi<V>(s, L.n, L.ptr.p, L.col.p, Sel::ac(L), Sel::anb(L), Sel::b(L), x, nullptr,
+                   nullptr, Sel::y(L), omega, 1);
+      std::swap(L.x32.p, L.y32.p);
+      std::swap(L.x64.p, L.y64.p);
+      x = Sel::x(L);
+    }
+    if (pre == 0) k::fill<V>(s, L.n, x, V(0));
+    k::residual_fast<V>(s, L.n, L.ptr.p, L.col.p, Sel::ac(L), Sel::anb(L), Sel::b(L), x,
+                        Sel::r(L));
+    if (C.kind == 3 && pre > 0)
+      k::restrict_jacobi<V>(s, C.n, L.child_ptr.p, L.child.p, Sel::r(L), Sel::ac(C), Sel::b(C),
+                            Sel::x(C), omega);
+    else
+      k::restrict_fast<V>(s, C.n, L.child_ptr.p, L.child.p, Sel::r(L), Sel::b(C));
+    V *xc = vcycle_fast<V>(ctx, S, l + 1);
+    if (post == 0) {
+      k::prolong_fast<V>(s, L.n, L.parent.p, xc, x);
+      return x;
+    }
+    V *y = Sel::y(L);
+    k::jacobi<V>(s, L.n, L.ptr.p, L.col.p, Sel::ac(L), Sel::anb(L), Sel::b(L), x, L.parent.p, xc,
+                 y, omega, 2);
+    for (int it = 1; it < post; ++it) {
+      k::jacobi<V>(s, L.n, L.ptr.p, L.col.p, Sel::ac(L), Sel::anb(L), Sel::b(L), y, nullptr,
+                   nullptr, x, omega, 1);
+      std::swap(x, y);
+    }
+    return y;
+  }
+  if (L.kind == 0) {
+    if (pre > 0) mcgs_sweeps<V>(ctx, L, pre, true);
+    else k::fill<V>(s, L.n, x, V(0));
+  } else {
+    if (pre > 0)
+      k::bmcgs<V>(s, L.nblk, L.bptr.p, L.boff.p, L.bncol, L.ptr.p, L.col.p, Sel::ac(L),
+                  Sel::anb(L), Sel::b(L), x, nullptr, nullptr, x, 0, pre);
+    else k::fill<V>(s, L.n, x, V(0));
+  }
+  k::residual_fast<V>(s, L.n, L.ptr.p, L.col.p, Sel::ac(L), Sel::anb(L), Sel::b(L), x, Sel::r(L));
+  if (C.kind == 3 && pre > 0)
+    k::restrict_jacobi<V>(s, C.n, L.child_ptr.p, L.child.p, Sel::r(L), Sel::ac(C), Sel::b(C),
+                          Sel::x(C), omega);
+  else
+    k::restrict_fast<V>(s, C.n, L.child_ptr.p, L.child.p, Sel::r(L), Sel::b(C));
+  V *xc = vcycle_fast<V>(ctx, S, l + 1);
+  if (L.kind == 1 && post > 0) {
+    V *y = Sel::y(L);
+    k::bmcgs<V>(s, L.nblk, L.bptr.p, L.boff.p, L.bncol, L.ptr.p, L.col.p, Sel::ac(L),
+                Sel::anb(L), Sel::b(L), x, L.parent.p, xc, y, 2, post);
+    return y;
+  }
+  k::prolong_fast<V>(s, L.n, L.parent.p, xc, x);
+  if (post > 0) mcgs_sweeps<V>(ctx, L, post, false);
+  return x;
 }
 
 void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
@@ -497,7 +568,11 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
   }
   P.lev.clear();
   P.lev.resize(nlev);
-  for (int32_t l = 0; l < nlev; ++l) build_fast_level(ctx, S, l, l == nlev - 1);
+  if (cfg.coarse_smoother != 0 && cfg.coarse_smoother != 1)
+    throw Error(UFVM_ERR_ARG, "coarse_smoother must be 0 or 1");
+  for (int32_t l = 0; l < nlev; ++l)
+    build_fast_level(ctx, S, l,
+                     l == nlev - 1 ? 2 : (l == 0 ? 0 : (cfg.coarse_smoother == 1 ? 3 : 1)));
   for (int32_t l = 0; l + 1 < nlev; ++l) link_levels(ctx, S, l);
   // dense inverse of the coarsest level (+ mean(diag)/n * 11^T when singular)
   RefLevel &RC = *S.lev[nlev - 1];
@@ -554,22 +629,16 @@ void pcg_solve_t(ufvm_ctx *ctx, System &S, int32_t *iters, double *relres) {
   int it = 0;
   if (nb > 0.0) {
     while (it < P.cfg.max_iter) {
-      vcycle_fast<V>(ctx, S, 0);
-      const V *z = Sel::x(L0);
-      k::fcg_dots<V>(s, n, z, P.w.p, P.r.p, ctx->partial.p);
-      k::reduce_groups(s, ctx->partial.p, k::nblocks(n), 5, scal);
-      k::fcg_scalars_after_dots(s, scal, n, P.singular, it == 0);
+      const V *z = vcycle_fast<V>(ctx, S, 0);
+      k::fcg_dots<V>(s, n, z, P.w.p, P.r.p, ctx->partial.p, scal, P.singular, it == 0);
       k::fcg_update_p<V>(s, n, z, scal, P.p.p, it == 0);
       {
         ProfScope ps(ctx, "spmv0",
                      ctx->prof_on ? 4.0 * (n + 1) + 12.0 * L0.nnz + 8.0 * 3 * n : 0.0);
         k::spmv_dot(s, n, L0.ptr.p, L0.col.p, L0.ac64.p, L0.anb64.p, P.p.p, P.w.p,
-                    ctx->partial.p);
+                    ctx->partial.p, scal);
       }
-      k::reduce_final(s, ctx->partial.p, k::nblocks(n), scal + 5);
-      k::fcg_scalars_after_spmv(s, scal);
       k::fcg_update_xr<V>(s, n, scal, P.p.p, P.w.p, P.x.p, P.r.p, Sel::b(L0), ctx->partial.p);
-      k::reduce_final(s, ctx->partial.p, k::nblocks(n), scal + 10);
       read_scal(ctx, 10, 1);
       ++it;
       rr = std::sqrt(ctx->hscal[10]);
@@ -583,7 +652,86 @@ void pcg_solve_t(ufvm_ctx *ctx, System &S, int32_t *iters, double *relres) {
   if (relres) *relres = nb > 0 ? rr / nb : 0.0;
 }
 
-}  // namespace
+void pcg_solve(ufvm_ctx *ctx, System &S, int32_t *iters, double *relres) {
+  if (!S.pcg.ready) throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first");
+  if (S.pcg.cfg.precision == 32) pcg_solve_t<float>(ctx, S, iters, relres);
+  else pcg_solve_t<double>(ctx, S, iters, relres);
+}
+
+// cfdApplyAMG.m:9-60
+void apply_amg(ufvm_ctx *ctx, System &S, const ufvm_amg_cfg &cfg, double *r0, double *r1,
+               int32_t *ncycles) {
+  if (cfg.cycle < 0 || cfg.cycle > 2) throw Error(UFVM_ERR_ARG, "bad cycle type");
+  if (cfg.smoother != UFVM_SMOOTHER_ILU && cfg.smoother != UFVM_SMOOTHER_SOR)
+    throw Error(UFVM_ERR_ARG, "smoother must be ILU or SOR");
+  if (cfg.rebuild || !S.hier_valid) agglomerate(ctx, S, cfg.max_coarse_levels);
+  // a freshly agglomerated hierarchy has zero coarse dphi (cfdSetupCoefficients.m:17)
+  for (int32_t l = 1; l < S.nlevels; ++l)
+    UFVM_HIP(hipMemsetAsync(S.lev[l]->dphi.p, 0, S.lev[l]->n * sizeof(double), ctx->stream));
+  const int32_t G = S.nlevels;
+  if (cfg.cycle != UFVM_CYCLE_V && G < 4)
+    throw Error(UFVM_ERR_UNSUPPORTED, "W- and F-cycles need at least 4 levels");
+  Cyc c{ctx, &S, &cfg};
+  const double initial = cyc_final(c);
+  double fin = initial;
+  int32_t nCycle = 1;
+  while (nCycle <= cfg.max_cycles && fin > cfg.termination * initial) {
+    if (cfg.cycle == UFVM_CYCLE_V) fin = vcycle(c, G);
+    else if (cfg.cycle == UFVM_CYCLE_W) fin = wcycle(c, G);
+    else fin = fcycle(c, G);
+    ++nCycle;
+  }
+  if (r0) *r0 = initial;
+  if (r1) *r1 = fin;
+  if (ncycles) *ncycles = nCycle;
+}
+
+// Galerkin coarse operators of an existing hierarchy recomputed from the
+// current level-1 coefficients (parents and coarse patterns unchanged)
+void galerkin_update(ufvm_ctx *ctx, System &S) {
+  if (S.nlevels < 2) throw Error(UFVM_ERR_STATE, "no AMG hierarchy to update");
+  for (int32_t l = 0; l + 1 < S.nlevels; ++l) {
+    RefLevel &F = *S.lev[l];
+    RefLevel &C = *S.lev[l + 1];
+    k::coarse_lhs(ctx->stream, C.n, F.child_ptr.p, F.child.p, F.ptr.p, F.ac.p, F.anb.p, F.slot.p,
+                  C.ptr.p, C.ac.p, C.anb.p);
+  }
+  UFVM_HIP(hipStreamSynchronize(ctx->stream));
+  S.hier_valid = true;
+}
+
+int32_t create_system(ufvm_ctx *ctx, int32_t n, int32_t nnz, const int32_t *ptr,
+                      const int32_t *col) {
+  if (n <= 0 || nnz < 0 || !ptr || (nnz > 0 && !col))
+    throw Error(UFVM_ERR_ARG, "bad system size or null pointer");
+  if (ptr[0] != 0 || ptr[n] != nnz) throw Error(UFVM_ERR_ARG, "ptr must run from 0 to nnz");
+  for (int32_t i = 0; i < n; ++i)
+    if (ptr[i + 1] < ptr[i]) throw Error(UFVM_ERR_ARG, "ptr not monotone");
+  for (int32_t e = 0; e < nnz; ++e)
+    if (col[e] < 0 || col[e] >= n) throw Error(UFVM_ERR_ARG, "column index out of range");
+  std::unique_ptr<System> S(new System());
+  S->n = n;
+  S->nnz = nnz;
+  S->lev.emplace_back(new RefLevel());
+  RefLevel &L = *S->lev[0];
+  L.g.n = n;
+  L.g.ptr.assign(ptr, ptr + n + 1);
+  L.g.col.assign(col, col + nnz);
+  upload_pattern(L, ctx->stream);
+  UFVM_HIP(hipMemsetAsync(L.ac.p, 0, n * sizeof(double), ctx->stream));
+  UFVM_HIP(hipMemsetAsync(L.anb.p, 0, std::max(nnz, 1) * sizeof(double), ctx->stream));
+  UFVM_HIP(hipMemsetAsync(L.bc.p, 0, n * sizeof(double), ctx->stream));
+  UFVM_HIP(hipMemsetAsync(L.dphi.p, 0, n * sizeof(double), ctx->stream));
+  UFVM_HIP(hipStreamSynchronize(ctx->stream));
+  const int32_t id = ctx->next_id++;
+  ctx->systems[id] = std::move(S);
+  return id;
+}
+
+}  // namespace detail
+}  // namespace ufvm
+
+using namespace ufvm::detail;
 
 // ======================================================================= C ABI
 
@@ -602,6 +750,8 @@ int ufvm_ctx_create(int32_t device, ufvm_ctx **out) {
     UFVM_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     ctx->scal.alloc(kScal);
     UFVM_HIP(hipMemsetAsync(ctx->scal.p, 0, kScal * sizeof(double), ctx->stream));
+    ctx->counter.alloc(4);
+    UFVM_HIP(hipMemsetAsync(ctx->counter.p, 0, 4 * sizeof(unsigned), ctx->stream));
     UFVM_HIP(hipHostMalloc((void **)&ctx->hscal, kScal * sizeof(double), hipHostMallocDefault));
     ctx->partial.alloc(1024);
     UFVM_HIP(hipStreamSynchronize(ctx->stream));
@@ -618,6 +768,7 @@ int ufvm_ctx_destroy(ufvm_ctx *ctx) {
   if (!ctx) return UFVM_OK;
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->cases.clear();
   ctx->systems.clear();
   for (auto &r : ctx->prof) {
     (void)hipEventDestroy(r.a);
@@ -626,6 +777,7 @@ int ufvm_ctx_destroy(ufvm_ctx *ctx) {
   for (auto e : ctx->event_pool) (void)hipEventDestroy(e);
   ctx->partial.release();
   ctx->scal.release();
+  ctx->counter.release();
   if (ctx->hscal) (void)hipHostFree(ctx->hscal);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -646,30 +798,8 @@ int ufvm_ctx_synchronize(ufvm_ctx *ctx) {
 int ufvm_system_create(ufvm_ctx *ctx, int32_t n, int32_t nnz, const int32_t *ptr,
                        const int32_t *col, int32_t *sys_out) {
   return guard(ctx, [&] {
-    if (n <= 0 || nnz < 0 || !ptr || (nnz > 0 && !col) || !sys_out)
-      throw Error(UFVM_ERR_ARG, "bad system size or null pointer");
-    if (ptr[0] != 0 || ptr[n] != nnz) throw Error(UFVM_ERR_ARG, "ptr must run from 0 to nnz");
-    for (int32_t i = 0; i < n; ++i)
-      if (ptr[i + 1] < ptr[i]) throw Error(UFVM_ERR_ARG, "ptr not monotone");
-    for (int32_t e = 0; e < nnz; ++e)
-      if (col[e] < 0 || col[e] >= n) throw Error(UFVM_ERR_ARG, "column index out of range");
-    std::unique_ptr<System> S(new System());
-    S->n = n;
-    S->nnz = nnz;
-    S->lev.emplace_back(new RefLevel());
-    RefLevel &L = *S->lev[0];
-    L.g.n = n;
-    L.g.ptr.assign(ptr, ptr + n + 1);
-    L.g.col.assign(col, col + nnz);
-    upload_pattern(L, ctx->stream);
-    UFVM_HIP(hipMemsetAsync(L.ac.p, 0, n * sizeof(double), ctx->stream));
-    UFVM_HIP(hipMemsetAsync(L.anb.p, 0, std::max(nnz, 1) * sizeof(double), ctx->stream));
-    UFVM_HIP(hipMemsetAsync(L.bc.p, 0, n * sizeof(double), ctx->stream));
-    UFVM_HIP(hipMemsetAsync(L.dphi.p, 0, n * sizeof(double), ctx->stream));
-    UFVM_HIP(hipStreamSynchronize(ctx->stream));
-    const int32_t id = ctx->next_id++;
-    ctx->systems[id] = std::move(S);
-    *sys_out = id;
+    if (!sys_out) throw Error(UFVM_ERR_ARG, "null sys_out");
+    *sys_out = create_system(ctx, n, nnz, ptr, col);
   });
 }
 
@@ -810,30 +940,7 @@ int ufvm_apply_amg(ufvm_ctx *ctx, int32_t sys, const ufvm_amg_cfg *cfg, double *
                    int32_t *ncycles) {
   return guard(ctx, [&] {
     if (!cfg) throw Error(UFVM_ERR_ARG, "null cfg");
-    if (cfg->cycle < 0 || cfg->cycle > 2) throw Error(UFVM_ERR_ARG, "bad cycle type");
-    if (cfg->smoother != UFVM_SMOOTHER_ILU && cfg->smoother != UFVM_SMOOTHER_SOR)
-      throw Error(UFVM_ERR_ARG, "smoother must be ILU or SOR");
-    System &S = sys_of(ctx, sys);
-    if (cfg->rebuild || !S.hier_valid) agglomerate(ctx, S, cfg->max_coarse_levels);
-    // a freshly agglomerated hierarchy has zero coarse dphi (cfdSetupCoefficients.m:17)
-    for (int32_t l = 1; l < S.nlevels; ++l)
-      UFVM_HIP(hipMemsetAsync(S.lev[l]->dphi.p, 0, S.lev[l]->n * sizeof(double), ctx->stream));
-    const int32_t G = S.nlevels;
-    if (cfg->cycle != UFVM_CYCLE_V && G < 4)
-      throw Error(UFVM_ERR_UNSUPPORTED, "W- and F-cycles need at least 4 levels");
-    Cyc c{ctx, &S, cfg};
-    const double initial = cyc_final(c);
-    double fin = initial;
-    int32_t nCycle = 1;
-    while (nCycle <= cfg->max_cycles && fin > cfg->termination * initial) {
-      if (cfg->cycle == UFVM_CYCLE_V) fin = vcycle(c, G);
-      else if (cfg->cycle == UFVM_CYCLE_W) fin = wcycle(c, G);
-      else fin = fcycle(c, G);
-      ++nCycle;
-    }
-    if (r0) *r0 = initial;
-    if (r1) *r1 = fin;
-    if (ncycles) *ncycles = nCycle;
+    apply_amg(ctx, sys_of(ctx, sys), *cfg, r0, r1, ncycles);
   });
 }
 
@@ -845,12 +952,7 @@ int ufvm_pcg_setup(ufvm_ctx *ctx, int32_t sys, const ufvm_pcg_cfg *cfg) {
 }
 
 int ufvm_pcg_solve(ufvm_ctx *ctx, int32_t sys, int32_t *iters, double *relres) {
-  return guard(ctx, [&] {
-    System &S = sys_of(ctx, sys);
-    if (!S.pcg.ready) throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first");
-    if (S.pcg.cfg.precision == 32) pcg_solve_t<float>(ctx, S, iters, relres);
-    else pcg_solve_t<double>(ctx, S, iters, relres);
-  });
+  return guard(ctx, [&] { pcg_solve(ctx, sys_of(ctx, sys), iters, relres); });
 }
 
 int ufvm_pcg_apply_vcycle(ufvm_ctx *ctx, int32_t sys, const double *r, double *z) {
@@ -866,12 +968,12 @@ int ufvm_pcg_apply_vcycle(ufvm_ctx *ctx, int32_t sys, const double *r, double *z
     k::permute_gather(s, n, L0.perm_d.p, P.w.p, P.p.p);
     if (P.cfg.precision == 32) {
       k::convert<double, float>(s, n, P.p.p, L0.b32.p);
-      vcycle_fast<float>(ctx, S, 0);
-      k::convert<float, double>(s, n, L0.x32.p, P.p.p);
+      const float *z32 = vcycle_fast<float>(ctx, S, 0);
+      k::convert<float, double>(s, n, z32, P.p.p);
     } else {
       k::convert<double, double>(s, n, P.p.p, L0.b64.p);
-      vcycle_fast<double>(ctx, S, 0);
-      k::convert<double, double>(s, n, L0.x64.p, P.p.p);
+      const double *z64 = vcycle_fast<double>(ctx, S, 0);
+      k::convert<double, double>(s, n, z64, P.p.p);
     }
     k::permute_scatter(s, n, L0.perm_d.p, P.p.p, P.w.p);
     UFVM_HIP(hipMemcpyAsync(z, P.w.p, n * sizeof(double), hipMemcpyDeviceToHost, s));

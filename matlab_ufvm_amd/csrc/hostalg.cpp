@@ -130,6 +130,32 @@ int32_t greedy_colour(const HostCSR &g, std::vector<int32_t> &colour) {
   return ncol;
 }
 
+int32_t greedy_colour_blocks(const HostCSR &g, int32_t rows_per_block,
+                             std::vector<int32_t> &colour) {
+  const int32_t n = g.n;
+  colour.assign(n, -1);
+  std::vector<int32_t> seen(64, -1);
+  int32_t ncol = 0;
+  for (int32_t i = 0; i < n; ++i) {
+    const int32_t lo = (i / rows_per_block) * rows_per_block;
+    const int32_t hi = std::min(n, lo + rows_per_block);
+    for (int32_t k = g.ptr[i]; k < g.ptr[i + 1]; ++k) {
+      const int32_t j = g.col[k];
+      if (j < lo || j >= hi) continue;
+      const int32_t c = colour[j];
+      if (c >= 0) {
+        if (c >= (int32_t)seen.size()) seen.resize(2 * c + 2, -1);
+        seen[c] = i;
+      }
+    }
+    int32_t c = 0;
+    while (c < (int32_t)seen.size() && seen[c] == i) ++c;
+    colour[i] = c;
+    ncol = std::max(ncol, c + 1);
+  }
+  return ncol;
+}
+
 void lex_schedule(const HostCSR &g, bool forward, Schedule &s) {
   const int32_t n = g.n;
   std::vector<int32_t> lev(n, 0);

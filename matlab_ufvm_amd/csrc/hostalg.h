@@ -27,6 +27,10 @@ void agglomerate_level(const HostCSR &g, const double *anb, AggResult &out);
 
 // greedy first-fit colouring in index order; returns number of colours
 int32_t greedy_colour(const HostCSR &g, std::vector<int32_t> &colour);
+// the same restricted to each block of `rows_per_block` consecutive rows
+// (only neighbours inside the row's own block constrain its colour)
+int32_t greedy_colour_blocks(const HostCSR &g, int32_t rows_per_block,
+                             std::vector<int32_t> &colour);
 
 // Level schedule of a lexicographic sweep.  forward: row i waits for its
 // neighbours j < i; backward: for neighbours j > i.  rows grouped by level,

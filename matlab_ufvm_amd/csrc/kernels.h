@@ -45,7 +45,22 @@ void coarse_lhs(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *c
 // ---- multicolour / precision-form kernels (V = float or double)
 template <typename V>
 void mcgs_rows(hipStream_t s, int32_t r0, int32_t r1, const int32_t *ptr, const int32_t *col,
-               const V *ac, const V *anb, const V *b, V *x, bool x_was_zero);
+               const V *ac, const V *anb, const V *b, V *x, int32_t zero_above);
+// block-local multicolour GS over blocks of at most kBRows rows (see kernels.hip)
+constexpr int kBRows = 1024;
+template <typename V>
+void bmcgs(hipStream_t s, int32_t nblk, const int32_t *bptr, const int32_t *boff, int32_t ncol,
+           const int32_t *ptr, const int32_t *col, const V *ac, const V *anb, const V *b,
+           const V *x_in, const int32_t *parent, const V *xc, V *x_out, int mode, int nsweep);
+// damped Jacobi sweep (mode 1: from x_in, 2: from x_in + xc[parent])
+template <typename V>
+void jacobi(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col, const V *ac,
+            const V *anb, const V *b, const V *x_in, const int32_t *parent, const V *xc, V *out,
+            V omega, int mode);
+// restriction fused with a Jacobi sweep from zero on the coarse level
+template <typename V>
+void restrict_jacobi(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *child,
+                     const V *r, const V *acc, V *bc, V *xc, V omega);
 template <typename V>
 void residual_fast(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col, const V *ac,
                    const V *anb, const V *b, const V *x, V *r);
@@ -70,28 +85,27 @@ void gj_inverse(hipStream_t s, int32_t n, double *a, double *colbuf);
 void csr_to_dense(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col,
                   const double *ac, const double *anb, double shift, double *a);
 
-// ---- FPCG (FP64, level-1 order of the multicolour permutation)
-// w = A p ; partial[blk] = p.w
+// ---- FPCG (FP64, level-1 order of the multicolour permutation).  Each call
+// is a streaming kernel writing per-block partial sums plus a one-block kernel
+// reducing them and updating the device scalars scal[] (layout in kernels.hip).
+// w = A p ; scal: delta = p.w, alpha = rho/delta
 void spmv_dot(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col, const double *ac,
-              const double *anb, const double *p, double *w, double *partial);
-// sums of z, z.w, z.r, w, r  -> partial[5*blk + {0..4}]
+              const double *anb, const double *p, double *w, double *partial, double *scal);
+// sums of z, z.w, z.r, w, r -> scal: m (mean of z if singular), rho, beta
 template <typename V>
 void fcg_dots(hipStream_t s, int32_t n, const V *z, const double *w, const double *r,
-              double *partial);
-// p = (z - m) - beta p  with m = zsum/n (singular) or 0; beta = (zw - m wsum)/delta
+              double *partial, double *scal, int singular, int first);
+// p = (z - m) - beta p
 template <typename V>
 void fcg_update_p(hipStream_t s, int32_t n, const V *z, const double *scal, double *p,
                   int first);
-// x += alpha p ; r -= alpha w ; partial = r.r ; rv = (V) r
+// x += alpha p ; r -= alpha w ; rv = (V) r ; scal: rr = r.r
 template <typename V>
-void fcg_update_xr(hipStream_t s, int32_t n, const double *scal, const double *p,
-                   const double *w, double *x, double *r, V *rv, double *partial);
-// reduce nparts partial groups of width `width` into out[0..width)
+void fcg_update_xr(hipStream_t s, int32_t n, double *scal, const double *p, const double *w,
+                   double *x, double *r, V *rv, double *partial);
+// reduce nblk partial groups of width `width` into out[0..width)
 void reduce_groups(hipStream_t s, const double *partial, int32_t nblk, int32_t width,
                    double *out);
-// scalar bookkeeping kernels of FPCG (all on device)
-void fcg_scalars_after_dots(hipStream_t s, double *scal, int32_t n, int singular, int first);
-void fcg_scalars_after_spmv(hipStream_t s, double *scal);
 // r.r and |b| helpers
 void dot_partial(hipStream_t s, int32_t n, const double *a, const double *b, double *partial);
 void sum_partial(hipStream_t s, int32_t n, const double *a, double *partial);

@@ -16,33 +16,10 @@
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
+#include "reduce.h"
 
 namespace ufvm {
 namespace k {
-
-constexpr int kWave = 64;
-
-template <typename T>
-__device__ inline T wave_sum(T v) {
-#pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_down(v, off, kWave);
-  return v;
-}
-
-// sum over a 256-thread block; result valid in thread 0
-template <typename T>
-__device__ inline T block_sum(T v, T *lds) {
-  v = wave_sum(v);
-  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-  if (lane == 0) lds[wid] = v;
-  __syncthreads();
-  T r = 0;
-  if (threadIdx.x == 0) {
-    for (int w = 0; w < kBlock / kWave; ++w) r += lds[w];
-  }
-  __syncthreads();
-  return r;
-}
 
 // ------------------------------------------------------------ reference mode
 
@@ -276,31 +253,151 @@ void coarse_lhs(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *c
 
 // --------------------------------------------------------- multicolour mode
 
-// Gauss-Seidel update of the contiguous rows [r0, r1) of one colour.  Rows
-// of one colour are mutually independent, so every read of x[col] sees the
-// value from the previous colour pass.  When the whole vector was zero on
-// entry and this is the first colour, the sum over neighbours vanishes.
+// Gauss-Seidel update of the contiguous rows [r0, r1) of one colour class.
+// Rows of one colour are mutually independent, so every read of x[col] sees
+// the value from an earlier colour pass.  Neighbours with index >= zero_above
+// belong to colours not yet visited in a sweep that started from x = 0 and
+// are read as 0 (so x needs no clearing first).
 template <typename V>
 __global__ void __launch_bounds__(kBlock)
 k_mcgs(int32_t r0, int32_t r1, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
        const V *__restrict__ ac, const V *__restrict__ anb, const V *__restrict__ b, V *x,
-       int zero) {
+       int32_t zero_above) {
   const int32_t i = r0 + blockIdx.x * kBlock + threadIdx.x;
   if (i >= r1) return;
   V v = b[i];
-  if (!zero) {
-    const int32_t k0 = ptr[i], k1 = ptr[i + 1];
-    for (int32_t k = k0; k < k1; ++k) v = v - anb[k] * x[col[k]];
+  const int32_t k0 = ptr[i], k1 = ptr[i + 1];
+  for (int32_t k = k0; k < k1; ++k) {
+    const int32_t j = col[k];
+    const V xj = (j < zero_above) ? x[j] : V(0);
+    v = v - anb[k] * xj;
   }
   x[i] = v / ac[i];
 }
 
 template <typename V>
 void mcgs_rows(hipStream_t s, int32_t r0, int32_t r1, const int32_t *ptr, const int32_t *col,
-               const V *ac, const V *anb, const V *b, V *x, bool x_was_zero) {
+               const V *ac, const V *anb, const V *b, V *x, int32_t zero_above) {
   if (r1 <= r0) return;
   hipLaunchKernelGGL(k_mcgs<V>, dim3(nblocks(r1 - r0)), dim3(kBlock), 0, s, r0, r1, ptr, col, ac,
-                     anb, b, x, x_was_zero ? 1 : 0);
+                     anb, b, x, zero_above);
+}
+
+// Block-local multicolour Gauss-Seidel (coarse levels).  Rows are numbered
+// block-major: block q owns rows [bptr[q], bptr[q+1]) (at most kBRows), sorted
+// by a colouring of the block's own subgraph; boff[q*(ncol+1)+c] is the first
+// row of colour c in block q.  The block's unknowns live in LDS during the
+// sweeps: neighbours inside the block are read there (Gauss-Seidel), those
+// outside from x_in (Jacobi coupling: their values from before this launch).
+// Each sweep visits the colours upward, then downward skipping the last one
+// (re-solving it would repeat the same arithmetic).  Initial in-block values:
+//   mode 0: zero (pre-smoothing from x = 0; out-of-block reads are 0 too)
+//   mode 1: x_in
+//   mode 2: x_in + xc[parent] (prolongation fused into post-smoothing)
+// Result: x_out[bptr[q] .. bptr[q+1]).  x_out must not alias x_in.
+template <typename V>
+__global__ void __launch_bounds__(kBlock)
+k_bmcgs(const int32_t *__restrict__ bptr, const int32_t *__restrict__ boff, int32_t ncol,
+        const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+        const V *__restrict__ ac, const V *__restrict__ anb, const V *__restrict__ b,
+        const V *__restrict__ x_in, const int32_t *__restrict__ parent,
+        const V *__restrict__ xc, V *__restrict__ x_out, int mode, int nsweep) {
+  __shared__ V xs[kBRows];
+  const int32_t q = blockIdx.x;
+  const int32_t r0 = bptr[q], r1 = bptr[q + 1];
+  for (int32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+    V v = V(0);
+    if (mode >= 1) v = x_in[i];
+    if (mode == 2) v = v + xc[parent[i]];
+    xs[i - r0] = v;
+  }
+  __syncthreads();
+  const int32_t *off = boff + (int64_t)q * (ncol + 1);
+  auto pass = [&](int c) {
+    for (int32_t i = off[c] + threadIdx.x; i < off[c + 1]; i += kBlock) {
+      V v = b[i];
+      for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+        const int32_t j = col[k];
+        V xj;
+        if (j >= r0 && j < r1) xj = xs[j - r0];
+        else if (mode == 0) xj = V(0);
+        else if (mode == 1) xj = x_in[j];
+        else xj = x_in[j] + xc[parent[j]];
+        v = v - anb[k] * xj;
+      }
+      xs[i - r0] = v / ac[i];
+    }
+    __syncthreads();
+  };
+  for (int s = 0; s < nsweep; ++s) {
+    for (int c = (s == 0 ? 0 : 1); c < ncol; ++c) pass(c);
+    for (int c = ncol - 2; c >= 0; --c) pass(c);
+  }
+  for (int32_t i = r0 + threadIdx.x; i < r1; i += kBlock) x_out[i] = xs[i - r0];
+}
+
+template <typename V>
+void bmcgs(hipStream_t s, int32_t nblk, const int32_t *bptr, const int32_t *boff, int32_t ncol,
+           const int32_t *ptr, const int32_t *col, const V *ac, const V *anb, const V *b,
+           const V *x_in, const int32_t *parent, const V *xc, V *x_out, int mode, int nsweep) {
+  if (nblk <= 0 || nsweep <= 0) return;
+  hipLaunchKernelGGL(k_bmcgs<V>, dim3(nblk), dim3(kBlock), 0, s, bptr, boff, ncol, ptr, col, ac,
+                     anb, b, x_in, parent, xc, x_out, mode, nsweep);
+}
+
+// Damped Jacobi sweep on a coarse level: out_i = v_i + w (b_i - (A v)_i) / ac_i
+// with v = x_in (mode 1) or x_in + xc[parent] (mode 2, prolongation fused into
+// the post-smoother).  Rows summed in cconn order.  out must not alias x_in.
+template <typename V>
+__global__ void __launch_bounds__(kBlock)
+k_jacobi(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+         const V *__restrict__ ac, const V *__restrict__ anb, const V *__restrict__ b,
+         const V *__restrict__ x_in, const int32_t *__restrict__ parent,
+         const V *__restrict__ xc, V *__restrict__ out, V omega, int mode) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  V vi = x_in[i];
+  if (mode == 2) vi = vi + xc[parent[i]];
+  V r = b[i] - ac[i] * vi;
+  for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+    const int32_t j = col[k];
+    V vj = x_in[j];
+    if (mode == 2) vj = vj + xc[parent[j]];
+    r = r - anb[k] * vj;
+  }
+  out[i] = vi + omega * r / ac[i];
+}
+
+template <typename V>
+void jacobi(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col, const V *ac,
+            const V *anb, const V *b, const V *x_in, const int32_t *parent, const V *xc, V *out,
+            V omega, int mode) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_jacobi<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, ptr, col, ac, anb, b,
+                     x_in, parent, xc, out, omega, mode);
+}
+
+// restriction fused with the first Jacobi sweep from zero on the coarse level:
+// bc[p] = sum of children's residuals; xc[p] = w bc[p] / ac[p]
+template <typename V>
+__global__ void __launch_bounds__(kBlock)
+k_restrict_jacobi(int32_t nc, const int32_t *__restrict__ cptr, const int32_t *__restrict__ child,
+                  const V *__restrict__ r, const V *__restrict__ acc, V *__restrict__ bc,
+                  V *__restrict__ xc, V omega) {
+  const int32_t p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= nc) return;
+  V s = 0;
+  for (int32_t c = cptr[p]; c < cptr[p + 1]; ++c) s = s + r[child[c]];
+  bc[p] = s;
+  xc[p] = omega * s / acc[p];
+}
+
+template <typename V>
+void restrict_jacobi(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *child,
+                     const V *r, const V *acc, V *bc, V *xc, V omega) {
+  if (nc <= 0) return;
+  hipLaunchKernelGGL(k_restrict_jacobi<V>, dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, cptr, child,
+                     r, acc, bc, xc, omega);
 }
 
 template <typename V>
@@ -441,6 +538,14 @@ void gj_inverse(hipStream_t s, int32_t n, double *a, double *colbuf) {
 
 // ------------------------------------------------------------------- FPCG
 
+// scal layout: 0 zsum 1 zw 2 zr 3 wsum 4 rsum 5 delta 6 beta 7 alpha 8 rho 9 m 10 rr
+// 12 scratch (mean of b)
+//
+// Each FPCG kernel writes per-block partial sums; a one-block kernel then sums
+// them in block order and does the scalar arithmetic (no grid-wide atomics or
+// device-scope fences in the streaming kernels).
+
+// w = A p ; partial = p.w
 __global__ void __launch_bounds__(kBlock)
 k_spmv_dot(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
            const double *__restrict__ ac, const double *__restrict__ anb,
@@ -459,64 +564,69 @@ k_spmv_dot(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict
   if (threadIdx.x == 0) partial[blockIdx.x] = d;
 }
 
-void spmv_dot(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col, const double *ac,
-              const double *anb, const double *p, double *w, double *partial) {
-  hipLaunchKernelGGL(k_spmv_dot, dim3(nblocks(n)), dim3(kBlock), 0, s, n, ptr, col, ac, anb, p, w,
-                     partial);
-}
-
+// partials of z, z.w, z.r, w, r
 template <typename V>
 __global__ void __launch_bounds__(kBlock)
 k_fcg_dots(int32_t n, const V *__restrict__ z, const double *__restrict__ w,
            const double *__restrict__ r, double *__restrict__ partial) {
   __shared__ double lds[kBlock / kWave];
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
-  double zs = 0, zw = 0, zr = 0, ws = 0, rs = 0;
+  double v[5] = {0, 0, 0, 0, 0};
   if (i < n) {
     const double zi = (double)z[i], wi = w[i], ri = r[i];
-    zs = zi;
-    zw = zi * wi;
-    zr = zi * ri;
-    ws = wi;
-    rs = ri;
+    v[0] = zi;
+    v[1] = zi * wi;
+    v[2] = zi * ri;
+    v[3] = wi;
+    v[4] = ri;
   }
-  zs = block_sum(zs, lds);
-  zw = block_sum(zw, lds);
-  zr = block_sum(zr, lds);
-  ws = block_sum(ws, lds);
-  rs = block_sum(rs, lds);
-  if (threadIdx.x == 0) {
-    double *o = partial + (int64_t)blockIdx.x * 5;
-    o[0] = zs; o[1] = zw; o[2] = zr; o[3] = ws; o[4] = rs;
+  for (int c = 0; c < 5; ++c) v[c] = block_sum(v[c], lds);
+  if (threadIdx.x == 0)
+    for (int c = 0; c < 5; ++c) partial[(int64_t)blockIdx.x * 5 + c] = v[c];
+}
+
+// reduction + scalar step.  mode 0: z-dots -> m, rho, beta; 1: delta -> alpha;
+// 2: rr
+__global__ void __launch_bounds__(kBlock)
+k_fcg_reduce(const double *__restrict__ partial, int32_t nblk, int mode, int32_t n, int singular,
+             int first, double *scal) {
+  __shared__ double lds[kBlock / kWave];
+  const int W = mode == 0 ? 5 : 1;
+  double t[5];
+  for (int c = 0; c < W; ++c) {
+    double a = 0.0;
+    for (int32_t q = threadIdx.x; q < nblk; q += kBlock) a += partial[(int64_t)q * W + c];
+    t[c] = block_sum(a, lds);
   }
+  if (threadIdx.x != 0) return;
+  if (mode == 0) {
+    for (int c = 0; c < 5; ++c) scal[c] = t[c];
+    const double m = singular ? t[0] / (double)n : 0.0;
+    scal[9] = m;
+    scal[8] = t[2] - m * t[4];
+    scal[6] = first ? 0.0 : (t[1] - m * t[3]) / scal[5];
+  } else if (mode == 1) {
+    scal[5] = t[0];
+    scal[7] = scal[8] / t[0];
+  } else {
+    scal[10] = t[0];
+  }
+}
+
+void spmv_dot(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col, const double *ac,
+              const double *anb, const double *p, double *w, double *partial, double *scal) {
+  hipLaunchKernelGGL(k_spmv_dot, dim3(nblocks(n)), dim3(kBlock), 0, s, n, ptr, col, ac, anb, p, w,
+                     partial);
+  hipLaunchKernelGGL(k_fcg_reduce, dim3(1), dim3(kBlock), 0, s, partial, nblocks(n), 1, n, 0, 0,
+                     scal);
 }
 
 template <typename V>
 void fcg_dots(hipStream_t s, int32_t n, const V *z, const double *w, const double *r,
-              double *partial) {
+              double *partial, double *scal, int singular, int first) {
   hipLaunchKernelGGL(k_fcg_dots<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, z, w, r, partial);
-}
-
-// scal layout: 0 zsum 1 zw 2 zr 3 wsum 4 rsum 5 delta 6 beta 7 alpha 8 rho 9 m 10 rr 11 spare
-__global__ void k_fcg_after_dots(double *scal, int32_t n, int singular, int first) {
-  if (threadIdx.x != 0) return;
-  const double m = singular ? scal[0] / (double)n : 0.0;
-  scal[9] = m;
-  scal[8] = scal[2] - m * scal[4];
-  scal[6] = first ? 0.0 : (scal[1] - m * scal[3]) / scal[5];
-}
-
-void fcg_scalars_after_dots(hipStream_t s, double *scal, int32_t n, int singular, int first) {
-  hipLaunchKernelGGL(k_fcg_after_dots, dim3(1), dim3(64), 0, s, scal, n, singular, first);
-}
-
-__global__ void k_fcg_after_spmv(double *scal) {
-  if (threadIdx.x != 0) return;
-  scal[7] = scal[8] / scal[5];
-}
-
-void fcg_scalars_after_spmv(hipStream_t s, double *scal) {
-  hipLaunchKernelGGL(k_fcg_after_spmv, dim3(1), dim3(64), 0, s, scal);
+  hipLaunchKernelGGL(k_fcg_reduce, dim3(1), dim3(kBlock), 0, s, partial, nblocks(n), 0, n,
+                     singular, first, scal);
 }
 
 template <typename V>
@@ -537,6 +647,7 @@ void fcg_update_p(hipStream_t s, int32_t n, const V *z, const double *scal, doub
                      first);
 }
 
+// x += alpha p ; r -= alpha w ; rv = (V) r ; partial = r.r
 template <typename V>
 __global__ void __launch_bounds__(kBlock)
 k_fcg_update_xr(int32_t n, const double *__restrict__ scal, const double *__restrict__ p,
@@ -558,10 +669,12 @@ k_fcg_update_xr(int32_t n, const double *__restrict__ scal, const double *__rest
 }
 
 template <typename V>
-void fcg_update_xr(hipStream_t s, int32_t n, const double *scal, const double *p,
-                   const double *w, double *x, double *r, V *rv, double *partial) {
+void fcg_update_xr(hipStream_t s, int32_t n, double *scal, const double *p, const double *w,
+                   double *x, double *r, V *rv, double *partial) {
   hipLaunchKernelGGL(k_fcg_update_xr<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, scal, p, w, x,
                      r, rv, partial);
+  hipLaunchKernelGGL(k_fcg_reduce, dim3(1), dim3(kBlock), 0, s, partial, nblocks(n), 2, n, 0, 0,
+                     scal);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -604,7 +717,15 @@ void axpy_const(hipStream_t s, int32_t n, const double *scal_slot, double mul, d
 // explicit instantiations
 #define UFVM_INST(V)                                                                            \
   template void mcgs_rows<V>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,   \
-                             const V *, const V *, const V *, V *, bool);                       \
+                             const V *, const V *, const V *, V *, int32_t);                    \
+  template void bmcgs<V>(hipStream_t, int32_t, const int32_t *, const int32_t *, int32_t,       \
+                         const int32_t *, const int32_t *, const V *, const V *, const V *,     \
+                         const V *, const int32_t *, const V *, V *, int, int);                 \
+  template void jacobi<V>(hipStream_t, int32_t, const int32_t *, const int32_t *, const V *,     \
+                          const V *, const V *, const V *, const int32_t *, const V *, V *, V, \
+                          int);                                                                \
+  template void restrict_jacobi<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,      \
+                                   const V *, const V *, V *, V *, V);                         \
   template void residual_fast<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,        \
                                  const V *, const V *, const V *, const V *, V *);              \
   template void restrict_fast<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,        \
@@ -613,10 +734,10 @@ void axpy_const(hipStream_t s, int32_t n, const double *scal_slot, double mul, d
   template void dense_solve<V>(hipStream_t, int32_t, const double *, const V *, V *);           \
   template void fill<V>(hipStream_t, int64_t, V *, V);                                          \
   template void fcg_dots<V>(hipStream_t, int32_t, const V *, const double *, const double *,    \
-                            double *);                                                          \
+                            double *, double *, int, int);                                      \
   template void fcg_update_p<V>(hipStream_t, int32_t, const V *, const double *, double *, int); \
-  template void fcg_update_xr<V>(hipStream_t, int32_t, const double *, const double *,          \
-                                 const double *, double *, double *, V *, double *);
+  template void fcg_update_xr<V>(hipStream_t, int32_t, double *, const double *, const double *, \
+                                 double *, double *, V *, double *);
 UFVM_INST(float)
 UFVM_INST(double)
 template void convert<double, float>(hipStream_t, int64_t, const double *, float *);

@@ -1,0 +1,406 @@
+// simple_api.hip -- C ABI of the SIMPLE outer iteration (ufvm_case_*,
+// ufvm_update_fields, ufvm_assemble_equation, ufvm_solve_equation,
+// ufvm_correct_equation, ufvm_simple_iterate) and its host orchestration.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+#include "kernels.h"
+#include "simple_kernels.h"
+#include "state.h"
+#include "ufvm.h"
+
+using namespace ufvm;
+using namespace ufvm::detail;
+
+namespace ufvm {
+
+struct Case {
+  int32_t N = 0, F = 0, Fi = 0, Fb = 0, P = 0;
+  DBuf<int32_t> owner, neighbour, cptr, ccol, own_nnz, nei_nnz, cf_ptr, cf_face, bpatch;
+  DBuf<int32_t> ptype, u_bc, p_bc;
+  DBuf<double> Sf, area, CN, eCN, gDiff, T, gf, fc, vol, cc, u_val;
+  DBuf<double> U, gradU, p, gradp, PP, gradPP, DU, DUT, mdot, gamma, fluxC1, fluxC2, ac_old;
+  sk::DevMesh dm{};
+  ufvm_simple_cfg cfg{};
+  std::vector<int32_t> h_ptype, h_u_bc, h_p_bc;
+  std::vector<double> h_u_val;
+  int32_t sysU = 0, sysP = 0;
+  bool p_hier_built = false;
+};
+
+}  // namespace ufvm
+
+namespace {
+
+Case &case_of(ufvm_ctx *ctx, int32_t id) {
+  auto it = ctx->cases.find(id);
+  if (it == ctx->cases.end()) throw Error(UFVM_ERR_ARG, "unknown case id");
+  return *static_cast<Case *>(it->second.get());
+}
+
+template <typename T>
+void up(DBuf<T> &d, const T *h, size_t n, hipStream_t s, const char *what) {
+  if (n && !h) throw Error(UFVM_ERR_ARG, std::string("null mesh array: ") + what);
+  d.upload(h, n, s);
+}
+
+// cfdInitializeCase.m: zero fields; boundary values from the BCs
+// (fixedValue -> value, empty -> owner minus its normal part, else owner);
+// mdot_f = rho_f (S_f . U_f) with Average interpolation (cfdInitializeMdotField.m:72-80)
+void init_fields(ufvm_ctx *ctx, Case &C) {
+  hipStream_t s = ctx->stream;
+  const int32_t NB = C.N + C.Fb;
+  std::vector<double> U((size_t)NB * 3, 0.0), zero((size_t)NB * 9, 0.0), mdot(C.F, 0.0);
+  std::vector<int32_t> owner(C.F);
+  std::vector<double> Sf((size_t)C.F * 3), gf(C.Fi);
+  C.owner.download(owner.data(), C.F, s);
+  C.Sf.download(Sf.data(), (size_t)C.F * 3, s);
+  C.gf.download(gf.data(), C.Fi, s);
+  std::vector<int32_t> bp(C.Fb);
+  C.bpatch.download(bp.data(), C.Fb, s);
+  UFVM_HIP(hipStreamSynchronize(s));
+  for (int32_t k = 0; k < C.Fb; ++k) {
+    const int p = bp[k];
+    const int32_t f = C.Fi + k, b = C.N + k, o = owner[f];
+    if (C.h_u_bc[p] == UFVM_BC_FIXED_VALUE) {
+      for (int d = 0; d < 3; ++d) U[b * 3 + d] = C.h_u_val[p * 3 + d];
+    } else if (C.h_u_bc[p] == UFVM_BC_EMPTY) {
+      const double *S = &Sf[f * 3];
+      const double nm = std::sqrt((S[0] * S[0] + S[1] * S[1]) + S[2] * S[2]);
+      double n[3] = {S[0] / nm, S[1] / nm, S[2] / nm};
+      const double un = (U[o * 3] * n[0] + U[o * 3 + 1] * n[1]) + U[o * 3 + 2] * n[2];
+      for (int d = 0; d < 3; ++d) U[b * 3 + d] = U[o * 3 + d] - un * n[d];
+    } else {
+      for (int d = 0; d < 3; ++d) U[b * 3 + d] = U[o * 3 + d];
+    }
+  }
+  // interior U is zero, so only boundary faces can carry an initial flux
+  for (int32_t f = C.Fi; f < C.F; ++f) {
+    const int32_t b = C.N + (f - C.Fi);
+    const double *S = &Sf[f * 3];
+    mdot[f] = 1.0 * ((S[0] * U[b * 3] + S[1] * U[b * 3 + 1]) + S[2] * U[b * 3 + 2]);
+  }
+  for (int32_t f = 0; f < C.Fi; ++f) {
+    const double g = gf[f];
+    const double rf = g * 1.0 + (1 - g) * 1.0;
+    const double *S = &Sf[f * 3];
+    mdot[f] = rf * ((S[0] * 0.0 + S[1] * 0.0) + S[2] * 0.0);
+  }
+  C.U.upload(U, s);
+  C.gradU.upload(zero, s);
+  C.p.upload(zero.data(), NB, s);
+  C.gradp.upload(zero.data(), (size_t)NB * 3, s);
+  C.PP.upload(zero.data(), NB, s);
+  C.gradPP.upload(zero.data(), (size_t)NB * 3, s);
+  C.DU.upload(zero.data(), (size_t)NB * 3, s);
+  C.DUT.upload(zero.data(), (size_t)NB * 3, s);
+  C.mdot.upload(mdot, s);
+  C.gamma.upload(zero.data(), C.F, s);
+  C.fluxC1.upload(zero.data(), C.F, s);
+  C.fluxC2.upload(zero.data(), C.F, s);
+  C.ac_old.upload(zero.data(), C.N, s);
+}
+
+struct FieldRef {
+  double *p;
+  size_t n;
+};
+
+FieldRef field_of(Case &C, const char *name) {
+  const size_t NB = (size_t)C.N + C.Fb;
+  const std::string n(name ? name : "");
+  if (n == "U") return {C.U.p, NB * 3};
+  if (n == "gradU") return {C.gradU.p, NB * 9};
+  if (n == "p") return {C.p.p, NB};
+  if (n == "gradp") return {C.gradp.p, NB * 3};
+  if (n == "PP") return {C.PP.p, NB};
+  if (n == "gradPP") return {C.gradPP.p, NB * 3};
+  if (n == "DU") return {C.DU.p, NB * 3};
+  if (n == "DUT") return {C.DUT.p, NB * 3};
+  if (n == "mdot_f") return {C.mdot.p, (size_t)C.F};
+  if (n == "gamma_f") return {C.gamma.p, (size_t)C.F};
+  throw Error(UFVM_ERR_ARG, "unknown field " + n);
+}
+
+int eq_of(const char *eq) {
+  if (eq && std::strcmp(eq, "U") == 0) return 0;
+  if (eq && std::strcmp(eq, "p") == 0) return 1;
+  throw Error(UFVM_ERR_ARG, "equation must be \"U\" or \"p\"");
+}
+
+// cfdUpdateEquations.m:14-24 for U then p
+void update_fields(ufvm_ctx *ctx, Case &C) {
+  hipStream_t s = ctx->stream;
+  sk::update_u_bc(s, C.dm, C.U.p);
+  sk::gradient(s, C.dm, C.U.p, 3, C.gradU.p);
+  sk::gamma_faces(s, C.dm, C.cfg.nu, C.gamma.p);
+  ensure_partial(ctx, (int64_t)(C.N + C.Fb) * 3, 2);
+  sk::scale_u(s, C.dm, C.U.p, ctx->partial.p, ctx->counter.p, ctx->scal.p);
+  sk::gradient(s, C.dm, C.p.p, 1, C.gradp.p);
+}
+
+void assemble(ufvm_ctx *ctx, Case &C, int eq, int comp, double *rms, double *mx) {
+  hipStream_t s = ctx->stream;
+  System &S = sys_of(ctx, eq == 0 ? C.sysU : C.sysP);
+  RefLevel &L = *S.lev[0];
+  ensure_partial(ctx, C.N, 2);
+  if (eq == 0) {
+    if (comp < 0 || comp > 2) throw Error(UFVM_ERR_ARG, "component must be 0, 1 or 2");
+    sk::assemble_u(s, C.dm, comp, C.cfg.urf_u, C.U.p, C.gradU.p, C.gradp.p, C.mdot.p, C.gamma.p,
+                   L.ac.p, L.anb.p, L.bc.p, C.ac_old.p, L.dphi.p, C.DU.p, C.DUT.p, ctx->partial.p,
+                   ctx->counter.p, ctx->scal.p);
+  } else {
+    sk::rhie_chow(s, C.dm, C.cfg.urf_u, C.U.p, C.DU.p, C.p.p, C.gradp.p, C.mdot.p, C.fluxC1.p,
+                  C.fluxC2.p);
+    sk::assemble_p(s, C.dm, C.fluxC1.p, C.fluxC2.p, C.mdot.p, L.ac.p, L.anb.p, L.bc.p, C.ac_old.p,
+                   L.dphi.p, ctx->partial.p, ctx->counter.p, ctx->scal.p);
+  }
+  S.hier_valid = false;
+  S.pcg.ready = false;
+  read_scal(ctx, sk::S_RMAX, 2);
+  const double m = ctx->hscal[sk::S_RMAX], sq = ctx->hscal[sk::S_RSQ];
+  if (mx) *mx = m;
+  if (rms) *rms = std::sqrt(sq / C.N);
+  if (std::isnan(sq)) throw Error(UFVM_ERR_NUMERIC, "divergence detected (NaN residual)");
+}
+
+void solve(ufvm_ctx *ctx, Case &C, int eq, double *r0, double *r1, int32_t *nc) {
+  if (eq == 0) {
+    System &S = sys_of(ctx, C.sysU);
+    solve_alg(ctx, *S.lev[0], UFVM_SMOOTHER_ILU, C.cfg.u_sweeps, 0.1, r0, r1);
+    if (nc) *nc = 0;
+    return;
+  }
+  System &S = sys_of(ctx, C.sysP);
+  if (C.cfg.p_solver == 0) {
+    ufvm_amg_cfg a = C.cfg.p_amg;
+    a.rebuild = 1;
+    apply_amg(ctx, S, a, r0, r1, nc);
+    return;
+  }
+  if (C.cfg.p_solver == 1 || !C.p_hier_built) {
+    agglomerate(ctx, S, C.cfg.p_amg.max_coarse_levels);
+    C.p_hier_built = true;
+  } else {
+    galerkin_update(ctx, S);   // first hierarchy's parents, current coefficients
+  }
+  pcg_setup(ctx, S, C.cfg.p_pcg);
+  double rel = 0.0;
+  pcg_solve(ctx, S, nc, &rel);
+  if (r0) *r0 = 1.0;
+  if (r1) *r1 = rel;
+}
+
+void correct(ufvm_ctx *ctx, Case &C, int eq, int comp) {
+  hipStream_t s = ctx->stream;
+  if (eq == 0) {
+    if (comp < 0 || comp > 2) throw Error(UFVM_ERR_ARG, "component must be 0, 1 or 2");
+    System &S = sys_of(ctx, C.sysU);
+    sk::correct_u(s, C.dm, comp, S.lev[0]->dphi.p, C.U.p);
+  } else {
+    System &S = sys_of(ctx, C.sysP);
+    sk::correct_p(s, C.dm, C.cfg.p_ref_cell, C.cfg.urf_p, S.lev[0]->dphi.p, C.PP.p, C.gradPP.p,
+                  C.DU.p, C.U.p, C.p.p, C.fluxC1.p, C.fluxC2.p, C.mdot.p);
+  }
+  UFVM_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+extern "C" {
+
+int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *m, const ufvm_simple_cfg *cfg,
+                     int32_t *case_out) {
+  return guard(ctx, [&] {
+    if (!m || !cfg || !case_out) throw Error(UFVM_ERR_ARG, "null argument");
+    if (m->n_cells <= 0 || m->n_interior < 0 || m->n_faces < m->n_interior || m->n_patches < 0)
+      throw Error(UFVM_ERR_ARG, "bad mesh sizes");
+    hipStream_t s = ctx->stream;
+    std::unique_ptr<Case> Cp(new Case());
+    Case &C = *Cp;
+    C.N = m->n_cells;
+    C.F = m->n_faces;
+    C.Fi = m->n_interior;
+    C.Fb = C.F - C.Fi;
+    C.P = m->n_patches;
+    C.cfg = *cfg;
+    // boundary faces -> patch, and patch checks
+    std::vector<int32_t> bp(C.Fb, -1);
+    for (int p = 0; p < C.P; ++p) {
+      const int32_t st = m->patch_start[p], nf = m->patch_nfaces[p];
+      if (st < C.Fi || nf < 0 || st + nf > C.F) throw Error(UFVM_ERR_ARG, "bad patch range");
+      for (int32_t f = st; f < st + nf; ++f) bp[f - C.Fi] = p;
+      const int t = m->patch_type[p];
+      if (t != UFVM_PATCH_WALL && t != UFVM_PATCH_EMPTY)
+        throw Error(UFVM_ERR_UNSUPPORTED, "only wall and empty patches are supported");
+      const int ub = cfg->u_bc[p], pb = cfg->p_bc[p];
+      if (t == UFVM_PATCH_WALL && !(ub == UFVM_BC_FIXED_VALUE && pb == UFVM_BC_ZERO_GRADIENT))
+        throw Error(UFVM_ERR_UNSUPPORTED, "walls need U fixedValue and p zeroGradient");
+      if (t == UFVM_PATCH_EMPTY && !(ub == UFVM_BC_EMPTY && pb == UFVM_BC_EMPTY))
+        throw Error(UFVM_ERR_UNSUPPORTED, "empty patches need empty U and p conditions");
+    }
+    for (int32_t k = 0; k < C.Fb; ++k)
+      if (bp[k] < 0) throw Error(UFVM_ERR_ARG, "boundary face outside every patch");
+    for (int32_t f = 0; f < C.F; ++f)
+      if (m->owner[f] < 0 || m->owner[f] >= C.N) throw Error(UFVM_ERR_ARG, "owner out of range");
+    for (int32_t f = 0; f < C.Fi; ++f)
+      if (m->neighbour[f] <= m->owner[f] || m->neighbour[f] >= C.N)
+        throw Error(UFVM_ERR_ARG, "interior faces need owner < neighbour < n_cells");
+    const int32_t nnz = m->cconn_ptr[C.N];
+    if (nnz != 2 * C.Fi) throw Error(UFVM_ERR_ARG, "cconn must hold two entries per interior face");
+    for (int32_t f = 0; f < C.Fi; ++f) {
+      const int32_t a = m->own_nnz[f], b = m->nei_nnz[f];
+      if (a < m->cconn_ptr[m->owner[f]] || a >= m->cconn_ptr[m->owner[f] + 1] ||
+          m->cconn[a] != m->neighbour[f] || b < m->cconn_ptr[m->neighbour[f]] ||
+          b >= m->cconn_ptr[m->neighbour[f] + 1] || m->cconn[b] != m->owner[f])
+        throw Error(UFVM_ERR_ARG, "own_nnz / nei_nnz inconsistent with cconn");
+    }
+    // faces of each cell, ascending (interior ones first by construction)
+    std::vector<int32_t> cf_ptr(C.N + 1, 0);
+    for (int32_t f = 0; f < C.F; ++f) {
+      ++cf_ptr[m->owner[f] + 1];
+      if (f < C.Fi) ++cf_ptr[m->neighbour[f] + 1];
+    }
+    for (int32_t i = 0; i < C.N; ++i) cf_ptr[i + 1] += cf_ptr[i];
+    std::vector<int32_t> cf_face(cf_ptr[C.N]), fillp(cf_ptr.begin(), cf_ptr.end() - 1);
+    for (int32_t f = 0; f < C.F; ++f) {
+      cf_face[fillp[m->owner[f]]++] = f;
+      if (f < C.Fi) cf_face[fillp[m->neighbour[f]]++] = f;
+    }
+    const size_t F3 = (size_t)C.F * 3;
+    up(C.owner, m->owner, C.F, s, "owner");
+    up(C.neighbour, m->neighbour, std::max(C.Fi, 1), s, "neighbour");
+    up(C.Sf, m->Sf, F3, s, "Sf");
+    up(C.area, m->area, C.F, s, "area");
+    up(C.CN, m->CN, F3, s, "CN");
+    up(C.eCN, m->eCN, F3, s, "eCN");
+    up(C.gDiff, m->gDiff, C.F, s, "gDiff");
+    up(C.T, m->T, F3, s, "T");
+    up(C.gf, m->gf, C.F, s, "gf");
+    up(C.fc, m->face_centroid, F3, s, "face_centroid");
+    up(C.vol, m->volume, C.N, s, "volume");
+    up(C.cc, m->centroid, (size_t)C.N * 3, s, "centroid");
+    up(C.cptr, m->cconn_ptr, C.N + 1, s, "cconn_ptr");
+    up(C.ccol, m->cconn, std::max(nnz, 1), s, "cconn");
+    up(C.own_nnz, m->own_nnz, std::max(C.Fi, 1), s, "own_nnz");
+    up(C.nei_nnz, m->nei_nnz, std::max(C.Fi, 1), s, "nei_nnz");
+    C.cf_ptr.upload(cf_ptr, s);
+    C.cf_face.upload(cf_face, s);
+    C.bpatch.upload(bp.empty() ? std::vector<int32_t>{0} : bp, s);
+    C.h_ptype.assign(m->patch_type, m->patch_type + C.P);
+    C.h_u_bc.assign(cfg->u_bc, cfg->u_bc + C.P);
+    C.h_p_bc.assign(cfg->p_bc, cfg->p_bc + C.P);
+    C.h_u_val.assign(cfg->u_value, cfg->u_value + 3 * C.P);
+    C.ptype.upload(C.h_ptype.empty() ? std::vector<int32_t>{0} : C.h_ptype, s);
+    C.u_bc.upload(C.h_u_bc.empty() ? std::vector<int32_t>{0} : C.h_u_bc, s);
+    C.p_bc.upload(C.h_p_bc.empty() ? std::vector<int32_t>{0} : C.h_p_bc, s);
+    C.u_val.upload(C.h_u_val.empty() ? std::vector<double>{0.0} : C.h_u_val, s);
+    C.cfg.u_bc = nullptr;
+    C.cfg.u_value = nullptr;
+    C.cfg.p_bc = nullptr;
+    if (C.cfg.p_ref_cell >= C.N) throw Error(UFVM_ERR_ARG, "p_ref_cell out of range");
+    sk::DevMesh &d = C.dm;
+    d.N = C.N; d.F = C.F; d.Fi = C.Fi;
+    d.owner = C.owner.p; d.neighbour = C.neighbour.p;
+    d.Sf = C.Sf.p; d.area = C.area.p; d.CN = C.CN.p; d.eCN = C.eCN.p; d.gDiff = C.gDiff.p;
+    d.T = C.T.p; d.gf = C.gf.p; d.fc = C.fc.p; d.vol = C.vol.p; d.cc = C.cc.p;
+    d.cptr = C.cptr.p; d.ccol = C.ccol.p; d.own_nnz = C.own_nnz.p; d.nei_nnz = C.nei_nnz.p;
+    d.cf_ptr = C.cf_ptr.p; d.cf_face = C.cf_face.p; d.bpatch = C.bpatch.p;
+    d.ptype = C.ptype.p; d.u_bc = C.u_bc.p; d.p_bc = C.p_bc.p; d.u_val = C.u_val.p;
+    init_fields(ctx, C);
+    C.sysU = create_system(ctx, C.N, nnz, m->cconn_ptr, m->cconn);
+    C.sysP = create_system(ctx, C.N, nnz, m->cconn_ptr, m->cconn);
+    UFVM_HIP(hipStreamSynchronize(s));
+    const int32_t id = ctx->next_id++;
+    ctx->cases[id] = std::shared_ptr<void>(Cp.release(), [](void *p) { delete static_cast<Case *>(p); });
+    *case_out = id;
+  });
+}
+
+int ufvm_case_destroy(ufvm_ctx *ctx, int32_t cas) {
+  return guard(ctx, [&] {
+    Case &C = case_of(ctx, cas);
+    UFVM_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->systems.erase(C.sysU);
+    ctx->systems.erase(C.sysP);
+    ctx->cases.erase(cas);
+  });
+}
+
+int ufvm_case_set_field(ufvm_ctx *ctx, int32_t cas, const char *name, const double *data,
+                        int64_t count) {
+  return guard(ctx, [&] {
+    FieldRef f = field_of(case_of(ctx, cas), name);
+    if (!data || count != (int64_t)f.n) throw Error(UFVM_ERR_ARG, "field size mismatch");
+    UFVM_HIP(hipMemcpyAsync(f.p, data, f.n * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+    UFVM_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int ufvm_case_get_field(ufvm_ctx *ctx, int32_t cas, const char *name, double *data,
+                        int64_t count) {
+  return guard(ctx, [&] {
+    FieldRef f = field_of(case_of(ctx, cas), name);
+    if (!data || count != (int64_t)f.n) throw Error(UFVM_ERR_ARG, "field size mismatch");
+    UFVM_HIP(hipMemcpyAsync(data, f.p, f.n * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    UFVM_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int ufvm_case_system(ufvm_ctx *ctx, int32_t cas, const char *eq, int32_t *sys_out) {
+  return guard(ctx, [&] {
+    Case &C = case_of(ctx, cas);
+    if (!sys_out) throw Error(UFVM_ERR_ARG, "null sys_out");
+    *sys_out = eq_of(eq) == 0 ? C.sysU : C.sysP;
+  });
+}
+
+int ufvm_update_fields(ufvm_ctx *ctx, int32_t cas) {
+  return guard(ctx, [&] {
+    update_fields(ctx, case_of(ctx, cas));
+    UFVM_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+int ufvm_assemble_equation(ufvm_ctx *ctx, int32_t cas, const char *eq, int32_t comp,
+                           double *rms, double *max) {
+  return guard(ctx, [&] { assemble(ctx, case_of(ctx, cas), eq_of(eq), comp, rms, max); });
+}
+
+int ufvm_solve_equation(ufvm_ctx *ctx, int32_t cas, const char *eq, double *r0, double *r1,
+                        int32_t *ncycles) {
+  return guard(ctx, [&] { solve(ctx, case_of(ctx, cas), eq_of(eq), r0, r1, ncycles); });
+}
+
+int ufvm_correct_equation(ufvm_ctx *ctx, int32_t cas, const char *eq, int32_t comp) {
+  return guard(ctx, [&] { correct(ctx, case_of(ctx, cas), eq_of(eq), comp); });
+}
+
+int ufvm_simple_iterate(ufvm_ctx *ctx, int32_t cas, int32_t n, double *rms) {
+  return guard(ctx, [&] {
+    Case &C = case_of(ctx, cas);
+    for (int32_t it = 0; it < n; ++it) {
+      update_fields(ctx, C);
+      double r[4];
+      for (int c = 0; c < 3; ++c) {
+        assemble(ctx, C, 0, c, &r[c], nullptr);
+        solve(ctx, C, 0, nullptr, nullptr, nullptr);
+        correct(ctx, C, 0, c);
+      }
+      for (int k = 0; k < C.cfg.n_correctors; ++k) {
+        assemble(ctx, C, 1, 0, &r[3], nullptr);
+        solve(ctx, C, 1, nullptr, nullptr, nullptr);
+        correct(ctx, C, 1, 0);
+      }
+      if (rms)
+        for (int q = 0; q < 4; ++q) rms[it * 4 + q] = r[q];
+    }
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,52 @@
+// simple_kernels.h -- gfx950 kernels of the SIMPLE outer iteration: field
+// boundary updates, Green-Gauss gradients, momentum and pressure-correction
+// assembly (Rhie-Chow), and the corrections.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ufvm {
+namespace sk {
+
+enum { PATCH_WALL = 0, PATCH_EMPTY = 1, PATCH_PATCH = 2 };
+enum { BC_FIXED_VALUE = 0, BC_ZERO_GRADIENT = 1, BC_EMPTY = 2 };
+
+// device pointers of the mesh, passed by value
+struct DevMesh {
+  int32_t N, F, Fi;
+  const int32_t *owner, *neighbour;
+  const double *Sf, *area, *CN, *eCN, *gDiff, *T, *gf, *fc, *vol, *cc;
+  const int32_t *cptr, *ccol, *own_nnz, *nei_nnz;
+  const int32_t *cf_ptr, *cf_face;   // faces of each cell, ascending
+  const int32_t *bpatch;             // patch of each boundary face
+  const int32_t *ptype, *u_bc, *p_bc;
+  const double *u_val;
+};
+
+// scalar slots written by the reductions of this file
+enum { S_UMAX = 0, S_UMIN = 1, S_SCALE = 2, S_RMAX = 3, S_RSQ = 4 };
+
+void update_u_bc(hipStream_t s, const DevMesh &m, double *U);
+void gradient(hipStream_t s, const DevMesh &m, const double *phi, int nc, double *grad);
+void gamma_faces(hipStream_t s, const DevMesh &m, double nu, double *gamma);
+void scale_u(hipStream_t s, const DevMesh &m, const double *U, double *partial,
+             unsigned *counter, double *scal);
+void assemble_u(hipStream_t s, const DevMesh &m, int comp, double urf, const double *U,
+                const double *gradU, const double *gradp, const double *mdot,
+                const double *gamma, double *ac, double *anb, double *bc, double *ac_old,
+                double *dphi, double *DU, double *DUT, double *partial, unsigned *counter,
+                double *scal);
+void correct_u(hipStream_t s, const DevMesh &m, int comp, const double *dphi, double *U);
+void rhie_chow(hipStream_t s, const DevMesh &m, double urf_u, const double *U,
+               const double *DU, const double *p, const double *gradp, double *mdot,
+               double *fluxC1, double *fluxC2);
+void assemble_p(hipStream_t s, const DevMesh &m, const double *fluxC1, const double *fluxC2,
+                const double *mdot, double *ac, double *anb, double *bc, double *ac_old,
+                double *dphi, double *partial, unsigned *counter, double *scal);
+void correct_p(hipStream_t s, const DevMesh &m, int32_t ref, double urf_p, const double *dphi,
+               double *PP, double *gradPP, const double *DU, double *U, double *p,
+               const double *fluxC1, const double *fluxC2, double *mdot);
+
+}  // namespace sk
+}  // namespace ufvm

@@ -1,0 +1,589 @@
+// simple_kernels.hip -- gfx950 kernels of the steady SIMPLE outer iteration.
+//
+// Assembly is cell-centric: one thread per cell walks the cell's faces in
+// ascending face order and accumulates ac, bc and its own anb row, so each
+// coefficient receives the face contributions in exactly the order of the
+// reference's face-loop scatter (uFVM/src/fvm/assemble/
+// cfdAssembleIntoGlobalMatrixFaceFluxes.m:25-55: interior faces ascending,
+// then boundary faces) with no atomics.  A face flux is evaluated by both of
+// its cells with identical arithmetic.  Built with -ffp-contract=off; every
+// expression keeps the reference's association.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "reduce.h"
+#include "simple_kernels.h"
+
+namespace ufvm {
+namespace sk {
+
+using k::kBlock;
+using k::nblocks;
+
+__device__ inline double mag3(const double *v) {
+  return sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]);
+}
+__device__ inline double dot3(const double *a, const double *b) {
+  return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2];
+}
+
+// Two per-block partial values (op 0: sum, 1: max, 2: min) written to
+// partial[2*block + c]; a one-block kernel then combines them in block order.
+__device__ inline void block_partials2(double v0, double v1, int op0, int op1, double *partial) {
+  __shared__ double lds[kBlock / k::kWave];
+  auto red = [&](double v, int op) {
+    if (op == 0) return k::block_sum(v, lds);
+    if (op == 1) return k::block_max(v, lds);
+    return -k::block_max(-v, lds);
+  };
+  v0 = red(v0, op0);
+  v1 = red(v1, op1);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = v0;
+    partial[2 * blockIdx.x + 1] = v1;
+  }
+}
+
+// mode 0: (max, min) of U -> scale; mode 1: (max, sum of squares) of the residual
+__global__ void __launch_bounds__(kBlock)
+k_reduce2(const double *__restrict__ partial, int32_t nblk, int mode, double *scal) {
+  __shared__ double lds[kBlock / k::kWave];
+  const int op0 = 1, op1 = mode == 0 ? 2 : 0;
+  double a0 = -INFINITY, a1 = op1 == 2 ? INFINITY : 0.0;
+  for (int32_t q = threadIdx.x; q < nblk; q += kBlock) {
+    a0 = fmax(a0, partial[2 * q]);
+    a1 = op1 == 2 ? fmin(a1, partial[2 * q + 1]) : a1 + partial[2 * q + 1];
+  }
+  a0 = k::block_max(a0, lds);
+  a1 = op1 == 2 ? -k::block_max(-a1, lds) : k::block_sum(a1, lds);
+  if (threadIdx.x != 0) return;
+  if (mode == 0) {
+    scal[S_UMAX] = a0;
+    scal[S_UMIN] = a1;
+    double sc = fmax(a0 - a1, fabs(a0));
+    if (sc < 1e-6) sc = 1.0;
+    scal[S_SCALE] = sc;
+  } else {
+    scal[S_RMAX] = a0;
+    scal[S_RSQ] = a1;
+  }
+}
+
+// ------------------------------------------------------ boundary conditions
+
+// fixedValue walls: U_b = value (cfdUpdateBoundaryConditions.m:103-139)
+__global__ void k_update_u_bc(DevMesh m, double *U) {
+  const int32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= m.F - m.Fi) return;
+  const int p = m.bpatch[k];
+  if (m.ptype[p] != PATCH_WALL || m.u_bc[p] != BC_FIXED_VALUE) return;
+  const int32_t b = m.N + k;
+  for (int d = 0; d < 3; ++d) U[b * 3 + d] = m.u_val[p * 3 + d] * 1.0;
+}
+
+void update_u_bc(hipStream_t s, const DevMesh &m, double *U) {
+  const int32_t Fb = m.F - m.Fi;
+  if (Fb > 0) hipLaunchKernelGGL(k_update_u_bc, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, U);
+}
+
+// ---------------------------------------------------------------- gradient
+
+// Green-Gauss, cfdComputeGradientGauss0.m:28-80.  Per (cell, component,
+// direction) the contributions arrive in ascending face order: interior
+// faces, then boundary faces.  The boundary contribution uses the boundary
+// value of component 1 for every component (linear indexing phi_b(k) into
+// the [Fb x nc] block, :59-64).
+template <int NC>
+__global__ void __launch_bounds__(kBlock) k_gauss_cells(DevMesh m, const double *phi, double *grad) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= m.N) return;
+  double g[NC * 3];
+#pragma unroll
+  for (int q = 0; q < NC * 3; ++q) g[q] = 0.0;
+  for (int32_t e = m.cf_ptr[i]; e < m.cf_ptr[i + 1]; ++e) {
+    const int32_t f = m.cf_face[e];
+    const double *S = &m.Sf[f * 3];
+    if (f < m.Fi) {
+      const int32_t o = m.owner[f], nb = m.neighbour[f];
+      const double w = m.gf[f];
+      const bool own = (o == i);
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const double phif = w * phi[nb * NC + c] + (1 - w) * phi[o * NC + c];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          if (own) g[c * 3 + d] = g[c * 3 + d] + phif * S[d];
+          else g[c * 3 + d] = g[c * 3 + d] - phif * S[d];
+        }
+      }
+    } else {
+      const double phib = phi[(m.N + (f - m.Fi)) * NC + 0];
+#pragma unroll
+      for (int c = 0; c < NC; ++c)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) g[c * 3 + d] = g[c * 3 + d] + phib * S[d];
+    }
+  }
+  const double v = m.vol[i];
+#pragma unroll
+  for (int q = 0; q < NC * 3; ++q) grad[i * NC * 3 + q] = g[q] / v;
+}
+
+// boundary rows: the owner's gradient, then on wall and 'patch' patches
+// g - (g.e)e + ((phi_b - phi_O)/|d|) e with d = C_f - C_O
+// (cfdUpdateGradient.m:104-134)
+template <int NC>
+__global__ void k_gauss_bnd(DevMesh m, const double *phi, double *grad) {
+  const int32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= m.F - m.Fi) return;
+  const int32_t f = m.Fi + k, o = m.owner[f], b = m.N + k;
+  const int p = m.bpatch[k];
+  const bool wall = (m.ptype[p] == PATCH_WALL || m.ptype[p] == PATCH_PATCH);
+  double d[3], e[3], dmag = 1.0;
+  if (wall) {
+    for (int q = 0; q < 3; ++q) d[q] = m.fc[f * 3 + q] - m.cc[o * 3 + q];
+    dmag = mag3(d);
+    for (int q = 0; q < 3; ++q) e[q] = d[q] / dmag;
+  }
+  for (int c = 0; c < NC; ++c) {
+    const double *g = &grad[(o * NC + c) * 3];
+    if (!wall) {
+      for (int q = 0; q < 3; ++q) grad[(b * NC + c) * 3 + q] = g[q];
+    } else {
+      const double ge = dot3(g, e);
+      const double t = (phi[b * NC + c] - phi[o * NC + c]) / dmag;
+      for (int q = 0; q < 3; ++q) grad[(b * NC + c) * 3 + q] = (g[q] - ge * e[q]) + t * e[q];
+    }
+  }
+}
+
+void gradient(hipStream_t s, const DevMesh &m, const double *phi, int nc, double *grad) {
+  const int32_t Fb = m.F - m.Fi;
+  if (nc == 3) {
+    hipLaunchKernelGGL(k_gauss_cells<3>, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, phi, grad);
+    if (Fb > 0) hipLaunchKernelGGL(k_gauss_bnd<3>, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, phi, grad);
+  } else {
+    hipLaunchKernelGGL(k_gauss_cells<1>, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, phi, grad);
+    if (Fb > 0) hipLaunchKernelGGL(k_gauss_bnd<1>, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, phi, grad);
+  }
+}
+
+// gamma_Ueq = Average(nu) on faces (cfdUpdateTermCoefficientField.m:44-70)
+__global__ void k_gamma(DevMesh m, double nu, double *gamma) {
+  const int32_t f = blockIdx.x * kBlock + threadIdx.x;
+  if (f >= m.F) return;
+  if (f < m.Fi) {
+    const double w = m.gf[f];
+    gamma[f] = w * nu + (1 - w) * nu;
+  } else {
+    gamma[f] = nu;
+  }
+}
+
+void gamma_faces(hipStream_t s, const DevMesh &m, double nu, double *gamma) {
+  hipLaunchKernelGGL(k_gamma, dim3(nblocks(m.F)), dim3(kBlock), 0, s, m, nu, gamma);
+}
+
+// phiScale = max(phiMax - phiMin, |phiMax|), 1 if < 1e-6 (cfdUpdateScale.m:17-23)
+__global__ void __launch_bounds__(kBlock) k_scale_u(int64_t n, const double *U, double *partial) {
+  const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double mx = -INFINITY, mn = INFINITY;
+  if (q < n) mx = mn = U[q];
+  block_partials2(mx, mn, 1, 2, partial);
+}
+
+void scale_u(hipStream_t s, const DevMesh &m, const double *U, double *partial,
+             unsigned *counter, double *scal) {
+  (void)counter;
+  const int64_t n = (int64_t)(m.N + m.F - m.Fi) * 3;
+  hipLaunchKernelGGL(k_scale_u, dim3(nblocks(n)), dim3(kBlock), 0, s, n, U, partial);
+  hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kBlock), 0, s, partial, nblocks(n), 0, scal);
+}
+
+// --------------------------------------------------------- momentum assembly
+
+// Average:Corrected face gradient of component c
+// (cfdInterpolateGradientsFromElementsToInteriorFaces.m:46-70; the average
+// weights are swapped relative to value interpolation, :48-50)
+template <int NC>
+__device__ inline void grad_face(const DevMesh &m, const double *phi, const double *grad, int c,
+                                 int32_t f, double *out) {
+  const int32_t o = m.owner[f], nb = m.neighbour[f];
+  const double w = m.gf[f];
+  double gfc[3], dcf[3], e[3];
+  for (int d = 0; d < 3; ++d)
+    gfc[d] = (1 - w) * grad[(nb * NC + c) * 3 + d] + w * grad[(o * NC + c) * 3 + d];
+  for (int d = 0; d < 3; ++d) dcf[d] = m.cc[nb * 3 + d] - m.cc[o * 3 + d];
+  const double dmag = mag3(dcf);
+  for (int d = 0; d < 3; ++d) e[d] = dcf[d] / dmag;
+  const double lgm = (phi[nb * NC + c] - phi[o * NC + c]) / dmag;
+  const double lam = dot3(gfc, e);
+  for (int d = 0; d < 3; ++d) out[d] = (gfc[d] - lam * e[d]) + lgm * e[d];
+}
+
+// One momentum component c, terms in equation order
+// (cfdAssembleEquationTerms.m:22-55):
+//   convection, sign +1: upwind C1 = max(mdot,0), C2 = -max(-mdot,0), walls
+//     and empty patches zero (cfdAssembleConvectionTerm.m:103-137, :238-278),
+//     then the divergence term per cell (cfdAssembleConvectionDivergenceTerm.m:38-47);
+//   diffusion laplacian(nu*U), sign -1: interior with corrected face gradient
+//     (cfdAssembleDiffusionTerm.m:94-130), fixedValue walls (:224-263),
+//     zeroGradient (:330-347), empty vector patches (:631-690);
+//   -grad(p), sign +1 (cfdAssemblePressureGradientTerm.m:31-40);
+//   false transient, fdt = 1e9, rho = 1 (cfdAssembleFalseTransientTerm.m:28-49);
+//   URF (cfdApplyURF.m:15-17); normalised residual
+//   (cfdComputeNormalizedResidual.m:64-88); DU = V/ac, DUT = ac_old/ac
+//   (cfdPostAssembleEquation.m:37-41).
+__global__ void __launch_bounds__(kBlock)
+k_assemble_u(DevMesh m, int c, double urf, const double *__restrict__ U,
+             const double *__restrict__ gradU, const double *__restrict__ gradp,
+             const double *__restrict__ mdot, const double *__restrict__ gamma, double *ac_out,
+             double *anb, double *bc_out, double *acold_out, double *dphi, double *DU,
+             double *DUT, double *partial, unsigned *counter, double *scal) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  double red[2] = {0.0, 0.0};
+  if (i < m.N) {
+    const int32_t N = m.N, Fi = m.Fi;
+    for (int32_t q = m.cptr[i]; q < m.cptr[i + 1]; ++q) anb[q] = 0.0;
+    double ac = 0.0, bc = 0.0, acold = 0.0;
+    const int32_t e0 = m.cf_ptr[i], e1 = m.cf_ptr[i + 1];
+    // convection
+    for (int32_t e = e0; e < e1; ++e) {
+      const int32_t f = m.cf_face[e];
+      if (f < Fi) {
+        const double psi = mdot[f];
+        const double C1 = 1.0 * fmax(psi, 0.0);
+        const double C2 = (-1.0) * fmax(-psi, 0.0);
+        const double T = (C1 * U[m.owner[f] * 3 + c] + C2 * U[m.neighbour[f] * 3 + c]) + 0.0;
+        if (m.owner[f] == i) {
+          ac = ac + C1;
+          anb[m.own_nnz[f]] = anb[m.own_nnz[f]] + C2;
+          bc = bc - T;
+        } else {
+          ac = ac - C2;
+          anb[m.nei_nnz[f]] = anb[m.nei_nnz[f]] - C1;
+          bc = bc + T;
+        }
+      } else {
+        ac = ac + 0.0;
+        bc = bc - 0.0;
+      }
+    }
+    {
+      double eff = 0.0;
+      for (int32_t e = e0; e < e1; ++e) {
+        const int32_t f = m.cf_face[e];
+        if (f < Fi && m.neighbour[f] == i) eff = eff - mdot[f];
+        else eff = eff + mdot[f];
+      }
+      const double phi = U[i * 3 + c];
+      const double CE = fmax(eff, 0.0) - eff;
+      const double VE = (-fmax(eff, 0.0)) * phi;
+      const double TE = CE * phi + VE;
+      ac = ac + CE;
+      bc = bc - TE;
+    }
+    // diffusion
+    for (int32_t e = e0; e < e1; ++e) {
+      const int32_t f = m.cf_face[e];
+      const double gam = gamma[f];
+      if (f < Fi) {
+        double gfc[3];
+        grad_face<3>(m, U, gradU, c, f, gfc);
+        const double C1 = (1.0 * gam) * m.gDiff[f];
+        const double C2 = ((-1.0) * gam) * m.gDiff[f];
+        const double V = ((-1.0) * gam) * dot3(gfc, &m.T[f * 3]);
+        const double T = (C1 * U[m.owner[f] * 3 + c] + C2 * U[m.neighbour[f] * 3 + c]) + V;
+        if (m.owner[f] == i) {
+          ac = ac + C1;
+          anb[m.own_nnz[f]] = anb[m.own_nnz[f]] + C2;
+          bc = bc - T;
+        } else {
+          ac = ac - C2;
+          anb[m.nei_nnz[f]] = anb[m.nei_nnz[f]] - C1;
+          bc = bc + T;
+        }
+      } else {
+        const int32_t k = f - Fi, b = N + k;
+        const int p = m.bpatch[k];
+        double C1 = 0.0, T = 0.0;
+        if (m.ptype[p] == PATCH_WALL && m.u_bc[p] == BC_FIXED_VALUE) {
+          C1 = (1.0 * gam) * m.gDiff[f];
+          const double C2 = ((-1.0) * gam) * m.gDiff[f];
+          const double V = ((-1.0) * gam) * dot3(&gradU[(b * 3 + c) * 3], &m.T[f * 3]);
+          T = (C1 * U[i * 3 + c] + C2 * U[b * 3 + c]) + V;
+        } else if (m.ptype[p] == PATCH_EMPTY) {
+          const double A = m.area[f];
+          double n[3], l[3] = {1.0, 1.0, 1.0}, a[3];
+          l[c] = 0.0;
+          for (int d = 0; d < 3; ++d) n[d] = m.Sf[f * 3 + d] / A;
+          const double dd = dot3(&m.CN[f * 3], n);
+          const double nc = n[c];
+          C1 = ((((1.0 * 2) * gam) * A) * (nc * nc)) / dd;
+          const double C2 = 0.0;
+          for (int d = 0; d < 3; ++d) a[d] = (U[i * 3 + d] * l[d]) * (n[d] * l[d]);
+          const double dl = (a[0] + a[1]) + a[2];
+          const double V = (((((-1.0) * 2) * gam) * A) * dl) * nc / dd;
+          T = (C1 * U[i * 3 + c] + C2 * U[b * 3 + c]) + V;
+        }
+        ac = ac + C1;
+        bc = bc - T;
+      }
+    }
+    const double vol = m.vol[i];
+    // -grad(p)
+    {
+      const double TE = (1.0 * vol) * gradp[i * 3 + c];
+      ac = ac + 0.0;
+      acold = acold + 0.0;
+      bc = bc - TE;
+    }
+    // false transient
+    {
+      const double fdt = 1e9;
+      const double CE = (vol * 1.0) / fdt;
+      const double CEOLD = ((-vol) * 1.0) / fdt;
+      const double TE = CE * U[i * 3 + c];
+      ac = ac + CE;
+      acold = acold + CEOLD;
+      bc = bc - TE;
+    }
+    ac = ac / urf;
+    ac_out[i] = ac;
+    bc_out[i] = bc;
+    acold_out[i] = acold;
+    dphi[i] = 0.0;
+    const double r = bc / (ac * scal[S_SCALE]);
+    red[0] = fabs(r);
+    red[1] = r * r;
+    DU[i * 3 + c] = vol / ac;
+    DUT[i * 3 + c] = acold / ac;
+  }
+  (void)counter;
+  block_partials2(red[0], red[1], 1, 0, partial);
+}
+
+// boundary rows of DU / DUT from their owners (cfdPostAssembleEquation.m:75-81)
+__global__ void k_du_bnd(DevMesh m, int c, double *DU, double *DUT) {
+  const int32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= m.F - m.Fi) return;
+  const int32_t o = m.owner[m.Fi + k], b = m.N + k;
+  DU[b * 3 + c] = DU[o * 3 + c];
+  DUT[b * 3 + c] = DUT[o * 3 + c];
+}
+
+void assemble_u(hipStream_t s, const DevMesh &m, int comp, double urf, const double *U,
+                const double *gradU, const double *gradp, const double *mdot,
+                const double *gamma, double *ac, double *anb, double *bc, double *ac_old,
+                double *dphi, double *DU, double *DUT, double *partial, unsigned *counter,
+                double *scal) {
+  hipLaunchKernelGGL(k_assemble_u, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, comp, urf, U, gradU,
+                     gradp, mdot, gamma, ac, anb, bc, ac_old, dphi, DU, DUT, partial, counter,
+                     scal);
+  hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kBlock), 0, s, partial, nblocks(m.N), 1, scal);
+  const int32_t Fb = m.F - m.Fi;
+  if (Fb > 0) hipLaunchKernelGGL(k_du_bnd, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, comp, DU, DUT);
+}
+
+// U(:,c) += dphi (cfdCorrectScalarEquation.m:16-24)
+__global__ void k_correct_u_cells(DevMesh m, int c, const double *dphi, double *U) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < m.N) U[i * 3 + c] = U[i * 3 + c] + dphi[i];
+}
+
+// empty patches: U_b,c = U_O,c - (U_O . n) n_c (cfdCorrectScalarEquation.m, cfdCorrectEmptyBC);
+// comp < 0: all three components (cfdCorrectVelocityField.m, empty patches)
+__global__ void k_empty_u(DevMesh m, int c, double *U) {
+  const int32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= m.F - m.Fi) return;
+  if (m.ptype[m.bpatch[k]] != PATCH_EMPTY) return;
+  const int32_t f = m.Fi + k, o = m.owner[f], b = m.N + k;
+  double n[3];
+  const double nm = mag3(&m.Sf[f * 3]);
+  for (int d = 0; d < 3; ++d) n[d] = m.Sf[f * 3 + d] / nm;
+  const double un = dot3(&U[o * 3], n);
+  if (c >= 0) {
+    U[b * 3 + c] = U[o * 3 + c] - un * n[c];
+  } else {
+    for (int d = 0; d < 3; ++d) U[b * 3 + d] = U[o * 3 + d] - un * n[d];
+  }
+}
+
+void correct_u(hipStream_t s, const DevMesh &m, int comp, const double *dphi, double *U) {
+  hipLaunchKernelGGL(k_correct_u_cells, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, comp, dphi, U);
+  const int32_t Fb = m.F - m.Fi;
+  if (Fb > 0) hipLaunchKernelGGL(k_empty_u, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, comp, U);
+}
+
+// ------------------------------------------------- pressure-correction path
+
+// Rhie-Chow face flux of every interior face (cfdAssembleMdotTerm.m:70-212):
+// mdot_f = C1 p_O + C2 p_N + V with C1 = rho_f geoDiff = -C2 and
+// V = rho_f Ubar.S - rho_f grad(p)_f.DUT + rho_f grad(p)bar_f.DUS
+//     + (1-urf_U)(mdot_prev - rho_f Ubar.S);  boundary fluxes of walls and
+// empty patches are zero (:281-350).
+__global__ void __launch_bounds__(kBlock)
+k_rhie_chow(DevMesh m, double urf_u, const double *__restrict__ U, const double *__restrict__ DU,
+            const double *__restrict__ p, const double *__restrict__ gradp, double *mdot,
+            double *fluxC1, double *fluxC2) {
+  const int32_t f = blockIdx.x * kBlock + threadIdx.x;
+  if (f >= m.F) return;
+  if (f >= m.Fi) {
+    fluxC1[f] = 0.0;
+    fluxC2[f] = 0.0;
+    return;
+  }
+  const int32_t o = m.owner[f], nb = m.neighbour[f];
+  const double w = m.gf[f];
+  double du[3], ub[3], gpb[3], gpf[3];
+  for (int d = 0; d < 3; ++d) du[d] = w * DU[nb * 3 + d] + (1 - w) * DU[o * 3 + d];
+  const double mprev = mdot[f];
+  for (int d = 0; d < 3; ++d) ub[d] = w * U[nb * 3 + d] + (1 - w) * U[o * 3 + d];
+  const double pos = mprev > 0 ? 1.0 : 0.0;
+  const double rho_f = 1.0 * pos + 1.0 * (1 - pos);
+  for (int d = 0; d < 3; ++d) gpb[d] = (1 - w) * gradp[nb * 3 + d] + w * gradp[o * 3 + d];
+  grad_face<1>(m, p, gradp, 0, f, gpf);
+  const double *Sf = &m.Sf[f * 3], *CF = &m.CN[f * 3], *eCF = &m.eCN[f * 3];
+  const double UbS = dot3(ub, Sf);
+  double V = 0.0 + rho_f * UbS;
+  double DUS[3], eDUS[3], DUE[3], DUT[3];
+  for (int d = 0; d < 3; ++d) DUS[d] = du[d] * Sf[d];
+  const double mDUS = mag3(DUS);
+  for (int d = 0; d < 3; ++d) eDUS[d] = DUS[d] / mDUS;
+  const double ee = dot3(eCF, eDUS);
+  for (int d = 0; d < 3; ++d) DUE[d] = mDUS * eCF[d] / ee;
+  const double geoDiff = mag3(DUE) / mag3(CF);
+  for (int d = 0; d < 3; ++d) DUT[d] = DUS[d] - DUE[d];
+  const double FC = 0.0 + rho_f * geoDiff;
+  const double FF = 0.0 - rho_f * geoDiff;
+  V = V - rho_f * dot3(gpf, DUT);
+  V = V + rho_f * dot3(gpb, DUS);
+  V = V + (1 - urf_u) * (mprev - rho_f * UbS);
+  mdot[f] = (FC * p[o] + FF * p[nb]) + V;
+  fluxC1[f] = FC;
+  fluxC2[f] = FF;
+}
+
+void rhie_chow(hipStream_t s, const DevMesh &m, double urf_u, const double *U,
+               const double *DU, const double *p, const double *gradp, double *mdot,
+               double *fluxC1, double *fluxC2) {
+  hipLaunchKernelGGL(k_rhie_chow, dim3(nblocks(m.F)), dim3(kBlock), 0, s, m, urf_u, U, DU, p,
+                     gradp, mdot, fluxC1, fluxC2);
+}
+
+// pressure-correction matrix from the face fluxes (T_f = mdot_f), element
+// fluxes all zero, residual = effective divergence of the new mdot
+// (cfdComputeNormalizedResidual.m:27-30: max and rms, not of |.|)
+__global__ void __launch_bounds__(kBlock)
+k_assemble_p(DevMesh m, const double *__restrict__ fluxC1, const double *__restrict__ fluxC2,
+             const double *__restrict__ mdot, double *ac_out, double *anb, double *bc_out,
+             double *acold_out, double *dphi, double *partial, unsigned *counter, double *scal) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  double red[2] = {-INFINITY, 0.0};
+  if (i < m.N) {
+    for (int32_t q = m.cptr[i]; q < m.cptr[i + 1]; ++q) anb[q] = 0.0;
+    double ac = 0.0, bc = 0.0, acold = 0.0, eff = 0.0;
+    for (int32_t e = m.cf_ptr[i]; e < m.cf_ptr[i + 1]; ++e) {
+      const int32_t f = m.cf_face[e];
+      if (f < m.Fi) {
+        const double C1 = fluxC1[f], C2 = fluxC2[f], T = mdot[f];
+        if (m.owner[f] == i) {
+          ac = ac + C1;
+          anb[m.own_nnz[f]] = anb[m.own_nnz[f]] + C2;
+          bc = bc - T;
+          eff = eff + T;
+        } else {
+          ac = ac - C2;
+          anb[m.nei_nnz[f]] = anb[m.nei_nnz[f]] - C1;
+          bc = bc + T;
+          eff = eff - T;
+        }
+      } else {
+        ac = ac + 0.0;
+        bc = bc - 0.0;
+        eff = eff + mdot[f];
+      }
+    }
+    ac = ac + 0.0;
+    acold = acold + 0.0;
+    bc = bc - 0.0;
+    ac_out[i] = ac;
+    bc_out[i] = bc;
+    acold_out[i] = acold;
+    dphi[i] = 0.0;
+    red[0] = eff;
+    red[1] = eff * eff;
+  }
+  (void)counter;
+  (void)scal;
+  block_partials2(red[0], red[1], 1, 0, partial);
+}
+
+void assemble_p(hipStream_t s, const DevMesh &m, const double *fluxC1, const double *fluxC2,
+                const double *mdot, double *ac, double *anb, double *bc, double *ac_old,
+                double *dphi, double *partial, unsigned *counter, double *scal) {
+  hipLaunchKernelGGL(k_assemble_p, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, fluxC1, fluxC2,
+                     mdot, ac, anb, bc, ac_old, dphi, partial, counter, scal);
+  hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kBlock), 0, s, partial, nblocks(m.N), 1, scal);
+}
+
+// PP = dphi - dphi(ref) (cfdCorrectPPField.m:26-52)
+__global__ void k_pp_cells(DevMesh m, int32_t ref, const double *dphi, double *PP) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= m.N) return;
+  const double ppc = ref >= 0 ? dphi[ref] : 0.0;
+  PP[i] = dphi[i] - ppc;
+}
+
+// boundary value = owner value (wall, empty patches of PP and p)
+__global__ void k_copy_owner(DevMesh m, double *phi) {
+  const int32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= m.F - m.Fi) return;
+  phi[m.N + k] = phi[m.owner[m.Fi + k]];
+}
+
+// U -= DU o grad(PP) on cells (cfdCorrectVelocityField.m:26-55)
+__global__ void k_correct_u_pp(DevMesh m, const double *DU, const double *gradPP, double *U) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= m.N) return;
+  for (int d = 0; d < 3; ++d) U[i * 3 + d] = U[i * 3 + d] - DU[i * 3 + d] * gradPP[i * 3 + d];
+}
+
+// p += urf_p (PP - PP(ref)) (cfdCorrectPressureEquation.m:22-54)
+__global__ void k_correct_p_cells(DevMesh m, int32_t ref, double urf_p, const double *PP,
+                                  double *p) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= m.N) return;
+  const double ppf = ref >= 0 ? PP[ref] : 0.0;
+  p[i] = p[i] + urf_p * (PP[i] - ppf);
+}
+
+// mdot += C1 PP_O + C2 PP_N; empty patches: mdot = 0 mdot (cfdCorrectMdotField.m:24-50, :328-346)
+__global__ void k_correct_mdot(DevMesh m, const double *fluxC1, const double *fluxC2,
+                               const double *PP, double *mdot) {
+  const int32_t f = blockIdx.x * kBlock + threadIdx.x;
+  if (f >= m.F) return;
+  if (f < m.Fi) {
+    mdot[f] = (mdot[f] + fluxC1[f] * PP[m.owner[f]]) + fluxC2[f] * PP[m.neighbour[f]];
+  } else if (m.ptype[m.bpatch[f - m.Fi]] == PATCH_EMPTY) {
+    mdot[f] = 0.0 * mdot[f];
+  }
+}
+
+void correct_p(hipStream_t s, const DevMesh &m, int32_t ref, double urf_p, const double *dphi,
+               double *PP, double *gradPP, const double *DU, double *U, double *p,
+               const double *fluxC1, const double *fluxC2, double *mdot) {
+  const int32_t Fb = m.F - m.Fi;
+  hipLaunchKernelGGL(k_pp_cells, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, ref, dphi, PP);
+  if (Fb > 0) hipLaunchKernelGGL(k_copy_owner, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, PP);
+  gradient(s, m, PP, 1, gradPP);
+  hipLaunchKernelGGL(k_correct_u_pp, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, DU, gradPP, U);
+  if (Fb > 0) hipLaunchKernelGGL(k_empty_u, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, -1, U);
+  hipLaunchKernelGGL(k_correct_p_cells, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, ref, urf_p, PP, p);
+  if (Fb > 0) hipLaunchKernelGGL(k_copy_owner, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, p);
+  hipLaunchKernelGGL(k_correct_mdot, dim3(nblocks(m.F)), dim3(kBlock), 0, s, m, fluxC1, fluxC2, PP,
+                     mdot);
+}
+
+}  // namespace sk
+}  // namespace ufvm

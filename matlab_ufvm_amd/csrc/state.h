@@ -83,13 +83,20 @@ struct RefLevel {
 // colour class is a contiguous row range, CSR kept in cconn order per row.
 struct FastLevel {
   int32_t n = 0, nnz = 0, ncol = 0;
-  std::vector<int32_t> col_off;   // colour c rows: [col_off[c], col_off[c+1])
+  // kind 0: global colouring, colour classes contiguous (level 1)
+  // kind 1: blocks of at most kBRows rows, block-local colouring (coarse levels)
+  // kind 2: coarsest level, solved densely
+  int kind = 0;
+  std::vector<int32_t> col_off;   // kind 0: colour c rows are [col_off[c], col_off[c+1])
+  int32_t nblk = 0, bncol = 0;    // kind 1: blocks and colours per block
+  std::vector<int32_t> h_bptr, h_boff;
+  DBuf<int32_t> bptr, boff;
   std::vector<int32_t> colour;    // reference numbering
   std::vector<int32_t> perm;      // perm[new] = reference index
   std::vector<int32_t> h_ptr;     // permuted CSR row pointer (host copy)
   DBuf<int32_t> ptr, col, perm_d;
-  DBuf<float> ac32, anb32, b32, x32, r32;
-  DBuf<double> ac64, anb64, b64, x64, r64;
+  DBuf<float> ac32, anb32, b32, x32, r32, y32;   // y: second iterate buffer (kind 1)
+  DBuf<double> ac64, anb64, b64, x64, r64, y64;
   DBuf<int32_t> parent, child_ptr, child;   // to / from the next level (new numbering)
 };
 
@@ -128,7 +135,11 @@ struct ufvm_ctx {
   int32_t next_id = 1;
   ufvm::DBuf<double> partial;   // per-block partial sums
   ufvm::DBuf<double> scal;      // device scalars
+  ufvm::DBuf<unsigned> counter; // last-block reduction counter (kept at zero between launches)
   double *hscal = nullptr;      // pinned host mirror of scal
+  // SIMPLE cases of this context (type-erased; owned here so that destroying
+  // the context releases them)
+  std::map<int32_t, std::shared_ptr<void>> cases;
   bool prof_on = false;
   std::vector<ufvm::ProfRec> prof;
   std::vector<hipEvent_t> event_pool;

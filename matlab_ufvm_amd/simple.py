@@ -1,0 +1,158 @@
+"""Host side of the SIMPLE outer iteration, mirroring the reference's calls.
+
+The reference drives one steady iteration from MATLAB
+(uFVM/src/fvm/top/cfdRunCase.m:92-142):
+
+    cfdUpdateFields
+    for U components: cfdAssembleEquation / cfdSolveEquation / cfdCorrectEquation
+    nCorrectors times for p: the same three calls
+
+`SimpleCase` exposes those calls with the same names and meaning on a mesh
+resident on the GPU (the fields, the two linear systems and the AMG hierarchy
+stay in HBM); every method goes through the C ABI of libufvm.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import AmgCfg, MeshDesc, PcgCfg, SimpleCfg, f64p, i32p
+
+PATCH = {"wall": 0, "empty": 1}
+BC = {"fixedValue": 0, "zeroGradient": 1, "empty": 2}
+P_SOLVER = {"amg": 0, "pcg": 1, "pcg-frozen": 2}
+
+
+def cavity_bcs(mesh, lid=(1.0, 0.0, 0.0)):
+    """Lid-driven cavity: fixedValue U on walls (lid moving), zeroGradient p,
+    empty front/back patches of 2-D meshes."""
+    out = {}
+    for p in mesh.patches:
+        if p.type == "empty":
+            out[p.name] = ("empty", (0.0, 0.0, 0.0), "empty")
+        elif p.name == "movingWall":
+            out[p.name] = ("fixedValue", tuple(lid), "zeroGradient")
+        else:
+            out[p.name] = ("fixedValue", (0.0, 0.0, 0.0), "zeroGradient")
+    return out
+
+
+class SimpleCase:
+    def __init__(self, ctx: _lib.Context, mesh, bcs, nu, urf_u=0.7, urf_p=0.3, n_correctors=2,
+                 p_ref=0, u_sweeps=10, p_solver="amg", p_cycle="W-Cycle", p_max_cycles=30,
+                 p_termination=0.1, p_max_coarse_levels=10, p_pre=1, p_post=3, p_smoother="ILU",
+                 pcg_precision=32, pcg_tol=1e-8, pcg_sweeps=1, pcg_max_coarse_cells=2000):
+        self.ctx = ctx
+        self.mesh = m = mesh
+        keep = self._keep = []
+
+        def a(x, t):
+            arr = np.ascontiguousarray(x, dtype=np.int32 if t is i32p else np.float64)
+            keep.append(arr)
+            return arr.ctypes.data_as(t)
+
+        d = MeshDesc()
+        d.n_cells, d.n_faces, d.n_interior = m.n_cells, m.n_faces, m.n_interior
+        d.owner, d.neighbour = a(m.owner, i32p), a(m.neighbour, i32p)
+        d.Sf, d.area, d.CN, d.eCN = a(m.Sf, f64p), a(m.area, f64p), a(m.CN, f64p), a(m.eCN, f64p)
+        d.gDiff, d.T, d.gf = a(m.gDiff, f64p), a(m.T, f64p), a(m.gf, f64p)
+        d.face_centroid = a(m.face_centroid, f64p)
+        d.volume, d.centroid = a(m.volume, f64p), a(m.centroid, f64p)
+        d.cconn_ptr, d.cconn = a(m.cconn_ptr, i32p), a(m.cconn, i32p)
+        d.own_nnz, d.nei_nnz = a(m.own_nnz, i32p), a(m.nei_nnz, i32p)
+        d.n_patches = len(m.patches)
+        d.patch_start = a([p.start for p in m.patches], i32p)
+        d.patch_nfaces = a([p.nfaces for p in m.patches], i32p)
+        d.patch_type = a([PATCH[p.type] for p in m.patches], i32p)
+        c = SimpleCfg()
+        c.nu, c.urf_u, c.urf_p = nu, urf_u, urf_p
+        c.n_correctors, c.p_ref_cell, c.u_sweeps = n_correctors, p_ref, u_sweeps
+        c.u_bc = a([BC[bcs[p.name][0]] for p in m.patches], i32p)
+        c.u_value = a(np.concatenate([np.asarray(bcs[p.name][1], float) for p in m.patches]), f64p)
+        c.p_bc = a([BC[bcs[p.name][2]] for p in m.patches], i32p)
+        c.p_solver = P_SOLVER[p_solver]
+        c.p_amg = AmgCfg(_lib.CYCLE[p_cycle], p_max_cycles, p_termination, p_max_coarse_levels,
+                         p_pre, p_post, _lib.SMOOTHER[p_smoother], 1)
+        c.p_pcg = PcgCfg(pcg_precision, pcg_sweeps, pcg_sweeps, 1, pcg_max_coarse_cells, -1, 500,
+                         pcg_tol)
+        cid = C.c_int32()
+        ctx.check(ctx.lib.ufvm_case_create(ctx.h, C.byref(d), C.byref(c), C.byref(cid)))
+        self.id = cid.value
+        self.n_correctors = n_correctors
+        self._keep = None
+
+    def close(self):
+        if self.id:
+            self.ctx.check(self.ctx.lib.ufvm_case_destroy(self.ctx.h, self.id))
+            self.id = 0
+
+    # -------------------------------------------------- reference call surface
+    def cfdUpdateFields(self):
+        self.ctx.check(self.ctx.lib.ufvm_update_fields(self.ctx.h, self.id))
+
+    def cfdAssembleEquation(self, name, comp=0):
+        rms, mx = C.c_double(), C.c_double()
+        self.ctx.check(self.ctx.lib.ufvm_assemble_equation(self.ctx.h, self.id, name.encode(),
+                                                           comp, C.byref(rms), C.byref(mx)))
+        return rms.value, mx.value
+
+    def cfdSolveEquation(self, name):
+        r0, r1, nc = C.c_double(), C.c_double(), C.c_int32()
+        self.ctx.check(self.ctx.lib.ufvm_solve_equation(self.ctx.h, self.id, name.encode(),
+                                                        C.byref(r0), C.byref(r1), C.byref(nc)))
+        return r0.value, r1.value, nc.value
+
+    def cfdCorrectEquation(self, name, comp=0):
+        self.ctx.check(self.ctx.lib.ufvm_correct_equation(self.ctx.h, self.id, name.encode(),
+                                                          comp))
+
+    def cfdAssembleAndCorrectEquation(self, name):
+        """cfdAssembleAndCorrectEquation.m:35-49: rms/max of each component."""
+        ncomp = 3 if name == "U" else 1
+        ncorr = self.n_correctors if name == "p" else 1
+        rms, mx = [0.0] * ncomp, [0.0] * ncomp
+        for c in range(ncomp):
+            for _ in range(ncorr):
+                rms[c], mx[c] = self.cfdAssembleEquation(name, c)
+                self.cfdSolveEquation(name)
+                self.cfdCorrectEquation(name, c)
+        return rms, mx
+
+    def iterate(self, n=1):
+        """n steady outer iterations (cfdRunCase.m:92-142); RMS residuals [n, 4]."""
+        out = np.zeros((n, 4))
+        self.ctx.check(self.ctx.lib.ufvm_simple_iterate(self.ctx.h, self.id, n,
+                                                        out.ctypes.data_as(f64p)))
+        return out
+
+    # ----------------------------------------------------------------- fields
+    def _size(self, name):
+        m = self.mesh
+        nb = m.n_cells + m.n_bfaces
+        return {"U": nb * 3, "gradU": nb * 9, "p": nb, "gradp": nb * 3, "PP": nb,
+                "gradPP": nb * 3, "DU": nb * 3, "DUT": nb * 3, "mdot_f": m.n_faces,
+                "gamma_f": m.n_faces}[name]
+
+    def field(self, name):
+        n = self._size(name)
+        out = np.empty(n)
+        self.ctx.check(self.ctx.lib.ufvm_case_get_field(self.ctx.h, self.id, name.encode(),
+                                                        out.ctypes.data_as(f64p), n))
+        m = self.mesh
+        nb = m.n_cells + m.n_bfaces
+        shape = {"U": (nb, 3), "gradU": (nb, 3, 3), "gradp": (nb, 3), "gradPP": (nb, 3),
+                 "DU": (nb, 3), "DUT": (nb, 3)}.get(name, (n,))
+        return out.reshape(shape)
+
+    def set_field(self, name, values):
+        v = np.ascontiguousarray(values, dtype=np.float64).reshape(-1)
+        self.ctx.check(self.ctx.lib.ufvm_case_set_field(self.ctx.h, self.id, name.encode(),
+                                                        v.ctypes.data_as(f64p), v.size))
+
+    def system(self, name):
+        sid = C.c_int32()
+        self.ctx.check(self.ctx.lib.ufvm_case_system(self.ctx.h, self.id, name.encode(),
+                                                     C.byref(sid)))
+        return sid.value

@@ -32,7 +32,11 @@ class Hier(C.Structure):
 
 class VCfg(C.Structure):
     _fields_ = [("nlev", C.c_int32), ("pre", C.c_int32), ("post", C.c_int32),
-                ("singular", C.c_int32)]
+                ("singular", C.c_int32), ("block_rows", C.c_int32),
+                ("coarse_smoother", C.c_int32)]
+
+
+BLOCK_ROWS = 1024   # rows per block of the coarse-level smoother
 
 
 def build():
@@ -57,6 +61,8 @@ def lib():
                                     C.c_int, C.c_int, f64p, f64p, C.POINTER(C.c_int)]
         L.orc_colour.argtypes = [C.c_int32, i32p, i32p, i32p]
         L.orc_colour.restype = C.c_int
+        L.orc_colour_blocks.argtypes = [C.c_int32, i32p, i32p, C.c_int32, i32p]
+        L.orc_colour_blocks.restype = C.c_int
         L.orc_vcycle.argtypes = [C.POINTER(Hier), C.POINTER(i32p), C.POINTER(C.c_int),
                                  C.POINTER(VCfg), f64p, f64p]
         L.orc_fpcg.argtypes = [C.POINTER(Hier), C.POINTER(i32p), C.POINTER(C.c_int),
@@ -158,11 +164,15 @@ class System:
 
     # ------------------------------------------------ precision-form oracle
     def _colour_all(self, nlev):
+        """level 0: global greedy colouring; coarse levels: block-local."""
         self.colours, self._ncol = [], []
         for l in range(nlev):
             L = self.H.lev[l]
             c = np.empty(L.n, np.int32)
-            nc = lib().orc_colour(L.n, L.ptr, L.col, _p(c, i32p))
+            if l == 0:
+                nc = lib().orc_colour(L.n, L.ptr, L.col, _p(c, i32p))
+            else:
+                nc = lib().orc_colour_blocks(L.n, L.ptr, L.col, BLOCK_ROWS, _p(c, i32p))
             self.colours.append(c)
             self._ncol.append(nc)
         arr = (i32p * 16)(*[_p(c, i32p) for c in self.colours])
@@ -175,19 +185,20 @@ class System:
                 return l + 1
         return self.H.nlevels
 
-    def vcycle(self, b, pre=1, post=1, max_coarse_cells=2000, singular=0):
+    def vcycle(self, b, pre=1, post=1, max_coarse_cells=2000, singular=0, coarse_smoother=1):
         nlev = self.pcg_levels(max_coarse_cells)
         arr, ncl = self._colour_all(nlev)
-        cfg = VCfg(nlev, pre, post, singular)
+        cfg = VCfg(nlev, pre, post, singular, BLOCK_ROWS, coarse_smoother)
         b = np.ascontiguousarray(b, np.float64)
         x = np.empty_like(b)
         lib().orc_vcycle(C.byref(self.H), arr, ncl, C.byref(cfg), _p(b, f64p), _p(x, f64p))
         return x
 
-    def fpcg(self, b, tol=1e-8, maxit=500, pre=1, post=1, max_coarse_cells=2000, singular=0):
+    def fpcg(self, b, tol=1e-8, maxit=500, pre=1, post=1, max_coarse_cells=2000, singular=0,
+             coarse_smoother=1):
         nlev = self.pcg_levels(max_coarse_cells)
         arr, ncl = self._colour_all(nlev)
-        cfg = VCfg(nlev, pre, post, singular)
+        cfg = VCfg(nlev, pre, post, singular, BLOCK_ROWS, coarse_smoother)
         b = np.ascontiguousarray(b, np.float64)
         x = np.empty_like(b)
         rr = C.c_double()
@@ -197,3 +208,170 @@ class System:
 
     def colours_of(self, l):
         return self.colours[l], self._ncol[l]
+
+
+# ----------------------------------------------------------------- SIMPLE step
+
+class OMesh(C.Structure):
+    _fields_ = [("N", C.c_int32), ("F", C.c_int32), ("Fi", C.c_int32),
+                ("owner", i32p), ("neighbour", i32p),
+                ("Sf", f64p), ("area", f64p), ("CN", f64p), ("eCN", f64p), ("gDiff", f64p),
+                ("T", f64p), ("gf", f64p), ("fc", f64p), ("vol", f64p), ("cc", f64p),
+                ("cptr", i32p), ("ccol", i32p), ("own_nnz", i32p), ("nei_nnz", i32p),
+                ("npatch", C.c_int32), ("pstart", i32p), ("pnf", i32p), ("ptype", i32p),
+                ("u_bc", i32p), ("p_bc", i32p), ("u_val", f64p)]
+
+
+class OCfg(C.Structure):
+    _fields_ = [("nu", C.c_double), ("urf_u", C.c_double), ("urf_p", C.c_double),
+                ("n_correctors", C.c_int32), ("p_ref", C.c_int32), ("u_sweeps", C.c_int32),
+                ("p_cycle", C.c_int32), ("p_max_cycles", C.c_int32), ("p_max_coarse", C.c_int32),
+                ("p_pre", C.c_int32), ("p_post", C.c_int32), ("p_smoother", C.c_int32),
+                ("p_termination", C.c_double)]
+
+
+class OState(C.Structure):
+    _fields_ = [("N", C.c_int32), ("Fb", C.c_int32),
+                ("U", f64p), ("gradU", f64p), ("p", f64p), ("gradp", f64p), ("PP", f64p),
+                ("gradPP", f64p), ("DU", f64p), ("DUT", f64p), ("mdot", f64p), ("gamma_f", f64p),
+                ("fluxC1", f64p), ("fluxC2", f64p), ("scaleU", C.c_double), ("H", Hier),
+                ("ac_old", f64p), ("rms", C.c_double * 4), ("mx", C.c_double * 4),
+                ("r0", C.c_double * 4), ("r1", C.c_double * 4), ("ncycles", C.c_int32)]
+
+
+PATCH_TYPE = {"wall": 0, "empty": 1, "patch": 2}
+BC = {"fixedValue": 0, "zeroGradient": 1, "empty": 2}
+
+
+class SimpleCase:
+    """The oracle's steady SIMPLE iteration on a processed mesh.
+
+    bcs: {patch name: (U bc, U value (3,), p bc)}; U bc in fixedValue /
+    zeroGradient / empty, p bc in zeroGradient / empty.
+    """
+
+    def __init__(self, mesh, bcs, nu, urf_u=0.7, urf_p=0.3, n_correctors=2, p_ref=0,
+                 u_sweeps=10, p_cycle="W-Cycle", p_max_cycles=30, p_termination=0.1,
+                 p_max_coarse=10, p_pre=1, p_post=3, p_smoother="ILU"):
+        L = lib()
+        for name, args in [("orc_simple_init", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState)]),
+                           ("orc_simple_free", [C.POINTER(OState)]),
+                           ("orc_simple_iteration", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState)]),
+                           ("orc_update_fields", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState)]),
+                           ("orc_assemble_u", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState), C.c_int]),
+                           ("orc_solve_u", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState), C.c_int]),
+                           ("orc_correct_u", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState), C.c_int]),
+                           ("orc_assemble_p", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState)]),
+                           ("orc_solve_p", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState)]),
+                           ("orc_correct_p", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState)]),
+                           ("orc_gradient", [C.POINTER(OMesh), f64p, C.c_int, f64p])]:
+            getattr(L, name).argtypes = args
+        m = mesh
+        self.mesh = m
+        keep = self._keep = []
+
+        def a(x, t):
+            arr = np.ascontiguousarray(x, dtype=np.int32 if t is i32p else np.float64)
+            keep.append(arr)
+            return arr.ctypes.data_as(t)
+
+        M = OMesh()
+        M.N, M.F, M.Fi = m.n_cells, m.n_faces, m.n_interior
+        M.owner, M.neighbour = a(m.owner, i32p), a(m.neighbour, i32p)
+        M.Sf, M.area, M.CN, M.eCN = a(m.Sf, f64p), a(m.area, f64p), a(m.CN, f64p), a(m.eCN, f64p)
+        M.gDiff, M.T, M.gf, M.fc = a(m.gDiff, f64p), a(m.T, f64p), a(m.gf, f64p), a(m.face_centroid, f64p)
+        M.vol, M.cc = a(m.volume, f64p), a(m.centroid, f64p)
+        M.cptr, M.ccol = a(m.cconn_ptr, i32p), a(m.cconn, i32p)
+        M.own_nnz, M.nei_nnz = a(m.own_nnz, i32p), a(m.nei_nnz, i32p)
+        M.npatch = len(m.patches)
+        M.pstart = a([p.start for p in m.patches], i32p)
+        M.pnf = a([p.nfaces for p in m.patches], i32p)
+        M.ptype = a([PATCH_TYPE[p.type] for p in m.patches], i32p)
+        M.u_bc = a([BC[bcs[p.name][0]] for p in m.patches], i32p)
+        M.p_bc = a([BC[bcs[p.name][2]] for p in m.patches], i32p)
+        M.u_val = a(np.concatenate([np.asarray(bcs[p.name][1], float) for p in m.patches]), f64p)
+        self.M = M
+        self.cfg = OCfg(nu, urf_u, urf_p, n_correctors, p_ref, u_sweeps,
+                        {"V-Cycle": 0, "W-Cycle": 1, "F-Cycle": 2}[p_cycle], p_max_cycles,
+                        p_max_coarse, p_pre, p_post, {"ILU": 0, "SOR": 1}[p_smoother], p_termination)
+        self.S = OState()
+        L.orc_simple_init(C.byref(self.M), C.byref(self.cfg), C.byref(self.S))
+
+    def __del__(self):
+        try:
+            lib().orc_simple_free(C.byref(self.S))
+        except Exception:
+            pass
+
+    def _args(self):
+        return C.byref(self.M), C.byref(self.cfg), C.byref(self.S)
+
+    def iterate(self, n=1):
+        for _ in range(n):
+            lib().orc_simple_iteration(*self._args())
+
+    def update_fields(self):
+        lib().orc_update_fields(*self._args())
+
+    def assemble_u(self, comp):
+        lib().orc_assemble_u(*self._args(), comp)
+
+    def solve_u(self, comp):
+        lib().orc_solve_u(*self._args(), comp)
+
+    def correct_u(self, comp):
+        lib().orc_correct_u(*self._args(), comp)
+
+    def assemble_p(self):
+        lib().orc_assemble_p(*self._args())
+
+    def solve_p(self):
+        lib().orc_solve_p(*self._args())
+
+    def correct_p(self):
+        lib().orc_correct_p(*self._args())
+
+    def field(self, name):
+        N, Fb, F = self.S.N, self.S.Fb, self.M.F
+        shapes = {"U": (N + Fb, 3), "gradU": (N + Fb, 3, 3), "p": (N + Fb,), "gradp": (N + Fb, 3),
+                  "PP": (N + Fb,), "gradPP": (N + Fb, 3), "DU": (N + Fb, 3), "DUT": (N + Fb, 3),
+                  "mdot": (F,), "gamma_f": (F,), "fluxC1": (F,), "fluxC2": (F,)}
+        ptr = getattr(self.S, name)
+        return np.ctypeslib.as_array(ptr, (int(np.prod(shapes[name])),)).reshape(shapes[name]).copy()
+
+    def set_field(self, name, values):
+        N, Fb, F = self.S.N, self.S.Fb, self.M.F
+        size = {"U": (N + Fb) * 3, "p": N + Fb, "PP": N + Fb, "DU": (N + Fb) * 3,
+                "DUT": (N + Fb) * 3, "mdot": F}[name]
+        np.ctypeslib.as_array(getattr(self.S, name), (size,))[:] = np.asarray(values).reshape(-1)
+
+    def system(self):
+        L = self.S.H.lev[0]
+        n, nnz = L.n, L.nnz
+        return dict(ac=np.ctypeslib.as_array(L.ac, (n,)).copy(),
+                    anb=np.ctypeslib.as_array(L.anb, (max(nnz, 1),))[:nnz].copy(),
+                    bc=np.ctypeslib.as_array(L.bc, (n,)).copy(),
+                    dphi=np.ctypeslib.as_array(L.dphi, (n,)).copy())
+
+    def set_dphi(self, x):
+        L = self.S.H.lev[0]
+        np.ctypeslib.as_array(L.dphi, (L.n,))[:] = x
+
+    def residuals(self):
+        S = self.S
+        return dict(rms=list(S.rms), max=list(S.mx), r0=list(S.r0), r1=list(S.r1),
+                    ncycles=S.ncycles)
+
+
+def cavity_bcs(mesh, lid=(1.0, 0.0, 0.0)):
+    """Lid-driven cavity boundary conditions (fixedValue walls, zeroGradient p,
+    empty front/back)."""
+    out = {}
+    for p in mesh.patches:
+        if p.type == "empty":
+            out[p.name] = ("empty", (0.0, 0.0, 0.0), "empty")
+        elif p.name == "movingWall":
+            out[p.name] = ("fixedValue", lid, "zeroGradient")
+        else:
+            out[p.name] = ("fixedValue", (0.0, 0.0, 0.0), "zeroGradient")
+    return out

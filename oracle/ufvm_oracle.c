@@ -468,6 +468,67 @@ int orc_colour(int32_t n, const int32_t *ptr, const int32_t *col, int32_t *colou
   return ncol;
 }
 
+int orc_colour_blocks(int32_t n, const int32_t *ptr, const int32_t *col, int32_t R,
+                      int32_t *colour) {
+  int32_t ncol = 0;
+  int32_t mark[1024];
+  for (int t = 0; t < 1024; ++t) mark[t] = -1;
+  for (int32_t i = 0; i < n; ++i) colour[i] = -1;
+  for (int32_t i = 0; i < n; ++i) {
+    const int32_t lo = (i / R) * R, hi = lo + R < n ? lo + R : n;
+    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+      const int32_t j = col[k];
+      if (j < lo || j >= hi) continue;
+      const int32_t c = colour[j];
+      if (c >= 0 && c < 1024) mark[c] = i;
+    }
+    int32_t c = 0;
+    while (c < 1024 && mark[c] == i) ++c;
+    colour[i] = c;
+    if (c + 1 > ncol) ncol = c + 1;
+  }
+  return ncol;
+}
+
+/* Block-local multicolour GS.  v0 = starting values of every row (mode 0: 0,
+ * mode 1: xin, mode 2: xin + xc[parent]); each block of R rows sweeps its own
+ * rows (colours up, then down) reading in-block neighbours' current values and
+ * out-of-block neighbours' v0; xout receives the block values. */
+static void bmcgs(const orc_level *L, const int32_t *colour, int ncol, int32_t R, const double *b,
+                  const double *xin, const int32_t *parent, const double *xc, int mode,
+                  int nsweep, double *xout) {
+  const int32_t n = L->n;
+  double *v0 = (double *)malloc((size_t)n * sizeof(double));
+  for (int32_t i = 0; i < n; ++i) {
+    double v = 0.0;
+    if (mode >= 1) v = xin[i];
+    if (mode == 2) v = v + xc[parent[i]];
+    v0[i] = v;
+    xout[i] = v;
+  }
+  for (int32_t lo = 0; lo < n; lo += R) {
+    const int32_t hi = lo + R < n ? lo + R : n;
+    for (int s = 0; s < nsweep; ++s) {
+      for (int dir = 0; dir < 2; ++dir) {
+        for (int cc = 0; cc < ncol; ++cc) {
+          const int c = dir == 0 ? cc : ncol - 1 - cc;
+          for (int32_t i = lo; i < hi; ++i) {
+            if (colour[i] != c) continue;
+            double v = b[i];
+            for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) {
+              const int32_t j = L->col[k];
+              const double xj = (j >= lo && j < hi) ? xout[j] : v0[j];
+              v = v - L->anb[k] * xj;
+            }
+            xout[i] = v / L->ac[i];
+          }
+        }
+      }
+    }
+  }
+  free(v0);
+}
+
 void orc_mcgs(const orc_level *L, const int32_t *colour, int ncolours, const double *b,
               double *x) {
   for (int c = 0; c < ncolours; ++c)
@@ -546,6 +607,23 @@ static void lu_free(orc_lu *F) {
   free(F->piv);
 }
 
+/* damped Jacobi sweep y = v + w (b - A v)/ac, v = x (+ xc[parent] if given) */
+static void jacobi(const orc_level *L, const double *b, const double *x, const int32_t *parent,
+                   const double *xc, double omega, double *y) {
+  for (int32_t i = 0; i < L->n; ++i) {
+    double vi = x[i];
+    if (parent) vi = vi + xc[parent[i]];
+    double r = b[i] - L->ac[i] * vi;
+    for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) {
+      const int32_t j = L->col[k];
+      double vj = x[j];
+      if (parent) vj = vj + xc[parent[j]];
+      r = r - L->anb[k] * vj;
+    }
+    y[i] = vi + omega * r / L->ac[i];
+  }
+}
+
 static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol,
                      const orc_vcfg *cfg, const orc_lu *lu, int l, const double *b, double *x) {
   const orc_level *L = &H->lev[l];
@@ -556,7 +634,42 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
     return;
   }
   memset(x, 0, (size_t)n * sizeof(double));
-  for (int s = 0; s < cfg->pre; ++s) orc_mcgs(L, colours[l], ncol[l], b, x);
+  const double omega = 2.0 / 3.0;
+  if (l > 0 && cfg->coarse_smoother == 1) {
+    double *y = (double *)malloc((size_t)n * sizeof(double));
+    for (int s = 0; s < cfg->pre; ++s) {
+      jacobi(L, b, x, NULL, NULL, omega, y);
+      memcpy(x, y, (size_t)n * sizeof(double));
+    }
+    double *r = (double *)malloc((size_t)n * sizeof(double));
+    for (int32_t i = 0; i < n; ++i) {
+      double s = b[i] - L->ac[i] * x[i];
+      for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) s = s - L->anb[k] * x[L->col[k]];
+      r[i] = s;
+    }
+    const orc_level *C = &H->lev[l + 1];
+    double *bc = (double *)calloc((size_t)C->n, sizeof(double));
+    double *xc = (double *)calloc((size_t)C->n, sizeof(double));
+    for (int32_t i = 0; i < n; ++i) bc[L->parents[i]] = bc[L->parents[i]] + r[i];
+    vcyc_rec(H, colours, ncol, cfg, lu, l + 1, bc, xc);
+    if (cfg->post == 0) {
+      for (int32_t i = 0; i < n; ++i) x[i] = x[i] + xc[L->parents[i]];
+    } else {
+      jacobi(L, b, x, L->parents, xc, omega, y);
+      memcpy(x, y, (size_t)n * sizeof(double));
+      for (int s = 1; s < cfg->post; ++s) {
+        jacobi(L, b, x, NULL, NULL, omega, y);
+        memcpy(x, y, (size_t)n * sizeof(double));
+      }
+    }
+    free(y); free(r); free(bc); free(xc);
+    return;
+  }
+  if (l == 0) {
+    for (int s = 0; s < cfg->pre; ++s) orc_mcgs(L, colours[l], ncol[l], b, x);
+  } else if (cfg->pre > 0) {
+    bmcgs(L, colours[l], ncol[l], cfg->block_rows, b, NULL, NULL, NULL, 0, cfg->pre, x);
+  }
   double *r = (double *)malloc((size_t)n * sizeof(double));
   for (int32_t i = 0; i < n; ++i) {
     double s = b[i] - L->ac[i] * x[i];
@@ -568,8 +681,16 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
   double *xc = (double *)calloc((size_t)C->n, sizeof(double));
   for (int32_t i = 0; i < n; ++i) bc[L->parents[i]] = bc[L->parents[i]] + r[i];
   vcyc_rec(H, colours, ncol, cfg, lu, l + 1, bc, xc);
-  for (int32_t i = 0; i < n; ++i) x[i] = x[i] + xc[L->parents[i]];
-  for (int s = 0; s < cfg->post; ++s) orc_mcgs(L, colours[l], ncol[l], b, x);
+  if (l > 0 && cfg->post > 0) {
+    double *y = (double *)malloc((size_t)n * sizeof(double));
+    bmcgs(L, colours[l], ncol[l], cfg->block_rows, b, x, L->parents, xc, 2, cfg->post, y);
+    memcpy(x, y, (size_t)n * sizeof(double));
+    free(y);
+  } else {
+    for (int32_t i = 0; i < n; ++i) x[i] = x[i] + xc[L->parents[i]];
+    if (l == 0)
+      for (int s = 0; s < cfg->post; ++s) orc_mcgs(L, colours[l], ncol[l], b, x);
+  }
   free(r); free(bc); free(xc);
 }
 

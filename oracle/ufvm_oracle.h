@@ -66,6 +66,9 @@ void orc_apply_amg(orc_hier *H, int cycle, int maxCycles, double rrf, int pre, i
 
 /* greedy first-fit colouring in index order; returns the number of colours */
 int orc_colour(int32_t n, const int32_t *ptr, const int32_t *col, int32_t *colour);
+/* the same within blocks of R consecutive rows (only in-block neighbours count) */
+int orc_colour_blocks(int32_t n, const int32_t *ptr, const int32_t *col, int32_t R,
+                      int32_t *colour);
 
 /* one symmetric multicolour Gauss-Seidel sweep (colours ascending, then
  * descending), each row summed in cconn order */
@@ -73,12 +76,17 @@ void orc_mcgs(const orc_level *L, const int32_t *colour, int ncolours, const dou
               double *x);
 
 /* preconditioner V-cycle: zero initial guess on every level, nu_pre/nu_post
- * symmetric MCGS sweeps, restriction by parents, coarsest level `ncoarse`
- * solved exactly (dense LU, +11^T/n when `singular`). */
+ * symmetric sweeps, restriction by parents, level nlev-1 solved exactly
+ * (dense LU, + mean(diag)/n 11^T when `singular`).  Level 0 uses global
+ * multicolour GS; levels 1..nlev-2 use block-local multicolour GS over blocks
+ * of block_rows consecutive rows (Jacobi coupling between blocks), with the
+ * prolongation applied to the post-smoother's starting values. */
 typedef struct {
   int32_t nlev;          /* levels used: 0..nlev-1, nlev-1 solved directly */
   int32_t pre, post;
   int32_t singular;
+  int32_t block_rows;
+  int32_t coarse_smoother;  /* levels 1..nlev-2: 0 block-local MCGS, 1 damped Jacobi (2/3) */
 } orc_vcfg;
 void orc_vcycle(const orc_hier *H, int32_t *const *colours, const int *ncolours,
                 const orc_vcfg *cfg, const double *b, double *x);

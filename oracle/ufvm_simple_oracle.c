@@ -1,0 +1,542 @@
+/*
+ * ufvm_simple_oracle.c -- CPU restatement of uFVM's steady SIMPLE iteration.
+ *
+ * TEST INFRASTRUCTURE ONLY (checker of the HIP implementation, CPU baseline).
+ *
+ * Every loop follows the reference's order (faces ascending, interior before
+ * boundary, terms in equation order) and every expression its association, so
+ * that the floating-point results are those of the reference's arithmetic.
+ * Paths are relative to uFVM/.  Built with -ffp-contract=off.
+ */
+#include "ufvm_simple_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define FB(m) ((m)->F - (m)->Fi)
+
+static double *dz(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
+
+static double mag3(const double *v) { return sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]); }
+static double dot3(const double *a, const double *b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
+
+/* patch of boundary face f (absolute index) */
+static int patch_of(const orc_mesh *m, int32_t f) {
+  for (int p = 0; p < m->npatch; ++p)
+    if (f >= m->pstart[p] && f < m->pstart[p] + m->pnf[p]) return p;
+  return -1;
+}
+
+void orc_simple_init(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s) {
+  const int32_t N = m->N, Fb = FB(m), NB = N + Fb, F = m->F;
+  memset(s, 0, sizeof(*s));
+  s->N = N;
+  s->Fb = Fb;
+  s->U = dz((size_t)NB * 3);
+  s->gradU = dz((size_t)NB * 9);
+  s->p = dz(NB);
+  s->gradp = dz((size_t)NB * 3);
+  s->PP = dz(NB);
+  s->gradPP = dz((size_t)NB * 3);
+  s->DU = dz((size_t)NB * 3);
+  s->DUT = dz((size_t)NB * 3);
+  s->mdot = dz(F);
+  s->gamma_f = dz(F);
+  s->fluxC1 = dz(F);
+  s->fluxC2 = dz(F);
+  s->ac_old = dz(N);
+  orc_level *L = &s->H.lev[0];
+  L->n = N;
+  L->nnz = m->cptr[N];
+  L->ptr = (int32_t *)m->cptr;
+  L->col = (int32_t *)m->ccol;
+  L->ac = dz(N);
+  L->anb = dz(L->nnz);
+  L->bc = dz(N);
+  L->dphi = dz(N);
+  L->parents = NULL;
+  s->H.nlevels = 1;
+  /* uniform zero initial fields; boundary values from the BCs
+   * (cfdInitializeBoundaryConditions.m: fixedValue -> value; zeroGradient ->
+   * owner value :200-226; empty -> owner minus its normal component :534-595) */
+  for (int p = 0; p < m->npatch; ++p) {
+    for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
+      const int32_t b = N + (f - m->Fi), o = m->owner[f];
+      if (m->u_bc[p] == ORC_BC_FIXED_VALUE) {
+        for (int d = 0; d < 3; ++d) s->U[b * 3 + d] = m->u_val[p * 3 + d];
+      } else if (m->u_bc[p] == ORC_BC_EMPTY) {
+        double n[3];
+        const double nm = mag3(&m->Sf[f * 3]);
+        for (int d = 0; d < 3; ++d) n[d] = m->Sf[f * 3 + d] / nm;
+        const double un = dot3(&s->U[o * 3], n);
+        for (int d = 0; d < 3; ++d) s->U[b * 3 + d] = s->U[o * 3 + d] - un * n[d];
+      } else {
+        for (int d = 0; d < 3; ++d) s->U[b * 3 + d] = s->U[o * 3 + d];
+      }
+      s->p[b] = s->p[o];
+    }
+  }
+  /* mdot_f = rho_f .* (Sf . U_f) with Average interpolation
+   * (cfdInitializeMdotField.m:72-80), rho = 1 */
+  for (int32_t f = 0; f < F; ++f) {
+    double uf[3], rf;
+    if (f < m->Fi) {
+      const int32_t o = m->owner[f], nb = m->neighbour[f];
+      const double g = m->gf[f];
+      for (int d = 0; d < 3; ++d) uf[d] = g * s->U[nb * 3 + d] + (1 - g) * s->U[o * 3 + d];
+      rf = g * 1.0 + (1 - g) * 1.0;
+    } else {
+      const int32_t b = N + (f - m->Fi);
+      for (int d = 0; d < 3; ++d) uf[d] = s->U[b * 3 + d];
+      rf = 1.0;
+    }
+    s->mdot[f] = rf * dot3(&m->Sf[f * 3], uf);
+  }
+  (void)c;
+}
+
+void orc_simple_free(orc_simple_state *s) {
+  orc_hier_free(&s->H);
+  orc_level *L = &s->H.lev[0];
+  free(L->ac); free(L->anb); free(L->bc); free(L->dphi);
+  free(s->U); free(s->gradU); free(s->p); free(s->gradp); free(s->PP); free(s->gradPP);
+  free(s->DU); free(s->DUT); free(s->mdot); free(s->gamma_f); free(s->fluxC1); free(s->fluxC2);
+  free(s->ac_old);
+  memset(s, 0, sizeof(*s));
+}
+
+/* cfdComputeGradientGauss0.m:28-80.  The boundary contribution reads
+ * phi_b(k) with linear indexing into the [Fb x nc] boundary block (:59-64),
+ * i.e. the FIRST component's boundary value for every component; kept. */
+static void gauss0(const orc_mesh *m, const double *phi, int nc, double *grad) {
+  const int32_t N = m->N, Fb = FB(m);
+  memset(grad, 0, (size_t)(N + Fb) * nc * 3 * sizeof(double));
+  for (int c = 0; c < nc; ++c) {
+    for (int32_t f = 0; f < m->Fi; ++f) {
+      const int32_t o = m->owner[f], nb = m->neighbour[f];
+      const double g = m->gf[f];
+      const double phif = g * phi[nb * nc + c] + (1 - g) * phi[o * nc + c];
+      for (int d = 0; d < 3; ++d) {
+        grad[(o * nc + c) * 3 + d] = grad[(o * nc + c) * 3 + d] + phif * m->Sf[f * 3 + d];
+        grad[(nb * nc + c) * 3 + d] = grad[(nb * nc + c) * 3 + d] - phif * m->Sf[f * 3 + d];
+      }
+    }
+  }
+  for (int c = 0; c < nc; ++c) {
+    for (int32_t k = 0; k < Fb; ++k) {
+      const int32_t f = m->Fi + k, o = m->owner[f];
+      const double phib = phi[(N + k) * nc + 0];
+      for (int d = 0; d < 3; ++d)
+        grad[(o * nc + c) * 3 + d] = grad[(o * nc + c) * 3 + d] + phib * m->Sf[f * 3 + d];
+    }
+  }
+  for (int c = 0; c < nc; ++c)
+    for (int32_t i = 0; i < N; ++i)
+      for (int d = 0; d < 3; ++d) grad[(i * nc + c) * 3 + d] = grad[(i * nc + c) * 3 + d] / m->vol[i];
+  for (int32_t k = 0; k < Fb; ++k) {
+    const int32_t o = m->owner[m->Fi + k];
+    for (int q = 0; q < nc * 3; ++q) grad[(N + k) * nc * 3 + q] = grad[o * nc * 3 + q];
+  }
+}
+
+/* cfdUpdateGradient.m:18-70: Gauss gradient, then wall (and 'patch') boundary
+ * rows replaced by g - (g.e)e + ((phi_b - phi_O)/|d|) e, d = C_f - C_O
+ * (:104-134) */
+void orc_gradient(const orc_mesh *m, const double *phi, int nc, double *grad) {
+  const int32_t N = m->N;
+  gauss0(m, phi, nc, grad);
+  for (int p = 0; p < m->npatch; ++p) {
+    if (m->ptype[p] != ORC_PATCH_WALL && m->ptype[p] != ORC_PATCH_PATCH) continue;
+    for (int c = 0; c < nc; ++c) {
+      for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
+        const int32_t b = N + (f - m->Fi), o = m->owner[f];
+        double d[3], e[3];
+        for (int q = 0; q < 3; ++q) d[q] = m->fc[f * 3 + q] - m->cc[o * 3 + q];
+        const double dmag = mag3(d);
+        for (int q = 0; q < 3; ++q) e[q] = d[q] / dmag;
+        const double *g = &grad[(o * nc + c) * 3];
+        const double ge = dot3(g, e);
+        const double t = (phi[b * nc + c] - phi[o * nc + c]) / dmag;
+        for (int q = 0; q < 3; ++q) grad[(b * nc + c) * 3 + q] = (g[q] - ge * e[q]) + t * e[q];
+      }
+    }
+  }
+}
+
+/* cfdUpdateEquations.m:14-24 for U then p: boundary conditions
+ * (fixedValue walls, cfdUpdateBoundaryConditions.m:103-139; zeroGradient and
+ * empty are no-ops :198-200, :388-390), gradients, gamma_Ueq = Average(nu)
+ * (cfdUpdateTermCoefficientField.m:44-70), phiScale (cfdUpdateScale.m:17-23) */
+void orc_update_fields(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s) {
+  const int32_t N = m->N, Fb = FB(m);
+  for (int p = 0; p < m->npatch; ++p) {
+    if (m->ptype[p] != ORC_PATCH_WALL || m->u_bc[p] != ORC_BC_FIXED_VALUE) continue;
+    for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
+      const int32_t b = N + (f - m->Fi);
+      for (int d = 0; d < 3; ++d) s->U[b * 3 + d] = m->u_val[p * 3 + d] * 1.0;
+    }
+  }
+  orc_gradient(m, s->U, 3, s->gradU);
+  for (int32_t f = 0; f < m->F; ++f) {
+    if (f < m->Fi) {
+      const double g = m->gf[f];
+      s->gamma_f[f] = g * c->nu + (1 - g) * c->nu;
+    } else {
+      s->gamma_f[f] = c->nu;
+    }
+  }
+  double mx = -INFINITY, mn = INFINITY;
+  for (int32_t q = 0; q < (N + Fb) * 3; ++q) {
+    if (s->U[q] > mx) mx = s->U[q];
+    if (s->U[q] < mn) mn = s->U[q];
+  }
+  double sc = (mx - mn) > fabs(mx) ? (mx - mn) : fabs(mx);
+  if (sc < 1e-6) sc = 1;
+  s->scaleU = sc;
+  orc_gradient(m, s->p, 1, s->gradp);
+}
+
+static void zero_system(orc_simple_state *s) {
+  orc_level *L = &s->H.lev[0];
+  memset(L->ac, 0, (size_t)L->n * sizeof(double));
+  memset(L->anb, 0, (size_t)(L->nnz ? L->nnz : 1) * sizeof(double));
+  memset(L->bc, 0, (size_t)L->n * sizeof(double));
+  memset(L->dphi, 0, (size_t)L->n * sizeof(double));
+  memset(s->ac_old, 0, (size_t)L->n * sizeof(double));
+}
+
+/* cfdAssembleIntoGlobalMatrixFaceFluxes.m:25-55 */
+static void scatter_faces(const orc_mesh *m, orc_simple_state *s, const double *C1,
+                          const double *C2, const double *T) {
+  orc_level *L = &s->H.lev[0];
+  for (int32_t f = 0; f < m->Fi; ++f) {
+    const int32_t o = m->owner[f], nb = m->neighbour[f];
+    L->ac[o] = L->ac[o] + C1[f];
+    L->anb[m->own_nnz[f]] = L->anb[m->own_nnz[f]] + C2[f];
+    L->bc[o] = L->bc[o] - T[f];
+    L->ac[nb] = L->ac[nb] - C2[f];
+    L->anb[m->nei_nnz[f]] = L->anb[m->nei_nnz[f]] - C1[f];
+    L->bc[nb] = L->bc[nb] + T[f];
+  }
+  for (int32_t f = m->Fi; f < m->F; ++f) {
+    const int32_t o = m->owner[f];
+    L->ac[o] = L->ac[o] + C1[f];
+    L->bc[o] = L->bc[o] - T[f];
+  }
+}
+
+/* Average:Corrected face gradient of component c of a field with nc
+ * components (cfdInterpolateGradientsFromElementsToInteriorFaces.m:46-70);
+ * the average weights are swapped relative to value interpolation (:48-50) */
+static void grad_face_corrected(const orc_mesh *m, const double *phi, const double *grad, int nc,
+                                int c, int32_t f, double *gf_out) {
+  const int32_t o = m->owner[f], nb = m->neighbour[f];
+  const double g = m->gf[f];
+  double gfc[3], dcf[3], e[3];
+  for (int d = 0; d < 3; ++d)
+    gfc[d] = (1 - g) * grad[(nb * nc + c) * 3 + d] + g * grad[(o * nc + c) * 3 + d];
+  for (int d = 0; d < 3; ++d) dcf[d] = m->cc[nb * 3 + d] - m->cc[o * 3 + d];
+  const double dmag = mag3(dcf);
+  for (int d = 0; d < 3; ++d) e[d] = dcf[d] / dmag;
+  const double lgm = (phi[nb * nc + c] - phi[o * nc + c]) / dmag;
+  const double lam = dot3(gfc, e);
+  for (int d = 0; d < 3; ++d) gf_out[d] = (gfc[d] - lam * e[d]) + lgm * e[d];
+}
+
+/* momentum component: convection (upwind) + divergence term, diffusion,
+ * -grad(p), false transient, URF (cfdAssembleEquationTerms.m:22-55) */
+void orc_assemble_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s, int comp) {
+  const int32_t N = m->N, F = m->F, Fi = m->Fi;
+  orc_level *L = &s->H.lev[0];
+  zero_system(s);
+  double *C1 = dz(F), *C2 = dz(F), *T = dz(F);
+  const double *U = s->U;
+  /* ---- Convection, sign +1 (cfdAssembleConvectionTerm.m:103-137; walls and
+   * empty patches give zero fluxes :238-278) */
+  {
+    const double sg = 1.0;
+    for (int32_t f = 0; f < Fi; ++f) {
+      const double psi = s->mdot[f];
+      C1[f] = sg * fmax(psi, 0.0);
+      C2[f] = (-sg) * fmax(-psi, 0.0);
+      const double V = 0.0;
+      T[f] = (C1[f] * U[m->owner[f] * 3 + comp] + C2[f] * U[m->neighbour[f] * 3 + comp]) + V;
+    }
+    for (int32_t f = Fi; f < F; ++f) C1[f] = C2[f] = T[f] = 0.0;
+    scatter_faces(m, s, C1, C2, T);
+    /* divergence term (cfdAssembleConvectionDivergenceTerm.m:38-47 with
+     * cfdComputeEffectiveDivergenceWithFluidTag.m:21-46) */
+    double *eff = dz(N);
+    for (int32_t f = 0; f < Fi; ++f) {
+      eff[m->owner[f]] = eff[m->owner[f]] + s->mdot[f];
+      eff[m->neighbour[f]] = eff[m->neighbour[f]] - s->mdot[f];
+    }
+    for (int32_t f = Fi; f < F; ++f) eff[m->owner[f]] = eff[m->owner[f]] + s->mdot[f];
+    for (int32_t i = 0; i < N; ++i) {
+      const double e = eff[i], phi = U[i * 3 + comp];
+      const double CE = fmax(e, 0.0) - e;
+      const double VE = (-fmax(e, 0.0)) * phi;
+      const double TE = CE * phi + VE;
+      L->ac[i] = L->ac[i] + CE;
+      L->bc[i] = L->bc[i] - TE;
+    }
+    free(eff);
+  }
+  /* ---- Diffusion laplacian(nu*U), sign -1 (cfdAssembleDiffusionTerm.m:94-130,
+   * wall fixedValue :224-263, zeroGradient :330-347, empty :631-690) */
+  {
+    const double sg = -1.0;
+    for (int32_t f = 0; f < Fi; ++f) {
+      double gfc[3];
+      grad_face_corrected(m, U, s->gradU, 3, comp, f, gfc);
+      const double gam = s->gamma_f[f];
+      C1[f] = ((-sg) * gam) * m->gDiff[f];
+      C2[f] = (sg * gam) * m->gDiff[f];
+      const double V = (sg * gam) * dot3(gfc, &m->T[f * 3]);
+      T[f] = (C1[f] * U[m->owner[f] * 3 + comp] + C2[f] * U[m->neighbour[f] * 3 + comp]) + V;
+    }
+    for (int p = 0; p < m->npatch; ++p) {
+      for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
+        const int32_t b = N + (f - Fi), o = m->owner[f];
+        const double gam = s->gamma_f[f];
+        if (m->ptype[p] == ORC_PATCH_WALL && m->u_bc[p] == ORC_BC_FIXED_VALUE) {
+          C1[f] = ((-sg) * gam) * m->gDiff[f];
+          C2[f] = (sg * gam) * m->gDiff[f];
+          const double V = (sg * gam) * dot3(&s->gradU[(b * 3 + comp) * 3], &m->T[f * 3]);
+          T[f] = (C1[f] * U[o * 3 + comp] + C2[f] * U[b * 3 + comp]) + V;
+        } else if (m->ptype[p] == ORC_PATCH_EMPTY) {
+          const double A = m->area[f];
+          double n[3], l[3] = {1.0, 1.0, 1.0};
+          l[comp] = 0.0;
+          for (int d = 0; d < 3; ++d) n[d] = m->Sf[f * 3 + d] / A;
+          const double dd = dot3(&m->CN[f * 3], n);
+          const double nc = n[comp];
+          C1[f] = (((((-sg) * 2) * gam) * A) * (nc * nc)) / dd;
+          C2[f] = 0.0;
+          double a[3];
+          for (int d = 0; d < 3; ++d) a[d] = (U[o * 3 + d] * l[d]) * (n[d] * l[d]);
+          const double dl = (a[0] + a[1]) + a[2];
+          const double V = ((((sg * 2) * gam) * A) * dl) * nc / dd;
+          T[f] = (C1[f] * U[o * 3 + comp] + C2[f] * U[b * 3 + comp]) + V;
+        } else {
+          C1[f] = C2[f] = T[f] = 0.0;
+        }
+      }
+    }
+    scatter_faces(m, s, C1, C2, T);
+  }
+  /* ---- -grad(p) on the right-hand side, sign +1
+   * (cfdAssemblePressureGradientTerm.m:31-40), element fluxes
+   * (cfdAssembleIntoGlobalMatrixElementFluxes.m:23-27) */
+  {
+    const double sg = 1.0;
+    for (int32_t i = 0; i < N; ++i) {
+      const double TE = (sg * m->vol[i]) * s->gradp[i * 3 + comp];
+      L->ac[i] = L->ac[i] + 0.0;
+      s->ac_old[i] = s->ac_old[i] + 0.0;
+      L->bc[i] = L->bc[i] - TE;
+    }
+  }
+  /* ---- false transient, fdt = 1e9, rho = rho_old = 1
+   * (cfdAssembleFalseTransientTerm.m:28-49; FLUXTEOLD is never scattered) */
+  {
+    const double fdt = 1e9;
+    for (int32_t i = 0; i < N; ++i) {
+      const double CE = (m->vol[i] * 1.0) / fdt;
+      const double CEOLD = ((-m->vol[i]) * 1.0) / fdt;
+      const double TE = CE * U[i * 3 + comp];
+      L->ac[i] = L->ac[i] + CE;
+      s->ac_old[i] = s->ac_old[i] + CEOLD;
+      L->bc[i] = L->bc[i] - TE;
+    }
+  }
+  /* ---- URF (cfdApplyURF.m:15-17) */
+  for (int32_t i = 0; i < N; ++i) L->ac[i] = L->ac[i] / c->urf_u;
+  /* ---- normalised residual (cfdComputeNormalizedResidual.m:64-88) */
+  {
+    double sq = 0.0, mx = 0.0;
+    for (int32_t i = 0; i < N; ++i) {
+      const double r = L->bc[i] / (L->ac[i] * s->scaleU);
+      if (fabs(r) > mx) mx = fabs(r);
+      sq = sq + r * r;
+    }
+    s->mx[comp] = mx;
+    s->rms[comp] = sqrt(sq / N);
+  }
+  /* ---- DU = V/ac, DUT = ac_old/ac, boundary rows from owners
+   * (cfdPostAssembleEquation.m:37-81) */
+  for (int32_t i = 0; i < N; ++i) {
+    s->DU[i * 3 + comp] = m->vol[i] / L->ac[i];
+    s->DUT[i * 3 + comp] = s->ac_old[i] / L->ac[i];
+  }
+  for (int32_t k = 0; k < FB(m); ++k) {
+    const int32_t o = m->owner[Fi + k];
+    s->DU[(N + k) * 3 + comp] = s->DU[o * 3 + comp];
+    s->DUT[(N + k) * 3 + comp] = s->DUT[o * 3 + comp];
+  }
+  free(C1); free(C2); free(T);
+}
+
+/* cfdSolver.m:11-12 -> cfdSolveAlgebraicSystem(1, 'ILU', 10) */
+void orc_solve_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s, int comp) {
+  (void)m;
+  orc_solve_algebraic_system(&s->H.lev[0], 0, c->u_sweeps, 0.1, &s->r0[comp], &s->r1[comp]);
+}
+
+/* cfdCorrectScalarEquation.m:16-24 (U(:,c) += dphi) and its empty-patch
+ * branch (U_b,c = U_O,c - (U_O . n) n_c) */
+void orc_correct_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s, int comp) {
+  const int32_t N = m->N;
+  const orc_level *L = &s->H.lev[0];
+  (void)c;
+  for (int32_t i = 0; i < N; ++i) s->U[i * 3 + comp] = s->U[i * 3 + comp] + L->dphi[i];
+  for (int p = 0; p < m->npatch; ++p) {
+    if (m->ptype[p] != ORC_PATCH_EMPTY) continue;
+    for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
+      const int32_t b = N + (f - m->Fi), o = m->owner[f];
+      double n[3];
+      const double nm = mag3(&m->Sf[f * 3]);
+      for (int d = 0; d < 3; ++d) n[d] = m->Sf[f * 3 + d] / nm;
+      const double un = dot3(&s->U[o * 3], n);
+      s->U[b * 3 + comp] = s->U[o * 3 + comp] - un * n[comp];
+    }
+  }
+}
+
+/* cfdAssembleMdotTerm.m:38-250: Rhie-Chow face fluxes (stored as the new
+ * mdot_f) and the pressure-correction matrix; boundary fluxes are zero for
+ * walls and empty patches (:281-350); the fix-pressure step is a no-op
+ * (:257-278); residual from the effective divergence
+ * (cfdComputeNormalizedResidual.m:27-30, cfdComputeEffectiveDivergence.m:21-54) */
+void orc_assemble_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s) {
+  const int32_t N = m->N, F = m->F, Fi = m->Fi;
+  orc_level *L = &s->H.lev[0];
+  zero_system(s);
+  double *T = dz(F);
+  const double *U = s->U, *DU = s->DU;
+  for (int32_t f = 0; f < Fi; ++f) {
+    const int32_t o = m->owner[f], nb = m->neighbour[f];
+    const double g = m->gf[f];
+    double du[3], ub[3], gpb[3], gpf[3];
+    for (int d = 0; d < 3; ++d) du[d] = g * DU[nb * 3 + d] + (1 - g) * DU[o * 3 + d];
+    const double mprev = s->mdot[f];
+    for (int d = 0; d < 3; ++d) ub[d] = g * U[nb * 3 + d] + (1 - g) * U[o * 3 + d];
+    const double pos = mprev > 0 ? 1.0 : 0.0;
+    const double rho_f = 1.0 * pos + 1.0 * (1 - pos);
+    /* Average (swapped weights) and Average:Corrected pressure gradients */
+    for (int d = 0; d < 3; ++d) gpb[d] = (1 - g) * s->gradp[nb * 3 + d] + g * s->gradp[o * 3 + d];
+    grad_face_corrected(m, s->p, s->gradp, 1, 0, f, gpf);
+    const double *Sf = &m->Sf[f * 3], *CF = &m->CN[f * 3], *eCF = &m->eCN[f * 3];
+    const double UbS = dot3(ub, Sf);
+    double V = 0.0 + rho_f * UbS;
+    double DUS[3], eDUS[3], DUE[3], DUT[3];
+    for (int d = 0; d < 3; ++d) DUS[d] = du[d] * Sf[d];
+    const double mDUS = mag3(DUS);
+    for (int d = 0; d < 3; ++d) eDUS[d] = DUS[d] / mDUS;
+    const double ee = dot3(eCF, eDUS);
+    for (int d = 0; d < 3; ++d) DUE[d] = mDUS * eCF[d] / ee;
+    const double geoDiff = mag3(DUE) / mag3(CF);
+    for (int d = 0; d < 3; ++d) DUT[d] = DUS[d] - DUE[d];
+    const double FC = 0.0 + rho_f * geoDiff;
+    const double FF = 0.0 - rho_f * geoDiff;
+    V = V - rho_f * dot3(gpf, DUT);
+    V = V + rho_f * dot3(gpb, DUS);
+    V = V + (1 - c->urf_u) * (mprev - rho_f * UbS);
+    const double md = (FC * s->p[o] + FF * s->p[nb]) + V;
+    s->mdot[f] = md;
+    s->fluxC1[f] = FC;
+    s->fluxC2[f] = FF;
+    T[f] = md;
+  }
+  for (int32_t f = Fi; f < F; ++f) s->fluxC1[f] = s->fluxC2[f] = T[f] = 0.0;
+  scatter_faces(m, s, s->fluxC1, s->fluxC2, T);
+  for (int32_t i = 0; i < N; ++i) {
+    L->ac[i] = L->ac[i] + 0.0;
+    s->ac_old[i] = s->ac_old[i] + 0.0;
+    L->bc[i] = L->bc[i] - 0.0;
+  }
+  double *eff = dz(N);
+  for (int32_t f = 0; f < Fi; ++f) {
+    eff[m->owner[f]] = eff[m->owner[f]] + s->mdot[f];
+    eff[m->neighbour[f]] = eff[m->neighbour[f]] - s->mdot[f];
+  }
+  for (int32_t f = Fi; f < F; ++f) eff[m->owner[f]] = eff[m->owner[f]] + s->mdot[f];
+  double mx = -INFINITY, sq = 0.0;
+  for (int32_t i = 0; i < N; ++i) {
+    if (eff[i] > mx) mx = eff[i];
+    sq = sq + eff[i] * eff[i];
+  }
+  s->mx[3] = mx;
+  s->rms[3] = sqrt(sq / N);
+  free(eff);
+  free(T);
+}
+
+/* cfdSolveEquation('p') -> cfdApplyAMG (rebuilds the hierarchy) */
+void orc_solve_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s) {
+  (void)m;
+  orc_hier_free(&s->H);
+  orc_agglomerate(&s->H, c->p_max_coarse);
+  orc_apply_amg(&s->H, c->p_cycle, c->p_max_cycles, c->p_termination, c->p_pre, c->p_post,
+                c->p_smoother, &s->r0[3], &s->r1[3], &s->ncycles);
+}
+
+/* cfdCorrectEquation.m:26-47 for p: PP field (cfdCorrectPPField.m:26-52 +
+ * wall/empty boundary copies + gradient), U (cfdCorrectVelocityField.m:26-55
+ * + empty patches), p (cfdCorrectPressureEquation.m:22-54 + boundary copies),
+ * mdot (cfdCorrectMdotField.m:24-50, empty patches zeroed :328-346) */
+void orc_correct_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s) {
+  const int32_t N = m->N, Fi = m->Fi;
+  const orc_level *L = &s->H.lev[0];
+  double ppc = 0.0;
+  if (c->p_ref >= 0) ppc = L->dphi[c->p_ref];
+  for (int32_t i = 0; i < N; ++i) s->PP[i] = L->dphi[i] - ppc;
+  for (int p = 0; p < m->npatch; ++p)
+    for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f)
+      s->PP[N + (f - Fi)] = s->PP[m->owner[f]];
+  orc_gradient(m, s->PP, 1, s->gradPP);
+  for (int32_t i = 0; i < N; ++i)
+    for (int d = 0; d < 3; ++d)
+      s->U[i * 3 + d] = s->U[i * 3 + d] - s->DU[i * 3 + d] * s->gradPP[i * 3 + d];
+  for (int p = 0; p < m->npatch; ++p) {
+    if (m->ptype[p] != ORC_PATCH_EMPTY) continue;
+    for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
+      const int32_t b = N + (f - Fi), o = m->owner[f];
+      double n[3];
+      const double nm = mag3(&m->Sf[f * 3]);
+      for (int d = 0; d < 3; ++d) n[d] = m->Sf[f * 3 + d] / nm;
+      const double un = dot3(&s->U[o * 3], n);
+      for (int d = 0; d < 3; ++d) s->U[b * 3 + d] = s->U[o * 3 + d] - un * n[d];
+    }
+  }
+  double ppf = 0.0;
+  if (c->p_ref >= 0) ppf = s->PP[c->p_ref];
+  for (int32_t i = 0; i < N; ++i) s->p[i] = s->p[i] + c->urf_p * (s->PP[i] - ppf);
+  for (int p = 0; p < m->npatch; ++p)
+    for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f)
+      s->p[N + (f - Fi)] = s->p[m->owner[f]];
+  for (int32_t f = 0; f < Fi; ++f)
+    s->mdot[f] = (s->mdot[f] + s->fluxC1[f] * s->PP[m->owner[f]]) +
+                 s->fluxC2[f] * s->PP[m->neighbour[f]];
+  for (int p = 0; p < m->npatch; ++p) {
+    if (m->ptype[p] != ORC_PATCH_EMPTY) continue;
+    for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) s->mdot[f] = 0.0 * s->mdot[f];
+  }
+}
+
+/* cfdRunCase.m:92-142, one steady iteration (equations U then p) */
+void orc_simple_iteration(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s) {
+  orc_update_fields(m, c, s);
+  for (int comp = 0; comp < 3; ++comp) {
+    orc_assemble_u(m, c, s, comp);
+    orc_solve_u(m, c, s, comp);
+    orc_correct_u(m, c, s, comp);
+  }
+  for (int k = 0; k < c->n_correctors; ++k) {
+    orc_assemble_p(m, c, s);
+    orc_solve_p(m, c, s);
+    orc_correct_p(m, c, s);
+  }
+  (void)patch_of;
+}

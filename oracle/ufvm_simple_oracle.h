@@ -1,0 +1,87 @@
+/*
+ * ufvm_simple_oracle.h -- CPU restatement of uFVM's steady incompressible
+ * SIMPLE outer iteration (momentum assembly, Rhie-Chow face fluxes,
+ * pressure-correction assembly, corrections, gradients, boundary updates).
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker of the HIP implementation and the
+ * CPU baseline); nothing in the product library links or calls it.
+ *
+ * Field layouts (0-based): cell fields hold N interior cells followed by the
+ * Fb boundary "elements" in boundary-face order (cfdSetupMeshField.m:36-55);
+ * vectors are [cell][3]; gradients of a vector are [cell][comp][3].
+ */
+#ifndef UFVM_SIMPLE_ORACLE_H
+#define UFVM_SIMPLE_ORACLE_H
+#include <stdint.h>
+
+#include "ufvm_oracle.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_PATCH_WALL = 0, ORC_PATCH_EMPTY = 1, ORC_PATCH_PATCH = 2 };
+enum { ORC_BC_FIXED_VALUE = 0, ORC_BC_ZERO_GRADIENT = 1, ORC_BC_EMPTY = 2 };
+
+typedef struct {
+  int32_t N, F, Fi;
+  const int32_t *owner, *neighbour;
+  const double *Sf, *area, *CN, *eCN, *gDiff, *T, *gf, *fc; /* fc = face centroid */
+  const double *vol, *cc;                                   /* cell volume, centroid */
+  const int32_t *cptr, *ccol;                               /* cconn CSR */
+  const int32_t *own_nnz, *nei_nnz;                         /* CSR slot of each interior face */
+  int32_t npatch;
+  const int32_t *pstart, *pnf, *ptype;                      /* patch ranges and physical type */
+  const int32_t *u_bc, *p_bc;                               /* per patch BC kinds */
+  const double *u_val;                                      /* per patch fixed U [3] */
+} orc_mesh;
+
+typedef struct {
+  double nu, urf_u, urf_p;
+  int32_t n_correctors;
+  int32_t p_ref;          /* 0-based pressure reference cell, -1 none */
+  int32_t u_sweeps;       /* ILU sweeps of the momentum solve (10) */
+  /* pressure-correction AMG (cfdApplyAMG settings) */
+  int32_t p_cycle, p_max_cycles, p_max_coarse, p_pre, p_post, p_smoother;
+  double p_termination;
+} orc_simple_cfg;
+
+typedef struct {
+  int32_t N, Fb;
+  double *U, *gradU;      /* [(N+Fb)*3], [(N+Fb)*9] */
+  double *p, *gradp;      /* [(N+Fb)], [(N+Fb)*3] */
+  double *PP, *gradPP;
+  double *DU, *DUT;       /* [(N+Fb)*3] */
+  double *mdot;           /* [F] */
+  double *gamma_f;        /* [F] */
+  double *fluxC1, *fluxC2;/* [F], kept from the last pressure assembly */
+  double scaleU;
+  orc_hier H;             /* level 1 = the equation being solved */
+  double *ac_old;
+  /* per equation component: U x,y,z and p */
+  double rms[4], mx[4], r0[4], r1[4];
+  int32_t ncycles;
+} orc_simple_state;
+
+void orc_simple_init(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s);
+void orc_simple_free(orc_simple_state *s);
+
+/* steps of one outer iteration (cfdRunCase.m:92-142) */
+void orc_update_fields(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s);
+void orc_assemble_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s, int comp);
+void orc_solve_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s, int comp);
+void orc_correct_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s, int comp);
+void orc_assemble_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s);
+void orc_solve_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s);
+void orc_correct_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s);
+/* one whole outer iteration */
+void orc_simple_iteration(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s);
+
+/* Green-Gauss gradient + wall-patch boundary gradients of a field with nc
+ * components (cfdUpdateGradient.m / cfdComputeGradientGauss0.m) */
+void orc_gradient(const orc_mesh *m, const double *phi, int nc, double *grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

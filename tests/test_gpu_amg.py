@@ -25,6 +25,8 @@ def to_scipy(ptr, col, ac, anb):
 CASES = {
     # 3-D jittered, renumbered hex cavity; Dirichlet on the lid
     "hex3d": lambda: (cavity_mesh(12, jitter=0.15, seed=31, renumber_seed=32), ("movingWall",)),
+    # large enough for several 1024-row blocks on the first coarse level
+    "hex3d24": lambda: (cavity_mesh(24, jitter=0.15, seed=36, renumber_seed=37), ("movingWall",)),
     # 40x40x1 plain cavity (lexicographic order)
     "hex2d": lambda: (cavity_mesh(40, nz=1, jitter=0.0, renumber_seed=None), ("movingWall",)),
     # jittered tets, Dirichlet on x=0
@@ -120,31 +122,38 @@ def test_apply_amg_bitexact(ctx, case, cycle, smoother, pre, post):
     ctx.system_destroy(sid)
 
 
-def test_pcg_vcycle_fp64_matches_oracle(ctx, case):
+@pytest.mark.parametrize("coarse_smoother,sweeps", [(0, 1), (1, 1), (1, 2)])
+def test_pcg_vcycle_fp64_matches_oracle(ctx, case, coarse_smoother, sweeps):
     _, m, ptr, col, ac, anb, b = case
     sid = make(ctx, ptr, col, ac, anb, b)
     ctx.agglomerate(sid, 10)
-    ctx.pcg_setup(sid, precision=64, pre_sweep=1, post_sweep=1, max_coarse_cells=200)
+    ctx.pcg_setup(sid, precision=64, pre_sweep=sweeps, post_sweep=sweeps, max_coarse_cells=200,
+                  coarse_smoother=coarse_smoother)
     S = oracle.System(ptr, col, ac, anb)
     S.agglomerate(10)
     r = np.random.default_rng(9).standard_normal(m.n_cells)
     z = ctx.pcg_apply_vcycle(sid, r)
-    z_ref = S.vcycle(r, pre=1, post=1, max_coarse_cells=200)
+    z_ref = S.vcycle(r, pre=sweeps, post=sweeps, max_coarse_cells=200,
+                     coarse_smoother=coarse_smoother)
     assert np.abs(z - z_ref).max() <= 1e-12 * np.abs(z_ref).max()
-    # same colourings as the oracle's greedy order
-    for l in range(len(S.colours) - 1):
+    # same colourings as the oracle's greedy order (level 1 always; the coarse
+    # levels when they use the block-local multicolour smoother)
+    nlev_col = len(S.colours) - 1 if coarse_smoother == 0 else 1
+    for l in range(nlev_col):
         nc, c = ctx.pcg_level_colours(sid, l)
         c_ref, nc_ref = S.colours_of(l)
         assert nc == nc_ref and np.array_equal(c, c_ref)
     ctx.system_destroy(sid)
 
 
-@pytest.mark.parametrize("precision,tol,err", [(64, 1e-12, 1e-10), (32, 1e-11, 1e-8)])
-def test_pcg_solve_matches_direct(ctx, case, precision, tol, err):
+@pytest.mark.parametrize("precision,tol,err,cs", [(64, 1e-12, 1e-10, 1), (32, 1e-11, 1e-8, 1),
+                                                  (32, 1e-11, 1e-8, 0)])
+def test_pcg_solve_matches_direct(ctx, case, precision, tol, err, cs):
     _, m, ptr, col, ac, anb, b = case
     sid = make(ctx, ptr, col, ac, anb, b)
     ctx.agglomerate(sid, 10)
-    ctx.pcg_setup(sid, precision=precision, max_coarse_cells=200, tol=tol, max_iter=300)
+    ctx.pcg_setup(sid, precision=precision, max_coarse_cells=200, tol=tol, max_iter=300,
+                  coarse_smoother=cs)
     it, rr = ctx.pcg_solve(sid)
     x = ctx.get_dphi(sid, 0)
     A = to_scipy(ptr, col, ac, anb)

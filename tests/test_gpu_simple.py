@@ -1,0 +1,121 @@
+"""GPU parity of the SIMPLE outer iteration against the oracle, through the C ABI.
+
+Every step of cfdRunCase's steady iteration -- field updates and gradients,
+momentum assembly, the ILU momentum solve, corrections, Rhie-Chow fluxes and
+pressure-correction assembly, the AMG pressure solve and the p' corrections --
+is compared bit for bit with the oracle, then whole iterations.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from matlab_ufvm_amd.simple import SimpleCase, cavity_bcs
+from matlab_ufvm_amd.workloads import cavity_mesh
+
+pytestmark = pytest.mark.gpu
+
+AMG = dict(p_cycle="V-Cycle", p_smoother="SOR", p_pre=2, p_post=2)
+
+MESHES = {
+    "cavity2d": lambda: (cavity_mesh(40, nz=1, jitter=0.0, renumber_seed=None), 0.01),
+    "cavity3d_jittered": lambda: (cavity_mesh(10, jitter=0.15, seed=51, renumber_seed=52), 0.0025),
+}
+
+
+@pytest.fixture(scope="module", params=sorted(MESHES))
+def pair(request, ctx):
+    m, nu = MESHES[request.param]()
+    print(f"\n[case {request.param}] cells={m.n_cells}", flush=True)
+    g = SimpleCase(ctx, m, cavity_bcs(m), nu=nu, **AMG)
+    o = oracle.SimpleCase(m, oracle.cavity_bcs(m), nu=nu, **AMG)
+    yield m, g, o
+    g.close()
+
+
+def gsys(ctx, g, name):
+    sid = g.system(name)
+    lv = ctx.level_get(sid, 0)
+    return dict(ac=lv["ac"], anb=lv["anb"], bc=ctx.get_bc(sid, 0), dphi=ctx.get_dphi(sid, 0))
+
+
+def same(a, b, what):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, what
+    bad = np.count_nonzero(a != b)
+    assert bad == 0, f"{what}: {bad} of {a.size} differ, max {np.abs(a - b).max():.3e}"
+
+
+def test_steps_bitexact(ctx, pair):
+    m, g, o = pair
+    for it in range(3):
+        g.cfdUpdateFields()
+        o.update_fields()
+        same(g.field("gradU"), o.field("gradU"), f"it{it} gradU")
+        same(g.field("gradp"), o.field("gradp"), f"it{it} gradp")
+        same(g.field("gamma_f"), o.field("gamma_f"), f"it{it} gamma_f")
+        for c in range(3):
+            rms, mx = g.cfdAssembleEquation("U", c)
+            o.assemble_u(c)
+            so, sg = o.system(), gsys(ctx, g, "U")
+            for k in ("ac", "anb", "bc"):
+                same(sg[k], so[k], f"it{it} U{c} {k}")
+            same(g.field("DU")[:, c], o.field("DU")[:, c], f"it{it} DU{c}")
+            r = o.residuals()
+            assert np.isclose(rms, r["rms"][c], rtol=1e-12, atol=1e-300)
+            assert mx == r["max"][c]
+            g.cfdSolveEquation("U")
+            o.solve_u(c)
+            same(gsys(ctx, g, "U")["dphi"], o.system()["dphi"], f"it{it} U{c} dphi")
+            g.cfdCorrectEquation("U", c)
+            o.correct_u(c)
+            same(g.field("U"), o.field("U"), f"it{it} U after U{c}")
+        for k in range(2):
+            rms, mx = g.cfdAssembleEquation("p")
+            o.assemble_p()
+            same(g.field("mdot_f"), o.field("mdot"), f"it{it} mdot (Rhie-Chow)")
+            so, sg = o.system(), gsys(ctx, g, "p")
+            for key in ("ac", "anb", "bc"):
+                same(sg[key], so[key], f"it{it} p {key}")
+            r = o.residuals()
+            assert mx == r["max"][3]
+            assert np.isclose(rms, r["rms"][3], rtol=1e-12, atol=1e-300)
+            r0, r1, nc = g.cfdSolveEquation("p")
+            o.solve_p()
+            assert nc == o.residuals()["ncycles"]
+            same(gsys(ctx, g, "p")["dphi"], o.system()["dphi"], f"it{it} p dphi")
+            g.cfdCorrectEquation("p")
+            o.correct_p()
+            for name, oname in (("PP", "PP"), ("gradPP", "gradPP"), ("U", "U"), ("p", "p"),
+                                ("mdot_f", "mdot")):
+                same(g.field(name), o.field(oname), f"it{it} corr{k} {name}")
+
+
+def test_iterations_match(ctx):
+    m = cavity_mesh(40, nz=1, jitter=0.0, renumber_seed=None)
+    g = SimpleCase(ctx, m, cavity_bcs(m), nu=0.01, **AMG)
+    o = oracle.SimpleCase(m, oracle.cavity_bcs(m), nu=0.01, **AMG)
+    res = g.iterate(40)
+    o.iterate(40)
+    U, Uo = g.field("U"), o.field("U")
+    err = np.linalg.norm(U - Uo) / np.linalg.norm(Uo)
+    print(f"U rel diff after 40 iterations: {err:.3e}; rms {res[-1]}")
+    assert err <= 1e-12
+    assert np.linalg.norm(g.field("p") - o.field("p")) <= 1e-12 * np.linalg.norm(o.field("p"))
+    assert np.all(res[-1, :2] < res[0, :2])
+    g.close()
+
+
+def test_precision_form_pressure_solver(ctx):
+    # the same SIMPLE iterations with the FP32-preconditioned AMG-PCG pressure
+    # solve reach the same fields as the reference AMG solve to within the
+    # solver tolerances
+    m = cavity_mesh(24, nz=1, jitter=0.1, seed=61, renumber_seed=62)
+    ref = SimpleCase(ctx, m, cavity_bcs(m), nu=0.01, p_termination=1e-10, p_max_cycles=200, **{
+        k: v for k, v in AMG.items()})
+    pcg = SimpleCase(ctx, m, cavity_bcs(m), nu=0.01, p_solver="pcg", pcg_tol=1e-10)
+    ref.iterate(30)
+    pcg.iterate(30)
+    U1, U2 = ref.field("U"), pcg.field("U")
+    assert np.linalg.norm(U1 - U2) <= 1e-6 * np.linalg.norm(U1)
+    ref.close()
+    pcg.close()

@@ -319,6 +319,24 @@ def test_vcycle_operator_is_symmetric(poisson16):
                        atol=1e-12 * np.abs(My).max())
 
 
+@pytest.mark.parametrize("coarse_smoother", [0, 1])
+def test_vcycle_symmetric_with_block_smoothed_levels(coarse_smoother):
+    # 13824 cells: the first coarse level has more than one 1024-row block, so
+    # the block-local smoother's Jacobi coupling is exercised
+    m = cavity_mesh(24, jitter=0.15, seed=36, renumber_seed=37)
+    ptr, col, ac, anb = poisson_system(m, dirichlet=("movingWall",))
+    S = oracle.System(ptr, col, ac, anb)
+    S.agglomerate(10)
+    assert S.level(1)["n"] > oracle.BLOCK_ROWS
+    rng = np.random.default_rng(6)
+    y, z = rng.standard_normal(m.n_cells), rng.standard_normal(m.n_cells)
+    for pre in (1, 2):
+        My = S.vcycle(y, pre=pre, post=pre, max_coarse_cells=100, coarse_smoother=coarse_smoother)
+        Mz = S.vcycle(z, pre=pre, post=pre, max_coarse_cells=100, coarse_smoother=coarse_smoother)
+        assert abs(z @ My - y @ Mz) <= 1e-11 * abs(z @ My)
+        assert y @ My > 0   # positive definite preconditioner
+
+
 def test_fpcg_matches_direct_solve(poisson16):
     m, ptr, col, ac, anb, b = poisson16
     S = oracle.System(ptr, col, ac, anb)
