@@ -384,6 +384,42 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
   FL.anb32.upload(a32, s);
   FL.b32.alloc(n); FL.x32.alloc(n); FL.r32.alloc(n); FL.y32.alloc(n);
   FL.b64.alloc(n); FL.x64.alloc(n); FL.r64.alloc(n); FL.y64.alloc(n);
+  // iterates start finite: the sliced-ELL padding multiplies x[row] by 0
+  for (DBuf<float> *v : {&FL.b32, &FL.x32, &FL.r32, &FL.y32})
+    UFVM_HIP(hipMemsetAsync(v->p, 0, n * sizeof(float), s));
+  for (DBuf<double> *v : {&FL.b64, &FL.x64, &FL.r64, &FL.y64})
+    UFVM_HIP(hipMemsetAsync(v->p, 0, n * sizeof(double), s));
+  // the FPCG's operator product runs on level 0's sliced-ELL copy whatever its
+  // kind (a single-level hierarchy makes level 0 the dense coarsest level)
+  if (kind == 0 || kind == 3 || l == 0) {
+    const int32_t nq = (n + 63) / 64;
+    FL.h_soff.assign(nq + 1, 0);
+    for (int32_t q = 0; q < nq; ++q) {
+      int32_t w = 0;
+      for (int32_t t = q * 64; t < std::min(n, q * 64 + 64); ++t)
+        w = std::max(w, FL.h_ptr[t + 1] - FL.h_ptr[t]);
+      FL.h_soff[q + 1] = FL.h_soff[q] + 64 * w;
+    }
+    const size_t tot = std::max(FL.h_soff[nq], 1);
+    // padding: col = the row itself, value 0 (rows past n in the last chunk
+    // are never processed)
+    std::vector<int32_t> scol(tot, 0);
+    std::vector<double> sv64(tot, 0.0);
+    for (int32_t t = 0; t < n; ++t) {
+      const int32_t q = t / 64, base = FL.h_soff[q] + (t & 63);
+      const int32_t w = (FL.h_soff[q + 1] - FL.h_soff[q]) / 64;
+      const int32_t len = FL.h_ptr[t + 1] - FL.h_ptr[t];
+      for (int32_t e = 0; e < w; ++e) {
+        scol[base + e * 64] = e < len ? col[FL.h_ptr[t] + e] : t;
+        sv64[base + e * 64] = e < len ? a64[FL.h_ptr[t] + e] : 0.0;
+      }
+    }
+    std::vector<float> sv32(sv64.begin(), sv64.end());
+    FL.soff.upload(FL.h_soff, s);
+    FL.scol.upload(scol, s);
+    FL.sval64.upload(sv64, s);
+    FL.sval32.upload(sv32, s);
+  }
   UFVM_HIP(hipStreamSynchronize(s));
 }
 
@@ -415,6 +451,7 @@ template <typename V>
 struct VecSel;
 template <>
 struct VecSel<float> {
+  static const float *sval(FastLevel &L) { return L.sval32.p; }
   static float *b(FastLevel &L) { return L.b32.p; }
   static float *x(FastLevel &L) { return L.x32.p; }
   static float *y(FastLevel &L) { return L.y32.p; }
@@ -424,6 +461,7 @@ struct VecSel<float> {
 };
 template <>
 struct VecSel<double> {
+  static const double *sval(FastLevel &L) { return L.sval64.p; }
   static double *b(FastLevel &L) { return L.b64.p; }
   static double *x(FastLevel &L) { return L.x64.p; }
   static double *y(FastLevel &L) { return L.y64.p; }
@@ -444,11 +482,13 @@ void mcgs_sweeps(ufvm_ctx *ctx, FastLevel &L, int nsweep, bool from_zero) {
     const int32_t r0 = L.col_off[c], r1 = L.col_off[c + 1];
     double bytes = 0.0;
     if (ctx->prof_on) {
+      // algorithmic bytes: the colour's CSR entries (col + value), its ac, b,
+      // x write, and one read of every other unknown
       const double R = r1 - r0, Z = L.h_ptr[r1] - L.h_ptr[r0];
-      bytes = 4.0 * (R + 1) + (4.0 + w) * Z + 3.0 * w * R + w * (L.n - R);
+      bytes = (4.0 + w) * Z + 3.0 * w * R + w * (L.n - R);
     }
     ProfScope ps(ctx, "mcgs0", bytes);
-    k::mcgs_rows<V>(ctx->stream, r0, r1, L.ptr.p, L.col.p, Sel::ac(L), Sel::anb(L), Sel::b(L),
+    k::mcgs_sell<V>(ctx->stream, r0, r1, L.soff.p, L.scol.p, Sel::sval(L), Sel::ac(L), Sel::b(L),
                     Sel::x(L), zero_above);
   };
   for (int sw = 0; sw < nsweep; ++sw) {
@@ -475,16 +515,15 @@ V *vcycle_fast(ufvm_ctx *ctx, System &S, int l) {
   const V omega = V(2.0 / 3.0);
   if (L.kind == 3) {
     // x already holds one Jacobi sweep from zero (fused into the restriction
-    // that produced b); further pre-sweeps ping-pong between x and y
+    // that produced b); further sweeps ping-pong between x and y
+    V *y = Sel::y(L);
     for (int it = 1; it < pre; ++it) {
-      k::jacobi<V>(s, L.n, L.ptr.p, L.col.p, Sel::ac(L), Sel::anb(L), Sel::b(L), x, nullptr,
-                   nullptr, Sel::y(L), omega, 1);
-      std::swap(L.x32.p, L.y32.p);
-      std::swap(L.x64.p, L.y64.p);
-      x = Sel::x(L);
+      k::jacobi_sell<V>(s, L.n, L.soff.p, L.scol.p, Sel::sval(L), Sel::ac(L), Sel::b(L), x, y,
+                        omega);
+      std::swap(x, y);
     }
     if (pre == 0) k::fill<V>(s, L.n, x, V(0));
-    k::residual_fast<V>(s, L.n, L.ptr.p, L.col.p, Sel::ac(L), Sel::anb(L), Sel::b(L), x,
+    k::residual_sell<V>(s, L.n, L.soff.p, L.scol.p, Sel::sval(L), Sel::ac(L), Sel::b(L), x,
                         Sel::r(L));
     if (C.kind == 3 && pre > 0)
       k::restrict_jacobi<V>(s, C.n, L.child_ptr.p, L.child.p, Sel::r(L), Sel::ac(C), Sel::b(C),
@@ -492,19 +531,13 @@ V *vcycle_fast(ufvm_ctx *ctx, System &S, int l) {
     else
       k::restrict_fast<V>(s, C.n, L.child_ptr.p, L.child.p, Sel::r(L), Sel::b(C));
     V *xc = vcycle_fast<V>(ctx, S, l + 1);
-    if (post == 0) {
-      k::prolong_fast<V>(s, L.n, L.parent.p, xc, x);
-      return x;
-    }
-    V *y = Sel::y(L);
-    k::jacobi<V>(s, L.n, L.ptr.p, L.col.p, Sel::ac(L), Sel::anb(L), Sel::b(L), x, L.parent.p, xc,
-                 y, omega, 2);
-    for (int it = 1; it < post; ++it) {
-      k::jacobi<V>(s, L.n, L.ptr.p, L.col.p, Sel::ac(L), Sel::anb(L), Sel::b(L), y, nullptr,
-                   nullptr, x, omega, 1);
+    k::prolong_fast<V>(s, L.n, L.parent.p, xc, x);
+    for (int it = 0; it < post; ++it) {
+      k::jacobi_sell<V>(s, L.n, L.soff.p, L.scol.p, Sel::sval(L), Sel::ac(L), Sel::b(L), x, y,
+                        omega);
       std::swap(x, y);
     }
-    return y;
+    return x;
   }
   if (L.kind == 0) {
     if (pre > 0) mcgs_sweeps<V>(ctx, L, pre, true);
@@ -515,7 +548,12 @@ V *vcycle_fast(ufvm_ctx *ctx, System &S, int l) {
                   Sel::anb(L), Sel::b(L), x, nullptr, nullptr, x, 0, pre);
     else k::fill<V>(s, L.n, x, V(0));
   }
-  k::residual_fast<V>(s, L.n, L.ptr.p, L.col.p, Sel::ac(L), Sel::anb(L), Sel::b(L), x, Sel::r(L));
+  if (L.kind == 0)
+    k::residual_sell<V>(s, L.n, L.soff.p, L.scol.p, Sel::sval(L), Sel::ac(L), Sel::b(L), x,
+                        Sel::r(L));
+  else
+    k::residual_fast<V>(s, L.n, L.ptr.p, L.col.p, Sel::ac(L), Sel::anb(L), Sel::b(L), x,
+                        Sel::r(L));
   if (C.kind == 3 && pre > 0)
     k::restrict_jacobi<V>(s, C.n, L.child_ptr.p, L.child.p, Sel::r(L), Sel::ac(C), Sel::b(C),
                           Sel::x(C), omega);
@@ -592,10 +630,10 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
   k::csr_to_dense(ctx->stream, nc, RC.ptr.p, RC.col.p, RC.ac.p, RC.anb.p, shift, P.ainv.p);
   k::gj_inverse(ctx->stream, nc, P.ainv.p, P.colbuf.p);
   const int32_t n0 = R0.n;
-  P.x.alloc(n0);
-  P.r.alloc(n0);
-  P.p.alloc(n0);
-  P.w.alloc(n0);
+  for (DBuf<double> *v : {&P.x, &P.r, &P.p, &P.w}) {
+    v->alloc(n0);
+    UFVM_HIP(hipMemsetAsync(v->p, 0, n0 * sizeof(double), ctx->stream));
+  }
   UFVM_HIP(hipStreamSynchronize(ctx->stream));
   P.ready = true;
 }
@@ -608,6 +646,7 @@ void pcg_solve_t(ufvm_ctx *ctx, System &S, int32_t *iters, double *relres) {
   FastLevel &L0 = *P.lev[0];
   hipStream_t s = ctx->stream;
   const int32_t n = R0.n;
+  if (!L0.soff.p || !L0.scol.p) throw Error(UFVM_ERR_STATE, "level 0 has no sliced-ELL copy");
   ensure_partial(ctx, n, 5);
   double *scal = ctx->scal.p;
   // r = P b (multicolour numbering), projected when singular
@@ -635,7 +674,7 @@ void pcg_solve_t(ufvm_ctx *ctx, System &S, int32_t *iters, double *relres) {
       {
         ProfScope ps(ctx, "spmv0",
                      ctx->prof_on ? 4.0 * (n + 1) + 12.0 * L0.nnz + 8.0 * 3 * n : 0.0);
-        k::spmv_dot(s, n, L0.ptr.p, L0.col.p, L0.ac64.p, L0.anb64.p, P.p.p, P.w.p,
+        k::spmv_dot(s, n, L0.soff.p, L0.scol.p, L0.ac64.p, L0.sval64.p, P.p.p, P.w.p,
                     ctx->partial.p, scal);
       }
       k::fcg_update_xr<V>(s, n, scal, P.p.p, P.w.p, P.x.p, P.r.p, Sel::b(L0), ctx->partial.p);

@@ -9,6 +9,7 @@ namespace ufvm {
 namespace k {
 
 constexpr int kBlock = 256;
+constexpr int kStreamBlocks = 2048;   // grid cap of the grid-stride FPCG kernels
 inline int nblocks(int64_t n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
 // ---- reference-order FP64 kernels (bit-exact with the reference arithmetic)
@@ -43,20 +44,23 @@ void coarse_lhs(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *c
                 const int32_t *cptr_c, double *cac, double *canb);
 
 // ---- multicolour / precision-form kernels (V = float or double)
+// sliced-ELL (64-row chunks, column-major inside a chunk) variants of the
+// colour pass, the residual and the damped Jacobi sweep
 template <typename V>
-void mcgs_rows(hipStream_t s, int32_t r0, int32_t r1, const int32_t *ptr, const int32_t *col,
-               const V *ac, const V *anb, const V *b, V *x, int32_t zero_above);
+void mcgs_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const int32_t *col,
+               const V *val, const V *ac, const V *b, V *x, int32_t zero_above);
+template <typename V>
+void residual_sell(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col,
+                   const V *val, const V *ac, const V *b, const V *x, V *r);
+template <typename V>
+void jacobi_sell(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, const V *val,
+                 const V *ac, const V *b, const V *x, V *out, V omega);
 // block-local multicolour GS over blocks of at most kBRows rows (see kernels.hip)
 constexpr int kBRows = 1024;
 template <typename V>
 void bmcgs(hipStream_t s, int32_t nblk, const int32_t *bptr, const int32_t *boff, int32_t ncol,
            const int32_t *ptr, const int32_t *col, const V *ac, const V *anb, const V *b,
            const V *x_in, const int32_t *parent, const V *xc, V *x_out, int mode, int nsweep);
-// damped Jacobi sweep (mode 1: from x_in, 2: from x_in + xc[parent])
-template <typename V>
-void jacobi(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col, const V *ac,
-            const V *anb, const V *b, const V *x_in, const int32_t *parent, const V *xc, V *out,
-            V omega, int mode);
 // restriction fused with a Jacobi sweep from zero on the coarse level
 template <typename V>
 void restrict_jacobi(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *child,
@@ -89,8 +93,8 @@ void csr_to_dense(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *c
 // is a streaming kernel writing per-block partial sums plus a one-block kernel
 // reducing them and updating the device scalars scal[] (layout in kernels.hip).
 // w = A p ; scal: delta = p.w, alpha = rho/delta
-void spmv_dot(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col, const double *ac,
-              const double *anb, const double *p, double *w, double *partial, double *scal);
+void spmv_dot(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, const double *ac,
+              const double *val, const double *p, double *w, double *partial, double *scal);
 // sums of z, z.w, z.r, w, r -> scal: m (mean of z if singular), rho, beta
 template <typename V>
 void fcg_dots(hipStream_t s, int32_t n, const V *z, const double *w, const double *r,

@@ -253,6 +253,42 @@ void coarse_lhs(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *c
 
 // --------------------------------------------------------- multicolour mode
 
+// ---- kernels on the sliced-ELL layout: rows in chunks of 64, chunk q holds
+// its entries column-major at off[q] + k*64 + lane for k < (off[q+1]-off[q])/64
+// (a wavefront reads 64 consecutive words per k).  Each row's entries keep its
+// cconn order and are followed by padding entries (col = the row itself,
+// value 0), so the loops run a uniform trip count with no exit test and the
+// loads of four entries are issued before their products are summed.
+
+// v -/+ sum_k val_k x[col_k] in entry order (NEG: subtract).  ZA: neighbours
+// with index >= zero_above read as 0.
+template <typename V, bool NEG, bool ZA>
+__device__ __forceinline__ V sell_row(const int32_t *__restrict__ col, const V *__restrict__ val,
+                                      const V *x, int32_t base, int32_t w, V v,
+                                      int32_t zero_above) {
+  int32_t k = 0;
+  for (; k + 4 <= w; k += 4) {
+    int32_t j[4];
+    V a[4], xj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      j[u] = col[base + (k + u) * 64];
+      a[u] = val[base + (k + u) * 64];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xj[u] = (!ZA || j[u] < zero_above) ? x[j[u]] : V(0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v = NEG ? v - a[u] * xj[u] : v + a[u] * xj[u];
+  }
+  for (; k < w; ++k) {
+    const int32_t j = col[base + k * 64];
+    const V a = val[base + k * 64];
+    const V xj = (!ZA || j < zero_above) ? x[j] : V(0);
+    v = NEG ? v - a * xj : v + a * xj;
+  }
+  return v;
+}
+
 // Gauss-Seidel update of the contiguous rows [r0, r1) of one colour class.
 // Rows of one colour are mutually independent, so every read of x[col] sees
 // the value from an earlier colour pass.  Neighbours with index >= zero_above
@@ -260,27 +296,70 @@ void coarse_lhs(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *c
 // are read as 0 (so x needs no clearing first).
 template <typename V>
 __global__ void __launch_bounds__(kBlock)
-k_mcgs(int32_t r0, int32_t r1, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
-       const V *__restrict__ ac, const V *__restrict__ anb, const V *__restrict__ b, V *x,
-       int32_t zero_above) {
-  const int32_t i = r0 + blockIdx.x * kBlock + threadIdx.x;
-  if (i >= r1) return;
-  V v = b[i];
-  const int32_t k0 = ptr[i], k1 = ptr[i + 1];
-  for (int32_t k = k0; k < k1; ++k) {
-    const int32_t j = col[k];
-    const V xj = (j < zero_above) ? x[j] : V(0);
-    v = v - anb[k] * xj;
-  }
+k_mcgs_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off,
+            const int32_t *__restrict__ col, const V *__restrict__ val,
+            const V *__restrict__ ac, const V *__restrict__ b, V *x, int32_t zero_above) {
+  const int32_t i = (r0 & ~63) + blockIdx.x * kBlock + threadIdx.x;
+  if (i < r0 || i >= r1) return;
+  const int32_t q = i >> 6;
+  const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
+  // zero_above beyond this colour's rows: every neighbour holds a value
+  const V v = (zero_above <= r1)
+                  ? sell_row<V, true, true>(col, val, x, base, w, b[i], zero_above)
+                  : sell_row<V, true, false>(col, val, x, base, w, b[i], zero_above);
   x[i] = v / ac[i];
 }
 
 template <typename V>
-void mcgs_rows(hipStream_t s, int32_t r0, int32_t r1, const int32_t *ptr, const int32_t *col,
-               const V *ac, const V *anb, const V *b, V *x, int32_t zero_above) {
+void mcgs_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const int32_t *col,
+               const V *val, const V *ac, const V *b, V *x, int32_t zero_above) {
   if (r1 <= r0) return;
-  hipLaunchKernelGGL(k_mcgs<V>, dim3(nblocks(r1 - r0)), dim3(kBlock), 0, s, r0, r1, ptr, col, ac,
-                     anb, b, x, zero_above);
+  const int32_t a = r0 & ~63;
+  hipLaunchKernelGGL(k_mcgs_sell<V>, dim3(nblocks(r1 - a)), dim3(kBlock), 0, s, r0, r1, off, col,
+                     val, ac, b, x, zero_above);
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kBlock)
+k_residual_sell(int32_t n, const int32_t *__restrict__ off, const int32_t *__restrict__ col,
+                const V *__restrict__ val, const V *__restrict__ ac, const V *__restrict__ b,
+                const V *__restrict__ x, V *__restrict__ r) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int32_t q = i >> 6;
+  const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
+  r[i] = sell_row<V, true, false>(col, val, x, base, w, b[i] - ac[i] * x[i], 0);
+}
+
+template <typename V>
+void residual_sell(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col,
+                   const V *val, const V *ac, const V *b, const V *x, V *r) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_residual_sell<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, off, col, val,
+                     ac, b, x, r);
+}
+
+// one damped Jacobi sweep: out = x + omega (b - A x) / ac
+template <typename V>
+__global__ void __launch_bounds__(kBlock)
+k_jacobi_sell(int32_t n, const int32_t *__restrict__ off, const int32_t *__restrict__ col,
+              const V *__restrict__ val, const V *__restrict__ ac, const V *__restrict__ b,
+              const V *__restrict__ x, V *__restrict__ out, V omega) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int32_t q = i >> 6;
+  const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
+  const V xi = x[i], d = ac[i];
+  const V v = sell_row<V, true, false>(col, val, x, base, w, b[i] - d * xi, 0);
+  out[i] = xi + omega * v / d;
+}
+
+template <typename V>
+void jacobi_sell(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, const V *val,
+                 const V *ac, const V *b, const V *x, V *out, V omega) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_jacobi_sell<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, off, col, val, ac,
+                     b, x, out, omega);
 }
 
 // Block-local multicolour Gauss-Seidel (coarse levels).  Rows are numbered
@@ -345,38 +424,6 @@ void bmcgs(hipStream_t s, int32_t nblk, const int32_t *bptr, const int32_t *boff
                      anb, b, x_in, parent, xc, x_out, mode, nsweep);
 }
 
-// Damped Jacobi sweep on a coarse level: out_i = v_i + w (b_i - (A v)_i) / ac_i
-// with v = x_in (mode 1) or x_in + xc[parent] (mode 2, prolongation fused into
-// the post-smoother).  Rows summed in cconn order.  out must not alias x_in.
-template <typename V>
-__global__ void __launch_bounds__(kBlock)
-k_jacobi(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
-         const V *__restrict__ ac, const V *__restrict__ anb, const V *__restrict__ b,
-         const V *__restrict__ x_in, const int32_t *__restrict__ parent,
-         const V *__restrict__ xc, V *__restrict__ out, V omega, int mode) {
-  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  V vi = x_in[i];
-  if (mode == 2) vi = vi + xc[parent[i]];
-  V r = b[i] - ac[i] * vi;
-  for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
-    const int32_t j = col[k];
-    V vj = x_in[j];
-    if (mode == 2) vj = vj + xc[parent[j]];
-    r = r - anb[k] * vj;
-  }
-  out[i] = vi + omega * r / ac[i];
-}
-
-template <typename V>
-void jacobi(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col, const V *ac,
-            const V *anb, const V *b, const V *x_in, const int32_t *parent, const V *xc, V *out,
-            V omega, int mode) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_jacobi<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, ptr, col, ac, anb, b,
-                     x_in, parent, xc, out, omega, mode);
-}
-
 // restriction fused with the first Jacobi sweep from zero on the coarse level:
 // bc[p] = sum of children's residuals; xc[p] = w bc[p] / ac[p]
 template <typename V>
@@ -429,8 +476,17 @@ k_dense_gemv(int32_t n, const double *__restrict__ ainv, const V *__restrict__ b
   const int lane = threadIdx.x & (kWave - 1);
   if (row >= n) return;
   const double *a = ainv + (int64_t)row * n;
-  double s = 0.0;
-  for (int32_t j = lane; j < n; j += kWave) s += a[j] * (double)b[j];
+  // four independent partial sums keep four loads in flight per lane
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  int32_t j = lane;
+  for (; j + 3 * kWave < n; j += 4 * kWave) {
+    s0 += a[j] * (double)b[j];
+    s1 += a[j + kWave] * (double)b[j + kWave];
+    s2 += a[j + 2 * kWave] * (double)b[j + 2 * kWave];
+    s3 += a[j + 3 * kWave] * (double)b[j + 3 * kWave];
+  }
+  for (; j < n; j += kWave) s0 += a[j] * (double)b[j];
+  double s = (s0 + s1) + (s2 + s3);
   s = wave_sum(s);
   if (lane == 0) x[row] = (V)s;
 }
@@ -545,20 +601,22 @@ void gj_inverse(hipStream_t s, int32_t n, double *a, double *colbuf) {
 // them in block order and does the scalar arithmetic (no grid-wide atomics or
 // device-scope fences in the streaming kernels).
 
-// w = A p ; partial = p.w
+// w = A p (sliced-ELL layout) ; partial = p.w.  Grid-stride over rows with at
+// most kStreamBlocks blocks, so the reduction step reads few partials.
 __global__ void __launch_bounds__(kBlock)
-k_spmv_dot(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
-           const double *__restrict__ ac, const double *__restrict__ anb,
-           const double *__restrict__ p, double *__restrict__ w, double *__restrict__ partial) {
+k_spmv_dot(int32_t n, const int32_t *__restrict__ off, const int32_t *__restrict__ col,
+           const double *__restrict__ ac,
+           const double *__restrict__ val, const double *__restrict__ p, double *__restrict__ w,
+           double *__restrict__ partial) {
   __shared__ double lds[kBlock / kWave];
-  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   double d = 0.0;
-  if (i < n) {
+  for (int32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const int32_t q = i >> 6;
+    const int32_t base = off[q] + (i & 63), wq = (off[q + 1] - off[q]) >> 6;
     const double pi = p[i];
-    double v = ac[i] * pi;
-    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) v = v + anb[k] * p[col[k]];
+    const double v = sell_row<double, false, false>(col, val, p, base, wq, ac[i] * pi, 0);
     w[i] = v;
-    d = pi * v;
+    d += pi * v;
   }
   d = block_sum(d, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = d;
@@ -570,15 +628,14 @@ __global__ void __launch_bounds__(kBlock)
 k_fcg_dots(int32_t n, const V *__restrict__ z, const double *__restrict__ w,
            const double *__restrict__ r, double *__restrict__ partial) {
   __shared__ double lds[kBlock / kWave];
-  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   double v[5] = {0, 0, 0, 0, 0};
-  if (i < n) {
+  for (int32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     const double zi = (double)z[i], wi = w[i], ri = r[i];
-    v[0] = zi;
-    v[1] = zi * wi;
-    v[2] = zi * ri;
-    v[3] = wi;
-    v[4] = ri;
+    v[0] += zi;
+    v[1] += zi * wi;
+    v[2] += zi * ri;
+    v[3] += wi;
+    v[4] += ri;
   }
   for (int c = 0; c < 5; ++c) v[c] = block_sum(v[c], lds);
   if (threadIdx.x == 0)
@@ -613,20 +670,23 @@ k_fcg_reduce(const double *__restrict__ partial, int32_t nblk, int mode, int32_t
   }
 }
 
-void spmv_dot(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col, const double *ac,
-              const double *anb, const double *p, double *w, double *partial, double *scal) {
-  hipLaunchKernelGGL(k_spmv_dot, dim3(nblocks(n)), dim3(kBlock), 0, s, n, ptr, col, ac, anb, p, w,
+static int stream_blocks(int32_t n) { return nblocks(n) < kStreamBlocks ? nblocks(n) : kStreamBlocks; }
+
+void spmv_dot(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, const double *ac,
+              const double *val, const double *p, double *w, double *partial, double *scal) {
+  const int g = stream_blocks(n);
+  hipLaunchKernelGGL(k_spmv_dot, dim3(g), dim3(kBlock), 0, s, n, off, col, ac, val, p, w,
                      partial);
-  hipLaunchKernelGGL(k_fcg_reduce, dim3(1), dim3(kBlock), 0, s, partial, nblocks(n), 1, n, 0, 0,
-                     scal);
+  hipLaunchKernelGGL(k_fcg_reduce, dim3(1), dim3(kBlock), 0, s, partial, g, 1, n, 0, 0, scal);
 }
 
 template <typename V>
 void fcg_dots(hipStream_t s, int32_t n, const V *z, const double *w, const double *r,
               double *partial, double *scal, int singular, int first) {
-  hipLaunchKernelGGL(k_fcg_dots<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, z, w, r, partial);
-  hipLaunchKernelGGL(k_fcg_reduce, dim3(1), dim3(kBlock), 0, s, partial, nblocks(n), 0, n,
-                     singular, first, scal);
+  const int g = stream_blocks(n);
+  hipLaunchKernelGGL(k_fcg_dots<V>, dim3(g), dim3(kBlock), 0, s, n, z, w, r, partial);
+  hipLaunchKernelGGL(k_fcg_reduce, dim3(1), dim3(kBlock), 0, s, partial, g, 0, n, singular, first,
+                     scal);
 }
 
 template <typename V>
@@ -654,15 +714,14 @@ k_fcg_update_xr(int32_t n, const double *__restrict__ scal, const double *__rest
                 const double *__restrict__ w, double *__restrict__ x, double *__restrict__ r,
                 V *__restrict__ rv, double *__restrict__ partial) {
   __shared__ double lds[kBlock / kWave];
-  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   double rr = 0.0;
-  if (i < n) {
-    const double a = scal[7];
+  const double a = scal[7];
+  for (int32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     x[i] = x[i] + a * p[i];
     const double ri = r[i] - a * w[i];
     r[i] = ri;
     rv[i] = (V)ri;
-    rr = ri * ri;
+    rr += ri * ri;
   }
   rr = block_sum(rr, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = rr;
@@ -671,10 +730,10 @@ k_fcg_update_xr(int32_t n, const double *__restrict__ scal, const double *__rest
 template <typename V>
 void fcg_update_xr(hipStream_t s, int32_t n, double *scal, const double *p, const double *w,
                    double *x, double *r, V *rv, double *partial) {
-  hipLaunchKernelGGL(k_fcg_update_xr<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, scal, p, w, x,
-                     r, rv, partial);
-  hipLaunchKernelGGL(k_fcg_reduce, dim3(1), dim3(kBlock), 0, s, partial, nblocks(n), 2, n, 0, 0,
-                     scal);
+  const int g = stream_blocks(n);
+  hipLaunchKernelGGL(k_fcg_update_xr<V>, dim3(g), dim3(kBlock), 0, s, n, scal, p, w, x, r, rv,
+                     partial);
+  hipLaunchKernelGGL(k_fcg_reduce, dim3(1), dim3(kBlock), 0, s, partial, g, 2, n, 0, 0, scal);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -716,14 +775,15 @@ void axpy_const(hipStream_t s, int32_t n, const double *scal_slot, double mul, d
 
 // explicit instantiations
 #define UFVM_INST(V)                                                                            \
-  template void mcgs_rows<V>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,   \
-                             const V *, const V *, const V *, V *, int32_t);                    \
   template void bmcgs<V>(hipStream_t, int32_t, const int32_t *, const int32_t *, int32_t,       \
                          const int32_t *, const int32_t *, const V *, const V *, const V *,     \
                          const V *, const int32_t *, const V *, V *, int, int);                 \
-  template void jacobi<V>(hipStream_t, int32_t, const int32_t *, const int32_t *, const V *,     \
-                          const V *, const V *, const V *, const int32_t *, const V *, V *, V, \
-                          int);                                                                \
+  template void mcgs_sell<V>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,    \
+                             const V *, const V *, const V *, V *, int32_t);                    \
+  template void residual_sell<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,         \
+                                 const V *, const V *, const V *, const V *, V *);              \
+  template void jacobi_sell<V>(hipStream_t, int32_t, const int32_t *, const int32_t *, const V *, \
+                               const V *, const V *, const V *, V *, V);                        \
   template void restrict_jacobi<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,      \
                                    const V *, const V *, V *, V *, V);                         \
   template void residual_fast<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,        \

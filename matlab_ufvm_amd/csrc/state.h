@@ -98,6 +98,12 @@ struct FastLevel {
   DBuf<float> ac32, anb32, b32, x32, r32, y32;   // y: second iterate buffer (kind 1)
   DBuf<double> ac64, anb64, b64, x64, r64, y64;
   DBuf<int32_t> parent, child_ptr, child;   // to / from the next level (new numbering)
+  // sliced-ELL copy of the off-diagonals (kinds 0 and 3): 64-row chunks, chunk
+  // q at [soff[q], soff[q+1]) column-major, padding col = -1
+  std::vector<int32_t> h_soff;
+  DBuf<int32_t> soff, scol;
+  DBuf<float> sval32;
+  DBuf<double> sval64;
 };
 
 struct PcgState {

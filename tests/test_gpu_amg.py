@@ -181,6 +181,24 @@ def test_pcg_singular_neumann(ctx):
     ctx.system_destroy(sid)
 
 
+@pytest.mark.parametrize("precision", [64, 32])
+def test_pcg_single_level(ctx, precision):
+    # every cell fits the dense coarsest level: the preconditioner is the exact
+    # inverse and the FPCG still runs its operator product on level 0
+    m = cavity_mesh(9, jitter=0.1, seed=44, renumber_seed=45)
+    ptr, col, ac, anb = poisson_system(m, dirichlet=("movingWall",))
+    b = gaussian_rhs(m, seed=46)
+    sid = make(ctx, ptr, col, ac, anb, b)
+    ctx.agglomerate(sid, 10)
+    ctx.pcg_setup(sid, precision=precision, max_coarse_cells=4096, tol=1e-12, max_iter=50)
+    it, rr = ctx.pcg_solve(sid)
+    x = ctx.get_dphi(sid, 0)
+    xs = spl.spsolve(to_scipy(ptr, col, ac, anb).tocsc(), b)
+    assert rr <= 1e-12 and it <= 4, (it, rr)
+    assert np.linalg.norm(x - xs) <= 1e-10 * np.linalg.norm(xs)
+    ctx.system_destroy(sid)
+
+
 def test_errors_are_status_codes(ctx):
     from matlab_ufvm_amd._lib import UfvmError
     with pytest.raises(UfvmError):
