@@ -35,23 +35,45 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def build_problem(n):
+def build_problem(n, workload, ctx):
+    """The pressure-correction system the solves run on.
+
+    pcorr-k5: the 128^3 lid-driven cavity (Re 400, nu = 1/400, lid u = 1) run
+    through four SIMPLE outer iterations on the GPU (FP32-preconditioned AMG-PCG
+    pressure solves), then the fifth iteration's momentum step and its
+    pressure-correction assembly (cfdAssembleMdotTerm.m:70-250); its matrix and
+    mass-imbalance right-hand side are the workload.  laplacian: the gDiff
+    Laplacian of the same mesh with a Gaussian right-hand side."""
     from matlab_ufvm_amd.workloads import cavity_mesh, gaussian_rhs, poisson_system
     t = time.time()
-    m = cavity_mesh(n, jitter=0.15, seed=2026, renumber_seed=2027)
-    ptr, col, ac, anb = poisson_system(m, dirichlet=())
-    b = gaussian_rhs(m, seed=2030)
-    log(f"mesh {n}^3: cells={m.n_cells} faces={m.n_faces} nnz={col.size} ({time.time()-t:.1f}s)")
-    return m, ptr, col, ac, anb, b
+    # own-face pyramid apex: with the reference's apex (global faces 1..6) the
+    # warped faces of this jittered mesh displace far cells' centroids by
+    # several cells and the SIMPLE momentum solve diverges
+    m = cavity_mesh(n, jitter=0.15, seed=2026, renumber_seed=2027, apex="own")
+    log(f"mesh {n}^3: cells={m.n_cells} faces={m.n_faces} ({time.time()-t:.1f}s)")
+    if workload == "laplacian":
+        ptr, col, ac, anb = poisson_system(m, dirichlet=())
+        return m, ptr, col, ac, anb, gaussian_rhs(m, seed=2030)
+    from matlab_ufvm_amd.simple import SimpleCase, cavity_bcs
+    t = time.time()
+    case = SimpleCase(ctx, m, cavity_bcs(m), nu=1.0 / 400.0, p_solver="pcg", pcg_tol=1e-8)
+    res = case.iterate(4)
+    case.cfdUpdateFields()
+    case.cfdAssembleAndCorrectEquation("U")
+    case.cfdAssembleEquation("p")
+    psid = case.system("p")
+    L = ctx.level_get(psid, 0)
+    b = ctx.get_bc(psid, 0)
+    case.close()
+    log(f"SIMPLE k=5 pressure-correction system ({time.time()-t:.1f}s; residuals after 4 "
+        f"iterations {np.array2string(res[-1], precision=3)})")
+    return m, L["ptr"], L["col"], L["ac"], L["anb"], b
 
 
-def timed_solves(ctx, sid, steps, warmup, prof=False):
+def timed_solves(ctx, sid, steps, warmup):
     for _ in range(warmup):
         ctx.pcg_solve(sid)
     lat, its = [], []
-    if prof:
-        ctx.prof_reset()
-        ctx.prof_enable(True)
     ctx.synchronize()          # device-wide: the timed region starts idle
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -61,9 +83,20 @@ def timed_solves(ctx, sid, steps, warmup, prof=False):
         its.append(it)
     ctx.synchronize()
     total = time.perf_counter() - t0
-    if prof:
-        ctx.prof_enable(False)
     return total, np.array(lat), its, rr
+
+
+def event_timed_solves(ctx, sid, steps):
+    """The same solves with HIP events recorded on the library's stream around
+    every finest-level colour pass and operator product (kernel durations for
+    the roofline); kept out of the headline timing, whose launches they slow."""
+    ctx.prof_reset()
+    ctx.prof_enable(True)
+    for _ in range(steps):
+        ctx.pcg_solve(sid)
+    ctx.synchronize()
+    ctx.prof_enable(False)
+    return ctx.prof_read("mcgs0"), ctx.prof_read("spmv0")
 
 
 def cpu_baseline(ptr, col, ac, anb, b, max_coarse, coarse_smoother):
@@ -91,6 +124,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--workload", choices=["pcorr-k5", "laplacian"], default="pcorr-k5")
     ap.add_argument("--pre", type=int, default=1)
     ap.add_argument("--post", type=int, default=1)
     ap.add_argument("--max-coarse", type=int, default=2000)
@@ -98,15 +132,19 @@ def main():
                     help="levels below the first: 0 block-local MCGS, 1 damped Jacobi")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--event-steps", type=int, default=5,
+                    help="solves re-run with per-kernel HIP events for the roofline")
+    ap.add_argument("--plain-cycles", type=int, default=20,
+                    help="timed V-cycles of the plain FP64 drop-in AMG (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs on one GPU: the pressure solve of one mesh "
-                                   "is not sharded across GPUs (see DESIGN.md); use --gpus 1"}))
+                                   "is not sharded across GPUs; use --gpus 1"}))
         sys.exit(2)
 
     from matlab_ufvm_amd._lib import Context
-    m, ptr, col, ac, anb, b = build_problem(args.n)
     ctx = Context(0)
+    m, ptr, col, ac, anb, b = build_problem(args.n, args.workload, ctx)
     sid = ctx.system_create(ptr, col)
     ctx.set_coefficients(sid, ac, anb, b)
     t = time.time()
@@ -122,12 +160,11 @@ def main():
                       max_coarse_cells=args.max_coarse, tol=1e-8, max_iter=500,
                       coarse_smoother=args.coarse_smoother)
         t_setup = time.time() - t
-        total, lat, its, rr = timed_solves(ctx, sid, args.steps, args.warmup, prof=(prec == 32))
+        total, lat, its, rr = timed_solves(ctx, sid, args.steps, args.warmup)
         x = ctx.get_dphi(sid, 0)
         res = dict(total=total, lat=lat, its=its, relres=rr, x=x, setup=t_setup)
         if prec == 32:
-            ms0, n0, by0 = ctx.prof_read("mcgs0")
-            res["prof"] = (ms0, n0, by0)
+            res["prof"], res["prof_spmv"] = event_timed_solves(ctx, sid, args.event_steps)
         results[prec] = res
         log(f"fp{prec}-precond: {args.steps / total:.2f} solves/s, iters {its[0]}, "
             f"p50 {np.median(lat)*1e3:.2f} ms, p99 {np.percentile(lat, 99)*1e3:.2f} ms,"
@@ -146,6 +183,35 @@ def main():
         d = x - x[0]
         return float(np.linalg.norm(d - d_ref) / np.linalg.norm(d_ref))
 
+    # plain FP64 drop-in: the reference's stationary AMG (V-cycle, agglomeration
+    # pairs, 2 lexicographic Gauss-Seidel sweeps pre and post, cfdApplyAMG.m) on
+    # the GPU, bit-exact with the reference arithmetic.  On this singular
+    # all-Neumann system it stalls before 1e-8 (its coarsest level is smoothed,
+    # not solved, so the constant mode drifts every cycle), so a bounded number
+    # of cycles is timed and the residual ratio and field error it reaches are
+    # reported beside the per-cycle time.
+    plain = None
+    if args.plain_cycles > 0:
+        zero = np.zeros(m.n_cells)
+        def plain_run(ncyc):
+            ctx.set_dphi(sid, 0, zero)
+            ctx.synchronize()
+            s = time.perf_counter()
+            r0, r1, nc = ctx.apply_amg(sid, "V-Cycle", max_cycles=ncyc, termination=1e-8,
+                                       max_coarse_levels=10, pre_sweep=2, post_sweep=2,
+                                       smoother="SOR", rebuild=0)
+            ctx.synchronize()
+            return time.perf_counter() - s, nc - 1, r1 / r0
+        plain_run(1)                          # level schedules built on first use
+        tp, ncyc, ratio = plain_run(args.plain_cycles)
+        plain = {"ms_per_cycle": round(tp / ncyc * 1e3, 3), "cycles": int(ncyc),
+                 "l1_residual_ratio": ratio,
+                 "field_rel_l2_err": ferr(ctx.get_dphi(sid, 0)),
+                 "reached_1e-8": bool(ratio <= 1e-8),
+                 "method": "cfdApplyAMG V-cycle, SOR 2/2, FP64, bit-exact reference arithmetic"}
+        log(f"plain fp64 AMG: {plain['ms_per_cycle']:.2f} ms/cycle, {ncyc} cycles, L1 ratio "
+            f"{ratio:.2e}, field err {plain['field_rel_l2_err']:.2e}")
+
     r32 = results[32]
     ms0, n0, by0 = r32["prof"]
     avg_ms = ms0 / max(n0, 1)
@@ -157,9 +223,12 @@ def main():
         "ms_per_step": round(r32["total"] / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 V-cycle, f64 FPCG", "data": "synthetic",
-        "config": {"workload": f"pressure solve, 3-D cavity {args.n}^3 jittered hex "
-                               f"(+-15%, seed 2026), permuted+RCM (seed 2027), all-wall "
-                               f"Neumann Laplacian (gDiff), Gaussian RHS (seed 2030)",
+        "config": {"workload": (f"pressure-correction solve, 3-D lid-driven cavity {args.n}^3 "
+                                f"jittered hex (+-15%, seed 2026), permuted+RCM (seed 2027), "
+                                f"own-face pyramid apex, "
+                                + ("matrix and RHS of SIMPLE outer iteration 5 (Re 400)"
+                                   if args.workload == "pcorr-k5" else
+                                   "all-wall Neumann gDiff Laplacian, Gaussian RHS (seed 2030)")),
                    "cells": int(m.n_cells), "faces": int(m.n_faces), "nnz_offdiag": int(col.size),
                    "tol": 1e-8, "amg_levels": sizes, "pre_sweep": args.pre,
                    "post_sweep": args.post, "max_coarse_cells": args.max_coarse,
@@ -170,13 +239,21 @@ def main():
         "iterations": int(r32["its"][0]),
         "field_rel_l2_err": ferr(r32["x"]),
         "setup_s": {"agglomerate": round(t_agg, 3), "pcg_setup": round(r32["setup"], 3)},
-        "roofline": {"bound": "hbm", "kernel": "k_mcgs<float> (level-1 colour pass)",
+        "roofline": {"bound": "hbm", "kernel": "k_mcgs_sell<float> (finest-level colour pass)",
                      "achieved": round(achieved / 1e9, 1), "peak": HBM_PEAK / 1e9,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
                      "avg_launch_us": round(avg_ms * 1e3, 2), "launches": int(n0),
                      "algorithmic_bytes_per_launch": round(by0 / max(n0, 1)),
-                     "traffic": None},
+                     "traffic": None,
+                     "measured": f"HIP events on the library stream, {args.event_steps} "
+                                 f"solves re-run after the timed ones"},
     }
+    ms1, n1, by1 = r32["prof_spmv"]
+    if n1:
+        out["roofline_spmv"] = {"kernel": "k_spmv_dot (FP64 finest-level operator product)",
+                                "achieved": round(by1 / n1 / (ms1 / n1 * 1e-3) / 1e9, 1),
+                                "avg_launch_us": round(ms1 / n1 * 1e3, 2),
+                                "algorithmic_bytes_per_launch": round(by1 / n1)}
     if 64 in results:
         r64 = results[64]
         v64 = args.steps / r64["total"]
@@ -185,6 +262,8 @@ def main():
                            "p99_ms": round(float(np.percentile(r64["lat"], 99)) * 1e3, 3),
                            "field_rel_l2_err": ferr(r64["x"])}
         out["speedup_fp32_vs_fp64_pcg"] = round(value / v64, 3)
+    if plain is not None:
+        out["plain_fp64_amg"] = plain
     if not args.no_cpu:
         cb, x_cpu = cpu_baseline(ptr, col, ac, anb, b, args.max_coarse, args.coarse_smoother)
         cb["field_rel_l2_err_vs_gpu_tight"] = ferr(x_cpu)

@@ -309,7 +309,8 @@ double fcycle(Cyc &c, int G) {  // cfdApplyFCycle.m:5-63
 // contiguous row range.  kind 1 (coarse levels): blocks of kBRows consecutive
 // reference rows, each sorted by a colouring of the block's own subgraph.
 // kind 2: the coarsest level, kept in reference order for the dense solve.
-// Rows keep their cconn order in every kind.
+// kind 3 (coarse levels smoothed by damped Jacobi): rows sorted by length
+// inside windows.  Rows keep their cconn order in every kind.
 void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
   hipStream_t s = ctx->stream;
   RefLevel &R = *S.lev[l];
@@ -353,8 +354,20 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
     FL.h_bptr[FL.nblk] = n;
     FL.bptr.upload(FL.h_bptr, s);
     FL.boff.upload(FL.h_boff, s);
+  } else if (kind == 3) {
+    // Jacobi-smoothed: row order is free, so rows are sorted by decreasing
+    // length inside windows of kSigma rows; the 64-row sliced-ELL chunks then
+    // carry little padding while neighbours stay close in memory
+    constexpr int32_t kSigma = 256;
+    FL.ncol = 1;
+    FL.colour.assign(n, 0);
+    for (int32_t i = 0; i < n; ++i) FL.perm[i] = i;
+    auto len = [&](int32_t i) { return R.g.ptr[i + 1] - R.g.ptr[i]; };
+    for (int32_t w0 = 0; w0 < n; w0 += kSigma)
+      std::stable_sort(FL.perm.begin() + w0, FL.perm.begin() + std::min(n, w0 + kSigma),
+                       [&](int32_t a, int32_t b) { return len(a) > len(b); });
   } else {
-    // kind 2 (dense coarsest) and kind 3 (Jacobi-smoothed): reference order
+    // kind 2 (dense coarsest): reference order
     FL.ncol = 1;
     FL.colour.assign(n, 0);
     for (int32_t i = 0; i < n; ++i) FL.perm[i] = i;
@@ -441,9 +454,21 @@ void link_levels(ufvm_ctx *ctx, System &S, int l) {
   std::vector<int32_t> child(F.n);
   std::vector<int32_t> fillc(cnt.begin(), cnt.end() - 1);
   for (int32_t i = 0; i < F.n; ++i) child[fillc[ipc[R.h_parents[i]]]++] = ipf[i];
+  // the same lists in 64-row chunks, column-major, padded with -1
+  const int32_t nq = (C.n + 63) / 64;
+  std::vector<int32_t> soff(nq + 1, 0);
+  for (int32_t q = 0; q < nq; ++q) {
+    int32_t w = 0;
+    for (int32_t p = q * 64; p < std::min(C.n, q * 64 + 64); ++p) w = std::max(w, cnt[p + 1] - cnt[p]);
+    soff[q + 1] = soff[q] + 64 * w;
+  }
+  std::vector<int32_t> schild(std::max(soff[nq], 1), -1);
+  for (int32_t p = 0; p < C.n; ++p)
+    for (int32_t e = cnt[p]; e < cnt[p + 1]; ++e)
+      schild[soff[p / 64] + (p & 63) + (e - cnt[p]) * 64] = child[e];
   F.parent.upload(parent, s);
-  F.child_ptr.upload(cnt, s);
-  F.child.upload(child, s);
+  F.csoff.upload(soff, s);
+  F.cschild.upload(schild, s);
   UFVM_HIP(hipStreamSynchronize(s));
 }
 
@@ -470,12 +495,14 @@ struct VecSel<double> {
   static const double *anb(FastLevel &L) { return L.anb64.p; }
 };
 
-// Symmetric multicolour Gauss-Seidel sweeps on level 1 (kind 0): colours
-// upward then downward; the downward pass skips the last colour and every
-// sweep after the first skips colour 0, because re-solving the colour just
-// solved repeats identical arithmetic.  from_zero: x holds no data yet.
+// Symmetric multicolour Gauss-Seidel sweeps on the finest level (kind 0):
+// colours upward then downward; the downward pass skips the last colour and
+// every sweep after the first skips colour 0, because re-solving the colour
+// just solved repeats identical arithmetic.  from_zero: x holds no data yet;
+// first_done: the first pass (colour 0 from zero) was already made by the
+// kernel that produced b.
 template <typename V>
-void mcgs_sweeps(ufvm_ctx *ctx, FastLevel &L, int nsweep, bool from_zero) {
+void mcgs_sweeps(ufvm_ctx *ctx, FastLevel &L, int nsweep, bool from_zero, bool first_done) {
   using Sel = VecSel<V>;
   const double w = sizeof(V);
   auto pass = [&](int c, int32_t zero_above) {
@@ -492,15 +519,16 @@ void mcgs_sweeps(ufvm_ctx *ctx, FastLevel &L, int nsweep, bool from_zero) {
                     Sel::x(L), zero_above);
   };
   for (int sw = 0; sw < nsweep; ++sw) {
-    for (int c = (sw == 0 ? 0 : 1); c < L.ncol; ++c)
+    for (int c = (sw == 0 ? (first_done ? 1 : 0) : 1); c < L.ncol; ++c)
       pass(c, (sw == 0 && from_zero) ? L.col_off[c + 1] : L.n);
     for (int c = L.ncol - 2; c >= 0; --c) pass(c, L.n);
   }
 }
 
 // One V-cycle from level l down; returns the level's solution vector.
+// first_done (finest level only): x already holds the first colour pass.
 template <typename V>
-V *vcycle_fast(ufvm_ctx *ctx, System &S, int l) {
+V *vcycle_fast(ufvm_ctx *ctx, System &S, int l, bool first_done = false) {
   using Sel = VecSel<V>;
   PcgState &P = S.pcg;
   FastLevel &L = *P.lev[l];
@@ -526,10 +554,10 @@ V *vcycle_fast(ufvm_ctx *ctx, System &S, int l) {
     k::residual_sell<V>(s, L.n, L.soff.p, L.scol.p, Sel::sval(L), Sel::ac(L), Sel::b(L), x,
                         Sel::r(L));
     if (C.kind == 3 && pre > 0)
-      k::restrict_jacobi<V>(s, C.n, L.child_ptr.p, L.child.p, Sel::r(L), Sel::ac(C), Sel::b(C),
+      k::restrict_jacobi<V>(s, C.n, L.csoff.p, L.cschild.p, Sel::r(L), Sel::ac(C), Sel::b(C),
                             Sel::x(C), omega);
     else
-      k::restrict_fast<V>(s, C.n, L.child_ptr.p, L.child.p, Sel::r(L), Sel::b(C));
+      k::restrict_fast<V>(s, C.n, L.csoff.p, L.cschild.p, Sel::r(L), Sel::b(C));
     V *xc = vcycle_fast<V>(ctx, S, l + 1);
     k::prolong_fast<V>(s, L.n, L.parent.p, xc, x);
     for (int it = 0; it < post; ++it) {
@@ -540,7 +568,7 @@ V *vcycle_fast(ufvm_ctx *ctx, System &S, int l) {
     return x;
   }
   if (L.kind == 0) {
-    if (pre > 0) mcgs_sweeps<V>(ctx, L, pre, true);
+    if (pre > 0) mcgs_sweeps<V>(ctx, L, pre, true, first_done);
     else k::fill<V>(s, L.n, x, V(0));
   } else {
     if (pre > 0)
@@ -555,10 +583,10 @@ V *vcycle_fast(ufvm_ctx *ctx, System &S, int l) {
     k::residual_fast<V>(s, L.n, L.ptr.p, L.col.p, Sel::ac(L), Sel::anb(L), Sel::b(L), x,
                         Sel::r(L));
   if (C.kind == 3 && pre > 0)
-    k::restrict_jacobi<V>(s, C.n, L.child_ptr.p, L.child.p, Sel::r(L), Sel::ac(C), Sel::b(C),
+    k::restrict_jacobi<V>(s, C.n, L.csoff.p, L.cschild.p, Sel::r(L), Sel::ac(C), Sel::b(C),
                           Sel::x(C), omega);
   else
-    k::restrict_fast<V>(s, C.n, L.child_ptr.p, L.child.p, Sel::r(L), Sel::b(C));
+    k::restrict_fast<V>(s, C.n, L.csoff.p, L.cschild.p, Sel::r(L), Sel::b(C));
   V *xc = vcycle_fast<V>(ctx, S, l + 1);
   if (L.kind == 1 && post > 0) {
     V *y = Sel::y(L);
@@ -567,7 +595,7 @@ V *vcycle_fast(ufvm_ctx *ctx, System &S, int l) {
     return y;
   }
   k::prolong_fast<V>(s, L.n, L.parent.p, xc, x);
-  if (post > 0) mcgs_sweeps<V>(ctx, L, post, false);
+  if (post > 0) mcgs_sweeps<V>(ctx, L, post, false, false);
   return x;
 }
 
@@ -667,22 +695,50 @@ void pcg_solve_t(ufvm_ctx *ctx, System &S, int32_t *iters, double *relres) {
   double rr = nb;
   int it = 0;
   if (nb > 0.0) {
-    while (it < P.cfg.max_iter) {
-      const V *z = vcycle_fast<V>(ctx, S, 0);
-      k::fcg_dots<V>(s, n, z, P.w.p, P.r.p, ctx->partial.p, scal, P.singular, it == 0);
-      k::fcg_update_p<V>(s, n, z, scal, P.p.p, it == 0);
+    // The host keeps one iteration queued beyond the one whose residual it
+    // waits for, so the GPU never idles on the convergence test.  The queued
+    // iteration after the converged one finds the device flag set and leaves
+    // x and r unchanged.
+    const double thr = (P.cfg.tol * nb) * (P.cfg.tol * nb);
+    k::fill<double>(s, 1, scal + 11, 0.0);
+    // iterations after the first find their first colour pass made by the
+    // previous iteration's x/r update (pre-smoothing by the colour-blocked
+    // Gauss-Seidel only)
+    const bool fuse0 = L0.kind == 0 && P.cfg.pre_sweep > 0;
+    auto issue = [&](int k) {
+      const V *z = vcycle_fast<V>(ctx, S, 0, fuse0 && k > 0);
+      k::fcg_dots<V>(s, n, z, P.w.p, P.r.p, ctx->partial.p, scal, P.singular, k == 0);
+      k::fcg_update_p<V>(s, n, z, scal, P.p.p, k == 0);
       {
         ProfScope ps(ctx, "spmv0",
-                     ctx->prof_on ? 4.0 * (n + 1) + 12.0 * L0.nnz + 8.0 * 3 * n : 0.0);
+                     ctx->prof_on ? 4.0 * (n / 64 + 1) + 12.0 * L0.nnz + 8.0 * 3 * n : 0.0);
         k::spmv_dot(s, n, L0.soff.p, L0.scol.p, L0.ac64.p, L0.sval64.p, P.p.p, P.w.p,
                     ctx->partial.p, scal);
       }
-      k::fcg_update_xr<V>(s, n, scal, P.p.p, P.w.p, P.x.p, P.r.p, Sel::b(L0), ctx->partial.p);
-      read_scal(ctx, 10, 1);
+      k::fcg_update_xr<V>(s, n, scal, P.p.p, P.w.p, P.x.p, P.r.p, Sel::b(L0), ctx->partial.p,
+                          thr, ctx->dring + 2 * (k & 1), k, fuse0 ? L0.col_off[1] : 0,
+                          Sel::ac(L0), Sel::x(L0));
+      UFVM_HIP(hipEventRecord(ctx->iter_ev[k & 1], s));
+    };
+    // Run one iteration ahead unless the residual history predicts that the
+    // next one converges (then the one after it would be issued for nothing).
+    int issued = 0;
+    bool ahead = true;
+    double prev2 = -1.0;
+    while (it < P.cfg.max_iter) {
+      while (issued < P.cfg.max_iter && issued < it + (ahead ? 2 : 1)) issue(issued++);
+      UFVM_HIP(hipEventSynchronize(ctx->iter_ev[it & 1]));
+      const volatile double *slot = ctx->hring + 2 * (it & 1);
+      if (slot[1] != (double)it) throw Error(UFVM_ERR_STATE, "FPCG residual ring out of step");
+      const double rr2 = slot[0];   // the value the device compared with thr
+      rr = std::sqrt(rr2);
       ++it;
-      rr = std::sqrt(ctx->hscal[10]);
       if (std::isnan(rr)) throw Error(UFVM_ERR_NUMERIC, "FPCG breakdown (NaN residual)");
-      if (rr <= P.cfg.tol * nb) break;
+      if (rr2 <= thr) break;
+      // at most iteration `it` is in flight now: predict its residual from the
+      // last reduction factor
+      if (prev2 > 0.0) ahead = rr2 * (rr2 / prev2) > 4.0 * thr;
+      prev2 = rr2;
     }
   }
   k::permute_scatter(s, n, L0.perm_d.p, P.x.p, R0.dphi.p);
@@ -792,6 +848,10 @@ int ufvm_ctx_create(int32_t device, ufvm_ctx **out) {
     ctx->counter.alloc(4);
     UFVM_HIP(hipMemsetAsync(ctx->counter.p, 0, 4 * sizeof(unsigned), ctx->stream));
     UFVM_HIP(hipHostMalloc((void **)&ctx->hscal, kScal * sizeof(double), hipHostMallocDefault));
+    UFVM_HIP(hipHostMalloc((void **)&ctx->hring, 8 * sizeof(double),
+                           hipHostMallocMapped | hipHostMallocCoherent));
+    UFVM_HIP(hipHostGetDevicePointer((void **)&ctx->dring, ctx->hring, 0));
+    for (hipEvent_t &e : ctx->iter_ev) UFVM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     ctx->partial.alloc(1024);
     UFVM_HIP(hipStreamSynchronize(ctx->stream));
   });
@@ -818,6 +878,9 @@ int ufvm_ctx_destroy(ufvm_ctx *ctx) {
   ctx->scal.release();
   ctx->counter.release();
   if (ctx->hscal) (void)hipHostFree(ctx->hscal);
+  if (ctx->hring) (void)hipHostFree(ctx->hring);
+  for (hipEvent_t e : ctx->iter_ev)
+    if (e) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return UFVM_OK;

@@ -61,15 +61,16 @@ template <typename V>
 void bmcgs(hipStream_t s, int32_t nblk, const int32_t *bptr, const int32_t *boff, int32_t ncol,
            const int32_t *ptr, const int32_t *col, const V *ac, const V *anb, const V *b,
            const V *x_in, const int32_t *parent, const V *xc, V *x_out, int mode, int nsweep);
-// restriction fused with a Jacobi sweep from zero on the coarse level
+// restriction over sliced-ELL child lists (off per 64 coarse rows, padding
+// -1), alone or fused with a Jacobi sweep from zero on the coarse level
 template <typename V>
-void restrict_jacobi(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *child,
+void restrict_jacobi(hipStream_t s, int32_t nc, const int32_t *off, const int32_t *child,
                      const V *r, const V *acc, V *bc, V *xc, V omega);
 template <typename V>
 void residual_fast(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col, const V *ac,
                    const V *anb, const V *b, const V *x, V *r);
 template <typename V>
-void restrict_fast(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *child,
+void restrict_fast(hipStream_t s, int32_t nc, const int32_t *off, const int32_t *child,
                    const V *r, V *bc);
 template <typename V>
 void prolong_fast(hipStream_t s, int32_t n, const int32_t *parent, const V *xc, V *x);
@@ -103,10 +104,14 @@ void fcg_dots(hipStream_t s, int32_t n, const V *z, const double *w, const doubl
 template <typename V>
 void fcg_update_p(hipStream_t s, int32_t n, const V *z, const double *scal, double *p,
                   int first);
-// x += alpha p ; r -= alpha w ; rv = (V) r ; scal: rr = r.r
+// x += alpha p ; r -= alpha w ; rv = (V) r ; scal: rr = r.r, convergence flag
+// scal[11] once rr <= thr (no update at all once it is set); ring: {rr, iter}
+// written to host-visible memory when not null.  Rows i < n0 also get
+// z0[i] = rv[i] / acv[i] (the next V-cycle's first colour pass from zero).
 template <typename V>
 void fcg_update_xr(hipStream_t s, int32_t n, double *scal, const double *p, const double *w,
-                   double *x, double *r, V *rv, double *partial);
+                   double *x, double *r, V *rv, double *partial, double thr, double *ring,
+                   int32_t iter, int32_t n0, const V *acv, V *z0);
 // reduce nblk partial groups of width `width` into out[0..width)
 void reduce_groups(hipStream_t s, const double *partial, int32_t nblk, int32_t width,
                    double *out);

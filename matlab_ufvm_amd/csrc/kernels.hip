@@ -189,13 +189,6 @@ void restrict_ref(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t 
                      r, bc_c);
 }
 
-template <typename V>
-void restrict_fast(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *child,
-                   const V *r, V *bc) {
-  if (nc <= 0) return;
-  hipLaunchKernelGGL(k_restrict<V>, dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, cptr, child, r,
-                     bc);
-}
 
 template <typename V>
 __global__ void __launch_bounds__(kBlock)
@@ -426,26 +419,56 @@ void bmcgs(hipStream_t s, int32_t nblk, const int32_t *bptr, const int32_t *boff
 
 // restriction fused with the first Jacobi sweep from zero on the coarse level:
 // bc[p] = sum of children's residuals; xc[p] = w bc[p] / ac[p]
-template <typename V>
+// Restriction over children stored like the sliced-ELL matrix: coarse rows in
+// chunks of 64, chunk q's child lists column-major at off[q] + k*64 + lane,
+// padded with -1.  bc[p] = sum of the children's residuals in list order
+// (increasing reference fine index); JAC: also the first Jacobi sweep from
+// zero on the coarse level, xc[p] = w bc[p] / ac[p].
+template <typename V, bool JAC>
 __global__ void __launch_bounds__(kBlock)
-k_restrict_jacobi(int32_t nc, const int32_t *__restrict__ cptr, const int32_t *__restrict__ child,
-                  const V *__restrict__ r, const V *__restrict__ acc, V *__restrict__ bc,
-                  V *__restrict__ xc, V omega) {
+k_restrict_sell(int32_t nc, const int32_t *__restrict__ off, const int32_t *__restrict__ child,
+                const V *__restrict__ r, const V *__restrict__ acc, V *__restrict__ bc,
+                V *__restrict__ xc, V omega) {
   const int32_t p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= nc) return;
+  const int32_t q = p >> 6;
+  const int32_t base = off[q] + (p & 63), w = (off[q + 1] - off[q]) >> 6;
   V s = 0;
-  for (int32_t c = cptr[p]; c < cptr[p + 1]; ++c) s = s + r[child[c]];
+  int32_t k = 0;
+  for (; k + 4 <= w; k += 4) {
+    int32_t c[4];
+    V v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) c[u] = child[base + (k + u) * 64];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = c[u] >= 0 ? r[c[u]] : V(0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) s = s + v[u];
+  }
+  for (; k < w; ++k) {
+    const int32_t c = child[base + k * 64];
+    s = s + (c >= 0 ? r[c] : V(0));
+  }
   bc[p] = s;
-  xc[p] = omega * s / acc[p];
+  if (JAC) xc[p] = omega * s / acc[p];
 }
 
 template <typename V>
-void restrict_jacobi(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *child,
+void restrict_jacobi(hipStream_t s, int32_t nc, const int32_t *off, const int32_t *child,
                      const V *r, const V *acc, V *bc, V *xc, V omega) {
   if (nc <= 0) return;
-  hipLaunchKernelGGL(k_restrict_jacobi<V>, dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, cptr, child,
-                     r, acc, bc, xc, omega);
+  hipLaunchKernelGGL((k_restrict_sell<V, true>), dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, off,
+                     child, r, acc, bc, xc, omega);
 }
+
+template <typename V>
+void restrict_fast(hipStream_t s, int32_t nc, const int32_t *off, const int32_t *child,
+                   const V *r, V *bc) {
+  if (nc <= 0) return;
+  hipLaunchKernelGGL((k_restrict_sell<V, false>), dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, off,
+                     child, r, nullptr, bc, nullptr, V(0));
+}
+
 
 template <typename V>
 __global__ void __launch_bounds__(kBlock)
@@ -643,10 +666,12 @@ k_fcg_dots(int32_t n, const V *__restrict__ z, const double *__restrict__ w,
 }
 
 // reduction + scalar step.  mode 0: z-dots -> m, rho, beta; 1: delta -> alpha;
-// 2: rr
+// 2: rr, and the convergence flag scal[11] once rr <= thr; after that flag is
+// set the mode-2 step leaves scal[10] alone.  ring (host-visible, may be null)
+// receives {rr, iter} for a host that polls without synchronising the stream.
 __global__ void __launch_bounds__(kBlock)
 k_fcg_reduce(const double *__restrict__ partial, int32_t nblk, int mode, int32_t n, int singular,
-             int first, double *scal) {
+             int first, double *scal, double thr, double *ring, int32_t iter) {
   __shared__ double lds[kBlock / kWave];
   const int W = mode == 0 ? 5 : 1;
   double t[5];
@@ -666,7 +691,14 @@ k_fcg_reduce(const double *__restrict__ partial, int32_t nblk, int mode, int32_t
     scal[5] = t[0];
     scal[7] = scal[8] / t[0];
   } else {
-    scal[10] = t[0];
+    if (scal[11] == 0.0) {
+      scal[10] = t[0];
+      if (t[0] <= thr) scal[11] = 1.0;
+    }
+    if (ring) {
+      ring[0] = scal[10];
+      ring[1] = (double)iter;
+    }
   }
 }
 
@@ -677,7 +709,8 @@ void spmv_dot(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, 
   const int g = stream_blocks(n);
   hipLaunchKernelGGL(k_spmv_dot, dim3(g), dim3(kBlock), 0, s, n, off, col, ac, val, p, w,
                      partial);
-  hipLaunchKernelGGL(k_fcg_reduce, dim3(1), dim3(kBlock), 0, s, partial, g, 1, n, 0, 0, scal);
+  hipLaunchKernelGGL(k_fcg_reduce, dim3(1), dim3(kBlock), 0, s, partial, g, 1, n, 0, 0, scal,
+                     0.0, nullptr, 0);
 }
 
 template <typename V>
@@ -686,7 +719,7 @@ void fcg_dots(hipStream_t s, int32_t n, const V *z, const double *w, const doubl
   const int g = stream_blocks(n);
   hipLaunchKernelGGL(k_fcg_dots<V>, dim3(g), dim3(kBlock), 0, s, n, z, w, r, partial);
   hipLaunchKernelGGL(k_fcg_reduce, dim3(1), dim3(kBlock), 0, s, partial, g, 0, n, singular, first,
-                     scal);
+                     scal, 0.0, nullptr, 0);
 }
 
 template <typename V>
@@ -707,20 +740,29 @@ void fcg_update_p(hipStream_t s, int32_t n, const V *z, const double *scal, doub
                      first);
 }
 
-// x += alpha p ; r -= alpha w ; rv = (V) r ; partial = r.r
+// x += alpha p ; r -= alpha w ; rv = (V) r ; partial = r.r.  Rows below n0
+// (colour class 0 of the finest level) also get the next V-cycle's first
+// Gauss-Seidel pass from zero, z0 = rv / acv: their neighbours all lie in later
+// colours, which that pass reads as 0.  Nothing changes once the convergence
+// flag scal[11] is set (an iteration issued after the converged one leaves x
+// and r as they were).
 template <typename V>
 __global__ void __launch_bounds__(kBlock)
 k_fcg_update_xr(int32_t n, const double *__restrict__ scal, const double *__restrict__ p,
                 const double *__restrict__ w, double *__restrict__ x, double *__restrict__ r,
-                V *__restrict__ rv, double *__restrict__ partial) {
+                V *__restrict__ rv, double *__restrict__ partial, int32_t n0,
+                const V *__restrict__ acv, V *__restrict__ z0) {
   __shared__ double lds[kBlock / kWave];
+  if (scal[11] != 0.0) return;
   double rr = 0.0;
   const double a = scal[7];
   for (int32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
     x[i] = x[i] + a * p[i];
     const double ri = r[i] - a * w[i];
     r[i] = ri;
-    rv[i] = (V)ri;
+    const V rvi = (V)ri;
+    rv[i] = rvi;
+    if (i < n0) z0[i] = rvi / acv[i];
     rr += ri * ri;
   }
   rr = block_sum(rr, lds);
@@ -729,11 +771,13 @@ k_fcg_update_xr(int32_t n, const double *__restrict__ scal, const double *__rest
 
 template <typename V>
 void fcg_update_xr(hipStream_t s, int32_t n, double *scal, const double *p, const double *w,
-                   double *x, double *r, V *rv, double *partial) {
+                   double *x, double *r, V *rv, double *partial, double thr, double *ring,
+                   int32_t iter, int32_t n0, const V *acv, V *z0) {
   const int g = stream_blocks(n);
   hipLaunchKernelGGL(k_fcg_update_xr<V>, dim3(g), dim3(kBlock), 0, s, n, scal, p, w, x, r, rv,
-                     partial);
-  hipLaunchKernelGGL(k_fcg_reduce, dim3(1), dim3(kBlock), 0, s, partial, g, 2, n, 0, 0, scal);
+                     partial, n0, acv, z0);
+  hipLaunchKernelGGL(k_fcg_reduce, dim3(1), dim3(kBlock), 0, s, partial, g, 2, n, 0, 0, scal, thr,
+                     ring, iter);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -797,7 +841,8 @@ void axpy_const(hipStream_t s, int32_t n, const double *scal_slot, double mul, d
                             double *, double *, int, int);                                      \
   template void fcg_update_p<V>(hipStream_t, int32_t, const V *, const double *, double *, int); \
   template void fcg_update_xr<V>(hipStream_t, int32_t, double *, const double *, const double *, \
-                                 double *, double *, V *, double *);
+                                 double *, double *, V *, double *, double, double *, int32_t,  \
+                                 int32_t, const V *, V *);
 UFVM_INST(float)
 UFVM_INST(double)
 template void convert<double, float>(hipStream_t, int64_t, const double *, float *);

@@ -97,7 +97,8 @@ struct FastLevel {
   DBuf<int32_t> ptr, col, perm_d;
   DBuf<float> ac32, anb32, b32, x32, r32, y32;   // y: second iterate buffer (kind 1)
   DBuf<double> ac64, anb64, b64, x64, r64, y64;
-  DBuf<int32_t> parent, child_ptr, child;   // to / from the next level (new numbering)
+  DBuf<int32_t> parent;           // coarse row of each row (next level's numbering)
+  DBuf<int32_t> csoff, cschild;   // child lists of the next level's rows, sliced-ELL
   // sliced-ELL copy of the off-diagonals (kinds 0 and 3): 64-row chunks, chunk
   // q at [soff[q], soff[q+1]) column-major, padding col = -1
   std::vector<int32_t> h_soff;
@@ -143,6 +144,10 @@ struct ufvm_ctx {
   ufvm::DBuf<double> scal;      // device scalars
   ufvm::DBuf<unsigned> counter; // last-block reduction counter (kept at zero between launches)
   double *hscal = nullptr;      // pinned host mirror of scal
+  // FPCG residual ring: pinned, host-coherent memory the iteration's last
+  // kernel writes {rr, iteration} into; dring is its device address
+  double *hring = nullptr, *dring = nullptr;
+  hipEvent_t iter_ev[2] = {nullptr, nullptr};
   // SIMPLE cases of this context (type-erased; owned here so that destroying
   // the context releases them)
   std::map<int32_t, std::shared_ptr<void>> cases;

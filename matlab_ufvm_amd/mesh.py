@@ -10,8 +10,14 @@ reference's processed geometry, including:
   (``cfdProcessOpenFoamMesh.m:23-52``);
 * the element "centre" (pyramid apex) taken as the mean centroid of GLOBAL
   faces ``1..nFaces(e)`` rather than the element's own faces
-  (``cfdProcessOpenFoamMesh.m:69-75``) -- kept because it changes the cell
-  centroids of non-planar (jittered) cells at the 1e-16..1e-12 level;
+  (``cfdProcessOpenFoamMesh.m:69-75``).  With planar faces the pyramid sums
+  do not depend on the apex, but the faces of a jittered hex are warped, and
+  then the centroid error grows with the apex's distance from the cell: on a
+  128^3 cavity jittered by 15% of a cell, cells far from global face 1 get
+  centroids displaced by several cells and some boundary faces get negative
+  ``gDiff``.  ``apex="reference"`` (the default) keeps this for parity;
+  ``apex="own"`` uses the mean centroid of the element's own faces, the
+  centre the reference's comment describes;
 * interior ``CN, eCN, gDiff, T, gf`` (``:108-132``) and boundary
   ``CN, gDiff = A^2/(CN.Sf), eCN, T, gf = 1, walldist`` (``:134-155``);
 * ``cconn{e}`` = neighbours in increasing interior-face order (``:235-241``,
@@ -168,8 +174,10 @@ def element_lists(n_cells, owner, neighbour, n_faces):
             own_nnz.astype(np.int32), nei_nnz.astype(np.int32))
 
 
-def process_mesh(pm: PolyMesh) -> Mesh:
+def process_mesh(pm: PolyMesh, apex: str = "reference") -> Mesh:
     """cfdProcessOpenFoamMesh restated (see module docstring)."""
+    if apex not in ("reference", "own"):
+        raise ValueError("apex must be 'reference' or 'own'")
     N, F, Fi = pm.n_cells, pm.n_faces, pm.n_interior_faces
     owner = pm.owner.astype(np.int32)
     nei = pm.neighbour.astype(np.int32)
@@ -178,8 +186,15 @@ def process_mesh(pm: PolyMesh) -> Mesh:
 
     # element volume / centroid (cfdProcessOpenFoamMesh.m:62-103)
     nf = np.diff(cf_ptr)
-    prefix = np.cumsum(fc, axis=0)        # sequential sums of global face centroids 1..k
-    centre = prefix[nf - 1] / nf[:, None]
+    if apex == "reference":
+        prefix = np.cumsum(fc, axis=0)    # sequential sums of global face centroids 1..k
+        centre = prefix[nf - 1] / nf[:, None]
+    else:
+        centre = np.zeros((N, 3))
+        for k in range(int(nf.max())):
+            cells = np.nonzero(nf > k)[0]
+            centre[cells] = centre[cells] + fc[cfaces[cf_ptr[cells] + k]]
+        centre = centre / nf[:, None]
     vcs = np.zeros((N, 3))
     vs = np.zeros(N)
     maxnf = int(nf.max())

@@ -21,12 +21,12 @@ from .meshgen import CAVITY_2D, CAVITY_3D, POISSON_3D, box_hex, box_tet
 
 
 def cavity_mesh(n: int, nz: int | None = None, jitter: float = 0.15, seed: int = 2026,
-                renumber_seed: int | None = 2027) -> Mesh:
+                renumber_seed: int | None = 2027, apex: str = "reference") -> Mesh:
     nz = n if nz is None else nz
     spec = CAVITY_2D if nz == 1 else CAVITY_3D
     lengths = (1.0, 1.0, 1.0 / n if nz == 1 else 1.0)
     return process_mesh(box_hex(n, n, nz, spec, lengths=lengths, jitter=jitter, seed=seed,
-                                renumber_seed=renumber_seed))
+                                renumber_seed=renumber_seed), apex=apex)
 
 
 def poisson_tet_mesh(n: int, jitter: float = 0.15, seed: int = 2028,

@@ -102,3 +102,35 @@ def test_uniform_closed_forms():
     # boundary: gDiff = A^2/(CN.Sf) = A/(h/2)
     assert np.allclose(m.gDiff[Fi:], h * h / (h / 2), rtol=1e-14)
     assert np.allclose(m.walldist[Fi:], h / 2, rtol=1e-14)
+
+
+def test_reference_apex_degrades_warped_meshes():
+    # The reference's pyramid apex (mean centroid of global faces 1..nFaces(e),
+    # cfdProcessOpenFoamMesh.m:69-75) makes the centroid error of a warped-face
+    # cell grow with the cell's distance from face 1.  On a 32^3 cavity jittered
+    # by 15% some boundary faces already get CN.Sf < 0 (negative gDiff); the
+    # element's own faces as apex keep every one positive, and the centroid it
+    # gives stays inside the cell's node bounding box.
+    from matlab_ufvm_amd.workloads import cavity_mesh
+    ref = cavity_mesh(32, jitter=0.15, seed=2026, renumber_seed=2027)
+    own = cavity_mesh(32, jitter=0.15, seed=2026, renumber_seed=2027, apex="own")
+    Fi = ref.n_interior
+    assert (ref.gDiff[Fi:] < 0).any()
+    assert (own.gDiff[Fi:] > 0).all() and (own.gDiff[:Fi] > 0).all()
+    # volumes do not depend on the apex (closed fan-triangulated surfaces)
+    assert np.allclose(ref.volume, own.volume, rtol=1e-9)
+    fid = np.repeat(np.arange(own.n_faces), np.diff(own.face_ptr))
+    pts = own.points[own.face_nodes]
+    inner = fid < Fi
+    cells = np.concatenate([own.owner[fid], own.neighbour[fid[inner]]])
+    p = np.concatenate([pts, pts[inner]])
+    lo = np.full((own.n_cells, 3), np.inf)
+    hi = np.full((own.n_cells, 3), -np.inf)
+    np.minimum.at(lo, cells, p)
+    np.maximum.at(hi, cells, p)
+    assert np.all(own.centroid >= lo) and np.all(own.centroid <= hi)
+    assert not (np.all(ref.centroid >= lo) and np.all(ref.centroid <= hi))
+    # a uniform (planar-face) mesh is apex-independent up to rounding
+    u = box_hex(5, 4, 3)
+    assert np.allclose(process_mesh(u).centroid, process_mesh(u, apex="own").centroid,
+                       rtol=0, atol=1e-15)
