@@ -5,7 +5,7 @@
  * its "API" for this path is a set of MATLAB functions operating on
  * `global Domain`.  Each entry point below replaces one of those functions
  * (cited as uFVM/<file>:<line>); a MEX gateway or a ctypes host marshals the
- * Domain pieces into these plain arrays (see INTEGRATION.md).
+ * Domain pieces into these plain arrays.
  *
  * Conventions
  *  - Every function returns 0 (UFVM_OK) or a UFVM_ERR_* status; on error the
@@ -235,8 +235,10 @@ int ufvm_correct_equation(ufvm_ctx *ctx, int32_t cas, const char *eq, int32_t co
 int ufvm_simple_iterate(ufvm_ctx *ctx, int32_t cas, int32_t n, double *rms);
 
 /* ------------------------------------------------------------- profiling */
-/* Per-kernel-class HIP-event timing on the context stream.  kind names:
- * "mcgs0" (level-0 smoother colour passes), "spmv0" (level-0 FP64 SpMV). */
+/* Per-kernel-class HIP-event timing on the context stream (no reference
+ * counterpart).  kind names: "mcgs0" (finest-level colour passes of the
+ * smoother), "spmv0" (finest-level FP64 operator product of the FPCG).
+ * on = 0 disables; on = k >= 1 times every k-th launch of each kind. */
 int ufvm_prof_enable(ufvm_ctx *ctx, int32_t on);
 int ufvm_prof_read(ufvm_ctx *ctx, const char *kind, double *total_ms, int64_t *launches,
                    double *bytes);

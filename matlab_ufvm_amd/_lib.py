@@ -284,7 +284,8 @@ class Context:
 
     # ------------------------------------------------------------ profiling
     def prof_enable(self, on=True):
-        self.check(self.lib.ufvm_prof_enable(self.h, 1 if on else 0))
+        """on: False/0 off, True/1 every launch, k > 1 every k-th launch per kind."""
+        self.check(self.lib.ufvm_prof_enable(self.h, int(on)))
 
     def prof_reset(self):
         self.check(self.lib.ufvm_prof_reset(self.h))

@@ -60,6 +60,10 @@ struct ProfScope {
   ProfRec rec;
   ProfScope(ufvm_ctx *c, const char *kind, double bytes) : ctx(c), on(c->prof_on) {
     if (!on) return;
+    // sampled launches only: fewer event records keep the host's launch queue
+    // ahead of the GPU, so an interval holds the kernel and not a launch wait
+    on = (ctx->prof_seen[kind]++ % ctx->prof_stride) == 0;
+    if (!on) return;
     rec.kind = kind;
     rec.bytes = bytes;
     rec.a = pool_event(ctx);
@@ -525,6 +529,10 @@ void mcgs_sweeps(ufvm_ctx *ctx, FastLevel &L, int nsweep, bool from_zero, bool f
   }
 }
 
+// Jacobi levels with at most this many rows fuse the prolongation into the
+// first post-smoothing sweep.
+constexpr int32_t kFusedProlongRows = 100000;
+
 // One V-cycle from level l down; returns the level's solution vector.
 // first_done (finest level only): x already holds the first colour pass.
 template <typename V>
@@ -546,12 +554,12 @@ V *vcycle_fast(ufvm_ctx *ctx, System &S, int l, bool first_done = false) {
     // that produced b); further sweeps ping-pong between x and y
     V *y = Sel::y(L);
     for (int it = 1; it < pre; ++it) {
-      k::jacobi_sell<V>(s, L.n, L.soff.p, L.scol.p, Sel::sval(L), Sel::ac(L), Sel::b(L), x, y,
-                        omega);
+      k::jacobi_sell<V>(s, L.n, L.soff.p, L.scol.p, Sel::sval(L), Sel::ac(L), Sel::b(L), x,
+                        nullptr, nullptr, y, omega);
       std::swap(x, y);
     }
     if (pre == 0) k::fill<V>(s, L.n, x, V(0));
-    k::residual_sell<V>(s, L.n, L.soff.p, L.scol.p, Sel::sval(L), Sel::ac(L), Sel::b(L), x,
+    k::residual_sell<V>(s, L.n, 0, L.soff.p, L.scol.p, Sel::sval(L), Sel::ac(L), Sel::b(L), x,
                         Sel::r(L));
     if (C.kind == 3 && pre > 0)
       k::restrict_jacobi<V>(s, C.n, L.csoff.p, L.cschild.p, Sel::r(L), Sel::ac(C), Sel::b(C),
@@ -559,10 +567,14 @@ V *vcycle_fast(ufvm_ctx *ctx, System &S, int l, bool first_done = false) {
     else
       k::restrict_fast<V>(s, C.n, L.csoff.p, L.cschild.p, Sel::r(L), Sel::b(C));
     V *xc = vcycle_fast<V>(ctx, S, l + 1);
-    k::prolong_fast<V>(s, L.n, L.parent.p, xc, x);
+    // small levels are latency-bound: the first post-sweep adds the coarse
+    // correction on the fly instead of a separate prolongation launch
+    const bool fuse = post > 0 && L.n <= kFusedProlongRows;
+    if (!fuse) k::prolong_fast<V>(s, 0, L.n, L.parent.p, xc, x);
     for (int it = 0; it < post; ++it) {
-      k::jacobi_sell<V>(s, L.n, L.soff.p, L.scol.p, Sel::sval(L), Sel::ac(L), Sel::b(L), x, y,
-                        omega);
+      const bool pro = fuse && it == 0;
+      k::jacobi_sell<V>(s, L.n, L.soff.p, L.scol.p, Sel::sval(L), Sel::ac(L), Sel::b(L), x,
+                        pro ? L.parent.p : nullptr, pro ? xc : nullptr, y, omega);
       std::swap(x, y);
     }
     return x;
@@ -576,9 +588,11 @@ V *vcycle_fast(ufvm_ctx *ctx, System &S, int l, bool first_done = false) {
                   Sel::anb(L), Sel::b(L), x, nullptr, nullptr, x, 0, pre);
     else k::fill<V>(s, L.n, x, V(0));
   }
+  // after a symmetric Gauss-Seidel sweep the rows of colour 0 (solved last)
+  // have a residual of rounding noise only: it is taken as 0
   if (L.kind == 0)
-    k::residual_sell<V>(s, L.n, L.soff.p, L.scol.p, Sel::sval(L), Sel::ac(L), Sel::b(L), x,
-                        Sel::r(L));
+    k::residual_sell<V>(s, L.n, pre > 0 ? L.col_off[1] : 0, L.soff.p, L.scol.p, Sel::sval(L),
+                        Sel::ac(L), Sel::b(L), x, Sel::r(L));
   else
     k::residual_fast<V>(s, L.n, L.ptr.p, L.col.p, Sel::ac(L), Sel::anb(L), Sel::b(L), x,
                         Sel::r(L));
@@ -594,7 +608,9 @@ V *vcycle_fast(ufvm_ctx *ctx, System &S, int l, bool first_done = false) {
                 Sel::anb(L), Sel::b(L), x, L.parent.p, xc, y, 2, post);
     return y;
   }
-  k::prolong_fast<V>(s, L.n, L.parent.p, xc, x);
+  // the post-sweep's first pass recomputes colour 0 from its neighbours
+  // alone, so only the other colours need the coarse correction
+  k::prolong_fast<V>(s, (L.kind == 0 && post > 0) ? L.col_off[1] : 0, L.n, L.parent.p, xc, x);
   if (post > 0) mcgs_sweeps<V>(ctx, L, post, false, false);
   return x;
 }
@@ -707,13 +723,13 @@ void pcg_solve_t(ufvm_ctx *ctx, System &S, int32_t *iters, double *relres) {
     const bool fuse0 = L0.kind == 0 && P.cfg.pre_sweep > 0;
     auto issue = [&](int k) {
       const V *z = vcycle_fast<V>(ctx, S, 0, fuse0 && k > 0);
-      k::fcg_dots<V>(s, n, z, P.w.p, P.r.p, ctx->partial.p, scal, P.singular, k == 0);
-      k::fcg_update_p<V>(s, n, z, scal, P.p.p, k == 0);
+      k::fcg_dots<V>(s, n, z, P.w.p, P.r.p, ctx->partial.p);
+      k::fcg_update_p<V>(s, n, z, ctx->partial.p, P.singular, k == 0, scal, P.p.p);
       {
         ProfScope ps(ctx, "spmv0",
                      ctx->prof_on ? 4.0 * (n / 64 + 1) + 12.0 * L0.nnz + 8.0 * 3 * n : 0.0);
         k::spmv_dot(s, n, L0.soff.p, L0.scol.p, L0.ac64.p, L0.sval64.p, P.p.p, P.w.p,
-                    ctx->partial.p, scal);
+                    ctx->partial.p);
       }
       k::fcg_update_xr<V>(s, n, scal, P.p.p, P.w.p, P.x.p, P.r.p, Sel::b(L0), ctx->partial.p,
                           thr, ctx->dring + 2 * (k & 1), k, fuse0 ? L0.col_off[1] : 0,
@@ -1096,7 +1112,11 @@ int ufvm_pcg_level_colours(ufvm_ctx *ctx, int32_t sys, int32_t level, int32_t *n
 }
 
 int ufvm_prof_enable(ufvm_ctx *ctx, int32_t on) {
-  return guard(ctx, [&] { ctx->prof_on = on != 0; });
+  return guard(ctx, [&] {
+    if (on < 0) throw Error(UFVM_ERR_ARG, "sampling stride must be >= 0");
+    ctx->prof_on = on != 0;
+    if (on) ctx->prof_stride = on;
+  });
 }
 
 int ufvm_prof_reset(ufvm_ctx *ctx) {
@@ -1107,6 +1127,7 @@ int ufvm_prof_reset(ufvm_ctx *ctx) {
       ctx->event_pool.push_back(r.b);
     }
     ctx->prof.clear();
+    ctx->prof_seen.clear();
   });
 }
 

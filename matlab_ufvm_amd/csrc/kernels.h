@@ -9,7 +9,7 @@ namespace ufvm {
 namespace k {
 
 constexpr int kBlock = 256;
-constexpr int kStreamBlocks = 2048;   // grid cap of the grid-stride FPCG kernels
+constexpr int kFoldBlocks = 512;   // grid cap of the grid-stride FCG kernels
 inline int nblocks(int64_t n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
 // ---- reference-order FP64 kernels (bit-exact with the reference arithmetic)
@@ -49,12 +49,15 @@ void coarse_lhs(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *c
 template <typename V>
 void mcgs_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const int32_t *col,
                const V *val, const V *ac, const V *b, V *x, int32_t zero_above);
+// rows below zero_rows get r = 0 without computing it
 template <typename V>
-void residual_sell(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col,
-                   const V *val, const V *ac, const V *b, const V *x, V *r);
+void residual_sell(hipStream_t s, int32_t n, int32_t zero_rows, const int32_t *off,
+                   const int32_t *col, const V *val, const V *ac, const V *b, const V *x, V *r);
+// parent/xc not null: the sweep starts from x + xc[parent] (fused prolongation)
 template <typename V>
 void jacobi_sell(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, const V *val,
-                 const V *ac, const V *b, const V *x, V *out, V omega);
+                 const V *ac, const V *b, const V *x, const int32_t *parent, const V *xc, V *out,
+                 V omega);
 // block-local multicolour GS over blocks of at most kBRows rows (see kernels.hip)
 constexpr int kBRows = 1024;
 template <typename V>
@@ -73,7 +76,9 @@ template <typename V>
 void restrict_fast(hipStream_t s, int32_t nc, const int32_t *off, const int32_t *child,
                    const V *r, V *bc);
 template <typename V>
-void prolong_fast(hipStream_t s, int32_t n, const int32_t *parent, const V *xc, V *x);
+// x[i] += xc[parent[i]] for rows r0 <= i < n
+void prolong_fast(hipStream_t s, int32_t r0, int32_t n, const int32_t *parent, const V *xc,
+                  V *x);
 template <typename V>
 void dense_solve(hipStream_t s, int32_t n, const double *ainv, const V *b, V *x);
 template <typename V>
@@ -93,21 +98,23 @@ void csr_to_dense(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *c
 // ---- FPCG (FP64, level-1 order of the multicolour permutation).  Each call
 // is a streaming kernel writing per-block partial sums plus a one-block kernel
 // reducing them and updating the device scalars scal[] (layout in kernels.hip).
-// w = A p ; scal: delta = p.w, alpha = rho/delta
+// FCG(1) iteration kernels (scalar slots of scal[] listed in kernels.hip).
+// w = A p (sliced-ELL finest level); partial[0, g) = per-block p.w
 void spmv_dot(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, const double *ac,
-              const double *val, const double *p, double *w, double *partial, double *scal);
-// sums of z, z.w, z.r, w, r -> scal: m (mean of z if singular), rho, beta
+              const double *val, const double *p, double *w, double *partial);
+// partial[c*g + b] = per-block sums of z, z.w, z.r, w, r
 template <typename V>
 void fcg_dots(hipStream_t s, int32_t n, const V *z, const double *w, const double *r,
-              double *partial, double *scal, int singular, int first);
-// p = (z - m) - beta p
+              double *partial);
+// folds fcg_dots' partials (m, rho, beta into scal) ; p = (z - m) - beta p
 template <typename V>
-void fcg_update_p(hipStream_t s, int32_t n, const V *z, const double *scal, double *p,
-                  int first);
-// x += alpha p ; r -= alpha w ; rv = (V) r ; scal: rr = r.r, convergence flag
-// scal[11] once rr <= thr (no update at all once it is set); ring: {rr, iter}
-// written to host-visible memory when not null.  Rows i < n0 also get
-// z0[i] = rv[i] / acv[i] (the next V-cycle's first colour pass from zero).
+void fcg_update_p(hipStream_t s, int32_t n, const V *z, const double *partial, int singular,
+                  int first, double *scal, double *p);
+// folds spmv_dot's partials (delta, alpha into scal) ; x += alpha p ;
+// r -= alpha w ; rv = (V) r ; then rr = r.r into scal[10] and the convergence
+// flag scal[11] once rr <= thr (no update at all once it is set); ring:
+// {rr, iter} written to host-visible memory when not null.  Rows i < n0 also
+// get z0[i] = rv[i] / acv[i] (the next V-cycle's first colour pass from zero).
 template <typename V>
 void fcg_update_xr(hipStream_t s, int32_t n, double *scal, const double *p, const double *w,
                    double *x, double *r, V *rv, double *partial, double thr, double *ring,

@@ -205,9 +205,20 @@ void prolong_ref(hipStream_t s, int32_t n, const int32_t *parent, const double *
 }
 
 template <typename V>
-void prolong_fast(hipStream_t s, int32_t n, const int32_t *parent, const V *xc, V *x) {
-  if (n <= 0) return;
-  hipLaunchKernelGGL(k_prolong<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, parent, xc, x);
+__global__ void __launch_bounds__(kBlock)
+k_prolong_from(int32_t r0, int32_t n, const int32_t *__restrict__ parent,
+               const V *__restrict__ xc, V *__restrict__ x) {
+  const int32_t i = r0 + blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  x[i] = x[i] + xc[parent[i]];
+}
+
+template <typename V>
+void prolong_fast(hipStream_t s, int32_t r0, int32_t n, const int32_t *parent, const V *xc,
+                  V *x) {
+  if (n <= r0) return;
+  hipLaunchKernelGGL(k_prolong_from<V>, dim3(nblocks(n - r0)), dim3(kBlock), 0, s, r0, n, parent,
+                     xc, x);
 }
 
 // One thread per coarse row p.  For its children i (increasing fine index):
@@ -312,47 +323,87 @@ void mcgs_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const 
                      val, ac, b, x, zero_above);
 }
 
+// r = b - A x; rows below zero_rows get r = 0 without computing it (the
+// colour class a Gauss-Seidel sweep solved last, whose residual is rounding
+// noise)
 template <typename V>
 __global__ void __launch_bounds__(kBlock)
-k_residual_sell(int32_t n, const int32_t *__restrict__ off, const int32_t *__restrict__ col,
-                const V *__restrict__ val, const V *__restrict__ ac, const V *__restrict__ b,
-                const V *__restrict__ x, V *__restrict__ r) {
+k_residual_sell(int32_t n, int32_t zero_rows, const int32_t *__restrict__ off,
+                const int32_t *__restrict__ col, const V *__restrict__ val,
+                const V *__restrict__ ac, const V *__restrict__ b, const V *__restrict__ x,
+                V *__restrict__ r) {
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
+  if (i < zero_rows) {
+    r[i] = V(0);
+    return;
+  }
   const int32_t q = i >> 6;
   const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
   r[i] = sell_row<V, true, false>(col, val, x, base, w, b[i] - ac[i] * x[i], 0);
 }
 
 template <typename V>
-void residual_sell(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col,
-                   const V *val, const V *ac, const V *b, const V *x, V *r) {
+void residual_sell(hipStream_t s, int32_t n, int32_t zero_rows, const int32_t *off,
+                   const int32_t *col, const V *val, const V *ac, const V *b, const V *x, V *r) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_residual_sell<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, off, col, val,
-                     ac, b, x, r);
+  hipLaunchKernelGGL(k_residual_sell<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, zero_rows, off,
+                     col, val, ac, b, x, r);
 }
 
-// one damped Jacobi sweep: out = x + omega (b - A x) / ac
-template <typename V>
+// v_j = x_j (+ xc[parent_j] when PRO: prolongation fused into the sweep)
+template <typename V, bool PRO>
+__device__ __forceinline__ V jac_val(const V *__restrict__ x, const int32_t *__restrict__ parent,
+                                     const V *__restrict__ xc, int32_t j) {
+  return PRO ? x[j] + xc[parent[j]] : x[j];
+}
+
+// one damped Jacobi sweep: out = v + omega (b - A v) / ac, v = x or, with
+// PRO, x + xc[parent] (the coarse correction added on the fly)
+template <typename V, bool PRO>
 __global__ void __launch_bounds__(kBlock)
 k_jacobi_sell(int32_t n, const int32_t *__restrict__ off, const int32_t *__restrict__ col,
               const V *__restrict__ val, const V *__restrict__ ac, const V *__restrict__ b,
-              const V *__restrict__ x, V *__restrict__ out, V omega) {
+              const V *__restrict__ x, const int32_t *__restrict__ parent,
+              const V *__restrict__ xc, V *__restrict__ out, V omega) {
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int32_t q = i >> 6;
   const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
-  const V xi = x[i], d = ac[i];
-  const V v = sell_row<V, true, false>(col, val, x, base, w, b[i] - d * xi, 0);
-  out[i] = xi + omega * v / d;
+  const V vi = jac_val<V, PRO>(x, parent, xc, i), d = ac[i];
+  V v = b[i] - d * vi;
+  int32_t k = 0;
+  for (; k + 4 <= w; k += 4) {
+    int32_t j[4];
+    V a[4], xj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      j[u] = col[base + (k + u) * 64];
+      a[u] = val[base + (k + u) * 64];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xj[u] = jac_val<V, PRO>(x, parent, xc, j[u]);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v = v - a[u] * xj[u];
+  }
+  for (; k < w; ++k) {
+    const int32_t j = col[base + k * 64];
+    v = v - val[base + k * 64] * jac_val<V, PRO>(x, parent, xc, j);
+  }
+  out[i] = vi + omega * v / d;
 }
 
 template <typename V>
 void jacobi_sell(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, const V *val,
-                 const V *ac, const V *b, const V *x, V *out, V omega) {
+                 const V *ac, const V *b, const V *x, const int32_t *parent, const V *xc, V *out,
+                 V omega) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_jacobi_sell<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, off, col, val, ac,
-                     b, x, out, omega);
+  if (parent)
+    hipLaunchKernelGGL((k_jacobi_sell<V, true>), dim3(nblocks(n)), dim3(kBlock), 0, s, n, off,
+                       col, val, ac, b, x, parent, xc, out, omega);
+  else
+    hipLaunchKernelGGL((k_jacobi_sell<V, false>), dim3(nblocks(n)), dim3(kBlock), 0, s, n, off,
+                       col, val, ac, b, x, parent, xc, out, omega);
 }
 
 // Block-local multicolour Gauss-Seidel (coarse levels).  Rows are numbered
@@ -624,16 +675,46 @@ void gj_inverse(hipStream_t s, int32_t n, double *a, double *colbuf) {
 // them in block order and does the scalar arithmetic (no grid-wide atomics or
 // device-scope fences in the streaming kernels).
 
-// w = A p (sliced-ELL layout) ; partial = p.w.  Grid-stride over rows with at
-// most kStreamBlocks blocks, so the reduction step reads few partials.
-__global__ void __launch_bounds__(kBlock)
+// ---- flexible CG (Notay FCG(1)) vector kernels.  Scalars live in scal[]:
+//   0..4  sums of z, z.w, z.r, w, r        5  delta = p.Ap     6  beta
+//   7     alpha = rho / delta              8  rho = (z - m).r  9  m (mean of z, singular)
+//   10    rr = r.r                         11 convergence flag
+// The streaming kernels run kFcgBlock-thread blocks, at most kFoldBlocks of
+// them (grid-stride; two blocks fill a CU), and write one partial per block;
+// the next kernel of the iteration re-reduces those few partials in every
+// block (same order in each block, so every block derives identical scalars)
+// instead of a separate reduction launch.
+
+constexpr int kFcgBlock = 1024;
+
+static int fold_blocks(int32_t n) {
+  const int b = nblocks(n, kFcgBlock);
+  return b < kFoldBlocks ? b : kFoldBlocks;
+}
+
+// W sums of g partials each (group c at part + c * g), returned to every
+// thread of every block with the same values
+template <int W>
+__device__ __forceinline__ void fold_sums(const double *__restrict__ part, int32_t g,
+                                          double (&t)[W], double *lds) {
+#pragma unroll
+  for (int c = 0; c < W; ++c) t[c] = 0.0;
+  for (int32_t q = threadIdx.x; q < g; q += kFcgBlock) {
+#pragma unroll
+    for (int c = 0; c < W; ++c) t[c] += part[(int64_t)c * g + q];
+  }
+  block_sums_all<kFcgBlock, W>(t, lds);
+}
+
+// w = A p (sliced-ELL layout) ; partial[b] = p.w over block b's rows
+__global__ void __launch_bounds__(kFcgBlock)
 k_spmv_dot(int32_t n, const int32_t *__restrict__ off, const int32_t *__restrict__ col,
            const double *__restrict__ ac,
            const double *__restrict__ val, const double *__restrict__ p, double *__restrict__ w,
            double *__restrict__ partial) {
-  __shared__ double lds[kBlock / kWave];
+  __shared__ double lds[kFcgBlock / kWave];
   double d = 0.0;
-  for (int32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+  for (int32_t i = blockIdx.x * kFcgBlock + threadIdx.x; i < n; i += gridDim.x * kFcgBlock) {
     const int32_t q = i >> 6;
     const int32_t base = off[q] + (i & 63), wq = (off[q + 1] - off[q]) >> 6;
     const double pi = p[i];
@@ -641,18 +722,24 @@ k_spmv_dot(int32_t n, const int32_t *__restrict__ off, const int32_t *__restrict
     w[i] = v;
     d += pi * v;
   }
-  d = block_sum(d, lds);
+  d = block_sum_n<kFcgBlock>(d, lds);
   if (threadIdx.x == 0) partial[blockIdx.x] = d;
 }
 
-// partials of z, z.w, z.r, w, r
+void spmv_dot(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, const double *ac,
+              const double *val, const double *p, double *w, double *partial) {
+  hipLaunchKernelGGL(k_spmv_dot, dim3(fold_blocks(n)), dim3(kFcgBlock), 0, s, n, off, col, ac,
+                     val, p, w, partial);
+}
+
+// partials of z, z.w, z.r, w, r: partial[c * g + b] for block b of g
 template <typename V>
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kFcgBlock)
 k_fcg_dots(int32_t n, const V *__restrict__ z, const double *__restrict__ w,
            const double *__restrict__ r, double *__restrict__ partial) {
-  __shared__ double lds[kBlock / kWave];
+  __shared__ double lds[5 * kFcgBlock / kWave];
   double v[5] = {0, 0, 0, 0, 0};
-  for (int32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+  for (int32_t i = blockIdx.x * kFcgBlock + threadIdx.x; i < n; i += gridDim.x * kFcgBlock) {
     const double zi = (double)z[i], wi = w[i], ri = r[i];
     v[0] += zi;
     v[1] += zi * wi;
@@ -660,103 +747,78 @@ k_fcg_dots(int32_t n, const V *__restrict__ z, const double *__restrict__ w,
     v[3] += wi;
     v[4] += ri;
   }
-  for (int c = 0; c < 5; ++c) v[c] = block_sum(v[c], lds);
+  block_sums_all<kFcgBlock, 5>(v, lds);
   if (threadIdx.x == 0)
-    for (int c = 0; c < 5; ++c) partial[(int64_t)blockIdx.x * 5 + c] = v[c];
-}
-
-// reduction + scalar step.  mode 0: z-dots -> m, rho, beta; 1: delta -> alpha;
-// 2: rr, and the convergence flag scal[11] once rr <= thr; after that flag is
-// set the mode-2 step leaves scal[10] alone.  ring (host-visible, may be null)
-// receives {rr, iter} for a host that polls without synchronising the stream.
-__global__ void __launch_bounds__(kBlock)
-k_fcg_reduce(const double *__restrict__ partial, int32_t nblk, int mode, int32_t n, int singular,
-             int first, double *scal, double thr, double *ring, int32_t iter) {
-  __shared__ double lds[kBlock / kWave];
-  const int W = mode == 0 ? 5 : 1;
-  double t[5];
-  for (int c = 0; c < W; ++c) {
-    double a = 0.0;
-    for (int32_t q = threadIdx.x; q < nblk; q += kBlock) a += partial[(int64_t)q * W + c];
-    t[c] = block_sum(a, lds);
-  }
-  if (threadIdx.x != 0) return;
-  if (mode == 0) {
-    for (int c = 0; c < 5; ++c) scal[c] = t[c];
-    const double m = singular ? t[0] / (double)n : 0.0;
-    scal[9] = m;
-    scal[8] = t[2] - m * t[4];
-    scal[6] = first ? 0.0 : (t[1] - m * t[3]) / scal[5];
-  } else if (mode == 1) {
-    scal[5] = t[0];
-    scal[7] = scal[8] / t[0];
-  } else {
-    if (scal[11] == 0.0) {
-      scal[10] = t[0];
-      if (t[0] <= thr) scal[11] = 1.0;
-    }
-    if (ring) {
-      ring[0] = scal[10];
-      ring[1] = (double)iter;
-    }
-  }
-}
-
-static int stream_blocks(int32_t n) { return nblocks(n) < kStreamBlocks ? nblocks(n) : kStreamBlocks; }
-
-void spmv_dot(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, const double *ac,
-              const double *val, const double *p, double *w, double *partial, double *scal) {
-  const int g = stream_blocks(n);
-  hipLaunchKernelGGL(k_spmv_dot, dim3(g), dim3(kBlock), 0, s, n, off, col, ac, val, p, w,
-                     partial);
-  hipLaunchKernelGGL(k_fcg_reduce, dim3(1), dim3(kBlock), 0, s, partial, g, 1, n, 0, 0, scal,
-                     0.0, nullptr, 0);
+    for (int c = 0; c < 5; ++c) partial[(int64_t)c * gridDim.x + blockIdx.x] = v[c];
 }
 
 template <typename V>
 void fcg_dots(hipStream_t s, int32_t n, const V *z, const double *w, const double *r,
-              double *partial, double *scal, int singular, int first) {
-  const int g = stream_blocks(n);
-  hipLaunchKernelGGL(k_fcg_dots<V>, dim3(g), dim3(kBlock), 0, s, n, z, w, r, partial);
-  hipLaunchKernelGGL(k_fcg_reduce, dim3(1), dim3(kBlock), 0, s, partial, g, 0, n, singular, first,
-                     scal, 0.0, nullptr, 0);
+              double *partial) {
+  hipLaunchKernelGGL(k_fcg_dots<V>, dim3(fold_blocks(n)), dim3(kFcgBlock), 0, s, n, z, w, r,
+                     partial);
+}
+
+// Folds the z-dots (m = mean of z when singular, rho = (z - m).r,
+// beta = (z - m).w / delta_prev), then p = (z - m) - beta p (p = z - m on the
+// first iteration).  Block 0 stores the scalars.
+template <typename V>
+__global__ void __launch_bounds__(kFcgBlock)
+k_fcg_update_p(int32_t n, const V *__restrict__ z, const double *__restrict__ partial,
+               int32_t g, int singular, int first, double *__restrict__ scal,
+               double *__restrict__ p) {
+  __shared__ double lds[5 * kFcgBlock / kWave];
+  double t[5];
+  fold_sums<5>(partial, g, t, lds);
+  const double m = singular ? t[0] / (double)n : 0.0;
+  const double rho = t[2] - m * t[4];
+  const double beta = first ? 0.0 : (t[1] - m * t[3]) / scal[5];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    for (int c = 0; c < 5; ++c) scal[c] = t[c];
+    scal[9] = m;
+    scal[8] = rho;
+    scal[6] = beta;
+  }
+  for (int32_t i = blockIdx.x * kFcgBlock + threadIdx.x; i < n; i += gridDim.x * kFcgBlock) {
+    const double zi = (double)z[i] - m;
+    p[i] = first ? zi : zi - beta * p[i];
+  }
 }
 
 template <typename V>
-__global__ void __launch_bounds__(kBlock)
-k_fcg_update_p(int32_t n, const V *__restrict__ z, const double *__restrict__ scal,
-               double *__restrict__ p, int first) {
-  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
-  const double m = scal[9];
-  const double zi = (double)z[i] - m;
-  p[i] = first ? zi : zi - scal[6] * p[i];
+void fcg_update_p(hipStream_t s, int32_t n, const V *z, const double *partial, int singular,
+                  int first, double *scal, double *p) {
+  const int g = fold_blocks(n);
+  hipLaunchKernelGGL(k_fcg_update_p<V>, dim3(g), dim3(kFcgBlock), 0, s, n, z, partial, g,
+                     singular, first, scal, p);
 }
 
+// Folds delta = p.Ap (partials of k_spmv_dot) and alpha = rho / delta, then
+// x += alpha p ; r -= alpha w ; rv = (V) r ; partial_out[b] = r.r.  Rows below
+// n0 (colour class 0 of the finest level) also get the next V-cycle's first
+// Gauss-Seidel pass from zero, z0 = rv / acv: their neighbours all lie in
+// later colours, which that pass reads as 0.  Nothing changes once the
+// convergence flag scal[11] is set (an iteration issued after the converged
+// one leaves x and r as they were).  partial_out must not overlap partial_in.
 template <typename V>
-void fcg_update_p(hipStream_t s, int32_t n, const V *z, const double *scal, double *p,
-                  int first) {
-  hipLaunchKernelGGL(k_fcg_update_p<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, z, scal, p,
-                     first);
-}
-
-// x += alpha p ; r -= alpha w ; rv = (V) r ; partial = r.r.  Rows below n0
-// (colour class 0 of the finest level) also get the next V-cycle's first
-// Gauss-Seidel pass from zero, z0 = rv / acv: their neighbours all lie in later
-// colours, which that pass reads as 0.  Nothing changes once the convergence
-// flag scal[11] is set (an iteration issued after the converged one leaves x
-// and r as they were).
-template <typename V>
-__global__ void __launch_bounds__(kBlock)
-k_fcg_update_xr(int32_t n, const double *__restrict__ scal, const double *__restrict__ p,
-                const double *__restrict__ w, double *__restrict__ x, double *__restrict__ r,
-                V *__restrict__ rv, double *__restrict__ partial, int32_t n0,
-                const V *__restrict__ acv, V *__restrict__ z0) {
-  __shared__ double lds[kBlock / kWave];
+__global__ void __launch_bounds__(kFcgBlock)
+k_fcg_update_xr(int32_t n, const double *__restrict__ partial_in, double *__restrict__ scal,
+                const double *__restrict__ p, const double *__restrict__ w,
+                double *__restrict__ x, double *__restrict__ r, V *__restrict__ rv,
+                double *__restrict__ partial_out, int32_t n0, const V *__restrict__ acv,
+                V *__restrict__ z0) {
+  __shared__ double lds[kFcgBlock / kWave];
   if (scal[11] != 0.0) return;
+  double t[1];
+  fold_sums<1>(partial_in, gridDim.x, t, lds);
+  const double delta = t[0];
+  const double a = scal[8] / delta;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    scal[5] = delta;
+    scal[7] = a;
+  }
   double rr = 0.0;
-  const double a = scal[7];
-  for (int32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+  for (int32_t i = blockIdx.x * kFcgBlock + threadIdx.x; i < n; i += gridDim.x * kFcgBlock) {
     x[i] = x[i] + a * p[i];
     const double ri = r[i] - a * w[i];
     r[i] = ri;
@@ -765,19 +827,41 @@ k_fcg_update_xr(int32_t n, const double *__restrict__ scal, const double *__rest
     if (i < n0) z0[i] = rvi / acv[i];
     rr += ri * ri;
   }
-  rr = block_sum(rr, lds);
-  if (threadIdx.x == 0) partial[blockIdx.x] = rr;
+  rr = block_sum_n<kFcgBlock>(rr, lds);
+  if (threadIdx.x == 0) partial_out[blockIdx.x] = rr;
+}
+
+// rr = r.r from the update's partials; sets the convergence flag scal[11]
+// once rr <= thr, after which scal[10] is left alone; ring (host-visible, may
+// be null) receives {rr, iter} for a host that polls without synchronising
+// the stream
+__global__ void __launch_bounds__(kFcgBlock)
+k_fcg_rr(const double *__restrict__ partial, int32_t g, double *scal, double thr, double *ring,
+         int32_t iter) {
+  __shared__ double lds[kFcgBlock / kWave];
+  double t[1];
+  fold_sums<1>(partial, g, t, lds);
+  if (threadIdx.x != 0) return;
+  if (scal[11] == 0.0) {
+    scal[10] = t[0];
+    if (t[0] <= thr) scal[11] = 1.0;
+  }
+  if (ring) {
+    ring[0] = scal[10];
+    ring[1] = (double)iter;
+  }
 }
 
 template <typename V>
 void fcg_update_xr(hipStream_t s, int32_t n, double *scal, const double *p, const double *w,
                    double *x, double *r, V *rv, double *partial, double thr, double *ring,
                    int32_t iter, int32_t n0, const V *acv, V *z0) {
-  const int g = stream_blocks(n);
-  hipLaunchKernelGGL(k_fcg_update_xr<V>, dim3(g), dim3(kBlock), 0, s, n, scal, p, w, x, r, rv,
-                     partial, n0, acv, z0);
-  hipLaunchKernelGGL(k_fcg_reduce, dim3(1), dim3(kBlock), 0, s, partial, g, 2, n, 0, 0, scal, thr,
-                     ring, iter);
+  // the spmv partials occupy partial[0, g); the r.r partials go after them
+  const int g = fold_blocks(n);
+  hipLaunchKernelGGL(k_fcg_update_xr<V>, dim3(g), dim3(kFcgBlock), 0, s, n, partial, scal, p, w,
+                     x, r, rv, partial + g, n0, acv, z0);
+  hipLaunchKernelGGL(k_fcg_rr, dim3(1), dim3(kFcgBlock), 0, s, partial + g, g, scal, thr, ring,
+                     iter);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -824,22 +908,25 @@ void axpy_const(hipStream_t s, int32_t n, const double *scal_slot, double mul, d
                          const V *, const int32_t *, const V *, V *, int, int);                 \
   template void mcgs_sell<V>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,    \
                              const V *, const V *, const V *, V *, int32_t);                    \
-  template void residual_sell<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,         \
-                                 const V *, const V *, const V *, const V *, V *);              \
+  template void residual_sell<V>(hipStream_t, int32_t, int32_t, const int32_t *,                \
+                                 const int32_t *, const V *, const V *, const V *, const V *,   \
+                                 V *);                                                          \
   template void jacobi_sell<V>(hipStream_t, int32_t, const int32_t *, const int32_t *, const V *, \
-                               const V *, const V *, const V *, V *, V);                        \
+                               const V *, const V *, const V *, const int32_t *, const V *,     \
+                               V *, V);                                                         \
   template void restrict_jacobi<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,      \
                                    const V *, const V *, V *, V *, V);                         \
   template void residual_fast<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,        \
                                  const V *, const V *, const V *, const V *, V *);              \
   template void restrict_fast<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,        \
                                  const V *, V *);                                               \
-  template void prolong_fast<V>(hipStream_t, int32_t, const int32_t *, const V *, V *);         \
+  template void prolong_fast<V>(hipStream_t, int32_t, int32_t, const int32_t *, const V *, V *); \
   template void dense_solve<V>(hipStream_t, int32_t, const double *, const V *, V *);           \
   template void fill<V>(hipStream_t, int64_t, V *, V);                                          \
   template void fcg_dots<V>(hipStream_t, int32_t, const V *, const double *, const double *,    \
-                            double *, double *, int, int);                                      \
-  template void fcg_update_p<V>(hipStream_t, int32_t, const V *, const double *, double *, int); \
+                            double *);                                                          \
+  template void fcg_update_p<V>(hipStream_t, int32_t, const V *, const double *, int, int,       \
+                                double *, double *);                                            \
   template void fcg_update_xr<V>(hipStream_t, int32_t, double *, const double *, const double *, \
                                  double *, double *, V *, double *, double, double *, int32_t,  \
                                  int32_t, const V *, V *);

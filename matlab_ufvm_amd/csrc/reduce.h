@@ -1,4 +1,4 @@
-// reduce.h -- deterministic wavefront / block / grid reductions (device code).
+// reduce.h -- deterministic wavefront / block reductions (device code).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -31,6 +31,42 @@ __device__ inline T block_sum(T v, T *lds) {
   return r;
 }
 
+// W sums over an NB-thread block, returned to every thread (each thread adds
+// the wave sums in the same order, so all get the same values).  lds holds
+// W * NB / kWave elements.
+template <int NB, int W, typename T>
+__device__ inline void block_sums_all(T (&v)[W], T *lds) {
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int c = 0; c < W; ++c) {
+    v[c] = wave_sum(v[c]);
+    if (lane == 0) lds[c * (NB / kWave) + wid] = v[c];
+  }
+  __syncthreads();
+  // not unrolled across c: W x NB/kWave LDS values held at once would spill
+#pragma unroll 1
+  for (int c = 0; c < W; ++c) {
+    T r = 0;
+    for (int w = 0; w < NB / kWave; ++w) r += lds[c * (NB / kWave) + w];
+    v[c] = r;
+  }
+  __syncthreads();
+}
+
+// sum over an NB-thread block; result valid in thread 0
+template <int NB, typename T>
+__device__ inline T block_sum_n(T v, T *lds) {
+  v = wave_sum(v);
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  T r = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < NB / kWave; ++w) r += lds[w];
+  __syncthreads();
+  return r;
+}
+
 template <typename T>
 __device__ inline T block_max(T v, T *lds) {
 #pragma unroll
@@ -44,57 +80,6 @@ __device__ inline T block_max(T v, T *lds) {
   }
   __syncthreads();
   return r;
-}
-
-// Deterministic grid reduction of W values per block: every block writes
-// its partial values, the last block to finish (counted with an atomic)
-// combines all partials in block order (op[c] = 0: sum, 1: max, 2: min) and
-// runs `fin` on the totals.  The counter is reset by that block, ready for
-// the next launch.
-template <int W, typename Fin>
-__device__ inline void grid_reduce_last(double (&v)[W], const int (&op)[W], double *partial,
-                                        unsigned *counter, Fin fin) {
-  __shared__ double lds[kBlock / kWave];
-  __shared__ bool last;
-  for (int c = 0; c < W; ++c) {
-    if (op[c] == 0) v[c] = block_sum(v[c], lds);
-    else if (op[c] == 1) v[c] = block_max(v[c], lds);
-    else v[c] = -block_max(-v[c], lds);
-  }
-  if (threadIdx.x == 0) {
-    for (int c = 0; c < W; ++c) partial[(int64_t)blockIdx.x * W + c] = v[c];
-    __threadfence();
-    const unsigned t = atomicAdd(counter, 1u);
-    last = (t == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (!last) return;
-  __threadfence();
-  double tot[W];
-  for (int c = 0; c < W; ++c) {
-    double a = op[c] == 0 ? 0.0 : __builtin_nontemporal_load(&partial[c]);
-    for (int32_t q = threadIdx.x; q < (int32_t)gridDim.x; q += kBlock) {
-      const double x = __builtin_nontemporal_load(&partial[(int64_t)q * W + c]);
-      if (op[c] == 0) a += x;
-      else if (op[c] == 1) a = fmax(a, x);
-      else a = fmin(a, x);
-    }
-    if (op[c] == 0) tot[c] = block_sum(a, lds);
-    else if (op[c] == 1) tot[c] = block_max(a, lds);
-    else tot[c] = -block_max(-a, lds);
-  }
-  if (threadIdx.x == 0) {
-    fin(tot);
-    *counter = 0u;
-  }
-}
-
-template <int W, typename Fin>
-__device__ inline void grid_reduce_last(double (&v)[W], double *partial, unsigned *counter,
-                                        Fin fin) {
-  int op[W];
-  for (int c = 0; c < W; ++c) op[c] = 0;
-  grid_reduce_last<W>(v, op, partial, counter, fin);
 }
 
 }  // namespace k

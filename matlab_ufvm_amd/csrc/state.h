@@ -152,6 +152,8 @@ struct ufvm_ctx {
   // the context releases them)
   std::map<int32_t, std::shared_ptr<void>> cases;
   bool prof_on = false;
+  int32_t prof_stride = 1;                  // time every prof_stride-th launch of a kind
+  std::map<std::string, int64_t> prof_seen; // launches seen per kind while enabled
   std::vector<ufvm::ProfRec> prof;
   std::vector<hipEvent_t> event_pool;
 };

@@ -672,6 +672,12 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
   }
   double *r = (double *)malloc((size_t)n * sizeof(double));
   for (int32_t i = 0; i < n; ++i) {
+    /* level 0 after a symmetric sweep: colour 0 was solved last and its
+     * residual (rounding noise) is taken as 0 */
+    if (l == 0 && cfg->pre > 0 && colours[0][i] == 0) {
+      r[i] = 0.0;
+      continue;
+    }
     double s = b[i] - L->ac[i] * x[i];
     for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) s = s - L->anb[k] * x[L->col[k]];
     r[i] = s;

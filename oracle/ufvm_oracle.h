@@ -76,7 +76,9 @@ void orc_mcgs(const orc_level *L, const int32_t *colour, int ncolours, const dou
               double *x);
 
 /* preconditioner V-cycle: zero initial guess on every level, nu_pre/nu_post
- * symmetric sweeps, restriction by parents, level nlev-1 solved exactly
+ * symmetric sweeps, restriction by parents (on level 0 the residual of the
+ * rows of colour 0, solved last by the pre-sweep, is taken as 0), level
+ * nlev-1 solved exactly
  * (dense LU, + mean(diag)/n 11^T when `singular`).  Level 0 uses global
  * multicolour GS; levels 1..nlev-2 use block-local multicolour GS over blocks
  * of block_rows consecutive rows (Jacobi coupling between blocks), with the
