@@ -44,9 +44,19 @@ def build_problem(n, workload, ctx):
     reference arithmetic), then the fifth iteration's momentum step and its
     pressure-correction assembly (cfdAssembleMdotTerm.m:70-250); its matrix and
     mass-imbalance right-hand side are the workload.  laplacian: the gDiff
-    Laplacian of the same mesh with a Gaussian right-hand side."""
-    from matlab_ufvm_amd.workloads import cavity_mesh, gaussian_rhs, poisson_system
+    Laplacian of the same mesh with a Gaussian right-hand side.  tet: the
+    tetrahedral Poisson problem (6 tets per hex of a jittered n^3 grid,
+    Dirichlet face, Gaussian right-hand side)."""
+    from matlab_ufvm_amd.workloads import (cavity_mesh, gaussian_rhs, poisson_system,
+                                           poisson_tet_mesh)
     t = time.time()
+    if workload == "tet":
+        # unit cube, 6 tets per hex of a jittered n^3 grid, unit-diffusivity
+        # Laplacian, p = 0 on the x = 0 face, zero-mean Gaussian right-hand side
+        m = poisson_tet_mesh(n)
+        log(f"tet mesh {n}^3 x 6: cells={m.n_cells} faces={m.n_faces} ({time.time()-t:.1f}s)")
+        ptr, col, ac, anb = poisson_system(m, dirichlet=("dirichlet",))
+        return m, ptr, col, ac, anb, gaussian_rhs(m, seed=2029)
     # own-face pyramid apex: with the reference's apex (global faces 1..6) the
     # warped faces of this jittered mesh displace far cells' centroids by
     # several cells and the SIMPLE momentum solve diverges
@@ -120,7 +130,7 @@ def event_timed_solves(ctx, sid, steps, stride=5):
     return ctx.prof_read("mcgs0"), ctx.prof_read("spmv0")
 
 
-def cpu_baseline(ptr, col, ac, anb, b, max_coarse, coarse_smoother):
+def cpu_baseline(ptr, col, ac, anb, b, max_coarse, coarse_smoother, tol, singular, smoother):
     """The oracle (sequential C, FP64 V-cycle FPCG, same algorithm) on one core:
     one full solve of the same 2.1M-cell system; hierarchy set-up not timed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -130,13 +140,25 @@ def cpu_baseline(ptr, col, ac, anb, b, max_coarse, coarse_smoother):
     S.agglomerate(10)
     t_setup = time.time() - t
     t = time.time()
-    x, it, rr = S.fpcg(b, tol=1e-8, max_coarse_cells=max_coarse, singular=1,
-                       coarse_smoother=coarse_smoother)
+    x, it, rr = S.fpcg(b, tol=tol, max_coarse_cells=max_coarse, singular=singular,
+                       coarse_smoother=coarse_smoother, smoother=smoother)
     dt = time.time() - t
     log(f"cpu oracle: setup {t_setup:.1f}s, solve {dt:.1f}s, {it} iterations, relres {rr:.1e}")
     return {"value": 1.0 / dt, "unit": "solves/s", "cores": 1, "kind": "port",
             "sample": f"1 full FP64 FPCG solve ({it} iterations) of the same {len(b)}-cell system,"
                       f" single thread, hierarchy set-up ({t_setup:.1f}s) excluded"}, x
+
+
+def workload_name(args):
+    if args.workload == "tet":
+        return (f"pressure-Poisson solve, unit cube {args.n}^3 jittered hex split into 6 tets "
+                f"(seed 2028), permuted+RCM (seed 2029), unit diffusivity, p = 0 on x = 0, "
+                f"zero-mean Gaussian RHS (seed 2029)")
+    return (f"pressure-correction solve, 3-D lid-driven cavity {args.n}^3 jittered hex "
+            f"(+-15%, seed 2026), permuted+RCM (seed 2027), own-face pyramid apex, "
+            + ("matrix and RHS of SIMPLE outer iteration 5 (Re 400)"
+               if args.workload == "pcorr-k5" else
+               "all-wall Neumann gDiff Laplacian, Gaussian RHS (seed 2030)"))
 
 
 def main():
@@ -145,7 +167,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=128)
-    ap.add_argument("--workload", choices=["pcorr-k5", "laplacian"], default="pcorr-k5")
+    ap.add_argument("--workload", choices=["pcorr-k5", "laplacian", "tet"], default="pcorr-k5")
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--smoother", choices=["MCGS", "ILU"], default="MCGS",
+                    help="finest-level smoother of the V-cycle")
     ap.add_argument("--pre", type=int, default=1)
     ap.add_argument("--post", type=int, default=1)
     ap.add_argument("--max-coarse", type=int, default=2000)
@@ -183,8 +208,8 @@ def main():
     for prec in ([32] if args.no_fp64 else [32, 64]):
         t = time.time()
         ctx.pcg_setup(sid, precision=prec, pre_sweep=args.pre, post_sweep=args.post,
-                      max_coarse_cells=args.max_coarse, tol=1e-8, max_iter=500,
-                      coarse_smoother=args.coarse_smoother)
+                      max_coarse_cells=args.max_coarse, tol=args.tol, max_iter=500,
+                      coarse_smoother=args.coarse_smoother, smoother=args.smoother)
         t_setup = time.time() - t
         total, lat, its, rr = timed_solves(ctx, sid, args.steps, args.warmup)
         x = ctx.get_dphi(sid, 0)
@@ -199,15 +224,19 @@ def main():
     # field error of each solve against a tight FP64 solve (shifted to p'(0) = 0 as
     # the pressure correction is, cfdCorrectPPField.m:44-50)
     ctx.pcg_setup(sid, precision=64, pre_sweep=args.pre, post_sweep=args.post,
-                  max_coarse_cells=args.max_coarse, tol=1e-12, max_iter=1000,
-                  coarse_smoother=args.coarse_smoother)
+                  max_coarse_cells=args.max_coarse, tol=min(1e-12, args.tol * 1e-2),
+                  max_iter=1000, coarse_smoother=args.coarse_smoother)
     ctx.pcg_solve(sid)
     x_tight = ctx.get_dphi(sid, 0)
-    d_ref = x_tight - x_tight[0]
+    singular = args.workload != "tet"
+
+    def shift(x):
+        return x - x[0] if singular else x
+
+    d_ref = shift(x_tight)
 
     def ferr(x):
-        d = x - x[0]
-        return float(np.linalg.norm(d - d_ref) / np.linalg.norm(d_ref))
+        return float(np.linalg.norm(shift(x) - d_ref) / np.linalg.norm(d_ref))
 
     # plain FP64 drop-in: the reference's stationary AMG (V-cycle, agglomeration
     # pairs, 2 lexicographic Gauss-Seidel sweeps pre and post, cfdApplyAMG.m) on
@@ -249,14 +278,10 @@ def main():
         "ms_per_step": round(r32["total"] / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 V-cycle, f64 FPCG", "data": "synthetic",
-        "config": {"workload": (f"pressure-correction solve, 3-D lid-driven cavity {args.n}^3 "
-                                f"jittered hex (+-15%, seed 2026), permuted+RCM (seed 2027), "
-                                f"own-face pyramid apex, "
-                                + ("matrix and RHS of SIMPLE outer iteration 5 (Re 400)"
-                                   if args.workload == "pcorr-k5" else
-                                   "all-wall Neumann gDiff Laplacian, Gaussian RHS (seed 2030)")),
+        "config": {"workload": workload_name(args),
                    "cells": int(m.n_cells), "faces": int(m.n_faces), "nnz_offdiag": int(col.size),
-                   "tol": 1e-8, "amg_levels": sizes, "pre_sweep": args.pre,
+                   "tol": args.tol, "amg_levels": sizes, "pre_sweep": args.pre,
+                   "smoother": args.smoother,
                    "post_sweep": args.post, "max_coarse_cells": args.max_coarse,
                    "coarse_smoother": ["block-MCGS", "jacobi"][args.coarse_smoother],
                    "hierarchy_cached": True},
@@ -291,13 +316,14 @@ def main():
         out["speedup_fp32_vs_fp64_pcg"] = round(value / v64, 3)
         # the target's measure: FP32-preconditioned field vs the FP64 run at the
         # same residual tolerance (shift-invariant, p'(0) = 0 as above)
-        d32, d64 = r32["x"] - r32["x"][0], r64["x"] - r64["x"][0]
+        d32, d64 = shift(r32["x"]), shift(r64["x"])
         out["field_rel_l2_err_vs_fp64_same_tol"] = float(np.linalg.norm(d32 - d64) /
                                                          np.linalg.norm(d64))
     if plain is not None:
         out["plain_fp64_amg"] = plain
     if not args.no_cpu:
-        cb, x_cpu = cpu_baseline(ptr, col, ac, anb, b, args.max_coarse, args.coarse_smoother)
+        cb, x_cpu = cpu_baseline(ptr, col, ac, anb, b, args.max_coarse, args.coarse_smoother,
+                                 args.tol, int(singular), args.smoother)
         cb["field_rel_l2_err_vs_gpu_tight"] = ferr(x_cpu)
         out["cpu_baseline"] = cb
     else:

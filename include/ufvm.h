@@ -134,7 +134,8 @@ int ufvm_apply_amg(ufvm_ctx *ctx, int32_t sys, const ufvm_amg_cfg *cfg, double *
 /* ------------------------------------------------ precision-form AMG-PCG */
 /* No reference counterpart: flexible PCG (FP64) preconditioned by one
  * AMG V-cycle over the reference's hierarchy, with multicolour Gauss-Seidel
- * smoothing and an FP64 dense solve on the first level with at most
+ * (or DILU) smoothing on the finest level, damped Jacobi or block multicolour
+ * GS below it, and an FP64 dense solve on the first level with at most
  * max_coarse_cells cells. */
 typedef struct {
   int32_t precision;         /* 32: FP32 V-cycle, 64: FP64 V-cycle */
@@ -147,6 +148,10 @@ typedef struct {
   int32_t nullspace;         /* -1 auto-detect (zero row sums), 0 no, 1 constant nullspace */
   int32_t max_iter;
   double tol;                /* stop when ||r||_2 <= tol * ||b||_2 */
+  int32_t smoother;          /* finest level: 0 multicolour Gauss-Seidel (colour classes in
+                                parallel), 1 the reference's DILU ('ILU' smoother,
+                                cfdFactorizeILU.m / cfdSolveILU.m) in its lexicographic
+                                order, level-scheduled */
 } ufvm_pcg_cfg;
 
 /* Build the solve-side data of the current hierarchy (colourings, permuted

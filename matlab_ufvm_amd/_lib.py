@@ -17,6 +17,7 @@ UFVM_OK = 0
 ERRORS = {1: "UFVM_ERR_ARG", 2: "UFVM_ERR_HIP", 3: "UFVM_ERR_STATE", 4: "UFVM_ERR_NUMERIC",
           5: "UFVM_ERR_UNSUPPORTED"}
 SMOOTHER = {"ILU": 0, "SOR": 1}
+PCG_SMOOTHER = {"MCGS": 0, "ILU": 1}
 CYCLE = {"V-Cycle": 0, "W-Cycle": 1, "F-Cycle": 2}
 
 i32p = C.POINTER(C.c_int32)
@@ -33,7 +34,7 @@ class PcgCfg(C.Structure):
     _fields_ = [("precision", C.c_int32), ("pre_sweep", C.c_int32), ("post_sweep", C.c_int32),
                 ("coarse_smoother", C.c_int32),
                 ("max_coarse_cells", C.c_int32), ("nullspace", C.c_int32),
-                ("max_iter", C.c_int32), ("tol", C.c_double)]
+                ("max_iter", C.c_int32), ("tol", C.c_double), ("smoother", C.c_int32)]
 
 
 class MeshDesc(C.Structure):
@@ -258,9 +259,11 @@ class Context:
         return r0.value, r1.value, nc.value
 
     def pcg_setup(self, sid, precision=32, pre_sweep=1, post_sweep=1, max_coarse_cells=2000,
-                  nullspace=-1, max_iter=500, tol=1e-8, coarse_smoother=1):
+                  nullspace=-1, max_iter=500, tol=1e-8, coarse_smoother=1, smoother="MCGS"):
+        """smoother (finest level): "MCGS" multicolour Gauss-Seidel, "ILU" the
+        reference's DILU in its lexicographic order (level-scheduled)."""
         cfg = PcgCfg(precision, pre_sweep, post_sweep, coarse_smoother, max_coarse_cells,
-                     nullspace, max_iter, tol)
+                     nullspace, max_iter, tol, PCG_SMOOTHER[smoother])
         self.check(self.lib.ufvm_pcg_setup(self.h, sid, C.byref(cfg)))
 
     def pcg_solve(self, sid):

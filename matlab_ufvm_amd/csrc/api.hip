@@ -371,7 +371,7 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
       std::stable_sort(FL.perm.begin() + w0, FL.perm.begin() + std::min(n, w0 + kSigma),
                        [&](int32_t a, int32_t b) { return len(a) > len(b); });
   } else {
-    // kind 2 (dense coarsest): reference order
+    // kind 2 (dense coarsest) and kind 4 (DILU finest level): reference order
     FL.ncol = 1;
     FL.colour.assign(n, 0);
     for (int32_t i = 0; i < n; ++i) FL.perm[i] = i;
@@ -401,6 +401,20 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
   FL.anb32.upload(a32, s);
   FL.b32.alloc(n); FL.x32.alloc(n); FL.r32.alloc(n); FL.y32.alloc(n);
   FL.b64.alloc(n); FL.x64.alloc(n); FL.r64.alloc(n); FL.y64.alloc(n);
+  if (kind == 4) {
+    // the reference's DILU factor (cfdFactorizeILU.m:10-47, row N-1 quirk
+    // kept) in FP64 over the lexicographic schedule, stored in both precisions
+    ensure_schedule(ctx, R);
+    if (!R.low_ok) throw Error(UFVM_ERR_NUMERIC, "ILU: the index for i in j is not found");
+    const auto &F = R.fwd.level_ptr;
+    for (size_t q = 0; q + 1 < F.size(); ++q)
+      k::dilu_factor_rows(s, R.fwd_rows.p + F[q], F[q + 1] - F[q], R.ac.p, R.anb.p, R.lptr.p,
+                          R.lnb.p, R.lpji.p, R.lpij.p, R.n - 2, R.dinv.p);
+    FL.dinv64.alloc(n);
+    FL.dinv32.alloc(n);
+    k::convert<double, double>(s, n, R.dinv.p, FL.dinv64.p);
+    k::convert<double, float>(s, n, R.dinv.p, FL.dinv32.p);
+  }
   // iterates start finite: the sliced-ELL padding multiplies x[row] by 0
   for (DBuf<float> *v : {&FL.b32, &FL.x32, &FL.r32, &FL.y32})
     UFVM_HIP(hipMemsetAsync(v->p, 0, n * sizeof(float), s));
@@ -481,6 +495,7 @@ struct VecSel;
 template <>
 struct VecSel<float> {
   static const float *sval(FastLevel &L) { return L.sval32.p; }
+  static const float *dinv(FastLevel &L) { return L.dinv32.p; }
   static float *b(FastLevel &L) { return L.b32.p; }
   static float *x(FastLevel &L) { return L.x32.p; }
   static float *y(FastLevel &L) { return L.y32.p; }
@@ -491,6 +506,7 @@ struct VecSel<float> {
 template <>
 struct VecSel<double> {
   static const double *sval(FastLevel &L) { return L.sval64.p; }
+  static const double *dinv(FastLevel &L) { return L.dinv64.p; }
   static double *b(FastLevel &L) { return L.b64.p; }
   static double *x(FastLevel &L) { return L.x64.p; }
   static double *y(FastLevel &L) { return L.y64.p; }
@@ -527,6 +543,32 @@ void mcgs_sweeps(ufvm_ctx *ctx, FastLevel &L, int nsweep, bool from_zero, bool f
       pass(c, (sw == 0 && from_zero) ? L.col_off[c + 1] : L.n);
     for (int c = L.ncol - 2; c >= 0; --c) pass(c, L.n);
   }
+}
+
+// One sweep of the reference's DILU smoother (cfdSolveILU.m:14-93) on a
+// kind-4 level: rc = b - A x (rc = b from zero), forward substitution, then
+// backward substitution with x += z, each over the lexicographic level
+// schedule of the reference level R.
+template <typename V>
+void dilu_sweep_fast(ufvm_ctx *ctx, RefLevel &R, FastLevel &L, V *x, bool from_zero) {
+  using Sel = VecSel<V>;
+  hipStream_t s = ctx->stream;
+  V *rc = Sel::r(L);
+  if (from_zero) {
+    k::fill<V>(s, L.n, x, V(0));
+    k::convert<V, V>(s, L.n, Sel::b(L), rc);
+  } else {
+    k::residual_sell<V>(s, L.n, 0, L.soff.p, L.scol.p, Sel::sval(L), Sel::ac(L), Sel::b(L), x,
+                        rc);
+  }
+  const auto &F = R.fwd.level_ptr;
+  for (size_t q = 0; q + 1 < F.size(); ++q)
+    k::dilu_fwd_rows<V>(s, R.fwd_rows.p + F[q], F[q + 1] - F[q], Sel::anb(L), R.lptr.p, R.lnb.p,
+                        R.lpji.p, Sel::dinv(L), rc);
+  const auto &B = R.bwd.level_ptr;
+  for (size_t q = 0; q + 1 < B.size(); ++q)
+    k::dilu_bwd_rows<V>(s, R.bwd_rows.p + B[q], B[q + 1] - B[q], L.n, L.ptr.p, L.col.p,
+                        Sel::anb(L), Sel::dinv(L), rc, x);
 }
 
 // Jacobi levels with at most this many rows fuse the prolongation into the
@@ -577,6 +619,22 @@ V *vcycle_fast(ufvm_ctx *ctx, System &S, int l, bool first_done = false) {
                         pro ? L.parent.p : nullptr, pro ? xc : nullptr, y, omega);
       std::swap(x, y);
     }
+    return x;
+  }
+  if (L.kind == 4) {
+    RefLevel &R = *S.lev[l];
+    if (pre == 0) k::fill<V>(s, L.n, x, V(0));
+    for (int it = 0; it < pre; ++it) dilu_sweep_fast<V>(ctx, R, L, x, it == 0);
+    k::residual_sell<V>(s, L.n, 0, L.soff.p, L.scol.p, Sel::sval(L), Sel::ac(L), Sel::b(L), x,
+                        Sel::r(L));
+    if (C.kind == 3 && pre > 0)
+      k::restrict_jacobi<V>(s, C.n, L.csoff.p, L.cschild.p, Sel::r(L), Sel::ac(C), Sel::b(C),
+                            Sel::x(C), omega);
+    else
+      k::restrict_fast<V>(s, C.n, L.csoff.p, L.cschild.p, Sel::r(L), Sel::b(C));
+    V *xc = vcycle_fast<V>(ctx, S, l + 1);
+    k::prolong_fast<V>(s, 0, L.n, L.parent.p, xc, x);
+    for (int it = 0; it < post; ++it) dilu_sweep_fast<V>(ctx, R, L, x, false);
     return x;
   }
   if (L.kind == 0) {
@@ -652,9 +710,12 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
   P.lev.resize(nlev);
   if (cfg.coarse_smoother != 0 && cfg.coarse_smoother != 1)
     throw Error(UFVM_ERR_ARG, "coarse_smoother must be 0 or 1");
+  if (cfg.smoother != 0 && cfg.smoother != 1) throw Error(UFVM_ERR_ARG, "smoother must be 0 or 1");
   for (int32_t l = 0; l < nlev; ++l)
     build_fast_level(ctx, S, l,
-                     l == nlev - 1 ? 2 : (l == 0 ? 0 : (cfg.coarse_smoother == 1 ? 3 : 1)));
+                     l == nlev - 1 ? 2
+                                   : (l == 0 ? (cfg.smoother == 1 ? 4 : 0)
+                                             : (cfg.coarse_smoother == 1 ? 3 : 1)));
   for (int32_t l = 0; l + 1 < nlev; ++l) link_levels(ctx, S, l);
   // dense inverse of the coarsest level (+ mean(diag)/n * 11^T when singular)
   RefLevel &RC = *S.lev[nlev - 1];

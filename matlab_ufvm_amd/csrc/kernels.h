@@ -27,12 +27,16 @@ void gs_rows(hipStream_t s, const int32_t *rows, int32_t count, const int32_t *p
 void dilu_factor_rows(hipStream_t s, const int32_t *rows, int32_t count, const double *ac,
                       const double *anb, const int32_t *lptr, const int32_t *lnb,
                       const int32_t *lpji, const int32_t *lpij, int32_t skip_row, double *dinv);
-void dilu_fwd_rows(hipStream_t s, const int32_t *rows, int32_t count, const double *anb,
-                   const int32_t *lptr, const int32_t *lnb, const int32_t *lpji,
-                   const double *dinv, double *rc);
+// DILU substitution sweeps over one level of the lexicographic schedule
+// (double: the reference arithmetic; float: the FP32 precision-form smoother)
+template <typename V>
+void dilu_fwd_rows(hipStream_t s, const int32_t *rows, int32_t count, const V *anb,
+                   const int32_t *lptr, const int32_t *lnb, const int32_t *lpji, const V *dinv,
+                   V *rc);
+template <typename V>
 void dilu_bwd_rows(hipStream_t s, const int32_t *rows, int32_t count, int32_t n,
-                   const int32_t *ptr, const int32_t *col, const double *anb, const double *dinv,
-                   double *rc, double *x);
+                   const int32_t *ptr, const int32_t *col, const V *anb, const V *dinv, V *rc,
+                   V *x);
 // bc_c[p] = sum over children (increasing fine index) of r
 void restrict_ref(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *child,
                   const double *r, double *bc_c);

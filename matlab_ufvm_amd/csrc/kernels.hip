@@ -118,58 +118,74 @@ void dilu_factor_rows(hipStream_t s, const int32_t *rows, int32_t count, const d
 }
 
 // rc_j -= a_ji * (dinv_i * rc_i) for lower i in increasing order
+template <typename V>
 __global__ void __launch_bounds__(kBlock)
-k_dilu_fwd(const int32_t *__restrict__ rows, int32_t count, const double *__restrict__ anb,
+k_dilu_fwd(const int32_t *__restrict__ rows, int32_t count, const V *__restrict__ anb,
            const int32_t *__restrict__ lptr, const int32_t *__restrict__ lnb,
-           const int32_t *__restrict__ lpji, const double *__restrict__ dinv, double *rc) {
+           const int32_t *__restrict__ lpji, const V *__restrict__ dinv, V *rc) {
   const int32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= count) return;
   const int32_t j = rows[t];
-  double v = rc[j];
+  V v = rc[j];
   for (int32_t e = lptr[j]; e < lptr[j + 1]; ++e) {
     const int32_t i = lnb[e];
-    const double mat1 = dinv[i] * rc[i];
+    const V mat1 = dinv[i] * rc[i];
     v = v - anb[lpji[e]] * mat1;
   }
   rc[j] = v;
 }
 
-void dilu_fwd_rows(hipStream_t s, const int32_t *rows, int32_t count, const double *anb,
-                   const int32_t *lptr, const int32_t *lnb, const int32_t *lpji,
-                   const double *dinv, double *rc) {
+template <typename V>
+void dilu_fwd_rows(hipStream_t s, const int32_t *rows, int32_t count, const V *anb,
+                   const int32_t *lptr, const int32_t *lnb, const int32_t *lpji, const V *dinv,
+                   V *rc) {
   if (count <= 0) return;
-  hipLaunchKernelGGL(k_dilu_fwd, dim3(nblocks(count)), dim3(kBlock), 0, s, rows, count, anb, lptr,
-                     lnb, lpji, dinv, rc);
+  hipLaunchKernelGGL(k_dilu_fwd<V>, dim3(nblocks(count)), dim3(kBlock), 0, s, rows, count, anb,
+                     lptr, lnb, lpji, dinv, rc);
 }
 
 // rc_i -= a_ij rc_j (j > i, cconn order) ; z = dinv_i rc_i ; rc_i = z ; x_i += z
+template <typename V>
 __global__ void __launch_bounds__(kBlock)
 k_dilu_bwd(const int32_t *__restrict__ rows, int32_t count, int32_t n,
            const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
-           const double *__restrict__ anb, const double *__restrict__ dinv, double *rc,
-           double *x) {
+           const V *__restrict__ anb, const V *__restrict__ dinv, V *rc, V *x) {
   const int32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= count) return;
   const int32_t i = rows[t];
-  double v = rc[i];
+  V v = rc[i];
   if (i < n - 1) {
     for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
       const int32_t j = col[k];
       if (j > i) v = v - anb[k] * rc[j];
     }
   }
-  const double z = dinv[i] * v;
+  const V z = dinv[i] * v;
   rc[i] = z;
   x[i] = x[i] + z;
 }
 
+template <typename V>
 void dilu_bwd_rows(hipStream_t s, const int32_t *rows, int32_t count, int32_t n,
-                   const int32_t *ptr, const int32_t *col, const double *anb, const double *dinv,
-                   double *rc, double *x) {
+                   const int32_t *ptr, const int32_t *col, const V *anb, const V *dinv, V *rc,
+                   V *x) {
   if (count <= 0) return;
-  hipLaunchKernelGGL(k_dilu_bwd, dim3(nblocks(count)), dim3(kBlock), 0, s, rows, count, n, ptr,
-                     col, anb, dinv, rc, x);
+  hipLaunchKernelGGL(k_dilu_bwd<V>, dim3(nblocks(count)), dim3(kBlock), 0, s, rows, count, n,
+                     ptr, col, anb, dinv, rc, x);
 }
+
+template void dilu_fwd_rows<double>(hipStream_t, const int32_t *, int32_t, const double *,
+                                    const int32_t *, const int32_t *, const int32_t *,
+                                    const double *, double *);
+template void dilu_fwd_rows<float>(hipStream_t, const int32_t *, int32_t, const float *,
+                                   const int32_t *, const int32_t *, const int32_t *,
+                                   const float *, float *);
+template void dilu_bwd_rows<double>(hipStream_t, const int32_t *, int32_t, int32_t,
+                                    const int32_t *, const int32_t *, const double *,
+                                    const double *, double *, double *);
+template void dilu_bwd_rows<float>(hipStream_t, const int32_t *, int32_t, int32_t,
+                                   const int32_t *, const int32_t *, const float *,
+                                   const float *, float *, float *);
 
 template <typename V>
 __global__ void __launch_bounds__(kBlock)
@@ -935,6 +951,7 @@ UFVM_INST(double)
 template void convert<double, float>(hipStream_t, int64_t, const double *, float *);
 template void convert<float, double>(hipStream_t, int64_t, const float *, double *);
 template void convert<double, double>(hipStream_t, int64_t, const double *, double *);
+template void convert<float, float>(hipStream_t, int64_t, const float *, float *);
 
 }  // namespace k
 }  // namespace ufvm

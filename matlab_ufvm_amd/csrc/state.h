@@ -86,6 +86,9 @@ struct FastLevel {
   // kind 0: global colouring, colour classes contiguous (level 1)
   // kind 1: blocks of at most kBRows rows, block-local colouring (coarse levels)
   // kind 2: coarsest level, solved densely
+  // kind 3: coarse level smoothed by damped Jacobi, rows sorted by length
+  // kind 4: finest level smoothed by the reference's DILU in its own
+  //         (lexicographic) order, level-scheduled; reference numbering
   int kind = 0;
   std::vector<int32_t> col_off;   // kind 0: colour c rows are [col_off[c], col_off[c+1])
   int32_t nblk = 0, bncol = 0;    // kind 1: blocks and colours per block
@@ -105,6 +108,9 @@ struct FastLevel {
   DBuf<int32_t> soff, scol;
   DBuf<float> sval32;
   DBuf<double> sval64;
+  // kind 4: DILU diagonal inverse (factored once in FP64 at set-up)
+  DBuf<float> dinv32;
+  DBuf<double> dinv64;
 };
 
 struct PcgState {

@@ -76,7 +76,7 @@ class SimpleCase:
         c.p_amg = AmgCfg(_lib.CYCLE[p_cycle], p_max_cycles, p_termination, p_max_coarse_levels,
                          p_pre, p_post, _lib.SMOOTHER[p_smoother], 1)
         c.p_pcg = PcgCfg(pcg_precision, pcg_sweeps, pcg_sweeps, 1, pcg_max_coarse_cells, -1, 500,
-                         pcg_tol)
+                         pcg_tol, 0)
         cid = C.c_int32()
         ctx.check(ctx.lib.ufvm_case_create(ctx.h, C.byref(d), C.byref(c), C.byref(cid)))
         self.id = cid.value

@@ -33,7 +33,7 @@ class Hier(C.Structure):
 class VCfg(C.Structure):
     _fields_ = [("nlev", C.c_int32), ("pre", C.c_int32), ("post", C.c_int32),
                 ("singular", C.c_int32), ("block_rows", C.c_int32),
-                ("coarse_smoother", C.c_int32)]
+                ("coarse_smoother", C.c_int32), ("smoother", C.c_int32)]
 
 
 BLOCK_ROWS = 1024   # rows per block of the coarse-level smoother
@@ -185,20 +185,23 @@ class System:
                 return l + 1
         return self.H.nlevels
 
-    def vcycle(self, b, pre=1, post=1, max_coarse_cells=2000, singular=0, coarse_smoother=1):
+    def vcycle(self, b, pre=1, post=1, max_coarse_cells=2000, singular=0, coarse_smoother=1,
+               smoother="MCGS"):
         nlev = self.pcg_levels(max_coarse_cells)
         arr, ncl = self._colour_all(nlev)
-        cfg = VCfg(nlev, pre, post, singular, BLOCK_ROWS, coarse_smoother)
+        cfg = VCfg(nlev, pre, post, singular, BLOCK_ROWS, coarse_smoother,
+                   {"MCGS": 0, "ILU": 1}[smoother])
         b = np.ascontiguousarray(b, np.float64)
         x = np.empty_like(b)
         lib().orc_vcycle(C.byref(self.H), arr, ncl, C.byref(cfg), _p(b, f64p), _p(x, f64p))
         return x
 
     def fpcg(self, b, tol=1e-8, maxit=500, pre=1, post=1, max_coarse_cells=2000, singular=0,
-             coarse_smoother=1):
+             coarse_smoother=1, smoother="MCGS"):
         nlev = self.pcg_levels(max_coarse_cells)
         arr, ncl = self._colour_all(nlev)
-        cfg = VCfg(nlev, pre, post, singular, BLOCK_ROWS, coarse_smoother)
+        cfg = VCfg(nlev, pre, post, singular, BLOCK_ROWS, coarse_smoother,
+                   {"MCGS": 0, "ILU": 1}[smoother])
         b = np.ascontiguousarray(b, np.float64)
         x = np.empty_like(b)
         rr = C.c_double()

@@ -246,12 +246,13 @@ void orc_factorize_ilu(const orc_level *L, double *dc, double *rc) {
   }
 }
 
-/* cfdSolveILU.m:14-93: residual, forward DILU substitution, backward
- * substitution with dphi += D*rc. */
-void orc_solve_ilu(const orc_level *L, const double *dc, double *rc, double *dphi) {
+/* cfdSolveILU.m:14-93 with right-hand side b: residual, forward DILU
+ * substitution, backward substitution with dphi += D*rc. */
+static void ilu_apply(const orc_level *L, const double *dc, const double *b, double *rc,
+                      double *dphi) {
   const int32_t n = L->n;
   for (int32_t i = 0; i < n; ++i) {
-    double s = -L->ac[i] * dphi[i] + L->bc[i];
+    double s = -L->ac[i] * dphi[i] + b[i];
     for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) s = s - L->anb[k] * dphi[L->col[k]];
     rc[i] = s;
   }
@@ -277,6 +278,11 @@ void orc_solve_ilu(const orc_level *L, const double *dc, double *rc, double *dph
     rc[i] = mat1;
     dphi[i] = dphi[i] + mat1;
   }
+}
+
+/* cfdSolveILU.m:14-93 on the level's own right-hand side */
+void orc_solve_ilu(const orc_level *L, const double *dc, double *rc, double *dphi) {
+  ilu_apply(L, dc, L->bc, rc, dphi);
 }
 
 static double l1(const double *r, int32_t n) {
@@ -625,7 +631,8 @@ static void jacobi(const orc_level *L, const double *b, const double *x, const i
 }
 
 static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol,
-                     const orc_vcfg *cfg, const orc_lu *lu, int l, const double *b, double *x) {
+                     const orc_vcfg *cfg, const orc_lu *lu, const double *dc0, int l,
+                     const double *b, double *x) {
   const orc_level *L = &H->lev[l];
   const int32_t n = L->n;
   if (l == cfg->nlev - 1) {
@@ -651,7 +658,7 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
     double *bc = (double *)calloc((size_t)C->n, sizeof(double));
     double *xc = (double *)calloc((size_t)C->n, sizeof(double));
     for (int32_t i = 0; i < n; ++i) bc[L->parents[i]] = bc[L->parents[i]] + r[i];
-    vcyc_rec(H, colours, ncol, cfg, lu, l + 1, bc, xc);
+    vcyc_rec(H, colours, ncol, cfg, lu, dc0, l + 1, bc, xc);
     if (cfg->post == 0) {
       for (int32_t i = 0; i < n; ++i) x[i] = x[i] + xc[L->parents[i]];
     } else {
@@ -665,6 +672,26 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
     free(y); free(r); free(bc); free(xc);
     return;
   }
+  if (l == 0 && cfg->smoother == 1) {
+    /* the reference's DILU in its lexicographic order, pre and post */
+    double *rc = (double *)malloc((size_t)n * sizeof(double));
+    for (int s = 0; s < cfg->pre; ++s) ilu_apply(L, dc0, b, rc, x);
+    double *r = (double *)malloc((size_t)n * sizeof(double));
+    for (int32_t i = 0; i < n; ++i) {
+      double s = b[i] - L->ac[i] * x[i];
+      for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) s = s - L->anb[k] * x[L->col[k]];
+      r[i] = s;
+    }
+    const orc_level *C = &H->lev[l + 1];
+    double *bc = (double *)calloc((size_t)C->n, sizeof(double));
+    double *xc = (double *)calloc((size_t)C->n, sizeof(double));
+    for (int32_t i = 0; i < n; ++i) bc[L->parents[i]] = bc[L->parents[i]] + r[i];
+    vcyc_rec(H, colours, ncol, cfg, lu, dc0, l + 1, bc, xc);
+    for (int32_t i = 0; i < n; ++i) x[i] = x[i] + xc[L->parents[i]];
+    for (int s = 0; s < cfg->post; ++s) ilu_apply(L, dc0, b, rc, x);
+    free(rc); free(r); free(bc); free(xc);
+    return;
+  }
   if (l == 0) {
     for (int s = 0; s < cfg->pre; ++s) orc_mcgs(L, colours[l], ncol[l], b, x);
   } else if (cfg->pre > 0) {
@@ -674,7 +701,7 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
   for (int32_t i = 0; i < n; ++i) {
     /* level 0 after a symmetric sweep: colour 0 was solved last and its
      * residual (rounding noise) is taken as 0 */
-    if (l == 0 && cfg->pre > 0 && colours[0][i] == 0) {
+    if (l == 0 && cfg->smoother == 0 && cfg->pre > 0 && colours[0][i] == 0) {
       r[i] = 0.0;
       continue;
     }
@@ -686,7 +713,7 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
   double *bc = (double *)calloc((size_t)C->n, sizeof(double));
   double *xc = (double *)calloc((size_t)C->n, sizeof(double));
   for (int32_t i = 0; i < n; ++i) bc[L->parents[i]] = bc[L->parents[i]] + r[i];
-  vcyc_rec(H, colours, ncol, cfg, lu, l + 1, bc, xc);
+  vcyc_rec(H, colours, ncol, cfg, lu, dc0, l + 1, bc, xc);
   if (l > 0 && cfg->post > 0) {
     double *y = (double *)malloc((size_t)n * sizeof(double));
     bmcgs(L, colours[l], ncol[l], cfg->block_rows, b, x, L->parents, xc, 2, cfg->post, y);
@@ -700,11 +727,24 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
   free(r); free(bc); free(xc);
 }
 
+/* DILU factor of level 0 when the finest smoother is DILU (else NULL) */
+static double *finest_dilu(const orc_hier *H, const orc_vcfg *cfg) {
+  if (cfg->smoother != 1 || cfg->nlev < 2) return NULL;
+  const orc_level *L = &H->lev[0];
+  double *dc = (double *)malloc((size_t)L->n * sizeof(double));
+  double *rc = (double *)malloc((size_t)L->n * sizeof(double));
+  orc_factorize_ilu(L, dc, rc);
+  free(rc);
+  return dc;
+}
+
 void orc_vcycle(const orc_hier *H, int32_t *const *colours, const int *ncolours,
                 const orc_vcfg *cfg, const double *b, double *x) {
   orc_lu lu;
   lu_build(&H->lev[cfg->nlev - 1], cfg->singular, &lu);
-  vcyc_rec(H, colours, ncolours, cfg, &lu, 0, b, x);
+  double *dc0 = finest_dilu(H, cfg);
+  vcyc_rec(H, colours, ncolours, cfg, &lu, dc0, 0, b, x);
+  free(dc0);
   lu_free(&lu);
 }
 
@@ -748,8 +788,9 @@ int orc_fpcg(const orc_hier *H, int32_t *const *colours, const int *ncolours,
   double delta = 1.0;
   orc_lu lu;
   lu_build(&H->lev[cfg->nlev - 1], cfg->singular, &lu);
+  double *dc0 = finest_dilu(H, cfg);
   while (it < maxit) {
-    vcyc_rec(H, colours, ncolours, cfg, &lu, 0, r, z);
+    vcyc_rec(H, colours, ncolours, cfg, &lu, dc0, 0, r, z);
     if (cfg->singular) project(z, n);
     if (it == 0) {
       memcpy(p, z, (size_t)n * sizeof(double));
@@ -769,6 +810,7 @@ int orc_fpcg(const orc_hier *H, int32_t *const *colours, const int *ncolours,
     rr = sqrt(dotv(r, r, n));
     if (rr <= tol * nb) break;
   }
+  free(dc0);
   lu_free(&lu);
 done:
   if (relres) *relres = nb > 0 ? rr / nb : 0.0;

@@ -89,6 +89,8 @@ typedef struct {
   int32_t singular;
   int32_t block_rows;
   int32_t coarse_smoother;  /* levels 1..nlev-2: 0 block-local MCGS, 1 damped Jacobi (2/3) */
+  int32_t smoother;         /* level 0: 0 symmetric multicolour GS, 1 the reference's DILU
+                               (cfdFactorizeILU / cfdSolveILU, lexicographic order) */
 } orc_vcfg;
 void orc_vcycle(const orc_hier *H, int32_t *const *colours, const int *ncolours,
                 const orc_vcfg *cfg, const double *b, double *x);

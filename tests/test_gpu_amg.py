@@ -136,6 +136,16 @@ def test_pcg_vcycle_fp64_matches_oracle(ctx, case, coarse_smoother, sweeps):
     z_ref = S.vcycle(r, pre=sweeps, post=sweeps, max_coarse_cells=200,
                      coarse_smoother=coarse_smoother)
     assert np.abs(z - z_ref).max() <= 1e-12 * np.abs(z_ref).max()
+    # the DILU finest smoother (reference ordering, level-scheduled)
+    if coarse_smoother == 1:
+        ctx.pcg_setup(sid, precision=64, pre_sweep=sweeps, post_sweep=sweeps,
+                      max_coarse_cells=200, coarse_smoother=1, smoother="ILU")
+        zi = ctx.pcg_apply_vcycle(sid, r)
+        zi_ref = S.vcycle(r, pre=sweeps, post=sweeps, max_coarse_cells=200, coarse_smoother=1,
+                          smoother="ILU")
+        assert np.abs(zi - zi_ref).max() <= 1e-12 * np.abs(zi_ref).max()
+        ctx.pcg_setup(sid, precision=64, pre_sweep=sweeps, post_sweep=sweeps,
+                      max_coarse_cells=200, coarse_smoother=coarse_smoother)
     # same colourings as the oracle's greedy order (level 1 always; the coarse
     # levels when they use the block-local multicolour smoother)
     nlev_col = len(S.colours) - 1 if coarse_smoother == 0 else 1
@@ -146,14 +156,17 @@ def test_pcg_vcycle_fp64_matches_oracle(ctx, case, coarse_smoother, sweeps):
     ctx.system_destroy(sid)
 
 
-@pytest.mark.parametrize("precision,tol,err,cs", [(64, 1e-12, 1e-10, 1), (32, 1e-11, 1e-8, 1),
-                                                  (32, 1e-11, 1e-8, 0)])
-def test_pcg_solve_matches_direct(ctx, case, precision, tol, err, cs):
+@pytest.mark.parametrize("precision,tol,err,cs,sm", [(64, 1e-12, 1e-10, 1, "MCGS"),
+                                                     (32, 1e-11, 1e-8, 1, "MCGS"),
+                                                     (32, 1e-11, 1e-8, 0, "MCGS"),
+                                                     (32, 1e-11, 1e-8, 1, "ILU"),
+                                                     (64, 1e-12, 1e-10, 1, "ILU")])
+def test_pcg_solve_matches_direct(ctx, case, precision, tol, err, cs, sm):
     _, m, ptr, col, ac, anb, b = case
     sid = make(ctx, ptr, col, ac, anb, b)
     ctx.agglomerate(sid, 10)
     ctx.pcg_setup(sid, precision=precision, max_coarse_cells=200, tol=tol, max_iter=300,
-                  coarse_smoother=cs)
+                  coarse_smoother=cs, smoother=sm)
     it, rr = ctx.pcg_solve(sid)
     x = ctx.get_dphi(sid, 0)
     A = to_scipy(ptr, col, ac, anb)

@@ -337,11 +337,71 @@ def test_vcycle_symmetric_with_block_smoothed_levels(coarse_smoother):
         assert y @ My > 0   # positive definite preconditioner
 
 
-def test_fpcg_matches_direct_solve(poisson16):
+@pytest.mark.parametrize("smoother,sweeps", [("MCGS", 1), ("MCGS", 2), ("ILU", 1), ("ILU", 2)])
+def test_two_level_vcycle_matches_dense_formula(poisson16, smoother, sweeps):
+    # With the coarse level solved exactly the V-cycle is, in matrices,
+    #   x1 = S^pre(0; b), xc = Ac^-1 P^T (b - A x1), x2 = x1 + P xc, z = S^post(x2; b)
+    # where one smoothing sweep from x is x + M^-1 (b - A x):
+    #   MCGS: M = D + L_c (colours up) then D + U_c (colours down), L_c / U_c the
+    #         couplings to lower / higher colour classes
+    #   ILU:  M = (D' + L) D'^-1 (D' + U), D' the DILU diagonal (pinned above)
+    # and Ac = P^T A P from the parents.
     m, ptr, col, ac, anb, b = poisson16
     S = oracle.System(ptr, col, ac, anb)
     S.agglomerate(10)
-    x, it, rr = S.fpcg(b, tol=1e-12, max_coarse_cells=200)
+    n1 = S.level(1)["n"]
+    A = to_scipy(ptr, col, ac, anb).toarray()
+    par = S.level(0)["parents"]
+    P = np.zeros((m.n_cells, n1))
+    P[np.arange(m.n_cells), par] = 1.0
+    Ac = P.T @ A @ P
+    if smoother == "MCGS":
+        S._colour_all(2)
+        c, _ = S.colours_of(0)
+        D = np.diag(np.diag(A))
+        off = A - D
+        lower = off * (c[None, :] < c[:, None])
+        upper = off * (c[None, :] > c[:, None])
+        Ms = [D + lower, D + upper]
+    else:
+        Sd = oracle.System(ptr, col, ac, anb, bc=b)
+        Dp = np.diag(1.0 / Sd.factorize_ilu())
+        Ms = [(Dp + np.tril(A, -1)) @ np.linalg.inv(Dp) @ (Dp + np.triu(A, 1))]
+
+    def sweep(x):
+        for M in Ms:
+            x = x + np.linalg.solve(M, b - A @ x)
+        return x
+
+    x = np.zeros(m.n_cells)
+    for _ in range(sweeps):
+        x = sweep(x)
+    x = x + P @ np.linalg.solve(Ac, P.T @ (b - A @ x))
+    for _ in range(sweeps):
+        x = sweep(x)
+    z = S.vcycle(b, pre=sweeps, post=sweeps, max_coarse_cells=n1, smoother=smoother)
+    assert np.abs(z - x).max() <= 1e-10 * np.abs(x).max()
+
+
+@pytest.mark.parametrize("smoother", ["MCGS", "ILU"])
+def test_vcycle_symmetric_for_each_finest_smoother(poisson16, smoother):
+    m, ptr, col, ac, anb, b = poisson16
+    S = oracle.System(ptr, col, ac, anb)
+    S.agglomerate(10)
+    rng = np.random.default_rng(7)
+    y, z = rng.standard_normal(m.n_cells), rng.standard_normal(m.n_cells)
+    My = S.vcycle(y, max_coarse_cells=50, smoother=smoother)
+    Mz = S.vcycle(z, max_coarse_cells=50, smoother=smoother)
+    assert abs(z @ My - y @ Mz) <= 1e-11 * abs(z @ My)
+    assert y @ My > 0
+
+
+@pytest.mark.parametrize("smoother", ["MCGS", "ILU"])
+def test_fpcg_matches_direct_solve(poisson16, smoother):
+    m, ptr, col, ac, anb, b = poisson16
+    S = oracle.System(ptr, col, ac, anb)
+    S.agglomerate(10)
+    x, it, rr = S.fpcg(b, tol=1e-12, max_coarse_cells=200, smoother=smoother)
     xs = spl.spsolve(to_scipy(ptr, col, ac, anb).tocsc(), b)
     assert rr <= 1e-12
     assert np.linalg.norm(x - xs) <= 1e-10 * np.linalg.norm(xs)
