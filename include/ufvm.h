@@ -239,6 +239,60 @@ int ufvm_correct_equation(ufvm_ctx *ctx, int32_t cas, const char *eq, int32_t co
  * NULL) receives 4 values per iteration: RMS residuals of Ux, Uy, Uz, p. */
 int ufvm_simple_iterate(ufvm_ctx *ctx, int32_t cas, int32_t n, double *rms);
 
+/* --------------------------------------------------------- serving form */
+/* No reference counterpart.  A single-GPU request loop over resident meshes:
+ * pressure-Poisson solves (the FP32-preconditioned AMG-PCG of ufvm_pcg_*) and
+ * SIMPLE steps (ufvm_simple_iterate).  Solves are continuously batched: up
+ * to max_active run at once, each on its own workspace and one of n_streams
+ * streams, and a worker thread advances each by one FCG iteration as the GPU
+ * finishes it; queued requests are admitted when a slot is free and the
+ * device keeps hbm_reserve of its memory free after the request's workspace.
+ * SIMPLE steps run on a second thread and stream, overlapping the solves.
+ * Results do not depend on what else runs: every solve's arithmetic depends
+ * only on its system and right-hand side.  Host arrays handed to a submit
+ * call must stay valid until the request is done. */
+typedef struct ufvm_server ufvm_server;
+typedef struct {
+  int32_t max_active;        /* solves on the GPU at once (>= 1) */
+  int32_t n_streams;         /* streams the active solves are spread over (>= 1) */
+  double hbm_reserve;        /* fraction of device memory admission keeps free */
+} ufvm_server_cfg;
+typedef struct {
+  int32_t status;            /* 0 queued, 1 running, 2 done, < 0: -UFVM_ERR_* */
+  int32_t iterations;        /* solve: FCG iterations */
+  double relres;             /* solve: ||r|| / ||b|| */
+  double queued_s;           /* submit -> admission (seconds) */
+  double run_s;              /* admission -> result on the host */
+} ufvm_request_info;
+
+int ufvm_server_create(int32_t device, const ufvm_server_cfg *cfg, ufvm_server **out);
+/* finishes every queued and running request, then frees everything */
+int ufvm_server_destroy(ufvm_server *srv);
+const char *ufvm_server_last_error(const ufvm_server *srv);
+/* a pressure-Poisson system (level-1 CSR in cconn order + coefficients): its
+ * hierarchy and precision-form copies are built now and stay resident */
+int ufvm_server_add_system(ufvm_server *srv, int32_t n, int32_t nnz, const int32_t *ptr,
+                           const int32_t *col, const double *ac, const double *anb,
+                           const ufvm_pcg_cfg *cfg, int32_t *sys_out);
+/* a SIMPLE case (as ufvm_case_create); its fields stay resident */
+int ufvm_server_add_case(ufvm_server *srv, const ufvm_mesh_desc *mesh, const ufvm_simple_cfg *cfg,
+                         int32_t *case_out);
+int ufvm_server_get_field(ufvm_server *srv, int32_t cas, const char *name, double *data,
+                          int64_t count);
+int ufvm_server_set_field(ufvm_server *srv, int32_t cas, const char *name, const double *data,
+                          int64_t count);
+/* solve A x = b (host arrays of the system's size, reference order) */
+int ufvm_server_submit_solve(ufvm_server *srv, int32_t sys, const double *b, double *x,
+                             int64_t *ticket);
+/* nsteps SIMPLE outer iterations; rms (may be NULL): 4 values per step */
+int ufvm_server_submit_step(ufvm_server *srv, int32_t cas, int32_t nsteps, double *rms,
+                            int64_t *ticket);
+int ufvm_server_query(ufvm_server *srv, int64_t ticket, ufvm_request_info *info);
+int ufvm_server_wait(ufvm_server *srv, int64_t ticket, ufvm_request_info *info);
+/* forget a finished request */
+int ufvm_server_release(ufvm_server *srv, int64_t ticket);
+int ufvm_server_stats(ufvm_server *srv, int32_t *queued, int32_t *active, int64_t *work_bytes);
+
 /* ------------------------------------------------------------- profiling */
 /* Per-kernel-class HIP-event timing on the context stream (no reference
  * counterpart).  kind names: "mcgs0" (finest-level colour passes of the

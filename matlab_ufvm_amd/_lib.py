@@ -37,6 +37,15 @@ class PcgCfg(C.Structure):
                 ("max_iter", C.c_int32), ("tol", C.c_double), ("smoother", C.c_int32)]
 
 
+class ServerCfg(C.Structure):
+    _fields_ = [("max_active", C.c_int32), ("n_streams", C.c_int32), ("hbm_reserve", C.c_double)]
+
+
+class RequestInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("relres", C.c_double),
+                ("queued_s", C.c_double), ("run_s", C.c_double)]
+
+
 class MeshDesc(C.Structure):
     _fields_ = [("n_cells", C.c_int32), ("n_faces", C.c_int32), ("n_interior", C.c_int32),
                 ("owner", i32p), ("neighbour", i32p),
@@ -95,6 +104,23 @@ _SIGS = {
     "ufvm_solve_equation": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, f64p, f64p, i32p]),
     "ufvm_correct_equation": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32]),
     "ufvm_simple_iterate": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f64p]),
+    "ufvm_server_create": (C.c_int, [C.c_int32, C.POINTER(ServerCfg), C.POINTER(C.c_void_p)]),
+    "ufvm_server_destroy": (C.c_int, [C.c_void_p]),
+    "ufvm_server_last_error": (C.c_char_p, [C.c_void_p]),
+    "ufvm_server_add_system": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, i32p, i32p, f64p, f64p,
+                                         C.POINTER(PcgCfg), i32p]),
+    "ufvm_server_add_case": (C.c_int, [C.c_void_p, C.POINTER(MeshDesc), C.POINTER(SimpleCfg),
+                                       i32p]),
+    "ufvm_server_get_field": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, f64p, C.c_int64]),
+    "ufvm_server_set_field": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, f64p, C.c_int64]),
+    "ufvm_server_submit_solve": (C.c_int, [C.c_void_p, C.c_int32, f64p, f64p,
+                                           C.POINTER(C.c_int64)]),
+    "ufvm_server_submit_step": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f64p,
+                                          C.POINTER(C.c_int64)]),
+    "ufvm_server_query": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(RequestInfo)]),
+    "ufvm_server_wait": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(RequestInfo)]),
+    "ufvm_server_release": (C.c_int, [C.c_void_p, C.c_int64]),
+    "ufvm_server_stats": (C.c_int, [C.c_void_p, i32p, i32p, C.POINTER(C.c_int64)]),
     "ufvm_prof_enable": (C.c_int, [C.c_void_p, C.c_int32]),
     "ufvm_prof_read": (C.c_int, [C.c_void_p, C.c_char_p, f64p, C.POINTER(C.c_int64), f64p]),
     "ufvm_prof_reset": (C.c_int, [C.c_void_p]),

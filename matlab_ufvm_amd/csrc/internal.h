@@ -30,7 +30,47 @@ int guard(ufvm_ctx *ctx, F &&f) {
   }
 }
 
+// event from the context's pool (created on first use)
+inline hipEvent_t pool_event(ufvm_ctx *ctx) {
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e;
+  UFVM_HIP(hipEventCreate(&e));
+  return e;
+}
+
+// HIP events around one launch of a kernel class on stream s while profiling
+// is enabled on the context (every prof_stride-th launch of each class)
+struct ProfScope {
+  ufvm_ctx *ctx;
+  hipStream_t s;
+  bool on;
+  ProfRec rec;
+  ProfScope(ufvm_ctx *c, hipStream_t st, const char *kind, double bytes)
+      : ctx(c), s(st), on(c->prof_on) {
+    if (!on) return;
+    // sampled launches only: fewer event records keep the host's launch queue
+    // ahead of the GPU, so an interval holds the kernel and not a launch wait
+    on = (ctx->prof_seen[kind]++ % ctx->prof_stride) == 0;
+    if (!on) return;
+    rec.kind = kind;
+    rec.bytes = bytes;
+    rec.a = pool_event(ctx);
+    rec.b = pool_event(ctx);
+    (void)hipEventRecord(rec.a, s);
+  }
+  ~ProfScope() {
+    if (!on) return;
+    (void)hipEventRecord(rec.b, s);
+    ctx->prof.push_back(rec);
+  }
+};
+
 System &sys_of(ufvm_ctx *ctx, int32_t id);
+void ensure_schedule(ufvm_ctx *ctx, RefLevel &L);
 void ensure_partial(ufvm_ctx *ctx, int64_t n, int width);
 void read_scal(ufvm_ctx *ctx, int first, int count);
 int32_t agglomerate(ufvm_ctx *ctx, System &S, int32_t max_coarse);

@@ -102,6 +102,8 @@ void csr_to_dense(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *c
 // ---- FPCG (FP64, level-1 order of the multicolour permutation).  Each call
 // is a streaming kernel writing per-block partial sums plus a one-block kernel
 // reducing them and updating the device scalars scal[] (layout in kernels.hip).
+// ring[0] = *src, ring[1] = -1 (a device scalar handed to host-visible memory)
+void ring_put(hipStream_t s, const double *src, double *ring);
 // FCG(1) iteration kernels (scalar slots of scal[] listed in kernels.hip).
 // w = A p (sliced-ELL finest level); partial[0, g) = per-block p.w
 void spmv_dot(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, const double *ac,

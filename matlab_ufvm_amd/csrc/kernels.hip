@@ -847,6 +847,17 @@ k_fcg_update_xr(int32_t n, const double *__restrict__ partial_in, double *__rest
   if (threadIdx.x == 0) partial_out[blockIdx.x] = rr;
 }
 
+__global__ void k_ring_put(const double *__restrict__ src, double *ring) {
+  if (threadIdx.x == 0) {
+    ring[0] = src[0];
+    ring[1] = -1.0;
+  }
+}
+
+void ring_put(hipStream_t s, const double *src, double *ring) {
+  hipLaunchKernelGGL(k_ring_put, dim3(1), dim3(64), 0, s, src, ring);
+}
+
 // rr = r.r from the update's partials; sets the convergence flag scal[11]
 // once rr <= thr, after which scal[10] is left alone; ring (host-visible, may
 // be null) receives {rr, iter} for a host that polls without synchronising

@@ -1,0 +1,60 @@
+// pcg.h -- precision-form AMG-preconditioned flexible CG (internal to libufvm).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <memory>
+
+#include "internal.h"
+
+namespace ufvm {
+namespace detail {
+
+// Device workspace for one solve on system S (whose ufvm_pcg_setup is done):
+// level vectors in the V-cycle precision, FCG vectors, scalars, partials,
+// the pinned result ring and its events.  *bytes receives the device bytes.
+std::unique_ptr<SolveWork> make_work(const System &S);
+// device bytes make_work(S) would allocate
+size_t work_bytes(const System &S);
+
+// One FCG(1) solve of S on workspace W and stream s, driven as a state
+// machine so that one host thread can interleave solves on several streams:
+// start() enqueues the set-up, poll() advances when the GPU has finished the
+// awaited step (or blocks for it when block is set) and returns true once the
+// solution has been written.  The host reads only the ring; it never waits on
+// the stream while another step of the same solve could be queued.
+class FcgRun {
+ public:
+  FcgRun(ufvm_ctx *ctx, System &S, SolveWork &W, hipStream_t s)
+      : ctx_(ctx), S_(&S), W_(&W), s_(s) {}
+  // b: right-hand side, x: solution, both device arrays in reference order
+  void start(const double *b, double *x);
+  bool poll(bool block);
+  bool done() const { return phase_ == 4; }
+  int32_t iterations() const { return it_; }
+  double relres() const { return nb_ > 0.0 ? rr_ / nb_ : 0.0; }
+
+ private:
+  template <typename V>
+  void start_t(const double *b);
+  template <typename V>
+  void issue_t(int k);
+  void issue(int k);
+  void finish();
+  bool ready(hipEvent_t e, bool block);
+
+  ufvm_ctx *ctx_;
+  System *S_;
+  SolveWork *W_;
+  hipStream_t s_;
+  double *x_out_ = nullptr;
+  int phase_ = 0;    // 1 set-up queued, 2 iterating, 3 result queued, 4 done
+  int it_ = 0, issued_ = 0;
+  bool ahead_ = true;
+  double prev2_ = -1.0, nb_ = 0.0, thr_ = 0.0, rr_ = 0.0;
+};
+
+// z = M^-1 r for host arrays in reference order (one preconditioner V-cycle)
+void pcg_apply_vcycle(ufvm_ctx *ctx, System &S, const double *r, double *z);
+
+}  // namespace detail
+}  // namespace ufvm

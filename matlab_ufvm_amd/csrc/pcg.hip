@@ -1,0 +1,670 @@
+// pcg.hip -- precision-form pressure solve: FP32 (or FP64) AMG V-cycle over
+// the reference's agglomeration hierarchy inside FP64 flexible CG (Notay
+// FCG(1)).  No reference counterpart; the hierarchy (parents, coarse
+// patterns and Galerkin operators) is the reference's, bit-exact.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "kernels.h"
+#include "pcg.h"
+
+namespace ufvm {
+namespace detail {
+
+// Precision-form copy of hierarchy level l.  kind 0 (level 1): unknowns
+// renumbered so every colour class of a global greedy colouring is one
+// contiguous row range.  kind 1 (coarse levels): blocks of kBRows consecutive
+// reference rows, each sorted by a colouring of the block's own subgraph.
+// kind 2: the coarsest level, kept in reference order for the dense solve.
+// kind 3 (coarse levels smoothed by damped Jacobi): rows sorted by length
+// inside windows.  Rows keep their cconn order in every kind.
+void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
+  hipStream_t s = ctx->stream;
+  RefLevel &R = *S.lev[l];
+  auto &FLp = S.pcg.lev[l];
+  FLp.reset(new FastLevel());
+  FastLevel &FL = *FLp;
+  const int32_t n = R.n;
+  FL.n = n;
+  FL.nnz = R.nnz;
+  FL.kind = kind;
+  std::vector<double> ac(n), anb(std::max(R.nnz, 1));
+  R.ac.download(ac.data(), n, s);
+  R.anb.download(anb.data(), R.nnz, s);
+  UFVM_HIP(hipStreamSynchronize(s));
+  FL.perm.resize(n);
+  if (kind == 0) {
+    FL.ncol = greedy_colour(R.g, FL.colour);
+    FL.col_off.assign(FL.ncol + 1, 0);
+    for (int32_t i = 0; i < n; ++i) ++FL.col_off[FL.colour[i] + 1];
+    for (int c = 0; c < FL.ncol; ++c) FL.col_off[c + 1] += FL.col_off[c];
+    std::vector<int32_t> fillp(FL.col_off.begin(), FL.col_off.end() - 1);
+    for (int32_t i = 0; i < n; ++i) FL.perm[fillp[FL.colour[i]]++] = i;
+  } else if (kind == 1) {
+    const int32_t B = k::kBRows;
+    FL.ncol = greedy_colour_blocks(R.g, B, FL.colour);
+    FL.bncol = FL.ncol;
+    FL.nblk = (n + B - 1) / B;
+    FL.h_bptr.assign(FL.nblk + 1, 0);
+    FL.h_boff.assign((size_t)FL.nblk * (FL.bncol + 1), 0);
+    for (int32_t q = 0; q < FL.nblk; ++q) {
+      const int32_t lo = q * B, hi = std::min(n, lo + B);
+      FL.h_bptr[q] = lo;
+      int32_t *off = &FL.h_boff[(size_t)q * (FL.bncol + 1)];
+      std::vector<int32_t> cnt(FL.bncol + 1, 0);
+      for (int32_t i = lo; i < hi; ++i) ++cnt[FL.colour[i] + 1];
+      off[0] = lo;
+      for (int c = 0; c < FL.bncol; ++c) off[c + 1] = off[c] + cnt[c + 1];
+      std::vector<int32_t> fillp(off, off + FL.bncol);
+      for (int32_t i = lo; i < hi; ++i) FL.perm[fillp[FL.colour[i]]++] = i;
+    }
+    FL.h_bptr[FL.nblk] = n;
+    FL.bptr.upload(FL.h_bptr, s);
+    FL.boff.upload(FL.h_boff, s);
+  } else if (kind == 3) {
+    // Jacobi-smoothed: row order is free, so rows are sorted by decreasing
+    // length inside windows of kSigma rows; the 64-row sliced-ELL chunks then
+    // carry little padding while neighbours stay close in memory
+    constexpr int32_t kSigma = 256;
+    FL.ncol = 1;
+    FL.colour.assign(n, 0);
+    for (int32_t i = 0; i < n; ++i) FL.perm[i] = i;
+    auto len = [&](int32_t i) { return R.g.ptr[i + 1] - R.g.ptr[i]; };
+    for (int32_t w0 = 0; w0 < n; w0 += kSigma)
+      std::stable_sort(FL.perm.begin() + w0, FL.perm.begin() + std::min(n, w0 + kSigma),
+                       [&](int32_t a, int32_t b) { return len(a) > len(b); });
+  } else {
+    // kind 2 (dense coarsest) and kind 4 (DILU finest level): reference order
+    FL.ncol = 1;
+    FL.colour.assign(n, 0);
+    for (int32_t i = 0; i < n; ++i) FL.perm[i] = i;
+  }
+  std::vector<int32_t> iperm(n);
+  for (int32_t t = 0; t < n; ++t) iperm[FL.perm[t]] = t;
+  FL.h_ptr.assign(n + 1, 0);
+  std::vector<int32_t> col(std::max(R.nnz, 1));
+  std::vector<double> a64(std::max(R.nnz, 1)), d64(n);
+  for (int32_t t = 0; t < n; ++t) {
+    const int32_t i = FL.perm[t];
+    FL.h_ptr[t + 1] = FL.h_ptr[t] + (R.g.ptr[i + 1] - R.g.ptr[i]);
+    int32_t o = FL.h_ptr[t];
+    for (int32_t kk = R.g.ptr[i]; kk < R.g.ptr[i + 1]; ++kk, ++o) {
+      col[o] = iperm[R.g.col[kk]];
+      a64[o] = anb[kk];
+    }
+    d64[t] = ac[i];
+  }
+  std::vector<float> a32(a64.begin(), a64.end()), d32(d64.begin(), d64.end());
+  FL.ptr.upload(FL.h_ptr, s);
+  FL.col.upload(col, s);
+  FL.perm_d.upload(FL.perm, s);
+  FL.ac64.upload(d64, s);
+  FL.anb64.upload(a64, s);
+  FL.ac32.upload(d32, s);
+  FL.anb32.upload(a32, s);
+  if (kind == 4) {
+    // the reference's DILU factor (cfdFactorizeILU.m:10-47, row N-1 quirk
+    // kept) in FP64 over the lexicographic schedule, stored in both precisions
+    ensure_schedule(ctx, R);
+    if (!R.low_ok) throw Error(UFVM_ERR_NUMERIC, "ILU: the index for i in j is not found");
+    const auto &F = R.fwd.level_ptr;
+    for (size_t q = 0; q + 1 < F.size(); ++q)
+      k::dilu_factor_rows(s, R.fwd_rows.p + F[q], F[q + 1] - F[q], R.ac.p, R.anb.p, R.lptr.p,
+                          R.lnb.p, R.lpji.p, R.lpij.p, R.n - 2, R.dinv.p);
+    FL.dinv64.alloc(n);
+    FL.dinv32.alloc(n);
+    k::convert<double, double>(s, n, R.dinv.p, FL.dinv64.p);
+    k::convert<double, float>(s, n, R.dinv.p, FL.dinv32.p);
+  }
+  // the FPCG's operator product runs on level 0's sliced-ELL copy whatever its
+  // kind (a single-level hierarchy makes level 0 the dense coarsest level)
+  if (kind == 0 || kind == 3 || l == 0) {
+    const int32_t nq = (n + 63) / 64;
+    FL.h_soff.assign(nq + 1, 0);
+    for (int32_t q = 0; q < nq; ++q) {
+      int32_t w = 0;
+      for (int32_t t = q * 64; t < std::min(n, q * 64 + 64); ++t)
+        w = std::max(w, FL.h_ptr[t + 1] - FL.h_ptr[t]);
+      FL.h_soff[q + 1] = FL.h_soff[q] + 64 * w;
+    }
+    const size_t tot = std::max(FL.h_soff[nq], 1);
+    // padding: col = the row itself, value 0 (rows past n in the last chunk
+    // are never processed)
+    std::vector<int32_t> scol(tot, 0);
+    std::vector<double> sv64(tot, 0.0);
+    for (int32_t t = 0; t < n; ++t) {
+      const int32_t q = t / 64, base = FL.h_soff[q] + (t & 63);
+      const int32_t w = (FL.h_soff[q + 1] - FL.h_soff[q]) / 64;
+      const int32_t len = FL.h_ptr[t + 1] - FL.h_ptr[t];
+      for (int32_t e = 0; e < w; ++e) {
+        scol[base + e * 64] = e < len ? col[FL.h_ptr[t] + e] : t;
+        sv64[base + e * 64] = e < len ? a64[FL.h_ptr[t] + e] : 0.0;
+      }
+    }
+    std::vector<float> sv32(sv64.begin(), sv64.end());
+    FL.soff.upload(FL.h_soff, s);
+    FL.scol.upload(scol, s);
+    FL.sval64.upload(sv64, s);
+    FL.sval32.upload(sv32, s);
+  }
+  UFVM_HIP(hipStreamSynchronize(s));
+}
+
+void link_levels(ufvm_ctx *ctx, System &S, int l) {
+  hipStream_t s = ctx->stream;
+  RefLevel &R = *S.lev[l];
+  FastLevel &F = *S.pcg.lev[l];
+  FastLevel &C = *S.pcg.lev[l + 1];
+  std::vector<int32_t> ipf(F.n), ipc(C.n);
+  for (int32_t t = 0; t < F.n; ++t) ipf[F.perm[t]] = t;
+  for (int32_t t = 0; t < C.n; ++t) ipc[C.perm[t]] = t;
+  std::vector<int32_t> parent(F.n);
+  for (int32_t t = 0; t < F.n; ++t) parent[t] = ipc[R.h_parents[F.perm[t]]];
+  // children of coarse (new) row q: the reference children of perm_c[q], which
+  // are listed in increasing reference fine index (the reference summation order)
+  std::vector<int32_t> cnt(C.n + 1, 0);
+  for (int32_t i = 0; i < F.n; ++i) ++cnt[ipc[R.h_parents[i]] + 1];
+  for (int32_t q = 0; q < C.n; ++q) cnt[q + 1] += cnt[q];
+  std::vector<int32_t> child(F.n);
+  std::vector<int32_t> fillc(cnt.begin(), cnt.end() - 1);
+  for (int32_t i = 0; i < F.n; ++i) child[fillc[ipc[R.h_parents[i]]]++] = ipf[i];
+  // the same lists in 64-row chunks, column-major, padded with -1
+  const int32_t nq = (C.n + 63) / 64;
+  std::vector<int32_t> soff(nq + 1, 0);
+  for (int32_t q = 0; q < nq; ++q) {
+    int32_t w = 0;
+    for (int32_t p = q * 64; p < std::min(C.n, q * 64 + 64); ++p) w = std::max(w, cnt[p + 1] - cnt[p]);
+    soff[q + 1] = soff[q] + 64 * w;
+  }
+  std::vector<int32_t> schild(std::max(soff[nq], 1), -1);
+  for (int32_t p = 0; p < C.n; ++p)
+    for (int32_t e = cnt[p]; e < cnt[p + 1]; ++e)
+      schild[soff[p / 64] + (p & 63) + (e - cnt[p]) * 64] = child[e];
+  F.parent.upload(parent, s);
+  F.csoff.upload(soff, s);
+  F.cschild.upload(schild, s);
+  UFVM_HIP(hipStreamSynchronize(s));
+}
+
+
+// matrix arrays of a level copy and vectors of a level workspace, by precision
+template <typename V>
+struct Prec;
+template <>
+struct Prec<float> {
+  static const float *ac(const FastLevel &L) { return L.ac32.p; }
+  static const float *anb(const FastLevel &L) { return L.anb32.p; }
+  static const float *sval(const FastLevel &L) { return L.sval32.p; }
+  static const float *dinv(const FastLevel &L) { return L.dinv32.p; }
+  static float *b(LevelWork &W) { return W.b32.p; }
+  static float *x(LevelWork &W) { return W.x32.p; }
+  static float *y(LevelWork &W) { return W.y32.p; }
+  static float *r(LevelWork &W) { return W.r32.p; }
+  static DBuf<float> &buf(LevelWork &W, int i) {
+    return i == 0 ? W.b32 : i == 1 ? W.x32 : i == 2 ? W.r32 : W.y32;
+  }
+};
+template <>
+struct Prec<double> {
+  static const double *ac(const FastLevel &L) { return L.ac64.p; }
+  static const double *anb(const FastLevel &L) { return L.anb64.p; }
+  static const double *sval(const FastLevel &L) { return L.sval64.p; }
+  static const double *dinv(const FastLevel &L) { return L.dinv64.p; }
+  static double *b(LevelWork &W) { return W.b64.p; }
+  static double *x(LevelWork &W) { return W.x64.p; }
+  static double *y(LevelWork &W) { return W.y64.p; }
+  static double *r(LevelWork &W) { return W.r64.p; }
+  static DBuf<double> &buf(LevelWork &W, int i) {
+    return i == 0 ? W.b64 : i == 1 ? W.x64 : i == 2 ? W.r64 : W.y64;
+  }
+};
+
+// Jacobi levels with at most this many rows fuse the prolongation into the
+// first post-smoothing sweep.
+constexpr int32_t kFusedProlongRows = 100000;
+
+// One preconditioner V-cycle of system S on workspace W and stream s.
+template <typename V>
+struct Cycle {
+  using P = Prec<V>;
+  ufvm_ctx *ctx;
+  System &S;
+  SolveWork &W;
+  hipStream_t s;
+
+  // Symmetric multicolour Gauss-Seidel sweeps on the finest level (kind 0):
+  // colours upward then downward; the downward pass skips the last colour
+  // and every sweep after the first skips colour 0, because re-solving the
+  // colour just solved repeats identical arithmetic.  from_zero: x holds no
+  // data yet; first_done: the first pass (colour 0 from zero) was already
+  // made by the kernel that produced b.
+  void mcgs_sweeps(const FastLevel &L, LevelWork &LW, int nsweep, bool from_zero,
+                   bool first_done) {
+    const double w = sizeof(V);
+    auto pass = [&](int c, int32_t zero_above) {
+      const int32_t r0 = L.col_off[c], r1 = L.col_off[c + 1];
+      double bytes = 0.0;
+      if (ctx->prof_on) {
+        // algorithmic bytes: the colour's entries (col + value), its ac, b,
+        // x write, and one read of every other unknown
+        const double R = r1 - r0, Z = L.h_ptr[r1] - L.h_ptr[r0];
+        bytes = (4.0 + w) * Z + 3.0 * w * R + w * (L.n - R);
+      }
+      ProfScope ps(ctx, s, "mcgs0", bytes);
+      k::mcgs_sell<V>(s, r0, r1, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW), P::x(LW),
+                      zero_above);
+    };
+    for (int sw = 0; sw < nsweep; ++sw) {
+      for (int c = (sw == 0 ? (first_done ? 1 : 0) : 1); c < L.ncol; ++c)
+        pass(c, (sw == 0 && from_zero) ? L.col_off[c + 1] : L.n);
+      for (int c = L.ncol - 2; c >= 0; --c) pass(c, L.n);
+    }
+  }
+
+  // One sweep of the reference's DILU smoother (cfdSolveILU.m:14-93) on a
+  // kind-4 level: rc = b - A x (rc = b from zero), forward substitution, then
+  // backward substitution with x += z, each over the lexicographic level
+  // schedule of the reference level R.
+  void dilu_sweep(const RefLevel &R, const FastLevel &L, LevelWork &LW, V *x, bool from_zero) {
+    V *rc = P::r(LW);
+    if (from_zero) {
+      k::fill<V>(s, L.n, x, V(0));
+      k::convert<V, V>(s, L.n, P::b(LW), rc);
+    } else {
+      k::residual_sell<V>(s, L.n, 0, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW), x, rc);
+    }
+    const auto &F = R.fwd.level_ptr;
+    for (size_t q = 0; q + 1 < F.size(); ++q)
+      k::dilu_fwd_rows<V>(s, R.fwd_rows.p + F[q], F[q + 1] - F[q], P::anb(L), R.lptr.p, R.lnb.p,
+                          R.lpji.p, P::dinv(L), rc);
+    const auto &B = R.bwd.level_ptr;
+    for (size_t q = 0; q + 1 < B.size(); ++q)
+      k::dilu_bwd_rows<V>(s, R.bwd_rows.p + B[q], B[q + 1] - B[q], L.n, L.ptr.p, L.col.p,
+                          P::anb(L), P::dinv(L), rc, x);
+  }
+
+  // residual of level l into r, then restriction to level l+1 (fused with the
+  // first Jacobi sweep there when that level is Jacobi-smoothed)
+  void down(const FastLevel &L, LevelWork &LW, int l, const V *x, int32_t zero_rows) {
+    const PcgState &Pc = S.pcg;
+    const FastLevel &C = *Pc.lev[l + 1];
+    LevelWork &CW = W.lev[l + 1];
+    if (L.kind == 1)
+      k::residual_fast<V>(s, L.n, L.ptr.p, L.col.p, P::ac(L), P::anb(L), P::b(LW), x, P::r(LW));
+    else
+      k::residual_sell<V>(s, L.n, zero_rows, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW),
+                          x, P::r(LW));
+    if (C.kind == 3 && Pc.cfg.pre_sweep > 0)
+      k::restrict_jacobi<V>(s, C.n, L.csoff.p, L.cschild.p, P::r(LW), P::ac(C), P::b(CW),
+                            P::x(CW), V(2.0 / 3.0));
+    else
+      k::restrict_fast<V>(s, C.n, L.csoff.p, L.cschild.p, P::r(LW), P::b(CW));
+  }
+
+  // One V-cycle from level l down; returns the level's solution vector.
+  // first_done (finest level only): x already holds the first colour pass.
+  V *vcycle(int l, bool first_done = false) {
+    const PcgState &Pc = S.pcg;
+    const FastLevel &L = *Pc.lev[l];
+    LevelWork &LW = W.lev[l];
+    const int pre = Pc.cfg.pre_sweep, post = Pc.cfg.post_sweep;
+    if (l == Pc.nlev - 1) {
+      k::dense_solve<V>(s, L.n, Pc.ainv.p, P::b(LW), P::x(LW));
+      return P::x(LW);
+    }
+    V *x = P::x(LW);
+    const V omega = V(2.0 / 3.0);
+    if (L.kind == 3) {
+      // x already holds one Jacobi sweep from zero (fused into the
+      // restriction that produced b); further sweeps ping-pong x and y
+      V *y = P::y(LW);
+      for (int it = 1; it < pre; ++it) {
+        k::jacobi_sell<V>(s, L.n, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW), x, nullptr,
+                          nullptr, y, omega);
+        std::swap(x, y);
+      }
+      if (pre == 0) k::fill<V>(s, L.n, x, V(0));
+      down(L, LW, l, x, 0);
+      V *xc = vcycle(l + 1);
+      // small levels are latency-bound: the first post-sweep adds the coarse
+      // correction on the fly instead of a separate prolongation launch
+      const bool fuse = post > 0 && L.n <= kFusedProlongRows;
+      if (!fuse) k::prolong_fast<V>(s, 0, L.n, L.parent.p, xc, x);
+      for (int it = 0; it < post; ++it) {
+        const bool pro = fuse && it == 0;
+        k::jacobi_sell<V>(s, L.n, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW), x,
+                          pro ? L.parent.p : nullptr, pro ? xc : nullptr, y, omega);
+        std::swap(x, y);
+      }
+      return x;
+    }
+    if (L.kind == 4) {
+      const RefLevel &R = *S.lev[l];
+      if (pre == 0) k::fill<V>(s, L.n, x, V(0));
+      for (int it = 0; it < pre; ++it) dilu_sweep(R, L, LW, x, it == 0);
+      down(L, LW, l, x, 0);
+      V *xc = vcycle(l + 1);
+      k::prolong_fast<V>(s, 0, L.n, L.parent.p, xc, x);
+      for (int it = 0; it < post; ++it) dilu_sweep(R, L, LW, x, false);
+      return x;
+    }
+    if (pre == 0) {
+      k::fill<V>(s, L.n, x, V(0));
+    } else if (L.kind == 0) {
+      mcgs_sweeps(L, LW, pre, true, first_done);
+    } else {
+      k::bmcgs<V>(s, L.nblk, L.bptr.p, L.boff.p, L.bncol, L.ptr.p, L.col.p, P::ac(L), P::anb(L),
+                  P::b(LW), x, nullptr, nullptr, x, 0, pre);
+    }
+    // after a symmetric Gauss-Seidel sweep the rows of colour 0 (solved
+    // last) have a residual of rounding noise only: it is taken as 0
+    down(L, LW, l, x, (L.kind == 0 && pre > 0) ? L.col_off[1] : 0);
+    V *xc = vcycle(l + 1);
+    if (L.kind == 1 && post > 0) {
+      V *y = P::y(LW);
+      k::bmcgs<V>(s, L.nblk, L.bptr.p, L.boff.p, L.bncol, L.ptr.p, L.col.p, P::ac(L), P::anb(L),
+                  P::b(LW), x, L.parent.p, xc, y, 2, post);
+      return y;
+    }
+    // the post-sweep's first pass recomputes colour 0 from its neighbours
+    // alone, so only the other colours need the coarse correction
+    k::prolong_fast<V>(s, (L.kind == 0 && post > 0) ? L.col_off[1] : 0, L.n, L.parent.p, xc, x);
+    if (post > 0) mcgs_sweeps(L, LW, post, false, false);
+    return x;
+  }
+};
+
+// --------------------------------------------------------------- workspaces
+
+size_t work_bytes(const System &S) {
+  const PcgState &P = S.pcg;
+  const size_t w = P.cfg.precision == 32 ? 4 : 8;
+  size_t bytes = 0;
+  for (int32_t l = 0; l < P.nlev; ++l) bytes += 4 * w * (size_t)P.lev[l]->n;
+  const size_t n0 = S.lev[0]->n;
+  bytes += 6 * 8 * n0;                                         // x r p w bref xref
+  bytes += 8 * (5 * (size_t)k::nblocks((int64_t)n0) + 16);     // partials
+  return bytes + 8 * kScal;
+}
+
+template <typename V>
+static void alloc_levels(const System &S, SolveWork &W) {
+  const PcgState &P = S.pcg;
+  W.lev.clear();
+  W.lev.resize(P.nlev);
+  for (int32_t l = 0; l < P.nlev; ++l)
+    for (int i = 0; i < 4; ++i) {
+      // iterates start finite: the sliced-ELL padding multiplies x[row] by 0
+      DBuf<V> &d = Prec<V>::buf(W.lev[l], i);
+      d.alloc(P.lev[l]->n);
+      UFVM_HIP(hipMemset(d.p, 0, d.n * sizeof(V)));
+    }
+}
+
+std::unique_ptr<SolveWork> make_work(const System &S) {
+  if (!S.pcg.ready) throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first");
+  std::unique_ptr<SolveWork> W(new SolveWork());
+  const int32_t n0 = S.lev[0]->n;
+  W->n0 = n0;
+  if (S.pcg.cfg.precision == 32) alloc_levels<float>(S, *W);
+  else alloc_levels<double>(S, *W);
+  for (DBuf<double> *v : {&W->x, &W->r, &W->p, &W->w, &W->bref, &W->xref}) {
+    v->alloc(n0);
+    UFVM_HIP(hipMemset(v->p, 0, n0 * sizeof(double)));
+  }
+  W->partial.alloc(5 * (size_t)k::nblocks((int64_t)n0) + 16);
+  W->scal.alloc(kScal);
+  UFVM_HIP(hipMemset(W->scal.p, 0, kScal * sizeof(double)));
+  UFVM_HIP(hipHostMalloc((void **)&W->hring, 8 * sizeof(double),
+                         hipHostMallocMapped | hipHostMallocCoherent));
+  UFVM_HIP(hipHostGetDevicePointer((void **)&W->dring, W->hring, 0));
+  for (int i = 0; i < 8; ++i) W->hring[i] = 0.0;
+  for (hipEvent_t &e : W->ev) UFVM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  W->bytes = work_bytes(S);
+  return W;
+}
+
+// -------------------------------------------------------------------- set-up
+
+void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
+  if (!S.hier_valid) throw Error(UFVM_ERR_STATE, "no AMG hierarchy: call ufvm_amg_agglomerate");
+  if (cfg.precision != 32 && cfg.precision != 64) throw Error(UFVM_ERR_ARG, "precision must be 32 or 64");
+  if (cfg.max_coarse_cells < 1 || cfg.max_coarse_cells > 4096)
+    throw Error(UFVM_ERR_ARG, "max_coarse_cells must be in [1, 4096]");
+  PcgState &P = S.pcg;
+  P.cfg = cfg;
+  int32_t nlev = S.nlevels;
+  for (int32_t l = 0; l < S.nlevels; ++l)
+    if (S.lev[l]->n <= cfg.max_coarse_cells) { nlev = l + 1; break; }
+  if (S.lev[nlev - 1]->n > 4096)
+    throw Error(UFVM_ERR_UNSUPPORTED, "coarsest level too large for the dense solve");
+  P.nlev = nlev;
+  // constant nullspace: all row sums vanish (pure-Neumann pressure correction)
+  RefLevel &R0 = *S.lev[0];
+  if (cfg.nullspace < 0) {
+    std::vector<double> ac(R0.n), anb(std::max(R0.nnz, 1));
+    R0.ac.download(ac.data(), R0.n, ctx->stream);
+    R0.anb.download(anb.data(), R0.nnz, ctx->stream);
+    UFVM_HIP(hipStreamSynchronize(ctx->stream));
+    bool sing = true;
+    for (int32_t i = 0; i < R0.n && sing; ++i) {
+      double rs = ac[i], mag = std::fabs(ac[i]);
+      for (int32_t kk = R0.g.ptr[i]; kk < R0.g.ptr[i + 1]; ++kk) {
+        rs += anb[kk];
+        mag += std::fabs(anb[kk]);
+      }
+      if (std::fabs(rs) > 1e-9 * mag) sing = false;
+    }
+    P.singular = sing ? 1 : 0;
+  } else {
+    P.singular = cfg.nullspace ? 1 : 0;
+  }
+  P.lev.clear();
+  P.lev.resize(nlev);
+  if (cfg.coarse_smoother != 0 && cfg.coarse_smoother != 1)
+    throw Error(UFVM_ERR_ARG, "coarse_smoother must be 0 or 1");
+  if (cfg.smoother != 0 && cfg.smoother != 1) throw Error(UFVM_ERR_ARG, "smoother must be 0 or 1");
+  for (int32_t l = 0; l < nlev; ++l)
+    build_fast_level(ctx, S, l,
+                     l == nlev - 1 ? 2
+                                   : (l == 0 ? (cfg.smoother == 1 ? 4 : 0)
+                                             : (cfg.coarse_smoother == 1 ? 3 : 1)));
+  for (int32_t l = 0; l + 1 < nlev; ++l) link_levels(ctx, S, l);
+  // dense inverse of the coarsest level (+ mean(diag)/n * 11^T when singular)
+  RefLevel &RC = *S.lev[nlev - 1];
+  const int32_t nc = RC.n;
+  double shift = 0.0;
+  if (P.singular) {
+    std::vector<double> ac(nc);
+    RC.ac.download(ac.data(), nc, ctx->stream);
+    UFVM_HIP(hipStreamSynchronize(ctx->stream));
+    for (double v : ac) shift += v;
+    shift /= (double)nc * nc;
+  }
+  P.ainv.alloc((size_t)nc * nc);
+  P.colbuf.alloc(nc);
+  // dense matrix in the multicolour numbering of the coarsest level (identity
+  // permutation there, since it has a single colour class)
+  k::csr_to_dense(ctx->stream, nc, RC.ptr.p, RC.col.p, RC.ac.p, RC.anb.p, shift, P.ainv.p);
+  k::gj_inverse(ctx->stream, nc, P.ainv.p, P.colbuf.p);
+  UFVM_HIP(hipStreamSynchronize(ctx->stream));
+  P.ready = true;
+  P.work0 = make_work(S);
+}
+
+
+// ---------------------------------------------------------------- FCG solve
+
+bool FcgRun::ready(hipEvent_t e, bool block) {
+  if (block) {
+    UFVM_HIP(hipEventSynchronize(e));
+    return true;
+  }
+  const hipError_t q = hipEventQuery(e);
+  if (q == hipSuccess) return true;
+  if (q == hipErrorNotReady) return false;
+  UFVM_HIP(q);
+  return false;
+}
+
+template <typename V>
+void FcgRun::start_t(const double *b) {
+  const PcgState &P = S_->pcg;
+  const FastLevel &L0 = *P.lev[0];
+  SolveWork &W = *W_;
+  const int32_t n = W.n0;
+  double *scal = W.scal.p;
+  // r = P b (V-cycle numbering), projected when singular; |r|^2 to the ring
+  k::permute_gather(s_, n, L0.perm_d.p, b, W.r.p);
+  if (P.singular) {
+    k::sum_partial(s_, n, W.r.p, W.partial.p);
+    k::reduce_final(s_, W.partial.p, k::nblocks(n), scal + 12);
+    k::axpy_const(s_, n, scal + 12, -1.0 / (double)n, W.r.p);
+  }
+  k::fill<double>(s_, n, W.x.p, 0.0);
+  k::fill<double>(s_, n, W.w.p, 0.0);
+  k::dot_partial(s_, n, W.r.p, W.r.p, W.partial.p);
+  k::reduce_final(s_, W.partial.p, k::nblocks(n), scal + 10);
+  k::convert<double, V>(s_, n, W.r.p, Prec<V>::b(W.lev[0]));
+  k::ring_put(s_, scal + 10, W.dring + 4);
+  UFVM_HIP(hipEventRecord(W.ev[2], s_));
+}
+
+void FcgRun::start(const double *b, double *x) {
+  if (!S_->pcg.ready) throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first");
+  if (!S_->pcg.lev[0]->soff.p) throw Error(UFVM_ERR_STATE, "level 0 has no sliced-ELL copy");
+  x_out_ = x;
+  it_ = issued_ = 0;
+  ahead_ = true;
+  prev2_ = -1.0;
+  nb_ = thr_ = rr_ = 0.0;
+  if (S_->pcg.cfg.precision == 32) start_t<float>(b);
+  else start_t<double>(b);
+  phase_ = 1;
+}
+
+// One FCG(1) iteration k: z = M^-1 r (V-cycle), dots, p update, w = A p,
+// x / r update.  The last kernel writes {r.r, k} to ring slot k & 1 and sets
+// the device convergence flag, after which a queued iteration changes nothing.
+template <typename V>
+void FcgRun::issue_t(int k) {
+  const PcgState &P = S_->pcg;
+  const FastLevel &L0 = *P.lev[0];
+  SolveWork &W = *W_;
+  const int32_t n = W.n0;
+  double *scal = W.scal.p;
+  // iterations after the first find their first colour pass made by the
+  // previous iteration's x/r update (multicolour pre-smoothing only)
+  const bool fuse0 = L0.kind == 0 && P.cfg.pre_sweep > 0;
+  Cycle<V> cyc{ctx_, *S_, W, s_};
+  const V *z = cyc.vcycle(0, fuse0 && k > 0);
+  k::fcg_dots<V>(s_, n, z, W.w.p, W.r.p, W.partial.p);
+  k::fcg_update_p<V>(s_, n, z, W.partial.p, P.singular, k == 0, scal, W.p.p);
+  {
+    ProfScope ps(ctx_, s_, "spmv0",
+                 ctx_->prof_on ? 4.0 * (n / 64 + 1) + 12.0 * L0.nnz + 8.0 * 3 * n : 0.0);
+    k::spmv_dot(s_, n, L0.soff.p, L0.scol.p, L0.ac64.p, L0.sval64.p, W.p.p, W.w.p, W.partial.p);
+  }
+  k::fcg_update_xr<V>(s_, n, scal, W.p.p, W.w.p, W.x.p, W.r.p, Prec<V>::b(W.lev[0]), W.partial.p,
+                      thr_, W.dring + 2 * (k & 1), k, fuse0 ? L0.col_off[1] : 0,
+                      Prec<V>::ac(L0), Prec<V>::x(W.lev[0]));
+  UFVM_HIP(hipEventRecord(W.ev[k & 1], s_));
+}
+
+void FcgRun::issue(int k) {
+  if (S_->pcg.cfg.precision == 32) issue_t<float>(k);
+  else issue_t<double>(k);
+}
+
+void FcgRun::finish() {
+  k::permute_scatter(s_, W_->n0, S_->pcg.lev[0]->perm_d.p, W_->x.p, x_out_);
+  UFVM_HIP(hipEventRecord(W_->ev[2], s_));
+  phase_ = 3;
+}
+
+bool FcgRun::poll(bool block) {
+  const ufvm_pcg_cfg &cfg = S_->pcg.cfg;
+  for (;;) {
+    if (phase_ == 4) return true;
+    if (phase_ == 3) {
+      if (!ready(W_->ev[2], block)) return false;
+      phase_ = 4;
+      continue;
+    }
+    if (phase_ == 1) {
+      if (!ready(W_->ev[2], block)) return false;
+      nb_ = std::sqrt(W_->hring[4]);
+      if (!(nb_ > 0.0) || cfg.max_iter <= 0) {
+        if (std::isnan(nb_)) throw Error(UFVM_ERR_NUMERIC, "NaN right-hand side");
+        finish();
+        continue;
+      }
+      thr_ = (cfg.tol * nb_) * (cfg.tol * nb_);
+      rr_ = nb_;
+      k::fill<double>(s_, 1, W_->scal.p + 11, 0.0);
+      phase_ = 2;
+    }
+    // phase 2: keep one iteration queued beyond the one awaited unless the
+    // residual history predicts that the awaited one converges
+    while (issued_ < cfg.max_iter && issued_ < it_ + (ahead_ ? 2 : 1)) issue(issued_++);
+    if (!ready(W_->ev[it_ & 1], block)) return false;
+    const volatile double *slot = W_->hring + 2 * (it_ & 1);
+    if (slot[1] != (double)it_) throw Error(UFVM_ERR_STATE, "FPCG residual ring out of step");
+    const double rr2 = slot[0];   // the value the device compared with thr
+    rr_ = std::sqrt(rr2);
+    ++it_;
+    if (std::isnan(rr_)) throw Error(UFVM_ERR_NUMERIC, "FPCG breakdown (NaN residual)");
+    if (rr2 <= thr_ || it_ >= cfg.max_iter) {
+      finish();
+      continue;
+    }
+    // at most iteration it_ is in flight: predict its residual from the last
+    // reduction factor
+    if (prev2_ > 0.0) ahead_ = rr2 * (rr2 / prev2_) > 4.0 * thr_;
+    prev2_ = rr2;
+  }
+}
+
+// ufvm_pcg_solve: the level-1 system's bc in, dphi out, on the context stream
+void pcg_solve(ufvm_ctx *ctx, System &S, int32_t *iters, double *relres) {
+  if (!S.pcg.ready || !S.pcg.work0) throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first");
+  RefLevel &R0 = *S.lev[0];
+  FcgRun run(ctx, S, *S.pcg.work0, ctx->stream);
+  run.start(R0.bc.p, R0.dphi.p);
+  while (!run.poll(true)) {
+  }
+  if (iters) *iters = run.iterations();
+  if (relres) *relres = run.relres();
+}
+
+template <typename V>
+static void apply_vcycle_t(ufvm_ctx *ctx, System &S, SolveWork &W) {
+  hipStream_t s = ctx->stream;
+  const int32_t n = W.n0;
+  const FastLevel &L0 = *S.pcg.lev[0];
+  k::permute_gather(s, n, L0.perm_d.p, W.bref.p, W.p.p);
+  k::convert<double, V>(s, n, W.p.p, Prec<V>::b(W.lev[0]));
+  Cycle<V> cyc{ctx, S, W, s};
+  const V *z = cyc.vcycle(0);
+  k::convert<V, double>(s, n, z, W.p.p);
+  k::permute_scatter(s, n, L0.perm_d.p, W.p.p, W.xref.p);
+}
+
+void pcg_apply_vcycle(ufvm_ctx *ctx, System &S, const double *r, double *z) {
+  if (!S.pcg.ready || !S.pcg.work0) throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first");
+  if (!r || !z) throw Error(UFVM_ERR_ARG, "null vector");
+  SolveWork &W = *S.pcg.work0;
+  hipStream_t s = ctx->stream;
+  const int32_t n = W.n0;
+  UFVM_HIP(hipMemcpyAsync(W.bref.p, r, n * sizeof(double), hipMemcpyHostToDevice, s));
+  if (S.pcg.cfg.precision == 32) apply_vcycle_t<float>(ctx, S, W);
+  else apply_vcycle_t<double>(ctx, S, W);
+  UFVM_HIP(hipMemcpyAsync(z, W.xref.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+  UFVM_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace detail
+}  // namespace ufvm

@@ -1,0 +1,490 @@
+// server.hip -- serving form: a single-GPU request loop for pressure-Poisson
+// solves and SIMPLE steps on resident meshes.
+//
+// Pressure solves are continuously batched: a worker thread keeps up to
+// max_active solves on the GPU, each on its own workspace and one of
+// n_streams streams, and advances every active solve one FCG iteration at a
+// time as the GPU finishes it (FcgRun state machines, polled without
+// blocking).  A request that arrives while others run is admitted as soon as
+// a slot is free and the device has memory for its workspace (workspaces of
+// finished solves are kept per system and reused).  SIMPLE steps run on a
+// second thread with its own stream, overlapping the solves on the device.
+// Each solve's arithmetic depends only on its system and right-hand side, so
+// a result is the same alone and in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+
+#include "internal.h"
+#include "pcg.h"
+#include "ufvm.h"
+
+namespace ufvm {
+namespace detail {
+
+static double now_s() {
+  using namespace std::chrono;
+  return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+struct Request {
+  int64_t ticket = 0;
+  int type = 0;                 // 0 pressure solve, 1 SIMPLE steps
+  int32_t id = 0;               // system or case
+  const double *b = nullptr;    // solve: right-hand side (host)
+  double *x = nullptr;          // solve: solution (host)
+  int32_t nsteps = 0;           // steps
+  double *rms = nullptr;        // steps: 4 RMS residuals per step (host, may be null)
+  int status = 0;               // 0 queued, 1 running, 2 done, < 0 error
+  std::string err;
+  int32_t iterations = 0;
+  double relres = 0.0;
+  double t_submit = 0.0, t_start = 0.0, t_done = 0.0;
+};
+
+struct Slot {
+  Request *req = nullptr;
+  System *sys = nullptr;
+  hipStream_t stream = nullptr;
+  std::unique_ptr<SolveWork> work;
+  std::unique_ptr<FcgRun> run;
+  hipEvent_t copied = nullptr;
+  int phase = 0;   // 1 solving, 2 result copy queued
+};
+
+}  // namespace detail
+}  // namespace ufvm
+
+using namespace ufvm;
+using namespace ufvm::detail;
+
+struct ufvm_server {
+  int device = 0;
+  ufvm_server_cfg cfg{};
+  ufvm_ctx *ctx_p = nullptr;   // pressure systems being served
+  ufvm_ctx *ctx_b = nullptr;   // builds new systems (then handed to ctx_p)
+  ufvm_ctx *ctx_s = nullptr;   // SIMPLE cases
+  std::vector<hipStream_t> streams;
+  std::vector<Slot> slots;
+  std::thread worker, stepper;
+  std::mutex mu;                       // requests, queues, pools, stop flag
+  std::condition_variable cv_work, cv_done;
+  std::mutex sys_mu;                   // ctx_p's system map (held only for lookups/inserts)
+  std::mutex build_mu;                 // ctx_b
+  std::mutex case_mu;                  // ctx_s (cases and their steps)
+  bool stop = false;
+  std::deque<Request *> pend_solve, pend_step;
+  std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
+  std::map<int32_t, std::vector<std::unique_ptr<SolveWork>>> pool;   // free workspaces
+  int64_t next_ticket = 1;
+  int32_t n_active = 0;
+  size_t work_bytes_held = 0;
+  std::string err;
+};
+
+namespace {
+
+template <typename F>
+int sguard(ufvm_server *srv, F &&f) {
+  if (!srv) return UFVM_ERR_ARG;
+  try {
+    UFVM_HIP(hipSetDevice(srv->device));
+    f();
+    return UFVM_OK;
+  } catch (const Error &e) {
+    std::lock_guard<std::mutex> g(srv->mu);
+    srv->err = e.what();
+    return e.code;
+  } catch (const std::exception &e) {
+    std::lock_guard<std::mutex> g(srv->mu);
+    srv->err = e.what();
+    return UFVM_ERR_ARG;
+  }
+}
+
+void check_rc(ufvm_ctx *ctx, int rc) {
+  if (rc != UFVM_OK) throw Error(rc, ufvm_last_error(ctx));
+}
+
+void complete(ufvm_server *srv, Request *r, int status, const std::string &msg) {
+  std::lock_guard<std::mutex> g(srv->mu);
+  r->status = status;
+  r->err = msg;
+  r->t_done = now_s();
+  srv->cv_done.notify_all();
+}
+
+// device memory that admission keeps free
+bool fits(ufvm_server *srv, size_t bytes) {
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return false;
+  const double reserve = srv->cfg.hbm_reserve * (double)total_b;
+  return (double)free_b - (double)bytes > reserve;
+}
+
+// Admit queued solves into free slots (worker thread).
+void admit(ufvm_server *srv) {
+  for (Slot &sl : srv->slots) {
+    if (sl.req) continue;
+    Request *r = nullptr;
+    std::unique_ptr<SolveWork> W;
+    System *S = nullptr;
+    {
+      std::lock_guard<std::mutex> g(srv->mu);
+      if (srv->pend_solve.empty()) return;
+      r = srv->pend_solve.front();
+      {
+        std::lock_guard<std::mutex> gs(srv->sys_mu);
+        auto it = srv->ctx_p->systems.find(r->id);
+        S = it == srv->ctx_p->systems.end() ? nullptr : it->second.get();
+      }
+      if (S) {
+        auto &fl = srv->pool[r->id];
+        if (!fl.empty()) {
+          W = std::move(fl.back());
+          fl.pop_back();
+        } else if (!fits(srv, work_bytes(*S))) {
+          return;   // admission by free memory: wait for running solves to finish
+        }
+      }
+      srv->pend_solve.pop_front();
+      r->status = 1;
+      r->t_start = now_s();
+    }
+    if (!S) {
+      complete(srv, r, -UFVM_ERR_ARG, "unknown system");
+      continue;
+    }
+    try {
+      if (!W) {
+        W = make_work(*S);
+        std::lock_guard<std::mutex> g(srv->mu);
+        srv->work_bytes_held += W->bytes;
+      }
+      UFVM_HIP(hipMemcpyAsync(W->bref.p, r->b, (size_t)W->n0 * sizeof(double),
+                              hipMemcpyHostToDevice, sl.stream));
+      sl.run.reset(new FcgRun(srv->ctx_p, *S, *W, sl.stream));
+      sl.run->start(W->bref.p, W->xref.p);
+      sl.req = r;
+      sl.sys = S;
+      sl.work = std::move(W);
+      sl.phase = 1;
+      std::lock_guard<std::mutex> g(srv->mu);
+      ++srv->n_active;
+    } catch (const Error &e) {
+      (void)hipStreamSynchronize(sl.stream);
+      complete(srv, r, -e.code, e.what());
+    }
+  }
+}
+
+// Advance every active solve without blocking; true if anything moved.
+bool progress(ufvm_server *srv) {
+  bool moved = false;
+  for (Slot &sl : srv->slots) {
+    if (!sl.req) continue;
+    Request *r = sl.req;
+    try {
+      if (sl.phase == 1 && sl.run->poll(false)) {
+        UFVM_HIP(hipMemcpyAsync(r->x, sl.work->xref.p, (size_t)sl.work->n0 * sizeof(double),
+                                hipMemcpyDeviceToHost, sl.stream));
+        UFVM_HIP(hipEventRecord(sl.copied, sl.stream));
+        sl.phase = 2;
+        moved = true;
+      }
+      if (sl.phase == 2) {
+        const hipError_t q = hipEventQuery(sl.copied);
+        if (q == hipErrorNotReady) continue;
+        UFVM_HIP(q);
+        {
+          std::lock_guard<std::mutex> g(srv->mu);
+          r->iterations = sl.run->iterations();
+          r->relres = sl.run->relres();
+          srv->pool[r->id].push_back(std::move(sl.work));
+          --srv->n_active;
+        }
+        complete(srv, r, 2, "");
+        sl.req = nullptr;
+        sl.run.reset();
+        sl.phase = 0;
+        moved = true;
+      }
+    } catch (const Error &e) {
+      (void)hipStreamSynchronize(sl.stream);
+      {
+        std::lock_guard<std::mutex> g(srv->mu);
+        if (sl.work) srv->pool[r->id].push_back(std::move(sl.work));
+        --srv->n_active;
+      }
+      complete(srv, r, -e.code, e.what());
+      sl.req = nullptr;
+      sl.run.reset();
+      sl.phase = 0;
+      moved = true;
+    }
+  }
+  return moved;
+}
+
+void worker_main(ufvm_server *srv) {
+  (void)hipSetDevice(srv->device);
+  for (;;) {
+    admit(srv);
+    const bool moved = progress(srv);
+    bool active = false;
+    for (Slot &sl : srv->slots) active = active || sl.req != nullptr;
+    std::unique_lock<std::mutex> g(srv->mu);
+    if (!active && srv->pend_solve.empty()) {
+      if (srv->stop) return;
+      srv->cv_work.wait(g, [&] { return srv->stop || !srv->pend_solve.empty(); });
+      continue;
+    }
+    g.unlock();
+    // the GPU is busy with every active solve's queued iteration: yield briefly
+    if (!moved) std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+}
+
+void stepper_main(ufvm_server *srv) {
+  (void)hipSetDevice(srv->device);
+  for (;;) {
+    Request *r = nullptr;
+    {
+      std::unique_lock<std::mutex> g(srv->mu);
+      srv->cv_work.wait(g, [&] { return srv->stop || !srv->pend_step.empty(); });
+      if (srv->pend_step.empty()) return;   // stop requested and nothing left
+      r = srv->pend_step.front();
+      srv->pend_step.pop_front();
+      r->status = 1;
+      r->t_start = now_s();
+    }
+    int rc;
+    {
+      std::lock_guard<std::mutex> g(srv->case_mu);
+      rc = ufvm_simple_iterate(srv->ctx_s, r->id, r->nsteps, r->rms);
+    }
+    complete(srv, r, rc == UFVM_OK ? 2 : -rc, rc == UFVM_OK ? "" : ufvm_last_error(srv->ctx_s));
+  }
+}
+
+Request *find(ufvm_server *srv, int64_t ticket) {
+  auto it = srv->reqs.find(ticket);
+  if (it == srv->reqs.end()) throw Error(UFVM_ERR_ARG, "unknown ticket");
+  return it->second.get();
+}
+
+void fill_info(const Request *r, ufvm_request_info *info) {
+  if (!info) return;
+  info->status = r->status;
+  info->iterations = r->iterations;
+  info->relres = r->relres;
+  info->queued_s = r->t_start > 0 ? r->t_start - r->t_submit : now_s() - r->t_submit;
+  info->run_s = r->t_done > 0 ? r->t_done - r->t_start : 0.0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ufvm_server_create(int32_t device, const ufvm_server_cfg *cfg, ufvm_server **out) {
+  if (!out || !cfg) return UFVM_ERR_ARG;
+  *out = nullptr;
+  if (cfg->max_active < 1 || cfg->n_streams < 1 || cfg->hbm_reserve < 0.0 ||
+      cfg->hbm_reserve >= 1.0)
+    return UFVM_ERR_ARG;
+  std::unique_ptr<ufvm_server> srv(new (std::nothrow) ufvm_server());
+  if (!srv) return UFVM_ERR_ARG;
+  srv->device = device;
+  srv->cfg = *cfg;
+  int rc = ufvm_ctx_create(device, &srv->ctx_p);
+  if (rc == UFVM_OK) rc = ufvm_ctx_create(device, &srv->ctx_b);
+  if (rc == UFVM_OK) rc = ufvm_ctx_create(device, &srv->ctx_s);
+  if (rc != UFVM_OK) {
+    if (srv->ctx_p) ufvm_ctx_destroy(srv->ctx_p);
+    if (srv->ctx_b) ufvm_ctx_destroy(srv->ctx_b);
+    return rc;
+  }
+  if (hipSetDevice(device) != hipSuccess) return UFVM_ERR_HIP;
+  srv->streams.resize(cfg->n_streams);
+  for (hipStream_t &s : srv->streams)
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return UFVM_ERR_HIP;
+  srv->slots.resize(cfg->max_active);
+  for (size_t i = 0; i < srv->slots.size(); ++i) {
+    srv->slots[i].stream = srv->streams[i % srv->streams.size()];
+    if (hipEventCreateWithFlags(&srv->slots[i].copied, hipEventDisableTiming) != hipSuccess)
+      return UFVM_ERR_HIP;
+  }
+  ufvm_server *p = srv.release();
+  p->worker = std::thread(worker_main, p);
+  p->stepper = std::thread(stepper_main, p);
+  *out = p;
+  return UFVM_OK;
+}
+
+int ufvm_server_destroy(ufvm_server *srv) {
+  if (!srv) return UFVM_ERR_ARG;
+  {
+    std::lock_guard<std::mutex> g(srv->mu);
+    srv->stop = true;
+  }
+  srv->cv_work.notify_all();
+  if (srv->worker.joinable()) srv->worker.join();
+  if (srv->stepper.joinable()) srv->stepper.join();
+  (void)hipSetDevice(srv->device);
+  for (Slot &sl : srv->slots) {
+    sl.run.reset();
+    sl.work.reset();
+    if (sl.copied) (void)hipEventDestroy(sl.copied);
+  }
+  for (hipStream_t s : srv->streams)
+    if (s) {
+      (void)hipStreamSynchronize(s);
+      (void)hipStreamDestroy(s);
+    }
+  srv->pool.clear();
+  if (srv->ctx_p) ufvm_ctx_destroy(srv->ctx_p);
+  if (srv->ctx_b) ufvm_ctx_destroy(srv->ctx_b);
+  if (srv->ctx_s) ufvm_ctx_destroy(srv->ctx_s);
+  delete srv;
+  return UFVM_OK;
+}
+
+const char *ufvm_server_last_error(const ufvm_server *srv) {
+  return srv ? srv->err.c_str() : "null server";
+}
+
+int ufvm_server_add_system(ufvm_server *srv, int32_t n, int32_t nnz, const int32_t *ptr,
+                           const int32_t *col, const double *ac, const double *anb,
+                           const ufvm_pcg_cfg *cfg, int32_t *sys_out) {
+  return sguard(srv, [&] {
+    if (!cfg || !sys_out) throw Error(UFVM_ERR_ARG, "null argument");
+    // build on a context of its own, so the solves in flight keep going,
+    // then hand the resident system to the serving context
+    std::lock_guard<std::mutex> gb(srv->build_mu);
+    ufvm_ctx *c = srv->ctx_b;
+    int32_t sid = 0;
+    check_rc(c, ufvm_system_create(c, n, nnz, ptr, col, &sid));
+    int rc = ufvm_system_set_coefficients(c, sid, ac, anb, nullptr);
+    int32_t nl = 0;
+    if (rc == UFVM_OK) rc = ufvm_amg_agglomerate(c, sid, 10, &nl);
+    if (rc == UFVM_OK) rc = ufvm_pcg_setup(c, sid, cfg);
+    if (rc != UFVM_OK) {
+      const std::string msg = ufvm_last_error(c);
+      ufvm_system_destroy(c, sid);
+      throw Error(rc, msg);
+    }
+    std::lock_guard<std::mutex> gs(srv->sys_mu);
+    const int32_t id = srv->ctx_p->next_id++;
+    srv->ctx_p->systems[id] = std::move(c->systems[sid]);
+    c->systems.erase(sid);
+    *sys_out = id;
+  });
+}
+
+int ufvm_server_add_case(ufvm_server *srv, const ufvm_mesh_desc *mesh, const ufvm_simple_cfg *cfg,
+                         int32_t *case_out) {
+  return sguard(srv, [&] {
+    std::lock_guard<std::mutex> g(srv->case_mu);
+    check_rc(srv->ctx_s, ufvm_case_create(srv->ctx_s, mesh, cfg, case_out));
+  });
+}
+
+int ufvm_server_get_field(ufvm_server *srv, int32_t cas, const char *name, double *data,
+                          int64_t count) {
+  return sguard(srv, [&] {
+    std::lock_guard<std::mutex> g(srv->case_mu);
+    check_rc(srv->ctx_s, ufvm_case_get_field(srv->ctx_s, cas, name, data, count));
+  });
+}
+
+int ufvm_server_set_field(ufvm_server *srv, int32_t cas, const char *name, const double *data,
+                          int64_t count) {
+  return sguard(srv, [&] {
+    std::lock_guard<std::mutex> g(srv->case_mu);
+    check_rc(srv->ctx_s, ufvm_case_set_field(srv->ctx_s, cas, name, data, count));
+  });
+}
+
+int ufvm_server_submit_solve(ufvm_server *srv, int32_t sys, const double *b, double *x,
+                             int64_t *ticket) {
+  return sguard(srv, [&] {
+    if (!b || !x || !ticket) throw Error(UFVM_ERR_ARG, "null argument");
+    std::unique_ptr<Request> r(new Request());
+    r->type = 0;
+    r->id = sys;
+    r->b = b;
+    r->x = x;
+    r->t_submit = now_s();
+    std::lock_guard<std::mutex> g(srv->mu);
+    if (srv->stop) throw Error(UFVM_ERR_STATE, "server stopping");
+    r->ticket = srv->next_ticket++;
+    *ticket = r->ticket;
+    srv->pend_solve.push_back(r.get());
+    srv->reqs[r->ticket] = std::move(r);
+    srv->cv_work.notify_all();
+  });
+}
+
+int ufvm_server_submit_step(ufvm_server *srv, int32_t cas, int32_t nsteps, double *rms,
+                            int64_t *ticket) {
+  return sguard(srv, [&] {
+    if (!ticket || nsteps < 1) throw Error(UFVM_ERR_ARG, "bad argument");
+    std::unique_ptr<Request> r(new Request());
+    r->type = 1;
+    r->id = cas;
+    r->nsteps = nsteps;
+    r->rms = rms;
+    r->t_submit = now_s();
+    std::lock_guard<std::mutex> g(srv->mu);
+    if (srv->stop) throw Error(UFVM_ERR_STATE, "server stopping");
+    r->ticket = srv->next_ticket++;
+    *ticket = r->ticket;
+    srv->pend_step.push_back(r.get());
+    srv->reqs[r->ticket] = std::move(r);
+    srv->cv_work.notify_all();
+  });
+}
+
+int ufvm_server_query(ufvm_server *srv, int64_t ticket, ufvm_request_info *info) {
+  return sguard(srv, [&] {
+    std::lock_guard<std::mutex> g(srv->mu);
+    fill_info(find(srv, ticket), info);
+  });
+}
+
+int ufvm_server_wait(ufvm_server *srv, int64_t ticket, ufvm_request_info *info) {
+  return sguard(srv, [&] {
+    std::unique_lock<std::mutex> g(srv->mu);
+    Request *r = find(srv, ticket);
+    srv->cv_done.wait(g, [&] { return r->status == 2 || r->status < 0; });
+    fill_info(r, info);
+  });
+}
+
+int ufvm_server_release(ufvm_server *srv, int64_t ticket) {
+  return sguard(srv, [&] {
+    std::lock_guard<std::mutex> g(srv->mu);
+    Request *r = find(srv, ticket);
+    if (r->status == 0 || r->status == 1) throw Error(UFVM_ERR_STATE, "request not finished");
+    srv->reqs.erase(ticket);
+  });
+}
+
+int ufvm_server_stats(ufvm_server *srv, int32_t *queued, int32_t *active, int64_t *work_bytes) {
+  return sguard(srv, [&] {
+    std::lock_guard<std::mutex> g(srv->mu);
+    if (queued) *queued = (int32_t)(srv->pend_solve.size() + srv->pend_step.size());
+    if (active) *active = srv->n_active;
+    if (work_bytes) *work_bytes = (int64_t)srv->work_bytes_held;
+  });
+}
+
+}  // extern "C"

@@ -34,6 +34,20 @@ struct DBuf {
   DBuf() = default;
   DBuf(const DBuf &) = delete;
   DBuf &operator=(const DBuf &) = delete;
+  DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DBuf &operator=(DBuf &&o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
   ~DBuf() { release(); }
   void release() {
     if (p) (void)hipFree(p);
@@ -98,8 +112,8 @@ struct FastLevel {
   std::vector<int32_t> perm;      // perm[new] = reference index
   std::vector<int32_t> h_ptr;     // permuted CSR row pointer (host copy)
   DBuf<int32_t> ptr, col, perm_d;
-  DBuf<float> ac32, anb32, b32, x32, r32, y32;   // y: second iterate buffer (kind 1)
-  DBuf<double> ac64, anb64, b64, x64, r64, y64;
+  DBuf<float> ac32, anb32;
+  DBuf<double> ac64, anb64;
   DBuf<int32_t> parent;           // coarse row of each row (next level's numbering)
   DBuf<int32_t> csoff, cschild;   // child lists of the next level's rows, sliced-ELL
   // sliced-ELL copy of the off-diagonals (kinds 0 and 3): 64-row chunks, chunk
@@ -113,6 +127,36 @@ struct FastLevel {
   DBuf<double> dinv64;
 };
 
+// Per-solve device workspace of one system: each level's right-hand side,
+// iterate, residual and second iterate (in the V-cycle precision) and the FCG
+// vectors.  Several workspaces of one system run independent solves at once.
+struct LevelWork {
+  DBuf<float> b32, x32, r32, y32;
+  DBuf<double> b64, x64, r64, y64;
+};
+
+struct SolveWork {
+  int32_t n0 = 0;
+  std::vector<LevelWork> lev;
+  DBuf<double> x, r, p, w;       // FCG vectors, finest level in its V-cycle numbering
+  DBuf<double> bref, xref;       // right-hand side / solution in reference numbering
+  DBuf<double> partial;          // per-block partial sums
+  DBuf<double> scal;             // device scalars
+  // pinned host-coherent ring the solve's kernels write {value, tag} pairs
+  // into: slots 0-1 even iterations, 2-3 odd iterations, 4-5 set-up norm
+  double *hring = nullptr, *dring = nullptr;
+  hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  size_t bytes = 0;              // device bytes held
+  SolveWork() = default;
+  SolveWork(const SolveWork &) = delete;
+  SolveWork &operator=(const SolveWork &) = delete;
+  ~SolveWork() {
+    if (hring) (void)hipHostFree(hring);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+  }
+};
+
 struct PcgState {
   bool ready = false;
   ufvm_pcg_cfg cfg{};
@@ -120,8 +164,7 @@ struct PcgState {
   int32_t singular = 0;
   std::vector<std::unique_ptr<FastLevel>> lev;
   DBuf<double> ainv, colbuf;
-  // outer FPCG vectors (level-1, multicolour numbering)
-  DBuf<double> x, r, p, w;
+  std::unique_ptr<SolveWork> work0;   // the workspace of ufvm_pcg_solve
 };
 
 struct System {
@@ -150,10 +193,6 @@ struct ufvm_ctx {
   ufvm::DBuf<double> scal;      // device scalars
   ufvm::DBuf<unsigned> counter; // last-block reduction counter (kept at zero between launches)
   double *hscal = nullptr;      // pinned host mirror of scal
-  // FPCG residual ring: pinned, host-coherent memory the iteration's last
-  // kernel writes {rr, iteration} into; dring is its device address
-  double *hring = nullptr, *dring = nullptr;
-  hipEvent_t iter_ev[2] = {nullptr, nullptr};
   // SIMPLE cases of this context (type-erased; owned here so that destroying
   // the context releases them)
   std::map<int32_t, std::shared_ptr<void>> cases;

@@ -39,44 +39,58 @@ def cavity_bcs(mesh, lid=(1.0, 0.0, 0.0)):
     return out
 
 
+def case_structs(mesh, bcs, nu, urf_u=0.7, urf_p=0.3, n_correctors=2, p_ref=0, u_sweeps=10,
+                 p_solver="amg", p_cycle="W-Cycle", p_max_cycles=30, p_termination=0.1,
+                 p_max_coarse_levels=10, p_pre=1, p_post=3, p_smoother="ILU", pcg_precision=32,
+                 pcg_tol=1e-8, pcg_sweeps=1, pcg_max_coarse_cells=2000):
+    """(MeshDesc, SimpleCfg, keep): the C structs of ufvm_case_create; the arrays
+    they point to live in `keep`, which must outlive the call."""
+    m = mesh
+    keep = []
+
+    def a(x, t):
+        arr = np.ascontiguousarray(x, dtype=np.int32 if t is i32p else np.float64)
+        keep.append(arr)
+        return arr.ctypes.data_as(t)
+
+    d = MeshDesc()
+    d.n_cells, d.n_faces, d.n_interior = m.n_cells, m.n_faces, m.n_interior
+    d.owner, d.neighbour = a(m.owner, i32p), a(m.neighbour, i32p)
+    d.Sf, d.area, d.CN, d.eCN = a(m.Sf, f64p), a(m.area, f64p), a(m.CN, f64p), a(m.eCN, f64p)
+    d.gDiff, d.T, d.gf = a(m.gDiff, f64p), a(m.T, f64p), a(m.gf, f64p)
+    d.face_centroid = a(m.face_centroid, f64p)
+    d.volume, d.centroid = a(m.volume, f64p), a(m.centroid, f64p)
+    d.cconn_ptr, d.cconn = a(m.cconn_ptr, i32p), a(m.cconn, i32p)
+    d.own_nnz, d.nei_nnz = a(m.own_nnz, i32p), a(m.nei_nnz, i32p)
+    d.n_patches = len(m.patches)
+    d.patch_start = a([p.start for p in m.patches], i32p)
+    d.patch_nfaces = a([p.nfaces for p in m.patches], i32p)
+    d.patch_type = a([PATCH[p.type] for p in m.patches], i32p)
+    c = SimpleCfg()
+    c.nu, c.urf_u, c.urf_p = nu, urf_u, urf_p
+    c.n_correctors, c.p_ref_cell, c.u_sweeps = n_correctors, p_ref, u_sweeps
+    c.u_bc = a([BC[bcs[p.name][0]] for p in m.patches], i32p)
+    c.u_value = a(np.concatenate([np.asarray(bcs[p.name][1], float) for p in m.patches]), f64p)
+    c.p_bc = a([BC[bcs[p.name][2]] for p in m.patches], i32p)
+    c.p_solver = P_SOLVER[p_solver]
+    c.p_amg = AmgCfg(_lib.CYCLE[p_cycle], p_max_cycles, p_termination, p_max_coarse_levels,
+                     p_pre, p_post, _lib.SMOOTHER[p_smoother], 1)
+    c.p_pcg = PcgCfg(pcg_precision, pcg_sweeps, pcg_sweeps, 1, pcg_max_coarse_cells, -1, 500,
+                     pcg_tol, 0)
+    return d, c, keep
+
+
 class SimpleCase:
     def __init__(self, ctx: _lib.Context, mesh, bcs, nu, urf_u=0.7, urf_p=0.3, n_correctors=2,
                  p_ref=0, u_sweeps=10, p_solver="amg", p_cycle="W-Cycle", p_max_cycles=30,
                  p_termination=0.1, p_max_coarse_levels=10, p_pre=1, p_post=3, p_smoother="ILU",
                  pcg_precision=32, pcg_tol=1e-8, pcg_sweeps=1, pcg_max_coarse_cells=2000):
         self.ctx = ctx
-        self.mesh = m = mesh
-        keep = self._keep = []
-
-        def a(x, t):
-            arr = np.ascontiguousarray(x, dtype=np.int32 if t is i32p else np.float64)
-            keep.append(arr)
-            return arr.ctypes.data_as(t)
-
-        d = MeshDesc()
-        d.n_cells, d.n_faces, d.n_interior = m.n_cells, m.n_faces, m.n_interior
-        d.owner, d.neighbour = a(m.owner, i32p), a(m.neighbour, i32p)
-        d.Sf, d.area, d.CN, d.eCN = a(m.Sf, f64p), a(m.area, f64p), a(m.CN, f64p), a(m.eCN, f64p)
-        d.gDiff, d.T, d.gf = a(m.gDiff, f64p), a(m.T, f64p), a(m.gf, f64p)
-        d.face_centroid = a(m.face_centroid, f64p)
-        d.volume, d.centroid = a(m.volume, f64p), a(m.centroid, f64p)
-        d.cconn_ptr, d.cconn = a(m.cconn_ptr, i32p), a(m.cconn, i32p)
-        d.own_nnz, d.nei_nnz = a(m.own_nnz, i32p), a(m.nei_nnz, i32p)
-        d.n_patches = len(m.patches)
-        d.patch_start = a([p.start for p in m.patches], i32p)
-        d.patch_nfaces = a([p.nfaces for p in m.patches], i32p)
-        d.patch_type = a([PATCH[p.type] for p in m.patches], i32p)
-        c = SimpleCfg()
-        c.nu, c.urf_u, c.urf_p = nu, urf_u, urf_p
-        c.n_correctors, c.p_ref_cell, c.u_sweeps = n_correctors, p_ref, u_sweeps
-        c.u_bc = a([BC[bcs[p.name][0]] for p in m.patches], i32p)
-        c.u_value = a(np.concatenate([np.asarray(bcs[p.name][1], float) for p in m.patches]), f64p)
-        c.p_bc = a([BC[bcs[p.name][2]] for p in m.patches], i32p)
-        c.p_solver = P_SOLVER[p_solver]
-        c.p_amg = AmgCfg(_lib.CYCLE[p_cycle], p_max_cycles, p_termination, p_max_coarse_levels,
-                         p_pre, p_post, _lib.SMOOTHER[p_smoother], 1)
-        c.p_pcg = PcgCfg(pcg_precision, pcg_sweeps, pcg_sweeps, 1, pcg_max_coarse_cells, -1, 500,
-                         pcg_tol, 0)
+        self.mesh = mesh
+        d, c, self._keep = case_structs(
+            mesh, bcs, nu, urf_u, urf_p, n_correctors, p_ref, u_sweeps, p_solver, p_cycle,
+            p_max_cycles, p_termination, p_max_coarse_levels, p_pre, p_post, p_smoother,
+            pcg_precision, pcg_tol, pcg_sweeps, pcg_max_coarse_cells)
         cid = C.c_int32()
         ctx.check(ctx.lib.ufvm_case_create(ctx.h, C.byref(d), C.byref(c), C.byref(cid)))
         self.id = cid.value
