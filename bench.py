@@ -100,6 +100,114 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n, "copies": 5}), flush=True)
 
 
+def serving_bench(args):
+    """Serving form: a Poisson-arrival stream of pressure solves and single
+    SIMPLE steps over a pool of resident jittered hex meshes (2-D and 3-D,
+    cell counts 50k..4M), FP32-preconditioned AMG-PCG at 1e-8, hierarchies
+    cached; reports completed requests/s and p50/p99 latency (submit ->
+    result on the host)."""
+    from matlab_ufvm_amd.serving import Server, request_stream, run_stream
+    from matlab_ufvm_amd.simple import cavity_bcs
+    from matlab_ufvm_amd.workloads import cavity_mesh, gaussian_rhs, poisson_system
+    pool = []
+    t = time.time()
+    for spec in args.serving_meshes.split(","):
+        dim, n = spec.split(":")
+        n = int(n)
+        m = cavity_mesh(n, nz=1 if dim == "2d" else None, jitter=0.15, seed=2026 + n,
+                        renumber_seed=2027 + n, apex="own")
+        pool.append((spec, m))
+    log(f"serving pool: {[(s_, m.n_cells) for s_, m in pool]} ({time.time()-t:.1f}s)")
+    srv = Server(max_active=args.serving_active, n_streams=args.serving_streams)
+    t = time.time()
+    sids, cids = [], []
+    for spec, m in pool:
+        ptr, col, ac, anb = poisson_system(m, dirichlet=())
+        sids.append(srv.add_system(ptr, col, ac, anb, tol=1e-8))
+        cids.append(srv.add_case(m, cavity_bcs(m), nu=1.0 / 400.0, p_cycle="V-Cycle",
+                                 p_pre=2, p_post=2, p_smoother="SOR"))
+    log(f"serving: systems and cases resident ({time.time()-t:.1f}s)")
+    sizes = [m.n_cells for _, m in pool]
+    # one right-hand side per mesh, made before the stream starts
+    rhs = [gaussian_rhs(m, seed=2030 + i) for i, (_, m) in enumerate(pool)]
+    # warm-up: one solve and one step per mesh (first-use set-up of the frozen
+    # hierarchies and workspaces), not timed
+    for i in range(len(pool)):
+        srv.wait(srv.submit_solve(sids[i], rhs[i]))
+        srv.wait(srv.submit_step(cids[i], 1))
+    reqs = request_stream(args.serving_requests, args.serving_rate, sizes, seed=2030)
+    wall = run_stream(srv, reqs, sids, cids, lambda r: rhs[r.mesh])
+    srv.close()
+    lat = np.array([r.latency for r in reqs])
+    out = {"metric": "serving: completed requests/s and p50/p99 latency (submit -> result)",
+           "value": round(len(reqs) / wall, 3), "unit": "requests/s", "n_gpus": 1,
+           "requests": len(reqs), "arrival_rate": args.serving_rate, "wall_s": round(wall, 3),
+           "latency_ms": {"p50": round(float(np.median(lat)) * 1e3, 2),
+                          "p99": round(float(np.percentile(lat, 99)) * 1e3, 2)},
+           "config": {"workload": "Poisson-arrival stream (seed 2030) of pressure solves "
+                                  "(all-wall Neumann Laplacian, Gaussian RHS) and single SIMPLE "
+                                  "steps (lid-driven cavity, Re 400) on a resident mesh pool",
+                      "meshes": {s_: int(m.n_cells) for s_, m in pool},
+                      "max_active": args.serving_active, "streams": args.serving_streams,
+                      "tol": 1e-8, "precision": "f32 V-cycle, f64 FPCG"},
+           "data": "synthetic"}
+    for kind in ("solve", "step"):
+        l = np.array([r.latency for r in reqs if r.kind == kind])
+        if l.size:
+            out[kind] = {"count": int(l.size), "p50_ms": round(float(np.median(l)) * 1e3, 2),
+                         "p99_ms": round(float(np.percentile(l, 99)) * 1e3, 2)}
+    print(json.dumps(out), flush=True)
+
+
+def simple_steps_bench(args):
+    """Precision form in the SIMPLE loop: the 128^3 lid-driven cavity (Re 400)
+    run for N outer iterations twice, with every pressure-correction solve by
+    the FP32-preconditioned AMG-PCG and by the FP64 one, both to 1e-8 with the
+    hierarchy rebuilt each solve (as the reference does); reports the time per
+    iteration and the per-field relative L2 difference after N iterations."""
+    from matlab_ufvm_amd._lib import Context
+    from matlab_ufvm_amd.simple import SimpleCase, cavity_bcs
+    from matlab_ufvm_amd.workloads import cavity_mesh
+    t = time.time()
+    m = cavity_mesh(args.n, jitter=0.15, seed=2026, renumber_seed=2027, apex="own")
+    log(f"mesh {args.n}^3: cells={m.n_cells} ({time.time()-t:.1f}s)")
+    ctx = Context(0)
+    fields, res = {}, {}
+    for prec in (32, 64):
+        case = SimpleCase(ctx, m, cavity_bcs(m), nu=1.0 / 400.0, p_solver=args.p_solver,
+                          pcg_precision=prec, pcg_tol=1e-8)
+        ctx.synchronize()
+        t = time.perf_counter()
+        rms = case.iterate(args.simple_steps)
+        ctx.synchronize()
+        dt = time.perf_counter() - t
+        fields[prec] = {k: case.field(k) for k in ("U", "p", "mdot_f")}
+        res[prec] = {"s_per_iteration": round(dt / args.simple_steps, 4),
+                     "final_rms": [float(v) for v in rms[-1]]}
+        log(f"fp{prec}-precond SIMPLE: {dt / args.simple_steps * 1e3:.1f} ms/iteration, "
+            f"final rms {np.array2string(rms[-1], precision=3)}")
+        case.close()
+    err = {}
+    for k in ("U", "p", "mdot_f"):
+        a, b = fields[32][k], fields[64][k]
+        err[k] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+    out = {"metric": "SIMPLE iterations/s with FP32-preconditioned pressure solves, and per-field "
+                     "relative L2 difference from the FP64-preconditioned run",
+           "value": round(1.0 / res[32]["s_per_iteration"], 3), "unit": "iterations/s",
+           "n_gpus": 1, "iterations": args.simple_steps,
+           "fp32": res[32], "fp64": res[64], "field_rel_l2_diff": err,
+           "config": {"workload": f"3-D lid-driven cavity {args.n}^3 jittered hex (+-15%, seed "
+                                  f"2026), permuted+RCM (seed 2027), own-face apex, Re 400, "
+                                  f"SIMPLE URF 0.7/0.3, 2 pressure correctors, momentum DILU",
+                      "cells": int(m.n_cells), "pressure_tol": 1e-8,
+                      "hierarchy": ("rebuilt every pressure solve" if args.p_solver == "pcg"
+                                    else "first solve's parents kept, Galerkin operators and "
+                                         "solve-side copies refreshed on the GPU")},
+           "data": "synthetic"}
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
 def timed_solves(ctx, sid, steps, warmup):
     for _ in range(warmup):
         ctx.pcg_solve(sid)
@@ -182,11 +290,29 @@ def main():
                     help="solves re-run with per-kernel HIP events for the roofline")
     ap.add_argument("--plain-cycles", type=int, default=20,
                     help="timed V-cycles of the plain FP64 drop-in AMG (0: skip)")
+    ap.add_argument("--simple-steps", type=int, default=0,
+                    help="run N SIMPLE iterations with FP32 and FP64 pressure solves instead")
+    ap.add_argument("--p-solver", choices=["pcg", "pcg-frozen"], default="pcg",
+                    help="SIMPLE mode: rebuild the hierarchy every solve, or keep the first")
+    ap.add_argument("--serving", action="store_true",
+                    help="run the serving-form stream instead of the solve benchmark")
+    ap.add_argument("--serving-meshes", default="3d:37,3d:64,3d:100,3d:128,2d:224,2d:512,"
+                                                "2d:1024,2d:2048")
+    ap.add_argument("--serving-requests", type=int, default=200)
+    ap.add_argument("--serving-rate", type=float, default=200.0)
+    ap.add_argument("--serving-active", type=int, default=8)
+    ap.add_argument("--serving-streams", type=int, default=4)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run 5 device copies of this many MiB (counter calibration)")
     args = ap.parse_args()
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib)
+        return
+    if args.serving:
+        serving_bench(args)
+        return
+    if args.simple_steps > 0:
+        simple_steps_bench(args)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs on one GPU: the pressure solve of one mesh "

@@ -73,6 +73,13 @@ int ufvm_system_set_coefficients(ufvm_ctx *ctx, int32_t sys, const double *ac,
 /* same, from device memory (e.g. a torch tensor on the context's device) */
 int ufvm_system_set_coefficients_dev(ufvm_ctx *ctx, int32_t sys, const double *ac_dev,
                                      const double *anb_dev, const double *bc_dev);
+/* New level-1 coefficients on the existing hierarchy (no reference
+ * counterpart: the reference re-agglomerates every solve).  Parents and
+ * coarse patterns are kept; the Galerkin coarse operators are recomputed
+ * (assembleAgglomeratedLHS.m:16-30 arithmetic) and, after ufvm_pcg_setup,
+ * the solve-side copies are refreshed on the GPU.  NULL leaves an array. */
+int ufvm_system_update_coefficients(ufvm_ctx *ctx, int32_t sys, const double *ac,
+                                    const double *anb, const double *bc);
 /* Domain.coefficients{level}.dphi read/write (cfdGetCoefficients.m:6-12) */
 int ufvm_system_set_dphi(ufvm_ctx *ctx, int32_t sys, int32_t level, const double *dphi);
 int ufvm_system_get_dphi(ufvm_ctx *ctx, int32_t sys, int32_t level, double *dphi);
@@ -288,6 +295,8 @@ int ufvm_server_submit_solve(ufvm_server *srv, int32_t sys, const double *b, dou
 int ufvm_server_submit_step(ufvm_server *srv, int32_t cas, int32_t nsteps, double *rms,
                             int64_t *ticket);
 int ufvm_server_query(ufvm_server *srv, int64_t ticket, ufvm_request_info *info);
+/* blocks until the request is done or failed; after a failure (status < 0)
+ * ufvm_server_last_error holds its message */
 int ufvm_server_wait(ufvm_server *srv, int64_t ticket, ufvm_request_info *info);
 /* forget a finished request */
 int ufvm_server_release(ufvm_server *srv, int64_t ticket);

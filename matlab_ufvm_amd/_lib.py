@@ -76,6 +76,7 @@ _SIGS = {
     "ufvm_system_set_coefficients": (C.c_int, [C.c_void_p, C.c_int32, f64p, f64p, f64p]),
     "ufvm_system_set_coefficients_dev": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p,
                                                    C.c_void_p, C.c_void_p]),
+    "ufvm_system_update_coefficients": (C.c_int, [C.c_void_p, C.c_int32, f64p, f64p, f64p]),
     "ufvm_system_set_dphi": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f64p]),
     "ufvm_system_get_dphi": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f64p]),
     "ufvm_system_set_bc": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f64p]),
@@ -219,6 +220,13 @@ class Context:
         n, nnz = C.c_int32(), C.c_int32()
         self.check(self.lib.ufvm_amg_level_info(self.h, sid, level, C.byref(n), C.byref(nnz)))
         return n.value, nnz.value
+
+    def update_coefficients(self, sid, ac=None, anb=None, bc=None):
+        """New level-1 values on the existing hierarchy (Galerkin update and,
+        after pcg_setup, a refresh of the solve-side copies)."""
+        arrs = [None if v is None else _f64(v) for v in (ac, anb, bc)]
+        self.check(self.lib.ufvm_system_update_coefficients(
+            self.h, sid, *[None if v is None else _p(v, f64p) for v in arrs]))
 
     def set_dphi(self, sid, level, x):
         x = _f64(x)

@@ -452,6 +452,24 @@ int ufvm_system_set_coefficients_dev(ufvm_ctx *ctx, int32_t sys, const double *a
   return guard(ctx, [&] { set_coeffs(ctx, sys, ac, anb, bc, hipMemcpyDeviceToDevice); });
 }
 
+int ufvm_system_update_coefficients(ufvm_ctx *ctx, int32_t sys, const double *ac,
+                                    const double *anb, const double *bc) {
+  return guard(ctx, [&] {
+    System &S = sys_of(ctx, sys);
+    if (!S.hier_valid || S.nlevels < 2)
+      throw Error(UFVM_ERR_STATE, "no AMG hierarchy: call ufvm_amg_agglomerate");
+    RefLevel &L = *S.lev[0];
+    hipStream_t s = ctx->stream;
+    if (ac) UFVM_HIP(hipMemcpyAsync(L.ac.p, ac, L.n * sizeof(double), hipMemcpyHostToDevice, s));
+    if (anb && L.nnz)
+      UFVM_HIP(hipMemcpyAsync(L.anb.p, anb, L.nnz * sizeof(double), hipMemcpyHostToDevice, s));
+    if (bc) UFVM_HIP(hipMemcpyAsync(L.bc.p, bc, L.n * sizeof(double), hipMemcpyHostToDevice, s));
+    UFVM_HIP(hipMemsetAsync(L.dphi.p, 0, L.n * sizeof(double), s));
+    galerkin_update(ctx, S);
+    if (S.pcg.ready) pcg_refresh(ctx, S);
+  });
+}
+
 int ufvm_system_set_dphi(ufvm_ctx *ctx, int32_t sys, int32_t level, const double *dphi) {
   return guard(ctx, [&] {
     RefLevel &L = level_of(sys_of(ctx, sys), level);

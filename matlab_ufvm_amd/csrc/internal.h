@@ -79,6 +79,7 @@ void solve_alg(ufvm_ctx *ctx, RefLevel &L, int smoother, int n_iter, double rrf,
 void apply_amg(ufvm_ctx *ctx, System &S, const ufvm_amg_cfg &cfg, double *r0, double *r1,
                int32_t *ncycles);
 void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg);
+void pcg_refresh(ufvm_ctx *ctx, System &S);
 void pcg_solve(ufvm_ctx *ctx, System &S, int32_t *iters, double *relres);
 void galerkin_update(ufvm_ctx *ctx, System &S);
 int32_t create_system(ufvm_ctx *ctx, int32_t n, int32_t nnz, const int32_t *ptr,

@@ -102,6 +102,9 @@ void csr_to_dense(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *c
 // ---- FPCG (FP64, level-1 order of the multicolour permutation).  Each call
 // is a streaming kernel writing per-block partial sums plus a one-block kernel
 // reducing them and updating the device scalars scal[] (layout in kernels.hip).
+// out[i] = idx[i] >= 0 ? (V) src[idx[i]] : 0
+template <typename V>
+void gather_vals(hipStream_t s, int64_t n, const int32_t *idx, const double *src, V *out);
 // ring[0] = *src, ring[1] = -1 (a device scalar handed to host-visible memory)
 void ring_put(hipStream_t s, const double *src, double *ring);
 // FCG(1) iteration kernels (scalar slots of scal[] listed in kernels.hip).
@@ -117,14 +120,17 @@ template <typename V>
 void fcg_update_p(hipStream_t s, int32_t n, const V *z, const double *partial, int singular,
                   int first, double *scal, double *p);
 // folds spmv_dot's partials (delta, alpha into scal) ; x += alpha p ;
-// r -= alpha w ; rv = (V) r ; then rr = r.r into scal[10] and the convergence
-// flag scal[11] once rr <= thr (no update at all once it is set); ring:
-// {rr, iter} written to host-visible memory when not null.  Rows i < n0 also
-// get z0[i] = rv[i] / acv[i] (the next V-cycle's first colour pass from zero).
+// r -= alpha w ; rv = (V) r ; then rr = r.r into scal[10], the convergence
+// flag scal[11] once rr <= scal[13] (no update at all once it is set), and
+// {rr, iteration} into ring slot (scal[14] & 1) when ring is not null.  Rows
+// i < n0 also get z0[i] = rv[i] / acv[i] (the next V-cycle's first colour pass
+// from zero).
 template <typename V>
 void fcg_update_xr(hipStream_t s, int32_t n, double *scal, const double *p, const double *w,
-                   double *x, double *r, V *rv, double *partial, double thr, double *ring,
-                   int32_t iter, int32_t n0, const V *acv, V *z0);
+                   double *x, double *r, V *rv, double *partial, double *ring, int32_t n0,
+                   const V *acv, V *z0);
+// arm a solve: scal[11] = 0 (flag), scal[13] = thr, scal[14] = 0 (iteration)
+void fcg_arm(hipStream_t s, double *scal, double thr);
 // reduce nblk partial groups of width `width` into out[0..width)
 void reduce_groups(hipStream_t s, const double *partial, int32_t nblk, int32_t width,
                    double *out);

@@ -847,6 +847,24 @@ k_fcg_update_xr(int32_t n, const double *__restrict__ partial_in, double *__rest
   if (threadIdx.x == 0) partial_out[blockIdx.x] = rr;
 }
 
+template <typename V>
+__global__ void k_gather_vals(int64_t n, const int32_t *__restrict__ idx,
+                              const double *__restrict__ src, V *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int32_t j = idx[i];
+  out[i] = j >= 0 ? (V)src[j] : V(0);
+}
+
+template <typename V>
+void gather_vals(hipStream_t s, int64_t n, const int32_t *idx, const double *src, V *out) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_gather_vals<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, idx, src, out);
+}
+template void gather_vals<float>(hipStream_t, int64_t, const int32_t *, const double *, float *);
+template void gather_vals<double>(hipStream_t, int64_t, const int32_t *, const double *,
+                                  double *);
+
 __global__ void k_ring_put(const double *__restrict__ src, double *ring) {
   if (threadIdx.x == 0) {
     ring[0] = src[0];
@@ -859,36 +877,51 @@ void ring_put(hipStream_t s, const double *src, double *ring) {
 }
 
 // rr = r.r from the update's partials; sets the convergence flag scal[11]
-// once rr <= thr, after which scal[10] is left alone; ring (host-visible, may
-// be null) receives {rr, iter} for a host that polls without synchronising
-// the stream
+// once rr <= scal[13], after which scal[10] is left alone; writes {rr, it} to
+// ring slot it & 1 (host-visible, may be null), it = scal[14] the iteration
+// counter, which it then advances.  Everything per solve lives in scal, so a
+// captured iteration replays unchanged.
 __global__ void __launch_bounds__(kFcgBlock)
-k_fcg_rr(const double *__restrict__ partial, int32_t g, double *scal, double thr, double *ring,
-         int32_t iter) {
+k_fcg_rr(const double *__restrict__ partial, int32_t g, double *scal, double *ring) {
   __shared__ double lds[kFcgBlock / kWave];
   double t[1];
   fold_sums<1>(partial, g, t, lds);
   if (threadIdx.x != 0) return;
   if (scal[11] == 0.0) {
     scal[10] = t[0];
-    if (t[0] <= thr) scal[11] = 1.0;
+    if (t[0] <= scal[13]) scal[11] = 1.0;
   }
+  const double it = scal[14];
   if (ring) {
-    ring[0] = scal[10];
-    ring[1] = (double)iter;
+    double *slot = ring + 2 * ((int64_t)it & 1);
+    slot[0] = scal[10];
+    slot[1] = it;
   }
+  scal[14] = it + 1.0;
+}
+
+// arm a solve: convergence flag off, threshold thr, iteration counter 0
+__global__ void k_fcg_arm(double *scal, double thr) {
+  if (threadIdx.x == 0) {
+    scal[11] = 0.0;
+    scal[13] = thr;
+    scal[14] = 0.0;
+  }
+}
+
+void fcg_arm(hipStream_t s, double *scal, double thr) {
+  hipLaunchKernelGGL(k_fcg_arm, dim3(1), dim3(64), 0, s, scal, thr);
 }
 
 template <typename V>
 void fcg_update_xr(hipStream_t s, int32_t n, double *scal, const double *p, const double *w,
-                   double *x, double *r, V *rv, double *partial, double thr, double *ring,
-                   int32_t iter, int32_t n0, const V *acv, V *z0) {
+                   double *x, double *r, V *rv, double *partial, double *ring, int32_t n0,
+                   const V *acv, V *z0) {
   // the spmv partials occupy partial[0, g); the r.r partials go after them
   const int g = fold_blocks(n);
   hipLaunchKernelGGL(k_fcg_update_xr<V>, dim3(g), dim3(kFcgBlock), 0, s, n, partial, scal, p, w,
                      x, r, rv, partial + g, n0, acv, z0);
-  hipLaunchKernelGGL(k_fcg_rr, dim3(1), dim3(kFcgBlock), 0, s, partial + g, g, scal, thr, ring,
-                     iter);
+  hipLaunchKernelGGL(k_fcg_rr, dim3(1), dim3(kFcgBlock), 0, s, partial + g, g, scal, ring);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -955,8 +988,8 @@ void axpy_const(hipStream_t s, int32_t n, const double *scal_slot, double mul, d
   template void fcg_update_p<V>(hipStream_t, int32_t, const V *, const double *, int, int,       \
                                 double *, double *);                                            \
   template void fcg_update_xr<V>(hipStream_t, int32_t, double *, const double *, const double *, \
-                                 double *, double *, V *, double *, double, double *, int32_t,  \
-                                 int32_t, const V *, V *);
+                                 double *, double *, V *, double *, double *, int32_t,          \
+                                 const V *, V *);
 UFVM_INST(float)
 UFVM_INST(double)
 template void convert<double, float>(hipStream_t, int64_t, const double *, float *);

@@ -24,8 +24,14 @@ size_t work_bytes(const System &S);
 // the stream while another step of the same solve could be queued.
 class FcgRun {
  public:
+  // Iterations are replayed as captured HIP graphs (one launch instead of
+  // ~30) on systems small enough for host launch cost to matter; on large
+  // ones the GPU is the bottleneck and eager launches leave fewer gaps.
+  // Never while kernel profiling is on.
+  static constexpr int32_t kGraphRows = 600000;
   FcgRun(ufvm_ctx *ctx, System &S, SolveWork &W, hipStream_t s)
-      : ctx_(ctx), S_(&S), W_(&W), s_(s) {}
+      : ctx_(ctx), S_(&S), W_(&W), s_(s),
+        graphs_(!ctx->prof_on && W.n0 < kGraphRows) {}
   // b: right-hand side, x: solution, both device arrays in reference order
   void start(const double *b, double *x);
   bool poll(bool block);
@@ -46,6 +52,7 @@ class FcgRun {
   System *S_;
   SolveWork *W_;
   hipStream_t s_;
+  bool graphs_;
   double *x_out_ = nullptr;
   int phase_ = 0;    // 1 set-up queued, 2 iterating, 3 result queued, 4 done
   int it_ = 0, issued_ = 0;

@@ -31,10 +31,6 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
   FL.n = n;
   FL.nnz = R.nnz;
   FL.kind = kind;
-  std::vector<double> ac(n), anb(std::max(R.nnz, 1));
-  R.ac.download(ac.data(), n, s);
-  R.anb.download(anb.data(), R.nnz, s);
-  UFVM_HIP(hipStreamSynchronize(s));
   FL.perm.resize(n);
   if (kind == 0) {
     FL.ncol = greedy_colour(R.g, FL.colour);
@@ -84,40 +80,31 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
   }
   std::vector<int32_t> iperm(n);
   for (int32_t t = 0; t < n; ++t) iperm[FL.perm[t]] = t;
+  // pattern on the host: permuted CSR and, per entry, its reference position
   FL.h_ptr.assign(n + 1, 0);
-  std::vector<int32_t> col(std::max(R.nnz, 1));
-  std::vector<double> a64(std::max(R.nnz, 1)), d64(n);
+  std::vector<int32_t> col(std::max(R.nnz, 1)), csrc(std::max(R.nnz, 1), -1);
   for (int32_t t = 0; t < n; ++t) {
     const int32_t i = FL.perm[t];
     FL.h_ptr[t + 1] = FL.h_ptr[t] + (R.g.ptr[i + 1] - R.g.ptr[i]);
     int32_t o = FL.h_ptr[t];
     for (int32_t kk = R.g.ptr[i]; kk < R.g.ptr[i + 1]; ++kk, ++o) {
       col[o] = iperm[R.g.col[kk]];
-      a64[o] = anb[kk];
+      csrc[o] = kk;
     }
-    d64[t] = ac[i];
   }
-  std::vector<float> a32(a64.begin(), a64.end()), d32(d64.begin(), d64.end());
   FL.ptr.upload(FL.h_ptr, s);
   FL.col.upload(col, s);
+  FL.csr_src.upload(csrc, s);
   FL.perm_d.upload(FL.perm, s);
-  FL.ac64.upload(d64, s);
-  FL.anb64.upload(a64, s);
-  FL.ac32.upload(d32, s);
-  FL.anb32.upload(a32, s);
+  FL.ac64.alloc(n);
+  FL.ac32.alloc(n);
+  FL.anb64.alloc(std::max(R.nnz, 1));
+  FL.anb32.alloc(std::max(R.nnz, 1));
   if (kind == 4) {
-    // the reference's DILU factor (cfdFactorizeILU.m:10-47, row N-1 quirk
-    // kept) in FP64 over the lexicographic schedule, stored in both precisions
     ensure_schedule(ctx, R);
     if (!R.low_ok) throw Error(UFVM_ERR_NUMERIC, "ILU: the index for i in j is not found");
-    const auto &F = R.fwd.level_ptr;
-    for (size_t q = 0; q + 1 < F.size(); ++q)
-      k::dilu_factor_rows(s, R.fwd_rows.p + F[q], F[q + 1] - F[q], R.ac.p, R.anb.p, R.lptr.p,
-                          R.lnb.p, R.lpji.p, R.lpij.p, R.n - 2, R.dinv.p);
     FL.dinv64.alloc(n);
     FL.dinv32.alloc(n);
-    k::convert<double, double>(s, n, R.dinv.p, FL.dinv64.p);
-    k::convert<double, float>(s, n, R.dinv.p, FL.dinv32.p);
   }
   // the FPCG's operator product runs on level 0's sliced-ELL copy whatever its
   // kind (a single-level hierarchy makes level 0 the dense coarsest level)
@@ -131,26 +118,56 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
       FL.h_soff[q + 1] = FL.h_soff[q] + 64 * w;
     }
     const size_t tot = std::max(FL.h_soff[nq], 1);
-    // padding: col = the row itself, value 0 (rows past n in the last chunk
-    // are never processed)
-    std::vector<int32_t> scol(tot, 0);
-    std::vector<double> sv64(tot, 0.0);
+    // padding: col = the row itself, value 0 (source -1; rows past n in the
+    // last chunk are never processed)
+    std::vector<int32_t> scol(tot, 0), ssrc(tot, -1);
     for (int32_t t = 0; t < n; ++t) {
       const int32_t q = t / 64, base = FL.h_soff[q] + (t & 63);
       const int32_t w = (FL.h_soff[q + 1] - FL.h_soff[q]) / 64;
       const int32_t len = FL.h_ptr[t + 1] - FL.h_ptr[t];
       for (int32_t e = 0; e < w; ++e) {
         scol[base + e * 64] = e < len ? col[FL.h_ptr[t] + e] : t;
-        sv64[base + e * 64] = e < len ? a64[FL.h_ptr[t] + e] : 0.0;
+        ssrc[base + e * 64] = e < len ? csrc[FL.h_ptr[t] + e] : -1;
       }
     }
-    std::vector<float> sv32(sv64.begin(), sv64.end());
     FL.soff.upload(FL.h_soff, s);
     FL.scol.upload(scol, s);
-    FL.sval64.upload(sv64, s);
-    FL.sval32.upload(sv32, s);
+    FL.sell_src.upload(ssrc, s);
+    FL.sval64.alloc(tot);
+    FL.sval32.alloc(tot);
   }
   UFVM_HIP(hipStreamSynchronize(s));
+}
+
+// Values of level copy l from the reference level's current coefficients
+// (device gathers through the maps build_fast_level recorded), both
+// precisions, and the DILU factor of a kind-4 level.
+void refresh_level(ufvm_ctx *ctx, System &S, int l) {
+  hipStream_t s = ctx->stream;
+  RefLevel &R = *S.lev[l];
+  FastLevel &FL = *S.pcg.lev[l];
+  const int32_t n = FL.n;
+  k::gather_vals<double>(s, n, FL.perm_d.p, R.ac.p, FL.ac64.p);
+  k::gather_vals<float>(s, n, FL.perm_d.p, R.ac.p, FL.ac32.p);
+  if (FL.nnz > 0) {
+    k::gather_vals<double>(s, FL.nnz, FL.csr_src.p, R.anb.p, FL.anb64.p);
+    k::gather_vals<float>(s, FL.nnz, FL.csr_src.p, R.anb.p, FL.anb32.p);
+  }
+  if (FL.sell_src.p) {
+    const int64_t slots = FL.h_soff.back();
+    k::gather_vals<double>(s, slots, FL.sell_src.p, R.anb.p, FL.sval64.p);
+    k::gather_vals<float>(s, slots, FL.sell_src.p, R.anb.p, FL.sval32.p);
+  }
+  if (FL.kind == 4) {
+    // the reference's DILU factor (cfdFactorizeILU.m:10-47, row N-1 quirk
+    // kept) in FP64 over the lexicographic schedule, stored in both precisions
+    const auto &F = R.fwd.level_ptr;
+    for (size_t q = 0; q + 1 < F.size(); ++q)
+      k::dilu_factor_rows(s, R.fwd_rows.p + F[q], F[q + 1] - F[q], R.ac.p, R.anb.p, R.lptr.p,
+                          R.lnb.p, R.lpji.p, R.lpij.p, R.n - 2, R.dinv.p);
+    k::convert<double, double>(s, n, R.dinv.p, FL.dinv64.p);
+    k::convert<double, float>(s, n, R.dinv.p, FL.dinv32.p);
+  }
 }
 
 void link_levels(ufvm_ctx *ctx, System &S, int l) {
@@ -429,6 +446,26 @@ std::unique_ptr<SolveWork> make_work(const System &S) {
 
 // -------------------------------------------------------------------- set-up
 
+// dense inverse of the coarsest level (+ mean(diag)/n * 11^T when singular),
+// in its reference numbering (the identity permutation of a kind-2 level)
+static void build_coarsest(ufvm_ctx *ctx, System &S) {
+  PcgState &P = S.pcg;
+  RefLevel &RC = *S.lev[P.nlev - 1];
+  const int32_t nc = RC.n;
+  double shift = 0.0;
+  if (P.singular) {
+    std::vector<double> ac(nc);
+    RC.ac.download(ac.data(), nc, ctx->stream);
+    UFVM_HIP(hipStreamSynchronize(ctx->stream));
+    for (double v : ac) shift += v;
+    shift /= (double)nc * nc;
+  }
+  P.ainv.alloc((size_t)nc * nc);
+  P.colbuf.alloc(nc);
+  k::csr_to_dense(ctx->stream, nc, RC.ptr.p, RC.col.p, RC.ac.p, RC.anb.p, shift, P.ainv.p);
+  k::gj_inverse(ctx->stream, nc, P.ainv.p, P.colbuf.p);
+}
+
 void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
   if (!S.hier_valid) throw Error(UFVM_ERR_STATE, "no AMG hierarchy: call ufvm_amg_agglomerate");
   if (cfg.precision != 32 && cfg.precision != 64) throw Error(UFVM_ERR_ARG, "precision must be 32 or 64");
@@ -473,28 +510,23 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
                                    : (l == 0 ? (cfg.smoother == 1 ? 4 : 0)
                                              : (cfg.coarse_smoother == 1 ? 3 : 1)));
   for (int32_t l = 0; l + 1 < nlev; ++l) link_levels(ctx, S, l);
-  // dense inverse of the coarsest level (+ mean(diag)/n * 11^T when singular)
-  RefLevel &RC = *S.lev[nlev - 1];
-  const int32_t nc = RC.n;
-  double shift = 0.0;
-  if (P.singular) {
-    std::vector<double> ac(nc);
-    RC.ac.download(ac.data(), nc, ctx->stream);
-    UFVM_HIP(hipStreamSynchronize(ctx->stream));
-    for (double v : ac) shift += v;
-    shift /= (double)nc * nc;
-  }
-  P.ainv.alloc((size_t)nc * nc);
-  P.colbuf.alloc(nc);
-  // dense matrix in the multicolour numbering of the coarsest level (identity
-  // permutation there, since it has a single colour class)
-  k::csr_to_dense(ctx->stream, nc, RC.ptr.p, RC.col.p, RC.ac.p, RC.anb.p, shift, P.ainv.p);
-  k::gj_inverse(ctx->stream, nc, P.ainv.p, P.colbuf.p);
+  for (int32_t l = 0; l < nlev; ++l) refresh_level(ctx, S, l);
+  build_coarsest(ctx, S);
   UFVM_HIP(hipStreamSynchronize(ctx->stream));
   P.ready = true;
   P.work0 = make_work(S);
 }
 
+// New coefficients on an unchanged hierarchy (same parents and patterns, e.g.
+// after galerkin_update): refresh every level copy's values and the coarsest
+// inverse on the GPU, keeping colourings, layouts and workspaces.
+void pcg_refresh(ufvm_ctx *ctx, System &S) {
+  PcgState &P = S.pcg;
+  if (!P.ready) throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first");
+  for (int32_t l = 0; l < P.nlev; ++l) refresh_level(ctx, S, l);
+  build_coarsest(ctx, S);
+  UFVM_HIP(hipStreamSynchronize(ctx->stream));
+}
 
 // ---------------------------------------------------------------- FCG solve
 
@@ -549,6 +581,8 @@ void FcgRun::start(const double *b, double *x) {
 // One FCG(1) iteration k: z = M^-1 r (V-cycle), dots, p update, w = A p,
 // x / r update.  The last kernel writes {r.r, k} to ring slot k & 1 and sets
 // the device convergence flag, after which a queued iteration changes nothing.
+// Only k == 0 versus k > 0 changes what is launched (every per-solve value
+// lives in device scalars), so iterations can be captured and replayed.
 template <typename V>
 void FcgRun::issue_t(int k) {
   const PcgState &P = S_->pcg;
@@ -569,14 +603,39 @@ void FcgRun::issue_t(int k) {
     k::spmv_dot(s_, n, L0.soff.p, L0.scol.p, L0.ac64.p, L0.sval64.p, W.p.p, W.w.p, W.partial.p);
   }
   k::fcg_update_xr<V>(s_, n, scal, W.p.p, W.w.p, W.x.p, W.r.p, Prec<V>::b(W.lev[0]), W.partial.p,
-                      thr_, W.dring + 2 * (k & 1), k, fuse0 ? L0.col_off[1] : 0,
-                      Prec<V>::ac(L0), Prec<V>::x(W.lev[0]));
-  UFVM_HIP(hipEventRecord(W.ev[k & 1], s_));
+                      W.dring, fuse0 ? L0.col_off[1] : 0, Prec<V>::ac(L0),
+                      Prec<V>::x(W.lev[0]));
 }
 
 void FcgRun::issue(int k) {
-  if (S_->pcg.cfg.precision == 32) issue_t<float>(k);
-  else issue_t<double>(k);
+  const bool f32 = S_->pcg.cfg.precision == 32;
+  if (!graphs_) {
+    if (f32) issue_t<float>(k);
+    else issue_t<double>(k);
+  } else {
+    hipGraphExec_t &gx = W_->iter_graph[k == 0 ? 0 : 1];
+    if (!gx) {
+      // capture this iteration's launches once per workspace, then replay
+      hipGraph_t g = nullptr;
+      // relaxed: other host threads (a server's SIMPLE thread, set-up of new
+      // systems) may allocate or synchronise while this stream captures
+      UFVM_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeRelaxed));
+      try {
+        if (f32) issue_t<float>(k);
+        else issue_t<double>(k);
+      } catch (...) {
+        (void)hipStreamEndCapture(s_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      UFVM_HIP(hipStreamEndCapture(s_, &g));
+      const hipError_t e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      UFVM_HIP(e);
+    }
+    UFVM_HIP(hipGraphLaunch(gx, s_));
+  }
+  UFVM_HIP(hipEventRecord(W_->ev[k & 1], s_));
 }
 
 void FcgRun::finish() {
@@ -604,7 +663,7 @@ bool FcgRun::poll(bool block) {
       }
       thr_ = (cfg.tol * nb_) * (cfg.tol * nb_);
       rr_ = nb_;
-      k::fill<double>(s_, 1, W_->scal.p + 11, 0.0);
+      k::fcg_arm(s_, W_->scal.p, thr_);
       phase_ = 2;
     }
     // phase 2: keep one iteration queued beyond the one awaited unless the

@@ -466,6 +466,7 @@ int ufvm_server_wait(ufvm_server *srv, int64_t ticket, ufvm_request_info *info) 
     Request *r = find(srv, ticket);
     srv->cv_done.wait(g, [&] { return r->status == 2 || r->status < 0; });
     fill_info(r, info);
+    if (r->status < 0) srv->err = r->err;   // the failed request's message
   });
 }
 

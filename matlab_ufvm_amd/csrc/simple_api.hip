@@ -186,10 +186,13 @@ void solve(ufvm_ctx *ctx, Case &C, int eq, double *r0, double *r1, int32_t *nc) 
   if (C.cfg.p_solver == 1 || !C.p_hier_built) {
     agglomerate(ctx, S, C.cfg.p_amg.max_coarse_levels);
     C.p_hier_built = true;
+    pcg_setup(ctx, S, C.cfg.p_pcg);
   } else {
-    galerkin_update(ctx, S);   // first hierarchy's parents, current coefficients
+    // first hierarchy's parents and layouts, current coefficients
+    galerkin_update(ctx, S);
+    if (S.pcg.ready) pcg_refresh(ctx, S);
+    else pcg_setup(ctx, S, C.cfg.p_pcg);
   }
-  pcg_setup(ctx, S, C.cfg.p_pcg);
   double rel = 0.0;
   pcg_solve(ctx, S, nc, &rel);
   if (r0) *r0 = 1.0;

@@ -112,6 +112,7 @@ struct FastLevel {
   std::vector<int32_t> perm;      // perm[new] = reference index
   std::vector<int32_t> h_ptr;     // permuted CSR row pointer (host copy)
   DBuf<int32_t> ptr, col, perm_d;
+  DBuf<int32_t> csr_src;          // reference CSR position of each entry
   DBuf<float> ac32, anb32;
   DBuf<double> ac64, anb64;
   DBuf<int32_t> parent;           // coarse row of each row (next level's numbering)
@@ -120,6 +121,7 @@ struct FastLevel {
   // q at [soff[q], soff[q+1]) column-major, padding col = -1
   std::vector<int32_t> h_soff;
   DBuf<int32_t> soff, scol;
+  DBuf<int32_t> sell_src;         // reference CSR position of each slot (-1: padding)
   DBuf<float> sval32;
   DBuf<double> sval64;
   // kind 4: DILU diagonal inverse (factored once in FP64 at set-up)
@@ -146,11 +148,15 @@ struct SolveWork {
   // into: slots 0-1 even iterations, 2-3 odd iterations, 4-5 set-up norm
   double *hring = nullptr, *dring = nullptr;
   hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+  // captured FCG iterations: [0] the first of a solve, [1] any later one
+  hipGraphExec_t iter_graph[2] = {nullptr, nullptr};
   size_t bytes = 0;              // device bytes held
   SolveWork() = default;
   SolveWork(const SolveWork &) = delete;
   SolveWork &operator=(const SolveWork &) = delete;
   ~SolveWork() {
+    for (hipGraphExec_t g : iter_graph)
+      if (g) (void)hipGraphExecDestroy(g);
     if (hring) (void)hipHostFree(hring);
     for (hipEvent_t e : ev)
       if (e) (void)hipEventDestroy(e);

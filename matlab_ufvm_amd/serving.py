@@ -136,7 +136,8 @@ class Server:
         self.check(self.lib.ufvm_server_wait(self.h, ticket, C.byref(info)))
         out = self._result(ticket, info)
         if info.status < 0:
-            raise _lib.UfvmError(-info.status, f"request {ticket} failed")
+            msg = self.lib.ufvm_server_last_error(self.h).decode()
+            raise _lib.UfvmError(-info.status, f"request {ticket} failed: {msg}")
         if release:
             self.check(self.lib.ufvm_server_release(self.h, ticket))
             del self._live[ticket]

@@ -218,3 +218,62 @@ def test_errors_are_status_codes(ctx):
         ctx.system_create(np.array([0, 1]), np.array([5]))   # column out of range
     with pytest.raises(UfvmError):
         ctx.pcg_solve(123456)                                # unknown system
+
+
+def test_graph_replays_match_eager_solve(ctx):
+    # FCG iterations are captured as HIP graphs and replayed; with kernel
+    # profiling on they are launched eagerly.  Every replay, with the
+    # program's own eager work in between, must give the eager solve's bits.
+    m = cavity_mesh(16, jitter=0.15, seed=47, renumber_seed=48)
+    ptr, col, ac, anb = poisson_system(m, dirichlet=())
+    b = gaussian_rhs(m, seed=49)
+    sid = make(ctx, ptr, col, ac, anb, b)
+    ctx.agglomerate(sid, 10)
+    ctx.pcg_setup(sid, precision=32, max_coarse_cells=200, tol=1e-9)
+    ctx.prof_enable(True)
+    it0, rr0 = ctx.pcg_solve(sid)
+    x0 = ctx.get_dphi(sid, 0)
+    ctx.prof_enable(False)
+    ctx.prof_reset()
+    for rep in range(4):
+        ctx.residual(sid, 0)                      # eager work between replays
+        it, rr = ctx.pcg_solve(sid)
+        assert it == it0 and rr == rr0, rep
+        assert np.array_equal(ctx.get_dphi(sid, 0), x0), rep
+    # a different right-hand side through the same captured graphs
+    b2 = gaussian_rhs(m, seed=50)
+    ctx.set_bc(sid, 0, b2)
+    ctx.pcg_solve(sid)
+    x2 = ctx.get_dphi(sid, 0)
+    A = to_scipy(ptr, col, ac, anb)
+    bp = b2 - b2.mean()
+    assert np.linalg.norm(bp - A @ x2) <= 1e-8 * np.linalg.norm(bp)
+    ctx.system_destroy(sid)
+
+
+def test_update_coefficients_refresh_matches_setup(ctx):
+    # new values on a kept hierarchy: the GPU refresh of the solve-side copies
+    # gives the same solve, bit for bit, as rebuilding them with pcg_setup
+    m = cavity_mesh(14, jitter=0.15, seed=61, renumber_seed=62)
+    ptr, col, ac, anb = poisson_system(m, dirichlet=("movingWall",))
+    b = gaussian_rhs(m, seed=63)
+    sid = make(ctx, ptr, col, ac, anb, b)
+    ctx.agglomerate(sid, 10)
+    for sm in ("MCGS", "ILU"):
+        ctx.pcg_setup(sid, precision=32, max_coarse_cells=200, tol=1e-10, smoother=sm)
+        ctx.pcg_solve(sid)
+        rng = np.random.default_rng(64)
+        scale = 1.0 + 0.5 * rng.random(len(ac))          # new coefficients, same pattern
+        anb2 = anb * np.sqrt(scale[np.repeat(np.arange(len(ac)), np.diff(ptr))] *
+                             scale[col])
+        ac2 = ac * scale
+        ctx.update_coefficients(sid, ac2, anb2, b)
+        it_r, rr_r = ctx.pcg_solve(sid)
+        x_r = ctx.get_dphi(sid, 0)
+        ctx.pcg_setup(sid, precision=32, max_coarse_cells=200, tol=1e-10, smoother=sm)
+        it_s, rr_s = ctx.pcg_solve(sid)
+        assert it_r == it_s and np.array_equal(ctx.get_dphi(sid, 0), x_r), sm
+        A2 = to_scipy(ptr, col, ac2, anb2)
+        assert np.linalg.norm(b - A2 @ x_r) <= 1e-9 * np.linalg.norm(b)
+        ctx.update_coefficients(sid, ac, anb, b)           # back for the next smoother
+    ctx.system_destroy(sid)
