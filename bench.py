@@ -238,7 +238,8 @@ def event_timed_solves(ctx, sid, steps, stride=5):
     return ctx.prof_read("mcgs0"), ctx.prof_read("spmv0")
 
 
-def cpu_baseline(ptr, col, ac, anb, b, max_coarse, coarse_smoother, tol, singular, smoother):
+def cpu_baseline(ptr, col, ac, anb, b, max_coarse, coarse_smoother, tol, singular, smoother,
+                 coarse_scale, cpre, cpost):
     """The oracle (sequential C, FP64 V-cycle FPCG, same algorithm) on one core:
     one full solve of the same 2.1M-cell system; hierarchy set-up not timed."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -249,7 +250,8 @@ def cpu_baseline(ptr, col, ac, anb, b, max_coarse, coarse_smoother, tol, singula
     t_setup = time.time() - t
     t = time.time()
     x, it, rr = S.fpcg(b, tol=tol, max_coarse_cells=max_coarse, singular=singular,
-                       coarse_smoother=coarse_smoother, smoother=smoother)
+                       coarse_smoother=coarse_smoother, smoother=smoother,
+                       coarse_scale=coarse_scale, cpre=cpre, cpost=cpost)
     dt = time.time() - t
     log(f"cpu oracle: setup {t_setup:.1f}s, solve {dt:.1f}s, {it} iterations, relres {rr:.1e}")
     return {"value": 1.0 / dt, "unit": "solves/s", "cores": 1, "kind": "port",
@@ -277,6 +279,11 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--workload", choices=["pcorr-k5", "laplacian", "tet"], default="pcorr-k5")
     ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--coarse-scale", type=float, default=2.0,
+                    help="factor on every coarse-grid correction (1: plain)")
+    ap.add_argument("--coarse-pre", type=int, default=-1,
+                    help="sweeps below the finest level before restriction (-1: --pre)")
+    ap.add_argument("--coarse-post", type=int, default=-1)
     ap.add_argument("--smoother", choices=["MCGS", "ILU"], default="MCGS",
                     help="finest-level smoother of the V-cycle")
     ap.add_argument("--pre", type=int, default=1)
@@ -335,7 +342,9 @@ def main():
         t = time.time()
         ctx.pcg_setup(sid, precision=prec, pre_sweep=args.pre, post_sweep=args.post,
                       max_coarse_cells=args.max_coarse, tol=args.tol, max_iter=500,
-                      coarse_smoother=args.coarse_smoother, smoother=args.smoother)
+                      coarse_smoother=args.coarse_smoother, smoother=args.smoother,
+                      coarse_scale=args.coarse_scale, coarse_pre_sweep=args.coarse_pre,
+                      coarse_post_sweep=args.coarse_post)
         t_setup = time.time() - t
         total, lat, its, rr = timed_solves(ctx, sid, args.steps, args.warmup)
         x = ctx.get_dphi(sid, 0)
@@ -407,7 +416,8 @@ def main():
         "config": {"workload": workload_name(args),
                    "cells": int(m.n_cells), "faces": int(m.n_faces), "nnz_offdiag": int(col.size),
                    "tol": args.tol, "amg_levels": sizes, "pre_sweep": args.pre,
-                   "smoother": args.smoother,
+                   "smoother": args.smoother, "coarse_scale": args.coarse_scale,
+                   "coarse_sweeps": [args.coarse_pre, args.coarse_post],
                    "post_sweep": args.post, "max_coarse_cells": args.max_coarse,
                    "coarse_smoother": ["block-MCGS", "jacobi"][args.coarse_smoother],
                    "hierarchy_cached": True},
@@ -449,7 +459,8 @@ def main():
         out["plain_fp64_amg"] = plain
     if not args.no_cpu:
         cb, x_cpu = cpu_baseline(ptr, col, ac, anb, b, args.max_coarse, args.coarse_smoother,
-                                 args.tol, int(singular), args.smoother)
+                                 args.tol, int(singular), args.smoother, args.coarse_scale,
+                                 args.coarse_pre, args.coarse_post)
         cb["field_rel_l2_err_vs_gpu_tight"] = ferr(x_cpu)
         out["cpu_baseline"] = cb
     else:

@@ -159,6 +159,11 @@ typedef struct {
                                 parallel), 1 the reference's DILU ('ILU' smoother,
                                 cfdFactorizeILU.m / cfdSolveILU.m) in its lexicographic
                                 order, level-scheduled */
+  double coarse_scale;       /* every coarse-grid correction is scaled by this factor
+                                (1: plain; about 2 suits the piecewise-constant
+                                prolongation of aggregation AMG); in (0, 4] */
+  int32_t coarse_pre_sweep;  /* sweeps below the finest level; -1: pre_sweep */
+  int32_t coarse_post_sweep; /* -1: post_sweep */
 } ufvm_pcg_cfg;
 
 /* Build the solve-side data of the current hierarchy (colourings, permuted

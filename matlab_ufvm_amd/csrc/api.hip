@@ -367,7 +367,10 @@ int ufvm_ctx_create(int32_t device, ufvm_ctx **out) {
     UFVM_HIP(hipMemsetAsync(ctx->scal.p, 0, kScal * sizeof(double), ctx->stream));
     ctx->counter.alloc(4);
     UFVM_HIP(hipMemsetAsync(ctx->counter.p, 0, 4 * sizeof(unsigned), ctx->stream));
-    UFVM_HIP(hipHostMalloc((void **)&ctx->hscal, kScal * sizeof(double), hipHostMallocDefault));
+    {
+      std::lock_guard<std::mutex> g(device_mgmt_mutex());
+      UFVM_HIP(hipHostMalloc((void **)&ctx->hscal, kScal * sizeof(double), hipHostMallocDefault));
+    }
     ctx->partial.alloc(1024);
     UFVM_HIP(hipStreamSynchronize(ctx->stream));
   });
@@ -393,8 +396,11 @@ int ufvm_ctx_destroy(ufvm_ctx *ctx) {
   ctx->partial.release();
   ctx->scal.release();
   ctx->counter.release();
-  if (ctx->hscal) (void)hipHostFree(ctx->hscal);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  {
+    std::lock_guard<std::mutex> g(device_mgmt_mutex());
+    if (ctx->hscal) (void)hipHostFree(ctx->hscal);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  }
   delete ctx;
   return UFVM_OK;
 }
@@ -406,6 +412,7 @@ void *ufvm_ctx_stream(ufvm_ctx *ctx) { return ctx ? (void *)ctx->stream : nullpt
 int ufvm_ctx_synchronize(ufvm_ctx *ctx) {
   return guard(ctx, [&] {
     UFVM_HIP(hipStreamSynchronize(ctx->stream));
+    std::lock_guard<std::mutex> g(device_mgmt_mutex());
     UFVM_HIP(hipDeviceSynchronize());
   });
 }

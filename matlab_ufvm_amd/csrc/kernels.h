@@ -69,16 +69,17 @@ void bmcgs(hipStream_t s, int32_t nblk, const int32_t *bptr, const int32_t *boff
            const int32_t *ptr, const int32_t *col, const V *ac, const V *anb, const V *b,
            const V *x_in, const int32_t *parent, const V *xc, V *x_out, int mode, int nsweep);
 // restriction over sliced-ELL child lists (off per 64 coarse rows, padding
-// -1), alone or fused with a Jacobi sweep from zero on the coarse level
+// -1), bc = scale * sum, alone or fused with a Jacobi sweep from zero on the
+// coarse level
 template <typename V>
 void restrict_jacobi(hipStream_t s, int32_t nc, const int32_t *off, const int32_t *child,
-                     const V *r, const V *acc, V *bc, V *xc, V omega);
+                     const V *r, const V *acc, V *bc, V *xc, V omega, V scale);
 template <typename V>
 void residual_fast(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col, const V *ac,
                    const V *anb, const V *b, const V *x, V *r);
 template <typename V>
 void restrict_fast(hipStream_t s, int32_t nc, const int32_t *off, const int32_t *child,
-                   const V *r, V *bc);
+                   const V *r, V *bc, V scale);
 template <typename V>
 // x[i] += xc[parent[i]] for rows r0 <= i < n
 void prolong_fast(hipStream_t s, int32_t r0, int32_t n, const int32_t *parent, const V *xc,

@@ -488,14 +488,15 @@ void bmcgs(hipStream_t s, int32_t nblk, const int32_t *bptr, const int32_t *boff
 // bc[p] = sum of children's residuals; xc[p] = w bc[p] / ac[p]
 // Restriction over children stored like the sliced-ELL matrix: coarse rows in
 // chunks of 64, chunk q's child lists column-major at off[q] + k*64 + lane,
-// padded with -1.  bc[p] = sum of the children's residuals in list order
-// (increasing reference fine index); JAC: also the first Jacobi sweep from
-// zero on the coarse level, xc[p] = w bc[p] / ac[p].
+// padded with -1.  bc[p] = scale * (sum of the children's residuals in list
+// order, increasing reference fine index); JAC: also the first Jacobi sweep
+// from zero on the coarse level, xc[p] = w bc[p] / ac[p].  The coarse cycle is
+// linear in bc, so scale multiplies the whole coarse correction.
 template <typename V, bool JAC>
 __global__ void __launch_bounds__(kBlock)
 k_restrict_sell(int32_t nc, const int32_t *__restrict__ off, const int32_t *__restrict__ child,
                 const V *__restrict__ r, const V *__restrict__ acc, V *__restrict__ bc,
-                V *__restrict__ xc, V omega) {
+                V *__restrict__ xc, V omega, V scale) {
   const int32_t p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= nc) return;
   const int32_t q = p >> 6;
@@ -516,24 +517,25 @@ k_restrict_sell(int32_t nc, const int32_t *__restrict__ off, const int32_t *__re
     const int32_t c = child[base + k * 64];
     s = s + (c >= 0 ? r[c] : V(0));
   }
-  bc[p] = s;
-  if (JAC) xc[p] = omega * s / acc[p];
+  const V t = scale * s;
+  bc[p] = t;
+  if (JAC) xc[p] = omega * t / acc[p];
 }
 
 template <typename V>
 void restrict_jacobi(hipStream_t s, int32_t nc, const int32_t *off, const int32_t *child,
-                     const V *r, const V *acc, V *bc, V *xc, V omega) {
+                     const V *r, const V *acc, V *bc, V *xc, V omega, V scale) {
   if (nc <= 0) return;
   hipLaunchKernelGGL((k_restrict_sell<V, true>), dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, off,
-                     child, r, acc, bc, xc, omega);
+                     child, r, acc, bc, xc, omega, scale);
 }
 
 template <typename V>
 void restrict_fast(hipStream_t s, int32_t nc, const int32_t *off, const int32_t *child,
-                   const V *r, V *bc) {
+                   const V *r, V *bc, V scale) {
   if (nc <= 0) return;
   hipLaunchKernelGGL((k_restrict_sell<V, false>), dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, off,
-                     child, r, nullptr, bc, nullptr, V(0));
+                     child, r, nullptr, bc, nullptr, V(0), scale);
 }
 
 
@@ -975,11 +977,11 @@ void axpy_const(hipStream_t s, int32_t n, const double *scal_slot, double mul, d
                                const V *, const V *, const V *, const int32_t *, const V *,     \
                                V *, V);                                                         \
   template void restrict_jacobi<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,      \
-                                   const V *, const V *, V *, V *, V);                         \
+                                   const V *, const V *, V *, V *, V, V);                      \
   template void residual_fast<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,        \
                                  const V *, const V *, const V *, const V *, V *);              \
   template void restrict_fast<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,        \
-                                 const V *, V *);                                               \
+                                 const V *, V *, V);                                            \
   template void prolong_fast<V>(hipStream_t, int32_t, int32_t, const int32_t *, const V *, V *); \
   template void dense_solve<V>(hipStream_t, int32_t, const double *, const V *, V *);           \
   template void fill<V>(hipStream_t, int64_t, V *, V);                                          \

@@ -11,8 +11,8 @@ namespace detail {
 
 // Device workspace for one solve on system S (whose ufvm_pcg_setup is done):
 // level vectors in the V-cycle precision, FCG vectors, scalars, partials,
-// the pinned result ring and its events.  *bytes receives the device bytes.
-std::unique_ptr<SolveWork> make_work(const System &S);
+// the pinned result ring and its events; cleared on stream s (synchronised).
+std::unique_ptr<SolveWork> make_work(const System &S, hipStream_t s);
 // device bytes make_work(S) would allocate
 size_t work_bytes(const System &S);
 

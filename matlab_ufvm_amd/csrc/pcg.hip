@@ -303,6 +303,16 @@ struct Cycle {
                           P::anb(L), P::dinv(L), rc, x);
   }
 
+  // smoothing sweeps of level l (the coarse levels may have their own counts)
+  int pre_of(int l) const {
+    const ufvm_pcg_cfg &c = S.pcg.cfg;
+    return (l == 0 || c.coarse_pre_sweep < 0) ? c.pre_sweep : c.coarse_pre_sweep;
+  }
+  int post_of(int l) const {
+    const ufvm_pcg_cfg &c = S.pcg.cfg;
+    return (l == 0 || c.coarse_post_sweep < 0) ? c.post_sweep : c.coarse_post_sweep;
+  }
+
   // residual of level l into r, then restriction to level l+1 (fused with the
   // first Jacobi sweep there when that level is Jacobi-smoothed)
   void down(const FastLevel &L, LevelWork &LW, int l, const V *x, int32_t zero_rows) {
@@ -314,11 +324,14 @@ struct Cycle {
     else
       k::residual_sell<V>(s, L.n, zero_rows, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW),
                           x, P::r(LW));
-    if (C.kind == 3 && Pc.cfg.pre_sweep > 0)
+    // the coarse correction is scaled by coarse_scale through the restricted
+    // right-hand side (the coarse cycle is linear in it)
+    const V scale = V(Pc.cfg.coarse_scale);
+    if (C.kind == 3 && pre_of(l + 1) > 0)
       k::restrict_jacobi<V>(s, C.n, L.csoff.p, L.cschild.p, P::r(LW), P::ac(C), P::b(CW),
-                            P::x(CW), V(2.0 / 3.0));
+                            P::x(CW), V(2.0 / 3.0), scale);
     else
-      k::restrict_fast<V>(s, C.n, L.csoff.p, L.cschild.p, P::r(LW), P::b(CW));
+      k::restrict_fast<V>(s, C.n, L.csoff.p, L.cschild.p, P::r(LW), P::b(CW), scale);
   }
 
   // One V-cycle from level l down; returns the level's solution vector.
@@ -327,7 +340,7 @@ struct Cycle {
     const PcgState &Pc = S.pcg;
     const FastLevel &L = *Pc.lev[l];
     LevelWork &LW = W.lev[l];
-    const int pre = Pc.cfg.pre_sweep, post = Pc.cfg.post_sweep;
+    const int pre = pre_of(l), post = post_of(l);
     if (l == Pc.nlev - 1) {
       k::dense_solve<V>(s, L.n, Pc.ainv.p, P::b(LW), P::x(LW));
       return P::x(LW);
@@ -408,7 +421,7 @@ size_t work_bytes(const System &S) {
 }
 
 template <typename V>
-static void alloc_levels(const System &S, SolveWork &W) {
+static void alloc_levels(const System &S, SolveWork &W, hipStream_t s) {
   const PcgState &P = S.pcg;
   W.lev.clear();
   W.lev.resize(P.nlev);
@@ -417,24 +430,26 @@ static void alloc_levels(const System &S, SolveWork &W) {
       // iterates start finite: the sliced-ELL padding multiplies x[row] by 0
       DBuf<V> &d = Prec<V>::buf(W.lev[l], i);
       d.alloc(P.lev[l]->n);
-      UFVM_HIP(hipMemset(d.p, 0, d.n * sizeof(V)));
+      UFVM_HIP(hipMemsetAsync(d.p, 0, d.n * sizeof(V), s));
     }
 }
 
-std::unique_ptr<SolveWork> make_work(const System &S) {
+std::unique_ptr<SolveWork> make_work(const System &S, hipStream_t s) {
   if (!S.pcg.ready) throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first");
   std::unique_ptr<SolveWork> W(new SolveWork());
   const int32_t n0 = S.lev[0]->n;
   W->n0 = n0;
-  if (S.pcg.cfg.precision == 32) alloc_levels<float>(S, *W);
-  else alloc_levels<double>(S, *W);
+  if (S.pcg.cfg.precision == 32) alloc_levels<float>(S, *W, s);
+  else alloc_levels<double>(S, *W, s);
   for (DBuf<double> *v : {&W->x, &W->r, &W->p, &W->w, &W->bref, &W->xref}) {
     v->alloc(n0);
-    UFVM_HIP(hipMemset(v->p, 0, n0 * sizeof(double)));
+    UFVM_HIP(hipMemsetAsync(v->p, 0, n0 * sizeof(double), s));
   }
   W->partial.alloc(5 * (size_t)k::nblocks((int64_t)n0) + 16);
   W->scal.alloc(kScal);
-  UFVM_HIP(hipMemset(W->scal.p, 0, kScal * sizeof(double)));
+  UFVM_HIP(hipMemsetAsync(W->scal.p, 0, kScal * sizeof(double), s));
+  UFVM_HIP(hipStreamSynchronize(s));
+  std::lock_guard<std::mutex> lk(device_mgmt_mutex());
   UFVM_HIP(hipHostMalloc((void **)&W->hring, 8 * sizeof(double),
                          hipHostMallocMapped | hipHostMallocCoherent));
   UFVM_HIP(hipHostGetDevicePointer((void **)&W->dring, W->hring, 0));
@@ -504,6 +519,8 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
   if (cfg.coarse_smoother != 0 && cfg.coarse_smoother != 1)
     throw Error(UFVM_ERR_ARG, "coarse_smoother must be 0 or 1");
   if (cfg.smoother != 0 && cfg.smoother != 1) throw Error(UFVM_ERR_ARG, "smoother must be 0 or 1");
+  if (!(cfg.coarse_scale > 0.0 && cfg.coarse_scale <= 4.0))
+    throw Error(UFVM_ERR_ARG, "coarse_scale must be in (0, 4]");
   for (int32_t l = 0; l < nlev; ++l)
     build_fast_level(ctx, S, l,
                      l == nlev - 1 ? 2
@@ -514,7 +531,7 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
   build_coarsest(ctx, S);
   UFVM_HIP(hipStreamSynchronize(ctx->stream));
   P.ready = true;
-  P.work0 = make_work(S);
+  P.work0 = make_work(S, ctx->stream);
 }
 
 // New coefficients on an unchanged hierarchy (same parents and patterns, e.g.
@@ -617,8 +634,9 @@ void FcgRun::issue(int k) {
     if (!gx) {
       // capture this iteration's launches once per workspace, then replay
       hipGraph_t g = nullptr;
-      // relaxed: other host threads (a server's SIMPLE thread, set-up of new
-      // systems) may allocate or synchronise while this stream captures
+      // no other host thread may allocate, free or synchronise the device
+      // while this stream captures (HIP refuses those during any capture)
+      std::lock_guard<std::mutex> lk(device_mgmt_mutex());
       UFVM_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeRelaxed));
       try {
         if (f32) issue_t<float>(k);

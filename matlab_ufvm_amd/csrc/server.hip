@@ -165,7 +165,7 @@ void admit(ufvm_server *srv) {
     }
     try {
       if (!W) {
-        W = make_work(*S);
+        W = make_work(*S, sl.stream);
         std::lock_guard<std::mutex> g(srv->mu);
         srv->work_bytes_held += W->bytes;
       }

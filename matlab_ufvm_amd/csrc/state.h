@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -26,6 +27,15 @@ struct Error : std::runtime_error {
     if (e_ != hipSuccess)                                                                 \
       throw ::ufvm::Error(UFVM_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
   } while (0)
+
+// Serialises device allocation, release and device-wide synchronisation with
+// HIP graph capture: several host threads (a server's worker and SIMPLE
+// threads) may capture while others set up, and HIP refuses such operations
+// while any stream captures.
+inline std::mutex &device_mgmt_mutex() {
+  static std::mutex m;
+  return m;
+}
 
 template <typename T>
 struct DBuf {
@@ -50,7 +60,10 @@ struct DBuf {
   }
   ~DBuf() { release(); }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      std::lock_guard<std::mutex> g(device_mgmt_mutex());
+      (void)hipFree(p);
+    }
     p = nullptr;
     n = 0;
   }
@@ -58,6 +71,7 @@ struct DBuf {
     if (count == n && p) return;
     release();
     if (count == 0) count = 1;
+    std::lock_guard<std::mutex> g(device_mgmt_mutex());
     UFVM_HIP(hipMalloc(&p, count * sizeof(T)));
     n = count;
   }
@@ -155,6 +169,7 @@ struct SolveWork {
   SolveWork(const SolveWork &) = delete;
   SolveWork &operator=(const SolveWork &) = delete;
   ~SolveWork() {
+    std::lock_guard<std::mutex> lk(device_mgmt_mutex());
     for (hipGraphExec_t g : iter_graph)
       if (g) (void)hipGraphExecDestroy(g);
     if (hring) (void)hipHostFree(hring);

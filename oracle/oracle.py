@@ -33,7 +33,8 @@ class Hier(C.Structure):
 class VCfg(C.Structure):
     _fields_ = [("nlev", C.c_int32), ("pre", C.c_int32), ("post", C.c_int32),
                 ("singular", C.c_int32), ("block_rows", C.c_int32),
-                ("coarse_smoother", C.c_int32), ("smoother", C.c_int32)]
+                ("coarse_smoother", C.c_int32), ("smoother", C.c_int32),
+                ("coarse_scale", C.c_double), ("cpre", C.c_int32), ("cpost", C.c_int32)]
 
 
 BLOCK_ROWS = 1024   # rows per block of the coarse-level smoother
@@ -186,22 +187,24 @@ class System:
         return self.H.nlevels
 
     def vcycle(self, b, pre=1, post=1, max_coarse_cells=2000, singular=0, coarse_smoother=1,
-               smoother="MCGS"):
+               smoother="MCGS", coarse_scale=2.0, cpre=-1, cpost=-1):
         nlev = self.pcg_levels(max_coarse_cells)
         arr, ncl = self._colour_all(nlev)
         cfg = VCfg(nlev, pre, post, singular, BLOCK_ROWS, coarse_smoother,
-                   {"MCGS": 0, "ILU": 1}[smoother])
+                   {"MCGS": 0, "ILU": 1}[smoother], coarse_scale, pre if cpre < 0 else cpre,
+                   post if cpost < 0 else cpost)
         b = np.ascontiguousarray(b, np.float64)
         x = np.empty_like(b)
         lib().orc_vcycle(C.byref(self.H), arr, ncl, C.byref(cfg), _p(b, f64p), _p(x, f64p))
         return x
 
     def fpcg(self, b, tol=1e-8, maxit=500, pre=1, post=1, max_coarse_cells=2000, singular=0,
-             coarse_smoother=1, smoother="MCGS"):
+             coarse_smoother=1, smoother="MCGS", coarse_scale=2.0, cpre=-1, cpost=-1):
         nlev = self.pcg_levels(max_coarse_cells)
         arr, ncl = self._colour_all(nlev)
         cfg = VCfg(nlev, pre, post, singular, BLOCK_ROWS, coarse_smoother,
-                   {"MCGS": 0, "ILU": 1}[smoother])
+                   {"MCGS": 0, "ILU": 1}[smoother], coarse_scale, pre if cpre < 0 else cpre,
+                   post if cpost < 0 else cpost)
         b = np.ascontiguousarray(b, np.float64)
         x = np.empty_like(b)
         rr = C.c_double()

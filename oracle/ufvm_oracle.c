@@ -644,7 +644,7 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
   const double omega = 2.0 / 3.0;
   if (l > 0 && cfg->coarse_smoother == 1) {
     double *y = (double *)malloc((size_t)n * sizeof(double));
-    for (int s = 0; s < cfg->pre; ++s) {
+    for (int s = 0; s < cfg->cpre; ++s) {
       jacobi(L, b, x, NULL, NULL, omega, y);
       memcpy(x, y, (size_t)n * sizeof(double));
     }
@@ -658,13 +658,14 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
     double *bc = (double *)calloc((size_t)C->n, sizeof(double));
     double *xc = (double *)calloc((size_t)C->n, sizeof(double));
     for (int32_t i = 0; i < n; ++i) bc[L->parents[i]] = bc[L->parents[i]] + r[i];
+    for (int32_t q = 0; q < C->n; ++q) bc[q] = cfg->coarse_scale * bc[q];
     vcyc_rec(H, colours, ncol, cfg, lu, dc0, l + 1, bc, xc);
-    if (cfg->post == 0) {
+    if (cfg->cpost == 0) {
       for (int32_t i = 0; i < n; ++i) x[i] = x[i] + xc[L->parents[i]];
     } else {
       jacobi(L, b, x, L->parents, xc, omega, y);
       memcpy(x, y, (size_t)n * sizeof(double));
-      for (int s = 1; s < cfg->post; ++s) {
+      for (int s = 1; s < cfg->cpost; ++s) {
         jacobi(L, b, x, NULL, NULL, omega, y);
         memcpy(x, y, (size_t)n * sizeof(double));
       }
@@ -686,6 +687,7 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
     double *bc = (double *)calloc((size_t)C->n, sizeof(double));
     double *xc = (double *)calloc((size_t)C->n, sizeof(double));
     for (int32_t i = 0; i < n; ++i) bc[L->parents[i]] = bc[L->parents[i]] + r[i];
+    for (int32_t q = 0; q < C->n; ++q) bc[q] = cfg->coarse_scale * bc[q];
     vcyc_rec(H, colours, ncol, cfg, lu, dc0, l + 1, bc, xc);
     for (int32_t i = 0; i < n; ++i) x[i] = x[i] + xc[L->parents[i]];
     for (int s = 0; s < cfg->post; ++s) ilu_apply(L, dc0, b, rc, x);
@@ -694,8 +696,8 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
   }
   if (l == 0) {
     for (int s = 0; s < cfg->pre; ++s) orc_mcgs(L, colours[l], ncol[l], b, x);
-  } else if (cfg->pre > 0) {
-    bmcgs(L, colours[l], ncol[l], cfg->block_rows, b, NULL, NULL, NULL, 0, cfg->pre, x);
+  } else if (cfg->cpre > 0) {
+    bmcgs(L, colours[l], ncol[l], cfg->block_rows, b, NULL, NULL, NULL, 0, cfg->cpre, x);
   }
   double *r = (double *)malloc((size_t)n * sizeof(double));
   for (int32_t i = 0; i < n; ++i) {
@@ -713,10 +715,11 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
   double *bc = (double *)calloc((size_t)C->n, sizeof(double));
   double *xc = (double *)calloc((size_t)C->n, sizeof(double));
   for (int32_t i = 0; i < n; ++i) bc[L->parents[i]] = bc[L->parents[i]] + r[i];
+    for (int32_t q = 0; q < C->n; ++q) bc[q] = cfg->coarse_scale * bc[q];
   vcyc_rec(H, colours, ncol, cfg, lu, dc0, l + 1, bc, xc);
-  if (l > 0 && cfg->post > 0) {
+  if (l > 0 && cfg->cpost > 0) {
     double *y = (double *)malloc((size_t)n * sizeof(double));
-    bmcgs(L, colours[l], ncol[l], cfg->block_rows, b, x, L->parents, xc, 2, cfg->post, y);
+    bmcgs(L, colours[l], ncol[l], cfg->block_rows, b, x, L->parents, xc, 2, cfg->cpost, y);
     memcpy(x, y, (size_t)n * sizeof(double));
     free(y);
   } else {

@@ -76,7 +76,7 @@ void orc_mcgs(const orc_level *L, const int32_t *colour, int ncolours, const dou
               double *x);
 
 /* preconditioner V-cycle: zero initial guess on every level, nu_pre/nu_post
- * symmetric sweeps, restriction by parents (on level 0 the residual of the
+ * symmetric sweeps, restriction by parents scaled by coarse_scale (on level 0 the residual of the
  * rows of colour 0, solved last by the pre-sweep, is taken as 0), level
  * nlev-1 solved exactly
  * (dense LU, + mean(diag)/n 11^T when `singular`).  Level 0 uses global
@@ -91,6 +91,9 @@ typedef struct {
   int32_t coarse_smoother;  /* levels 1..nlev-2: 0 block-local MCGS, 1 damped Jacobi (2/3) */
   int32_t smoother;         /* level 0: 0 symmetric multicolour GS, 1 the reference's DILU
                                (cfdFactorizeILU / cfdSolveILU, lexicographic order) */
+  double coarse_scale;      /* restricted right-hand sides (hence every coarse correction)
+                               scaled by this factor */
+  int32_t cpre, cpost;      /* sweeps on levels 1..nlev-2 */
 } orc_vcfg;
 void orc_vcycle(const orc_hier *H, int32_t *const *colours, const int *ncolours,
                 const orc_vcfg *cfg, const double *b, double *x);

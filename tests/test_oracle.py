@@ -337,15 +337,16 @@ def test_vcycle_symmetric_with_block_smoothed_levels(coarse_smoother):
         assert y @ My > 0   # positive definite preconditioner
 
 
-@pytest.mark.parametrize("smoother,sweeps", [("MCGS", 1), ("MCGS", 2), ("ILU", 1), ("ILU", 2)])
-def test_two_level_vcycle_matches_dense_formula(poisson16, smoother, sweeps):
+@pytest.mark.parametrize("smoother,sweeps,w", [("MCGS", 1, 1.0), ("MCGS", 2, 2.0),
+                                               ("ILU", 1, 2.0), ("ILU", 2, 1.5)])
+def test_two_level_vcycle_matches_dense_formula(poisson16, smoother, sweeps, w):
     # With the coarse level solved exactly the V-cycle is, in matrices,
-    #   x1 = S^pre(0; b), xc = Ac^-1 P^T (b - A x1), x2 = x1 + P xc, z = S^post(x2; b)
+    #   x1 = S^pre(0; b), xc = Ac^-1 P^T (b - A x1), x2 = x1 + w P xc, z = S^post(x2; b)
     # where one smoothing sweep from x is x + M^-1 (b - A x):
     #   MCGS: M = D + L_c (colours up) then D + U_c (colours down), L_c / U_c the
     #         couplings to lower / higher colour classes
     #   ILU:  M = (D' + L) D'^-1 (D' + U), D' the DILU diagonal (pinned above)
-    # and Ac = P^T A P from the parents.
+    # Ac = P^T A P from the parents and w the coarse-correction scale.
     m, ptr, col, ac, anb, b = poisson16
     S = oracle.System(ptr, col, ac, anb)
     S.agglomerate(10)
@@ -376,10 +377,11 @@ def test_two_level_vcycle_matches_dense_formula(poisson16, smoother, sweeps):
     x = np.zeros(m.n_cells)
     for _ in range(sweeps):
         x = sweep(x)
-    x = x + P @ np.linalg.solve(Ac, P.T @ (b - A @ x))
+    x = x + w * (P @ np.linalg.solve(Ac, P.T @ (b - A @ x)))
     for _ in range(sweeps):
         x = sweep(x)
-    z = S.vcycle(b, pre=sweeps, post=sweeps, max_coarse_cells=n1, smoother=smoother)
+    z = S.vcycle(b, pre=sweeps, post=sweeps, max_coarse_cells=n1, smoother=smoother,
+                 coarse_scale=w)
     assert np.abs(z - x).max() <= 1e-10 * np.abs(x).max()
 
 
