@@ -289,6 +289,8 @@ def main():
     ap.add_argument("--pre", type=int, default=1)
     ap.add_argument("--post", type=int, default=1)
     ap.add_argument("--max-coarse", type=int, default=2000)
+    ap.add_argument("--launch-mode", choices=["auto", "eager", "graphs"], default="auto",
+                    help="FCG iterations as captured HIP graphs or kernel by kernel")
     ap.add_argument("--coarse-smoother", type=int, default=1,
                     help="levels below the first: 0 block-local MCGS, 1 damped Jacobi")
     ap.add_argument("--no-cpu", action="store_true")
@@ -344,7 +346,7 @@ def main():
                       max_coarse_cells=args.max_coarse, tol=args.tol, max_iter=500,
                       coarse_smoother=args.coarse_smoother, smoother=args.smoother,
                       coarse_scale=args.coarse_scale, coarse_pre_sweep=args.coarse_pre,
-                      coarse_post_sweep=args.coarse_post)
+                      coarse_post_sweep=args.coarse_post, launch_mode=args.launch_mode)
         t_setup = time.time() - t
         total, lat, its, rr = timed_solves(ctx, sid, args.steps, args.warmup)
         x = ctx.get_dphi(sid, 0)
@@ -418,6 +420,7 @@ def main():
                    "tol": args.tol, "amg_levels": sizes, "pre_sweep": args.pre,
                    "smoother": args.smoother, "coarse_scale": args.coarse_scale,
                    "coarse_sweeps": [args.coarse_pre, args.coarse_post],
+                   "launch_mode": args.launch_mode,
                    "post_sweep": args.post, "max_coarse_cells": args.max_coarse,
                    "coarse_smoother": ["block-MCGS", "jacobi"][args.coarse_smoother],
                    "hierarchy_cached": True},

@@ -164,6 +164,9 @@ typedef struct {
                                 prolongation of aggregation AMG); in (0, 4] */
   int32_t coarse_pre_sweep;  /* sweeps below the finest level; -1: pre_sweep */
   int32_t coarse_post_sweep; /* -1: post_sweep */
+  int32_t launch_mode;       /* FCG iterations: 0 auto (replayed as captured HIP graphs below
+                                600000 cells, launched kernel by kernel above), 1 always
+                                kernel by kernel, 2 always graphs */
 } ufvm_pcg_cfg;
 
 /* Build the solve-side data of the current hierarchy (colourings, permuted

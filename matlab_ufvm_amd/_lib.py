@@ -30,13 +30,16 @@ class AmgCfg(C.Structure):
                 ("post_sweep", C.c_int32), ("smoother", C.c_int32), ("rebuild", C.c_int32)]
 
 
+LAUNCH_MODE = {"auto": 0, "eager": 1, "graphs": 2}
+
+
 class PcgCfg(C.Structure):
     _fields_ = [("precision", C.c_int32), ("pre_sweep", C.c_int32), ("post_sweep", C.c_int32),
                 ("coarse_smoother", C.c_int32),
                 ("max_coarse_cells", C.c_int32), ("nullspace", C.c_int32),
                 ("max_iter", C.c_int32), ("tol", C.c_double), ("smoother", C.c_int32),
                 ("coarse_scale", C.c_double), ("coarse_pre_sweep", C.c_int32),
-                ("coarse_post_sweep", C.c_int32)]
+                ("coarse_post_sweep", C.c_int32), ("launch_mode", C.c_int32)]
 
 
 class ServerCfg(C.Structure):
@@ -296,14 +299,16 @@ class Context:
 
     def pcg_setup(self, sid, precision=32, pre_sweep=1, post_sweep=1, max_coarse_cells=2000,
                   nullspace=-1, max_iter=500, tol=1e-8, coarse_smoother=1, smoother="MCGS",
-                  coarse_scale=2.0, coarse_pre_sweep=-1, coarse_post_sweep=-1):
+                  coarse_scale=2.0, coarse_pre_sweep=-1, coarse_post_sweep=-1,
+                  launch_mode="auto"):
         """smoother (finest level): "MCGS" multicolour Gauss-Seidel, "ILU" the
         reference's DILU in its lexicographic order (level-scheduled);
         coarse_scale: factor on every coarse-grid correction; coarse_*_sweep:
-        sweeps below the finest level (-1: as on the finest)."""
+        sweeps below the finest level (-1: as on the finest); launch_mode:
+        "auto" (captured graphs below 600k cells), "eager" or "graphs"."""
         cfg = PcgCfg(precision, pre_sweep, post_sweep, coarse_smoother, max_coarse_cells,
                      nullspace, max_iter, tol, PCG_SMOOTHER[smoother], coarse_scale,
-                     coarse_pre_sweep, coarse_post_sweep)
+                     coarse_pre_sweep, coarse_post_sweep, LAUNCH_MODE[launch_mode])
         self.check(self.lib.ufvm_pcg_setup(self.h, sid, C.byref(cfg)))
 
     def pcg_solve(self, sid):

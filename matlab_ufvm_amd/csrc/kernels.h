@@ -86,6 +86,7 @@ void prolong_fast(hipStream_t s, int32_t r0, int32_t n, const int32_t *parent, c
                   V *x);
 template <typename V>
 void dense_solve(hipStream_t s, int32_t n, const double *ainv, const V *b, V *x);
+
 template <typename V>
 void fill(hipStream_t s, int64_t n, V *x, V v);
 template <typename A, typename B>
@@ -120,6 +121,8 @@ void fcg_dots(hipStream_t s, int32_t n, const V *z, const double *w, const doubl
 template <typename V>
 void fcg_update_p(hipStream_t s, int32_t n, const V *z, const double *partial, int singular,
                   int first, double *scal, double *p);
+// blocks of the FCG streaming kernels (each writes one partial per block)
+int fold_blocks(int32_t n);
 // folds spmv_dot's partials (delta, alpha into scal) ; x += alpha p ;
 // r -= alpha w ; rv = (V) r ; then rr = r.r into scal[10], the convergence
 // flag scal[11] once rr <= scal[13] (no update at all once it is set), and

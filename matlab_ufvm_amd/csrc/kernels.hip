@@ -343,6 +343,16 @@ void mcgs_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const 
 // colour class a Gauss-Seidel sweep solved last, whose residual is rounding
 // noise)
 template <typename V>
+__device__ __forceinline__ void residual_row(int32_t i, const int32_t *__restrict__ off,
+                                             const int32_t *__restrict__ col,
+                                             const V *__restrict__ val, const V *__restrict__ ac,
+                                             const V *b, const V *x, V *r) {
+  const int32_t q = i >> 6;
+  const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
+  r[i] = sell_row<V, true, false>(col, val, x, base, w, b[i] - ac[i] * x[i], 0);
+}
+
+template <typename V>
 __global__ void __launch_bounds__(kBlock)
 k_residual_sell(int32_t n, int32_t zero_rows, const int32_t *__restrict__ off,
                 const int32_t *__restrict__ col, const V *__restrict__ val,
@@ -354,9 +364,7 @@ k_residual_sell(int32_t n, int32_t zero_rows, const int32_t *__restrict__ off,
     r[i] = V(0);
     return;
   }
-  const int32_t q = i >> 6;
-  const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
-  r[i] = sell_row<V, true, false>(col, val, x, base, w, b[i] - ac[i] * x[i], 0);
+  residual_row<V>(i, off, col, val, ac, b, x, r);
 }
 
 template <typename V>
@@ -369,21 +377,20 @@ void residual_sell(hipStream_t s, int32_t n, int32_t zero_rows, const int32_t *o
 
 // v_j = x_j (+ xc[parent_j] when PRO: prolongation fused into the sweep)
 template <typename V, bool PRO>
-__device__ __forceinline__ V jac_val(const V *__restrict__ x, const int32_t *__restrict__ parent,
-                                     const V *__restrict__ xc, int32_t j) {
+__device__ __forceinline__ V jac_val(const V *x, const int32_t *__restrict__ parent, const V *xc,
+                                     int32_t j) {
   return PRO ? x[j] + xc[parent[j]] : x[j];
 }
 
 // one damped Jacobi sweep: out = v + omega (b - A v) / ac, v = x or, with
 // PRO, x + xc[parent] (the coarse correction added on the fly)
 template <typename V, bool PRO>
-__global__ void __launch_bounds__(kBlock)
-k_jacobi_sell(int32_t n, const int32_t *__restrict__ off, const int32_t *__restrict__ col,
-              const V *__restrict__ val, const V *__restrict__ ac, const V *__restrict__ b,
-              const V *__restrict__ x, const int32_t *__restrict__ parent,
-              const V *__restrict__ xc, V *__restrict__ out, V omega) {
-  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= n) return;
+__device__ __forceinline__ void jacobi_row(int32_t i, const int32_t *__restrict__ off,
+                                           const int32_t *__restrict__ col,
+                                           const V *__restrict__ val, const V *__restrict__ ac,
+                                           const V *b, const V *x,
+                                           const int32_t *__restrict__ parent, const V *xc,
+                                           V *out, V omega) {
   const int32_t q = i >> 6;
   const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
   const V vi = jac_val<V, PRO>(x, parent, xc, i), d = ac[i];
@@ -407,6 +414,17 @@ k_jacobi_sell(int32_t n, const int32_t *__restrict__ off, const int32_t *__restr
     v = v - val[base + k * 64] * jac_val<V, PRO>(x, parent, xc, j);
   }
   out[i] = vi + omega * v / d;
+}
+
+template <typename V, bool PRO>
+__global__ void __launch_bounds__(kBlock)
+k_jacobi_sell(int32_t n, const int32_t *__restrict__ off, const int32_t *__restrict__ col,
+              const V *__restrict__ val, const V *__restrict__ ac, const V *__restrict__ b,
+              const V *__restrict__ x, const int32_t *__restrict__ parent,
+              const V *__restrict__ xc, V *__restrict__ out, V omega) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  jacobi_row<V, PRO>(i, off, col, val, ac, b, x, parent, xc, out, omega);
 }
 
 template <typename V>
@@ -493,12 +511,10 @@ void bmcgs(hipStream_t s, int32_t nblk, const int32_t *bptr, const int32_t *boff
 // from zero on the coarse level, xc[p] = w bc[p] / ac[p].  The coarse cycle is
 // linear in bc, so scale multiplies the whole coarse correction.
 template <typename V, bool JAC>
-__global__ void __launch_bounds__(kBlock)
-k_restrict_sell(int32_t nc, const int32_t *__restrict__ off, const int32_t *__restrict__ child,
-                const V *__restrict__ r, const V *__restrict__ acc, V *__restrict__ bc,
-                V *__restrict__ xc, V omega, V scale) {
-  const int32_t p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= nc) return;
+__device__ __forceinline__ void restrict_row(int32_t p, const int32_t *__restrict__ off,
+                                             const int32_t *__restrict__ child, const V *r,
+                                             const V *__restrict__ acc, V *bc, V *xc, V omega,
+                                             V scale) {
   const int32_t q = p >> 6;
   const int32_t base = off[q] + (p & 63), w = (off[q + 1] - off[q]) >> 6;
   V s = 0;
@@ -520,6 +536,16 @@ k_restrict_sell(int32_t nc, const int32_t *__restrict__ off, const int32_t *__re
   const V t = scale * s;
   bc[p] = t;
   if (JAC) xc[p] = omega * t / acc[p];
+}
+
+template <typename V, bool JAC>
+__global__ void __launch_bounds__(kBlock)
+k_restrict_sell(int32_t nc, const int32_t *__restrict__ off, const int32_t *__restrict__ child,
+                const V *__restrict__ r, const V *__restrict__ acc, V *__restrict__ bc,
+                V *__restrict__ xc, V omega, V scale) {
+  const int32_t p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= nc) return;
+  restrict_row<V, JAC>(p, off, child, r, acc, bc, xc, omega, scale);
 }
 
 template <typename V>
@@ -561,12 +587,8 @@ void residual_fast(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *
 
 // x = Ainv * b, one wavefront per row, FP64 accumulation
 template <typename V>
-__global__ void __launch_bounds__(kBlock)
-k_dense_gemv(int32_t n, const double *__restrict__ ainv, const V *__restrict__ b,
-             V *__restrict__ x) {
-  const int32_t row = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
-  const int lane = threadIdx.x & (kWave - 1);
-  if (row >= n) return;
+__device__ __forceinline__ void dense_row(int32_t row, int lane, int32_t n,
+                                          const double *__restrict__ ainv, const V *b, V *x) {
   const double *a = ainv + (int64_t)row * n;
   // four independent partial sums keep four loads in flight per lane
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
@@ -581,6 +603,16 @@ k_dense_gemv(int32_t n, const double *__restrict__ ainv, const V *__restrict__ b
   double s = (s0 + s1) + (s2 + s3);
   s = wave_sum(s);
   if (lane == 0) x[row] = (V)s;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kBlock)
+k_dense_gemv(int32_t n, const double *__restrict__ ainv, const V *__restrict__ b,
+             V *__restrict__ x) {
+  const int32_t row = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x & (kWave - 1);
+  if (row >= n) return;
+  dense_row<V>(row, lane, n, ainv, b, x);
 }
 
 template <typename V>
@@ -705,7 +737,7 @@ void gj_inverse(hipStream_t s, int32_t n, double *a, double *colbuf) {
 
 constexpr int kFcgBlock = 1024;
 
-static int fold_blocks(int32_t n) {
+int fold_blocks(int32_t n) {
   const int b = nblocks(n, kFcgBlock);
   return b < kFoldBlocks ? b : kFoldBlocks;
 }
@@ -757,14 +789,29 @@ k_fcg_dots(int32_t n, const V *__restrict__ z, const double *__restrict__ w,
            const double *__restrict__ r, double *__restrict__ partial) {
   __shared__ double lds[5 * kFcgBlock / kWave];
   double v[5] = {0, 0, 0, 0, 0};
-  for (int32_t i = blockIdx.x * kFcgBlock + threadIdx.x; i < n; i += gridDim.x * kFcgBlock) {
-    const double zi = (double)z[i], wi = w[i], ri = r[i];
+  auto add = [&](double zi, double wi, double ri) {
     v[0] += zi;
     v[1] += zi * wi;
     v[2] += zi * ri;
     v[3] += wi;
     v[4] += ri;
+  };
+  const int32_t G = gridDim.x * kFcgBlock;
+  int32_t i = blockIdx.x * kFcgBlock + threadIdx.x;
+  // four grid-stride rows per trip: their loads are in flight together, and
+  // they are summed in the same order as one row per trip
+  for (; i + 3 * G < n; i += 4 * G) {
+    double zi[4], wi[4], ri[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      zi[u] = (double)z[i + u * G];
+      wi[u] = w[i + u * G];
+      ri[u] = r[i + u * G];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) add(zi[u], wi[u], ri[u]);
   }
+  for (; i < n; i += G) add((double)z[i], w[i], r[i]);
   block_sums_all<kFcgBlock, 5>(v, lds);
   if (threadIdx.x == 0)
     for (int c = 0; c < 5; ++c) partial[(int64_t)c * gridDim.x + blockIdx.x] = v[c];
@@ -786,6 +833,19 @@ k_fcg_update_p(int32_t n, const V *__restrict__ z, const double *__restrict__ pa
                int32_t g, int singular, int first, double *__restrict__ scal,
                double *__restrict__ p) {
   __shared__ double lds[5 * kFcgBlock / kWave];
+  const int32_t G = gridDim.x * kFcgBlock;
+  int32_t i = blockIdx.x * kFcgBlock + threadIdx.x;
+  // the first four grid-stride rows are loaded before the fold, so that their
+  // latency overlaps the partials' (the scalars are not needed to load them)
+  double zi[4], pi[4];
+  const bool ahead = i + 3 * G < n;
+  if (ahead) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      zi[u] = (double)z[i + u * G];
+      pi[u] = first ? 0.0 : p[i + u * G];
+    }
+  }
   double t[5];
   fold_sums<5>(partial, g, t, lds);
   const double m = singular ? t[0] / (double)n : 0.0;
@@ -797,7 +857,25 @@ k_fcg_update_p(int32_t n, const V *__restrict__ z, const double *__restrict__ pa
     scal[8] = rho;
     scal[6] = beta;
   }
-  for (int32_t i = blockIdx.x * kFcgBlock + threadIdx.x; i < n; i += gridDim.x * kFcgBlock) {
+  auto put = [&](int32_t j, double zj, double pj) {
+    const double d = zj - m;
+    p[j] = first ? d : d - beta * pj;
+  };
+  if (ahead) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) put(i + u * G, zi[u], pi[u]);
+    i += 4 * G;
+  }
+  for (; i + 3 * G < n; i += 4 * G) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      zi[u] = (double)z[i + u * G];
+      pi[u] = first ? 0.0 : p[i + u * G];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) put(i + u * G, zi[u], pi[u]);
+  }
+  for (; i < n; i += G) {
     const double zi = (double)z[i] - m;
     p[i] = first ? zi : zi - beta * p[i];
   }
@@ -827,6 +905,21 @@ k_fcg_update_xr(int32_t n, const double *__restrict__ partial_in, double *__rest
                 V *__restrict__ z0) {
   __shared__ double lds[kFcgBlock / kWave];
   if (scal[11] != 0.0) return;
+  const int32_t G = gridDim.x * kFcgBlock;
+  int32_t i = blockIdx.x * kFcgBlock + threadIdx.x;
+  // the first four grid-stride rows are loaded before the fold (their latency
+  // overlaps the partials')
+  double xi[4], pi[4], ri[4], wi[4];
+  const bool ahead = i + 3 * G < n;
+  if (ahead) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      xi[u] = x[i + u * G];
+      pi[u] = p[i + u * G];
+      ri[u] = r[i + u * G];
+      wi[u] = w[i + u * G];
+    }
+  }
   double t[1];
   fold_sums<1>(partial_in, gridDim.x, t, lds);
   const double delta = t[0];
@@ -836,15 +929,32 @@ k_fcg_update_xr(int32_t n, const double *__restrict__ partial_in, double *__rest
     scal[7] = a;
   }
   double rr = 0.0;
-  for (int32_t i = blockIdx.x * kFcgBlock + threadIdx.x; i < n; i += gridDim.x * kFcgBlock) {
-    x[i] = x[i] + a * p[i];
-    const double ri = r[i] - a * w[i];
+  auto row = [&](int32_t i, double xi, double pi, double ri0, double wi) {
+    x[i] = xi + a * pi;
+    const double ri = ri0 - a * wi;
     r[i] = ri;
     const V rvi = (V)ri;
     rv[i] = rvi;
     if (i < n0) z0[i] = rvi / acv[i];
     rr += ri * ri;
+  };
+  if (ahead) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) row(i + u * G, xi[u], pi[u], ri[u], wi[u]);
+    i += 4 * G;
   }
+  for (; i + 3 * G < n; i += 4 * G) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      xi[u] = x[i + u * G];
+      pi[u] = p[i + u * G];
+      ri[u] = r[i + u * G];
+      wi[u] = w[i + u * G];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) row(i + u * G, xi[u], pi[u], ri[u], wi[u]);
+  }
+  for (; i < n; i += G) row(i, x[i], p[i], r[i], w[i]);
   rr = block_sum_n<kFcgBlock>(rr, lds);
   if (threadIdx.x == 0) partial_out[blockIdx.x] = rr;
 }
@@ -919,7 +1029,8 @@ template <typename V>
 void fcg_update_xr(hipStream_t s, int32_t n, double *scal, const double *p, const double *w,
                    double *x, double *r, V *rv, double *partial, double *ring, int32_t n0,
                    const V *acv, V *z0) {
-  // the spmv partials occupy partial[0, g); the r.r partials go after them
+  // the operator-product partials occupy partial[0, g); the r.r partials go
+  // after them
   const int g = fold_blocks(n);
   hipLaunchKernelGGL(k_fcg_update_xr<V>, dim3(g), dim3(kFcgBlock), 0, s, n, partial, scal, p, w,
                      x, r, rv, partial + g, n0, acv, z0);

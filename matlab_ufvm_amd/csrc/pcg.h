@@ -31,7 +31,8 @@ class FcgRun {
   static constexpr int32_t kGraphRows = 600000;
   FcgRun(ufvm_ctx *ctx, System &S, SolveWork &W, hipStream_t s)
       : ctx_(ctx), S_(&S), W_(&W), s_(s),
-        graphs_(!ctx->prof_on && W.n0 < kGraphRows) {}
+        graphs_(!ctx->prof_on && (S.pcg.cfg.launch_mode == 2 ||
+                                  (S.pcg.cfg.launch_mode == 0 && W.n0 < kGraphRows))) {}
   // b: right-hand side, x: solution, both device arrays in reference order
   void start(const double *b, double *x);
   bool poll(bool block);

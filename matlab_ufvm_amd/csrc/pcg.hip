@@ -521,6 +521,8 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
   if (cfg.smoother != 0 && cfg.smoother != 1) throw Error(UFVM_ERR_ARG, "smoother must be 0 or 1");
   if (!(cfg.coarse_scale > 0.0 && cfg.coarse_scale <= 4.0))
     throw Error(UFVM_ERR_ARG, "coarse_scale must be in (0, 4]");
+  if (cfg.launch_mode < 0 || cfg.launch_mode > 2)
+    throw Error(UFVM_ERR_ARG, "launch_mode must be 0, 1 or 2");
   for (int32_t l = 0; l < nlev; ++l)
     build_fast_level(ctx, S, l,
                      l == nlev - 1 ? 2
