@@ -100,6 +100,75 @@ def copy_calibration(mib):
     print(json.dumps({"copy_calibration_bytes": n, "copies": 5}), flush=True)
 
 
+def _pmc_load(d):
+    """{(kernel, grid): [value per dispatch]} from the *counter_collection.csv
+    files of a rocprofv3 --pmc run written with --output-format csv (rows of
+    one dispatch and counter are summed)."""
+    import csv
+    import glob
+    import collections
+    per = collections.defaultdict(float)
+    meta = {}
+    for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        with open(f, newline="") as fh:
+            for raw in csv.DictReader(fh):
+                row = {k.lower(): v for k, v in raw.items()}
+                did = (f, row["dispatch_id"])
+                per[did] += float(row["counter_value"])
+                meta[did] = (row["kernel_name"], int(row.get("grid_size", 0) or 0))
+    out = collections.defaultdict(list)
+    for did, v in per.items():
+        out[meta[did]].append(v)
+    return out
+
+
+def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_mib):
+    """HBM bytes per dispatch of each kernel from two rocprofv3 --pmc runs
+    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by what the same
+    counter reads for the device copies of copy_calibration (5 x copy_mib MiB
+    read and written)."""
+    moved = 5 * (copy_mib << 20)
+    scale = {}
+    for key, d in (("fetch", copy_fetch_dir), ("write", copy_write_dir)):
+        kib = sum(v for (k, _), vs in _pmc_load(d).items() if "copyBuffer" in k for v in vs)
+        scale[key] = moved / (kib * 1024.0)
+    print(f"calibration: device copies of {5} x {copy_mib} MiB; bytes = KiB * 1024 * factor, "
+          f"factor FETCH_SIZE {scale['fetch']:.3f}, WRITE_SIZE {scale['write']:.3f}")
+    fe, wr = _pmc_load(fetch_dir), _pmc_load(write_dir)
+    print(f"{'kernel':60s} {'grid':>9s} {'dispatches':>10s} {'fetch MB':>10s} {'write MB':>10s}"
+          f" {'HBM MB/dispatch':>15s}")
+    rows = []
+    for key, fv in fe.items():
+        wv = wr.get(key, [0.0])
+        fm = sum(fv) / len(fv) * 1024 * scale["fetch"] / 1e6
+        wm = sum(wv) / len(wv) * 1024 * scale["write"] / 1e6
+        rows.append((fm + wm, key, len(fv), fm, wm))
+    for tot, (name, grid), cnt, fm, wm in sorted(rows, reverse=True):
+        if tot < 0.5:
+            continue
+        short = name.split("(")[0].replace("void ", "").replace("ufvm::k::", "")
+        print(f"{short[:60]:60s} {grid:9d} {cnt:10d} {fm:10.2f} {wm:10.2f} {tot:15.2f}")
+
+
+PMC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                           "vcycle_kernels_laplacian_cavity128_pmc.txt")
+
+
+def committed_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed --pmc-summary
+    table (dispatch-weighted over its grid sizes), or None."""
+    if not os.path.exists(PMC_PROFILE):
+        return None
+    tot = cnt = 0.0
+    with open(PMC_PROFILE) as fh:
+        for line in fh:
+            parts = line.split()
+            if len(parts) >= 6 and parts[0] == kernel:
+                tot += int(parts[2]) * float(parts[5])
+                cnt += int(parts[2])
+    return tot / cnt * 1e6 if cnt else None
+
+
 def serving_bench(args):
     """Serving form: a Poisson-arrival stream of pressure solves and single
     SIMPLE steps over a pool of resident jittered hex meshes (2-D and 3-D,
@@ -313,7 +382,13 @@ def main():
     ap.add_argument("--serving-streams", type=int, default=4)
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="only run 5 device copies of this many MiB (counter calibration)")
+    ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
+                    help="only summarise four rocprofv3 --pmc output folders (CSV): "
+                         "FETCH_SIZE and WRITE_SIZE of a run and of --copy-calibration-mib")
     args = ap.parse_args()
+    if args.pmc_summary:
+        pmc_summary(*args.pmc_summary, args.copy_calibration_mib or 256)
+        return
     if args.copy_calibration_mib > 0:
         copy_calibration(args.copy_calibration_mib)
         return
@@ -409,6 +484,7 @@ def main():
     avg_ms = ms0 / max(n0, 1)
     achieved = (by0 / max(n0, 1)) / (avg_ms * 1e-3) if n0 else 0.0
     value = args.steps / r32["total"]
+    traffic = committed_traffic("k_mcgs_sell<float>") if args.n == 128 else None
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -434,10 +510,14 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK, 4),
                      "avg_launch_us": round(avg_ms * 1e3, 2), "launches": int(n0),
                      "algorithmic_bytes_per_launch": round(by0 / max(n0, 1)),
-                     "traffic": None,
-                     "measured": f"HIP events on the library stream around every 5th "
-                                 f"launch, {args.event_steps} solves re-run after the "
-                                 f"timed ones"},
+                     "traffic": (None if traffic is None else round(traffic)),
+                     "traffic_source": (None if traffic is None else
+                                        "rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, "
+                                        "copy-calibrated: profiles/" +
+                                        os.path.basename(PMC_PROFILE)),
+                     "measured": f"HIP events on the library stream around the back-to-back "
+                                 f"colour passes of every 5th finest-level smoothing call, "
+                                 f"{args.event_steps} solves re-run after the timed ones"},
     }
     ms1, n1, by1 = r32["prof_spmv"]
     if n1:

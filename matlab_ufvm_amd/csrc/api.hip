@@ -644,7 +644,7 @@ int ufvm_prof_read(ufvm_ctx *ctx, const char *kind, double *total_ms, int64_t *l
       UFVM_HIP(hipEventElapsedTime(&t, r.a, r.b));
       ms += t;
       by += r.bytes;
-      ++cnt;
+      cnt += r.launches;
     }
     if (total_ms) *total_ms = ms;
     if (launches) *launches = cnt;

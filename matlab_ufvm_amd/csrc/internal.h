@@ -62,6 +62,16 @@ struct ProfScope {
     rec.b = pool_event(ctx);
     (void)hipEventRecord(rec.a, s);
   }
+  // a scope spanning several back-to-back launches of one kernel class:
+  // begin with bytes 0, then add(bytes) once per launch
+  void add(double b) {
+    if (!on) return;
+    if (!added) rec.launches = 0;
+    added = true;
+    rec.bytes += b;
+    ++rec.launches;
+  }
+  bool added = false;
   ~ProfScope() {
     if (!on) return;
     (void)hipEventRecord(rec.b, s);

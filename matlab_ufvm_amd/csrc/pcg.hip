@@ -261,16 +261,17 @@ struct Cycle {
   void mcgs_sweeps(const FastLevel &L, LevelWork &LW, int nsweep, bool from_zero,
                    bool first_done) {
     const double w = sizeof(V);
+    // one event pair around the back-to-back colour passes of this call (the
+    // event overhead is spread over several launches)
+    ProfScope ps(ctx, s, "mcgs0", 0.0);
     auto pass = [&](int c, int32_t zero_above) {
       const int32_t r0 = L.col_off[c], r1 = L.col_off[c + 1];
-      double bytes = 0.0;
       if (ctx->prof_on) {
         // algorithmic bytes: the colour's entries (col + value), its ac, b,
         // x write, and one read of every other unknown
         const double R = r1 - r0, Z = L.h_ptr[r1] - L.h_ptr[r0];
-        bytes = (4.0 + w) * Z + 3.0 * w * R + w * (L.n - R);
+        ps.add((4.0 + w) * Z + 3.0 * w * R + w * (L.n - R));
       }
-      ProfScope ps(ctx, s, "mcgs0", bytes);
       k::mcgs_sell<V>(s, r0, r1, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW), P::x(LW),
                       zero_above);
     };

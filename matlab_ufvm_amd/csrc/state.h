@@ -200,6 +200,7 @@ struct ProfRec {
   std::string kind;
   hipEvent_t a, b;
   double bytes;
+  int64_t launches = 1;   // kernel launches between the two events
 };
 
 }  // namespace ufvm
