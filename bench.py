@@ -194,7 +194,8 @@ def serving_bench(args):
         ptr, col, ac, anb = poisson_system(m, dirichlet=())
         sids.append(srv.add_system(ptr, col, ac, anb, tol=1e-8))
         cids.append(srv.add_case(m, cavity_bcs(m), nu=1.0 / 400.0, p_cycle="V-Cycle",
-                                 p_pre=2, p_post=2, p_smoother="SOR"))
+                                 p_pre=2, p_post=2, p_smoother="SOR",
+                                 u_smoother=args.serving_u_smoother))
     log(f"serving: systems and cases resident ({time.time()-t:.1f}s)")
     sizes = [m.n_cells for _, m in pool]
     # one right-hand side per mesh, made before the stream starts
@@ -244,7 +245,7 @@ def simple_steps_bench(args):
     fields, res = {}, {}
     for prec in (32, 64):
         case = SimpleCase(ctx, m, cavity_bcs(m), nu=1.0 / 400.0, p_solver=args.p_solver,
-                          pcg_precision=prec, pcg_tol=1e-8)
+                          pcg_precision=prec, pcg_tol=1e-8, u_smoother=args.u_smoother)
         ctx.synchronize()
         t = time.perf_counter()
         rms = case.iterate(args.simple_steps)
@@ -267,7 +268,9 @@ def simple_steps_bench(args):
            "fp32": res[32], "fp64": res[64], "field_rel_l2_diff": err,
            "config": {"workload": f"3-D lid-driven cavity {args.n}^3 jittered hex (+-15%, seed "
                                   f"2026), permuted+RCM (seed 2027), own-face apex, Re 400, "
-                                  f"SIMPLE URF 0.7/0.3, 2 pressure correctors, momentum DILU",
+                                  f"SIMPLE URF 0.7/0.3, 2 pressure correctors, momentum "
+                                  f"{'DILU' if args.u_smoother == 'ILU' else 'multicolour GS'} "
+                                  f"(10 sweeps, 0.1 reduction)",
                       "cells": int(m.n_cells), "pressure_tol": 1e-8,
                       "hierarchy": ("rebuilt every pressure solve" if args.p_solver == "pcg"
                                     else "first solve's parents kept, Galerkin operators and "
@@ -370,6 +373,8 @@ def main():
                     help="timed V-cycles of the plain FP64 drop-in AMG (0: skip)")
     ap.add_argument("--simple-steps", type=int, default=0,
                     help="run N SIMPLE iterations with FP32 and FP64 pressure solves instead")
+    ap.add_argument("--u-smoother", choices=["ILU", "MCGS"], default="ILU",
+                    help="momentum solves of --simple-steps: the reference's DILU or multicolour GS")
     ap.add_argument("--p-solver", choices=["pcg", "pcg-frozen"], default="pcg",
                     help="SIMPLE mode: rebuild the hierarchy every solve, or keep the first")
     ap.add_argument("--serving", action="store_true",
@@ -377,6 +382,8 @@ def main():
     ap.add_argument("--serving-meshes", default="3d:37,3d:64,3d:100,3d:128,2d:224,2d:512,"
                                                 "2d:1024,2d:2048")
     ap.add_argument("--serving-requests", type=int, default=200)
+    ap.add_argument("--serving-u-smoother", choices=["ILU", "MCGS"], default="MCGS",
+                    help="momentum solves of the SIMPLE-step requests")
     ap.add_argument("--serving-rate", type=float, default=200.0)
     ap.add_argument("--serving-active", type=int, default=8)
     ap.add_argument("--serving-streams", type=int, default=4)

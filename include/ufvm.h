@@ -39,6 +39,10 @@ extern "C" {
 
 #define UFVM_SMOOTHER_ILU 0    /* 'ILU' (DILU) -- cfdSolveAlgebraicSystem.m:29-46 */
 #define UFVM_SMOOTHER_SOR 1    /* 'SOR' (symmetric GS) -- cfdSolveAlgebraicSystem.m:47-62 */
+#define UFVM_SMOOTHER_MCGS 2   /* symmetric multicolour GS: greedy first-fit colouring in index
+                                  order, colours ascending then descending; the colour classes
+                                  run in parallel (no reference counterpart: the GPU form of
+                                  'SOR' for solves that need not follow the reference's order) */
 
 #define UFVM_CYCLE_V 0
 #define UFVM_CYCLE_W 1
@@ -221,6 +225,8 @@ typedef struct {
   int32_t p_solver;
   ufvm_amg_cfg p_amg;
   ufvm_pcg_cfg p_pcg;
+  int32_t u_smoother;                     /* momentum solves: UFVM_SMOOTHER_ILU (the
+                                             reference's, default) or UFVM_SMOOTHER_MCGS */
 } ufvm_simple_cfg;
 
 /* Upload a mesh and create its fields (uniform zero initial conditions,

@@ -16,7 +16,7 @@ LIB_PATH = os.path.join(HERE, "libufvm.so")
 UFVM_OK = 0
 ERRORS = {1: "UFVM_ERR_ARG", 2: "UFVM_ERR_HIP", 3: "UFVM_ERR_STATE", 4: "UFVM_ERR_NUMERIC",
           5: "UFVM_ERR_UNSUPPORTED"}
-SMOOTHER = {"ILU": 0, "SOR": 1}
+SMOOTHER = {"ILU": 0, "SOR": 1, "MCGS": 2}
 PCG_SMOOTHER = {"MCGS": 0, "ILU": 1}
 CYCLE = {"V-Cycle": 0, "W-Cycle": 1, "F-Cycle": 2}
 
@@ -66,7 +66,7 @@ class SimpleCfg(C.Structure):
     _fields_ = [("nu", C.c_double), ("urf_u", C.c_double), ("urf_p", C.c_double),
                 ("n_correctors", C.c_int32), ("p_ref_cell", C.c_int32), ("u_sweeps", C.c_int32),
                 ("u_bc", i32p), ("u_value", f64p), ("p_bc", i32p), ("p_solver", C.c_int32),
-                ("p_amg", AmgCfg), ("p_pcg", PcgCfg)]
+                ("p_amg", AmgCfg), ("p_pcg", PcgCfg), ("u_smoother", C.c_int32)]
 
 
 # name -> (restype, argtypes)

@@ -96,6 +96,31 @@ void sor_sweep(ufvm_ctx *ctx, RefLevel &L) {
                L.anb.p, L.bc.p, L.dphi.p);
 }
 
+// Symmetric multicolour Gauss-Seidel sweep in the reference numbering: the
+// classes of a greedy first-fit colouring (index order) upward, then
+// downward without the last class (re-solving it repeats its arithmetic).
+// Each row is the reference's Gauss-Seidel row update (k_gs_rows).
+void mcgs_sweep(ufvm_ctx *ctx, RefLevel &L) {
+  if (!L.mc) {
+    std::vector<int32_t> colour;
+    const int32_t nc = greedy_colour(L.g, colour);
+    L.mc_ptr.assign(nc + 1, 0);
+    for (int32_t c : colour) ++L.mc_ptr[c + 1];
+    for (int32_t c = 0; c < nc; ++c) L.mc_ptr[c + 1] += L.mc_ptr[c];
+    std::vector<int32_t> rows(std::max(L.n, 1)), at(L.mc_ptr.begin(), L.mc_ptr.end() - 1);
+    for (int32_t i = 0; i < L.n; ++i) rows[at[colour[i]]++] = i;
+    L.mc_rows.upload(rows, ctx->stream);
+    L.mc = true;
+  }
+  const int32_t nc = (int32_t)L.mc_ptr.size() - 1;
+  auto pass = [&](int32_t c) {
+    k::gs_rows(ctx->stream, L.mc_rows.p + L.mc_ptr[c], L.mc_ptr[c + 1] - L.mc_ptr[c], L.ptr.p,
+               L.col.p, L.ac.p, L.anb.p, L.bc.p, L.dphi.p);
+  };
+  for (int32_t c = 0; c < nc; ++c) pass(c);
+  for (int32_t c = nc - 2; c >= 0; --c) pass(c);
+}
+
 void dilu_factor(ufvm_ctx *ctx, RefLevel &L) {
   if (!L.low_ok) throw Error(UFVM_ERR_NUMERIC, "ILU: the index for i in j is not found");
   const auto &F = L.fwd.level_ptr;
@@ -122,9 +147,10 @@ void dilu_sweep(ufvm_ctx *ctx, RefLevel &L) {
 // cfdSolveAlgebraicSystem.m:11-66
 void solve_alg(ufvm_ctx *ctx, RefLevel &L, int smoother, int n_iter, double rrf, double *r0,
                double *r1) {
-  if (smoother != UFVM_SMOOTHER_ILU && smoother != UFVM_SMOOTHER_SOR)
-    throw Error(UFVM_ERR_ARG, "smoother must be ILU or SOR");
-  ensure_schedule(ctx, L);
+  if (smoother != UFVM_SMOOTHER_ILU && smoother != UFVM_SMOOTHER_SOR &&
+      smoother != UFVM_SMOOTHER_MCGS)
+    throw Error(UFVM_ERR_ARG, "smoother must be ILU, SOR or MCGS");
+  if (smoother != UFVM_SMOOTHER_MCGS) ensure_schedule(ctx, L);
   const double initial = residual_l1(ctx, L, L.r.p) / L.n;
   double fin = initial;
   if (std::isnan(initial)) throw Error(UFVM_ERR_NUMERIC, "NaN residual");
@@ -132,7 +158,8 @@ void solve_alg(ufvm_ctx *ctx, RefLevel &L, int smoother, int n_iter, double rrf,
   int iter = 1;
   while (iter <= n_iter && fin > rrf * initial) {
     if (smoother == UFVM_SMOOTHER_ILU) dilu_sweep(ctx, L);
-    else sor_sweep(ctx, L);
+    else if (smoother == UFVM_SMOOTHER_SOR) sor_sweep(ctx, L);
+    else mcgs_sweep(ctx, L);
     fin = residual_l1(ctx, L, L.r.p) / L.n;
     ++iter;
   }

@@ -534,6 +534,7 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
   build_coarsest(ctx, S);
   UFVM_HIP(hipStreamSynchronize(ctx->stream));
   P.ready = true;
+  P.stale = false;
   P.work0 = make_work(S, ctx->stream);
 }
 
@@ -546,6 +547,7 @@ void pcg_refresh(ufvm_ctx *ctx, System &S) {
   for (int32_t l = 0; l < P.nlev; ++l) refresh_level(ctx, S, l);
   build_coarsest(ctx, S);
   UFVM_HIP(hipStreamSynchronize(ctx->stream));
+  P.stale = false;
 }
 
 // ---------------------------------------------------------------- FCG solve
@@ -586,7 +588,8 @@ void FcgRun::start_t(const double *b) {
 }
 
 void FcgRun::start(const double *b, double *x) {
-  if (!S_->pcg.ready) throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first");
+  if (!S_->pcg.ready || S_->pcg.stale)
+    throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first (or refresh after new coefficients)");
   if (!S_->pcg.lev[0]->soff.p) throw Error(UFVM_ERR_STATE, "level 0 has no sliced-ELL copy");
   x_out_ = x;
   it_ = issued_ = 0;
@@ -710,7 +713,8 @@ bool FcgRun::poll(bool block) {
 
 // ufvm_pcg_solve: the level-1 system's bc in, dphi out, on the context stream
 void pcg_solve(ufvm_ctx *ctx, System &S, int32_t *iters, double *relres) {
-  if (!S.pcg.ready || !S.pcg.work0) throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first");
+  if (!S.pcg.ready || S.pcg.stale || !S.pcg.work0)
+    throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first (or refresh after new coefficients)");
   RefLevel &R0 = *S.lev[0];
   FcgRun run(ctx, S, *S.pcg.work0, ctx->stream);
   run.start(R0.bc.p, R0.dphi.p);
@@ -734,7 +738,8 @@ static void apply_vcycle_t(ufvm_ctx *ctx, System &S, SolveWork &W) {
 }
 
 void pcg_apply_vcycle(ufvm_ctx *ctx, System &S, const double *r, double *z) {
-  if (!S.pcg.ready || !S.pcg.work0) throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first");
+  if (!S.pcg.ready || S.pcg.stale || !S.pcg.work0)
+    throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first (or refresh after new coefficients)");
   if (!r || !z) throw Error(UFVM_ERR_ARG, "null vector");
   SolveWork &W = *S.pcg.work0;
   hipStream_t s = ctx->stream;

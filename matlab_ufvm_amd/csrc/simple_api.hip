@@ -160,8 +160,10 @@ void assemble(ufvm_ctx *ctx, Case &C, int eq, int comp, double *rms, double *mx)
     sk::assemble_p(s, C.dm, C.fluxC1.p, C.fluxC2.p, C.mdot.p, L.ac.p, L.anb.p, L.bc.p, C.ac_old.p,
                    L.dphi.p, ctx->partial.p, ctx->counter.p, ctx->scal.p);
   }
+  // new values: the coarse operators and the solve-side copies are stale (a
+  // kept hierarchy is updated from them by galerkin_update + pcg_refresh)
   S.hier_valid = false;
-  S.pcg.ready = false;
+  S.pcg.stale = true;
   read_scal(ctx, sk::S_RMAX, 2);
   const double m = ctx->hscal[sk::S_RMAX], sq = ctx->hscal[sk::S_RSQ];
   if (mx) *mx = m;
@@ -172,7 +174,7 @@ void assemble(ufvm_ctx *ctx, Case &C, int eq, int comp, double *rms, double *mx)
 void solve(ufvm_ctx *ctx, Case &C, int eq, double *r0, double *r1, int32_t *nc) {
   if (eq == 0) {
     System &S = sys_of(ctx, C.sysU);
-    solve_alg(ctx, *S.lev[0], UFVM_SMOOTHER_ILU, C.cfg.u_sweeps, 0.1, r0, r1);
+    solve_alg(ctx, *S.lev[0], C.cfg.u_smoother, C.cfg.u_sweeps, 0.1, r0, r1);
     if (nc) *nc = 0;
     return;
   }
@@ -232,6 +234,8 @@ int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *m, const ufvm_simple_c
     C.Fb = C.F - C.Fi;
     C.P = m->n_patches;
     C.cfg = *cfg;
+    if (cfg->u_smoother != UFVM_SMOOTHER_ILU && cfg->u_smoother != UFVM_SMOOTHER_MCGS)
+      throw Error(UFVM_ERR_ARG, "u_smoother must be UFVM_SMOOTHER_ILU or UFVM_SMOOTHER_MCGS");
     // boundary faces -> patch, and patch checks
     std::vector<int32_t> bp(C.Fb, -1);
     for (int p = 0; p < C.P; ++p) {

@@ -105,6 +105,11 @@ struct RefLevel {
   bool low_ok = false;
   LowerLists low;
   DBuf<int32_t> lptr, lnb, lpji, lpij;
+  // multicolour GS: rows of colour c are mc_rows[mc_ptr[c] .. mc_ptr[c+1]),
+  // increasing index; built on first use
+  bool mc = false;
+  std::vector<int32_t> mc_ptr;
+  DBuf<int32_t> mc_rows;
 };
 
 // One level of the precision-form V-cycle: unknowns renumbered so that each
@@ -179,7 +184,8 @@ struct SolveWork {
 };
 
 struct PcgState {
-  bool ready = false;
+  bool ready = false;        // layouts, colourings and workspace built for the hierarchy
+  bool stale = false;        // level-1 values changed since the last setup or refresh
   ufvm_pcg_cfg cfg{};
   int32_t nlev = 0;          // levels used; nlev-1 solved densely
   int32_t singular = 0;

@@ -74,7 +74,11 @@ class Server:
         return sid.value
 
     def add_case(self, mesh, bcs, nu, **kw) -> int:
+        """A resident SIMPLE case; keyword arguments as SimpleCase.  Serving
+        defaults: FP32-preconditioned AMG-PCG pressure solves on the first
+        step's hierarchy ("pcg-frozen") and multicolour-GS momentum solves."""
         kw.setdefault("p_solver", "pcg-frozen")
+        kw.setdefault("u_smoother", "MCGS")
         d, c, keep = case_structs(mesh, bcs, nu, **kw)
         cid = C.c_int32()
         self.check(self.lib.ufvm_server_add_case(self.h, C.byref(d), C.byref(c), C.byref(cid)))

@@ -142,7 +142,8 @@ class System:
 
     def solve_algebraic_system(self, l, smoother, n_iter, rrf=0.1):
         r0, r1 = C.c_double(), C.c_double()
-        lib().orc_solve_algebraic_system(C.byref(self.H.lev[l]), {"ILU": 0, "SOR": 1}[smoother],
+        lib().orc_solve_algebraic_system(C.byref(self.H.lev[l]),
+                                         {"ILU": 0, "SOR": 1, "MCGS": 2}[smoother],
                                          n_iter, rrf, C.byref(r0), C.byref(r1))
         return r0.value, r1.value
 
@@ -233,7 +234,7 @@ class OCfg(C.Structure):
                 ("n_correctors", C.c_int32), ("p_ref", C.c_int32), ("u_sweeps", C.c_int32),
                 ("p_cycle", C.c_int32), ("p_max_cycles", C.c_int32), ("p_max_coarse", C.c_int32),
                 ("p_pre", C.c_int32), ("p_post", C.c_int32), ("p_smoother", C.c_int32),
-                ("p_termination", C.c_double)]
+                ("p_termination", C.c_double), ("u_smoother", C.c_int32)]
 
 
 class OState(C.Structure):
@@ -258,7 +259,7 @@ class SimpleCase:
 
     def __init__(self, mesh, bcs, nu, urf_u=0.7, urf_p=0.3, n_correctors=2, p_ref=0,
                  u_sweeps=10, p_cycle="W-Cycle", p_max_cycles=30, p_termination=0.1,
-                 p_max_coarse=10, p_pre=1, p_post=3, p_smoother="ILU"):
+                 p_max_coarse=10, p_pre=1, p_post=3, p_smoother="ILU", u_smoother="ILU"):
         L = lib()
         for name, args in [("orc_simple_init", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState)]),
                            ("orc_simple_free", [C.POINTER(OState)]),
@@ -299,7 +300,8 @@ class SimpleCase:
         self.M = M
         self.cfg = OCfg(nu, urf_u, urf_p, n_correctors, p_ref, u_sweeps,
                         {"V-Cycle": 0, "W-Cycle": 1, "F-Cycle": 2}[p_cycle], p_max_cycles,
-                        p_max_coarse, p_pre, p_post, {"ILU": 0, "SOR": 1}[p_smoother], p_termination)
+                        p_max_coarse, p_pre, p_post, {"ILU": 0, "SOR": 1}[p_smoother], p_termination,
+                        {"ILU": 0, "MCGS": 2}[u_smoother])
         self.S = OState()
         L.orc_simple_init(C.byref(self.M), C.byref(self.cfg), C.byref(self.S))
 

@@ -311,7 +311,7 @@ void orc_solve_algebraic_system(orc_level *L, int smoother, int nIter, double rr
       ++iter;
     }
     free(dc); free(rc);
-  } else {
+  } else if (smoother == 1) {
     int iter = 1;
     while (iter <= nIter && final_ > rrf * initial) {
       orc_sor(L, L->dphi);
@@ -319,6 +319,19 @@ void orc_solve_algebraic_system(orc_level *L, int smoother, int nIter, double rr
       final_ = l1(r, n) / n;
       ++iter;
     }
+  } else {
+    /* symmetric multicolour GS (no reference counterpart): the same
+       termination loop with orc_mcgs sweeps over a greedy colouring */
+    int32_t *colour = (int32_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int32_t));
+    const int nc = orc_colour(n, L->ptr, L->col, colour);
+    int iter = 1;
+    while (iter <= nIter && final_ > rrf * initial) {
+      orc_mcgs(L, colour, nc, L->bc, L->dphi);
+      orc_residual(L, r);
+      final_ = l1(r, n) / n;
+      ++iter;
+    }
+    free(colour);
   }
   free(r);
   if (r0) *r0 = initial;

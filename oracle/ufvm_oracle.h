@@ -54,7 +54,8 @@ void orc_factorize_ilu(const orc_level *L, double *dc, double *rc);
 /* cfdSolveILU.m:14-93 */
 void orc_solve_ilu(const orc_level *L, const double *dc, double *rc, double *dphi);
 
-/* cfdSolveAlgebraicSystem.m:11-66; smoother 0 = 'ILU', 1 = 'SOR' */
+/* cfdSolveAlgebraicSystem.m:11-66; smoother 0 = 'ILU', 1 = 'SOR', 2 = symmetric
+ * multicolour GS (orc_mcgs over orc_colour; no reference counterpart) */
 void orc_solve_algebraic_system(orc_level *L, int smoother, int nIter, double rrf,
                                 double *r0, double *r1);
 

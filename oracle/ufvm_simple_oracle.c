@@ -381,7 +381,8 @@ void orc_assemble_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
 /* cfdSolver.m:11-12 -> cfdSolveAlgebraicSystem(1, 'ILU', 10) */
 void orc_solve_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s, int comp) {
   (void)m;
-  orc_solve_algebraic_system(&s->H.lev[0], 0, c->u_sweeps, 0.1, &s->r0[comp], &s->r1[comp]);
+  orc_solve_algebraic_system(&s->H.lev[0], c->u_smoother, c->u_sweeps, 0.1, &s->r0[comp],
+                             &s->r1[comp]);
 }
 
 /* cfdCorrectScalarEquation.m:16-24 (U(:,c) += dphi) and its empty-patch

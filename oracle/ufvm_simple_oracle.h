@@ -44,6 +44,7 @@ typedef struct {
   /* pressure-correction AMG (cfdApplyAMG settings) */
   int32_t p_cycle, p_max_cycles, p_max_coarse, p_pre, p_post, p_smoother;
   double p_termination;
+  int32_t u_smoother;     /* momentum solve: 0 the reference's DILU, 2 multicolour GS */
 } orc_simple_cfg;
 
 typedef struct {

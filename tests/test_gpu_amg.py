@@ -80,7 +80,7 @@ def test_hierarchy_bitexact(ctx, case):
     ctx.system_destroy(sid)
 
 
-@pytest.mark.parametrize("smoother", ["SOR", "ILU"])
+@pytest.mark.parametrize("smoother", ["SOR", "ILU", "MCGS"])
 def test_solve_algebraic_system_bitexact(ctx, case, smoother):
     _, m, ptr, col, ac, anb, b = case
     sid = make(ctx, ptr, col, ac, anb, b)

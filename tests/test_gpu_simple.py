@@ -119,3 +119,34 @@ def test_precision_form_pressure_solver(ctx):
     assert np.linalg.norm(U1 - U2) <= 1e-6 * np.linalg.norm(U1)
     ref.close()
     pcg.close()
+
+
+def test_mcgs_momentum_iterations_bitexact(ctx):
+    # the multicolour-GS momentum solve (u_smoother "MCGS") on the GPU gives
+    # the oracle's fields bit for bit over whole SIMPLE iterations
+    m = cavity_mesh(12, jitter=0.15, seed=65, renumber_seed=66)
+    g = SimpleCase(ctx, m, cavity_bcs(m), nu=0.0025, u_smoother="MCGS", **AMG)
+    o = oracle.SimpleCase(m, oracle.cavity_bcs(m), nu=0.0025, u_smoother="MCGS", **AMG)
+    res = g.iterate(5)
+    for _ in range(5):
+        o.iterate(1)
+    for name, oname in (("U", "U"), ("p", "p"), ("mdot_f", "mdot")):
+        same(g.field(name), o.field(oname), f"MCGS momentum {name}")
+    assert np.all(np.isfinite(res))
+    g.close()
+
+
+def test_frozen_hierarchy_pressure_solver(ctx):
+    # "pcg-frozen" keeps the first solve's hierarchy and refreshes its values
+    # on the GPU after every assembly; it reaches the fields of the
+    # rebuild-every-solve path to within the solver tolerances
+    m = cavity_mesh(20, nz=1, jitter=0.1, seed=67, renumber_seed=68)
+    a = SimpleCase(ctx, m, cavity_bcs(m), nu=0.01, p_solver="pcg", pcg_tol=1e-10)
+    b = SimpleCase(ctx, m, cavity_bcs(m), nu=0.01, p_solver="pcg-frozen", pcg_tol=1e-10)
+    ra = a.iterate(20)
+    rb = b.iterate(20)
+    assert np.allclose(ra[0], rb[0], rtol=1e-9)
+    U1, U2 = a.field("U"), b.field("U")
+    assert np.linalg.norm(U1 - U2) <= 1e-6 * np.linalg.norm(U1)
+    a.close()
+    b.close()

@@ -199,6 +199,38 @@ def test_sor_is_forward_then_backward_gauss_seidel(poisson16):
     assert np.abs(S.level(0)["dphi"] - x2).max() <= 1e-12 * np.abs(x2).max()
 
 
+def test_mcgs_solve_is_symmetric_gauss_seidel_in_colour_order(poisson16):
+    # One MCGS sweep is forward then backward Gauss-Seidel in the ordering
+    # that lists the colour classes one after another (rows of one class are
+    # uncoupled, so their order inside the class does not matter): scipy
+    # triangular solves on the permuted matrix reproduce it.
+    m, ptr, col, ac, anb, b = poisson16
+    A = to_scipy(ptr, col, ac, anb).tocsr()
+    n = m.n_cells
+    colour = np.full(n, -1)
+    for i in range(n):                          # greedy first fit in index order
+        used = {colour[j] for j in col[ptr[i]:ptr[i + 1]] if colour[j] >= 0}
+        c = 0
+        while c in used:
+            c += 1
+        colour[i] = c
+    order = np.argsort(colour, kind="stable")
+    P = A[order][:, order]
+    x0 = np.random.default_rng(2).standard_normal(n)
+    S = oracle.System(ptr, col, ac, anb, bc=b, dphi=x0)
+    S.solve_algebraic_system(0, "MCGS", 1, rrf=0.0)
+    bp, xp = b[order], x0[order]
+    x1 = spl.spsolve_triangular(sp.tril(P, 0, format="csr"), bp - sp.triu(P, 1) @ xp, lower=True)
+    x2 = spl.spsolve_triangular(sp.triu(P, 0, format="csr"), bp - sp.tril(P, -1) @ x1, lower=False)
+    got = S.level(0)["dphi"][order]
+    assert colour.max() >= 1
+    assert np.abs(got - x2).max() <= 1e-12 * np.abs(x2).max()
+    # and many sweeps converge to the solution
+    S.solve_algebraic_system(0, "MCGS", 4000, rrf=0.0)
+    xs = spl.spsolve(A.tocsc(), b)
+    assert np.abs(S.level(0)["dphi"] - xs).max() <= 1e-7 * np.abs(xs).max()
+
+
 def ilu0_pivots(A):
     """ILU(0) by IKJ elimination on a dense copy, dropping fill outside the pattern."""
     D = A.toarray()
