@@ -95,8 +95,10 @@ void permute_gather(hipStream_t s, int32_t n, const int32_t *idx, const double *
 void permute_scatter(hipStream_t s, int32_t n, const int32_t *idx, const double *src,
                      double *dst);
 
-// dense Gauss-Jordan inverse in place (row-major n x n), no pivoting
-void gj_inverse(hipStream_t s, int32_t n, double *a, double *colbuf);
+// dense Gauss-Jordan inverse in place (row-major n x n), no pivoting, in
+// blocks of 64 columns; scratch holds gj_scratch(n) doubles
+size_t gj_scratch(int32_t n);
+void gj_inverse(hipStream_t s, int32_t n, double *a, double *scratch);
 // a = csr (+ shift*11^T) as dense row-major
 void csr_to_dense(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col,
                   const double *ac, const double *anb, double shift, double *a);

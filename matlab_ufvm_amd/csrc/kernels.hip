@@ -687,32 +687,94 @@ void csr_to_dense(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *c
   hipLaunchKernelGGL(k_csr_to_dense, dim3(n), dim3(kBlock), 0, s, n, ptr, col, ac, anb, shift, a);
 }
 
-// step k, part 1: save column k, scale pivot row (pivot entry becomes 1/p)
-__global__ void k_gj_pivot(int32_t n, int32_t kk, double *a, double *colbuf) {
-  const double p = a[(int64_t)kk * n + kk];
-  for (int32_t i = threadIdx.x; i < n; i += blockDim.x) colbuf[i] = a[(int64_t)i * n + kk];
+// ---- blocked Gauss-Jordan inversion (no pivoting: the coarsest operators are
+// symmetric positive definite, shifted when singular):
+// per diagonal block K of at most kGjB rows, P = A_KK^-1, then
+//   A_KJ <- P A_KJ,  A_IJ <- A_IJ - A_IK (P A_KJ),  A_IK <- -A_IK P,  A_KK <- P
+// for I, J outside K: one block step is kGjB one-column steps at once, and
+// the rank-kGjB update replaces kGjB launches over the whole matrix.
+
+constexpr int kGjB = 64;
+
+// P = inverse of the b x b diagonal block at (k0, k0): one-column Gauss-Jordan
+// steps in LDS (save column k, scale row k so the pivot becomes 1/p, subtract
+// column k's multiple of row k from every other row)
+__global__ void __launch_bounds__(kBlock)
+k_gjb_diag(int32_t n, int32_t k0, int32_t b, const double *__restrict__ a,
+           double *__restrict__ pbuf) {
+  __shared__ double t[kGjB][kGjB + 1];
+  __shared__ double colv[kGjB];
+  for (int32_t e = threadIdx.x; e < b * b; e += kBlock)
+    t[e / b][e % b] = a[(int64_t)(k0 + e / b) * n + k0 + e % b];
   __syncthreads();
-  const double inv = 1.0 / p;
-  for (int32_t j = threadIdx.x; j < n; j += blockDim.x) {
-    double v = (j == kk) ? 1.0 : a[(int64_t)kk * n + j];
-    a[(int64_t)kk * n + j] = v * inv;
+  for (int32_t k = 0; k < b; ++k) {
+    const double inv = 1.0 / t[k][k];
+    for (int32_t i = threadIdx.x; i < b; i += kBlock) colv[i] = t[i][k];
+    __syncthreads();
+    for (int32_t j = threadIdx.x; j < b; j += kBlock) t[k][j] = ((j == k) ? 1.0 : t[k][j]) * inv;
+    __syncthreads();
+    for (int32_t e = threadIdx.x; e < b * b; e += kBlock) {
+      const int32_t i = e / b, j = e % b;
+      if (i != k) t[i][j] = ((j == k) ? 0.0 : t[i][j]) - colv[i] * t[k][j];
+    }
+    __syncthreads();
   }
+  for (int32_t e = threadIdx.x; e < b * b; e += kBlock) pbuf[e] = t[e / b][e % b];
 }
 
-// step k, part 2: eliminate column k from every other row
-__global__ void k_gj_elim(int32_t n, int32_t kk, double *a, const double *colbuf) {
+// rbuf[s][j] = sum_r P[s][r] A[k0+r][j] (the row panel times P), and
+// cbuf[i][s] = A[i][k0+s] (the old column panel); grid (columns, b)
+__global__ void __launch_bounds__(kBlock)
+k_gjb_panels(int32_t n, int32_t k0, int32_t b, const double *__restrict__ a,
+             const double *__restrict__ pbuf, double *__restrict__ rbuf,
+             double *__restrict__ cbuf) {
+  const int32_t j = blockIdx.x * kBlock + threadIdx.x;
+  const int32_t s = blockIdx.y;
+  if (j >= n) return;
+  double v = 0.0;
+  for (int32_t r = 0; r < b; ++r) v += pbuf[s * b + r] * a[(int64_t)(k0 + r) * n + j];
+  rbuf[(int64_t)s * n + j] = v;
+  cbuf[(int64_t)j * b + s] = a[(int64_t)j * n + k0 + s];
+}
+
+// every element from the panels: grid (columns, rows)
+__global__ void __launch_bounds__(kBlock)
+k_gjb_update(int32_t n, int32_t k0, int32_t b, double *__restrict__ a,
+             const double *__restrict__ pbuf, const double *__restrict__ rbuf,
+             const double *__restrict__ cbuf) {
+  __shared__ double ci[kGjB];
   const int32_t i = blockIdx.y;
   const int32_t j = blockIdx.x * kBlock + threadIdx.x;
-  if (i == kk || j >= n) return;
-  const double f = colbuf[i];
-  const double base = (j == kk) ? 0.0 : a[(int64_t)i * n + j];
-  a[(int64_t)i * n + j] = base - f * a[(int64_t)kk * n + j];
+  for (int32_t s = threadIdx.x; s < b; s += kBlock) ci[s] = cbuf[(int64_t)i * b + s];
+  __syncthreads();
+  if (j >= n) return;
+  const bool ik = i >= k0 && i < k0 + b, jk = j >= k0 && j < k0 + b;
+  double v;
+  if (ik && jk) {
+    v = pbuf[(i - k0) * b + (j - k0)];
+  } else if (ik) {
+    v = rbuf[(int64_t)(i - k0) * n + j];
+  } else if (jk) {
+    v = 0.0;
+    for (int32_t s = 0; s < b; ++s) v -= ci[s] * pbuf[s * b + (j - k0)];
+  } else {
+    v = a[(int64_t)i * n + j];
+    for (int32_t s = 0; s < b; ++s) v -= ci[s] * rbuf[(int64_t)s * n + j];
+  }
+  a[(int64_t)i * n + j] = v;
 }
 
-void gj_inverse(hipStream_t s, int32_t n, double *a, double *colbuf) {
-  for (int32_t kk = 0; kk < n; ++kk) {
-    hipLaunchKernelGGL(k_gj_pivot, dim3(1), dim3(kBlock), 0, s, n, kk, a, colbuf);
-    hipLaunchKernelGGL(k_gj_elim, dim3(nblocks(n), n), dim3(kBlock), 0, s, n, kk, a, colbuf);
+size_t gj_scratch(int32_t n) { return (size_t)kGjB * kGjB + 2 * (size_t)kGjB * n + 8; }
+
+void gj_inverse(hipStream_t s, int32_t n, double *a, double *scratch) {
+  double *pbuf = scratch, *rbuf = pbuf + kGjB * kGjB, *cbuf = rbuf + (size_t)kGjB * n;
+  for (int32_t k0 = 0; k0 < n; k0 += kGjB) {
+    const int32_t b = n - k0 < kGjB ? n - k0 : kGjB;
+    hipLaunchKernelGGL(k_gjb_diag, dim3(1), dim3(kBlock), 0, s, n, k0, b, a, pbuf);
+    hipLaunchKernelGGL(k_gjb_panels, dim3(nblocks(n), b), dim3(kBlock), 0, s, n, k0, b, a, pbuf,
+                       rbuf, cbuf);
+    hipLaunchKernelGGL(k_gjb_update, dim3(nblocks(n), n), dim3(kBlock), 0, s, n, k0, b, a, pbuf,
+                       rbuf, cbuf);
   }
 }
 

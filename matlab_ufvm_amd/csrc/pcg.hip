@@ -477,7 +477,7 @@ static void build_coarsest(ufvm_ctx *ctx, System &S) {
     shift /= (double)nc * nc;
   }
   P.ainv.alloc((size_t)nc * nc);
-  P.colbuf.alloc(nc);
+  P.colbuf.alloc(k::gj_scratch(nc));
   k::csr_to_dense(ctx->stream, nc, RC.ptr.p, RC.col.p, RC.ac.p, RC.anb.p, shift, P.ainv.p);
   k::gj_inverse(ctx->stream, nc, P.ainv.p, P.colbuf.p);
 }
