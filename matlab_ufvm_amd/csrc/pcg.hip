@@ -489,6 +489,7 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
     throw Error(UFVM_ERR_ARG, "max_coarse_cells must be in [1, 4096]");
   PcgState &P = S.pcg;
   P.cfg = cfg;
+  if (P.cfg.coarse_scale == 0.0) P.cfg.coarse_scale = 2.0;   // zero-initialised config
   int32_t nlev = S.nlevels;
   for (int32_t l = 0; l < S.nlevels; ++l)
     if (S.lev[l]->n <= cfg.max_coarse_cells) { nlev = l + 1; break; }
@@ -520,8 +521,8 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
   if (cfg.coarse_smoother != 0 && cfg.coarse_smoother != 1)
     throw Error(UFVM_ERR_ARG, "coarse_smoother must be 0 or 1");
   if (cfg.smoother != 0 && cfg.smoother != 1) throw Error(UFVM_ERR_ARG, "smoother must be 0 or 1");
-  if (!(cfg.coarse_scale > 0.0 && cfg.coarse_scale <= 4.0))
-    throw Error(UFVM_ERR_ARG, "coarse_scale must be in (0, 4]");
+  if (!(P.cfg.coarse_scale > 0.0 && P.cfg.coarse_scale <= 4.0))
+    throw Error(UFVM_ERR_ARG, "coarse_scale must be in (0, 4] (0: the default 2)");
   if (cfg.launch_mode < 0 || cfg.launch_mode > 2)
     throw Error(UFVM_ERR_ARG, "launch_mode must be 0, 1 or 2");
   for (int32_t l = 0; l < nlev; ++l)
