@@ -487,14 +487,26 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
   if (cfg.precision != 32 && cfg.precision != 64) throw Error(UFVM_ERR_ARG, "precision must be 32 or 64");
   if (cfg.max_coarse_cells < 1 || cfg.max_coarse_cells > 4096)
     throw Error(UFVM_ERR_ARG, "max_coarse_cells must be in [1, 4096]");
-  PcgState &P = S.pcg;
-  P.cfg = cfg;
-  if (P.cfg.coarse_scale == 0.0) P.cfg.coarse_scale = 2.0;   // zero-initialised config
+  if (cfg.coarse_smoother != 0 && cfg.coarse_smoother != 1)
+    throw Error(UFVM_ERR_ARG, "coarse_smoother must be 0 or 1");
+  if (cfg.smoother != 0 && cfg.smoother != 1) throw Error(UFVM_ERR_ARG, "smoother must be 0 or 1");
+  const double cscale = cfg.coarse_scale == 0.0 ? 2.0 : cfg.coarse_scale;   // 0: default
+  if (!(cscale > 0.0 && cscale <= 4.0))
+    throw Error(UFVM_ERR_ARG, "coarse_scale must be in (0, 4] (0: the default 2)");
+  if (cfg.launch_mode < 0 || cfg.launch_mode > 2)
+    throw Error(UFVM_ERR_ARG, "launch_mode must be 0, 1 or 2");
+  if (cfg.nullspace < -1 || cfg.nullspace > 1)
+    throw Error(UFVM_ERR_ARG, "nullspace must be -1, 0 or 1");
   int32_t nlev = S.nlevels;
   for (int32_t l = 0; l < S.nlevels; ++l)
     if (S.lev[l]->n <= cfg.max_coarse_cells) { nlev = l + 1; break; }
   if (S.lev[nlev - 1]->n > 4096)
     throw Error(UFVM_ERR_UNSUPPORTED, "coarsest level too large for the dense solve");
+  // the configuration is valid: from here the previous set-up is replaced
+  PcgState &P = S.pcg;
+  P.ready = false;
+  P.cfg = cfg;
+  P.cfg.coarse_scale = cscale;
   P.nlev = nlev;
   // constant nullspace: all row sums vanish (pure-Neumann pressure correction)
   RefLevel &R0 = *S.lev[0];
@@ -518,13 +530,6 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
   }
   P.lev.clear();
   P.lev.resize(nlev);
-  if (cfg.coarse_smoother != 0 && cfg.coarse_smoother != 1)
-    throw Error(UFVM_ERR_ARG, "coarse_smoother must be 0 or 1");
-  if (cfg.smoother != 0 && cfg.smoother != 1) throw Error(UFVM_ERR_ARG, "smoother must be 0 or 1");
-  if (!(P.cfg.coarse_scale > 0.0 && P.cfg.coarse_scale <= 4.0))
-    throw Error(UFVM_ERR_ARG, "coarse_scale must be in (0, 4] (0: the default 2)");
-  if (cfg.launch_mode < 0 || cfg.launch_mode > 2)
-    throw Error(UFVM_ERR_ARG, "launch_mode must be 0, 1 or 2");
   for (int32_t l = 0; l < nlev; ++l)
     build_fast_level(ctx, S, l,
                      l == nlev - 1 ? 2

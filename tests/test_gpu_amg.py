@@ -277,3 +277,24 @@ def test_update_coefficients_refresh_matches_setup(ctx):
         assert np.linalg.norm(b - A2 @ x_r) <= 1e-9 * np.linalg.norm(b)
         ctx.update_coefficients(sid, ac, anb, b)           # back for the next smoother
     ctx.system_destroy(sid)
+
+
+def test_invalid_pcg_config_keeps_previous_setup(ctx):
+    # a rejected configuration leaves the previous set-up usable
+    from matlab_ufvm_amd._lib import UfvmError
+    m = cavity_mesh(10, jitter=0.15, seed=75, renumber_seed=76)
+    ptr, col, ac, anb = poisson_system(m, dirichlet=("movingWall",))
+    b = gaussian_rhs(m, seed=77)
+    sid = make(ctx, ptr, col, ac, anb, b)
+    ctx.agglomerate(sid, 10)
+    ctx.pcg_setup(sid, precision=32, max_coarse_cells=200, tol=1e-9)
+    it0, rr0 = ctx.pcg_solve(sid)
+    x0 = ctx.get_dphi(sid, 0)
+    for bad in (dict(coarse_scale=5.0), dict(smoother="MCGS", coarse_smoother=7),
+                 dict(launch_mode="graphs", nullspace=3)):
+        with pytest.raises(UfvmError):
+            ctx.pcg_setup(sid, precision=32, max_coarse_cells=200, tol=1e-9, **bad)
+        it, rr = ctx.pcg_solve(sid)
+        assert it == it0 and rr == rr0
+        assert np.array_equal(ctx.get_dphi(sid, 0), x0)
+    ctx.system_destroy(sid)
