@@ -310,6 +310,49 @@ def event_timed_solves(ctx, sid, steps, stride=5):
     return ctx.prof_read("mcgs0"), ctx.prof_read("spmv0")
 
 
+def batched_solves(ctx, sid, b, x_single, args):
+    """k solves of the same system at once (ufvm_pcg_solve_batch), inputs and
+    outputs resident in device memory: the system's right-hand side and k-1
+    seeded variants of it (10% Gaussian noise of its RMS added).  Reports the
+    throughput and per-batch latency, and whether the batch gave the first
+    right-hand side the bits of its single solve."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    k = args.batch
+    rng = np.random.default_rng(2031)
+    rms = float(np.sqrt(np.mean(b * b)))
+    B = np.ascontiguousarray(
+        np.stack([b] + [b + 0.1 * rms * rng.standard_normal(b.size) for _ in range(k - 1)]))
+    X = np.empty_like(B)
+    nbytes = C.c_size_t(B.nbytes)
+    bd, xd = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(bd), nbytes) == 0 and hip.hipMalloc(C.byref(xd), nbytes) == 0
+    assert hip.hipMemcpy(bd, C.c_void_p(B.ctypes.data), nbytes, 1) == 0   # host to device
+    ctx.pcg_solve_batch_ptr(sid, k, bd.value, xd.value)     # warm-up (workspace)
+    nb = max(3, args.steps // 2)
+    lat = []
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(nb):
+        s = time.perf_counter()
+        its, rel = ctx.pcg_solve_batch_ptr(sid, k, bd.value, xd.value)
+        lat.append(time.perf_counter() - s)
+    ctx.synchronize()
+    total = time.perf_counter() - t0
+    assert hip.hipMemcpy(C.c_void_p(X.ctypes.data), xd, nbytes, 2) == 0   # device to host
+    hip.hipFree(bd)
+    hip.hipFree(xd)
+    same = bool(np.array_equal(X[0], x_single))
+    log(f"batched k={k}: {k * nb / total:.2f} solves/s, batch p50 {np.median(lat)*1e3:.2f} ms, "
+        f"iterations {its.tolist()}, first solution bit-identical to its single solve: {same}")
+    return {"k": k, "solves_per_s": round(k * nb / total, 3), "batches": nb,
+            "batch_ms_p50": round(float(np.median(lat)) * 1e3, 3),
+            "batch_ms_p99": round(float(np.percentile(lat, 99)) * 1e3, 3),
+            "iterations": [int(v) for v in its], "max_relres": float(rel.max()),
+            "same_bits_as_single_solve": same,
+            "inputs": "system RHS + k-1 variants (10% seeded Gaussian noise), device-resident"}
+
+
 def cavity2d_bench(args):
     """The plain method, CPU-runnable: the 2-D lid-driven cavity on a 40x40x1 hex
     mesh (Re 100: lid u = 1, nu = 0.01), SIMPLE with URF 0.7 / 0.3, pressure by
@@ -436,6 +479,8 @@ def main():
     ap.add_argument("--pre", type=int, default=1)
     ap.add_argument("--post", type=int, default=1)
     ap.add_argument("--max-coarse", type=int, default=2000)
+    ap.add_argument("--batch", type=int, default=8, choices=[1, 2, 4, 8],
+                    help="also time k solves of the system at once (ufvm_pcg_solve_batch)")
     ap.add_argument("--launch-mode", choices=["auto", "eager", "graphs"], default="auto",
                     help="FCG iterations as captured HIP graphs or kernel by kernel")
     ap.add_argument("--coarse-smoother", type=int, default=1,
@@ -517,6 +562,8 @@ def main():
         res = dict(total=total, lat=lat, its=its, relres=rr, x=x, setup=t_setup)
         if prec == 32:
             res["prof"], res["prof_spmv"] = event_timed_solves(ctx, sid, args.event_steps)
+            if args.batch > 1:
+                res["batch"] = batched_solves(ctx, sid, b, x, args)
         results[prec] = res
         log(f"fp{prec}-precond: {args.steps / total:.2f} solves/s, iters {its[0]}, "
             f"p50 {np.median(lat)*1e3:.2f} ms, p99 {np.percentile(lat, 99)*1e3:.2f} ms,"
@@ -627,6 +674,9 @@ def main():
                                 "achieved": round(by1 / n1 / (ms1 / n1 * 1e-3) / 1e9, 1),
                                 "avg_launch_us": round(ms1 / n1 * 1e3, 2),
                                 "algorithmic_bytes_per_launch": round(by1 / n1)}
+    if "batch" in r32:
+        out["batched"] = r32["batch"]
+        out["batched"]["speedup_vs_single"] = round(r32["batch"]["solves_per_s"] / value, 3)
     if 64 in results:
         r64 = results[64]
         v64 = args.steps / r64["total"]

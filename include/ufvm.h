@@ -179,6 +179,15 @@ int ufvm_pcg_setup(ufvm_ctx *ctx, int32_t sys, const ufvm_pcg_cfg *cfg);
 /* Solve A dphi = bc for level 1 from dphi = 0.  On return dphi holds the
  * solution (device; read with ufvm_system_get_dphi).  */
 int ufvm_pcg_solve(ufvm_ctx *ctx, int32_t sys, int32_t *iters, double *relres);
+/* k = 1, 2, 4 or 8 independent solves of the same system at once (no
+ * reference counterpart: throughput form of ufvm_pcg_solve).  b: k right-hand
+ * sides, x: k solutions, each n values in level-1 order one after another, in
+ * host or device memory.  Every kernel of an iteration serves all k (the
+ * matrices are read once per k solves); each solution, iteration count and
+ * relative residual is the one ufvm_pcg_solve gives for that right-hand side.
+ * Needs the multicolour GS finest smoother and Jacobi coarse levels. */
+int ufvm_pcg_solve_batch(ufvm_ctx *ctx, int32_t sys, int32_t k, const double *b, double *x,
+                         int32_t *iters, double *relres);
 /* Apply one preconditioner V-cycle: z = M^-1 r (host arrays, level-1 order). */
 int ufvm_pcg_apply_vcycle(ufvm_ctx *ctx, int32_t sys, const double *r, double *z);
 /* number of colours of a level's multicolour ordering (after ufvm_pcg_setup) */

@@ -97,6 +97,8 @@ _SIGS = {
     "ufvm_apply_amg": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(AmgCfg), f64p, f64p, i32p]),
     "ufvm_pcg_setup": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(PcgCfg)]),
     "ufvm_pcg_solve": (C.c_int, [C.c_void_p, C.c_int32, i32p, f64p]),
+    "ufvm_pcg_solve_batch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
+                                       i32p, f64p]),
     "ufvm_pcg_apply_vcycle": (C.c_int, [C.c_void_p, C.c_int32, f64p, f64p]),
     "ufvm_pcg_level_colours": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, i32p, i32p]),
     "ufvm_case_create": (C.c_int, [C.c_void_p, C.POINTER(MeshDesc), C.POINTER(SimpleCfg), i32p]),
@@ -315,6 +317,25 @@ class Context:
         it, rr = C.c_int32(), C.c_double()
         self.check(self.lib.ufvm_pcg_solve(self.h, sid, C.byref(it), C.byref(rr)))
         return it.value, rr.value
+
+    def pcg_solve_batch(self, sid, B):
+        """k = len(B) (1, 2, 4 or 8) solves of the system at once; B: (k, n)
+        right-hand sides in level-1 order.  Returns (X (k, n), iterations (k,),
+        relative residuals (k,))."""
+        B = np.ascontiguousarray(B, np.float64)
+        X = np.empty_like(B)
+        its, rel = self.pcg_solve_batch_ptr(sid, B.shape[0], B.ctypes.data, X.ctypes.data)
+        return X, its, rel
+
+    def pcg_solve_batch_ptr(self, sid, k, b_addr, x_addr):
+        """As pcg_solve_batch with the k x n arrays given by address (host or
+        device memory, e.g. torch tensors' data_ptr())."""
+        its = np.zeros(k, np.int32)
+        rel = np.zeros(k)
+        self.check(self.lib.ufvm_pcg_solve_batch(self.h, sid, k, C.c_void_p(b_addr),
+                                                 C.c_void_p(x_addr), its.ctypes.data_as(i32p),
+                                                 rel.ctypes.data_as(f64p)))
+        return its, rel
 
     def pcg_apply_vcycle(self, sid, r):
         r = _f64(r)

@@ -15,7 +15,8 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libufvm.so")
 
-SOURCES = ["api.hip", "kernels.hip", "simple_api.hip", "simple_kernels.hip", "pcg.hip", "server.hip", "hostalg.cpp"]
+SOURCES = ["api.hip", "kernels.hip", "kernels_batch.hip", "simple_api.hip", "simple_kernels.hip",
+           "pcg.hip", "server.hip", "hostalg.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

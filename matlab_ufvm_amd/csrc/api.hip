@@ -622,6 +622,11 @@ int ufvm_pcg_solve(ufvm_ctx *ctx, int32_t sys, int32_t *iters, double *relres) {
   return guard(ctx, [&] { pcg_solve(ctx, sys_of(ctx, sys), iters, relres); });
 }
 
+int ufvm_pcg_solve_batch(ufvm_ctx *ctx, int32_t sys, int32_t k, const double *b, double *x,
+                         int32_t *iters, double *relres) {
+  return guard(ctx, [&] { pcg_solve_batch(ctx, sys_of(ctx, sys), k, b, x, iters, relres); });
+}
+
 int ufvm_pcg_apply_vcycle(ufvm_ctx *ctx, int32_t sys, const double *r, double *z) {
   return guard(ctx, [&] { pcg_apply_vcycle(ctx, sys_of(ctx, sys), r, z); });
 }

@@ -10,6 +10,7 @@ namespace k {
 
 constexpr int kBlock = 256;
 constexpr int kFoldBlocks = 512;   // grid cap of the grid-stride FCG kernels
+constexpr int kFcgBlock = 1024;    // threads per block of the FCG kernels
 inline int nblocks(int64_t n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
 // ---- reference-order FP64 kernels (bit-exact with the reference arithmetic)
@@ -115,16 +116,55 @@ void ring_put(hipStream_t s, const double *src, double *ring);
 // w = A p (sliced-ELL finest level); partial[0, g) = per-block p.w
 void spmv_dot(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, const double *ac,
               const double *val, const double *p, double *w, double *partial);
+// Strides between the right-hand sides of a batch of k solves (the FCG
+// kernels below run the k solves as grid rows blockIdx.y, each exactly as a
+// single solve): FP64 vectors, V-cycle-precision vectors, partials, device
+// scalars and ring slots.  All zero for one solve.
+struct BatchStrides {
+  int64_t ld = 0, ldv = 0, ps = 0, ss = 0, rs = 0;
+};
 // partial[c*g + b] = per-block sums of z, z.w, z.r, w, r
 template <typename V>
 void fcg_dots(hipStream_t s, int32_t n, const V *z, const double *w, const double *r,
-              double *partial);
+              double *partial, int k = 1, BatchStrides bs = {});
 // folds fcg_dots' partials (m, rho, beta into scal) ; p = (z - m) - beta p
 template <typename V>
 void fcg_update_p(hipStream_t s, int32_t n, const V *z, const double *partial, int singular,
-                  int first, double *scal, double *p);
+                  int first, double *scal, double *p, int k = 1, BatchStrides bs = {});
 // blocks of the FCG streaming kernels (each writes one partial per block)
 int fold_blocks(int32_t n);
+
+// ---- K right-hand sides of one system at once (kernels_batch.hip): vectors
+// RHS-major (vector r at offset r * ld), k in {1, 2, 4, 8}; each vector gets
+// the arithmetic of the single-vector kernel above
+template <typename V>
+void mcgs_sell_b(hipStream_t s, int k, int32_t r0, int32_t r1, const int32_t *off,
+                 const int32_t *col, const V *val, const V *ac, const V *b, V *x, int64_t ld,
+                 int32_t zero_above);
+template <typename V>
+void residual_sell_b(hipStream_t s, int k, int32_t n, int32_t zero_rows, const int32_t *off,
+                     const int32_t *col, const V *val, const V *ac, const V *b, const V *x,
+                     V *r, int64_t ld);
+// parent / xc null: no prolongation fused
+template <typename V>
+void jacobi_sell_b(hipStream_t s, int k, int32_t n, const int32_t *off, const int32_t *col,
+                   const V *val, const V *ac, const V *b, const V *x, int64_t ld,
+                   const int32_t *parent, const V *xc, int64_t ldc, V *out, V omega);
+// acc null: plain restriction, else fused with the first Jacobi sweep (xc)
+template <typename V>
+void restrict_b(hipStream_t s, int k, int32_t nc, const int32_t *off, const int32_t *child,
+                const V *r, int64_t ldf, const V *acc, V *bc, V *xc, int64_t ldc, V omega,
+                V scale);
+template <typename V>
+void prolong_b(hipStream_t s, int k, int32_t r0, int32_t n, const int32_t *parent, const V *xc,
+               int64_t ldc, V *x, int64_t ld);
+template <typename V>
+void dense_solve_b(hipStream_t s, int k, int32_t n, const double *ainv, const V *b, int64_t ld,
+                   V *x);
+// w_r = A p_r and the p_r.w_r block partials at partial + r * pstride
+void spmv_dot_b(hipStream_t s, int k, int32_t n, const int32_t *off, const int32_t *col,
+                const double *ac, const double *val, const double *p, double *w, int64_t ld,
+                double *partial, int64_t pstride);
 // folds spmv_dot's partials (delta, alpha into scal) ; x += alpha p ;
 // r -= alpha w ; rv = (V) r ; then rr = r.r into scal[10], the convergence
 // flag scal[11] once rr <= scal[13] (no update at all once it is set), and
@@ -134,9 +174,9 @@ int fold_blocks(int32_t n);
 template <typename V>
 void fcg_update_xr(hipStream_t s, int32_t n, double *scal, const double *p, const double *w,
                    double *x, double *r, V *rv, double *partial, double *ring, int32_t n0,
-                   const V *acv, V *z0);
+                   const V *acv, V *z0, int k = 1, BatchStrides bs = {});
 // arm a solve: scal[11] = 0 (flag), scal[13] = thr, scal[14] = 0 (iteration)
-void fcg_arm(hipStream_t s, double *scal, double thr);
+void fcg_arm(hipStream_t s, double *scal, double thr, int done = 0);
 // reduce nblk partial groups of width `width` into out[0..width)
 void reduce_groups(hipStream_t s, const double *partial, int32_t nblk, int32_t width,
                    double *out);

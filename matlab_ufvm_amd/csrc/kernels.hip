@@ -797,8 +797,6 @@ void gj_inverse(hipStream_t s, int32_t n, double *a, double *scratch) {
 // block (same order in each block, so every block derives identical scalars)
 // instead of a separate reduction launch.
 
-constexpr int kFcgBlock = 1024;
-
 int fold_blocks(int32_t n) {
   const int b = nblocks(n, kFcgBlock);
   return b < kFoldBlocks ? b : kFoldBlocks;
@@ -848,8 +846,13 @@ void spmv_dot(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, 
 template <typename V>
 __global__ void __launch_bounds__(kFcgBlock)
 k_fcg_dots(int32_t n, const V *__restrict__ z, const double *__restrict__ w,
-           const double *__restrict__ r, double *__restrict__ partial) {
+           const double *__restrict__ r, double *__restrict__ partial, BatchStrides bs) {
   __shared__ double lds[5 * kFcgBlock / kWave];
+  // right-hand side blockIdx.y of a batch
+  z += blockIdx.y * bs.ldv;
+  w += blockIdx.y * bs.ld;
+  r += blockIdx.y * bs.ld;
+  partial += blockIdx.y * bs.ps;
   double v[5] = {0, 0, 0, 0, 0};
   auto add = [&](double zi, double wi, double ri) {
     v[0] += zi;
@@ -881,9 +884,9 @@ k_fcg_dots(int32_t n, const V *__restrict__ z, const double *__restrict__ w,
 
 template <typename V>
 void fcg_dots(hipStream_t s, int32_t n, const V *z, const double *w, const double *r,
-              double *partial) {
-  hipLaunchKernelGGL(k_fcg_dots<V>, dim3(fold_blocks(n)), dim3(kFcgBlock), 0, s, n, z, w, r,
-                     partial);
+              double *partial, int k, BatchStrides bs) {
+  hipLaunchKernelGGL(k_fcg_dots<V>, dim3(fold_blocks(n), k), dim3(kFcgBlock), 0, s, n, z, w, r,
+                     partial, bs);
 }
 
 // Folds the z-dots (m = mean of z when singular, rho = (z - m).r,
@@ -893,8 +896,12 @@ template <typename V>
 __global__ void __launch_bounds__(kFcgBlock)
 k_fcg_update_p(int32_t n, const V *__restrict__ z, const double *__restrict__ partial,
                int32_t g, int singular, int first, double *__restrict__ scal,
-               double *__restrict__ p) {
+               double *__restrict__ p, BatchStrides bs) {
   __shared__ double lds[5 * kFcgBlock / kWave];
+  z += blockIdx.y * bs.ldv;
+  p += blockIdx.y * bs.ld;
+  partial += blockIdx.y * bs.ps;
+  scal += blockIdx.y * bs.ss;
   const int32_t G = gridDim.x * kFcgBlock;
   int32_t i = blockIdx.x * kFcgBlock + threadIdx.x;
   // the first four grid-stride rows are loaded before the fold, so that their
@@ -945,10 +952,10 @@ k_fcg_update_p(int32_t n, const V *__restrict__ z, const double *__restrict__ pa
 
 template <typename V>
 void fcg_update_p(hipStream_t s, int32_t n, const V *z, const double *partial, int singular,
-                  int first, double *scal, double *p) {
+                  int first, double *scal, double *p, int k, BatchStrides bs) {
   const int g = fold_blocks(n);
-  hipLaunchKernelGGL(k_fcg_update_p<V>, dim3(g), dim3(kFcgBlock), 0, s, n, z, partial, g,
-                     singular, first, scal, p);
+  hipLaunchKernelGGL(k_fcg_update_p<V>, dim3(g, k), dim3(kFcgBlock), 0, s, n, z, partial, g,
+                     singular, first, scal, p, bs);
 }
 
 // Folds delta = p.Ap (partials of k_spmv_dot) and alpha = rho / delta, then
@@ -964,9 +971,18 @@ k_fcg_update_xr(int32_t n, const double *__restrict__ partial_in, double *__rest
                 const double *__restrict__ p, const double *__restrict__ w,
                 double *__restrict__ x, double *__restrict__ r, V *__restrict__ rv,
                 double *__restrict__ partial_out, int32_t n0, const V *__restrict__ acv,
-                V *__restrict__ z0) {
+                V *__restrict__ z0, BatchStrides bs) {
   __shared__ double lds[kFcgBlock / kWave];
+  scal += blockIdx.y * bs.ss;
   if (scal[11] != 0.0) return;
+  partial_in += blockIdx.y * bs.ps;
+  partial_out += blockIdx.y * bs.ps;
+  p += blockIdx.y * bs.ld;
+  w += blockIdx.y * bs.ld;
+  x += blockIdx.y * bs.ld;
+  r += blockIdx.y * bs.ld;
+  rv += blockIdx.y * bs.ldv;
+  z0 += blockIdx.y * bs.ldv;
   const int32_t G = gridDim.x * kFcgBlock;
   int32_t i = blockIdx.x * kFcgBlock + threadIdx.x;
   // the first four grid-stride rows are loaded before the fold (their latency
@@ -1056,8 +1072,12 @@ void ring_put(hipStream_t s, const double *src, double *ring) {
 // counter, which it then advances.  Everything per solve lives in scal, so a
 // captured iteration replays unchanged.
 __global__ void __launch_bounds__(kFcgBlock)
-k_fcg_rr(const double *__restrict__ partial, int32_t g, double *scal, double *ring) {
+k_fcg_rr(const double *__restrict__ partial, int32_t g, double *scal, double *ring,
+         BatchStrides bs) {
   __shared__ double lds[kFcgBlock / kWave];
+  partial += blockIdx.y * bs.ps;
+  scal += blockIdx.y * bs.ss;
+  if (ring) ring += blockIdx.y * bs.rs;
   double t[1];
   fold_sums<1>(partial, g, t, lds);
   if (threadIdx.x != 0) return;
@@ -1074,29 +1094,30 @@ k_fcg_rr(const double *__restrict__ partial, int32_t g, double *scal, double *ri
   scal[14] = it + 1.0;
 }
 
-// arm a solve: convergence flag off, threshold thr, iteration counter 0
-__global__ void k_fcg_arm(double *scal, double thr) {
+// arm a solve: convergence flag off (on: done, for a zero right-hand side in
+// a batch), threshold thr, iteration counter 0
+__global__ void k_fcg_arm(double *scal, double thr, int done) {
   if (threadIdx.x == 0) {
-    scal[11] = 0.0;
+    scal[11] = done ? 1.0 : 0.0;
     scal[13] = thr;
     scal[14] = 0.0;
   }
 }
 
-void fcg_arm(hipStream_t s, double *scal, double thr) {
-  hipLaunchKernelGGL(k_fcg_arm, dim3(1), dim3(64), 0, s, scal, thr);
+void fcg_arm(hipStream_t s, double *scal, double thr, int done) {
+  hipLaunchKernelGGL(k_fcg_arm, dim3(1), dim3(64), 0, s, scal, thr, done);
 }
 
 template <typename V>
 void fcg_update_xr(hipStream_t s, int32_t n, double *scal, const double *p, const double *w,
                    double *x, double *r, V *rv, double *partial, double *ring, int32_t n0,
-                   const V *acv, V *z0) {
+                   const V *acv, V *z0, int k, BatchStrides bs) {
   // the operator-product partials occupy partial[0, g); the r.r partials go
   // after them
   const int g = fold_blocks(n);
-  hipLaunchKernelGGL(k_fcg_update_xr<V>, dim3(g), dim3(kFcgBlock), 0, s, n, partial, scal, p, w,
-                     x, r, rv, partial + g, n0, acv, z0);
-  hipLaunchKernelGGL(k_fcg_rr, dim3(1), dim3(kFcgBlock), 0, s, partial + g, g, scal, ring);
+  hipLaunchKernelGGL(k_fcg_update_xr<V>, dim3(g, k), dim3(kFcgBlock), 0, s, n, partial, scal, p,
+                     w, x, r, rv, partial + g, n0, acv, z0, bs);
+  hipLaunchKernelGGL(k_fcg_rr, dim3(1, k), dim3(kFcgBlock), 0, s, partial + g, g, scal, ring, bs);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -1159,12 +1180,12 @@ void axpy_const(hipStream_t s, int32_t n, const double *scal_slot, double mul, d
   template void dense_solve<V>(hipStream_t, int32_t, const double *, const V *, V *);           \
   template void fill<V>(hipStream_t, int64_t, V *, V);                                          \
   template void fcg_dots<V>(hipStream_t, int32_t, const V *, const double *, const double *,    \
-                            double *);                                                          \
+                            double *, int, BatchStrides);                                       \
   template void fcg_update_p<V>(hipStream_t, int32_t, const V *, const double *, int, int,       \
-                                double *, double *);                                            \
+                                double *, double *, int, BatchStrides);                         \
   template void fcg_update_xr<V>(hipStream_t, int32_t, double *, const double *, const double *, \
                                  double *, double *, V *, double *, double *, int32_t,          \
-                                 const V *, V *);
+                                 const V *, V *, int, BatchStrides);
 UFVM_INST(float)
 UFVM_INST(double)
 template void convert<double, float>(hipStream_t, int64_t, const double *, float *);

@@ -3,18 +3,21 @@
 #include <hip/hip_runtime.h>
 
 #include <memory>
+#include <vector>
 
 #include "internal.h"
+#include "kernels.h"
 
 namespace ufvm {
 namespace detail {
 
-// Device workspace for one solve on system S (whose ufvm_pcg_setup is done):
-// level vectors in the V-cycle precision, FCG vectors, scalars, partials,
-// the pinned result ring and its events; cleared on stream s (synchronised).
-std::unique_ptr<SolveWork> make_work(const System &S, hipStream_t s);
-// device bytes make_work(S) would allocate
-size_t work_bytes(const System &S);
+// Device workspace for one solve (k = 1) or a batch of k = 2, 4 or 8 solves
+// on system S (whose ufvm_pcg_setup is done): level vectors in the V-cycle
+// precision, FCG vectors, scalars, partials, the pinned result ring and its
+// events; cleared on stream s (synchronised).
+std::unique_ptr<SolveWork> make_work(const System &S, hipStream_t s, int k = 1);
+// device bytes make_work(S, s, k) would allocate
+size_t work_bytes(const System &S, int k = 1);
 
 // One FCG(1) solve of S on workspace W and stream s, driven as a state
 // machine so that one host thread can interleave solves on several streams:
@@ -60,6 +63,44 @@ class FcgRun {
   bool ahead_ = true;
   double prev2_ = -1.0, nb_ = 0.0, thr_ = 0.0, rr_ = 0.0;
 };
+
+// FCG(1) for the k right-hand sides of a batch workspace (make_work(S, s, k))
+// at once: every kernel of an iteration covers all k (the matrix is read once
+// per k solves) and each right-hand side gets the bits of its own solve.  A
+// right-hand side that has converged stays frozen while the others continue.
+class FcgBatch {
+ public:
+  FcgBatch(ufvm_ctx *ctx, System &S, SolveWork &W, hipStream_t s);
+  // B: k right-hand sides, X: k solutions, each n values in reference order
+  // one after another, host or device memory (X written when poll returns true)
+  void start(const double *B, double *X);
+  bool poll(bool block);
+  int32_t iterations(int q) const { return its_[q]; }
+  double relres(int q) const { return nb_[q] > 0.0 ? rr_[q] / nb_[q] : 0.0; }
+
+ private:
+  template <typename V>
+  void start_t();
+  template <typename V>
+  void issue_t(int it);
+  void finish();
+  k::BatchStrides strides() const;
+
+  ufvm_ctx *ctx_;
+  System *S_;
+  SolveWork *W_;
+  hipStream_t s_;
+  int k_;
+  double *X_ = nullptr;
+  int phase_ = 0, it_ = 0, issued_ = 0;
+  std::vector<double> nb_, thr_, rr_;
+  std::vector<int32_t> its_;
+  std::vector<char> done_;
+};
+
+// blocking batch solve on the context stream (workspace cached per k)
+void pcg_solve_batch(ufvm_ctx *ctx, System &S, int k, const double *B, double *X,
+                     int32_t *iters, double *relres);
 
 // z = M^-1 r for host arrays in reference order (one preconditioner V-cycle)
 void pcg_apply_vcycle(ufvm_ctx *ctx, System &S, const double *r, double *z);

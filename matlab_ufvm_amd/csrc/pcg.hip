@@ -410,53 +410,77 @@ struct Cycle {
 
 // --------------------------------------------------------------- workspaces
 
-size_t work_bytes(const System &S) {
+// stride between the right-hand sides of a k-solve batch in an n-row vector
+// (one solve: n; a batch: n rounded up to 64 rows, so every vector starts on
+// a 256-byte boundary)
+static int64_t batch_ld(int64_t n, int k) { return k == 1 ? n : (n + 63) / 64 * 64; }
+// partial sums per right-hand side: the start-up reductions use one partial
+// per 256-row block, the FCG kernels at most 6 per 1024-row block
+static int64_t partial_stride(int32_t n0) { return 5 * (int64_t)k::nblocks((int64_t)n0) + 16; }
+// ring doubles per right-hand side: {r.r, iteration} of even and odd
+// iterations, then the start-up norm
+constexpr int kRingPer = 6;
+
+size_t work_bytes(const System &S, int k) {
   const PcgState &P = S.pcg;
   const size_t w = P.cfg.precision == 32 ? 4 : 8;
   size_t bytes = 0;
-  for (int32_t l = 0; l < P.nlev; ++l) bytes += 4 * w * (size_t)P.lev[l]->n;
+  for (int32_t l = 0; l < P.nlev; ++l) bytes += 4 * w * k * (size_t)batch_ld(P.lev[l]->n, k);
   const size_t n0 = S.lev[0]->n;
-  bytes += 6 * 8 * n0;                                         // x r p w bref xref
-  bytes += 8 * (5 * (size_t)k::nblocks((int64_t)n0) + 16);     // partials
-  return bytes + 8 * kScal;
+  bytes += 4 * 8 * k * (size_t)batch_ld(n0, k);                   // x r p w
+  bytes += 2 * 8 * k * n0;                                        // bref xref
+  bytes += 8 * k * (size_t)partial_stride((int32_t)n0);           // partials
+  return bytes + 8 * k * kScal;
 }
 
 template <typename V>
 static void alloc_levels(const System &S, SolveWork &W, hipStream_t s) {
   const PcgState &P = S.pcg;
+  const int k = W.nrhs;
   W.lev.clear();
   W.lev.resize(P.nlev);
-  for (int32_t l = 0; l < P.nlev; ++l)
+  for (int32_t l = 0; l < P.nlev; ++l) {
+    W.lev[l].ld = batch_ld(P.lev[l]->n, k);
     for (int i = 0; i < 4; ++i) {
       // iterates start finite: the sliced-ELL padding multiplies x[row] by 0
       DBuf<V> &d = Prec<V>::buf(W.lev[l], i);
-      d.alloc(P.lev[l]->n);
+      d.alloc((size_t)W.lev[l].ld * k);
       UFVM_HIP(hipMemsetAsync(d.p, 0, d.n * sizeof(V), s));
     }
+  }
 }
 
-std::unique_ptr<SolveWork> make_work(const System &S, hipStream_t s) {
+std::unique_ptr<SolveWork> make_work(const System &S, hipStream_t s, int k) {
   if (!S.pcg.ready) throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first");
+  if (k != 1 && k != 2 && k != 4 && k != 8)
+    throw Error(UFVM_ERR_ARG, "a batch holds 1, 2, 4 or 8 right-hand sides");
   std::unique_ptr<SolveWork> W(new SolveWork());
   const int32_t n0 = S.lev[0]->n;
   W->n0 = n0;
+  W->nrhs = k;
+  W->ld0 = batch_ld(n0, k);
   if (S.pcg.cfg.precision == 32) alloc_levels<float>(S, *W, s);
   else alloc_levels<double>(S, *W, s);
-  for (DBuf<double> *v : {&W->x, &W->r, &W->p, &W->w, &W->bref, &W->xref}) {
-    v->alloc(n0);
-    UFVM_HIP(hipMemsetAsync(v->p, 0, n0 * sizeof(double), s));
+  for (DBuf<double> *v : {&W->x, &W->r, &W->p, &W->w}) {
+    v->alloc((size_t)W->ld0 * k);
+    UFVM_HIP(hipMemsetAsync(v->p, 0, v->n * sizeof(double), s));
   }
-  W->partial.alloc(5 * (size_t)k::nblocks((int64_t)n0) + 16);
-  W->scal.alloc(kScal);
-  UFVM_HIP(hipMemsetAsync(W->scal.p, 0, kScal * sizeof(double), s));
+  for (DBuf<double> *v : {&W->bref, &W->xref}) {
+    v->alloc((size_t)n0 * k);
+    UFVM_HIP(hipMemsetAsync(v->p, 0, v->n * sizeof(double), s));
+  }
+  W->partial.alloc((size_t)partial_stride(n0) * k);
+  W->scal.alloc((size_t)kScal * k);
+  UFVM_HIP(hipMemsetAsync(W->scal.p, 0, W->scal.n * sizeof(double), s));
   UFVM_HIP(hipStreamSynchronize(s));
   std::lock_guard<std::mutex> lk(device_mgmt_mutex());
-  UFVM_HIP(hipHostMalloc((void **)&W->hring, 8 * sizeof(double),
+  const int nring = kRingPer * k + 2;
+  UFVM_HIP(hipHostMalloc((void **)&W->hring, nring * sizeof(double),
                          hipHostMallocMapped | hipHostMallocCoherent));
   UFVM_HIP(hipHostGetDevicePointer((void **)&W->dring, W->hring, 0));
-  for (int i = 0; i < 8; ++i) W->hring[i] = 0.0;
+  for (int i = 0; i < nring; ++i) W->hring[i] = 0.0;
   for (hipEvent_t &e : W->ev) UFVM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
-  W->bytes = work_bytes(S);
+  W->bytes = work_bytes(S, k);
   return W;
 }
 
@@ -505,6 +529,7 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
   // the configuration is valid: from here the previous set-up is replaced
   PcgState &P = S.pcg;
   P.ready = false;
+  P.work_batch.clear();
   P.cfg = cfg;
   P.cfg.coarse_scale = cscale;
   P.nlev = nlev;
@@ -714,6 +739,276 @@ bool FcgRun::poll(bool block) {
     // reduction factor
     if (prev2_ > 0.0) ahead_ = rr2 * (rr2 / prev2_) > 4.0 * thr_;
     prev2_ = rr2;
+  }
+}
+
+// ------------------------------------------------------------ batched solves
+
+// The V-cycle of Cycle<V> for the k right-hand sides of a batch workspace:
+// finest level multicolour GS, damped-Jacobi coarse levels, dense coarsest
+// level (the configurations a batch accepts), every launch covering all k.
+template <typename V>
+struct CycleB {
+  using P = Prec<V>;
+  System &S;
+  SolveWork &W;
+  hipStream_t s;
+  int k() const { return W.nrhs; }
+
+  int pre_of(int l) const {
+    const ufvm_pcg_cfg &c = S.pcg.cfg;
+    return (l == 0 || c.coarse_pre_sweep < 0) ? c.pre_sweep : c.coarse_pre_sweep;
+  }
+  int post_of(int l) const {
+    const ufvm_pcg_cfg &c = S.pcg.cfg;
+    return (l == 0 || c.coarse_post_sweep < 0) ? c.post_sweep : c.coarse_post_sweep;
+  }
+
+  void mcgs_sweeps(const FastLevel &L, LevelWork &LW, int nsweep, bool from_zero,
+                   bool first_done) {
+    auto pass = [&](int c, int32_t zero_above) {
+      k::mcgs_sell_b<V>(s, k(), L.col_off[c], L.col_off[c + 1], L.soff.p, L.scol.p, P::sval(L),
+                        P::ac(L), P::b(LW), P::x(LW), LW.ld, zero_above);
+    };
+    for (int sw = 0; sw < nsweep; ++sw) {
+      for (int c = (sw == 0 ? (first_done ? 1 : 0) : 1); c < L.ncol; ++c)
+        pass(c, (sw == 0 && from_zero) ? L.col_off[c + 1] : L.n);
+      for (int c = L.ncol - 2; c >= 0; --c) pass(c, L.n);
+    }
+  }
+
+  void down(const FastLevel &L, LevelWork &LW, int l, const V *x, int32_t zero_rows) {
+    const PcgState &Pc = S.pcg;
+    const FastLevel &C = *Pc.lev[l + 1];
+    LevelWork &CW = W.lev[l + 1];
+    k::residual_sell_b<V>(s, k(), L.n, zero_rows, L.soff.p, L.scol.p, P::sval(L), P::ac(L),
+                          P::b(LW), x, P::r(LW), LW.ld);
+    const bool jac = C.kind == 3 && pre_of(l + 1) > 0;
+    k::restrict_b<V>(s, k(), C.n, L.csoff.p, L.cschild.p, P::r(LW), LW.ld,
+                     jac ? P::ac(C) : nullptr, P::b(CW), P::x(CW), CW.ld, V(2.0 / 3.0),
+                     V(Pc.cfg.coarse_scale));
+  }
+
+  V *vcycle(int l, bool first_done = false) {
+    const PcgState &Pc = S.pcg;
+    const FastLevel &L = *Pc.lev[l];
+    LevelWork &LW = W.lev[l];
+    const int pre = pre_of(l), post = post_of(l);
+    if (l == Pc.nlev - 1) {
+      k::dense_solve_b<V>(s, k(), L.n, Pc.ainv.p, P::b(LW), LW.ld, P::x(LW));
+      return P::x(LW);
+    }
+    V *x = P::x(LW);
+    const V omega = V(2.0 / 3.0);
+    if (L.kind == 3) {
+      V *y = P::y(LW);
+      for (int it = 1; it < pre; ++it) {
+        k::jacobi_sell_b<V>(s, k(), L.n, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW), x,
+                            LW.ld, nullptr, nullptr, 0, y, omega);
+        std::swap(x, y);
+      }
+      if (pre == 0) k::fill<V>(s, LW.ld * k(), x, V(0));
+      down(L, LW, l, x, 0);
+      V *xc = vcycle(l + 1);
+      const int64_t ldc = W.lev[l + 1].ld;
+      const bool fuse = post > 0 && L.n <= kFusedProlongRows;
+      if (!fuse) k::prolong_b<V>(s, k(), 0, L.n, L.parent.p, xc, ldc, x, LW.ld);
+      for (int it = 0; it < post; ++it) {
+        const bool pro = fuse && it == 0;
+        k::jacobi_sell_b<V>(s, k(), L.n, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW), x,
+                            LW.ld, pro ? L.parent.p : nullptr, pro ? xc : nullptr, ldc, y,
+                            omega);
+        std::swap(x, y);
+      }
+      return x;
+    }
+    // kind 0: the finest level
+    if (pre == 0) k::fill<V>(s, LW.ld * k(), x, V(0));
+    else mcgs_sweeps(L, LW, pre, true, first_done);
+    down(L, LW, l, x, pre > 0 ? L.col_off[1] : 0);
+    V *xc = vcycle(l + 1);
+    k::prolong_b<V>(s, k(), post > 0 ? L.col_off[1] : 0, L.n, L.parent.p, xc, W.lev[l + 1].ld, x,
+                    LW.ld);
+    if (post > 0) mcgs_sweeps(L, LW, post, false, false);
+    return x;
+  }
+};
+
+FcgBatch::FcgBatch(ufvm_ctx *ctx, System &S, SolveWork &W, hipStream_t s)
+    : ctx_(ctx), S_(&S), W_(&W), s_(s), k_(W.nrhs), nb_(k_), thr_(k_), rr_(k_), its_(k_),
+      done_(k_) {
+  const PcgState &P = S.pcg;
+  for (int32_t l = 1; l + 1 < P.nlev; ++l)
+    if (P.lev[l]->kind != 3)
+      throw Error(UFVM_ERR_UNSUPPORTED, "batched solves need coarse_smoother 1 (Jacobi)");
+  if (P.nlev > 1 && P.lev[0]->kind != 0)
+    throw Error(UFVM_ERR_UNSUPPORTED, "batched solves need the multicolour GS finest smoother");
+}
+
+k::BatchStrides FcgBatch::strides() const {
+  k::BatchStrides bs;
+  bs.ld = W_->ld0;
+  bs.ldv = W_->lev[0].ld;
+  bs.ps = partial_stride(W_->n0);
+  bs.ss = kScal;
+  bs.rs = kRingPer;
+  return bs;
+}
+
+template <typename V>
+void FcgBatch::start_t() {
+  const PcgState &P = S_->pcg;
+  const FastLevel &L0 = *P.lev[0];
+  SolveWork &W = *W_;
+  const int32_t n = W.n0;
+  const int64_t ps = partial_stride(n);
+  // per right-hand side, the start-up of a single solve (same kernels and
+  // partial layout, so the same bits)
+  for (int q = 0; q < k_; ++q) {
+    double *r = W.r.p + q * W.ld0, *part = W.partial.p + q * ps, *scal = W.scal.p + q * kScal;
+    k::permute_gather(s_, n, L0.perm_d.p, W.bref.p + (int64_t)q * n, r);
+    if (P.singular) {
+      k::sum_partial(s_, n, r, part);
+      k::reduce_final(s_, part, k::nblocks(n), scal + 12);
+      k::axpy_const(s_, n, scal + 12, -1.0 / (double)n, r);
+    }
+    k::dot_partial(s_, n, r, r, part);
+    k::reduce_final(s_, part, k::nblocks(n), scal + 10);
+    k::convert<double, V>(s_, n, r, Prec<V>::b(W.lev[0]) + q * W.lev[0].ld);
+    k::ring_put(s_, scal + 10, W.dring + q * kRingPer + 4);
+  }
+  k::fill<double>(s_, (int64_t)W.ld0 * k_, W.x.p, 0.0);
+  k::fill<double>(s_, (int64_t)W.ld0 * k_, W.w.p, 0.0);
+  UFVM_HIP(hipEventRecord(W.ev[2], s_));
+}
+
+void FcgBatch::start(const double *B, double *X) {
+  const PcgState &P = S_->pcg;
+  if (!P.ready || P.stale)
+    throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first (or refresh after new coefficients)");
+  if (!B || !X) throw Error(UFVM_ERR_ARG, "null right-hand sides or solutions");
+  X_ = X;
+  it_ = issued_ = 0;
+  UFVM_HIP(hipMemcpyAsync(W_->bref.p, B, sizeof(double) * W_->n0 * k_, hipMemcpyDefault, s_));
+  if (P.cfg.precision == 32) start_t<float>();
+  else start_t<double>();
+  phase_ = 1;
+}
+
+template <typename V>
+void FcgBatch::issue_t(int it) {
+  const PcgState &P = S_->pcg;
+  const FastLevel &L0 = *P.lev[0];
+  SolveWork &W = *W_;
+  const int32_t n = W.n0;
+  const k::BatchStrides bs = strides();
+  const bool fuse0 = L0.kind == 0 && P.cfg.pre_sweep > 0;
+  CycleB<V> cyc{*S_, W, s_};
+  const V *z = cyc.vcycle(0, fuse0 && it > 0);
+  k::fcg_dots<V>(s_, n, z, W.w.p, W.r.p, W.partial.p, k_, bs);
+  k::fcg_update_p<V>(s_, n, z, W.partial.p, P.singular, it == 0, W.scal.p, W.p.p, k_, bs);
+  k::spmv_dot_b(s_, k_, n, L0.soff.p, L0.scol.p, L0.ac64.p, L0.sval64.p, W.p.p, W.w.p, W.ld0,
+                W.partial.p, bs.ps);
+  k::fcg_update_xr<V>(s_, n, W.scal.p, W.p.p, W.w.p, W.x.p, W.r.p, Prec<V>::b(W.lev[0]),
+                      W.partial.p, W.dring, fuse0 ? L0.col_off[1] : 0, Prec<V>::ac(L0),
+                      Prec<V>::x(W.lev[0]), k_, bs);
+  UFVM_HIP(hipEventRecord(W.ev[it & 1], s_));
+}
+
+void FcgBatch::finish() {
+  const FastLevel &L0 = *S_->pcg.lev[0];
+  SolveWork &W = *W_;
+  for (int q = 0; q < k_; ++q)
+    k::permute_scatter(s_, W.n0, L0.perm_d.p, W.x.p + q * W.ld0, W.xref.p + (int64_t)q * W.n0);
+  UFVM_HIP(hipMemcpyAsync(X_, W.xref.p, sizeof(double) * W.n0 * k_, hipMemcpyDefault, s_));
+  UFVM_HIP(hipEventRecord(W.ev[2], s_));
+  phase_ = 3;
+}
+
+bool FcgBatch::poll(bool block) {
+  const ufvm_pcg_cfg &cfg = S_->pcg.cfg;
+  auto ready = [&](hipEvent_t e) {
+    if (block) {
+      UFVM_HIP(hipEventSynchronize(e));
+      return true;
+    }
+    const hipError_t q = hipEventQuery(e);
+    if (q == hipSuccess) return true;
+    if (q != hipErrorNotReady) UFVM_HIP(q);
+    return false;
+  };
+  for (;;) {
+    if (phase_ == 4) return true;
+    if (phase_ == 3) {
+      if (!ready(W_->ev[2])) return false;
+      phase_ = 4;
+      continue;
+    }
+    if (phase_ == 1) {
+      if (!ready(W_->ev[2])) return false;
+      int left = 0;
+      for (int q = 0; q < k_; ++q) {
+        nb_[q] = std::sqrt(W_->hring[q * kRingPer + 4]);
+        if (std::isnan(nb_[q])) throw Error(UFVM_ERR_NUMERIC, "NaN right-hand side");
+        thr_[q] = (cfg.tol * nb_[q]) * (cfg.tol * nb_[q]);
+        rr_[q] = nb_[q];
+        its_[q] = 0;
+        // a zero right-hand side is solved by x = 0: its updates stay off
+        done_[q] = !(nb_[q] > 0.0) || cfg.max_iter <= 0;
+        k::fcg_arm(s_, W_->scal.p + q * kScal, thr_[q], done_[q] ? 1 : 0);
+        left += !done_[q];
+      }
+      if (left == 0) {
+        finish();
+        continue;
+      }
+      phase_ = 2;
+    }
+    // phase 2: one iteration queued beyond the awaited one
+    while (issued_ < cfg.max_iter && issued_ < it_ + 2) {
+      if (S_->pcg.cfg.precision == 32) issue_t<float>(issued_);
+      else issue_t<double>(issued_);
+      ++issued_;
+    }
+    if (!ready(W_->ev[it_ & 1])) return false;
+    int left = 0;
+    for (int q = 0; q < k_; ++q) {
+      const volatile double *slot = W_->hring + q * kRingPer + 2 * (it_ & 1);
+      if (slot[1] != (double)it_) throw Error(UFVM_ERR_STATE, "FPCG residual ring out of step");
+      if (done_[q]) continue;
+      const double rr2 = slot[0];
+      rr_[q] = std::sqrt(rr2);
+      its_[q] = it_ + 1;
+      if (std::isnan(rr_[q])) throw Error(UFVM_ERR_NUMERIC, "FPCG breakdown (NaN residual)");
+      if (rr2 <= thr_[q]) done_[q] = true;
+      left += !done_[q];
+    }
+    ++it_;
+    if (left == 0 || it_ >= cfg.max_iter) {
+      finish();
+      continue;
+    }
+  }
+}
+
+// ufvm_pcg_solve_batch: k right-hand sides (RHS-major, reference order, host
+// or device memory) in, k solutions out, on the context stream
+void pcg_solve_batch(ufvm_ctx *ctx, System &S, int k, const double *B, double *X,
+                     int32_t *iters, double *relres) {
+  PcgState &P = S.pcg;
+  if (!P.ready || P.stale)
+    throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first (or refresh after new coefficients)");
+  if (k != 1 && k != 2 && k != 4 && k != 8)
+    throw Error(UFVM_ERR_ARG, "a batch holds 1, 2, 4 or 8 right-hand sides");
+  std::unique_ptr<SolveWork> &W = P.work_batch[k];
+  if (!W) W = make_work(S, ctx->stream, k);
+  FcgBatch run(ctx, S, *W, ctx->stream);
+  run.start(B, X);
+  while (!run.poll(true)) {
+  }
+  for (int q = 0; q < k; ++q) {
+    if (iters) iters[q] = run.iterations(q);
+    if (relres) relres[q] = run.relres(q);
   }
 }
 

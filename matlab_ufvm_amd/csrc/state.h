@@ -154,10 +154,13 @@ struct FastLevel {
 struct LevelWork {
   DBuf<float> b32, x32, r32, y32;
   DBuf<double> b64, x64, r64, y64;
+  int64_t ld = 0;                // stride between the right-hand sides of a batch
 };
 
 struct SolveWork {
   int32_t n0 = 0;
+  int32_t nrhs = 1;              // right-hand sides solved together (vectors RHS-major)
+  int64_t ld0 = 0;               // stride between them in the finest-level vectors
   std::vector<LevelWork> lev;
   DBuf<double> x, r, p, w;       // FCG vectors, finest level in its V-cycle numbering
   DBuf<double> bref, xref;       // right-hand side / solution in reference numbering
@@ -192,6 +195,7 @@ struct PcgState {
   std::vector<std::unique_ptr<FastLevel>> lev;
   DBuf<double> ainv, colbuf;
   std::unique_ptr<SolveWork> work0;   // the workspace of ufvm_pcg_solve
+  std::map<int, std::unique_ptr<SolveWork>> work_batch;   // of ufvm_pcg_solve_batch, by k
 };
 
 struct System {

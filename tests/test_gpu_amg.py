@@ -298,3 +298,49 @@ def test_invalid_pcg_config_keeps_previous_setup(ctx):
         assert it == it0 and rr == rr0
         assert np.array_equal(ctx.get_dphi(sid, 0), x0)
     ctx.system_destroy(sid)
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_batched_solves_match_single_solves(ctx, prec):
+    # k right-hand sides solved together give each one the bits, iteration
+    # count and residual of its own solve; a zero right-hand side gives x = 0
+    for m, dirichlet in ((cavity_mesh(14, jitter=0.15, seed=81, renumber_seed=82), ()),
+                         (cavity_mesh(48, nz=1, jitter=0.1, seed=83, renumber_seed=84),
+                          ("movingWall",))):
+        ptr, col, ac, anb = poisson_system(m, dirichlet=dirichlet)
+        B = np.stack([gaussian_rhs(m, seed=90 + q) * (1.0 + q) for q in range(8)])
+        B[5] = 0.0
+        sid = make(ctx, ptr, col, ac, anb, B[0])
+        ctx.agglomerate(sid, 10)
+        ctx.pcg_setup(sid, precision=prec, max_coarse_cells=200, tol=1e-9)
+        single = []
+        for q in range(8):
+            ctx.set_bc(sid, 0, B[q])
+            it, rr = ctx.pcg_solve(sid)
+            single.append((it, rr, ctx.get_dphi(sid, 0)))
+        print(f"\n[batch] n={m.n_cells} iterations {[s[0] for s in single]}", flush=True)
+        for k in (2, 4, 8):
+            for start in range(0, 8, k):
+                X, its, rel = ctx.pcg_solve_batch(sid, B[start:start + k])
+                for q in range(k):
+                    it, rr, x = single[start + q]
+                    assert its[q] == it and rel[q] == rr, (k, start + q)
+                    assert np.array_equal(X[q], x), (k, start + q)
+        assert not single[5][2].any() and single[5][0] == 0
+        ctx.system_destroy(sid)
+
+
+def test_batched_solve_rejects_unsupported(ctx):
+    from matlab_ufvm_amd._lib import UfvmError
+    m = cavity_mesh(10, jitter=0.1, seed=85, renumber_seed=86)
+    ptr, col, ac, anb = poisson_system(m, dirichlet=("movingWall",))
+    b = gaussian_rhs(m, seed=87)
+    sid = make(ctx, ptr, col, ac, anb, b)
+    ctx.agglomerate(sid, 10)
+    ctx.pcg_setup(sid, precision=32, max_coarse_cells=200)
+    with pytest.raises(UfvmError):
+        ctx.pcg_solve_batch(sid, np.stack([b, b, b]))          # k = 3
+    ctx.pcg_setup(sid, precision=32, max_coarse_cells=200, smoother="ILU")
+    with pytest.raises(UfvmError):
+        ctx.pcg_solve_batch(sid, np.stack([b, b]))             # DILU finest smoother
+    ctx.system_destroy(sid)
