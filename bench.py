@@ -187,7 +187,8 @@ def serving_bench(args):
                         renumber_seed=2027 + n, apex="own")
         pool.append((spec, m))
     log(f"serving pool: {[(s_, m.n_cells) for s_, m in pool]} ({time.time()-t:.1f}s)")
-    srv = Server(max_active=args.serving_active, n_streams=args.serving_streams)
+    srv = Server(max_active=args.serving_active, n_streams=args.serving_streams,
+                 max_batch=args.serving_max_batch)
     t = time.time()
     sids, cids = [], []
     for spec, m in pool:
@@ -219,6 +220,8 @@ def serving_bench(args):
                                   "steps (lid-driven cavity, Re 400) on a resident mesh pool",
                       "meshes": {s_: int(m.n_cells) for s_, m in pool},
                       "max_active": args.serving_active, "streams": args.serving_streams,
+                      "max_batch": args.serving_max_batch,
+                      "momentum_smoother": args.serving_u_smoother,
                       "tol": 1e-8, "precision": "f32 V-cycle, f64 FPCG"},
            "data": "synthetic"}
     for kind in ("solve", "step"):
@@ -226,6 +229,9 @@ def serving_bench(args):
         if l.size:
             out[kind] = {"count": int(l.size), "p50_ms": round(float(np.median(l)) * 1e3, 2),
                          "p99_ms": round(float(np.percentile(l, 99)) * 1e3, 2)}
+    bs = [r.extra.get("batch", 1) for r in reqs if r.kind == "solve"]
+    if bs:
+        out["solve"]["batch_sizes"] = {str(k): bs.count(k) for k in sorted(set(bs))}
     print(json.dumps(out), flush=True)
 
 
@@ -502,6 +508,8 @@ def main():
     ap.add_argument("--serving-meshes", default="3d:37,3d:64,3d:100,3d:128,2d:224,2d:512,"
                                                 "2d:1024,2d:2048")
     ap.add_argument("--serving-requests", type=int, default=200)
+    ap.add_argument("--serving-max-batch", type=int, default=8, choices=[1, 2, 4, 8],
+                    help="queued solves of one mesh admitted together as one batch")
     ap.add_argument("--serving-u-smoother", choices=["ILU", "MCGS"], default="MCGS",
                     help="momentum solves of the SIMPLE-step requests")
     ap.add_argument("--serving-rate", type=float, default=200.0)

@@ -286,6 +286,10 @@ typedef struct {
   int32_t max_active;        /* solves on the GPU at once (>= 1) */
   int32_t n_streams;         /* streams the active solves are spread over (>= 1) */
   double hbm_reserve;        /* fraction of device memory admission keeps free */
+  int32_t max_batch;         /* queued solves of one system admitted together as one batch of
+                                up to this many (2, 4 or 8; 1: no batching; 0: 8).  A
+                                batch reads the matrices once per iteration for all its
+                                solves and gives each the bits of its own solve */
 } ufvm_server_cfg;
 typedef struct {
   int32_t status;            /* 0 queued, 1 running, 2 done, < 0: -UFVM_ERR_* */
@@ -293,6 +297,7 @@ typedef struct {
   double relres;             /* solve: ||r|| / ||b|| */
   double queued_s;           /* submit -> admission (seconds) */
   double run_s;              /* admission -> result on the host */
+  int32_t batch;             /* solve: solves in the batch it ran in (1: alone) */
 } ufvm_request_info;
 
 int ufvm_server_create(int32_t device, const ufvm_server_cfg *cfg, ufvm_server **out);

@@ -72,7 +72,8 @@ class FcgBatch {
  public:
   FcgBatch(ufvm_ctx *ctx, System &S, SolveWork &W, hipStream_t s);
   // B: k right-hand sides, X: k solutions, each n values in reference order
-  // one after another, host or device memory (X written when poll returns true)
+  // one after another, host or device memory (X written when poll returns
+  // true); B / X may be the workspace's own bref / xref
   void start(const double *B, double *X);
   bool poll(bool block);
   int32_t iterations(int q) const { return its_[q]; }
@@ -98,6 +99,9 @@ class FcgBatch {
   std::vector<char> done_;
 };
 
+// whether S's precision-form set-up can run batched solves (multicolour GS
+// finest level, Jacobi coarse levels)
+bool batch_supported(const System &S);
 // blocking batch solve on the context stream (workspace cached per k)
 void pcg_solve_batch(ufvm_ctx *ctx, System &S, int k, const double *B, double *X,
                      int32_t *iters, double *relres);

@@ -834,6 +834,15 @@ struct CycleB {
   }
 };
 
+bool batch_supported(const System &S) {
+  const PcgState &P = S.pcg;
+  if (!P.ready) return false;
+  if (P.nlev > 1 && P.lev[0]->kind != 0) return false;
+  for (int32_t l = 1; l + 1 < P.nlev; ++l)
+    if (P.lev[l]->kind != 3) return false;
+  return true;
+}
+
 FcgBatch::FcgBatch(ufvm_ctx *ctx, System &S, SolveWork &W, hipStream_t s)
     : ctx_(ctx), S_(&S), W_(&W), s_(s), k_(W.nrhs), nb_(k_), thr_(k_), rr_(k_), its_(k_),
       done_(k_) {
@@ -889,7 +898,8 @@ void FcgBatch::start(const double *B, double *X) {
   if (!B || !X) throw Error(UFVM_ERR_ARG, "null right-hand sides or solutions");
   X_ = X;
   it_ = issued_ = 0;
-  UFVM_HIP(hipMemcpyAsync(W_->bref.p, B, sizeof(double) * W_->n0 * k_, hipMemcpyDefault, s_));
+  if (B != W_->bref.p)
+    UFVM_HIP(hipMemcpyAsync(W_->bref.p, B, sizeof(double) * W_->n0 * k_, hipMemcpyDefault, s_));
   if (P.cfg.precision == 32) start_t<float>();
   else start_t<double>();
   phase_ = 1;
@@ -920,7 +930,8 @@ void FcgBatch::finish() {
   SolveWork &W = *W_;
   for (int q = 0; q < k_; ++q)
     k::permute_scatter(s_, W.n0, L0.perm_d.p, W.x.p + q * W.ld0, W.xref.p + (int64_t)q * W.n0);
-  UFVM_HIP(hipMemcpyAsync(X_, W.xref.p, sizeof(double) * W.n0 * k_, hipMemcpyDefault, s_));
+  if (X_ != W.xref.p)
+    UFVM_HIP(hipMemcpyAsync(X_, W.xref.p, sizeof(double) * W.n0 * k_, hipMemcpyDefault, s_));
   UFVM_HIP(hipEventRecord(W.ev[2], s_));
   phase_ = 3;
 }

@@ -46,18 +46,21 @@ struct Request {
   int status = 0;               // 0 queued, 1 running, 2 done, < 0 error
   std::string err;
   int32_t iterations = 0;
+  int32_t batch = 0;
   double relres = 0.0;
   double t_submit = 0.0, t_start = 0.0, t_done = 0.0;
 };
 
+// One admitted solve, or a batch of k solves of one system (FcgBatch).
 struct Slot {
-  Request *req = nullptr;
+  std::vector<Request *> reqs;   // empty: free
   System *sys = nullptr;
   hipStream_t stream = nullptr;
   std::unique_ptr<SolveWork> work;
-  std::unique_ptr<FcgRun> run;
+  std::unique_ptr<FcgRun> run;     // k == 1
+  std::unique_ptr<FcgBatch> brun;  // k > 1
   hipEvent_t copied = nullptr;
-  int phase = 0;   // 1 solving, 2 result copy queued
+  int phase = 0;   // 1 solving, 2 result copies queued
 };
 
 }  // namespace detail
@@ -83,7 +86,8 @@ struct ufvm_server {
   bool stop = false;
   std::deque<Request *> pend_solve, pend_step;
   std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
-  std::map<int32_t, std::vector<std::unique_ptr<SolveWork>>> pool;   // free workspaces
+  // free workspaces by (system, batch size)
+  std::map<std::pair<int32_t, int>, std::vector<std::unique_ptr<SolveWork>>> pool;
   int64_t next_ticket = 1;
   int32_t n_active = 0;
   size_t work_bytes_held = 0;
@@ -130,58 +134,78 @@ bool fits(ufvm_server *srv, size_t bytes) {
   return (double)free_b - (double)bytes > reserve;
 }
 
-// Admit queued solves into free slots (worker thread).
+// Admit queued solves into free slots (worker thread).  The oldest queued
+// solve is admitted together with later queued solves of the same system, as
+// one batch of the largest size in {8, 4, 2} (up to max_batch) they fill.
 void admit(ufvm_server *srv) {
   for (Slot &sl : srv->slots) {
-    if (sl.req) continue;
-    Request *r = nullptr;
+    if (!sl.reqs.empty()) continue;
+    std::vector<Request *> group;
     std::unique_ptr<SolveWork> W;
     System *S = nullptr;
     {
       std::lock_guard<std::mutex> g(srv->mu);
       if (srv->pend_solve.empty()) return;
-      r = srv->pend_solve.front();
+      Request *r = srv->pend_solve.front();
       {
         std::lock_guard<std::mutex> gs(srv->sys_mu);
         auto it = srv->ctx_p->systems.find(r->id);
         S = it == srv->ctx_p->systems.end() ? nullptr : it->second.get();
       }
+      group.push_back(r);
+      if (S && srv->cfg.max_batch > 1 && batch_supported(*S))
+        for (auto it = srv->pend_solve.begin() + 1;
+             it != srv->pend_solve.end() && (int)group.size() < srv->cfg.max_batch; ++it)
+          if ((*it)->id == r->id) group.push_back(*it);
+      int k = 1;
+      while (k * 2 <= (int)group.size()) k *= 2;
+      group.resize(k);
       if (S) {
-        auto &fl = srv->pool[r->id];
+        auto &fl = srv->pool[{r->id, k}];
         if (!fl.empty()) {
           W = std::move(fl.back());
           fl.pop_back();
-        } else if (!fits(srv, work_bytes(*S))) {
+        } else if (!fits(srv, work_bytes(*S, k))) {
           return;   // admission by free memory: wait for running solves to finish
         }
       }
-      srv->pend_solve.pop_front();
-      r->status = 1;
-      r->t_start = now_s();
+      for (Request *q : group) {
+        srv->pend_solve.erase(std::find(srv->pend_solve.begin(), srv->pend_solve.end(), q));
+        q->status = 1;
+        q->t_start = now_s();
+      }
     }
     if (!S) {
-      complete(srv, r, -UFVM_ERR_ARG, "unknown system");
+      complete(srv, group[0], -UFVM_ERR_ARG, "unknown system");
       continue;
     }
+    const int k = (int)group.size();
     try {
       if (!W) {
-        W = make_work(*S, sl.stream);
+        W = make_work(*S, sl.stream, k);
         std::lock_guard<std::mutex> g(srv->mu);
         srv->work_bytes_held += W->bytes;
       }
-      UFVM_HIP(hipMemcpyAsync(W->bref.p, r->b, (size_t)W->n0 * sizeof(double),
-                              hipMemcpyHostToDevice, sl.stream));
-      sl.run.reset(new FcgRun(srv->ctx_p, *S, *W, sl.stream));
-      sl.run->start(W->bref.p, W->xref.p);
-      sl.req = r;
+      for (int q = 0; q < k; ++q)
+        UFVM_HIP(hipMemcpyAsync(W->bref.p + (int64_t)q * W->n0, group[q]->b,
+                                (size_t)W->n0 * sizeof(double), hipMemcpyHostToDevice,
+                                sl.stream));
+      if (k == 1) {
+        sl.run.reset(new FcgRun(srv->ctx_p, *S, *W, sl.stream));
+        sl.run->start(W->bref.p, W->xref.p);
+      } else {
+        sl.brun.reset(new FcgBatch(srv->ctx_p, *S, *W, sl.stream));
+        sl.brun->start(W->bref.p, W->xref.p);
+      }
+      sl.reqs = group;
       sl.sys = S;
       sl.work = std::move(W);
       sl.phase = 1;
       std::lock_guard<std::mutex> g(srv->mu);
-      ++srv->n_active;
+      srv->n_active += k;
     } catch (const Error &e) {
       (void)hipStreamSynchronize(sl.stream);
-      complete(srv, r, -e.code, e.what());
+      for (Request *q : group) complete(srv, q, -e.code, e.what());
     }
   }
 }
@@ -190,12 +214,19 @@ void admit(ufvm_server *srv) {
 bool progress(ufvm_server *srv) {
   bool moved = false;
   for (Slot &sl : srv->slots) {
-    if (!sl.req) continue;
-    Request *r = sl.req;
+    if (sl.reqs.empty()) continue;
+    const int k = (int)sl.reqs.size();
+    auto release = [&](bool keep_work) {
+      std::lock_guard<std::mutex> g(srv->mu);
+      if (keep_work && sl.work) srv->pool[{sl.reqs[0]->id, k}].push_back(std::move(sl.work));
+      srv->n_active -= k;
+    };
     try {
-      if (sl.phase == 1 && sl.run->poll(false)) {
-        UFVM_HIP(hipMemcpyAsync(r->x, sl.work->xref.p, (size_t)sl.work->n0 * sizeof(double),
-                                hipMemcpyDeviceToHost, sl.stream));
+      if (sl.phase == 1 && (k == 1 ? sl.run->poll(false) : sl.brun->poll(false))) {
+        for (int q = 0; q < k; ++q)
+          UFVM_HIP(hipMemcpyAsync(sl.reqs[q]->x, sl.work->xref.p + (int64_t)q * sl.work->n0,
+                                  (size_t)sl.work->n0 * sizeof(double), hipMemcpyDeviceToHost,
+                                  sl.stream));
         UFVM_HIP(hipEventRecord(sl.copied, sl.stream));
         sl.phase = 2;
         moved = true;
@@ -206,27 +237,27 @@ bool progress(ufvm_server *srv) {
         UFVM_HIP(q);
         {
           std::lock_guard<std::mutex> g(srv->mu);
-          r->iterations = sl.run->iterations();
-          r->relres = sl.run->relres();
-          srv->pool[r->id].push_back(std::move(sl.work));
-          --srv->n_active;
+          for (int i = 0; i < k; ++i) {
+            sl.reqs[i]->iterations = k == 1 ? sl.run->iterations() : sl.brun->iterations(i);
+            sl.reqs[i]->batch = k;
+            sl.reqs[i]->relres = k == 1 ? sl.run->relres() : sl.brun->relres(i);
+          }
         }
-        complete(srv, r, 2, "");
-        sl.req = nullptr;
+        release(true);
+        for (Request *r : sl.reqs) complete(srv, r, 2, "");
+        sl.reqs.clear();
         sl.run.reset();
+        sl.brun.reset();
         sl.phase = 0;
         moved = true;
       }
     } catch (const Error &e) {
       (void)hipStreamSynchronize(sl.stream);
-      {
-        std::lock_guard<std::mutex> g(srv->mu);
-        if (sl.work) srv->pool[r->id].push_back(std::move(sl.work));
-        --srv->n_active;
-      }
-      complete(srv, r, -e.code, e.what());
-      sl.req = nullptr;
+      release(true);
+      for (Request *r : sl.reqs) complete(srv, r, -e.code, e.what());
+      sl.reqs.clear();
       sl.run.reset();
+      sl.brun.reset();
       sl.phase = 0;
       moved = true;
     }
@@ -240,7 +271,7 @@ void worker_main(ufvm_server *srv) {
     admit(srv);
     const bool moved = progress(srv);
     bool active = false;
-    for (Slot &sl : srv->slots) active = active || sl.req != nullptr;
+    for (Slot &sl : srv->slots) active = active || !sl.reqs.empty();
     std::unique_lock<std::mutex> g(srv->mu);
     if (!active && srv->pend_solve.empty()) {
       if (srv->stop) return;
@@ -288,6 +319,7 @@ void fill_info(const Request *r, ufvm_request_info *info) {
   info->relres = r->relres;
   info->queued_s = r->t_start > 0 ? r->t_start - r->t_submit : now_s() - r->t_submit;
   info->run_s = r->t_done > 0 ? r->t_done - r->t_start : 0.0;
+  info->batch = r->batch;
 }
 
 }  // namespace
@@ -300,10 +332,13 @@ int ufvm_server_create(int32_t device, const ufvm_server_cfg *cfg, ufvm_server *
   if (cfg->max_active < 1 || cfg->n_streams < 1 || cfg->hbm_reserve < 0.0 ||
       cfg->hbm_reserve >= 1.0)
     return UFVM_ERR_ARG;
+  const int32_t mb = cfg->max_batch == 0 ? 8 : cfg->max_batch;
+  if (mb != 1 && mb != 2 && mb != 4 && mb != 8) return UFVM_ERR_ARG;
   std::unique_ptr<ufvm_server> srv(new (std::nothrow) ufvm_server());
   if (!srv) return UFVM_ERR_ARG;
   srv->device = device;
   srv->cfg = *cfg;
+  srv->cfg.max_batch = mb;
   int rc = ufvm_ctx_create(device, &srv->ctx_p);
   if (rc == UFVM_OK) rc = ufvm_ctx_create(device, &srv->ctx_b);
   if (rc == UFVM_OK) rc = ufvm_ctx_create(device, &srv->ctx_s);
@@ -341,6 +376,7 @@ int ufvm_server_destroy(ufvm_server *srv) {
   (void)hipSetDevice(srv->device);
   for (Slot &sl : srv->slots) {
     sl.run.reset();
+    sl.brun.reset();
     sl.work.reset();
     if (sl.copied) (void)hipEventDestroy(sl.copied);
   }

@@ -2,10 +2,12 @@
 
 `Server` wraps the ufvm_server_* C ABI (include/ufvm.h).  The request loop
 itself is native: a worker thread inside libufvm.so continuously batches
-pressure solves (up to `max_active` on the GPU at once, each advanced one FCG
-iteration at a time as the GPU finishes it, spread over `n_streams` streams)
-and admits queued requests as slots and device memory free up; a second
-thread runs SIMPLE steps on its own stream.
+pressure solves (up to `max_active` slots on the GPU at once, each advanced one
+FCG iteration at a time as the GPU finishes it, spread over `n_streams`
+streams; queued solves of one system share a slot as one batch of up to
+`max_batch`, whose kernels read the matrices once for all of them) and admits
+queued requests as slots and device memory free up; a second thread runs
+SIMPLE steps on its own stream.
 
 `request_stream` and `run_stream` drive the mixed workload of the serving
 configuration: Poisson arrivals, pressure solves and single SIMPLE steps on a
@@ -28,11 +30,15 @@ STATUS = {0: "queued", 1: "running", 2: "done"}
 
 class Server:
     def __init__(self, device: int = 0, max_active: int = 8, n_streams: int = 4,
-                 hbm_reserve: float = 0.05):
+                 hbm_reserve: float = 0.05, max_batch: int = 8):
+        """max_active: solves (or batches of solves) on the GPU at once;
+        max_batch: queued solves of one system admitted together as one batch
+        (1, 2, 4 or 8; each still gets the bits of its own solve)."""
         self.lib = _lib.load()
         h = C.c_void_p()
         rc = self.lib.ufvm_server_create(device, C.byref(ServerCfg(max_active, n_streams,
-                                                                   hbm_reserve)), C.byref(h))
+                                                                   hbm_reserve, max_batch)),
+                                         C.byref(h))
         if rc != _lib.UFVM_OK:
             raise _lib.UfvmError(rc, "ufvm_server_create failed (no GPU visible?)")
         self.h = h
@@ -123,7 +129,8 @@ class Server:
     def _result(self, ticket, info):
         kind = self._live[ticket][0]
         out = {"status": info.status, "iterations": info.iterations, "relres": info.relres,
-               "queued_s": info.queued_s, "run_s": info.run_s, "kind": kind}
+               "queued_s": info.queued_s, "run_s": info.run_s, "kind": kind,
+               "batch": info.batch}
         if info.status == 2:
             if kind == "solve":
                 out["x"] = self._live[ticket][2]
@@ -198,4 +205,5 @@ def run_stream(srv: Server, reqs, systems, cases, rhs):
         res = srv.wait(t)
         r.latency = res["queued_s"] + res["run_s"]
         r.iterations = res["iterations"]
+        r.extra["batch"] = res["batch"]
     return time.perf_counter() - t0

@@ -85,7 +85,7 @@ def test_steps_same_alone_and_in_mix(meshes):
 
 
 def test_admission_queue_and_errors(meshes):
-    srv = Server(max_active=1, n_streams=1)
+    srv = Server(max_active=1, n_streams=1, max_batch=1)
     try:
         sid = _register(srv, meshes[:1])[0]
         bs = [gaussian_rhs(meshes[0], seed=90 + i) for i in range(5)]
@@ -102,6 +102,30 @@ def test_admission_queue_and_errors(meshes):
             srv.wait(_submit_unknown(srv, sid + 99, bs[0]))
         r = srv.wait(srv.submit_solve(sid, bs[0]))
         assert np.array_equal(r["x"], res[0]["x"])
+    finally:
+        srv.close()
+
+
+def test_batched_admission_same_bits(meshes):
+    # one slot: solves queued behind the first are admitted as batches of the
+    # same system, and every result has the bits of the solve made alone
+    srv = Server(max_active=1, n_streams=1, max_batch=8)
+    try:
+        sids = _register(srv, meshes[:2])
+        rhs = [gaussian_rhs(meshes[i % 2], seed=100 + i) for i in range(18)]
+        alone = []
+        for i, b in enumerate(rhs):
+            alone.append(srv.wait(srv.submit_solve(sids[i % 2], b)))
+        ts = [srv.submit_solve(sids[i % 2], b) for i, b in enumerate(rhs)]
+        batches = []
+        for i, t in enumerate(ts):
+            r = srv.wait(t)
+            batches.append(r["batch"])
+            assert r["iterations"] == alone[i]["iterations"] and r["relres"] == alone[i]["relres"]
+            assert np.array_equal(r["x"], alone[i]["x"]), i
+        print(f"\n[serving] batch sizes {batches}", flush=True)
+        assert all(a["batch"] == 1 for a in alone)
+        assert max(batches) > 1
     finally:
         srv.close()
 
