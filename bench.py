@@ -188,7 +188,7 @@ def serving_bench(args):
         pool.append((spec, m))
     log(f"serving pool: {[(s_, m.n_cells) for s_, m in pool]} ({time.time()-t:.1f}s)")
     srv = Server(max_active=args.serving_active, n_streams=args.serving_streams,
-                 max_batch=args.serving_max_batch)
+                 max_batch=args.serving_max_batch, n_steppers=args.serving_steppers)
     t = time.time()
     sids, cids = [], []
     for spec, m in pool:
@@ -221,6 +221,7 @@ def serving_bench(args):
                       "meshes": {s_: int(m.n_cells) for s_, m in pool},
                       "max_active": args.serving_active, "streams": args.serving_streams,
                       "max_batch": args.serving_max_batch,
+                      "step_lanes": args.serving_steppers,
                       "momentum_smoother": args.serving_u_smoother,
                       "tol": 1e-8, "precision": "f32 V-cycle, f64 FPCG"},
            "data": "synthetic"}
@@ -508,6 +509,8 @@ def main():
     ap.add_argument("--serving-meshes", default="3d:37,3d:64,3d:100,3d:128,2d:224,2d:512,"
                                                 "2d:1024,2d:2048")
     ap.add_argument("--serving-requests", type=int, default=200)
+    ap.add_argument("--serving-steppers", type=int, default=2,
+                    help="SIMPLE-step lanes of the server")
     ap.add_argument("--serving-max-batch", type=int, default=8, choices=[1, 2, 4, 8],
                     help="queued solves of one mesh admitted together as one batch")
     ap.add_argument("--serving-u-smoother", choices=["ILU", "MCGS"], default="MCGS",

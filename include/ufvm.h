@@ -290,6 +290,10 @@ typedef struct {
                                 up to this many (2, 4 or 8; 1: no batching; 0: 8).  A
                                 batch reads the matrices once per iteration for all its
                                 solves and gives each the bits of its own solve */
+  int32_t n_steppers;        /* SIMPLE-step lanes (0: 2): each owns the cases assigned to it
+                                (the least-loaded lane by cells at ufvm_server_add_case), a
+                                context, a stream and a thread, so steps of cases in
+                                different lanes overlap on the device */
 } ufvm_server_cfg;
 typedef struct {
   int32_t status;            /* 0 queued, 1 running, 2 done, < 0: -UFVM_ERR_* */

@@ -44,7 +44,7 @@ class PcgCfg(C.Structure):
 
 class ServerCfg(C.Structure):
     _fields_ = [("max_active", C.c_int32), ("n_streams", C.c_int32), ("hbm_reserve", C.c_double),
-                ("max_batch", C.c_int32)]
+                ("max_batch", C.c_int32), ("n_steppers", C.c_int32)]
 
 
 class RequestInfo(C.Structure):

@@ -7,8 +7,10 @@
 // time as the GPU finishes it (FcgRun state machines, polled without
 // blocking).  A request that arrives while others run is admitted as soon as
 // a slot is free and the device has memory for its workspace (workspaces of
-// finished solves are kept per system and reused).  SIMPLE steps run on a
-// second thread with its own stream, overlapping the solves on the device.
+// finished solves are kept per system and reused); queued solves of one system
+// share a slot as one batch.  SIMPLE steps run in lanes: each lane owns some
+// cases, a context (stream) and a thread, so steps of cases in different
+// lanes overlap each other and the solves on the device.
 // Each solve's arithmetic depends only on its system and right-hand side, so
 // a result is the same alone and in any mix.
 #include <hip/hip_runtime.h>
@@ -74,17 +76,24 @@ struct ufvm_server {
   ufvm_server_cfg cfg{};
   ufvm_ctx *ctx_p = nullptr;   // pressure systems being served
   ufvm_ctx *ctx_b = nullptr;   // builds new systems (then handed to ctx_p)
-  ufvm_ctx *ctx_s = nullptr;   // SIMPLE cases
+  // SIMPLE-step lanes: lane i owns the cases of ctx_s[i] and serves
+  // pend_step[i] on its own thread
+  std::vector<ufvm_ctx *> ctx_s;
+  std::vector<std::unique_ptr<std::mutex>> case_mu;   // per lane: its context's cases
+  std::vector<std::deque<Request *>> pend_step;
+  std::vector<int64_t> lane_cells;
+  std::unordered_map<int32_t, std::pair<int, int32_t>> case_of;   // server case -> lane, case
+  int32_t next_case = 0;
   std::vector<hipStream_t> streams;
   std::vector<Slot> slots;
-  std::thread worker, stepper;
+  std::thread worker;
+  std::vector<std::thread> steppers;
   std::mutex mu;                       // requests, queues, pools, stop flag
   std::condition_variable cv_work, cv_done;
   std::mutex sys_mu;                   // ctx_p's system map (held only for lookups/inserts)
   std::mutex build_mu;                 // ctx_b
-  std::mutex case_mu;                  // ctx_s (cases and their steps)
   bool stop = false;
-  std::deque<Request *> pend_solve, pend_step;
+  std::deque<Request *> pend_solve;
   std::unordered_map<int64_t, std::unique_ptr<Request>> reqs;
   // free workspaces by (system, batch size)
   std::map<std::pair<int32_t, int>, std::vector<std::unique_ptr<SolveWork>>> pool;
@@ -284,26 +293,38 @@ void worker_main(ufvm_server *srv) {
   }
 }
 
-void stepper_main(ufvm_server *srv) {
+void stepper_main(ufvm_server *srv, int lane) {
   (void)hipSetDevice(srv->device);
+  ufvm_ctx *ctx = srv->ctx_s[lane];
   for (;;) {
     Request *r = nullptr;
+    int32_t cas = 0;
     {
       std::unique_lock<std::mutex> g(srv->mu);
-      srv->cv_work.wait(g, [&] { return srv->stop || !srv->pend_step.empty(); });
-      if (srv->pend_step.empty()) return;   // stop requested and nothing left
-      r = srv->pend_step.front();
-      srv->pend_step.pop_front();
+      auto &q = srv->pend_step[lane];
+      srv->cv_work.wait(g, [&] { return srv->stop || !q.empty(); });
+      if (q.empty()) return;   // stop requested and nothing left
+      r = q.front();
+      q.pop_front();
       r->status = 1;
       r->t_start = now_s();
+      cas = srv->case_of.at(r->id).second;
     }
     int rc;
     {
-      std::lock_guard<std::mutex> g(srv->case_mu);
-      rc = ufvm_simple_iterate(srv->ctx_s, r->id, r->nsteps, r->rms);
+      std::lock_guard<std::mutex> g(*srv->case_mu[lane]);
+      rc = ufvm_simple_iterate(ctx, cas, r->nsteps, r->rms);
     }
-    complete(srv, r, rc == UFVM_OK ? 2 : -rc, rc == UFVM_OK ? "" : ufvm_last_error(srv->ctx_s));
+    complete(srv, r, rc == UFVM_OK ? 2 : -rc, rc == UFVM_OK ? "" : ufvm_last_error(ctx));
   }
+}
+
+// lane and lane-local case of a server case id
+std::pair<int, int32_t> case_lane(ufvm_server *srv, int32_t cas) {
+  std::lock_guard<std::mutex> g(srv->mu);
+  auto it = srv->case_of.find(cas);
+  if (it == srv->case_of.end()) throw Error(UFVM_ERR_ARG, "unknown case");
+  return it->second;
 }
 
 Request *find(ufvm_server *srv, int64_t ticket) {
@@ -326,53 +347,8 @@ void fill_info(const Request *r, ufvm_request_info *info) {
 
 extern "C" {
 
-int ufvm_server_create(int32_t device, const ufvm_server_cfg *cfg, ufvm_server **out) {
-  if (!out || !cfg) return UFVM_ERR_ARG;
-  *out = nullptr;
-  if (cfg->max_active < 1 || cfg->n_streams < 1 || cfg->hbm_reserve < 0.0 ||
-      cfg->hbm_reserve >= 1.0)
-    return UFVM_ERR_ARG;
-  const int32_t mb = cfg->max_batch == 0 ? 8 : cfg->max_batch;
-  if (mb != 1 && mb != 2 && mb != 4 && mb != 8) return UFVM_ERR_ARG;
-  std::unique_ptr<ufvm_server> srv(new (std::nothrow) ufvm_server());
-  if (!srv) return UFVM_ERR_ARG;
-  srv->device = device;
-  srv->cfg = *cfg;
-  srv->cfg.max_batch = mb;
-  int rc = ufvm_ctx_create(device, &srv->ctx_p);
-  if (rc == UFVM_OK) rc = ufvm_ctx_create(device, &srv->ctx_b);
-  if (rc == UFVM_OK) rc = ufvm_ctx_create(device, &srv->ctx_s);
-  if (rc != UFVM_OK) {
-    if (srv->ctx_p) ufvm_ctx_destroy(srv->ctx_p);
-    if (srv->ctx_b) ufvm_ctx_destroy(srv->ctx_b);
-    return rc;
-  }
-  if (hipSetDevice(device) != hipSuccess) return UFVM_ERR_HIP;
-  srv->streams.resize(cfg->n_streams);
-  for (hipStream_t &s : srv->streams)
-    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return UFVM_ERR_HIP;
-  srv->slots.resize(cfg->max_active);
-  for (size_t i = 0; i < srv->slots.size(); ++i) {
-    srv->slots[i].stream = srv->streams[i % srv->streams.size()];
-    if (hipEventCreateWithFlags(&srv->slots[i].copied, hipEventDisableTiming) != hipSuccess)
-      return UFVM_ERR_HIP;
-  }
-  ufvm_server *p = srv.release();
-  p->worker = std::thread(worker_main, p);
-  p->stepper = std::thread(stepper_main, p);
-  *out = p;
-  return UFVM_OK;
-}
-
-int ufvm_server_destroy(ufvm_server *srv) {
-  if (!srv) return UFVM_ERR_ARG;
-  {
-    std::lock_guard<std::mutex> g(srv->mu);
-    srv->stop = true;
-  }
-  srv->cv_work.notify_all();
-  if (srv->worker.joinable()) srv->worker.join();
-  if (srv->stepper.joinable()) srv->stepper.join();
+// frees whatever a (possibly partly built) server holds; threads joined first
+static void free_server(ufvm_server *srv) {
   (void)hipSetDevice(srv->device);
   for (Slot &sl : srv->slots) {
     sl.run.reset();
@@ -388,8 +364,72 @@ int ufvm_server_destroy(ufvm_server *srv) {
   srv->pool.clear();
   if (srv->ctx_p) ufvm_ctx_destroy(srv->ctx_p);
   if (srv->ctx_b) ufvm_ctx_destroy(srv->ctx_b);
-  if (srv->ctx_s) ufvm_ctx_destroy(srv->ctx_s);
+  for (ufvm_ctx *c : srv->ctx_s)
+    if (c) ufvm_ctx_destroy(c);
   delete srv;
+}
+
+int ufvm_server_create(int32_t device, const ufvm_server_cfg *cfg, ufvm_server **out) {
+  if (!out || !cfg) return UFVM_ERR_ARG;
+  *out = nullptr;
+  if (cfg->max_active < 1 || cfg->n_streams < 1 || cfg->hbm_reserve < 0.0 ||
+      cfg->hbm_reserve >= 1.0 || cfg->n_steppers < 0 || cfg->n_steppers > 16)
+    return UFVM_ERR_ARG;
+  const int32_t mb = cfg->max_batch == 0 ? 8 : cfg->max_batch;
+  if (mb != 1 && mb != 2 && mb != 4 && mb != 8) return UFVM_ERR_ARG;
+  const int32_t lanes = cfg->n_steppers == 0 ? 2 : cfg->n_steppers;
+  ufvm_server *srv = new (std::nothrow) ufvm_server();
+  if (!srv) return UFVM_ERR_ARG;
+  srv->device = device;
+  srv->cfg = *cfg;
+  srv->cfg.max_batch = mb;
+  srv->cfg.n_steppers = lanes;
+  int rc = ufvm_ctx_create(device, &srv->ctx_p);
+  if (rc == UFVM_OK) rc = ufvm_ctx_create(device, &srv->ctx_b);
+  for (int i = 0; i < lanes && rc == UFVM_OK; ++i) {
+    ufvm_ctx *c = nullptr;
+    rc = ufvm_ctx_create(device, &c);
+    if (rc == UFVM_OK) srv->ctx_s.push_back(c);
+  }
+  if (rc == UFVM_OK && hipSetDevice(device) != hipSuccess) rc = UFVM_ERR_HIP;
+  if (rc == UFVM_OK) {
+    srv->streams.assign(cfg->n_streams, nullptr);
+    for (hipStream_t &s : srv->streams)
+      if (rc == UFVM_OK && hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess)
+        rc = UFVM_ERR_HIP;
+  }
+  if (rc == UFVM_OK) {
+    srv->slots.resize(cfg->max_active);
+    for (size_t i = 0; i < srv->slots.size() && rc == UFVM_OK; ++i) {
+      srv->slots[i].stream = srv->streams[i % srv->streams.size()];
+      if (hipEventCreateWithFlags(&srv->slots[i].copied, hipEventDisableTiming) != hipSuccess)
+        rc = UFVM_ERR_HIP;
+    }
+  }
+  if (rc != UFVM_OK) {
+    free_server(srv);
+    return rc;
+  }
+  for (int i = 0; i < lanes; ++i) srv->case_mu.emplace_back(new std::mutex());
+  srv->pend_step.resize(lanes);
+  srv->lane_cells.assign(lanes, 0);
+  srv->worker = std::thread(worker_main, srv);
+  for (int i = 0; i < lanes; ++i) srv->steppers.emplace_back(stepper_main, srv, i);
+  *out = srv;
+  return UFVM_OK;
+}
+
+int ufvm_server_destroy(ufvm_server *srv) {
+  if (!srv) return UFVM_ERR_ARG;
+  {
+    std::lock_guard<std::mutex> g(srv->mu);
+    srv->stop = true;
+  }
+  srv->cv_work.notify_all();
+  if (srv->worker.joinable()) srv->worker.join();
+  for (std::thread &t : srv->steppers)
+    if (t.joinable()) t.join();
+  free_server(srv);
   return UFVM_OK;
 }
 
@@ -428,24 +468,43 @@ int ufvm_server_add_system(ufvm_server *srv, int32_t n, int32_t nnz, const int32
 int ufvm_server_add_case(ufvm_server *srv, const ufvm_mesh_desc *mesh, const ufvm_simple_cfg *cfg,
                          int32_t *case_out) {
   return sguard(srv, [&] {
-    std::lock_guard<std::mutex> g(srv->case_mu);
-    check_rc(srv->ctx_s, ufvm_case_create(srv->ctx_s, mesh, cfg, case_out));
+    if (!mesh || !case_out) throw Error(UFVM_ERR_ARG, "null argument");
+    int lane = 0;
+    {
+      std::lock_guard<std::mutex> g(srv->mu);
+      for (int i = 1; i < (int)srv->lane_cells.size(); ++i)
+        if (srv->lane_cells[i] < srv->lane_cells[lane]) lane = i;
+    }
+    int32_t local = 0;
+    {
+      std::lock_guard<std::mutex> g(*srv->case_mu[lane]);
+      check_rc(srv->ctx_s[lane], ufvm_case_create(srv->ctx_s[lane], mesh, cfg, &local));
+    }
+    std::lock_guard<std::mutex> g(srv->mu);
+    srv->lane_cells[lane] += mesh->n_cells;
+    const int32_t id = srv->next_case++;
+    srv->case_of[id] = {lane, local};
+    *case_out = id;
   });
 }
 
 int ufvm_server_get_field(ufvm_server *srv, int32_t cas, const char *name, double *data,
                           int64_t count) {
   return sguard(srv, [&] {
-    std::lock_guard<std::mutex> g(srv->case_mu);
-    check_rc(srv->ctx_s, ufvm_case_get_field(srv->ctx_s, cas, name, data, count));
+    const auto lc = case_lane(srv, cas);
+    std::lock_guard<std::mutex> g(*srv->case_mu[lc.first]);
+    check_rc(srv->ctx_s[lc.first],
+             ufvm_case_get_field(srv->ctx_s[lc.first], lc.second, name, data, count));
   });
 }
 
 int ufvm_server_set_field(ufvm_server *srv, int32_t cas, const char *name, const double *data,
                           int64_t count) {
   return sguard(srv, [&] {
-    std::lock_guard<std::mutex> g(srv->case_mu);
-    check_rc(srv->ctx_s, ufvm_case_set_field(srv->ctx_s, cas, name, data, count));
+    const auto lc = case_lane(srv, cas);
+    std::lock_guard<std::mutex> g(*srv->case_mu[lc.first]);
+    check_rc(srv->ctx_s[lc.first],
+             ufvm_case_set_field(srv->ctx_s[lc.first], lc.second, name, data, count));
   });
 }
 
@@ -481,9 +540,11 @@ int ufvm_server_submit_step(ufvm_server *srv, int32_t cas, int32_t nsteps, doubl
     r->t_submit = now_s();
     std::lock_guard<std::mutex> g(srv->mu);
     if (srv->stop) throw Error(UFVM_ERR_STATE, "server stopping");
+    auto lc = srv->case_of.find(cas);
+    if (lc == srv->case_of.end()) throw Error(UFVM_ERR_ARG, "unknown case");
     r->ticket = srv->next_ticket++;
     *ticket = r->ticket;
-    srv->pend_step.push_back(r.get());
+    srv->pend_step[lc->second.first].push_back(r.get());
     srv->reqs[r->ticket] = std::move(r);
     srv->cv_work.notify_all();
   });
@@ -518,7 +579,9 @@ int ufvm_server_release(ufvm_server *srv, int64_t ticket) {
 int ufvm_server_stats(ufvm_server *srv, int32_t *queued, int32_t *active, int64_t *work_bytes) {
   return sguard(srv, [&] {
     std::lock_guard<std::mutex> g(srv->mu);
-    if (queued) *queued = (int32_t)(srv->pend_solve.size() + srv->pend_step.size());
+    size_t steps = 0;
+    for (const auto &q : srv->pend_step) steps += q.size();
+    if (queued) *queued = (int32_t)(srv->pend_solve.size() + steps);
     if (active) *active = srv->n_active;
     if (work_bytes) *work_bytes = (int64_t)srv->work_bytes_held;
   });

@@ -6,8 +6,8 @@ pressure solves (up to `max_active` slots on the GPU at once, each advanced one
 FCG iteration at a time as the GPU finishes it, spread over `n_streams`
 streams; queued solves of one system share a slot as one batch of up to
 `max_batch`, whose kernels read the matrices once for all of them) and admits
-queued requests as slots and device memory free up; a second thread runs
-SIMPLE steps on its own stream.
+queued requests as slots and device memory free up; SIMPLE steps run in
+`n_steppers` lanes, each with its own cases, stream and thread.
 
 `request_stream` and `run_stream` drive the mixed workload of the serving
 configuration: Poisson arrivals, pressure solves and single SIMPLE steps on a
@@ -30,14 +30,16 @@ STATUS = {0: "queued", 1: "running", 2: "done"}
 
 class Server:
     def __init__(self, device: int = 0, max_active: int = 8, n_streams: int = 4,
-                 hbm_reserve: float = 0.05, max_batch: int = 8):
+                 hbm_reserve: float = 0.05, max_batch: int = 8, n_steppers: int = 2):
         """max_active: solves (or batches of solves) on the GPU at once;
         max_batch: queued solves of one system admitted together as one batch
-        (1, 2, 4 or 8; each still gets the bits of its own solve)."""
+        (1, 2, 4 or 8; each still gets the bits of its own solve); n_steppers:
+        SIMPLE-step lanes (each owns some cases, a stream and a thread)."""
         self.lib = _lib.load()
         h = C.c_void_p()
         rc = self.lib.ufvm_server_create(device, C.byref(ServerCfg(max_active, n_streams,
-                                                                   hbm_reserve, max_batch)),
+                                                                   hbm_reserve, max_batch,
+                                                                   n_steppers)),
                                          C.byref(h))
         if rc != _lib.UFVM_OK:
             raise _lib.UfvmError(rc, "ufvm_server_create failed (no GPU visible?)")
