@@ -476,16 +476,18 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--workload", choices=["pcorr-k5", "laplacian", "tet"], default="pcorr-k5")
     ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--coarse-scale", type=float, default=2.0,
+    ap.add_argument("--coarse-scale", type=float, default=2.5,
                     help="factor on every coarse-grid correction (1: plain)")
-    ap.add_argument("--coarse-pre", type=int, default=-1,
+    ap.add_argument("--coarse-pre", type=int, default=2,
                     help="sweeps below the finest level before restriction (-1: --pre)")
-    ap.add_argument("--coarse-post", type=int, default=-1)
+    ap.add_argument("--coarse-post", type=int, default=2)
     ap.add_argument("--smoother", choices=["MCGS", "ILU"], default="MCGS",
                     help="finest-level smoother of the V-cycle")
     ap.add_argument("--pre", type=int, default=1)
     ap.add_argument("--post", type=int, default=1)
     ap.add_argument("--max-coarse", type=int, default=2000)
+    ap.add_argument("--sweep", default="",
+                    help="only time FP32 solves under settings scale:pre:post:cpre:cpost,...")
     ap.add_argument("--batch", type=int, default=8, choices=[1, 2, 4, 8],
                     help="also time k solves of the system at once (ufvm_pcg_solve_batch)")
     ap.add_argument("--launch-mode", choices=["auto", "eager", "graphs"], default="auto",
@@ -558,6 +560,23 @@ def main():
     level_nz = [ctx.level_info(sid, l) for l in range(nl)]
     sizes = [n for n, _ in level_nz]
     log(f"hierarchy: {nl} levels {sizes} ({t_agg:.2f}s)")
+
+    if args.sweep:
+        # FP32-preconditioned solves of the same system under several V-cycle
+        # settings ("scale:pre:post:cpre:cpost", comma separated); one JSON line each
+        for spec in args.sweep.split(","):
+            cs, pr, po, cpr, cpo = spec.split(":")
+            ctx.pcg_setup(sid, precision=32, pre_sweep=int(pr), post_sweep=int(po),
+                          max_coarse_cells=args.max_coarse, tol=args.tol, max_iter=500,
+                          coarse_smoother=args.coarse_smoother, smoother=args.smoother,
+                          coarse_scale=float(cs), coarse_pre_sweep=int(cpr),
+                          coarse_post_sweep=int(cpo))
+            total, lat, its, rr = timed_solves(ctx, sid, args.steps, args.warmup)
+            line = {"sweep": spec, "solves_per_s": round(args.steps / total, 2),
+                    "iterations": int(its[0]), "p50_ms": round(float(np.median(lat)) * 1e3, 3)}
+            log(json.dumps(line))
+            print(json.dumps(line), flush=True)
+        return
 
     results = {}
     for prec in ([32] if args.no_fp64 else [32, 64]):

@@ -164,8 +164,8 @@ typedef struct {
                                 cfdFactorizeILU.m / cfdSolveILU.m) in its lexicographic
                                 order, level-scheduled */
   double coarse_scale;       /* every coarse-grid correction is scaled by this factor
-                                (1: plain; about 2 suits the piecewise-constant
-                                prolongation of aggregation AMG); in (0, 4], 0 = 2 */
+                                (1: plain; 2-2.5 suits the piecewise-constant
+                                prolongation of aggregation AMG); in (0, 4], 0 = 2.5 */
   int32_t coarse_pre_sweep;  /* sweeps below the finest level; -1: pre_sweep */
   int32_t coarse_post_sweep; /* -1: post_sweep */
   int32_t launch_mode;       /* FCG iterations: 0 auto (replayed as captured HIP graphs below

@@ -302,13 +302,15 @@ class Context:
 
     def pcg_setup(self, sid, precision=32, pre_sweep=1, post_sweep=1, max_coarse_cells=2000,
                   nullspace=-1, max_iter=500, tol=1e-8, coarse_smoother=1, smoother="MCGS",
-                  coarse_scale=2.0, coarse_pre_sweep=-1, coarse_post_sweep=-1,
+                  coarse_scale=2.5, coarse_pre_sweep=2, coarse_post_sweep=2,
                   launch_mode="auto"):
         """smoother (finest level): "MCGS" multicolour Gauss-Seidel, "ILU" the
         reference's DILU in its lexicographic order (level-scheduled);
         coarse_scale: factor on every coarse-grid correction; coarse_*_sweep:
-        sweeps below the finest level (-1: as on the finest); launch_mode:
-        "auto" (captured graphs below 600k cells), "eager" or "graphs"."""
+        sweeps below the finest level (-1: as on the finest); the defaults
+        2.5 and 2 / 2 were tuned on the 2.1M-cell hex and 5.3M-cell tet
+        systems.  launch_mode: "auto" (captured graphs below 600k cells),
+        "eager" or "graphs"."""
         cfg = PcgCfg(precision, pre_sweep, post_sweep, coarse_smoother, max_coarse_cells,
                      nullspace, max_iter, tol, PCG_SMOOTHER[smoother], coarse_scale,
                      coarse_pre_sweep, coarse_post_sweep, LAUNCH_MODE[launch_mode])

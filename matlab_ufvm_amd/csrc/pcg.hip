@@ -514,9 +514,9 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
   if (cfg.coarse_smoother != 0 && cfg.coarse_smoother != 1)
     throw Error(UFVM_ERR_ARG, "coarse_smoother must be 0 or 1");
   if (cfg.smoother != 0 && cfg.smoother != 1) throw Error(UFVM_ERR_ARG, "smoother must be 0 or 1");
-  const double cscale = cfg.coarse_scale == 0.0 ? 2.0 : cfg.coarse_scale;   // 0: default
+  const double cscale = cfg.coarse_scale == 0.0 ? 2.5 : cfg.coarse_scale;   // 0: default
   if (!(cscale > 0.0 && cscale <= 4.0))
-    throw Error(UFVM_ERR_ARG, "coarse_scale must be in (0, 4] (0: the default 2)");
+    throw Error(UFVM_ERR_ARG, "coarse_scale must be in (0, 4] (0: the default 2.5)");
   if (cfg.launch_mode < 0 || cfg.launch_mode > 2)
     throw Error(UFVM_ERR_ARG, "launch_mode must be 0, 1 or 2");
   if (cfg.nullspace < -1 || cfg.nullspace > 1)

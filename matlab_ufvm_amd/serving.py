@@ -66,13 +66,13 @@ class Server:
     # --------------------------------------------------------- registration
     def add_system(self, ptr, col, ac, anb, precision=32, tol=1e-8, pre_sweep=1, post_sweep=1,
                    max_coarse_cells=2000, max_iter=500, coarse_smoother=1, smoother=0,
-                   coarse_scale=2.0) -> int:
+                   coarse_scale=2.5, coarse_sweeps=2) -> int:
         ptr = np.ascontiguousarray(ptr, np.int32)
         col = np.ascontiguousarray(col, np.int32)
         ac = np.ascontiguousarray(ac, np.float64)
         anb = np.ascontiguousarray(anb, np.float64)
         cfg = PcgCfg(precision, pre_sweep, post_sweep, coarse_smoother, max_coarse_cells, -1,
-                     max_iter, tol, smoother, coarse_scale, -1, -1)
+                     max_iter, tol, smoother, coarse_scale, coarse_sweeps, coarse_sweeps)
         sid = C.c_int32()
         n = len(ptr) - 1
         self.check(self.lib.ufvm_server_add_system(

@@ -188,7 +188,7 @@ class System:
         return self.H.nlevels
 
     def vcycle(self, b, pre=1, post=1, max_coarse_cells=2000, singular=0, coarse_smoother=1,
-               smoother="MCGS", coarse_scale=2.0, cpre=-1, cpost=-1):
+               smoother="MCGS", coarse_scale=2.5, cpre=2, cpost=2):
         nlev = self.pcg_levels(max_coarse_cells)
         arr, ncl = self._colour_all(nlev)
         cfg = VCfg(nlev, pre, post, singular, BLOCK_ROWS, coarse_smoother,
@@ -200,7 +200,7 @@ class System:
         return x
 
     def fpcg(self, b, tol=1e-8, maxit=500, pre=1, post=1, max_coarse_cells=2000, singular=0,
-             coarse_smoother=1, smoother="MCGS", coarse_scale=2.0, cpre=-1, cpost=-1):
+             coarse_smoother=1, smoother="MCGS", coarse_scale=2.5, cpre=2, cpost=2):
         nlev = self.pcg_levels(max_coarse_cells)
         arr, ncl = self._colour_all(nlev)
         cfg = VCfg(nlev, pre, post, singular, BLOCK_ROWS, coarse_smoother,
