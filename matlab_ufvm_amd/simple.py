@@ -91,6 +91,7 @@ class SimpleCase:
                  pcg_coarse_scale=2.5, u_smoother="ILU", pcg_coarse_sweeps=2):
         self.ctx = ctx
         self.mesh = mesh
+        self.bcs = bcs
         d, c, self._keep = case_structs(
             mesh, bcs, nu, urf_u, urf_p, n_correctors, p_ref, u_sweeps, p_solver, p_cycle,
             p_max_cycles, p_termination, p_max_coarse_levels, p_pre, p_post, p_smoother,
