@@ -1,4 +1,4 @@
-"""OpenFOAM ASCII polyMesh reader (host side).
+"""OpenFOAM ASCII polyMesh reader and writer (host side).
 
 Reads ``constant/polyMesh/{points,faces,owner,neighbour,boundary}`` into 0-based
 numpy arrays.  Behaviour follows ``src/utilities/IO/cfdReadPolyMesh.m:25-133``
@@ -131,3 +131,64 @@ def read_polymesh(case_dir: str) -> PolyMesh:
     patches = read_boundary(os.path.join(d, "boundary"))
     n_cells = int(max(own.max(), nei.max() if nei.size else -1)) + 1
     return PolyMesh(pts, ptr, nodes, own, nei, patches, n_cells)
+
+
+# ------------------------------------------------------------------- writer
+
+_BANNER = """/*--------------------------------*- C++ -*----------------------------------*\\
+  =========                 |
+  \\\\      /  F ield         | OpenFOAM: The Open Source CFD Toolbox
+   \\\\    /   O peration     |
+    \\\\  /    A nd           |
+     \\\\/     M anipulation  |
+\\*---------------------------------------------------------------------------*/"""
+
+
+def _foam_header(cls: str, obj: str, note: str = "") -> str:
+    lines = [_BANNER, "FoamFile", "{", "    version     2.0;", "    format      ascii;",
+             f"    class       {cls};"]
+    if note:
+        lines.append(f'    note        "{note}";')
+    lines += ['    location    "constant/polyMesh";', f"    object      {obj};", "}",
+              "// * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * * //",
+              ""]
+    return "\n".join(lines) + "\n"
+
+
+_FOOTER = "\n// ************************************************************************* //\n"
+
+
+def write_polymesh(pm: PolyMesh, case_dir: str) -> None:
+    """Write ``<case_dir>/constant/polyMesh/{points,faces,owner,neighbour,
+    boundary}`` in OpenFOAM ASCII (the file set read_polymesh reads; points
+    with 17 significant digits, so a write-read round trip is exact)."""
+    d = os.path.join(case_dir, "constant", "polyMesh")
+    os.makedirs(d, exist_ok=True)
+    note = (f"nPoints:{pm.points.shape[0]}  nCells:{pm.n_cells}  nFaces:{pm.n_faces}  "
+            f"nInternalFaces:{pm.n_interior_faces}")
+    with open(os.path.join(d, "points"), "w") as f:
+        f.write(_foam_header("vectorField", "points"))
+        f.write(f"\n{pm.points.shape[0]}\n(\n")
+        f.writelines("(%r %r %r)\n" % tuple(float(c) for c in p) for p in pm.points)
+        f.write(")\n" + _FOOTER)
+    with open(os.path.join(d, "faces"), "w") as f:
+        f.write(_foam_header("faceList", "faces"))
+        f.write(f"\n{pm.n_faces}\n(\n")
+        for k in range(pm.n_faces):
+            nodes = pm.face_nodes[pm.face_ptr[k]:pm.face_ptr[k + 1]]
+            f.write(f"{nodes.size}(" + " ".join(str(int(v)) for v in nodes) + ")\n")
+        f.write(")\n" + _FOOTER)
+    for name, arr in (("owner", pm.owner), ("neighbour", pm.neighbour)):
+        with open(os.path.join(d, name), "w") as f:
+            f.write(_foam_header("labelList", name, note))
+            f.write(f"\n{arr.shape[0]}\n(\n")
+            f.writelines(f"{int(v)}\n" for v in arr)
+            f.write(")\n" + _FOOTER)
+    with open(os.path.join(d, "boundary"), "w") as f:
+        f.write(_foam_header("polyBoundaryMesh", "boundary"))
+        f.write(f"\n{len(pm.patches)}\n(\n")
+        for p in pm.patches:
+            f.write(f"    {p.name}\n    {{\n        type            {p.type};\n"
+                    f"        nFaces          {p.nfaces};\n        startFace       {p.start};\n"
+                    f"    }}\n")
+        f.write(")\n" + _FOOTER)

@@ -1,5 +1,5 @@
 """OpenFOAM field files: the reader parses the reference's own time-directory
-output (fixtures: tests/golden/cylinder_0.15/{U,p,mdot_f}, written by
+output (fixtures: tests/golden/cylinder/0.15/{U,p,mdot_f}, written by
 cfdWriteOpenFoamField.m in the reference's cylinder tutorial), the writer
 reproduces those files byte for byte from the parsed values (which pins the
 layout and MATLAB's num2str formatting), and the lossless mode round-trips
@@ -11,7 +11,7 @@ import pytest
 
 from matlab_ufvm_amd import fieldio
 
-GOLD = os.path.join(os.path.dirname(__file__), "golden", "cylinder_0.15")
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "cylinder", "0.15")
 
 
 @pytest.mark.parametrize("name", ["U", "p", "mdot_f"])
@@ -98,3 +98,36 @@ def test_restart_from_time_directory_is_exact(tmp_path):
     c.o.iterate(4)
     for name in ("U", "p", "mdot_f"):
         assert np.array_equal(c.field(name), a.field(name)), name
+
+
+def test_polymesh_read_write_round_trip(tmp_path):
+    # the reference's cylinder tutorial mesh (fixture, OpenFOAM ASCII): its
+    # sizes agree with the field files above, and write -> read keeps it exactly
+    from matlab_ufvm_amd.polymesh import read_polymesh, write_polymesh
+    case = os.path.join(os.path.dirname(__file__), "golden", "cylinder")
+    pm = read_polymesh(case)
+    U = fieldio.read_field(os.path.join(GOLD, "U"))
+    assert pm.n_cells == U.internal.shape[0] == 479
+    assert pm.n_interior_faces == 906
+    assert [p.name for p in pm.patches] == list(U.boundary)
+    write_polymesh(pm, str(tmp_path))
+    q = read_polymesh(str(tmp_path))
+    for a in ("points", "face_ptr", "face_nodes", "owner", "neighbour"):
+        assert np.array_equal(getattr(pm, a), getattr(q, a)), a
+    assert [(p.name, p.type, p.start, p.nfaces) for p in pm.patches] == \
+           [(p.name, p.type, p.start, p.nfaces) for p in q.patches]
+    assert q.n_cells == pm.n_cells
+
+
+def test_cylinder_cells_are_closed():
+    # geometry of a real tutorial mesh: every cell's outward face area
+    # vectors sum to zero, and volumes are positive
+    from matlab_ufvm_amd.mesh import process_mesh
+    from matlab_ufvm_amd.polymesh import read_polymesh
+    m = process_mesh(read_polymesh(os.path.join(os.path.dirname(__file__), "golden", "cylinder")),
+                     apex="own")
+    net = np.zeros((m.n_cells, 3))
+    np.add.at(net, m.owner, m.Sf)
+    np.add.at(net, m.neighbour, -m.Sf[:m.n_interior])
+    assert np.abs(net).max() <= 1e-12 * np.abs(m.Sf).max()
+    assert np.all(m.volume > 0)
