@@ -71,12 +71,26 @@ double residual_l1(ufvm_ctx *ctx, RefLevel &L, double *out_r) {
   return ctx->hscal[0];
 }
 
+// A level-scheduled sweep runs in one workgroup (a barrier between levels)
+// when no level holds more rows than this: two passes of the workgroup per
+// level at most, cheaper than a launch per level.
+constexpr int32_t kOneWgWidest = 2 * k::kSweepBlock;
+
+static int32_t nlev(const Schedule &S) { return (int32_t)S.level_ptr.size() - 1; }
+
 void ensure_schedule(ufvm_ctx *ctx, RefLevel &L) {
   if (L.sched) return;
   lex_schedule(L.g, true, L.fwd);
   lex_schedule(L.g, false, L.bwd);
   L.fwd_rows.upload(L.fwd.rows, ctx->stream);
   L.bwd_rows.upload(L.bwd.rows, ctx->stream);
+  L.fwd_lp.upload(L.fwd.level_ptr, ctx->stream);
+  L.bwd_lp.upload(L.bwd.level_ptr, ctx->stream);
+  int32_t widest = 0;
+  for (const Schedule *S : {&L.fwd, &L.bwd})
+    for (size_t q = 0; q + 1 < S->level_ptr.size(); ++q)
+      widest = std::max(widest, S->level_ptr[q + 1] - S->level_ptr[q]);
+  L.one_wg = widest <= kOneWgWidest;
   L.low_ok = lower_lists(L.g, L.low);
   L.lptr.upload(L.low.ptr, ctx->stream);
   L.lnb.upload(L.low.nb, ctx->stream);
@@ -86,6 +100,13 @@ void ensure_schedule(ufvm_ctx *ctx, RefLevel &L) {
 }
 
 void sor_sweep(ufvm_ctx *ctx, RefLevel &L) {
+  if (L.one_wg) {
+    k::gs_sweep_block(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.ptr.p, L.col.p,
+                      L.ac.p, L.anb.p, L.bc.p, L.dphi.p);
+    k::gs_sweep_block(ctx->stream, L.bwd_rows.p, L.bwd_lp.p, nlev(L.bwd), L.ptr.p, L.col.p,
+                      L.ac.p, L.anb.p, L.bc.p, L.dphi.p);
+    return;
+  }
   const auto &F = L.fwd.level_ptr;
   for (size_t l = 0; l + 1 < F.size(); ++l)
     k::gs_rows(ctx->stream, L.fwd_rows.p + F[l], F[l + 1] - F[l], L.ptr.p, L.col.p, L.ac.p,
@@ -125,6 +146,11 @@ void dilu_factor(ufvm_ctx *ctx, RefLevel &L) {
   if (!L.low_ok) throw Error(UFVM_ERR_NUMERIC, "ILU: the index for i in j is not found");
   const auto &F = L.fwd.level_ptr;
   const int32_t skip = L.n - 2;  // cfdFactorizeILU.m:21 skips 1-based row N-1
+  if (L.one_wg) {
+    k::dilu_factor_block(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.ac.p, L.anb.p,
+                         L.lptr.p, L.lnb.p, L.lpji.p, L.lpij.p, skip, L.dinv.p);
+    return;
+  }
   for (size_t l = 0; l + 1 < F.size(); ++l)
     k::dilu_factor_rows(ctx->stream, L.fwd_rows.p + F[l], F[l + 1] - F[l], L.ac.p, L.anb.p,
                         L.lptr.p, L.lnb.p, L.lpji.p, L.lpij.p, skip, L.dinv.p);
@@ -134,6 +160,13 @@ void dilu_sweep(ufvm_ctx *ctx, RefLevel &L) {
   ensure_partial(ctx, L.n, 1);
   k::residual_ref(ctx->stream, L.n, L.ptr.p, L.col.p, L.ac.p, L.anb.p, L.bc.p, L.dphi.p, L.rc.p,
                   ctx->partial.p);
+  if (L.one_wg) {
+    k::dilu_fwd_block<double>(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.anb.p,
+                              L.lptr.p, L.lnb.p, L.lpji.p, L.dinv.p, L.rc.p);
+    k::dilu_bwd_block<double>(ctx->stream, L.bwd_rows.p, L.bwd_lp.p, nlev(L.bwd), L.n, L.ptr.p,
+                              L.col.p, L.anb.p, L.dinv.p, L.rc.p, L.dphi.p);
+    return;
+  }
   const auto &F = L.fwd.level_ptr;
   for (size_t l = 0; l + 1 < F.size(); ++l)
     k::dilu_fwd_rows(ctx->stream, L.fwd_rows.p + F[l], F[l + 1] - F[l], L.anb.p, L.lptr.p,

@@ -11,6 +11,7 @@ namespace k {
 constexpr int kBlock = 256;
 constexpr int kFoldBlocks = 512;   // grid cap of the grid-stride FCG kernels
 constexpr int kFcgBlock = 1024;    // threads per block of the FCG kernels
+constexpr int kSweepBlock = 1024;  // threads of the one-workgroup level-scheduled sweeps
 inline int nblocks(int64_t n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
 // ---- reference-order FP64 kernels (bit-exact with the reference arithmetic)
@@ -24,6 +25,24 @@ void reduce_final(hipStream_t s, const double *partial, int32_t np, double *out)
 void gs_rows(hipStream_t s, const int32_t *rows, int32_t count, const int32_t *ptr,
              const int32_t *col, const double *ac, const double *anb, const double *b,
              double *x);
+// A whole level-scheduled sweep in one workgroup (levels lp[q] .. lp[q+1]
+// of rows, device arrays, a barrier between levels): for schedules of many
+// small levels, one launch instead of one per level, same arithmetic
+void gs_sweep_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
+                    const int32_t *ptr, const int32_t *col, const double *ac, const double *anb,
+                    const double *b, double *x);
+void dilu_factor_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
+                       const double *ac, const double *anb, const int32_t *lptr,
+                       const int32_t *lnb, const int32_t *lpji, const int32_t *lpij,
+                       int32_t skip_row, double *dinv);
+template <typename V>
+void dilu_fwd_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
+                    const V *anb, const int32_t *lptr, const int32_t *lnb, const int32_t *lpji,
+                    const V *dinv, V *rc);
+template <typename V>
+void dilu_bwd_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
+                    int32_t n, const int32_t *ptr, const int32_t *col, const V *anb,
+                    const V *dinv, V *rc, V *x);
 // DILU factorisation of the listed rows (pull form of cfdFactorizeILU)
 void dilu_factor_rows(hipStream_t s, const int32_t *rows, int32_t count, const double *ac,
                       const double *anb, const int32_t *lptr, const int32_t *lnb,

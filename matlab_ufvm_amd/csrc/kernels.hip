@@ -71,16 +71,47 @@ void reduce_groups(hipStream_t s, const double *partial, int32_t nblk, int32_t w
                      out);
 }
 
+__device__ __forceinline__ void gs_row(int32_t i, const int32_t *__restrict__ ptr,
+                                       const int32_t *__restrict__ col,
+                                       const double *__restrict__ ac,
+                                       const double *__restrict__ anb, const double *b,
+                                       double *x) {
+  double v = b[i];
+  for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) v = v - anb[k] * x[col[k]];
+  x[i] = v / ac[i];
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_gs_rows(const int32_t *__restrict__ rows, int32_t count, const int32_t *__restrict__ ptr,
           const int32_t *__restrict__ col, const double *__restrict__ ac,
           const double *__restrict__ anb, const double *__restrict__ b, double *x) {
   const int32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= count) return;
-  const int32_t i = rows[t];
-  double v = b[i];
-  for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) v = v - anb[k] * x[col[k]];
-  x[i] = v / ac[i];
+  gs_row(rows[t], ptr, col, ac, anb, b, x);
+}
+
+// A whole level-scheduled sweep in one workgroup: the rows of each level in
+// parallel, a workgroup barrier between levels (levels lp[q] .. lp[q+1] of
+// rows).  For schedules of many small levels this replaces a launch per
+// level; every row is computed exactly as by the per-level kernel.
+__global__ void __launch_bounds__(kSweepBlock)
+k_gs_sweep_block(const int32_t *__restrict__ rows, const int32_t *__restrict__ lp, int32_t nlev,
+                 const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+                 const double *__restrict__ ac, const double *__restrict__ anb, const double *b,
+                 double *x) {
+  for (int32_t q = 0; q < nlev; ++q) {
+    for (int32_t t = lp[q] + threadIdx.x; t < lp[q + 1]; t += kSweepBlock)
+      gs_row(rows[t], ptr, col, ac, anb, b, x);
+    __syncthreads();
+  }
+}
+
+void gs_sweep_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
+                    const int32_t *ptr, const int32_t *col, const double *ac, const double *anb,
+                    const double *b, double *x) {
+  if (nlev <= 0) return;
+  hipLaunchKernelGGL(k_gs_sweep_block, dim3(1), dim3(kSweepBlock), 0, s, rows, lp, nlev, ptr,
+                     col, ac, anb, b, x);
 }
 
 void gs_rows(hipStream_t s, const int32_t *rows, int32_t count, const int32_t *ptr,
@@ -92,14 +123,13 @@ void gs_rows(hipStream_t s, const int32_t *rows, int32_t count, const int32_t *p
 }
 
 // d_j = ac_j - sum_{i<j, increasing i, i != skip} (a_ji * dinv_i) * a_ij ; dinv_j = 1/d_j
-__global__ void __launch_bounds__(kBlock)
-k_dilu_factor(const int32_t *__restrict__ rows, int32_t count, const double *__restrict__ ac,
-              const double *__restrict__ anb, const int32_t *__restrict__ lptr,
-              const int32_t *__restrict__ lnb, const int32_t *__restrict__ lpji,
-              const int32_t *__restrict__ lpij, int32_t skip_row, double *dinv) {
-  const int32_t t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= count) return;
-  const int32_t j = rows[t];
+__device__ __forceinline__ void dilu_factor_row(int32_t j, const double *__restrict__ ac,
+                                                const double *__restrict__ anb,
+                                                const int32_t *__restrict__ lptr,
+                                                const int32_t *__restrict__ lnb,
+                                                const int32_t *__restrict__ lpji,
+                                                const int32_t *__restrict__ lpij,
+                                                int32_t skip_row, double *dinv) {
   double d = ac[j];
   for (int32_t e = lptr[j]; e < lptr[j + 1]; ++e) {
     const int32_t i = lnb[e];
@@ -107,6 +137,38 @@ k_dilu_factor(const int32_t *__restrict__ rows, int32_t count, const double *__r
     d = d - (anb[lpji[e]] * dinv[i]) * anb[lpij[e]];
   }
   dinv[j] = 1.0 / d;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_dilu_factor(const int32_t *__restrict__ rows, int32_t count, const double *__restrict__ ac,
+              const double *__restrict__ anb, const int32_t *__restrict__ lptr,
+              const int32_t *__restrict__ lnb, const int32_t *__restrict__ lpji,
+              const int32_t *__restrict__ lpij, int32_t skip_row, double *dinv) {
+  const int32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= count) return;
+  dilu_factor_row(rows[t], ac, anb, lptr, lnb, lpji, lpij, skip_row, dinv);
+}
+
+__global__ void __launch_bounds__(kSweepBlock)
+k_dilu_factor_block(const int32_t *__restrict__ rows, const int32_t *__restrict__ lp,
+                    int32_t nlev, const double *__restrict__ ac, const double *__restrict__ anb,
+                    const int32_t *__restrict__ lptr, const int32_t *__restrict__ lnb,
+                    const int32_t *__restrict__ lpji, const int32_t *__restrict__ lpij,
+                    int32_t skip_row, double *dinv) {
+  for (int32_t q = 0; q < nlev; ++q) {
+    for (int32_t t = lp[q] + threadIdx.x; t < lp[q + 1]; t += kSweepBlock)
+      dilu_factor_row(rows[t], ac, anb, lptr, lnb, lpji, lpij, skip_row, dinv);
+    __syncthreads();
+  }
+}
+
+void dilu_factor_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
+                       const double *ac, const double *anb, const int32_t *lptr,
+                       const int32_t *lnb, const int32_t *lpji, const int32_t *lpij,
+                       int32_t skip_row, double *dinv) {
+  if (nlev <= 0) return;
+  hipLaunchKernelGGL(k_dilu_factor_block, dim3(1), dim3(kSweepBlock), 0, s, rows, lp, nlev, ac,
+                     anb, lptr, lnb, lpji, lpij, skip_row, dinv);
 }
 
 void dilu_factor_rows(hipStream_t s, const int32_t *rows, int32_t count, const double *ac,
@@ -119,13 +181,11 @@ void dilu_factor_rows(hipStream_t s, const int32_t *rows, int32_t count, const d
 
 // rc_j -= a_ji * (dinv_i * rc_i) for lower i in increasing order
 template <typename V>
-__global__ void __launch_bounds__(kBlock)
-k_dilu_fwd(const int32_t *__restrict__ rows, int32_t count, const V *__restrict__ anb,
-           const int32_t *__restrict__ lptr, const int32_t *__restrict__ lnb,
-           const int32_t *__restrict__ lpji, const V *__restrict__ dinv, V *rc) {
-  const int32_t t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= count) return;
-  const int32_t j = rows[t];
+__device__ __forceinline__ void dilu_fwd_row(int32_t j, const V *__restrict__ anb,
+                                             const int32_t *__restrict__ lptr,
+                                             const int32_t *__restrict__ lnb,
+                                             const int32_t *__restrict__ lpji,
+                                             const V *__restrict__ dinv, V *rc) {
   V v = rc[j];
   for (int32_t e = lptr[j]; e < lptr[j + 1]; ++e) {
     const int32_t i = lnb[e];
@@ -133,6 +193,38 @@ k_dilu_fwd(const int32_t *__restrict__ rows, int32_t count, const V *__restrict_
     v = v - anb[lpji[e]] * mat1;
   }
   rc[j] = v;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kBlock)
+k_dilu_fwd(const int32_t *__restrict__ rows, int32_t count, const V *__restrict__ anb,
+           const int32_t *__restrict__ lptr, const int32_t *__restrict__ lnb,
+           const int32_t *__restrict__ lpji, const V *__restrict__ dinv, V *rc) {
+  const int32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= count) return;
+  dilu_fwd_row<V>(rows[t], anb, lptr, lnb, lpji, dinv, rc);
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kSweepBlock)
+k_dilu_fwd_block(const int32_t *__restrict__ rows, const int32_t *__restrict__ lp, int32_t nlev,
+                 const V *__restrict__ anb, const int32_t *__restrict__ lptr,
+                 const int32_t *__restrict__ lnb, const int32_t *__restrict__ lpji,
+                 const V *__restrict__ dinv, V *rc) {
+  for (int32_t q = 0; q < nlev; ++q) {
+    for (int32_t t = lp[q] + threadIdx.x; t < lp[q + 1]; t += kSweepBlock)
+      dilu_fwd_row<V>(rows[t], anb, lptr, lnb, lpji, dinv, rc);
+    __syncthreads();
+  }
+}
+
+template <typename V>
+void dilu_fwd_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
+                    const V *anb, const int32_t *lptr, const int32_t *lnb, const int32_t *lpji,
+                    const V *dinv, V *rc) {
+  if (nlev <= 0) return;
+  hipLaunchKernelGGL(k_dilu_fwd_block<V>, dim3(1), dim3(kSweepBlock), 0, s, rows, lp, nlev, anb,
+                     lptr, lnb, lpji, dinv, rc);
 }
 
 template <typename V>
@@ -146,13 +238,11 @@ void dilu_fwd_rows(hipStream_t s, const int32_t *rows, int32_t count, const V *a
 
 // rc_i -= a_ij rc_j (j > i, cconn order) ; z = dinv_i rc_i ; rc_i = z ; x_i += z
 template <typename V>
-__global__ void __launch_bounds__(kBlock)
-k_dilu_bwd(const int32_t *__restrict__ rows, int32_t count, int32_t n,
-           const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
-           const V *__restrict__ anb, const V *__restrict__ dinv, V *rc, V *x) {
-  const int32_t t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= count) return;
-  const int32_t i = rows[t];
+__device__ __forceinline__ void dilu_bwd_row(int32_t i, int32_t n,
+                                             const int32_t *__restrict__ ptr,
+                                             const int32_t *__restrict__ col,
+                                             const V *__restrict__ anb,
+                                             const V *__restrict__ dinv, V *rc, V *x) {
   V v = rc[i];
   if (i < n - 1) {
     for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
@@ -166,6 +256,37 @@ k_dilu_bwd(const int32_t *__restrict__ rows, int32_t count, int32_t n,
 }
 
 template <typename V>
+__global__ void __launch_bounds__(kBlock)
+k_dilu_bwd(const int32_t *__restrict__ rows, int32_t count, int32_t n,
+           const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+           const V *__restrict__ anb, const V *__restrict__ dinv, V *rc, V *x) {
+  const int32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= count) return;
+  dilu_bwd_row<V>(rows[t], n, ptr, col, anb, dinv, rc, x);
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kSweepBlock)
+k_dilu_bwd_block(const int32_t *__restrict__ rows, const int32_t *__restrict__ lp, int32_t nlev,
+                 int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+                 const V *__restrict__ anb, const V *__restrict__ dinv, V *rc, V *x) {
+  for (int32_t q = 0; q < nlev; ++q) {
+    for (int32_t t = lp[q] + threadIdx.x; t < lp[q + 1]; t += kSweepBlock)
+      dilu_bwd_row<V>(rows[t], n, ptr, col, anb, dinv, rc, x);
+    __syncthreads();
+  }
+}
+
+template <typename V>
+void dilu_bwd_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
+                    int32_t n, const int32_t *ptr, const int32_t *col, const V *anb,
+                    const V *dinv, V *rc, V *x) {
+  if (nlev <= 0) return;
+  hipLaunchKernelGGL(k_dilu_bwd_block<V>, dim3(1), dim3(kSweepBlock), 0, s, rows, lp, nlev, n,
+                     ptr, col, anb, dinv, rc, x);
+}
+
+template <typename V>
 void dilu_bwd_rows(hipStream_t s, const int32_t *rows, int32_t count, int32_t n,
                    const int32_t *ptr, const int32_t *col, const V *anb, const V *dinv, V *rc,
                    V *x) {
@@ -174,6 +295,18 @@ void dilu_bwd_rows(hipStream_t s, const int32_t *rows, int32_t count, int32_t n,
                      ptr, col, anb, dinv, rc, x);
 }
 
+template void dilu_fwd_block<double>(hipStream_t, const int32_t *, const int32_t *, int32_t,
+                                     const double *, const int32_t *, const int32_t *,
+                                     const int32_t *, const double *, double *);
+template void dilu_fwd_block<float>(hipStream_t, const int32_t *, const int32_t *, int32_t,
+                                    const float *, const int32_t *, const int32_t *,
+                                    const int32_t *, const float *, float *);
+template void dilu_bwd_block<double>(hipStream_t, const int32_t *, const int32_t *, int32_t,
+                                     int32_t, const int32_t *, const int32_t *, const double *,
+                                     const double *, double *, double *);
+template void dilu_bwd_block<float>(hipStream_t, const int32_t *, const int32_t *, int32_t,
+                                    int32_t, const int32_t *, const int32_t *, const float *,
+                                    const float *, float *, float *);
 template void dilu_fwd_rows<double>(hipStream_t, const int32_t *, int32_t, const double *,
                                     const int32_t *, const int32_t *, const int32_t *,
                                     const double *, double *);

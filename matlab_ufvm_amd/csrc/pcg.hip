@@ -162,9 +162,13 @@ void refresh_level(ufvm_ctx *ctx, System &S, int l) {
     // the reference's DILU factor (cfdFactorizeILU.m:10-47, row N-1 quirk
     // kept) in FP64 over the lexicographic schedule, stored in both precisions
     const auto &F = R.fwd.level_ptr;
-    for (size_t q = 0; q + 1 < F.size(); ++q)
-      k::dilu_factor_rows(s, R.fwd_rows.p + F[q], F[q + 1] - F[q], R.ac.p, R.anb.p, R.lptr.p,
-                          R.lnb.p, R.lpji.p, R.lpij.p, R.n - 2, R.dinv.p);
+    if (R.one_wg)
+      k::dilu_factor_block(s, R.fwd_rows.p, R.fwd_lp.p, (int32_t)F.size() - 1, R.ac.p, R.anb.p,
+                           R.lptr.p, R.lnb.p, R.lpji.p, R.lpij.p, R.n - 2, R.dinv.p);
+    else
+      for (size_t q = 0; q + 1 < F.size(); ++q)
+        k::dilu_factor_rows(s, R.fwd_rows.p + F[q], F[q + 1] - F[q], R.ac.p, R.anb.p, R.lptr.p,
+                            R.lnb.p, R.lpji.p, R.lpij.p, R.n - 2, R.dinv.p);
     k::convert<double, double>(s, n, R.dinv.p, FL.dinv64.p);
     k::convert<double, float>(s, n, R.dinv.p, FL.dinv32.p);
   }
@@ -295,10 +299,17 @@ struct Cycle {
       k::residual_sell<V>(s, L.n, 0, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW), x, rc);
     }
     const auto &F = R.fwd.level_ptr;
+    const auto &B = R.bwd.level_ptr;
+    if (R.one_wg) {
+      k::dilu_fwd_block<V>(s, R.fwd_rows.p, R.fwd_lp.p, (int32_t)F.size() - 1, P::anb(L),
+                           R.lptr.p, R.lnb.p, R.lpji.p, P::dinv(L), rc);
+      k::dilu_bwd_block<V>(s, R.bwd_rows.p, R.bwd_lp.p, (int32_t)B.size() - 1, L.n, L.ptr.p,
+                           L.col.p, P::anb(L), P::dinv(L), rc, x);
+      return;
+    }
     for (size_t q = 0; q + 1 < F.size(); ++q)
       k::dilu_fwd_rows<V>(s, R.fwd_rows.p + F[q], F[q + 1] - F[q], P::anb(L), R.lptr.p, R.lnb.p,
                           R.lpji.p, P::dinv(L), rc);
-    const auto &B = R.bwd.level_ptr;
     for (size_t q = 0; q + 1 < B.size(); ++q)
       k::dilu_bwd_rows<V>(s, R.bwd_rows.p + B[q], B[q + 1] - B[q], L.n, L.ptr.p, L.col.p,
                           P::anb(L), P::dinv(L), rc, x);

@@ -102,6 +102,11 @@ struct RefLevel {
   bool sched = false;
   Schedule fwd, bwd;
   DBuf<int32_t> fwd_rows, bwd_rows;
+  // the schedules' level pointers on the device, and whether every level is
+  // small enough that a whole sweep runs in one workgroup (one launch
+  // instead of one per level)
+  DBuf<int32_t> fwd_lp, bwd_lp;
+  bool one_wg = false;
   bool low_ok = false;
   LowerLists low;
   DBuf<int32_t> lptr, lnb, lpji, lpij;
