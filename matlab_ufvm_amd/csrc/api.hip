@@ -218,13 +218,18 @@ int32_t agglomerate(ufvm_ctx *ctx, System &S, int32_t max_coarse) {
     F.parents.upload(ag.parents, s);
     F.child_ptr.upload(ag.child_ptr, s);
     F.child.upload(ag.child, s);
-    F.slot.upload(ag.entry_slot.empty() ? std::vector<int32_t>{-1} : ag.entry_slot, s);
+    {
+      std::vector<int32_t> gptr, gsrc;
+      galerkin_lists(F.g, ag, gptr, gsrc);
+      F.gal_out = (int32_t)gptr.size() - 1;
+      F.gal_ptr.upload(gptr, s);
+      F.gal_src.upload(gsrc, s);
+    }
     if ((int32_t)S.lev.size() < iLevel) S.lev.emplace_back(new RefLevel());
     RefLevel &C = *S.lev[iLevel - 1];
     C.g = std::move(ag.coarse);
     upload_pattern(C, s);
-    k::coarse_lhs(s, C.n, F.child_ptr.p, F.child.p, F.ptr.p, F.ac.p, F.anb.p, F.slot.p, C.ptr.p,
-                  C.ac.p, C.anb.p);
+    k::coarse_lhs(s, F.gal_out, C.n, F.gal_ptr.p, F.gal_src.p, F.ac.p, F.anb.p, C.ac.p, C.anb.p);
     UFVM_HIP(hipMemsetAsync(C.bc.p, 0, C.n * sizeof(double), s));
     UFVM_HIP(hipMemsetAsync(C.dphi.p, 0, C.n * sizeof(double), s));
     S.nlevels = iLevel;
@@ -368,8 +373,8 @@ void galerkin_update(ufvm_ctx *ctx, System &S) {
   for (int32_t l = 0; l + 1 < S.nlevels; ++l) {
     RefLevel &F = *S.lev[l];
     RefLevel &C = *S.lev[l + 1];
-    k::coarse_lhs(ctx->stream, C.n, F.child_ptr.p, F.child.p, F.ptr.p, F.ac.p, F.anb.p, F.slot.p,
-                  C.ptr.p, C.ac.p, C.anb.p);
+    k::coarse_lhs(ctx->stream, F.gal_out, C.n, F.gal_ptr.p, F.gal_src.p, F.ac.p, F.anb.p, C.ac.p,
+                  C.anb.p);
   }
   UFVM_HIP(hipStreamSynchronize(ctx->stream));
   S.hier_valid = true;

@@ -109,6 +109,31 @@ void agglomerate_level(const HostCSR &g, const double *anb, AggResult &out) {
   }
 }
 
+void galerkin_lists(const HostCSR &g, const AggResult &ag, std::vector<int32_t> &ptr,
+                    std::vector<int32_t> &src) {
+  const int32_t nc = ag.nc;
+  const size_t nout = (size_t)nc + ag.coarse.col.size();
+  ptr.assign(nout + 1, 0);
+  for (int32_t i = 0; i < g.n; ++i) ++ptr[ag.parents[i] + 1];
+  for (int32_t i = 0; i < g.n; ++i)
+    for (int32_t k = g.ptr[i]; k < g.ptr[i + 1]; ++k) {
+      const int32_t sl = ag.entry_slot[k];
+      ++ptr[(sl < 0 ? ag.parents[i] : nc + sl) + 1];
+    }
+  for (size_t o = 0; o < nout; ++o) ptr[o + 1] += ptr[o];
+  src.assign(std::max<size_t>(ptr[nout], 1), 0);
+  std::vector<int32_t> at(ptr.begin(), ptr.end() - 1);
+  for (int32_t p = 0; p < nc; ++p)
+    for (int32_t c = ag.child_ptr[p]; c < ag.child_ptr[p + 1]; ++c) {
+      const int32_t i = ag.child[c];
+      src[at[p]++] = -1 - i;
+      for (int32_t k = g.ptr[i]; k < g.ptr[i + 1]; ++k) {
+        const int32_t sl = ag.entry_slot[k];
+        src[at[sl < 0 ? p : nc + sl]++] = k;
+      }
+    }
+}
+
 int32_t greedy_colour(const HostCSR &g, std::vector<int32_t> &colour) {
   const int32_t n = g.n;
   colour.assign(n, -1);

@@ -25,6 +25,14 @@ struct AggResult {
 // cfdAgglomerateLevel.m:3-104 on one level (anb values in cconn order).
 void agglomerate_level(const HostCSR &g, const double *anb, AggResult &out);
 
+// The Galerkin coarse operator as gather lists, one per coarse value in
+// the reference's accumulation order (assembleAgglomeratedLHS.m:16-30: fine
+// rows by parent, children in increasing order, entries in cconn order).
+// Value o < nc is AC[o], value nc + t is ANB slot t.  Sources: e >= 0 is
+// fine anb[e], e < 0 is fine ac[-1 - e].
+void galerkin_lists(const HostCSR &g, const AggResult &ag, std::vector<int32_t> &ptr,
+                    std::vector<int32_t> &src);
+
 // greedy first-fit colouring in index order; returns number of colours
 int32_t greedy_colour(const HostCSR &g, std::vector<int32_t> &colour);
 // the same restricted to each block of `rows_per_block` consecutive rows

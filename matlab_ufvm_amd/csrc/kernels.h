@@ -62,10 +62,10 @@ void restrict_ref(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t 
                   const double *r, double *bc_c);
 // x[i] += xc[parent[i]]
 void prolong_ref(hipStream_t s, int32_t n, const int32_t *parent, const double *xc, double *x);
-// Galerkin coarse operator in the reference accumulation order
-void coarse_lhs(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *child,
-                const int32_t *fptr, const double *fac, const double *fanb, const int32_t *slot,
-                const int32_t *cptr_c, double *cac, double *canb);
+// Galerkin coarse operator in the reference accumulation order, from the
+// gather lists of galerkin_lists (nout = nc + coarse nnz values)
+void coarse_lhs(hipStream_t s, int32_t nout, int32_t nc, const int32_t *ptr, const int32_t *src,
+                const double *fac, const double *fanb, double *cac, double *canb);
 
 // ---- multicolour / precision-form kernels (V = float or double)
 // sliced-ELL (64-row chunks, column-major inside a chunk) variants of the

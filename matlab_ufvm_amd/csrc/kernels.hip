@@ -370,38 +370,44 @@ void prolong_fast(hipStream_t s, int32_t r0, int32_t n, const int32_t *parent, c
                      xc, x);
 }
 
-// One thread per coarse row p.  For its children i (increasing fine index):
-// AC += ac_i; then per entry k of row i (cconn order): slot < 0 (same parent)
-// -> AC += anb_k, else ANB[slot] += anb_k.  Row p's ANB entries are written by
-// this thread only, in the reference's accumulation order.
+// Galerkin coarse operator, one thread per coarse value (the nc diagonal
+// values, then the off-diagonal slots): the value is 0 plus its sources in
+// list order, which is the reference's accumulation order for that value.
+// Sources e >= 0 are fine anb[e], e < 0 fine ac[-1 - e].  Four sources are
+// loaded ahead of their sums.
 __global__ void __launch_bounds__(kBlock)
-k_coarse_lhs(int32_t nc, const int32_t *__restrict__ cptr, const int32_t *__restrict__ child,
-             const int32_t *__restrict__ fptr, const double *__restrict__ fac,
-             const double *__restrict__ fanb, const int32_t *__restrict__ slot,
-             const int32_t *__restrict__ cptr_c, double *__restrict__ cac,
+k_coarse_lhs(int32_t nout, int32_t nc, const int32_t *__restrict__ ptr,
+             const int32_t *__restrict__ src, const double *__restrict__ fac,
+             const double *__restrict__ fanb, double *__restrict__ cac,
              double *__restrict__ canb) {
-  const int32_t p = blockIdx.x * kBlock + threadIdx.x;
-  if (p >= nc) return;
-  for (int32_t t = cptr_c[p]; t < cptr_c[p + 1]; ++t) canb[t] = 0.0;
-  double a = 0.0;
-  for (int32_t c = cptr[p]; c < cptr[p + 1]; ++c) {
-    const int32_t i = child[c];
-    a = a + fac[i];
-    for (int32_t k = fptr[i]; k < fptr[i + 1]; ++k) {
-      const int32_t sl = slot[k];
-      if (sl < 0) a = a + fanb[k];
-      else canb[sl] = canb[sl] + fanb[k];
+  const int32_t o = blockIdx.x * kBlock + threadIdx.x;
+  if (o >= nout) return;
+  const int32_t m0 = ptr[o], m1 = ptr[o + 1];
+  double v = 0.0;
+  int32_t m = m0;
+  for (; m + 4 <= m1; m += 4) {
+    double x[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int32_t e = src[m + u];
+      x[u] = e >= 0 ? fanb[e] : fac[-1 - e];
     }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v = v + x[u];
   }
-  cac[p] = a;
+  for (; m < m1; ++m) {
+    const int32_t e = src[m];
+    v = v + (e >= 0 ? fanb[e] : fac[-1 - e]);
+  }
+  if (o < nc) cac[o] = v;
+  else canb[o - nc] = v;
 }
 
-void coarse_lhs(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *child,
-                const int32_t *fptr, const double *fac, const double *fanb, const int32_t *slot,
-                const int32_t *cptr_c, double *cac, double *canb) {
-  if (nc <= 0) return;
-  hipLaunchKernelGGL(k_coarse_lhs, dim3(nblocks(nc)), dim3(kBlock), 0, s, nc, cptr, child, fptr,
-                     fac, fanb, slot, cptr_c, cac, canb);
+void coarse_lhs(hipStream_t s, int32_t nout, int32_t nc, const int32_t *ptr, const int32_t *src,
+                const double *fac, const double *fanb, double *cac, double *canb) {
+  if (nout <= 0) return;
+  hipLaunchKernelGGL(k_coarse_lhs, dim3(nblocks(nout)), dim3(kBlock), 0, s, nout, nc, ptr, src,
+                     fac, fanb, cac, canb);
 }
 
 // --------------------------------------------------------- multicolour mode
@@ -828,73 +834,168 @@ void csr_to_dense(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *c
 // the rank-kGjB update replaces kGjB launches over the whole matrix.
 
 constexpr int kGjB = 64;
+constexpr int kGjT = 256;  // threads of the block-step kernels: 16 x 16, a 4 x 4 tile each
 
-// P = inverse of the b x b diagonal block at (k0, k0): one-column Gauss-Jordan
-// steps in LDS (save column k, scale row k so the pivot becomes 1/p, subtract
-// column k's multiple of row k from every other row)
-__global__ void __launch_bounds__(kBlock)
+// P = inverse of the b x b diagonal block at (k0, k0) by b one-column
+// Gauss-Jordan steps (scale row k so the pivot becomes 1/p, subtract column
+// k's multiple of row k from every other row).  The block, padded to 64 x 64
+// with the identity (which the steps leave unchanged), lives in registers as
+// 4 x 4 tiles; step k's raw row and column go through double-buffered LDS, so
+// each step needs one barrier.
+__global__ void __launch_bounds__(kGjT)
 k_gjb_diag(int32_t n, int32_t k0, int32_t b, const double *__restrict__ a,
            double *__restrict__ pbuf) {
-  __shared__ double t[kGjB][kGjB + 1];
-  __shared__ double colv[kGjB];
-  for (int32_t e = threadIdx.x; e < b * b; e += kBlock)
-    t[e / b][e % b] = a[(int64_t)(k0 + e / b) * n + k0 + e % b];
-  __syncthreads();
+  __shared__ double rowb[2][kGjB], colb[2][kGjB];
+  const int32_t ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+  double t[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int32_t i = ty * 4 + r, j = tx * 4 + c;
+      t[r][c] = (i < b && j < b) ? a[(int64_t)(k0 + i) * n + k0 + j] : (i == j ? 1.0 : 0.0);
+    }
   for (int32_t k = 0; k < b; ++k) {
-    const double inv = 1.0 / t[k][k];
-    for (int32_t i = threadIdx.x; i < b; i += kBlock) colv[i] = t[i][k];
-    __syncthreads();
-    for (int32_t j = threadIdx.x; j < b; j += kBlock) t[k][j] = ((j == k) ? 1.0 : t[k][j]) * inv;
-    __syncthreads();
-    for (int32_t e = threadIdx.x; e < b * b; e += kBlock) {
-      const int32_t i = e / b, j = e % b;
-      if (i != k) t[i][j] = ((j == k) ? 0.0 : t[i][j]) - colv[i] * t[k][j];
+    const int q = k & 1;
+    if (ty == (k >> 2)) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        if (r == (k & 3))
+#pragma unroll
+          for (int c = 0; c < 4; ++c) rowb[q][tx * 4 + c] = t[r][c];
+    }
+    if (tx == (k >> 2)) {
+#pragma unroll
+      for (int c = 0; c < 4; ++c)
+        if (c == (k & 3))
+#pragma unroll
+          for (int r = 0; r < 4; ++r) colb[q][ty * 4 + r] = t[r][c];
     }
     __syncthreads();
+    const double inv = 1.0 / rowb[q][k];
+    double rk[4], ck[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int32_t j = tx * 4 + c;
+      rk[c] = ((j == k) ? 1.0 : rowb[q][j]) * inv;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ck[r] = colb[q][ty * 4 + r];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int32_t i = ty * 4 + r;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const int32_t j = tx * 4 + c;
+        t[r][c] = (i == k) ? rk[c] : ((j == k) ? 0.0 : t[r][c]) - ck[r] * rk[c];
+      }
+    }
   }
-  for (int32_t e = threadIdx.x; e < b * b; e += kBlock) pbuf[e] = t[e / b][e % b];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int32_t i = ty * 4 + r, j = tx * 4 + c;
+      if (i < b && j < b) pbuf[i * b + j] = t[r][c];
+    }
 }
 
-// rbuf[s][j] = sum_r P[s][r] A[k0+r][j] (the row panel times P), and
-// cbuf[i][s] = A[i][k0+s] (the old column panel); grid (columns, b)
-__global__ void __launch_bounds__(kBlock)
+// rbuf[s][j] = sum_r P[s][r] A[k0+r][j] (the row panel times P, r in
+// order), and cbuf[j][s] = A[j][k0+s] (the old column panel): thread j of
+// grid row g handles s = 16 g .. 16 g + 15; P is staged in LDS
+__global__ void __launch_bounds__(kGjT)
 k_gjb_panels(int32_t n, int32_t k0, int32_t b, const double *__restrict__ a,
              const double *__restrict__ pbuf, double *__restrict__ rbuf,
              double *__restrict__ cbuf) {
-  const int32_t j = blockIdx.x * kBlock + threadIdx.x;
-  const int32_t s = blockIdx.y;
+  __shared__ double ps[16][kGjB];
+  const int32_t s0 = blockIdx.y * 16;
+  for (int32_t e = threadIdx.x; e < 16 * kGjB; e += kGjT) {
+    const int32_t s = s0 + e / kGjB, r = e % kGjB;
+    ps[e / kGjB][r] = (s < b && r < b) ? pbuf[s * b + r] : 0.0;
+  }
+  __syncthreads();
+  const int32_t j = blockIdx.x * kGjT + threadIdx.x;
   if (j >= n) return;
-  double v = 0.0;
-  for (int32_t r = 0; r < b; ++r) v += pbuf[s * b + r] * a[(int64_t)(k0 + r) * n + j];
-  rbuf[(int64_t)s * n + j] = v;
-  cbuf[(int64_t)j * b + s] = a[(int64_t)j * n + k0 + s];
+  double v[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) v[u] = 0.0;
+  int32_t r = 0;
+  for (; r + 8 <= b; r += 8) {  // eight loads in flight before their products
+    double x[8];
+#pragma unroll
+    for (int w = 0; w < 8; ++w) x[w] = a[(int64_t)(k0 + r + w) * n + j];
+#pragma unroll
+    for (int w = 0; w < 8; ++w)
+#pragma unroll
+      for (int u = 0; u < 16; ++u) v[u] += ps[u][r + w] * x[w];
+  }
+  for (; r < b; ++r) {
+    const double x = a[(int64_t)(k0 + r) * n + j];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) v[u] += ps[u][r] * x;
+  }
+#pragma unroll
+  for (int u = 0; u < 16; ++u)
+    if (s0 + u < b) {
+      rbuf[(int64_t)(s0 + u) * n + j] = v[u];
+      cbuf[(int64_t)j * b + s0 + u] = a[(int64_t)j * n + k0 + s0 + u];
+    }
 }
 
-// every element from the panels: grid (columns, rows)
-__global__ void __launch_bounds__(kBlock)
+// every element from the panels, 64 x 64 output tiles (a 4 x 4 tile per
+// thread), both panels staged in LDS; each element sums its b terms in
+// order s = 0 .. b-1
+__global__ void __launch_bounds__(kGjT)
 k_gjb_update(int32_t n, int32_t k0, int32_t b, double *__restrict__ a,
              const double *__restrict__ pbuf, const double *__restrict__ rbuf,
              const double *__restrict__ cbuf) {
-  __shared__ double ci[kGjB];
-  const int32_t i = blockIdx.y;
-  const int32_t j = blockIdx.x * kBlock + threadIdx.x;
-  for (int32_t s = threadIdx.x; s < b; s += kBlock) ci[s] = cbuf[(int64_t)i * b + s];
-  __syncthreads();
-  if (j >= n) return;
-  const bool ik = i >= k0 && i < k0 + b, jk = j >= k0 && j < k0 + b;
-  double v;
-  if (ik && jk) {
-    v = pbuf[(i - k0) * b + (j - k0)];
-  } else if (ik) {
-    v = rbuf[(int64_t)(i - k0) * n + j];
-  } else if (jk) {
-    v = 0.0;
-    for (int32_t s = 0; s < b; ++s) v -= ci[s] * pbuf[s * b + (j - k0)];
-  } else {
-    v = a[(int64_t)i * n + j];
-    for (int32_t s = 0; s < b; ++s) v -= ci[s] * rbuf[(int64_t)s * n + j];
+  __shared__ double cs[kGjB][kGjB + 1];  // [row of tile][s]
+  __shared__ double rs[kGjB][kGjB + 1];  // [s][column of tile]
+  const int32_t i0 = blockIdx.y * kGjB, j0 = blockIdx.x * kGjB;
+  for (int32_t e = threadIdx.x; e < kGjB * kGjB; e += kGjT) {
+    const int32_t u = e / kGjB, w = e % kGjB;
+    const int32_t i = i0 + u, s = w;
+    cs[u][w] = (i < n && s < b) ? cbuf[(int64_t)i * b + s] : 0.0;
+    const int32_t j = j0 + w, s2 = u;
+    double r = 0.0;
+    if (j < n && s2 < b)
+      r = (j >= k0 && j < k0 + b) ? pbuf[s2 * b + (j - k0)] : rbuf[(int64_t)s2 * n + j];
+    rs[u][w] = r;
   }
-  a[(int64_t)i * n + j] = v;
+  __syncthreads();
+  const int32_t ty = threadIdx.x >> 4, tx = threadIdx.x & 15;
+  double v[4][4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int32_t i = i0 + ty * 4 + r, j = j0 + tx * 4 + c;
+      const bool jk = j >= k0 && j < k0 + b;
+      v[r][c] = (i < n && j < n && !jk) ? a[(int64_t)i * n + j] : 0.0;
+    }
+  for (int32_t s = 0; s < b; ++s) {
+    double ci[4], rj[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) ci[r] = cs[ty * 4 + r][s];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) rj[c] = rs[s][tx * 4 + c];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int c = 0; c < 4; ++c) v[r][c] -= ci[r] * rj[c];
+  }
+#pragma unroll
+  for (int r = 0; r < 4; ++r)
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const int32_t i = i0 + ty * 4 + r, j = j0 + tx * 4 + c;
+      if (i >= n || j >= n) continue;
+      const bool ik = i >= k0 && i < k0 + b, jk = j >= k0 && j < k0 + b;
+      double x = v[r][c];
+      if (ik && jk) x = pbuf[(i - k0) * b + (j - k0)];
+      else if (ik) x = rbuf[(int64_t)(i - k0) * n + j];
+      a[(int64_t)i * n + j] = x;
+    }
 }
 
 size_t gj_scratch(int32_t n) { return (size_t)kGjB * kGjB + 2 * (size_t)kGjB * n + 8; }
@@ -903,10 +1004,11 @@ void gj_inverse(hipStream_t s, int32_t n, double *a, double *scratch) {
   double *pbuf = scratch, *rbuf = pbuf + kGjB * kGjB, *cbuf = rbuf + (size_t)kGjB * n;
   for (int32_t k0 = 0; k0 < n; k0 += kGjB) {
     const int32_t b = n - k0 < kGjB ? n - k0 : kGjB;
-    hipLaunchKernelGGL(k_gjb_diag, dim3(1), dim3(kBlock), 0, s, n, k0, b, a, pbuf);
-    hipLaunchKernelGGL(k_gjb_panels, dim3(nblocks(n), b), dim3(kBlock), 0, s, n, k0, b, a, pbuf,
-                       rbuf, cbuf);
-    hipLaunchKernelGGL(k_gjb_update, dim3(nblocks(n), n), dim3(kBlock), 0, s, n, k0, b, a, pbuf,
+    const int32_t tiles = (n + kGjB - 1) / kGjB;
+    hipLaunchKernelGGL(k_gjb_diag, dim3(1), dim3(kGjT), 0, s, n, k0, b, a, pbuf);
+    hipLaunchKernelGGL(k_gjb_panels, dim3(nblocks(n, kGjT), (b + 15) / 16), dim3(kGjT), 0, s, n,
+                       k0, b, a, pbuf, rbuf, cbuf);
+    hipLaunchKernelGGL(k_gjb_update, dim3(tiles, tiles), dim3(kGjT), 0, s, n, k0, b, a, pbuf,
                        rbuf, cbuf);
   }
 }

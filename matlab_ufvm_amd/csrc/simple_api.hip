@@ -21,10 +21,14 @@ namespace ufvm {
 
 struct Case {
   int32_t N = 0, F = 0, Fi = 0, Fb = 0, P = 0;
-  DBuf<int32_t> owner, neighbour, cptr, ccol, own_nnz, nei_nnz, cf_ptr, cf_face, bpatch;
+  DBuf<int32_t> owner, neighbour, cptr, ccol, own_nnz, nei_nnz, cf_ptr, cf_face, cf_slot, bpatch;
   DBuf<int32_t> ptype, u_bc, p_bc;
   DBuf<double> Sf, area, CN, eCN, gDiff, T, gf, fc, vol, cc, u_val;
   DBuf<double> U, gradU, p, gradp, PP, gradPP, DU, DUT, mdot, gamma, fluxC1, fluxC2, ac_old;
+  DBuf<double> uface;  // momentum face fluxes, 6 per face
+  // 1 when the pressure-correction coefficients changed since the kept
+  // hierarchy was last refreshed from them (set by the assembly)
+  DBuf<int32_t> p_changed;
   sk::DevMesh dm{};
   ufvm_simple_cfg cfg{};
   std::vector<int32_t> h_ptype, h_u_bc, h_p_bc;
@@ -152,13 +156,13 @@ void assemble(ufvm_ctx *ctx, Case &C, int eq, int comp, double *rms, double *mx)
   if (eq == 0) {
     if (comp < 0 || comp > 2) throw Error(UFVM_ERR_ARG, "component must be 0, 1 or 2");
     sk::assemble_u(s, C.dm, comp, C.cfg.urf_u, C.U.p, C.gradU.p, C.gradp.p, C.mdot.p, C.gamma.p,
-                   L.ac.p, L.anb.p, L.bc.p, C.ac_old.p, L.dphi.p, C.DU.p, C.DUT.p, ctx->partial.p,
+                   C.uface.p, L.ac.p, L.anb.p, L.bc.p, C.ac_old.p, L.dphi.p, C.DU.p, C.DUT.p, ctx->partial.p,
                    ctx->counter.p, ctx->scal.p);
   } else {
     sk::rhie_chow(s, C.dm, C.cfg.urf_u, C.U.p, C.DU.p, C.p.p, C.gradp.p, C.mdot.p, C.fluxC1.p,
                   C.fluxC2.p);
     sk::assemble_p(s, C.dm, C.fluxC1.p, C.fluxC2.p, C.mdot.p, L.ac.p, L.anb.p, L.bc.p, C.ac_old.p,
-                   L.dphi.p, ctx->partial.p, ctx->counter.p, ctx->scal.p);
+                   L.dphi.p, ctx->partial.p, ctx->scal.p, C.p_changed.p);
   }
   // new values: the coarse operators and the solve-side copies are stale (a
   // kept hierarchy is updated from them by galerkin_update + pcg_refresh)
@@ -190,10 +194,22 @@ void solve(ufvm_ctx *ctx, Case &C, int eq, double *r0, double *r1, int32_t *nc) 
     C.p_hier_built = true;
     pcg_setup(ctx, S, C.cfg.p_pcg);
   } else {
-    // first hierarchy's parents and layouts, current coefficients
-    galerkin_update(ctx, S);
-    if (S.pcg.ready) pcg_refresh(ctx, S);
-    else pcg_setup(ctx, S, C.cfg.p_pcg);
+    // first hierarchy's parents and layouts, current coefficients.  When no
+    // coefficient's bits changed since the last refresh (the correctors of
+    // one SIMPLE iteration share DU, so their matrices are identical), the
+    // coarse operators and solve-side copies are still exact.
+    int32_t changed = 1;
+    C.p_changed.download(&changed, 1, ctx->stream);
+    UFVM_HIP(hipStreamSynchronize(ctx->stream));
+    if (!changed && S.pcg.ready && S.nlevels >= 2) {
+      S.hier_valid = true;
+      S.pcg.stale = false;
+    } else {
+      galerkin_update(ctx, S);
+      if (S.pcg.ready) pcg_refresh(ctx, S);
+      else pcg_setup(ctx, S, C.cfg.p_pcg);
+      C.p_changed.upload(std::vector<int32_t>{0}, ctx->stream);
+    }
   }
   double rel = 0.0;
   pcg_solve(ctx, S, nc, &rel);
@@ -275,9 +291,14 @@ int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *m, const ufvm_simple_c
     }
     for (int32_t i = 0; i < C.N; ++i) cf_ptr[i + 1] += cf_ptr[i];
     std::vector<int32_t> cf_face(cf_ptr[C.N]), fillp(cf_ptr.begin(), cf_ptr.end() - 1);
+    std::vector<int32_t> cf_slot(cf_ptr[C.N], 0);
     for (int32_t f = 0; f < C.F; ++f) {
+      if (f < C.Fi) cf_slot[fillp[m->owner[f]]] = m->own_nnz[f];
       cf_face[fillp[m->owner[f]]++] = f;
-      if (f < C.Fi) cf_face[fillp[m->neighbour[f]]++] = f;
+      if (f < C.Fi) {
+        cf_slot[fillp[m->neighbour[f]]] = -1 - m->nei_nnz[f];
+        cf_face[fillp[m->neighbour[f]]++] = f;
+      }
     }
     const size_t F3 = (size_t)C.F * 3;
     up(C.owner, m->owner, C.F, s, "owner");
@@ -298,6 +319,9 @@ int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *m, const ufvm_simple_c
     up(C.nei_nnz, m->nei_nnz, std::max(C.Fi, 1), s, "nei_nnz");
     C.cf_ptr.upload(cf_ptr, s);
     C.cf_face.upload(cf_face, s);
+    C.cf_slot.upload(cf_slot, s);
+    C.uface.alloc((size_t)C.F * 6);
+    C.p_changed.upload(std::vector<int32_t>{1}, s);
     C.bpatch.upload(bp.empty() ? std::vector<int32_t>{0} : bp, s);
     C.h_ptype.assign(m->patch_type, m->patch_type + C.P);
     C.h_u_bc.assign(cfg->u_bc, cfg->u_bc + C.P);
@@ -317,7 +341,8 @@ int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *m, const ufvm_simple_c
     d.Sf = C.Sf.p; d.area = C.area.p; d.CN = C.CN.p; d.eCN = C.eCN.p; d.gDiff = C.gDiff.p;
     d.T = C.T.p; d.gf = C.gf.p; d.fc = C.fc.p; d.vol = C.vol.p; d.cc = C.cc.p;
     d.cptr = C.cptr.p; d.ccol = C.ccol.p; d.own_nnz = C.own_nnz.p; d.nei_nnz = C.nei_nnz.p;
-    d.cf_ptr = C.cf_ptr.p; d.cf_face = C.cf_face.p; d.bpatch = C.bpatch.p;
+    d.cf_ptr = C.cf_ptr.p; d.cf_face = C.cf_face.p; d.cf_slot = C.cf_slot.p;
+    d.bpatch = C.bpatch.p;
     d.ptype = C.ptype.p; d.u_bc = C.u_bc.p; d.p_bc = C.p_bc.p; d.u_val = C.u_val.p;
     init_fields(ctx, C);
     C.sysU = create_system(ctx, C.N, nnz, m->cconn_ptr, m->cconn);

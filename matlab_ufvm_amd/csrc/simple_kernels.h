@@ -19,6 +19,9 @@ struct DevMesh {
   const double *Sf, *area, *CN, *eCN, *gDiff, *T, *gf, *fc, *vol, *cc;
   const int32_t *cptr, *ccol, *own_nnz, *nei_nnz;
   const int32_t *cf_ptr, *cf_face;   // faces of each cell, ascending
+  const int32_t *cf_slot;            // per cell-face entry, interior faces: the cell's anb
+                                     // slot of the face (>= 0, the cell owns the face) or
+                                     // -1 - slot (the cell is the neighbour)
   const int32_t *bpatch;             // patch of each boundary face
   const int32_t *ptype, *u_bc, *p_bc;
   const double *u_val;
@@ -34,16 +37,18 @@ void scale_u(hipStream_t s, const DevMesh &m, const double *U, double *partial,
              unsigned *counter, double *scal);
 void assemble_u(hipStream_t s, const DevMesh &m, int comp, double urf, const double *U,
                 const double *gradU, const double *gradp, const double *mdot,
-                const double *gamma, double *ac, double *anb, double *bc, double *ac_old,
-                double *dphi, double *DU, double *DUT, double *partial, unsigned *counter,
-                double *scal);
+                const double *gamma, double *uf, double *ac, double *anb, double *bc,
+                double *ac_old, double *dphi, double *DU, double *DUT, double *partial,
+                unsigned *counter, double *scal);
 void correct_u(hipStream_t s, const DevMesh &m, int comp, const double *dphi, double *U);
 void rhie_chow(hipStream_t s, const DevMesh &m, double urf_u, const double *U,
                const double *DU, const double *p, const double *gradp, double *mdot,
                double *fluxC1, double *fluxC2);
+// pressure-correction coefficients; *changed is set to 1 when an ac or anb
+// value's bits differ from the value it overwrites (never cleared here)
 void assemble_p(hipStream_t s, const DevMesh &m, const double *fluxC1, const double *fluxC2,
                 const double *mdot, double *ac, double *anb, double *bc, double *ac_old,
-                double *dphi, double *partial, unsigned *counter, double *scal);
+                double *dphi, double *partial, double *scal, int32_t *changed);
 void correct_p(hipStream_t s, const DevMesh &m, int32_t ref, double urf_p, const double *dphi,
                double *PP, double *gradPP, const double *DU, double *U, double *p,
                const double *fluxC1, const double *fluxC2, double *mdot);

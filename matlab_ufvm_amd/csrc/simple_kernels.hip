@@ -5,9 +5,11 @@
 // coefficient receives the face contributions in exactly the order of the
 // reference's face-loop scatter (uFVM/src/fvm/assemble/
 // cfdAssembleIntoGlobalMatrixFaceFluxes.m:25-55: interior faces ascending,
-// then boundary faces) with no atomics.  A face flux is evaluated by both of
-// its cells with identical arithmetic.  Built with -ffp-contract=off; every
-// expression keeps the reference's association.
+// then boundary faces) with no atomics.  Face fluxes are evaluated once per
+// face by a face kernel and gathered by both cells; cf_slot gives each
+// cell-face entry its side and anb slot, so each anb value is written once.
+// Built with -ffp-contract=off; every expression keeps the reference's
+// association.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -93,6 +95,70 @@ void update_u_bc(hipStream_t s, const DevMesh &m, double *U) {
 // faces, then boundary faces.  The boundary contribution uses the boundary
 // value of component 1 for every component (linear indexing phi_b(k) into
 // the [Fb x nc] block, :59-64).
+
+// one face of a cell's Green-Gauss sum: face f, side slot sl, Sf, weight gf
+// and the owner / neighbour values (boundary faces: po[0] = phi_b's first
+// component)
+template <int NC>
+struct GEntry {
+  int32_t f, sl;
+  double S[3], w, po[NC], pn[NC];
+};
+
+// the face's contribution to g: phi_f S_f added (owner) or subtracted
+// (neighbour); boundary faces phi_b S_b
+template <int NC>
+__device__ __forceinline__ void gauss_entry(const GEntry<NC> &x, int32_t Fi, double (&g)[NC * 3]) {
+  if (x.f < Fi) {
+    const bool own = x.sl >= 0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      const double phif = x.w * x.pn[c] + (1 - x.w) * x.po[c];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        if (own) g[c * 3 + d] = g[c * 3 + d] + phif * x.S[d];
+        else g[c * 3 + d] = g[c * 3 + d] - phif * x.S[d];
+      }
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) g[c * 3 + d] = g[c * 3 + d] + x.po[0] * x.S[d];
+  }
+}
+
+// loads entry e of cell i (two dependent stages: face, then its data)
+template <int NC>
+__device__ __forceinline__ void gauss_face(const DevMesh &m, int32_t e, GEntry<NC> &x) {
+  x.f = m.cf_face[e];
+  x.sl = m.cf_slot[e];
+}
+
+template <int NC>
+__device__ __forceinline__ void gauss_data(const DevMesh &m, const double *phi, int32_t i,
+                                           GEntry<NC> &x) {
+  const int32_t f = x.f;
+#pragma unroll
+  for (int d = 0; d < 3; ++d) x.S[d] = m.Sf[f * 3 + d];
+  if (f < m.Fi) {
+    x.w = m.gf[f];
+    const int32_t o = x.sl >= 0 ? i : m.owner[f];
+    const int32_t nb = x.sl >= 0 ? m.neighbour[f] : i;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      x.po[c] = phi[o * NC + c];
+      x.pn[c] = phi[nb * NC + c];
+    }
+  } else {
+    x.w = 0.0;
+    x.po[0] = phi[(m.N + (f - m.Fi)) * NC + 0];
+  }
+}
+
+// Green-Gauss gradient of the cells (cfdComputeGradientGauss0.m:28-80): the
+// cell's faces in ascending order (interior faces first), then / volume.
+// Scalar fields load their entries in groups of 8 before the sums.
 template <int NC>
 __global__ void __launch_bounds__(kBlock) k_gauss_cells(DevMesh m, const double *phi, double *grad) {
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -100,28 +166,44 @@ __global__ void __launch_bounds__(kBlock) k_gauss_cells(DevMesh m, const double 
   double g[NC * 3];
 #pragma unroll
   for (int q = 0; q < NC * 3; ++q) g[q] = 0.0;
-  for (int32_t e = m.cf_ptr[i]; e < m.cf_ptr[i + 1]; ++e) {
-    const int32_t f = m.cf_face[e];
-    const double *S = &m.Sf[f * 3];
-    if (f < m.Fi) {
-      const int32_t o = m.owner[f], nb = m.neighbour[f];
-      const double w = m.gf[f];
-      const bool own = (o == i);
+  const int32_t e0 = m.cf_ptr[i], e1 = m.cf_ptr[i + 1];
+  if constexpr (NC == 1) {
+    for (int32_t e = e0; e < e1; e += 8) {
+      GEntry<1> x[8];
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        const double phif = w * phi[nb * NC + c] + (1 - w) * phi[o * NC + c];
+      for (int u = 0; u < 8; ++u)
+        if (e + u < e1) gauss_face<1>(m, e + u, x[u]);
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-          if (own) g[c * 3 + d] = g[c * 3 + d] + phif * S[d];
-          else g[c * 3 + d] = g[c * 3 + d] - phif * S[d];
+      for (int u = 0; u < 8; ++u)
+        if (e + u < e1) gauss_data<1>(m, phi, i, x[u]);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (e + u < e1) gauss_entry<1>(x[u], m.Fi, g);
+    }
+  } else {
+    for (int32_t e = e0; e < e1; ++e) {
+      const int32_t f = m.cf_face[e];
+      const double *S = &m.Sf[f * 3];
+      if (f < m.Fi) {
+        const int32_t o = m.owner[f], nb = m.neighbour[f];
+        const double w = m.gf[f];
+        const bool own = (o == i);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          const double phif = w * phi[nb * NC + c] + (1 - w) * phi[o * NC + c];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            if (own) g[c * 3 + d] = g[c * 3 + d] + phif * S[d];
+            else g[c * 3 + d] = g[c * 3 + d] - phif * S[d];
+          }
         }
+      } else {
+        const double phib = phi[(m.N + (f - m.Fi)) * NC + 0];
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) g[c * 3 + d] = g[c * 3 + d] + phib * S[d];
       }
-    } else {
-      const double phib = phi[(m.N + (f - m.Fi)) * NC + 0];
-#pragma unroll
-      for (int c = 0; c < NC; ++c)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) g[c * 3 + d] = g[c * 3 + d] + phib * S[d];
     }
   }
   const double v = m.vol[i];
@@ -221,113 +303,197 @@ __device__ inline void grad_face(const DevMesh &m, const double *phi, const doub
   for (int d = 0; d < 3; ++d) out[d] = (gfc[d] - lam * e[d]) + lgm * e[d];
 }
 
-// One momentum component c, terms in equation order
-// (cfdAssembleEquationTerms.m:22-55):
-//   convection, sign +1: upwind C1 = max(mdot,0), C2 = -max(-mdot,0), walls
-//     and empty patches zero (cfdAssembleConvectionTerm.m:103-137, :238-278),
-//     then the divergence term per cell (cfdAssembleConvectionDivergenceTerm.m:38-47);
-//   diffusion laplacian(nu*U), sign -1: interior with corrected face gradient
-//     (cfdAssembleDiffusionTerm.m:94-130), fixedValue walls (:224-263),
-//     zeroGradient (:330-347), empty vector patches (:631-690);
-//   -grad(p), sign +1 (cfdAssemblePressureGradientTerm.m:31-40);
-//   false transient, fdt = 1e9, rho = 1 (cfdAssembleFalseTransientTerm.m:28-49);
-//   URF (cfdApplyURF.m:15-17); normalised residual
-//   (cfdComputeNormalizedResidual.m:64-88); DU = V/ac, DUT = ac_old/ac
-//   (cfdPostAssembleEquation.m:37-41).
+// Face fluxes of momentum component c, six per face (uf[6 f + q]):
+//   convection, sign +1 (cfdAssembleConvectionTerm.m:103-137): q 0..2 =
+//     C1 = max(mdot,0), C2 = -max(-mdot,0), T = C1 U_O + C2 U_N; walls and
+//     empty patches contribute zero (:238-278; left 0 here);
+//   diffusion laplacian(nu*U), sign -1: q 3..5 = C1, C2, T; interior with the
+//     corrected face gradient (cfdAssembleDiffusionTerm.m:94-130), fixedValue
+//     walls (:224-263), zeroGradient (:330-347, zero), empty vector patches
+//     (:631-690); boundary faces have C2 = 0.
+__global__ void __launch_bounds__(kBlock)
+k_u_faces(DevMesh m, int c, const double *__restrict__ U, const double *__restrict__ gradU,
+          const double *__restrict__ mdot, const double *__restrict__ gamma,
+          double *__restrict__ uf) {
+  const int32_t f = blockIdx.x * kBlock + threadIdx.x;
+  if (f >= m.F) return;
+  const int32_t N = m.N, Fi = m.Fi;
+  const double gam = gamma[f];
+  double out[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (f < Fi) {
+    const int32_t o = m.owner[f], nb = m.neighbour[f];
+    const double psi = mdot[f];
+    const double C1 = 1.0 * fmax(psi, 0.0);
+    const double C2 = (-1.0) * fmax(-psi, 0.0);
+    out[0] = C1;
+    out[1] = C2;
+    out[2] = (C1 * U[o * 3 + c] + C2 * U[nb * 3 + c]) + 0.0;
+    double gfc[3];
+    grad_face<3>(m, U, gradU, c, f, gfc);
+    const double D1 = (1.0 * gam) * m.gDiff[f];
+    const double D2 = ((-1.0) * gam) * m.gDiff[f];
+    const double V = ((-1.0) * gam) * dot3(gfc, &m.T[f * 3]);
+    out[3] = D1;
+    out[4] = D2;
+    out[5] = (D1 * U[o * 3 + c] + D2 * U[nb * 3 + c]) + V;
+  } else {
+    const int32_t k = f - Fi, b = N + k, i = m.owner[f];
+    const int p = m.bpatch[k];
+    double C1 = 0.0, T = 0.0;
+    if (m.ptype[p] == PATCH_WALL && m.u_bc[p] == BC_FIXED_VALUE) {
+      C1 = (1.0 * gam) * m.gDiff[f];
+      const double C2 = ((-1.0) * gam) * m.gDiff[f];
+      const double V = ((-1.0) * gam) * dot3(&gradU[(b * 3 + c) * 3], &m.T[f * 3]);
+      T = (C1 * U[i * 3 + c] + C2 * U[b * 3 + c]) + V;
+    } else if (m.ptype[p] == PATCH_EMPTY) {
+      const double A = m.area[f];
+      double n[3], l[3] = {1.0, 1.0, 1.0}, a[3];
+      l[c] = 0.0;
+      for (int d = 0; d < 3; ++d) n[d] = m.Sf[f * 3 + d] / A;
+      const double dd = dot3(&m.CN[f * 3], n);
+      const double nc = n[c];
+      C1 = ((((1.0 * 2) * gam) * A) * (nc * nc)) / dd;
+      const double C2 = 0.0;
+      for (int d = 0; d < 3; ++d) a[d] = (U[i * 3 + d] * l[d]) * (n[d] * l[d]);
+      const double dl = (a[0] + a[1]) + a[2];
+      const double V = (((((-1.0) * 2) * gam) * A) * dl) * nc / dd;
+      T = (C1 * U[i * 3 + c] + C2 * U[b * 3 + c]) + V;
+    }
+    out[3] = C1;
+    out[5] = T;
+  }
+#pragma unroll
+  for (int q = 0; q < 6; ++q) uf[(int64_t)f * 6 + q] = out[q];
+}
+
+// One momentum component c from the face fluxes, terms in equation order
+// (cfdAssembleEquationTerms.m:22-55): convection over the cell's faces, then
+// the divergence term per cell (cfdAssembleConvectionDivergenceTerm.m:38-47),
+// diffusion over the faces (anb = (0 + convection) + diffusion, the two
+// contributions each slot receives), -grad(p), sign +1
+// (cfdAssemblePressureGradientTerm.m:31-40); false transient, fdt = 1e9,
+// rho = 1 (cfdAssembleFalseTransientTerm.m:28-49); URF (cfdApplyURF.m:15-17);
+// normalised residual (cfdComputeNormalizedResidual.m:64-88); DU = V/ac,
+// DUT = ac_old/ac (cfdPostAssembleEquation.m:37-41).
+// the terms of one cell-face entry of k_assemble_u: face f, side slot sl,
+// the six face fluxes q and mdot md
+struct UEntry {
+  int32_t f, sl;
+  double q[6], md;
+};
+
+__device__ __forceinline__ void load_uentry(const DevMesh &m, const double *__restrict__ uf,
+                                            const double *__restrict__ mdot, int32_t e,
+                                            UEntry &x) {
+  x.f = m.cf_face[e];
+  x.sl = m.cf_slot[e];
+#pragma unroll
+  for (int q = 0; q < 6; ++q) x.q[q] = uf[(int64_t)x.f * 6 + q];
+  x.md = mdot[x.f];
+}
+
+// one entry's share of each face pass of k_assemble_u
+__device__ __forceinline__ void conv_entry(const UEntry &x, int32_t Fi, double &ac, double &bc) {
+  if (x.f < Fi) {
+    if (x.sl >= 0) {
+      ac = ac + x.q[0];
+      bc = bc - x.q[2];
+    } else {
+      ac = ac - x.q[1];
+      bc = bc + x.q[2];
+    }
+  } else {
+    ac = ac + 0.0;
+    bc = bc - 0.0;
+  }
+}
+
+__device__ __forceinline__ void div_entry(const UEntry &x, int32_t Fi, double &eff) {
+  if (x.f < Fi && x.sl < 0) eff = eff - x.md;
+  else eff = eff + x.md;
+}
+
+__device__ __forceinline__ void div_term(double eff, double phi, double &ac, double &bc) {
+  const double CE = fmax(eff, 0.0) - eff;
+  const double VE = (-fmax(eff, 0.0)) * phi;
+  const double TE = CE * phi + VE;
+  ac = ac + CE;
+  bc = bc - TE;
+}
+
+__device__ __forceinline__ void diff_entry(const UEntry &x, int32_t Fi, double *anb, double &ac,
+                                           double &bc) {
+  const double *q = x.q;
+  if (x.f < Fi) {
+    const int32_t sl = x.sl;
+    if (sl >= 0) {
+      ac = ac + q[3];
+      anb[sl] = (0.0 + q[1]) + q[4];
+      bc = bc - q[5];
+    } else {
+      ac = ac - q[4];
+      anb[-1 - sl] = (0.0 - q[0]) - q[3];
+      bc = bc + q[5];
+    }
+  } else {
+    ac = ac + q[3];
+    bc = bc - q[5];
+  }
+}
+
+// One momentum component c from the face fluxes, terms in equation order
+// (cfdAssembleEquationTerms.m:22-55): convection over the cell's faces, then
+// the divergence term per cell (cfdAssembleConvectionDivergenceTerm.m:38-47),
+// diffusion over the faces (anb = (0 + convection) + diffusion, the two
+// contributions each slot receives), -grad(p), sign +1
+// (cfdAssemblePressureGradientTerm.m:31-40); false transient, fdt = 1e9,
+// rho = 1 (cfdAssembleFalseTransientTerm.m:28-49); URF (cfdApplyURF.m:15-17);
+// normalised residual (cfdComputeNormalizedResidual.m:64-88); DU = V/ac,
+// DUT = ac_old/ac (cfdPostAssembleEquation.m:37-41).  The entries of a cell
+// with at most 8 faces are loaded together before the passes.
 __global__ void __launch_bounds__(kBlock)
 k_assemble_u(DevMesh m, int c, double urf, const double *__restrict__ U,
-             const double *__restrict__ gradU, const double *__restrict__ gradp,
-             const double *__restrict__ mdot, const double *__restrict__ gamma, double *ac_out,
-             double *anb, double *bc_out, double *acold_out, double *dphi, double *DU,
-             double *DUT, double *partial, unsigned *counter, double *scal) {
+             const double *__restrict__ gradp, const double *__restrict__ mdot,
+             const double *__restrict__ uf, double *ac_out, double *anb, double *bc_out,
+             double *acold_out, double *dphi, double *DU, double *DUT, double *partial,
+             double *scal) {
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   double red[2] = {0.0, 0.0};
   if (i < m.N) {
-    const int32_t N = m.N, Fi = m.Fi;
-    for (int32_t q = m.cptr[i]; q < m.cptr[i + 1]; ++q) anb[q] = 0.0;
-    double ac = 0.0, bc = 0.0, acold = 0.0;
-    const int32_t e0 = m.cf_ptr[i], e1 = m.cf_ptr[i + 1];
-    // convection
-    for (int32_t e = e0; e < e1; ++e) {
-      const int32_t f = m.cf_face[e];
-      if (f < Fi) {
-        const double psi = mdot[f];
-        const double C1 = 1.0 * fmax(psi, 0.0);
-        const double C2 = (-1.0) * fmax(-psi, 0.0);
-        const double T = (C1 * U[m.owner[f] * 3 + c] + C2 * U[m.neighbour[f] * 3 + c]) + 0.0;
-        if (m.owner[f] == i) {
-          ac = ac + C1;
-          anb[m.own_nnz[f]] = anb[m.own_nnz[f]] + C2;
-          bc = bc - T;
-        } else {
-          ac = ac - C2;
-          anb[m.nei_nnz[f]] = anb[m.nei_nnz[f]] - C1;
-          bc = bc + T;
-        }
-      } else {
-        ac = ac + 0.0;
-        bc = bc - 0.0;
-      }
-    }
-    {
-      double eff = 0.0;
+    const int32_t Fi = m.Fi;
+    double ac = 0.0, bc = 0.0, acold = 0.0, eff = 0.0;
+    const int32_t e0 = m.cf_ptr[i], e1 = m.cf_ptr[i + 1], nf = e1 - e0;
+    const double phi = U[i * 3 + c];
+    if (nf <= 8) {
+      UEntry x[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (u < nf) load_uentry(m, uf, mdot, e0 + u, x[u]);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (u < nf) conv_entry(x[u], Fi, ac, bc);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (u < nf) div_entry(x[u], Fi, eff);
+      div_term(eff, phi, ac, bc);
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+        if (u < nf) diff_entry(x[u], Fi, anb, ac, bc);
+    } else {
+      // many faces: the same passes, loading one entry at a time
+      UEntry x;
       for (int32_t e = e0; e < e1; ++e) {
-        const int32_t f = m.cf_face[e];
-        if (f < Fi && m.neighbour[f] == i) eff = eff - mdot[f];
-        else eff = eff + mdot[f];
+        load_uentry(m, uf, mdot, e, x);
+        conv_entry(x, Fi, ac, bc);
       }
-      const double phi = U[i * 3 + c];
-      const double CE = fmax(eff, 0.0) - eff;
-      const double VE = (-fmax(eff, 0.0)) * phi;
-      const double TE = CE * phi + VE;
-      ac = ac + CE;
-      bc = bc - TE;
-    }
-    // diffusion
-    for (int32_t e = e0; e < e1; ++e) {
-      const int32_t f = m.cf_face[e];
-      const double gam = gamma[f];
-      if (f < Fi) {
-        double gfc[3];
-        grad_face<3>(m, U, gradU, c, f, gfc);
-        const double C1 = (1.0 * gam) * m.gDiff[f];
-        const double C2 = ((-1.0) * gam) * m.gDiff[f];
-        const double V = ((-1.0) * gam) * dot3(gfc, &m.T[f * 3]);
-        const double T = (C1 * U[m.owner[f] * 3 + c] + C2 * U[m.neighbour[f] * 3 + c]) + V;
-        if (m.owner[f] == i) {
-          ac = ac + C1;
-          anb[m.own_nnz[f]] = anb[m.own_nnz[f]] + C2;
-          bc = bc - T;
-        } else {
-          ac = ac - C2;
-          anb[m.nei_nnz[f]] = anb[m.nei_nnz[f]] - C1;
-          bc = bc + T;
-        }
-      } else {
-        const int32_t k = f - Fi, b = N + k;
-        const int p = m.bpatch[k];
-        double C1 = 0.0, T = 0.0;
-        if (m.ptype[p] == PATCH_WALL && m.u_bc[p] == BC_FIXED_VALUE) {
-          C1 = (1.0 * gam) * m.gDiff[f];
-          const double C2 = ((-1.0) * gam) * m.gDiff[f];
-          const double V = ((-1.0) * gam) * dot3(&gradU[(b * 3 + c) * 3], &m.T[f * 3]);
-          T = (C1 * U[i * 3 + c] + C2 * U[b * 3 + c]) + V;
-        } else if (m.ptype[p] == PATCH_EMPTY) {
-          const double A = m.area[f];
-          double n[3], l[3] = {1.0, 1.0, 1.0}, a[3];
-          l[c] = 0.0;
-          for (int d = 0; d < 3; ++d) n[d] = m.Sf[f * 3 + d] / A;
-          const double dd = dot3(&m.CN[f * 3], n);
-          const double nc = n[c];
-          C1 = ((((1.0 * 2) * gam) * A) * (nc * nc)) / dd;
-          const double C2 = 0.0;
-          for (int d = 0; d < 3; ++d) a[d] = (U[i * 3 + d] * l[d]) * (n[d] * l[d]);
-          const double dl = (a[0] + a[1]) + a[2];
-          const double V = (((((-1.0) * 2) * gam) * A) * dl) * nc / dd;
-          T = (C1 * U[i * 3 + c] + C2 * U[b * 3 + c]) + V;
-        }
-        ac = ac + C1;
-        bc = bc - T;
+      for (int32_t e = e0; e < e1; ++e) {
+        load_uentry(m, uf, mdot, e, x);
+        div_entry(x, Fi, eff);
+      }
+      div_term(eff, phi, ac, bc);
+      for (int32_t e = e0; e < e1; ++e) {
+        load_uentry(m, uf, mdot, e, x);
+        diff_entry(x, Fi, anb, ac, bc);
       }
     }
     const double vol = m.vol[i];
@@ -359,7 +525,6 @@ k_assemble_u(DevMesh m, int c, double urf, const double *__restrict__ U,
     DU[i * 3 + c] = vol / ac;
     DUT[i * 3 + c] = acold / ac;
   }
-  (void)counter;
   block_partials2(red[0], red[1], 1, 0, partial);
 }
 
@@ -374,12 +539,14 @@ __global__ void k_du_bnd(DevMesh m, int c, double *DU, double *DUT) {
 
 void assemble_u(hipStream_t s, const DevMesh &m, int comp, double urf, const double *U,
                 const double *gradU, const double *gradp, const double *mdot,
-                const double *gamma, double *ac, double *anb, double *bc, double *ac_old,
-                double *dphi, double *DU, double *DUT, double *partial, unsigned *counter,
-                double *scal) {
-  hipLaunchKernelGGL(k_assemble_u, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, comp, urf, U, gradU,
-                     gradp, mdot, gamma, ac, anb, bc, ac_old, dphi, DU, DUT, partial, counter,
-                     scal);
+                const double *gamma, double *uf, double *ac, double *anb, double *bc,
+                double *ac_old, double *dphi, double *DU, double *DUT, double *partial,
+                unsigned *counter, double *scal) {
+  (void)counter;
+  hipLaunchKernelGGL(k_u_faces, dim3(nblocks(m.F)), dim3(kBlock), 0, s, m, comp, U, gradU, mdot,
+                     gamma, uf);
+  hipLaunchKernelGGL(k_assemble_u, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, comp, urf, U, gradp,
+                     mdot, uf, ac, anb, bc, ac_old, dphi, DU, DUT, partial, scal);
   hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kBlock), 0, s, partial, nblocks(m.N), 1, scal);
   const int32_t Fb = m.F - m.Fi;
   if (Fb > 0) hipLaunchKernelGGL(k_du_bnd, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, comp, DU, DUT);
@@ -477,53 +644,77 @@ void rhie_chow(hipStream_t s, const DevMesh &m, double urf_u, const double *U,
 __global__ void __launch_bounds__(kBlock)
 k_assemble_p(DevMesh m, const double *__restrict__ fluxC1, const double *__restrict__ fluxC2,
              const double *__restrict__ mdot, double *ac_out, double *anb, double *bc_out,
-             double *acold_out, double *dphi, double *partial, unsigned *counter, double *scal) {
+             double *acold_out, double *dphi, double *partial, int32_t *changed) {
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   double red[2] = {-INFINITY, 0.0};
+  bool differs = false;  // a coefficient's bits differ from the array's previous value
+  auto put = [&](double *a, int32_t q, double v) {
+    differs |= __double_as_longlong(a[q]) != __double_as_longlong(v);
+    a[q] = v;
+  };
   if (i < m.N) {
-    for (int32_t q = m.cptr[i]; q < m.cptr[i + 1]; ++q) anb[q] = 0.0;
     double ac = 0.0, bc = 0.0, acold = 0.0, eff = 0.0;
-    for (int32_t e = m.cf_ptr[i]; e < m.cf_ptr[i + 1]; ++e) {
-      const int32_t f = m.cf_face[e];
-      if (f < m.Fi) {
-        const double C1 = fluxC1[f], C2 = fluxC2[f], T = mdot[f];
-        if (m.owner[f] == i) {
-          ac = ac + C1;
-          anb[m.own_nnz[f]] = anb[m.own_nnz[f]] + C2;
-          bc = bc - T;
-          eff = eff + T;
-        } else {
-          ac = ac - C2;
-          anb[m.nei_nnz[f]] = anb[m.nei_nnz[f]] - C1;
-          bc = bc + T;
-          eff = eff - T;
+    const int32_t e0 = m.cf_ptr[i], e1 = m.cf_ptr[i + 1];
+    // entries in groups of four: faces first, then their fluxes, then the sums
+    for (int32_t e = e0; e < e1; e += 4) {
+      int32_t f[4], sl[4];
+      double C1[4], C2[4], T[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (e + u < e1) {
+          f[u] = m.cf_face[e + u];
+          sl[u] = m.cf_slot[e + u];
         }
-      } else {
-        ac = ac + 0.0;
-        bc = bc - 0.0;
-        eff = eff + mdot[f];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        if (e + u < e1) {
+          T[u] = mdot[f[u]];
+          if (f[u] < m.Fi) {
+            C1[u] = fluxC1[f[u]];
+            C2[u] = fluxC2[f[u]];
+          }
+        }
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (e + u >= e1) break;
+        if (f[u] < m.Fi) {
+          if (sl[u] >= 0) {  // each anb slot has this one contribution
+            ac = ac + C1[u];
+            put(anb, sl[u], 0.0 + C2[u]);
+            bc = bc - T[u];
+            eff = eff + T[u];
+          } else {
+            ac = ac - C2[u];
+            put(anb, -1 - sl[u], 0.0 - C1[u]);
+            bc = bc + T[u];
+            eff = eff - T[u];
+          }
+        } else {
+          ac = ac + 0.0;
+          bc = bc - 0.0;
+          eff = eff + T[u];
+        }
       }
     }
     ac = ac + 0.0;
     acold = acold + 0.0;
     bc = bc - 0.0;
-    ac_out[i] = ac;
+    put(ac_out, i, ac);
     bc_out[i] = bc;
     acold_out[i] = acold;
     dphi[i] = 0.0;
     red[0] = eff;
     red[1] = eff * eff;
   }
-  (void)counter;
-  (void)scal;
+  if (differs) *changed = 1;  // the same value from every writer
   block_partials2(red[0], red[1], 1, 0, partial);
 }
 
 void assemble_p(hipStream_t s, const DevMesh &m, const double *fluxC1, const double *fluxC2,
                 const double *mdot, double *ac, double *anb, double *bc, double *ac_old,
-                double *dphi, double *partial, unsigned *counter, double *scal) {
+                double *dphi, double *partial, double *scal, int32_t *changed) {
   hipLaunchKernelGGL(k_assemble_p, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, fluxC1, fluxC2,
-                     mdot, ac, anb, bc, ac_old, dphi, partial, counter, scal);
+                     mdot, ac, anb, bc, ac_old, dphi, partial, changed);
   hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kBlock), 0, s, partial, nblocks(m.N), 1, scal);
 }
 

@@ -97,7 +97,11 @@ struct RefLevel {
   DBuf<double> ac, anb, bc, dphi, r, rc, dinv;
   // map to the next coarser level (defined below the coarsest)
   std::vector<int32_t> h_parents;
-  DBuf<int32_t> parents, child_ptr, child, slot;
+  DBuf<int32_t> parents, child_ptr, child;
+  // the coarse operator's gather lists (galerkin_lists), values of the next
+  // coarser level: nc diagonal values then its off-diagonal slots
+  DBuf<int32_t> gal_ptr, gal_src;
+  int32_t gal_out = 0;
   // lexicographic schedules (SOR / DILU), built on first use
   bool sched = false;
   Schedule fwd, bwd;
