@@ -162,7 +162,8 @@ typedef struct {
   int32_t smoother;          /* finest level: 0 multicolour Gauss-Seidel (colour classes in
                                 parallel), 1 the reference's DILU ('ILU' smoother,
                                 cfdFactorizeILU.m / cfdSolveILU.m) in its lexicographic
-                                order, level-scheduled */
+                                order, level-scheduled, 2 DILU in the colour order of 0
+                                (one level per colour class) */
   double coarse_scale;       /* every coarse-grid correction is scaled by this factor
                                 (1: plain; 2-2.5 suits the piecewise-constant
                                 prolongation of aggregation AMG); in (0, 4], 0 = 2.5 */

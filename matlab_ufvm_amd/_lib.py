@@ -17,7 +17,7 @@ UFVM_OK = 0
 ERRORS = {1: "UFVM_ERR_ARG", 2: "UFVM_ERR_HIP", 3: "UFVM_ERR_STATE", 4: "UFVM_ERR_NUMERIC",
           5: "UFVM_ERR_UNSUPPORTED"}
 SMOOTHER = {"ILU": 0, "SOR": 1, "MCGS": 2}
-PCG_SMOOTHER = {"MCGS": 0, "ILU": 1}
+PCG_SMOOTHER = {"MCGS": 0, "ILU": 1, "MCILU": 2}
 CYCLE = {"V-Cycle": 0, "W-Cycle": 1, "F-Cycle": 2}
 
 i32p = C.POINTER(C.c_int32)
@@ -305,7 +305,8 @@ class Context:
                   coarse_scale=2.5, coarse_pre_sweep=2, coarse_post_sweep=2,
                   launch_mode="auto"):
         """smoother (finest level): "MCGS" multicolour Gauss-Seidel, "ILU" the
-        reference's DILU in its lexicographic order (level-scheduled);
+        reference's DILU in its lexicographic order (level-scheduled), "MCILU"
+        DILU in the colour order of "MCGS" (one dependency level per colour);
         coarse_scale: factor on every coarse-grid correction; coarse_*_sweep:
         sweeps below the finest level (-1: as on the finest); the defaults
         2.5 and 2 / 2 were tuned on the 2.1M-cell hex and 5.3M-cell tet

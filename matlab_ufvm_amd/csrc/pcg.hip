@@ -20,7 +20,8 @@ namespace detail {
 // reference rows, each sorted by a colouring of the block's own subgraph.
 // kind 2: the coarsest level, kept in reference order for the dense solve.
 // kind 3 (coarse levels smoothed by damped Jacobi): rows sorted by length
-// inside windows.  Rows keep their cconn order in every kind.
+// inside windows.  kind 5 (finest level, DILU in colour order): kind 0's
+// numbering.  Rows keep their cconn order in every kind.
 void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
   hipStream_t s = ctx->stream;
   RefLevel &R = *S.lev[l];
@@ -32,7 +33,7 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
   FL.nnz = R.nnz;
   FL.kind = kind;
   FL.perm.resize(n);
-  if (kind == 0) {
+  if (kind == 0 || kind == 5) {
     FL.ncol = greedy_colour(R.g, FL.colour);
     FL.col_off.assign(FL.ncol + 1, 0);
     for (int32_t i = 0; i < n; ++i) ++FL.col_off[FL.colour[i] + 1];
@@ -106,9 +107,28 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
     FL.dinv64.alloc(n);
     FL.dinv32.alloc(n);
   }
+  if (kind == 5) {
+    // lower = earlier colour = smaller permuted index (a colour class has no
+    // internal edges)
+    HostCSR pg;
+    pg.n = n;
+    pg.ptr = FL.h_ptr;
+    pg.col.assign(col.begin(), col.begin() + R.nnz);
+    LowerLists low;
+    if (!lower_lists(pg, low)) throw Error(UFVM_ERR_NUMERIC, "ILU: the index for i in j is not found");
+    FL.lptr.upload(low.ptr, s);
+    FL.lnb.upload(low.nb.empty() ? std::vector<int32_t>{0} : low.nb, s);
+    FL.lpji.upload(low.pos_ji.empty() ? std::vector<int32_t>{0} : low.pos_ji, s);
+    FL.lpij.upload(low.pos_ij.empty() ? std::vector<int32_t>{0} : low.pos_ij, s);
+    std::vector<int32_t> iota(n);
+    for (int32_t t = 0; t < n; ++t) iota[t] = t;
+    FL.iota.upload(iota, s);
+    FL.dinv64.alloc(n);
+    FL.dinv32.alloc(n);
+  }
   // the FPCG's operator product runs on level 0's sliced-ELL copy whatever its
   // kind (a single-level hierarchy makes level 0 the dense coarsest level)
-  if (kind == 0 || kind == 3 || l == 0) {
+  if (kind == 0 || kind == 3 || kind == 5 || l == 0) {
     const int32_t nq = (n + 63) / 64;
     FL.h_soff.assign(nq + 1, 0);
     for (int32_t q = 0; q < nq; ++q) {
@@ -171,6 +191,16 @@ void refresh_level(ufvm_ctx *ctx, System &S, int l) {
                             R.lnb.p, R.lpji.p, R.lpij.p, R.n - 2, R.dinv.p);
     k::convert<double, double>(s, n, R.dinv.p, FL.dinv64.p);
     k::convert<double, float>(s, n, R.dinv.p, FL.dinv32.p);
+  }
+  if (FL.kind == 5) {
+    // DILU in colour order, FP64, one launch per colour:
+    // d_j = a_jj - sum over earlier-colour neighbours i (increasing i) of
+    // (a_ji / d_i) a_ij
+    for (int c = 0; c < FL.ncol; ++c)
+      k::dilu_factor_rows(s, FL.iota.p + FL.col_off[c], FL.col_off[c + 1] - FL.col_off[c],
+                          FL.ac64.p, FL.anb64.p, FL.lptr.p, FL.lnb.p, FL.lpji.p, FL.lpij.p, -1,
+                          FL.dinv64.p);
+    k::convert<double, float>(s, n, FL.dinv64.p, FL.dinv32.p);
   }
 }
 
@@ -315,6 +345,27 @@ struct Cycle {
                           P::anb(L), P::dinv(L), rc, x);
   }
 
+  // One sweep of DILU in colour order on a kind-5 level: rc = b - A x (rc = b
+  // from zero); forward substitution over the colours upward
+  // (rc_j -= a_ji (rc_i / d_i), earlier colours i); backward substitution over
+  // the colours downward (z_i = (rc_i - sum of a_ij z_j, later colours j) / d_i,
+  // x += z).  Each colour is one launch.
+  void mc_dilu_sweep(const FastLevel &L, LevelWork &LW, V *x, bool from_zero) {
+    V *rc = P::r(LW);
+    if (from_zero) {
+      k::fill<V>(s, L.n, x, V(0));
+      k::convert<V, V>(s, L.n, P::b(LW), rc);
+    } else {
+      k::residual_sell<V>(s, L.n, 0, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW), x, rc);
+    }
+    for (int c = 0; c < L.ncol; ++c)
+      k::dilu_fwd_rows<V>(s, L.iota.p + L.col_off[c], L.col_off[c + 1] - L.col_off[c], P::anb(L),
+                          L.lptr.p, L.lnb.p, L.lpji.p, P::dinv(L), rc);
+    for (int c = L.ncol - 1; c >= 0; --c)
+      k::dilu_bwd_rows<V>(s, L.iota.p + L.col_off[c], L.col_off[c + 1] - L.col_off[c], L.n,
+                          L.ptr.p, L.col.p, P::anb(L), P::dinv(L), rc, x);
+  }
+
   // smoothing sweeps of level l (the coarse levels may have their own counts)
   int pre_of(int l) const {
     const ufvm_pcg_cfg &c = S.pcg.cfg;
@@ -383,14 +434,18 @@ struct Cycle {
       }
       return x;
     }
-    if (L.kind == 4) {
+    if (L.kind == 4 || L.kind == 5) {
       const RefLevel &R = *S.lev[l];
+      auto sweep = [&](bool from_zero) {
+        if (L.kind == 4) dilu_sweep(R, L, LW, x, from_zero);
+        else mc_dilu_sweep(L, LW, x, from_zero);
+      };
       if (pre == 0) k::fill<V>(s, L.n, x, V(0));
-      for (int it = 0; it < pre; ++it) dilu_sweep(R, L, LW, x, it == 0);
+      for (int it = 0; it < pre; ++it) sweep(it == 0);
       down(L, LW, l, x, 0);
       V *xc = vcycle(l + 1);
       k::prolong_fast<V>(s, 0, L.n, L.parent.p, xc, x);
-      for (int it = 0; it < post; ++it) dilu_sweep(R, L, LW, x, false);
+      for (int it = 0; it < post; ++it) sweep(false);
       return x;
     }
     if (pre == 0) {
@@ -524,7 +579,7 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
     throw Error(UFVM_ERR_ARG, "max_coarse_cells must be in [1, 4096]");
   if (cfg.coarse_smoother != 0 && cfg.coarse_smoother != 1)
     throw Error(UFVM_ERR_ARG, "coarse_smoother must be 0 or 1");
-  if (cfg.smoother != 0 && cfg.smoother != 1) throw Error(UFVM_ERR_ARG, "smoother must be 0 or 1");
+  if (cfg.smoother < 0 || cfg.smoother > 2) throw Error(UFVM_ERR_ARG, "smoother must be 0, 1 or 2");
   const double cscale = cfg.coarse_scale == 0.0 ? 2.5 : cfg.coarse_scale;   // 0: default
   if (!(cscale > 0.0 && cscale <= 4.0))
     throw Error(UFVM_ERR_ARG, "coarse_scale must be in (0, 4] (0: the default 2.5)");
@@ -569,7 +624,7 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
   for (int32_t l = 0; l < nlev; ++l)
     build_fast_level(ctx, S, l,
                      l == nlev - 1 ? 2
-                                   : (l == 0 ? (cfg.smoother == 1 ? 4 : 0)
+                                   : (l == 0 ? (cfg.smoother == 1 ? 4 : cfg.smoother == 2 ? 5 : 0)
                                              : (cfg.coarse_smoother == 1 ? 3 : 1)));
   for (int32_t l = 0; l + 1 < nlev; ++l) link_levels(ctx, S, l);
   for (int32_t l = 0; l < nlev; ++l) refresh_level(ctx, S, l);

@@ -131,6 +131,9 @@ struct FastLevel {
   // kind 3: coarse level smoothed by damped Jacobi, rows sorted by length
   // kind 4: finest level smoothed by the reference's DILU in its own
   //         (lexicographic) order, level-scheduled; reference numbering
+  // kind 5: finest level smoothed by DILU in colour order: kind 0's
+  //         numbering, so a colour class is one level of the lower-triangular
+  //         dependency graph
   int kind = 0;
   std::vector<int32_t> col_off;   // kind 0: colour c rows are [col_off[c], col_off[c+1])
   int32_t nblk = 0, bncol = 0;    // kind 1: blocks and colours per block
@@ -152,9 +155,12 @@ struct FastLevel {
   DBuf<int32_t> sell_src;         // reference CSR position of each slot (-1: padding)
   DBuf<float> sval32;
   DBuf<double> sval64;
-  // kind 4: DILU diagonal inverse (factored once in FP64 at set-up)
+  // kinds 4, 5: DILU diagonal inverse (factored in FP64 at set-up / refresh)
   DBuf<float> dinv32;
   DBuf<double> dinv64;
+  // kind 5: row numbers 0..n-1 (the row lists of the colour levels) and the
+  // lower neighbour lists of the permuted pattern (lower_lists)
+  DBuf<int32_t> iota, lptr, lnb, lpji, lpij;
 };
 
 // Per-solve device workspace of one system: each level's right-hand side,

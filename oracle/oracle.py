@@ -70,6 +70,9 @@ def lib():
                                C.POINTER(VCfg), f64p, C.c_double, C.c_int, f64p, f64p]
         L.orc_fpcg.restype = C.c_int
         L.orc_factorize_ilu.argtypes = [C.POINTER(Level), f64p, f64p]
+        L.orc_mcilu_factor.argtypes = [C.POINTER(Level), i32p, C.c_int, f64p]
+        L.orc_mcilu_sweep.argtypes = [C.POINTER(Level), i32p, C.c_int, f64p, f64p, C.c_int,
+                                      f64p, f64p]
         L.orc_sor.argtypes = [C.POINTER(Level), f64p]
         L.orc_hier_free.argtypes = [C.POINTER(Hier)]
         _lib = L
@@ -164,6 +167,24 @@ class System:
         lib().orc_factorize_ilu(C.byref(L), _p(dc, f64p), _p(rc, f64p))
         return dc
 
+    def mcilu(self, b, l=0, x=None, sweeps=1):
+        """DILU in colour order (rows by greedy colour, then index) on level
+        l: returns (dinv, x after `sweeps` sweeps, the first from x = 0 when x
+        is None)."""
+        L = self.H.lev[l]
+        c = np.empty(L.n, np.int32)
+        nc = lib().orc_colour(L.n, L.ptr, L.col, _p(c, i32p))
+        dinv = np.empty(L.n)
+        lib().orc_mcilu_factor(C.byref(L), _p(c, i32p), nc, _p(dinv, f64p))
+        b = np.ascontiguousarray(b, np.float64)
+        rc = np.empty(L.n)
+        from_zero = x is None
+        x = np.zeros(L.n) if from_zero else np.array(x, np.float64)
+        for s in range(sweeps):
+            lib().orc_mcilu_sweep(C.byref(L), _p(c, i32p), nc, _p(dinv, f64p), _p(b, f64p),
+                                  1 if (s == 0 and from_zero) else 0, _p(rc, f64p), _p(x, f64p))
+        return dinv, x
+
     # ------------------------------------------------ precision-form oracle
     def _colour_all(self, nlev):
         """level 0: global greedy colouring; coarse levels: block-local."""
@@ -192,7 +213,8 @@ class System:
         nlev = self.pcg_levels(max_coarse_cells)
         arr, ncl = self._colour_all(nlev)
         cfg = VCfg(nlev, pre, post, singular, BLOCK_ROWS, coarse_smoother,
-                   {"MCGS": 0, "ILU": 1}[smoother], coarse_scale, pre if cpre < 0 else cpre,
+                   {"MCGS": 0, "ILU": 1, "MCILU": 2}[smoother], coarse_scale,
+                   pre if cpre < 0 else cpre,
                    post if cpost < 0 else cpost)
         b = np.ascontiguousarray(b, np.float64)
         x = np.empty_like(b)
@@ -204,7 +226,8 @@ class System:
         nlev = self.pcg_levels(max_coarse_cells)
         arr, ncl = self._colour_all(nlev)
         cfg = VCfg(nlev, pre, post, singular, BLOCK_ROWS, coarse_smoother,
-                   {"MCGS": 0, "ILU": 1}[smoother], coarse_scale, pre if cpre < 0 else cpre,
+                   {"MCGS": 0, "ILU": 1, "MCILU": 2}[smoother], coarse_scale,
+                   pre if cpre < 0 else cpre,
                    post if cpost < 0 else cpost)
         b = np.ascontiguousarray(b, np.float64)
         x = np.empty_like(b)

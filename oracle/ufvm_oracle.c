@@ -558,6 +558,104 @@ void orc_mcgs(const orc_level *L, const int32_t *colour, int ncolours, const dou
       if (colour[i] == c) gs_row(L, b, x, i);
 }
 
+/* ---- DILU in colour order (the kind-5 finest smoother of the GPU path) */
+
+/* rows in the order (colour, index): order[t] = row, pos[row] = t */
+static void mc_order(const int32_t *colour, int ncol, int32_t n, int32_t *order, int32_t *pos) {
+  int32_t *start = (int32_t *)calloc((size_t)ncol + 1, sizeof(int32_t));
+  for (int32_t i = 0; i < n; ++i) ++start[colour[i] + 1];
+  for (int c = 0; c < ncol; ++c) start[c + 1] += start[c];
+  for (int32_t i = 0; i < n; ++i) order[start[colour[i]]++] = i;
+  for (int32_t t = 0; t < n; ++t) pos[order[t]] = t;
+  free(start);
+}
+
+/* first entry of row r with column c (-1 if none) */
+static int32_t entry_of(const orc_level *L, int32_t r, int32_t c) {
+  for (int32_t k = L->ptr[r]; k < L->ptr[r + 1]; ++k)
+    if (L->col[k] == c) return k;
+  return -1;
+}
+
+/* row j's lower neighbours (earlier in the order), increasing position:
+ * their rows into nb[], returns the count */
+static int lower_of(const orc_level *L, const int32_t *pos, int32_t j, int32_t *nb) {
+  int m = 0;
+  for (int32_t k = L->ptr[j]; k < L->ptr[j + 1]; ++k)
+    if (pos[L->col[k]] < pos[j]) nb[m++] = L->col[k];
+  for (int a = 1; a < m; ++a) /* insertion sort by position, stable */
+    for (int b = a; b > 0 && pos[nb[b - 1]] > pos[nb[b]]; --b) {
+      int32_t t = nb[b]; nb[b] = nb[b - 1]; nb[b - 1] = t;
+    }
+  return m;
+}
+
+static int32_t max_row(const orc_level *L) {
+  int32_t w = 1;
+  for (int32_t i = 0; i < L->n; ++i)
+    if (L->ptr[i + 1] - L->ptr[i] > w) w = L->ptr[i + 1] - L->ptr[i];
+  return w;
+}
+
+void orc_mcilu_factor(const orc_level *L, const int32_t *colour, int ncol, double *dinv) {
+  const int32_t n = L->n;
+  int32_t *order = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+  int32_t *pos = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+  int32_t *nb = (int32_t *)malloc((size_t)max_row(L) * sizeof(int32_t));
+  mc_order(colour, ncol, n, order, pos);
+  for (int32_t t = 0; t < n; ++t) {
+    const int32_t j = order[t];
+    double d = L->ac[j];
+    const int m = lower_of(L, pos, j, nb);
+    for (int e = 0; e < m; ++e) {
+      const int32_t i = nb[e];
+      d = d - (L->anb[entry_of(L, j, i)] * dinv[i]) * L->anb[entry_of(L, i, j)];
+    }
+    dinv[j] = 1.0 / d;
+  }
+  free(order); free(pos); free(nb);
+}
+
+void orc_mcilu_sweep(const orc_level *L, const int32_t *colour, int ncol, const double *dinv,
+                     const double *b, int from_zero, double *rc, double *x) {
+  const int32_t n = L->n;
+  int32_t *order = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+  int32_t *pos = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+  int32_t *nb = (int32_t *)malloc((size_t)max_row(L) * sizeof(int32_t));
+  mc_order(colour, ncol, n, order, pos);
+  if (from_zero) {
+    for (int32_t i = 0; i < n; ++i) { x[i] = 0.0; rc[i] = b[i]; }
+  } else {
+    for (int32_t i = 0; i < n; ++i) {
+      double s = b[i] - L->ac[i] * x[i];
+      for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) s = s - L->anb[k] * x[L->col[k]];
+      rc[i] = s;
+    }
+  }
+  for (int32_t t = 0; t < n; ++t) { /* forward, colours upward */
+    const int32_t j = order[t];
+    double v = rc[j];
+    const int m = lower_of(L, pos, j, nb);
+    for (int e = 0; e < m; ++e) {
+      const int32_t i = nb[e];
+      const double mat1 = dinv[i] * rc[i];
+      v = v - L->anb[entry_of(L, j, i)] * mat1;
+    }
+    rc[j] = v;
+  }
+  for (int32_t t = n - 1; t >= 0; --t) { /* backward, colours downward */
+    const int32_t i = order[t];
+    double v = rc[i];
+    if (t < n - 1)
+      for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k)
+        if (pos[L->col[k]] > t) v = v - L->anb[k] * rc[L->col[k]];
+    const double z = dinv[i] * v;
+    rc[i] = z;
+    x[i] = x[i] + z;
+  }
+  free(order); free(pos); free(nb);
+}
+
 /* dense LU with partial pivoting of a (row-major n x n), in place */
 typedef struct {
   int32_t n;
@@ -686,10 +784,14 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
     free(y); free(r); free(bc); free(xc);
     return;
   }
-  if (l == 0 && cfg->smoother == 1) {
-    /* the reference's DILU in its lexicographic order, pre and post */
+  if (l == 0 && (cfg->smoother == 1 || cfg->smoother == 2)) {
+    /* DILU pre and post: the reference's in its lexicographic order (1), or
+     * in colour order (2) */
     double *rc = (double *)malloc((size_t)n * sizeof(double));
-    for (int s = 0; s < cfg->pre; ++s) ilu_apply(L, dc0, b, rc, x);
+    for (int s = 0; s < cfg->pre; ++s) {
+      if (cfg->smoother == 1) ilu_apply(L, dc0, b, rc, x);
+      else orc_mcilu_sweep(L, colours[0], ncol[0], dc0, b, s == 0, rc, x);
+    }
     double *r = (double *)malloc((size_t)n * sizeof(double));
     for (int32_t i = 0; i < n; ++i) {
       double s = b[i] - L->ac[i] * x[i];
@@ -703,7 +805,10 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
     for (int32_t q = 0; q < C->n; ++q) bc[q] = cfg->coarse_scale * bc[q];
     vcyc_rec(H, colours, ncol, cfg, lu, dc0, l + 1, bc, xc);
     for (int32_t i = 0; i < n; ++i) x[i] = x[i] + xc[L->parents[i]];
-    for (int s = 0; s < cfg->post; ++s) ilu_apply(L, dc0, b, rc, x);
+    for (int s = 0; s < cfg->post; ++s) {
+      if (cfg->smoother == 1) ilu_apply(L, dc0, b, rc, x);
+      else orc_mcilu_sweep(L, colours[0], ncol[0], dc0, b, 0, rc, x);
+    }
     free(rc); free(r); free(bc); free(xc);
     return;
   }
@@ -744,10 +849,15 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
 }
 
 /* DILU factor of level 0 when the finest smoother is DILU (else NULL) */
-static double *finest_dilu(const orc_hier *H, const orc_vcfg *cfg) {
-  if (cfg->smoother != 1 || cfg->nlev < 2) return NULL;
+static double *finest_dilu(const orc_hier *H, const orc_vcfg *cfg, int32_t *const *colours,
+                           const int *ncolours) {
+  if ((cfg->smoother != 1 && cfg->smoother != 2) || cfg->nlev < 2) return NULL;
   const orc_level *L = &H->lev[0];
   double *dc = (double *)malloc((size_t)L->n * sizeof(double));
+  if (cfg->smoother == 2) {
+    orc_mcilu_factor(L, colours[0], ncolours[0], dc);
+    return dc;
+  }
   double *rc = (double *)malloc((size_t)L->n * sizeof(double));
   orc_factorize_ilu(L, dc, rc);
   free(rc);
@@ -758,7 +868,7 @@ void orc_vcycle(const orc_hier *H, int32_t *const *colours, const int *ncolours,
                 const orc_vcfg *cfg, const double *b, double *x) {
   orc_lu lu;
   lu_build(&H->lev[cfg->nlev - 1], cfg->singular, &lu);
-  double *dc0 = finest_dilu(H, cfg);
+  double *dc0 = finest_dilu(H, cfg, colours, ncolours);
   vcyc_rec(H, colours, ncolours, cfg, &lu, dc0, 0, b, x);
   free(dc0);
   lu_free(&lu);
@@ -804,7 +914,7 @@ int orc_fpcg(const orc_hier *H, int32_t *const *colours, const int *ncolours,
   double delta = 1.0;
   orc_lu lu;
   lu_build(&H->lev[cfg->nlev - 1], cfg->singular, &lu);
-  double *dc0 = finest_dilu(H, cfg);
+  double *dc0 = finest_dilu(H, cfg, colours, ncolours);
   while (it < maxit) {
     vcyc_rec(H, colours, ncolours, cfg, &lu, dc0, 0, r, z);
     if (cfg->singular) project(z, n);

@@ -91,11 +91,21 @@ typedef struct {
   int32_t block_rows;
   int32_t coarse_smoother;  /* levels 1..nlev-2: 0 block-local MCGS, 1 damped Jacobi (2/3) */
   int32_t smoother;         /* level 0: 0 symmetric multicolour GS, 1 the reference's DILU
-                               (cfdFactorizeILU / cfdSolveILU, lexicographic order) */
+                               (cfdFactorizeILU / cfdSolveILU, lexicographic order),
+                               2 DILU in colour order (rows by colour, then index) */
   double coarse_scale;      /* restricted right-hand sides (hence every coarse correction)
                                scaled by this factor */
   int32_t cpre, cpost;      /* sweeps on levels 1..nlev-2 */
 } orc_vcfg;
+
+/* DILU in colour order on one level (no reference counterpart): rows in the
+ * order (colour, index); the lower neighbours of row j are its neighbours of
+ * an earlier colour, taken in that order.  dinv_j = 1 / (a_jj - sum of
+ * (a_ji dinv_i) a_ij).  orc_mcilu_factor fills dinv[n]; orc_mcilu_sweep runs
+ * one sweep on x (from_zero: x = 0 and rc = b, else rc = b - A x), rc[n] work. */
+void orc_mcilu_factor(const orc_level *L, const int32_t *colour, int ncol, double *dinv);
+void orc_mcilu_sweep(const orc_level *L, const int32_t *colour, int ncol, const double *dinv,
+                     const double *b, int from_zero, double *rc, double *x);
 void orc_vcycle(const orc_hier *H, int32_t *const *colours, const int *ncolours,
                 const orc_vcfg *cfg, const double *b, double *x);
 

@@ -136,14 +136,16 @@ def test_pcg_vcycle_fp64_matches_oracle(ctx, case, coarse_smoother, sweeps):
     z_ref = S.vcycle(r, pre=sweeps, post=sweeps, max_coarse_cells=200,
                      coarse_smoother=coarse_smoother)
     assert np.abs(z - z_ref).max() <= 1e-12 * np.abs(z_ref).max()
-    # the DILU finest smoother (reference ordering, level-scheduled)
+    # the DILU finest smoothers (reference ordering, level-scheduled; colour
+    # ordering, one level per colour)
     if coarse_smoother == 1:
-        ctx.pcg_setup(sid, precision=64, pre_sweep=sweeps, post_sweep=sweeps,
-                      max_coarse_cells=200, coarse_smoother=1, smoother="ILU")
-        zi = ctx.pcg_apply_vcycle(sid, r)
-        zi_ref = S.vcycle(r, pre=sweeps, post=sweeps, max_coarse_cells=200, coarse_smoother=1,
-                          smoother="ILU")
-        assert np.abs(zi - zi_ref).max() <= 1e-12 * np.abs(zi_ref).max()
+        for sm in ("ILU", "MCILU"):
+            ctx.pcg_setup(sid, precision=64, pre_sweep=sweeps, post_sweep=sweeps,
+                          max_coarse_cells=200, coarse_smoother=1, smoother=sm)
+            zi = ctx.pcg_apply_vcycle(sid, r)
+            zi_ref = S.vcycle(r, pre=sweeps, post=sweeps, max_coarse_cells=200,
+                              coarse_smoother=1, smoother=sm)
+            assert np.abs(zi - zi_ref).max() <= 1e-12 * np.abs(zi_ref).max(), sm
         ctx.pcg_setup(sid, precision=64, pre_sweep=sweeps, post_sweep=sweeps,
                       max_coarse_cells=200, coarse_smoother=coarse_smoother)
     # same colourings as the oracle's greedy order (level 1 always; the coarse
@@ -160,7 +162,9 @@ def test_pcg_vcycle_fp64_matches_oracle(ctx, case, coarse_smoother, sweeps):
                                                      (32, 1e-11, 1e-8, 1, "MCGS"),
                                                      (32, 1e-11, 1e-8, 0, "MCGS"),
                                                      (32, 1e-11, 1e-8, 1, "ILU"),
-                                                     (64, 1e-12, 1e-10, 1, "ILU")])
+                                                     (64, 1e-12, 1e-10, 1, "ILU"),
+                                                     (32, 1e-11, 1e-8, 1, "MCILU"),
+                                                     (64, 1e-12, 1e-10, 1, "MCILU")])
 def test_pcg_solve_matches_direct(ctx, case, precision, tol, err, cs, sm):
     _, m, ptr, col, ac, anb, b = case
     sid = make(ctx, ptr, col, ac, anb, b)

@@ -12,7 +12,7 @@ import scipy.sparse as sp
 import scipy.sparse.linalg as spl
 
 import oracle
-from matlab_ufvm_amd.workloads import cavity_mesh, gaussian_rhs, poisson_system
+from matlab_ufvm_amd.workloads import cavity_mesh, gaussian_rhs, poisson_system, poisson_tet_mesh
 
 
 def chain(anb_links):
@@ -277,6 +277,70 @@ def test_dilu_sweep_applies_factored_preconditioner(poisson16):
     assert np.abs(S.level(0)["dphi"] - x1).max() <= 1e-11 * np.abs(x1).max()
 
 
+def colour_order(S):
+    """rows by (greedy colour, index): the order of the colour-order DILU"""
+    S._colour_all(1)
+    c, _ = S.colours_of(0)
+    return np.lexsort((np.arange(c.size), c)), c
+
+
+def mcilu_matrix(A, order, dinv):
+    """M = (D' + L) D'^-1 (D' + U) of the colour-order DILU, in the original
+    numbering (L, U the strictly lower / upper parts in the colour order)"""
+    Ap = A[np.ix_(order, order)]
+    Dp = np.diag(1.0 / dinv[order])
+    Mp = (Dp + np.tril(Ap, -1)) @ np.linalg.inv(Dp) @ (Dp + np.triu(Ap, 1))
+    M = np.empty_like(Mp)
+    M[np.ix_(order, order)] = Mp
+    return M
+
+
+def test_mcilu_equals_ilu0_of_colour_permuted_matrix_on_triangle_free_graph():
+    # a 2-D grid has no triangles, so DILU pivots are ILU(0) pivots; in colour
+    # order (no quirk row) they are those of the colour-permuted matrix
+    m = cavity_mesh(6, nz=1, jitter=0.1, seed=5, renumber_seed=None)
+    ptr, col, ac, anb = poisson_system(m, dirichlet=("movingWall",))
+    A = to_scipy(ptr, col, ac, anb)
+    S = oracle.System(ptr, col, ac, anb)
+    order, c = colour_order(S)
+    dinv, _ = S.mcilu(np.zeros(m.n_cells))
+    Ap = A.toarray()[np.ix_(order, order)]
+    d_ref = ilu0_pivots(sp.csr_matrix(Ap))
+    assert np.allclose(1.0 / dinv[order], d_ref, rtol=1e-13)
+    # colour 0 rows have no earlier-colour neighbours: d = a_ii exactly
+    assert np.array_equal(dinv[c == 0], 1.0 / ac[c == 0])
+
+
+def test_mcilu_keeps_the_diagonal_on_tets():
+    # DILU's defining property, triangles or not: diag(M) = diag(A)
+    m = poisson_tet_mesh(3, jitter=0.1, seed=9, renumber_seed=10)
+    ptr, col, ac, anb = poisson_system(m, dirichlet=("dirichlet",))
+    A = to_scipy(ptr, col, ac, anb).toarray()
+    S = oracle.System(ptr, col, ac, anb)
+    order, _ = colour_order(S)
+    dinv, _ = S.mcilu(np.zeros(m.n_cells))
+    M = mcilu_matrix(A, order, dinv)
+    assert np.allclose(np.diag(M), np.diag(A), rtol=1e-12)
+    # with triangles the colour-ordered DILU differs from A off the diagonal
+    # (fill a_ji a_ik / d_i onto existing entries), unlike ILU(0)
+    assert not np.allclose(M, A, rtol=1e-9, atol=0)
+
+
+def test_mcilu_sweep_applies_factored_preconditioner(poisson16):
+    m, ptr, col, ac, anb, b = poisson16
+    A = to_scipy(ptr, col, ac, anb).toarray()
+    S = oracle.System(ptr, col, ac, anb)
+    order, _ = colour_order(S)
+    x0 = np.random.default_rng(4).standard_normal(m.n_cells)
+    dinv, x1 = S.mcilu(b, x=x0)
+    M = mcilu_matrix(A, order, dinv)
+    ref = x0 + np.linalg.solve(M, b - A @ x0)
+    assert np.abs(x1 - ref).max() <= 1e-11 * np.abs(ref).max()
+    # from x = 0 the sweep is M^-1 b
+    _, z = S.mcilu(b)
+    assert np.abs(z - np.linalg.solve(M, b)).max() <= 1e-11 * np.abs(z).max()
+
+
 def test_solve_algebraic_system_early_exit(poisson16):
     m, ptr, col, ac, anb, b = poisson16
     # the loop runs while final > rrf*initial: rrf = 1 runs no sweep at all
@@ -370,7 +434,8 @@ def test_vcycle_symmetric_with_block_smoothed_levels(coarse_smoother):
 
 
 @pytest.mark.parametrize("smoother,sweeps,w", [("MCGS", 1, 1.0), ("MCGS", 2, 2.0),
-                                               ("ILU", 1, 2.0), ("ILU", 2, 1.5)])
+                                               ("ILU", 1, 2.0), ("ILU", 2, 1.5),
+                                               ("MCILU", 1, 2.5), ("MCILU", 2, 1.0)])
 def test_two_level_vcycle_matches_dense_formula(poisson16, smoother, sweeps, w):
     # With the coarse level solved exactly the V-cycle is, in matrices,
     #   x1 = S^pre(0; b), xc = Ac^-1 P^T (b - A x1), x2 = x1 + w P xc, z = S^post(x2; b)
@@ -378,6 +443,7 @@ def test_two_level_vcycle_matches_dense_formula(poisson16, smoother, sweeps, w):
     #   MCGS: M = D + L_c (colours up) then D + U_c (colours down), L_c / U_c the
     #         couplings to lower / higher colour classes
     #   ILU:  M = (D' + L) D'^-1 (D' + U), D' the DILU diagonal (pinned above)
+    #   MCILU: the same in colour order (pinned above)
     # Ac = P^T A P from the parents and w the coarse-correction scale.
     m, ptr, col, ac, anb, b = poisson16
     S = oracle.System(ptr, col, ac, anb)
@@ -396,10 +462,15 @@ def test_two_level_vcycle_matches_dense_formula(poisson16, smoother, sweeps, w):
         lower = off * (c[None, :] < c[:, None])
         upper = off * (c[None, :] > c[:, None])
         Ms = [D + lower, D + upper]
-    else:
+    elif smoother == "ILU":
         Sd = oracle.System(ptr, col, ac, anb, bc=b)
         Dp = np.diag(1.0 / Sd.factorize_ilu())
         Ms = [(Dp + np.tril(A, -1)) @ np.linalg.inv(Dp) @ (Dp + np.triu(A, 1))]
+    else:
+        Sd = oracle.System(ptr, col, ac, anb)
+        order, _ = colour_order(Sd)
+        dinv, _ = Sd.mcilu(b)
+        Ms = [mcilu_matrix(A, order, dinv)]
 
     def sweep(x):
         for M in Ms:
@@ -417,7 +488,7 @@ def test_two_level_vcycle_matches_dense_formula(poisson16, smoother, sweeps, w):
     assert np.abs(z - x).max() <= 1e-10 * np.abs(x).max()
 
 
-@pytest.mark.parametrize("smoother", ["MCGS", "ILU"])
+@pytest.mark.parametrize("smoother", ["MCGS", "ILU", "MCILU"])
 def test_vcycle_symmetric_for_each_finest_smoother(poisson16, smoother):
     m, ptr, col, ac, anb, b = poisson16
     S = oracle.System(ptr, col, ac, anb)
@@ -430,7 +501,7 @@ def test_vcycle_symmetric_for_each_finest_smoother(poisson16, smoother):
     assert y @ My > 0
 
 
-@pytest.mark.parametrize("smoother", ["MCGS", "ILU"])
+@pytest.mark.parametrize("smoother", ["MCGS", "ILU", "MCILU"])
 def test_fpcg_matches_direct_solve(poisson16, smoother):
     m, ptr, col, ac, anb, b = poisson16
     S = oracle.System(ptr, col, ac, anb)
