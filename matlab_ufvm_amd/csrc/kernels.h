@@ -43,6 +43,17 @@ template <typename V>
 void dilu_bwd_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
                     int32_t n, const int32_t *ptr, const int32_t *col, const V *anb,
                     const V *dinv, V *rc, V *x);
+// DILU in colour order on a sliced-ELL level whose colours are contiguous
+// row ranges: forward pass of the colour rows [r0, r1) (rc from b - A x and
+// the earlier colours; zero: x = 0) and backward pass (z from the later
+// colours, x += z; zero: x = z)
+template <typename V>
+void mcilu_fwd_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const int32_t *col,
+                    const V *val, const V *ac, const V *b, const V *x, const V *dinv, V *rc,
+                    bool zero);
+template <typename V>
+void mcilu_bwd_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const int32_t *col,
+                    const V *val, const V *dinv, V *rc, V *x, bool zero);
 // DILU factorisation of the listed rows (pull form of cfdFactorizeILU)
 void dilu_factor_rows(hipStream_t s, const int32_t *rows, int32_t count, const double *ac,
                       const double *anb, const int32_t *lptr, const int32_t *lnb,

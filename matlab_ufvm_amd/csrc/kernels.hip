@@ -478,6 +478,130 @@ void mcgs_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const 
                      val, ac, b, x, zero_above);
 }
 
+// ---- DILU in colour order on the sliced-ELL copy of a level whose colour
+// classes are contiguous row ranges (earlier colours = lower rows).  Forward
+// pass over the colour rows [r0, r1):
+//   rc_i = (b_i - ac_i x_i - sum_k a_ik x_k) - sum_{k < r0} a_ik (dinv_k rc_k)
+// both sums in entry order (ZERO: x = 0, the residual part is b_i).
+// Backward pass over [r0, r1), colours downward:
+//   z_i = dinv_i (rc_i - sum_{k >= r1} a_ik rc_k),  rc_i = z_i,  x_i += z_i
+// (ZERO: x_i = z_i).  Padding entries (col = the row) fall in neither sum.
+template <typename V, bool ZERO>
+__global__ void __launch_bounds__(kBlock)
+k_mcilu_fwd_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off,
+                 const int32_t *__restrict__ col, const V *__restrict__ val,
+                 const V *__restrict__ ac, const V *__restrict__ b, const V *__restrict__ x,
+                 const V *__restrict__ dinv, V *rc) {
+  const int32_t i = (r0 & ~63) + blockIdx.x * kBlock + threadIdx.x;
+  if (i < r0 || i >= r1) return;
+  const int32_t q = i >> 6;
+  const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
+  V res = ZERO ? b[i] : b[i] - ac[i] * x[i];
+  V low = V(0);
+  int32_t k = 0;
+  for (; k + 4 <= w; k += 4) {
+    int32_t j[4];
+    V a[4], xj[4], lj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      j[u] = col[base + (k + u) * 64];
+      a[u] = val[base + (k + u) * 64];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (!ZERO) xj[u] = x[j[u]];
+      lj[u] = j[u] < r0 ? dinv[j[u]] * rc[j[u]] : V(0);
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (!ZERO) res = res - a[u] * xj[u];
+      if (j[u] < r0) low = low + a[u] * lj[u];
+    }
+  }
+  for (; k < w; ++k) {
+    const int32_t j = col[base + k * 64];
+    const V a = val[base + k * 64];
+    if (!ZERO) res = res - a * x[j];
+    if (j < r0) low = low + a * (dinv[j] * rc[j]);
+  }
+  rc[i] = res - low;
+}
+
+template <typename V, bool ZERO>
+__global__ void __launch_bounds__(kBlock)
+k_mcilu_bwd_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off,
+                 const int32_t *__restrict__ col, const V *__restrict__ val,
+                 const V *__restrict__ dinv, V *rc, V *x) {
+  const int32_t i = (r0 & ~63) + blockIdx.x * kBlock + threadIdx.x;
+  if (i < r0 || i >= r1) return;
+  const int32_t q = i >> 6;
+  const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
+  V v = rc[i];
+  int32_t k = 0;
+  for (; k + 4 <= w; k += 4) {
+    int32_t j[4];
+    V a[4], rj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      j[u] = col[base + (k + u) * 64];
+      a[u] = val[base + (k + u) * 64];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) rj[u] = j[u] >= r1 ? rc[j[u]] : V(0);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (j[u] >= r1) v = v - a[u] * rj[u];
+  }
+  for (; k < w; ++k) {
+    const int32_t j = col[base + k * 64];
+    if (j >= r1) v = v - val[base + k * 64] * rc[j];
+  }
+  const V z = dinv[i] * v;
+  rc[i] = z;
+  x[i] = ZERO ? z : x[i] + z;
+}
+
+template <typename V>
+void mcilu_fwd_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const int32_t *col,
+                    const V *val, const V *ac, const V *b, const V *x, const V *dinv, V *rc,
+                    bool zero) {
+  if (r1 <= r0) return;
+  const int32_t a = r0 & ~63;
+  if (zero)
+    hipLaunchKernelGGL((k_mcilu_fwd_sell<V, true>), dim3(nblocks(r1 - a)), dim3(kBlock), 0, s, r0,
+                       r1, off, col, val, ac, b, x, dinv, rc);
+  else
+    hipLaunchKernelGGL((k_mcilu_fwd_sell<V, false>), dim3(nblocks(r1 - a)), dim3(kBlock), 0, s,
+                       r0, r1, off, col, val, ac, b, x, dinv, rc);
+}
+
+template <typename V>
+void mcilu_bwd_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const int32_t *col,
+                    const V *val, const V *dinv, V *rc, V *x, bool zero) {
+  if (r1 <= r0) return;
+  const int32_t a = r0 & ~63;
+  if (zero)
+    hipLaunchKernelGGL((k_mcilu_bwd_sell<V, true>), dim3(nblocks(r1 - a)), dim3(kBlock), 0, s, r0,
+                       r1, off, col, val, dinv, rc, x);
+  else
+    hipLaunchKernelGGL((k_mcilu_bwd_sell<V, false>), dim3(nblocks(r1 - a)), dim3(kBlock), 0, s,
+                       r0, r1, off, col, val, dinv, rc, x);
+}
+
+template void mcilu_fwd_sell<float>(hipStream_t, int32_t, int32_t, const int32_t *,
+                                    const int32_t *, const float *, const float *, const float *,
+                                    const float *, const float *, float *, bool);
+template void mcilu_fwd_sell<double>(hipStream_t, int32_t, int32_t, const int32_t *,
+                                     const int32_t *, const double *, const double *,
+                                     const double *, const double *, const double *, double *,
+                                     bool);
+template void mcilu_bwd_sell<float>(hipStream_t, int32_t, int32_t, const int32_t *,
+                                    const int32_t *, const float *, const float *, float *,
+                                    float *, bool);
+template void mcilu_bwd_sell<double>(hipStream_t, int32_t, int32_t, const int32_t *,
+                                     const int32_t *, const double *, const double *, double *,
+                                     double *, bool);
+
 // r = b - A x; rows below zero_rows get r = 0 without computing it (the
 // colour class a Gauss-Seidel sweep solved last, whose residual is rounding
 // noise)

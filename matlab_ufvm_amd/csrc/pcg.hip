@@ -345,25 +345,18 @@ struct Cycle {
                           P::anb(L), P::dinv(L), rc, x);
   }
 
-  // One sweep of DILU in colour order on a kind-5 level: rc = b - A x (rc = b
-  // from zero); forward substitution over the colours upward
-  // (rc_j -= a_ji (rc_i / d_i), earlier colours i); backward substitution over
-  // the colours downward (z_i = (rc_i - sum of a_ij z_j, later colours j) / d_i,
-  // x += z).  Each colour is one launch.
+  // One sweep of DILU in colour order on a kind-5 level: a forward pass over
+  // the colours upward (the residual b - A x fused in; from zero: x = 0) and
+  // a backward pass over the colours downward (x += z; from zero: x = z), one
+  // launch per colour each way on the sliced-ELL copy.
   void mc_dilu_sweep(const FastLevel &L, LevelWork &LW, V *x, bool from_zero) {
     V *rc = P::r(LW);
-    if (from_zero) {
-      k::fill<V>(s, L.n, x, V(0));
-      k::convert<V, V>(s, L.n, P::b(LW), rc);
-    } else {
-      k::residual_sell<V>(s, L.n, 0, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW), x, rc);
-    }
     for (int c = 0; c < L.ncol; ++c)
-      k::dilu_fwd_rows<V>(s, L.iota.p + L.col_off[c], L.col_off[c + 1] - L.col_off[c], P::anb(L),
-                          L.lptr.p, L.lnb.p, L.lpji.p, P::dinv(L), rc);
+      k::mcilu_fwd_sell<V>(s, L.col_off[c], L.col_off[c + 1], L.soff.p, L.scol.p, P::sval(L),
+                           P::ac(L), P::b(LW), x, P::dinv(L), rc, from_zero);
     for (int c = L.ncol - 1; c >= 0; --c)
-      k::dilu_bwd_rows<V>(s, L.iota.p + L.col_off[c], L.col_off[c + 1] - L.col_off[c], L.n,
-                          L.ptr.p, L.col.p, P::anb(L), P::dinv(L), rc, x);
+      k::mcilu_bwd_sell<V>(s, L.col_off[c], L.col_off[c + 1], L.soff.p, L.scol.p, P::sval(L),
+                           P::dinv(L), rc, x, from_zero);
   }
 
   // smoothing sweeps of level l (the coarse levels may have their own counts)

@@ -621,39 +621,33 @@ void orc_mcilu_sweep(const orc_level *L, const int32_t *colour, int ncol, const 
   const int32_t n = L->n;
   int32_t *order = (int32_t *)malloc((size_t)n * sizeof(int32_t));
   int32_t *pos = (int32_t *)malloc((size_t)n * sizeof(int32_t));
-  int32_t *nb = (int32_t *)malloc((size_t)max_row(L) * sizeof(int32_t));
   mc_order(colour, ncol, n, order, pos);
-  if (from_zero) {
-    for (int32_t i = 0; i < n; ++i) { x[i] = 0.0; rc[i] = b[i]; }
-  } else {
-    for (int32_t i = 0; i < n; ++i) {
-      double s = b[i] - L->ac[i] * x[i];
-      for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) s = s - L->anb[k] * x[L->col[k]];
-      rc[i] = s;
-    }
-  }
-  for (int32_t t = 0; t < n; ++t) { /* forward, colours upward */
+  /* forward, colours upward: rc_j = (b - A x)_j - sum over earlier-colour
+   * neighbours (entry order) of a_ji (dinv_i rc_i); from zero x = 0 and the
+   * residual part is b_j */
+  for (int32_t t = 0; t < n; ++t) {
     const int32_t j = order[t];
-    double v = rc[j];
-    const int m = lower_of(L, pos, j, nb);
-    for (int e = 0; e < m; ++e) {
-      const int32_t i = nb[e];
-      const double mat1 = dinv[i] * rc[i];
-      v = v - L->anb[entry_of(L, j, i)] * mat1;
+    double res = from_zero ? b[j] : b[j] - L->ac[j] * x[j];
+    double low = 0.0;
+    for (int32_t k = L->ptr[j]; k < L->ptr[j + 1]; ++k) {
+      const int32_t i = L->col[k];
+      if (!from_zero) res = res - L->anb[k] * x[i];
+      if (colour[i] < colour[j]) low = low + L->anb[k] * (dinv[i] * rc[i]);
     }
-    rc[j] = v;
+    rc[j] = res - low;
   }
-  for (int32_t t = n - 1; t >= 0; --t) { /* backward, colours downward */
+  /* backward, colours downward: z_i = dinv_i (rc_i - sum over later-colour
+   * neighbours of a_ij z_j), x += z */
+  for (int32_t t = n - 1; t >= 0; --t) {
     const int32_t i = order[t];
     double v = rc[i];
-    if (t < n - 1)
-      for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k)
-        if (pos[L->col[k]] > t) v = v - L->anb[k] * rc[L->col[k]];
+    for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k)
+      if (colour[L->col[k]] > colour[i]) v = v - L->anb[k] * rc[L->col[k]];
     const double z = dinv[i] * v;
     rc[i] = z;
-    x[i] = x[i] + z;
+    x[i] = from_zero ? z : x[i] + z;
   }
-  free(order); free(pos); free(nb);
+  free(order); free(pos);
 }
 
 /* dense LU with partial pivoting of a (row-major n x n), in place */

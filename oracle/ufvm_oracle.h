@@ -100,9 +100,12 @@ typedef struct {
 
 /* DILU in colour order on one level (no reference counterpart): rows in the
  * order (colour, index); the lower neighbours of row j are its neighbours of
- * an earlier colour, taken in that order.  dinv_j = 1 / (a_jj - sum of
- * (a_ji dinv_i) a_ij).  orc_mcilu_factor fills dinv[n]; orc_mcilu_sweep runs
- * one sweep on x (from_zero: x = 0 and rc = b, else rc = b - A x), rc[n] work. */
+ * an earlier colour.  orc_mcilu_factor fills dinv[n]: dinv_j = 1 / (a_jj -
+ * sum of (a_ji dinv_i) a_ij), lower neighbours in increasing order.
+ * orc_mcilu_sweep runs one sweep on x: forward rc_j = (b - A x)_j - sum of
+ * a_ji (dinv_i rc_i) over lower neighbours in entry order, backward
+ * z_i = dinv_i (rc_i - sum of a_ij z_j over upper neighbours), x += z
+ * (from_zero: x = 0 before, x = z after); rc[n] is work space. */
 void orc_mcilu_factor(const orc_level *L, const int32_t *colour, int ncol, double *dinv);
 void orc_mcilu_sweep(const orc_level *L, const int32_t *colour, int ncol, const double *dinv,
                      const double *b, int from_zero, double *rc, double *x);
