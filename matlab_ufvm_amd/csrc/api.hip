@@ -60,6 +60,7 @@ void upload_pattern(RefLevel &L, hipStream_t s) {
   L.dinv.alloc(L.n);
   L.sched = false;
   L.low_ok = false;
+  L.cs = false;  // a new pattern: schedules and colour chunks are rebuilt on use
 }
 
 double residual_l1(ufvm_ctx *ctx, RefLevel &L, double *out_r) {
@@ -117,26 +118,80 @@ void sor_sweep(ufvm_ctx *ctx, RefLevel &L) {
                L.anb.p, L.bc.p, L.dphi.p);
 }
 
+// Colour classes of a greedy first-fit colouring (index order) and their
+// sliced-ELL chunks (built on first use; the pattern is fixed)
+static void ensure_colour_chunks(ufvm_ctx *ctx, RefLevel &L) {
+  if (L.cs) return;
+  hipStream_t s = ctx->stream;
+  std::vector<int32_t> colour;
+  const int32_t nc = greedy_colour(L.g, colour);
+  std::vector<int32_t> mc_ptr(nc + 1, 0), mc_rows(std::max(L.n, 1), 0);
+  for (int32_t c : colour) ++mc_ptr[c + 1];
+  for (int32_t c = 0; c < nc; ++c) mc_ptr[c + 1] += mc_ptr[c];
+  std::vector<int32_t> at(mc_ptr.begin(), mc_ptr.end() - 1);
+  for (int32_t i = 0; i < L.n; ++i) mc_rows[at[colour[i]]++] = i;
+  L.cs_cptr.assign(nc + 1, 0);
+  std::vector<int32_t> row;
+  for (int32_t c = 0; c < nc; ++c) {
+    for (int32_t t = mc_ptr[c]; t < mc_ptr[c + 1]; ++t) row.push_back(mc_rows[t]);
+    while (row.size() % 64) row.push_back(-1);
+    L.cs_cptr[c + 1] = (int32_t)(row.size() / 64);
+  }
+  const int32_t nq = (int32_t)(row.size() / 64);
+  std::vector<int32_t> off(nq + 1, 0);
+  for (int32_t q = 0; q < nq; ++q) {
+    int32_t w = 0;
+    for (int32_t u = 0; u < 64; ++u) {
+      const int32_t i = row[(size_t)q * 64 + u];
+      if (i >= 0) w = std::max(w, L.g.ptr[i + 1] - L.g.ptr[i]);
+    }
+    off[q + 1] = off[q] + 64 * w;
+  }
+  const size_t slots = std::max(off[nq], 1);
+  std::vector<int32_t> col(slots, -1), src(slots, -1);
+  for (int32_t q = 0; q < nq; ++q) {
+    const int32_t w = (off[q + 1] - off[q]) / 64;
+    for (int32_t u = 0; u < 64; ++u) {
+      const int32_t i = row[(size_t)q * 64 + u];
+      if (i < 0) continue;
+      for (int32_t e = 0; e < w && L.g.ptr[i] + e < L.g.ptr[i + 1]; ++e) {
+        col[off[q] + (size_t)e * 64 + u] = L.g.col[L.g.ptr[i] + e];
+        src[off[q] + (size_t)e * 64 + u] = L.g.ptr[i] + e;
+      }
+    }
+  }
+  if (row.empty()) row.push_back(-1);
+  L.cs_row.upload(row, s);
+  L.cs_off.upload(off, s);
+  L.cs_col.upload(col, s);
+  L.cs_src.upload(src, s);
+  L.cs_val.alloc(slots);
+  L.cs_slots = (int64_t)nq * 64;
+  L.cs = true;
+}
+
+// sum of |r|, r = b - A x, over the colour chunks (each row's arithmetic is
+// k_residual_ref's; the per-block sums group the rows by colour)
+static double residual_l1_chunks(ufvm_ctx *ctx, RefLevel &L) {
+  ensure_partial(ctx, (int32_t)std::max<int64_t>(L.cs_slots, 1), 1);
+  k::residual_csell(ctx->stream, L.cs_slots, L.cs_row.p, L.cs_off.p, L.cs_col.p, L.cs_val.p,
+                    L.ac.p, L.bc.p, L.dphi.p, L.r.p, ctx->partial.p);
+  k::reduce_final(ctx->stream, ctx->partial.p, k::nblocks(L.cs_slots), ctx->scal.p);
+  read_scal(ctx, 0, 1);
+  return ctx->hscal[0];
+}
+
 // Symmetric multicolour Gauss-Seidel sweep in the reference numbering: the
 // classes of a greedy first-fit colouring (index order) upward, then
 // downward without the last class (re-solving it repeats its arithmetic).
-// Each row is the reference's Gauss-Seidel row update (k_gs_rows).
+// Each row is the reference's Gauss-Seidel row update (k_gs_rows' arithmetic,
+// on the colour chunks; cs_val must hold the current anb).
 void mcgs_sweep(ufvm_ctx *ctx, RefLevel &L) {
-  if (!L.mc) {
-    std::vector<int32_t> colour;
-    const int32_t nc = greedy_colour(L.g, colour);
-    L.mc_ptr.assign(nc + 1, 0);
-    for (int32_t c : colour) ++L.mc_ptr[c + 1];
-    for (int32_t c = 0; c < nc; ++c) L.mc_ptr[c + 1] += L.mc_ptr[c];
-    std::vector<int32_t> rows(std::max(L.n, 1)), at(L.mc_ptr.begin(), L.mc_ptr.end() - 1);
-    for (int32_t i = 0; i < L.n; ++i) rows[at[colour[i]]++] = i;
-    L.mc_rows.upload(rows, ctx->stream);
-    L.mc = true;
-  }
-  const int32_t nc = (int32_t)L.mc_ptr.size() - 1;
+  ensure_colour_chunks(ctx, L);
+  const int32_t nc = (int32_t)L.cs_cptr.size() - 1;
   auto pass = [&](int32_t c) {
-    k::gs_rows(ctx->stream, L.mc_rows.p + L.mc_ptr[c], L.mc_ptr[c + 1] - L.mc_ptr[c], L.ptr.p,
-               L.col.p, L.ac.p, L.anb.p, L.bc.p, L.dphi.p);
+    k::gs_csell(ctx->stream, L.cs_cptr[c], L.cs_cptr[c + 1], L.cs_row.p, L.cs_off.p, L.cs_col.p,
+                L.cs_val.p, L.ac.p, L.bc.p, L.dphi.p);
   };
   for (int32_t c = 0; c < nc; ++c) pass(c);
   for (int32_t c = nc - 2; c >= 0; --c) pass(c);
@@ -184,7 +239,13 @@ void solve_alg(ufvm_ctx *ctx, RefLevel &L, int smoother, int n_iter, double rrf,
       smoother != UFVM_SMOOTHER_MCGS)
     throw Error(UFVM_ERR_ARG, "smoother must be ILU, SOR or MCGS");
   if (smoother != UFVM_SMOOTHER_MCGS) ensure_schedule(ctx, L);
-  const double initial = residual_l1(ctx, L, L.r.p) / L.n;
+  const bool chunks = smoother == UFVM_SMOOTHER_MCGS;
+  if (chunks) {
+    ensure_colour_chunks(ctx, L);
+    k::gather_vals<double>(ctx->stream, (int64_t)L.cs_val.n, L.cs_src.p, L.anb.p, L.cs_val.p);
+  }
+  auto l1 = [&]() { return chunks ? residual_l1_chunks(ctx, L) : residual_l1(ctx, L, L.r.p); };
+  const double initial = l1() / L.n;
   double fin = initial;
   if (std::isnan(initial)) throw Error(UFVM_ERR_NUMERIC, "NaN residual");
   if (smoother == UFVM_SMOOTHER_ILU) dilu_factor(ctx, L);
@@ -193,7 +254,7 @@ void solve_alg(ufvm_ctx *ctx, RefLevel &L, int smoother, int n_iter, double rrf,
     if (smoother == UFVM_SMOOTHER_ILU) dilu_sweep(ctx, L);
     else if (smoother == UFVM_SMOOTHER_SOR) sor_sweep(ctx, L);
     else mcgs_sweep(ctx, L);
-    fin = residual_l1(ctx, L, L.r.p) / L.n;
+    fin = l1() / L.n;
     ++iter;
   }
   if (r0) *r0 = initial;

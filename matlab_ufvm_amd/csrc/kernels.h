@@ -73,6 +73,14 @@ void restrict_ref(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t 
                   const double *r, double *bc_c);
 // x[i] += xc[parent[i]]
 void prolong_ref(hipStream_t s, int32_t n, const int32_t *parent, const double *xc, double *x);
+// the reference level as colour-class sliced-ELL chunks: r = b - A x of
+// every row (per-block |r| sums) and Gauss-Seidel over the chunks [q0, q1)
+void residual_csell(hipStream_t s, int64_t nslot, const int32_t *row, const int32_t *off,
+                    const int32_t *col, const double *val, const double *ac, const double *bc,
+                    const double *x, double *r, double *partial);
+void gs_csell(hipStream_t s, int32_t q0, int32_t q1, const int32_t *row, const int32_t *off,
+              const int32_t *col, const double *val, const double *ac, const double *b,
+              double *x);
 // Galerkin coarse operator in the reference accumulation order, from the
 // gather lists of galerkin_lists (nout = nc + coarse nnz values)
 void coarse_lhs(hipStream_t s, int32_t nout, int32_t nc, const int32_t *ptr, const int32_t *src,

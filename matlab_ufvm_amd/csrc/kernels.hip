@@ -41,6 +41,86 @@ k_residual_ref(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__rest
   if (threadIdx.x == 0) partial[blockIdx.x] = a;
 }
 
+// ---- the reference level's rows as colour-class sliced-ELL chunks (slot
+// row t of chunk t / 64 is row row[t], -1 for padding rows; entries in cconn
+// order, padding entries col = -1).  Each row's arithmetic is that of
+// k_residual_ref / k_gs_rows.
+__device__ __forceinline__ double csell_row(const int32_t *__restrict__ off,
+                                            const int32_t *__restrict__ col,
+                                            const double *__restrict__ val, const double *x,
+                                            int32_t t, double v) {
+  const int32_t q = t >> 6;
+  const int32_t base = off[q] + (t & 63), w = (off[q + 1] - off[q]) >> 6;
+  int32_t k = 0;
+  for (; k + 4 <= w; k += 4) {
+    int32_t j[4];
+    double a[4], xj[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      j[u] = col[base + (k + u) * 64];
+      a[u] = val[base + (k + u) * 64];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xj[u] = j[u] >= 0 ? x[j[u]] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (j[u] >= 0) v = v - a[u] * xj[u];
+  }
+  for (; k < w; ++k) {
+    const int32_t j = col[base + k * 64];
+    if (j >= 0) v = v - val[base + k * 64] * x[j];
+  }
+  return v;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_residual_csell(int64_t nslot, const int32_t *__restrict__ row, const int32_t *__restrict__ off,
+                 const int32_t *__restrict__ col, const double *__restrict__ val,
+                 const double *__restrict__ ac, const double *__restrict__ bc,
+                 const double *__restrict__ x, double *__restrict__ r,
+                 double *__restrict__ partial) {
+  __shared__ double lds[kBlock / kWave];
+  const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  double a = 0.0;
+  const int32_t i = t < nslot ? row[t] : -1;
+  if (i >= 0) {
+    const double s = csell_row(off, col, val, x, (int32_t)t, bc[i] - ac[i] * x[i]);
+    r[i] = s;
+    a = fabs(s);
+  }
+  a = block_sum(a, lds);
+  if (threadIdx.x == 0) partial[blockIdx.x] = a;
+}
+
+void residual_csell(hipStream_t s, int64_t nslot, const int32_t *row, const int32_t *off,
+                    const int32_t *col, const double *val, const double *ac, const double *bc,
+                    const double *x, double *r, double *partial) {
+  if (nslot <= 0) return;
+  hipLaunchKernelGGL(k_residual_csell, dim3(nblocks(nslot)), dim3(kBlock), 0, s, nslot, row, off,
+                     col, val, ac, bc, x, r, partial);
+}
+
+// Gauss-Seidel rows of the chunks [q0, q1) (one colour class)
+__global__ void __launch_bounds__(kBlock)
+k_gs_csell(int32_t q0, int32_t q1, const int32_t *__restrict__ row,
+           const int32_t *__restrict__ off, const int32_t *__restrict__ col,
+           const double *__restrict__ val, const double *__restrict__ ac,
+           const double *__restrict__ b, double *x) {
+  const int32_t t = q0 * 64 + blockIdx.x * kBlock + threadIdx.x;
+  if (t >= q1 * 64) return;
+  const int32_t i = row[t];
+  if (i < 0) return;
+  x[i] = csell_row(off, col, val, x, t, b[i]) / ac[i];
+}
+
+void gs_csell(hipStream_t s, int32_t q0, int32_t q1, const int32_t *row, const int32_t *off,
+              const int32_t *col, const double *val, const double *ac, const double *b,
+              double *x) {
+  if (q1 <= q0) return;
+  hipLaunchKernelGGL(k_gs_csell, dim3(nblocks((int64_t)(q1 - q0) * 64)), dim3(kBlock), 0, s, q0,
+                     q1, row, off, col, val, ac, b, x);
+}
+
 void residual_ref(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col,
                   const double *ac, const double *anb, const double *bc, const double *x,
                   double *r, double *partial) {

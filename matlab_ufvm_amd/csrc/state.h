@@ -114,11 +114,16 @@ struct RefLevel {
   bool low_ok = false;
   LowerLists low;
   DBuf<int32_t> lptr, lnb, lpji, lpij;
-  // multicolour GS: rows of colour c are mc_rows[mc_ptr[c] .. mc_ptr[c+1]),
-  // increasing index; built on first use
-  bool mc = false;
-  std::vector<int32_t> mc_ptr;
-  DBuf<int32_t> mc_rows;
+  // multicolour GS: the classes of a greedy colouring (rows in increasing
+  // index) as sliced-ELL chunks of 64 rows, built on first use (a chunk holds
+  // rows of one colour; cs_row = -1 pads a colour to whole chunks): colour c
+  // is chunks [cs_cptr[c], cs_cptr[c+1]); entries in cconn order, padding
+  // col = -1; values gathered from anb through cs_src
+  bool cs = false;
+  std::vector<int32_t> cs_cptr;
+  DBuf<int32_t> cs_row, cs_off, cs_col, cs_src;
+  DBuf<double> cs_val;
+  int64_t cs_slots = 0;
 };
 
 // One level of the precision-form V-cycle: unknowns renumbered so that each

@@ -103,6 +103,33 @@ def test_solve_algebraic_system_bitexact(ctx, case, smoother):
     ctx.system_destroy(sid)
 
 
+def test_mcgs_after_reagglomeration_uses_the_new_pattern(ctx):
+    # a coarse level rebuilt with another pattern gets a new colouring and new
+    # colour chunks (solve, re-agglomerate from other coefficients, solve)
+    m = cavity_mesh(10, jitter=0.15, seed=51, renumber_seed=52)
+    ptr, col, ac, anb = poisson_system(m, dirichlet=("movingWall",))
+    b = gaussian_rhs(m, seed=53)
+    sid = make(ctx, ptr, col, ac, anb, b)
+    ctx.agglomerate(sid, 10)
+    first = ctx.level_get(sid, 1)
+    ctx.set_bc(sid, 1, np.ones(first["ac"].size))
+    ctx.solve_algebraic_system(sid, 1, "MCGS", 1, rrf=0.0)
+    # strengthen every third coupling: a different aggregation
+    anb2 = anb * np.where(np.arange(anb.size) % 3 == 0, 4.0, 1.0)
+    ctx.set_coefficients(sid, ac, anb2, b)
+    ctx.agglomerate(sid, 10)
+    S = oracle.System(ptr, col, ac, anb2, bc=b)
+    S.agglomerate(10)
+    second = S.level(1)
+    assert second["n"] != first["ac"].size or not np.array_equal(second["col"], first["col"])
+    ctx.set_bc(sid, 1, np.ones(second["n"]))
+    np.ctypeslib.as_array(S.H.lev[1].bc, (second["n"],))[:] = 1.0
+    ctx.solve_algebraic_system(sid, 1, "MCGS", 2, rrf=0.0)
+    S.solve_algebraic_system(1, "MCGS", 2, rrf=0.0)
+    assert np.array_equal(ctx.get_dphi(sid, 1), S.level(1)["dphi"])
+    ctx.system_destroy(sid)
+
+
 @pytest.mark.parametrize("cycle,smoother,pre,post", [("V-Cycle", "SOR", 2, 2),
                                                      ("W-Cycle", "ILU", 1, 3),
                                                      ("F-Cycle", "SOR", 1, 3)])
