@@ -14,6 +14,16 @@ constexpr int kFcgBlock = 1024;    // threads per block of the FCG kernels
 constexpr int kSweepBlock = 1024;  // threads of the one-workgroup level-scheduled sweeps
 inline int nblocks(int64_t n, int b = kBlock) { return (int)((n + b - 1) / b); }
 
+// The GPU hands workgroup b to XCD b % 8 (each XCD has its own L2).  This
+// maps it to the b-th of a contiguous share of the grid per XCD instead, so
+// that rows next to each other -- and the neighbour values they read -- meet
+// in one L2.  A bijection of [0, G) for any G.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int32_t xcd_block(int32_t b, int32_t G) {
+  const int32_t q = G / kXcds, r = G % kXcds, x = b % kXcds, i = b / kXcds;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
 // ---- reference-order FP64 kernels (bit-exact with the reference arithmetic)
 // r = bc - ac*x - sum_k anb_k x_col(k) (cconn order); partial[blk] = sum |r|
 void residual_ref(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col,

@@ -538,7 +538,7 @@ __global__ void __launch_bounds__(kBlock)
 k_mcgs_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off,
             const int32_t *__restrict__ col, const V *__restrict__ val,
             const V *__restrict__ ac, const V *__restrict__ b, V *x, int32_t zero_above) {
-  const int32_t i = (r0 & ~63) + blockIdx.x * kBlock + threadIdx.x;
+  const int32_t i = (r0 & ~63) + xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i < r0 || i >= r1) return;
   const int32_t q = i >> 6;
   const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
@@ -572,7 +572,7 @@ k_mcilu_fwd_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off,
                  const int32_t *__restrict__ col, const V *__restrict__ val,
                  const V *__restrict__ ac, const V *__restrict__ b, const V *__restrict__ x,
                  const V *__restrict__ dinv, V *rc) {
-  const int32_t i = (r0 & ~63) + blockIdx.x * kBlock + threadIdx.x;
+  const int32_t i = (r0 & ~63) + xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i < r0 || i >= r1) return;
   const int32_t q = i >> 6;
   const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
@@ -612,7 +612,7 @@ __global__ void __launch_bounds__(kBlock)
 k_mcilu_bwd_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off,
                  const int32_t *__restrict__ col, const V *__restrict__ val,
                  const V *__restrict__ dinv, V *rc, V *x) {
-  const int32_t i = (r0 & ~63) + blockIdx.x * kBlock + threadIdx.x;
+  const int32_t i = (r0 & ~63) + xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i < r0 || i >= r1) return;
   const int32_t q = i >> 6;
   const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
@@ -701,7 +701,7 @@ k_residual_sell(int32_t n, int32_t zero_rows, const int32_t *__restrict__ off,
                 const int32_t *__restrict__ col, const V *__restrict__ val,
                 const V *__restrict__ ac, const V *__restrict__ b, const V *__restrict__ x,
                 V *__restrict__ r) {
-  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const int32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= n) return;
   if (i < zero_rows) {
     r[i] = V(0);
@@ -765,7 +765,7 @@ k_jacobi_sell(int32_t n, const int32_t *__restrict__ off, const int32_t *__restr
               const V *__restrict__ val, const V *__restrict__ ac, const V *__restrict__ b,
               const V *__restrict__ x, const int32_t *__restrict__ parent,
               const V *__restrict__ xc, V *__restrict__ out, V omega) {
-  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const int32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i >= n) return;
   jacobi_row<V, PRO>(i, off, col, val, ac, b, x, parent, xc, out, omega);
 }
@@ -886,7 +886,7 @@ __global__ void __launch_bounds__(kBlock)
 k_restrict_sell(int32_t nc, const int32_t *__restrict__ off, const int32_t *__restrict__ child,
                 const V *__restrict__ r, const V *__restrict__ acc, V *__restrict__ bc,
                 V *__restrict__ xc, V omega, V scale) {
-  const int32_t p = blockIdx.x * kBlock + threadIdx.x;
+  const int32_t p = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (p >= nc) return;
   restrict_row<V, JAC>(p, off, child, r, acc, bc, xc, omega, scale);
 }
