@@ -101,6 +101,11 @@ void ensure_schedule(ufvm_ctx *ctx, RefLevel &L) {
 }
 
 void sor_sweep(ufvm_ctx *ctx, RefLevel &L) {
+  if (L.one_wg && L.n <= k::kLdsRows) {
+    k::sor_lds(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.bwd_rows.p, L.bwd_lp.p,
+               nlev(L.bwd), L.n, L.ptr.p, L.col.p, L.ac.p, L.anb.p, L.bc.p, L.dphi.p);
+    return;
+  }
   if (L.one_wg) {
     k::gs_sweep_block(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.ptr.p, L.col.p,
                       L.ac.p, L.anb.p, L.bc.p, L.dphi.p);
@@ -201,6 +206,11 @@ void dilu_factor(ufvm_ctx *ctx, RefLevel &L) {
   if (!L.low_ok) throw Error(UFVM_ERR_NUMERIC, "ILU: the index for i in j is not found");
   const auto &F = L.fwd.level_ptr;
   const int32_t skip = L.n - 2;  // cfdFactorizeILU.m:21 skips 1-based row N-1
+  if (L.one_wg && L.n <= k::kLdsRows) {
+    k::dilu_factor_lds(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.n, L.ac.p, L.anb.p,
+                       L.lptr.p, L.lnb.p, L.lpji.p, L.lpij.p, skip, L.dinv.p);
+    return;
+  }
   if (L.one_wg) {
     k::dilu_factor_block(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.ac.p, L.anb.p,
                          L.lptr.p, L.lnb.p, L.lpji.p, L.lpij.p, skip, L.dinv.p);
@@ -215,6 +225,12 @@ void dilu_sweep(ufvm_ctx *ctx, RefLevel &L) {
   ensure_partial(ctx, L.n, 1);
   k::residual_ref(ctx->stream, L.n, L.ptr.p, L.col.p, L.ac.p, L.anb.p, L.bc.p, L.dphi.p, L.rc.p,
                   ctx->partial.p);
+  if (L.one_wg && L.n <= k::kLdsRows) {
+    k::dilu_lds(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.bwd_rows.p, L.bwd_lp.p,
+                nlev(L.bwd), L.n, L.ptr.p, L.col.p, L.anb.p, L.lptr.p, L.lnb.p, L.lpji.p,
+                L.dinv.p, L.rc.p, L.dphi.p);
+    return;
+  }
   if (L.one_wg) {
     k::dilu_fwd_block<double>(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.anb.p,
                               L.lptr.p, L.lnb.p, L.lpji.p, L.dinv.p, L.rc.p);

@@ -41,6 +41,19 @@ void gs_rows(hipStream_t s, const int32_t *rows, int32_t count, const int32_t *p
 void gs_sweep_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
                     const int32_t *ptr, const int32_t *col, const double *ac, const double *anb,
                     const double *b, double *x);
+// the same with the level's unknowns (x, rc, dinv) in LDS: n <= kLdsRows
+constexpr int32_t kLdsRows = 8192;
+void sor_lds(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf, const int32_t *br,
+             const int32_t *blp, int32_t nbk, int32_t n, const int32_t *ptr, const int32_t *col,
+             const double *ac, const double *anb, const double *b, double *x);
+void dilu_lds(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf, const int32_t *br,
+              const int32_t *blp, int32_t nbk, int32_t n, const int32_t *ptr, const int32_t *col,
+              const double *anb, const int32_t *lptr, const int32_t *lnb, const int32_t *lpji,
+              const double *dinv, double *rc, double *x);
+void dilu_factor_lds(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
+                     int32_t n, const double *ac, const double *anb, const int32_t *lptr,
+                     const int32_t *lnb, const int32_t *lpji, const int32_t *lpij,
+                     int32_t skip_row, double *dinv);
 void dilu_factor_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
                        const double *ac, const double *anb, const int32_t *lptr,
                        const int32_t *lnb, const int32_t *lpji, const int32_t *lpij,

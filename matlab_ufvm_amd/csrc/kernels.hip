@@ -186,6 +186,129 @@ k_gs_sweep_block(const int32_t *__restrict__ rows, const int32_t *__restrict__ l
   }
 }
 
+// The forward then backward sweep (SOR) of a level whose n unknowns fit in
+// LDS: x is loaded once, every level reads and writes it there (the level
+// chain runs at LDS latency), and it is stored back at the end.  Row
+// arithmetic as gs_row.
+__global__ void __launch_bounds__(kSweepBlock)
+k_sor_lds(const int32_t *__restrict__ fr, const int32_t *__restrict__ flp, int32_t nf,
+          const int32_t *__restrict__ br, const int32_t *__restrict__ blp, int32_t nbk, int32_t n,
+          const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+          const double *__restrict__ ac, const double *__restrict__ anb,
+          const double *__restrict__ b, double *__restrict__ x) {
+  extern __shared__ double xs[];
+  for (int32_t i = threadIdx.x; i < n; i += kSweepBlock) xs[i] = x[i];
+  __syncthreads();
+  auto sweep = [&](const int32_t *rows, const int32_t *lp, int32_t nlev) {
+    for (int32_t q = 0; q < nlev; ++q) {
+      for (int32_t t = lp[q] + threadIdx.x; t < lp[q + 1]; t += kSweepBlock) {
+        const int32_t i = rows[t];
+        double v = b[i];
+        for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) v = v - anb[k] * xs[col[k]];
+        xs[i] = v / ac[i];
+      }
+      __syncthreads();
+    }
+  };
+  sweep(fr, flp, nf);
+  sweep(br, blp, nbk);
+  for (int32_t i = threadIdx.x; i < n; i += kSweepBlock) x[i] = xs[i];
+}
+
+void sor_lds(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf, const int32_t *br,
+             const int32_t *blp, int32_t nbk, int32_t n, const int32_t *ptr, const int32_t *col,
+             const double *ac, const double *anb, const double *b, double *x) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_sor_lds, dim3(1), dim3(kSweepBlock), (size_t)n * sizeof(double), s, fr, flp,
+                     nf, br, blp, nbk, n, ptr, col, ac, anb, b, x);
+}
+
+// DILU forward and backward substitution (cfdSolveILU's, as dilu_fwd_row /
+// dilu_bwd_row) of a level whose rc fits in LDS; x += z in memory
+__global__ void __launch_bounds__(kSweepBlock)
+k_dilu_lds(const int32_t *__restrict__ fr, const int32_t *__restrict__ flp, int32_t nf,
+           const int32_t *__restrict__ br, const int32_t *__restrict__ blp, int32_t nbk, int32_t n,
+           const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+           const double *__restrict__ anb, const int32_t *__restrict__ lptr,
+           const int32_t *__restrict__ lnb, const int32_t *__restrict__ lpji,
+           const double *__restrict__ dinv, double *__restrict__ rc, double *__restrict__ x) {
+  extern __shared__ double rs[];
+  for (int32_t i = threadIdx.x; i < n; i += kSweepBlock) rs[i] = rc[i];
+  __syncthreads();
+  for (int32_t q = 0; q < nf; ++q) {
+    for (int32_t t = flp[q] + threadIdx.x; t < flp[q + 1]; t += kSweepBlock) {
+      const int32_t j = fr[t];
+      double v = rs[j];
+      for (int32_t e = lptr[j]; e < lptr[j + 1]; ++e) {
+        const int32_t i = lnb[e];
+        const double mat1 = dinv[i] * rs[i];
+        v = v - anb[lpji[e]] * mat1;
+      }
+      rs[j] = v;
+    }
+    __syncthreads();
+  }
+  for (int32_t q = 0; q < nbk; ++q) {
+    for (int32_t t = blp[q] + threadIdx.x; t < blp[q + 1]; t += kSweepBlock) {
+      const int32_t i = br[t];
+      double v = rs[i];
+      if (i < n - 1) {
+        for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+          const int32_t j = col[k];
+          if (j > i) v = v - anb[k] * rs[j];
+        }
+      }
+      const double z = dinv[i] * v;
+      rs[i] = z;
+      x[i] = x[i] + z;
+    }
+    __syncthreads();
+  }
+  for (int32_t i = threadIdx.x; i < n; i += kSweepBlock) rc[i] = rs[i];
+}
+
+void dilu_lds(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf, const int32_t *br,
+              const int32_t *blp, int32_t nbk, int32_t n, const int32_t *ptr, const int32_t *col,
+              const double *anb, const int32_t *lptr, const int32_t *lnb, const int32_t *lpji,
+              const double *dinv, double *rc, double *x) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_dilu_lds, dim3(1), dim3(kSweepBlock), (size_t)n * sizeof(double), s, fr,
+                     flp, nf, br, blp, nbk, n, ptr, col, anb, lptr, lnb, lpji, dinv, rc, x);
+}
+
+// DILU factor (dilu_factor_row) of a level whose dinv fits in LDS
+__global__ void __launch_bounds__(kSweepBlock)
+k_dilu_factor_lds(const int32_t *__restrict__ rows, const int32_t *__restrict__ lp, int32_t nlev,
+                  int32_t n, const double *__restrict__ ac, const double *__restrict__ anb,
+                  const int32_t *__restrict__ lptr, const int32_t *__restrict__ lnb,
+                  const int32_t *__restrict__ lpji, const int32_t *__restrict__ lpij,
+                  int32_t skip_row, double *__restrict__ dinv) {
+  extern __shared__ double ds[];
+  for (int32_t q = 0; q < nlev; ++q) {
+    for (int32_t t = lp[q] + threadIdx.x; t < lp[q + 1]; t += kSweepBlock) {
+      const int32_t j = rows[t];
+      double d = ac[j];
+      for (int32_t e = lptr[j]; e < lptr[j + 1]; ++e) {
+        const int32_t i = lnb[e];
+        if (i == skip_row) continue;
+        d = d - (anb[lpji[e]] * ds[i]) * anb[lpij[e]];
+      }
+      ds[j] = 1.0 / d;
+    }
+    __syncthreads();
+  }
+  for (int32_t i = threadIdx.x; i < n; i += kSweepBlock) dinv[i] = ds[i];
+}
+
+void dilu_factor_lds(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
+                     int32_t n, const double *ac, const double *anb, const int32_t *lptr,
+                     const int32_t *lnb, const int32_t *lpji, const int32_t *lpij,
+                     int32_t skip_row, double *dinv) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_dilu_factor_lds, dim3(1), dim3(kSweepBlock), (size_t)n * sizeof(double), s,
+                     rows, lp, nlev, n, ac, anb, lptr, lnb, lpji, lpij, skip_row, dinv);
+}
+
 void gs_sweep_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
                     const int32_t *ptr, const int32_t *col, const double *ac, const double *anb,
                     const double *b, double *x) {
