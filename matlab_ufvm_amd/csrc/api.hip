@@ -100,27 +100,27 @@ void ensure_schedule(ufvm_ctx *ctx, RefLevel &L) {
   L.sched = true;
 }
 
-void sor_sweep(ufvm_ctx *ctx, RefLevel &L) {
+void sor_sweep(ufvm_ctx *ctx, RefLevel &L, const double *stop = nullptr) {
   if (L.one_wg && L.n <= k::kLdsRows) {
     k::sor_lds(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.bwd_rows.p, L.bwd_lp.p,
-               nlev(L.bwd), L.n, L.ptr.p, L.col.p, L.ac.p, L.anb.p, L.bc.p, L.dphi.p);
+               nlev(L.bwd), L.n, L.ptr.p, L.col.p, L.ac.p, L.anb.p, L.bc.p, L.dphi.p, stop);
     return;
   }
   if (L.one_wg) {
     k::gs_sweep_block(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.ptr.p, L.col.p,
-                      L.ac.p, L.anb.p, L.bc.p, L.dphi.p);
+                      L.ac.p, L.anb.p, L.bc.p, L.dphi.p, stop);
     k::gs_sweep_block(ctx->stream, L.bwd_rows.p, L.bwd_lp.p, nlev(L.bwd), L.ptr.p, L.col.p,
-                      L.ac.p, L.anb.p, L.bc.p, L.dphi.p);
+                      L.ac.p, L.anb.p, L.bc.p, L.dphi.p, stop);
     return;
   }
   const auto &F = L.fwd.level_ptr;
   for (size_t l = 0; l + 1 < F.size(); ++l)
     k::gs_rows(ctx->stream, L.fwd_rows.p + F[l], F[l + 1] - F[l], L.ptr.p, L.col.p, L.ac.p,
-               L.anb.p, L.bc.p, L.dphi.p);
+               L.anb.p, L.bc.p, L.dphi.p, stop);
   const auto &B = L.bwd.level_ptr;
   for (size_t l = 0; l + 1 < B.size(); ++l)
     k::gs_rows(ctx->stream, L.bwd_rows.p + B[l], B[l + 1] - B[l], L.ptr.p, L.col.p, L.ac.p,
-               L.anb.p, L.bc.p, L.dphi.p);
+               L.anb.p, L.bc.p, L.dphi.p, stop);
 }
 
 // Colour classes of a greedy first-fit colouring (index order) and their
@@ -191,12 +191,12 @@ static double residual_l1_chunks(ufvm_ctx *ctx, RefLevel &L) {
 // downward without the last class (re-solving it repeats its arithmetic).
 // Each row is the reference's Gauss-Seidel row update (k_gs_rows' arithmetic,
 // on the colour chunks; cs_val must hold the current anb).
-void mcgs_sweep(ufvm_ctx *ctx, RefLevel &L) {
+void mcgs_sweep(ufvm_ctx *ctx, RefLevel &L, const double *stop = nullptr) {
   ensure_colour_chunks(ctx, L);
   const int32_t nc = (int32_t)L.cs_cptr.size() - 1;
   auto pass = [&](int32_t c) {
     k::gs_csell(ctx->stream, L.cs_cptr[c], L.cs_cptr[c + 1], L.cs_row.p, L.cs_off.p, L.cs_col.p,
-                L.cs_val.p, L.ac.p, L.bc.p, L.dphi.p);
+                L.cs_val.p, L.ac.p, L.bc.p, L.dphi.p, stop);
   };
   for (int32_t c = 0; c < nc; ++c) pass(c);
   for (int32_t c = nc - 2; c >= 0; --c) pass(c);
@@ -221,36 +221,67 @@ void dilu_factor(ufvm_ctx *ctx, RefLevel &L) {
                         L.lptr.p, L.lnb.p, L.lpji.p, L.lpij.p, skip, L.dinv.p);
 }
 
-void dilu_sweep(ufvm_ctx *ctx, RefLevel &L) {
+void dilu_sweep(ufvm_ctx *ctx, RefLevel &L, const double *stop = nullptr) {
   ensure_partial(ctx, L.n, 1);
   k::residual_ref(ctx->stream, L.n, L.ptr.p, L.col.p, L.ac.p, L.anb.p, L.bc.p, L.dphi.p, L.rc.p,
-                  ctx->partial.p);
+                  ctx->partial.p, stop);
   if (L.one_wg && L.n <= k::kLdsRows) {
     k::dilu_lds(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.bwd_rows.p, L.bwd_lp.p,
                 nlev(L.bwd), L.n, L.ptr.p, L.col.p, L.anb.p, L.lptr.p, L.lnb.p, L.lpji.p,
-                L.dinv.p, L.rc.p, L.dphi.p);
+                L.dinv.p, L.rc.p, L.dphi.p, stop);
     return;
   }
   if (L.one_wg) {
     k::dilu_fwd_block<double>(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.anb.p,
-                              L.lptr.p, L.lnb.p, L.lpji.p, L.dinv.p, L.rc.p);
+                              L.lptr.p, L.lnb.p, L.lpji.p, L.dinv.p, L.rc.p, stop);
     k::dilu_bwd_block<double>(ctx->stream, L.bwd_rows.p, L.bwd_lp.p, nlev(L.bwd), L.n, L.ptr.p,
-                              L.col.p, L.anb.p, L.dinv.p, L.rc.p, L.dphi.p);
+                              L.col.p, L.anb.p, L.dinv.p, L.rc.p, L.dphi.p, stop);
     return;
   }
   const auto &F = L.fwd.level_ptr;
   for (size_t l = 0; l + 1 < F.size(); ++l)
     k::dilu_fwd_rows(ctx->stream, L.fwd_rows.p + F[l], F[l + 1] - F[l], L.anb.p, L.lptr.p,
-                     L.lnb.p, L.lpji.p, L.dinv.p, L.rc.p);
+                     L.lnb.p, L.lpji.p, L.dinv.p, L.rc.p, stop);
   const auto &B = L.bwd.level_ptr;
   for (size_t l = 0; l + 1 < B.size(); ++l)
     k::dilu_bwd_rows(ctx->stream, L.bwd_rows.p + B[l], B[l + 1] - B[l], L.n, L.ptr.p, L.col.p,
-                     L.anb.p, L.dinv.p, L.rc.p, L.dphi.p);
+                     L.anb.p, L.dinv.p, L.rc.p, L.dphi.p, stop);
+}
+
+// cfdSolveAlgebraicSystem.m:11-66 with the loop test on the device: the
+// same sweeps and residuals, each launched n_iter times, and every kernel
+// after the stop flag is set returns at once.  A NaN initial residual is
+// recorded (alg[3]) and reported by the cycle loop (check_alg_nan).
+static void solve_alg_device(ufvm_ctx *ctx, RefLevel &L, int smoother, int n_iter, double rrf,
+                             bool chunks) {
+  hipStream_t s = ctx->stream;
+  double *st = ctx->alg.p;
+  auto residual = [&](const double *stop) -> int32_t {
+    if (chunks) {
+      ensure_partial(ctx, (int32_t)std::max<int64_t>(L.cs_slots, 1), 1);
+      k::residual_csell(s, L.cs_slots, L.cs_row.p, L.cs_off.p, L.cs_col.p, L.cs_val.p, L.ac.p,
+                        L.bc.p, L.dphi.p, L.r.p, ctx->partial.p, stop);
+      return k::nblocks(L.cs_slots);
+    }
+    ensure_partial(ctx, L.n, 1);
+    k::residual_ref(s, L.n, L.ptr.p, L.col.p, L.ac.p, L.anb.p, L.bc.p, L.dphi.p, L.r.p,
+                    ctx->partial.p, stop);
+    return k::nblocks(L.n);
+  };
+  k::alg_test(s, ctx->partial.p, residual(nullptr), L.n, rrf, 0, st);
+  if (smoother == UFVM_SMOOTHER_ILU) dilu_factor(ctx, L);
+  const double *stop = st + 2;
+  for (int iter = 1; iter <= n_iter; ++iter) {
+    if (smoother == UFVM_SMOOTHER_ILU) dilu_sweep(ctx, L, stop);
+    else if (smoother == UFVM_SMOOTHER_SOR) sor_sweep(ctx, L, stop);
+    else mcgs_sweep(ctx, L, stop);
+    k::alg_test(s, ctx->partial.p, residual(stop), L.n, rrf, 1, st);
+  }
 }
 
 // cfdSolveAlgebraicSystem.m:11-66
 void solve_alg(ufvm_ctx *ctx, RefLevel &L, int smoother, int n_iter, double rrf, double *r0,
-               double *r1) {
+               double *r1, bool device_test) {
   if (smoother != UFVM_SMOOTHER_ILU && smoother != UFVM_SMOOTHER_SOR &&
       smoother != UFVM_SMOOTHER_MCGS)
     throw Error(UFVM_ERR_ARG, "smoother must be ILU, SOR or MCGS");
@@ -259,6 +290,12 @@ void solve_alg(ufvm_ctx *ctx, RefLevel &L, int smoother, int n_iter, double rrf,
   if (chunks) {
     ensure_colour_chunks(ctx, L);
     k::gather_vals<double>(ctx->stream, (int64_t)L.cs_val.n, L.cs_src.p, L.anb.p, L.cs_val.p);
+  }
+  if (device_test) {
+    // a smoothing call inside an AMG cycle: nobody reads its residuals, so
+    // the stopping test runs on the device and the host does not wait
+    solve_alg_device(ctx, L, smoother, n_iter, rrf, chunks);
+    return;
   }
   auto l1 = [&]() { return chunks ? residual_l1_chunks(ctx, L) : residual_l1(ctx, L, L.r.p); };
   const double initial = l1() / L.n;
@@ -331,7 +368,8 @@ RefLevel &L1(Cyc &c, int level) { return *c.S->lev[level - 1]; }  // 1-based
 // cfdRestrict.m:5-12
 void cyc_restrict(Cyc &c, int level, bool presmooth) {
   RefLevel &F = L1(c, level);
-  if (presmooth) solve_alg(c.ctx, F, c.cfg->smoother, c.cfg->pre_sweep, 0.1, nullptr, nullptr);
+  if (presmooth)
+    solve_alg(c.ctx, F, c.cfg->smoother, c.cfg->pre_sweep, 0.1, nullptr, nullptr, true);
   ensure_partial(c.ctx, F.n, 1);
   k::residual_ref(c.ctx->stream, F.n, F.ptr.p, F.col.p, F.ac.p, F.anb.p, F.bc.p, F.dphi.p, F.r.p,
                   c.ctx->partial.p);
@@ -344,11 +382,13 @@ void cyc_prolong(Cyc &c, int level) {
   RefLevel &F = L1(c, level - 1);
   RefLevel &C = L1(c, level);
   k::prolong_ref(c.ctx->stream, F.n, F.parents.p, C.dphi.p, F.dphi.p);
-  solve_alg(c.ctx, F, c.cfg->smoother, c.cfg->post_sweep, c.cfg->termination, nullptr, nullptr);
+  solve_alg(c.ctx, F, c.cfg->smoother, c.cfg->post_sweep, c.cfg->termination, nullptr, nullptr,
+            true);
 }
 
 void cyc_coarsest(Cyc &c, int level) {
-  solve_alg(c.ctx, L1(c, level), c.cfg->smoother, c.cfg->post_sweep, 0.1, nullptr, nullptr);
+  solve_alg(c.ctx, L1(c, level), c.cfg->smoother, c.cfg->post_sweep, 0.1, nullptr, nullptr,
+            true);
 }
 
 double cyc_final(Cyc &c) { return residual_l1(c.ctx, L1(c, 1), L1(c, 1).r.p); }
@@ -429,6 +469,15 @@ void apply_amg(ufvm_ctx *ctx, System &S, const ufvm_amg_cfg &cfg, double *r0, do
   if (cfg.cycle != UFVM_CYCLE_V && G < 4)
     throw Error(UFVM_ERR_UNSUPPORTED, "W- and F-cycles need at least 4 levels");
   Cyc c{ctx, &S, &cfg};
+  // the cycle's smoothing calls keep their stopping test on the device
+  // (solve_alg_device); a NaN initial residual there is reported here
+  if (!ctx->alg.p) ctx->alg.alloc(4);
+  UFVM_HIP(hipMemsetAsync(ctx->alg.p, 0, 4 * sizeof(double), ctx->stream));
+  auto check_nan = [&]() {
+    double seen = 0.0;
+    UFVM_HIP(hipMemcpy(&seen, ctx->alg.p + 3, sizeof(double), hipMemcpyDeviceToHost));
+    if (seen != 0.0) throw Error(UFVM_ERR_NUMERIC, "NaN residual");
+  };
   const double initial = cyc_final(c);
   double fin = initial;
   int32_t nCycle = 1;
@@ -436,6 +485,7 @@ void apply_amg(ufvm_ctx *ctx, System &S, const ufvm_amg_cfg &cfg, double *r0, do
     if (cfg.cycle == UFVM_CYCLE_V) fin = vcycle(c, G);
     else if (cfg.cycle == UFVM_CYCLE_W) fin = wcycle(c, G);
     else fin = fcycle(c, G);
+    check_nan();
     ++nCycle;
   }
   if (r0) *r0 = initial;

@@ -85,7 +85,7 @@ void ensure_partial(ufvm_ctx *ctx, int64_t n, int width);
 void read_scal(ufvm_ctx *ctx, int first, int count);
 int32_t agglomerate(ufvm_ctx *ctx, System &S, int32_t max_coarse);
 void solve_alg(ufvm_ctx *ctx, RefLevel &L, int smoother, int n_iter, double rrf, double *r0,
-               double *r1);
+               double *r1, bool device_test = false);
 void apply_amg(ufvm_ctx *ctx, System &S, const ufvm_amg_cfg &cfg, double *r0, double *r1,
                int32_t *ncycles);
 void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg);

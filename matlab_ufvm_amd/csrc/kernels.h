@@ -26,30 +26,34 @@ __device__ __forceinline__ int32_t xcd_block(int32_t b, int32_t G) {
 
 // ---- reference-order FP64 kernels (bit-exact with the reference arithmetic)
 // r = bc - ac*x - sum_k anb_k x_col(k) (cconn order); partial[blk] = sum |r|
+// device-side stopping test of a smoothing call: phase 0 (initial residual)
+// or 1 (after a sweep) on st[0..3] = initial, final, stop, NaN seen
+void alg_test(hipStream_t s, const double *partial, int32_t np, int32_t n, double rrf, int phase,
+              double *st);
 void residual_ref(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col,
                   const double *ac, const double *anb, const double *bc, const double *x,
-                  double *r, double *partial);
+                  double *r, double *partial, const double *stop = nullptr);
 // out[0] = sum(partial[0..np)) in a fixed order
 void reduce_final(hipStream_t s, const double *partial, int32_t np, double *out);
 // Gauss-Seidel update of the listed rows (one level of a lexicographic sweep)
 void gs_rows(hipStream_t s, const int32_t *rows, int32_t count, const int32_t *ptr,
              const int32_t *col, const double *ac, const double *anb, const double *b,
-             double *x);
+             double *x, const double *stop = nullptr);
 // A whole level-scheduled sweep in one workgroup (levels lp[q] .. lp[q+1]
 // of rows, device arrays, a barrier between levels): for schedules of many
 // small levels, one launch instead of one per level, same arithmetic
 void gs_sweep_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
                     const int32_t *ptr, const int32_t *col, const double *ac, const double *anb,
-                    const double *b, double *x);
+                    const double *b, double *x, const double *stop = nullptr);
 // the same with the level's unknowns (x, rc, dinv) in LDS: n <= kLdsRows
 constexpr int32_t kLdsRows = 8192;
 void sor_lds(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf, const int32_t *br,
              const int32_t *blp, int32_t nbk, int32_t n, const int32_t *ptr, const int32_t *col,
-             const double *ac, const double *anb, const double *b, double *x);
+             const double *ac, const double *anb, const double *b, double *x, const double *stop = nullptr);
 void dilu_lds(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf, const int32_t *br,
               const int32_t *blp, int32_t nbk, int32_t n, const int32_t *ptr, const int32_t *col,
               const double *anb, const int32_t *lptr, const int32_t *lnb, const int32_t *lpji,
-              const double *dinv, double *rc, double *x);
+              const double *dinv, double *rc, double *x, const double *stop = nullptr);
 void dilu_factor_lds(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
                      int32_t n, const double *ac, const double *anb, const int32_t *lptr,
                      const int32_t *lnb, const int32_t *lpji, const int32_t *lpij,
@@ -61,11 +65,11 @@ void dilu_factor_block(hipStream_t s, const int32_t *rows, const int32_t *lp, in
 template <typename V>
 void dilu_fwd_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
                     const V *anb, const int32_t *lptr, const int32_t *lnb, const int32_t *lpji,
-                    const V *dinv, V *rc);
+                    const V *dinv, V *rc, const double *stop = nullptr);
 template <typename V>
 void dilu_bwd_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
                     int32_t n, const int32_t *ptr, const int32_t *col, const V *anb,
-                    const V *dinv, V *rc, V *x);
+                    const V *dinv, V *rc, V *x, const double *stop = nullptr);
 // DILU in colour order on a sliced-ELL level whose colours are contiguous
 // row ranges: forward pass of the colour rows [r0, r1) (rc from b - A x and
 // the earlier colours; zero: x = 0) and backward pass (z from the later
@@ -86,11 +90,11 @@ void dilu_factor_rows(hipStream_t s, const int32_t *rows, int32_t count, const d
 template <typename V>
 void dilu_fwd_rows(hipStream_t s, const int32_t *rows, int32_t count, const V *anb,
                    const int32_t *lptr, const int32_t *lnb, const int32_t *lpji, const V *dinv,
-                   V *rc);
+                   V *rc, const double *stop = nullptr);
 template <typename V>
 void dilu_bwd_rows(hipStream_t s, const int32_t *rows, int32_t count, int32_t n,
                    const int32_t *ptr, const int32_t *col, const V *anb, const V *dinv, V *rc,
-                   V *x);
+                   V *x, const double *stop = nullptr);
 // bc_c[p] = sum over children (increasing fine index) of r
 void restrict_ref(hipStream_t s, int32_t nc, const int32_t *cptr, const int32_t *child,
                   const double *r, double *bc_c);
@@ -100,10 +104,10 @@ void prolong_ref(hipStream_t s, int32_t n, const int32_t *parent, const double *
 // every row (per-block |r| sums) and Gauss-Seidel over the chunks [q0, q1)
 void residual_csell(hipStream_t s, int64_t nslot, const int32_t *row, const int32_t *off,
                     const int32_t *col, const double *val, const double *ac, const double *bc,
-                    const double *x, double *r, double *partial);
+                    const double *x, double *r, double *partial, const double *stop = nullptr);
 void gs_csell(hipStream_t s, int32_t q0, int32_t q1, const int32_t *row, const int32_t *off,
               const int32_t *col, const double *val, const double *ac, const double *b,
-              double *x);
+              double *x, const double *stop = nullptr);
 // Galerkin coarse operator in the reference accumulation order, from the
 // gather lists of galerkin_lists (nout = nc + coarse nnz values)
 void coarse_lhs(hipStream_t s, int32_t nout, int32_t nc, const int32_t *ptr, const int32_t *src,

@@ -27,7 +27,8 @@ __global__ void __launch_bounds__(kBlock)
 k_residual_ref(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
                const double *__restrict__ ac, const double *__restrict__ anb,
                const double *__restrict__ bc, const double *__restrict__ x,
-               double *__restrict__ r, double *__restrict__ partial) {
+               double *__restrict__ r, double *__restrict__ partial, const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;  // stopped by the device-side test
   __shared__ double lds[kBlock / kWave];
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   double a = 0.0;
@@ -78,7 +79,8 @@ k_residual_csell(int64_t nslot, const int32_t *__restrict__ row, const int32_t *
                  const int32_t *__restrict__ col, const double *__restrict__ val,
                  const double *__restrict__ ac, const double *__restrict__ bc,
                  const double *__restrict__ x, double *__restrict__ r,
-                 double *__restrict__ partial) {
+                 double *__restrict__ partial, const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;  // stopped by the device-side test
   __shared__ double lds[kBlock / kWave];
   const int64_t t = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double a = 0.0;
@@ -94,10 +96,10 @@ k_residual_csell(int64_t nslot, const int32_t *__restrict__ row, const int32_t *
 
 void residual_csell(hipStream_t s, int64_t nslot, const int32_t *row, const int32_t *off,
                     const int32_t *col, const double *val, const double *ac, const double *bc,
-                    const double *x, double *r, double *partial) {
+                    const double *x, double *r, double *partial, const double *stop) {
   if (nslot <= 0) return;
   hipLaunchKernelGGL(k_residual_csell, dim3(nblocks(nslot)), dim3(kBlock), 0, s, nslot, row, off,
-                     col, val, ac, bc, x, r, partial);
+                     col, val, ac, bc, x, r, partial, stop);
 }
 
 // Gauss-Seidel rows of the chunks [q0, q1) (one colour class)
@@ -105,7 +107,8 @@ __global__ void __launch_bounds__(kBlock)
 k_gs_csell(int32_t q0, int32_t q1, const int32_t *__restrict__ row,
            const int32_t *__restrict__ off, const int32_t *__restrict__ col,
            const double *__restrict__ val, const double *__restrict__ ac,
-           const double *__restrict__ b, double *x) {
+           const double *__restrict__ b, double *x, const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;  // stopped by the device-side test
   const int32_t t = q0 * 64 + blockIdx.x * kBlock + threadIdx.x;
   if (t >= q1 * 64) return;
   const int32_t i = row[t];
@@ -115,18 +118,51 @@ k_gs_csell(int32_t q0, int32_t q1, const int32_t *__restrict__ row,
 
 void gs_csell(hipStream_t s, int32_t q0, int32_t q1, const int32_t *row, const int32_t *off,
               const int32_t *col, const double *val, const double *ac, const double *b,
-              double *x) {
+              double *x, const double *stop) {
   if (q1 <= q0) return;
   hipLaunchKernelGGL(k_gs_csell, dim3(nblocks((int64_t)(q1 - q0) * 64)), dim3(kBlock), 0, s, q0,
-                     q1, row, off, col, val, ac, b, x);
+                     q1, row, off, col, val, ac, b, x, stop);
 }
 
 void residual_ref(hipStream_t s, int32_t n, const int32_t *ptr, const int32_t *col,
                   const double *ac, const double *anb, const double *bc, const double *x,
-                  double *r, double *partial) {
+                  double *r, double *partial, const double *stop) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_residual_ref, dim3(nblocks(n)), dim3(kBlock), 0, s, n, ptr, col, ac, anb,
-                     bc, x, r, partial);
+                     bc, x, r, partial, stop);
+}
+
+// Device-side stopping test of a smoothing call (mean |r| after each sweep,
+// cfdSolveAlgebraicSystem.m:25-27): sums the residual kernel's per-block |r|
+// partials as reduce_final does, then
+//   phase 0 (initial): st[0] = st[1] = m, st[2] = !(m > rrf m), st[3] |= isnan(m)
+//   phase 1 (after a sweep; nothing once st[2] is set): st[1] = m,
+//           st[2] = !(m > rrf st[0])
+// with m = sum / n.  st[2] is the stop flag the sweep kernels test.
+__global__ void __launch_bounds__(kBlock)
+k_alg_test(const double *__restrict__ partial, int32_t np, int32_t n, double rrf, int phase,
+           double *st) {
+  __shared__ double lds[kBlock / kWave];
+  if (phase == 1 && st[2] != 0.0) return;
+  double v = 0.0;
+  for (int32_t b = threadIdx.x; b < np; b += kBlock) v += partial[b];
+  v = block_sum(v, lds);
+  if (threadIdx.x == 0) {
+    const double m = v / n;
+    st[1] = m;
+    if (phase == 0) {
+      st[0] = m;
+      st[2] = (m > rrf * m) ? 0.0 : 1.0;
+      if (isnan(m)) st[3] = 1.0;
+    } else if (!(m > rrf * st[0])) {
+      st[2] = 1.0;
+    }
+  }
+}
+
+void alg_test(hipStream_t s, const double *partial, int32_t np, int32_t n, double rrf, int phase,
+              double *st) {
+  hipLaunchKernelGGL(k_alg_test, dim3(1), dim3(kBlock), 0, s, partial, np, n, rrf, phase, st);
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -164,7 +200,8 @@ __device__ __forceinline__ void gs_row(int32_t i, const int32_t *__restrict__ pt
 __global__ void __launch_bounds__(kBlock)
 k_gs_rows(const int32_t *__restrict__ rows, int32_t count, const int32_t *__restrict__ ptr,
           const int32_t *__restrict__ col, const double *__restrict__ ac,
-          const double *__restrict__ anb, const double *__restrict__ b, double *x) {
+          const double *__restrict__ anb, const double *__restrict__ b, double *x, const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;  // stopped by the device-side test
   const int32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= count) return;
   gs_row(rows[t], ptr, col, ac, anb, b, x);
@@ -178,7 +215,8 @@ __global__ void __launch_bounds__(kSweepBlock)
 k_gs_sweep_block(const int32_t *__restrict__ rows, const int32_t *__restrict__ lp, int32_t nlev,
                  const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
                  const double *__restrict__ ac, const double *__restrict__ anb, const double *b,
-                 double *x) {
+                 double *x, const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;  // stopped by the device-side test
   for (int32_t q = 0; q < nlev; ++q) {
     for (int32_t t = lp[q] + threadIdx.x; t < lp[q + 1]; t += kSweepBlock)
       gs_row(rows[t], ptr, col, ac, anb, b, x);
@@ -195,7 +233,8 @@ k_sor_lds(const int32_t *__restrict__ fr, const int32_t *__restrict__ flp, int32
           const int32_t *__restrict__ br, const int32_t *__restrict__ blp, int32_t nbk, int32_t n,
           const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
           const double *__restrict__ ac, const double *__restrict__ anb,
-          const double *__restrict__ b, double *__restrict__ x) {
+          const double *__restrict__ b, double *__restrict__ x, const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;  // stopped by the device-side test
   extern __shared__ double xs[];
   for (int32_t i = threadIdx.x; i < n; i += kSweepBlock) xs[i] = x[i];
   __syncthreads();
@@ -217,10 +256,10 @@ k_sor_lds(const int32_t *__restrict__ fr, const int32_t *__restrict__ flp, int32
 
 void sor_lds(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf, const int32_t *br,
              const int32_t *blp, int32_t nbk, int32_t n, const int32_t *ptr, const int32_t *col,
-             const double *ac, const double *anb, const double *b, double *x) {
+             const double *ac, const double *anb, const double *b, double *x, const double *stop) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_sor_lds, dim3(1), dim3(kSweepBlock), (size_t)n * sizeof(double), s, fr, flp,
-                     nf, br, blp, nbk, n, ptr, col, ac, anb, b, x);
+                     nf, br, blp, nbk, n, ptr, col, ac, anb, b, x, stop);
 }
 
 // DILU forward and backward substitution (cfdSolveILU's, as dilu_fwd_row /
@@ -231,7 +270,8 @@ k_dilu_lds(const int32_t *__restrict__ fr, const int32_t *__restrict__ flp, int3
            const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
            const double *__restrict__ anb, const int32_t *__restrict__ lptr,
            const int32_t *__restrict__ lnb, const int32_t *__restrict__ lpji,
-           const double *__restrict__ dinv, double *__restrict__ rc, double *__restrict__ x) {
+           const double *__restrict__ dinv, double *__restrict__ rc, double *__restrict__ x, const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;  // stopped by the device-side test
   extern __shared__ double rs[];
   for (int32_t i = threadIdx.x; i < n; i += kSweepBlock) rs[i] = rc[i];
   __syncthreads();
@@ -270,10 +310,10 @@ k_dilu_lds(const int32_t *__restrict__ fr, const int32_t *__restrict__ flp, int3
 void dilu_lds(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf, const int32_t *br,
               const int32_t *blp, int32_t nbk, int32_t n, const int32_t *ptr, const int32_t *col,
               const double *anb, const int32_t *lptr, const int32_t *lnb, const int32_t *lpji,
-              const double *dinv, double *rc, double *x) {
+              const double *dinv, double *rc, double *x, const double *stop) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_dilu_lds, dim3(1), dim3(kSweepBlock), (size_t)n * sizeof(double), s, fr,
-                     flp, nf, br, blp, nbk, n, ptr, col, anb, lptr, lnb, lpji, dinv, rc, x);
+                     flp, nf, br, blp, nbk, n, ptr, col, anb, lptr, lnb, lpji, dinv, rc, x, stop);
 }
 
 // DILU factor (dilu_factor_row) of a level whose dinv fits in LDS
@@ -311,18 +351,18 @@ void dilu_factor_lds(hipStream_t s, const int32_t *rows, const int32_t *lp, int3
 
 void gs_sweep_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
                     const int32_t *ptr, const int32_t *col, const double *ac, const double *anb,
-                    const double *b, double *x) {
+                    const double *b, double *x, const double *stop) {
   if (nlev <= 0) return;
   hipLaunchKernelGGL(k_gs_sweep_block, dim3(1), dim3(kSweepBlock), 0, s, rows, lp, nlev, ptr,
-                     col, ac, anb, b, x);
+                     col, ac, anb, b, x, stop);
 }
 
 void gs_rows(hipStream_t s, const int32_t *rows, int32_t count, const int32_t *ptr,
              const int32_t *col, const double *ac, const double *anb, const double *b,
-             double *x) {
+             double *x, const double *stop) {
   if (count <= 0) return;
   hipLaunchKernelGGL(k_gs_rows, dim3(nblocks(count)), dim3(kBlock), 0, s, rows, count, ptr, col,
-                     ac, anb, b, x);
+                     ac, anb, b, x, stop);
 }
 
 // d_j = ac_j - sum_{i<j, increasing i, i != skip} (a_ji * dinv_i) * a_ij ; dinv_j = 1/d_j
@@ -402,7 +442,8 @@ template <typename V>
 __global__ void __launch_bounds__(kBlock)
 k_dilu_fwd(const int32_t *__restrict__ rows, int32_t count, const V *__restrict__ anb,
            const int32_t *__restrict__ lptr, const int32_t *__restrict__ lnb,
-           const int32_t *__restrict__ lpji, const V *__restrict__ dinv, V *rc) {
+           const int32_t *__restrict__ lpji, const V *__restrict__ dinv, V *rc, const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;  // stopped by the device-side test
   const int32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= count) return;
   dilu_fwd_row<V>(rows[t], anb, lptr, lnb, lpji, dinv, rc);
@@ -413,7 +454,8 @@ __global__ void __launch_bounds__(kSweepBlock)
 k_dilu_fwd_block(const int32_t *__restrict__ rows, const int32_t *__restrict__ lp, int32_t nlev,
                  const V *__restrict__ anb, const int32_t *__restrict__ lptr,
                  const int32_t *__restrict__ lnb, const int32_t *__restrict__ lpji,
-                 const V *__restrict__ dinv, V *rc) {
+                 const V *__restrict__ dinv, V *rc, const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;  // stopped by the device-side test
   for (int32_t q = 0; q < nlev; ++q) {
     for (int32_t t = lp[q] + threadIdx.x; t < lp[q + 1]; t += kSweepBlock)
       dilu_fwd_row<V>(rows[t], anb, lptr, lnb, lpji, dinv, rc);
@@ -424,19 +466,19 @@ k_dilu_fwd_block(const int32_t *__restrict__ rows, const int32_t *__restrict__ l
 template <typename V>
 void dilu_fwd_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
                     const V *anb, const int32_t *lptr, const int32_t *lnb, const int32_t *lpji,
-                    const V *dinv, V *rc) {
+                    const V *dinv, V *rc, const double *stop) {
   if (nlev <= 0) return;
   hipLaunchKernelGGL(k_dilu_fwd_block<V>, dim3(1), dim3(kSweepBlock), 0, s, rows, lp, nlev, anb,
-                     lptr, lnb, lpji, dinv, rc);
+                     lptr, lnb, lpji, dinv, rc, stop);
 }
 
 template <typename V>
 void dilu_fwd_rows(hipStream_t s, const int32_t *rows, int32_t count, const V *anb,
                    const int32_t *lptr, const int32_t *lnb, const int32_t *lpji, const V *dinv,
-                   V *rc) {
+                   V *rc, const double *stop) {
   if (count <= 0) return;
   hipLaunchKernelGGL(k_dilu_fwd<V>, dim3(nblocks(count)), dim3(kBlock), 0, s, rows, count, anb,
-                     lptr, lnb, lpji, dinv, rc);
+                     lptr, lnb, lpji, dinv, rc, stop);
 }
 
 // rc_i -= a_ij rc_j (j > i, cconn order) ; z = dinv_i rc_i ; rc_i = z ; x_i += z
@@ -462,7 +504,8 @@ template <typename V>
 __global__ void __launch_bounds__(kBlock)
 k_dilu_bwd(const int32_t *__restrict__ rows, int32_t count, int32_t n,
            const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
-           const V *__restrict__ anb, const V *__restrict__ dinv, V *rc, V *x) {
+           const V *__restrict__ anb, const V *__restrict__ dinv, V *rc, V *x, const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;  // stopped by the device-side test
   const int32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= count) return;
   dilu_bwd_row<V>(rows[t], n, ptr, col, anb, dinv, rc, x);
@@ -472,7 +515,8 @@ template <typename V>
 __global__ void __launch_bounds__(kSweepBlock)
 k_dilu_bwd_block(const int32_t *__restrict__ rows, const int32_t *__restrict__ lp, int32_t nlev,
                  int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
-                 const V *__restrict__ anb, const V *__restrict__ dinv, V *rc, V *x) {
+                 const V *__restrict__ anb, const V *__restrict__ dinv, V *rc, V *x, const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;  // stopped by the device-side test
   for (int32_t q = 0; q < nlev; ++q) {
     for (int32_t t = lp[q] + threadIdx.x; t < lp[q + 1]; t += kSweepBlock)
       dilu_bwd_row<V>(rows[t], n, ptr, col, anb, dinv, rc, x);
@@ -483,45 +527,45 @@ k_dilu_bwd_block(const int32_t *__restrict__ rows, const int32_t *__restrict__ l
 template <typename V>
 void dilu_bwd_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
                     int32_t n, const int32_t *ptr, const int32_t *col, const V *anb,
-                    const V *dinv, V *rc, V *x) {
+                    const V *dinv, V *rc, V *x, const double *stop) {
   if (nlev <= 0) return;
   hipLaunchKernelGGL(k_dilu_bwd_block<V>, dim3(1), dim3(kSweepBlock), 0, s, rows, lp, nlev, n,
-                     ptr, col, anb, dinv, rc, x);
+                     ptr, col, anb, dinv, rc, x, stop);
 }
 
 template <typename V>
 void dilu_bwd_rows(hipStream_t s, const int32_t *rows, int32_t count, int32_t n,
                    const int32_t *ptr, const int32_t *col, const V *anb, const V *dinv, V *rc,
-                   V *x) {
+                   V *x, const double *stop) {
   if (count <= 0) return;
   hipLaunchKernelGGL(k_dilu_bwd<V>, dim3(nblocks(count)), dim3(kBlock), 0, s, rows, count, n,
-                     ptr, col, anb, dinv, rc, x);
+                     ptr, col, anb, dinv, rc, x, stop);
 }
 
 template void dilu_fwd_block<double>(hipStream_t, const int32_t *, const int32_t *, int32_t,
                                      const double *, const int32_t *, const int32_t *,
-                                     const int32_t *, const double *, double *);
+                                     const int32_t *, const double *, double *, const double *);
 template void dilu_fwd_block<float>(hipStream_t, const int32_t *, const int32_t *, int32_t,
                                     const float *, const int32_t *, const int32_t *,
-                                    const int32_t *, const float *, float *);
+                                    const int32_t *, const float *, float *, const double *);
 template void dilu_bwd_block<double>(hipStream_t, const int32_t *, const int32_t *, int32_t,
                                      int32_t, const int32_t *, const int32_t *, const double *,
-                                     const double *, double *, double *);
+                                     const double *, double *, double *, const double *);
 template void dilu_bwd_block<float>(hipStream_t, const int32_t *, const int32_t *, int32_t,
                                     int32_t, const int32_t *, const int32_t *, const float *,
-                                    const float *, float *, float *);
+                                    const float *, float *, float *, const double *);
 template void dilu_fwd_rows<double>(hipStream_t, const int32_t *, int32_t, const double *,
                                     const int32_t *, const int32_t *, const int32_t *,
-                                    const double *, double *);
+                                    const double *, double *, const double *);
 template void dilu_fwd_rows<float>(hipStream_t, const int32_t *, int32_t, const float *,
                                    const int32_t *, const int32_t *, const int32_t *,
-                                   const float *, float *);
+                                   const float *, float *, const double *);
 template void dilu_bwd_rows<double>(hipStream_t, const int32_t *, int32_t, int32_t,
                                     const int32_t *, const int32_t *, const double *,
-                                    const double *, double *, double *);
+                                    const double *, double *, double *, const double *);
 template void dilu_bwd_rows<float>(hipStream_t, const int32_t *, int32_t, int32_t,
                                    const int32_t *, const int32_t *, const float *,
-                                   const float *, float *, float *);
+                                   const float *, float *, float *, const double *);
 
 template <typename V>
 __global__ void __launch_bounds__(kBlock)

@@ -243,6 +243,7 @@ struct ufvm_ctx {
   int32_t next_id = 1;
   ufvm::DBuf<double> partial;   // per-block partial sums
   ufvm::DBuf<double> scal;      // device scalars
+  ufvm::DBuf<double> alg;       // smoothing calls inside AMG cycles: initial, final, stop, NaN
   ufvm::DBuf<unsigned> counter; // last-block reduction counter (kept at zero between launches)
   double *hscal = nullptr;      // pinned host mirror of scal
   // SIMPLE cases of this context (type-erased; owned here so that destroying
