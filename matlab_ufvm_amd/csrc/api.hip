@@ -474,8 +474,10 @@ void apply_amg(ufvm_ctx *ctx, System &S, const ufvm_amg_cfg &cfg, double *r0, do
   if (!ctx->alg.p) ctx->alg.alloc(4);
   UFVM_HIP(hipMemsetAsync(ctx->alg.p, 0, 4 * sizeof(double), ctx->stream));
   auto check_nan = [&]() {
-    double seen = 0.0;
-    UFVM_HIP(hipMemcpy(&seen, ctx->alg.p + 3, sizeof(double), hipMemcpyDeviceToHost));
+    double seen = 0.0;  // on the context's stream: never the null stream
+    UFVM_HIP(hipMemcpyAsync(&seen, ctx->alg.p + 3, sizeof(double), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    UFVM_HIP(hipStreamSynchronize(ctx->stream));
     if (seen != 0.0) throw Error(UFVM_ERR_NUMERIC, "NaN residual");
   };
   const double initial = cyc_final(c);
