@@ -511,7 +511,7 @@ def main():
     ap.add_argument("--serving-meshes", default="3d:37,3d:64,3d:100,3d:128,2d:224,2d:512,"
                                                 "2d:1024,2d:2048")
     ap.add_argument("--serving-requests", type=int, default=200)
-    ap.add_argument("--serving-steppers", type=int, default=2,
+    ap.add_argument("--serving-steppers", type=int, default=4,
                     help="SIMPLE-step lanes of the server")
     ap.add_argument("--serving-max-batch", type=int, default=8, choices=[1, 2, 4, 8],
                     help="queued solves of one mesh admitted together as one batch")

@@ -291,7 +291,7 @@ typedef struct {
                                 up to this many (2, 4 or 8; 1: no batching; 0: 8).  A
                                 batch reads the matrices once per iteration for all its
                                 solves and gives each the bits of its own solve */
-  int32_t n_steppers;        /* SIMPLE-step lanes (0: 2): each owns the cases assigned to it
+  int32_t n_steppers;        /* SIMPLE-step lanes (0: 4): each owns the cases assigned to it
                                 (the least-loaded lane by cells at ufvm_server_add_case), a
                                 context, a stream and a thread, so steps of cases in
                                 different lanes overlap on the device */

@@ -377,7 +377,7 @@ int ufvm_server_create(int32_t device, const ufvm_server_cfg *cfg, ufvm_server *
     return UFVM_ERR_ARG;
   const int32_t mb = cfg->max_batch == 0 ? 8 : cfg->max_batch;
   if (mb != 1 && mb != 2 && mb != 4 && mb != 8) return UFVM_ERR_ARG;
-  const int32_t lanes = cfg->n_steppers == 0 ? 2 : cfg->n_steppers;
+  const int32_t lanes = cfg->n_steppers == 0 ? 4 : cfg->n_steppers;
   ufvm_server *srv = new (std::nothrow) ufvm_server();
   if (!srv) return UFVM_ERR_ARG;
   srv->device = device;

@@ -30,7 +30,7 @@ STATUS = {0: "queued", 1: "running", 2: "done"}
 
 class Server:
     def __init__(self, device: int = 0, max_active: int = 8, n_streams: int = 4,
-                 hbm_reserve: float = 0.05, max_batch: int = 8, n_steppers: int = 2):
+                 hbm_reserve: float = 0.05, max_batch: int = 8, n_steppers: int = 4):
         """max_active: solves (or batches of solves) on the GPU at once;
         max_batch: queued solves of one system admitted together as one batch
         (1, 2, 4 or 8; each still gets the bits of its own solve); n_steppers:
