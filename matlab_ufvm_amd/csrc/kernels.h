@@ -49,7 +49,8 @@ void gs_sweep_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32
 constexpr int32_t kLdsRows = 8192;
 void sor_lds(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf, const int32_t *br,
              const int32_t *blp, int32_t nbk, int32_t n, const int32_t *ptr, const int32_t *col,
-             const double *ac, const double *anb, const double *b, double *x, const double *stop = nullptr);
+             const double *ac, const double *anb, const double *b, double *x,
+             const double *stop = nullptr);
 void dilu_lds(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf, const int32_t *br,
               const int32_t *blp, int32_t nbk, int32_t n, const int32_t *ptr, const int32_t *col,
               const double *anb, const int32_t *lptr, const int32_t *lnb, const int32_t *lpji,

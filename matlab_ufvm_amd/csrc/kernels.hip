@@ -27,7 +27,8 @@ __global__ void __launch_bounds__(kBlock)
 k_residual_ref(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
                const double *__restrict__ ac, const double *__restrict__ anb,
                const double *__restrict__ bc, const double *__restrict__ x,
-               double *__restrict__ r, double *__restrict__ partial, const double *__restrict__ stop) {
+               double *__restrict__ r, double *__restrict__ partial,
+               const double *__restrict__ stop) {
   if (stop && *stop != 0.0) return;  // stopped by the device-side test
   __shared__ double lds[kBlock / kWave];
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
@@ -200,7 +201,8 @@ __device__ __forceinline__ void gs_row(int32_t i, const int32_t *__restrict__ pt
 __global__ void __launch_bounds__(kBlock)
 k_gs_rows(const int32_t *__restrict__ rows, int32_t count, const int32_t *__restrict__ ptr,
           const int32_t *__restrict__ col, const double *__restrict__ ac,
-          const double *__restrict__ anb, const double *__restrict__ b, double *x, const double *__restrict__ stop) {
+          const double *__restrict__ anb, const double *__restrict__ b, double *x,
+          const double *__restrict__ stop) {
   if (stop && *stop != 0.0) return;  // stopped by the device-side test
   const int32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= count) return;
@@ -270,7 +272,8 @@ k_dilu_lds(const int32_t *__restrict__ fr, const int32_t *__restrict__ flp, int3
            const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
            const double *__restrict__ anb, const int32_t *__restrict__ lptr,
            const int32_t *__restrict__ lnb, const int32_t *__restrict__ lpji,
-           const double *__restrict__ dinv, double *__restrict__ rc, double *__restrict__ x, const double *__restrict__ stop) {
+           const double *__restrict__ dinv, double *__restrict__ rc, double *__restrict__ x,
+           const double *__restrict__ stop) {
   if (stop && *stop != 0.0) return;  // stopped by the device-side test
   extern __shared__ double rs[];
   for (int32_t i = threadIdx.x; i < n; i += kSweepBlock) rs[i] = rc[i];
@@ -442,7 +445,8 @@ template <typename V>
 __global__ void __launch_bounds__(kBlock)
 k_dilu_fwd(const int32_t *__restrict__ rows, int32_t count, const V *__restrict__ anb,
            const int32_t *__restrict__ lptr, const int32_t *__restrict__ lnb,
-           const int32_t *__restrict__ lpji, const V *__restrict__ dinv, V *rc, const double *__restrict__ stop) {
+           const int32_t *__restrict__ lpji, const V *__restrict__ dinv, V *rc,
+           const double *__restrict__ stop) {
   if (stop && *stop != 0.0) return;  // stopped by the device-side test
   const int32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= count) return;
@@ -504,7 +508,8 @@ template <typename V>
 __global__ void __launch_bounds__(kBlock)
 k_dilu_bwd(const int32_t *__restrict__ rows, int32_t count, int32_t n,
            const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
-           const V *__restrict__ anb, const V *__restrict__ dinv, V *rc, V *x, const double *__restrict__ stop) {
+           const V *__restrict__ anb, const V *__restrict__ dinv, V *rc, V *x,
+           const double *__restrict__ stop) {
   if (stop && *stop != 0.0) return;  // stopped by the device-side test
   const int32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= count) return;
@@ -515,7 +520,8 @@ template <typename V>
 __global__ void __launch_bounds__(kSweepBlock)
 k_dilu_bwd_block(const int32_t *__restrict__ rows, const int32_t *__restrict__ lp, int32_t nlev,
                  int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
-                 const V *__restrict__ anb, const V *__restrict__ dinv, V *rc, V *x, const double *__restrict__ stop) {
+                 const V *__restrict__ anb, const V *__restrict__ dinv, V *rc, V *x,
+                 const double *__restrict__ stop) {
   if (stop && *stop != 0.0) return;  // stopped by the device-side test
   for (int32_t q = 0; q < nlev; ++q) {
     for (int32_t t = lp[q] + threadIdx.x; t < lp[q + 1]; t += kSweepBlock)
