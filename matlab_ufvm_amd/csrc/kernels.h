@@ -46,7 +46,8 @@ void gs_sweep_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32
                     const int32_t *ptr, const int32_t *col, const double *ac, const double *anb,
                     const double *b, double *x, const double *stop = nullptr);
 // the same with the level's unknowns (x, rc, dinv) in LDS: n <= kLdsRows
-constexpr int32_t kLdsRows = 8192;
+// (48 KB of doubles, inside the 64 KB any workgroup may allocate)
+constexpr int32_t kLdsRows = 6144;
 void sor_lds(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf, const int32_t *br,
              const int32_t *blp, int32_t nbk, int32_t n, const int32_t *ptr, const int32_t *col,
              const double *ac, const double *anb, const double *b, double *x,

@@ -103,6 +103,23 @@ def test_solve_algebraic_system_bitexact(ctx, case, smoother):
     ctx.system_destroy(sid)
 
 
+@pytest.mark.parametrize("smoother", ["SOR", "ILU"])
+def test_lds_sweeps_near_their_size_limit(ctx, smoother):
+    # 76 x 76 = 5,776 cells: every lexicographic level is narrow (one
+    # workgroup) and the unknowns fit in LDS; the sweeps keep the oracle's bits
+    m = cavity_mesh(76, nz=1, jitter=0.1, seed=61, renumber_seed=None)
+    ptr, col, ac, anb = poisson_system(m, dirichlet=("movingWall",))
+    b = gaussian_rhs(m, seed=62)
+    sid = make(ctx, ptr, col, ac, anb, b)
+    x0 = np.random.default_rng(63).standard_normal(m.n_cells) * 1e-3
+    ctx.set_dphi(sid, 0, x0)
+    ctx.solve_algebraic_system(sid, 0, smoother, 3, rrf=0.0)
+    S = oracle.System(ptr, col, ac, anb, bc=b, dphi=x0)
+    S.solve_algebraic_system(0, smoother, 3, rrf=0.0)
+    assert np.array_equal(ctx.get_dphi(sid, 0), S.level(0)["dphi"])
+    ctx.system_destroy(sid)
+
+
 def test_mcgs_after_reagglomeration_uses_the_new_pattern(ctx):
     # a coarse level rebuilt with another pattern gets a new colouring and new
     # colour chunks (solve, re-agglomerate from other coefficients, solve)
