@@ -16,6 +16,9 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libufvm_oracle.so")
+# the same source built with OpenMP: the FPCG solve's loops on all host cores
+# (used only to time the CPU baseline; its dot products round differently)
+LIB_PATH_OMP = os.path.join(HERE, "libufvm_oracle_omp.so")
 
 i32p = C.POINTER(C.c_int32)
 f64p = C.POINTER(C.c_double)
@@ -44,15 +47,23 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
-_lib = None
+_libs = {}
+_openmp = False
+
+
+def use_openmp(on: bool) -> None:
+    """Route the following oracle calls to the OpenMP build (True) or the
+    sequential one (False, the default and the only one the tests check)."""
+    global _openmp
+    _openmp = bool(on)
 
 
 def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
+    path = LIB_PATH_OMP if _openmp else LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
             build()
-        L = C.CDLL(LIB_PATH)
+        L = C.CDLL(path)
         L.orc_residual.argtypes = [C.POINTER(Level), f64p]
         L.orc_agglomerate.argtypes = [C.POINTER(Hier), C.c_int]
         L.orc_agglomerate.restype = C.c_int
@@ -75,8 +86,8 @@ def lib():
                                       f64p, f64p]
         L.orc_sor.argtypes = [C.POINTER(Level), f64p]
         L.orc_hier_free.argtypes = [C.POINTER(Hier)]
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _p(a, t):

@@ -550,12 +550,21 @@ static void bmcgs(const orc_level *L, const int32_t *colour, int ncol, int32_t R
 
 void orc_mcgs(const orc_level *L, const int32_t *colour, int ncolours, const double *b,
               double *x) {
-  for (int c = 0; c < ncolours; ++c)
+  /* the rows of one colour are independent: the OpenMP build splits them */
+  for (int c = 0; c < ncolours; ++c) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
     for (int32_t i = 0; i < L->n; ++i)
       if (colour[i] == c) gs_row(L, b, x, i);
-  for (int c = ncolours - 1; c >= 0; --c)
+  }
+  for (int c = ncolours - 1; c >= 0; --c) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
     for (int32_t i = 0; i < L->n; ++i)
       if (colour[i] == c) gs_row(L, b, x, i);
+  }
 }
 
 /* ---- DILU in colour order (the kind-5 finest smoother of the GPU path) */
@@ -721,6 +730,9 @@ static void lu_free(orc_lu *F) {
 /* damped Jacobi sweep y = v + w (b - A v)/ac, v = x (+ xc[parent] if given) */
 static void jacobi(const orc_level *L, const double *b, const double *x, const int32_t *parent,
                    const double *xc, double omega, double *y) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
   for (int32_t i = 0; i < L->n; ++i) {
     double vi = x[i];
     if (parent) vi = vi + xc[parent[i]];
@@ -754,6 +766,9 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
       memcpy(x, y, (size_t)n * sizeof(double));
     }
     double *r = (double *)malloc((size_t)n * sizeof(double));
+#ifdef _OPENMP
+    #pragma omp parallel for schedule(static)
+    #endif
     for (int32_t i = 0; i < n; ++i) {
       double s = b[i] - L->ac[i] * x[i];
       for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) s = s - L->anb[k] * x[L->col[k]];
@@ -787,6 +802,9 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
       else orc_mcilu_sweep(L, colours[0], ncol[0], dc0, b, s == 0, rc, x);
     }
     double *r = (double *)malloc((size_t)n * sizeof(double));
+#ifdef _OPENMP
+    #pragma omp parallel for schedule(static)
+    #endif
     for (int32_t i = 0; i < n; ++i) {
       double s = b[i] - L->ac[i] * x[i];
       for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) s = s - L->anb[k] * x[L->col[k]];
@@ -812,6 +830,9 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
     bmcgs(L, colours[l], ncol[l], cfg->block_rows, b, NULL, NULL, NULL, 0, cfg->cpre, x);
   }
   double *r = (double *)malloc((size_t)n * sizeof(double));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
   for (int32_t i = 0; i < n; ++i) {
     /* level 0 after a symmetric sweep: colour 0 was solved last and its
      * residual (rounding noise) is taken as 0 */
@@ -868,20 +889,33 @@ void orc_vcycle(const orc_hier *H, int32_t *const *colours, const int *ncolours,
   lu_free(&lu);
 }
 
+/* (the OpenMP build sums in per-thread parts: its dots round differently) */
 static double dotv(const double *a, const double *b, int32_t n) {
   double s = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : s) schedule(static)
+#endif
   for (int32_t i = 0; i < n; ++i) s += a[i] * b[i];
   return s;
 }
 
 static void project(double *v, int32_t n) {
   double m = 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for reduction(+ : m) schedule(static)
+#endif
   for (int32_t i = 0; i < n; ++i) m += v[i];
   m /= n;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
   for (int32_t i = 0; i < n; ++i) v[i] -= m;
 }
 
 static void spmv(const orc_level *L, const double *x, double *y) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
   for (int32_t i = 0; i < L->n; ++i) {
     double s = L->ac[i] * x[i];
     for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) s = s + L->anb[k] * x[L->col[k]];
@@ -916,12 +950,18 @@ int orc_fpcg(const orc_hier *H, int32_t *const *colours, const int *ncolours,
       memcpy(p, z, (size_t)n * sizeof(double));
     } else {
       double beta = dotv(z, w, n) / delta;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
       for (int32_t i = 0; i < n; ++i) p[i] = z[i] - beta * p[i];
     }
     double rho = dotv(z, r, n);
     spmv(L, p, w);
     delta = dotv(p, w, n);
     double alpha = rho / delta;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
     for (int32_t i = 0; i < n; ++i) {
       x[i] += alpha * p[i];
       r[i] -= alpha * w[i];

@@ -523,3 +523,22 @@ def test_fpcg_singular_neumann():
     A = to_scipy(ptr, col, ac, anb)
     assert np.linalg.norm(b - A @ x) <= 1e-10 * np.linalg.norm(b)
     assert abs(x.mean()) <= 1e-8 * np.abs(x).max()
+
+
+def test_openmp_build_solves_the_same_system(poisson16):
+    # the OpenMP build (CPU-baseline timing only) runs the same FPCG: the same
+    # iteration count to within one and a solution within the tolerance;
+    # the sequential build stays the default
+    m, ptr, col, ac, anb, b = poisson16
+    S = oracle.System(ptr, col, ac, anb)
+    S.agglomerate(10)
+    x1, it1, rr1 = S.fpcg(b, tol=1e-10, max_coarse_cells=200)
+    oracle.use_openmp(True)
+    try:
+        x2, it2, rr2 = S.fpcg(b, tol=1e-10, max_coarse_cells=200)
+    finally:
+        oracle.use_openmp(False)
+    assert abs(it1 - it2) <= 1 and rr2 <= 1e-10
+    assert np.linalg.norm(x1 - x2) <= 1e-8 * np.linalg.norm(x1)
+    x3, it3, rr3 = S.fpcg(b, tol=1e-10, max_coarse_cells=200)
+    assert it3 == it1 and np.array_equal(x3, x1)
