@@ -206,6 +206,9 @@ int ufvm_pcg_level_colours(ufvm_ctx *ctx, int32_t sys, int32_t level, int32_t *n
 #define UFVM_BC_FIXED_VALUE 0
 #define UFVM_BC_ZERO_GRADIENT 1
 #define UFVM_BC_EMPTY 2
+/* U on a wall: zero, with the reference's vector noSlip wall diffusion
+ * (cfdAssembleDiffusionTerm.m:479-562, TM from cfdGetUCoef.m:41-44) */
+#define UFVM_BC_NO_SLIP 3
 typedef struct {
   int32_t n_cells, n_faces, n_interior;
   const int32_t *owner, *neighbour;

@@ -23,7 +23,7 @@ struct Case {
   int32_t N = 0, F = 0, Fi = 0, Fb = 0, P = 0;
   DBuf<int32_t> owner, neighbour, cptr, ccol, own_nnz, nei_nnz, cf_ptr, cf_face, cf_slot, bpatch;
   DBuf<int32_t> ptype, u_bc, p_bc;
-  DBuf<double> Sf, area, CN, eCN, gDiff, T, gf, fc, vol, cc, u_val;
+  DBuf<double> Sf, area, CN, eCN, gDiff, T, gf, fc, vol, cc, u_val, walldist, noslip_sum;
   DBuf<double> U, gradU, p, gradp, PP, gradPP, DU, DUT, mdot, gamma, fluxC1, fluxC2, ac_old;
   DBuf<double> uface;  // momentum face fluxes, 6 per face
   // 1 when the pressure-correction coefficients changed since the kept
@@ -73,6 +73,10 @@ void init_fields(ufvm_ctx *ctx, Case &C) {
     const int32_t f = C.Fi + k, b = C.N + k, o = owner[f];
     if (C.h_u_bc[p] == UFVM_BC_FIXED_VALUE) {
       for (int d = 0; d < 3; ++d) U[b * 3 + d] = C.h_u_val[p * 3 + d];
+    } else if (C.h_u_bc[p] == UFVM_BC_NO_SLIP) {
+      // noSlip's value '[0;0;0]' (cfdDefineEquation.m:290-295), set once
+      // (cfdInitializeBoundaryConditions.m:239-267) and never updated
+      for (int d = 0; d < 3; ++d) U[b * 3 + d] = 0.0;
     } else if (C.h_u_bc[p] == UFVM_BC_EMPTY) {
       const double *S = &Sf[f * 3];
       const double nm = std::sqrt((S[0] * S[0] + S[1] * S[1]) + S[2] * S[2]);
@@ -262,8 +266,9 @@ int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *m, const ufvm_simple_c
       if (t != UFVM_PATCH_WALL && t != UFVM_PATCH_EMPTY)
         throw Error(UFVM_ERR_UNSUPPORTED, "only wall and empty patches are supported");
       const int ub = cfg->u_bc[p], pb = cfg->p_bc[p];
-      if (t == UFVM_PATCH_WALL && !(ub == UFVM_BC_FIXED_VALUE && pb == UFVM_BC_ZERO_GRADIENT))
-        throw Error(UFVM_ERR_UNSUPPORTED, "walls need U fixedValue and p zeroGradient");
+      if (t == UFVM_PATCH_WALL &&
+          !((ub == UFVM_BC_FIXED_VALUE || ub == UFVM_BC_NO_SLIP) && pb == UFVM_BC_ZERO_GRADIENT))
+        throw Error(UFVM_ERR_UNSUPPORTED, "walls need U fixedValue or noSlip and p zeroGradient");
       if (t == UFVM_PATCH_EMPTY && !(ub == UFVM_BC_EMPTY && pb == UFVM_BC_EMPTY))
         throw Error(UFVM_ERR_UNSUPPORTED, "empty patches need empty U and p conditions");
     }
@@ -331,6 +336,25 @@ int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *m, const ufvm_simple_c
     C.u_bc.upload(C.h_u_bc.empty() ? std::vector<int32_t>{0} : C.h_u_bc, s);
     C.p_bc.upload(C.h_p_bc.empty() ? std::vector<int32_t>{0} : C.h_p_bc, s);
     C.u_val.upload(C.h_u_val.empty() ? std::vector<double>{0.0} : C.h_u_val, s);
+    {
+      // boundary-face walldist = (CN . Sf) / |Sf| (cfdProcessOpenFoamMesh.m:153)
+      // and, per patch and component, the sum in face order of (Sf_c/area)^2
+      // (the noSlip wall's patch-wide dot(nx',nx'))
+      std::vector<double> wd(C.F, 0.0), ns(std::max(3 * C.P, 1), 0.0);
+      for (int32_t f = C.Fi; f < C.F; ++f) {
+        const double *S = &m->Sf[(size_t)f * 3], *D = &m->CN[(size_t)f * 3];
+        const double dot = (D[0] * S[0] + D[1] * S[1]) + D[2] * S[2];
+        wd[f] = dot / std::sqrt((S[0] * S[0] + S[1] * S[1]) + S[2] * S[2]);
+      }
+      for (int p = 0; p < C.P; ++p)
+        for (int32_t f = m->patch_start[p]; f < m->patch_start[p] + m->patch_nfaces[p]; ++f)
+          for (int d = 0; d < 3; ++d) {
+            const double nc = m->Sf[(size_t)f * 3 + d] / m->area[f];
+            ns[p * 3 + d] = ns[p * 3 + d] + nc * nc;
+          }
+      C.walldist.upload(wd, s);
+      C.noslip_sum.upload(ns, s);
+    }
     C.cfg.u_bc = nullptr;
     C.cfg.u_value = nullptr;
     C.cfg.p_bc = nullptr;
@@ -344,6 +368,7 @@ int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *m, const ufvm_simple_c
     d.cf_ptr = C.cf_ptr.p; d.cf_face = C.cf_face.p; d.cf_slot = C.cf_slot.p;
     d.bpatch = C.bpatch.p;
     d.ptype = C.ptype.p; d.u_bc = C.u_bc.p; d.p_bc = C.p_bc.p; d.u_val = C.u_val.p;
+    d.walldist = C.walldist.p; d.noslip_sum = C.noslip_sum.p;
     init_fields(ctx, C);
     C.sysU = create_system(ctx, C.N, nnz, m->cconn_ptr, m->cconn);
     C.sysP = create_system(ctx, C.N, nnz, m->cconn_ptr, m->cconn);

@@ -10,7 +10,7 @@ namespace ufvm {
 namespace sk {
 
 enum { PATCH_WALL = 0, PATCH_EMPTY = 1, PATCH_PATCH = 2 };
-enum { BC_FIXED_VALUE = 0, BC_ZERO_GRADIENT = 1, BC_EMPTY = 2 };
+enum { BC_FIXED_VALUE = 0, BC_ZERO_GRADIENT = 1, BC_EMPTY = 2, BC_NO_SLIP = 3 };
 
 // device pointers of the mesh, passed by value
 struct DevMesh {
@@ -25,6 +25,8 @@ struct DevMesh {
   const int32_t *bpatch;             // patch of each boundary face
   const int32_t *ptype, *u_bc, *p_bc;
   const double *u_val;
+  const double *walldist;            // [F] (CN . Sf) / |Sf| of boundary faces, 0 inside
+  const double *noslip_sum;          // [patches][3] sum over the patch of (Sf_c / area)^2
 };
 
 // scalar slots written by the reductions of this file

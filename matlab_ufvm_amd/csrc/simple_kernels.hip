@@ -345,6 +345,22 @@ k_u_faces(DevMesh m, int c, const double *__restrict__ U, const double *__restri
       const double C2 = ((-1.0) * gam) * m.gDiff[f];
       const double V = ((-1.0) * gam) * dot3(&gradU[(b * 3 + c) * 3], &m.T[f * 3]);
       T = (C1 * U[i * 3 + c] + C2 * U[b * 3 + c]) + V;
+    } else if (m.ptype[p] == PATCH_WALL && m.u_bc[p] == BC_NO_SLIP) {
+      // vector noSlip wall (cfdAssembleDiffusionTerm.m:479-562): TM =
+      // gamma_b / walldist (cfdGetUCoef.m:41-44), and dot(nx',nx') over the
+      // patch's rows is one patch-wide sum of n_c^2 (kept as the reference has it)
+      const double A = m.area[f], dw = m.walldist[f], TM = gam / dw;
+      double n[3];
+      for (int d = 0; d < 3; ++d) n[d] = m.Sf[f * 3 + d] / A;
+      const double Sc = m.noslip_sum[p * 3 + c];
+      const int j = c == 0 ? 1 : 0, k2 = c == 2 ? 1 : 2;
+      const double g = ((((1.0 * gam) * A) / dw) * TM);
+      C1 = g * (1 - Sc);
+      const double C2 = 0.0;
+      const double V = g * ((U[b * 3 + c] * (1 - Sc) +
+                             ((U[i * 3 + j] - U[b * 3 + j]) * n[j]) * n[c]) -
+                            ((U[i * 3 + k2] - U[b * 3 + k2]) * n[k2]) * n[c]);
+      T = (C1 * U[i * 3 + c] + C2 * U[b * 3 + c]) + V;
     } else if (m.ptype[p] == PATCH_EMPTY) {
       const double A = m.area[f];
       double n[3], l[3] = {1.0, 1.0, 1.0}, a[3];

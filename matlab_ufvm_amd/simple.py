@@ -21,13 +21,14 @@ from . import _lib
 from ._lib import AmgCfg, MeshDesc, PcgCfg, SimpleCfg, f64p, i32p
 
 PATCH = {"wall": 0, "empty": 1}
-BC = {"fixedValue": 0, "zeroGradient": 1, "empty": 2}
+BC = {"fixedValue": 0, "zeroGradient": 1, "empty": 2, "noSlip": 3}
 P_SOLVER = {"amg": 0, "pcg": 1, "pcg-frozen": 2}
 
 
-def cavity_bcs(mesh, lid=(1.0, 0.0, 0.0)):
-    """Lid-driven cavity: fixedValue U on walls (lid moving), zeroGradient p,
-    empty front/back patches of 2-D meshes."""
+def cavity_bcs(mesh, lid=(1.0, 0.0, 0.0), walls="fixedValue"):
+    """Lid-driven cavity: the lid fixedValue U, the fixed walls fixedValue
+    (0 0 0) or noSlip (`walls`), zeroGradient p, empty front/back patches of
+    2-D meshes."""
     out = {}
     for p in mesh.patches:
         if p.type == "empty":
@@ -35,7 +36,7 @@ def cavity_bcs(mesh, lid=(1.0, 0.0, 0.0)):
         elif p.name == "movingWall":
             out[p.name] = ("fixedValue", tuple(lid), "zeroGradient")
         else:
-            out[p.name] = ("fixedValue", (0.0, 0.0, 0.0), "zeroGradient")
+            out[p.name] = (walls, (0.0, 0.0, 0.0), "zeroGradient")
     return out
 
 

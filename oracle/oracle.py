@@ -281,7 +281,7 @@ class OState(C.Structure):
 
 
 PATCH_TYPE = {"wall": 0, "empty": 1, "patch": 2}
-BC = {"fixedValue": 0, "zeroGradient": 1, "empty": 2}
+BC = {"fixedValue": 0, "zeroGradient": 1, "empty": 2, "noSlip": 3}
 
 
 class SimpleCase:
@@ -405,9 +405,9 @@ class SimpleCase:
                     ncycles=S.ncycles)
 
 
-def cavity_bcs(mesh, lid=(1.0, 0.0, 0.0)):
-    """Lid-driven cavity boundary conditions (fixedValue walls, zeroGradient p,
-    empty front/back)."""
+def cavity_bcs(mesh, lid=(1.0, 0.0, 0.0), walls="fixedValue"):
+    """Lid-driven cavity boundary conditions (the lid fixedValue, the fixed
+    walls fixedValue (0 0 0) or noSlip, zeroGradient p, empty front/back)."""
     out = {}
     for p in mesh.patches:
         if p.type == "empty":
@@ -415,5 +415,5 @@ def cavity_bcs(mesh, lid=(1.0, 0.0, 0.0)):
         elif p.name == "movingWall":
             out[p.name] = ("fixedValue", lid, "zeroGradient")
         else:
-            out[p.name] = ("fixedValue", (0.0, 0.0, 0.0), "zeroGradient")
+            out[p.name] = (walls, (0.0, 0.0, 0.0), "zeroGradient")
     return out

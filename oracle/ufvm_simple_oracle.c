@@ -22,6 +22,22 @@ static double mag3(const double *v) { return sqrt((v[0] * v[0] + v[1] * v[1]) + 
 static double dot3(const double *a, const double *b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
 
 /* patch of boundary face f (absolute index) */
+/* face walldist = (CN . Sf) / |Sf| (cfdProcessOpenFoamMesh.m:153) */
+static double walldist(const orc_mesh *m, int32_t f) {
+  return dot3(&m->CN[f * 3], &m->Sf[f * 3]) / mag3(&m->Sf[f * 3]);
+}
+
+/* sum over patch p's faces, in face order, of n_c^2, n = Sf / area: the
+ * patch-wide dot(nx',nx') of cfdAssembleDiffusionTerm.m:535-551 */
+static double noslip_sum(const orc_mesh *m, int p, int comp) {
+  double s = 0.0;
+  for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
+    const double nc = m->Sf[f * 3 + comp] / m->area[f];
+    s = s + nc * nc;
+  }
+  return s;
+}
+
 static int patch_of(const orc_mesh *m, int32_t f) {
   for (int p = 0; p < m->npatch; ++p)
     if (f >= m->pstart[p] && f < m->pstart[p] + m->pnf[p]) return p;
@@ -65,6 +81,10 @@ void orc_simple_init(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_stat
       const int32_t b = N + (f - m->Fi), o = m->owner[f];
       if (m->u_bc[p] == ORC_BC_FIXED_VALUE) {
         for (int d = 0; d < 3; ++d) s->U[b * 3 + d] = m->u_val[p * 3 + d];
+      } else if (m->u_bc[p] == ORC_BC_NO_SLIP) {
+        /* the noSlip value '[0;0;0]' (cfdDefineEquation.m:290-295), set once
+         * (cfdInitializeBoundaryConditions.m:239-267) and never updated */
+        for (int d = 0; d < 3; ++d) s->U[b * 3 + d] = 0.0;
       } else if (m->u_bc[p] == ORC_BC_EMPTY) {
         double n[3];
         const double nm = mag3(&m->Sf[f * 3]);
@@ -304,6 +324,22 @@ void orc_assemble_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
           C1[f] = ((-sg) * gam) * m->gDiff[f];
           C2[f] = (sg * gam) * m->gDiff[f];
           const double V = (sg * gam) * dot3(&s->gradU[(b * 3 + comp) * 3], &m->T[f * 3]);
+          T[f] = (C1[f] * U[o * 3 + comp] + C2[f] * U[b * 3 + comp]) + V;
+        } else if (m->ptype[p] == ORC_PATCH_WALL && m->u_bc[p] == ORC_BC_NO_SLIP) {
+          /* cfdAssembleDiffusionTerm.m:479-562 with TM = gamma_b / walldist
+           * (cfdGetUCoef.m:41-44, laminar).  dot(nx',nx') on the patch's
+           * 1 x F rows is one patch-wide sum of n_c^2 (hazard H7) */
+          const double A = m->area[f], d = walldist(m, f), TM = gam / d;
+          double n[3];
+          for (int q = 0; q < 3; ++q) n[q] = m->Sf[f * 3 + q] / A;
+          const double Sc = noslip_sum(m, p, comp);
+          const int j = comp == 0 ? 1 : 0, k = comp == 2 ? 1 : 2;
+          const double g = ((((-sg) * gam) * A) / d) * TM;
+          C1[f] = g * (1 - Sc);
+          C2[f] = 0.0;
+          const double V = g * ((U[b * 3 + comp] * (1 - Sc) +
+                                 ((U[o * 3 + j] - U[b * 3 + j]) * n[j]) * n[comp]) -
+                                ((U[o * 3 + k] - U[b * 3 + k]) * n[k]) * n[comp]);
           T[f] = (C1[f] * U[o * 3 + comp] + C2[f] * U[b * 3 + comp]) + V;
         } else if (m->ptype[p] == ORC_PATCH_EMPTY) {
           const double A = m->area[f];

@@ -136,6 +136,26 @@ def test_mcgs_momentum_iterations_bitexact(ctx):
     g.close()
 
 
+def test_noslip_walls_bitexact(ctx):
+    # the reference's vector noSlip wall (patch-wide sums of n_c^2, TM =
+    # nu / walldist) on the GPU gives the oracle's fields bit for bit, and it
+    # is a different flow from the fixedValue-wall one
+    m = cavity_mesh(8, jitter=0.15, seed=71, renumber_seed=72)
+    g = SimpleCase(ctx, m, cavity_bcs(m, walls="noSlip"), nu=0.01, **AMG)
+    o = oracle.SimpleCase(m, oracle.cavity_bcs(m, walls="noSlip"), nu=0.01, **AMG)
+    res = g.iterate(4)
+    for _ in range(4):
+        o.iterate(1)
+    for name, oname in (("U", "U"), ("p", "p"), ("mdot_f", "mdot")):
+        same(g.field(name), o.field(oname), f"noSlip {name}")
+    assert np.all(np.isfinite(res))
+    f = SimpleCase(ctx, m, cavity_bcs(m), nu=0.01, **AMG)
+    f.iterate(4)
+    assert not np.array_equal(f.field("U"), g.field("U"))
+    f.close()
+    g.close()
+
+
 def test_frozen_hierarchy_pressure_solver(ctx):
     # "pcg-frozen" keeps the first solve's hierarchy and refreshes its values
     # on the GPU after every assembly; it reaches the fields of the

@@ -92,6 +92,33 @@ def test_uniform_mesh_closed_forms():
     assert np.allclose(S["ac"][interior], (4 * nu * A / h + V / 1e9) / 0.7, rtol=1e-13)
 
 
+def test_noslip_wall_closed_forms():
+    # the reference's vector noSlip wall (cfdAssembleDiffusionTerm.m:479-562):
+    # C1 = nu A / d * TM * (1 - S_c), TM = nu / d (cfdGetUCoef.m:41-44), with
+    # S_c the PATCH-WIDE sum of n_c^2 (dot(nx',nx') on 1 x F rows, hazard H7).
+    # On a uniform n x n grid the fixed-wall patch (left, right, bottom) has
+    # S_x = 2n and S_y = n, and d = h/2, so a bottom-wall cell's diagonal from
+    # rest is urf^-1 (3 nu A/h + nu^2 A/(h/2)^2 (1 - S_c) + V/fdt).
+    n, nu = 6, 0.01
+    m = cavity_mesh(n, nz=1, jitter=0.0, renumber_seed=None)
+    case = oracle.SimpleCase(m, oracle.cavity_bcs(m, walls="noSlip"), nu=nu)
+    h, A, V = 1.0 / n, (1.0 / n) ** 2, (1.0 / n) ** 3
+    c = m.centroid[:m.n_cells]
+    bottom = np.where((np.abs(c[:, 1] - h / 2) < 1e-12) & (c[:, 0] > h) & (c[:, 0] < 1 - h))[0]
+    assert bottom.size == n - 2
+    case.update_fields()
+    for comp, S_c in ((0, 2 * n), (1, n)):
+        case.assemble_u(comp)
+        ac = case.system()["ac"]
+        wall = nu * nu * A / (h / 2) ** 2 * (1 - S_c)
+        assert np.allclose(ac[bottom], (3 * nu * A / h + wall + V / 1e9) / 0.7, rtol=1e-12), comp
+    # the wall's U stays zero (noSlip's '[0;0;0]', never updated)
+    U = case.field("U")
+    fw = next(p for p in m.patches if p.name == "fixedWalls")
+    b0 = m.n_cells + fw.start - m.n_interior
+    assert np.all(U[b0:b0 + fw.nfaces] == 0.0)
+
+
 def test_gradient_of_linear_field_is_exact():
     m = cavity_mesh(6, nz=1, jitter=0.0, renumber_seed=None)
     case = oracle.SimpleCase(m, oracle.cavity_bcs(m), nu=0.01)
