@@ -49,8 +49,9 @@ void read_scal(ufvm_ctx *ctx, int first, int count) {
 void upload_pattern(RefLevel &L, hipStream_t s) {
   L.n = L.g.n;
   L.nnz = (int32_t)L.g.col.size();
-  L.ptr.upload(L.g.ptr, s);
-  L.col.upload(L.g.col, s);
+  // L.g lives as long as the level: the copies need not be waited for here
+  L.ptr.upload_async(L.g.ptr, s);
+  L.col.upload_async(L.g.col, s);
   L.ac.alloc(L.n);
   L.anb.alloc(std::max(L.nnz, 1));
   L.bc.alloc(L.n);
@@ -320,24 +321,27 @@ int32_t agglomerate(ufvm_ctx *ctx, System &S, int32_t max_coarse) {
   S.pcg.ready = false;
   int32_t iLevel = 1;
   std::vector<double> h_anb;
+  std::vector<std::unique_ptr<AggResult>> keep;
   while (iLevel <= max_coarse) {
     iLevel = iLevel + 1;
     RefLevel &F = *S.lev[iLevel - 2];
     h_anb.resize(std::max(F.nnz, 1));
     F.anb.download(h_anb.data(), F.nnz, s);
     UFVM_HIP(hipStreamSynchronize(s));
-    AggResult ag;
+    AggResult &ag = *keep.emplace_back(new AggResult());
     agglomerate_level(F.g, h_anb.data(), ag);
     F.h_parents = ag.parents;
-    F.parents.upload(ag.parents, s);
-    F.child_ptr.upload(ag.child_ptr, s);
-    F.child.upload(ag.child, s);
+    // the host arrays stay alive (keep) until the stream is synchronised:
+    // the next level's download or the end of this function
+    F.parents.upload_async(F.h_parents, s);
+    F.child_ptr.upload_async(ag.child_ptr, s);
+    F.child.upload_async(ag.child, s);
     {
-      std::vector<int32_t> gptr, gsrc;
+      std::vector<int32_t> &gptr = ag.gal_ptr, &gsrc = ag.gal_src;
       galerkin_lists(F.g, ag, gptr, gsrc);
       F.gal_out = (int32_t)gptr.size() - 1;
-      F.gal_ptr.upload(gptr, s);
-      F.gal_src.upload(gsrc, s);
+      F.gal_ptr.upload_async(gptr, s);
+      F.gal_src.upload_async(gsrc, s);
     }
     if ((int32_t)S.lev.size() < iLevel) S.lev.emplace_back(new RefLevel());
     RefLevel &C = *S.lev[iLevel - 1];

@@ -20,6 +20,7 @@ struct AggResult {
   std::vector<int32_t> child_ptr;   // coarse row -> children, increasing fine index
   std::vector<int32_t> child;
   std::vector<int32_t> entry_slot;  // fine nnz -> coarse nnz slot, -1 if same parent
+  std::vector<int32_t> gal_ptr, gal_src;  // galerkin_lists of this level, when built
 };
 
 // cfdAgglomerateLevel.m:3-104 on one level (anb values in cconn order).

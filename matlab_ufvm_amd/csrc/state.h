@@ -84,6 +84,12 @@ struct DBuf {
     }
   }
   void upload(const std::vector<T> &v, hipStream_t s) { upload(v.data(), v.size(), s); }
+  // enqueued only: h must stay valid until the stream has been synchronised
+  void upload_async(const std::vector<T> &v, hipStream_t s) {
+    alloc(v.size());
+    if (!v.empty())
+      UFVM_HIP(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s));
+  }
   void download(T *h, size_t count, hipStream_t s) const {
     if (count) UFVM_HIP(hipMemcpyAsync(h, p, count * sizeof(T), hipMemcpyDeviceToHost, s));
   }
