@@ -874,7 +874,7 @@ k_residual_sell(int32_t n, int32_t zero_rows, const int32_t *__restrict__ off,
                 const int32_t *__restrict__ col, const V *__restrict__ val,
                 const V *__restrict__ ac, const V *__restrict__ b, const V *__restrict__ x,
                 V *__restrict__ r) {
-  const int32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   if (i < zero_rows) {
     r[i] = V(0);
@@ -938,7 +938,7 @@ k_jacobi_sell(int32_t n, const int32_t *__restrict__ off, const int32_t *__restr
               const V *__restrict__ val, const V *__restrict__ ac, const V *__restrict__ b,
               const V *__restrict__ x, const int32_t *__restrict__ parent,
               const V *__restrict__ xc, V *__restrict__ out, V omega) {
-  const int32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   jacobi_row<V, PRO>(i, off, col, val, ac, b, x, parent, xc, out, omega);
 }
@@ -1059,7 +1059,7 @@ __global__ void __launch_bounds__(kBlock)
 k_restrict_sell(int32_t nc, const int32_t *__restrict__ off, const int32_t *__restrict__ child,
                 const V *__restrict__ r, const V *__restrict__ acc, V *__restrict__ bc,
                 V *__restrict__ xc, V omega, V scale) {
-  const int32_t p = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int32_t p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= nc) return;
   restrict_row<V, JAC>(p, off, child, r, acc, bc, xc, omega, scale);
 }

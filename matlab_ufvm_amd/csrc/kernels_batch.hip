@@ -80,7 +80,7 @@ k_residual_sell_k(int32_t n, int32_t zero_rows, const int32_t *__restrict__ off,
                   const int32_t *__restrict__ col, const V *__restrict__ val,
                   const V *__restrict__ ac, const V *__restrict__ b, const V *__restrict__ x,
                   V *__restrict__ rr, int64_t ld) {
-  const int32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   if (i < zero_rows) {
 #pragma unroll
@@ -105,7 +105,7 @@ k_jacobi_sell_k(int32_t n, const int32_t *__restrict__ off, const int32_t *__res
                 const V *__restrict__ val, const V *__restrict__ ac, const V *__restrict__ b,
                 const V *__restrict__ x, int64_t ld, const int32_t *__restrict__ parent,
                 const V *__restrict__ xc, int64_t ldc, V *__restrict__ out, V omega) {
-  const int32_t i = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   const int32_t q = i >> 6;
   const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(kBlock)
 k_restrict_sell_k(int32_t nc, const int32_t *__restrict__ off, const int32_t *__restrict__ child,
                   const V *__restrict__ rf, int64_t ldf, const V *__restrict__ acc,
                   V *__restrict__ bc, V *__restrict__ xc, int64_t ldc, V omega, V scale) {
-  const int32_t p = xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  const int32_t p = blockIdx.x * kBlock + threadIdx.x;
   if (p >= nc) return;
   const int32_t q = p >> 6;
   const int32_t base = off[q] + (p & 63), w = (off[q + 1] - off[q]) >> 6;
