@@ -256,7 +256,8 @@ void dilu_sweep(ufvm_ctx *ctx, RefLevel &L, const double *stop = nullptr) {
 static void solve_alg_device(ufvm_ctx *ctx, RefLevel &L, int smoother, int n_iter, double rrf,
                              bool chunks) {
   hipStream_t s = ctx->stream;
-  double *st = ctx->alg.p;
+  double *st = ctx->alg.p;  // allocated and cleared by apply_amg, the only caller
+  if (!st) throw Error(UFVM_ERR_STATE, "device-side smoothing test outside an AMG solve");
   auto residual = [&](const double *stop) -> int32_t {
     if (chunks) {
       ensure_partial(ctx, (int32_t)std::max<int64_t>(L.cs_slots, 1), 1);
