@@ -131,3 +131,23 @@ def test_cylinder_cells_are_closed():
     np.add.at(net, m.neighbour, -m.Sf[:m.n_interior])
     assert np.abs(net).max() <= 1e-12 * np.abs(m.Sf).max()
     assert np.all(m.volume > 0)
+
+
+def test_meshtool_processes_the_cylinder_case(tmp_path, capsys):
+    # the standalone mesh tool: polyMesh in, processed geometry out (the same
+    # arrays as process_mesh), with a summary on stdout
+    import json
+    from matlab_ufvm_amd import meshtool
+    from matlab_ufvm_amd.mesh import process_mesh
+    from matlab_ufvm_amd.polymesh import read_polymesh
+    case = os.path.join(os.path.dirname(__file__), "golden", "cylinder")
+    out = str(tmp_path / "cyl.npz")
+    assert meshtool.main([case, "--out", out]) == 0
+    info = json.loads(capsys.readouterr().out)
+    m = process_mesh(read_polymesh(case))
+    assert info["cells"] == m.n_cells and info["faces"] == m.n_faces
+    assert [p["name"] for p in info["patches"]] == [p.name for p in m.patches]
+    d = np.load(out)
+    for k in meshtool.ARRAYS:
+        assert np.array_equal(d[k], getattr(m, k)), k
+    assert int(d["n_interior"]) == m.n_interior
