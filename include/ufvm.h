@@ -240,6 +240,9 @@ typedef struct {
   ufvm_pcg_cfg p_pcg;
   int32_t u_smoother;                     /* momentum solves: UFVM_SMOOTHER_ILU (the
                                              reference's, default) or UFVM_SMOOTHER_MCGS */
+  int32_t stress_term;                    /* 1: the momentum equation has
+                                             + div(nu*transp(grad(U))) (the reference's
+                                             tutorials; cfdAssembleExplicitStressTerm.m) */
 } ufvm_simple_cfg;
 
 /* Upload a mesh and create its fields (uniform zero initial conditions,

@@ -67,7 +67,8 @@ class SimpleCfg(C.Structure):
     _fields_ = [("nu", C.c_double), ("urf_u", C.c_double), ("urf_p", C.c_double),
                 ("n_correctors", C.c_int32), ("p_ref_cell", C.c_int32), ("u_sweeps", C.c_int32),
                 ("u_bc", i32p), ("u_value", f64p), ("p_bc", i32p), ("p_solver", C.c_int32),
-                ("p_amg", AmgCfg), ("p_pcg", PcgCfg), ("u_smoother", C.c_int32)]
+                ("p_amg", AmgCfg), ("p_pcg", PcgCfg), ("u_smoother", C.c_int32),
+                ("stress_term", C.c_int32)]
 
 
 # name -> (restype, argtypes)

@@ -26,6 +26,7 @@ struct Case {
   DBuf<double> Sf, area, CN, eCN, gDiff, T, gf, fc, vol, cc, u_val, walldist, noslip_sum;
   DBuf<double> U, gradU, p, gradp, PP, gradPP, DU, DUT, mdot, gamma, fluxC1, fluxC2, ac_old;
   DBuf<double> uface;  // momentum face fluxes, 6 per face
+  DBuf<double> ustress;  // explicit stress flux per face (stress_term only)
   // 1 when the pressure-correction coefficients changed since the kept
   // hierarchy was last refreshed from them (set by the assembly)
   DBuf<int32_t> p_changed;
@@ -160,7 +161,7 @@ void assemble(ufvm_ctx *ctx, Case &C, int eq, int comp, double *rms, double *mx)
   if (eq == 0) {
     if (comp < 0 || comp > 2) throw Error(UFVM_ERR_ARG, "component must be 0, 1 or 2");
     sk::assemble_u(s, C.dm, comp, C.cfg.urf_u, C.U.p, C.gradU.p, C.gradp.p, C.mdot.p, C.gamma.p,
-                   C.uface.p, L.ac.p, L.anb.p, L.bc.p, C.ac_old.p, L.dphi.p, C.DU.p, C.DUT.p, ctx->partial.p,
+                   C.uface.p, C.cfg.stress_term ? C.ustress.p : nullptr, L.ac.p, L.anb.p, L.bc.p, C.ac_old.p, L.dphi.p, C.DU.p, C.DUT.p, ctx->partial.p,
                    ctx->counter.p, ctx->scal.p);
   } else {
     sk::rhie_chow(s, C.dm, C.cfg.urf_u, C.U.p, C.DU.p, C.p.p, C.gradp.p, C.mdot.p, C.fluxC1.p,
@@ -326,6 +327,9 @@ int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *m, const ufvm_simple_c
     C.cf_face.upload(cf_face, s);
     C.cf_slot.upload(cf_slot, s);
     C.uface.alloc((size_t)C.F * 6);
+    if (C.cfg.stress_term != 0 && C.cfg.stress_term != 1)
+      throw Error(UFVM_ERR_ARG, "stress_term must be 0 or 1");
+    if (C.cfg.stress_term) C.ustress.alloc(C.F);
     C.p_changed.upload(std::vector<int32_t>{1}, s);
     C.bpatch.upload(bp.empty() ? std::vector<int32_t>{0} : bp, s);
     C.h_ptype.assign(m->patch_type, m->patch_type + C.P);

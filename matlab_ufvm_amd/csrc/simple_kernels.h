@@ -39,8 +39,8 @@ void scale_u(hipStream_t s, const DevMesh &m, const double *U, double *partial,
              unsigned *counter, double *scal);
 void assemble_u(hipStream_t s, const DevMesh &m, int comp, double urf, const double *U,
                 const double *gradU, const double *gradp, const double *mdot,
-                const double *gamma, double *uf, double *ac, double *anb, double *bc,
-                double *ac_old, double *dphi, double *DU, double *DUT, double *partial,
+                const double *gamma, double *uf, double *stress, double *ac, double *anb,
+                double *bc, double *ac_old, double *dphi, double *DU, double *DUT, double *partial,
                 unsigned *counter, double *scal);
 void correct_u(hipStream_t s, const DevMesh &m, int comp, const double *dphi, double *U);
 void rhie_chow(hipStream_t s, const DevMesh &m, double urf_u, const double *U,

@@ -314,11 +314,28 @@ __device__ inline void grad_face(const DevMesh &m, const double *phi, const doub
 __global__ void __launch_bounds__(kBlock)
 k_u_faces(DevMesh m, int c, const double *__restrict__ U, const double *__restrict__ gradU,
           const double *__restrict__ mdot, const double *__restrict__ gamma,
-          double *__restrict__ uf) {
+          double *__restrict__ uf, double *__restrict__ stress) {
   const int32_t f = blockIdx.x * kBlock + threadIdx.x;
   if (f >= m.F) return;
   const int32_t N = m.N, Fi = m.Fi;
   const double gam = gamma[f];
+  if (stress) {
+    // + div(nu*transp(grad(U))), explicit, sign -1 (cfdAssembleExplicitStressTerm.m:14-52):
+    // (sign gamma_f) ((g_x Sf_x + g_y Sf_y) + g_z Sf_z), g_d the Average
+    // interpolation of d(U_c)/dx_d (boundary faces: the boundary row) -- the
+    // gradient of U_c itself, as the reference computes it
+    double g[3];
+    if (f < Fi) {
+      const int32_t o = m.owner[f], nb = m.neighbour[f];
+      const double w = m.gf[f];
+      for (int d = 0; d < 3; ++d)
+        g[d] = w * gradU[(nb * 3 + c) * 3 + d] + (1 - w) * gradU[(o * 3 + c) * 3 + d];
+    } else {
+      const int32_t b = N + (f - Fi);
+      for (int d = 0; d < 3; ++d) g[d] = gradU[(b * 3 + c) * 3 + d];
+    }
+    stress[f] = ((-1.0) * gam) * dot3(g, &m.Sf[f * 3]);
+  }
   double out[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (f < Fi) {
     const int32_t o = m.owner[f], nb = m.neighbour[f];
@@ -470,9 +487,9 @@ __device__ __forceinline__ void diff_entry(const UEntry &x, int32_t Fi, double *
 __global__ void __launch_bounds__(kBlock)
 k_assemble_u(DevMesh m, int c, double urf, const double *__restrict__ U,
              const double *__restrict__ gradp, const double *__restrict__ mdot,
-             const double *__restrict__ uf, double *ac_out, double *anb, double *bc_out,
-             double *acold_out, double *dphi, double *DU, double *DUT, double *partial,
-             double *scal) {
+             const double *__restrict__ uf, const double *__restrict__ stress, double *ac_out,
+             double *anb, double *bc_out, double *acold_out, double *dphi, double *DU,
+             double *DUT, double *partial, double *scal) {
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   double red[2] = {0.0, 0.0};
   if (i < m.N) {
@@ -510,6 +527,22 @@ k_assemble_u(DevMesh m, int c, double urf, const double *__restrict__ U,
       for (int32_t e = e0; e < e1; ++e) {
         load_uentry(m, uf, mdot, e, x);
         diff_entry(x, Fi, anb, ac, bc);
+      }
+    }
+    if (stress) {
+      // the explicit stress fluxes, scattered as the reference's face loop
+      // does (C1 = C2 = 0: the diagonal takes +-0, the off-diagonals keep
+      // their values)
+      for (int32_t e = e0; e < e1; ++e) {
+        const int32_t f = m.cf_face[e];
+        const double Ts = stress[f];
+        if (f < Fi && m.cf_slot[e] < 0) {
+          ac = ac - 0.0;
+          bc = bc + Ts;
+        } else {
+          ac = ac + 0.0;
+          bc = bc - Ts;
+        }
       }
     }
     const double vol = m.vol[i];
@@ -555,14 +588,14 @@ __global__ void k_du_bnd(DevMesh m, int c, double *DU, double *DUT) {
 
 void assemble_u(hipStream_t s, const DevMesh &m, int comp, double urf, const double *U,
                 const double *gradU, const double *gradp, const double *mdot,
-                const double *gamma, double *uf, double *ac, double *anb, double *bc,
-                double *ac_old, double *dphi, double *DU, double *DUT, double *partial,
+                const double *gamma, double *uf, double *stress, double *ac, double *anb,
+                double *bc, double *ac_old, double *dphi, double *DU, double *DUT, double *partial,
                 unsigned *counter, double *scal) {
   (void)counter;
   hipLaunchKernelGGL(k_u_faces, dim3(nblocks(m.F)), dim3(kBlock), 0, s, m, comp, U, gradU, mdot,
-                     gamma, uf);
+                     gamma, uf, stress);
   hipLaunchKernelGGL(k_assemble_u, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, comp, urf, U, gradp,
-                     mdot, uf, ac, anb, bc, ac_old, dphi, DU, DUT, partial, scal);
+                     mdot, uf, stress, ac, anb, bc, ac_old, dphi, DU, DUT, partial, scal);
   hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kBlock), 0, s, partial, nblocks(m.N), 1, scal);
   const int32_t Fb = m.F - m.Fi;
   if (Fb > 0) hipLaunchKernelGGL(k_du_bnd, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, comp, DU, DUT);

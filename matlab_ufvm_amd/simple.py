@@ -44,10 +44,12 @@ def case_structs(mesh, bcs, nu, urf_u=0.7, urf_p=0.3, n_correctors=2, p_ref=0, u
                  p_solver="amg", p_cycle="W-Cycle", p_max_cycles=30, p_termination=0.1,
                  p_max_coarse_levels=10, p_pre=1, p_post=3, p_smoother="ILU", pcg_precision=32,
                  pcg_tol=1e-8, pcg_sweeps=1, pcg_max_coarse_cells=2000, pcg_coarse_scale=2.5,
-                 u_smoother="ILU", pcg_coarse_sweeps=2):
+                 u_smoother="ILU", pcg_coarse_sweeps=2, stress=False):
     """(MeshDesc, SimpleCfg, keep): the C structs of ufvm_case_create; the arrays
     they point to live in `keep`, which must outlive the call.  u_smoother:
-    "ILU" (the reference's DILU momentum solve) or "MCGS" (multicolour GS)."""
+    "ILU" (the reference's DILU momentum solve) or "MCGS" (multicolour GS);
+    stress: the momentum equation has + div(nu*transp(grad(U))), as the
+    reference's tutorials do."""
     m = mesh
     keep = []
 
@@ -81,6 +83,7 @@ def case_structs(mesh, bcs, nu, urf_u=0.7, urf_p=0.3, n_correctors=2, p_ref=0, u
     c.p_pcg = PcgCfg(pcg_precision, pcg_sweeps, pcg_sweeps, 1, pcg_max_coarse_cells, -1, 500,
                      pcg_tol, 0, pcg_coarse_scale, pcg_coarse_sweeps, pcg_coarse_sweeps)
     c.u_smoother = _lib.SMOOTHER[u_smoother]
+    c.stress_term = int(bool(stress))
     return d, c, keep
 
 
@@ -89,7 +92,7 @@ class SimpleCase:
                  p_ref=0, u_sweeps=10, p_solver="amg", p_cycle="W-Cycle", p_max_cycles=30,
                  p_termination=0.1, p_max_coarse_levels=10, p_pre=1, p_post=3, p_smoother="ILU",
                  pcg_precision=32, pcg_tol=1e-8, pcg_sweeps=1, pcg_max_coarse_cells=2000,
-                 pcg_coarse_scale=2.5, u_smoother="ILU", pcg_coarse_sweeps=2):
+                 pcg_coarse_scale=2.5, u_smoother="ILU", pcg_coarse_sweeps=2, stress=False):
         self.ctx = ctx
         self.mesh = mesh
         self.bcs = bcs
@@ -97,7 +100,7 @@ class SimpleCase:
             mesh, bcs, nu, urf_u, urf_p, n_correctors, p_ref, u_sweeps, p_solver, p_cycle,
             p_max_cycles, p_termination, p_max_coarse_levels, p_pre, p_post, p_smoother,
             pcg_precision, pcg_tol, pcg_sweeps, pcg_max_coarse_cells, pcg_coarse_scale,
-            u_smoother, pcg_coarse_sweeps)
+            u_smoother, pcg_coarse_sweeps, stress)
         cid = C.c_int32()
         ctx.check(ctx.lib.ufvm_case_create(ctx.h, C.byref(d), C.byref(c), C.byref(cid)))
         self.id = cid.value

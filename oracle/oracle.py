@@ -268,7 +268,7 @@ class OCfg(C.Structure):
                 ("n_correctors", C.c_int32), ("p_ref", C.c_int32), ("u_sweeps", C.c_int32),
                 ("p_cycle", C.c_int32), ("p_max_cycles", C.c_int32), ("p_max_coarse", C.c_int32),
                 ("p_pre", C.c_int32), ("p_post", C.c_int32), ("p_smoother", C.c_int32),
-                ("p_termination", C.c_double), ("u_smoother", C.c_int32)]
+                ("p_termination", C.c_double), ("u_smoother", C.c_int32), ("stress", C.c_int32)]
 
 
 class OState(C.Structure):
@@ -293,7 +293,8 @@ class SimpleCase:
 
     def __init__(self, mesh, bcs, nu, urf_u=0.7, urf_p=0.3, n_correctors=2, p_ref=0,
                  u_sweeps=10, p_cycle="W-Cycle", p_max_cycles=30, p_termination=0.1,
-                 p_max_coarse=10, p_pre=1, p_post=3, p_smoother="ILU", u_smoother="ILU"):
+                 p_max_coarse=10, p_pre=1, p_post=3, p_smoother="ILU", u_smoother="ILU",
+                 stress=False):
         L = lib()
         for name, args in [("orc_simple_init", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState)]),
                            ("orc_simple_free", [C.POINTER(OState)]),
@@ -335,7 +336,7 @@ class SimpleCase:
         self.cfg = OCfg(nu, urf_u, urf_p, n_correctors, p_ref, u_sweeps,
                         {"V-Cycle": 0, "W-Cycle": 1, "F-Cycle": 2}[p_cycle], p_max_cycles,
                         p_max_coarse, p_pre, p_post, {"ILU": 0, "SOR": 1}[p_smoother], p_termination,
-                        {"ILU": 0, "MCGS": 2}[u_smoother])
+                        {"ILU": 0, "MCGS": 2}[u_smoother], int(bool(stress)))
         self.S = OState()
         L.orc_simple_init(C.byref(self.M), C.byref(self.cfg), C.byref(self.S))
 

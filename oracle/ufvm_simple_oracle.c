@@ -362,6 +362,30 @@ void orc_assemble_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
     }
     scatter_faces(m, s, C1, C2, T);
   }
+  /* ---- + div(nu*transp(grad(U))), explicit, sign -1
+   * (cfdAssembleExplicitStressTerm.m:14-52): on every face
+   * T = (sign gamma_f) ((g_x Sf_x + g_y Sf_y) + g_z Sf_z), g_d the Average
+   * interpolation of phiGradient(:,d,comp) = d(U_comp)/dx_d (boundary faces:
+   * the boundary element's row), i.e. grad(U_comp).Sf and not the transpose's
+   * sum_j d(U_j)/dx_comp Sf_j, as the reference computes it */
+  if (c->stress) {
+    const double sg = -1.0;
+    for (int32_t f = 0; f < F; ++f) {
+      double g[3];
+      if (f < Fi) {
+        const int32_t o = m->owner[f], nb = m->neighbour[f];
+        const double w = m->gf[f];
+        for (int d = 0; d < 3; ++d)
+          g[d] = w * s->gradU[(nb * 3 + comp) * 3 + d] + (1 - w) * s->gradU[(o * 3 + comp) * 3 + d];
+      } else {
+        const int32_t b = N + (f - Fi);
+        for (int d = 0; d < 3; ++d) g[d] = s->gradU[(b * 3 + comp) * 3 + d];
+      }
+      C1[f] = C2[f] = 0.0;
+      T[f] = (sg * s->gamma_f[f]) * dot3(g, &m->Sf[f * 3]);
+    }
+    scatter_faces(m, s, C1, C2, T);
+  }
   /* ---- -grad(p) on the right-hand side, sign +1
    * (cfdAssemblePressureGradientTerm.m:31-40), element fluxes
    * (cfdAssembleIntoGlobalMatrixElementFluxes.m:23-27) */

@@ -47,6 +47,7 @@ typedef struct {
   int32_t p_cycle, p_max_cycles, p_max_coarse, p_pre, p_post, p_smoother;
   double p_termination;
   int32_t u_smoother;     /* momentum solve: 0 the reference's DILU, 2 multicolour GS */
+  int32_t stress;         /* 1: the momentum equation has + div(nu*transp(grad(U))) */
 } orc_simple_cfg;
 
 typedef struct {

@@ -156,6 +156,24 @@ def test_noslip_walls_bitexact(ctx):
     g.close()
 
 
+def test_explicit_stress_term_bitexact(ctx):
+    # + div(nu*transp(grad(U))) (the reference's tutorials' momentum equation)
+    # on the GPU gives the oracle's fields bit for bit over SIMPLE iterations
+    m = cavity_mesh(10, jitter=0.15, seed=73, renumber_seed=74)
+    g = SimpleCase(ctx, m, cavity_bcs(m), nu=0.005, stress=True, **AMG)
+    o = oracle.SimpleCase(m, oracle.cavity_bcs(m), nu=0.005, stress=True, **AMG)
+    g.iterate(4)
+    for _ in range(4):
+        o.iterate(1)
+    for name, oname in (("U", "U"), ("p", "p"), ("mdot_f", "mdot")):
+        same(g.field(name), o.field(oname), f"stress term {name}")
+    f = SimpleCase(ctx, m, cavity_bcs(m), nu=0.005, **AMG)
+    f.iterate(4)
+    assert not np.array_equal(f.field("U"), g.field("U"))
+    f.close()
+    g.close()
+
+
 def test_frozen_hierarchy_pressure_solver(ctx):
     # "pcg-frozen" keeps the first solve's hierarchy and refreshes its values
     # on the GPU after every assembly; it reaches the fields of the

@@ -119,6 +119,41 @@ def test_noslip_wall_closed_forms():
     assert np.all(U[b0:b0 + fw.nfaces] == 0.0)
 
 
+def test_explicit_stress_term_pins():
+    # + div(nu*transp(grad(U))) (cfdAssembleExplicitStressTerm.m:14-52) is
+    # sum over faces of -nu_f (Average grad(U_c))_f . Sf.  (1) For a linear U on
+    # a uniform grid every cell away from the boundary has an exact gradient,
+    # so its face fluxes sum to -nu G . sum Sf = 0 over the closed cell.  (2)
+    # The reference takes phiGradient(:,d,comp) = d(U_comp)/dx_d, the gradient
+    # of U_comp itself, so component x does not see U_y (a true transpose term
+    # would, through d(U_y)/dx).
+    n, nu = 8, 0.01
+    m = cavity_mesh(n, nz=1, jitter=0.0, renumber_seed=None)
+    c = m.centroid
+    NB = m.n_cells + m.n_bfaces
+    pts = np.concatenate([c[:m.n_cells], m.face_centroid[m.n_interior:]])
+    assert pts.shape[0] == NB
+
+    def bc_x(stress, uy):
+        case = oracle.SimpleCase(m, oracle.cavity_bcs(m), nu=nu, stress=stress)
+        U = np.zeros((NB, 3))
+        U[:, 0] = 0.3 * pts[:, 0] + 0.7 * pts[:, 1]
+        U[:, 1] = uy(pts)
+        case.set_field("U", U)
+        case.update_fields()
+        case.assemble_u(0)
+        return case.system()["bc"]
+    h = 1.0 / n
+    deep = (c[:m.n_cells, 0] > 2 * h) & (c[:m.n_cells, 0] < 1 - 2 * h) & \
+           (c[:m.n_cells, 1] > 2 * h) & (c[:m.n_cells, 1] < 1 - 2 * h)
+    zero = lambda p: 0.0 * p[:, 0]
+    off, on = bc_x(False, zero), bc_x(True, zero)
+    assert np.abs(on - off)[deep].max() <= 1e-15
+    assert np.abs(on - off)[~deep].max() > 1e-6      # the term is active at the walls
+    wavy = lambda p: np.sin(3 * p[:, 0]) * np.cos(2 * p[:, 1])
+    assert np.array_equal(bc_x(True, wavy), bc_x(True, zero))
+
+
 def test_gradient_of_linear_field_is_exact():
     m = cavity_mesh(6, nz=1, jitter=0.0, renumber_seed=None)
     case = oracle.SimpleCase(m, oracle.cavity_bcs(m), nu=0.01)
