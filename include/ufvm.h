@@ -203,12 +203,18 @@ int ufvm_pcg_level_colours(ufvm_ctx *ctx, int32_t sys, int32_t level, int32_t *n
  * boundary faces follow, grouped by patch. */
 #define UFVM_PATCH_WALL 0
 #define UFVM_PATCH_EMPTY 1
+/* a symmetry plane (U and p conditions UFVM_BC_SYMMETRY): on the steady path
+ * the reference's symmetry branches do its empty branches' arithmetic
+ * (cfdAssembleDiffusionTerm.m:567-626, cfdCorrectScalarEquation.m:469-505,
+ * cfdCorrectVelocityField.m:274-305, cfdCorrectMdotField.m:306-322) */
+#define UFVM_PATCH_SYMMETRY 3
 #define UFVM_BC_FIXED_VALUE 0
 #define UFVM_BC_ZERO_GRADIENT 1
 #define UFVM_BC_EMPTY 2
 /* U on a wall: zero, with the reference's vector noSlip wall diffusion
  * (cfdAssembleDiffusionTerm.m:479-562, TM from cfdGetUCoef.m:41-44) */
 #define UFVM_BC_NO_SLIP 3
+#define UFVM_BC_SYMMETRY 4
 typedef struct {
   int32_t n_cells, n_faces, n_interior;
   const int32_t *owner, *neighbour;

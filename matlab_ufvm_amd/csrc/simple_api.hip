@@ -78,7 +78,7 @@ void init_fields(ufvm_ctx *ctx, Case &C) {
       // noSlip's value '[0;0;0]' (cfdDefineEquation.m:290-295), set once
       // (cfdInitializeBoundaryConditions.m:239-267) and never updated
       for (int d = 0; d < 3; ++d) U[b * 3 + d] = 0.0;
-    } else if (C.h_u_bc[p] == UFVM_BC_EMPTY) {
+    } else if (C.h_u_bc[p] == UFVM_BC_EMPTY || C.h_u_bc[p] == UFVM_BC_SYMMETRY) {
       const double *S = &Sf[f * 3];
       const double nm = std::sqrt((S[0] * S[0] + S[1] * S[1]) + S[2] * S[2]);
       double n[3] = {S[0] / nm, S[1] / nm, S[2] / nm};
@@ -264,14 +264,16 @@ int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *m, const ufvm_simple_c
       if (st < C.Fi || nf < 0 || st + nf > C.F) throw Error(UFVM_ERR_ARG, "bad patch range");
       for (int32_t f = st; f < st + nf; ++f) bp[f - C.Fi] = p;
       const int t = m->patch_type[p];
-      if (t != UFVM_PATCH_WALL && t != UFVM_PATCH_EMPTY)
-        throw Error(UFVM_ERR_UNSUPPORTED, "only wall and empty patches are supported");
+      if (t != UFVM_PATCH_WALL && t != UFVM_PATCH_EMPTY && t != UFVM_PATCH_SYMMETRY)
+        throw Error(UFVM_ERR_UNSUPPORTED, "only wall, empty and symmetry patches are supported");
       const int ub = cfg->u_bc[p], pb = cfg->p_bc[p];
       if (t == UFVM_PATCH_WALL &&
           !((ub == UFVM_BC_FIXED_VALUE || ub == UFVM_BC_NO_SLIP) && pb == UFVM_BC_ZERO_GRADIENT))
         throw Error(UFVM_ERR_UNSUPPORTED, "walls need U fixedValue or noSlip and p zeroGradient");
       if (t == UFVM_PATCH_EMPTY && !(ub == UFVM_BC_EMPTY && pb == UFVM_BC_EMPTY))
         throw Error(UFVM_ERR_UNSUPPORTED, "empty patches need empty U and p conditions");
+      if (t == UFVM_PATCH_SYMMETRY && !(ub == UFVM_BC_SYMMETRY && pb == UFVM_BC_SYMMETRY))
+        throw Error(UFVM_ERR_UNSUPPORTED, "symmetry patches need symmetry U and p conditions");
     }
     for (int32_t k = 0; k < C.Fb; ++k)
       if (bp[k] < 0) throw Error(UFVM_ERR_ARG, "boundary face outside every patch");

@@ -74,6 +74,16 @@ k_reduce2(const double *__restrict__ partial, int32_t nblk, int mode, double *sc
 // ------------------------------------------------------ boundary conditions
 
 // fixedValue walls: U_b = value (cfdUpdateBoundaryConditions.m:103-139)
+// empty and symmetry patches: every symmetry branch of the reference's steady
+// path does its empty branch's arithmetic (diffusion
+// cfdAssembleDiffusionTerm.m:567-626 vs :631-690, zero convection, no gradient
+// update, U_b = U_O - (U_O . n) n in cfdCorrectScalarEquation.m:469-505 and
+// cfdCorrectVelocityField.m:274-305, mdot_b = 0 mdot_b in
+// cfdCorrectMdotField.m:306-322, owner copies of p and PP)
+__device__ inline bool plane(const DevMesh &m, int p) {
+  return m.ptype[p] == PATCH_EMPTY || m.ptype[p] == PATCH_SYMMETRY;
+}
+
 __global__ void k_update_u_bc(DevMesh m, double *U) {
   const int32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= m.F - m.Fi) return;
@@ -378,7 +388,7 @@ k_u_faces(DevMesh m, int c, const double *__restrict__ U, const double *__restri
                              ((U[i * 3 + j] - U[b * 3 + j]) * n[j]) * n[c]) -
                             ((U[i * 3 + k2] - U[b * 3 + k2]) * n[k2]) * n[c]);
       T = (C1 * U[i * 3 + c] + C2 * U[b * 3 + c]) + V;
-    } else if (m.ptype[p] == PATCH_EMPTY) {
+    } else if (plane(m, p)) {
       const double A = m.area[f];
       double n[3], l[3] = {1.0, 1.0, 1.0}, a[3];
       l[c] = 0.0;
@@ -612,7 +622,7 @@ __global__ void k_correct_u_cells(DevMesh m, int c, const double *dphi, double *
 __global__ void k_empty_u(DevMesh m, int c, double *U) {
   const int32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= m.F - m.Fi) return;
-  if (m.ptype[m.bpatch[k]] != PATCH_EMPTY) return;
+  if (!plane(m, m.bpatch[k])) return;
   const int32_t f = m.Fi + k, o = m.owner[f], b = m.N + k;
   double n[3];
   const double nm = mag3(&m.Sf[f * 3]);
@@ -805,7 +815,7 @@ __global__ void k_correct_mdot(DevMesh m, const double *fluxC1, const double *fl
   if (f >= m.F) return;
   if (f < m.Fi) {
     mdot[f] = (mdot[f] + fluxC1[f] * PP[m.owner[f]]) + fluxC2[f] * PP[m.neighbour[f]];
-  } else if (m.ptype[m.bpatch[f - m.Fi]] == PATCH_EMPTY) {
+  } else if (plane(m, m.bpatch[f - m.Fi])) {
     mdot[f] = 0.0 * mdot[f];
   }
 }

@@ -20,19 +20,19 @@ import numpy as np
 from . import _lib
 from ._lib import AmgCfg, MeshDesc, PcgCfg, SimpleCfg, f64p, i32p
 
-PATCH = {"wall": 0, "empty": 1}
-BC = {"fixedValue": 0, "zeroGradient": 1, "empty": 2, "noSlip": 3}
+PATCH = {"wall": 0, "empty": 1, "symmetry": 3}
+BC = {"fixedValue": 0, "zeroGradient": 1, "empty": 2, "noSlip": 3, "symmetry": 4}
 P_SOLVER = {"amg": 0, "pcg": 1, "pcg-frozen": 2}
 
 
 def cavity_bcs(mesh, lid=(1.0, 0.0, 0.0), walls="fixedValue"):
     """Lid-driven cavity: the lid fixedValue U, the fixed walls fixedValue
     (0 0 0) or noSlip (`walls`), zeroGradient p, empty front/back patches of
-    2-D meshes."""
+    2-D meshes, symmetry conditions on symmetry planes."""
     out = {}
     for p in mesh.patches:
-        if p.type == "empty":
-            out[p.name] = ("empty", (0.0, 0.0, 0.0), "empty")
+        if p.type in ("empty", "symmetry"):
+            out[p.name] = (p.type, (0.0, 0.0, 0.0), p.type)
         elif p.name == "movingWall":
             out[p.name] = ("fixedValue", tuple(lid), "zeroGradient")
         else:

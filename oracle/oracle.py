@@ -280,8 +280,8 @@ class OState(C.Structure):
                 ("r0", C.c_double * 4), ("r1", C.c_double * 4), ("ncycles", C.c_int32)]
 
 
-PATCH_TYPE = {"wall": 0, "empty": 1, "patch": 2}
-BC = {"fixedValue": 0, "zeroGradient": 1, "empty": 2, "noSlip": 3}
+PATCH_TYPE = {"wall": 0, "empty": 1, "patch": 2, "symmetry": 3}
+BC = {"fixedValue": 0, "zeroGradient": 1, "empty": 2, "noSlip": 3, "symmetry": 4}
 
 
 class SimpleCase:
@@ -408,11 +408,12 @@ class SimpleCase:
 
 def cavity_bcs(mesh, lid=(1.0, 0.0, 0.0), walls="fixedValue"):
     """Lid-driven cavity boundary conditions (the lid fixedValue, the fixed
-    walls fixedValue (0 0 0) or noSlip, zeroGradient p, empty front/back)."""
+    walls fixedValue (0 0 0) or noSlip, zeroGradient p, empty front/back,
+    symmetry conditions on symmetry patches)."""
     out = {}
     for p in mesh.patches:
-        if p.type == "empty":
-            out[p.name] = ("empty", (0.0, 0.0, 0.0), "empty")
+        if p.type in ("empty", "symmetry"):
+            out[p.name] = (p.type, (0.0, 0.0, 0.0), p.type)
         elif p.name == "movingWall":
             out[p.name] = ("fixedValue", lid, "zeroGradient")
         else:

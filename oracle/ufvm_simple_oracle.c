@@ -19,6 +19,20 @@
 static double *dz(size_t n) { return (double *)calloc(n ? n : 1, sizeof(double)); }
 
 static double mag3(const double *v) { return sqrt((v[0] * v[0] + v[1] * v[1]) + v[2] * v[2]); }
+
+/* empty and symmetry patches: on the steady path every symmetry branch of the
+ * reference does the empty branch's arithmetic -- diffusion
+ * (cfdAssembleDiffusionTerm.m:567-626 vs :631-690), zero convection
+ * (cfdAssembleConvectionTerm.m:285-300), no gradient update
+ * (cfdUpdateGradient.m:260), no mdot assembly (cfdAssembleMdotTerm.m:342),
+ * U_b = U_O - (U_O . n) n at initialisation and in both corrections
+ * (cfdInitializeBoundaryConditions.m:471-530, cfdCorrectScalarEquation.m:469-505,
+ * cfdCorrectVelocityField.m:274-305), p_b and PP_b owner copies
+ * (cfdCorrectPressureEquation.m:238-261, cfdCorrectPPField.m:278-300), mdot_b =
+ * 0 mdot_b (cfdCorrectMdotField.m:306-322) */
+static int plane(const orc_mesh *m, int p) {
+  return m->ptype[p] == ORC_PATCH_EMPTY || m->ptype[p] == ORC_PATCH_SYMMETRY;
+}
 static double dot3(const double *a, const double *b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
 
 /* patch of boundary face f (absolute index) */
@@ -85,7 +99,7 @@ void orc_simple_init(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_stat
         /* the noSlip value '[0;0;0]' (cfdDefineEquation.m:290-295), set once
          * (cfdInitializeBoundaryConditions.m:239-267) and never updated */
         for (int d = 0; d < 3; ++d) s->U[b * 3 + d] = 0.0;
-      } else if (m->u_bc[p] == ORC_BC_EMPTY) {
+      } else if (m->u_bc[p] == ORC_BC_EMPTY || m->u_bc[p] == ORC_BC_SYMMETRY) {
         double n[3];
         const double nm = mag3(&m->Sf[f * 3]);
         for (int d = 0; d < 3; ++d) n[d] = m->Sf[f * 3 + d] / nm;
@@ -341,7 +355,7 @@ void orc_assemble_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
                                  ((U[o * 3 + j] - U[b * 3 + j]) * n[j]) * n[comp]) -
                                 ((U[o * 3 + k] - U[b * 3 + k]) * n[k]) * n[comp]);
           T[f] = (C1[f] * U[o * 3 + comp] + C2[f] * U[b * 3 + comp]) + V;
-        } else if (m->ptype[p] == ORC_PATCH_EMPTY) {
+        } else if (plane(m, p)) {
           const double A = m->area[f];
           double n[3], l[3] = {1.0, 1.0, 1.0};
           l[comp] = 0.0;
@@ -453,7 +467,7 @@ void orc_correct_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state 
   (void)c;
   for (int32_t i = 0; i < N; ++i) s->U[i * 3 + comp] = s->U[i * 3 + comp] + L->dphi[i];
   for (int p = 0; p < m->npatch; ++p) {
-    if (m->ptype[p] != ORC_PATCH_EMPTY) continue;
+    if (!plane(m, p)) continue;
     for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
       const int32_t b = N + (f - m->Fi), o = m->owner[f];
       double n[3];
@@ -561,7 +575,7 @@ void orc_correct_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state 
     for (int d = 0; d < 3; ++d)
       s->U[i * 3 + d] = s->U[i * 3 + d] - s->DU[i * 3 + d] * s->gradPP[i * 3 + d];
   for (int p = 0; p < m->npatch; ++p) {
-    if (m->ptype[p] != ORC_PATCH_EMPTY) continue;
+    if (!plane(m, p)) continue;
     for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
       const int32_t b = N + (f - Fi), o = m->owner[f];
       double n[3];
@@ -581,7 +595,7 @@ void orc_correct_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state 
     s->mdot[f] = (s->mdot[f] + s->fluxC1[f] * s->PP[m->owner[f]]) +
                  s->fluxC2[f] * s->PP[m->neighbour[f]];
   for (int p = 0; p < m->npatch; ++p) {
-    if (m->ptype[p] != ORC_PATCH_EMPTY) continue;
+    if (!plane(m, p)) continue;
     for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) s->mdot[f] = 0.0 * s->mdot[f];
   }
 }

@@ -20,10 +20,11 @@
 extern "C" {
 #endif
 
-enum { ORC_PATCH_WALL = 0, ORC_PATCH_EMPTY = 1, ORC_PATCH_PATCH = 2 };
+enum { ORC_PATCH_WALL = 0, ORC_PATCH_EMPTY = 1, ORC_PATCH_PATCH = 2, ORC_PATCH_SYMMETRY = 3 };
 /* ORC_BC_NO_SLIP: U = 0 on the wall and the reference's vector noSlip wall
  * diffusion (cfdAssembleDiffusionTerm.m:479-562, cfdGetUCoef.m:41-44) */
-enum { ORC_BC_FIXED_VALUE = 0, ORC_BC_ZERO_GRADIENT = 1, ORC_BC_EMPTY = 2, ORC_BC_NO_SLIP = 3 };
+enum { ORC_BC_FIXED_VALUE = 0, ORC_BC_ZERO_GRADIENT = 1, ORC_BC_EMPTY = 2, ORC_BC_NO_SLIP = 3,
+       ORC_BC_SYMMETRY = 4 };
 
 typedef struct {
   int32_t N, F, Fi;

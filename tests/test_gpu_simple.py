@@ -156,6 +156,26 @@ def test_noslip_walls_bitexact(ctx):
     g.close()
 
 
+def test_symmetry_plane_bitexact(ctx):
+    # a symmetry plane (z = 0) of a jittered 3-D cavity: the GPU gives the
+    # oracle's fields bit for bit
+    from matlab_ufvm_amd.mesh import process_mesh
+    from matlab_ufvm_amd.meshgen import box_hex
+    spec = [("movingWall", "wall", ["y1"]), ("fixedWalls", "wall", ["x0", "x1", "y0", "z1"]),
+            ("midPlane", "symmetry", ["z0"])]
+    m = process_mesh(box_hex(8, 8, 8, spec, jitter=0.15, seed=81, renumber_seed=82),
+                     apex="reference")
+    g = SimpleCase(ctx, m, cavity_bcs(m), nu=0.01, **AMG)
+    o = oracle.SimpleCase(m, oracle.cavity_bcs(m), nu=0.01, **AMG)
+    res = g.iterate(4)
+    for _ in range(4):
+        o.iterate(1)
+    for name, oname in (("U", "U"), ("p", "p"), ("mdot_f", "mdot")):
+        same(g.field(name), o.field(oname), f"symmetry {name}")
+    assert np.all(np.isfinite(res))
+    g.close()
+
+
 def test_explicit_stress_term_bitexact(ctx):
     # + div(nu*transp(grad(U))) (the reference's tutorials' momentum equation)
     # on the GPU gives the oracle's fields bit for bit over SIMPLE iterations

@@ -119,6 +119,68 @@ def test_noslip_wall_closed_forms():
     assert np.all(U[b0:b0 + fw.nfaces] == 0.0)
 
 
+SYMMETRY_3D = [("movingWall", "wall", ["y1"]), ("fixedWalls", "wall", ["x0", "x1", "y0", "z1"]),
+               ("midPlane", "symmetry", ["z0"])]
+
+
+def _box(n, nz, spec, **kw):
+    from matlab_ufvm_amd.mesh import process_mesh
+    from matlab_ufvm_amd.meshgen import box_hex
+    lz = 1.0 / n if nz == 1 else 1.0
+    return process_mesh(box_hex(n, n, nz, spec, lengths=(1.0, 1.0, lz), **kw), apex="reference")
+
+
+def test_symmetry_plane_closed_forms():
+    # the reference's symmetry diffusion (cfdAssembleDiffusionTerm.m:567-626):
+    # C1 = 2 nu A n_c^2 / d, d = CN.n = h/2, and V = 0 when n is along an axis.
+    # A cell beside the z = 0 plane (and no other boundary) of a uniform box
+    # has, from rest, diagonal urf^-1 (5 nu A/h + V/fdt) for U_x (the plane
+    # slips) and urf^-1 (9 nu A/h + V/fdt) for U_z (4 nu A/h from the plane)
+    n, nu = 6, 0.01
+    m = _box(n, n, SYMMETRY_3D)
+    case = oracle.SimpleCase(m, oracle.cavity_bcs(m), nu=nu)
+    h, A, V = 1.0 / n, (1.0 / n) ** 2, (1.0 / n) ** 3
+    c = m.centroid[:m.n_cells]
+    inner = (c[:, 0] > h) & (c[:, 0] < 1 - h) & (c[:, 1] > h) & (c[:, 1] < 1 - h)
+    beside = np.where(inner & (np.abs(c[:, 2] - h / 2) < 1e-12))[0]
+    assert beside.size == (n - 2) ** 2
+    case.update_fields()
+    for comp, k in ((0, 5), (1, 5), (2, 9)):
+        case.assemble_u(comp)
+        ac = case.system()["ac"]
+        assert np.allclose(ac[beside], (k * nu * A / h + V / 1e9) / 0.7, rtol=1e-12), comp
+    # after iterations: no flux through the plane, U_b tangential (U_O minus
+    # its normal part: the plane slips), p_b the owner's
+    case.iterate(5)
+    U, p, mdot = case.field("U"), case.field("p"), case.field("mdot")
+    sym = next(q for q in m.patches if q.type == "symmetry")
+    f = np.arange(sym.start, sym.start + sym.nfaces)
+    b, o = m.n_cells + f - m.n_interior, m.owner[f]
+    nrm = m.Sf[f] / np.linalg.norm(m.Sf[f], axis=1)[:, None]
+    assert np.all(mdot[f] == 0.0)
+    assert np.abs((U[b] * nrm).sum(1)).max() < 1e-15
+    assert np.array_equal(U[b][:, :2], U[o][:, :2])
+    assert np.abs(U[b][:, 0]).max() > 1e-3
+    assert np.array_equal(p[b], p[o])
+
+
+def test_symmetry_labels_equal_empty_on_2d_planes():
+    # on the steady path every symmetry branch of the reference does its empty
+    # branch's arithmetic, so the 2-D cavity with its front and back planes
+    # labelled symmetry gives the empty-labelled case's bits
+    spec = [("movingWall", "wall", ["y1"]), ("fixedWalls", "wall", ["x0", "x1", "y0"]),
+            ("frontAndBack", "symmetry", ["z0", "z1"])]
+    me = cavity_mesh(12, nz=1, jitter=0.0, renumber_seed=None)
+    ms = _box(12, 1, spec)
+    a = oracle.SimpleCase(me, oracle.cavity_bcs(me), nu=0.01)
+    b = oracle.SimpleCase(ms, oracle.cavity_bcs(ms), nu=0.01)
+    assert oracle.cavity_bcs(ms)["frontAndBack"] == ("symmetry", (0.0, 0.0, 0.0), "symmetry")
+    a.iterate(5)
+    b.iterate(5)
+    for name in ("U", "p", "mdot"):
+        assert np.array_equal(a.field(name), b.field(name)), name
+
+
 def test_explicit_stress_term_pins():
     # + div(nu*transp(grad(U))) (cfdAssembleExplicitStressTerm.m:14-52) is
     # sum over faces of -nu_f (Average grad(U_c))_f . Sf.  (1) For a linear U on
