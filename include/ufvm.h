@@ -215,6 +215,11 @@ int ufvm_pcg_level_colours(ufvm_ctx *ctx, int32_t sys, int32_t level, int32_t *n
  * (cfdAssembleDiffusionTerm.m:479-562, TM from cfdGetUCoef.m:41-44) */
 #define UFVM_BC_NO_SLIP 3
 #define UFVM_BC_SYMMETRY 4
+/* U on a wall: slip -- no diffusion or mass flux through the wall, and U_b the
+ * owner's U minus its normal part after each correction
+ * (cfdAssembleDiffusionTerm.m:456-478, cfdCorrectScalarEquation.m:239-275,
+ * cfdCorrectVelocityField.m:133-160, cfdCorrectMdotField.m:132-149) */
+#define UFVM_BC_SLIP 5
 typedef struct {
   int32_t n_cells, n_faces, n_interior;
   const int32_t *owner, *neighbour;

@@ -268,8 +268,10 @@ int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *m, const ufvm_simple_c
         throw Error(UFVM_ERR_UNSUPPORTED, "only wall, empty and symmetry patches are supported");
       const int ub = cfg->u_bc[p], pb = cfg->p_bc[p];
       if (t == UFVM_PATCH_WALL &&
-          !((ub == UFVM_BC_FIXED_VALUE || ub == UFVM_BC_NO_SLIP) && pb == UFVM_BC_ZERO_GRADIENT))
-        throw Error(UFVM_ERR_UNSUPPORTED, "walls need U fixedValue or noSlip and p zeroGradient");
+          !((ub == UFVM_BC_FIXED_VALUE || ub == UFVM_BC_NO_SLIP || ub == UFVM_BC_SLIP) &&
+            pb == UFVM_BC_ZERO_GRADIENT))
+        throw Error(UFVM_ERR_UNSUPPORTED,
+                    "walls need U fixedValue, noSlip or slip and p zeroGradient");
       if (t == UFVM_PATCH_EMPTY && !(ub == UFVM_BC_EMPTY && pb == UFVM_BC_EMPTY))
         throw Error(UFVM_ERR_UNSUPPORTED, "empty patches need empty U and p conditions");
       if (t == UFVM_PATCH_SYMMETRY && !(ub == UFVM_BC_SYMMETRY && pb == UFVM_BC_SYMMETRY))

@@ -10,7 +10,8 @@ namespace ufvm {
 namespace sk {
 
 enum { PATCH_WALL = 0, PATCH_EMPTY = 1, PATCH_PATCH = 2, PATCH_SYMMETRY = 3 };
-enum { BC_FIXED_VALUE = 0, BC_ZERO_GRADIENT = 1, BC_EMPTY = 2, BC_NO_SLIP = 3, BC_SYMMETRY = 4 };
+enum { BC_FIXED_VALUE = 0, BC_ZERO_GRADIENT = 1, BC_EMPTY = 2, BC_NO_SLIP = 3, BC_SYMMETRY = 4,
+       BC_SLIP = 5 };
 
 // device pointers of the mesh, passed by value
 struct DevMesh {

@@ -84,6 +84,14 @@ __device__ inline bool plane(const DevMesh &m, int p) {
   return m.ptype[p] == PATCH_EMPTY || m.ptype[p] == PATCH_SYMMETRY;
 }
 
+// patches whose corrections make U_b tangential and zero mdot_b: empty and
+// symmetry planes, and slip walls (cfdCorrectScalarEquation.m:239-275,
+// cfdCorrectVelocityField.m:133-160, cfdCorrectMdotField.m:132-149); a slip
+// wall's diffusion and mass fluxes are zero and its gradient rows are a wall's
+__device__ inline bool tangential(const DevMesh &m, int p) {
+  return plane(m, p) || (m.ptype[p] == PATCH_WALL && m.u_bc[p] == BC_SLIP);
+}
+
 __global__ void k_update_u_bc(DevMesh m, double *U) {
   const int32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= m.F - m.Fi) return;
@@ -622,7 +630,7 @@ __global__ void k_correct_u_cells(DevMesh m, int c, const double *dphi, double *
 __global__ void k_empty_u(DevMesh m, int c, double *U) {
   const int32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= m.F - m.Fi) return;
-  if (!plane(m, m.bpatch[k])) return;
+  if (!tangential(m, m.bpatch[k])) return;
   const int32_t f = m.Fi + k, o = m.owner[f], b = m.N + k;
   double n[3];
   const double nm = mag3(&m.Sf[f * 3]);
@@ -815,7 +823,7 @@ __global__ void k_correct_mdot(DevMesh m, const double *fluxC1, const double *fl
   if (f >= m.F) return;
   if (f < m.Fi) {
     mdot[f] = (mdot[f] + fluxC1[f] * PP[m.owner[f]]) + fluxC2[f] * PP[m.neighbour[f]];
-  } else if (plane(m, m.bpatch[f - m.Fi])) {
+  } else if (tangential(m, m.bpatch[f - m.Fi])) {
     mdot[f] = 0.0 * mdot[f];
   }
 }

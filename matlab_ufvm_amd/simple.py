@@ -21,7 +21,7 @@ from . import _lib
 from ._lib import AmgCfg, MeshDesc, PcgCfg, SimpleCfg, f64p, i32p
 
 PATCH = {"wall": 0, "empty": 1, "symmetry": 3}
-BC = {"fixedValue": 0, "zeroGradient": 1, "empty": 2, "noSlip": 3, "symmetry": 4}
+BC = {"fixedValue": 0, "zeroGradient": 1, "empty": 2, "noSlip": 3, "symmetry": 4, "slip": 5}
 P_SOLVER = {"amg": 0, "pcg": 1, "pcg-frozen": 2}
 
 

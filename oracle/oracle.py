@@ -281,7 +281,7 @@ class OState(C.Structure):
 
 
 PATCH_TYPE = {"wall": 0, "empty": 1, "patch": 2, "symmetry": 3}
-BC = {"fixedValue": 0, "zeroGradient": 1, "empty": 2, "noSlip": 3, "symmetry": 4}
+BC = {"fixedValue": 0, "zeroGradient": 1, "empty": 2, "noSlip": 3, "symmetry": 4, "slip": 5}
 
 
 class SimpleCase:

@@ -33,6 +33,15 @@ static double mag3(const double *v) { return sqrt((v[0] * v[0] + v[1] * v[1]) + 
 static int plane(const orc_mesh *m, int p) {
   return m->ptype[p] == ORC_PATCH_EMPTY || m->ptype[p] == ORC_PATCH_SYMMETRY;
 }
+
+/* corrections making U_b = U_O - (U_O . n) n and mdot_b = 0 mdot_b: the planes
+ * above and slip walls (cfdCorrectScalarEquation.m:239-275,
+ * cfdCorrectVelocityField.m:133-160, cfdCorrectMdotField.m:132-149); a slip
+ * wall's diffusion (cfdAssembleDiffusionTerm.m:456-478) and mdot
+ * (cfdAssembleMdotTerm.m:355-375) fluxes are zero, its gradient rows a wall's */
+static int tangential(const orc_mesh *m, int p) {
+  return plane(m, p) || (m->ptype[p] == ORC_PATCH_WALL && m->u_bc[p] == ORC_BC_SLIP);
+}
 static double dot3(const double *a, const double *b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
 
 /* patch of boundary face f (absolute index) */
@@ -467,7 +476,7 @@ void orc_correct_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state 
   (void)c;
   for (int32_t i = 0; i < N; ++i) s->U[i * 3 + comp] = s->U[i * 3 + comp] + L->dphi[i];
   for (int p = 0; p < m->npatch; ++p) {
-    if (!plane(m, p)) continue;
+    if (!tangential(m, p)) continue;
     for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
       const int32_t b = N + (f - m->Fi), o = m->owner[f];
       double n[3];
@@ -575,7 +584,7 @@ void orc_correct_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state 
     for (int d = 0; d < 3; ++d)
       s->U[i * 3 + d] = s->U[i * 3 + d] - s->DU[i * 3 + d] * s->gradPP[i * 3 + d];
   for (int p = 0; p < m->npatch; ++p) {
-    if (!plane(m, p)) continue;
+    if (!tangential(m, p)) continue;
     for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
       const int32_t b = N + (f - Fi), o = m->owner[f];
       double n[3];
@@ -595,7 +604,7 @@ void orc_correct_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state 
     s->mdot[f] = (s->mdot[f] + s->fluxC1[f] * s->PP[m->owner[f]]) +
                  s->fluxC2[f] * s->PP[m->neighbour[f]];
   for (int p = 0; p < m->npatch; ++p) {
-    if (!plane(m, p)) continue;
+    if (!tangential(m, p)) continue;
     for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) s->mdot[f] = 0.0 * s->mdot[f];
   }
 }

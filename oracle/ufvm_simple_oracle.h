@@ -24,7 +24,7 @@ enum { ORC_PATCH_WALL = 0, ORC_PATCH_EMPTY = 1, ORC_PATCH_PATCH = 2, ORC_PATCH_S
 /* ORC_BC_NO_SLIP: U = 0 on the wall and the reference's vector noSlip wall
  * diffusion (cfdAssembleDiffusionTerm.m:479-562, cfdGetUCoef.m:41-44) */
 enum { ORC_BC_FIXED_VALUE = 0, ORC_BC_ZERO_GRADIENT = 1, ORC_BC_EMPTY = 2, ORC_BC_NO_SLIP = 3,
-       ORC_BC_SYMMETRY = 4 };
+       ORC_BC_SYMMETRY = 4, ORC_BC_SLIP = 5 };
 
 typedef struct {
   int32_t N, F, Fi;

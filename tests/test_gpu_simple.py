@@ -176,6 +176,21 @@ def test_symmetry_plane_bitexact(ctx):
     g.close()
 
 
+def test_slip_walls_bitexact(ctx):
+    # the reference's wall slip on a jittered 3-D cavity's fixed walls: the
+    # GPU gives the oracle's fields bit for bit
+    m = cavity_mesh(8, jitter=0.15, seed=91, renumber_seed=92)
+    g = SimpleCase(ctx, m, cavity_bcs(m, walls="slip"), nu=0.01, **AMG)
+    o = oracle.SimpleCase(m, oracle.cavity_bcs(m, walls="slip"), nu=0.01, **AMG)
+    res = g.iterate(4)
+    for _ in range(4):
+        o.iterate(1)
+    for name, oname in (("U", "U"), ("p", "p"), ("mdot_f", "mdot")):
+        same(g.field(name), o.field(oname), f"slip {name}")
+    assert np.all(np.isfinite(res))
+    g.close()
+
+
 def test_explicit_stress_term_bitexact(ctx):
     # + div(nu*transp(grad(U))) (the reference's tutorials' momentum equation)
     # on the GPU gives the oracle's fields bit for bit over SIMPLE iterations

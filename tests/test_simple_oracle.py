@@ -181,6 +181,41 @@ def test_symmetry_labels_equal_empty_on_2d_planes():
         assert np.array_equal(a.field(name), b.field(name)), name
 
 
+def test_slip_wall_closed_forms():
+    # the reference's wall slip (cfdAssembleDiffusionTerm.m:456-478): no
+    # diffusion through the wall, so a bottom-wall cell of a uniform 2-D
+    # cavity has, from rest, diagonal urf^-1 (3 nu A/h + V/fdt) for U_x and
+    # U_y (the fixedValue wall adds nu A/(h/2)); after iterations the slip
+    # walls carry no flux, U_b is tangential (U_O minus its normal part) and
+    # the flow differs from the fixedValue-wall one
+    n, nu = 6, 0.01
+    m = cavity_mesh(n, nz=1, jitter=0.0, renumber_seed=None)
+    case = oracle.SimpleCase(m, oracle.cavity_bcs(m, walls="slip"), nu=nu)
+    h = 1.0 / n
+    A, V = h * h, h ** 3
+    c = m.centroid[:m.n_cells]
+    bottom = np.where((np.abs(c[:, 1] - h / 2) < 1e-12) & (c[:, 0] > h) & (c[:, 0] < 1 - h))[0]
+    assert bottom.size == n - 2
+    case.update_fields()
+    for comp in (0, 1):
+        case.assemble_u(comp)
+        ac = case.system()["ac"]
+        assert np.allclose(ac[bottom], (3 * nu * A / h + V / 1e9) / 0.7, rtol=1e-12), comp
+    case.iterate(5)
+    U, p, mdot = case.field("U"), case.field("p"), case.field("mdot")
+    fw = next(q for q in m.patches if q.name == "fixedWalls")
+    f = np.arange(fw.start, fw.start + fw.nfaces)
+    b, o = m.n_cells + f - m.n_interior, m.owner[f]
+    nrm = m.Sf[f] / np.linalg.norm(m.Sf[f], axis=1)[:, None]
+    assert np.all(mdot[f] == 0.0)
+    assert np.abs((U[b] * nrm).sum(1)).max() < 1e-15
+    assert np.abs(U[b]).max() > 1e-3
+    assert np.array_equal(p[b], p[o])
+    fixed = oracle.SimpleCase(m, oracle.cavity_bcs(m), nu=nu)
+    fixed.iterate(5)
+    assert np.abs(fixed.field("U") - U).max() > 1e-3
+
+
 def test_explicit_stress_term_pins():
     # + div(nu*transp(grad(U))) (cfdAssembleExplicitStressTerm.m:14-52) is
     # sum over faces of -nu_f (Average grad(U_c))_f . Sf.  (1) For a linear U on
