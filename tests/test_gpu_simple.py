@@ -191,6 +191,27 @@ def test_slip_walls_bitexact(ctx):
     g.close()
 
 
+def test_symmetry_slip_and_stress_together_bitexact(ctx):
+    # a symmetry plane, slip walls and the explicit stress term in one case
+    # (the stress term's boundary rows read the planes' and walls' gradient
+    # rows): the GPU gives the oracle's fields bit for bit
+    from matlab_ufvm_amd.mesh import process_mesh
+    from matlab_ufvm_amd.meshgen import box_hex
+    spec = [("movingWall", "wall", ["y1"]), ("fixedWalls", "wall", ["x0", "x1", "y0", "z1"]),
+            ("midPlane", "symmetry", ["z0"])]
+    m = process_mesh(box_hex(7, 7, 7, spec, jitter=0.15, seed=93, renumber_seed=94),
+                     apex="reference")
+    g = SimpleCase(ctx, m, cavity_bcs(m, walls="slip"), nu=0.01, stress=True, **AMG)
+    o = oracle.SimpleCase(m, oracle.cavity_bcs(m, walls="slip"), nu=0.01, stress=True, **AMG)
+    res = g.iterate(4)
+    for _ in range(4):
+        o.iterate(1)
+    for name, oname in (("U", "U"), ("p", "p"), ("mdot_f", "mdot")):
+        same(g.field(name), o.field(oname), f"combined {name}")
+    assert np.all(np.isfinite(res))
+    g.close()
+
+
 def test_explicit_stress_term_bitexact(ctx):
     # + div(nu*transp(grad(U))) (the reference's tutorials' momentum equation)
     # on the GPU gives the oracle's fields bit for bit over SIMPLE iterations
