@@ -150,8 +150,12 @@ def _parse_value(spec: str, vector: bool):
 def read_field(path: str) -> FoamField:
     text = open(path).read()
     head = re.search(r"FoamFile\s*\{(.*?)\}", text, re.S).group(1)
-    get = lambda key: re.search(key + r"\s+\"?([^;\"]*)\"?;", head).group(1).strip()
-    cls, loc, name = get("class"), get("location"), get("object")
+    def get(key, default=None):
+        m = re.search(key + r"\s+\"?([^;\"]*)\"?;", head)
+        return m.group(1).strip() if m else default
+
+    # a 0 directory's files may omit "location"
+    cls, loc, name = get("class"), get("location", ""), get("object")
     vector = cls == "volVectorField"
     dims = re.search(r"dimensions\s+(\[[^\]]*\])", text).group(1)
     istart = text.index("internalField")

@@ -260,7 +260,7 @@ class OMesh(C.Structure):
                 ("T", f64p), ("gf", f64p), ("fc", f64p), ("vol", f64p), ("cc", f64p),
                 ("cptr", i32p), ("ccol", i32p), ("own_nnz", i32p), ("nei_nnz", i32p),
                 ("npatch", C.c_int32), ("pstart", i32p), ("pnf", i32p), ("ptype", i32p),
-                ("u_bc", i32p), ("p_bc", i32p), ("u_val", f64p)]
+                ("u_bc", i32p), ("p_bc", i32p), ("u_val", f64p), ("p_val", f64p)]
 
 
 class OCfg(C.Structure):
@@ -268,7 +268,8 @@ class OCfg(C.Structure):
                 ("n_correctors", C.c_int32), ("p_ref", C.c_int32), ("u_sweeps", C.c_int32),
                 ("p_cycle", C.c_int32), ("p_max_cycles", C.c_int32), ("p_max_coarse", C.c_int32),
                 ("p_pre", C.c_int32), ("p_post", C.c_int32), ("p_smoother", C.c_int32),
-                ("p_termination", C.c_double), ("u_smoother", C.c_int32), ("stress", C.c_int32)]
+                ("p_termination", C.c_double), ("u_smoother", C.c_int32), ("stress", C.c_int32),
+                ("u_init", C.c_double * 3), ("p_init", C.c_double)]
 
 
 class OState(C.Structure):
@@ -277,24 +278,27 @@ class OState(C.Structure):
                 ("gradPP", f64p), ("DU", f64p), ("DUT", f64p), ("mdot", f64p), ("gamma_f", f64p),
                 ("fluxC1", f64p), ("fluxC2", f64p), ("scaleU", C.c_double), ("H", Hier),
                 ("ac_old", f64p), ("rms", C.c_double * 4), ("mx", C.c_double * 4),
-                ("r0", C.c_double * 4), ("r1", C.c_double * 4), ("ncycles", C.c_int32)]
+                ("r0", C.c_double * 4), ("r1", C.c_double * 4), ("ncycles", C.c_int32),
+                ("scale_zero", C.c_int32)]
 
 
-PATCH_TYPE = {"wall": 0, "empty": 1, "patch": 2, "symmetry": 3}
+PATCH_TYPE = {"wall": 0, "empty": 1, "patch": 2, "symmetry": 3, "inlet": 4, "outlet": 5}
 BC = {"fixedValue": 0, "zeroGradient": 1, "empty": 2, "noSlip": 3, "symmetry": 4, "slip": 5}
 
 
 class SimpleCase:
     """The oracle's steady SIMPLE iteration on a processed mesh.
 
-    bcs: {patch name: (U bc, U value (3,), p bc)}; U bc in fixedValue /
-    zeroGradient / empty, p bc in zeroGradient / empty.
+    bcs: {patch name: (U bc, U value (3,), p bc[, p value])}: U bc in
+    fixedValue / zeroGradient / noSlip / slip / empty / symmetry, p bc in
+    zeroGradient / fixedValue / empty / symmetry.  u_init, p_init: the uniform
+    internal fields of the 0 directory.
     """
 
     def __init__(self, mesh, bcs, nu, urf_u=0.7, urf_p=0.3, n_correctors=2, p_ref=0,
                  u_sweeps=10, p_cycle="W-Cycle", p_max_cycles=30, p_termination=0.1,
                  p_max_coarse=10, p_pre=1, p_post=3, p_smoother="ILU", u_smoother="ILU",
-                 stress=False):
+                 stress=False, u_init=(0.0, 0.0, 0.0), p_init=0.0):
         L = lib()
         for name, args in [("orc_simple_init", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState)]),
                            ("orc_simple_free", [C.POINTER(OState)]),
@@ -332,11 +336,13 @@ class SimpleCase:
         M.u_bc = a([BC[bcs[p.name][0]] for p in m.patches], i32p)
         M.p_bc = a([BC[bcs[p.name][2]] for p in m.patches], i32p)
         M.u_val = a(np.concatenate([np.asarray(bcs[p.name][1], float) for p in m.patches]), f64p)
+        M.p_val = a([float(bcs[p.name][3]) if len(bcs[p.name]) > 3 else 0.0 for p in m.patches], f64p)
         self.M = M
         self.cfg = OCfg(nu, urf_u, urf_p, n_correctors, p_ref, u_sweeps,
                         {"V-Cycle": 0, "W-Cycle": 1, "F-Cycle": 2}[p_cycle], p_max_cycles,
                         p_max_coarse, p_pre, p_post, {"ILU": 0, "SOR": 1}[p_smoother], p_termination,
-                        {"ILU": 0, "MCGS": 2}[u_smoother], int(bool(stress)))
+                        {"ILU": 0, "MCGS": 2}[u_smoother], int(bool(stress)),
+                        (C.c_double * 3)(*u_init), float(p_init))
         self.S = OState()
         L.orc_simple_init(C.byref(self.M), C.byref(self.cfg), C.byref(self.S))
 
@@ -349,9 +355,16 @@ class SimpleCase:
     def _args(self):
         return C.byref(self.M), C.byref(self.cfg), C.byref(self.S)
 
+    def _check_cycles(self):
+        # the reference's W- and F-cycles index levels gridLevel-3 and below
+        # (cfdApplyWCycle.m, cfdApplyFCycle.m) and fail with fewer than 4
+        if self.S.ncycles == -1:
+            raise ValueError("W/F cycles need at least 4 AMG levels")
+
     def iterate(self, n=1):
         for _ in range(n):
             lib().orc_simple_iteration(*self._args())
+            self._check_cycles()
 
     def update_fields(self):
         lib().orc_update_fields(*self._args())
@@ -370,6 +383,7 @@ class SimpleCase:
 
     def solve_p(self):
         lib().orc_solve_p(*self._args())
+        self._check_cycles()
 
     def correct_p(self):
         lib().orc_correct_p(*self._args())

@@ -34,14 +34,32 @@ static int plane(const orc_mesh *m, int p) {
   return m->ptype[p] == ORC_PATCH_EMPTY || m->ptype[p] == ORC_PATCH_SYMMETRY;
 }
 
-/* corrections making U_b = U_O - (U_O . n) n and mdot_b = 0 mdot_b: the planes
- * above and slip walls (cfdCorrectScalarEquation.m:239-275,
- * cfdCorrectVelocityField.m:133-160, cfdCorrectMdotField.m:132-149); a slip
- * wall's diffusion (cfdAssembleDiffusionTerm.m:456-478) and mdot
- * (cfdAssembleMdotTerm.m:355-375) fluxes are zero, its gradient rows a wall's */
-static int tangential(const orc_mesh *m, int p) {
+/* the momentum corrections dispatch on U's condition
+ * (cfdCorrectScalarEquation.m:31-107): U_b,c = U_O,c - (U_O . n) n_c on the
+ * planes above and on slip walls (:239-293); an owner copy of U_b,c on
+ * zeroGradient outlets (:376-407); no change on fixedValue / noSlip walls and
+ * fixedValue inlets (:116-120, :229-233, :301-303).  A slip wall's diffusion
+ * (cfdAssembleDiffusionTerm.m:456-478) and mdot (cfdAssembleMdotTerm.m:355-375)
+ * fluxes are zero, its gradient rows a wall's. */
+static int u_tangential(const orc_mesh *m, int p) {
   return plane(m, p) || (m->ptype[p] == ORC_PATCH_WALL && m->u_bc[p] == ORC_BC_SLIP);
 }
+
+/* inlet and outlet patches carry a mass flux in the pressure-correction
+ * assembly (cfdAssembleMdotTerm.m:415-486 'inlet' with p zeroGradient,
+ * :799-906 'outlet' with p fixedValue); walls and planes do not (:355-410) */
+static int p_flux(const orc_mesh *m, int p) {
+  return m->ptype[p] == ORC_PATCH_INLET || m->ptype[p] == ORC_PATCH_OUTLET;
+}
+
+/* outlet with a fixed pressure: PP_b = 0 (cfdCorrectPPField.m:249-272), p_b
+ * kept (cfdCorrectPressureEquation.m:230-233), the owners' U written to rows
+ * iBFaces of U (cfdCorrectVelocityField.m:247-272) and mdot_b = rho_b (U_O . S_b)
+ * (cfdCorrectMdotField.m:264-304) */
+static int p_fixed_outlet(const orc_mesh *m, int p) {
+  return m->ptype[p] == ORC_PATCH_OUTLET && m->p_bc[p] == ORC_BC_FIXED_VALUE;
+}
+
 static double dot3(const double *a, const double *b) { return (a[0] * b[0] + a[1] * b[1]) + a[2] * b[2]; }
 
 /* patch of boundary face f (absolute index) */
@@ -96,19 +114,27 @@ void orc_simple_init(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_stat
   L->dphi = dz(N);
   L->parents = NULL;
   s->H.nlevels = 1;
-  /* uniform zero initial fields; boundary values from the BCs
-   * (cfdInitializeBoundaryConditions.m: fixedValue -> value; zeroGradient ->
-   * owner value :200-226; empty -> owner minus its normal component :534-595) */
+  /* uniform initial fields (the 0 directory's internalField); boundary values
+   * from the BCs (cfdInitializeBoundaryConditions.m): fixedValue walls and
+   * inlets the value (:107-148, :321-357), noSlip zero (:239-275), slip walls,
+   * empty and symmetry planes the owner minus its normal component (:276-320,
+   * :471-595), zeroGradient outlets an owner copy of every component
+   * (:426-470); p_b an owner copy, or the value on fixed-pressure outlets
+   * (:386-425) */
+  for (int32_t i = 0; i < N; ++i) {
+    for (int d = 0; d < 3; ++d) s->U[i * 3 + d] = c->u_init[d];
+    s->p[i] = c->p_init;
+  }
   for (int p = 0; p < m->npatch; ++p) {
     for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
       const int32_t b = N + (f - m->Fi), o = m->owner[f];
       if (m->u_bc[p] == ORC_BC_FIXED_VALUE) {
         for (int d = 0; d < 3; ++d) s->U[b * 3 + d] = m->u_val[p * 3 + d];
       } else if (m->u_bc[p] == ORC_BC_NO_SLIP) {
-        /* the noSlip value '[0;0;0]' (cfdDefineEquation.m:290-295), set once
-         * (cfdInitializeBoundaryConditions.m:239-267) and never updated */
+        /* the noSlip value '[0;0;0]' (cfdDefineEquation.m:290-295), set at
+         * every boundary update (cfdUpdateBoundaryConditions.m:213-278) */
         for (int d = 0; d < 3; ++d) s->U[b * 3 + d] = 0.0;
-      } else if (m->u_bc[p] == ORC_BC_EMPTY || m->u_bc[p] == ORC_BC_SYMMETRY) {
+      } else if (u_tangential(m, p)) {
         double n[3];
         const double nm = mag3(&m->Sf[f * 3]);
         for (int d = 0; d < 3; ++d) n[d] = m->Sf[f * 3 + d] / nm;
@@ -117,7 +143,7 @@ void orc_simple_init(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_stat
       } else {
         for (int d = 0; d < 3; ++d) s->U[b * 3 + d] = s->U[o * 3 + d];
       }
-      s->p[b] = s->p[o];
+      s->p[b] = m->p_bc[p] == ORC_BC_FIXED_VALUE ? m->p_val[p] : s->p[o];
     }
   }
   /* mdot_f = rho_f .* (Sf . U_f) with Average interpolation
@@ -136,7 +162,6 @@ void orc_simple_init(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_stat
     }
     s->mdot[f] = rf * dot3(&m->Sf[f * 3], uf);
   }
-  (void)c;
 }
 
 void orc_simple_free(orc_simple_state *s) {
@@ -183,15 +208,23 @@ static void gauss0(const orc_mesh *m, const double *phi, int nc, double *grad) {
   }
 }
 
-/* cfdUpdateGradient.m:18-70: Gauss gradient, then wall (and 'patch') boundary
- * rows replaced by g - (g.e)e + ((phi_b - phi_O)/|d|) e, d = C_f - C_O
- * (:104-134) */
+/* cfdUpdateGradient.m:18-70: Gauss gradient, then the boundary rows of wall,
+ * 'patch', inlet and outlet patches replaced by
+ * g - (g.e)e + ((phi_b - phi_O)/|d|) e, d = C_f - C_O (walls :81-134, inlets
+ * :142-195, outlets :202-255).  The inlet and outlet rows index phi linearly
+ * (phi(iBElement) - phi(iOwner), :187 and :247), i.e. the first component's
+ * difference for every component of a vector; kept. */
 void orc_gradient(const orc_mesh *m, const double *phi, int nc, double *grad) {
   const int32_t N = m->N;
   gauss0(m, phi, nc, grad);
   for (int p = 0; p < m->npatch; ++p) {
-    if (m->ptype[p] != ORC_PATCH_WALL && m->ptype[p] != ORC_PATCH_PATCH) continue;
+    const int t = m->ptype[p];
+    if (t != ORC_PATCH_WALL && t != ORC_PATCH_PATCH && t != ORC_PATCH_INLET &&
+        t != ORC_PATCH_OUTLET)
+      continue;
+    const int lin = t == ORC_PATCH_INLET || t == ORC_PATCH_OUTLET;
     for (int c = 0; c < nc; ++c) {
+      const int cd = lin ? 0 : c;
       for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
         const int32_t b = N + (f - m->Fi), o = m->owner[f];
         double d[3], e[3];
@@ -200,24 +233,31 @@ void orc_gradient(const orc_mesh *m, const double *phi, int nc, double *grad) {
         for (int q = 0; q < 3; ++q) e[q] = d[q] / dmag;
         const double *g = &grad[(o * nc + c) * 3];
         const double ge = dot3(g, e);
-        const double t = (phi[b * nc + c] - phi[o * nc + c]) / dmag;
+        const double t = (phi[b * nc + cd] - phi[o * nc + cd]) / dmag;
         for (int q = 0; q < 3; ++q) grad[(b * nc + c) * 3 + q] = (g[q] - ge * e[q]) + t * e[q];
       }
     }
   }
 }
 
-/* cfdUpdateEquations.m:14-24 for U then p: boundary conditions
- * (fixedValue walls, cfdUpdateBoundaryConditions.m:103-139; zeroGradient and
- * empty are no-ops :198-200, :388-390), gradients, gamma_Ueq = Average(nu)
- * (cfdUpdateTermCoefficientField.m:44-70), phiScale (cfdUpdateScale.m:17-23) */
+/* cfdUpdateEquations.m:14-24 for U then p: boundary conditions (fixedValue
+ * walls and inlets, cfdUpdateBoundaryConditions.m:103-139, :291-328; noSlip
+ * walls the zero value :213-278; zeroGradient, slip, empty and symmetry are
+ * no-ops; fixed-pressure outlets p_b = value, :335-373), gradients,
+ * gamma_Ueq = Average(nu) (cfdUpdateTermCoefficientField.m:44-70), phiScale
+ * (cfdUpdateScale.m:17-23) */
 void orc_update_fields(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s) {
   const int32_t N = m->N, Fb = FB(m);
   for (int p = 0; p < m->npatch; ++p) {
-    if (m->ptype[p] != ORC_PATCH_WALL || m->u_bc[p] != ORC_BC_FIXED_VALUE) continue;
+    const int t = m->ptype[p];
+    const int uval = (t == ORC_PATCH_WALL || t == ORC_PATCH_INLET) && m->u_bc[p] == ORC_BC_FIXED_VALUE;
+    const int noslip = t == ORC_PATCH_WALL && m->u_bc[p] == ORC_BC_NO_SLIP;
     for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
       const int32_t b = N + (f - m->Fi);
-      for (int d = 0; d < 3; ++d) s->U[b * 3 + d] = m->u_val[p * 3 + d] * 1.0;
+      if (uval)
+        for (int d = 0; d < 3; ++d) s->U[b * 3 + d] = m->u_val[p * 3 + d] * 1.0;
+      if (noslip)
+        for (int d = 0; d < 3; ++d) s->U[b * 3 + d] = 0.0 * 1.0;
     }
   }
   orc_gradient(m, s->U, 3, s->gradU);
@@ -234,9 +274,20 @@ void orc_update_fields(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_st
     if (s->U[q] > mx) mx = s->U[q];
     if (s->U[q] < mn) mn = s->U[q];
   }
+  if (s->scale_zero) {
+    /* the zero-filled rows of the grown U (cfdUpdateScale.m:16-17 takes the
+     * max and min over every row) */
+    if (0.0 > mx) mx = 0.0;
+    if (0.0 < mn) mn = 0.0;
+  }
   double sc = (mx - mn) > fabs(mx) ? (mx - mn) : fabs(mx);
   if (sc < 1e-6) sc = 1;
   s->scaleU = sc;
+  for (int p = 0; p < m->npatch; ++p) {
+    if (!p_fixed_outlet(m, p)) continue;
+    for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f)
+      s->p[N + (f - m->Fi)] = m->p_val[p] * 1.0;
+  }
   orc_gradient(m, s->p, 1, s->gradp);
 }
 
@@ -295,8 +346,10 @@ void orc_assemble_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
   zero_system(s);
   double *C1 = dz(F), *C2 = dz(F), *T = dz(F);
   const double *U = s->U;
-  /* ---- Convection, sign +1 (cfdAssembleConvectionTerm.m:103-137; walls and
-   * empty patches give zero fluxes :238-278) */
+  /* ---- Convection, sign +1 (cfdAssembleConvectionTerm.m:103-137; fixedValue
+   * inlets and zeroGradient outlets the same upwind fluxes with U_b in place
+   * of U_N, :144-232; walls, empty and symmetry patches give zero fluxes
+   * :238-302) */
   {
     const double sg = 1.0;
     for (int32_t f = 0; f < Fi; ++f) {
@@ -307,6 +360,16 @@ void orc_assemble_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
       T[f] = (C1[f] * U[m->owner[f] * 3 + comp] + C2[f] * U[m->neighbour[f] * 3 + comp]) + V;
     }
     for (int32_t f = Fi; f < F; ++f) C1[f] = C2[f] = T[f] = 0.0;
+    for (int p = 0; p < m->npatch; ++p) {
+      if (m->ptype[p] != ORC_PATCH_INLET && m->ptype[p] != ORC_PATCH_OUTLET) continue;
+      for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
+        const int32_t b = N + (f - Fi);
+        const double psi = s->mdot[f];
+        C1[f] = sg * fmax(psi, 0.0);
+        C2[f] = (-sg) * fmax(-psi, 0.0);
+        T[f] = (C1[f] * U[m->owner[f] * 3 + comp] + C2[f] * U[b * 3 + comp]) + 0.0;
+      }
+    }
     scatter_faces(m, s, C1, C2, T);
     /* divergence term (cfdAssembleConvectionDivergenceTerm.m:38-47 with
      * cfdComputeEffectiveDivergenceWithFluidTag.m:21-46) */
@@ -327,7 +390,8 @@ void orc_assemble_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
     free(eff);
   }
   /* ---- Diffusion laplacian(nu*U), sign -1 (cfdAssembleDiffusionTerm.m:94-130,
-   * wall fixedValue :224-263, zeroGradient :330-347, empty :631-690) */
+   * wall fixedValue :224-263 and inlet fixedValue :135-173 (the same fluxes),
+   * zeroGradient walls :330-347 and outlets :201-218 (zero), empty :631-690) */
   {
     const double sg = -1.0;
     for (int32_t f = 0; f < Fi; ++f) {
@@ -343,7 +407,8 @@ void orc_assemble_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
       for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
         const int32_t b = N + (f - Fi), o = m->owner[f];
         const double gam = s->gamma_f[f];
-        if (m->ptype[p] == ORC_PATCH_WALL && m->u_bc[p] == ORC_BC_FIXED_VALUE) {
+        if ((m->ptype[p] == ORC_PATCH_WALL || m->ptype[p] == ORC_PATCH_INLET) &&
+            m->u_bc[p] == ORC_BC_FIXED_VALUE) {
           C1[f] = ((-sg) * gam) * m->gDiff[f];
           C2[f] = (sg * gam) * m->gDiff[f];
           const double V = (sg * gam) * dot3(&s->gradU[(b * 3 + comp) * 3], &m->T[f * 3]);
@@ -468,17 +533,24 @@ void orc_solve_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s
                              &s->r1[comp]);
 }
 
-/* cfdCorrectScalarEquation.m:16-24 (U(:,c) += dphi) and its empty-patch
- * branch (U_b,c = U_O,c - (U_O . n) n_c) */
+/* cfdCorrectScalarEquation.m:16-24 (U(:,c) += dphi) and its boundary
+ * branches, chosen by U's condition (:31-107): U_b,c = U_O,c - (U_O . n) n_c
+ * on slip walls, empty and symmetry planes, U_b,c = U_O,c on zeroGradient
+ * outlets, nothing on fixedValue / noSlip walls and fixedValue inlets */
 void orc_correct_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s, int comp) {
   const int32_t N = m->N;
   const orc_level *L = &s->H.lev[0];
   (void)c;
   for (int32_t i = 0; i < N; ++i) s->U[i * 3 + comp] = s->U[i * 3 + comp] + L->dphi[i];
   for (int p = 0; p < m->npatch; ++p) {
-    if (!tangential(m, p)) continue;
+    const int copy = m->ptype[p] == ORC_PATCH_OUTLET && m->u_bc[p] == ORC_BC_ZERO_GRADIENT;
+    if (!u_tangential(m, p) && !copy) continue;
     for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
       const int32_t b = N + (f - m->Fi), o = m->owner[f];
+      if (copy) {
+        s->U[b * 3 + comp] = s->U[o * 3 + comp];
+        continue;
+      }
       double n[3];
       const double nm = mag3(&m->Sf[f * 3]);
       for (int d = 0; d < 3; ++d) n[d] = m->Sf[f * 3 + d] / nm;
@@ -490,7 +562,8 @@ void orc_correct_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state 
 
 /* cfdAssembleMdotTerm.m:38-250: Rhie-Chow face fluxes (stored as the new
  * mdot_f) and the pressure-correction matrix; boundary fluxes are zero for
- * walls and empty patches (:281-350); the fix-pressure step is a no-op
+ * walls and planes, inlets and fixed-pressure outlets carry theirs
+ * (:281-348); the fix-pressure step is a no-op
  * (:257-278); residual from the effective divergence
  * (cfdComputeNormalizedResidual.m:27-30, cfdComputeEffectiveDivergence.m:21-54) */
 void orc_assemble_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s) {
@@ -534,6 +607,40 @@ void orc_assemble_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
     T[f] = md;
   }
   for (int32_t f = Fi; f < F; ++f) s->fluxC1[f] = s->fluxC2[f] = T[f] = 0.0;
+  for (int p = 0; p < m->npatch; ++p) {
+    if (!p_flux(m, p)) continue;
+    for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
+      const int32_t o = m->owner[f], b = N + (f - Fi);
+      const double *Sb = &m->Sf[f * 3];
+      double C1 = 0.0, C2 = 0.0, V = 0.0;
+      if (m->ptype[p] == ORC_PATCH_INLET) {
+        /* cfdAssembleMdotTermInletInletBC (:415-486): term I only */
+        V = V + 1.0 * dot3(Sb, &U[b * 3]);
+      } else {
+        /* cfdAssembleMdotTermOutletFixedValueBC (:799-906): terms I-III with
+         * the boundary rows of DU and grad(p), FLUXCb2 = 0 */
+        const double *CN = &m->CN[f * 3], *eCN = &m->eCN[f * 3];
+        double DUS[3], eDUS[3], DUE[3], DUT[3];
+        for (int d = 0; d < 3; ++d) DUS[d] = DU[b * 3 + d] * Sb[d];
+        const double mDUS = mag3(DUS);
+        for (int d = 0; d < 3; ++d) eDUS[d] = DUS[d] / mDUS;
+        const double ee = dot3(eCN, eDUS);
+        for (int d = 0; d < 3; ++d) DUE[d] = (mDUS / ee) * eCN[d];
+        const double geoDiff = mag3(DUE) / mag3(CN);
+        for (int d = 0; d < 3; ++d) DUT[d] = DUS[d] - DUE[d];
+        V = V + 1.0 * dot3(&U[b * 3], Sb);
+        C1 = C1 + 1.0 * geoDiff;
+        C2 = C2 - 0.0;
+        V = V - 1.0 * dot3(&s->gradp[b * 3], DUT);
+        V = V + 1.0 * dot3(&s->gradp[b * 3], DUS);
+      }
+      const double md = (C1 * s->p[o] + C2 * s->p[b]) + V;
+      s->mdot[f] = md;
+      s->fluxC1[f] = C1;
+      s->fluxC2[f] = C2;
+      T[f] = md;
+    }
+  }
   scatter_faces(m, s, s->fluxC1, s->fluxC2, T);
   for (int32_t i = 0; i < N; ++i) {
     L->ac[i] = L->ac[i] + 0.0;
@@ -566,10 +673,22 @@ void orc_solve_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s
                 c->p_smoother, &s->r0[3], &s->r1[3], &s->ncycles);
 }
 
-/* cfdCorrectEquation.m:26-47 for p: PP field (cfdCorrectPPField.m:26-52 +
- * wall/empty boundary copies + gradient), U (cfdCorrectVelocityField.m:26-55
- * + empty patches), p (cfdCorrectPressureEquation.m:22-54 + boundary copies),
- * mdot (cfdCorrectMdotField.m:24-50, empty patches zeroed :328-346) */
+/* cfdCorrectEquation.m:26-47 for p, each boundary branch chosen by the patch
+ * type and p's condition as the reference dispatches them:
+ *  - PP (cfdCorrectPPField.m:26-52): cells dphi - dphi(ref); PP_b an owner
+ *    copy (walls, zeroGradient inlets, planes :131-186, :278-335), 0 on
+ *    fixed-pressure outlets (:249-272); then its gradient;
+ *  - U (cfdCorrectVelocityField.m:26-121): cells -= DU o grad(PP); U_b
+ *    projected on empty and symmetry planes (:274-343); the owners' U
+ *    written to rows iBFaces of U on fixed-pressure outlets (:247-272, the
+ *    face index used as the row: kept); walls with p zeroGradient and
+ *    zeroGradient inlets are no-ops (:173-182);
+ *  - p (cfdCorrectPressureEquation.m:22-54): cells += urf (PP - PP(ref));
+ *    p_b an owner copy except on fixed-pressure outlets (:131-290);
+ *  - mdot (cfdCorrectMdotField.m:24-50): interior += C1 PP_O + C2 PP_N;
+ *    planes 0 mdot_b (:306-346); fixed-pressure outlets rho_b (U_O . S_b)
+ *    (:264-304); walls with p zeroGradient and incompressible inlets are
+ *    no-ops (:156-208). */
 void orc_correct_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s) {
   const int32_t N = m->N, Fi = m->Fi;
   const orc_level *L = &s->H.lev[0];
@@ -578,34 +697,71 @@ void orc_correct_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state 
   for (int32_t i = 0; i < N; ++i) s->PP[i] = L->dphi[i] - ppc;
   for (int p = 0; p < m->npatch; ++p)
     for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f)
-      s->PP[N + (f - Fi)] = s->PP[m->owner[f]];
+      s->PP[N + (f - Fi)] = p_fixed_outlet(m, p) ? 0.0 : s->PP[m->owner[f]];
   orc_gradient(m, s->PP, 1, s->gradPP);
   for (int32_t i = 0; i < N; ++i)
     for (int d = 0; d < 3; ++d)
       s->U[i * 3 + d] = s->U[i * 3 + d] - s->DU[i * 3 + d] * s->gradPP[i * 3 + d];
   for (int p = 0; p < m->npatch; ++p) {
-    if (!tangential(m, p)) continue;
-    for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
-      const int32_t b = N + (f - Fi), o = m->owner[f];
-      double n[3];
-      const double nm = mag3(&m->Sf[f * 3]);
-      for (int d = 0; d < 3; ++d) n[d] = m->Sf[f * 3 + d] / nm;
-      const double un = dot3(&s->U[o * 3], n);
-      for (int d = 0; d < 3; ++d) s->U[b * 3 + d] = s->U[o * 3 + d] - un * n[d];
+    if (plane(m, p)) {
+      for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) {
+        const int32_t b = N + (f - Fi), o = m->owner[f];
+        double n[3];
+        const double nm = mag3(&m->Sf[f * 3]);
+        for (int d = 0; d < 3; ++d) n[d] = m->Sf[f * 3 + d] / nm;
+        const double un = dot3(&s->U[o * 3], n);
+        for (int d = 0; d < 3; ++d) s->U[b * 3 + d] = s->U[o * 3 + d] - un * n[d];
+      }
+    } else if (p_fixed_outlet(m, p)) {
+      /* U(iBFaces,:) = U(iOwners,:): the 1-based face index f+1 is the row,
+       * all owners read before any row is written; rows past the field (the
+       * reference would grow U) are not written */
+      const int32_t f0 = m->pstart[p], nf = m->pnf[p];
+      double *tmp = dz((size_t)nf * 3);
+      for (int32_t k = 0; k < nf; ++k)
+        for (int d = 0; d < 3; ++d) tmp[k * 3 + d] = s->U[m->owner[f0 + k] * 3 + d];
+      for (int32_t k = 0; k < nf; ++k)
+        if (f0 + k < N + FB(m))
+          for (int d = 0; d < 3; ++d) s->U[(f0 + k) * 3 + d] = tmp[k * 3 + d];
+      free(tmp);
+      if (f0 + nf > N + FB(m)) {
+        /* rows N+Fb .. f0+nf-1 now exist; any of them no outlet face writes
+         * stays zero */
+        int32_t hi = f0 + nf - 1, covered = 0;
+        for (int q = 0; q < m->npatch; ++q) {
+          if (!p_fixed_outlet(m, q)) continue;
+          const int32_t e = m->pstart[q] + m->pnf[q] - 1;
+          if (e > hi) hi = e;
+        }
+        for (int32_t r = N + FB(m); r <= hi; ++r) {
+          int hit = 0;
+          for (int q = 0; q < m->npatch && !hit; ++q)
+            hit = p_fixed_outlet(m, q) && r >= m->pstart[q] && r < m->pstart[q] + m->pnf[q];
+          covered += hit;
+        }
+        if (covered < hi - (N + FB(m)) + 1) s->scale_zero = 1;
+      }
     }
   }
   double ppf = 0.0;
   if (c->p_ref >= 0) ppf = s->PP[c->p_ref];
   for (int32_t i = 0; i < N; ++i) s->p[i] = s->p[i] + c->urf_p * (s->PP[i] - ppf);
-  for (int p = 0; p < m->npatch; ++p)
+  for (int p = 0; p < m->npatch; ++p) {
+    if (p_fixed_outlet(m, p)) continue;
     for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f)
       s->p[N + (f - Fi)] = s->p[m->owner[f]];
+  }
   for (int32_t f = 0; f < Fi; ++f)
     s->mdot[f] = (s->mdot[f] + s->fluxC1[f] * s->PP[m->owner[f]]) +
                  s->fluxC2[f] * s->PP[m->neighbour[f]];
   for (int p = 0; p < m->npatch; ++p) {
-    if (!tangential(m, p)) continue;
-    for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f) s->mdot[f] = 0.0 * s->mdot[f];
+    if (plane(m, p)) {
+      for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f)
+        s->mdot[f] = 0.0 * s->mdot[f];
+    } else if (p_fixed_outlet(m, p)) {
+      for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f)
+        s->mdot[f] = 1.0 * dot3(&s->U[m->owner[f] * 3], &m->Sf[f * 3]);
+    }
   }
 }
 

@@ -20,7 +20,10 @@
 extern "C" {
 #endif
 
-enum { ORC_PATCH_WALL = 0, ORC_PATCH_EMPTY = 1, ORC_PATCH_PATCH = 2, ORC_PATCH_SYMMETRY = 3 };
+/* physical patch types of the boundary file (cfdReadPolyMesh.m:100-130);
+ * ORC_PATCH_INLET / ORC_PATCH_OUTLET are 'inlet' and 'outlet' */
+enum { ORC_PATCH_WALL = 0, ORC_PATCH_EMPTY = 1, ORC_PATCH_PATCH = 2, ORC_PATCH_SYMMETRY = 3,
+       ORC_PATCH_INLET = 4, ORC_PATCH_OUTLET = 5 };
 /* ORC_BC_NO_SLIP: U = 0 on the wall and the reference's vector noSlip wall
  * diffusion (cfdAssembleDiffusionTerm.m:479-562, cfdGetUCoef.m:41-44) */
 enum { ORC_BC_FIXED_VALUE = 0, ORC_BC_ZERO_GRADIENT = 1, ORC_BC_EMPTY = 2, ORC_BC_NO_SLIP = 3,
@@ -37,6 +40,7 @@ typedef struct {
   const int32_t *pstart, *pnf, *ptype;                      /* patch ranges and physical type */
   const int32_t *u_bc, *p_bc;                               /* per patch BC kinds */
   const double *u_val;                                      /* per patch fixed U [3] */
+  const double *p_val;                                      /* per patch fixed p */
 } orc_mesh;
 
 typedef struct {
@@ -49,6 +53,8 @@ typedef struct {
   double p_termination;
   int32_t u_smoother;     /* momentum solve: 0 the reference's DILU, 2 multicolour GS */
   int32_t stress;         /* 1: the momentum equation has + div(nu*transp(grad(U))) */
+  double u_init[3];       /* uniform internal U of the 0 directory */
+  double p_init;          /* uniform internal p */
 } orc_simple_cfg;
 
 typedef struct {
@@ -66,6 +72,10 @@ typedef struct {
   /* per equation component: U x,y,z and p */
   double rms[4], mx[4], r0[4], r1[4];
   int32_t ncycles;
+  /* 1 once a fixed-pressure outlet's velocity correction has written U rows
+   * past the field's end with unwritten rows between (the reference's U grows
+   * and zero-fills them; only cfdUpdateScale's max/min sees those rows) */
+  int32_t scale_zero;
 } orc_simple_state;
 
 void orc_simple_init(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s);

@@ -1,16 +1,18 @@
 """Pins of the SIMPLE-step oracle (oracle/ufvm_simple_oracle.c).
 
-No golden vectors of the reference exist for this path (MATLAB is absent and
-the tutorial snapshots use the noSlip wall branch with ~5 digits), so the
-oracle is pinned by closed forms on uniform meshes, conservation invariants
-and the Re = 100 lid-driven-cavity benchmark of Ghia, Ghia & Shin (1982).
+The reference's own output for this path is its cylinder tutorial's time
+directories (tests/test_tutorial_cylinder.py: every written value
+reproduced).  Here the oracle is pinned further by closed forms on uniform
+and jittered meshes, conservation invariants, the reference's boundary
+dispatch of each correction stage, and the Re = 100 lid-driven-cavity
+benchmark of Ghia, Ghia & Shin (1982).
 """
 import numpy as np
 import pytest
 import scipy.sparse as sp
 
 import oracle
-from matlab_ufvm_amd.workloads import cavity_mesh
+from matlab_ufvm_amd.workloads import cavity_mesh, channel_bcs, channel_mesh
 
 
 @pytest.fixture(scope="module")
@@ -172,8 +174,8 @@ def test_symmetry_labels_equal_empty_on_2d_planes():
             ("frontAndBack", "symmetry", ["z0", "z1"])]
     me = cavity_mesh(12, nz=1, jitter=0.0, renumber_seed=None)
     ms = _box(12, 1, spec)
-    a = oracle.SimpleCase(me, oracle.cavity_bcs(me), nu=0.01)
-    b = oracle.SimpleCase(ms, oracle.cavity_bcs(ms), nu=0.01)
+    a = oracle.SimpleCase(me, oracle.cavity_bcs(me), nu=0.01, p_cycle="V-Cycle")
+    b = oracle.SimpleCase(ms, oracle.cavity_bcs(ms), nu=0.01, p_cycle="V-Cycle")
     assert oracle.cavity_bcs(ms)["frontAndBack"] == ("symmetry", (0.0, 0.0, 0.0), "symmetry")
     a.iterate(5)
     b.iterate(5)
@@ -190,7 +192,7 @@ def test_slip_wall_closed_forms():
     # the flow differs from the fixedValue-wall one
     n, nu = 6, 0.01
     m = cavity_mesh(n, nz=1, jitter=0.0, renumber_seed=None)
-    case = oracle.SimpleCase(m, oracle.cavity_bcs(m, walls="slip"), nu=nu)
+    case = oracle.SimpleCase(m, oracle.cavity_bcs(m, walls="slip"), nu=nu, p_cycle="V-Cycle")
     h = 1.0 / n
     A, V = h * h, h ** 3
     c = m.centroid[:m.n_cells]
@@ -211,7 +213,7 @@ def test_slip_wall_closed_forms():
     assert np.abs((U[b] * nrm).sum(1)).max() < 1e-15
     assert np.abs(U[b]).max() > 1e-3
     assert np.array_equal(p[b], p[o])
-    fixed = oracle.SimpleCase(m, oracle.cavity_bcs(m), nu=nu)
+    fixed = oracle.SimpleCase(m, oracle.cavity_bcs(m), nu=nu, p_cycle="V-Cycle")
     fixed.iterate(5)
     assert np.abs(fixed.field("U") - U).max() > 1e-3
 
@@ -294,3 +296,117 @@ def test_reference_w_cycle_diverges_on_3d_cavity():
     v.iterate(2)
     assert w.residuals()["rms"][3] > 1e10
     assert v.residuals()["rms"][3] < 1e-3
+
+
+def _patch_rows(m, name):
+    p = next(q for q in m.patches if q.name == name)
+    f = np.arange(p.start, p.start + p.nfaces)
+    return f, m.n_cells + f - m.n_interior, m.owner[f]
+
+
+def _normals(m, f):
+    S = m.Sf[f]
+    return S / np.sqrt(((S[:, 0] * S[:, 0] + S[:, 1] * S[:, 1]) + S[:, 2] * S[:, 2]))[:, None]
+
+
+def test_correction_stages_follow_the_reference_dispatch():
+    # Which boundary rows each correction writes, per patch kind, as the
+    # reference dispatches them.  Momentum (cfdCorrectScalarEquation.m:31-107,
+    # on U's condition): slip walls and symmetry planes get
+    # U_b,c = U_O,c - (U_O . n) n_c, zeroGradient outlets U_b,c = U_O,c,
+    # fixedValue / noSlip walls and the fixedValue inlet keep U_b.  Pressure
+    # (cfdCorrectVelocityField.m:57-121 and cfdCorrectMdotField.m:56-120, on
+    # p's condition): walls with zeroGradient p -- slip ones included -- and
+    # the zeroGradient-p inlet keep U_b and mdot_b; symmetry planes project
+    # U_b on the corrected U_O and zero mdot_b; the fixed-pressure outlet
+    # copies the corrected owners' U into rows iBFaces and sets
+    # mdot_b = U_O . S_b.
+    m = channel_mesh(6, 4, 3)
+    case = oracle.SimpleCase(m, channel_bcs(), nu=0.01, p_ref=-1, u_init=(1.0, 0.0, 0.0),
+                             stress=True, p_cycle="V-Cycle")
+    case.iterate(2)
+    case.update_fields()
+    N = m.n_cells
+    for comp in range(3):
+        case.assemble_u(comp)
+        case.solve_u(comp)
+        U0 = case.field("U")
+        case.correct_u(comp)
+        U1 = case.field("U")
+        for name in ("inlet", "lowerWall", "back"):          # fixedValue, noSlip, fixedValue
+            _, b, _ = _patch_rows(m, name)
+            assert np.array_equal(U1[b], U0[b]), (comp, name)
+        for name in ("upperWall", "front"):                  # slip wall, symmetry plane
+            f, b, o = _patch_rows(m, name)
+            n = _normals(m, f)
+            un = (U1[o, 0] * n[:, 0] + U1[o, 1] * n[:, 1]) + U1[o, 2] * n[:, 2]
+            assert np.array_equal(U1[b, comp], U1[o, comp] - un * n[:, comp]), (comp, name)
+            others = [d for d in range(3) if d != comp]
+            assert np.array_equal(U1[b][:, others], U0[b][:, others])
+        f, b, o = _patch_rows(m, "outlet")
+        assert np.array_equal(U1[b, comp], U1[o, comp])
+    case.assemble_p()
+    case.solve_p()
+    U0, md0 = case.field("U"), case.field("mdot")
+    DU = case.field("DU")
+    case.correct_p()
+    U1, md1, G = case.field("U"), case.field("mdot"), case.field("gradPP")
+    assert np.abs(G[:N]).max() > 0
+    assert np.array_equal(U1[:N], U0[:N] - DU[:N] * G[:N])
+    # the whole boundary block as the reference's branches leave it
+    NB = N + m.n_bfaces
+    want = U0.copy()
+    f, b, o = _patch_rows(m, "front")
+    n = _normals(m, f)
+    un = (U1[o, 0] * n[:, 0] + U1[o, 1] * n[:, 1]) + U1[o, 2] * n[:, 2]
+    want[b] = U1[o] - un[:, None] * n
+    fo, bo, oo = _patch_rows(m, "outlet")
+    inside = fo < NB           # rows iBFaces inside the field: another patch's U_b
+    assert 0 < inside.sum() < fo.size
+    want[fo[inside]] = U1[oo[inside]]
+    assert np.array_equal(U1[N:], want[N:])
+    for name in ("inlet", "lowerWall", "upperWall", "back"):
+        fw, _, _ = _patch_rows(m, name)
+        assert np.array_equal(md1[fw], md0[fw]), name
+    assert np.all(md1[f] == 0.0)
+    S = m.Sf[fo]
+    assert np.array_equal(md1[fo], (U1[oo, 0] * S[:, 0] + U1[oo, 1] * S[:, 1]) + U1[oo, 2] * S[:, 2])
+    # rows past the end: the reference's U grows, zero-filling rows no outlet
+    # face writes, which its scale (max/min over all rows) then sees; here the
+    # outlet's rows run contiguously past the end, so no row stays zero
+    assert fo.max() >= NB and set(range(NB, fo.max() + 1)) <= set(fo.tolist())
+    assert case.S.scale_zero == 0
+    big = channel_mesh(8, 5, 4)
+    fb, _, _ = _patch_rows(big, "outlet")
+    assert fb.min() > big.n_cells + big.n_bfaces          # a gap of zero rows
+    bc = oracle.SimpleCase(big, channel_bcs(), nu=0.01, p_ref=-1, u_init=(1.0, 0.0, 0.0),
+                           p_cycle="V-Cycle")
+    bc.iterate(1)
+    assert bc.S.scale_zero == 1
+
+
+def test_slip_wall_keeps_its_momentum_stage_velocity_through_the_pressure_stage():
+    # A slip wall with zeroGradient p takes the reference's no-op wall branch
+    # in the pressure stage (cfdCorrectVelocityField.m:67-72 -> :173-175), so
+    # after the correctors U_b is still the projection made by the last
+    # momentum correction, not one of the corrected owner velocity.
+    n, nu = 6, 0.01
+    m = cavity_mesh(n, nz=1, jitter=0.0, renumber_seed=None)
+    case = oracle.SimpleCase(m, oracle.cavity_bcs(m, walls="slip"), nu=nu, p_cycle="V-Cycle")
+    case.iterate(2)
+    case.update_fields()
+    for comp in range(3):
+        case.assemble_u(comp)
+        case.solve_u(comp)
+        case.correct_u(comp)
+    f, b, o = _patch_rows(m, "fixedWalls")
+    Ub = case.field("U")[b]
+    for _ in range(2):
+        case.assemble_p()
+        case.solve_p()
+        case.correct_p()
+    U = case.field("U")
+    assert np.array_equal(U[b], Ub)
+    nrm = _normals(m, f)
+    proj = U[o] - ((U[o] * nrm).sum(1))[:, None] * nrm
+    assert np.abs(proj - Ub).max() > 1e-6     # the owners moved; U_b did not follow

@@ -1,0 +1,107 @@
+"""The reference's cylinder tutorial (uFVM/tutorials/incompressible/cylinder)
+run by the oracle, checked against the output the reference wrote for it.
+
+The case is the reference's own: its polyMesh (479 cells of a non-orthogonal
+2-D mesh around a cylinder), 0/U and 0/p (inlet fixedValue (10 0 0) with
+zeroGradient p; zeroGradient outlet at p = 0; noSlip cylinder wall; symmetry
+top and bottom; empty sides), nu = 1.4775e-5, two correctors, URF 0.7 / 0.3,
+the momentum equation div(U*U) = laplacian(nu*U) + div(nu*transp(grad(U)))
+- grad(p) (run.m), and the reference's default pressure solver (W-cycle,
+ILU, 1/3 sweeps, termination 0.1: cfdDefineEquation.m:118-136).  Time
+directories 0.05, 0.15 and 0.45 (fixtures, written by cfdWriteResults.m
+every 5 steady iterations, deltaT 0.01) hold U, p and mdot_f as MATLAB's
+num2str wrote them.  The oracle's fields, written the same way, must give the
+same strings: every value agrees with the reference's to the written digits.
+"""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from matlab_ufvm_amd.fieldio import num2str
+from matlab_ufvm_amd.foamcase import read_case
+
+CASE = os.path.join(os.path.dirname(__file__), "golden", "cylinder")
+
+
+def written(path):
+    """the internalField entries of a reference field file, as written"""
+    t = open(path).read()
+    body = t[t.index("internalField"):t.index("boundaryField")]
+    body = body[body.index("(") + 1:].replace("(", " ").replace(")", " ")
+    return re.findall(r"[-+0-9.eE]+|NaN|Inf", body)
+
+
+@pytest.fixture(scope="module")
+def case():
+    return read_case(CASE)
+
+
+def test_case_settings(case):
+    m = case.mesh
+    assert (m.n_cells, m.n_interior, m.n_faces) == (479, 906, 1966)
+    assert case.nu == 1.4775e-5 and case.n_correctors == 2
+    assert (case.urf_u, case.urf_p) == (0.7, 0.3)
+    assert case.p_ref == -1          # an inlet and an outlet: no reference cell
+    assert case.u_init == (10.0, 0.0, 0.0) and case.p_init == 0.0
+    assert case.bcs["inlet"] == ("fixedValue", (10.0, 0.0, 0.0), "zeroGradient", 0.0)
+    assert case.bcs["outlet"][::2] == ("zeroGradient", "fixedValue")
+    assert [p.type for p in m.patches] == ["symmetry", "inlet", "wall", "outlet", "empty",
+                                           "empty", "symmetry"]
+
+
+def test_oracle_writes_the_reference_output(case):
+    c = oracle.SimpleCase(case.mesh, case.bcs, stress=True, **case.simple_kwargs())
+    m = case.mesh
+    N = m.n_cells
+    done = 0
+    for t in ("0.05", "0.15", "0.45"):
+        it = int(round(float(t) / 0.01))
+        c.iterate(it - done)
+        done = it
+        for name, vals in (("U", c.field("U")[:N].reshape(-1)), ("p", c.field("p")[:N]),
+                           ("mdot_f", c.field("mdot")[:m.n_interior])):
+            ref = written(os.path.join(CASE, t, name))
+            assert len(ref) == vals.size, (t, name)
+            ours = [num2str(v) for v in vals]
+            bad = [k for k in range(vals.size) if ours[k] != ref[k]]
+            assert not bad, (t, name, len(bad), [(ours[k], ref[k]) for k in bad[:5]])
+    # the quirks that make U_z nonzero on this 2-D mesh are reproduced too
+    assert np.abs(c.field("U")[:N, 2]).max() > 1e-4
+
+
+def test_outlet_pressure_enters_mdot_only_through_the_owner(case):
+    # cfdAssembleMdotTermOutletFixedValueBC (cfdAssembleMdotTerm.m:799-906):
+    # FLUXCb1 = rho_b geoDiff, FLUXCb2 = 0, so mdot_b = geoDiff p_C + (terms of
+    # U_b and grad p_b).  With p = c everywhere (cells and the outlet value)
+    # grad p vanishes to round-off, and raising c from 0 raises each outlet
+    # face's mdot_b by geoDiff c -- geoDiff of DU_b, Sb, eCN, CN recomputed
+    # here -- where a pressure difference p_C - p_b would give 0.
+    m = case.mesh
+    N, Fi = m.n_cells, m.n_interior
+    out = next(p for p in m.patches if p.name == "outlet")
+    f = np.arange(out.start, out.start + out.nfaces)
+    b = N + f - Fi
+    res = {}
+    for cval in (0.0, 7.0):
+        bcs = dict(case.bcs)
+        bcs["outlet"] = bcs["outlet"][:3] + (cval,)
+        kw = case.simple_kwargs()
+        kw["p_init"] = cval
+        c = oracle.SimpleCase(m, bcs, stress=True, **kw)
+        c.update_fields()
+        for comp in range(3):
+            c.assemble_u(comp)
+        DU = c.field("DU")
+        c.assemble_p()
+        res[cval] = (c.field("mdot")[f], c.field("fluxC1")[f], c.field("fluxC2")[f], DU[b])
+    S, CN, eCN = m.Sf[f], m.CN[f], m.eCN[f]
+    DUS = res[0.0][3] * S
+    mag = np.sqrt((DUS ** 2).sum(1))
+    ee = (eCN * (DUS / mag[:, None])).sum(1)
+    geo = np.sqrt((((mag / ee)[:, None] * eCN) ** 2).sum(1)) / np.sqrt((CN ** 2).sum(1))
+    assert np.allclose(res[7.0][1], geo, rtol=1e-13) and np.all(res[7.0][2] == 0.0)
+    assert np.allclose(res[7.0][0] - res[0.0][0], 7.0 * geo, rtol=1e-9, atol=1e-12)
+    assert np.abs(geo).min() > 1e-6
