@@ -208,6 +208,17 @@ int ufvm_pcg_level_colours(ufvm_ctx *ctx, int32_t sys, int32_t level, int32_t *n
  * (cfdAssembleDiffusionTerm.m:567-626, cfdCorrectScalarEquation.m:469-505,
  * cfdCorrectVelocityField.m:274-305, cfdCorrectMdotField.m:306-322) */
 #define UFVM_PATCH_SYMMETRY 3
+/* 'inlet' and 'outlet' patches of the boundary file: an inlet takes U
+ * fixedValue and p zeroGradient (cfdAssembleMdotTerm.m:415-486,
+ * cfdAssembleConvectionTerm.m:144-186, cfdAssembleDiffusionTerm.m:135-173);
+ * an outlet U zeroGradient and p fixedValue (cfdAssembleMdotTerm.m:799-906,
+ * cfdAssembleConvectionTerm.m:191-232, cfdAssembleDiffusionTerm.m:201-218),
+ * with the reference's corrections (cfdCorrectVelocityField.m:247-272 writes
+ * the owners' U into rows numbered by the outlet's FACE indices: kept; rows
+ * past the end of U are not stored, and their zero fill enters the velocity
+ * scale as in the reference; an outlet face index below n_cells is refused) */
+#define UFVM_PATCH_INLET 4
+#define UFVM_PATCH_OUTLET 5
 #define UFVM_BC_FIXED_VALUE 0
 #define UFVM_BC_ZERO_GRADIENT 1
 #define UFVM_BC_EMPTY 2
@@ -215,10 +226,13 @@ int ufvm_pcg_level_colours(ufvm_ctx *ctx, int32_t sys, int32_t level, int32_t *n
  * (cfdAssembleDiffusionTerm.m:479-562, TM from cfdGetUCoef.m:41-44) */
 #define UFVM_BC_NO_SLIP 3
 #define UFVM_BC_SYMMETRY 4
-/* U on a wall: slip -- no diffusion or mass flux through the wall, and U_b the
- * owner's U minus its normal part after each correction
- * (cfdAssembleDiffusionTerm.m:456-478, cfdCorrectScalarEquation.m:239-275,
- * cfdCorrectVelocityField.m:133-160, cfdCorrectMdotField.m:132-149) */
+/* U on a wall: slip -- no diffusion or mass flux through the wall
+ * (cfdAssembleDiffusionTerm.m:456-478, cfdAssembleMdotTerm.m:395-410), and
+ * U_b,c the owner's U_c minus its normal part after each momentum correction
+ * (cfdCorrectScalarEquation.m:239-293).  The pressure-stage corrections
+ * dispatch on p's condition (cfdCorrectVelocityField.m:57-121,
+ * cfdCorrectMdotField.m:56-120): with p zeroGradient the wall takes the
+ * no-op wall branch, so U_b keeps its momentum-stage value. */
 #define UFVM_BC_SLIP 5
 typedef struct {
   int32_t n_cells, n_faces, n_interior;
@@ -233,8 +247,13 @@ typedef struct {
 
 /* Steady incompressible SIMPLE settings: theEquation fields of U and p
  * (uFVM/src/fvm/top/cfdDefineEquation.m:84-136), fvSolution SIMPLE block and
- * the per-patch boundary conditions (U: fixedValue with a value, or empty;
- * p: zeroGradient or empty).  Pressure-correction solver: 0 = cfdApplyAMG
+ * the per-patch boundary conditions, by patch type:
+ *   wall:     U fixedValue (u_value), noSlip or slip; p zeroGradient
+ *   empty:    U and p empty
+ *   symmetry: U and p symmetry
+ *   inlet:    U fixedValue (u_value); p zeroGradient
+ *   outlet:   U zeroGradient; p fixedValue (p_value)
+ * Pressure-correction solver: 0 = cfdApplyAMG
  * (stationary cycles, hierarchy rebuilt every solve as the reference does),
  * 1 = AMG-PCG with the hierarchy rebuilt every solve, 2 = AMG-PCG with the
  * hierarchy of the first solve kept. */
@@ -254,10 +273,15 @@ typedef struct {
   int32_t stress_term;                    /* 1: the momentum equation has
                                              + div(nu*transp(grad(U))) (the reference's
                                              tutorials; cfdAssembleExplicitStressTerm.m) */
+  const double *p_value;                  /* [n_patches] fixed p of outlets; NULL: 0 */
+  double u_init[3];                       /* uniform internal U of the 0 directory
+                                             (cfdDefineEquation.m:68-82) */
+  double p_init;                          /* uniform internal p */
 } ufvm_simple_cfg;
 
-/* Upload a mesh and create its fields (uniform zero initial conditions,
- * boundary values and mdot_f initialised as cfdInitializeCase.m:17-83). */
+/* Upload a mesh and create its fields (the uniform initial fields u_init and
+ * p_init; boundary values and mdot_f initialised as cfdInitializeCase.m:17-83,
+ * cfdInitializeBoundaryConditions.m, cfdInitializeMdotField.m:72-80). */
 int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *mesh, const ufvm_simple_cfg *cfg,
                      int32_t *case_out);
 int ufvm_case_destroy(ufvm_ctx *ctx, int32_t cas);

@@ -68,7 +68,8 @@ class SimpleCfg(C.Structure):
                 ("n_correctors", C.c_int32), ("p_ref_cell", C.c_int32), ("u_sweeps", C.c_int32),
                 ("u_bc", i32p), ("u_value", f64p), ("p_bc", i32p), ("p_solver", C.c_int32),
                 ("p_amg", AmgCfg), ("p_pcg", PcgCfg), ("u_smoother", C.c_int32),
-                ("stress_term", C.c_int32)]
+                ("stress_term", C.c_int32), ("p_value", f64p), ("u_init", C.c_double * 3),
+                ("p_init", C.c_double)]
 
 
 # name -> (restype, argtypes)

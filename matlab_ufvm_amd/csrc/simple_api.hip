@@ -23,7 +23,7 @@ struct Case {
   int32_t N = 0, F = 0, Fi = 0, Fb = 0, P = 0;
   DBuf<int32_t> owner, neighbour, cptr, ccol, own_nnz, nei_nnz, cf_ptr, cf_face, cf_slot, bpatch;
   DBuf<int32_t> ptype, u_bc, p_bc;
-  DBuf<double> Sf, area, CN, eCN, gDiff, T, gf, fc, vol, cc, u_val, walldist, noslip_sum;
+  DBuf<double> Sf, area, CN, eCN, gDiff, T, gf, fc, vol, cc, u_val, p_val, walldist, noslip_sum;
   DBuf<double> U, gradU, p, gradp, PP, gradPP, DU, DUT, mdot, gamma, fluxC1, fluxC2, ac_old;
   DBuf<double> uface;  // momentum face fluxes, 6 per face
   DBuf<double> ustress;  // explicit stress flux per face (stress_term only)
@@ -33,9 +33,13 @@ struct Case {
   sk::DevMesh dm{};
   ufvm_simple_cfg cfg{};
   std::vector<int32_t> h_ptype, h_u_bc, h_p_bc;
-  std::vector<double> h_u_val;
+  std::vector<double> h_u_val, h_p_val;
   int32_t sysU = 0, sysP = 0;
   bool p_hier_built = false;
+  // a fixed-pressure outlet's velocity correction writes U rows past the end
+  // with unwritten rows between them (outlet_gap); once it has run, the
+  // reference's grown U holds zero rows that its scale sees (scale_zero)
+  bool outlet_gap = false, scale_zero = false;
 };
 
 }  // namespace ufvm
@@ -54,31 +58,39 @@ void up(DBuf<T> &d, const T *h, size_t n, hipStream_t s, const char *what) {
   d.upload(h, n, s);
 }
 
-// cfdInitializeCase.m: zero fields; boundary values from the BCs
-// (fixedValue -> value, empty -> owner minus its normal part, else owner);
-// mdot_f = rho_f (S_f . U_f) with Average interpolation (cfdInitializeMdotField.m:72-80)
+// cfdInitializeCase.m: the uniform internal fields u_init / p_init; boundary
+// values from the BCs (cfdInitializeBoundaryConditions.m: fixedValue walls and
+// inlets the value; noSlip zero; slip walls, empty and symmetry planes the
+// owner minus its normal part; zeroGradient outlets an owner copy; p_b an owner
+// copy or, on fixed-pressure outlets, the value); mdot_f = rho_f (S_f . U_f)
+// with Average interpolation (cfdInitializeMdotField.m:72-80)
 void init_fields(ufvm_ctx *ctx, Case &C) {
   hipStream_t s = ctx->stream;
   const int32_t NB = C.N + C.Fb;
-  std::vector<double> U((size_t)NB * 3, 0.0), zero((size_t)NB * 9, 0.0), mdot(C.F, 0.0);
-  std::vector<int32_t> owner(C.F);
-  std::vector<double> Sf((size_t)C.F * 3), gf(C.Fi);
+  std::vector<double> U((size_t)NB * 3, 0.0), P(NB, 0.0), zero((size_t)NB * 9, 0.0), mdot(C.F, 0.0);
+  std::vector<int32_t> owner(C.F), nbr(std::max(C.Fi, 1));
+  std::vector<double> Sf((size_t)C.F * 3), gf(std::max(C.Fi, 1));
   C.owner.download(owner.data(), C.F, s);
+  if (C.Fi > 0) C.neighbour.download(nbr.data(), C.Fi, s);
   C.Sf.download(Sf.data(), (size_t)C.F * 3, s);
-  C.gf.download(gf.data(), C.Fi, s);
+  if (C.Fi > 0) C.gf.download(gf.data(), C.Fi, s);
   std::vector<int32_t> bp(C.Fb);
-  C.bpatch.download(bp.data(), C.Fb, s);
+  if (C.Fb > 0) C.bpatch.download(bp.data(), C.Fb, s);
   UFVM_HIP(hipStreamSynchronize(s));
+  for (int32_t i = 0; i < C.N; ++i) {
+    for (int d = 0; d < 3; ++d) U[i * 3 + d] = C.cfg.u_init[d];
+    P[i] = C.cfg.p_init;
+  }
   for (int32_t k = 0; k < C.Fb; ++k) {
     const int p = bp[k];
     const int32_t f = C.Fi + k, b = C.N + k, o = owner[f];
-    if (C.h_u_bc[p] == UFVM_BC_FIXED_VALUE) {
+    const int t = C.h_ptype[p], ub = C.h_u_bc[p];
+    if (ub == UFVM_BC_FIXED_VALUE) {
       for (int d = 0; d < 3; ++d) U[b * 3 + d] = C.h_u_val[p * 3 + d];
-    } else if (C.h_u_bc[p] == UFVM_BC_NO_SLIP) {
-      // noSlip's value '[0;0;0]' (cfdDefineEquation.m:290-295), set once
-      // (cfdInitializeBoundaryConditions.m:239-267) and never updated
+    } else if (ub == UFVM_BC_NO_SLIP) {
+      // noSlip's value '[0;0;0]' (cfdDefineEquation.m:290-295)
       for (int d = 0; d < 3; ++d) U[b * 3 + d] = 0.0;
-    } else if (C.h_u_bc[p] == UFVM_BC_EMPTY || C.h_u_bc[p] == UFVM_BC_SYMMETRY) {
+    } else if (t == UFVM_PATCH_EMPTY || t == UFVM_PATCH_SYMMETRY || ub == UFVM_BC_SLIP) {
       const double *S = &Sf[f * 3];
       const double nm = std::sqrt((S[0] * S[0] + S[1] * S[1]) + S[2] * S[2]);
       double n[3] = {S[0] / nm, S[1] / nm, S[2] / nm};
@@ -87,22 +99,26 @@ void init_fields(ufvm_ctx *ctx, Case &C) {
     } else {
       for (int d = 0; d < 3; ++d) U[b * 3 + d] = U[o * 3 + d];
     }
+    P[b] = C.h_p_bc[p] == UFVM_BC_FIXED_VALUE ? C.h_p_val[p] : P[o];
   }
-  // interior U is zero, so only boundary faces can carry an initial flux
-  for (int32_t f = C.Fi; f < C.F; ++f) {
-    const int32_t b = C.N + (f - C.Fi);
+  for (int32_t f = 0; f < C.F; ++f) {
     const double *S = &Sf[f * 3];
-    mdot[f] = 1.0 * ((S[0] * U[b * 3] + S[1] * U[b * 3 + 1]) + S[2] * U[b * 3 + 2]);
-  }
-  for (int32_t f = 0; f < C.Fi; ++f) {
-    const double g = gf[f];
-    const double rf = g * 1.0 + (1 - g) * 1.0;
-    const double *S = &Sf[f * 3];
-    mdot[f] = rf * ((S[0] * 0.0 + S[1] * 0.0) + S[2] * 0.0);
+    double uf[3], rf;
+    if (f < C.Fi) {
+      const double g = gf[f];
+      const int32_t o = owner[f], nb = nbr[f];
+      for (int d = 0; d < 3; ++d) uf[d] = g * U[nb * 3 + d] + (1 - g) * U[o * 3 + d];
+      rf = g * 1.0 + (1 - g) * 1.0;
+    } else {
+      const int32_t b = C.N + (f - C.Fi);
+      for (int d = 0; d < 3; ++d) uf[d] = U[b * 3 + d];
+      rf = 1.0;
+    }
+    mdot[f] = rf * ((S[0] * uf[0] + S[1] * uf[1]) + S[2] * uf[2]);
   }
   C.U.upload(U, s);
   C.gradU.upload(zero, s);
-  C.p.upload(zero.data(), NB, s);
+  C.p.upload(P, s);
   C.gradp.upload(zero.data(), (size_t)NB * 3, s);
   C.PP.upload(zero.data(), NB, s);
   C.gradPP.upload(zero.data(), (size_t)NB * 3, s);
@@ -148,8 +164,9 @@ void update_fields(ufvm_ctx *ctx, Case &C) {
   sk::update_u_bc(s, C.dm, C.U.p);
   sk::gradient(s, C.dm, C.U.p, 3, C.gradU.p);
   sk::gamma_faces(s, C.dm, C.cfg.nu, C.gamma.p);
-  ensure_partial(ctx, (int64_t)(C.N + C.Fb) * 3, 2);
-  sk::scale_u(s, C.dm, C.U.p, ctx->partial.p, ctx->counter.p, ctx->scal.p);
+  ensure_partial(ctx, (int64_t)(C.N + C.Fb) * 3 + 1, 2);
+  sk::scale_u(s, C.dm, C.U.p, C.scale_zero, ctx->partial.p, ctx->counter.p, ctx->scal.p);
+  sk::update_p_bc(s, C.dm, C.p.p);
   sk::gradient(s, C.dm, C.p.p, 1, C.gradp.p);
 }
 
@@ -232,6 +249,7 @@ void correct(ufvm_ctx *ctx, Case &C, int eq, int comp) {
     System &S = sys_of(ctx, C.sysP);
     sk::correct_p(s, C.dm, C.cfg.p_ref_cell, C.cfg.urf_p, S.lev[0]->dphi.p, C.PP.p, C.gradPP.p,
                   C.DU.p, C.U.p, C.p.p, C.fluxC1.p, C.fluxC2.p, C.mdot.p);
+    if (C.outlet_gap) C.scale_zero = true;
   }
   UFVM_HIP(hipStreamSynchronize(s));
 }
@@ -264,8 +282,10 @@ int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *m, const ufvm_simple_c
       if (st < C.Fi || nf < 0 || st + nf > C.F) throw Error(UFVM_ERR_ARG, "bad patch range");
       for (int32_t f = st; f < st + nf; ++f) bp[f - C.Fi] = p;
       const int t = m->patch_type[p];
-      if (t != UFVM_PATCH_WALL && t != UFVM_PATCH_EMPTY && t != UFVM_PATCH_SYMMETRY)
-        throw Error(UFVM_ERR_UNSUPPORTED, "only wall, empty and symmetry patches are supported");
+      if (t != UFVM_PATCH_WALL && t != UFVM_PATCH_EMPTY && t != UFVM_PATCH_SYMMETRY &&
+          t != UFVM_PATCH_INLET && t != UFVM_PATCH_OUTLET)
+        throw Error(UFVM_ERR_UNSUPPORTED,
+                    "only wall, empty, symmetry, inlet and outlet patches are supported");
       const int ub = cfg->u_bc[p], pb = cfg->p_bc[p];
       if (t == UFVM_PATCH_WALL &&
           !((ub == UFVM_BC_FIXED_VALUE || ub == UFVM_BC_NO_SLIP || ub == UFVM_BC_SLIP) &&
@@ -276,6 +296,37 @@ int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *m, const ufvm_simple_c
         throw Error(UFVM_ERR_UNSUPPORTED, "empty patches need empty U and p conditions");
       if (t == UFVM_PATCH_SYMMETRY && !(ub == UFVM_BC_SYMMETRY && pb == UFVM_BC_SYMMETRY))
         throw Error(UFVM_ERR_UNSUPPORTED, "symmetry patches need symmetry U and p conditions");
+      if (t == UFVM_PATCH_INLET && !(ub == UFVM_BC_FIXED_VALUE && pb == UFVM_BC_ZERO_GRADIENT))
+        throw Error(UFVM_ERR_UNSUPPORTED, "inlets need U fixedValue and p zeroGradient");
+      if (t == UFVM_PATCH_OUTLET) {
+        if (!(ub == UFVM_BC_ZERO_GRADIENT && pb == UFVM_BC_FIXED_VALUE))
+          throw Error(UFVM_ERR_UNSUPPORTED, "outlets need U zeroGradient and p fixedValue");
+        // the reference's outlet velocity correction writes rows numbered by
+        // the face index; below n_cells those rows are cell values
+        if (st < C.N)
+          throw Error(UFVM_ERR_UNSUPPORTED,
+                      "an outlet face index below n_cells: the reference's outlet velocity "
+                      "correction would overwrite cell values");
+      }
+    }
+    {
+      // rows past the end of U written by the outlets, and whether any row
+      // between the end and the last of them stays unwritten
+      const int32_t NB = C.N + C.Fb;
+      int32_t hi = -1;
+      std::vector<char> hit;
+      for (int p = 0; p < C.P; ++p)
+        if (m->patch_type[p] == UFVM_PATCH_OUTLET)
+          hi = std::max(hi, m->patch_start[p] + m->patch_nfaces[p] - 1);
+      if (hi >= NB) {
+        hit.assign((size_t)(hi - NB + 1), 0);
+        for (int p = 0; p < C.P; ++p)
+          if (m->patch_type[p] == UFVM_PATCH_OUTLET)
+            for (int32_t f = std::max(m->patch_start[p], NB);
+                 f < m->patch_start[p] + m->patch_nfaces[p]; ++f)
+              hit[f - NB] = 1;
+        for (char h : hit) C.outlet_gap |= !h;
+      }
     }
     for (int32_t k = 0; k < C.Fb; ++k)
       if (bp[k] < 0) throw Error(UFVM_ERR_ARG, "boundary face outside every patch");
@@ -340,10 +391,13 @@ int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *m, const ufvm_simple_c
     C.h_u_bc.assign(cfg->u_bc, cfg->u_bc + C.P);
     C.h_p_bc.assign(cfg->p_bc, cfg->p_bc + C.P);
     C.h_u_val.assign(cfg->u_value, cfg->u_value + 3 * C.P);
+    C.h_p_val.assign(C.P, 0.0);
+    if (cfg->p_value) C.h_p_val.assign(cfg->p_value, cfg->p_value + C.P);
     C.ptype.upload(C.h_ptype.empty() ? std::vector<int32_t>{0} : C.h_ptype, s);
     C.u_bc.upload(C.h_u_bc.empty() ? std::vector<int32_t>{0} : C.h_u_bc, s);
     C.p_bc.upload(C.h_p_bc.empty() ? std::vector<int32_t>{0} : C.h_p_bc, s);
     C.u_val.upload(C.h_u_val.empty() ? std::vector<double>{0.0} : C.h_u_val, s);
+    C.p_val.upload(C.h_p_val.empty() ? std::vector<double>{0.0} : C.h_p_val, s);
     {
       // boundary-face walldist = (CN . Sf) / |Sf| (cfdProcessOpenFoamMesh.m:153)
       // and, per patch and component, the sum in face order of (Sf_c/area)^2
@@ -366,6 +420,7 @@ int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *m, const ufvm_simple_c
     C.cfg.u_bc = nullptr;
     C.cfg.u_value = nullptr;
     C.cfg.p_bc = nullptr;
+    C.cfg.p_value = nullptr;
     if (C.cfg.p_ref_cell >= C.N) throw Error(UFVM_ERR_ARG, "p_ref_cell out of range");
     sk::DevMesh &d = C.dm;
     d.N = C.N; d.F = C.F; d.Fi = C.Fi;
@@ -376,6 +431,7 @@ int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *m, const ufvm_simple_c
     d.cf_ptr = C.cf_ptr.p; d.cf_face = C.cf_face.p; d.cf_slot = C.cf_slot.p;
     d.bpatch = C.bpatch.p;
     d.ptype = C.ptype.p; d.u_bc = C.u_bc.p; d.p_bc = C.p_bc.p; d.u_val = C.u_val.p;
+    d.p_val = C.p_val.p;
     d.walldist = C.walldist.p; d.noslip_sum = C.noslip_sum.p;
     init_fields(ctx, C);
     C.sysU = create_system(ctx, C.N, nnz, m->cconn_ptr, m->cconn);

@@ -9,7 +9,8 @@
 namespace ufvm {
 namespace sk {
 
-enum { PATCH_WALL = 0, PATCH_EMPTY = 1, PATCH_PATCH = 2, PATCH_SYMMETRY = 3 };
+enum { PATCH_WALL = 0, PATCH_EMPTY = 1, PATCH_PATCH = 2, PATCH_SYMMETRY = 3, PATCH_INLET = 4,
+       PATCH_OUTLET = 5 };
 enum { BC_FIXED_VALUE = 0, BC_ZERO_GRADIENT = 1, BC_EMPTY = 2, BC_NO_SLIP = 3, BC_SYMMETRY = 4,
        BC_SLIP = 5 };
 
@@ -26,6 +27,7 @@ struct DevMesh {
   const int32_t *bpatch;             // patch of each boundary face
   const int32_t *ptype, *u_bc, *p_bc;
   const double *u_val;
+  const double *p_val;               // [patches] fixed p of outlets
   const double *walldist;            // [F] (CN . Sf) / |Sf| of boundary faces, 0 inside
   const double *noslip_sum;          // [patches][3] sum over the patch of (Sf_c / area)^2
 };
@@ -34,9 +36,12 @@ struct DevMesh {
 enum { S_UMAX = 0, S_UMIN = 1, S_SCALE = 2, S_RMAX = 3, S_RSQ = 4 };
 
 void update_u_bc(hipStream_t s, const DevMesh &m, double *U);
+void update_p_bc(hipStream_t s, const DevMesh &m, double *p);
 void gradient(hipStream_t s, const DevMesh &m, const double *phi, int nc, double *grad);
 void gamma_faces(hipStream_t s, const DevMesh &m, double nu, double *gamma);
-void scale_u(hipStream_t s, const DevMesh &m, const double *U, double *partial,
+// zero: the max and min also take the value 0 (the zero-filled rows the
+// reference's U gains past its end from a fixed-pressure outlet's correction)
+void scale_u(hipStream_t s, const DevMesh &m, const double *U, bool zero, double *partial,
              unsigned *counter, double *scal);
 void assemble_u(hipStream_t s, const DevMesh &m, int comp, double urf, const double *U,
                 const double *gradU, const double *gradp, const double *mdot,

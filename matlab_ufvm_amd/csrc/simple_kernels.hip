@@ -73,7 +73,6 @@ k_reduce2(const double *__restrict__ partial, int32_t nblk, int mode, double *sc
 
 // ------------------------------------------------------ boundary conditions
 
-// fixedValue walls: U_b = value (cfdUpdateBoundaryConditions.m:103-139)
 // empty and symmetry patches: every symmetry branch of the reference's steady
 // path does its empty branch's arithmetic (diffusion
 // cfdAssembleDiffusionTerm.m:567-626 vs :631-690, zero convection, no gradient
@@ -84,21 +83,51 @@ __device__ inline bool plane(const DevMesh &m, int p) {
   return m.ptype[p] == PATCH_EMPTY || m.ptype[p] == PATCH_SYMMETRY;
 }
 
-// patches whose corrections make U_b tangential and zero mdot_b: empty and
-// symmetry planes, and slip walls (cfdCorrectScalarEquation.m:239-275,
-// cfdCorrectVelocityField.m:133-160, cfdCorrectMdotField.m:132-149); a slip
-// wall's diffusion and mass fluxes are zero and its gradient rows are a wall's
-__device__ inline bool tangential(const DevMesh &m, int p) {
+// the momentum corrections dispatch on U's condition
+// (cfdCorrectScalarEquation.m:31-107): U_b,c = U_O,c - (U_O . n) n_c on the
+// planes and on slip walls (:239-293); a slip wall's diffusion and mass fluxes
+// are zero and its gradient rows are a wall's
+__device__ inline bool u_tangential(const DevMesh &m, int p) {
   return plane(m, p) || (m.ptype[p] == PATCH_WALL && m.u_bc[p] == BC_SLIP);
 }
 
+// inlets and outlets carry a boundary mass flux in the pressure-correction
+// assembly (cfdAssembleMdotTerm.m:415-486, :799-906); walls and planes do not
+__device__ inline bool p_flux(const DevMesh &m, int p) {
+  return m.ptype[p] == PATCH_INLET || m.ptype[p] == PATCH_OUTLET;
+}
+
+// an outlet with a fixed pressure (its p_bc is fixedValue)
+__device__ inline bool p_fixed_outlet(const DevMesh &m, int p) {
+  return m.ptype[p] == PATCH_OUTLET && m.p_bc[p] == BC_FIXED_VALUE;
+}
+
+// fixedValue walls and inlets: U_b = value (cfdUpdateBoundaryConditions.m:103-139,
+// :291-328); noSlip walls: U_b = their zero value (:213-278)
 __global__ void k_update_u_bc(DevMesh m, double *U) {
   const int32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= m.F - m.Fi) return;
   const int p = m.bpatch[k];
-  if (m.ptype[p] != PATCH_WALL || m.u_bc[p] != BC_FIXED_VALUE) return;
   const int32_t b = m.N + k;
-  for (int d = 0; d < 3; ++d) U[b * 3 + d] = m.u_val[p * 3 + d] * 1.0;
+  const bool wall_or_inlet = m.ptype[p] == PATCH_WALL || m.ptype[p] == PATCH_INLET;
+  if (wall_or_inlet && m.u_bc[p] == BC_FIXED_VALUE) {
+    for (int d = 0; d < 3; ++d) U[b * 3 + d] = m.u_val[p * 3 + d] * 1.0;
+  } else if (m.ptype[p] == PATCH_WALL && m.u_bc[p] == BC_NO_SLIP) {
+    for (int d = 0; d < 3; ++d) U[b * 3 + d] = 0.0 * 1.0;
+  }
+}
+
+// fixed-pressure outlets: p_b = value (cfdUpdateBoundaryConditions.m:335-373)
+__global__ void k_update_p_bc(DevMesh m, double *p) {
+  const int32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= m.F - m.Fi) return;
+  const int q = m.bpatch[k];
+  if (p_fixed_outlet(m, q)) p[m.N + k] = m.p_val[q] * 1.0;
+}
+
+void update_p_bc(hipStream_t s, const DevMesh &m, double *p) {
+  const int32_t Fb = m.F - m.Fi;
+  if (Fb > 0) hipLaunchKernelGGL(k_update_p_bc, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, p);
 }
 
 void update_u_bc(hipStream_t s, const DevMesh &m, double *U) {
@@ -229,16 +258,19 @@ __global__ void __launch_bounds__(kBlock) k_gauss_cells(DevMesh m, const double 
   for (int q = 0; q < NC * 3; ++q) grad[i * NC * 3 + q] = g[q] / v;
 }
 
-// boundary rows: the owner's gradient, then on wall and 'patch' patches
-// g - (g.e)e + ((phi_b - phi_O)/|d|) e with d = C_f - C_O
-// (cfdUpdateGradient.m:104-134)
+// boundary rows: the owner's gradient, then on wall, 'patch', inlet and outlet
+// patches g - (g.e)e + ((phi_b - phi_O)/|d|) e with d = C_f - C_O
+// (cfdUpdateGradient.m:81-134 walls, :142-195 inlets, :202-255 outlets).  The
+// inlet and outlet rows index phi linearly (phi(iBElement) - phi(iOwner)), so
+// every component of a vector takes its first component's difference: kept.
 template <int NC>
 __global__ void k_gauss_bnd(DevMesh m, const double *phi, double *grad) {
   const int32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= m.F - m.Fi) return;
   const int32_t f = m.Fi + k, o = m.owner[f], b = m.N + k;
   const int p = m.bpatch[k];
-  const bool wall = (m.ptype[p] == PATCH_WALL || m.ptype[p] == PATCH_PATCH);
+  const bool lin = (m.ptype[p] == PATCH_INLET || m.ptype[p] == PATCH_OUTLET);
+  const bool wall = (m.ptype[p] == PATCH_WALL || m.ptype[p] == PATCH_PATCH || lin);
   double d[3], e[3], dmag = 1.0;
   if (wall) {
     for (int q = 0; q < 3; ++q) d[q] = m.fc[f * 3 + q] - m.cc[o * 3 + q];
@@ -251,7 +283,8 @@ __global__ void k_gauss_bnd(DevMesh m, const double *phi, double *grad) {
       for (int q = 0; q < 3; ++q) grad[(b * NC + c) * 3 + q] = g[q];
     } else {
       const double ge = dot3(g, e);
-      const double t = (phi[b * NC + c] - phi[o * NC + c]) / dmag;
+      const int cd = lin ? 0 : c;
+      const double t = (phi[b * NC + cd] - phi[o * NC + cd]) / dmag;
       for (int q = 0; q < 3; ++q) grad[(b * NC + c) * 3 + q] = (g[q] - ge * e[q]) + t * e[q];
     }
   }
@@ -285,19 +318,23 @@ void gamma_faces(hipStream_t s, const DevMesh &m, double nu, double *gamma) {
 }
 
 // phiScale = max(phiMax - phiMin, |phiMax|), 1 if < 1e-6 (cfdUpdateScale.m:17-23)
-__global__ void __launch_bounds__(kBlock) k_scale_u(int64_t n, const double *U, double *partial) {
+// (zero: one more value, 0, for the zero-filled rows of a grown U)
+__global__ void __launch_bounds__(kBlock) k_scale_u(int64_t n, bool zero, const double *U,
+                                                    double *partial) {
   const int64_t q = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   double mx = -INFINITY, mn = INFINITY;
   if (q < n) mx = mn = U[q];
+  else if (q == n && zero) mx = mn = 0.0;
   block_partials2(mx, mn, 1, 2, partial);
 }
 
-void scale_u(hipStream_t s, const DevMesh &m, const double *U, double *partial,
+void scale_u(hipStream_t s, const DevMesh &m, const double *U, bool zero, double *partial,
              unsigned *counter, double *scal) {
   (void)counter;
   const int64_t n = (int64_t)(m.N + m.F - m.Fi) * 3;
-  hipLaunchKernelGGL(k_scale_u, dim3(nblocks(n)), dim3(kBlock), 0, s, n, U, partial);
-  hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kBlock), 0, s, partial, nblocks(n), 0, scal);
+  const int64_t nb = nblocks(n + (zero ? 1 : 0));
+  hipLaunchKernelGGL(k_scale_u, dim3(nb), dim3(kBlock), 0, s, n, zero, U, partial);
+  hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kBlock), 0, s, partial, nb, 0, scal);
 }
 
 // --------------------------------------------------------- momentum assembly
@@ -323,8 +360,9 @@ __device__ inline void grad_face(const DevMesh &m, const double *phi, const doub
 
 // Face fluxes of momentum component c, six per face (uf[6 f + q]):
 //   convection, sign +1 (cfdAssembleConvectionTerm.m:103-137): q 0..2 =
-//     C1 = max(mdot,0), C2 = -max(-mdot,0), T = C1 U_O + C2 U_N; walls and
-//     empty patches contribute zero (:238-278; left 0 here);
+//     C1 = max(mdot,0), C2 = -max(-mdot,0), T = C1 U_O + C2 U_N (U_b on
+//     inlets and outlets, :144-232); walls and planes contribute zero
+//     (:238-302; left 0 here);
 //   diffusion laplacian(nu*U), sign -1: q 3..5 = C1, C2, T; interior with the
 //     corrected face gradient (cfdAssembleDiffusionTerm.m:94-130), fixedValue
 //     walls (:224-263), zeroGradient (:330-347, zero), empty vector patches
@@ -374,8 +412,20 @@ k_u_faces(DevMesh m, int c, const double *__restrict__ U, const double *__restri
   } else {
     const int32_t k = f - Fi, b = N + k, i = m.owner[f];
     const int p = m.bpatch[k];
+    if (m.ptype[p] == PATCH_INLET || m.ptype[p] == PATCH_OUTLET) {
+      // fixedValue inlets and zeroGradient outlets: the interior upwind fluxes
+      // with U_b in place of U_N (cfdAssembleConvectionTerm.m:144-232)
+      const double psi = mdot[f];
+      const double C1 = 1.0 * fmax(psi, 0.0);
+      const double C2 = (-1.0) * fmax(-psi, 0.0);
+      out[0] = C1;
+      out[1] = C2;
+      out[2] = (C1 * U[i * 3 + c] + C2 * U[b * 3 + c]) + 0.0;
+    }
     double C1 = 0.0, T = 0.0;
-    if (m.ptype[p] == PATCH_WALL && m.u_bc[p] == BC_FIXED_VALUE) {
+    if ((m.ptype[p] == PATCH_WALL || m.ptype[p] == PATCH_INLET) && m.u_bc[p] == BC_FIXED_VALUE) {
+      // fixedValue walls (cfdAssembleDiffusionTerm.m:224-263) and inlets
+      // (:135-173) have the same fluxes; zeroGradient outlets none (:201-218)
       C1 = (1.0 * gam) * m.gDiff[f];
       const double C2 = ((-1.0) * gam) * m.gDiff[f];
       const double V = ((-1.0) * gam) * dot3(&gradU[(b * 3 + c) * 3], &m.T[f * 3]);
@@ -454,8 +504,9 @@ __device__ __forceinline__ void conv_entry(const UEntry &x, int32_t Fi, double &
       bc = bc + x.q[2];
     }
   } else {
-    ac = ac + 0.0;
-    bc = bc - 0.0;
+    // boundary faces: ac += C1, bc -= T (zero except on inlets and outlets)
+    ac = ac + x.q[0];
+    bc = bc - x.q[2];
   }
 }
 
@@ -625,28 +676,31 @@ __global__ void k_correct_u_cells(DevMesh m, int c, const double *dphi, double *
   if (i < m.N) U[i * 3 + c] = U[i * 3 + c] + dphi[i];
 }
 
-// empty patches: U_b,c = U_O,c - (U_O . n) n_c (cfdCorrectScalarEquation.m, cfdCorrectEmptyBC);
-// comp < 0: all three components (cfdCorrectVelocityField.m, empty patches)
-__global__ void k_empty_u(DevMesh m, int c, double *U) {
+// boundary rows of the momentum correction of component c, by U's condition
+// (cfdCorrectScalarEquation.m:31-107): slip walls, empty and symmetry planes
+// U_b,c = U_O,c - (U_O . n) n_c (:239-293, :469-584); zeroGradient outlets
+// U_b,c = U_O,c (:376-407); fixedValue / noSlip walls and inlets unchanged
+__global__ void k_bnd_u(DevMesh m, int c, double *U) {
   const int32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= m.F - m.Fi) return;
-  if (!tangential(m, m.bpatch[k])) return;
+  const int p = m.bpatch[k];
   const int32_t f = m.Fi + k, o = m.owner[f], b = m.N + k;
+  if (m.ptype[p] == PATCH_OUTLET && m.u_bc[p] == BC_ZERO_GRADIENT) {
+    U[b * 3 + c] = U[o * 3 + c];
+    return;
+  }
+  if (!u_tangential(m, p)) return;
   double n[3];
   const double nm = mag3(&m.Sf[f * 3]);
   for (int d = 0; d < 3; ++d) n[d] = m.Sf[f * 3 + d] / nm;
   const double un = dot3(&U[o * 3], n);
-  if (c >= 0) {
-    U[b * 3 + c] = U[o * 3 + c] - un * n[c];
-  } else {
-    for (int d = 0; d < 3; ++d) U[b * 3 + d] = U[o * 3 + d] - un * n[d];
-  }
+  U[b * 3 + c] = U[o * 3 + c] - un * n[c];
 }
 
 void correct_u(hipStream_t s, const DevMesh &m, int comp, const double *dphi, double *U) {
   hipLaunchKernelGGL(k_correct_u_cells, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, comp, dphi, U);
   const int32_t Fb = m.F - m.Fi;
-  if (Fb > 0) hipLaunchKernelGGL(k_empty_u, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, comp, U);
+  if (Fb > 0) hipLaunchKernelGGL(k_bnd_u, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, comp, U);
 }
 
 // ------------------------------------------------- pressure-correction path
@@ -655,7 +709,8 @@ void correct_u(hipStream_t s, const DevMesh &m, int comp, const double *dphi, do
 // mdot_f = C1 p_O + C2 p_N + V with C1 = rho_f geoDiff = -C2 and
 // V = rho_f Ubar.S - rho_f grad(p)_f.DUT + rho_f grad(p)bar_f.DUS
 //     + (1-urf_U)(mdot_prev - rho_f Ubar.S);  boundary fluxes of walls and
-// empty patches are zero (:281-350).
+// planes are zero, inlets and fixed-pressure outlets get theirs and a new
+// mdot_b (:281-348).
 __global__ void __launch_bounds__(kBlock)
 k_rhie_chow(DevMesh m, double urf_u, const double *__restrict__ U, const double *__restrict__ DU,
             const double *__restrict__ p, const double *__restrict__ gradp, double *mdot,
@@ -663,8 +718,39 @@ k_rhie_chow(DevMesh m, double urf_u, const double *__restrict__ U, const double 
   const int32_t f = blockIdx.x * kBlock + threadIdx.x;
   if (f >= m.F) return;
   if (f >= m.Fi) {
-    fluxC1[f] = 0.0;
-    fluxC2[f] = 0.0;
+    const int32_t k = f - m.Fi, b = m.N + k, o = m.owner[f];
+    const int q = m.bpatch[k];
+    if (!p_flux(m, q)) {
+      fluxC1[f] = 0.0;
+      fluxC2[f] = 0.0;
+      return;
+    }
+    const double *Sb = &m.Sf[f * 3];
+    double C1 = 0.0, C2 = 0.0, V = 0.0;
+    if (m.ptype[q] == PATCH_INLET) {
+      // zeroGradient-p inlet: term I only (cfdAssembleMdotTerm.m:415-486)
+      V = V + 1.0 * dot3(Sb, &U[b * 3]);
+    } else {
+      // fixed-pressure outlet: terms I-III from the boundary rows of DU and
+      // grad(p), FLUXCb2 = 0 (cfdAssembleMdotTerm.m:799-906)
+      const double *CNb = &m.CN[f * 3], *eCNb = &m.eCN[f * 3];
+      double DUS[3], eDUS[3], DUE[3], DUT[3];
+      for (int d = 0; d < 3; ++d) DUS[d] = DU[b * 3 + d] * Sb[d];
+      const double mDUS = mag3(DUS);
+      for (int d = 0; d < 3; ++d) eDUS[d] = DUS[d] / mDUS;
+      const double ee = dot3(eCNb, eDUS);
+      for (int d = 0; d < 3; ++d) DUE[d] = (mDUS / ee) * eCNb[d];
+      const double geoDiff = mag3(DUE) / mag3(CNb);
+      for (int d = 0; d < 3; ++d) DUT[d] = DUS[d] - DUE[d];
+      V = V + 1.0 * dot3(&U[b * 3], Sb);
+      C1 = C1 + 1.0 * geoDiff;
+      C2 = C2 - 0.0;
+      V = V - 1.0 * dot3(&gradp[b * 3], DUT);
+      V = V + 1.0 * dot3(&gradp[b * 3], DUS);
+    }
+    mdot[f] = (C1 * p[o] + C2 * p[b]) + V;
+    fluxC1[f] = C1;
+    fluxC2[f] = C2;
     return;
   }
   const int32_t o = m.owner[f], nb = m.neighbour[f];
@@ -756,6 +842,11 @@ k_assemble_p(DevMesh m, const double *__restrict__ fluxC1, const double *__restr
             bc = bc + T[u];
             eff = eff - T[u];
           }
+        } else if (p_flux(m, m.bpatch[f[u] - m.Fi])) {
+          // inlet / outlet: ac += FLUXC1, bc -= FLUXT = mdot_b
+          ac = ac + fluxC1[f[u]];
+          bc = bc - T[u];
+          eff = eff + T[u];
         } else {
           ac = ac + 0.0;
           bc = bc - 0.0;
@@ -793,11 +884,20 @@ __global__ void k_pp_cells(DevMesh m, int32_t ref, const double *dphi, double *P
   PP[i] = dphi[i] - ppc;
 }
 
-// boundary value = owner value (wall, empty patches of PP and p)
-__global__ void k_copy_owner(DevMesh m, double *phi) {
+// PP_b (cfdCorrectPPField.m:57-335): 0 on fixed-pressure outlets (:249-272),
+// the owner's value on every other patch
+__global__ void k_pp_bnd(DevMesh m, double *PP) {
   const int32_t k = blockIdx.x * kBlock + threadIdx.x;
   if (k >= m.F - m.Fi) return;
-  phi[m.N + k] = phi[m.owner[m.Fi + k]];
+  PP[m.N + k] = p_fixed_outlet(m, m.bpatch[k]) ? 0.0 : PP[m.owner[m.Fi + k]];
+}
+
+// p_b (cfdCorrectPressureEquation.m:60-290): the owner's value except on
+// fixed-pressure outlets, whose p_b is kept (:230-233)
+__global__ void k_p_bnd(DevMesh m, double *p) {
+  const int32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= m.F - m.Fi) return;
+  if (!p_fixed_outlet(m, m.bpatch[k])) p[m.N + k] = p[m.owner[m.Fi + k]];
 }
 
 // U -= DU o grad(PP) on cells (cfdCorrectVelocityField.m:26-55)
@@ -805,6 +905,37 @@ __global__ void k_correct_u_pp(DevMesh m, const double *DU, const double *gradPP
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= m.N) return;
   for (int d = 0; d < 3; ++d) U[i * 3 + d] = U[i * 3 + d] - DU[i * 3 + d] * gradPP[i * 3 + d];
+}
+
+// boundary rows of U after the pressure correction, patches dispatched on p's
+// condition in patch order (cfdCorrectVelocityField.m:57-121):
+//  - empty and symmetry planes: U_b = U_O - (U_O . n) n (:274-343);
+//  - fixed-pressure outlets: U(iBFaces,:) = U(iOwners,:) (:247-272), i.e. the
+//    row numbered by the outlet face's own index gets that face's owner's U;
+//    such a row is another patch's boundary element here (outlet faces below
+//    n_cells are refused at case creation, rows past the end are not stored);
+//  - walls with zeroGradient p and zeroGradient-p inlets: nothing (:173-182).
+// Row r = N + k is written by its own patch (if a plane) and by the outlet
+// whose face has index r; the later patch in order wins, as in the loop.
+// Only interior rows of U are read, so the rows are independent.
+__global__ void k_bnd_u_p(DevMesh m, double *U) {
+  const int32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= m.F - m.Fi) return;
+  const int32_t r = m.N + k;
+  const int pk = m.bpatch[k];
+  int q = -1;
+  if (r >= m.Fi && r < m.F && p_fixed_outlet(m, m.bpatch[r - m.Fi])) q = m.bpatch[r - m.Fi];
+  if (q >= 0 && (!plane(m, pk) || q > pk)) {
+    const int32_t o = m.owner[r];
+    for (int d = 0; d < 3; ++d) U[r * 3 + d] = U[o * 3 + d];
+  } else if (plane(m, pk)) {
+    const int32_t f = m.Fi + k, o = m.owner[f];
+    double n[3];
+    const double nm = mag3(&m.Sf[f * 3]);
+    for (int d = 0; d < 3; ++d) n[d] = m.Sf[f * 3 + d] / nm;
+    const double un = dot3(&U[o * 3], n);
+    for (int d = 0; d < 3; ++d) U[r * 3 + d] = U[o * 3 + d] - un * n[d];
+  }
 }
 
 // p += urf_p (PP - PP(ref)) (cfdCorrectPressureEquation.m:22-54)
@@ -816,15 +947,23 @@ __global__ void k_correct_p_cells(DevMesh m, int32_t ref, double urf_p, const do
   p[i] = p[i] + urf_p * (PP[i] - ppf);
 }
 
-// mdot += C1 PP_O + C2 PP_N; empty patches: mdot = 0 mdot (cfdCorrectMdotField.m:24-50, :328-346)
+// mdot += C1 PP_O + C2 PP_N (cfdCorrectMdotField.m:24-50); boundary faces by
+// p's condition (:56-120): planes 0 mdot_b (:306-346), fixed-pressure outlets
+// rho_b (U_O . S_b) with the corrected owner U (:264-304), walls with
+// zeroGradient p and incompressible inlets unchanged (:156-208)
 __global__ void k_correct_mdot(DevMesh m, const double *fluxC1, const double *fluxC2,
-                               const double *PP, double *mdot) {
+                               const double *PP, const double *U, double *mdot) {
   const int32_t f = blockIdx.x * kBlock + threadIdx.x;
   if (f >= m.F) return;
   if (f < m.Fi) {
     mdot[f] = (mdot[f] + fluxC1[f] * PP[m.owner[f]]) + fluxC2[f] * PP[m.neighbour[f]];
-  } else if (tangential(m, m.bpatch[f - m.Fi])) {
+    return;
+  }
+  const int q = m.bpatch[f - m.Fi];
+  if (plane(m, q)) {
     mdot[f] = 0.0 * mdot[f];
+  } else if (p_fixed_outlet(m, q)) {
+    mdot[f] = 1.0 * dot3(&U[m.owner[f] * 3], &m.Sf[f * 3]);
   }
 }
 
@@ -833,14 +972,14 @@ void correct_p(hipStream_t s, const DevMesh &m, int32_t ref, double urf_p, const
                const double *fluxC1, const double *fluxC2, double *mdot) {
   const int32_t Fb = m.F - m.Fi;
   hipLaunchKernelGGL(k_pp_cells, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, ref, dphi, PP);
-  if (Fb > 0) hipLaunchKernelGGL(k_copy_owner, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, PP);
+  if (Fb > 0) hipLaunchKernelGGL(k_pp_bnd, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, PP);
   gradient(s, m, PP, 1, gradPP);
   hipLaunchKernelGGL(k_correct_u_pp, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, DU, gradPP, U);
-  if (Fb > 0) hipLaunchKernelGGL(k_empty_u, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, -1, U);
+  if (Fb > 0) hipLaunchKernelGGL(k_bnd_u_p, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, U);
   hipLaunchKernelGGL(k_correct_p_cells, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, ref, urf_p, PP, p);
-  if (Fb > 0) hipLaunchKernelGGL(k_copy_owner, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, p);
+  if (Fb > 0) hipLaunchKernelGGL(k_p_bnd, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, p);
   hipLaunchKernelGGL(k_correct_mdot, dim3(nblocks(m.F)), dim3(kBlock), 0, s, m, fluxC1, fluxC2, PP,
-                     mdot);
+                     U, mdot);
 }
 
 }  // namespace sk

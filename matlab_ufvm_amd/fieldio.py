@@ -211,7 +211,7 @@ def case_fields(case, time: str = "0"):
     fU, fp = {}, {}
     fm = {}
     for pt in m.patches:
-        ubc, _, pbc = case.bcs[pt.name]
+        ubc, pbc = case.bcs[pt.name][0], case.bcs[pt.name][2]
         bel = slice(N + pt.start - Fi, N + pt.start - Fi + pt.nfaces)
         if ubc == "fixedValue":
             fU[pt.name] = BoundaryEntry("fixedValue", *_entry(U[bel]))

@@ -20,7 +20,7 @@ import numpy as np
 from . import _lib
 from ._lib import AmgCfg, MeshDesc, PcgCfg, SimpleCfg, f64p, i32p
 
-PATCH = {"wall": 0, "empty": 1, "symmetry": 3}
+PATCH = {"wall": 0, "empty": 1, "symmetry": 3, "inlet": 4, "outlet": 5}
 BC = {"fixedValue": 0, "zeroGradient": 1, "empty": 2, "noSlip": 3, "symmetry": 4, "slip": 5}
 P_SOLVER = {"amg": 0, "pcg": 1, "pcg-frozen": 2}
 
@@ -44,12 +44,15 @@ def case_structs(mesh, bcs, nu, urf_u=0.7, urf_p=0.3, n_correctors=2, p_ref=0, u
                  p_solver="amg", p_cycle="W-Cycle", p_max_cycles=30, p_termination=0.1,
                  p_max_coarse_levels=10, p_pre=1, p_post=3, p_smoother="ILU", pcg_precision=32,
                  pcg_tol=1e-8, pcg_sweeps=1, pcg_max_coarse_cells=2000, pcg_coarse_scale=2.5,
-                 u_smoother="ILU", pcg_coarse_sweeps=2, stress=False):
+                 u_smoother="ILU", pcg_coarse_sweeps=2, stress=False, u_init=(0.0, 0.0, 0.0),
+                 p_init=0.0):
     """(MeshDesc, SimpleCfg, keep): the C structs of ufvm_case_create; the arrays
-    they point to live in `keep`, which must outlive the call.  u_smoother:
-    "ILU" (the reference's DILU momentum solve) or "MCGS" (multicolour GS);
-    stress: the momentum equation has + div(nu*transp(grad(U))), as the
-    reference's tutorials do."""
+    they point to live in `keep`, which must outlive the call.  bcs:
+    {patch: (U bc, U value (3,), p bc[, p value])}.  u_smoother: "ILU" (the
+    reference's DILU momentum solve) or "MCGS" (multicolour GS); stress: the
+    momentum equation has + div(nu*transp(grad(U))), as the reference's
+    tutorials do; u_init / p_init: the uniform internal fields of the 0
+    directory."""
     m = mesh
     keep = []
 
@@ -84,6 +87,10 @@ def case_structs(mesh, bcs, nu, urf_u=0.7, urf_p=0.3, n_correctors=2, p_ref=0, u
                      pcg_tol, 0, pcg_coarse_scale, pcg_coarse_sweeps, pcg_coarse_sweeps)
     c.u_smoother = _lib.SMOOTHER[u_smoother]
     c.stress_term = int(bool(stress))
+    c.p_value = a([float(bcs[p.name][3]) if len(bcs[p.name]) > 3 else 0.0 for p in m.patches],
+                  f64p)
+    c.u_init = (C.c_double * 3)(*[float(v) for v in u_init])
+    c.p_init = float(p_init)
     return d, c, keep
 
 
@@ -92,7 +99,8 @@ class SimpleCase:
                  p_ref=0, u_sweeps=10, p_solver="amg", p_cycle="W-Cycle", p_max_cycles=30,
                  p_termination=0.1, p_max_coarse_levels=10, p_pre=1, p_post=3, p_smoother="ILU",
                  pcg_precision=32, pcg_tol=1e-8, pcg_sweeps=1, pcg_max_coarse_cells=2000,
-                 pcg_coarse_scale=2.5, u_smoother="ILU", pcg_coarse_sweeps=2, stress=False):
+                 pcg_coarse_scale=2.5, u_smoother="ILU", pcg_coarse_sweeps=2, stress=False,
+                 u_init=(0.0, 0.0, 0.0), p_init=0.0):
         self.ctx = ctx
         self.mesh = mesh
         self.bcs = bcs
@@ -100,7 +108,7 @@ class SimpleCase:
             mesh, bcs, nu, urf_u, urf_p, n_correctors, p_ref, u_sweeps, p_solver, p_cycle,
             p_max_cycles, p_termination, p_max_coarse_levels, p_pre, p_post, p_smoother,
             pcg_precision, pcg_tol, pcg_sweeps, pcg_max_coarse_cells, pcg_coarse_scale,
-            u_smoother, pcg_coarse_sweeps, stress)
+            u_smoother, pcg_coarse_sweeps, stress, u_init, p_init)
         cid = C.c_int32()
         ctx.check(ctx.lib.ufvm_case_create(ctx.h, C.byref(d), C.byref(c), C.byref(cid)))
         self.id = cid.value
