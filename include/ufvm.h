@@ -98,6 +98,21 @@ int ufvm_system_get_bc(ufvm_ctx *ctx, int32_t sys, int32_t level, double *bc);
 int ufvm_compute_residuals(ufvm_ctx *ctx, int32_t sys, int32_t level, double *r_out,
                            double *l1_out);
 
+/* No reference counterpart: for callers that cannot allocate device memory
+ * (work captured into HIP graphs, tight serving loops, harnesses that forbid
+ * allocation).  The residual r and its L1 norm sum(abs(r)) of
+ * ufvm_compute_residuals, with the same bits, written to DEVICE memory of the
+ * context's GPU (r_dev: n doubles, l1_dev: one double) and enqueued on the
+ * context's stream without waiting for it.  The call allocates nothing: its
+ * per-block partial sums live in a static device array of the library sized
+ * for at most UFVM_NOALLOC_MAX_ROWS rows (1 MiB per GPU, held until the
+ * process ends); larger levels are refused (UFVM_ERR_ARG).  Every call on one
+ * GPU shares that array, so calls that overlap on one GPU (other contexts'
+ * streams, other host threads) race. */
+#define UFVM_NOALLOC_MAX_ROWS 33554432
+int ufvm_compute_residuals_noalloc(ufvm_ctx *ctx, int32_t sys, int32_t level, double *r_dev,
+                                   double *l1_dev);
+
 /* ----------------------------------------------------- AMG (agglomeration) */
 /* cfdAgglomerate(maxCoarseLevels) (uFVM/src/fvm/solve/AMG/cfdAgglomerate.m:7-17):
  * builds the coarse levels from the current level-1 coefficients with the

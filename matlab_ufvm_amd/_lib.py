@@ -90,6 +90,8 @@ _SIGS = {
     "ufvm_system_set_bc": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f64p]),
     "ufvm_system_get_bc": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f64p]),
     "ufvm_compute_residuals": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f64p, f64p]),
+    "ufvm_compute_residuals_noalloc": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p,
+                                                 C.c_void_p]),
     "ufvm_amg_agglomerate": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, i32p]),
     "ufvm_amg_num_levels": (C.c_int, [C.c_void_p, C.c_int32, i32p]),
     "ufvm_amg_level_info": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, i32p, i32p]),
@@ -264,6 +266,14 @@ class Context:
         l1 = C.c_double()
         self.check(self.lib.ufvm_compute_residuals(self.h, sid, level, _p(r, f64p), C.byref(l1)))
         return r, l1.value
+
+    def residual_noalloc_ptr(self, sid, level, r_addr, l1_addr):
+        """ufvm_compute_residuals_noalloc: r and its L1 norm into device memory
+        given by address (e.g. torch tensors' data_ptr()), enqueued on the
+        context's stream; nothing is allocated and nothing waited for."""
+        self.check(self.lib.ufvm_compute_residuals_noalloc(self.h, sid, level,
+                                                           C.c_void_p(r_addr),
+                                                           C.c_void_p(l1_addr)))
 
     def agglomerate(self, sid, max_coarse_levels=10) -> int:
         nl = C.c_int32()

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libufvm.so")
 
 SOURCES = ["api.hip", "kernels.hip", "kernels_batch.hip", "simple_api.hip", "simple_kernels.hip",
-           "pcg.hip", "server.hip", "hostalg.cpp"]
+           "pcg.hip", "server.hip", "noalloc.hip", "hostalg.cpp"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

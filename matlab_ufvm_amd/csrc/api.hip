@@ -571,6 +571,7 @@ int ufvm_ctx_create(int32_t device, ufvm_ctx **out) {
       UFVM_HIP(hipHostMalloc((void **)&ctx->hscal, kScal * sizeof(double), hipHostMallocDefault));
     }
     ctx->partial.alloc(1024);
+    noalloc_setup(device);
     UFVM_HIP(hipStreamSynchronize(ctx->stream));
   });
   if (rc != UFVM_OK) {

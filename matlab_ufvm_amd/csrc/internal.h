@@ -80,6 +80,8 @@ struct ProfScope {
 };
 
 System &sys_of(ufvm_ctx *ctx, int32_t id);
+RefLevel &level_of(System &S, int32_t level);
+void noalloc_setup(int device);
 void ensure_schedule(ufvm_ctx *ctx, RefLevel &L);
 void ensure_partial(ufvm_ctx *ctx, int64_t n, int width);
 void read_scal(ufvm_ctx *ctx, int first, int count);
