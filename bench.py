@@ -169,12 +169,12 @@ def committed_traffic(kernel):
     return tot / cnt * 1e6 if cnt else None
 
 
-def serving_bench(args):
+def serving_leg(args):
     """Serving form: a Poisson-arrival stream of pressure solves and single
     SIMPLE steps over a pool of resident jittered hex meshes (2-D and 3-D,
     cell counts 50k..4M), FP32-preconditioned AMG-PCG at 1e-8, hierarchies
-    cached; reports completed requests/s and p50/p99 latency (submit ->
-    result on the host)."""
+    cached; returns completed requests/s and p50/p99 latency (submit ->
+    result on the host) under that mixed-size load."""
     from matlab_ufvm_amd.serving import Server, request_stream, run_stream
     from matlab_ufvm_amd.simple import cavity_bcs
     from matlab_ufvm_amd.workloads import cavity_mesh, gaussian_rhs, poisson_system
@@ -233,35 +233,31 @@ def serving_bench(args):
     bs = [r.extra.get("batch", 1) for r in reqs if r.kind == "solve"]
     if bs:
         out["solve"]["batch_sizes"] = {str(k): bs.count(k) for k in sorted(set(bs))}
-    print(json.dumps(out), flush=True)
+    log(f"serving: {out['value']} requests/s, p50 {out['latency_ms']['p50']} ms, "
+        f"p99 {out['latency_ms']['p99']} ms over {len(reqs)} requests")
+    return out
 
 
-def simple_steps_bench(args):
+def simple_leg(args, ctx, m, steps, p_solver, u_smoother):
     """Precision form in the SIMPLE loop: the 128^3 lid-driven cavity (Re 400)
-    run for N outer iterations twice, with every pressure-correction solve by
-    the FP32-preconditioned AMG-PCG and by the FP64 one, both to 1e-8 with the
-    hierarchy rebuilt each solve (as the reference does); reports the time per
-    iteration and the per-field relative L2 difference after N iterations."""
-    from matlab_ufvm_amd._lib import Context
+    run for `steps` outer iterations twice, with every pressure-correction
+    solve by the FP32-preconditioned AMG-PCG and by the FP64 one, both to
+    1e-8; returns the time per iteration and the per-field relative L2
+    difference after the iterations."""
     from matlab_ufvm_amd.simple import SimpleCase, cavity_bcs
-    from matlab_ufvm_amd.workloads import cavity_mesh
-    t = time.time()
-    m = cavity_mesh(args.n, jitter=0.15, seed=2026, renumber_seed=2027, apex="own")
-    log(f"mesh {args.n}^3: cells={m.n_cells} ({time.time()-t:.1f}s)")
-    ctx = Context(0)
     fields, res = {}, {}
     for prec in (32, 64):
-        case = SimpleCase(ctx, m, cavity_bcs(m), nu=1.0 / 400.0, p_solver=args.p_solver,
-                          pcg_precision=prec, pcg_tol=1e-8, u_smoother=args.u_smoother)
+        case = SimpleCase(ctx, m, cavity_bcs(m), nu=1.0 / 400.0, p_solver=p_solver,
+                          pcg_precision=prec, pcg_tol=1e-8, u_smoother=u_smoother)
         ctx.synchronize()
         t = time.perf_counter()
-        rms = case.iterate(args.simple_steps)
+        rms = case.iterate(steps)
         ctx.synchronize()
         dt = time.perf_counter() - t
         fields[prec] = {k: case.field(k) for k in ("U", "p", "mdot_f")}
-        res[prec] = {"s_per_iteration": round(dt / args.simple_steps, 4),
+        res[prec] = {"s_per_iteration": round(dt / steps, 4),
                      "final_rms": [float(v) for v in rms[-1]]}
-        log(f"fp{prec}-precond SIMPLE: {dt / args.simple_steps * 1e3:.1f} ms/iteration, "
+        log(f"fp{prec}-precond SIMPLE: {dt / steps * 1e3:.1f} ms/iteration, "
             f"final rms {np.array2string(rms[-1], precision=3)}")
         case.close()
     err = {}
@@ -271,20 +267,20 @@ def simple_steps_bench(args):
     out = {"metric": "SIMPLE iterations/s with FP32-preconditioned pressure solves, and per-field "
                      "relative L2 difference from the FP64-preconditioned run",
            "value": round(1.0 / res[32]["s_per_iteration"], 3), "unit": "iterations/s",
-           "n_gpus": 1, "iterations": args.simple_steps,
+           "n_gpus": 1, "iterations": steps,
            "fp32": res[32], "fp64": res[64], "field_rel_l2_diff": err,
            "config": {"workload": f"3-D lid-driven cavity {args.n}^3 jittered hex (+-15%, seed "
                                   f"2026), permuted+RCM (seed 2027), own-face apex, Re 400, "
                                   f"SIMPLE URF 0.7/0.3, 2 pressure correctors, momentum "
-                                  f"{'DILU' if args.u_smoother == 'ILU' else 'multicolour GS'} "
+                                  f"{'DILU' if u_smoother == 'ILU' else 'multicolour GS'} "
                                   f"(10 sweeps, 0.1 reduction)",
                       "cells": int(m.n_cells), "pressure_tol": 1e-8,
-                      "hierarchy": ("rebuilt every pressure solve" if args.p_solver == "pcg"
+                      "hierarchy": ("rebuilt every pressure solve" if p_solver == "pcg"
                                     else "first solve's parents kept, Galerkin operators and "
                                          "solve-side copies refreshed on the GPU")},
            "data": "synthetic"}
-    ctx.close()
-    print(json.dumps(out), flush=True)
+    log(f"SIMPLE precision form: field differences after {steps} steps {err}")
+    return out
 
 
 def timed_solves(ctx, sid, steps, warmup):
@@ -435,6 +431,23 @@ def survey_bytes_per_iteration(levels, nlev, pre, post, cpre, cpost):
     return total + b_pass(n0, z0, 8) + 7 * 8 * n0 + 2 * 12 * n0
 
 
+def plain_oracle_check(ptr, col, ac, anb, b, ncyc, dphi_gpu):
+    """The oracle's cfdApplyAMG (V-cycle, SOR 2/2, FP64) on the same system for
+    the same number of cycles from dphi = 0: the GPU's plain drop-in must give
+    its dphi bit for bit, stall included (checker only, untimed)."""
+    import oracle   # checker
+    t = time.time()
+    S = oracle.System(ptr, col, ac, anb, bc=b)
+    r0, r1, nc = S.apply_amg("V-Cycle", max_cycles=ncyc, termination=1e-8, max_coarse_levels=10,
+                             pre_sweep=2, post_sweep=2, smoother="SOR")
+    x = S.level(0)["dphi"]
+    out = {"cycles": int(nc - 1), "l1_residual_ratio": r1 / r0,
+           "dphi_bit_identical_to_gpu": bool(np.array_equal(x, dphi_gpu)),
+           "seconds": round(time.time() - t, 1)}
+    log(f"plain fp64 AMG on the oracle: {out}")
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -533,6 +546,10 @@ def main():
                     help="SIMPLE mode: rebuild the hierarchy every solve, or keep the first")
     ap.add_argument("--serving", action="store_true",
                     help="run the serving-form stream instead of the solve benchmark")
+    ap.add_argument("--no-serving-leg", action="store_true",
+                    help="leave the serving-form stream out of the default line")
+    ap.add_argument("--simple-leg-steps", type=int, default=50,
+                    help="SIMPLE steps of the default line's precision-form leg (0: skip)")
     ap.add_argument("--serving-meshes", default="3d:37,3d:64,3d:100,3d:128,2d:224,2d:512,"
                                                 "2d:1024,2d:2048")
     ap.add_argument("--serving-requests", type=int, default=200)
@@ -564,10 +581,16 @@ def main():
         copy_calibration(args.copy_calibration_mib)
         return
     if args.serving:
-        serving_bench(args)
+        print(json.dumps(serving_leg(args)), flush=True)
         return
     if args.simple_steps > 0:
-        simple_steps_bench(args)
+        from matlab_ufvm_amd._lib import Context
+        from matlab_ufvm_amd.workloads import cavity_mesh
+        m = cavity_mesh(args.n, jitter=0.15, seed=2026, renumber_seed=2027, apex="own")
+        ctx = Context(0)
+        print(json.dumps(simple_leg(args, ctx, m, args.simple_steps, args.p_solver,
+                                    args.u_smoother)), flush=True)
+        ctx.close()
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(json.dumps({"error": "this build runs on one GPU: the pressure solve of one mesh "
@@ -662,6 +685,7 @@ def main():
             return time.perf_counter() - s, nc - 1, r1 / r0
         plain_run(1)                          # level schedules built on first use
         tp, ncyc, ratio = plain_run(args.plain_cycles)
+        plain_dphi = ctx.get_dphi(sid, 0)
         plain = {"ms_per_cycle": round(tp / ncyc * 1e3, 3), "cycles": int(ncyc),
                  "l1_residual_ratio": ratio,
                  "field_rel_l2_err": ferr(ctx.get_dphi(sid, 0)),
@@ -746,16 +770,29 @@ def main():
         out["field_rel_l2_err_vs_fp64_same_tol"] = float(np.linalg.norm(d32 - d64) /
                                                          np.linalg.norm(d64))
     if plain is not None:
+        # the plain path never reaches 1e-8 on this system; any solve of it
+        # would take at least one cycle, so one cycle's time over the FP32
+        # solve's bounds the solves/s ratio from below
+        plain["speedup_fp32_pcg_lower_bound"] = round(plain["ms_per_cycle"] /
+                                                      out["latency_ms"]["p50"], 2)
         out["plain_fp64_amg"] = plain
+    if args.simple_leg_steps > 0 and args.workload == "pcorr-k5":
+        out["simple_precision_form"] = simple_leg(args, ctx, m, args.simple_leg_steps,
+                                                  "pcg-frozen", "ILU")
     if not args.no_cpu:
         cb, x_cpu = cpu_baseline(ptr, col, ac, anb, b, args.max_coarse, args.coarse_smoother,
                                  args.tol, int(singular), args.smoother, args.coarse_scale,
                                  args.coarse_pre, args.coarse_post)
         cb["field_rel_l2_err_vs_gpu_tight"] = ferr(x_cpu)
         out["cpu_baseline"] = cb
+        if plain is not None:
+            out["plain_fp64_amg"]["oracle"] = plain_oracle_check(ptr, col, ac, anb, b,
+                                                                plain["cycles"], plain_dphi)
     else:
         out["cpu_baseline"] = None
     ctx.close()
+    if not args.no_serving_leg and args.workload == "pcorr-k5":
+        out["serving"] = serving_leg(args)
     print(json.dumps(out), flush=True)
 
 

@@ -149,3 +149,38 @@ def test_request_stream_completes(meshes):
               f"{np.median(lat)*1e3:.1f} ms, p99 {np.percentile(lat, 99)*1e3:.1f} ms", flush=True)
     finally:
         srv.close()
+
+
+def test_large_meshes_same_alone_and_in_mix():
+    # meshes of the serving configuration's size range (>= 50k cells, 2-D and
+    # 3-D): pressure solves and SIMPLE steps give the same bits alone and in a
+    # mix where they overlap on the device
+    big = [cavity_mesh(40, jitter=0.15, seed=111, renumber_seed=112),            # 64,000 cells
+           cavity_mesh(256, nz=1, jitter=0.1, seed=113, renumber_seed=114)]       # 65,536 cells
+    print(f"\n[serving large] cells {[m.n_cells for m in big]}", flush=True)
+    srv = Server(max_active=4, n_streams=4)
+    try:
+        sids = _register(srv, big)
+        kw = dict(nu=1.0 / 400.0, p_cycle="V-Cycle", p_pre=2, p_post=2, p_smoother="SOR",
+                  u_smoother="MCGS")
+        a = srv.add_case(big[0], cavity_bcs(big[0]), **kw)
+        b = srv.add_case(big[0], cavity_bcs(big[0]), **kw)
+        rhs = [[gaussian_rhs(m, seed=120 + 5 * i + j) for j in range(2)]
+               for i, m in enumerate(big)]
+        alone = {(i, j): srv.wait(srv.submit_solve(sids[i], rhs[i][j]))
+                 for i in range(2) for j in range(2)}
+        ra = srv.wait(srv.submit_step(a, 2))
+        tb = srv.submit_step(b, 2)
+        ts = [((i, j), srv.submit_solve(sids[i], rhs[i][j]))
+              for _ in range(2) for i in range(2) for j in range(2)]
+        for k, t in ts:
+            r = srv.wait(t)
+            assert r["relres"] <= 1e-8
+            assert r["iterations"] == alone[k]["iterations"]
+            assert np.array_equal(r["x"], alone[k]["x"]), k
+        rb = srv.wait(tb)
+        assert np.array_equal(ra["rms"], rb["rms"])
+        for name in ("U", "p", "mdot_f"):
+            assert np.array_equal(srv.field(a, name), srv.field(b, name)), name
+    finally:
+        srv.close()
