@@ -8,7 +8,10 @@ incompressible loop, restricted to this path:
 * ``0/U`` and ``0/p``: the uniform internal fields (``cfdDefineEquation.m:68-82``)
   and, per patch, the boundary condition type and value
   (``cfdReadTimeDirectory.m:13-133``);
-* ``constant/transportProperties``: ``nu``;
+* ``constant/transportProperties``: ``nu`` (tutorials written
+  ``div(U*U) = laplacian(nu*U) ...``, density 1), or ``mu`` and ``rho``
+  (tutorials written ``div(rho*U*U) = laplacian(mu*U) ...``: mu is the
+  diffusion coefficient, rho the density of the convected mass flux);
 * ``system/fvSolution``: ``nCorrectors`` and ``pRefCell`` of the SIMPLE block
   (``cfdReadSystem.m:150-190``) and the relaxation factors, p from ``fields``
   and U from ``equations`` (``cfdDefineEquation.m:84-99``, 0.7 when absent).
@@ -35,20 +38,21 @@ PLANE = ("empty", "symmetry")
 class FoamCase:
     mesh: Mesh
     bcs: dict                     # patch -> (U type, U value (3,), p type, p value)
-    nu: float
+    nu: float                     # the laplacian coefficient (nu, or mu)
     u_init: tuple
     p_init: float
     n_correctors: int = 1
     urf_u: float = 0.7
     urf_p: float = 0.7
     p_ref: int = -1               # 0-based, -1 none
+    rho: float = 1.0
     extra: dict = field(default_factory=dict)
 
     def simple_kwargs(self) -> dict:
         """keyword arguments of SimpleCase (GPU) and oracle.SimpleCase"""
         return dict(nu=self.nu, urf_u=self.urf_u, urf_p=self.urf_p,
                     n_correctors=self.n_correctors, p_ref=self.p_ref, u_init=self.u_init,
-                    p_init=self.p_init)
+                    p_init=self.p_init, rho=self.rho)
 
 
 def _strip(text: str) -> str:
@@ -90,7 +94,14 @@ def read_case(case_dir: str, apex: str = "reference") -> FoamCase:
         pval = float(pe.value) if pe.value is not None else 0.0
         bcs[pt.name] = (ue.type, uval, pe.type, pval)
     tp = _strip(open(os.path.join(case_dir, "constant", "transportProperties")).read())
-    nu = float(re.search(r"(?<![\w.])nu\b[^;]*?([-+0-9.eE]+)\s*;", tp).group(1))
+
+    def prop(name):
+        mm = re.search(r"(?<![\w.])" + name + r"\b[^;]*?([-+0-9.eE]+)\s*;", tp)
+        return float(mm.group(1)) if mm else None
+
+    nu, rho = prop("nu"), 1.0
+    if nu is None:
+        nu, rho = prop("mu"), prop("rho")
     fvs = _strip(open(os.path.join(case_dir, "system", "fvSolution")).read())
     simple = _block(fvs, "SIMPLE")
     nc = _entry(simple, "nCorrectors")
@@ -107,4 +118,4 @@ def read_case(case_dir: str, apex: str = "reference") -> FoamCase:
     return FoamCase(mesh=mesh, bcs=bcs, nu=nu, u_init=u_init, p_init=float(p0.internal),
                     n_correctors=int(float(nc)) if nc is not None else 1,
                     urf_u=float(urf_u) if urf_u is not None else 0.7,
-                    urf_p=float(urf_p) if urf_p is not None else 0.7, p_ref=p_ref)
+                    urf_p=float(urf_p) if urf_p is not None else 0.7, p_ref=p_ref, rho=rho)

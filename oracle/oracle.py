@@ -269,14 +269,16 @@ class OCfg(C.Structure):
                 ("p_cycle", C.c_int32), ("p_max_cycles", C.c_int32), ("p_max_coarse", C.c_int32),
                 ("p_pre", C.c_int32), ("p_post", C.c_int32), ("p_smoother", C.c_int32),
                 ("p_termination", C.c_double), ("u_smoother", C.c_int32), ("stress", C.c_int32),
-                ("u_init", C.c_double * 3), ("p_init", C.c_double)]
+                ("u_init", C.c_double * 3), ("p_init", C.c_double), ("rho", C.c_double),
+                ("transient", C.c_int32), ("dt", C.c_double)]
 
 
 class OState(C.Structure):
     _fields_ = [("N", C.c_int32), ("Fb", C.c_int32),
                 ("U", f64p), ("gradU", f64p), ("p", f64p), ("gradp", f64p), ("PP", f64p),
                 ("gradPP", f64p), ("DU", f64p), ("DUT", f64p), ("mdot", f64p), ("gamma_f", f64p),
-                ("fluxC1", f64p), ("fluxC2", f64p), ("scaleU", C.c_double), ("H", Hier),
+                ("fluxC1", f64p), ("fluxC2", f64p), ("U_old", f64p), ("mdot_old", f64p),
+                ("scaleU", C.c_double), ("H", Hier),
                 ("ac_old", f64p), ("rms", C.c_double * 4), ("mx", C.c_double * 4),
                 ("r0", C.c_double * 4), ("r1", C.c_double * 4), ("ncycles", C.c_int32),
                 ("scale_zero", C.c_int32)]
@@ -298,7 +300,8 @@ class SimpleCase:
     def __init__(self, mesh, bcs, nu, urf_u=0.7, urf_p=0.3, n_correctors=2, p_ref=0,
                  u_sweeps=10, p_cycle="W-Cycle", p_max_cycles=30, p_termination=0.1,
                  p_max_coarse=10, p_pre=1, p_post=3, p_smoother="ILU", u_smoother="ILU",
-                 stress=False, u_init=(0.0, 0.0, 0.0), p_init=0.0):
+                 stress=False, u_init=(0.0, 0.0, 0.0), p_init=0.0, rho=1.0, transient=False,
+                 dt=1.0):
         L = lib()
         for name, args in [("orc_simple_init", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState)]),
                            ("orc_simple_free", [C.POINTER(OState)]),
@@ -310,7 +313,10 @@ class SimpleCase:
                            ("orc_assemble_p", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState)]),
                            ("orc_solve_p", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState)]),
                            ("orc_correct_p", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState)]),
-                           ("orc_gradient", [C.POINTER(OMesh), f64p, C.c_int, f64p])]:
+                           ("orc_gradient", [C.POINTER(OMesh), f64p, C.c_int, f64p]),
+                           ("orc_transient_update", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState)]),
+                           ("orc_transient_step", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState),
+                                                   C.c_int, C.c_double, C.c_double])]:
             getattr(L, name).argtypes = args
         m = mesh
         self.mesh = m
@@ -342,7 +348,8 @@ class SimpleCase:
                         {"V-Cycle": 0, "W-Cycle": 1, "F-Cycle": 2}[p_cycle], p_max_cycles,
                         p_max_coarse, p_pre, p_post, {"ILU": 0, "SOR": 1}[p_smoother], p_termination,
                         {"ILU": 0, "MCGS": 2}[u_smoother], int(bool(stress)),
-                        (C.c_double * 3)(*u_init), float(p_init))
+                        (C.c_double * 3)(*u_init), float(p_init), float(rho), int(bool(transient)),
+                        float(dt))
         self.S = OState()
         L.orc_simple_init(C.byref(self.M), C.byref(self.cfg), C.byref(self.S))
 
@@ -369,6 +376,15 @@ class SimpleCase:
     def update_fields(self):
         lib().orc_update_fields(*self._args())
 
+    def transient_update(self):
+        lib().orc_transient_update(*self._args())
+
+    def transient_step(self, max_inner=20, u_tol=1e-6, p_tol=1e-6):
+        """one time step (cfdRunCase.m:24-86); returns the outer iterations run"""
+        n = lib().orc_transient_step(*self._args(), max_inner, u_tol, p_tol)
+        self._check_cycles()
+        return n
+
     def assemble_u(self, comp):
         lib().orc_assemble_u(*self._args(), comp)
 
@@ -392,14 +408,15 @@ class SimpleCase:
         N, Fb, F = self.S.N, self.S.Fb, self.M.F
         shapes = {"U": (N + Fb, 3), "gradU": (N + Fb, 3, 3), "p": (N + Fb,), "gradp": (N + Fb, 3),
                   "PP": (N + Fb,), "gradPP": (N + Fb, 3), "DU": (N + Fb, 3), "DUT": (N + Fb, 3),
-                  "mdot": (F,), "gamma_f": (F,), "fluxC1": (F,), "fluxC2": (F,)}
+                  "mdot": (F,), "gamma_f": (F,), "fluxC1": (F,), "fluxC2": (F,),
+                  "U_old": (N + Fb, 3), "mdot_old": (F,)}
         ptr = getattr(self.S, name)
         return np.ctypeslib.as_array(ptr, (int(np.prod(shapes[name])),)).reshape(shapes[name]).copy()
 
     def set_field(self, name, values):
         N, Fb, F = self.S.N, self.S.Fb, self.M.F
         size = {"U": (N + Fb) * 3, "p": N + Fb, "PP": N + Fb, "DU": (N + Fb) * 3,
-                "DUT": (N + Fb) * 3, "mdot": F}[name]
+                "DUT": (N + Fb) * 3, "mdot": F, "U_old": (N + Fb) * 3, "mdot_old": F}[name]
         np.ctypeslib.as_array(getattr(self.S, name), (size,))[:] = np.asarray(values).reshape(-1)
 
     def system(self):

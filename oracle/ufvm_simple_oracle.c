@@ -102,6 +102,8 @@ void orc_simple_init(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_stat
   s->gamma_f = dz(F);
   s->fluxC1 = dz(F);
   s->fluxC2 = dz(F);
+  s->U_old = dz((size_t)NB * 3);
+  s->mdot_old = dz(F);
   s->ac_old = dz(N);
   orc_level *L = &s->H.lev[0];
   L->n = N;
@@ -146,22 +148,27 @@ void orc_simple_init(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_stat
       s->p[b] = m->p_bc[p] == ORC_BC_FIXED_VALUE ? m->p_val[p] : s->p[o];
     }
   }
-  /* mdot_f = rho_f .* (Sf . U_f) with Average interpolation
-   * (cfdInitializeMdotField.m:72-80), rho = 1 */
+  /* mdot_f = rho_f .* (Sf . U_f) with Average interpolation of U and rho
+   * (cfdInitializeMdotField.m:72-80; the transient form subtracts
+   * rho_f DeltaVol / dt with DeltaVol = 0 on every face,
+   * cfdProcessOpenFoamMesh.m:17, so the bits are the same) */
+  const double rho = c->rho == 0.0 ? 1.0 : c->rho;
   for (int32_t f = 0; f < F; ++f) {
     double uf[3], rf;
     if (f < m->Fi) {
       const int32_t o = m->owner[f], nb = m->neighbour[f];
       const double g = m->gf[f];
       for (int d = 0; d < 3; ++d) uf[d] = g * s->U[nb * 3 + d] + (1 - g) * s->U[o * 3 + d];
-      rf = g * 1.0 + (1 - g) * 1.0;
+      rf = g * rho + (1 - g) * rho;
     } else {
       const int32_t b = N + (f - m->Fi);
       for (int d = 0; d < 3; ++d) uf[d] = s->U[b * 3 + d];
-      rf = 1.0;
+      rf = rho;
     }
     s->mdot[f] = rf * dot3(&m->Sf[f * 3], uf);
   }
+  memcpy(s->U_old, s->U, (size_t)NB * 3 * sizeof(double));
+  memcpy(s->mdot_old, s->mdot, (size_t)F * sizeof(double));
 }
 
 void orc_simple_free(orc_simple_state *s) {
@@ -170,6 +177,7 @@ void orc_simple_free(orc_simple_state *s) {
   free(L->ac); free(L->anb); free(L->bc); free(L->dphi);
   free(s->U); free(s->gradU); free(s->p); free(s->gradp); free(s->PP); free(s->gradPP);
   free(s->DU); free(s->DUT); free(s->mdot); free(s->gamma_f); free(s->fluxC1); free(s->fluxC2);
+  free(s->U_old); free(s->mdot_old);
   free(s->ac_old);
   memset(s, 0, sizeof(*s));
 }
@@ -346,6 +354,26 @@ void orc_assemble_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
   zero_system(s);
   double *C1 = dz(F), *C2 = dz(F), *T = dz(F);
   const double *U = s->U;
+  const double rho = c->rho == 0.0 ? 1.0 : c->rho;
+  /* ---- ddt(rho*U), Euler, sign +1, the first term of the equation
+   * (cfdAssembleTransientTermEuler.m:13-44): FLUXCE = V rho / dt, FLUXCEOLD =
+   * -V rho_old / dt and FLUXTE = FLUXCE .* phi' + FLUXCEOLD .* phi_old' -- a
+   * column times a row, i.e. an N x N matrix whose linear index iElement in
+   * cfdAssembleIntoGlobalMatrixElementFluxes.m:23-27 reads column 1: every
+   * cell's source takes cell 1's phi and phi_old.  Kept (without the N x N
+   * matrix). */
+  if (c->transient) {
+    const double sg = 1.0;
+    const double phi1 = U[0 * 3 + comp], old1 = s->U_old[0 * 3 + comp];
+    for (int32_t i = 0; i < N; ++i) {
+      const double CE = ((sg * m->vol[i]) * rho) / c->dt;
+      const double CEOLD = (((-sg) * m->vol[i]) * rho) / c->dt;
+      const double TE = CE * phi1 + CEOLD * old1;
+      L->ac[i] = L->ac[i] + CE;
+      s->ac_old[i] = s->ac_old[i] + CEOLD;
+      L->bc[i] = L->bc[i] - TE;
+    }
+  }
   /* ---- Convection, sign +1 (cfdAssembleConvectionTerm.m:103-137; fixedValue
    * inlets and zeroGradient outlets the same upwind fluxes with U_b in place
    * of U_N, :144-232; walls, empty and symmetry patches give zero fluxes
@@ -486,13 +514,14 @@ void orc_assemble_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
       L->bc[i] = L->bc[i] - TE;
     }
   }
-  /* ---- false transient, fdt = 1e9, rho = rho_old = 1
-   * (cfdAssembleFalseTransientTerm.m:28-49; FLUXTEOLD is never scattered) */
-  {
+  /* ---- false transient of steady runs, fdt = 1e9, rho = rho_old
+   * (cfdAssembleFalseTransientTerm.m:28-49; FLUXTEOLD is never scattered;
+   * cfdAssembleEquationTerms.m:44-47: not in transient runs) */
+  if (!c->transient) {
     const double fdt = 1e9;
     for (int32_t i = 0; i < N; ++i) {
-      const double CE = (m->vol[i] * 1.0) / fdt;
-      const double CEOLD = ((-m->vol[i]) * 1.0) / fdt;
+      const double CE = (m->vol[i] * rho) / fdt;
+      const double CEOLD = ((-m->vol[i]) * rho) / fdt;
       const double TE = CE * U[i * 3 + comp];
       L->ac[i] = L->ac[i] + CE;
       s->ac_old[i] = s->ac_old[i] + CEOLD;
@@ -501,11 +530,19 @@ void orc_assemble_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
   }
   /* ---- URF (cfdApplyURF.m:15-17) */
   for (int32_t i = 0; i < N; ++i) L->ac[i] = L->ac[i] / c->urf_u;
-  /* ---- normalised residual (cfdComputeNormalizedResidual.m:64-88) */
+  /* ---- normalised residual (cfdComputeNormalizedResidual.m:64-88); in a
+   * transient run the diagonal less V rho / dt, floored at 1e-6 of it
+   * (:55-76) */
   {
     double sq = 0.0, mx = 0.0;
     for (int32_t i = 0; i < N; ++i) {
-      const double r = L->bc[i] / (L->ac[i] * s->scaleU);
+      double a = L->ac[i];
+      if (c->transient) {
+        const double at = (m->vol[i] * rho) / c->dt;
+        a = a - at;
+        if (a < 1e-6 * at) a = at;
+      }
+      const double r = L->bc[i] / (a * s->scaleU);
       if (fabs(r) > mx) mx = fabs(r);
       sq = sq + r * r;
     }
@@ -572,6 +609,15 @@ void orc_assemble_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
   zero_system(s);
   double *T = dz(F);
   const double *U = s->U, *DU = s->DU;
+  const double rho = c->rho == 0.0 ? 1.0 : c->rho;
+  /* transient terms VI-VII: DUT = mean([DUT1 DUT2 DUT3], 2) per element
+   * (cfdAssembleMdotTerm.m:196) */
+  double *DUTm = NULL;
+  if (c->transient) {
+    DUTm = dz((size_t)(N + FB(m)));
+    for (int32_t i = 0; i < N + FB(m); ++i)
+      DUTm[i] = ((s->DUT[i * 3] + s->DUT[i * 3 + 1]) + s->DUT[i * 3 + 2]) / 3;
+  }
   for (int32_t f = 0; f < Fi; ++f) {
     const int32_t o = m->owner[f], nb = m->neighbour[f];
     const double g = m->gf[f];
@@ -579,8 +625,9 @@ void orc_assemble_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
     for (int d = 0; d < 3; ++d) du[d] = g * DU[nb * 3 + d] + (1 - g) * DU[o * 3 + d];
     const double mprev = s->mdot[f];
     for (int d = 0; d < 3; ++d) ub[d] = g * U[nb * 3 + d] + (1 - g) * U[o * 3 + d];
+    /* rho_f: 'Upwind' by the previous mdot_f (cfdAssembleMdotTerm.m:118-121) */
     const double pos = mprev > 0 ? 1.0 : 0.0;
-    const double rho_f = 1.0 * pos + 1.0 * (1 - pos);
+    const double rho_f = rho * pos + rho * (1 - pos);
     /* Average (swapped weights) and Average:Corrected pressure gradients */
     for (int d = 0; d < 3; ++d) gpb[d] = (1 - g) * s->gradp[nb * 3 + d] + g * s->gradp[o * 3 + d];
     grad_face_corrected(m, s->p, s->gradp, 1, 0, f, gpf);
@@ -599,6 +646,16 @@ void orc_assemble_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
     const double FF = 0.0 - rho_f * geoDiff;
     V = V - rho_f * dot3(gpf, DUT);
     V = V + rho_f * dot3(gpb, DUS);
+    if (c->transient) {
+      /* terms VI-VII (:175-201): DUT_f (mdot_old - rho_old_f [v_old]_f . S_f),
+       * DUT_f and [v_old]_f Average-interpolated, rho_old_f 'Upwind' by the
+       * previous mdot_f */
+      const double dut = g * DUTm[nb] + (1 - g) * DUTm[o];
+      const double rho_old_f = rho * pos + rho * (1 - pos);
+      double uo[3];
+      for (int d = 0; d < 3; ++d) uo[d] = g * s->U_old[nb * 3 + d] + (1 - g) * s->U_old[o * 3 + d];
+      V = V + dut * (s->mdot_old[f] - rho_old_f * dot3(uo, Sf));
+    }
     V = V + (1 - c->urf_u) * (mprev - rho_f * UbS);
     const double md = (FC * s->p[o] + FF * s->p[nb]) + V;
     s->mdot[f] = md;
@@ -615,7 +672,7 @@ void orc_assemble_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
       double C1 = 0.0, C2 = 0.0, V = 0.0;
       if (m->ptype[p] == ORC_PATCH_INLET) {
         /* cfdAssembleMdotTermInletInletBC (:415-486): term I only */
-        V = V + 1.0 * dot3(Sb, &U[b * 3]);
+        V = V + rho * dot3(Sb, &U[b * 3]);
       } else {
         /* cfdAssembleMdotTermOutletFixedValueBC (:799-906): terms I-III with
          * the boundary rows of DU and grad(p), FLUXCb2 = 0 */
@@ -628,11 +685,11 @@ void orc_assemble_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
         for (int d = 0; d < 3; ++d) DUE[d] = (mDUS / ee) * eCN[d];
         const double geoDiff = mag3(DUE) / mag3(CN);
         for (int d = 0; d < 3; ++d) DUT[d] = DUS[d] - DUE[d];
-        V = V + 1.0 * dot3(&U[b * 3], Sb);
-        C1 = C1 + 1.0 * geoDiff;
+        V = V + rho * dot3(&U[b * 3], Sb);
+        C1 = C1 + rho * geoDiff;
         C2 = C2 - 0.0;
-        V = V - 1.0 * dot3(&s->gradp[b * 3], DUT);
-        V = V + 1.0 * dot3(&s->gradp[b * 3], DUS);
+        V = V - rho * dot3(&s->gradp[b * 3], DUT);
+        V = V + rho * dot3(&s->gradp[b * 3], DUS);
       }
       const double md = (C1 * s->p[o] + C2 * s->p[b]) + V;
       s->mdot[f] = md;
@@ -662,6 +719,7 @@ void orc_assemble_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
   s->rms[3] = sqrt(sq / N);
   free(eff);
   free(T);
+  free(DUTm);
 }
 
 /* cfdSolveEquation('p') -> cfdApplyAMG (rebuilds the hierarchy) */
@@ -760,9 +818,28 @@ void orc_correct_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state 
         s->mdot[f] = 0.0 * s->mdot[f];
     } else if (p_fixed_outlet(m, p)) {
       for (int32_t f = m->pstart[p]; f < m->pstart[p] + m->pnf[p]; ++f)
-        s->mdot[f] = 1.0 * dot3(&s->U[m->owner[f] * 3], &m->Sf[f * 3]);
+        s->mdot[f] = (c->rho == 0.0 ? 1.0 : c->rho) * dot3(&s->U[m->owner[f] * 3], &m->Sf[f * 3]);
     }
   }
+}
+
+void orc_transient_update(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s) {
+  (void)c;
+  memcpy(s->U_old, s->U, (size_t)(m->N + FB(m)) * 3 * sizeof(double));
+  memcpy(s->mdot_old, s->mdot, (size_t)m->F * sizeof(double));
+}
+
+int orc_transient_step(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s,
+                       int max_inner, double u_tol, double p_tol) {
+  orc_transient_update(m, c, s);
+  int it = 0;
+  while (it < max_inner) {
+    orc_simple_iteration(m, c, s);
+    ++it;
+    if (s->rms[0] <= u_tol && s->rms[1] <= u_tol && s->rms[2] <= u_tol && s->rms[3] <= p_tol)
+      break;
+  }
+  return it;
 }
 
 /* cfdRunCase.m:92-142, one steady iteration (equations U then p) */

@@ -55,6 +55,9 @@ typedef struct {
   int32_t stress;         /* 1: the momentum equation has + div(nu*transp(grad(U))) */
   double u_init[3];       /* uniform internal U of the 0 directory */
   double p_init;          /* uniform internal p */
+  double rho;             /* constant density of ddt(rho*U) / div(rho*U*U) (0: 1) */
+  int32_t transient;      /* 1: ddt(rho*U) with the Euler scheme, time step dt */
+  double dt;
 } orc_simple_cfg;
 
 typedef struct {
@@ -66,6 +69,7 @@ typedef struct {
   double *mdot;           /* [F] */
   double *gamma_f;        /* [F] */
   double *fluxC1, *fluxC2;/* [F], kept from the last pressure assembly */
+  double *U_old, *mdot_old; /* U and mdot_f of the previous time step ('Step1') */
   double scaleU;
   orc_hier H;             /* level 1 = the equation being solved */
   double *ac_old;
@@ -91,6 +95,15 @@ void orc_solve_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s
 void orc_correct_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s);
 /* one whole outer iteration */
 void orc_simple_iteration(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s);
+/* cfdTransientUpdate (cfdUpdateTransientEquation.m, cfdUpdateTransientMdot.m):
+ * the previous time step's U and mdot_f become the current ones */
+void orc_transient_update(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s);
+/* one time step of cfdRunCase's transient branch (cfdRunCase.m:24-86): the
+ * transient update, then up to max_inner outer iterations, ending after the
+ * first whose RMS residuals (U components, p) are all <= u_tol / p_tol
+ * (cfdCheckConvergence.m); returns the iterations run */
+int orc_transient_step(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s,
+                       int max_inner, double u_tol, double p_tol);
 
 /* Green-Gauss gradient + wall-patch boundary gradients of a field with nc
  * components (cfdUpdateGradient.m / cfdComputeGradientGauss0.m) */

@@ -410,3 +410,99 @@ def test_slip_wall_keeps_its_momentum_stage_velocity_through_the_pressure_stage(
     nrm = _normals(m, f)
     proj = U[o] - ((U[o] * nrm).sum(1))[:, None] * nrm
     assert np.abs(proj - Ub).max() > 1e-6     # the owners moved; U_b did not follow
+
+
+def test_euler_term_closed_forms_and_cell_one_source():
+    # ddt(U), Euler (cfdAssembleTransientTermEuler.m:13-44): the diagonal gains
+    # V rho / dt in place of the false transient's V rho / 1e9 (uniform 2-D
+    # cavity from rest: interior diagonal urf^-1 (4 nu A/h + V rho/dt)); the
+    # source of EVERY cell is FLUXCE_i phi_1 + FLUXCEOLD_i phi_old_1, cell 1's
+    # values (FLUXCE .* phi' is N x N, read through its first column), so the
+    # assembled right-hand side does not depend on phi_old elsewhere
+    n, nu, dt, rho = 6, 0.01, 0.05, 1.3
+    m = cavity_mesh(n, nz=1, jitter=0.0, renumber_seed=None)
+    case = oracle.SimpleCase(m, oracle.cavity_bcs(m), nu=nu, transient=True, dt=dt, rho=rho,
+                             p_cycle="V-Cycle")
+    case.update_fields()
+    case.assemble_u(0)
+    S = case.system()
+    h, A, V = 1.0 / n, (1.0 / n) ** 2, (1.0 / n) ** 3
+    interior = [i for i in range(m.n_cells) if np.diff(m.cconn_ptr)[i] == 4]
+    assert np.allclose(S["ac"][interior], (4 * nu * A / h + V * rho / dt) / 0.7, rtol=1e-13)
+    # a moving state: three iterations of a transient step, then perturb U_old
+    case.transient_step(max_inner=3, u_tol=0.0, p_tol=0.0)
+    case.update_fields()
+    base = case.field("U_old").copy()
+    rng = np.random.default_rng(5)
+    out = {}
+    for tag in ("base", "others", "cell1"):
+        old = base.copy()
+        if tag == "others":
+            old[1:m.n_cells] += rng.standard_normal((m.n_cells - 1, 3))
+        if tag == "cell1":
+            old[0] += 0.5
+        case.set_field("U_old", old)
+        case.assemble_u(0)
+        out[tag] = case.system()["bc"]
+    assert np.array_equal(out["base"], out["others"])
+    assert np.count_nonzero(out["cell1"] != out["base"]) == m.n_cells
+
+
+def test_rhie_chow_transient_terms_are_linear_in_the_old_fields():
+    # terms VI-VII (cfdAssembleMdotTerm.m:175-201): mdot_f gains
+    # DUT_f (mdot_old_f - rho_old_f [v_old]_f . S_f) with DUT = mean of the
+    # three DUT components per element, DUT_f and [v_old]_f interpolated with
+    # the (unswapped) Average weights.  On a jittered mesh (g != 1/2) a change
+    # of mdot_old by D moves mdot_f by DUT_f D, and a change of U_old by E by
+    # -DUT_f rho [E]_f . S_f; a steady run has no such dependence.
+    m = cavity_mesh(6, jitter=0.15, seed=7, renumber_seed=8)
+    rho, Fi = 1.2, m.n_interior
+    rng = np.random.default_rng(9)
+    for transient in (True, False):
+        case = oracle.SimpleCase(m, oracle.cavity_bcs(m), nu=0.01, transient=transient, dt=0.1,
+                                 rho=rho, p_cycle="V-Cycle")
+        case.iterate(2)
+        case.update_fields()
+        for comp in range(3):
+            case.assemble_u(comp)
+            case.solve_u(comp)
+            case.correct_u(comp)
+        mdot0, mold, Uold = case.field("mdot"), case.field("mdot_old"), case.field("U_old")
+        DUT = case.field("DUT")
+        D = rng.standard_normal(m.n_faces) * np.abs(mold).max()
+        E = rng.standard_normal(Uold.shape)
+
+        def assemble(mo, uo):
+            case.set_field("mdot", mdot0)
+            case.set_field("mdot_old", mo)
+            case.set_field("U_old", uo)
+            case.assemble_p()
+            return case.field("mdot")[:Fi]
+
+        a = assemble(mold, Uold)
+        b = assemble(mold + D, Uold)
+        c = assemble(mold, Uold + E)
+        if not transient:
+            assert np.array_equal(a, b) and np.array_equal(a, c)
+            continue
+        o, nb, g = m.owner[:Fi], m.neighbour, m.gf[:Fi]
+        dm = ((DUT[:, 0] + DUT[:, 1]) + DUT[:, 2]) / 3
+        dut_f = g * dm[nb] + (1 - g) * dm[o]
+        assert np.abs(dut_f).max() > 0 and np.abs(g - 0.5).max() > 0.01
+        assert np.allclose(b - a, dut_f * D[:Fi], rtol=1e-7, atol=1e-12 * np.abs(a).max())
+        Ef = g[:, None] * E[nb] + (1 - g[:, None]) * E[o]
+        want = -dut_f * rho * (Ef * m.Sf[:Fi]).sum(1)
+        assert np.allclose(c - a, want, rtol=1e-7, atol=1e-12 * np.abs(a).max())
+
+
+def test_transient_step_ends_at_convergence():
+    # cfdRunCase.m:24-86: at most 20 outer iterations per time step, ending
+    # after the first whose RMS residuals meet residualControl
+    m = cavity_mesh(8, nz=1, jitter=0.0, renumber_seed=None)
+    case = oracle.SimpleCase(m, oracle.cavity_bcs(m), nu=0.01, transient=True, dt=0.02,
+                             p_cycle="V-Cycle")
+    assert case.transient_step(max_inner=20, u_tol=0.0, p_tol=0.0) == 20
+    n = case.transient_step(max_inner=20, u_tol=1e-2, p_tol=1e-2)
+    assert 1 <= n < 20
+    r = case.residuals()["rms"]
+    assert max(r[:3]) <= 1e-2 and r[3] <= 1e-2
