@@ -292,6 +292,17 @@ typedef struct {
   double u_init[3];                       /* uniform internal U of the 0 directory
                                              (cfdDefineEquation.m:68-82) */
   double p_init;                          /* uniform internal p */
+  double rho;                             /* constant density of div(rho*U*U) and
+                                             ddt(rho*U) (0: 1); nu is then the
+                                             laplacian's coefficient (mu) */
+  int32_t transient;                      /* 1: the momentum equation has ddt(rho*U) with
+                                             the Euler scheme (cfdAssembleTransientTermEuler.m,
+                                             every cell's source taking cell 1's U and U_old
+                                             as the reference's does) and the Rhie-Chow
+                                             fluxes terms VI-VII (cfdAssembleMdotTerm.m:
+                                             175-201); no false transient.  Steps are
+                                             advanced by ufvm_transient_step */
+  double dt;                              /* time step of the transient form */
 } ufvm_simple_cfg;
 
 /* Upload a mesh and create its fields (the uniform initial fields u_init and
@@ -301,8 +312,8 @@ int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *mesh, const ufvm_simpl
                      int32_t *case_out);
 int ufvm_case_destroy(ufvm_ctx *ctx, int32_t cas);
 /* Field access (cfdGetMeshField / cfdSetMeshField.m): "U", "p", "PP", "DU",
- * "DUT", "mdot_f", "gradU", "gradp", "gradPP", "gamma_f".  count must equal
- * the field size. */
+ * "DUT", "mdot_f", "gradU", "gradp", "gradPP", "gamma_f", and the previous time
+ * step's "U_old", "mdot_old".  count must equal the field size. */
 int ufvm_case_set_field(ufvm_ctx *ctx, int32_t cas, const char *name, const double *data,
                         int64_t count);
 int ufvm_case_get_field(ufvm_ctx *ctx, int32_t cas, const char *name, double *data,
@@ -325,6 +336,15 @@ int ufvm_correct_equation(ufvm_ctx *ctx, int32_t cas, const char *eq, int32_t co
 /* n steady outer iterations of cfdRunCase (cfdRunCase.m:92-142); rms (may be
  * NULL) receives 4 values per iteration: RMS residuals of Ux, Uy, Uz, p. */
 int ufvm_simple_iterate(ufvm_ctx *ctx, int32_t cas, int32_t n, double *rms);
+/* One time step of cfdRunCase's transient branch (cfdRunCase.m:24-86) for a
+ * case created with transient = 1: the previous step's U and mdot_f are kept
+ * (cfdTransientUpdate: fields "U_old" and "mdot_old"), then outer iterations
+ * run until the first whose RMS residuals are all <= u_tol (U components) and
+ * p_tol (p) (cfdCheckConvergence.m), at most max_inner (the reference's 20).
+ * *n_inner (may be NULL) receives the iterations run, rms (may be NULL, room
+ * for 4 * max_inner) their residuals. */
+int ufvm_transient_step(ufvm_ctx *ctx, int32_t cas, int32_t max_inner, double u_tol,
+                        double p_tol, double *rms, int32_t *n_inner);
 
 /* --------------------------------------------------------- serving form */
 /* No reference counterpart.  A single-GPU request loop over resident meshes:

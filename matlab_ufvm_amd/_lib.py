@@ -69,7 +69,8 @@ class SimpleCfg(C.Structure):
                 ("u_bc", i32p), ("u_value", f64p), ("p_bc", i32p), ("p_solver", C.c_int32),
                 ("p_amg", AmgCfg), ("p_pcg", PcgCfg), ("u_smoother", C.c_int32),
                 ("stress_term", C.c_int32), ("p_value", f64p), ("u_init", C.c_double * 3),
-                ("p_init", C.c_double)]
+                ("p_init", C.c_double), ("rho", C.c_double), ("transient", C.c_int32),
+                ("dt", C.c_double)]
 
 
 # name -> (restype, argtypes)
@@ -117,6 +118,8 @@ _SIGS = {
     "ufvm_solve_equation": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, f64p, f64p, i32p]),
     "ufvm_correct_equation": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, C.c_int32]),
     "ufvm_simple_iterate": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f64p]),
+    "ufvm_transient_step": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                      f64p, i32p]),
     "ufvm_server_create": (C.c_int, [C.c_int32, C.POINTER(ServerCfg), C.POINTER(C.c_void_p)]),
     "ufvm_server_destroy": (C.c_int, [C.c_void_p]),
     "ufvm_server_last_error": (C.c_char_p, [C.c_void_p]),

@@ -25,6 +25,7 @@ struct Case {
   DBuf<int32_t> ptype, u_bc, p_bc;
   DBuf<double> Sf, area, CN, eCN, gDiff, T, gf, fc, vol, cc, u_val, p_val, walldist, noslip_sum;
   DBuf<double> U, gradU, p, gradp, PP, gradPP, DU, DUT, mdot, gamma, fluxC1, fluxC2, ac_old;
+  DBuf<double> U_old, mdot_old;  // the previous time step's U and mdot_f ('Step1')
   DBuf<double> uface;  // momentum face fluxes, 6 per face
   DBuf<double> ustress;  // explicit stress flux per face (stress_term only)
   // 1 when the pressure-correction coefficients changed since the kept
@@ -40,6 +41,11 @@ struct Case {
   // with unwritten rows between them (outlet_gap); once it has run, the
   // reference's grown U holds zero rows that its scale sees (scale_zero)
   bool outlet_gap = false, scale_zero = false;
+  double rho = 1.0;
+
+  sk::Flow flow() const {
+    return sk::Flow{rho, cfg.transient, cfg.dt, U_old.p, mdot_old.p};
+  }
 };
 
 }  // namespace ufvm
@@ -108,11 +114,11 @@ void init_fields(ufvm_ctx *ctx, Case &C) {
       const double g = gf[f];
       const int32_t o = owner[f], nb = nbr[f];
       for (int d = 0; d < 3; ++d) uf[d] = g * U[nb * 3 + d] + (1 - g) * U[o * 3 + d];
-      rf = g * 1.0 + (1 - g) * 1.0;
+      rf = g * C.rho + (1 - g) * C.rho;
     } else {
       const int32_t b = C.N + (f - C.Fi);
       for (int d = 0; d < 3; ++d) uf[d] = U[b * 3 + d];
-      rf = 1.0;
+      rf = C.rho;
     }
     mdot[f] = rf * ((S[0] * uf[0] + S[1] * uf[1]) + S[2] * uf[2]);
   }
@@ -125,6 +131,8 @@ void init_fields(ufvm_ctx *ctx, Case &C) {
   C.DU.upload(zero.data(), (size_t)NB * 3, s);
   C.DUT.upload(zero.data(), (size_t)NB * 3, s);
   C.mdot.upload(mdot, s);
+  C.U_old.upload(U, s);
+  C.mdot_old.upload(mdot, s);
   C.gamma.upload(zero.data(), C.F, s);
   C.fluxC1.upload(zero.data(), C.F, s);
   C.fluxC2.upload(zero.data(), C.F, s);
@@ -149,6 +157,8 @@ FieldRef field_of(Case &C, const char *name) {
   if (n == "DUT") return {C.DUT.p, NB * 3};
   if (n == "mdot_f") return {C.mdot.p, (size_t)C.F};
   if (n == "gamma_f") return {C.gamma.p, (size_t)C.F};
+  if (n == "U_old") return {C.U_old.p, NB * 3};
+  if (n == "mdot_old") return {C.mdot_old.p, (size_t)C.F};
   throw Error(UFVM_ERR_ARG, "unknown field " + n);
 }
 
@@ -177,12 +187,12 @@ void assemble(ufvm_ctx *ctx, Case &C, int eq, int comp, double *rms, double *mx)
   ensure_partial(ctx, C.N, 2);
   if (eq == 0) {
     if (comp < 0 || comp > 2) throw Error(UFVM_ERR_ARG, "component must be 0, 1 or 2");
-    sk::assemble_u(s, C.dm, comp, C.cfg.urf_u, C.U.p, C.gradU.p, C.gradp.p, C.mdot.p, C.gamma.p,
+    sk::assemble_u(s, C.dm, C.flow(), comp, C.cfg.urf_u, C.U.p, C.gradU.p, C.gradp.p, C.mdot.p, C.gamma.p,
                    C.uface.p, C.cfg.stress_term ? C.ustress.p : nullptr, L.ac.p, L.anb.p, L.bc.p, C.ac_old.p, L.dphi.p, C.DU.p, C.DUT.p, ctx->partial.p,
                    ctx->counter.p, ctx->scal.p);
   } else {
-    sk::rhie_chow(s, C.dm, C.cfg.urf_u, C.U.p, C.DU.p, C.p.p, C.gradp.p, C.mdot.p, C.fluxC1.p,
-                  C.fluxC2.p);
+    sk::rhie_chow(s, C.dm, C.flow(), C.cfg.urf_u, C.U.p, C.DU.p, C.DUT.p, C.p.p, C.gradp.p,
+                  C.mdot.p, C.fluxC1.p, C.fluxC2.p);
     sk::assemble_p(s, C.dm, C.fluxC1.p, C.fluxC2.p, C.mdot.p, L.ac.p, L.anb.p, L.bc.p, C.ac_old.p,
                    L.dphi.p, ctx->partial.p, ctx->scal.p, C.p_changed.p);
   }
@@ -247,11 +257,27 @@ void correct(ufvm_ctx *ctx, Case &C, int eq, int comp) {
     sk::correct_u(s, C.dm, comp, S.lev[0]->dphi.p, C.U.p);
   } else {
     System &S = sys_of(ctx, C.sysP);
-    sk::correct_p(s, C.dm, C.cfg.p_ref_cell, C.cfg.urf_p, S.lev[0]->dphi.p, C.PP.p, C.gradPP.p,
+    sk::correct_p(s, C.dm, C.rho, C.cfg.p_ref_cell, C.cfg.urf_p, S.lev[0]->dphi.p, C.PP.p, C.gradPP.p,
                   C.DU.p, C.U.p, C.p.p, C.fluxC1.p, C.fluxC2.p, C.mdot.p);
     if (C.outlet_gap) C.scale_zero = true;
   }
   UFVM_HIP(hipStreamSynchronize(s));
+}
+
+// one outer iteration (cfdRunCase.m:92-142 / :42-57): field updates, the
+// momentum components, the pressure correctors; r gets the RMS residuals
+void outer_iteration(ufvm_ctx *ctx, Case &C, double r[4]) {
+  update_fields(ctx, C);
+  for (int c = 0; c < 3; ++c) {
+    assemble(ctx, C, 0, c, &r[c], nullptr);
+    solve(ctx, C, 0, nullptr, nullptr, nullptr);
+    correct(ctx, C, 0, c);
+  }
+  for (int k = 0; k < C.cfg.n_correctors; ++k) {
+    assemble(ctx, C, 1, 0, &r[3], nullptr);
+    solve(ctx, C, 1, nullptr, nullptr, nullptr);
+    correct(ctx, C, 1, 0);
+  }
 }
 
 }  // namespace
@@ -273,6 +299,11 @@ int ufvm_case_create(ufvm_ctx *ctx, const ufvm_mesh_desc *m, const ufvm_simple_c
     C.Fb = C.F - C.Fi;
     C.P = m->n_patches;
     C.cfg = *cfg;
+    C.rho = cfg->rho == 0.0 ? 1.0 : cfg->rho;
+    if (!(C.rho > 0.0)) throw Error(UFVM_ERR_ARG, "rho must be positive");
+    if (cfg->transient != 0 && cfg->transient != 1)
+      throw Error(UFVM_ERR_ARG, "transient must be 0 or 1");
+    if (cfg->transient && !(cfg->dt > 0.0)) throw Error(UFVM_ERR_ARG, "dt must be positive");
     if (cfg->u_smoother != UFVM_SMOOTHER_ILU && cfg->u_smoother != UFVM_SMOOTHER_MCGS)
       throw Error(UFVM_ERR_ARG, "u_smoother must be UFVM_SMOOTHER_ILU or UFVM_SMOOTHER_MCGS");
     // boundary faces -> patch, and patch checks
@@ -506,21 +537,36 @@ int ufvm_simple_iterate(ufvm_ctx *ctx, int32_t cas, int32_t n, double *rms) {
   return guard(ctx, [&] {
     Case &C = case_of(ctx, cas);
     for (int32_t it = 0; it < n; ++it) {
-      update_fields(ctx, C);
       double r[4];
-      for (int c = 0; c < 3; ++c) {
-        assemble(ctx, C, 0, c, &r[c], nullptr);
-        solve(ctx, C, 0, nullptr, nullptr, nullptr);
-        correct(ctx, C, 0, c);
-      }
-      for (int k = 0; k < C.cfg.n_correctors; ++k) {
-        assemble(ctx, C, 1, 0, &r[3], nullptr);
-        solve(ctx, C, 1, nullptr, nullptr, nullptr);
-        correct(ctx, C, 1, 0);
-      }
+      outer_iteration(ctx, C, r);
       if (rms)
         for (int q = 0; q < 4; ++q) rms[it * 4 + q] = r[q];
     }
+  });
+}
+
+int ufvm_transient_step(ufvm_ctx *ctx, int32_t cas, int32_t max_inner, double u_tol,
+                        double p_tol, double *rms, int32_t *n_inner) {
+  return guard(ctx, [&] {
+    Case &C = case_of(ctx, cas);
+    if (!C.cfg.transient) throw Error(UFVM_ERR_STATE, "the case was not created transient");
+    if (max_inner < 1) throw Error(UFVM_ERR_ARG, "max_inner must be >= 1");
+    // cfdTransientUpdate: the previous step's fields ('Step1')
+    UFVM_HIP(hipMemcpyAsync(C.U_old.p, C.U.p, (size_t)(C.N + C.Fb) * 3 * sizeof(double),
+                            hipMemcpyDeviceToDevice, ctx->stream));
+    UFVM_HIP(hipMemcpyAsync(C.mdot_old.p, C.mdot.p, (size_t)C.F * sizeof(double),
+                            hipMemcpyDeviceToDevice, ctx->stream));
+    int32_t it = 0;
+    while (it < max_inner) {
+      double r[4];
+      outer_iteration(ctx, C, r);
+      if (rms)
+        for (int q = 0; q < 4; ++q) rms[it * 4 + q] = r[q];
+      ++it;
+      // cfdCheckConvergence.m: converged when no RMS residual exceeds its tolerance
+      if (r[0] <= u_tol && r[1] <= u_tol && r[2] <= u_tol && r[3] <= p_tol) break;
+    }
+    if (n_inner) *n_inner = it;
   });
 }
 

@@ -32,6 +32,15 @@ struct DevMesh {
   const double *noslip_sum;          // [patches][3] sum over the patch of (Sf_c / area)^2
 };
 
+// the density and the transient form of the momentum equation
+struct Flow {
+  double rho;              // constant density
+  int transient;           // 1: ddt(rho*U), Euler, time step dt; 0: false transient
+  double dt;
+  const double *U_old;     // the previous time step's U and mdot_f
+  const double *mdot_old;
+};
+
 // scalar slots written by the reductions of this file
 enum { S_UMAX = 0, S_UMIN = 1, S_SCALE = 2, S_RMAX = 3, S_RSQ = 4 };
 
@@ -43,21 +52,23 @@ void gamma_faces(hipStream_t s, const DevMesh &m, double nu, double *gamma);
 // reference's U gains past its end from a fixed-pressure outlet's correction)
 void scale_u(hipStream_t s, const DevMesh &m, const double *U, bool zero, double *partial,
              unsigned *counter, double *scal);
-void assemble_u(hipStream_t s, const DevMesh &m, int comp, double urf, const double *U,
+void assemble_u(hipStream_t s, const DevMesh &m, const Flow &fl, int comp, double urf,
+                const double *U,
                 const double *gradU, const double *gradp, const double *mdot,
                 const double *gamma, double *uf, double *stress, double *ac, double *anb,
                 double *bc, double *ac_old, double *dphi, double *DU, double *DUT, double *partial,
                 unsigned *counter, double *scal);
 void correct_u(hipStream_t s, const DevMesh &m, int comp, const double *dphi, double *U);
-void rhie_chow(hipStream_t s, const DevMesh &m, double urf_u, const double *U,
-               const double *DU, const double *p, const double *gradp, double *mdot,
-               double *fluxC1, double *fluxC2);
+void rhie_chow(hipStream_t s, const DevMesh &m, const Flow &fl, double urf_u, const double *U,
+               const double *DU, const double *DUT, const double *p, const double *gradp,
+               double *mdot, double *fluxC1, double *fluxC2);
 // pressure-correction coefficients; *changed is set to 1 when an ac or anb
 // value's bits differ from the value it overwrites (never cleared here)
 void assemble_p(hipStream_t s, const DevMesh &m, const double *fluxC1, const double *fluxC2,
                 const double *mdot, double *ac, double *anb, double *bc, double *ac_old,
                 double *dphi, double *partial, double *scal, int32_t *changed);
-void correct_p(hipStream_t s, const DevMesh &m, int32_t ref, double urf_p, const double *dphi,
+void correct_p(hipStream_t s, const DevMesh &m, double rho, int32_t ref, double urf_p,
+               const double *dphi,
                double *PP, double *gradPP, const double *DU, double *U, double *p,
                const double *fluxC1, const double *fluxC2, double *mdot);
 

@@ -554,7 +554,7 @@ __device__ __forceinline__ void diff_entry(const UEntry &x, int32_t Fi, double *
 // DUT = ac_old/ac (cfdPostAssembleEquation.m:37-41).  The entries of a cell
 // with at most 8 faces are loaded together before the passes.
 __global__ void __launch_bounds__(kBlock)
-k_assemble_u(DevMesh m, int c, double urf, const double *__restrict__ U,
+k_assemble_u(DevMesh m, Flow fl, int c, double urf, const double *__restrict__ U,
              const double *__restrict__ gradp, const double *__restrict__ mdot,
              const double *__restrict__ uf, const double *__restrict__ stress, double *ac_out,
              double *anb, double *bc_out, double *acold_out, double *dphi, double *DU,
@@ -566,6 +566,20 @@ k_assemble_u(DevMesh m, int c, double urf, const double *__restrict__ U,
     double ac = 0.0, bc = 0.0, acold = 0.0, eff = 0.0;
     const int32_t e0 = m.cf_ptr[i], e1 = m.cf_ptr[i + 1], nf = e1 - e0;
     const double phi = U[i * 3 + c];
+    if (fl.transient) {
+      // ddt(rho*U), Euler, the equation's first term
+      // (cfdAssembleTransientTermEuler.m:13-44): FLUXCE = V rho / dt,
+      // FLUXCEOLD = -V rho_old / dt, and the source FLUXCE .* phi' +
+      // FLUXCEOLD .* phi_old' is N x N, read by element index through its
+      // first column: every cell takes cell 1's phi and phi_old (kept)
+      const double v = m.vol[i];
+      const double CE = ((1.0 * v) * fl.rho) / fl.dt;
+      const double CEOLD = (((-1.0) * v) * fl.rho) / fl.dt;
+      const double TE = CE * U[0 * 3 + c] + CEOLD * fl.U_old[0 * 3 + c];
+      ac = ac + CE;
+      acold = acold + CEOLD;
+      bc = bc - TE;
+    }
     if (nf <= 8) {
       UEntry x[8];
 #pragma unroll
@@ -622,11 +636,12 @@ k_assemble_u(DevMesh m, int c, double urf, const double *__restrict__ U,
       acold = acold + 0.0;
       bc = bc - TE;
     }
-    // false transient
-    {
+    // false transient of steady runs (not in transient ones,
+    // cfdAssembleEquationTerms.m:44-47)
+    if (!fl.transient) {
       const double fdt = 1e9;
-      const double CE = (vol * 1.0) / fdt;
-      const double CEOLD = ((-vol) * 1.0) / fdt;
+      const double CE = (vol * fl.rho) / fdt;
+      const double CEOLD = ((-vol) * fl.rho) / fdt;
       const double TE = CE * U[i * 3 + c];
       ac = ac + CE;
       acold = acold + CEOLD;
@@ -637,7 +652,15 @@ k_assemble_u(DevMesh m, int c, double urf, const double *__restrict__ U,
     bc_out[i] = bc;
     acold_out[i] = acold;
     dphi[i] = 0.0;
-    const double r = bc / (ac * scal[S_SCALE]);
+    // transient runs normalise by ac - V rho / dt, floored at 1e-6 of it
+    // (cfdComputeNormalizedResidual.m:55-76)
+    double an = ac;
+    if (fl.transient) {
+      const double at = (vol * fl.rho) / fl.dt;
+      an = an - at;
+      if (an < 1e-6 * at) an = at;
+    }
+    const double r = bc / (an * scal[S_SCALE]);
     red[0] = fabs(r);
     red[1] = r * r;
     DU[i * 3 + c] = vol / ac;
@@ -655,7 +678,8 @@ __global__ void k_du_bnd(DevMesh m, int c, double *DU, double *DUT) {
   DUT[b * 3 + c] = DUT[o * 3 + c];
 }
 
-void assemble_u(hipStream_t s, const DevMesh &m, int comp, double urf, const double *U,
+void assemble_u(hipStream_t s, const DevMesh &m, const Flow &fl, int comp, double urf,
+                const double *U,
                 const double *gradU, const double *gradp, const double *mdot,
                 const double *gamma, double *uf, double *stress, double *ac, double *anb,
                 double *bc, double *ac_old, double *dphi, double *DU, double *DUT, double *partial,
@@ -663,7 +687,7 @@ void assemble_u(hipStream_t s, const DevMesh &m, int comp, double urf, const dou
   (void)counter;
   hipLaunchKernelGGL(k_u_faces, dim3(nblocks(m.F)), dim3(kBlock), 0, s, m, comp, U, gradU, mdot,
                      gamma, uf, stress);
-  hipLaunchKernelGGL(k_assemble_u, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, comp, urf, U, gradp,
+  hipLaunchKernelGGL(k_assemble_u, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, fl, comp, urf, U, gradp,
                      mdot, uf, stress, ac, anb, bc, ac_old, dphi, DU, DUT, partial, scal);
   hipLaunchKernelGGL(k_reduce2, dim3(1), dim3(kBlock), 0, s, partial, nblocks(m.N), 1, scal);
   const int32_t Fb = m.F - m.Fi;
@@ -712,7 +736,8 @@ void correct_u(hipStream_t s, const DevMesh &m, int comp, const double *dphi, do
 // planes are zero, inlets and fixed-pressure outlets get theirs and a new
 // mdot_b (:281-348).
 __global__ void __launch_bounds__(kBlock)
-k_rhie_chow(DevMesh m, double urf_u, const double *__restrict__ U, const double *__restrict__ DU,
+k_rhie_chow(DevMesh m, Flow fl, double urf_u, const double *__restrict__ U,
+            const double *__restrict__ DU, const double *__restrict__ DUTc,
             const double *__restrict__ p, const double *__restrict__ gradp, double *mdot,
             double *fluxC1, double *fluxC2) {
   const int32_t f = blockIdx.x * kBlock + threadIdx.x;
@@ -729,7 +754,7 @@ k_rhie_chow(DevMesh m, double urf_u, const double *__restrict__ U, const double 
     double C1 = 0.0, C2 = 0.0, V = 0.0;
     if (m.ptype[q] == PATCH_INLET) {
       // zeroGradient-p inlet: term I only (cfdAssembleMdotTerm.m:415-486)
-      V = V + 1.0 * dot3(Sb, &U[b * 3]);
+      V = V + fl.rho * dot3(Sb, &U[b * 3]);
     } else {
       // fixed-pressure outlet: terms I-III from the boundary rows of DU and
       // grad(p), FLUXCb2 = 0 (cfdAssembleMdotTerm.m:799-906)
@@ -742,11 +767,11 @@ k_rhie_chow(DevMesh m, double urf_u, const double *__restrict__ U, const double 
       for (int d = 0; d < 3; ++d) DUE[d] = (mDUS / ee) * eCNb[d];
       const double geoDiff = mag3(DUE) / mag3(CNb);
       for (int d = 0; d < 3; ++d) DUT[d] = DUS[d] - DUE[d];
-      V = V + 1.0 * dot3(&U[b * 3], Sb);
-      C1 = C1 + 1.0 * geoDiff;
+      V = V + fl.rho * dot3(&U[b * 3], Sb);
+      C1 = C1 + fl.rho * geoDiff;
       C2 = C2 - 0.0;
-      V = V - 1.0 * dot3(&gradp[b * 3], DUT);
-      V = V + 1.0 * dot3(&gradp[b * 3], DUS);
+      V = V - fl.rho * dot3(&gradp[b * 3], DUT);
+      V = V + fl.rho * dot3(&gradp[b * 3], DUS);
     }
     mdot[f] = (C1 * p[o] + C2 * p[b]) + V;
     fluxC1[f] = C1;
@@ -759,8 +784,9 @@ k_rhie_chow(DevMesh m, double urf_u, const double *__restrict__ U, const double 
   for (int d = 0; d < 3; ++d) du[d] = w * DU[nb * 3 + d] + (1 - w) * DU[o * 3 + d];
   const double mprev = mdot[f];
   for (int d = 0; d < 3; ++d) ub[d] = w * U[nb * 3 + d] + (1 - w) * U[o * 3 + d];
+  // rho_f: 'Upwind' by the previous mdot_f (cfdAssembleMdotTerm.m:118-121)
   const double pos = mprev > 0 ? 1.0 : 0.0;
-  const double rho_f = 1.0 * pos + 1.0 * (1 - pos);
+  const double rho_f = fl.rho * pos + fl.rho * (1 - pos);
   for (int d = 0; d < 3; ++d) gpb[d] = (1 - w) * gradp[nb * 3 + d] + w * gradp[o * 3 + d];
   grad_face<1>(m, p, gradp, 0, f, gpf);
   const double *Sf = &m.Sf[f * 3], *CF = &m.CN[f * 3], *eCF = &m.eCN[f * 3];
@@ -778,17 +804,30 @@ k_rhie_chow(DevMesh m, double urf_u, const double *__restrict__ U, const double 
   const double FF = 0.0 - rho_f * geoDiff;
   V = V - rho_f * dot3(gpf, DUT);
   V = V + rho_f * dot3(gpb, DUS);
+  if (fl.transient) {
+    // terms VI-VII (:175-201): DUT_f (mdot_old - rho_old_f [v_old]_f . S_f),
+    // DUT = mean of the three DUT components per element; DUT_f and
+    // [v_old]_f Average-interpolated, rho_old_f 'Upwind' by the previous mdot_f
+    auto dmean = [&](int32_t e) {
+      return ((DUTc[e * 3] + DUTc[e * 3 + 1]) + DUTc[e * 3 + 2]) / 3;
+    };
+    const double dut = w * dmean(nb) + (1 - w) * dmean(o);
+    const double rho_old_f = fl.rho * pos + fl.rho * (1 - pos);
+    double uo[3];
+    for (int d = 0; d < 3; ++d) uo[d] = w * fl.U_old[nb * 3 + d] + (1 - w) * fl.U_old[o * 3 + d];
+    V = V + dut * (fl.mdot_old[f] - rho_old_f * dot3(uo, Sf));
+  }
   V = V + (1 - urf_u) * (mprev - rho_f * UbS);
   mdot[f] = (FC * p[o] + FF * p[nb]) + V;
   fluxC1[f] = FC;
   fluxC2[f] = FF;
 }
 
-void rhie_chow(hipStream_t s, const DevMesh &m, double urf_u, const double *U,
-               const double *DU, const double *p, const double *gradp, double *mdot,
-               double *fluxC1, double *fluxC2) {
-  hipLaunchKernelGGL(k_rhie_chow, dim3(nblocks(m.F)), dim3(kBlock), 0, s, m, urf_u, U, DU, p,
-                     gradp, mdot, fluxC1, fluxC2);
+void rhie_chow(hipStream_t s, const DevMesh &m, const Flow &fl, double urf_u, const double *U,
+               const double *DU, const double *DUT, const double *p, const double *gradp,
+               double *mdot, double *fluxC1, double *fluxC2) {
+  hipLaunchKernelGGL(k_rhie_chow, dim3(nblocks(m.F)), dim3(kBlock), 0, s, m, fl, urf_u, U, DU,
+                     DUT, p, gradp, mdot, fluxC1, fluxC2);
 }
 
 // pressure-correction matrix from the face fluxes (T_f = mdot_f), element
@@ -951,8 +990,9 @@ __global__ void k_correct_p_cells(DevMesh m, int32_t ref, double urf_p, const do
 // p's condition (:56-120): planes 0 mdot_b (:306-346), fixed-pressure outlets
 // rho_b (U_O . S_b) with the corrected owner U (:264-304), walls with
 // zeroGradient p and incompressible inlets unchanged (:156-208)
-__global__ void k_correct_mdot(DevMesh m, const double *fluxC1, const double *fluxC2,
-                               const double *PP, const double *U, double *mdot) {
+__global__ void k_correct_mdot(DevMesh m, double rho, const double *fluxC1,
+                               const double *fluxC2, const double *PP, const double *U,
+                               double *mdot) {
   const int32_t f = blockIdx.x * kBlock + threadIdx.x;
   if (f >= m.F) return;
   if (f < m.Fi) {
@@ -963,11 +1003,12 @@ __global__ void k_correct_mdot(DevMesh m, const double *fluxC1, const double *fl
   if (plane(m, q)) {
     mdot[f] = 0.0 * mdot[f];
   } else if (p_fixed_outlet(m, q)) {
-    mdot[f] = 1.0 * dot3(&U[m.owner[f] * 3], &m.Sf[f * 3]);
+    mdot[f] = rho * dot3(&U[m.owner[f] * 3], &m.Sf[f * 3]);
   }
 }
 
-void correct_p(hipStream_t s, const DevMesh &m, int32_t ref, double urf_p, const double *dphi,
+void correct_p(hipStream_t s, const DevMesh &m, double rho, int32_t ref, double urf_p,
+               const double *dphi,
                double *PP, double *gradPP, const double *DU, double *U, double *p,
                const double *fluxC1, const double *fluxC2, double *mdot) {
   const int32_t Fb = m.F - m.Fi;
@@ -978,8 +1019,8 @@ void correct_p(hipStream_t s, const DevMesh &m, int32_t ref, double urf_p, const
   if (Fb > 0) hipLaunchKernelGGL(k_bnd_u_p, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, U);
   hipLaunchKernelGGL(k_correct_p_cells, dim3(nblocks(m.N)), dim3(kBlock), 0, s, m, ref, urf_p, PP, p);
   if (Fb > 0) hipLaunchKernelGGL(k_p_bnd, dim3(nblocks(Fb)), dim3(kBlock), 0, s, m, p);
-  hipLaunchKernelGGL(k_correct_mdot, dim3(nblocks(m.F)), dim3(kBlock), 0, s, m, fluxC1, fluxC2, PP,
-                     U, mdot);
+  hipLaunchKernelGGL(k_correct_mdot, dim3(nblocks(m.F)), dim3(kBlock), 0, s, m, rho, fluxC1,
+                     fluxC2, PP, U, mdot);
 }
 
 }  // namespace sk

@@ -45,14 +45,15 @@ def case_structs(mesh, bcs, nu, urf_u=0.7, urf_p=0.3, n_correctors=2, p_ref=0, u
                  p_max_coarse_levels=10, p_pre=1, p_post=3, p_smoother="ILU", pcg_precision=32,
                  pcg_tol=1e-8, pcg_sweeps=1, pcg_max_coarse_cells=2000, pcg_coarse_scale=2.5,
                  u_smoother="ILU", pcg_coarse_sweeps=2, stress=False, u_init=(0.0, 0.0, 0.0),
-                 p_init=0.0):
+                 p_init=0.0, rho=1.0, transient=False, dt=1.0):
     """(MeshDesc, SimpleCfg, keep): the C structs of ufvm_case_create; the arrays
     they point to live in `keep`, which must outlive the call.  bcs:
     {patch: (U bc, U value (3,), p bc[, p value])}.  u_smoother: "ILU" (the
     reference's DILU momentum solve) or "MCGS" (multicolour GS); stress: the
     momentum equation has + div(nu*transp(grad(U))), as the reference's
     tutorials do; u_init / p_init: the uniform internal fields of the 0
-    directory."""
+    directory; rho: the constant density (nu is then the laplacian's
+    coefficient); transient / dt: ddt(rho*U) with the Euler scheme."""
     m = mesh
     keep = []
 
@@ -91,6 +92,9 @@ def case_structs(mesh, bcs, nu, urf_u=0.7, urf_p=0.3, n_correctors=2, p_ref=0, u
                   f64p)
     c.u_init = (C.c_double * 3)(*[float(v) for v in u_init])
     c.p_init = float(p_init)
+    c.rho = float(rho)
+    c.transient = int(bool(transient))
+    c.dt = float(dt)
     return d, c, keep
 
 
@@ -100,7 +104,7 @@ class SimpleCase:
                  p_termination=0.1, p_max_coarse_levels=10, p_pre=1, p_post=3, p_smoother="ILU",
                  pcg_precision=32, pcg_tol=1e-8, pcg_sweeps=1, pcg_max_coarse_cells=2000,
                  pcg_coarse_scale=2.5, u_smoother="ILU", pcg_coarse_sweeps=2, stress=False,
-                 u_init=(0.0, 0.0, 0.0), p_init=0.0):
+                 u_init=(0.0, 0.0, 0.0), p_init=0.0, rho=1.0, transient=False, dt=1.0):
         self.ctx = ctx
         self.mesh = mesh
         self.bcs = bcs
@@ -108,7 +112,7 @@ class SimpleCase:
             mesh, bcs, nu, urf_u, urf_p, n_correctors, p_ref, u_sweeps, p_solver, p_cycle,
             p_max_cycles, p_termination, p_max_coarse_levels, p_pre, p_post, p_smoother,
             pcg_precision, pcg_tol, pcg_sweeps, pcg_max_coarse_cells, pcg_coarse_scale,
-            u_smoother, pcg_coarse_sweeps, stress, u_init, p_init)
+            u_smoother, pcg_coarse_sweeps, stress, u_init, p_init, rho, transient, dt)
         cid = C.c_int32()
         ctx.check(ctx.lib.ufvm_case_create(ctx.h, C.byref(d), C.byref(c), C.byref(cid)))
         self.id = cid.value
@@ -152,6 +156,14 @@ class SimpleCase:
                 self.cfdCorrectEquation(name, c)
         return rms, mx
 
+    def transient_step(self, max_inner=20, u_tol=1e-6, p_tol=1e-6):
+        """one time step (cfdRunCase.m:24-86): (iterations run, RMS residuals [it, 4])"""
+        rms = np.zeros((max_inner, 4))
+        n = C.c_int32()
+        self.ctx.check(self.ctx.lib.ufvm_transient_step(self.ctx.h, self.id, max_inner, u_tol, p_tol,
+                                                        rms.ctypes.data_as(f64p), C.byref(n)))
+        return n.value, rms[:n.value]
+
     def iterate(self, n=1):
         """n steady outer iterations (cfdRunCase.m:92-142); RMS residuals [n, 4]."""
         out = np.zeros((n, 4))
@@ -165,7 +177,7 @@ class SimpleCase:
         nb = m.n_cells + m.n_bfaces
         return {"U": nb * 3, "gradU": nb * 9, "p": nb, "gradp": nb * 3, "PP": nb,
                 "gradPP": nb * 3, "DU": nb * 3, "DUT": nb * 3, "mdot_f": m.n_faces,
-                "gamma_f": m.n_faces}[name]
+                "gamma_f": m.n_faces, "U_old": nb * 3, "mdot_old": m.n_faces}[name]
 
     def field(self, name):
         n = self._size(name)
@@ -175,7 +187,7 @@ class SimpleCase:
         m = self.mesh
         nb = m.n_cells + m.n_bfaces
         shape = {"U": (nb, 3), "gradU": (nb, 3, 3), "gradp": (nb, 3), "gradPP": (nb, 3),
-                 "DU": (nb, 3), "DUT": (nb, 3)}.get(name, (n,))
+                 "DU": (nb, 3), "DUT": (nb, 3), "U_old": (nb, 3)}.get(name, (n,))
         return out.reshape(shape)
 
     def set_field(self, name, values):

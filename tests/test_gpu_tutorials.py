@@ -13,6 +13,11 @@ the boundary branches it needs, through the C ABI.
 * The precision form inside SIMPLE: FP32-preconditioned AMG-PCG pressure
   solves against the oracle's FP64 ones at the same tolerance on a jittered
   3-D cavity, per-field relative L2 <= 1e-8.
+* The bump tutorial (density 1.225): bit-exact and the reference's written
+  values.
+* Transient time steps (ddt(rho*U), Euler, Rhie-Chow terms VI-VII) on a
+  jittered cavity and the inlet/outlet channel: bit-exact step by step and
+  over whole time steps with their early exit.
 """
 import os
 import re
@@ -161,4 +166,82 @@ def test_precision_form_simple_steps_vs_fp64(ctx):
         errs[name] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
     print(f"\nFP32-preconditioned vs FP64 SIMPLE, {steps} steps: {errs}", flush=True)
     assert max(errs.values()) <= 1e-8, errs
+    g.close()
+
+
+BUMP = os.path.join(os.path.dirname(__file__), "golden", "bump")
+
+
+def test_bump_tutorial_bitexact_and_reference_output(ctx):
+    # density 1.225, laplacian coefficient mu = 1.8e-5, one corrector
+    case = read_case(BUMP)
+    m = case.mesh
+    print(f"\n[bump tutorial] cells={m.n_cells}", flush=True)
+    g = SimpleCase(ctx, m, case.bcs, stress=True, **case.simple_kwargs())
+    o = oracle.SimpleCase(m, case.bcs, stress=True, **case.simple_kwargs())
+    N, done = m.n_cells, 0
+    for t in ("0.05", "0.5", "1.1"):
+        it = int(round(float(t) / 0.05))
+        g.iterate(it - done)
+        o.iterate(it - done)
+        done = it
+        for name, oname in (("U", "U"), ("p", "p"), ("mdot_f", "mdot")):
+            same(g.field(name), o.field(oname), f"t={t} {name}")
+        for name, vals in (("U", g.field("U")[:N].reshape(-1)), ("p", g.field("p")[:N]),
+                           ("mdot_f", g.field("mdot_f")[:m.n_interior])):
+            assert [num2str(v) for v in vals] == written(os.path.join(BUMP, t, name))[:vals.size]
+    g.close()
+
+
+@pytest.mark.parametrize("which", ["cavity", "channel"])
+def test_transient_steps_bitexact(ctx, which):
+    # ddt(rho*U) Euler with the Rhie-Chow terms VI-VII: every step of a time
+    # step's iterations bit-exact, then whole time steps with their early exit
+    if which == "cavity":
+        m = cavity_mesh(10, jitter=0.15, seed=131, renumber_seed=132)
+        bcs, extra = cavity_bcs(m), dict(p_ref=0)
+        obcs = oracle.cavity_bcs(m)
+    else:
+        m = channel_mesh(10, 6, 5, seed=133, renumber_seed=134)
+        bcs = obcs = channel_bcs(u_in=(1.0, 0.05, 0.0), p_out=0.1)
+        extra = dict(p_ref=-1, u_init=(1.0, 0.0, 0.0), p_init=0.1)
+    print(f"\n[transient {which}] cells={m.n_cells}", flush=True)
+    args = dict(nu=0.01, rho=1.3, transient=True, dt=0.02, stress=True, **SOR_V, **extra)
+    g = SimpleCase(ctx, m, bcs, **args)
+    o = oracle.SimpleCase(m, obcs, **args)
+    # first time step, its first two iterations step by step
+    n0, _ = g.transient_step(max_inner=1, u_tol=0.0, p_tol=0.0)
+    o.transient_step(max_inner=1, u_tol=0.0, p_tol=0.0)
+    assert n0 == 1
+    for name in ("U_old", "mdot_old"):
+        same(g.field(name), o.field(name), name)
+    g.cfdUpdateFields()
+    o.update_fields()
+    for c in range(3):
+        g.cfdAssembleEquation("U", c)
+        o.assemble_u(c)
+        same(g.field("DU")[:, c], o.field("DU")[:, c], f"DU{c}")
+        same(g.field("DUT")[:, c], o.field("DUT")[:, c], f"DUT{c}")
+        g.cfdSolveEquation("U")
+        o.solve_u(c)
+        g.cfdCorrectEquation("U", c)
+        o.correct_u(c)
+        same(g.field("U"), o.field("U"), f"U after U{c}")
+    for k in range(2):
+        g.cfdAssembleEquation("p")
+        o.assemble_p()
+        same(g.field("mdot_f"), o.field("mdot"), f"corr{k} mdot (Rhie-Chow VI-VII)")
+        g.cfdSolveEquation("p")
+        o.solve_p()
+        g.cfdCorrectEquation("p")
+        o.correct_p()
+        for name, oname in (("U", "U"), ("p", "p"), ("mdot_f", "mdot")):
+            same(g.field(name), o.field(oname), f"corr{k} {name}")
+    # whole time steps: a loose tolerance ends some early, as the oracle's do
+    for _ in range(3):
+        ng, rg = g.transient_step(max_inner=20, u_tol=1e-3, p_tol=1e-3)
+        no = o.transient_step(max_inner=20, u_tol=1e-3, p_tol=1e-3)
+        assert ng == no and np.all(np.isfinite(rg))
+        for name, oname in (("U", "U"), ("p", "p"), ("mdot_f", "mdot")):
+            same(g.field(name), o.field(oname), f"time step {name}")
     g.close()
