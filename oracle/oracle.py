@@ -15,7 +15,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libufvm_oracle.so")
+# UFVM_ORACLE_LIB selects another build of the same source (the sanitizer
+# build of the CPU tests)
+LIB_PATH = os.environ.get("UFVM_ORACLE_LIB") or os.path.join(HERE, "libufvm_oracle.so")
 # the same source built with OpenMP: the FPCG solve's loops on all host cores
 # (used only to time the CPU baseline; its dot products round differently)
 LIB_PATH_OMP = os.path.join(HERE, "libufvm_oracle_omp.so")

@@ -146,7 +146,7 @@ int orc_agglomerate_level(orc_level *F, orc_level *C) {
   C->ptr[0] = 0;
   for (int32_t p = 0; p < nc; ++p) {
     C->ptr[p + 1] = C->ptr[p] + cnt[p];
-    memcpy(C->col + C->ptr[p], lst[p], (size_t)cnt[p] * sizeof(int32_t));
+    if (cnt[p]) memcpy(C->col + C->ptr[p], lst[p], (size_t)cnt[p] * sizeof(int32_t));
     free(lst[p]);
   }
   free(lst); free(cnt); free(cap);
