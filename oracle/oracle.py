@@ -198,6 +198,16 @@ class System:
                                   1 if (s == 0 and from_zero) else 0, _p(rc, f64p), _p(x, f64p))
         return dinv, x
 
+    def mcilu0_factor(self, l=0):
+        """ILU(0) in colour order on level l: (colour, dinv, F) with F the
+        factor's off-diagonal values in the level's CSR order"""
+        L = self.H.lev[l]
+        c = np.empty(L.n, np.int32)
+        nc = lib().orc_colour(L.n, L.ptr, L.col, _p(c, i32p))
+        dinv, F = np.empty(L.n), np.empty(max(L.nnz, 1))
+        lib().orc_mcilu0_factor(C.byref(L), _p(c, i32p), nc, _p(dinv, f64p), _p(F, f64p))
+        return c, dinv, F[:L.nnz]
+
     # ------------------------------------------------ precision-form oracle
     def _colour_all(self, nlev):
         """level 0: global greedy colouring; coarse levels: block-local."""
@@ -226,7 +236,7 @@ class System:
         nlev = self.pcg_levels(max_coarse_cells)
         arr, ncl = self._colour_all(nlev)
         cfg = VCfg(nlev, pre, post, singular, BLOCK_ROWS, coarse_smoother,
-                   {"MCGS": 0, "ILU": 1, "MCILU": 2}[smoother], coarse_scale,
+                   {"MCGS": 0, "ILU": 1, "MCILU": 2, "MCILU0": 3}[smoother], coarse_scale,
                    pre if cpre < 0 else cpre,
                    post if cpost < 0 else cpost)
         b = np.ascontiguousarray(b, np.float64)
@@ -239,7 +249,7 @@ class System:
         nlev = self.pcg_levels(max_coarse_cells)
         arr, ncl = self._colour_all(nlev)
         cfg = VCfg(nlev, pre, post, singular, BLOCK_ROWS, coarse_smoother,
-                   {"MCGS": 0, "ILU": 1, "MCILU": 2}[smoother], coarse_scale,
+                   {"MCGS": 0, "ILU": 1, "MCILU": 2, "MCILU0": 3}[smoother], coarse_scale,
                    pre if cpre < 0 else cpre,
                    post if cpost < 0 else cpost)
         b = np.ascontiguousarray(b, np.float64)

@@ -625,8 +625,56 @@ void orc_mcilu_factor(const orc_level *L, const int32_t *colour, int ncol, doubl
   free(order); free(pos); free(nb);
 }
 
+/* ILU(0) in colour order (smoother 3 of the precision-form V-cycle; no
+ * reference counterpart: cfg3 asks for ILU(0) beside multicolour GS).  Row by
+ * row in (colour, index) order, IKJ elimination restricted to the pattern:
+ * w = row j of A; for each earlier-colour neighbour k of j in increasing
+ * position, l = w_jk dinv_k, F_jk = w_jk (the undivided L entry), and for each
+ * later entry m of row k (pos m > pos k, entry order): m = j updates the
+ * diagonal d = d - l F_kj, any other m in row j updates w_jm = w_jm - l F_km;
+ * then F_jm = w_jm for the later entries of row j and dinv_j = 1 / d.  The
+ * preconditioner M = (D + L~) D^-1 (D + U~) with L~, U~ the F values is L U of
+ * ILU(0), and applying it is orc_mcilu_sweep_f with F.  On a graph without
+ * triangles no off-diagonal changes (F = anb) and the factor is
+ * orc_mcilu_factor's bit for bit. */
+void orc_mcilu0_factor(const orc_level *L, const int32_t *colour, int ncol, double *dinv,
+                       double *F) {
+  const int32_t n = L->n;
+  int32_t *order = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+  int32_t *pos = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+  int32_t *nb = (int32_t *)malloc((size_t)max_row(L) * sizeof(int32_t));
+  mc_order(colour, ncol, n, order, pos);
+  for (int32_t t = 0; t < n; ++t) {
+    const int32_t j = order[t];
+    for (int32_t k = L->ptr[j]; k < L->ptr[j + 1]; ++k) F[k] = L->anb[k];
+    double d = L->ac[j];
+    const int m = lower_of(L, pos, j, nb);
+    for (int e = 0; e < m; ++e) {
+      const int32_t k = nb[e];
+      const double l = F[entry_of(L, j, k)] * dinv[k];
+      for (int32_t q = L->ptr[k]; q < L->ptr[k + 1]; ++q) {
+        const int32_t c = L->col[q];
+        if (pos[c] <= pos[k]) continue;
+        if (c == j) {
+          d = d - l * F[q];
+        } else {
+          const int32_t r = entry_of(L, j, c);
+          if (r >= 0) F[r] = F[r] - l * F[q];
+        }
+      }
+    }
+    dinv[j] = 1.0 / d;
+  }
+  free(order); free(pos); free(nb);
+}
+
 void orc_mcilu_sweep(const orc_level *L, const int32_t *colour, int ncol, const double *dinv,
                      const double *b, int from_zero, double *rc, double *x) {
+  orc_mcilu_sweep_f(L, colour, ncol, dinv, L->anb, b, from_zero, rc, x);
+}
+
+void orc_mcilu_sweep_f(const orc_level *L, const int32_t *colour, int ncol, const double *dinv,
+                       const double *F, const double *b, int from_zero, double *rc, double *x) {
   const int32_t n = L->n;
   int32_t *order = (int32_t *)malloc((size_t)n * sizeof(int32_t));
   int32_t *pos = (int32_t *)malloc((size_t)n * sizeof(int32_t));
@@ -641,7 +689,7 @@ void orc_mcilu_sweep(const orc_level *L, const int32_t *colour, int ncol, const 
     for (int32_t k = L->ptr[j]; k < L->ptr[j + 1]; ++k) {
       const int32_t i = L->col[k];
       if (!from_zero) res = res - L->anb[k] * x[i];
-      if (colour[i] < colour[j]) low = low + L->anb[k] * (dinv[i] * rc[i]);
+      if (colour[i] < colour[j]) low = low + F[k] * (dinv[i] * rc[i]);
     }
     rc[j] = res - low;
   }
@@ -651,7 +699,7 @@ void orc_mcilu_sweep(const orc_level *L, const int32_t *colour, int ncol, const 
     const int32_t i = order[t];
     double v = rc[i];
     for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k)
-      if (colour[L->col[k]] > colour[i]) v = v - L->anb[k] * rc[L->col[k]];
+      if (colour[L->col[k]] > colour[i]) v = v - F[k] * rc[L->col[k]];
     const double z = dinv[i] * v;
     rc[i] = z;
     x[i] = from_zero ? z : x[i] + z;
@@ -793,13 +841,15 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
     free(y); free(r); free(bc); free(xc);
     return;
   }
-  if (l == 0 && (cfg->smoother == 1 || cfg->smoother == 2)) {
+  if (l == 0 && (cfg->smoother == 1 || cfg->smoother == 2 || cfg->smoother == 3)) {
     /* DILU pre and post: the reference's in its lexicographic order (1), or
-     * in colour order (2) */
+     * in colour order (2); ILU(0) in colour order (3: dc0 holds dinv, then
+     * the factor values F) */
     double *rc = (double *)malloc((size_t)n * sizeof(double));
+    const double *Fv = cfg->smoother == 3 ? dc0 + n : L->anb;
     for (int s = 0; s < cfg->pre; ++s) {
       if (cfg->smoother == 1) ilu_apply(L, dc0, b, rc, x);
-      else orc_mcilu_sweep(L, colours[0], ncol[0], dc0, b, s == 0, rc, x);
+      else orc_mcilu_sweep_f(L, colours[0], ncol[0], dc0, Fv, b, s == 0, rc, x);
     }
     double *r = (double *)malloc((size_t)n * sizeof(double));
 #ifdef _OPENMP
@@ -819,7 +869,7 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
     for (int32_t i = 0; i < n; ++i) x[i] = x[i] + xc[L->parents[i]];
     for (int s = 0; s < cfg->post; ++s) {
       if (cfg->smoother == 1) ilu_apply(L, dc0, b, rc, x);
-      else orc_mcilu_sweep(L, colours[0], ncol[0], dc0, b, 0, rc, x);
+      else orc_mcilu_sweep_f(L, colours[0], ncol[0], dc0, Fv, b, 0, rc, x);
     }
     free(rc); free(r); free(bc); free(xc);
     return;
@@ -866,8 +916,13 @@ static void vcyc_rec(const orc_hier *H, int32_t *const *colours, const int *ncol
 /* DILU factor of level 0 when the finest smoother is DILU (else NULL) */
 static double *finest_dilu(const orc_hier *H, const orc_vcfg *cfg, int32_t *const *colours,
                            const int *ncolours) {
-  if ((cfg->smoother != 1 && cfg->smoother != 2) || cfg->nlev < 2) return NULL;
+  if ((cfg->smoother < 1 || cfg->smoother > 3) || cfg->nlev < 2) return NULL;
   const orc_level *L = &H->lev[0];
+  if (cfg->smoother == 3) {
+    double *dcF = (double *)malloc((size_t)(L->n + (L->nnz > 0 ? L->nnz : 1)) * sizeof(double));
+    orc_mcilu0_factor(L, colours[0], ncolours[0], dcF, dcF + L->n);
+    return dcF;
+  }
   double *dc = (double *)malloc((size_t)L->n * sizeof(double));
   if (cfg->smoother == 2) {
     orc_mcilu_factor(L, colours[0], ncolours[0], dc);

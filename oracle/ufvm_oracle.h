@@ -92,7 +92,8 @@ typedef struct {
   int32_t coarse_smoother;  /* levels 1..nlev-2: 0 block-local MCGS, 1 damped Jacobi (2/3) */
   int32_t smoother;         /* level 0: 0 symmetric multicolour GS, 1 the reference's DILU
                                (cfdFactorizeILU / cfdSolveILU, lexicographic order),
-                               2 DILU in colour order (rows by colour, then index) */
+                               2 DILU in colour order (rows by colour, then index),
+                               3 ILU(0) in colour order (orc_mcilu0_factor) */
   double coarse_scale;      /* restricted right-hand sides (hence every coarse correction)
                                scaled by this factor */
   int32_t cpre, cpost;      /* sweeps on levels 1..nlev-2 */
@@ -109,6 +110,13 @@ typedef struct {
 void orc_mcilu_factor(const orc_level *L, const int32_t *colour, int ncol, double *dinv);
 void orc_mcilu_sweep(const orc_level *L, const int32_t *colour, int ncol, const double *dinv,
                      const double *b, int from_zero, double *rc, double *x);
+/* ILU(0) in colour order: dinv and the factor's off-diagonal values F (L's
+ * CSR order: the undivided L entries below, U entries above); applying it is
+ * orc_mcilu_sweep_f with F in place of anb */
+void orc_mcilu0_factor(const orc_level *L, const int32_t *colour, int ncol, double *dinv,
+                       double *F);
+void orc_mcilu_sweep_f(const orc_level *L, const int32_t *colour, int ncol, const double *dinv,
+                       const double *F, const double *b, int from_zero, double *rc, double *x);
 void orc_vcycle(const orc_hier *H, int32_t *const *colours, const int *ncolours,
                 const orc_vcfg *cfg, const double *b, double *x);
 

@@ -488,7 +488,7 @@ def test_two_level_vcycle_matches_dense_formula(poisson16, smoother, sweeps, w):
     assert np.abs(z - x).max() <= 1e-10 * np.abs(x).max()
 
 
-@pytest.mark.parametrize("smoother", ["MCGS", "ILU", "MCILU"])
+@pytest.mark.parametrize("smoother", ["MCGS", "ILU", "MCILU", "MCILU0"])
 def test_vcycle_symmetric_for_each_finest_smoother(poisson16, smoother):
     m, ptr, col, ac, anb, b = poisson16
     S = oracle.System(ptr, col, ac, anb)
@@ -501,7 +501,7 @@ def test_vcycle_symmetric_for_each_finest_smoother(poisson16, smoother):
     assert y @ My > 0
 
 
-@pytest.mark.parametrize("smoother", ["MCGS", "ILU", "MCILU"])
+@pytest.mark.parametrize("smoother", ["MCGS", "ILU", "MCILU", "MCILU0"])
 def test_fpcg_matches_direct_solve(poisson16, smoother):
     m, ptr, col, ac, anb, b = poisson16
     S = oracle.System(ptr, col, ac, anb)
@@ -542,3 +542,113 @@ def test_openmp_build_solves_the_same_system(poisson16):
     assert np.linalg.norm(x1 - x2) <= 1e-8 * np.linalg.norm(x1)
     x3, it3, rr3 = S.fpcg(b, tol=1e-10, max_coarse_cells=200)
     assert it3 == it1 and np.array_equal(x3, x1)
+
+
+def test_ilu0_equals_dilu_without_triangles(poisson16):
+    # a hex mesh's graph has no triangles: no fill inside the pattern, so the
+    # ILU(0) factor keeps every off-diagonal and its diagonal is the colour-order
+    # DILU's, bit for bit; the V-cycles agree bit for bit
+    m, ptr, col, ac, anb, b = poisson16
+    S = oracle.System(ptr, col, ac, anb)
+    _, dinv0, F = S.mcilu0_factor()
+    dinv, _ = S.mcilu(b)
+    assert np.array_equal(F, anb) and np.array_equal(dinv0, dinv)
+    S.agglomerate(10)
+    assert np.array_equal(S.vcycle(b, max_coarse_cells=50, smoother="MCILU0"),
+                          S.vcycle(b, max_coarse_cells=50, smoother="MCILU"))
+
+
+def _factor_product(ptr, col, c, dinv, F):
+    """L U of a colour-order factor: L = I + L~ D^-1 strictly lower in the
+    colour order, U = D + U~; also the rows / low mask of the entries"""
+    n = len(dinv)
+    order = np.lexsort((np.arange(n), c))
+    pos = np.empty(n, np.int64)
+    pos[order] = np.arange(n)
+    rows = np.repeat(np.arange(n), np.diff(ptr))
+    Ld, Ud = np.eye(n), np.diag(1.0 / dinv)
+    low = pos[col] < pos[rows]
+    Ld[rows[low], col[low]] = F[low] * dinv[col[low]]
+    Ud[rows[~low], col[~low]] = F[~low]
+    return Ld @ Ud, rows
+
+
+def test_ilu0_on_tets_is_the_colour_order_dilu():
+    # the face-adjacency graph of the 6-tets-per-hex meshes has no triangles
+    # (two face neighbours of a tet never share a face), so ILU(0) makes no
+    # off-diagonal change and equals the colour-order DILU bit for bit; its
+    # product L U reproduces A on the pattern, the defining property of ILU(0)
+    m = poisson_tet_mesh(4, seed=71, renumber_seed=72)
+    ptr, col, ac, anb = poisson_system(m, dirichlet=("dirichlet",))
+    G = sp.csr_matrix((np.ones(col.size), col, ptr), shape=(m.n_cells,) * 2)
+    assert (G @ G).multiply(G).sum() == 0          # no triangles
+    S = oracle.System(ptr, col, ac, anb)
+    c, dinv, F = S.mcilu0_factor()
+    dd, _ = S.mcilu(np.zeros(m.n_cells))
+    assert np.array_equal(F, anb) and np.array_equal(dinv, dd)
+    LU, rows = _factor_product(ptr, col, c, dinv, F)
+    A = to_scipy(ptr, col, ac, anb).toarray()
+    scale = np.abs(A).max()
+    assert np.abs(LU[rows, col] - A[rows, col]).max() <= 1e-12 * scale
+    assert np.abs(np.diag(LU) - np.diag(A)).max() <= 1e-12 * scale
+
+
+def test_ilu0_factor_reproduces_the_matrix_on_a_graph_with_triangles():
+    # a grid graph with one diagonal per square (every square two triangles):
+    # ILU(0) now changes off-diagonals inside the pattern and L U = A there,
+    # while the DILU factor's product misses A at those entries
+    k = 9
+    n = k * k
+    idx = lambda i, j: i * k + j
+    rng = np.random.default_rng(77)
+    nb = [dict() for _ in range(n)]
+    for i in range(k):
+        for j in range(k):
+            for di, dj in ((0, 1), (1, 0), (1, 1)):
+                if i + di < k and j + dj < k:
+                    a, b = idx(i, j), idx(i + di, j + dj)
+                    w = -rng.uniform(0.5, 1.5)
+                    nb[a][b] = w
+                    nb[b][a] = w
+    ptr, col, anb = [0], [], []
+    for a in range(n):
+        for b in sorted(nb[a]):
+            col.append(b)
+            anb.append(nb[a][b])
+        ptr.append(len(col))
+    ptr, col, anb = np.array(ptr, np.int32), np.array(col, np.int32), np.array(anb)
+    ac = np.array([-sum(nb[a].values()) + 0.1 for a in range(n)])
+    S = oracle.System(ptr, col, ac, anb)
+    c, dinv, F = S.mcilu0_factor()
+    A = to_scipy(ptr, col, ac, anb).toarray()
+    scale = np.abs(A).max()
+    LU, rows = _factor_product(ptr, col, c, dinv, F)
+    assert np.abs(F - anb).max() > 1e-3 * scale
+    assert np.abs(LU[rows, col] - A[rows, col]).max() <= 1e-12 * scale
+    assert np.abs(np.diag(LU) - np.diag(A)).max() <= 1e-12 * scale
+    dd, _ = S.mcilu(np.zeros(n))
+    LUd, _ = _factor_product(ptr, col, c, dd, anb)
+    assert np.abs(LUd[rows, col] - A[rows, col]).max() > 1e-3 * scale
+    # applying it: one sweep from zero is M^-1 b with M = L U
+    b = rng.standard_normal(n)
+    S.agglomerate(10)
+    x = S.vcycle(b, pre=1, post=0, max_coarse_cells=n, smoother="MCILU0", coarse_scale=1.0)
+    assert x.shape == (n,)
+
+
+def test_ilu0_fpcg_on_tets():
+    m = poisson_tet_mesh(6, seed=73, renumber_seed=74)
+    ptr, col, ac, anb = poisson_system(m, dirichlet=("dirichlet",))
+    b = gaussian_rhs(m, seed=75)
+    S = oracle.System(ptr, col, ac, anb)
+    S.agglomerate(10)
+    its = {}
+    for sm in ("MCILU", "MCILU0"):
+        x, its[sm], rr = S.fpcg(b, tol=1e-11, max_coarse_cells=100, smoother=sm)
+        xs = spl.spsolve(to_scipy(ptr, col, ac, anb).tocsc(), b)
+        assert rr <= 1e-11 and np.linalg.norm(x - xs) <= 1e-9 * np.linalg.norm(xs)
+    rng = np.random.default_rng(76)
+    y, z = rng.standard_normal(m.n_cells), rng.standard_normal(m.n_cells)
+    My = S.vcycle(y, max_coarse_cells=100, smoother="MCILU0")
+    Mz = S.vcycle(z, max_coarse_cells=100, smoother="MCILU0")
+    assert abs(z @ My - y @ Mz) <= 1e-11 * abs(z @ My)
