@@ -519,7 +519,7 @@ def main():
     ap.add_argument("--coarse-pre", type=int, default=2,
                     help="sweeps below the finest level before restriction (-1: --pre)")
     ap.add_argument("--coarse-post", type=int, default=2)
-    ap.add_argument("--smoother", choices=["MCGS", "ILU", "MCILU"], default="MCGS",
+    ap.add_argument("--smoother", choices=["MCGS", "ILU", "MCILU", "MCILU0"], default="MCGS",
                     help="finest-level smoother of the V-cycle")
     ap.add_argument("--pre", type=int, default=1)
     ap.add_argument("--post", type=int, default=1)

@@ -178,7 +178,10 @@ typedef struct {
                                 parallel), 1 the reference's DILU ('ILU' smoother,
                                 cfdFactorizeILU.m / cfdSolveILU.m) in its lexicographic
                                 order, level-scheduled, 2 DILU in the colour order of 0
-                                (one level per colour class) */
+                                (one level per colour class), 3 ILU(0) in that colour
+                                order (the factor's off-diagonals updated inside the
+                                pattern; equal to 2 when the matrix graph has no
+                                triangles, as on hexahedral and 6-tets-per-hex meshes) */
   double coarse_scale;       /* every coarse-grid correction is scaled by this factor
                                 (1: plain; 2-2.5 suits the piecewise-constant
                                 prolongation of aggregation AMG); in (0, 4], 0 = 2.5 */

@@ -17,7 +17,7 @@ UFVM_OK = 0
 ERRORS = {1: "UFVM_ERR_ARG", 2: "UFVM_ERR_HIP", 3: "UFVM_ERR_STATE", 4: "UFVM_ERR_NUMERIC",
           5: "UFVM_ERR_UNSUPPORTED"}
 SMOOTHER = {"ILU": 0, "SOR": 1, "MCGS": 2}
-PCG_SMOOTHER = {"MCGS": 0, "ILU": 1, "MCILU": 2}
+PCG_SMOOTHER = {"MCGS": 0, "ILU": 1, "MCILU": 2, "MCILU0": 3}
 CYCLE = {"V-Cycle": 0, "W-Cycle": 1, "F-Cycle": 2}
 
 i32p = C.POINTER(C.c_int32)
@@ -321,7 +321,9 @@ class Context:
                   launch_mode="auto"):
         """smoother (finest level): "MCGS" multicolour Gauss-Seidel, "ILU" the
         reference's DILU in its lexicographic order (level-scheduled), "MCILU"
-        DILU in the colour order of "MCGS" (one dependency level per colour);
+        DILU in the colour order of "MCGS" (one dependency level per colour),
+        "MCILU0" ILU(0) in that colour order (equal to "MCILU" on graphs
+        without triangles);
         coarse_scale: factor on every coarse-grid correction; coarse_*_sweep:
         sweeps below the finest level (-1: as on the finest); the defaults
         2.5 and 2 / 2 were tuned on the 2.1M-cell hex and 5.3M-cell tet

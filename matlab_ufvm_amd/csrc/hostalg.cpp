@@ -252,4 +252,37 @@ bool lower_lists(const HostCSR &g, LowerLists &L) {
   return ok;
 }
 
+void ilu0_lists(const HostCSR &g, const LowerLists &L, Ilu0Lists &U) {
+  const int32_t n = g.n;
+  U.ptr.assign(L.nb.size() + 1, 0);
+  U.src.clear();
+  U.dst.clear();
+  // first[c]: the first slot of column c in the current row j (stamped by j)
+  std::vector<int32_t> first(n, -1), stamp(n, -1);
+  for (int32_t j = 0; j < n; ++j) {
+    for (int32_t t = g.ptr[j]; t < g.ptr[j + 1]; ++t) {
+      const int32_t c = g.col[t];
+      if (stamp[c] != j) {
+        stamp[c] = j;
+        first[c] = t;
+      }
+    }
+    for (int32_t o = L.ptr[j]; o < L.ptr[j + 1]; ++o) {
+      const int32_t k = L.nb[o];
+      for (int32_t q = g.ptr[k]; q < g.ptr[k + 1]; ++q) {
+        const int32_t c = g.col[q];
+        if (c <= k) continue;
+        if (c == j) {
+          U.src.push_back(q);
+          U.dst.push_back(-1);
+        } else if (stamp[c] == j) {
+          U.src.push_back(q);
+          U.dst.push_back(first[c]);
+        }
+      }
+      U.ptr[o + 1] = (int32_t)U.src.size();
+    }
+  }
+}
+
 }  // namespace ufvm

@@ -60,4 +60,16 @@ struct LowerLists {
 };
 bool lower_lists(const HostCSR &g, LowerLists &L);
 
+// For ILU(0) in IKJ form over a numbering whose lower neighbours are the
+// smaller indices: for every lower record o of row j (LowerLists order, k =
+// nb[o]), the entries (k, c) of row k with c > k, in entry order, that land in
+// row j -- on the diagonal (c == j) or on the first entry (j, c) of row j.
+// Entries (k, c) with no such target drop out (ILU(0) keeps no fill).
+struct Ilu0Lists {
+  std::vector<int32_t> ptr;       // [lower records + 1]
+  std::vector<int32_t> src;       // CSR slot of (k, c) in row k
+  std::vector<int32_t> dst;       // CSR slot of (j, c) in row j, -1: the diagonal
+};
+void ilu0_lists(const HostCSR &g, const LowerLists &L, Ilu0Lists &U);
+
 }  // namespace ufvm

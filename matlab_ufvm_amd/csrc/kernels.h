@@ -75,14 +75,22 @@ void dilu_bwd_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32
 // DILU in colour order on a sliced-ELL level whose colours are contiguous
 // row ranges: forward pass of the colour rows [r0, r1) (rc from b - A x and
 // the earlier colours; zero: x = 0) and backward pass (z from the later
-// colours, x += z; zero: x = z)
+// colours, x += z; zero: x = z).  ILU(0): lval = the factor's off-diagonals
+// (the lower sum; nullptr for DILU, where they are A's), and the backward
+// pass gets the factor as val.
 template <typename V>
 void mcilu_fwd_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const int32_t *col,
-                    const V *val, const V *ac, const V *b, const V *x, const V *dinv, V *rc,
-                    bool zero);
+                    const V *val, const V *lval, const V *ac, const V *b, const V *x,
+                    const V *dinv, V *rc, bool zero);
 template <typename V>
 void mcilu_bwd_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const int32_t *col,
                     const V *val, const V *dinv, V *rc, V *x, bool zero);
+// ILU(0) factorisation of the rows [r0, r1) of one colour class (the factor's
+// off-diagonals overwrite F, which holds A's on entry; update lists Ilu0Lists)
+void ilu0_factor_rows(hipStream_t s, int32_t r0, int32_t r1, const double *ac,
+                      const int32_t *lptr, const int32_t *lnb, const int32_t *lpji,
+                      const int32_t *hptr, const int32_t *hsrc, const int32_t *hdst, double *F,
+                      double *dinv);
 // DILU factorisation of the listed rows (pull form of cfdFactorizeILU)
 void dilu_factor_rows(hipStream_t s, const int32_t *rows, int32_t count, const double *ac,
                       const double *anb, const int32_t *lptr, const int32_t *lnb,

@@ -425,6 +425,43 @@ void dilu_factor_rows(hipStream_t s, const int32_t *rows, int32_t count, const d
                      lptr, lnb, lpji, lpij, skip_row, dinv);
 }
 
+// ILU(0) (IKJ form) of the rows [r0, r1) of one colour class, in a numbering
+// where earlier colours are smaller indices: F holds A's off-diagonals on
+// entry and the factor's on return (L~ = strictly lower part, U~ = strictly
+// upper part, M = (D + L~) D^-1 (D + U~)).  For each lower neighbour k of j
+// in increasing order, l = F_jk dinv_k, then every listed entry (k, c), c > k,
+// updates the diagonal (c == j) or F_jc.  Rows of one colour never read each
+// other's values, and rows of earlier colours are final.
+__global__ void __launch_bounds__(kBlock)
+k_ilu0_factor(int32_t r0, int32_t r1, const double *__restrict__ ac,
+              const int32_t *__restrict__ lptr, const int32_t *__restrict__ lnb,
+              const int32_t *__restrict__ lpji, const int32_t *__restrict__ hptr,
+              const int32_t *__restrict__ hsrc, const int32_t *__restrict__ hdst, double *F,
+              double *dinv) {
+  const int32_t j = r0 + blockIdx.x * kBlock + threadIdx.x;
+  if (j >= r1) return;
+  double d = ac[j];
+  for (int32_t o = lptr[j]; o < lptr[j + 1]; ++o) {
+    const double l = F[lpji[o]] * dinv[lnb[o]];
+    for (int32_t h = hptr[o]; h < hptr[o + 1]; ++h) {
+      const int32_t t = hdst[h];
+      const double u = F[hsrc[h]];
+      if (t < 0) d = d - l * u;
+      else F[t] = F[t] - l * u;
+    }
+  }
+  dinv[j] = 1.0 / d;
+}
+
+void ilu0_factor_rows(hipStream_t s, int32_t r0, int32_t r1, const double *ac,
+                      const int32_t *lptr, const int32_t *lnb, const int32_t *lpji,
+                      const int32_t *hptr, const int32_t *hsrc, const int32_t *hdst, double *F,
+                      double *dinv) {
+  if (r1 <= r0) return;
+  hipLaunchKernelGGL(k_ilu0_factor, dim3(nblocks(r1 - r0)), dim3(kBlock), 0, s, r0, r1, ac, lptr,
+                     lnb, lpji, hptr, hsrc, hdst, F, dinv);
+}
+
 // rc_j -= a_ji * (dinv_i * rc_i) for lower i in increasing order
 template <typename V>
 __device__ __forceinline__ void dilu_fwd_row(int32_t j, const V *__restrict__ anb,
@@ -739,12 +776,16 @@ void mcgs_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const 
 // Backward pass over [r0, r1), colours downward:
 //   z_i = dinv_i (rc_i - sum_{k >= r1} a_ik rc_k),  rc_i = z_i,  x_i += z_i
 // (ZERO: x_i = z_i).  Padding entries (col = the row) fall in neither sum.
-template <typename V, bool ZERO>
+// For ILU(0) the sums over lower / upper neighbours use the factor's values
+// instead of A's: SEP takes the lower ones from lval while the residual keeps
+// val (A); with ZERO, or for the backward pass, the caller passes the factor
+// as val.
+template <typename V, bool ZERO, bool SEP>
 __global__ void __launch_bounds__(kBlock)
 k_mcilu_fwd_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off,
                  const int32_t *__restrict__ col, const V *__restrict__ val,
-                 const V *__restrict__ ac, const V *__restrict__ b, const V *__restrict__ x,
-                 const V *__restrict__ dinv, V *rc) {
+                 const V *__restrict__ lval, const V *__restrict__ ac, const V *__restrict__ b,
+                 const V *__restrict__ x, const V *__restrict__ dinv, V *rc) {
   const int32_t i = (r0 & ~63) + xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i < r0 || i >= r1) return;
   const int32_t q = i >> 6;
@@ -754,11 +795,12 @@ k_mcilu_fwd_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off,
   int32_t k = 0;
   for (; k + 4 <= w; k += 4) {
     int32_t j[4];
-    V a[4], xj[4], lj[4];
+    V a[4], f[4], xj[4], lj[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       j[u] = col[base + (k + u) * 64];
       a[u] = val[base + (k + u) * 64];
+      f[u] = SEP ? lval[base + (k + u) * 64] : a[u];
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
@@ -768,14 +810,15 @@ k_mcilu_fwd_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off,
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       if (!ZERO) res = res - a[u] * xj[u];
-      if (j[u] < r0) low = low + a[u] * lj[u];
+      if (j[u] < r0) low = low + f[u] * lj[u];
     }
   }
   for (; k < w; ++k) {
     const int32_t j = col[base + k * 64];
     const V a = val[base + k * 64];
+    const V f = SEP ? lval[base + k * 64] : a;
     if (!ZERO) res = res - a * x[j];
-    if (j < r0) low = low + a * (dinv[j] * rc[j]);
+    if (j < r0) low = low + f * (dinv[j] * rc[j]);
   }
   rc[i] = res - low;
 }
@@ -816,16 +859,21 @@ k_mcilu_bwd_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off,
 
 template <typename V>
 void mcilu_fwd_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const int32_t *col,
-                    const V *val, const V *ac, const V *b, const V *x, const V *dinv, V *rc,
-                    bool zero) {
+                    const V *val, const V *lval, const V *ac, const V *b, const V *x,
+                    const V *dinv, V *rc, bool zero) {
   if (r1 <= r0) return;
   const int32_t a = r0 & ~63;
+  // from zero only the lower sum reads values
+  const V *v0 = (zero && lval) ? lval : val;
   if (zero)
-    hipLaunchKernelGGL((k_mcilu_fwd_sell<V, true>), dim3(nblocks(r1 - a)), dim3(kBlock), 0, s, r0,
-                       r1, off, col, val, ac, b, x, dinv, rc);
+    hipLaunchKernelGGL((k_mcilu_fwd_sell<V, true, false>), dim3(nblocks(r1 - a)), dim3(kBlock), 0,
+                       s, r0, r1, off, col, v0, v0, ac, b, x, dinv, rc);
+  else if (lval && lval != val)
+    hipLaunchKernelGGL((k_mcilu_fwd_sell<V, false, true>), dim3(nblocks(r1 - a)), dim3(kBlock), 0,
+                       s, r0, r1, off, col, val, lval, ac, b, x, dinv, rc);
   else
-    hipLaunchKernelGGL((k_mcilu_fwd_sell<V, false>), dim3(nblocks(r1 - a)), dim3(kBlock), 0, s,
-                       r0, r1, off, col, val, ac, b, x, dinv, rc);
+    hipLaunchKernelGGL((k_mcilu_fwd_sell<V, false, false>), dim3(nblocks(r1 - a)), dim3(kBlock),
+                       0, s, r0, r1, off, col, val, val, ac, b, x, dinv, rc);
 }
 
 template <typename V>
@@ -843,11 +891,11 @@ void mcilu_bwd_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, c
 
 template void mcilu_fwd_sell<float>(hipStream_t, int32_t, int32_t, const int32_t *,
                                     const int32_t *, const float *, const float *, const float *,
-                                    const float *, const float *, float *, bool);
+                                    const float *, const float *, const float *, float *, bool);
 template void mcilu_fwd_sell<double>(hipStream_t, int32_t, int32_t, const int32_t *,
                                      const int32_t *, const double *, const double *,
-                                     const double *, const double *, const double *, double *,
-                                     bool);
+                                     const double *, const double *, const double *,
+                                     const double *, double *, bool);
 template void mcilu_bwd_sell<float>(hipStream_t, int32_t, int32_t, const int32_t *,
                                     const int32_t *, const float *, const float *, float *,
                                     float *, bool);

@@ -21,7 +21,8 @@ namespace detail {
 // kind 2: the coarsest level, kept in reference order for the dense solve.
 // kind 3 (coarse levels smoothed by damped Jacobi): rows sorted by length
 // inside windows.  kind 5 (finest level, DILU in colour order): kind 0's
-// numbering.  Rows keep their cconn order in every kind.
+// numbering; kind 6 (ILU(0) in colour order) the same.  Rows keep their cconn
+// order in every kind.
 void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
   hipStream_t s = ctx->stream;
   RefLevel &R = *S.lev[l];
@@ -33,7 +34,7 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
   FL.nnz = R.nnz;
   FL.kind = kind;
   FL.perm.resize(n);
-  if (kind == 0 || kind == 5) {
+  if (kind == 0 || kind == 5 || kind == 6) {
     FL.ncol = greedy_colour(R.g, FL.colour);
     FL.col_off.assign(FL.ncol + 1, 0);
     for (int32_t i = 0; i < n; ++i) ++FL.col_off[FL.colour[i] + 1];
@@ -107,7 +108,7 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
     FL.dinv64.alloc(n);
     FL.dinv32.alloc(n);
   }
-  if (kind == 5) {
+  if (kind == 5 || kind == 6) {
     // lower = earlier colour = smaller permuted index (a colour class has no
     // internal edges)
     HostCSR pg;
@@ -125,10 +126,18 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
     FL.iota.upload(iota, s);
     FL.dinv64.alloc(n);
     FL.dinv32.alloc(n);
+    if (kind == 6) {
+      Ilu0Lists U;
+      ilu0_lists(pg, low, U);
+      FL.hptr.upload(U.ptr, s);
+      FL.hsrc.upload(U.src.empty() ? std::vector<int32_t>{0} : U.src, s);
+      FL.hdst.upload(U.dst.empty() ? std::vector<int32_t>{0} : U.dst, s);
+      FL.f64.alloc(std::max(R.nnz, 1));
+    }
   }
   // the FPCG's operator product runs on level 0's sliced-ELL copy whatever its
   // kind (a single-level hierarchy makes level 0 the dense coarsest level)
-  if (kind == 0 || kind == 3 || kind == 5 || l == 0) {
+  if (kind == 0 || kind == 3 || kind == 5 || kind == 6 || l == 0) {
     const int32_t nq = (n + 63) / 64;
     FL.h_soff.assign(nq + 1, 0);
     for (int32_t q = 0; q < nq; ++q) {
@@ -140,7 +149,7 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
     const size_t tot = std::max(FL.h_soff[nq], 1);
     // padding: col = the row itself, value 0 (source -1; rows past n in the
     // last chunk are never processed)
-    std::vector<int32_t> scol(tot, 0), ssrc(tot, -1);
+    std::vector<int32_t> scol(tot, 0), ssrc(tot, -1), fsrc(tot, -1);
     for (int32_t t = 0; t < n; ++t) {
       const int32_t q = t / 64, base = FL.h_soff[q] + (t & 63);
       const int32_t w = (FL.h_soff[q + 1] - FL.h_soff[q]) / 64;
@@ -148,6 +157,7 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
       for (int32_t e = 0; e < w; ++e) {
         scol[base + e * 64] = e < len ? col[FL.h_ptr[t] + e] : t;
         ssrc[base + e * 64] = e < len ? csrc[FL.h_ptr[t] + e] : -1;
+        fsrc[base + e * 64] = e < len ? FL.h_ptr[t] + e : -1;
       }
     }
     FL.soff.upload(FL.h_soff, s);
@@ -155,6 +165,11 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
     FL.sell_src.upload(ssrc, s);
     FL.sval64.alloc(tot);
     FL.sval32.alloc(tot);
+    if (kind == 6) {
+      FL.fsrc.upload(fsrc, s);
+      FL.sf64.alloc(tot);
+      FL.sf32.alloc(tot);
+    }
   }
   UFVM_HIP(hipStreamSynchronize(s));
 }
@@ -201,6 +216,18 @@ void refresh_level(ufvm_ctx *ctx, System &S, int l) {
                           FL.ac64.p, FL.anb64.p, FL.lptr.p, FL.lnb.p, FL.lpji.p, FL.lpij.p, -1,
                           FL.dinv64.p);
     k::convert<double, float>(s, n, FL.dinv64.p, FL.dinv32.p);
+  }
+  if (FL.kind == 6) {
+    // ILU(0) in colour order, FP64, one launch per colour; then the factor's
+    // off-diagonals into the sliced-ELL layout in both precisions
+    if (FL.nnz > 0) k::convert<double, double>(s, FL.nnz, FL.anb64.p, FL.f64.p);
+    for (int c = 0; c < FL.ncol; ++c)
+      k::ilu0_factor_rows(s, FL.col_off[c], FL.col_off[c + 1], FL.ac64.p, FL.lptr.p, FL.lnb.p,
+                          FL.lpji.p, FL.hptr.p, FL.hsrc.p, FL.hdst.p, FL.f64.p, FL.dinv64.p);
+    k::convert<double, float>(s, n, FL.dinv64.p, FL.dinv32.p);
+    const int64_t slots = FL.h_soff.back();
+    k::gather_vals<double>(s, slots, FL.fsrc.p, FL.f64.p, FL.sf64.p);
+    k::gather_vals<float>(s, slots, FL.fsrc.p, FL.f64.p, FL.sf32.p);
   }
 }
 
@@ -250,6 +277,7 @@ struct Prec<float> {
   static const float *anb(const FastLevel &L) { return L.anb32.p; }
   static const float *sval(const FastLevel &L) { return L.sval32.p; }
   static const float *dinv(const FastLevel &L) { return L.dinv32.p; }
+  static const float *sfac(const FastLevel &L) { return L.sf32.p; }
   static float *b(LevelWork &W) { return W.b32.p; }
   static float *x(LevelWork &W) { return W.x32.p; }
   static float *y(LevelWork &W) { return W.y32.p; }
@@ -264,6 +292,7 @@ struct Prec<double> {
   static const double *anb(const FastLevel &L) { return L.anb64.p; }
   static const double *sval(const FastLevel &L) { return L.sval64.p; }
   static const double *dinv(const FastLevel &L) { return L.dinv64.p; }
+  static const double *sfac(const FastLevel &L) { return L.sf64.p; }
   static double *b(LevelWork &W) { return W.b64.p; }
   static double *x(LevelWork &W) { return W.x64.p; }
   static double *y(LevelWork &W) { return W.y64.p; }
@@ -345,17 +374,19 @@ struct Cycle {
                           P::anb(L), P::dinv(L), rc, x);
   }
 
-  // One sweep of DILU in colour order on a kind-5 level: a forward pass over
-  // the colours upward (the residual b - A x fused in; from zero: x = 0) and
-  // a backward pass over the colours downward (x += z; from zero: x = z), one
-  // launch per colour each way on the sliced-ELL copy.
+  // One sweep of DILU (kind 5) or ILU(0) (kind 6) in colour order: a forward
+  // pass over the colours upward (the residual b - A x fused in; from zero:
+  // x = 0) and a backward pass over the colours downward (x += z; from zero:
+  // x = z), one launch per colour each way on the sliced-ELL copy.  ILU(0)'s
+  // substitutions read the factor's off-diagonals, the residual A's.
   void mc_dilu_sweep(const FastLevel &L, LevelWork &LW, V *x, bool from_zero) {
     V *rc = P::r(LW);
+    const V *fac = L.kind == 6 ? P::sfac(L) : P::sval(L);
     for (int c = 0; c < L.ncol; ++c)
       k::mcilu_fwd_sell<V>(s, L.col_off[c], L.col_off[c + 1], L.soff.p, L.scol.p, P::sval(L),
-                           P::ac(L), P::b(LW), x, P::dinv(L), rc, from_zero);
+                           fac, P::ac(L), P::b(LW), x, P::dinv(L), rc, from_zero);
     for (int c = L.ncol - 1; c >= 0; --c)
-      k::mcilu_bwd_sell<V>(s, L.col_off[c], L.col_off[c + 1], L.soff.p, L.scol.p, P::sval(L),
+      k::mcilu_bwd_sell<V>(s, L.col_off[c], L.col_off[c + 1], L.soff.p, L.scol.p, fac,
                            P::dinv(L), rc, x, from_zero);
   }
 
@@ -427,7 +458,7 @@ struct Cycle {
       }
       return x;
     }
-    if (L.kind == 4 || L.kind == 5) {
+    if (L.kind == 4 || L.kind == 5 || L.kind == 6) {
       const RefLevel &R = *S.lev[l];
       auto sweep = [&](bool from_zero) {
         if (L.kind == 4) dilu_sweep(R, L, LW, x, from_zero);
@@ -572,7 +603,8 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
     throw Error(UFVM_ERR_ARG, "max_coarse_cells must be in [1, 4096]");
   if (cfg.coarse_smoother != 0 && cfg.coarse_smoother != 1)
     throw Error(UFVM_ERR_ARG, "coarse_smoother must be 0 or 1");
-  if (cfg.smoother < 0 || cfg.smoother > 2) throw Error(UFVM_ERR_ARG, "smoother must be 0, 1 or 2");
+  if (cfg.smoother < 0 || cfg.smoother > 3)
+    throw Error(UFVM_ERR_ARG, "smoother must be 0, 1, 2 or 3");
   const double cscale = cfg.coarse_scale == 0.0 ? 2.5 : cfg.coarse_scale;   // 0: default
   if (!(cscale > 0.0 && cscale <= 4.0))
     throw Error(UFVM_ERR_ARG, "coarse_scale must be in (0, 4] (0: the default 2.5)");
@@ -617,7 +649,10 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
   for (int32_t l = 0; l < nlev; ++l)
     build_fast_level(ctx, S, l,
                      l == nlev - 1 ? 2
-                                   : (l == 0 ? (cfg.smoother == 1 ? 4 : cfg.smoother == 2 ? 5 : 0)
+                                   : (l == 0 ? (cfg.smoother == 1   ? 4
+                                                : cfg.smoother == 2 ? 5
+                                                : cfg.smoother == 3 ? 6
+                                                                    : 0)
                                              : (cfg.coarse_smoother == 1 ? 3 : 1)));
   for (int32_t l = 0; l + 1 < nlev; ++l) link_levels(ctx, S, l);
   for (int32_t l = 0; l < nlev; ++l) refresh_level(ctx, S, l);

@@ -145,6 +145,8 @@ struct FastLevel {
   // kind 5: finest level smoothed by DILU in colour order: kind 0's
   //         numbering, so a colour class is one level of the lower-triangular
   //         dependency graph
+  // kind 6: finest level smoothed by ILU(0) in colour order: kind 5's
+  //         numbering and lists, plus the factor's own off-diagonal values
   int kind = 0;
   std::vector<int32_t> col_off;   // kind 0: colour c rows are [col_off[c], col_off[c+1])
   int32_t nblk = 0, bncol = 0;    // kind 1: blocks and colours per block
@@ -172,6 +174,12 @@ struct FastLevel {
   // kind 5: row numbers 0..n-1 (the row lists of the colour levels) and the
   // lower neighbour lists of the permuted pattern (lower_lists)
   DBuf<int32_t> iota, lptr, lnb, lpji, lpij;
+  // kind 6: the ILU(0) factor's off-diagonals in the permuted CSR layout
+  // (f64, factored in FP64) and in the sliced-ELL layout (sf64, sf32; slot ->
+  // CSR position fsrc, -1 padding), and the update lists (Ilu0Lists)
+  DBuf<double> f64, sf64;
+  DBuf<float> sf32;
+  DBuf<int32_t> fsrc, hptr, hsrc, hdst;
 };
 
 // Per-solve device workspace of one system: each level's right-hand side,

@@ -1,9 +1,10 @@
 // hostalg_check.cpp -- the product's host graph algorithms (hostalg.cpp:
 // agglomeration, Galerkin gather lists, colourings, level schedules, DILU
-// lower lists) on seeded graphs, built with AddressSanitizer and
+// lower lists, ILU(0) update lists) on seeded graphs, built with AddressSanitizer and
 // UndefinedBehaviorSanitizer by tests/test_sanitizers.py.  Every result is
 // checked against its invariants and the agglomeration against the oracle's
-// (oracle/ufvm_oracle.c, compiled into this program); exit status 0 = pass.
+// (oracle/ufvm_oracle.c, compiled into this program), the ILU(0) factor that
+// the update lists produce against the oracle's; exit status 0 = pass.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -28,9 +29,12 @@ static int failures = 0;
     }                                                                      \
   } while (0)
 
-// n1 x n2 x n3 grid, 7-point stencil, rows numbered by a seeded permutation,
-// neighbours in increasing index order; anb < 0 with seeded magnitudes
-static void grid(int n1, int n2, int n3, unsigned seed, HostCSR &g, std::vector<double> &anb) {
+// n1 x n2 x n3 grid, 7-point stencil (tri: plus the (i+1, j+1) diagonal of
+// every square, so the graph has triangles), rows numbered by a seeded
+// permutation, neighbours in increasing index order; anb < 0 with seeded
+// magnitudes
+static void grid(int n1, int n2, int n3, unsigned seed, HostCSR &g, std::vector<double> &anb,
+                 bool tri = false) {
   const int n = n1 * n2 * n3;
   std::mt19937 rng(seed);
   std::vector<int> perm(n);
@@ -45,6 +49,10 @@ static void grid(int n1, int n2, int n3, unsigned seed, HostCSR &g, std::vector<
         if (i + 1 < n1) { nb[a].push_back(id(i + 1, j, k)); nb[id(i + 1, j, k)].push_back(a); }
         if (j + 1 < n2) { nb[a].push_back(id(i, j + 1, k)); nb[id(i, j + 1, k)].push_back(a); }
         if (k + 1 < n3) { nb[a].push_back(id(i, j, k + 1)); nb[id(i, j, k + 1)].push_back(a); }
+        if (tri && i + 1 < n1 && j + 1 < n2) {
+          nb[a].push_back(id(i + 1, j + 1, k));
+          nb[id(i + 1, j + 1, k)].push_back(a);
+        }
       }
   g.n = n;
   g.ptr.assign(1, 0);
@@ -152,6 +160,55 @@ static void check_level(const HostCSR &g, const std::vector<double> &anb) {
     }
 }
 
+// ILU(0) in index order from the update lists, in the arithmetic of the
+// factorisation kernel (k_ilu0_factor), against the oracle's ILU(0) with every
+// row its own colour (the same order): equal bit for bit
+static void check_ilu0(const HostCSR &g, const std::vector<double> &anb) {
+  LowerLists L;
+  CHECK(lower_lists(g, L));
+  Ilu0Lists U;
+  ilu0_lists(g, L, U);
+  CHECK(U.ptr.size() == L.nb.size() + 1 && U.src.size() == U.dst.size());
+  const int n = g.n;
+  std::vector<double> ac(n);
+  for (int a = 0; a < n; ++a) {
+    double s = 0.5;
+    for (int t = g.ptr[a]; t < g.ptr[a + 1]; ++t) s -= anb[t];
+    ac[a] = s;
+  }
+  std::vector<double> F(anb), dinv(n, 0.0);
+  for (int j = 0; j < n; ++j) {
+    double d = ac[j];
+    for (int o = L.ptr[j]; o < L.ptr[j + 1]; ++o) {
+      const double l = F[L.pos_ji[o]] * dinv[L.nb[o]];
+      for (int h = U.ptr[o]; h < U.ptr[o + 1]; ++h) {
+        CHECK(U.src[h] >= g.ptr[L.nb[o]] && U.src[h] < g.ptr[L.nb[o] + 1]);
+        CHECK(U.dst[h] < 0 || (U.dst[h] >= g.ptr[j] && U.dst[h] < g.ptr[j + 1]));
+        const int t = U.dst[h];
+        const double u = F[U.src[h]];
+        if (t < 0) d = d - l * u;
+        else F[t] = F[t] - l * u;
+      }
+    }
+    dinv[j] = 1.0 / d;
+  }
+  std::vector<int32_t> colour(n);
+  std::iota(colour.begin(), colour.end(), 0);
+  std::vector<double> zero(n, 0.0), Fo(anb.size()), dinvo(n);
+  orc_level O{};
+  O.n = n;
+  O.nnz = (int32_t)g.col.size();
+  O.ptr = const_cast<int32_t *>(g.ptr.data());
+  O.col = const_cast<int32_t *>(g.col.data());
+  O.ac = ac.data();
+  O.anb = const_cast<double *>(anb.data());
+  O.bc = zero.data();
+  O.dphi = zero.data();
+  orc_mcilu0_factor(&O, colour.data(), n, dinvo.data(), Fo.data());
+  for (int a = 0; a < n; ++a) CHECK(dinv[a] == dinvo[a]);
+  for (size_t t = 0; t < anb.size(); ++t) CHECK(F[t] == Fo[t]);
+}
+
 int main() {
   const int sizes[][3] = {{1, 1, 1}, {2, 1, 1}, {5, 4, 3}, {17, 9, 6}, {24, 24, 24}};
   unsigned seed = 7;
@@ -160,6 +217,10 @@ int main() {
     std::vector<double> anb;
     grid(sz[0], sz[1], sz[2], seed++, g, anb);
     check_level(g, anb);
+    check_ilu0(g, anb);
+    grid(sz[0], sz[1], sz[2], seed++, g, anb, true);
+    check_level(g, anb);
+    check_ilu0(g, anb);
   }
   std::printf("hostalg_check: %d failures\n", failures);
   return failures ? 1 : 0;

@@ -183,13 +183,17 @@ def test_pcg_vcycle_fp64_matches_oracle(ctx, case, coarse_smoother, sweeps):
     # the DILU finest smoothers (reference ordering, level-scheduled; colour
     # ordering, one level per colour)
     if coarse_smoother == 1:
-        for sm in ("ILU", "MCILU"):
+        zs = {}
+        for sm in ("ILU", "MCILU", "MCILU0"):
             ctx.pcg_setup(sid, precision=64, pre_sweep=sweeps, post_sweep=sweeps,
                           max_coarse_cells=200, coarse_smoother=1, smoother=sm)
-            zi = ctx.pcg_apply_vcycle(sid, r)
+            zi = zs[sm] = ctx.pcg_apply_vcycle(sid, r)
             zi_ref = S.vcycle(r, pre=sweeps, post=sweeps, max_coarse_cells=200,
                               coarse_smoother=1, smoother=sm)
             assert np.abs(zi - zi_ref).max() <= 1e-12 * np.abs(zi_ref).max(), sm
+        # hex and 6-tets-per-hex graphs have no triangles: ILU(0) keeps A's
+        # off-diagonals and is the colour-order DILU, bit for bit
+        assert np.array_equal(zs["MCILU0"], zs["MCILU"])
         ctx.pcg_setup(sid, precision=64, pre_sweep=sweeps, post_sweep=sweeps,
                       max_coarse_cells=200, coarse_smoother=coarse_smoother)
     # same colourings as the oracle's greedy order (level 1 always; the coarse
@@ -219,6 +223,63 @@ def test_pcg_solve_matches_direct(ctx, case, precision, tol, err, cs, sm):
     x = ctx.get_dphi(sid, 0)
     A = to_scipy(ptr, col, ac, anb)
     xs = spl.spsolve(A.tocsc(), b)
+    assert rr <= tol
+    assert np.linalg.norm(x - xs) <= err * np.linalg.norm(xs), (it, rr)
+    ctx.system_destroy(sid)
+
+
+def triangulated_grid(k, seed):
+    """k x k grid graph with one diagonal per square (every square two
+    triangles): symmetric negative off-diagonals, rows numbered by a seeded
+    permutation, diagonal = 0.05 + sum |a_ij| (CSR, neighbours increasing)"""
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(k * k)
+    nb = [dict() for _ in range(k * k)]
+    for i in range(k):
+        for j in range(k):
+            for di, dj in ((0, 1), (1, 0), (1, 1)):
+                if i + di < k and j + dj < k:
+                    a, b = perm[i * k + j], perm[(i + di) * k + j + dj]
+                    w = -rng.uniform(0.5, 1.5)
+                    nb[a][b] = nb[b][a] = w
+    ptr, col, anb = [0], [], []
+    for a in range(k * k):
+        for b in sorted(nb[a]):
+            col.append(b)
+            anb.append(nb[a][b])
+        ptr.append(len(col))
+    ac = np.array([0.05 - sum(nb[a].values()) for a in range(k * k)])
+    return np.array(ptr, np.int32), np.array(col, np.int32), ac, np.array(anb)
+
+
+@pytest.mark.parametrize("precision,tol,err", [(64, 1e-12, 1e-10), (32, 1e-11, 1e-8)])
+def test_ilu0_on_a_graph_with_triangles(ctx, precision, tol, err):
+    # ILU(0) updates off-diagonals inside the pattern here: its V-cycle
+    # matches the oracle's and differs from the colour-order DILU's; the FPCG
+    # it preconditions solves the system
+    ptr, col, ac, anb = triangulated_grid(48, seed=91)
+    n = len(ptr) - 1
+    b = np.random.default_rng(92).standard_normal(n)
+    print(f"\n[triangulated grid] n={n} nnz={col.size} precision={precision}", flush=True)
+    sid = make(ctx, ptr, col, ac, anb, b)
+    ctx.agglomerate(sid, 10)
+    if precision == 64:
+        S = oracle.System(ptr, col, ac, anb)
+        S.agglomerate(10)
+        zs = {}
+        for sm in ("MCILU0", "MCILU"):
+            ctx.pcg_setup(sid, precision=64, pre_sweep=1, post_sweep=1, max_coarse_cells=200,
+                          coarse_smoother=1, smoother=sm)
+            zi = zs[sm] = ctx.pcg_apply_vcycle(sid, b)
+            zi_ref = S.vcycle(b, pre=1, post=1, max_coarse_cells=200, coarse_smoother=1,
+                              smoother=sm)
+            assert np.abs(zi - zi_ref).max() <= 1e-12 * np.abs(zi_ref).max(), sm
+        assert np.abs(zs["MCILU0"] - zs["MCILU"]).max() > 1e-6 * np.abs(zs["MCILU"]).max()
+    ctx.pcg_setup(sid, precision=precision, max_coarse_cells=200, tol=tol, max_iter=300,
+                  coarse_smoother=1, smoother="MCILU0")
+    it, rr = ctx.pcg_solve(sid)
+    x = ctx.get_dphi(sid, 0)
+    xs = spl.spsolve(to_scipy(ptr, col, ac, anb).tocsc(), b)
     assert rr <= tol
     assert np.linalg.norm(x - xs) <= err * np.linalg.norm(xs), (it, rr)
     ctx.system_destroy(sid)
