@@ -326,6 +326,8 @@ class SimpleCase:
                            ("orc_solve_p", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState)]),
                            ("orc_correct_p", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState)]),
                            ("orc_gradient", [C.POINTER(OMesh), f64p, C.c_int, f64p]),
+                           ("orc_face_gradients", [C.POINTER(OMesh), f64p, f64p, C.c_int, C.c_int,
+                                                   f64p]),
                            ("orc_transient_update", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState)]),
                            ("orc_transient_step", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState),
                                                    C.c_int, C.c_double, C.c_double])]:
@@ -425,10 +427,22 @@ class SimpleCase:
         ptr = getattr(self.S, name)
         return np.ctypeslib.as_array(ptr, (int(np.prod(shapes[name])),)).reshape(shapes[name]).copy()
 
+    def face_gradients(self, phi, grad, nc=1, c=0):
+        """'Average:Corrected' gradients of the interior faces, (Fi, 3), of
+        component c of phi ((N+Fb) x nc) with cell gradients grad
+        ((N+Fb) x nc x 3)"""
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        grad = np.ascontiguousarray(grad, dtype=np.float64)
+        out = np.zeros((self.M.Fi, 3))
+        lib().orc_face_gradients(C.byref(self.M), phi.ctypes.data_as(f64p),
+                                 grad.ctypes.data_as(f64p), nc, c, out.ctypes.data_as(f64p))
+        return out
+
     def set_field(self, name, values):
         N, Fb, F = self.S.N, self.S.Fb, self.M.F
         size = {"U": (N + Fb) * 3, "p": N + Fb, "PP": N + Fb, "DU": (N + Fb) * 3,
-                "DUT": (N + Fb) * 3, "mdot": F, "U_old": (N + Fb) * 3, "mdot_old": F}[name]
+                "DUT": (N + Fb) * 3, "mdot": F, "U_old": (N + Fb) * 3, "mdot_old": F,
+                "gradp": (N + Fb) * 3, "gradU": (N + Fb) * 9}[name]
         np.ctypeslib.as_array(getattr(self.S, name), (size,))[:] = np.asarray(values).reshape(-1)
 
     def system(self):

@@ -346,6 +346,11 @@ static void grad_face_corrected(const orc_mesh *m, const double *phi, const doub
   for (int d = 0; d < 3; ++d) gf_out[d] = (gfc[d] - lam * e[d]) + lgm * e[d];
 }
 
+void orc_face_gradients(const orc_mesh *m, const double *phi, const double *grad, int nc, int c,
+                        double *out) {
+  for (int32_t f = 0; f < m->Fi; ++f) grad_face_corrected(m, phi, grad, nc, c, f, out + 3 * f);
+}
+
 /* momentum component: convection (upwind) + divergence term, diffusion,
  * -grad(p), false transient, URF (cfdAssembleEquationTerms.m:22-55) */
 void orc_assemble_u(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s, int comp) {

@@ -109,6 +109,12 @@ int orc_transient_step(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_st
  * components (cfdUpdateGradient.m / cfdComputeGradientGauss0.m) */
 void orc_gradient(const orc_mesh *m, const double *phi, int nc, double *grad);
 
+/* the 'Average:Corrected' gradient of every interior face, component c of an
+ * nc-component field (the face gradient of the diffusion correction and of
+ * the Rhie-Chow interpolation): out[f*3 + d] */
+void orc_face_gradients(const orc_mesh *m, const double *phi, const double *grad, int nc, int c,
+                        double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,11 +1,15 @@
 """Pins of the SIMPLE-step oracle (oracle/ufvm_simple_oracle.c).
 
-The reference's own output for this path is its cylinder tutorial's time
-directories (tests/test_tutorial_cylinder.py: every written value
+The reference's own output for this path is the time directories of its
+cylinder and bump tutorials (tests/test_tutorials.py: every written value
 reproduced).  Here the oracle is pinned further by closed forms on uniform
-and jittered meshes, conservation invariants, the reference's boundary
-dispatch of each correction stage, and the Re = 100 lid-driven-cavity
-benchmark of Ghia, Ghia & Shin (1982).
+and jittered meshes -- on the jittered ones the Rhie-Chow pressure terms,
+the over-relaxed p' coefficients, the 'Average:Corrected' face gradient and
+its swapped interpolation weights, the non-orthogonal diffusion correction
+and upwind convection under a prescribed mass flux, each chosen so that a
+plausible mistake in that term fails it -- conservation invariants, the
+reference's boundary dispatch of each correction stage, and the Re = 100
+lid-driven-cavity benchmark of Ghia, Ghia & Shin (1982).
 """
 import numpy as np
 import pytest
@@ -506,3 +510,179 @@ def test_transient_step_ends_at_convergence():
     assert 1 <= n < 20
     r = case.residuals()["rms"]
     assert max(r[:3]) <= 1e-2 and r[3] <= 1e-2
+
+
+# ---- closed forms on jittered (non-orthogonal, gf != 1/2) meshes -----------
+
+def _jittered_case(nu, urf_u=0.7, rho=1.0, seed=51):
+    m = cavity_mesh(8, jitter=0.15, seed=seed, renumber_seed=seed + 1)
+    case = oracle.SimpleCase(m, oracle.cavity_bcs(m), nu=nu, urf_u=urf_u, rho=rho)
+    case.update_fields()        # face diffusivities; the fields are replaced below
+    x = np.vstack([m.centroid, m.face_centroid[m.n_interior:]])
+    return m, case, x
+
+
+def test_rhie_chow_pressure_terms_cancel_for_a_linear_pressure():
+    # p = a.x with every cell gradient exactly a: the 'Average:Corrected' face
+    # gradient is a, so -rho_f geoDiff (p_N - p_O) = -rho_f a.DUE cancels
+    # rho_f a.(DUS - DUT) and mdot_f = rho_f Ubar.S + (1 - urf)(mdot_prev -
+    # rho_f Ubar.S) on every interior face (cfdAssembleMdotTerm.m:118-170)
+    rho, urf = 1.3, 0.7
+    m, case, x = _jittered_case(0.01, urf_u=urf, rho=rho)
+    N, Fi = m.n_cells, m.n_interior
+    rng = np.random.default_rng(53)
+    a = np.array([0.7, -1.1, 0.4])
+    case.set_field("p", x @ a + 0.3)
+    case.set_field("gradp", np.tile(a, (len(x), 1)))
+    U = rng.uniform(-1, 1, (len(x), 3))
+    DU = rng.uniform(0.5, 1.5, (len(x), 3)) * 1e-2
+    mprev = rng.uniform(-1, 1, m.n_faces) * 1e-2
+    case.set_field("U", U)
+    case.set_field("DU", DU)
+    case.set_field("mdot", mprev)
+    case.assemble_p()
+    md = case.field("mdot")[:Fi]
+    o, nb, g, S = m.owner[:Fi], m.neighbour, m.gf[:Fi], m.Sf[:Fi]
+    ub = g[:, None] * U[nb] + (1 - g)[:, None] * U[o]
+    UbS = (ub * S).sum(1)
+    expect = rho * UbS + (1 - urf) * (mprev[:Fi] - rho * UbS)
+    # the cancelling terms are of size rho |a| |DU S|
+    scale = rho * np.linalg.norm(a) * np.linalg.norm(DU[o] * S, axis=1).max()
+    assert np.abs(md - expect).max() <= 1e-12 * scale
+    # and they do not vanish one by one: the pressure-difference term alone
+    C1 = case.field("fluxC1")[:Fi]
+    pn = x @ a + 0.3
+    assert np.abs(C1 * (pn[o] - pn[nb])).max() > 1e-3 * scale
+
+
+def test_corrected_face_gradient_is_exact_along_cf_and_interpolates_with_swapped_weights():
+    # 'Average:Corrected' (cfdInterpolateGradientsFromElementsToInteriorFaces.m:
+    # 46-70): along e_CF the face gradient is (phi_N - phi_O)/|d_CF|, exact
+    # for a linear phi whatever the cell gradients; across e_CF it keeps the
+    # interpolated cell gradients, whose weights are swapped (:25-27: g on
+    # the owner) -- for cell gradients B x that is B at the mirror image,
+    # about the CF midpoint, of the value-interpolation point g x_N + (1-g) x_O
+    m, case, x = _jittered_case(0.01)
+    Fi = m.n_interior
+    rng = np.random.default_rng(54)
+    a = np.array([0.3, 1.7, -0.9])
+    B = rng.standard_normal((3, 3))
+    phi = x @ a - 0.4
+    G = x @ B.T
+    gf = case.face_gradients(phi, G)
+    o, nb, g = m.owner[:Fi], m.neighbour, m.gf[:Fi]
+    d = m.centroid[nb] - m.centroid[o]
+    e = d / np.linalg.norm(d, axis=1)[:, None]
+    scale = np.linalg.norm(a) + np.abs(G).max()
+    assert np.abs((gf * e).sum(1) - e @ a).max() <= 1e-13 * scale
+
+    def across(v):
+        return v - (v * e).sum(1)[:, None] * e
+
+    mirror = (1 - g)[:, None] * x[nb] + g[:, None] * x[o]
+    plain = g[:, None] * x[nb] + (1 - g)[:, None] * x[o]
+    assert np.abs(g - 0.5).max() > 0.02
+    assert np.abs(across(gf) - across(mirror @ B.T)).max() <= 1e-13 * scale
+    assert np.abs(across(gf) - across(plain @ B.T)).max() > 1e-3 * scale
+
+
+def test_diffusion_of_a_linear_velocity_balances_on_closed_cells():
+    # U_c = A_c.x with exact cell gradients: each interior face carries
+    # nu (gDiff (U_N - U_O) + grad(U)_f.T) = nu A_c.(E + T) = nu A_c.S_f
+    # (cfdAssembleDiffusionTerm.m:94-130), which sums to zero over a closed
+    # cell; with no flux, pressure or relaxation only the false transient's
+    # -V U / 1e9 stays in bc (cfdAssembleFalseTransientTerm.m)
+    nu = 0.37
+    m, case, x = _jittered_case(nu, urf_u=1.0)
+    N = m.n_cells
+    rng = np.random.default_rng(55)
+    A = rng.standard_normal((3, 3))
+    U = x @ A.T + np.array([0.2, -0.1, 0.05])
+    grad = np.repeat(A[None, :, :], len(x), axis=0)
+    case.set_field("U", U)
+    case.set_field("gradU", grad)
+    case.set_field("p", np.zeros(len(x)))
+    case.set_field("gradp", np.zeros((len(x), 3)))
+    case.set_field("mdot", np.zeros(m.n_faces))
+    inner = np.ones(N, bool)
+    inner[m.owner[m.n_interior:]] = False
+    assert inner.sum() == 6 ** 3
+    h2 = np.abs(m.Sf[:m.n_interior]).max()
+    for c in range(3):
+        case.assemble_u(c)
+        bc = case.system()["bc"]
+        expect = -(m.volume * 1.0) / 1e9 * U[:N, c]
+        scale = nu * np.linalg.norm(A[c]) * 6 * h2
+        assert np.abs(bc - expect)[inner].max() <= 1e-12 * scale, c
+        # the boundary cells see their walls: not balanced
+        assert np.abs(bc - expect)[~inner].max() > 1e-3 * scale
+
+
+def test_upwind_convection_under_a_prescribed_mass_flux():
+    # cfdAssembleConvectionTerm.m (upwind: FLUXC1 = max(mdot, 0), FLUXC2 =
+    # -max(-mdot, 0)), scattered by cfdAssembleIntoGlobalMatrixFaceFluxes.m,
+    # plus cfdAssembleConvectionDivergenceTerm.m (ac += max(div, 0) - div,
+    # bc += div U) and the false transient: with m_if the flux out of cell i
+    # through face f,
+    #   anb_ij = -max(-m_if, 0),   ac_i = sum max(m_if, 0) + max(div_i, 0) - div_i + V_i / 1e9,
+    #   bc_i = sum max(-m_if, 0) (U_j - U_i) - V_i U_i / 1e9
+    m, case, x = _jittered_case(0.0, urf_u=1.0, seed=57)
+    N, Fi = m.n_cells, m.n_interior
+    rng = np.random.default_rng(58)
+    U = rng.uniform(-1, 1, (len(x), 3))
+    mdot = np.zeros(m.n_faces)
+    mdot[:Fi] = rng.uniform(-1, 1, Fi) * 1e-2
+    case.set_field("U", U)
+    case.set_field("gradU", np.zeros((len(x), 3, 3)))
+    case.set_field("p", np.zeros(len(x)))
+    case.set_field("gradp", np.zeros((len(x), 3)))
+    case.set_field("mdot", mdot)
+    o, nb, md = m.owner[:Fi], m.neighbour, mdot[:Fi]
+    anb = np.zeros(m.cconn.size)
+    anb[m.own_nnz] = -np.maximum(-md, 0)
+    anb[m.nei_nnz] = -np.maximum(md, 0)
+    div = np.zeros(N)
+    np.add.at(div, o, md)
+    np.add.at(div, nb, -md)
+    out = np.zeros(N)
+    np.add.at(out, o, np.maximum(md, 0))
+    np.add.at(out, nb, np.maximum(-md, 0))
+    ac = out + np.maximum(div, 0) - div + m.volume / 1e9
+    for c in range(3):
+        inflow = np.zeros(N)
+        np.add.at(inflow, o, np.maximum(-md, 0) * (U[nb, c] - U[o, c]))
+        np.add.at(inflow, nb, np.maximum(md, 0) * (U[o, c] - U[nb, c]))
+        bc = inflow - m.volume / 1e9 * U[:N, c]
+        case.assemble_u(c)
+        S = case.system()
+        assert np.array_equal(S["anb"], anb)
+        assert np.abs(S["ac"] - ac).max() <= 1e-15 * 6e-2
+        assert np.abs(S["bc"] - bc).max() <= 1e-15 * 6e-2 * 2
+
+
+def test_pressure_correction_coefficients_with_isotropic_du_are_over_relaxed():
+    # with DU_x = DU_y = DU_z in every cell, DUS = du_f S_f and the
+    # over-relaxed split DUE = |DUS| / (e_DUS.e_CF) e_CF
+    # (cfdAssembleMdotTerm.m:144-165) makes the p' coefficient
+    # rho |DUE| / |d_CF| = rho du_f |S_f|^2 / (S_f.d_CF) -- not the
+    # minimum-correction |S_f| / |d_CF| of the diffusion term's gDiff; mdot
+    # itself does not depend on |DUE| (the corrected face gradient cancels it)
+    rho = 1.3
+    m, case, x = _jittered_case(0.01, rho=rho, seed=59)
+    Fi = m.n_interior
+    rng = np.random.default_rng(60)
+    du = rng.uniform(0.5, 1.5, len(x)) * 1e-2
+    case.set_field("DU", np.repeat(du[:, None], 3, axis=1))
+    case.set_field("U", rng.uniform(-1, 1, (len(x), 3)))
+    case.set_field("mdot", rng.uniform(-1, 1, m.n_faces) * 1e-2)
+    case.assemble_p()
+    g, o, nb = m.gf[:Fi], m.owner[:Fi], m.neighbour
+    duf = g * du[nb] + (1 - g) * du[o]
+    S = m.Sf[:Fi]
+    d = m.centroid[nb] - m.centroid[o]
+    expect = rho * duf * (S * S).sum(1) / (S * d).sum(1)
+    C1, C2 = case.field("fluxC1")[:Fi], case.field("fluxC2")[:Fi]
+    assert np.abs(C1 - expect).max() <= 1e-13 * expect.max()
+    assert np.array_equal(C2, -C1)
+    # non-orthogonal faces: the minimum-correction coefficient differs
+    assert np.abs(rho * duf * m.gDiff[:Fi] - expect).max() > 1e-3 * expect.max()
