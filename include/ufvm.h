@@ -348,6 +348,11 @@ int ufvm_simple_iterate(ufvm_ctx *ctx, int32_t cas, int32_t n, double *rms);
  * for 4 * max_inner) their residuals. */
 int ufvm_transient_step(ufvm_ctx *ctx, int32_t cas, int32_t max_inner, double u_tol,
                         double p_tol, double *rms, int32_t *n_inner);
+/* cfdTransientUpdate (uFVM/src/fvm/update/cfdTransientUpdate.m) on the path's
+ * fields alone: U and mdot_f become the previous step's "U_old" and
+ * "mdot_old" (the first part of ufvm_transient_step, for a caller that runs
+ * the time loop itself, as cfdRunCase.m:24-86 does through the .m shims). */
+int ufvm_transient_update(ufvm_ctx *ctx, int32_t cas);
 
 /* --------------------------------------------------------- serving form */
 /* No reference counterpart.  A single-GPU request loop over resident meshes:

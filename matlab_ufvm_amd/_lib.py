@@ -120,6 +120,7 @@ _SIGS = {
     "ufvm_simple_iterate": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, f64p]),
     "ufvm_transient_step": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                       f64p, i32p]),
+    "ufvm_transient_update": (C.c_int, [C.c_void_p, C.c_int32]),
     "ufvm_server_create": (C.c_int, [C.c_int32, C.POINTER(ServerCfg), C.POINTER(C.c_void_p)]),
     "ufvm_server_destroy": (C.c_int, [C.c_void_p]),
     "ufvm_server_last_error": (C.c_char_p, [C.c_void_p]),

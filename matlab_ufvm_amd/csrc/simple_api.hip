@@ -545,17 +545,29 @@ int ufvm_simple_iterate(ufvm_ctx *ctx, int32_t cas, int32_t n, double *rms) {
   });
 }
 
+// cfdTransientUpdate: the previous step's fields ('Step1')
+static void transient_update(ufvm_ctx *ctx, Case &C) {
+  UFVM_HIP(hipMemcpyAsync(C.U_old.p, C.U.p, (size_t)(C.N + C.Fb) * 3 * sizeof(double),
+                          hipMemcpyDeviceToDevice, ctx->stream));
+  UFVM_HIP(hipMemcpyAsync(C.mdot_old.p, C.mdot.p, (size_t)C.F * sizeof(double),
+                          hipMemcpyDeviceToDevice, ctx->stream));
+}
+
+int ufvm_transient_update(ufvm_ctx *ctx, int32_t cas) {
+  return guard(ctx, [&] {
+    Case &C = case_of(ctx, cas);
+    if (!C.cfg.transient) throw Error(UFVM_ERR_STATE, "the case was not created transient");
+    transient_update(ctx, C);
+  });
+}
+
 int ufvm_transient_step(ufvm_ctx *ctx, int32_t cas, int32_t max_inner, double u_tol,
                         double p_tol, double *rms, int32_t *n_inner) {
   return guard(ctx, [&] {
     Case &C = case_of(ctx, cas);
     if (!C.cfg.transient) throw Error(UFVM_ERR_STATE, "the case was not created transient");
     if (max_inner < 1) throw Error(UFVM_ERR_ARG, "max_inner must be >= 1");
-    // cfdTransientUpdate: the previous step's fields ('Step1')
-    UFVM_HIP(hipMemcpyAsync(C.U_old.p, C.U.p, (size_t)(C.N + C.Fb) * 3 * sizeof(double),
-                            hipMemcpyDeviceToDevice, ctx->stream));
-    UFVM_HIP(hipMemcpyAsync(C.mdot_old.p, C.mdot.p, (size_t)C.F * sizeof(double),
-                            hipMemcpyDeviceToDevice, ctx->stream));
+    transient_update(ctx, C);
     int32_t it = 0;
     while (it < max_inner) {
       double r[4];

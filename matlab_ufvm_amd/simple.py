@@ -156,6 +156,10 @@ class SimpleCase:
                 self.cfdCorrectEquation(name, c)
         return rms, mx
 
+    def cfdTransientUpdate(self):
+        """U, mdot_f -> the previous step's U_old, mdot_old (cfdTransientUpdate.m)"""
+        self.ctx.check(self.ctx.lib.ufvm_transient_update(self.ctx.h, self.id))
+
     def transient_step(self, max_inner=20, u_tol=1e-6, p_tol=1e-6):
         """one time step (cfdRunCase.m:24-86): (iterations run, RMS residuals [it, 4])"""
         rms = np.zeros((max_inner, 4))

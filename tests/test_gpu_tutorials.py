@@ -244,4 +244,13 @@ def test_transient_steps_bitexact(ctx, which):
         assert ng == no and np.all(np.isfinite(rg))
         for name, oname in (("U", "U"), ("p", "p"), ("mdot_f", "mdot")):
             same(g.field(name), o.field(oname), f"time step {name}")
+    # a time step driven by the caller, as cfdRunCase does through the .m
+    # shims: the transient update, then outer iterations one call each
+    g.cfdTransientUpdate()
+    o.transient_update()
+    g.iterate(2)
+    o.iterate(2)
+    for name, oname in (("U_old", "U_old"), ("mdot_old", "mdot_old"), ("U", "U"), ("p", "p"),
+                        ("mdot_f", "mdot")):
+        same(g.field(name), o.field(oname), f"caller-driven step {name}")
     g.close()
