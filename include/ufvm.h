@@ -207,6 +207,19 @@ int ufvm_pcg_solve(ufvm_ctx *ctx, int32_t sys, int32_t *iters, double *relres);
  * Needs the multicolour GS finest smoother and Jacobi coarse levels. */
 int ufvm_pcg_solve_batch(ufvm_ctx *ctx, int32_t sys, int32_t k, const double *b, double *x,
                          int32_t *iters, double *relres);
+/* g solves of g different systems of the context at once (no reference
+ * counterpart: the serving form's grouped launches, exposed for callers and
+ * tests).  b[q] / x[q]: host arrays of system sys[q]'s size, level-1 order.
+ * The solves advance in lock step, one FCG iteration at a time, and each
+ * phase of an iteration -- every colour pass, residual, restriction, Jacobi
+ * sweep, prolongation and dense solve of the V-cycles and every FCG kernel --
+ * is one launch serving all of them.  Each solution, iteration count and
+ * relative residual is the one ufvm_pcg_solve gives for that system and
+ * right-hand side.  Needs, on every system, the multicolour GS finest
+ * smoother, Jacobi coarse levels with at least one pre-sweep, and one V-cycle
+ * precision and max_iter across the group; a system may appear once. */
+int ufvm_pcg_solve_group(ufvm_ctx *ctx, int32_t g, const int32_t *sys, const double *const *b,
+                         double *const *x, int32_t *iters, double *relres);
 /* Apply one preconditioner V-cycle: z = M^-1 r (host arrays, level-1 order). */
 int ufvm_pcg_apply_vcycle(ufvm_ctx *ctx, int32_t sys, const double *r, double *z);
 /* number of colours of a level's multicolour ordering (after ufvm_pcg_setup) */

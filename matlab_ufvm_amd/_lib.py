@@ -105,6 +105,8 @@ _SIGS = {
     "ufvm_pcg_solve": (C.c_int, [C.c_void_p, C.c_int32, i32p, f64p]),
     "ufvm_pcg_solve_batch": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_void_p, C.c_void_p,
                                        i32p, f64p]),
+    "ufvm_pcg_solve_group": (C.c_int, [C.c_void_p, C.c_int32, i32p, C.POINTER(f64p),
+                                       C.POINTER(f64p), i32p, f64p]),
     "ufvm_pcg_apply_vcycle": (C.c_int, [C.c_void_p, C.c_int32, f64p, f64p]),
     "ufvm_pcg_level_colours": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, i32p, i32p]),
     "ufvm_case_create": (C.c_int, [C.c_void_p, C.POINTER(MeshDesc), C.POINTER(SimpleCfg), i32p]),
@@ -358,6 +360,24 @@ class Context:
                                                  C.c_void_p(x_addr), its.ctypes.data_as(i32p),
                                                  rel.ctypes.data_as(f64p)))
         return its, rel
+
+    def pcg_solve_group(self, sids, bs):
+        """len(sids) solves of different systems at once, every phase of an
+        iteration one launch for all (ufvm_pcg_solve_group); bs: their
+        right-hand sides in level-1 order.  Returns (solutions, iterations,
+        relative residuals)."""
+        g = len(sids)
+        bs = [_f64(b) for b in bs]
+        xs = [np.empty_like(b) for b in bs]
+        ids = np.ascontiguousarray(sids, np.int32)
+        bp = (f64p * g)(*[_p(b, f64p) for b in bs])
+        xp = (f64p * g)(*[_p(x, f64p) for x in xs])
+        its = np.zeros(g, np.int32)
+        rel = np.zeros(g)
+        self.check(self.lib.ufvm_pcg_solve_group(self.h, g, _p(ids, i32p), bp, xp,
+                                                 its.ctypes.data_as(i32p),
+                                                 rel.ctypes.data_as(f64p)))
+        return xs, its, rel
 
     def pcg_apply_vcycle(self, sid, r):
         r = _f64(r)

@@ -800,6 +800,16 @@ int ufvm_pcg_solve_batch(ufvm_ctx *ctx, int32_t sys, int32_t k, const double *b,
   return guard(ctx, [&] { pcg_solve_batch(ctx, sys_of(ctx, sys), k, b, x, iters, relres); });
 }
 
+int ufvm_pcg_solve_group(ufvm_ctx *ctx, int32_t g, const int32_t *sys, const double *const *b,
+                         double *const *x, int32_t *iters, double *relres) {
+  return guard(ctx, [&] {
+    if (g < 1 || !sys) throw Error(UFVM_ERR_ARG, "a group needs g >= 1 systems");
+    std::vector<System *> v;
+    for (int32_t q = 0; q < g; ++q) v.push_back(&sys_of(ctx, sys[q]));
+    pcg_solve_group(ctx, v, b, x, iters, relres);
+  });
+}
+
 int ufvm_pcg_apply_vcycle(ufvm_ctx *ctx, int32_t sys, const double *r, double *z) {
   return guard(ctx, [&] { pcg_apply_vcycle(ctx, sys_of(ctx, sys), r, z); });
 }

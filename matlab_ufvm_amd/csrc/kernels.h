@@ -260,5 +260,49 @@ void dot_partial(hipStream_t s, int32_t n, const double *a, const double *b, dou
 void sum_partial(hipStream_t s, int32_t n, const double *a, double *partial);
 void axpy_const(hipStream_t s, int32_t n, const double *scal_slot, double mul, double *x);
 
+// ---- grouped launches (serving form).  One launch covers the same phase of
+// g solves of different systems: blockIdx.y selects the member, whose
+// arguments are entry y of a device table; blocks past the member's own
+// block count nb return, and so does every block of a member whose solve
+// has converged (*done != 0, the solve's flag scal[11]).  Each member keeps
+// the rows, block decomposition and reduction order of its own single-solve
+// launch, so its result keeps the bits of its own solve.
+enum GOp : int32_t {
+  G_MCGS,          // colour class [r0, n) Gauss-Seidel pass; aux = zero_above; out = x
+  G_RESID,         // r = b - A x, rows < aux set to 0; out = r
+  G_JACOBI,        // damped Jacobi (omega); parent != null: x + xc[parent] as input
+  G_RESTRICT,      // bc = scale * sum of children (off/col: child lists, x: fine r);
+                   // ac != null: also out2 = omega bc / ac (first Jacobi sweep)
+  G_PROLONG,       // x[i] += xc[parent[i]] for rows [r0, n); out = x
+  G_DENSE,         // x = Ainv b (FP64 inverse, one wavefront per row); out = x
+};
+template <typename V>
+struct GArgs {
+  int32_t n, r0, aux, nb;
+  const int32_t *off, *col, *parent;
+  const V *val, *ac, *b, *x, *xc;
+  V *out, *out2;
+  const double *ainv;
+  V omega, scale;
+  const double *done;
+};
+enum GFcgOp : int32_t { G_DOTS, G_UPDATE_P, G_UPDATE_P_FIRST, G_SPMV, G_UPDATE_XR, G_RR };
+template <typename V>
+struct GFcgArgs {
+  int32_t n, nb, n0, singular;   // rows, fold_blocks(n), colour-0 rows, constant nullspace
+  const V *z, *acv;              // preconditioned residual; finest diagonal (V-cycle copy)
+  V *rv, *z0;                    // V-cycle right-hand side and iterate of the finest level
+  const int32_t *off, *col;      // FP64 finest operator (sliced-ELL)
+  const double *ac, *val;
+  double *p, *w, *x, *r, *partial, *scal, *ring;
+};
+// blocks of a dense coarsest solve of n rows (one 64-lane wavefront per row)
+inline int32_t dense_blocks(int32_t n) { return (n + kBlock / 64 - 1) / (kBlock / 64); }
+// gx: the largest member block count (grid x), g: members (grid y)
+template <typename V>
+void group_row(hipStream_t s, GOp op, int32_t gx, int32_t g, const GArgs<V> *tab);
+template <typename V>
+void group_fcg(hipStream_t s, GFcgOp op, int32_t gx, int32_t g, const GFcgArgs<V> *tab);
+
 }  // namespace k
 }  // namespace ufvm

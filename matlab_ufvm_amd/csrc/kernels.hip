@@ -1476,15 +1476,22 @@ __device__ __forceinline__ void fold_sums(const double *__restrict__ part, int32
   block_sums_all<kFcgBlock, W>(t, lds);
 }
 
+// The FCG kernels' bodies take the block index blk and the block count nb of
+// the launch (blockIdx.x / gridDim.x for a single solve; a grouped launch
+// passes each member's own count), which fix the grid-stride row assignment
+// and the reduction tree.
+
 // w = A p (sliced-ELL layout) ; partial[b] = p.w over block b's rows
-__global__ void __launch_bounds__(kFcgBlock)
-k_spmv_dot(int32_t n, const int32_t *__restrict__ off, const int32_t *__restrict__ col,
-           const double *__restrict__ ac,
-           const double *__restrict__ val, const double *__restrict__ p, double *__restrict__ w,
-           double *__restrict__ partial) {
-  __shared__ double lds[kFcgBlock / kWave];
+__device__ __forceinline__ void spmv_dot_body(int32_t blk, int32_t nb, int32_t n,
+                                              const int32_t *__restrict__ off,
+                                              const int32_t *__restrict__ col,
+                                              const double *__restrict__ ac,
+                                              const double *__restrict__ val,
+                                              const double *__restrict__ p,
+                                              double *__restrict__ w,
+                                              double *__restrict__ partial, double *lds) {
   double d = 0.0;
-  for (int32_t i = blockIdx.x * kFcgBlock + threadIdx.x; i < n; i += gridDim.x * kFcgBlock) {
+  for (int32_t i = blk * kFcgBlock + threadIdx.x; i < n; i += nb * kFcgBlock) {
     const int32_t q = i >> 6;
     const int32_t base = off[q] + (i & 63), wq = (off[q + 1] - off[q]) >> 6;
     const double pi = p[i];
@@ -1493,7 +1500,16 @@ k_spmv_dot(int32_t n, const int32_t *__restrict__ off, const int32_t *__restrict
     d += pi * v;
   }
   d = block_sum_n<kFcgBlock>(d, lds);
-  if (threadIdx.x == 0) partial[blockIdx.x] = d;
+  if (threadIdx.x == 0) partial[blk] = d;
+}
+
+__global__ void __launch_bounds__(kFcgBlock)
+k_spmv_dot(int32_t n, const int32_t *__restrict__ off, const int32_t *__restrict__ col,
+           const double *__restrict__ ac,
+           const double *__restrict__ val, const double *__restrict__ p, double *__restrict__ w,
+           double *__restrict__ partial) {
+  __shared__ double lds[kFcgBlock / kWave];
+  spmv_dot_body(blockIdx.x, gridDim.x, n, off, col, ac, val, p, w, partial, lds);
 }
 
 void spmv_dot(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, const double *ac,
@@ -1504,15 +1520,11 @@ void spmv_dot(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, 
 
 // partials of z, z.w, z.r, w, r: partial[c * g + b] for block b of g
 template <typename V>
-__global__ void __launch_bounds__(kFcgBlock)
-k_fcg_dots(int32_t n, const V *__restrict__ z, const double *__restrict__ w,
-           const double *__restrict__ r, double *__restrict__ partial, BatchStrides bs) {
-  __shared__ double lds[5 * kFcgBlock / kWave];
-  // right-hand side blockIdx.y of a batch
-  z += blockIdx.y * bs.ldv;
-  w += blockIdx.y * bs.ld;
-  r += blockIdx.y * bs.ld;
-  partial += blockIdx.y * bs.ps;
+__device__ __forceinline__ void fcg_dots_body(int32_t blk, int32_t nb, int32_t n,
+                                              const V *__restrict__ z,
+                                              const double *__restrict__ w,
+                                              const double *__restrict__ r,
+                                              double *__restrict__ partial, double *lds) {
   double v[5] = {0, 0, 0, 0, 0};
   auto add = [&](double zi, double wi, double ri) {
     v[0] += zi;
@@ -1521,8 +1533,8 @@ k_fcg_dots(int32_t n, const V *__restrict__ z, const double *__restrict__ w,
     v[3] += wi;
     v[4] += ri;
   };
-  const int32_t G = gridDim.x * kFcgBlock;
-  int32_t i = blockIdx.x * kFcgBlock + threadIdx.x;
+  const int32_t G = nb * kFcgBlock;
+  int32_t i = blk * kFcgBlock + threadIdx.x;
   // four grid-stride rows per trip: their loads are in flight together, and
   // they are summed in the same order as one row per trip
   for (; i + 3 * G < n; i += 4 * G) {
@@ -1539,7 +1551,20 @@ k_fcg_dots(int32_t n, const V *__restrict__ z, const double *__restrict__ w,
   for (; i < n; i += G) add((double)z[i], w[i], r[i]);
   block_sums_all<kFcgBlock, 5>(v, lds);
   if (threadIdx.x == 0)
-    for (int c = 0; c < 5; ++c) partial[(int64_t)c * gridDim.x + blockIdx.x] = v[c];
+    for (int c = 0; c < 5; ++c) partial[(int64_t)c * nb + blk] = v[c];
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kFcgBlock)
+k_fcg_dots(int32_t n, const V *__restrict__ z, const double *__restrict__ w,
+           const double *__restrict__ r, double *__restrict__ partial, BatchStrides bs) {
+  __shared__ double lds[5 * kFcgBlock / kWave];
+  // right-hand side blockIdx.y of a batch
+  z += blockIdx.y * bs.ldv;
+  w += blockIdx.y * bs.ld;
+  r += blockIdx.y * bs.ld;
+  partial += blockIdx.y * bs.ps;
+  fcg_dots_body<V>(blockIdx.x, gridDim.x, n, z, w, r, partial, lds);
 }
 
 template <typename V>
@@ -1553,17 +1578,14 @@ void fcg_dots(hipStream_t s, int32_t n, const V *z, const double *w, const doubl
 // beta = (z - m).w / delta_prev), then p = (z - m) - beta p (p = z - m on the
 // first iteration).  Block 0 stores the scalars.
 template <typename V>
-__global__ void __launch_bounds__(kFcgBlock)
-k_fcg_update_p(int32_t n, const V *__restrict__ z, const double *__restrict__ partial,
-               int32_t g, int singular, int first, double *__restrict__ scal,
-               double *__restrict__ p, BatchStrides bs) {
-  __shared__ double lds[5 * kFcgBlock / kWave];
-  z += blockIdx.y * bs.ldv;
-  p += blockIdx.y * bs.ld;
-  partial += blockIdx.y * bs.ps;
-  scal += blockIdx.y * bs.ss;
-  const int32_t G = gridDim.x * kFcgBlock;
-  int32_t i = blockIdx.x * kFcgBlock + threadIdx.x;
+__device__ __forceinline__ void fcg_update_p_body(int32_t blk, int32_t nb, int32_t n,
+                                                  const V *__restrict__ z,
+                                                  const double *__restrict__ partial, int32_t g,
+                                                  int singular, int first,
+                                                  double *__restrict__ scal,
+                                                  double *__restrict__ p, double *lds) {
+  const int32_t G = nb * kFcgBlock;
+  int32_t i = blk * kFcgBlock + threadIdx.x;
   // the first four grid-stride rows are loaded before the fold, so that their
   // latency overlaps the partials' (the scalars are not needed to load them)
   double zi[4], pi[4];
@@ -1580,7 +1602,7 @@ k_fcg_update_p(int32_t n, const V *__restrict__ z, const double *__restrict__ pa
   const double m = singular ? t[0] / (double)n : 0.0;
   const double rho = t[2] - m * t[4];
   const double beta = first ? 0.0 : (t[1] - m * t[3]) / scal[5];
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blk == 0 && threadIdx.x == 0) {
     for (int c = 0; c < 5; ++c) scal[c] = t[c];
     scal[9] = m;
     scal[8] = rho;
@@ -1611,6 +1633,19 @@ k_fcg_update_p(int32_t n, const V *__restrict__ z, const double *__restrict__ pa
 }
 
 template <typename V>
+__global__ void __launch_bounds__(kFcgBlock)
+k_fcg_update_p(int32_t n, const V *__restrict__ z, const double *__restrict__ partial,
+               int32_t g, int singular, int first, double *__restrict__ scal,
+               double *__restrict__ p, BatchStrides bs) {
+  __shared__ double lds[5 * kFcgBlock / kWave];
+  z += blockIdx.y * bs.ldv;
+  p += blockIdx.y * bs.ld;
+  partial += blockIdx.y * bs.ps;
+  scal += blockIdx.y * bs.ss;
+  fcg_update_p_body<V>(blockIdx.x, gridDim.x, n, z, partial, g, singular, first, scal, p, lds);
+}
+
+template <typename V>
 void fcg_update_p(hipStream_t s, int32_t n, const V *z, const double *partial, int singular,
                   int first, double *scal, double *p, int k, BatchStrides bs) {
   const int g = fold_blocks(n);
@@ -1626,25 +1661,14 @@ void fcg_update_p(hipStream_t s, int32_t n, const V *z, const double *partial, i
 // convergence flag scal[11] is set (an iteration issued after the converged
 // one leaves x and r as they were).  partial_out must not overlap partial_in.
 template <typename V>
-__global__ void __launch_bounds__(kFcgBlock)
-k_fcg_update_xr(int32_t n, const double *__restrict__ partial_in, double *__restrict__ scal,
-                const double *__restrict__ p, const double *__restrict__ w,
-                double *__restrict__ x, double *__restrict__ r, V *__restrict__ rv,
-                double *__restrict__ partial_out, int32_t n0, const V *__restrict__ acv,
-                V *__restrict__ z0, BatchStrides bs) {
-  __shared__ double lds[kFcgBlock / kWave];
-  scal += blockIdx.y * bs.ss;
-  if (scal[11] != 0.0) return;
-  partial_in += blockIdx.y * bs.ps;
-  partial_out += blockIdx.y * bs.ps;
-  p += blockIdx.y * bs.ld;
-  w += blockIdx.y * bs.ld;
-  x += blockIdx.y * bs.ld;
-  r += blockIdx.y * bs.ld;
-  rv += blockIdx.y * bs.ldv;
-  z0 += blockIdx.y * bs.ldv;
-  const int32_t G = gridDim.x * kFcgBlock;
-  int32_t i = blockIdx.x * kFcgBlock + threadIdx.x;
+__device__ __forceinline__ void fcg_update_xr_body(
+    int32_t blk, int32_t nb, int32_t n, const double *__restrict__ partial_in,
+    double *__restrict__ scal, const double *__restrict__ p, const double *__restrict__ w,
+    double *__restrict__ x, double *__restrict__ r, V *__restrict__ rv,
+    double *__restrict__ partial_out, int32_t n0, const V *__restrict__ acv, V *__restrict__ z0,
+    double *lds) {
+  const int32_t G = nb * kFcgBlock;
+  int32_t i = blk * kFcgBlock + threadIdx.x;
   // the first four grid-stride rows are loaded before the fold (their latency
   // overlaps the partials')
   double xi[4], pi[4], ri[4], wi[4];
@@ -1659,10 +1683,10 @@ k_fcg_update_xr(int32_t n, const double *__restrict__ partial_in, double *__rest
     }
   }
   double t[1];
-  fold_sums<1>(partial_in, gridDim.x, t, lds);
+  fold_sums<1>(partial_in, nb, t, lds);
   const double delta = t[0];
   const double a = scal[8] / delta;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (blk == 0 && threadIdx.x == 0) {
     scal[5] = delta;
     scal[7] = a;
   }
@@ -1694,7 +1718,29 @@ k_fcg_update_xr(int32_t n, const double *__restrict__ partial_in, double *__rest
   }
   for (; i < n; i += G) row(i, x[i], p[i], r[i], w[i]);
   rr = block_sum_n<kFcgBlock>(rr, lds);
-  if (threadIdx.x == 0) partial_out[blockIdx.x] = rr;
+  if (threadIdx.x == 0) partial_out[blk] = rr;
+}
+
+template <typename V>
+__global__ void __launch_bounds__(kFcgBlock)
+k_fcg_update_xr(int32_t n, const double *__restrict__ partial_in, double *__restrict__ scal,
+                const double *__restrict__ p, const double *__restrict__ w,
+                double *__restrict__ x, double *__restrict__ r, V *__restrict__ rv,
+                double *__restrict__ partial_out, int32_t n0, const V *__restrict__ acv,
+                V *__restrict__ z0, BatchStrides bs) {
+  __shared__ double lds[kFcgBlock / kWave];
+  scal += blockIdx.y * bs.ss;
+  if (scal[11] != 0.0) return;
+  partial_in += blockIdx.y * bs.ps;
+  partial_out += blockIdx.y * bs.ps;
+  p += blockIdx.y * bs.ld;
+  w += blockIdx.y * bs.ld;
+  x += blockIdx.y * bs.ld;
+  r += blockIdx.y * bs.ld;
+  rv += blockIdx.y * bs.ldv;
+  z0 += blockIdx.y * bs.ldv;
+  fcg_update_xr_body<V>(blockIdx.x, gridDim.x, n, partial_in, scal, p, w, x, r, rv, partial_out,
+                        n0, acv, z0, lds);
 }
 
 template <typename V>
@@ -1731,13 +1777,8 @@ void ring_put(hipStream_t s, const double *src, double *ring) {
 // ring slot it & 1 (host-visible, may be null), it = scal[14] the iteration
 // counter, which it then advances.  Everything per solve lives in scal, so a
 // captured iteration replays unchanged.
-__global__ void __launch_bounds__(kFcgBlock)
-k_fcg_rr(const double *__restrict__ partial, int32_t g, double *scal, double *ring,
-         BatchStrides bs) {
-  __shared__ double lds[kFcgBlock / kWave];
-  partial += blockIdx.y * bs.ps;
-  scal += blockIdx.y * bs.ss;
-  if (ring) ring += blockIdx.y * bs.rs;
+__device__ __forceinline__ void fcg_rr_body(const double *__restrict__ partial, int32_t g,
+                                            double *scal, double *ring, double *lds) {
   double t[1];
   fold_sums<1>(partial, g, t, lds);
   if (threadIdx.x != 0) return;
@@ -1752,6 +1793,16 @@ k_fcg_rr(const double *__restrict__ partial, int32_t g, double *scal, double *ri
     slot[1] = it;
   }
   scal[14] = it + 1.0;
+}
+
+__global__ void __launch_bounds__(kFcgBlock)
+k_fcg_rr(const double *__restrict__ partial, int32_t g, double *scal, double *ring,
+         BatchStrides bs) {
+  __shared__ double lds[kFcgBlock / kWave];
+  partial += blockIdx.y * bs.ps;
+  scal += blockIdx.y * bs.ss;
+  if (ring) ring += blockIdx.y * bs.rs;
+  fcg_rr_body(partial, g, scal, ring, lds);
 }
 
 // arm a solve: convergence flag off (on: done, for a zero right-hand side in
@@ -1816,6 +1867,110 @@ __global__ void k_axpy_const(int32_t n, const double *__restrict__ slot, double 
 void axpy_const(hipStream_t s, int32_t n, const double *scal_slot, double mul, double *x) {
   hipLaunchKernelGGL(k_axpy_const, dim3(nblocks(n)), dim3(kBlock), 0, s, n, scal_slot, mul, x);
 }
+
+// ---------------------------------------------------------- grouped launches
+
+template <typename V, GOp OP>
+__global__ void __launch_bounds__(kBlock) k_group_row(const GArgs<V> *__restrict__ tab) {
+  const GArgs<V> a = tab[blockIdx.y];
+  if ((int32_t)blockIdx.x >= a.nb || *a.done != 0.0) return;
+  if (OP == G_MCGS) {
+    // the member's rows in k_mcgs_sell's XCD-aware order over its own blocks
+    const int32_t i = (a.r0 & ~63) + xcd_block(blockIdx.x, a.nb) * kBlock + threadIdx.x;
+    if (i < a.r0 || i >= a.n) return;
+    const int32_t q = i >> 6;
+    const int32_t base = a.off[q] + (i & 63), w = (a.off[q + 1] - a.off[q]) >> 6;
+    const V v = (a.aux <= a.n)
+                    ? sell_row<V, true, true>(a.col, a.val, a.out, base, w, a.b[i], a.aux)
+                    : sell_row<V, true, false>(a.col, a.val, a.out, base, w, a.b[i], a.aux);
+    a.out[i] = v / a.ac[i];
+  } else if (OP == G_RESID) {
+    const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    if (i < a.aux) a.out[i] = V(0);
+    else residual_row<V>(i, a.off, a.col, a.val, a.ac, a.b, a.x, a.out);
+  } else if (OP == G_JACOBI) {
+    const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    if (a.parent)
+      jacobi_row<V, true>(i, a.off, a.col, a.val, a.ac, a.b, a.x, a.parent, a.xc, a.out, a.omega);
+    else
+      jacobi_row<V, false>(i, a.off, a.col, a.val, a.ac, a.b, a.x, nullptr, nullptr, a.out,
+                           a.omega);
+  } else if (OP == G_RESTRICT) {
+    const int32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= a.n) return;
+    if (a.ac)
+      restrict_row<V, true>(p, a.off, a.col, a.x, a.ac, a.out, a.out2, a.omega, a.scale);
+    else
+      restrict_row<V, false>(p, a.off, a.col, a.x, nullptr, a.out, nullptr, V(0), a.scale);
+  } else if (OP == G_PROLONG) {
+    const int32_t i = a.r0 + blockIdx.x * kBlock + threadIdx.x;
+    if (i >= a.n) return;
+    a.out[i] = a.out[i] + a.xc[a.parent[i]];
+  } else {
+    const int32_t row = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+    if (row >= a.n) return;
+    dense_row<V>(row, threadIdx.x & (kWave - 1), a.n, a.ainv, a.b, a.out);
+  }
+}
+
+template <typename V, GFcgOp OP>
+__global__ void __launch_bounds__(kFcgBlock) k_group_fcg(const GFcgArgs<V> *__restrict__ tab) {
+  __shared__ double lds[(OP == G_DOTS || OP == G_UPDATE_P || OP == G_UPDATE_P_FIRST ? 5 : 1) *
+                        kFcgBlock / kWave];
+  const GFcgArgs<V> a = tab[blockIdx.y];
+  if ((int32_t)blockIdx.x >= a.nb || a.scal[11] != 0.0) return;
+  if (OP == G_DOTS) {
+    fcg_dots_body<V>(blockIdx.x, a.nb, a.n, a.z, a.w, a.r, a.partial, lds);
+  } else if (OP == G_UPDATE_P || OP == G_UPDATE_P_FIRST) {
+    fcg_update_p_body<V>(blockIdx.x, a.nb, a.n, a.z, a.partial, a.nb, a.singular,
+                         OP == G_UPDATE_P_FIRST, a.scal, a.p, lds);
+  } else if (OP == G_SPMV) {
+    spmv_dot_body(blockIdx.x, a.nb, a.n, a.off, a.col, a.ac, a.val, a.p, a.w, a.partial, lds);
+  } else if (OP == G_UPDATE_XR) {
+    // the operator-product partials occupy partial[0, nb); r.r's follow
+    fcg_update_xr_body<V>(blockIdx.x, a.nb, a.n, a.partial, a.scal, a.p, a.w, a.x, a.r, a.rv,
+                          a.partial + a.nb, a.n0, a.acv, a.z0, lds);
+  } else {
+    fcg_rr_body(a.partial + a.nb, a.nb, a.scal, a.ring, lds);
+  }
+}
+
+template <typename V>
+void group_row(hipStream_t s, GOp op, int32_t gx, int32_t g, const GArgs<V> *tab) {
+  if (gx <= 0 || g <= 0) return;
+  const dim3 grid(gx, g), blk(kBlock);
+  switch (op) {
+    case G_MCGS: hipLaunchKernelGGL((k_group_row<V, G_MCGS>), grid, blk, 0, s, tab); break;
+    case G_RESID: hipLaunchKernelGGL((k_group_row<V, G_RESID>), grid, blk, 0, s, tab); break;
+    case G_JACOBI: hipLaunchKernelGGL((k_group_row<V, G_JACOBI>), grid, blk, 0, s, tab); break;
+    case G_RESTRICT: hipLaunchKernelGGL((k_group_row<V, G_RESTRICT>), grid, blk, 0, s, tab); break;
+    case G_PROLONG: hipLaunchKernelGGL((k_group_row<V, G_PROLONG>), grid, blk, 0, s, tab); break;
+    case G_DENSE: hipLaunchKernelGGL((k_group_row<V, G_DENSE>), grid, blk, 0, s, tab); break;
+  }
+}
+
+template <typename V>
+void group_fcg(hipStream_t s, GFcgOp op, int32_t gx, int32_t g, const GFcgArgs<V> *tab) {
+  if (gx <= 0 || g <= 0) return;
+  const dim3 grid(op == G_RR ? 1 : gx, g), blk(kFcgBlock);
+  switch (op) {
+    case G_DOTS: hipLaunchKernelGGL((k_group_fcg<V, G_DOTS>), grid, blk, 0, s, tab); break;
+    case G_UPDATE_P: hipLaunchKernelGGL((k_group_fcg<V, G_UPDATE_P>), grid, blk, 0, s, tab); break;
+    case G_UPDATE_P_FIRST:
+      hipLaunchKernelGGL((k_group_fcg<V, G_UPDATE_P_FIRST>), grid, blk, 0, s, tab);
+      break;
+    case G_SPMV: hipLaunchKernelGGL((k_group_fcg<V, G_SPMV>), grid, blk, 0, s, tab); break;
+    case G_UPDATE_XR: hipLaunchKernelGGL((k_group_fcg<V, G_UPDATE_XR>), grid, blk, 0, s, tab); break;
+    case G_RR: hipLaunchKernelGGL((k_group_fcg<V, G_RR>), grid, blk, 0, s, tab); break;
+  }
+}
+
+template void group_row<float>(hipStream_t, GOp, int32_t, int32_t, const GArgs<float> *);
+template void group_row<double>(hipStream_t, GOp, int32_t, int32_t, const GArgs<double> *);
+template void group_fcg<float>(hipStream_t, GFcgOp, int32_t, int32_t, const GFcgArgs<float> *);
+template void group_fcg<double>(hipStream_t, GFcgOp, int32_t, int32_t, const GFcgArgs<double> *);
 
 // explicit instantiations
 #define UFVM_INST(V)                                                                            \

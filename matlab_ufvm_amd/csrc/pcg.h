@@ -102,6 +102,89 @@ class FcgBatch {
 // whether S's precision-form set-up can run batched solves (multicolour GS
 // finest level, Jacobi coarse levels)
 bool batch_supported(const System &S);
+
+// FCG(1) for g solves of different systems at once (serving form).  The
+// members advance in lock step, one FCG iteration at a time, and each phase
+// of an iteration is one launch for all of them: every colour pass,
+// residual, restriction, Jacobi sweep, prolongation and dense solve of the
+// V-cycle at each level, and every FCG kernel (k::group_row, k::group_fcg,
+// with a device table of each member's arguments).  Members whose V-cycles
+// are shorter skip the launches of levels they do not have.  A member keeps
+// the arithmetic, block decomposition and reduction order of its own
+// FcgRun, so its solution, iteration count and residual are those of its
+// own solve; a member that converges is finished at once (its launches
+// then skip it) while the others continue.  Members need group_supported
+// systems with the same V-cycle precision and max_iter, each on its own
+// single-right-hand-side workspace.
+class FcgGroup {
+ public:
+  struct Member {
+    System *S;
+    SolveWork *W;
+    const double *b;          // device, reference order (may be W->bref)
+    double *x;                // device, reference order (may be W->xref)
+    double *x_host = nullptr; // if set, x is also copied here when the member finishes
+  };
+  FcgGroup(ufvm_ctx *ctx, std::vector<Member> members, hipStream_t s);
+  ~FcgGroup();
+  FcgGroup(const FcgGroup &) = delete;
+  FcgGroup &operator=(const FcgGroup &) = delete;
+  void start();
+  // advance without blocking (or block for the awaited step); true once every
+  // member has finished and its copies are done
+  bool poll(bool block);
+  int size() const { return (int)m_.size(); }
+  // member q's solution (and host copy) is written, or it failed
+  bool member_done(int q);
+  const std::string &member_error(int q) const { return st_[q].err; }
+  int32_t iterations(int q) const { return st_[q].its; }
+  double relres(int q) const { return st_[q].nb > 0.0 ? st_[q].rr / st_[q].nb : 0.0; }
+  int64_t launches() const { return launches_; }   // grouped launches issued
+
+ private:
+  struct MemberState {
+    double nb = 0.0, thr = 0.0, rr = 0.0, prev2 = -1.0;
+    int32_t its = 0;
+    bool active = false, finished = false, ahead = true;
+    hipEvent_t ev = nullptr;
+    std::string err;
+  };
+  struct Step {                 // one grouped launch
+    int32_t op, gx, g;
+    size_t table;               // byte offset of its argument table in tables_
+  };
+  template <typename V>
+  void build();
+  template <typename V>
+  void issue_t(int k);
+  void issue(int k);
+  void finish_member(int q);
+
+  ufvm_ctx *ctx_;
+  std::vector<Member> m_;
+  std::vector<MemberState> st_;
+  hipStream_t s_;
+  int precision_ = 32;
+  int32_t max_iter_ = 0;
+  bool graphs_ = false;
+  std::vector<Step> steps_[2];  // V-cycle launches of the first / later iterations
+  size_t fcg_table_ = 0;
+  int32_t fcg_gx_ = 0;
+  DBuf<char> tables_;
+  hipEvent_t ev_[3] = {nullptr, nullptr, nullptr};
+  hipGraphExec_t graph_[2] = {nullptr, nullptr};
+  int phase_ = 0, it_ = 0, issued_ = 0;
+  int64_t launches_ = 0;
+};
+
+// whether S's precision-form set-up can run in a group (batch_supported and
+// at least one pre-smoothing sweep on every level)
+bool group_supported(const System &S);
+// blocking group solve on the context stream (ufvm_pcg_solve_group): each
+// system's own workspace, host right-hand sides and solutions
+void pcg_solve_group(ufvm_ctx *ctx, const std::vector<System *> &sys,
+                     const double *const *b, double *const *x, int32_t *iters,
+                     double *relres);
 // blocking batch solve on the context stream (workspace cached per k)
 void pcg_solve_batch(ufvm_ctx *ctx, System &S, int k, const double *B, double *X,
                      int32_t *iters, double *relres);

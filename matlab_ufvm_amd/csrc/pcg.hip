@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
 
 #include "kernels.h"
 #include "pcg.h"
@@ -689,27 +690,33 @@ bool FcgRun::ready(hipEvent_t e, bool block) {
   return false;
 }
 
+// The start of one solve on a single-right-hand-side workspace: r = P b
+// (V-cycle numbering), projected when singular, x = w = 0, |r|^2 to the ring
+// slot 4 and r as the V-cycle's right-hand side (FcgRun and FcgGroup).
 template <typename V>
-void FcgRun::start_t(const double *b) {
-  const PcgState &P = S_->pcg;
+static void fcg_setup(hipStream_t s, const System &S, SolveWork &W, const double *b) {
+  const PcgState &P = S.pcg;
   const FastLevel &L0 = *P.lev[0];
-  SolveWork &W = *W_;
   const int32_t n = W.n0;
   double *scal = W.scal.p;
-  // r = P b (V-cycle numbering), projected when singular; |r|^2 to the ring
-  k::permute_gather(s_, n, L0.perm_d.p, b, W.r.p);
+  k::permute_gather(s, n, L0.perm_d.p, b, W.r.p);
   if (P.singular) {
-    k::sum_partial(s_, n, W.r.p, W.partial.p);
-    k::reduce_final(s_, W.partial.p, k::nblocks(n), scal + 12);
-    k::axpy_const(s_, n, scal + 12, -1.0 / (double)n, W.r.p);
+    k::sum_partial(s, n, W.r.p, W.partial.p);
+    k::reduce_final(s, W.partial.p, k::nblocks(n), scal + 12);
+    k::axpy_const(s, n, scal + 12, -1.0 / (double)n, W.r.p);
   }
-  k::fill<double>(s_, n, W.x.p, 0.0);
-  k::fill<double>(s_, n, W.w.p, 0.0);
-  k::dot_partial(s_, n, W.r.p, W.r.p, W.partial.p);
-  k::reduce_final(s_, W.partial.p, k::nblocks(n), scal + 10);
-  k::convert<double, V>(s_, n, W.r.p, Prec<V>::b(W.lev[0]));
-  k::ring_put(s_, scal + 10, W.dring + 4);
-  UFVM_HIP(hipEventRecord(W.ev[2], s_));
+  k::fill<double>(s, n, W.x.p, 0.0);
+  k::fill<double>(s, n, W.w.p, 0.0);
+  k::dot_partial(s, n, W.r.p, W.r.p, W.partial.p);
+  k::reduce_final(s, W.partial.p, k::nblocks(n), scal + 10);
+  k::convert<double, V>(s, n, W.r.p, Prec<V>::b(W.lev[0]));
+  k::ring_put(s, scal + 10, W.dring + 4);
+}
+
+template <typename V>
+void FcgRun::start_t(const double *b) {
+  fcg_setup<V>(s_, *S_, *W_, b);
+  UFVM_HIP(hipEventRecord(W_->ev[2], s_));
 }
 
 void FcgRun::start(const double *b, double *x) {
@@ -1155,6 +1162,496 @@ void pcg_apply_vcycle(ufvm_ctx *ctx, System &S, const double *r, double *z) {
   else apply_vcycle_t<double>(ctx, S, W);
   UFVM_HIP(hipMemcpyAsync(z, W.xref.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
   UFVM_HIP(hipStreamSynchronize(s));
+}
+
+// ------------------------------------------------------------- group solves
+
+bool group_supported(const System &S) {
+  if (!batch_supported(S)) return false;
+  const PcgState &P = S.pcg;
+  for (int32_t l = 0; l + 1 < P.nlev; ++l) {
+    const int pre = (l == 0 || P.cfg.coarse_pre_sweep < 0) ? P.cfg.pre_sweep
+                                                          : P.cfg.coarse_pre_sweep;
+    if (pre < 1) return false;
+  }
+  return true;
+}
+
+namespace {
+
+// Phase key of a V-cycle launch: launches of different members with equal
+// keys become one launch.  Keys order the descent by level, then every dense
+// coarsest solve, then the ascent by level from the coarsest; inside a level
+// they follow Cycle<V>'s order, so each member's launches keep their order.
+struct GKey {
+  int32_t stage, level, a, b, c;
+  bool operator<(const GKey &o) const {
+    if (stage != o.stage) return stage < o.stage;
+    if (level != o.level) return level < o.level;
+    if (a != o.a) return a < o.a;
+    if (b != o.b) return b < o.b;
+    return c < o.c;
+  }
+};
+
+// Records the launches of one member's V-cycle exactly as Cycle<V> makes them
+// for the kinds a group accepts (multicolour GS finest level, Jacobi coarse
+// levels, dense coarsest level), with each launch's arguments.
+template <typename V>
+struct GroupRec {
+  using P = Prec<V>;
+  using A = k::GArgs<V>;
+  System &S;
+  SolveWork &W;
+  const double *done;
+  std::map<GKey, std::pair<int32_t, std::vector<std::pair<int, A>>>> &out;
+  int member;
+
+  int pre_of(int l) const {
+    const ufvm_pcg_cfg &c = S.pcg.cfg;
+    return (l == 0 || c.coarse_pre_sweep < 0) ? c.pre_sweep : c.coarse_pre_sweep;
+  }
+  int post_of(int l) const {
+    const ufvm_pcg_cfg &c = S.pcg.cfg;
+    return (l == 0 || c.coarse_post_sweep < 0) ? c.post_sweep : c.coarse_post_sweep;
+  }
+  A blank() const {
+    A a{};
+    a.done = done;
+    return a;
+  }
+  void add(const GKey &key, k::GOp op, const A &a) {
+    auto &e = out[key];
+    if (!e.second.empty() && e.first != (int32_t)op)
+      throw Error(UFVM_ERR_STATE, "group schedule: two operations share a phase");
+    e.first = (int32_t)op;
+    e.second.emplace_back(member, a);
+  }
+  static int32_t up(int l) { return 1000 - l; }
+
+  void pass(const FastLevel &L, LevelWork &LW, const GKey &key, int c, int32_t zero_above) {
+    const int32_t r0 = L.col_off[c], r1 = L.col_off[c + 1];
+    if (r1 <= r0) return;
+    A a = blank();
+    a.n = r1;
+    a.r0 = r0;
+    a.aux = zero_above;
+    a.nb = k::nblocks(r1 - (r0 & ~63));
+    a.off = L.soff.p;
+    a.col = L.scol.p;
+    a.val = P::sval(L);
+    a.ac = P::ac(L);
+    a.b = P::b(LW);
+    a.out = P::x(LW);
+    add(key, k::G_MCGS, a);
+  }
+  void mcgs_sweeps(const FastLevel &L, LevelWork &LW, int nsweep, bool from_zero,
+                   bool first_done, int32_t stage, int32_t level) {
+    for (int sw = 0; sw < nsweep; ++sw) {
+      for (int c = (sw == 0 ? (first_done ? 1 : 0) : 1); c < L.ncol; ++c)
+        pass(L, LW, {stage, level, 1, 2 * sw, c}, c,
+             (sw == 0 && from_zero) ? L.col_off[c + 1] : L.n);
+      for (int c = L.ncol - 2; c >= 0; --c) pass(L, LW, {stage, level, 1, 2 * sw + 1, 1000 - c}, c, L.n);
+    }
+  }
+  void down(const FastLevel &L, LevelWork &LW, int l, const V *x, int32_t zero_rows) {
+    const PcgState &Pc = S.pcg;
+    const FastLevel &C = *Pc.lev[l + 1];
+    LevelWork &CW = W.lev[l + 1];
+    A r = blank();
+    r.n = L.n;
+    r.aux = zero_rows;
+    r.nb = k::nblocks(L.n);
+    r.off = L.soff.p;
+    r.col = L.scol.p;
+    r.val = P::sval(L);
+    r.ac = P::ac(L);
+    r.b = P::b(LW);
+    r.x = x;
+    r.out = P::r(LW);
+    add({0, l, 2, 0, 0}, k::G_RESID, r);
+    A t = blank();
+    t.n = C.n;
+    t.nb = k::nblocks(C.n);
+    t.off = L.csoff.p;
+    t.col = L.cschild.p;
+    t.x = P::r(LW);
+    t.out = P::b(CW);
+    t.scale = V(Pc.cfg.coarse_scale);
+    if (C.kind == 3 && pre_of(l + 1) > 0) {
+      t.ac = P::ac(C);
+      t.out2 = P::x(CW);
+      t.omega = V(2.0 / 3.0);
+    }
+    add({0, l, 3, 0, 0}, k::G_RESTRICT, t);
+  }
+  const V *vcycle(int l, bool first_done) {
+    const PcgState &Pc = S.pcg;
+    const FastLevel &L = *Pc.lev[l];
+    LevelWork &LW = W.lev[l];
+    const int pre = pre_of(l), post = post_of(l);
+    if (l == Pc.nlev - 1) {
+      A a = blank();
+      a.n = L.n;
+      a.nb = k::dense_blocks(L.n);
+      a.ainv = Pc.ainv.p;
+      a.b = P::b(LW);
+      a.out = P::x(LW);
+      add({1, 0, 0, 0, 0}, k::G_DENSE, a);
+      return P::x(LW);
+    }
+    V *x = P::x(LW);
+    const V omega = V(2.0 / 3.0);
+    auto jac = [&](const GKey &key, const V *in, const int32_t *parent, const V *xc, V *outp) {
+      A a = blank();
+      a.n = L.n;
+      a.nb = k::nblocks(L.n);
+      a.off = L.soff.p;
+      a.col = L.scol.p;
+      a.val = P::sval(L);
+      a.ac = P::ac(L);
+      a.b = P::b(LW);
+      a.x = in;
+      a.parent = parent;
+      a.xc = xc;
+      a.out = outp;
+      a.omega = omega;
+      add(key, k::G_JACOBI, a);
+    };
+    auto prolong = [&](int32_t r0, const V *xc, V *xp) {
+      if (L.n <= r0) return;
+      A a = blank();
+      a.n = L.n;
+      a.r0 = r0;
+      a.nb = k::nblocks(L.n - r0);
+      a.parent = L.parent.p;
+      a.xc = xc;
+      a.out = xp;
+      add({2, up(l), 0, 0, 0}, k::G_PROLONG, a);
+    };
+    if (L.kind == 3) {
+      V *y = P::y(LW);
+      for (int it = 1; it < pre; ++it) {
+        jac({0, l, 1, it, 0}, x, nullptr, nullptr, y);
+        std::swap(x, y);
+      }
+      down(L, LW, l, x, 0);
+      const V *xc = vcycle(l + 1, false);
+      const bool fuse = post > 0 && L.n <= kFusedProlongRows;
+      if (!fuse) prolong(0, xc, x);
+      for (int it = 0; it < post; ++it) {
+        const bool pro = fuse && it == 0;
+        jac({2, up(l), 1, it, 0}, x, pro ? L.parent.p : nullptr, pro ? xc : nullptr, y);
+        std::swap(x, y);
+      }
+      return x;
+    }
+    // kind 0: the finest level, multicolour Gauss-Seidel
+    mcgs_sweeps(L, LW, pre, true, first_done, 0, l);
+    down(L, LW, l, x, pre > 0 ? L.col_off[1] : 0);
+    const V *xc = vcycle(l + 1, false);
+    prolong(post > 0 ? L.col_off[1] : 0, xc, x);
+    if (post > 0) mcgs_sweeps(L, LW, post, false, false, 2, up(l));
+    return x;
+  }
+};
+
+}  // namespace
+
+FcgGroup::FcgGroup(ufvm_ctx *ctx, std::vector<Member> members, hipStream_t s)
+    : ctx_(ctx), m_(std::move(members)), st_(m_.size()), s_(s) {
+  if (m_.empty()) throw Error(UFVM_ERR_ARG, "an empty group");
+  for (size_t q = 0; q < m_.size(); ++q) {
+    const Member &mb = m_[q];
+    if (!mb.S || !mb.W || !mb.b || !mb.x) throw Error(UFVM_ERR_ARG, "incomplete group member");
+    const PcgState &P = mb.S->pcg;
+    if (!P.ready || P.stale)
+      throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first (or refresh after new coefficients)");
+    if (!group_supported(*mb.S))
+      throw Error(UFVM_ERR_UNSUPPORTED,
+                  "group solves need the multicolour GS finest smoother, Jacobi coarse levels "
+                  "and pre-smoothing on every level");
+    if (mb.W->nrhs != 1 || mb.W->n0 != mb.S->lev[0]->n)
+      throw Error(UFVM_ERR_ARG, "a group member needs a single-solve workspace of its system");
+    if (q == 0) {
+      precision_ = P.cfg.precision;
+      max_iter_ = P.cfg.max_iter;
+    } else if (P.cfg.precision != precision_ || P.cfg.max_iter != max_iter_) {
+      throw Error(UFVM_ERR_UNSUPPORTED, "group members need one V-cycle precision and max_iter");
+    }
+    for (size_t o = 0; o < q; ++o)
+      if (m_[o].W == mb.W) throw Error(UFVM_ERR_ARG, "two group members share a workspace");
+  }
+  bool small = true;
+  for (const Member &mb : m_) small = small && mb.W->n0 < FcgRun::kGraphRows;
+  const int mode = m_[0].S->pcg.cfg.launch_mode;
+  graphs_ = !ctx->prof_on && (mode == 2 || (mode == 0 && small));
+  for (hipEvent_t &e : ev_) UFVM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  for (MemberState &ms : st_) UFVM_HIP(hipEventCreateWithFlags(&ms.ev, hipEventDisableTiming));
+  if (precision_ == 32) build<float>();
+  else build<double>();
+}
+
+FcgGroup::~FcgGroup() {
+  for (hipGraphExec_t &g : graph_)
+    if (g) (void)hipGraphExecDestroy(g);
+  for (hipEvent_t &e : ev_)
+    if (e) (void)hipEventDestroy(e);
+  for (MemberState &ms : st_)
+    if (ms.ev) (void)hipEventDestroy(ms.ev);
+}
+
+// Both iteration schedules (the first, whose pre-smoother starts from colour
+// 0, and the later ones, whose colour-0 pass the previous x/r update made)
+// and the FCG table, in one device buffer of argument tables.
+template <typename V>
+void FcgGroup::build() {
+  std::vector<char> host;
+  auto put = [&](const void *p, size_t bytes) {
+    const size_t at = (host.size() + 255) & ~(size_t)255;
+    host.resize(at + bytes);
+    std::memcpy(host.data() + at, p, bytes);
+    return at;
+  };
+  for (int kind = 0; kind < 2; ++kind) {
+    std::map<GKey, std::pair<int32_t, std::vector<std::pair<int, k::GArgs<V>>>>> sched;
+    for (int q = 0; q < (int)m_.size(); ++q) {
+      System &S = *m_[q].S;
+      SolveWork &W = *m_[q].W;
+      const FastLevel &L0 = *S.pcg.lev[0];
+      const bool fuse0 = L0.kind == 0 && S.pcg.cfg.pre_sweep > 0;
+      GroupRec<V> rec{S, W, W.scal.p + 11, sched, q};
+      const V *z = rec.vcycle(0, fuse0 && kind == 1);
+      // the V-cycle's result is the finest level's x on every path
+      if (z != Prec<V>::x(W.lev[0]))
+        throw Error(UFVM_ERR_STATE, "group schedule: unexpected V-cycle result vector");
+    }
+    for (auto &e : sched) {
+      std::vector<k::GArgs<V>> tab;
+      int32_t gx = 0;
+      for (auto &pm : e.second.second) {
+        tab.push_back(pm.second);
+        gx = std::max(gx, pm.second.nb);
+      }
+      const size_t at = put(tab.data(), tab.size() * sizeof(k::GArgs<V>));
+      steps_[kind].push_back({e.second.first, gx, (int32_t)tab.size(), at});
+    }
+  }
+  std::vector<k::GFcgArgs<V>> f(m_.size());
+  for (size_t q = 0; q < m_.size(); ++q) {
+    System &S = *m_[q].S;
+    SolveWork &W = *m_[q].W;
+    const FastLevel &L0 = *S.pcg.lev[0];
+    const bool fuse0 = L0.kind == 0 && S.pcg.cfg.pre_sweep > 0;
+    k::GFcgArgs<V> &a = f[q];
+    a.n = W.n0;
+    a.nb = k::fold_blocks(W.n0);
+    a.n0 = fuse0 ? L0.col_off[1] : 0;
+    a.singular = S.pcg.singular;
+    a.z = Prec<V>::x(W.lev[0]);
+    a.acv = Prec<V>::ac(L0);
+    a.rv = Prec<V>::b(W.lev[0]);
+    a.z0 = Prec<V>::x(W.lev[0]);
+    a.off = L0.soff.p;
+    a.col = L0.scol.p;
+    a.ac = L0.ac64.p;
+    a.val = L0.sval64.p;
+    a.p = W.p.p;
+    a.w = W.w.p;
+    a.x = W.x.p;
+    a.r = W.r.p;
+    a.partial = W.partial.p;
+    a.scal = W.scal.p;
+    a.ring = W.dring;
+    fcg_gx_ = std::max(fcg_gx_, a.nb);
+  }
+  fcg_table_ = put(f.data(), f.size() * sizeof(k::GFcgArgs<V>));
+  tables_.upload(host.data(), host.size(), s_);
+}
+
+template <typename V>
+void FcgGroup::issue_t(int k) {
+  const int kind = k == 0 ? 0 : 1;
+  for (const Step &st : steps_[kind]) {
+    k::group_row<V>(s_, (k::GOp)st.op, st.gx, st.g,
+                    reinterpret_cast<const k::GArgs<V> *>(tables_.p + st.table));
+  }
+  const auto *ft = reinterpret_cast<const k::GFcgArgs<V> *>(tables_.p + fcg_table_);
+  const int32_t g = (int32_t)m_.size();
+  k::group_fcg<V>(s_, k::G_DOTS, fcg_gx_, g, ft);
+  k::group_fcg<V>(s_, k == 0 ? k::G_UPDATE_P_FIRST : k::G_UPDATE_P, fcg_gx_, g, ft);
+  k::group_fcg<V>(s_, k::G_SPMV, fcg_gx_, g, ft);
+  k::group_fcg<V>(s_, k::G_UPDATE_XR, fcg_gx_, g, ft);
+  k::group_fcg<V>(s_, k::G_RR, fcg_gx_, g, ft);
+}
+
+void FcgGroup::issue(int k) {
+  const bool f32 = precision_ == 32;
+  if (!graphs_) {
+    if (f32) issue_t<float>(k);
+    else issue_t<double>(k);
+  } else {
+    hipGraphExec_t &gx = graph_[k == 0 ? 0 : 1];
+    if (!gx) {
+      hipGraph_t g = nullptr;
+      std::lock_guard<std::mutex> lk(device_mgmt_mutex());
+      UFVM_HIP(hipStreamBeginCapture(s_, hipStreamCaptureModeRelaxed));
+      try {
+        if (f32) issue_t<float>(k);
+        else issue_t<double>(k);
+      } catch (...) {
+        (void)hipStreamEndCapture(s_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      UFVM_HIP(hipStreamEndCapture(s_, &g));
+      const hipError_t e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      UFVM_HIP(e);
+    }
+    UFVM_HIP(hipGraphLaunch(gx, s_));
+  }
+  launches_ += (int64_t)steps_[k == 0 ? 0 : 1].size() + 5;
+  UFVM_HIP(hipEventRecord(ev_[k & 1], s_));
+}
+
+void FcgGroup::start() {
+  for (size_t q = 0; q < m_.size(); ++q) {
+    if (precision_ == 32) fcg_setup<float>(s_, *m_[q].S, *m_[q].W, m_[q].b);
+    else fcg_setup<double>(s_, *m_[q].S, *m_[q].W, m_[q].b);
+  }
+  UFVM_HIP(hipEventRecord(ev_[2], s_));
+  it_ = issued_ = 0;
+  phase_ = 1;
+}
+
+// the member's solution into its x (and host copy); its launches skip it from
+// now on (its flag is set, or armed so for a member that never iterates)
+void FcgGroup::finish_member(int q) {
+  MemberState &ms = st_[q];
+  const Member &mb = m_[q];
+  k::permute_scatter(s_, mb.W->n0, mb.S->pcg.lev[0]->perm_d.p, mb.W->x.p, mb.x);
+  if (mb.x_host)
+    UFVM_HIP(hipMemcpyAsync(mb.x_host, mb.x, (size_t)mb.W->n0 * sizeof(double),
+                            hipMemcpyDeviceToHost, s_));
+  UFVM_HIP(hipEventRecord(ms.ev, s_));
+  ms.active = false;
+  ms.finished = true;
+}
+
+bool FcgGroup::member_done(int q) {
+  MemberState &ms = st_[q];
+  if (!ms.finished) return false;
+  const hipError_t e = hipEventQuery(ms.ev);
+  if (e == hipSuccess) return true;
+  if (e != hipErrorNotReady) UFVM_HIP(e);
+  return false;
+}
+
+bool FcgGroup::poll(bool block) {
+  auto ready = [&](hipEvent_t e) {
+    if (block) {
+      UFVM_HIP(hipEventSynchronize(e));
+      return true;
+    }
+    const hipError_t q = hipEventQuery(e);
+    if (q == hipSuccess) return true;
+    if (q != hipErrorNotReady) UFVM_HIP(q);
+    return false;
+  };
+  for (;;) {
+    if (phase_ == 4) return true;
+    if (phase_ == 3) {
+      for (size_t q = 0; q < m_.size(); ++q)
+        if (!ready(st_[q].ev)) return false;
+      phase_ = 4;
+      continue;
+    }
+    if (phase_ == 1) {
+      if (!ready(ev_[2])) return false;
+      int left = 0;
+      for (size_t q = 0; q < m_.size(); ++q) {
+        MemberState &ms = st_[q];
+        const ufvm_pcg_cfg &cfg = m_[q].S->pcg.cfg;
+        ms.nb = std::sqrt(m_[q].W->hring[4]);
+        ms.rr = ms.nb;
+        ms.thr = (cfg.tol * ms.nb) * (cfg.tol * ms.nb);
+        const bool nan = std::isnan(ms.nb);
+        if (nan) ms.err = "NaN right-hand side";
+        // a zero right-hand side is solved by x = 0; its launches stay off
+        const bool idle = nan || !(ms.nb > 0.0) || cfg.max_iter <= 0;
+        k::fcg_arm(s_, m_[q].W->scal.p, ms.thr, idle ? 1 : 0);
+        if (idle) finish_member((int)q);
+        else ms.active = true;
+        left += ms.active;
+      }
+      phase_ = left ? 2 : 3;
+      continue;
+    }
+    // phase 2: one iteration queued beyond the awaited one unless every
+    // active member's residual history predicts that the awaited one converges
+    bool ahead = false;
+    for (const MemberState &ms : st_) ahead = ahead || (ms.active && ms.ahead);
+    while (issued_ < max_iter_ && issued_ < it_ + (ahead ? 2 : 1)) issue(issued_++);
+    if (!ready(ev_[it_ & 1])) return false;
+    int left = 0;
+    for (size_t q = 0; q < m_.size(); ++q) {
+      MemberState &ms = st_[q];
+      if (!ms.active) continue;
+      const volatile double *slot = m_[q].W->hring + 2 * (it_ & 1);
+      if (slot[1] != (double)it_) throw Error(UFVM_ERR_STATE, "FPCG residual ring out of step");
+      const double rr2 = slot[0];
+      ms.rr = std::sqrt(rr2);
+      ms.its = it_ + 1;
+      if (std::isnan(ms.rr)) {
+        // the member's own solve would stop here with this error; the device
+        // flag keeps its launches off from now on
+        ms.err = "FPCG breakdown (NaN residual)";
+        k::fcg_arm(s_, m_[q].W->scal.p, ms.thr, 1);
+        finish_member((int)q);
+        continue;
+      }
+      if (rr2 <= ms.thr || ms.its >= max_iter_) {
+        finish_member((int)q);
+        continue;
+      }
+      if (ms.prev2 > 0.0) ms.ahead = rr2 * (rr2 / ms.prev2) > 4.0 * ms.thr;
+      ms.prev2 = rr2;
+      ++left;
+    }
+    ++it_;
+    if (left == 0) phase_ = 3;
+  }
+}
+
+void pcg_solve_group(ufvm_ctx *ctx, const std::vector<System *> &sys, const double *const *b,
+                     double *const *x, int32_t *iters, double *relres) {
+  if (sys.empty()) throw Error(UFVM_ERR_ARG, "an empty group");
+  if (!b || !x) throw Error(UFVM_ERR_ARG, "null right-hand sides or solutions");
+  std::vector<FcgGroup::Member> m;
+  for (size_t q = 0; q < sys.size(); ++q) {
+    System &S = *sys[q];
+    if (!S.pcg.ready || S.pcg.stale || !S.pcg.work0)
+      throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first (or refresh after new coefficients)");
+    if (!b[q] || !x[q]) throw Error(UFVM_ERR_ARG, "null right-hand side or solution");
+    for (size_t o = 0; o < q; ++o)
+      if (sys[o] == sys[q])
+        throw Error(UFVM_ERR_ARG, "a system appears twice (ufvm_pcg_solve_batch solves several "
+                                  "right-hand sides of one system)");
+    SolveWork &W = *S.pcg.work0;
+    UFVM_HIP(hipMemcpyAsync(W.bref.p, b[q], (size_t)W.n0 * sizeof(double),
+                            hipMemcpyHostToDevice, ctx->stream));
+    m.push_back({&S, &W, W.bref.p, W.xref.p, x[q]});
+  }
+  FcgGroup grp(ctx, m, ctx->stream);
+  grp.start();
+  while (!grp.poll(true)) {
+  }
+  for (int q = 0; q < grp.size(); ++q) {
+    if (!grp.member_error(q).empty()) throw Error(UFVM_ERR_NUMERIC, grp.member_error(q));
+    if (iters) iters[q] = grp.iterations(q);
+    if (relres) relres[q] = grp.relres(q);
+  }
 }
 
 }  // namespace detail
