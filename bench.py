@@ -188,7 +188,8 @@ def serving_leg(args):
         pool.append((spec, m))
     log(f"serving pool: {[(s_, m.n_cells) for s_, m in pool]} ({time.time()-t:.1f}s)")
     srv = Server(max_active=args.serving_active, n_streams=args.serving_streams,
-                 max_batch=args.serving_max_batch, n_steppers=args.serving_steppers)
+                 max_batch=args.serving_max_batch, n_steppers=args.serving_steppers,
+                 max_group=args.serving_max_group, group_span=args.serving_group_span)
     t = time.time()
     sids, cids = [], []
     for spec, m in pool:
@@ -209,6 +210,7 @@ def serving_leg(args):
     reqs = request_stream(args.serving_requests, args.serving_rate, sizes, seed=2030)
     wall = run_stream(srv, reqs, sids, cids, lambda r: rhs[r.mesh])
     srv.close()
+    saturated = saturated_solves(args, pool, rhs) if args.serving_burst > 0 else None
     lat = np.array([r.latency for r in reqs])
     out = {"metric": "serving: completed requests/s and p50/p99 latency (submit -> result)",
            "value": round(len(reqs) / wall, 3), "unit": "requests/s", "n_gpus": 1,
@@ -221,6 +223,8 @@ def serving_leg(args):
                       "meshes": {s_: int(m.n_cells) for s_, m in pool},
                       "max_active": args.serving_active, "streams": args.serving_streams,
                       "max_batch": args.serving_max_batch,
+                      "max_group": args.serving_max_group,
+                      "group_span": args.serving_group_span,
                       "step_lanes": args.serving_steppers,
                       "momentum_smoother": args.serving_u_smoother,
                       "tol": 1e-8, "precision": "f32 V-cycle, f64 FPCG"},
@@ -231,11 +235,90 @@ def serving_leg(args):
             out[kind] = {"count": int(l.size), "p50_ms": round(float(np.median(l)) * 1e3, 2),
                          "p99_ms": round(float(np.percentile(l, 99)) * 1e3, 2)}
     bs = [r.extra.get("batch", 1) for r in reqs if r.kind == "solve"]
+    gs = [r.extra.get("group", 1) for r in reqs if r.kind == "solve"]
     if bs:
+        # how many solves each solve's launches served: its batch (same
+        # system) or its group (different systems)
         out["solve"]["batch_sizes"] = {str(k): bs.count(k) for k in sorted(set(bs))}
+        out["solve"]["group_sizes"] = {str(k): gs.count(k) for k in sorted(set(gs))}
+        shared = sum(1 for b_, g_ in zip(bs, gs) if b_ > 1 or g_ > 1)
+        out["solve"]["launches_shared_frac"] = round(shared / len(bs), 3)
+    if saturated:
+        out["saturated_solves"] = saturated
     log(f"serving: {out['value']} requests/s, p50 {out['latency_ms']['p50']} ms, "
         f"p99 {out['latency_ms']['p99']} ms over {len(reqs)} requests")
     return out
+
+
+def saturated_solves(args, pool, rhs):
+    """A burst of solve requests submitted at once over the same mesh pool
+    (the queue stays full, so admission forms groups of solves of different
+    meshes), served with groups of up to --serving-max-group and without
+    groups; batches off in both, so that only grouping differs.  Every
+    grouped result is checked against the ungrouped one bit for bit."""
+    from matlab_ufvm_amd.workloads import cavity_mesh, gaussian_rhs
+    # eight distinct meshes of about 50k cells (four 3-D, four 2-D): solves
+    # there are launch- and latency-bound, where groups pay
+    fleet = []
+    for dim, n in (("3d", 36), ("3d", 37), ("3d", 38), ("3d", 39),
+                   ("2d", 216), ("2d", 220), ("2d", 224), ("2d", 228)):
+        m = cavity_mesh(n, nz=1 if dim == "2d" else None, jitter=0.15, seed=3026 + n,
+                        renumber_seed=3027 + n, apex="own")
+        fleet.append((f"{dim}:{n}", m))
+    frhs = [gaussian_rhs(m, seed=3030 + i) for i, (_, m) in enumerate(fleet)]
+    return {"pool": burst(args, pool, rhs), "fleet_50k": burst(args, fleet, frhs, uniform=True)}
+
+
+def burst(args, pool, rhs, uniform=False):
+    """one saturated burst over `pool`, grouped and ungrouped (see
+    saturated_solves); uniform: each request's mesh drawn uniformly from the
+    pool (not by log-uniform size)"""
+    from matlab_ufvm_amd.serving import Server, request_stream
+    from matlab_ufvm_amd.workloads import poisson_system
+    sizes = [m.n_cells for _, m in pool]
+    res = {"meshes": sizes}
+    xs = {}
+    for label, mg, lm in (("grouped", args.serving_max_group, "auto"),
+                          ("grouped_graphs", args.serving_max_group, "graphs"),
+                          ("ungrouped", 1, "auto")):
+        srv = Server(max_active=args.serving_active, n_streams=args.serving_streams,
+                     max_batch=1, n_steppers=1, max_group=mg,
+                     group_span=args.serving_group_span)
+        sids = []
+        for _, m in pool:
+            ptr, col, ac, anb = poisson_system(m, dirichlet=())
+            sids.append(srv.add_system(ptr, col, ac, anb, tol=1e-8, launch_mode=lm))
+        for i in range(len(pool)):
+            srv.wait(srv.submit_solve(sids[i], rhs[i]))     # first-use set-up, untimed
+        reqs = request_stream(args.serving_burst, 1e9, sizes, seed=2031, p_step=0.0)
+        if uniform:
+            pick = np.random.default_rng(2032).integers(0, len(pool), len(reqs))
+            for r, q in zip(reqs, pick):
+                r.mesh = int(q)
+        outs = {}
+        t0 = time.perf_counter()
+        tickets = [srv.submit_solve(sids[r.mesh], rhs[r.mesh]) for r in reqs]
+        for r, t in zip(reqs, tickets):
+            o = srv.wait(t)
+            r.latency = o["queued_s"] + o["run_s"]
+            outs[t] = o
+        wall = time.perf_counter() - t0
+        srv.close()
+        lat = np.array([r.latency for r in reqs])
+        gs = [o["group"] for o in outs.values()]
+        xs[label] = [outs[t]["x"] for t in tickets]
+        res[label] = {"solves_per_s": round(len(reqs) / wall, 2),
+                      "p50_ms": round(float(np.median(lat)) * 1e3, 2),
+                      "p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 2),
+                      "group_sizes": {str(k): gs.count(k) for k in sorted(set(gs))}}
+        log(f"saturated solves, {label}: {res[label]}")
+    res["requests"] = args.serving_burst
+    res["same_bits_grouped_and_alone"] = all(
+        np.array_equal(a, b) and np.array_equal(a, c)
+        for a, b, c in zip(xs["grouped"], xs["ungrouped"], xs["grouped_graphs"]))
+    res["speedup_grouped"] = round(res["grouped"]["solves_per_s"] /
+                                   res["ungrouped"]["solves_per_s"], 3)
+    return res
 
 
 def simple_leg(args, ctx, m, steps, p_solver, u_smoother):
@@ -552,11 +635,18 @@ def main():
                     help="SIMPLE steps of the default line's precision-form leg (0: skip)")
     ap.add_argument("--serving-meshes", default="3d:37,3d:64,3d:100,3d:128,2d:224,2d:512,"
                                                 "2d:1024,2d:2048")
-    ap.add_argument("--serving-requests", type=int, default=200)
+    ap.add_argument("--serving-requests", type=int, default=500)
     ap.add_argument("--serving-steppers", type=int, default=4,
                     help="SIMPLE-step lanes of the server")
     ap.add_argument("--serving-max-batch", type=int, default=8, choices=[1, 2, 4, 8],
                     help="queued solves of one mesh admitted together as one batch")
+    ap.add_argument("--serving-max-group", type=int, default=8,
+                    help="queued solves of different meshes admitted together as one group "
+                         "(1: none)")
+    ap.add_argument("--serving-group-span", type=float, default=4.0,
+                    help="largest cell-count ratio inside a group")
+    ap.add_argument("--serving-burst", type=int, default=200,
+                    help="solve requests of the saturated burst (0: skip)")
     ap.add_argument("--serving-u-smoother", choices=["ILU", "MCGS"], default="MCGS",
                     help="momentum solves of the SIMPLE-step requests")
     ap.add_argument("--serving-rate", type=float, default=200.0)

@@ -392,6 +392,14 @@ typedef struct {
                                 (the least-loaded lane by cells at ufvm_server_add_case), a
                                 context, a stream and a thread, so steps of cases in
                                 different lanes overlap on the device */
+  int32_t max_group;         /* queued solves of different systems below 600k cells (where
+                                launches and latency dominate) admitted together as one
+                                group of up to this many (0: 8; 1: no groups) when no batch
+                                forms: every phase of their FCG iterations is one launch for
+                                all of them (ufvm_pcg_solve_group), and each keeps the bits
+                                of its own solve */
+  double group_span;         /* a group's systems differ in cell count by at most this factor
+                                (0: 4), so that small solves are not paced by large ones */
 } ufvm_server_cfg;
 typedef struct {
   int32_t status;            /* 0 queued, 1 running, 2 done, < 0: -UFVM_ERR_* */
@@ -400,6 +408,8 @@ typedef struct {
   double queued_s;           /* submit -> admission (seconds) */
   double run_s;              /* admission -> result on the host */
   int32_t batch;             /* solve: solves in the batch it ran in (1: alone) */
+  int32_t group;             /* solve: solves of different systems whose launches it shared
+                                (1: none) */
 } ufvm_request_info;
 
 int ufvm_server_create(int32_t device, const ufvm_server_cfg *cfg, ufvm_server **out);

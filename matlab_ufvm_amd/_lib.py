@@ -44,12 +44,14 @@ class PcgCfg(C.Structure):
 
 class ServerCfg(C.Structure):
     _fields_ = [("max_active", C.c_int32), ("n_streams", C.c_int32), ("hbm_reserve", C.c_double),
-                ("max_batch", C.c_int32), ("n_steppers", C.c_int32)]
+                ("max_batch", C.c_int32), ("n_steppers", C.c_int32), ("max_group", C.c_int32),
+                ("group_span", C.c_double)]
 
 
 class RequestInfo(C.Structure):
     _fields_ = [("status", C.c_int32), ("iterations", C.c_int32), ("relres", C.c_double),
-                ("queued_s", C.c_double), ("run_s", C.c_double), ("batch", C.c_int32)]
+                ("queued_s", C.c_double), ("run_s", C.c_double), ("batch", C.c_int32),
+                ("group", C.c_int32)]
 
 
 class MeshDesc(C.Structure):

@@ -1382,10 +1382,9 @@ FcgGroup::FcgGroup(ufvm_ctx *ctx, std::vector<Member> members, hipStream_t s)
     for (size_t o = 0; o < q; ++o)
       if (m_[o].W == mb.W) throw Error(UFVM_ERR_ARG, "two group members share a workspace");
   }
-  bool small = true;
-  for (const Member &mb : m_) small = small && mb.W->n0 < FcgRun::kGraphRows;
-  const int mode = m_[0].S->pcg.cfg.launch_mode;
-  graphs_ = !ctx->prof_on && (mode == 2 || (mode == 0 && small));
+  // a group lives for one solve each, so a captured iteration would be
+  // replayed only some tens of times: graphs only when asked for (launch_mode 2)
+  graphs_ = !ctx->prof_on && m_[0].S->pcg.cfg.launch_mode == 2;
   for (hipEvent_t &e : ev_) UFVM_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   for (MemberState &ms : st_) UFVM_HIP(hipEventCreateWithFlags(&ms.ev, hipEventDisableTiming));
   if (precision_ == 32) build<float>();

@@ -12,7 +12,10 @@
 // cases, a context (stream) and a thread, so steps of cases in different
 // lanes overlap each other and the solves on the device.
 // Each solve's arithmetic depends only on its system and right-hand side, so
-// a result is the same alone and in any mix.
+// a result is the same alone and in any mix.  When no batch forms, queued
+// solves of different systems of comparable size are admitted together as one
+// group (FcgGroup): every phase of their FCG iterations is one launch for
+// all of them, and each finishes (and is answered) as soon as it converges.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -49,11 +52,13 @@ struct Request {
   std::string err;
   int32_t iterations = 0;
   int32_t batch = 0;
+  int32_t group = 0;
   double relres = 0.0;
   double t_submit = 0.0, t_start = 0.0, t_done = 0.0;
 };
 
-// One admitted solve, or a batch of k solves of one system (FcgBatch).
+// One admitted solve, a batch of k solves of one system (FcgBatch), or a
+// group of solves of different systems (FcgGroup).
 struct Slot {
   std::vector<Request *> reqs;   // empty: free
   System *sys = nullptr;
@@ -61,6 +66,9 @@ struct Slot {
   std::unique_ptr<SolveWork> work;
   std::unique_ptr<FcgRun> run;     // k == 1
   std::unique_ptr<FcgBatch> brun;  // k > 1
+  std::unique_ptr<FcgGroup> grun;  // a group: one workspace per member, pooled only
+  std::vector<std::unique_ptr<SolveWork>> gwork;   // once the whole group has drained
+  std::vector<char> answered;      // group members already completed
   hipEvent_t copied = nullptr;
   int phase = 0;   // 1 solving, 2 result copies queued
 };
@@ -143,40 +151,82 @@ bool fits(ufvm_server *srv, size_t bytes) {
   return (double)free_b - (double)bytes > reserve;
 }
 
+System *lookup(ufvm_server *srv, int32_t id) {
+  std::lock_guard<std::mutex> gs(srv->sys_mu);
+  auto it = srv->ctx_p->systems.find(id);
+  return it == srv->ctx_p->systems.end() ? nullptr : it->second.get();
+}
+
 // Admit queued solves into free slots (worker thread).  The oldest queued
 // solve is admitted together with later queued solves of the same system, as
 // one batch of the largest size in {8, 4, 2} (up to max_batch) they fill.
+// Otherwise later queued solves of other systems within group_span of its
+// size join it as one group (up to max_group, one solve per system), as far
+// as the device has room for their workspaces.
 void admit(ufvm_server *srv) {
   for (Slot &sl : srv->slots) {
     if (!sl.reqs.empty()) continue;
     std::vector<Request *> group;
-    std::unique_ptr<SolveWork> W;
-    System *S = nullptr;
+    std::vector<System *> sys;
+    std::vector<std::unique_ptr<SolveWork>> works;   // null: to be made
+    int k = 1;
     {
       std::lock_guard<std::mutex> g(srv->mu);
       if (srv->pend_solve.empty()) return;
       Request *r = srv->pend_solve.front();
-      {
-        std::lock_guard<std::mutex> gs(srv->sys_mu);
-        auto it = srv->ctx_p->systems.find(r->id);
-        S = it == srv->ctx_p->systems.end() ? nullptr : it->second.get();
-      }
+      System *S = lookup(srv, r->id);
       group.push_back(r);
+      sys.push_back(S);
       if (S && srv->cfg.max_batch > 1 && batch_supported(*S))
         for (auto it = srv->pend_solve.begin() + 1;
              it != srv->pend_solve.end() && (int)group.size() < srv->cfg.max_batch; ++it)
           if ((*it)->id == r->id) group.push_back(*it);
-      int k = 1;
       while (k * 2 <= (int)group.size()) k *= 2;
       group.resize(k);
-      if (S) {
-        auto &fl = srv->pool[{r->id, k}];
-        if (!fl.empty()) {
-          W = std::move(fl.back());
-          fl.pop_back();
-        } else if (!fits(srv, work_bytes(*S, k))) {
-          return;   // admission by free memory: wait for running solves to finish
+      // groups of launch-bound solves only: on systems as large as the graph
+      // threshold the kernels are bandwidth-bound, and solves on separate
+      // streams overlap better than in lock step
+      if (S && k == 1 && srv->cfg.max_group > 1 && group_supported(*S) &&
+          S->lev[0]->n < FcgRun::kGraphRows) {
+        const double n0 = S->lev[0]->n;
+        for (auto it = srv->pend_solve.begin() + 1;
+             it != srv->pend_solve.end() && (int)group.size() < srv->cfg.max_group; ++it) {
+          System *T = lookup(srv, (*it)->id);
+          if (!T || !group_supported(*T) || T->lev[0]->n >= FcgRun::kGraphRows ||
+              std::find(sys.begin(), sys.end(), T) != sys.end())
+            continue;
+          if (T->pcg.cfg.precision != S->pcg.cfg.precision ||
+              T->pcg.cfg.max_iter != S->pcg.cfg.max_iter)
+            continue;
+          const double n1 = T->lev[0]->n;
+          if (std::max(n0, n1) > srv->cfg.group_span * std::min(n0, n1)) continue;
+          group.push_back(*it);
+          sys.push_back(T);
         }
+      }
+      if (S) {
+        // workspaces (one k-solve workspace for a single solve or a batch, one
+        // each for a group's members): pooled ones, then new ones while the
+        // device has room; a group is cut where the room ends
+        const size_t nw = k > 1 ? 1 : group.size();
+        size_t need = 0;
+        for (size_t q = 0; q < nw; ++q) {
+          auto &fl = srv->pool[{group[q]->id, k}];
+          if (!fl.empty()) {
+            works.push_back(std::move(fl.back()));
+            fl.pop_back();
+            continue;
+          }
+          need += work_bytes(*sys[q], k);
+          if (!fits(srv, need)) {
+            if (k > 1) group.clear();
+            else group.resize(q);
+            sys.resize(group.size());
+            break;
+          }
+          works.push_back(nullptr);
+        }
+        if (group.empty()) return;   // admission by free memory: wait for running solves
       }
       for (Request *q : group) {
         srv->pend_solve.erase(std::find(srv->pend_solve.begin(), srv->pend_solve.end(), q));
@@ -184,39 +234,121 @@ void admit(ufvm_server *srv) {
         q->t_start = now_s();
       }
     }
-    if (!S) {
+    if (!sys[0]) {
       complete(srv, group[0], -UFVM_ERR_ARG, "unknown system");
       continue;
     }
-    const int k = (int)group.size();
+    const int g = (int)group.size();
     try {
-      if (!W) {
-        W = make_work(*S, sl.stream, k);
-        std::lock_guard<std::mutex> g(srv->mu);
-        srv->work_bytes_held += W->bytes;
-      }
-      for (int q = 0; q < k; ++q)
-        UFVM_HIP(hipMemcpyAsync(W->bref.p + (int64_t)q * W->n0, group[q]->b,
-                                (size_t)W->n0 * sizeof(double), hipMemcpyHostToDevice,
-                                sl.stream));
-      if (k == 1) {
-        sl.run.reset(new FcgRun(srv->ctx_p, *S, *W, sl.stream));
-        sl.run->start(W->bref.p, W->xref.p);
+      for (int q = 0; q < (int)works.size(); ++q)
+        if (!works[q]) {
+          works[q] = make_work(*sys[q], sl.stream, k);
+          std::lock_guard<std::mutex> lk(srv->mu);
+          srv->work_bytes_held += works[q]->bytes;
+        }
+      if (g > 1 && k == 1) {
+        std::vector<FcgGroup::Member> mem;
+        for (int q = 0; q < g; ++q) {
+          SolveWork &W = *works[q];
+          UFVM_HIP(hipMemcpyAsync(W.bref.p, group[q]->b, (size_t)W.n0 * sizeof(double),
+                                  hipMemcpyHostToDevice, sl.stream));
+          mem.push_back({sys[q], &W, W.bref.p, W.xref.p, group[q]->x});
+        }
+        sl.grun.reset(new FcgGroup(srv->ctx_p, mem, sl.stream));
+        sl.grun->start();
+        sl.gwork = std::move(works);
+        sl.answered.assign(g, 0);
       } else {
-        sl.brun.reset(new FcgBatch(srv->ctx_p, *S, *W, sl.stream));
-        sl.brun->start(W->bref.p, W->xref.p);
+        std::unique_ptr<SolveWork> &W = works[0];
+        for (int q = 0; q < k; ++q)
+          UFVM_HIP(hipMemcpyAsync(W->bref.p + (int64_t)q * W->n0, group[q]->b,
+                                  (size_t)W->n0 * sizeof(double), hipMemcpyHostToDevice,
+                                  sl.stream));
+        if (k == 1) {
+          sl.run.reset(new FcgRun(srv->ctx_p, *sys[0], *W, sl.stream));
+          sl.run->start(W->bref.p, W->xref.p);
+        } else {
+          sl.brun.reset(new FcgBatch(srv->ctx_p, *sys[0], *W, sl.stream));
+          sl.brun->start(W->bref.p, W->xref.p);
+        }
+        sl.work = std::move(W);
       }
       sl.reqs = group;
-      sl.sys = S;
-      sl.work = std::move(W);
+      sl.sys = sys[0];
       sl.phase = 1;
-      std::lock_guard<std::mutex> g(srv->mu);
-      srv->n_active += k;
+      std::lock_guard<std::mutex> lk(srv->mu);
+      srv->n_active += g;
     } catch (const Error &e) {
       (void)hipStreamSynchronize(sl.stream);
+      sl.grun.reset();
+      {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        for (size_t q = 0; q < works.size(); ++q)
+          if (works[q]) srv->pool[{group[q]->id, k}].push_back(std::move(works[q]));
+        for (size_t q = 0; q < sl.gwork.size(); ++q)
+          if (sl.gwork[q]) srv->pool[{group[q]->id, 1}].push_back(std::move(sl.gwork[q]));
+        sl.gwork.clear();
+      }
       for (Request *q : group) complete(srv, q, -e.code, e.what());
     }
   }
+}
+
+// Advance a group slot: answer each member as soon as its solution is on the
+// host; pool the workspaces once the whole group has drained.
+bool progress_group(ufvm_server *srv, Slot &sl) {
+  bool moved = false;
+  const int g = (int)sl.reqs.size();
+  auto drain = [&]() {
+    std::lock_guard<std::mutex> lk(srv->mu);
+    for (int q = 0; q < g; ++q)
+      if (sl.gwork[q]) srv->pool[{sl.reqs[q]->id, 1}].push_back(std::move(sl.gwork[q]));
+    sl.gwork.clear();
+  };
+  try {
+    const bool all = sl.grun->poll(false);
+    for (int q = 0; q < g; ++q) {
+      if (sl.answered[q] || !sl.grun->member_done(q)) continue;
+      Request *r = sl.reqs[q];
+      const std::string &err = sl.grun->member_error(q);
+      {
+        std::lock_guard<std::mutex> lk(srv->mu);
+        r->iterations = sl.grun->iterations(q);
+        r->relres = sl.grun->relres(q);
+        r->batch = 1;
+        r->group = g;
+        srv->n_active -= 1;
+      }
+      complete(srv, r, err.empty() ? 2 : -UFVM_ERR_NUMERIC, err);
+      sl.answered[q] = 1;
+      moved = true;
+    }
+    if (all) {
+      for (int q = 0; q < g; ++q)
+        if (!sl.answered[q]) throw Error(UFVM_ERR_STATE, "group drained with a member unanswered");
+      drain();
+      sl.grun.reset();
+      sl.reqs.clear();
+      sl.phase = 0;
+      moved = true;
+    }
+  } catch (const Error &e) {
+    (void)hipStreamSynchronize(sl.stream);
+    for (int q = 0; q < g; ++q)
+      if (!sl.answered[q]) {
+        {
+          std::lock_guard<std::mutex> lk(srv->mu);
+          srv->n_active -= 1;
+        }
+        complete(srv, sl.reqs[q], -e.code, e.what());
+      }
+    drain();
+    sl.grun.reset();
+    sl.reqs.clear();
+    sl.phase = 0;
+    moved = true;
+  }
+  return moved;
 }
 
 // Advance every active solve without blocking; true if anything moved.
@@ -224,6 +356,10 @@ bool progress(ufvm_server *srv) {
   bool moved = false;
   for (Slot &sl : srv->slots) {
     if (sl.reqs.empty()) continue;
+    if (sl.grun) {
+      moved = progress_group(srv, sl) || moved;
+      continue;
+    }
     const int k = (int)sl.reqs.size();
     auto release = [&](bool keep_work) {
       std::lock_guard<std::mutex> g(srv->mu);
@@ -249,6 +385,7 @@ bool progress(ufvm_server *srv) {
           for (int i = 0; i < k; ++i) {
             sl.reqs[i]->iterations = k == 1 ? sl.run->iterations() : sl.brun->iterations(i);
             sl.reqs[i]->batch = k;
+            sl.reqs[i]->group = 1;
             sl.reqs[i]->relres = k == 1 ? sl.run->relres() : sl.brun->relres(i);
           }
         }
@@ -341,6 +478,7 @@ void fill_info(const Request *r, ufvm_request_info *info) {
   info->queued_s = r->t_start > 0 ? r->t_start - r->t_submit : now_s() - r->t_submit;
   info->run_s = r->t_done > 0 ? r->t_done - r->t_start : 0.0;
   info->batch = r->batch;
+  info->group = r->group;
 }
 
 }  // namespace
@@ -353,6 +491,8 @@ static void free_server(ufvm_server *srv) {
   for (Slot &sl : srv->slots) {
     sl.run.reset();
     sl.brun.reset();
+    sl.grun.reset();
+    sl.gwork.clear();
     sl.work.reset();
     if (sl.copied) (void)hipEventDestroy(sl.copied);
   }
@@ -377,12 +517,17 @@ int ufvm_server_create(int32_t device, const ufvm_server_cfg *cfg, ufvm_server *
     return UFVM_ERR_ARG;
   const int32_t mb = cfg->max_batch == 0 ? 8 : cfg->max_batch;
   if (mb != 1 && mb != 2 && mb != 4 && mb != 8) return UFVM_ERR_ARG;
+  const int32_t mg = cfg->max_group == 0 ? 8 : cfg->max_group;
+  const double span = cfg->group_span == 0.0 ? 4.0 : cfg->group_span;
+  if (mg < 1 || mg > 64 || !(span >= 1.0)) return UFVM_ERR_ARG;
   const int32_t lanes = cfg->n_steppers == 0 ? 4 : cfg->n_steppers;
   ufvm_server *srv = new (std::nothrow) ufvm_server();
   if (!srv) return UFVM_ERR_ARG;
   srv->device = device;
   srv->cfg = *cfg;
   srv->cfg.max_batch = mb;
+  srv->cfg.max_group = mg;
+  srv->cfg.group_span = span;
   srv->cfg.n_steppers = lanes;
   int rc = ufvm_ctx_create(device, &srv->ctx_p);
   if (rc == UFVM_OK) rc = ufvm_ctx_create(device, &srv->ctx_b);

@@ -30,17 +30,21 @@ STATUS = {0: "queued", 1: "running", 2: "done"}
 
 class Server:
     def __init__(self, device: int = 0, max_active: int = 8, n_streams: int = 4,
-                 hbm_reserve: float = 0.05, max_batch: int = 8, n_steppers: int = 4):
-        """max_active: solves (or batches of solves) on the GPU at once;
-        max_batch: queued solves of one system admitted together as one batch
-        (1, 2, 4 or 8; each still gets the bits of its own solve); n_steppers:
-        SIMPLE-step lanes (each owns some cases, a stream and a thread)."""
+                 hbm_reserve: float = 0.05, max_batch: int = 8, n_steppers: int = 4,
+                 max_group: int = 8, group_span: float = 4.0):
+        """max_active: solves (or batches or groups of solves) on the GPU at
+        once; max_batch: queued solves of one system admitted together as one
+        batch (1, 2, 4 or 8); max_group: queued solves of different systems,
+        within a factor group_span of each other in size, admitted together as
+        one group whose iterations are one launch per phase for all of them
+        (1: none).  Each solve keeps the bits of its own solve either way.
+        n_steppers: SIMPLE-step lanes (each owns some cases, a stream and a
+        thread)."""
         self.lib = _lib.load()
         h = C.c_void_p()
-        rc = self.lib.ufvm_server_create(device, C.byref(ServerCfg(max_active, n_streams,
-                                                                   hbm_reserve, max_batch,
-                                                                   n_steppers)),
-                                         C.byref(h))
+        cfg = ServerCfg(max_active, n_streams, hbm_reserve, max_batch, n_steppers, max_group,
+                        group_span)
+        rc = self.lib.ufvm_server_create(device, C.byref(cfg), C.byref(h))
         if rc != _lib.UFVM_OK:
             raise _lib.UfvmError(rc, "ufvm_server_create failed (no GPU visible?)")
         self.h = h
@@ -66,13 +70,16 @@ class Server:
     # --------------------------------------------------------- registration
     def add_system(self, ptr, col, ac, anb, precision=32, tol=1e-8, pre_sweep=1, post_sweep=1,
                    max_coarse_cells=2000, max_iter=500, coarse_smoother=1, smoother=0,
-                   coarse_scale=2.5, coarse_sweeps=2) -> int:
+                   coarse_scale=2.5, coarse_sweeps=2, launch_mode="auto") -> int:
+        """launch_mode: "auto" (graphs below 600k cells for single and batched
+        solves, eager groups), "eager", or "graphs" (groups captured too)"""
         ptr = np.ascontiguousarray(ptr, np.int32)
         col = np.ascontiguousarray(col, np.int32)
         ac = np.ascontiguousarray(ac, np.float64)
         anb = np.ascontiguousarray(anb, np.float64)
         cfg = PcgCfg(precision, pre_sweep, post_sweep, coarse_smoother, max_coarse_cells, -1,
-                     max_iter, tol, smoother, coarse_scale, coarse_sweeps, coarse_sweeps)
+                     max_iter, tol, smoother, coarse_scale, coarse_sweeps, coarse_sweeps,
+                     _lib.LAUNCH_MODE[launch_mode])
         sid = C.c_int32()
         n = len(ptr) - 1
         self.check(self.lib.ufvm_server_add_system(
@@ -132,7 +139,7 @@ class Server:
         kind = self._live[ticket][0]
         out = {"status": info.status, "iterations": info.iterations, "relres": info.relres,
                "queued_s": info.queued_s, "run_s": info.run_s, "kind": kind,
-               "batch": info.batch}
+               "batch": info.batch, "group": info.group}
         if info.status == 2:
             if kind == "solve":
                 out["x"] = self._live[ticket][2]
@@ -208,4 +215,5 @@ def run_stream(srv: Server, reqs, systems, cases, rhs):
         r.latency = res["queued_s"] + res["run_s"]
         r.iterations = res["iterations"]
         r.extra["batch"] = res["batch"]
+        r.extra["group"] = res["group"]
     return time.perf_counter() - t0

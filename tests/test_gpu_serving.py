@@ -130,6 +130,39 @@ def test_batched_admission_same_bits(meshes):
         srv.close()
 
 
+def test_grouped_admission_same_bits(meshes):
+    # one slot and no batches: solves of different systems queued behind the
+    # first are admitted as groups (each phase of their iterations one launch
+    # for all), and every result has the bits of the solve made alone
+    srv = Server(max_active=1, n_streams=1, max_batch=1, max_group=8)
+    try:
+        sids = _register(srv, meshes)
+        rhs = [gaussian_rhs(meshes[i % 3], seed=120 + i) for i in range(12)]
+        alone = [srv.wait(srv.submit_solve(sids[i % 3], b)) for i, b in enumerate(rhs)]
+        assert all(a["group"] == 1 for a in alone)
+        ts = [srv.submit_solve(sids[i % 3], b) for i, b in enumerate(rhs)]
+        res = [srv.wait(t) for t in ts]
+        groups = [r["group"] for r in res]
+        print(f"\n[serving] group sizes {groups}", flush=True)
+        assert max(groups) > 1
+        for a, r in zip(alone, res):
+            assert r["iterations"] == a["iterations"] and r["relres"] == a["relres"]
+            assert np.array_equal(r["x"], a["x"])
+    finally:
+        srv.close()
+    # a span below the meshes' size ratios forms no group
+    srv = Server(max_active=1, n_streams=1, max_batch=1, max_group=8, group_span=1.1)
+    try:
+        sids = _register(srv, meshes)
+        ts = [srv.submit_solve(sids[i % 3], b) for i, b in enumerate(rhs)]
+        res = [srv.wait(t) for t in ts]
+        assert all(r["group"] == 1 for r in res)
+        for a, r in zip(alone, res):
+            assert np.array_equal(r["x"], a["x"])
+    finally:
+        srv.close()
+
+
 def _submit_unknown(srv, sid, b):
     srv._sizes[sid] = b.size
     return srv.submit_solve(sid, b)
