@@ -445,7 +445,9 @@ int ufvm_server_stats(ufvm_server *srv, int32_t *queued, int32_t *active, int64_
 /* ------------------------------------------------------------- profiling */
 /* Per-kernel-class HIP-event timing on the context stream (no reference
  * counterpart).  kind names: "mcgs0" (finest-level colour passes of the
- * smoother), "spmv0" (finest-level FP64 operator product of the FPCG).
+ * smoother), "spmv0" (finest-level FP64 operator product of the FPCG),
+ * "vcycle" (one whole preconditioner V-cycle), "tail" (the V-cycle below and
+ * including the first level of at most 40,000 rows).
  * on = 0 disables; on = k >= 1 times every k-th launch of each kind. */
 int ufvm_prof_enable(ufvm_ctx *ctx, int32_t on);
 int ufvm_prof_read(ufvm_ctx *ctx, const char *kind, double *total_ms, int64_t *launches,

@@ -50,7 +50,7 @@ struct ProfScope {
   bool on;
   ProfRec rec;
   ProfScope(ufvm_ctx *c, hipStream_t st, const char *kind, double bytes)
-      : ctx(c), s(st), on(c->prof_on) {
+      : ctx(c), s(st), on(c->prof_on && kind && *kind) {
     if (!on) return;
     // sampled launches only: fewer event records keep the host's launch queue
     // ahead of the GPU, so an interval holds the kernel and not a launch wait

@@ -306,6 +306,7 @@ struct Prec<double> {
 // Jacobi levels with at most this many rows fuse the prolongation into the
 // first post-smoothing sweep.
 constexpr int32_t kFusedProlongRows = 100000;
+constexpr int32_t kTailRows = 40000;   // profiling: levels from here down are "tail"
 
 // One preconditioner V-cycle of system S on workspace W and stream s.
 template <typename V>
@@ -446,7 +447,14 @@ struct Cycle {
       }
       if (pre == 0) k::fill<V>(s, L.n, x, V(0));
       down(L, LW, l, x, 0);
-      V *xc = vcycle(l + 1);
+      V *xc;
+      {
+        // the latency-bound tail: every level from the first of at most
+        // kTailRows rows down (events only when profiling)
+        const bool tail = Pc.lev[l + 1]->n <= kTailRows && L.n > kTailRows;
+        ProfScope ps(ctx, s, tail ? "tail" : "", 0.0);
+        xc = vcycle(l + 1);
+      }
       // small levels are latency-bound: the first post-sweep adds the coarse
       // correction on the fly instead of a separate prolongation launch
       const bool fuse = post > 0 && L.n <= kFusedProlongRows;
@@ -749,7 +757,11 @@ void FcgRun::issue_t(int k) {
   // previous iteration's x/r update (multicolour pre-smoothing only)
   const bool fuse0 = L0.kind == 0 && P.cfg.pre_sweep > 0;
   Cycle<V> cyc{ctx_, *S_, W, s_};
-  const V *z = cyc.vcycle(0, fuse0 && k > 0);
+  const V *z;
+  {
+    ProfScope ps(ctx_, s_, "vcycle", 0.0);
+    z = cyc.vcycle(0, fuse0 && k > 0);
+  }
   k::fcg_dots<V>(s_, n, z, W.w.p, W.r.p, W.partial.p);
   k::fcg_update_p<V>(s_, n, z, W.partial.p, P.singular, k == 0, scal, W.p.p);
   {
