@@ -613,6 +613,8 @@ def main():
                     help="also time k solves of the system at once (ufvm_pcg_solve_batch)")
     ap.add_argument("--launch-mode", choices=["auto", "eager", "graphs"], default="auto",
                     help="FCG iterations as captured HIP graphs or kernel by kernel")
+    ap.add_argument("--index-width", type=int, choices=[0, 4], default=0,
+                    help="sliced-ELL column indices: 0 16-bit offsets where they fit, 4 int32")
     ap.add_argument("--coarse-smoother", type=int, default=1,
                     help="levels below the first: 0 block-local MCGS, 1 damped Jacobi")
     ap.add_argument("--no-cpu", action="store_true")
@@ -687,7 +689,7 @@ def main():
                                    "is not sharded across GPUs; use --gpus 1"}))
         sys.exit(2)
 
-    from matlab_ufvm_amd._lib import Context
+    from matlab_ufvm_amd._lib import Context, UfvmError
     ctx = Context(0)
     m, ptr, col, ac, anb, b = build_problem(args.n, args.workload, ctx)
     sid = ctx.system_create(ptr, col)
@@ -723,8 +725,16 @@ def main():
                       max_coarse_cells=args.max_coarse, tol=args.tol, max_iter=500,
                       coarse_smoother=args.coarse_smoother, smoother=args.smoother,
                       coarse_scale=args.coarse_scale, coarse_pre_sweep=args.coarse_pre,
-                      coarse_post_sweep=args.coarse_post, launch_mode=args.launch_mode)
+                      coarse_post_sweep=args.coarse_post, launch_mode=args.launch_mode,
+                      index_width=args.index_width)
         t_setup = time.time() - t
+        if prec == 32:
+            index_bytes = []
+            for lev in range(nl):
+                try:
+                    index_bytes.append(ctx.pcg_level_layout(sid, lev)[0])
+                except UfvmError:      # past the last level of the precision form
+                    break
         total, lat, its, rr = timed_solves(ctx, sid, args.steps, args.warmup)
         x = ctx.get_dphi(sid, 0)
         res = dict(total=total, lat=lat, its=its, relres=rr, x=x, setup=t_setup)
@@ -802,6 +812,7 @@ def main():
                    "smoother": args.smoother, "coarse_scale": args.coarse_scale,
                    "coarse_sweeps": [args.coarse_pre, args.coarse_post],
                    "launch_mode": args.launch_mode,
+                   "index_bytes_per_level": index_bytes,
                    "post_sweep": args.post, "max_coarse_cells": args.max_coarse,
                    "coarse_smoother": ["block-MCGS", "jacobi"][args.coarse_smoother],
                    "hierarchy_cached": True},

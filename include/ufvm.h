@@ -190,6 +190,11 @@ typedef struct {
   int32_t launch_mode;       /* FCG iterations: 0 auto (replayed as captured HIP graphs below
                                 600000 cells, launched kernel by kernel above), 1 always
                                 kernel by kernel, 2 always graphs */
+  int32_t index_width;       /* column indices of the sliced-ELL level copies: 0 auto (a
+                                uint16 offset per entry from a base per 64-row chunk on
+                                every level whose chunks each span fewer than 65536
+                                columns, int32 elsewhere), 4 always int32.  Results are
+                                the same either way. */
 } ufvm_pcg_cfg;
 
 /* Build the solve-side data of the current hierarchy (colourings, permuted
@@ -225,6 +230,10 @@ int ufvm_pcg_apply_vcycle(ufvm_ctx *ctx, int32_t sys, const double *r, double *z
 /* number of colours of a level's multicolour ordering (after ufvm_pcg_setup) */
 int ufvm_pcg_level_colours(ufvm_ctx *ctx, int32_t sys, int32_t level, int32_t *ncolours,
                            int32_t *colour);
+/* bytes per column index of a level's sliced-ELL copy (2, 4, or 0 when the
+ * level has none) and its slots (entries plus padding) */
+int ufvm_pcg_level_layout(ufvm_ctx *ctx, int32_t sys, int32_t level, int32_t *index_bytes,
+                          int64_t *slots);
 
 
 /* ------------------------------------------------- SIMPLE outer iteration */

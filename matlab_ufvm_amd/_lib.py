@@ -39,7 +39,8 @@ class PcgCfg(C.Structure):
                 ("max_coarse_cells", C.c_int32), ("nullspace", C.c_int32),
                 ("max_iter", C.c_int32), ("tol", C.c_double), ("smoother", C.c_int32),
                 ("coarse_scale", C.c_double), ("coarse_pre_sweep", C.c_int32),
-                ("coarse_post_sweep", C.c_int32), ("launch_mode", C.c_int32)]
+                ("coarse_post_sweep", C.c_int32), ("launch_mode", C.c_int32),
+                ("index_width", C.c_int32)]
 
 
 class ServerCfg(C.Structure):
@@ -111,6 +112,8 @@ _SIGS = {
                                        C.POINTER(f64p), i32p, f64p]),
     "ufvm_pcg_apply_vcycle": (C.c_int, [C.c_void_p, C.c_int32, f64p, f64p]),
     "ufvm_pcg_level_colours": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, i32p, i32p]),
+    "ufvm_pcg_level_layout": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, i32p,
+                                        C.POINTER(C.c_int64)]),
     "ufvm_case_create": (C.c_int, [C.c_void_p, C.POINTER(MeshDesc), C.POINTER(SimpleCfg), i32p]),
     "ufvm_case_destroy": (C.c_int, [C.c_void_p, C.c_int32]),
     "ufvm_case_set_field": (C.c_int, [C.c_void_p, C.c_int32, C.c_char_p, f64p, C.c_int64]),
@@ -323,7 +326,7 @@ class Context:
     def pcg_setup(self, sid, precision=32, pre_sweep=1, post_sweep=1, max_coarse_cells=2000,
                   nullspace=-1, max_iter=500, tol=1e-8, coarse_smoother=1, smoother="MCGS",
                   coarse_scale=2.5, coarse_pre_sweep=2, coarse_post_sweep=2,
-                  launch_mode="auto"):
+                  launch_mode="auto", index_width=0):
         """smoother (finest level): "MCGS" multicolour Gauss-Seidel, "ILU" the
         reference's DILU in its lexicographic order (level-scheduled), "MCILU"
         DILU in the colour order of "MCGS" (one dependency level per colour),
@@ -333,10 +336,11 @@ class Context:
         sweeps below the finest level (-1: as on the finest); the defaults
         2.5 and 2 / 2 were tuned on the 2.1M-cell hex and 5.3M-cell tet
         systems.  launch_mode: "auto" (captured graphs below 600k cells),
-        "eager" or "graphs"."""
+        "eager" or "graphs".  index_width: 0 (16-bit column offsets on the
+        levels where they fit) or 4 (int32 columns everywhere)."""
         cfg = PcgCfg(precision, pre_sweep, post_sweep, coarse_smoother, max_coarse_cells,
                      nullspace, max_iter, tol, PCG_SMOOTHER[smoother], coarse_scale,
-                     coarse_pre_sweep, coarse_post_sweep, LAUNCH_MODE[launch_mode])
+                     coarse_pre_sweep, coarse_post_sweep, LAUNCH_MODE[launch_mode], index_width)
         self.check(self.lib.ufvm_pcg_setup(self.h, sid, C.byref(cfg)))
 
     def pcg_solve(self, sid):
@@ -394,6 +398,13 @@ class Context:
         self.check(self.lib.ufvm_pcg_level_colours(self.h, sid, level, C.byref(nc),
                                                    _p(col, i32p)))
         return nc.value, col
+
+    def pcg_level_layout(self, sid, level):
+        """(bytes per column index of the level's sliced-ELL copy: 2, 4 or 0
+        without one, slots including padding)"""
+        ib, sl = C.c_int32(), C.c_int64()
+        self.check(self.lib.ufvm_pcg_level_layout(self.h, sid, level, C.byref(ib), C.byref(sl)))
+        return ib.value, sl.value
 
     # ------------------------------------------------------------ profiling
     def prof_enable(self, on=True):

@@ -826,6 +826,18 @@ int ufvm_pcg_level_colours(ufvm_ctx *ctx, int32_t sys, int32_t level, int32_t *n
   });
 }
 
+int ufvm_pcg_level_layout(ufvm_ctx *ctx, int32_t sys, int32_t level, int32_t *index_bytes,
+                          int64_t *slots) {
+  return guard(ctx, [&] {
+    System &S = sys_of(ctx, sys);
+    if (!S.pcg.ready) throw Error(UFVM_ERR_STATE, "call ufvm_pcg_setup first");
+    if (level < 0 || level >= S.pcg.nlev) throw Error(UFVM_ERR_ARG, "level out of range");
+    const FastLevel &L = *S.pcg.lev[level];
+    if (index_bytes) *index_bytes = L.sd16.p ? 2 : (L.scol.p ? 4 : 0);
+    if (slots) *slots = L.h_soff.empty() ? 0 : L.h_soff.back();
+  });
+}
+
 int ufvm_prof_enable(ufvm_ctx *ctx, int32_t on) {
   return guard(ctx, [&] {
     if (on < 0) throw Error(UFVM_ERR_ARG, "sampling stride must be >= 0");

@@ -285,4 +285,38 @@ void ilu0_lists(const HostCSR &g, const LowerLists &L, Ilu0Lists &U) {
   }
 }
 
+bool pack_sell_columns(const std::vector<int32_t> &ptr, const std::vector<int32_t> &soff,
+                       const std::vector<int32_t> &scol, int32_t n, std::vector<uint16_t> &d16,
+                       std::vector<int32_t> &base) {
+  const int32_t nq = (n + 63) / 64;
+  d16.assign(scol.size(), 0);
+  base.assign(std::max(nq, 1), 0);
+  for (int32_t q = 0; q < nq; ++q) {
+    const int32_t s0 = soff[q], w = (soff[q + 1] - s0) / 64;
+    const int32_t t0 = q * 64, t1 = std::min(n, t0 + 64);
+    int64_t lo = INT64_MAX, hi = INT64_MIN;
+    for (int32_t t = t0; t < t1; ++t) {
+      const int32_t len = ptr[t + 1] - ptr[t];
+      const int32_t pad = len > 0 ? scol[s0 + (t - t0) + (len - 1) * 64] : t;
+      for (int32_t e = 0; e < w; ++e) {
+        const int32_t v = e < len ? scol[s0 + (t - t0) + e * 64] : pad;
+        lo = std::min<int64_t>(lo, v);
+        hi = std::max<int64_t>(hi, v);
+      }
+    }
+    if (w == 0 || lo > hi) continue;
+    if (hi - lo > 65535) return false;
+    base[q] = (int32_t)lo;
+    for (int32_t t = t0; t < t1; ++t) {
+      const int32_t len = ptr[t + 1] - ptr[t];
+      const int32_t pad = len > 0 ? scol[s0 + (t - t0) + (len - 1) * 64] : t;
+      for (int32_t e = 0; e < w; ++e) {
+        const int32_t v = e < len ? scol[s0 + (t - t0) + e * 64] : pad;
+        d16[s0 + (t - t0) + e * 64] = (uint16_t)(v - lo);
+      }
+    }
+  }
+  return true;
+}
+
 }  // namespace ufvm

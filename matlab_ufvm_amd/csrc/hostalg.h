@@ -72,4 +72,15 @@ struct Ilu0Lists {
 };
 void ilu0_lists(const HostCSR &g, const LowerLists &L, Ilu0Lists &U);
 
+// 16-bit form of a sliced-ELL column array of n rows (64-row chunks, chunk q's
+// slots at [soff[q], soff[q+1]) column-major; row t has ptr[t+1] - ptr[t]
+// real entries, the rest of its slots are padding).  On success base[q] +
+// d16[slot] is scol[slot] for every real entry, padding slots take the row's
+// last real column (the row itself when it has none), and slots of rows past
+// n take base[q].  False, leaving d16 and base unspecified, when a chunk's
+// columns span 65536 or more.
+bool pack_sell_columns(const std::vector<int32_t> &ptr, const std::vector<int32_t> &soff,
+                       const std::vector<int32_t> &scol, int32_t n, std::vector<uint16_t> &d16,
+                       std::vector<int32_t> &base);
+
 }  // namespace ufvm

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 namespace ufvm {
 namespace k {
@@ -22,6 +23,42 @@ constexpr int kXcds = 8;
 __device__ __forceinline__ int32_t xcd_block(int32_t b, int32_t G) {
   const int32_t q = G / kXcds, r = G % kXcds, x = b % kXcds, i = b / kXcds;
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// Column indices of a sliced-ELL copy: one int32 per entry (col), or, when
+// every 64-row chunk's columns lie within 65536 of the chunk's smallest, a
+// uint16 offset per entry (d16) from a base per chunk (base).  Exactly one
+// form is set; padding entries point at a valid row with value 0.
+struct SellCols {
+  const int32_t *col = nullptr;
+  const uint16_t *d16 = nullptr;
+  const int32_t *base = nullptr;
+};
+
+// Device view of chunk q's column indices: col[e] is the column of entry e
+// (an index into the level's entry stream that lies in chunk q).
+template <bool PK>
+struct ColSrc;
+template <>
+struct ColSrc<false> {
+  const int32_t *c;
+  __device__ __forceinline__ ColSrc(const SellCols &s, int32_t) : c(s.col) {}
+  __device__ __forceinline__ int32_t operator[](int32_t e) const { return c[e]; }
+};
+template <>
+struct ColSrc<true> {
+  const uint16_t *d;
+  int32_t b;
+  __device__ __forceinline__ ColSrc(const SellCols &s, int32_t q) : d(s.d16), b(s.base[q]) {}
+  __device__ __forceinline__ int32_t operator[](int32_t e) const { return b + (int32_t)d[e]; }
+};
+// f(std::bool_constant<packed>): launchers pick the kernel of cols' form
+template <typename F>
+inline void with_cols(const SellCols &cols, F &&f) {
+  if (cols.d16)
+    f(std::true_type{});
+  else
+    f(std::false_type{});
 }
 
 // ---- reference-order FP64 kernels (bit-exact with the reference arithmetic)
@@ -79,12 +116,12 @@ void dilu_bwd_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32
 // (the lower sum; nullptr for DILU, where they are A's), and the backward
 // pass gets the factor as val.
 template <typename V>
-void mcilu_fwd_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const int32_t *col,
-                    const V *val, const V *lval, const V *ac, const V *b, const V *x,
+void mcilu_fwd_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off,
+                    const SellCols &cols, const V *val, const V *lval, const V *ac, const V *b, const V *x,
                     const V *dinv, V *rc, bool zero);
 template <typename V>
-void mcilu_bwd_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const int32_t *col,
-                    const V *val, const V *dinv, V *rc, V *x, bool zero);
+void mcilu_bwd_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off,
+                    const SellCols &cols, const V *val, const V *dinv, V *rc, V *x, bool zero);
 // ILU(0) factorisation of the rows [r0, r1) of one colour class (the factor's
 // off-diagonals overwrite F, which holds A's on entry; update lists Ilu0Lists)
 void ilu0_factor_rows(hipStream_t s, int32_t r0, int32_t r1, const double *ac,
@@ -127,16 +164,16 @@ void coarse_lhs(hipStream_t s, int32_t nout, int32_t nc, const int32_t *ptr, con
 // sliced-ELL (64-row chunks, column-major inside a chunk) variants of the
 // colour pass, the residual and the damped Jacobi sweep
 template <typename V>
-void mcgs_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const int32_t *col,
+void mcgs_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const SellCols &cols,
                const V *val, const V *ac, const V *b, V *x, int32_t zero_above);
 // rows below zero_rows get r = 0 without computing it
 template <typename V>
 void residual_sell(hipStream_t s, int32_t n, int32_t zero_rows, const int32_t *off,
-                   const int32_t *col, const V *val, const V *ac, const V *b, const V *x, V *r);
+                   const SellCols &cols, const V *val, const V *ac, const V *b, const V *x, V *r);
 // parent/xc not null: the sweep starts from x + xc[parent] (fused prolongation)
 template <typename V>
-void jacobi_sell(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, const V *val,
-                 const V *ac, const V *b, const V *x, const int32_t *parent, const V *xc, V *out,
+void jacobi_sell(hipStream_t s, int32_t n, const int32_t *off, const SellCols &cols,
+                 const V *val, const V *ac, const V *b, const V *x, const int32_t *parent, const V *xc, V *out,
                  V omega);
 // block-local multicolour GS over blocks of at most kBRows rows (see kernels.hip)
 constexpr int kBRows = 1024;
@@ -189,8 +226,8 @@ void gather_vals(hipStream_t s, int64_t n, const int32_t *idx, const double *src
 void ring_put(hipStream_t s, const double *src, double *ring);
 // FCG(1) iteration kernels (scalar slots of scal[] listed in kernels.hip).
 // w = A p (sliced-ELL finest level); partial[0, g) = per-block p.w
-void spmv_dot(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, const double *ac,
-              const double *val, const double *p, double *w, double *partial);
+void spmv_dot(hipStream_t s, int32_t n, const int32_t *off, const SellCols &cols,
+              const double *ac, const double *val, const double *p, double *w, double *partial);
 // Strides between the right-hand sides of a batch of k solves (the FCG
 // kernels below run the k solves as grid rows blockIdx.y, each exactly as a
 // single solve): FP64 vectors, V-cycle-precision vectors, partials, device
@@ -214,15 +251,15 @@ int fold_blocks(int32_t n);
 // the arithmetic of the single-vector kernel above
 template <typename V>
 void mcgs_sell_b(hipStream_t s, int k, int32_t r0, int32_t r1, const int32_t *off,
-                 const int32_t *col, const V *val, const V *ac, const V *b, V *x, int64_t ld,
+                 const SellCols &cols, const V *val, const V *ac, const V *b, V *x, int64_t ld,
                  int32_t zero_above);
 template <typename V>
 void residual_sell_b(hipStream_t s, int k, int32_t n, int32_t zero_rows, const int32_t *off,
-                     const int32_t *col, const V *val, const V *ac, const V *b, const V *x,
+                     const SellCols &cols, const V *val, const V *ac, const V *b, const V *x,
                      V *r, int64_t ld);
 // parent / xc null: no prolongation fused
 template <typename V>
-void jacobi_sell_b(hipStream_t s, int k, int32_t n, const int32_t *off, const int32_t *col,
+void jacobi_sell_b(hipStream_t s, int k, int32_t n, const int32_t *off, const SellCols &cols,
                    const V *val, const V *ac, const V *b, const V *x, int64_t ld,
                    const int32_t *parent, const V *xc, int64_t ldc, V *out, V omega);
 // acc null: plain restriction, else fused with the first Jacobi sweep (xc)
@@ -237,7 +274,7 @@ template <typename V>
 void dense_solve_b(hipStream_t s, int k, int32_t n, const double *ainv, const V *b, int64_t ld,
                    V *x);
 // w_r = A p_r and the p_r.w_r block partials at partial + r * pstride
-void spmv_dot_b(hipStream_t s, int k, int32_t n, const int32_t *off, const int32_t *col,
+void spmv_dot_b(hipStream_t s, int k, int32_t n, const int32_t *off, const SellCols &cols,
                 const double *ac, const double *val, const double *p, double *w, int64_t ld,
                 double *partial, int64_t pstride);
 // folds spmv_dot's partials (delta, alpha into scal) ; x += alpha p ;
@@ -271,7 +308,7 @@ enum GOp : int32_t {
   G_MCGS,          // colour class [r0, n) Gauss-Seidel pass; aux = zero_above; out = x
   G_RESID,         // r = b - A x, rows < aux set to 0; out = r
   G_JACOBI,        // damped Jacobi (omega); parent != null: x + xc[parent] as input
-  G_RESTRICT,      // bc = scale * sum of children (off/col: child lists, x: fine r);
+  G_RESTRICT,      // bc = scale * sum of children (off/child: child lists, x: fine r);
                    // ac != null: also out2 = omega bc / ac (first Jacobi sweep)
   G_PROLONG,       // x[i] += xc[parent[i]] for rows [r0, n); out = x
   G_DENSE,         // x = Ainv b (FP64 inverse, one wavefront per row); out = x
@@ -279,7 +316,8 @@ enum GOp : int32_t {
 template <typename V>
 struct GArgs {
   int32_t n, r0, aux, nb;
-  const int32_t *off, *col, *parent;
+  const int32_t *off, *child, *parent;
+  SellCols cols;
   const V *val, *ac, *b, *x, *xc;
   V *out, *out2;
   const double *ainv;
@@ -292,7 +330,8 @@ struct GFcgArgs {
   int32_t n, nb, n0, singular;   // rows, fold_blocks(n), colour-0 rows, constant nullspace
   const V *z, *acv;              // preconditioned residual; finest diagonal (V-cycle copy)
   V *rv, *z0;                    // V-cycle right-hand side and iterate of the finest level
-  const int32_t *off, *col;      // FP64 finest operator (sliced-ELL)
+  const int32_t *off;            // FP64 finest operator (sliced-ELL)
+  SellCols cols;
   const double *ac, *val;
   double *p, *w, *x, *r, *partial, *scal, *ring;
 };

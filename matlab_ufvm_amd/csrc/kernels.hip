@@ -705,14 +705,16 @@ void coarse_lhs(hipStream_t s, int32_t nout, int32_t nc, const int32_t *ptr, con
 // ---- kernels on the sliced-ELL layout: rows in chunks of 64, chunk q holds
 // its entries column-major at off[q] + k*64 + lane for k < (off[q+1]-off[q])/64
 // (a wavefront reads 64 consecutive words per k).  Each row's entries keep its
-// cconn order and are followed by padding entries (col = the row itself,
-// value 0), so the loops run a uniform trip count with no exit test and the
-// loads of four entries are issued before their products are summed.
+// cconn order and are followed by padding entries (value 0; their column is
+// the row itself, or in the 16-bit form the row's last real column), so the
+// loops run a uniform trip count with no exit test and the loads of four
+// entries are issued before their products are summed.  Column indices come
+// through ColSrc: int32 per entry, or a chunk base plus a uint16 per entry.
 
 // v -/+ sum_k val_k x[col_k] in entry order (NEG: subtract).  ZA: neighbours
 // with index >= zero_above read as 0.
-template <typename V, bool NEG, bool ZA>
-__device__ __forceinline__ V sell_row(const int32_t *__restrict__ col, const V *__restrict__ val,
+template <typename V, bool NEG, bool ZA, typename CS>
+__device__ __forceinline__ V sell_row(const CS &col, const V *__restrict__ val,
                                       const V *x, int32_t base, int32_t w, V v,
                                       int32_t zero_above) {
   int32_t k = 0;
@@ -743,15 +745,15 @@ __device__ __forceinline__ V sell_row(const int32_t *__restrict__ col, const V *
 // the value from an earlier colour pass.  Neighbours with index >= zero_above
 // belong to colours not yet visited in a sweep that started from x = 0 and
 // are read as 0 (so x needs no clearing first).
-template <typename V>
-__global__ void __launch_bounds__(kBlock)
-k_mcgs_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off,
-            const int32_t *__restrict__ col, const V *__restrict__ val,
-            const V *__restrict__ ac, const V *__restrict__ b, V *x, int32_t zero_above) {
-  const int32_t i = (r0 & ~63) + xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-  if (i < r0 || i >= r1) return;
+// row i of the colour class ending at r1
+template <typename V, bool PK>
+__device__ __forceinline__ void mcgs_row(int32_t i, int32_t r1, const int32_t *__restrict__ off,
+                                         const SellCols &cols, const V *__restrict__ val,
+                                         const V *__restrict__ ac, const V *__restrict__ b, V *x,
+                                         int32_t zero_above) {
   const int32_t q = i >> 6;
   const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
+  const ColSrc<PK> col(cols, q);
   // zero_above beyond this colour's rows: every neighbour holds a value
   const V v = (zero_above <= r1)
                   ? sell_row<V, true, true>(col, val, x, base, w, b[i], zero_above)
@@ -759,13 +761,25 @@ k_mcgs_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off,
   x[i] = v / ac[i];
 }
 
+template <typename V, bool PK>
+__global__ void __launch_bounds__(kBlock)
+k_mcgs_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off, SellCols cols,
+            const V *__restrict__ val, const V *__restrict__ ac, const V *__restrict__ b, V *x,
+            int32_t zero_above) {
+  const int32_t i = (r0 & ~63) + xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i < r0 || i >= r1) return;
+  mcgs_row<V, PK>(i, r1, off, cols, val, ac, b, x, zero_above);
+}
+
 template <typename V>
-void mcgs_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const int32_t *col,
+void mcgs_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const SellCols &cols,
                const V *val, const V *ac, const V *b, V *x, int32_t zero_above) {
   if (r1 <= r0) return;
   const int32_t a = r0 & ~63;
-  hipLaunchKernelGGL(k_mcgs_sell<V>, dim3(nblocks(r1 - a)), dim3(kBlock), 0, s, r0, r1, off, col,
-                     val, ac, b, x, zero_above);
+  with_cols(cols, [&](auto pk) {
+    hipLaunchKernelGGL((k_mcgs_sell<V, decltype(pk)::value>), dim3(nblocks(r1 - a)),
+                       dim3(kBlock), 0, s, r0, r1, off, cols, val, ac, b, x, zero_above);
+  });
 }
 
 // ---- DILU in colour order on the sliced-ELL copy of a level whose colour
@@ -780,16 +794,17 @@ void mcgs_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const 
 // instead of A's: SEP takes the lower ones from lval while the residual keeps
 // val (A); with ZERO, or for the backward pass, the caller passes the factor
 // as val.
-template <typename V, bool ZERO, bool SEP>
+template <typename V, bool ZERO, bool SEP, bool PK>
 __global__ void __launch_bounds__(kBlock)
-k_mcilu_fwd_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off,
-                 const int32_t *__restrict__ col, const V *__restrict__ val,
-                 const V *__restrict__ lval, const V *__restrict__ ac, const V *__restrict__ b,
-                 const V *__restrict__ x, const V *__restrict__ dinv, V *rc) {
+k_mcilu_fwd_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off, SellCols cols,
+                 const V *__restrict__ val, const V *__restrict__ lval, const V *__restrict__ ac,
+                 const V *__restrict__ b, const V *__restrict__ x, const V *__restrict__ dinv,
+                 V *rc) {
   const int32_t i = (r0 & ~63) + xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i < r0 || i >= r1) return;
   const int32_t q = i >> 6;
   const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
+  const ColSrc<PK> col(cols, q);
   V res = ZERO ? b[i] : b[i] - ac[i] * x[i];
   V low = V(0);
   int32_t k = 0;
@@ -823,15 +838,15 @@ k_mcilu_fwd_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off,
   rc[i] = res - low;
 }
 
-template <typename V, bool ZERO>
+template <typename V, bool ZERO, bool PK>
 __global__ void __launch_bounds__(kBlock)
-k_mcilu_bwd_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off,
-                 const int32_t *__restrict__ col, const V *__restrict__ val,
-                 const V *__restrict__ dinv, V *rc, V *x) {
+k_mcilu_bwd_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off, SellCols cols,
+                 const V *__restrict__ val, const V *__restrict__ dinv, V *rc, V *x) {
   const int32_t i = (r0 & ~63) + xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i < r0 || i >= r1) return;
   const int32_t q = i >> 6;
   const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
+  const ColSrc<PK> col(cols, q);
   V v = rc[i];
   int32_t k = 0;
   for (; k + 4 <= w; k += 4) {
@@ -858,85 +873,95 @@ k_mcilu_bwd_sell(int32_t r0, int32_t r1, const int32_t *__restrict__ off,
 }
 
 template <typename V>
-void mcilu_fwd_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const int32_t *col,
-                    const V *val, const V *lval, const V *ac, const V *b, const V *x,
-                    const V *dinv, V *rc, bool zero) {
+void mcilu_fwd_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off,
+                    const SellCols &cols, const V *val, const V *lval, const V *ac, const V *b,
+                    const V *x, const V *dinv, V *rc, bool zero) {
   if (r1 <= r0) return;
-  const int32_t a = r0 & ~63;
+  const dim3 g(nblocks(r1 - (r0 & ~63))), t(kBlock);
   // from zero only the lower sum reads values
   const V *v0 = (zero && lval) ? lval : val;
-  if (zero)
-    hipLaunchKernelGGL((k_mcilu_fwd_sell<V, true, false>), dim3(nblocks(r1 - a)), dim3(kBlock), 0,
-                       s, r0, r1, off, col, v0, v0, ac, b, x, dinv, rc);
-  else if (lval && lval != val)
-    hipLaunchKernelGGL((k_mcilu_fwd_sell<V, false, true>), dim3(nblocks(r1 - a)), dim3(kBlock), 0,
-                       s, r0, r1, off, col, val, lval, ac, b, x, dinv, rc);
-  else
-    hipLaunchKernelGGL((k_mcilu_fwd_sell<V, false, false>), dim3(nblocks(r1 - a)), dim3(kBlock),
-                       0, s, r0, r1, off, col, val, val, ac, b, x, dinv, rc);
+  with_cols(cols, [&](auto pk) {
+    constexpr bool PK = decltype(pk)::value;
+    if (zero)
+      hipLaunchKernelGGL((k_mcilu_fwd_sell<V, true, false, PK>), g, t, 0, s, r0, r1, off, cols,
+                         v0, v0, ac, b, x, dinv, rc);
+    else if (lval && lval != val)
+      hipLaunchKernelGGL((k_mcilu_fwd_sell<V, false, true, PK>), g, t, 0, s, r0, r1, off, cols,
+                         val, lval, ac, b, x, dinv, rc);
+    else
+      hipLaunchKernelGGL((k_mcilu_fwd_sell<V, false, false, PK>), g, t, 0, s, r0, r1, off, cols,
+                         val, val, ac, b, x, dinv, rc);
+  });
 }
 
 template <typename V>
-void mcilu_bwd_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const int32_t *col,
-                    const V *val, const V *dinv, V *rc, V *x, bool zero) {
+void mcilu_bwd_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off,
+                    const SellCols &cols, const V *val, const V *dinv, V *rc, V *x, bool zero) {
   if (r1 <= r0) return;
-  const int32_t a = r0 & ~63;
-  if (zero)
-    hipLaunchKernelGGL((k_mcilu_bwd_sell<V, true>), dim3(nblocks(r1 - a)), dim3(kBlock), 0, s, r0,
-                       r1, off, col, val, dinv, rc, x);
-  else
-    hipLaunchKernelGGL((k_mcilu_bwd_sell<V, false>), dim3(nblocks(r1 - a)), dim3(kBlock), 0, s,
-                       r0, r1, off, col, val, dinv, rc, x);
+  const dim3 g(nblocks(r1 - (r0 & ~63))), t(kBlock);
+  with_cols(cols, [&](auto pk) {
+    constexpr bool PK = decltype(pk)::value;
+    if (zero)
+      hipLaunchKernelGGL((k_mcilu_bwd_sell<V, true, PK>), g, t, 0, s, r0, r1, off, cols, val,
+                         dinv, rc, x);
+    else
+      hipLaunchKernelGGL((k_mcilu_bwd_sell<V, false, PK>), g, t, 0, s, r0, r1, off, cols, val,
+                         dinv, rc, x);
+  });
 }
 
 template void mcilu_fwd_sell<float>(hipStream_t, int32_t, int32_t, const int32_t *,
-                                    const int32_t *, const float *, const float *, const float *,
-                                    const float *, const float *, const float *, float *, bool);
+                                    const SellCols &, const float *, const float *,
+                                    const float *, const float *, const float *, const float *,
+                                    float *, bool);
 template void mcilu_fwd_sell<double>(hipStream_t, int32_t, int32_t, const int32_t *,
-                                     const int32_t *, const double *, const double *,
+                                     const SellCols &, const double *, const double *,
                                      const double *, const double *, const double *,
                                      const double *, double *, bool);
 template void mcilu_bwd_sell<float>(hipStream_t, int32_t, int32_t, const int32_t *,
-                                    const int32_t *, const float *, const float *, float *,
+                                    const SellCols &, const float *, const float *, float *,
                                     float *, bool);
 template void mcilu_bwd_sell<double>(hipStream_t, int32_t, int32_t, const int32_t *,
-                                     const int32_t *, const double *, const double *, double *,
+                                     const SellCols &, const double *, const double *, double *,
                                      double *, bool);
 
 // r = b - A x; rows below zero_rows get r = 0 without computing it (the
 // colour class a Gauss-Seidel sweep solved last, whose residual is rounding
 // noise)
-template <typename V>
+template <typename V, bool PK>
 __device__ __forceinline__ void residual_row(int32_t i, const int32_t *__restrict__ off,
-                                             const int32_t *__restrict__ col,
-                                             const V *__restrict__ val, const V *__restrict__ ac,
-                                             const V *b, const V *x, V *r) {
+                                             const SellCols &cols, const V *__restrict__ val,
+                                             const V *__restrict__ ac, const V *b, const V *x,
+                                             V *r) {
   const int32_t q = i >> 6;
   const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
+  const ColSrc<PK> col(cols, q);
   r[i] = sell_row<V, true, false>(col, val, x, base, w, b[i] - ac[i] * x[i], 0);
 }
 
-template <typename V>
+template <typename V, bool PK>
 __global__ void __launch_bounds__(kBlock)
-k_residual_sell(int32_t n, int32_t zero_rows, const int32_t *__restrict__ off,
-                const int32_t *__restrict__ col, const V *__restrict__ val,
-                const V *__restrict__ ac, const V *__restrict__ b, const V *__restrict__ x,
-                V *__restrict__ r) {
+k_residual_sell(int32_t n, int32_t zero_rows, const int32_t *__restrict__ off, SellCols cols,
+                const V *__restrict__ val, const V *__restrict__ ac, const V *__restrict__ b,
+                const V *__restrict__ x, V *__restrict__ r) {
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
   if (i < zero_rows) {
     r[i] = V(0);
     return;
   }
-  residual_row<V>(i, off, col, val, ac, b, x, r);
+  residual_row<V, PK>(i, off, cols, val, ac, b, x, r);
 }
 
 template <typename V>
 void residual_sell(hipStream_t s, int32_t n, int32_t zero_rows, const int32_t *off,
-                   const int32_t *col, const V *val, const V *ac, const V *b, const V *x, V *r) {
+                   const SellCols &cols, const V *val, const V *ac, const V *b, const V *x,
+                   V *r) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_residual_sell<V>, dim3(nblocks(n)), dim3(kBlock), 0, s, n, zero_rows, off,
-                     col, val, ac, b, x, r);
+  with_cols(cols, [&](auto pk) {
+    hipLaunchKernelGGL((k_residual_sell<V, decltype(pk)::value>), dim3(nblocks(n)), dim3(kBlock),
+                       0, s, n, zero_rows, off, cols, val, ac, b, x, r);
+  });
 }
 
 // v_j = x_j (+ xc[parent_j] when PRO: prolongation fused into the sweep)
@@ -948,15 +973,15 @@ __device__ __forceinline__ V jac_val(const V *x, const int32_t *__restrict__ par
 
 // one damped Jacobi sweep: out = v + omega (b - A v) / ac, v = x or, with
 // PRO, x + xc[parent] (the coarse correction added on the fly)
-template <typename V, bool PRO>
+template <typename V, bool PRO, bool PK>
 __device__ __forceinline__ void jacobi_row(int32_t i, const int32_t *__restrict__ off,
-                                           const int32_t *__restrict__ col,
-                                           const V *__restrict__ val, const V *__restrict__ ac,
-                                           const V *b, const V *x,
+                                           const SellCols &cols, const V *__restrict__ val,
+                                           const V *__restrict__ ac, const V *b, const V *x,
                                            const int32_t *__restrict__ parent, const V *xc,
                                            V *out, V omega) {
   const int32_t q = i >> 6;
   const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
+  const ColSrc<PK> col(cols, q);
   const V vi = jac_val<V, PRO>(x, parent, xc, i), d = ac[i];
   V v = b[i] - d * vi;
   int32_t k = 0;
@@ -980,28 +1005,32 @@ __device__ __forceinline__ void jacobi_row(int32_t i, const int32_t *__restrict_
   out[i] = vi + omega * v / d;
 }
 
-template <typename V, bool PRO>
+template <typename V, bool PRO, bool PK>
 __global__ void __launch_bounds__(kBlock)
-k_jacobi_sell(int32_t n, const int32_t *__restrict__ off, const int32_t *__restrict__ col,
+k_jacobi_sell(int32_t n, const int32_t *__restrict__ off, SellCols cols,
               const V *__restrict__ val, const V *__restrict__ ac, const V *__restrict__ b,
               const V *__restrict__ x, const int32_t *__restrict__ parent,
               const V *__restrict__ xc, V *__restrict__ out, V omega) {
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  jacobi_row<V, PRO>(i, off, col, val, ac, b, x, parent, xc, out, omega);
+  jacobi_row<V, PRO, PK>(i, off, cols, val, ac, b, x, parent, xc, out, omega);
 }
 
 template <typename V>
-void jacobi_sell(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, const V *val,
-                 const V *ac, const V *b, const V *x, const int32_t *parent, const V *xc, V *out,
-                 V omega) {
+void jacobi_sell(hipStream_t s, int32_t n, const int32_t *off, const SellCols &cols,
+                 const V *val, const V *ac, const V *b, const V *x, const int32_t *parent,
+                 const V *xc, V *out, V omega) {
   if (n <= 0) return;
-  if (parent)
-    hipLaunchKernelGGL((k_jacobi_sell<V, true>), dim3(nblocks(n)), dim3(kBlock), 0, s, n, off,
-                       col, val, ac, b, x, parent, xc, out, omega);
-  else
-    hipLaunchKernelGGL((k_jacobi_sell<V, false>), dim3(nblocks(n)), dim3(kBlock), 0, s, n, off,
-                       col, val, ac, b, x, parent, xc, out, omega);
+  const dim3 g(nblocks(n)), t(kBlock);
+  with_cols(cols, [&](auto pk) {
+    constexpr bool PK = decltype(pk)::value;
+    if (parent)
+      hipLaunchKernelGGL((k_jacobi_sell<V, true, PK>), g, t, 0, s, n, off, cols, val, ac, b, x,
+                         parent, xc, out, omega);
+    else
+      hipLaunchKernelGGL((k_jacobi_sell<V, false, PK>), g, t, 0, s, n, off, cols, val, ac, b, x,
+                         parent, xc, out, omega);
+  });
 }
 
 // Block-local multicolour Gauss-Seidel (coarse levels).  Rows are numbered
@@ -1482,9 +1511,10 @@ __device__ __forceinline__ void fold_sums(const double *__restrict__ part, int32
 // and the reduction tree.
 
 // w = A p (sliced-ELL layout) ; partial[b] = p.w over block b's rows
+template <bool PK>
 __device__ __forceinline__ void spmv_dot_body(int32_t blk, int32_t nb, int32_t n,
                                               const int32_t *__restrict__ off,
-                                              const int32_t *__restrict__ col,
+                                              const SellCols &cols,
                                               const double *__restrict__ ac,
                                               const double *__restrict__ val,
                                               const double *__restrict__ p,
@@ -1494,6 +1524,7 @@ __device__ __forceinline__ void spmv_dot_body(int32_t blk, int32_t nb, int32_t n
   for (int32_t i = blk * kFcgBlock + threadIdx.x; i < n; i += nb * kFcgBlock) {
     const int32_t q = i >> 6;
     const int32_t base = off[q] + (i & 63), wq = (off[q + 1] - off[q]) >> 6;
+    const ColSrc<PK> col(cols, q);
     const double pi = p[i];
     const double v = sell_row<double, false, false>(col, val, p, base, wq, ac[i] * pi, 0);
     w[i] = v;
@@ -1503,19 +1534,22 @@ __device__ __forceinline__ void spmv_dot_body(int32_t blk, int32_t nb, int32_t n
   if (threadIdx.x == 0) partial[blk] = d;
 }
 
+template <bool PK>
 __global__ void __launch_bounds__(kFcgBlock)
-k_spmv_dot(int32_t n, const int32_t *__restrict__ off, const int32_t *__restrict__ col,
+k_spmv_dot(int32_t n, const int32_t *__restrict__ off, SellCols cols,
            const double *__restrict__ ac,
            const double *__restrict__ val, const double *__restrict__ p, double *__restrict__ w,
            double *__restrict__ partial) {
   __shared__ double lds[kFcgBlock / kWave];
-  spmv_dot_body(blockIdx.x, gridDim.x, n, off, col, ac, val, p, w, partial, lds);
+  spmv_dot_body<PK>(blockIdx.x, gridDim.x, n, off, cols, ac, val, p, w, partial, lds);
 }
 
-void spmv_dot(hipStream_t s, int32_t n, const int32_t *off, const int32_t *col, const double *ac,
-              const double *val, const double *p, double *w, double *partial) {
-  hipLaunchKernelGGL(k_spmv_dot, dim3(fold_blocks(n)), dim3(kFcgBlock), 0, s, n, off, col, ac,
-                     val, p, w, partial);
+void spmv_dot(hipStream_t s, int32_t n, const int32_t *off, const SellCols &cols,
+              const double *ac, const double *val, const double *p, double *w, double *partial) {
+  with_cols(cols, [&](auto pk) {
+    hipLaunchKernelGGL((k_spmv_dot<decltype(pk)::value>), dim3(fold_blocks(n)), dim3(kFcgBlock),
+                       0, s, n, off, cols, ac, val, p, w, partial);
+  });
 }
 
 // partials of z, z.w, z.r, w, r: partial[c * g + b] for block b of g
@@ -1876,34 +1910,42 @@ __global__ void __launch_bounds__(kBlock) k_group_row(const GArgs<V> *__restrict
   if ((int32_t)blockIdx.x >= a.nb || *a.done != 0.0) return;
   if (OP == G_MCGS) {
     // the member's rows in k_mcgs_sell's XCD-aware order over its own blocks
+    // (each member's column form is uniform over its blocks)
     const int32_t i = (a.r0 & ~63) + xcd_block(blockIdx.x, a.nb) * kBlock + threadIdx.x;
     if (i < a.r0 || i >= a.n) return;
-    const int32_t q = i >> 6;
-    const int32_t base = a.off[q] + (i & 63), w = (a.off[q + 1] - a.off[q]) >> 6;
-    const V v = (a.aux <= a.n)
-                    ? sell_row<V, true, true>(a.col, a.val, a.out, base, w, a.b[i], a.aux)
-                    : sell_row<V, true, false>(a.col, a.val, a.out, base, w, a.b[i], a.aux);
-    a.out[i] = v / a.ac[i];
+    if (a.cols.d16) mcgs_row<V, true>(i, a.n, a.off, a.cols, a.val, a.ac, a.b, a.out, a.aux);
+    else mcgs_row<V, false>(i, a.n, a.off, a.cols, a.val, a.ac, a.b, a.out, a.aux);
   } else if (OP == G_RESID) {
     const int32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.n) return;
     if (i < a.aux) a.out[i] = V(0);
-    else residual_row<V>(i, a.off, a.col, a.val, a.ac, a.b, a.x, a.out);
+    else if (a.cols.d16) residual_row<V, true>(i, a.off, a.cols, a.val, a.ac, a.b, a.x, a.out);
+    else residual_row<V, false>(i, a.off, a.cols, a.val, a.ac, a.b, a.x, a.out);
   } else if (OP == G_JACOBI) {
     const int32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.n) return;
-    if (a.parent)
-      jacobi_row<V, true>(i, a.off, a.col, a.val, a.ac, a.b, a.x, a.parent, a.xc, a.out, a.omega);
-    else
-      jacobi_row<V, false>(i, a.off, a.col, a.val, a.ac, a.b, a.x, nullptr, nullptr, a.out,
-                           a.omega);
+    if (a.parent) {
+      if (a.cols.d16)
+        jacobi_row<V, true, true>(i, a.off, a.cols, a.val, a.ac, a.b, a.x, a.parent, a.xc, a.out,
+                                  a.omega);
+      else
+        jacobi_row<V, true, false>(i, a.off, a.cols, a.val, a.ac, a.b, a.x, a.parent, a.xc,
+                                   a.out, a.omega);
+    } else {
+      if (a.cols.d16)
+        jacobi_row<V, false, true>(i, a.off, a.cols, a.val, a.ac, a.b, a.x, nullptr, nullptr,
+                                   a.out, a.omega);
+      else
+        jacobi_row<V, false, false>(i, a.off, a.cols, a.val, a.ac, a.b, a.x, nullptr, nullptr,
+                                    a.out, a.omega);
+    }
   } else if (OP == G_RESTRICT) {
     const int32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= a.n) return;
     if (a.ac)
-      restrict_row<V, true>(p, a.off, a.col, a.x, a.ac, a.out, a.out2, a.omega, a.scale);
+      restrict_row<V, true>(p, a.off, a.child, a.x, a.ac, a.out, a.out2, a.omega, a.scale);
     else
-      restrict_row<V, false>(p, a.off, a.col, a.x, nullptr, a.out, nullptr, V(0), a.scale);
+      restrict_row<V, false>(p, a.off, a.child, a.x, nullptr, a.out, nullptr, V(0), a.scale);
   } else if (OP == G_PROLONG) {
     const int32_t i = a.r0 + blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.n) return;
@@ -1927,7 +1969,12 @@ __global__ void __launch_bounds__(kFcgBlock) k_group_fcg(const GFcgArgs<V> *__re
     fcg_update_p_body<V>(blockIdx.x, a.nb, a.n, a.z, a.partial, a.nb, a.singular,
                          OP == G_UPDATE_P_FIRST, a.scal, a.p, lds);
   } else if (OP == G_SPMV) {
-    spmv_dot_body(blockIdx.x, a.nb, a.n, a.off, a.col, a.ac, a.val, a.p, a.w, a.partial, lds);
+    if (a.cols.d16)
+      spmv_dot_body<true>(blockIdx.x, a.nb, a.n, a.off, a.cols, a.ac, a.val, a.p, a.w, a.partial,
+                          lds);
+    else
+      spmv_dot_body<false>(blockIdx.x, a.nb, a.n, a.off, a.cols, a.ac, a.val, a.p, a.w,
+                           a.partial, lds);
   } else if (OP == G_UPDATE_XR) {
     // the operator-product partials occupy partial[0, nb); r.r's follow
     fcg_update_xr_body<V>(blockIdx.x, a.nb, a.n, a.partial, a.scal, a.p, a.w, a.x, a.r, a.rv,
@@ -1977,14 +2024,14 @@ template void group_fcg<double>(hipStream_t, GFcgOp, int32_t, int32_t, const GFc
   template void bmcgs<V>(hipStream_t, int32_t, const int32_t *, const int32_t *, int32_t,       \
                          const int32_t *, const int32_t *, const V *, const V *, const V *,     \
                          const V *, const int32_t *, const V *, V *, int, int);                 \
-  template void mcgs_sell<V>(hipStream_t, int32_t, int32_t, const int32_t *, const int32_t *,    \
+  template void mcgs_sell<V>(hipStream_t, int32_t, int32_t, const int32_t *, const SellCols &,   \
                              const V *, const V *, const V *, V *, int32_t);                    \
   template void residual_sell<V>(hipStream_t, int32_t, int32_t, const int32_t *,                \
-                                 const int32_t *, const V *, const V *, const V *, const V *,   \
+                                 const SellCols &, const V *, const V *, const V *, const V *,  \
                                  V *);                                                          \
-  template void jacobi_sell<V>(hipStream_t, int32_t, const int32_t *, const int32_t *, const V *, \
-                               const V *, const V *, const V *, const int32_t *, const V *,     \
-                               V *, V);                                                         \
+  template void jacobi_sell<V>(hipStream_t, int32_t, const int32_t *, const SellCols &,         \
+                               const V *, const V *, const V *, const V *, const int32_t *,     \
+                               const V *, V *, V);                                              \
   template void restrict_jacobi<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,      \
                                    const V *, const V *, V *, V *, V, V);                      \
   template void residual_fast<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,        \

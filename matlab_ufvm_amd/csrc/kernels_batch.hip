@@ -19,9 +19,8 @@ namespace k {
 
 // v[r] -= sum_k val_k x_r[col_k] in entry order, four entries' loads at a
 // time; ZA: neighbours with index >= zero_above read as 0
-template <typename V, int K, bool ZA>
-__device__ __forceinline__ void sell_rows_k(const int32_t *__restrict__ col,
-                                            const V *__restrict__ val, const V *x, int64_t ld,
+template <typename V, int K, bool ZA, typename CS>
+__device__ __forceinline__ void sell_rows_k(const CS &col, const V *__restrict__ val, const V *x, int64_t ld,
                                             int32_t base, int32_t w, V (&v)[K],
                                             int32_t zero_above) {
   int32_t k = 0;
@@ -54,16 +53,16 @@ __device__ __forceinline__ void sell_rows_k(const int32_t *__restrict__ col,
 }
 
 // Gauss-Seidel pass over the colour class [r0, r1) (k_mcgs_sell for K vectors)
-template <typename V, int K>
+template <typename V, int K, bool PK>
 __global__ void __launch_bounds__(kBlock)
-k_mcgs_sell_k(int32_t r0, int32_t r1, const int32_t *__restrict__ off,
-              const int32_t *__restrict__ col, const V *__restrict__ val,
-              const V *__restrict__ ac, const V *__restrict__ b, V *x, int64_t ld,
-              int32_t zero_above) {
+k_mcgs_sell_k(int32_t r0, int32_t r1, const int32_t *__restrict__ off, SellCols cols,
+              const V *__restrict__ val, const V *__restrict__ ac, const V *__restrict__ b, V *x,
+              int64_t ld, int32_t zero_above) {
   const int32_t i = (r0 & ~63) + xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   if (i < r0 || i >= r1) return;
   const int32_t q = i >> 6;
   const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
+  const ColSrc<PK> col(cols, q);
   V v[K];
 #pragma unroll
   for (int r = 0; r < K; ++r) v[r] = b[r * ld + i];
@@ -74,11 +73,10 @@ k_mcgs_sell_k(int32_t r0, int32_t r1, const int32_t *__restrict__ off,
   for (int r = 0; r < K; ++r) x[r * ld + i] = v[r] / d;
 }
 
-template <typename V, int K>
+template <typename V, int K, bool PK>
 __global__ void __launch_bounds__(kBlock)
-k_residual_sell_k(int32_t n, int32_t zero_rows, const int32_t *__restrict__ off,
-                  const int32_t *__restrict__ col, const V *__restrict__ val,
-                  const V *__restrict__ ac, const V *__restrict__ b, const V *__restrict__ x,
+k_residual_sell_k(int32_t n, int32_t zero_rows, const int32_t *__restrict__ off, SellCols cols,
+                  const V *__restrict__ val, const V *__restrict__ ac, const V *__restrict__ b, const V *__restrict__ x,
                   V *__restrict__ rr, int64_t ld) {
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
@@ -89,6 +87,7 @@ k_residual_sell_k(int32_t n, int32_t zero_rows, const int32_t *__restrict__ off,
   }
   const int32_t q = i >> 6;
   const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
+  const ColSrc<PK> col(cols, q);
   const V d = ac[i];
   V v[K];
 #pragma unroll
@@ -99,9 +98,9 @@ k_residual_sell_k(int32_t n, int32_t zero_rows, const int32_t *__restrict__ off,
 }
 
 // damped Jacobi for K vectors; PRO: x + xc[parent] as the sweep's input
-template <typename V, bool PRO, int K>
+template <typename V, bool PRO, int K, bool PK>
 __global__ void __launch_bounds__(kBlock)
-k_jacobi_sell_k(int32_t n, const int32_t *__restrict__ off, const int32_t *__restrict__ col,
+k_jacobi_sell_k(int32_t n, const int32_t *__restrict__ off, SellCols cols,
                 const V *__restrict__ val, const V *__restrict__ ac, const V *__restrict__ b,
                 const V *__restrict__ x, int64_t ld, const int32_t *__restrict__ parent,
                 const V *__restrict__ xc, int64_t ldc, V *__restrict__ out, V omega) {
@@ -109,6 +108,7 @@ k_jacobi_sell_k(int32_t n, const int32_t *__restrict__ off, const int32_t *__res
   if (i >= n) return;
   const int32_t q = i >> 6;
   const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
+  const ColSrc<PK> col(cols, q);
   const V d = ac[i];
   const int32_t pi = PRO ? parent[i] : 0;
   V vi[K], v[K];
@@ -246,9 +246,9 @@ k_dense_gemv_k(int32_t n, const double *__restrict__ ainv, const V *__restrict__
 // w_r = A p_r (FP64, sliced-ELL) and partial[r * pstride + block] = p_r.w_r
 // over the block's rows: the grid-stride row assignment and the block sum of
 // k_spmv_dot, per vector
-template <int K>
+template <int K, bool PK>
 __global__ void __launch_bounds__(kFcgBlock)
-k_spmv_dot_k(int32_t n, const int32_t *__restrict__ off, const int32_t *__restrict__ col,
+k_spmv_dot_k(int32_t n, const int32_t *__restrict__ off, SellCols cols,
              const double *__restrict__ ac, const double *__restrict__ val,
              const double *__restrict__ p, double *__restrict__ w, int64_t ld,
              double *__restrict__ partial, int64_t pstride) {
@@ -259,6 +259,7 @@ k_spmv_dot_k(int32_t n, const int32_t *__restrict__ off, const int32_t *__restri
   for (int32_t i = blockIdx.x * kFcgBlock + threadIdx.x; i < n; i += gridDim.x * kFcgBlock) {
     const int32_t q = i >> 6;
     const int32_t base = off[q] + (i & 63), wq = (off[q + 1] - off[q]) >> 6;
+    const ColSrc<PK> col(cols, q);
     const double a0 = ac[i];
     double pi[K], v[K];
 #pragma unroll
@@ -306,53 +307,62 @@ k_spmv_dot_k(int32_t n, const int32_t *__restrict__ off, const int32_t *__restri
 
 template <typename V>
 void mcgs_sell_b(hipStream_t s, int k, int32_t r0, int32_t r1, const int32_t *off,
-                 const int32_t *col, const V *val, const V *ac, const V *b, V *x, int64_t ld,
+                 const SellCols &cols, const V *val, const V *ac, const V *b, V *x, int64_t ld,
                  int32_t zero_above) {
   if (r1 <= r0) return;
   const int32_t a = r0 & ~63;
   const dim3 g(nblocks(r1 - a)), t(kBlock);
-  switch (k) {
-    case 2: hipLaunchKernelGGL((k_mcgs_sell_k<V, 2>), g, t, 0, s, r0, r1, off, col, val, ac, b, x, ld, zero_above); break;
-    case 4: hipLaunchKernelGGL((k_mcgs_sell_k<V, 4>), g, t, 0, s, r0, r1, off, col, val, ac, b, x, ld, zero_above); break;
-    case 8: hipLaunchKernelGGL((k_mcgs_sell_k<V, 8>), g, t, 0, s, r0, r1, off, col, val, ac, b, x, ld, zero_above); break;
-    default: hipLaunchKernelGGL((k_mcgs_sell_k<V, 1>), g, t, 0, s, r0, r1, off, col, val, ac, b, x, ld, zero_above); break;
-  }
+  with_cols(cols, [&](auto pk) {
+    constexpr bool P = decltype(pk)::value;
+    switch (k) {
+      case 2: hipLaunchKernelGGL((k_mcgs_sell_k<V, 2, P>), g, t, 0, s, r0, r1, off, cols, val, ac, b, x, ld, zero_above); break;
+      case 4: hipLaunchKernelGGL((k_mcgs_sell_k<V, 4, P>), g, t, 0, s, r0, r1, off, cols, val, ac, b, x, ld, zero_above); break;
+      case 8: hipLaunchKernelGGL((k_mcgs_sell_k<V, 8, P>), g, t, 0, s, r0, r1, off, cols, val, ac, b, x, ld, zero_above); break;
+      default: hipLaunchKernelGGL((k_mcgs_sell_k<V, 1, P>), g, t, 0, s, r0, r1, off, cols, val, ac, b, x, ld, zero_above); break;
+    }
+  });
 }
 
 template <typename V>
 void residual_sell_b(hipStream_t s, int k, int32_t n, int32_t zero_rows, const int32_t *off,
-                     const int32_t *col, const V *val, const V *ac, const V *b, const V *x,
+                     const SellCols &cols, const V *val, const V *ac, const V *b, const V *x,
                      V *r, int64_t ld) {
   if (n <= 0) return;
   const dim3 g(nblocks(n)), t(kBlock);
-  switch (k) {
-    case 2: hipLaunchKernelGGL((k_residual_sell_k<V, 2>), g, t, 0, s, n, zero_rows, off, col, val, ac, b, x, r, ld); break;
-    case 4: hipLaunchKernelGGL((k_residual_sell_k<V, 4>), g, t, 0, s, n, zero_rows, off, col, val, ac, b, x, r, ld); break;
-    case 8: hipLaunchKernelGGL((k_residual_sell_k<V, 8>), g, t, 0, s, n, zero_rows, off, col, val, ac, b, x, r, ld); break;
-    default: hipLaunchKernelGGL((k_residual_sell_k<V, 1>), g, t, 0, s, n, zero_rows, off, col, val, ac, b, x, r, ld); break;
-  }
+  with_cols(cols, [&](auto pk) {
+    constexpr bool P = decltype(pk)::value;
+    switch (k) {
+      case 2: hipLaunchKernelGGL((k_residual_sell_k<V, 2, P>), g, t, 0, s, n, zero_rows, off, cols, val, ac, b, x, r, ld); break;
+      case 4: hipLaunchKernelGGL((k_residual_sell_k<V, 4, P>), g, t, 0, s, n, zero_rows, off, cols, val, ac, b, x, r, ld); break;
+      case 8: hipLaunchKernelGGL((k_residual_sell_k<V, 8, P>), g, t, 0, s, n, zero_rows, off, cols, val, ac, b, x, r, ld); break;
+      default: hipLaunchKernelGGL((k_residual_sell_k<V, 1, P>), g, t, 0, s, n, zero_rows, off, cols, val, ac, b, x, r, ld); break;
+    }
+  });
 }
 
 template <typename V, bool PRO>
-static void jacobi_b_t(hipStream_t s, int k, int32_t n, const int32_t *off, const int32_t *col,
+static void jacobi_b_t(hipStream_t s, int k, int32_t n, const int32_t *off, const SellCols &cols,
                        const V *val, const V *ac, const V *b, const V *x, int64_t ld,
                        const int32_t *parent, const V *xc, int64_t ldc, V *out, V omega) {
   const dim3 g(nblocks(n)), t(kBlock);
-  switch (k) {
-    case 2: hipLaunchKernelGGL((k_jacobi_sell_k<V, PRO, 2>), g, t, 0, s, n, off, col, val, ac, b, x, ld, parent, xc, ldc, out, omega); break;
-    case 4: hipLaunchKernelGGL((k_jacobi_sell_k<V, PRO, 4>), g, t, 0, s, n, off, col, val, ac, b, x, ld, parent, xc, ldc, out, omega); break;
-    case 8: hipLaunchKernelGGL((k_jacobi_sell_k<V, PRO, 8>), g, t, 0, s, n, off, col, val, ac, b, x, ld, parent, xc, ldc, out, omega); break;
-    default: hipLaunchKernelGGL((k_jacobi_sell_k<V, PRO, 1>), g, t, 0, s, n, off, col, val, ac, b, x, ld, parent, xc, ldc, out, omega); break;
-  }
+  with_cols(cols, [&](auto pk) {
+    constexpr bool P = decltype(pk)::value;
+    switch (k) {
+      case 2: hipLaunchKernelGGL((k_jacobi_sell_k<V, PRO, 2, P>), g, t, 0, s, n, off, cols, val, ac, b, x, ld, parent, xc, ldc, out, omega); break;
+      case 4: hipLaunchKernelGGL((k_jacobi_sell_k<V, PRO, 4, P>), g, t, 0, s, n, off, cols, val, ac, b, x, ld, parent, xc, ldc, out, omega); break;
+      case 8: hipLaunchKernelGGL((k_jacobi_sell_k<V, PRO, 8, P>), g, t, 0, s, n, off, cols, val, ac, b, x, ld, parent, xc, ldc, out, omega); break;
+      default: hipLaunchKernelGGL((k_jacobi_sell_k<V, PRO, 1, P>), g, t, 0, s, n, off, cols, val, ac, b, x, ld, parent, xc, ldc, out, omega); break;
+    }
+  });
 }
 
 template <typename V>
-void jacobi_sell_b(hipStream_t s, int k, int32_t n, const int32_t *off, const int32_t *col,
+void jacobi_sell_b(hipStream_t s, int k, int32_t n, const int32_t *off, const SellCols &cols,
                    const V *val, const V *ac, const V *b, const V *x, int64_t ld,
                    const int32_t *parent, const V *xc, int64_t ldc, V *out, V omega) {
   if (n <= 0) return;
-  if (parent) jacobi_b_t<V, true>(s, k, n, off, col, val, ac, b, x, ld, parent, xc, ldc, out, omega);
-  else jacobi_b_t<V, false>(s, k, n, off, col, val, ac, b, x, ld, parent, xc, ldc, out, omega);
+  if (parent) jacobi_b_t<V, true>(s, k, n, off, cols, val, ac, b, x, ld, parent, xc, ldc, out, omega);
+  else jacobi_b_t<V, false>(s, k, n, off, cols, val, ac, b, x, ld, parent, xc, ldc, out, omega);
 }
 
 template <typename V, bool JAC>
@@ -403,26 +413,29 @@ void dense_solve_b(hipStream_t s, int k, int32_t n, const double *ainv, const V 
   }
 }
 
-void spmv_dot_b(hipStream_t s, int k, int32_t n, const int32_t *off, const int32_t *col,
+void spmv_dot_b(hipStream_t s, int k, int32_t n, const int32_t *off, const SellCols &cols,
                 const double *ac, const double *val, const double *p, double *w, int64_t ld,
                 double *partial, int64_t pstride) {
   const dim3 g(fold_blocks(n)), t(kFcgBlock);
-  switch (k) {
-    case 2: hipLaunchKernelGGL(k_spmv_dot_k<2>, g, t, 0, s, n, off, col, ac, val, p, w, ld, partial, pstride); break;
-    case 4: hipLaunchKernelGGL(k_spmv_dot_k<4>, g, t, 0, s, n, off, col, ac, val, p, w, ld, partial, pstride); break;
-    case 8: hipLaunchKernelGGL(k_spmv_dot_k<8>, g, t, 0, s, n, off, col, ac, val, p, w, ld, partial, pstride); break;
-    default: hipLaunchKernelGGL(k_spmv_dot_k<1>, g, t, 0, s, n, off, col, ac, val, p, w, ld, partial, pstride); break;
-  }
+  with_cols(cols, [&](auto pk) {
+    constexpr bool P = decltype(pk)::value;
+    switch (k) {
+      case 2: hipLaunchKernelGGL((k_spmv_dot_k<2, P>), g, t, 0, s, n, off, cols, ac, val, p, w, ld, partial, pstride); break;
+      case 4: hipLaunchKernelGGL((k_spmv_dot_k<4, P>), g, t, 0, s, n, off, cols, ac, val, p, w, ld, partial, pstride); break;
+      case 8: hipLaunchKernelGGL((k_spmv_dot_k<8, P>), g, t, 0, s, n, off, cols, ac, val, p, w, ld, partial, pstride); break;
+      default: hipLaunchKernelGGL((k_spmv_dot_k<1, P>), g, t, 0, s, n, off, cols, ac, val, p, w, ld, partial, pstride); break;
+    }
+  });
 }
 
 #define UFVM_INST_B(V)                                                                          \
   template void mcgs_sell_b<V>(hipStream_t, int, int32_t, int32_t, const int32_t *,             \
-                               const int32_t *, const V *, const V *, const V *, V *, int64_t,  \
+                               const SellCols &, const V *, const V *, const V *, V *, int64_t, \
                                int32_t);                                                        \
   template void residual_sell_b<V>(hipStream_t, int, int32_t, int32_t, const int32_t *,         \
-                                   const int32_t *, const V *, const V *, const V *, const V *, \
-                                   V *, int64_t);                                               \
-  template void jacobi_sell_b<V>(hipStream_t, int, int32_t, const int32_t *, const int32_t *,   \
+                                   const SellCols &, const V *, const V *, const V *,           \
+                                   const V *, V *, int64_t);                                    \
+  template void jacobi_sell_b<V>(hipStream_t, int, int32_t, const int32_t *, const SellCols &,  \
                                  const V *, const V *, const V *, const V *, int64_t,           \
                                  const int32_t *, const V *, int64_t, V *, V);                  \
   template void restrict_b<V>(hipStream_t, int, int32_t, const int32_t *, const int32_t *,      \

@@ -15,6 +15,20 @@
 namespace ufvm {
 namespace detail {
 
+// the column indices of a level's sliced-ELL copy, in whichever form it has
+static k::SellCols sell_cols(const FastLevel &L) {
+  k::SellCols c;
+  if (L.sd16.p) {
+    c.d16 = L.sd16.p;
+    c.base = L.scbase.p;
+  } else {
+    c.col = L.scol.p;
+  }
+  return c;
+}
+// bytes per sliced-ELL column index (2: the 16-bit form)
+static double index_bytes(const FastLevel &L) { return L.sd16.p ? 2.0 : 4.0; }
+
 // Precision-form copy of hierarchy level l.  kind 0 (level 1): unknowns
 // renumbered so every colour class of a global greedy colouring is one
 // contiguous row range.  kind 1 (coarse levels): blocks of kBRows consecutive
@@ -162,7 +176,14 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
       }
     }
     FL.soff.upload(FL.h_soff, s);
-    FL.scol.upload(scol, s);
+    std::vector<uint16_t> d16;
+    std::vector<int32_t> cbase;
+    if (S.pcg.cfg.index_width == 0 && pack_sell_columns(FL.h_ptr, FL.h_soff, scol, n, d16, cbase)) {
+      FL.sd16.upload(d16, s);
+      FL.scbase.upload(cbase, s);
+    } else {
+      FL.scol.upload(scol, s);
+    }
     FL.sell_src.upload(ssrc, s);
     FL.sval64.alloc(tot);
     FL.sval32.alloc(tot);
@@ -335,9 +356,9 @@ struct Cycle {
         // algorithmic bytes: the colour's entries (col + value), its ac, b,
         // x write, and one read of every other unknown
         const double R = r1 - r0, Z = L.h_ptr[r1] - L.h_ptr[r0];
-        ps.add((4.0 + w) * Z + 3.0 * w * R + w * (L.n - R));
+        ps.add((index_bytes(L) + w) * Z + 3.0 * w * R + w * (L.n - R));
       }
-      k::mcgs_sell<V>(s, r0, r1, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW), P::x(LW),
+      k::mcgs_sell<V>(s, r0, r1, L.soff.p, sell_cols(L), P::sval(L), P::ac(L), P::b(LW), P::x(LW),
                       zero_above);
     };
     for (int sw = 0; sw < nsweep; ++sw) {
@@ -357,7 +378,7 @@ struct Cycle {
       k::fill<V>(s, L.n, x, V(0));
       k::convert<V, V>(s, L.n, P::b(LW), rc);
     } else {
-      k::residual_sell<V>(s, L.n, 0, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW), x, rc);
+      k::residual_sell<V>(s, L.n, 0, L.soff.p, sell_cols(L), P::sval(L), P::ac(L), P::b(LW), x, rc);
     }
     const auto &F = R.fwd.level_ptr;
     const auto &B = R.bwd.level_ptr;
@@ -385,10 +406,10 @@ struct Cycle {
     V *rc = P::r(LW);
     const V *fac = L.kind == 6 ? P::sfac(L) : P::sval(L);
     for (int c = 0; c < L.ncol; ++c)
-      k::mcilu_fwd_sell<V>(s, L.col_off[c], L.col_off[c + 1], L.soff.p, L.scol.p, P::sval(L),
+      k::mcilu_fwd_sell<V>(s, L.col_off[c], L.col_off[c + 1], L.soff.p, sell_cols(L), P::sval(L),
                            fac, P::ac(L), P::b(LW), x, P::dinv(L), rc, from_zero);
     for (int c = L.ncol - 1; c >= 0; --c)
-      k::mcilu_bwd_sell<V>(s, L.col_off[c], L.col_off[c + 1], L.soff.p, L.scol.p, fac,
+      k::mcilu_bwd_sell<V>(s, L.col_off[c], L.col_off[c + 1], L.soff.p, sell_cols(L), fac,
                            P::dinv(L), rc, x, from_zero);
   }
 
@@ -411,7 +432,7 @@ struct Cycle {
     if (L.kind == 1)
       k::residual_fast<V>(s, L.n, L.ptr.p, L.col.p, P::ac(L), P::anb(L), P::b(LW), x, P::r(LW));
     else
-      k::residual_sell<V>(s, L.n, zero_rows, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW),
+      k::residual_sell<V>(s, L.n, zero_rows, L.soff.p, sell_cols(L), P::sval(L), P::ac(L), P::b(LW),
                           x, P::r(LW));
     // the coarse correction is scaled by coarse_scale through the restricted
     // right-hand side (the coarse cycle is linear in it)
@@ -441,7 +462,7 @@ struct Cycle {
       // restriction that produced b); further sweeps ping-pong x and y
       V *y = P::y(LW);
       for (int it = 1; it < pre; ++it) {
-        k::jacobi_sell<V>(s, L.n, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW), x, nullptr,
+        k::jacobi_sell<V>(s, L.n, L.soff.p, sell_cols(L), P::sval(L), P::ac(L), P::b(LW), x, nullptr,
                           nullptr, y, omega);
         std::swap(x, y);
       }
@@ -461,7 +482,7 @@ struct Cycle {
       if (!fuse) k::prolong_fast<V>(s, 0, L.n, L.parent.p, xc, x);
       for (int it = 0; it < post; ++it) {
         const bool pro = fuse && it == 0;
-        k::jacobi_sell<V>(s, L.n, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW), x,
+        k::jacobi_sell<V>(s, L.n, L.soff.p, sell_cols(L), P::sval(L), P::ac(L), P::b(LW), x,
                           pro ? L.parent.p : nullptr, pro ? xc : nullptr, y, omega);
         std::swap(x, y);
       }
@@ -621,6 +642,8 @@ void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
     throw Error(UFVM_ERR_ARG, "launch_mode must be 0, 1 or 2");
   if (cfg.nullspace < -1 || cfg.nullspace > 1)
     throw Error(UFVM_ERR_ARG, "nullspace must be -1, 0 or 1");
+  if (cfg.index_width != 0 && cfg.index_width != 4)
+    throw Error(UFVM_ERR_ARG, "index_width must be 0 (auto) or 4");
   int32_t nlev = S.nlevels;
   for (int32_t l = 0; l < S.nlevels; ++l)
     if (S.lev[l]->n <= cfg.max_coarse_cells) { nlev = l + 1; break; }
@@ -766,8 +789,9 @@ void FcgRun::issue_t(int k) {
   k::fcg_update_p<V>(s_, n, z, W.partial.p, P.singular, k == 0, scal, W.p.p);
   {
     ProfScope ps(ctx_, s_, "spmv0",
-                 ctx_->prof_on ? 4.0 * (n / 64 + 1) + 12.0 * L0.nnz + 8.0 * 3 * n : 0.0);
-    k::spmv_dot(s_, n, L0.soff.p, L0.scol.p, L0.ac64.p, L0.sval64.p, W.p.p, W.w.p, W.partial.p);
+                 ctx_->prof_on ? 4.0 * (n / 64 + 1) + (8.0 + index_bytes(L0)) * L0.nnz + 8.0 * 3 * n
+                               : 0.0);
+    k::spmv_dot(s_, n, L0.soff.p, sell_cols(L0), L0.ac64.p, L0.sval64.p, W.p.p, W.w.p, W.partial.p);
   }
   k::fcg_update_xr<V>(s_, n, scal, W.p.p, W.w.p, W.x.p, W.r.p, Prec<V>::b(W.lev[0]), W.partial.p,
                       W.dring, fuse0 ? L0.col_off[1] : 0, Prec<V>::ac(L0),
@@ -880,7 +904,7 @@ struct CycleB {
   void mcgs_sweeps(const FastLevel &L, LevelWork &LW, int nsweep, bool from_zero,
                    bool first_done) {
     auto pass = [&](int c, int32_t zero_above) {
-      k::mcgs_sell_b<V>(s, k(), L.col_off[c], L.col_off[c + 1], L.soff.p, L.scol.p, P::sval(L),
+      k::mcgs_sell_b<V>(s, k(), L.col_off[c], L.col_off[c + 1], L.soff.p, sell_cols(L), P::sval(L),
                         P::ac(L), P::b(LW), P::x(LW), LW.ld, zero_above);
     };
     for (int sw = 0; sw < nsweep; ++sw) {
@@ -894,7 +918,7 @@ struct CycleB {
     const PcgState &Pc = S.pcg;
     const FastLevel &C = *Pc.lev[l + 1];
     LevelWork &CW = W.lev[l + 1];
-    k::residual_sell_b<V>(s, k(), L.n, zero_rows, L.soff.p, L.scol.p, P::sval(L), P::ac(L),
+    k::residual_sell_b<V>(s, k(), L.n, zero_rows, L.soff.p, sell_cols(L), P::sval(L), P::ac(L),
                           P::b(LW), x, P::r(LW), LW.ld);
     const bool jac = C.kind == 3 && pre_of(l + 1) > 0;
     k::restrict_b<V>(s, k(), C.n, L.csoff.p, L.cschild.p, P::r(LW), LW.ld,
@@ -916,7 +940,7 @@ struct CycleB {
     if (L.kind == 3) {
       V *y = P::y(LW);
       for (int it = 1; it < pre; ++it) {
-        k::jacobi_sell_b<V>(s, k(), L.n, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW), x,
+        k::jacobi_sell_b<V>(s, k(), L.n, L.soff.p, sell_cols(L), P::sval(L), P::ac(L), P::b(LW), x,
                             LW.ld, nullptr, nullptr, 0, y, omega);
         std::swap(x, y);
       }
@@ -928,7 +952,7 @@ struct CycleB {
       if (!fuse) k::prolong_b<V>(s, k(), 0, L.n, L.parent.p, xc, ldc, x, LW.ld);
       for (int it = 0; it < post; ++it) {
         const bool pro = fuse && it == 0;
-        k::jacobi_sell_b<V>(s, k(), L.n, L.soff.p, L.scol.p, P::sval(L), P::ac(L), P::b(LW), x,
+        k::jacobi_sell_b<V>(s, k(), L.n, L.soff.p, sell_cols(L), P::sval(L), P::ac(L), P::b(LW), x,
                             LW.ld, pro ? L.parent.p : nullptr, pro ? xc : nullptr, ldc, y,
                             omega);
         std::swap(x, y);
@@ -1030,7 +1054,7 @@ void FcgBatch::issue_t(int it) {
   const V *z = cyc.vcycle(0, fuse0 && it > 0);
   k::fcg_dots<V>(s_, n, z, W.w.p, W.r.p, W.partial.p, k_, bs);
   k::fcg_update_p<V>(s_, n, z, W.partial.p, P.singular, it == 0, W.scal.p, W.p.p, k_, bs);
-  k::spmv_dot_b(s_, k_, n, L0.soff.p, L0.scol.p, L0.ac64.p, L0.sval64.p, W.p.p, W.w.p, W.ld0,
+  k::spmv_dot_b(s_, k_, n, L0.soff.p, sell_cols(L0), L0.ac64.p, L0.sval64.p, W.p.p, W.w.p, W.ld0,
                 W.partial.p, bs.ps);
   k::fcg_update_xr<V>(s_, n, W.scal.p, W.p.p, W.w.p, W.x.p, W.r.p, Prec<V>::b(W.lev[0]),
                       W.partial.p, W.dring, fuse0 ? L0.col_off[1] : 0, Prec<V>::ac(L0),
@@ -1250,7 +1274,7 @@ struct GroupRec {
     a.aux = zero_above;
     a.nb = k::nblocks(r1 - (r0 & ~63));
     a.off = L.soff.p;
-    a.col = L.scol.p;
+    a.cols = sell_cols(L);
     a.val = P::sval(L);
     a.ac = P::ac(L);
     a.b = P::b(LW);
@@ -1275,7 +1299,7 @@ struct GroupRec {
     r.aux = zero_rows;
     r.nb = k::nblocks(L.n);
     r.off = L.soff.p;
-    r.col = L.scol.p;
+    r.cols = sell_cols(L);
     r.val = P::sval(L);
     r.ac = P::ac(L);
     r.b = P::b(LW);
@@ -1286,7 +1310,7 @@ struct GroupRec {
     t.n = C.n;
     t.nb = k::nblocks(C.n);
     t.off = L.csoff.p;
-    t.col = L.cschild.p;
+    t.child = L.cschild.p;
     t.x = P::r(LW);
     t.out = P::b(CW);
     t.scale = V(Pc.cfg.coarse_scale);
@@ -1319,7 +1343,7 @@ struct GroupRec {
       a.n = L.n;
       a.nb = k::nblocks(L.n);
       a.off = L.soff.p;
-      a.col = L.scol.p;
+      a.cols = sell_cols(L);
       a.val = P::sval(L);
       a.ac = P::ac(L);
       a.b = P::b(LW);
@@ -1464,7 +1488,7 @@ void FcgGroup::build() {
     a.rv = Prec<V>::b(W.lev[0]);
     a.z0 = Prec<V>::x(W.lev[0]);
     a.off = L0.soff.p;
-    a.col = L0.scol.p;
+    a.cols = sell_cols(L0);
     a.ac = L0.ac64.p;
     a.val = L0.sval64.p;
     a.p = W.p.p;

@@ -161,10 +161,13 @@ struct FastLevel {
   DBuf<double> ac64, anb64;
   DBuf<int32_t> parent;           // coarse row of each row (next level's numbering)
   DBuf<int32_t> csoff, cschild;   // child lists of the next level's rows, sliced-ELL
-  // sliced-ELL copy of the off-diagonals (kinds 0 and 3): 64-row chunks, chunk
-  // q at [soff[q], soff[q+1]) column-major, padding col = -1
+  // sliced-ELL copy of the off-diagonals (kinds 0, 3, 5, 6 and level 0):
+  // 64-row chunks, chunk q at [soff[q], soff[q+1]) column-major, padding
+  // value 0.  Columns as int32 (scol), or, when every chunk's columns span
+  // less than 65536, as uint16 offsets (sd16) from a base per chunk (scbase)
   std::vector<int32_t> h_soff;
-  DBuf<int32_t> soff, scol;
+  DBuf<int32_t> soff, scol, scbase;
+  DBuf<uint16_t> sd16;
   DBuf<int32_t> sell_src;         // reference CSR position of each slot (-1: padding)
   DBuf<float> sval32;
   DBuf<double> sval64;

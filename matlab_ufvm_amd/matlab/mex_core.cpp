@@ -329,6 +329,7 @@ ufvm_pcg_cfg pcg_cfg(const Source &os) {
   q.coarse_scale = scalar(os, "pcgCoarseScale", 2.5);
   q.coarse_pre_sweep = q.coarse_post_sweep = iscalar(os, "pcgCoarseSweeps", 2);
   q.launch_mode = 0;
+  q.index_width = 0;   // 16-bit column offsets where they fit
   return q;
 }
 

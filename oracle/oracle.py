@@ -282,7 +282,8 @@ class OCfg(C.Structure):
                 ("p_pre", C.c_int32), ("p_post", C.c_int32), ("p_smoother", C.c_int32),
                 ("p_termination", C.c_double), ("u_smoother", C.c_int32), ("stress", C.c_int32),
                 ("u_init", C.c_double * 3), ("p_init", C.c_double), ("rho", C.c_double),
-                ("transient", C.c_int32), ("dt", C.c_double)]
+                ("transient", C.c_int32), ("dt", C.c_double), ("p_solver", C.c_int32),
+                ("pcg_tol", C.c_double)]
 
 
 class OState(C.Structure):
@@ -306,14 +307,16 @@ class SimpleCase:
     bcs: {patch name: (U bc, U value (3,), p bc[, p value])}: U bc in
     fixedValue / zeroGradient / noSlip / slip / empty / symmetry, p bc in
     zeroGradient / fixedValue / empty / symmetry.  u_init, p_init: the uniform
-    internal fields of the 0 directory.
+    internal fields of the 0 directory.  p_solver: "amg" (cfdApplyAMG with the
+    p_* settings) or "fcg64" (an FP64 flexible CG preconditioned by the FP64
+    V-cycle of System.fpcg's defaults, to ||r||_2 <= pcg_tol ||b||_2).
     """
 
     def __init__(self, mesh, bcs, nu, urf_u=0.7, urf_p=0.3, n_correctors=2, p_ref=0,
                  u_sweeps=10, p_cycle="W-Cycle", p_max_cycles=30, p_termination=0.1,
                  p_max_coarse=10, p_pre=1, p_post=3, p_smoother="ILU", u_smoother="ILU",
                  stress=False, u_init=(0.0, 0.0, 0.0), p_init=0.0, rho=1.0, transient=False,
-                 dt=1.0):
+                 dt=1.0, p_solver="amg", pcg_tol=1e-8):
         L = lib()
         for name, args in [("orc_simple_init", [C.POINTER(OMesh), C.POINTER(OCfg), C.POINTER(OState)]),
                            ("orc_simple_free", [C.POINTER(OState)]),
@@ -363,7 +366,7 @@ class SimpleCase:
                         p_max_coarse, p_pre, p_post, {"ILU": 0, "SOR": 1}[p_smoother], p_termination,
                         {"ILU": 0, "MCGS": 2}[u_smoother], int(bool(stress)),
                         (C.c_double * 3)(*u_init), float(p_init), float(rho), int(bool(transient)),
-                        float(dt))
+                        float(dt), {"amg": 0, "fcg64": 1}[p_solver], float(pcg_tol))
         self.S = OState()
         L.orc_simple_init(C.byref(self.M), C.byref(self.cfg), C.byref(self.S))
 

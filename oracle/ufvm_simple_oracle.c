@@ -727,11 +727,55 @@ void orc_assemble_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state
   free(DUTm);
 }
 
+/* the FP64 FCG pressure solve of p_solver 1 on s->H (agglomerated): level 1's
+ * dphi receives the solution; r1 = the relative residual */
+static void fcg_solve_p(const orc_simple_cfg *c, orc_simple_state *s) {
+  orc_hier *H = &s->H;
+  orc_level *L = &H->lev[0];
+  int nlev = H->nlevels;
+  for (int l = 0; l < H->nlevels; ++l)
+    if (H->lev[l].n <= 2000) {
+      nlev = l + 1;
+      break;
+    }
+  /* constant nullspace: every row sum vanishes relative to the row's size */
+  int singular = 1;
+  for (int32_t i = 0; i < L->n && singular; ++i) {
+    double rs = L->ac[i], mag = fabs(L->ac[i]);
+    for (int32_t k = L->ptr[i]; k < L->ptr[i + 1]; ++k) {
+      rs += L->anb[k];
+      mag += fabs(L->anb[k]);
+    }
+    if (fabs(rs) > 1e-9 * mag) singular = 0;
+  }
+  int32_t *colours[16];
+  int ncol[16];
+  for (int l = 0; l < nlev; ++l) {
+    const orc_level *V = &H->lev[l];
+    colours[l] = (int32_t *)malloc(sizeof(int32_t) * (V->n > 0 ? V->n : 1));
+    ncol[l] = l == 0 ? orc_colour(V->n, V->ptr, V->col, colours[l])
+                     : orc_colour_blocks(V->n, V->ptr, V->col, 1024, colours[l]);
+  }
+  const orc_vcfg v = {nlev, 1, 1, singular, 1024, 1, 0, 2.5, 2, 2};
+  double *x = (double *)malloc(sizeof(double) * (L->n > 0 ? L->n : 1));
+  double rr = 0.0;
+  s->ncycles = orc_fpcg(H, colours, ncol, &v, L->bc, c->pcg_tol, 500, x, &rr);
+  memcpy(L->dphi, x, sizeof(double) * L->n);
+  s->r0[3] = 1.0;
+  s->r1[3] = rr;
+  free(x);
+  for (int l = 0; l < nlev; ++l) free(colours[l]);
+}
+
 /* cfdSolveEquation('p') -> cfdApplyAMG (rebuilds the hierarchy) */
 void orc_solve_p(const orc_mesh *m, const orc_simple_cfg *c, orc_simple_state *s) {
   (void)m;
   orc_hier_free(&s->H);
   orc_agglomerate(&s->H, c->p_max_coarse);
+  if (c->p_solver == 1) {
+    fcg_solve_p(c, s);
+    return;
+  }
   orc_apply_amg(&s->H, c->p_cycle, c->p_max_cycles, c->p_termination, c->p_pre, c->p_post,
                 c->p_smoother, &s->r0[3], &s->r1[3], &s->ncycles);
 }

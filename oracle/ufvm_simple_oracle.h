@@ -58,6 +58,16 @@ typedef struct {
   double rho;             /* constant density of ddt(rho*U) / div(rho*U*U) (0: 1) */
   int32_t transient;      /* 1: ddt(rho*U) with the Euler scheme, time step dt */
   double dt;
+  /* pressure-correction solve: 0 cfdApplyAMG with the settings above; 1 an
+   * FP64 flexible CG preconditioned by one orc_vcycle (no reference
+   * counterpart: the precision form's algorithm with an FP64 V-cycle) on a
+   * hierarchy rebuilt every solve, from dphi = 0 to ||r||_2 <= pcg_tol
+   * ||b||_2: symmetric multicolour GS 1/1 on level 1, damped Jacobi 2/2
+   * below, corrections scaled by 2.5, dense solve on the first level of at
+   * most 2000 cells, the constant nullspace projected when every row sum
+   * vanishes */
+  int32_t p_solver;
+  double pcg_tol;
 } orc_simple_cfg;
 
 typedef struct {

@@ -1,6 +1,7 @@
 // hostalg_check.cpp -- the product's host graph algorithms (hostalg.cpp:
 // agglomeration, Galerkin gather lists, colourings, level schedules, DILU
-// lower lists, ILU(0) update lists) on seeded graphs, built with AddressSanitizer and
+// lower lists, ILU(0) update lists, 16-bit sliced-ELL columns) on seeded
+// graphs, built with AddressSanitizer and
 // UndefinedBehaviorSanitizer by tests/test_sanitizers.py.  Every result is
 // checked against its invariants and the agglomeration against the oracle's
 // (oracle/ufvm_oracle.c, compiled into this program), the ILU(0) factor that
@@ -209,6 +210,104 @@ static void check_ilu0(const HostCSR &g, const std::vector<double> &anb) {
   for (size_t t = 0; t < anb.size(); ++t) CHECK(F[t] == Fo[t]);
 }
 
+// the sliced-ELL layout of build_fast_level (64-row chunks, width = the
+// chunk's longest row, padding column = the row itself) packed to 16 bits:
+// every slot of a processed row decodes to its column (padding: the row's
+// last real column, or the row when it has none), and packing succeeds
+// exactly when every chunk's decoded columns span at most 65535
+static void check_pack(const HostCSR &g) {
+  const int32_t n = g.n, nq = (n + 63) / 64;
+  std::vector<int32_t> soff(nq + 1, 0);
+  for (int32_t q = 0; q < nq; ++q) {
+    int32_t w = 0;
+    for (int32_t t = q * 64; t < std::min(n, q * 64 + 64); ++t)
+      w = std::max(w, g.ptr[t + 1] - g.ptr[t]);
+    soff[q + 1] = soff[q] + 64 * w;
+  }
+  std::vector<int32_t> scol(std::max(soff[nq], 1), 0);
+  for (int32_t t = 0; t < n; ++t) {
+    const int32_t q = t / 64, w = (soff[q + 1] - soff[q]) / 64, len = g.ptr[t + 1] - g.ptr[t];
+    for (int32_t e = 0; e < w; ++e)
+      scol[soff[q] + (t & 63) + e * 64] = e < len ? g.col[g.ptr[t] + e] : t;
+  }
+  bool fits = true;
+  for (int32_t q = 0; q < nq; ++q) {
+    int32_t lo = n, hi = -1;
+    const int32_t w = (soff[q + 1] - soff[q]) / 64;
+    for (int32_t t = q * 64; t < std::min(n, q * 64 + 64) && w > 0; ++t) {
+      const int32_t len = g.ptr[t + 1] - g.ptr[t];
+      for (int32_t e = 0; e < std::max(len, 1); ++e) {
+        const int32_t c = len ? g.col[g.ptr[t] + e] : t;
+        lo = std::min(lo, c);
+        hi = std::max(hi, c);
+      }
+    }
+    if (hi >= lo && hi - lo > 65535) fits = false;
+  }
+  std::vector<uint16_t> d16;
+  std::vector<int32_t> base;
+  const bool ok = pack_sell_columns(g.ptr, soff, scol, n, d16, base);
+  CHECK(ok == fits);
+  if (!ok) return;
+  CHECK(d16.size() == scol.size() && (int32_t)base.size() == std::max(nq, 1));
+  for (int32_t q = 0; q < nq; ++q) {
+    const int32_t w = (soff[q + 1] - soff[q]) / 64;
+    for (int32_t u = 0; u < 64; ++u) {
+      const int32_t t = q * 64 + u;
+      const int32_t len = t < n ? g.ptr[t + 1] - g.ptr[t] : 0;
+      for (int32_t e = 0; e < w; ++e) {
+        const int32_t got = base[q] + d16[soff[q] + u + e * 64];
+        CHECK(got >= 0 && got < n);
+        if (t >= n) CHECK(got == base[q]);
+        else if (e < len) CHECK(got == g.col[g.ptr[t] + e]);
+        else CHECK(got == (len ? g.col[g.ptr[t] + len - 1] : t));
+      }
+    }
+  }
+}
+
+// a chain of n rows (i -- i+1), plus the edge (0, n-1) when far is set
+static void chain(int32_t n, bool far, HostCSR &g) {
+  std::vector<std::vector<int32_t>> nb(n);
+  for (int32_t i = 0; i + 1 < n; ++i) {
+    nb[i].push_back(i + 1);
+    nb[i + 1].push_back(i);
+  }
+  if (far && n > 2) {
+    nb[0].push_back(n - 1);
+    nb[n - 1].push_back(0);
+  }
+  g.n = n;
+  g.ptr.assign(1, 0);
+  g.col.clear();
+  for (int32_t a = 0; a < n; ++a) {
+    std::sort(nb[a].begin(), nb[a].end());
+    g.col.insert(g.col.end(), nb[a].begin(), nb[a].end());
+    g.ptr.push_back((int32_t)g.col.size());
+  }
+}
+
+// g renumbered so each colour class of a greedy colouring is contiguous (the
+// finest level's multicolour numbering)
+static HostCSR colour_sorted(const HostCSR &g) {
+  std::vector<int32_t> colour;
+  greedy_colour(g, colour);
+  std::vector<int32_t> perm(g.n), iperm(g.n);
+  std::iota(perm.begin(), perm.end(), 0);
+  std::stable_sort(perm.begin(), perm.end(),
+                   [&](int32_t a, int32_t b) { return colour[a] < colour[b]; });
+  for (int32_t t = 0; t < g.n; ++t) iperm[perm[t]] = t;
+  HostCSR h;
+  h.n = g.n;
+  h.ptr.assign(1, 0);
+  for (int32_t t = 0; t < g.n; ++t) {
+    const int32_t i = perm[t];
+    for (int32_t k = g.ptr[i]; k < g.ptr[i + 1]; ++k) h.col.push_back(iperm[g.col[k]]);
+    h.ptr.push_back((int32_t)h.col.size());
+  }
+  return h;
+}
+
 int main() {
   const int sizes[][3] = {{1, 1, 1}, {2, 1, 1}, {5, 4, 3}, {17, 9, 6}, {24, 24, 24}};
   unsigned seed = 7;
@@ -221,6 +320,25 @@ int main() {
     grid(sz[0], sz[1], sz[2], seed++, g, anb, true);
     check_level(g, anb);
     check_ilu0(g, anb);
+    check_pack(g);
+    check_pack(colour_sorted(g));
+  }
+  {
+    HostCSR g;
+    chain(70000, false, g);
+    check_pack(g);
+    chain(70000, true, g);       // chunk 0 and the last chunk span 69999 columns
+    check_pack(g);
+    chain(65600, true, g);       // the edge spans 65599
+    check_pack(g);
+    chain(65536, true, g);       // spans 65535: still fits
+    check_pack(g);
+    chain(65537, true, g);       // spans 65536: does not
+    check_pack(g);
+    g.n = 100;                   // rows without neighbours: padding is the row itself
+    g.ptr.assign(101, 0);
+    g.col.clear();
+    check_pack(g);
   }
   std::printf("hostalg_check: %d failures\n", failures);
   return failures ? 1 : 0;

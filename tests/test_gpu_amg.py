@@ -453,3 +453,84 @@ def test_batched_solve_rejects_unsupported(ctx):
     with pytest.raises(UfvmError):
         ctx.pcg_solve_batch(sid, np.stack([b, b]))             # DILU finest smoother
     ctx.system_destroy(sid)
+
+
+def _randomly_numbered(ptr, col, ac, anb, seed):
+    # the same system with its rows in a seeded random order (entries keep
+    # their order inside each row); p[new] = old
+    n = len(ptr) - 1
+    p = np.random.default_rng(seed).permutation(n)
+    ip = np.empty(n, np.int64)
+    ip[p] = np.arange(n)
+    nptr = np.concatenate([[0], np.cumsum(np.diff(ptr)[p])]).astype(np.int32)
+    idx = np.concatenate([np.arange(ptr[i], ptr[i + 1]) for i in p])
+    return (nptr, ip[col[idx]].astype(np.int32), ac[p], anb[idx]), p
+
+
+def _layout(ctx, sid):
+    from matlab_ufvm_amd._lib import UfvmError
+    widths = []
+    for lev in range(ctx.num_levels(sid)):
+        try:
+            widths.append(ctx.pcg_level_layout(sid, lev)[0])
+        except UfvmError:          # past the precision form's last level
+            break
+    return widths
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_16bit_columns_give_the_same_bits(ctx, prec):
+    # The sliced-ELL level copies store a column as a base per 64-row chunk
+    # plus a uint16 offset per entry wherever every chunk spans fewer than
+    # 65536 columns.  Only the storage of the indices changes: every finest
+    # smoother, both coarse smoothers, the batched and the grouped forms give
+    # the bits of the int32 form.  The randomly numbered 137k-row system has
+    # chunks spanning more than 65535 columns on its finest level, which
+    # keeps int32 there while its coarse levels take 16 bits.
+    systems = []
+    m = cavity_mesh(20, jitter=0.15, seed=141, renumber_seed=142)
+    systems.append(("hex3d", poisson_system(m, dirichlet=("movingWall",)),
+                    np.stack([gaussian_rhs(m, seed=150 + q) for q in range(4)])))
+    m = poisson_tet_mesh(8, jitter=0.1, seed=143, renumber_seed=144)
+    systems.append(("tet", poisson_system(m, dirichlet=("dirichlet",)),
+                    np.stack([gaussian_rhs(m, seed=150 + q) for q in range(4)])))
+    m = cavity_mesh(370, nz=1, jitter=0.1, seed=161, renumber_seed=None)
+    wide, p = _randomly_numbered(*poisson_system(m, dirichlet=()), seed=162)
+    systems.append(("wide2d", wide,
+                    np.stack([gaussian_rhs(m, seed=150 + q)[p] for q in range(4)])))
+    sids = []
+    for name, (ptr, col, ac, anb), B in systems:
+        sid = make(ctx, ptr, col, ac, anb, B[0])
+        ctx.agglomerate(sid, 10)
+        sids.append(sid)
+    for sm, cs in (("MCGS", 1), ("MCGS", 0), ("MCILU", 1), ("MCILU0", 1)):
+        res = {}
+        for iw in (4, 0):
+            out, widths = [], {}
+            for (name, _, B), sid in zip(systems, sids):
+                ctx.pcg_setup(sid, precision=prec, max_coarse_cells=200, tol=1e-9, smoother=sm,
+                              coarse_smoother=cs, index_width=iw)
+                w = widths[name] = _layout(ctx, sid)
+                if iw == 4:
+                    assert set(w) <= {0, 4}, (name, w)
+                else:
+                    assert w[0] == (4 if name == "wide2d" else 2), (name, w)
+                    # Jacobi coarse levels have sliced-ELL copies, block GS ones do not
+                    assert (2 in w[1:]) == (cs == 1), (name, w)
+                ctx.set_bc(sid, 0, B[0])
+                it, rr = ctx.pcg_solve(sid)
+                out.append((it, rr, ctx.get_dphi(sid, 0)))
+                if sm == "MCGS" and cs == 1:
+                    X, its, rel = ctx.pcg_solve_batch(sid, B)
+                    out += [(its[q], rel[q], X[q]) for q in range(4)]
+            if sm == "MCGS" and cs == 1:
+                xs, its, rel = ctx.pcg_solve_group(sids, [B[1] for _, _, B in systems])
+                out += [(its[q], rel[q], xs[q]) for q in range(len(sids))]
+            res[iw] = out
+        print(f"\n[16-bit] prec={prec} {sm}/{cs}: widths {widths} "
+              f"iterations {[o[0] for o in res[0]]}", flush=True)
+        assert len(res[4]) == len(res[0])
+        for (it4, rr4, x4), (it2, rr2, x2) in zip(res[4], res[0]):
+            assert it4 == it2 and rr4 == rr2 and np.array_equal(x4, x2), (sm, cs)
+    for sid in sids:
+        ctx.system_destroy(sid)

@@ -91,15 +91,45 @@ def test_steps_bitexact(ctx, pair):
 
 
 def test_iterations_match(ctx):
+    # the CPU-runnable configuration whole: 40 x 40 x 1 cavity, Re 100, 200
+    # SIMPLE iterations, V-cycle SOR 2/2 -- bit-identical
     m = cavity_mesh(40, nz=1, jitter=0.0, renumber_seed=None)
     g = SimpleCase(ctx, m, cavity_bcs(m), nu=0.01, **AMG)
     o = oracle.SimpleCase(m, oracle.cavity_bcs(m), nu=0.01, **AMG)
-    res = g.iterate(40)
-    o.iterate(40)
+    res = g.iterate(200)
+    o.iterate(200)
     for name, oname in (("U", "U"), ("p", "p"), ("mdot_f", "mdot")):
-        same(g.field(name), o.field(oname), f"40 iterations {name}")
+        same(g.field(name), o.field(oname), f"200 iterations {name}")
     assert res[-1, 0] < 0.01 * res[0, 0]
     g.close()
+
+
+@pytest.mark.parametrize("name", ["cavity3d_16", "cavity2d_64"])
+def test_fp32_preconditioned_simple_within_1e8_of_fp64(ctx, name):
+    # The precision form's measure at test size: SIMPLE iterations whose
+    # pressure-correction solves are the FP32-preconditioned AMG-PCG to
+    # ||r|| <= 1e-8 ||b||, against the oracle's SIMPLE whose solves are the
+    # FP64 FCG at the same tolerance: relative L2 error <= 1e-8 for U, p and
+    # mdot_f.  The FP64-preconditioned GPU solve is measured beside it.
+    if name == "cavity3d_16":
+        m, nu = cavity_mesh(16, jitter=0.15, seed=171, renumber_seed=172), 1.0 / 400.0
+    else:
+        m, nu = cavity_mesh(64, nz=1, jitter=0.1, seed=173, renumber_seed=174), 0.01
+    o = oracle.SimpleCase(m, oracle.cavity_bcs(m), nu=nu, p_solver="fcg64", pcg_tol=1e-8)
+    o.iterate(20)
+    errs = {}
+    for prec in (32, 64):
+        g = SimpleCase(ctx, m, cavity_bcs(m), nu=nu, p_solver="pcg", pcg_precision=prec,
+                       pcg_tol=1e-8)
+        g.iterate(20)
+        errs[prec] = {name_g: float(np.linalg.norm(g.field(name_g) - o.field(name_o)) /
+                                    np.linalg.norm(o.field(name_o)))
+                      for name_g, name_o in (("U", "U"), ("p", "p"), ("mdot_f", "mdot"))}
+        g.close()
+    print(f"\n[precision form] {name} cells={m.n_cells} 20 iterations, fields vs the oracle's "
+          f"FP64 FCG at 1e-8: FP32 V-cycle {errs[32]}, FP64 V-cycle {errs[64]}", flush=True)
+    assert max(errs[32].values()) <= 1e-8, errs
+    assert max(errs[64].values()) <= 1e-8, errs
 
 
 def test_precision_form_pressure_solver(ctx):
