@@ -52,6 +52,9 @@ struct ColSrc<true> {
   __device__ __forceinline__ ColSrc(const SellCols &s, int32_t q) : d(s.d16), b(s.base[q]) {}
   __device__ __forceinline__ int32_t operator[](int32_t e) const { return b + (int32_t)d[e]; }
 };
+// entries per run of the sliced-ELL row loops (loads issued together)
+constexpr int kSellRun = 8;
+
 // f(std::bool_constant<packed>): launchers pick the kernel of cols' form
 template <typename F>
 inline void with_cols(const SellCols &cols, F &&f) {

@@ -47,6 +47,28 @@ k_residual_ref(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__rest
 // row t of chunk t / 64 is row row[t], -1 for padding rows; entries in cconn
 // order, padding entries col = -1).  Each row's arithmetic is that of
 // k_residual_ref / k_gs_rows.
+// v - sum over the N slots from e0 of val x[col], padding (col -1) skipped;
+// the run's loads are issued together, padding reading x[0]
+template <int N>
+__device__ __forceinline__ double csell_run(const int32_t *__restrict__ col,
+                                            const double *__restrict__ val, const double *x,
+                                            int32_t e0, double v) {
+  int32_t j[N];
+  double a[N], xj[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    j[u] = col[e0 + u * 64];
+    a[u] = val[e0 + u * 64];
+  }
+#pragma unroll
+  for (int u = 0; u < N; ++u) xj[u] = x[j[u] >= 0 ? j[u] : 0];
+#pragma unroll
+  for (int u = 0; u < N; ++u) asm volatile("" : "+v"(xj[u]));
+#pragma unroll
+  for (int u = 0; u < N; ++u) v = j[u] >= 0 ? v - a[u] * xj[u] : v;
+  return v;
+}
+
 __device__ __forceinline__ double csell_row(const int32_t *__restrict__ off,
                                             const int32_t *__restrict__ col,
                                             const double *__restrict__ val, const double *x,
@@ -54,25 +76,18 @@ __device__ __forceinline__ double csell_row(const int32_t *__restrict__ off,
   const int32_t q = t >> 6;
   const int32_t base = off[q] + (t & 63), w = (off[q + 1] - off[q]) >> 6;
   int32_t k = 0;
-  for (; k + 4 <= w; k += 4) {
-    int32_t j[4];
-    double a[4], xj[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      j[u] = col[base + (k + u) * 64];
-      a[u] = val[base + (k + u) * 64];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) xj[u] = j[u] >= 0 ? x[j[u]] : 0.0;
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (j[u] >= 0) v = v - a[u] * xj[u];
+  for (; k + kSellRun <= w; k += kSellRun) v = csell_run<kSellRun>(col, val, x, base + k * 64, v);
+  const int32_t e0 = base + k * 64;
+  switch (w - k) {
+    case 1: return csell_run<1>(col, val, x, e0, v);
+    case 2: return csell_run<2>(col, val, x, e0, v);
+    case 3: return csell_run<3>(col, val, x, e0, v);
+    case 4: return csell_run<4>(col, val, x, e0, v);
+    case 5: return csell_run<5>(col, val, x, e0, v);
+    case 6: return csell_run<6>(col, val, x, e0, v);
+    case 7: return csell_run<7>(col, val, x, e0, v);
+    default: return v;
   }
-  for (; k < w; ++k) {
-    const int32_t j = col[base + k * 64];
-    if (j >= 0) v = v - val[base + k * 64] * x[j];
-  }
-  return v;
 }
 
 __global__ void __launch_bounds__(kBlock)
@@ -711,33 +726,58 @@ void coarse_lhs(hipStream_t s, int32_t nout, int32_t nc, const int32_t *ptr, con
 // entries are issued before their products are summed.  Column indices come
 // through ColSrc: int32 per entry, or a chunk base plus a uint16 per entry.
 
-// v -/+ sum_k val_k x[col_k] in entry order (NEG: subtract).  ZA: neighbours
-// with index >= zero_above read as 0.
+// A row's entries are taken in runs: the run's index and value loads are
+// issued together, then its neighbour gathers, then its products in entry
+// order, so each run waits for one round trip to its entries and one to its
+// neighbours.  Runs have kSellRun entries, the last one the remainder (w is
+// the same for the 64 rows of a chunk, so a wavefront takes one branch).
+
+// v -/+ sum over the N entries from slot e0 of val x[col] (NEG: subtract);
+// ZA: neighbours with index >= zero_above read as 0
+template <int N, typename V, bool NEG, bool ZA, typename CS>
+__device__ __forceinline__ V sell_run(const CS &col, const V *__restrict__ val, const V *x,
+                                      int32_t e0, V v, int32_t zero_above) {
+  int32_t j[N];
+  V a[N], xj[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    j[u] = col[e0 + u * 64];
+    a[u] = val[e0 + u * 64];
+  }
+#pragma unroll
+  for (int u = 0; u < N; ++u) xj[u] = x[j[u]];
+  if (ZA) {
+    // the gathers stay issued together here, not moved under the guards
+#pragma unroll
+    for (int u = 0; u < N; ++u) asm volatile("" : "+v"(xj[u]));
+#pragma unroll
+    for (int u = 0; u < N; ++u) xj[u] = j[u] < zero_above ? xj[u] : V(0);
+  }
+#pragma unroll
+  for (int u = 0; u < N; ++u) v = NEG ? v - a[u] * xj[u] : v + a[u] * xj[u];
+  return v;
+}
+
+// v -/+ sum_k val_k x[col_k] over the row's w entries from slot base, in
+// entry order
 template <typename V, bool NEG, bool ZA, typename CS>
 __device__ __forceinline__ V sell_row(const CS &col, const V *__restrict__ val,
                                       const V *x, int32_t base, int32_t w, V v,
                                       int32_t zero_above) {
   int32_t k = 0;
-  for (; k + 4 <= w; k += 4) {
-    int32_t j[4];
-    V a[4], xj[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      j[u] = col[base + (k + u) * 64];
-      a[u] = val[base + (k + u) * 64];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) xj[u] = (!ZA || j[u] < zero_above) ? x[j[u]] : V(0);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) v = NEG ? v - a[u] * xj[u] : v + a[u] * xj[u];
+  for (; k + kSellRun <= w; k += kSellRun)
+    v = sell_run<kSellRun, V, NEG, ZA>(col, val, x, base + k * 64, v, zero_above);
+  const int32_t e0 = base + k * 64;
+  switch (w - k) {
+    case 1: return sell_run<1, V, NEG, ZA>(col, val, x, e0, v, zero_above);
+    case 2: return sell_run<2, V, NEG, ZA>(col, val, x, e0, v, zero_above);
+    case 3: return sell_run<3, V, NEG, ZA>(col, val, x, e0, v, zero_above);
+    case 4: return sell_run<4, V, NEG, ZA>(col, val, x, e0, v, zero_above);
+    case 5: return sell_run<5, V, NEG, ZA>(col, val, x, e0, v, zero_above);
+    case 6: return sell_run<6, V, NEG, ZA>(col, val, x, e0, v, zero_above);
+    case 7: return sell_run<7, V, NEG, ZA>(col, val, x, e0, v, zero_above);
+    default: return v;
   }
-  for (; k < w; ++k) {
-    const int32_t j = col[base + k * 64];
-    const V a = val[base + k * 64];
-    const V xj = (!ZA || j < zero_above) ? x[j] : V(0);
-    v = NEG ? v - a * xj : v + a * xj;
-  }
-  return v;
 }
 
 // Gauss-Seidel update of the contiguous rows [r0, r1) of one colour class.
@@ -754,11 +794,12 @@ __device__ __forceinline__ void mcgs_row(int32_t i, int32_t r1, const int32_t *_
   const int32_t q = i >> 6;
   const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
   const ColSrc<PK> col(cols, q);
+  const V d = ac[i];
   // zero_above beyond this colour's rows: every neighbour holds a value
   const V v = (zero_above <= r1)
                   ? sell_row<V, true, true>(col, val, x, base, w, b[i], zero_above)
                   : sell_row<V, true, false>(col, val, x, base, w, b[i], zero_above);
-  x[i] = v / ac[i];
+  x[i] = v / d;
 }
 
 template <typename V, bool PK>
@@ -971,6 +1012,26 @@ __device__ __forceinline__ V jac_val(const V *x, const int32_t *__restrict__ par
   return PRO ? x[j] + xc[parent[j]] : x[j];
 }
 
+// sell_run for the Jacobi sweep: v - sum over the N entries from slot e0 of
+// val * (x + xc[parent] when PRO)
+template <int N, typename V, bool PRO, typename CS>
+__device__ __forceinline__ V jacobi_run(const CS &col, const V *__restrict__ val, const V *x,
+                                        const int32_t *__restrict__ parent, const V *xc,
+                                        int32_t e0, V v) {
+  int32_t j[N];
+  V a[N], xj[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    j[u] = col[e0 + u * 64];
+    a[u] = val[e0 + u * 64];
+  }
+#pragma unroll
+  for (int u = 0; u < N; ++u) xj[u] = jac_val<V, PRO>(x, parent, xc, j[u]);
+#pragma unroll
+  for (int u = 0; u < N; ++u) v = v - a[u] * xj[u];
+  return v;
+}
+
 // one damped Jacobi sweep: out = v + omega (b - A v) / ac, v = x or, with
 // PRO, x + xc[parent] (the coarse correction added on the fly)
 template <typename V, bool PRO, bool PK>
@@ -985,22 +1046,18 @@ __device__ __forceinline__ void jacobi_row(int32_t i, const int32_t *__restrict_
   const V vi = jac_val<V, PRO>(x, parent, xc, i), d = ac[i];
   V v = b[i] - d * vi;
   int32_t k = 0;
-  for (; k + 4 <= w; k += 4) {
-    int32_t j[4];
-    V a[4], xj[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      j[u] = col[base + (k + u) * 64];
-      a[u] = val[base + (k + u) * 64];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) xj[u] = jac_val<V, PRO>(x, parent, xc, j[u]);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) v = v - a[u] * xj[u];
-  }
-  for (; k < w; ++k) {
-    const int32_t j = col[base + k * 64];
-    v = v - val[base + k * 64] * jac_val<V, PRO>(x, parent, xc, j);
+  for (; k + kSellRun <= w; k += kSellRun)
+    v = jacobi_run<kSellRun, V, PRO>(col, val, x, parent, xc, base + k * 64, v);
+  const int32_t e0 = base + k * 64;
+  switch (w - k) {
+    case 1: v = jacobi_run<1, V, PRO>(col, val, x, parent, xc, e0, v); break;
+    case 2: v = jacobi_run<2, V, PRO>(col, val, x, parent, xc, e0, v); break;
+    case 3: v = jacobi_run<3, V, PRO>(col, val, x, parent, xc, e0, v); break;
+    case 4: v = jacobi_run<4, V, PRO>(col, val, x, parent, xc, e0, v); break;
+    case 5: v = jacobi_run<5, V, PRO>(col, val, x, parent, xc, e0, v); break;
+    case 6: v = jacobi_run<6, V, PRO>(col, val, x, parent, xc, e0, v); break;
+    case 7: v = jacobi_run<7, V, PRO>(col, val, x, parent, xc, e0, v); break;
+    default: break;
   }
   out[i] = vi + omega * v / d;
 }
@@ -1103,6 +1160,24 @@ void bmcgs(hipStream_t s, int32_t nblk, const int32_t *bptr, const int32_t *boff
 // order, increasing reference fine index); JAC: also the first Jacobi sweep
 // from zero on the coarse level, xc[p] = w bc[p] / ac[p].  The coarse cycle is
 // linear in bc, so scale multiplies the whole coarse correction.
+// s + the residuals of the N child slots from e0 in list order (padding -1
+// adds 0); the run's loads are issued together, padding reading child 0
+template <int N, typename V>
+__device__ __forceinline__ V restrict_run(const int32_t *__restrict__ child, const V *r, int32_t e0,
+                                          V s) {
+  int32_t c[N];
+  V v[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) c[u] = child[e0 + u * 64];
+#pragma unroll
+  for (int u = 0; u < N; ++u) v[u] = r[c[u] >= 0 ? c[u] : 0];
+#pragma unroll
+  for (int u = 0; u < N; ++u) asm volatile("" : "+v"(v[u]));
+#pragma unroll
+  for (int u = 0; u < N; ++u) s = s + (c[u] >= 0 ? v[u] : V(0));
+  return s;
+}
+
 template <typename V, bool JAC>
 __device__ __forceinline__ void restrict_row(int32_t p, const int32_t *__restrict__ off,
                                              const int32_t *__restrict__ child, const V *r,
@@ -1112,19 +1187,17 @@ __device__ __forceinline__ void restrict_row(int32_t p, const int32_t *__restric
   const int32_t base = off[q] + (p & 63), w = (off[q + 1] - off[q]) >> 6;
   V s = 0;
   int32_t k = 0;
-  for (; k + 4 <= w; k += 4) {
-    int32_t c[4];
-    V v[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) c[u] = child[base + (k + u) * 64];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = c[u] >= 0 ? r[c[u]] : V(0);
-#pragma unroll
-    for (int u = 0; u < 4; ++u) s = s + v[u];
-  }
-  for (; k < w; ++k) {
-    const int32_t c = child[base + k * 64];
-    s = s + (c >= 0 ? r[c] : V(0));
+  for (; k + kSellRun <= w; k += kSellRun) s = restrict_run<kSellRun>(child, r, base + k * 64, s);
+  const int32_t e0 = base + k * 64;
+  switch (w - k) {
+    case 1: s = restrict_run<1>(child, r, e0, s); break;
+    case 2: s = restrict_run<2>(child, r, e0, s); break;
+    case 3: s = restrict_run<3>(child, r, e0, s); break;
+    case 4: s = restrict_run<4>(child, r, e0, s); break;
+    case 5: s = restrict_run<5>(child, r, e0, s); break;
+    case 6: s = restrict_run<6>(child, r, e0, s); break;
+    case 7: s = restrict_run<7>(child, r, e0, s); break;
+    default: break;
   }
   const V t = scale * s;
   bc[p] = t;

@@ -17,38 +17,53 @@
 namespace ufvm {
 namespace k {
 
-// v[r] -= sum_k val_k x_r[col_k] in entry order, four entries' loads at a
-// time; ZA: neighbours with index >= zero_above read as 0
-template <typename V, int K, bool ZA, typename CS>
-__device__ __forceinline__ void sell_rows_k(const CS &col, const V *__restrict__ val, const V *x, int64_t ld,
-                                            int32_t base, int32_t w, V (&v)[K],
-                                            int32_t zero_above) {
-  int32_t k = 0;
-  for (; k + 4 <= w; k += 4) {
-    int32_t j[4];
-    V a[4];
+// v[r] -= sum over the N entries from slot e0 of val x_r[col] in entry order,
+// the run's loads issued together (the runs of the single-vector sell_row);
+// ZA: neighbours with index >= zero_above read as 0
+template <int N, typename V, int K, bool ZA, typename CS>
+__device__ __forceinline__ void sell_run_k(const CS &col, const V *__restrict__ val, const V *x,
+                                           int64_t ld, int32_t e0, V (&v)[K],
+                                           int32_t zero_above) {
+  int32_t j[N];
+  V a[N];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      j[u] = col[base + (k + u) * 64];
-      a[u] = val[base + (k + u) * 64];
-    }
-#pragma unroll
-    for (int r = 0; r < K; ++r) {
-      V xj[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) xj[u] = (!ZA || j[u] < zero_above) ? x[r * ld + j[u]] : V(0);
-#pragma unroll
-      for (int u = 0; u < 4; ++u) v[r] = v[r] - a[u] * xj[u];
-    }
+  for (int u = 0; u < N; ++u) {
+    j[u] = col[e0 + u * 64];
+    a[u] = val[e0 + u * 64];
   }
-  for (; k < w; ++k) {
-    const int32_t j = col[base + k * 64];
-    const V a = val[base + k * 64];
 #pragma unroll
-    for (int r = 0; r < K; ++r) {
-      const V xj = (!ZA || j < zero_above) ? x[r * ld + j] : V(0);
-      v[r] = v[r] - a * xj;
+  for (int r = 0; r < K; ++r) {
+    V xj[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u) xj[u] = x[r * ld + j[u]];
+    if (ZA) {
+#pragma unroll
+      for (int u = 0; u < N; ++u) asm volatile("" : "+v"(xj[u]));
+#pragma unroll
+      for (int u = 0; u < N; ++u) xj[u] = j[u] < zero_above ? xj[u] : V(0);
     }
+#pragma unroll
+    for (int u = 0; u < N; ++u) v[r] = v[r] - a[u] * xj[u];
+  }
+}
+
+template <typename V, int K, bool ZA, typename CS>
+__device__ __forceinline__ void sell_rows_k(const CS &col, const V *__restrict__ val,
+                                            const V *x, int64_t ld, int32_t base, int32_t w,
+                                            V (&v)[K], int32_t zero_above) {
+  int32_t k = 0;
+  for (; k + kSellRun <= w; k += kSellRun)
+    sell_run_k<kSellRun, V, K, ZA>(col, val, x, ld, base + k * 64, v, zero_above);
+  const int32_t e0 = base + k * 64;
+  switch (w - k) {
+    case 1: sell_run_k<1, V, K, ZA>(col, val, x, ld, e0, v, zero_above); break;
+    case 2: sell_run_k<2, V, K, ZA>(col, val, x, ld, e0, v, zero_above); break;
+    case 3: sell_run_k<3, V, K, ZA>(col, val, x, ld, e0, v, zero_above); break;
+    case 4: sell_run_k<4, V, K, ZA>(col, val, x, ld, e0, v, zero_above); break;
+    case 5: sell_run_k<5, V, K, ZA>(col, val, x, ld, e0, v, zero_above); break;
+    case 6: sell_run_k<6, V, K, ZA>(col, val, x, ld, e0, v, zero_above); break;
+    case 7: sell_run_k<7, V, K, ZA>(col, val, x, ld, e0, v, zero_above); break;
+    default: break;
   }
 }
 
@@ -97,6 +112,36 @@ k_residual_sell_k(int32_t n, int32_t zero_rows, const int32_t *__restrict__ off,
   for (int r = 0; r < K; ++r) rr[r * ld + i] = v[r];
 }
 
+// v[r] -= sum over the N entries from slot e0 of val (x_r + xc_r[parent] when
+// PRO), the run's loads issued together
+template <int N, typename V, bool PRO, int K, typename CS>
+__device__ __forceinline__ void jacobi_run_k(const CS &col, const V *__restrict__ val,
+                                             const V *__restrict__ x, int64_t ld,
+                                             const int32_t *__restrict__ parent,
+                                             const V *__restrict__ xc, int64_t ldc, int32_t e0,
+                                             V (&v)[K]) {
+  int32_t j[N], pj[N];
+  V a[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    j[u] = col[e0 + u * 64];
+    a[u] = val[e0 + u * 64];
+  }
+  if (PRO) {
+#pragma unroll
+    for (int u = 0; u < N; ++u) pj[u] = parent[j[u]];
+  }
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    V xj[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u)
+      xj[u] = PRO ? x[r * ld + j[u]] + xc[r * ldc + pj[u]] : x[r * ld + j[u]];
+#pragma unroll
+    for (int u = 0; u < N; ++u) v[r] = v[r] - a[u] * xj[u];
+  }
+}
+
 // damped Jacobi for K vectors; PRO: x + xc[parent] as the sweep's input
 template <typename V, bool PRO, int K, bool PK>
 __global__ void __launch_bounds__(kBlock)
@@ -118,40 +163,42 @@ k_jacobi_sell_k(int32_t n, const int32_t *__restrict__ off, SellCols cols,
     v[r] = b[r * ld + i] - d * vi[r];
   }
   int32_t k = 0;
-  for (; k + 4 <= w; k += 4) {
-    int32_t j[4], pj[4];
-    V a[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      j[u] = col[base + (k + u) * 64];
-      a[u] = val[base + (k + u) * 64];
-    }
-    if (PRO) {
-#pragma unroll
-      for (int u = 0; u < 4; ++u) pj[u] = parent[j[u]];
-    }
-#pragma unroll
-    for (int r = 0; r < K; ++r) {
-      V xj[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-        xj[u] = PRO ? x[r * ld + j[u]] + xc[r * ldc + pj[u]] : x[r * ld + j[u]];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) v[r] = v[r] - a[u] * xj[u];
-    }
-  }
-  for (; k < w; ++k) {
-    const int32_t j = col[base + k * 64];
-    const V a = val[base + k * 64];
-    const int32_t pj = PRO ? parent[j] : 0;
-#pragma unroll
-    for (int r = 0; r < K; ++r) {
-      const V xj = PRO ? x[r * ld + j] + xc[r * ldc + pj] : x[r * ld + j];
-      v[r] = v[r] - a * xj;
-    }
+  for (; k + kSellRun <= w; k += kSellRun)
+    jacobi_run_k<kSellRun, V, PRO, K>(col, val, x, ld, parent, xc, ldc, base + k * 64, v);
+  const int32_t e0 = base + k * 64;
+  switch (w - k) {
+    case 1: jacobi_run_k<1, V, PRO, K>(col, val, x, ld, parent, xc, ldc, e0, v); break;
+    case 2: jacobi_run_k<2, V, PRO, K>(col, val, x, ld, parent, xc, ldc, e0, v); break;
+    case 3: jacobi_run_k<3, V, PRO, K>(col, val, x, ld, parent, xc, ldc, e0, v); break;
+    case 4: jacobi_run_k<4, V, PRO, K>(col, val, x, ld, parent, xc, ldc, e0, v); break;
+    case 5: jacobi_run_k<5, V, PRO, K>(col, val, x, ld, parent, xc, ldc, e0, v); break;
+    case 6: jacobi_run_k<6, V, PRO, K>(col, val, x, ld, parent, xc, ldc, e0, v); break;
+    case 7: jacobi_run_k<7, V, PRO, K>(col, val, x, ld, parent, xc, ldc, e0, v); break;
+    default: break;
   }
 #pragma unroll
   for (int r = 0; r < K; ++r) out[r * ld + i] = vi[r] + omega * v[r] / d;
+}
+
+// s[r] += the residuals r of the N child slots from e0 in list order (padding
+// -1 adds 0), the run's child loads issued together
+template <int N, typename V, int K>
+__device__ __forceinline__ void restrict_run_k(const int32_t *__restrict__ child,
+                                               const V *__restrict__ rf, int64_t ldf, int32_t e0,
+                                               V (&s)[K]) {
+  int32_t c[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) c[u] = child[e0 + u * 64];
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    V v[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u) v[u] = rf[r * ldf + (c[u] >= 0 ? c[u] : 0)];
+#pragma unroll
+    for (int u = 0; u < N; ++u) asm volatile("" : "+v"(v[u]));
+#pragma unroll
+    for (int u = 0; u < N; ++u) s[r] = s[r] + (c[u] >= 0 ? v[u] : V(0));
+  }
 }
 
 // restriction over sliced-ELL child lists (k_restrict_sell for K vectors)
@@ -168,23 +215,18 @@ k_restrict_sell_k(int32_t nc, const int32_t *__restrict__ off, const int32_t *__
 #pragma unroll
   for (int r = 0; r < K; ++r) s[r] = V(0);
   int32_t k = 0;
-  for (; k + 4 <= w; k += 4) {
-    int32_t c[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) c[u] = child[base + (k + u) * 64];
-#pragma unroll
-    for (int r = 0; r < K; ++r) {
-      V v[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) v[u] = c[u] >= 0 ? rf[r * ldf + c[u]] : V(0);
-#pragma unroll
-      for (int u = 0; u < 4; ++u) s[r] = s[r] + v[u];
-    }
-  }
-  for (; k < w; ++k) {
-    const int32_t c = child[base + k * 64];
-#pragma unroll
-    for (int r = 0; r < K; ++r) s[r] = s[r] + (c >= 0 ? rf[r * ldf + c] : V(0));
+  for (; k + kSellRun <= w; k += kSellRun)
+    restrict_run_k<kSellRun, V, K>(child, rf, ldf, base + k * 64, s);
+  const int32_t e0 = base + k * 64;
+  switch (w - k) {
+    case 1: restrict_run_k<1, V, K>(child, rf, ldf, e0, s); break;
+    case 2: restrict_run_k<2, V, K>(child, rf, ldf, e0, s); break;
+    case 3: restrict_run_k<3, V, K>(child, rf, ldf, e0, s); break;
+    case 4: restrict_run_k<4, V, K>(child, rf, ldf, e0, s); break;
+    case 5: restrict_run_k<5, V, K>(child, rf, ldf, e0, s); break;
+    case 6: restrict_run_k<6, V, K>(child, rf, ldf, e0, s); break;
+    case 7: restrict_run_k<7, V, K>(child, rf, ldf, e0, s); break;
+    default: break;
   }
   const V a = JAC ? acc[p] : V(1);
 #pragma unroll
@@ -243,6 +285,29 @@ k_dense_gemv_k(int32_t n, const double *__restrict__ ainv, const V *__restrict__
   }
 }
 
+// v[r] += sum over the N entries from slot e0 of val p_r[col], the run's loads
+// issued together
+template <int N, int K, typename CS>
+__device__ __forceinline__ void spmv_run_k(const CS &col, const double *__restrict__ val,
+                                           const double *__restrict__ p, int64_t ld, int32_t e0,
+                                           double (&v)[K]) {
+  int32_t j[N];
+  double a[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    j[u] = col[e0 + u * 64];
+    a[u] = val[e0 + u * 64];
+  }
+#pragma unroll
+  for (int r = 0; r < K; ++r) {
+    double xj[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u) xj[u] = p[r * ld + j[u]];
+#pragma unroll
+    for (int u = 0; u < N; ++u) v[r] = v[r] + a[u] * xj[u];
+  }
+}
+
 // w_r = A p_r (FP64, sliced-ELL) and partial[r * pstride + block] = p_r.w_r
 // over the block's rows: the grid-stride row assignment and the block sum of
 // k_spmv_dot, per vector
@@ -268,28 +333,18 @@ k_spmv_dot_k(int32_t n, const int32_t *__restrict__ off, SellCols cols,
       v[r] = a0 * pi[r];
     }
     int32_t k = 0;
-    for (; k + 4 <= wq; k += 4) {
-      int32_t j[4];
-      double a[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        j[u] = col[base + (k + u) * 64];
-        a[u] = val[base + (k + u) * 64];
-      }
-#pragma unroll
-      for (int r = 0; r < K; ++r) {
-        double xj[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) xj[u] = p[r * ld + j[u]];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) v[r] = v[r] + a[u] * xj[u];
-      }
-    }
-    for (; k < wq; ++k) {
-      const int32_t j = col[base + k * 64];
-      const double a = val[base + k * 64];
-#pragma unroll
-      for (int r = 0; r < K; ++r) v[r] = v[r] + a * p[r * ld + j];
+    for (; k + kSellRun <= wq; k += kSellRun)
+      spmv_run_k<kSellRun, K>(col, val, p, ld, base + k * 64, v);
+    const int32_t e0 = base + k * 64;
+    switch (wq - k) {
+      case 1: spmv_run_k<1, K>(col, val, p, ld, e0, v); break;
+      case 2: spmv_run_k<2, K>(col, val, p, ld, e0, v); break;
+      case 3: spmv_run_k<3, K>(col, val, p, ld, e0, v); break;
+      case 4: spmv_run_k<4, K>(col, val, p, ld, e0, v); break;
+      case 5: spmv_run_k<5, K>(col, val, p, ld, e0, v); break;
+      case 6: spmv_run_k<6, K>(col, val, p, ld, e0, v); break;
+      case 7: spmv_run_k<7, K>(col, val, p, ld, e0, v); break;
+      default: break;
     }
 #pragma unroll
     for (int r = 0; r < K; ++r) {
