@@ -122,6 +122,36 @@ def _pmc_load(d):
     return out
 
 
+def trace_summary(d, regex):
+    """Average kernel duration per (kernel, grid size) from the
+    *kernel_trace.csv files of a rocprofv3 --kernel-trace run written with
+    --output-format csv, restricted to kernel names matching regex: the
+    per-level breakdown that the name-only stats table merges."""
+    import csv
+    import glob
+    import re
+    import collections
+    pat = re.compile(regex)
+    dur = collections.defaultdict(list)
+    for f in glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True):
+        with open(f, newline="") as fh:
+            for raw in csv.DictReader(fh):
+                row = {k.lower(): v for k, v in raw.items()}
+                name = row["kernel_name"]
+                if not pat.search(name):
+                    continue
+                grid = int(row.get("grid_size_x", row.get("grid_size", 0)) or 0)
+                dur[(name, grid)].append((int(row["end_timestamp"]) -
+                                          int(row["start_timestamp"])) * 1e-3)
+    print(f"{'kernel':60s} {'grid':>9s} {'dispatches':>10s} {'avg us':>9s} {'min us':>9s}"
+          f" {'total ms':>9s}")
+    rows = sorted(dur.items(), key=lambda kv: -sum(kv[1]))
+    for (name, grid), v in rows:
+        short = name.split("(")[0].replace("void ", "").replace("ufvm::k::", "")
+        print(f"{short[:60]:60s} {grid:9d} {len(v):10d} {sum(v) / len(v):9.2f} {min(v):9.2f}"
+              f" {sum(v) * 1e-3:9.2f}")
+
+
 def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_mib):
     """HBM bytes per dispatch of each kernel from two rocprofv3 --pmc runs
     (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by what the same
@@ -662,7 +692,13 @@ def main():
     ap.add_argument("--pmc-summary", nargs=4, metavar=("FETCH", "WRITE", "COPY_FETCH", "COPY_WRITE"),
                     help="only summarise four rocprofv3 --pmc output folders (CSV): "
                          "FETCH_SIZE and WRITE_SIZE of a run and of --copy-calibration-mib")
+    ap.add_argument("--trace-summary", nargs=2, metavar=("DIR", "REGEX"),
+                    help="only summarise a rocprofv3 --kernel-trace output folder (CSV): "
+                         "average duration per kernel and grid size of kernels matching REGEX")
     args = ap.parse_args()
+    if args.trace_summary:
+        trace_summary(*args.trace_summary)
+        return
     if args.pmc_summary:
         pmc_summary(*args.pmc_summary, args.copy_calibration_mib or 256)
         return

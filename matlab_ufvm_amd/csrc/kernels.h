@@ -169,15 +169,17 @@ void coarse_lhs(hipStream_t s, int32_t nout, int32_t nc, const int32_t *ptr, con
 template <typename V>
 void mcgs_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const SellCols &cols,
                const V *val, const V *ac, const V *b, V *x, int32_t zero_above);
-// rows below zero_rows get r = 0 without computing it
+// rows below zero_rows get r = 0 without computing it.  wide: every entry of a
+// row loaded in one round (levels whose rows are longer than a run)
 template <typename V>
 void residual_sell(hipStream_t s, int32_t n, int32_t zero_rows, const int32_t *off,
-                   const SellCols &cols, const V *val, const V *ac, const V *b, const V *x, V *r);
+                   const SellCols &cols, const V *val, const V *ac, const V *b, const V *x, V *r,
+                   bool wide = false);
 // parent/xc not null: the sweep starts from x + xc[parent] (fused prolongation)
 template <typename V>
 void jacobi_sell(hipStream_t s, int32_t n, const int32_t *off, const SellCols &cols,
                  const V *val, const V *ac, const V *b, const V *x, const int32_t *parent, const V *xc, V *out,
-                 V omega);
+                 V omega, bool wide = false);
 // block-local multicolour GS over blocks of at most kBRows rows (see kernels.hip)
 constexpr int kBRows = 1024;
 template <typename V>

@@ -780,6 +780,48 @@ __device__ __forceinline__ V sell_row(const CS &col, const V *__restrict__ val,
   }
 }
 
+// Wide rows.  The coarse levels' rows hold 13-16 entries on average (up to
+// ~35), so runs of kSellRun wait for two or three round trips per row.  The
+// wide form issues the index and value loads of every entry of the row at
+// once, then every neighbour gather (xv(j): the neighbour's value), then the
+// products in entry order: the arithmetic and its order are those of
+// sell_row.  The row's w entries are loaded as N (w rounded up to a multiple
+// of kSellRun); slots past w repeat entry w-1 (same address, in bounds) and
+// are not summed, so the loads form one block with no branches between them.
+// w is the same for the 64 rows of a chunk, so a wavefront takes one branch.
+constexpr int kWideMax = 32;
+
+template <int N, typename V, bool NEG, typename CS, typename XV>
+__device__ __forceinline__ V sell_row_n(const CS &col, const V *__restrict__ val, const XV &xv,
+                                        int32_t base, int32_t w, V v) {
+  int32_t j[N];
+  V a[N], xj[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    const int32_t e = base + (u < w ? u : w - 1) * 64;
+    j[u] = col[e];
+    a[u] = val[e];
+  }
+#pragma unroll
+  for (int u = 0; u < N; ++u) xj[u] = xv(j[u]);
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    const V t = NEG ? v - a[u] * xj[u] : v + a[u] * xj[u];
+    v = u < w ? t : v;
+  }
+  return v;
+}
+
+// v -/+ sum over the row's w entries (kSellRun < w <= kWideMax) in one round
+template <typename V, bool NEG, typename CS, typename XV>
+__device__ __forceinline__ V sell_row_wide(const CS &col, const V *__restrict__ val, const XV &xv,
+                                           int32_t base, int32_t w, V v) {
+  static_assert(kWideMax == 4 * kSellRun, "wide buckets");
+  if (w <= 2 * kSellRun) return sell_row_n<2 * kSellRun, V, NEG>(col, val, xv, base, w, v);
+  if (w <= 3 * kSellRun) return sell_row_n<3 * kSellRun, V, NEG>(col, val, xv, base, w, v);
+  return sell_row_n<4 * kSellRun, V, NEG>(col, val, xv, base, w, v);
+}
+
 // Gauss-Seidel update of the contiguous rows [r0, r1) of one colour class.
 // Rows of one colour are mutually independent, so every read of x[col] sees
 // the value from an earlier colour pass.  Neighbours with index >= zero_above
@@ -969,18 +1011,28 @@ template void mcilu_bwd_sell<double>(hipStream_t, int32_t, int32_t, const int32_
 // r = b - A x; rows below zero_rows get r = 0 without computing it (the
 // colour class a Gauss-Seidel sweep solved last, whose residual is rounding
 // noise)
-template <typename V, bool PK>
+template <typename V, bool PK, bool WIDE = false>
+__device__ __forceinline__ V residual_val(int32_t i, const int32_t *__restrict__ off,
+                                          const SellCols &cols, const V *__restrict__ val,
+                                          const V *__restrict__ ac, const V *b, const V *x) {
+  const int32_t q = i >> 6;
+  const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
+  const ColSrc<PK> col(cols, q);
+  const V v0 = b[i] - ac[i] * x[i];
+  if (WIDE && w > kSellRun && w <= kWideMax)
+    return sell_row_wide<V, true>(col, val, [x](int32_t j) { return x[j]; }, base, w, v0);
+  return sell_row<V, true, false>(col, val, x, base, w, v0, 0);
+}
+
+template <typename V, bool PK, bool WIDE = false>
 __device__ __forceinline__ void residual_row(int32_t i, const int32_t *__restrict__ off,
                                              const SellCols &cols, const V *__restrict__ val,
                                              const V *__restrict__ ac, const V *b, const V *x,
                                              V *r) {
-  const int32_t q = i >> 6;
-  const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
-  const ColSrc<PK> col(cols, q);
-  r[i] = sell_row<V, true, false>(col, val, x, base, w, b[i] - ac[i] * x[i], 0);
+  r[i] = residual_val<V, PK, WIDE>(i, off, cols, val, ac, b, x);
 }
 
-template <typename V, bool PK>
+template <typename V, bool PK, bool WIDE>
 __global__ void __launch_bounds__(kBlock)
 k_residual_sell(int32_t n, int32_t zero_rows, const int32_t *__restrict__ off, SellCols cols,
                 const V *__restrict__ val, const V *__restrict__ ac, const V *__restrict__ b,
@@ -991,17 +1043,22 @@ k_residual_sell(int32_t n, int32_t zero_rows, const int32_t *__restrict__ off, S
     r[i] = V(0);
     return;
   }
-  residual_row<V, PK>(i, off, cols, val, ac, b, x, r);
+  residual_row<V, PK, WIDE>(i, off, cols, val, ac, b, x, r);
 }
 
 template <typename V>
 void residual_sell(hipStream_t s, int32_t n, int32_t zero_rows, const int32_t *off,
                    const SellCols &cols, const V *val, const V *ac, const V *b, const V *x,
-                   V *r) {
+                   V *r, bool wide) {
   if (n <= 0) return;
   with_cols(cols, [&](auto pk) {
-    hipLaunchKernelGGL((k_residual_sell<V, decltype(pk)::value>), dim3(nblocks(n)), dim3(kBlock),
-                       0, s, n, zero_rows, off, cols, val, ac, b, x, r);
+    constexpr bool PK = decltype(pk)::value;
+    if (wide)
+      hipLaunchKernelGGL((k_residual_sell<V, PK, true>), dim3(nblocks(n)), dim3(kBlock), 0, s, n,
+                         zero_rows, off, cols, val, ac, b, x, r);
+    else
+      hipLaunchKernelGGL((k_residual_sell<V, PK, false>), dim3(nblocks(n)), dim3(kBlock), 0, s,
+                         n, zero_rows, off, cols, val, ac, b, x, r);
   });
 }
 
@@ -1034,7 +1091,7 @@ __device__ __forceinline__ V jacobi_run(const CS &col, const V *__restrict__ val
 
 // one damped Jacobi sweep: out = v + omega (b - A v) / ac, v = x or, with
 // PRO, x + xc[parent] (the coarse correction added on the fly)
-template <typename V, bool PRO, bool PK>
+template <typename V, bool PRO, bool PK, bool WIDE = false>
 __device__ __forceinline__ void jacobi_row(int32_t i, const int32_t *__restrict__ off,
                                            const SellCols &cols, const V *__restrict__ val,
                                            const V *__restrict__ ac, const V *b, const V *x,
@@ -1045,6 +1102,13 @@ __device__ __forceinline__ void jacobi_row(int32_t i, const int32_t *__restrict_
   const ColSrc<PK> col(cols, q);
   const V vi = jac_val<V, PRO>(x, parent, xc, i), d = ac[i];
   V v = b[i] - d * vi;
+  if (WIDE && w > kSellRun && w <= kWideMax) {
+    v = sell_row_wide<V, true>(
+        col, val, [x, parent, xc](int32_t j) { return jac_val<V, PRO>(x, parent, xc, j); }, base,
+        w, v);
+    out[i] = vi + omega * v / d;
+    return;
+  }
   int32_t k = 0;
   for (; k + kSellRun <= w; k += kSellRun)
     v = jacobi_run<kSellRun, V, PRO>(col, val, x, parent, xc, base + k * 64, v);
@@ -1062,7 +1126,7 @@ __device__ __forceinline__ void jacobi_row(int32_t i, const int32_t *__restrict_
   out[i] = vi + omega * v / d;
 }
 
-template <typename V, bool PRO, bool PK>
+template <typename V, bool PRO, bool PK, bool WIDE>
 __global__ void __launch_bounds__(kBlock)
 k_jacobi_sell(int32_t n, const int32_t *__restrict__ off, SellCols cols,
               const V *__restrict__ val, const V *__restrict__ ac, const V *__restrict__ b,
@@ -1070,23 +1134,29 @@ k_jacobi_sell(int32_t n, const int32_t *__restrict__ off, SellCols cols,
               const V *__restrict__ xc, V *__restrict__ out, V omega) {
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  jacobi_row<V, PRO, PK>(i, off, cols, val, ac, b, x, parent, xc, out, omega);
+  jacobi_row<V, PRO, PK, WIDE>(i, off, cols, val, ac, b, x, parent, xc, out, omega);
 }
 
 template <typename V>
 void jacobi_sell(hipStream_t s, int32_t n, const int32_t *off, const SellCols &cols,
                  const V *val, const V *ac, const V *b, const V *x, const int32_t *parent,
-                 const V *xc, V *out, V omega) {
+                 const V *xc, V *out, V omega, bool wide) {
   if (n <= 0) return;
   const dim3 g(nblocks(n)), t(kBlock);
   with_cols(cols, [&](auto pk) {
     constexpr bool PK = decltype(pk)::value;
-    if (parent)
-      hipLaunchKernelGGL((k_jacobi_sell<V, true, PK>), g, t, 0, s, n, off, cols, val, ac, b, x,
-                         parent, xc, out, omega);
+    if (parent && wide)
+      hipLaunchKernelGGL((k_jacobi_sell<V, true, PK, true>), g, t, 0, s, n, off, cols, val, ac, b,
+                         x, parent, xc, out, omega);
+    else if (parent)
+      hipLaunchKernelGGL((k_jacobi_sell<V, true, PK, false>), g, t, 0, s, n, off, cols, val, ac,
+                         b, x, parent, xc, out, omega);
+    else if (wide)
+      hipLaunchKernelGGL((k_jacobi_sell<V, false, PK, true>), g, t, 0, s, n, off, cols, val, ac,
+                         b, x, parent, xc, out, omega);
     else
-      hipLaunchKernelGGL((k_jacobi_sell<V, false, PK>), g, t, 0, s, n, off, cols, val, ac, b, x,
-                         parent, xc, out, omega);
+      hipLaunchKernelGGL((k_jacobi_sell<V, false, PK, false>), g, t, 0, s, n, off, cols, val, ac,
+                         b, x, parent, xc, out, omega);
   });
 }
 
@@ -1178,6 +1248,27 @@ __device__ __forceinline__ V restrict_run(const int32_t *__restrict__ child, con
   return s;
 }
 
+// the sum of restrict_run over a child list of w slots (kSellRun < w <= N)
+// in one round: every slot's index load, then every residual load, then the
+// sum in list order; slots past w repeat slot w-1 and are not summed
+template <int N, typename V>
+__device__ __forceinline__ V restrict_wide(const int32_t *__restrict__ child, const V *r,
+                                           int32_t base, int32_t w) {
+  int32_t c[N];
+  V v[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) c[u] = child[base + (u < w ? u : w - 1) * 64];
+#pragma unroll
+  for (int u = 0; u < N; ++u) v[u] = r[c[u] >= 0 ? c[u] : 0];
+  V s = 0;
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    const V t = s + (c[u] >= 0 ? v[u] : V(0));
+    s = u < w ? t : s;
+  }
+  return s;
+}
+
 template <typename V, bool JAC>
 __device__ __forceinline__ void restrict_row(int32_t p, const int32_t *__restrict__ off,
                                              const int32_t *__restrict__ child, const V *r,
@@ -1186,6 +1277,15 @@ __device__ __forceinline__ void restrict_row(int32_t p, const int32_t *__restric
   const int32_t q = p >> 6;
   const int32_t base = off[q] + (p & 63), w = (off[q + 1] - off[q]) >> 6;
   V s = 0;
+  if (w > kSellRun && w <= kWideMax) {
+    if (w <= 2 * kSellRun) s = restrict_wide<2 * kSellRun>(child, r, base, w);
+    else if (w <= 3 * kSellRun) s = restrict_wide<3 * kSellRun>(child, r, base, w);
+    else s = restrict_wide<4 * kSellRun>(child, r, base, w);
+    const V t = scale * s;
+    bc[p] = t;
+    if (JAC) xc[p] = omega * t / acc[p];
+    return;
+  }
   int32_t k = 0;
   for (; k + kSellRun <= w; k += kSellRun) s = restrict_run<kSellRun>(child, r, base + k * 64, s);
   const int32_t e0 = base + k * 64;
@@ -2101,10 +2201,10 @@ template void group_fcg<double>(hipStream_t, GFcgOp, int32_t, int32_t, const GFc
                              const V *, const V *, const V *, V *, int32_t);                    \
   template void residual_sell<V>(hipStream_t, int32_t, int32_t, const int32_t *,                \
                                  const SellCols &, const V *, const V *, const V *, const V *,  \
-                                 V *);                                                          \
+                                 V *, bool);                                                    \
   template void jacobi_sell<V>(hipStream_t, int32_t, const int32_t *, const SellCols &,         \
                                const V *, const V *, const V *, const V *, const int32_t *,     \
-                               const V *, V *, V);                                              \
+                               const V *, V *, V, bool);                                        \
   template void restrict_jacobi<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,      \
                                    const V *, const V *, V *, V *, V, V);                      \
   template void residual_fast<V>(hipStream_t, int32_t, const int32_t *, const int32_t *,        \

@@ -48,6 +48,7 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
   FL.n = n;
   FL.nnz = R.nnz;
   FL.kind = kind;
+  FL.wide = kind == 3 && (int64_t)R.nnz > (int64_t)k::kSellRun * n;
   FL.perm.resize(n);
   if (kind == 0 || kind == 5 || kind == 6) {
     FL.ncol = greedy_colour(R.g, FL.colour);
@@ -429,15 +430,16 @@ struct Cycle {
     const PcgState &Pc = S.pcg;
     const FastLevel &C = *Pc.lev[l + 1];
     LevelWork &CW = W.lev[l + 1];
+    // the coarse correction is scaled by coarse_scale through the restricted
+    // right-hand side (the coarse cycle is linear in it)
+    const V scale = V(Pc.cfg.coarse_scale);
+    const bool jac = C.kind == 3 && pre_of(l + 1) > 0;
     if (L.kind == 1)
       k::residual_fast<V>(s, L.n, L.ptr.p, L.col.p, P::ac(L), P::anb(L), P::b(LW), x, P::r(LW));
     else
       k::residual_sell<V>(s, L.n, zero_rows, L.soff.p, sell_cols(L), P::sval(L), P::ac(L), P::b(LW),
-                          x, P::r(LW));
-    // the coarse correction is scaled by coarse_scale through the restricted
-    // right-hand side (the coarse cycle is linear in it)
-    const V scale = V(Pc.cfg.coarse_scale);
-    if (C.kind == 3 && pre_of(l + 1) > 0)
+                          x, P::r(LW), L.wide);
+    if (jac)
       k::restrict_jacobi<V>(s, C.n, L.csoff.p, L.cschild.p, P::r(LW), P::ac(C), P::b(CW),
                             P::x(CW), V(2.0 / 3.0), scale);
     else
@@ -463,7 +465,7 @@ struct Cycle {
       V *y = P::y(LW);
       for (int it = 1; it < pre; ++it) {
         k::jacobi_sell<V>(s, L.n, L.soff.p, sell_cols(L), P::sval(L), P::ac(L), P::b(LW), x, nullptr,
-                          nullptr, y, omega);
+                          nullptr, y, omega, L.wide);
         std::swap(x, y);
       }
       if (pre == 0) k::fill<V>(s, L.n, x, V(0));
@@ -483,7 +485,7 @@ struct Cycle {
       for (int it = 0; it < post; ++it) {
         const bool pro = fuse && it == 0;
         k::jacobi_sell<V>(s, L.n, L.soff.p, sell_cols(L), P::sval(L), P::ac(L), P::b(LW), x,
-                          pro ? L.parent.p : nullptr, pro ? xc : nullptr, y, omega);
+                          pro ? L.parent.p : nullptr, pro ? xc : nullptr, y, omega, L.wide);
         std::swap(x, y);
       }
       return x;

@@ -148,6 +148,9 @@ struct FastLevel {
   // kind 6: finest level smoothed by ILU(0) in colour order: kind 5's
   //         numbering and lists, plus the factor's own off-diagonal values
   int kind = 0;
+  // rows longer than a run on average (the coarse levels): the residual and
+  // Jacobi kernels load a row's entries in one round (k::kWideMax)
+  bool wide = false;
   std::vector<int32_t> col_off;   // kind 0: colour c rows are [col_off[c], col_off[c+1])
   int32_t nblk = 0, bncol = 0;    // kind 1: blocks and colours per block
   std::vector<int32_t> h_bptr, h_boff;
