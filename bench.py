@@ -83,21 +83,38 @@ def build_problem(n, workload, ctx):
 
 
 def copy_calibration(mib):
-    """A device-to-device copy of a known size (the runtime's blit kernel), run
-    alone so that a counter profile of it scales FETCH_SIZE / WRITE_SIZE."""
+    """Kernels of known traffic, run alone so that a counter profile of them
+    scales FETCH_SIZE / WRITE_SIZE: a device-to-device copy of mib MiB (the
+    runtime's blit kernel), then the calibration library's float and double
+    stream copies and float gather over buffers of the same size, 5 launches
+    each."""
     import ctypes as C
     hip = C.CDLL("libamdhip64.so")
     n = mib << 20
-    a, b = C.c_void_p(), C.c_void_p()
-    assert hip.hipMalloc(C.byref(a), C.c_size_t(n)) == 0
-    assert hip.hipMalloc(C.byref(b), C.c_size_t(n)) == 0
-    assert hip.hipMemset(a, 1, C.c_size_t(n)) == 0
+    bufs = [C.c_void_p() for _ in range(3)]
+    for p in bufs:
+        assert hip.hipMalloc(C.byref(p), C.c_size_t(n)) == 0
+        assert hip.hipMemset(p, 1, C.c_size_t(n)) == 0
+    a, b, c = bufs
     for _ in range(5):
         assert hip.hipMemcpy(b, a, C.c_size_t(n), 3) == 0      # hipMemcpyDeviceToDevice
     assert hip.hipDeviceSynchronize() == 0
-    hip.hipFree(a)
-    hip.hipFree(b)
-    print(json.dumps({"copy_calibration_bytes": n, "copies": 5}), flush=True)
+    from matlab_ufvm_amd.build import CALIB_LIB
+    cal = C.CDLL(CALIB_LIB)
+    cal.ufvm_calibration_run.argtypes = [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int64,
+                                         C.c_int]
+    for kind in (0, 1, 2):
+        assert cal.ufvm_calibration_run(kind, a, b, c, n, 5) == 0
+    for p in bufs:
+        hip.hipFree(p)
+    print(json.dumps({"copy_calibration_bytes": n, "launches_per_kernel": 5}), flush=True)
+
+
+# bytes each calibration kernel reads and writes per launch, per byte of
+# buffer (copy_calibration): blit copy, float / double stream, float gather
+# (index stream + gathered values)
+CALIBRATION_KERNELS = (("copyBuffer", 1.0, 1.0), ("k_cal_stream<float>", 1.0, 1.0),
+                       ("k_cal_stream<double>", 1.0, 1.0), ("k_cal_gather", 2.0, 1.0))
 
 
 def _pmc_load(d):
@@ -154,16 +171,28 @@ def trace_summary(d, regex):
 
 def pmc_summary(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, copy_mib):
     """HBM bytes per dispatch of each kernel from two rocprofv3 --pmc runs
-    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch), each scaled by what the same
-    counter reads for the device copies of copy_calibration (5 x copy_mib MiB
-    read and written)."""
-    moved = 5 * (copy_mib << 20)
-    scale = {}
-    for key, d in (("fetch", copy_fetch_dir), ("write", copy_write_dir)):
-        kib = sum(v for (k, _), vs in _pmc_load(d).items() if "copyBuffer" in k for v in vs)
-        scale[key] = moved / (kib * 1024.0)
-    print(f"calibration: device copies of {5} x {copy_mib} MiB; bytes = KiB * 1024 * factor, "
-          f"factor FETCH_SIZE {scale['fetch']:.3f}, WRITE_SIZE {scale['write']:.3f}")
+    (FETCH_SIZE, WRITE_SIZE; KiB per dispatch).  The counters of the
+    calibration kernels of copy_calibration (copy_mib MiB buffers) give, per
+    kernel, the factor between the bytes it moves and what the counter reads;
+    the float gather's factors (an index stream, gathered values and a
+    streamed write, as in the solver's kernels) scale the table."""
+    n = copy_mib << 20
+    cfe, cwr = _pmc_load(copy_fetch_dir), _pmc_load(copy_write_dir)
+    factors = {}
+    for tag, rd, wt in CALIBRATION_KERNELS:
+        fk = [v for (k, _), vs in cfe.items() if tag in k for v in vs]
+        wk = [v for (k, _), vs in cwr.items() if tag in k for v in vs]
+        if not fk or not wk:
+            continue
+        f = rd * n / (sum(fk) / len(fk) * 1024.0)
+        w = wt * n / (sum(wk) / len(wk) * 1024.0)
+        factors[tag] = (f, w)
+        print(f"calibration {tag:22s}: {len(fk)} dispatches of {rd * n / 2**20:.0f} MiB read, "
+              f"{wt * n / 2**20:.0f} MiB written; factor FETCH_SIZE {f:.3f}, WRITE_SIZE {w:.3f}")
+    fs, ws = factors.get("k_cal_gather", factors.get("copyBuffer", (1.0, 1.0)))
+    scale = {"fetch": fs, "write": ws}
+    print(f"table: bytes = KiB * 1024 * factor, factor FETCH_SIZE {fs:.3f}, WRITE_SIZE {ws:.3f} "
+          f"(the float gather's)")
     fe, wr = _pmc_load(fetch_dir), _pmc_load(write_dir)
     print(f"{'kernel':60s} {'grid':>9s} {'dispatches':>10s} {'fetch MB':>10s} {'write MB':>10s}"
           f" {'HBM MB/dispatch':>15s}")
@@ -184,18 +213,26 @@ PMC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles
                            "vcycle_kernels_laplacian_cavity128_pmc.txt")
 
 
-def committed_traffic(kernel):
-    """HBM bytes per dispatch of `kernel` from the committed --pmc-summary
-    table (dispatch-weighted over its grid sizes), or None."""
+def committed_traffic(kernel, grid=None):
+    """HBM bytes per dispatch from the committed --pmc-summary table: the
+    rows whose kernel name starts with `kernel` (and whose grid size is
+    `grid`, if given), dispatch-weighted, or None.  The table's first 60
+    columns hold the kernel name."""
     if not os.path.exists(PMC_PROFILE):
         return None
     tot = cnt = 0.0
     with open(PMC_PROFILE) as fh:
         for line in fh:
-            parts = line.split()
-            if len(parts) >= 6 and parts[0] == kernel:
-                tot += int(parts[2]) * float(parts[5])
-                cnt += int(parts[2])
+            name, rest = line[:60].strip(), line[60:].split()
+            if len(rest) != 5 or not name.startswith(kernel):
+                continue
+            try:
+                g, n_disp, total = int(rest[0]), int(rest[1]), float(rest[4])
+            except ValueError:
+                continue
+            if grid is None or g == grid:
+                tot += n_disp * total
+                cnt += n_disp
     return tot / cnt * 1e6 if cnt else None
 
 
@@ -835,7 +872,8 @@ def main():
     avg_ms = ms0 / max(n0, 1)
     achieved = (by0 / max(n0, 1)) / (avg_ms * 1e-3) if n0 else 0.0
     value = args.steps / r32["total"]
-    traffic = committed_traffic("k_mcgs_sell<float>") if args.n == 128 else None
+    traffic = (committed_traffic("k_mcgs_sell<float")
+               if args.n == 128 and args.workload != "tet" else None)
     out = {
         "metric": METRIC, "value": round(value, 3), "unit": "solves/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -865,8 +903,9 @@ def main():
                      "traffic": (None if traffic is None else round(traffic)),
                      "traffic_source": (None if traffic is None else
                                         "rocprofv3 FETCH_SIZE + WRITE_SIZE per dispatch, "
-                                        "copy-calibrated: profiles/" +
-                                        os.path.basename(PMC_PROFILE)),
+                                        "scaled by the factors the same counters read for "
+                                        "kernels of known traffic (x2.000 / x1.000): "
+                                        "profiles/" + os.path.basename(PMC_PROFILE)),
                      "measured": f"HIP events on the library stream around the back-to-back "
                                  f"colour passes of every 5th finest-level smoothing call, "
                                  f"{args.event_steps} solves re-run after the timed ones"},

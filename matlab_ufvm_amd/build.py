@@ -33,14 +33,23 @@ def hipcc() -> str:
     return os.path.join(rocm, "bin", "hipcc")
 
 
-def build(verbose: bool = False) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, *srcs, "-o", tmp]
+# kernels of known traffic that calibrate the profiler's memory counters
+# (measurement tooling loaded by bench.py only; not part of the solver)
+CALIB_LIB = os.path.join(HERE, "libufvm_calib.so")
+
+
+def _compile(sources, out, verbose):
+    tmp = out + ".tmp"
+    cmd = [hipcc(), *HIPCC_FLAGS, "-I", os.path.join(ROOT, "include"), "-I", CSRC, *sources, "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
+
+
+def build(verbose: bool = False) -> str:
+    _compile([os.path.join(CSRC, s) for s in SOURCES], LIB, verbose)
+    _compile([os.path.join(CSRC, "calibrate.hip")], CALIB_LIB, verbose)
     return LIB
 
 

@@ -25,6 +25,16 @@ __device__ __forceinline__ int32_t xcd_block(int32_t b, int32_t G) {
   return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
 }
 
+// Rows of the FP64 operator product (k_spmv_dot and its batched form): block
+// b of nb takes spmv_trips(n, nb) consecutive kFcgBlock-row chunks from chunk
+// xcd_block(b, nb) * trips, so each XCD takes one contiguous share of the
+// rows and the neighbour values that share gathers stay in its own L2 (with
+// the rows interleaved over the XCDs, every L2 fetched most of the vector).
+__host__ __device__ __forceinline__ int32_t spmv_trips(int32_t n, int32_t nb) {
+  const int32_t chunks = (n + kFcgBlock - 1) / kFcgBlock;
+  return (chunks + nb - 1) / nb;
+}
+
 // Column indices of a sliced-ELL copy: one int32 per entry (col), or, when
 // every 64-row chunk's columns lie within 65536 of the chunk's smallest, a
 // uint16 offset per entry (d16) from a base per chunk (base).  Exactly one

@@ -1694,7 +1694,10 @@ __device__ __forceinline__ void spmv_dot_body(int32_t blk, int32_t nb, int32_t n
                                               double *__restrict__ w,
                                               double *__restrict__ partial, double *lds) {
   double d = 0.0;
-  for (int32_t i = blk * kFcgBlock + threadIdx.x; i < n; i += nb * kFcgBlock) {
+  const int32_t T = spmv_trips(n, nb), c0 = xcd_block(blk, nb) * T;
+  for (int32_t t = 0; t < T; ++t) {
+    const int32_t i = (c0 + t) * kFcgBlock + threadIdx.x;
+    if (i >= n) break;
     const int32_t q = i >> 6;
     const int32_t base = off[q] + (i & 63), wq = (off[q + 1] - off[q]) >> 6;
     const ColSrc<PK> col(cols, q);

@@ -321,7 +321,10 @@ k_spmv_dot_k(int32_t n, const int32_t *__restrict__ off, SellCols cols,
   double d[K];
 #pragma unroll
   for (int r = 0; r < K; ++r) d[r] = 0.0;
-  for (int32_t i = blockIdx.x * kFcgBlock + threadIdx.x; i < n; i += gridDim.x * kFcgBlock) {
+  const int32_t T = spmv_trips(n, gridDim.x), c0 = xcd_block(blockIdx.x, gridDim.x) * T;
+  for (int32_t t = 0; t < T; ++t) {
+    const int32_t i = (c0 + t) * kFcgBlock + threadIdx.x;
+    if (i >= n) break;
     const int32_t q = i >> 6;
     const int32_t base = off[q] + (i & 63), wq = (off[q + 1] - off[q]) >> 6;
     const ColSrc<PK> col(cols, q);
