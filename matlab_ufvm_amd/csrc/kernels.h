@@ -54,6 +54,7 @@ struct ColSrc<false> {
   const int32_t *c;
   __device__ __forceinline__ ColSrc(const SellCols &s, int32_t) : c(s.col) {}
   __device__ __forceinline__ int32_t operator[](int32_t e) const { return c[e]; }
+  __device__ __forceinline__ int32_t nt(int32_t e) const { return __builtin_nontemporal_load(c + e); }
 };
 template <>
 struct ColSrc<true> {
@@ -61,6 +62,9 @@ struct ColSrc<true> {
   int32_t b;
   __device__ __forceinline__ ColSrc(const SellCols &s, int32_t q) : d(s.d16), b(s.base[q]) {}
   __device__ __forceinline__ int32_t operator[](int32_t e) const { return b + (int32_t)d[e]; }
+  __device__ __forceinline__ int32_t nt(int32_t e) const {
+    return b + (int32_t)__builtin_nontemporal_load(d + e);
+  }
 };
 // entries per run of the sliced-ELL row loops (loads issued together)
 constexpr int kSellRun = 8;

@@ -734,15 +734,15 @@ void coarse_lhs(hipStream_t s, int32_t nout, int32_t nc, const int32_t *ptr, con
 
 // v -/+ sum over the N entries from slot e0 of val x[col] (NEG: subtract);
 // ZA: neighbours with index >= zero_above read as 0
-template <int N, typename V, bool NEG, bool ZA, typename CS>
+template <int N, typename V, bool NEG, bool ZA, bool NT, typename CS>
 __device__ __forceinline__ V sell_run(const CS &col, const V *__restrict__ val, const V *x,
                                       int32_t e0, V v, int32_t zero_above) {
   int32_t j[N];
   V a[N], xj[N];
 #pragma unroll
   for (int u = 0; u < N; ++u) {
-    j[u] = col[e0 + u * 64];
-    a[u] = val[e0 + u * 64];
+    j[u] = NT ? col.nt(e0 + u * 64) : col[e0 + u * 64];
+    a[u] = NT ? __builtin_nontemporal_load(val + e0 + u * 64) : val[e0 + u * 64];
   }
 #pragma unroll
   for (int u = 0; u < N; ++u) xj[u] = x[j[u]];
@@ -760,22 +760,22 @@ __device__ __forceinline__ V sell_run(const CS &col, const V *__restrict__ val, 
 
 // v -/+ sum_k val_k x[col_k] over the row's w entries from slot base, in
 // entry order
-template <typename V, bool NEG, bool ZA, typename CS>
+template <typename V, bool NEG, bool ZA, typename CS, bool NT = false>
 __device__ __forceinline__ V sell_row(const CS &col, const V *__restrict__ val,
                                       const V *x, int32_t base, int32_t w, V v,
                                       int32_t zero_above) {
   int32_t k = 0;
   for (; k + kSellRun <= w; k += kSellRun)
-    v = sell_run<kSellRun, V, NEG, ZA>(col, val, x, base + k * 64, v, zero_above);
+    v = sell_run<kSellRun, V, NEG, ZA, NT>(col, val, x, base + k * 64, v, zero_above);
   const int32_t e0 = base + k * 64;
   switch (w - k) {
-    case 1: return sell_run<1, V, NEG, ZA>(col, val, x, e0, v, zero_above);
-    case 2: return sell_run<2, V, NEG, ZA>(col, val, x, e0, v, zero_above);
-    case 3: return sell_run<3, V, NEG, ZA>(col, val, x, e0, v, zero_above);
-    case 4: return sell_run<4, V, NEG, ZA>(col, val, x, e0, v, zero_above);
-    case 5: return sell_run<5, V, NEG, ZA>(col, val, x, e0, v, zero_above);
-    case 6: return sell_run<6, V, NEG, ZA>(col, val, x, e0, v, zero_above);
-    case 7: return sell_run<7, V, NEG, ZA>(col, val, x, e0, v, zero_above);
+    case 1: return sell_run<1, V, NEG, ZA, NT>(col, val, x, e0, v, zero_above);
+    case 2: return sell_run<2, V, NEG, ZA, NT>(col, val, x, e0, v, zero_above);
+    case 3: return sell_run<3, V, NEG, ZA, NT>(col, val, x, e0, v, zero_above);
+    case 4: return sell_run<4, V, NEG, ZA, NT>(col, val, x, e0, v, zero_above);
+    case 5: return sell_run<5, V, NEG, ZA, NT>(col, val, x, e0, v, zero_above);
+    case 6: return sell_run<6, V, NEG, ZA, NT>(col, val, x, e0, v, zero_above);
+    case 7: return sell_run<7, V, NEG, ZA, NT>(col, val, x, e0, v, zero_above);
     default: return v;
   }
 }
@@ -1021,7 +1021,7 @@ __device__ __forceinline__ V residual_val(int32_t i, const int32_t *__restrict__
   const V v0 = b[i] - ac[i] * x[i];
   if (WIDE && w > kSellRun && w <= kWideMax)
     return sell_row_wide<V, true>(col, val, [x](int32_t j) { return x[j]; }, base, w, v0);
-  return sell_row<V, true, false>(col, val, x, base, w, v0, 0);
+  return sell_row<V, true, false, ColSrc<PK>, true>(col, val, x, base, w, v0, 0);
 }
 
 template <typename V, bool PK, bool WIDE = false>
@@ -1702,7 +1702,8 @@ __device__ __forceinline__ void spmv_dot_body(int32_t blk, int32_t nb, int32_t n
     const int32_t base = off[q] + (i & 63), wq = (off[q + 1] - off[q]) >> 6;
     const ColSrc<PK> col(cols, q);
     const double pi = p[i];
-    const double v = sell_row<double, false, false>(col, val, p, base, wq, ac[i] * pi, 0);
+    const double v =
+        sell_row<double, false, false, ColSrc<PK>, true>(col, val, p, base, wq, ac[i] * pi, 0);
     w[i] = v;
     d += pi * v;
   }
