@@ -1753,8 +1753,8 @@ __device__ __forceinline__ void fcg_dots_body(int32_t blk, int32_t nb, int32_t n
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       zi[u] = (double)z[i + u * G];
-      wi[u] = w[i + u * G];
-      ri[u] = r[i + u * G];
+      wi[u] = __builtin_nontemporal_load(w + i + u * G);
+      ri[u] = __builtin_nontemporal_load(r + i + u * G);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) add(zi[u], wi[u], ri[u]);
@@ -1887,10 +1887,10 @@ __device__ __forceinline__ void fcg_update_xr_body(
   if (ahead) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      xi[u] = x[i + u * G];
-      pi[u] = p[i + u * G];
-      ri[u] = r[i + u * G];
-      wi[u] = w[i + u * G];
+      xi[u] = __builtin_nontemporal_load(x + i + u * G);
+      pi[u] = __builtin_nontemporal_load(p + i + u * G);
+      ri[u] = __builtin_nontemporal_load(r + i + u * G);
+      wi[u] = __builtin_nontemporal_load(w + i + u * G);
     }
   }
   double t[1];
@@ -1903,9 +1903,9 @@ __device__ __forceinline__ void fcg_update_xr_body(
   }
   double rr = 0.0;
   auto row = [&](int32_t i, double xi, double pi, double ri0, double wi) {
-    x[i] = xi + a * pi;
+    __builtin_nontemporal_store(xi + a * pi, x + i);
     const double ri = ri0 - a * wi;
-    r[i] = ri;
+    __builtin_nontemporal_store(ri, r + i);
     const V rvi = (V)ri;
     rv[i] = rvi;
     if (i < n0) z0[i] = rvi / acv[i];
@@ -1919,10 +1919,10 @@ __device__ __forceinline__ void fcg_update_xr_body(
   for (; i + 3 * G < n; i += 4 * G) {
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
-      xi[u] = x[i + u * G];
-      pi[u] = p[i + u * G];
-      ri[u] = r[i + u * G];
-      wi[u] = w[i + u * G];
+      xi[u] = __builtin_nontemporal_load(x + i + u * G);
+      pi[u] = __builtin_nontemporal_load(p + i + u * G);
+      ri[u] = __builtin_nontemporal_load(r + i + u * G);
+      wi[u] = __builtin_nontemporal_load(w + i + u * G);
     }
 #pragma unroll
     for (int u = 0; u < 4; ++u) row(i + u * G, xi[u], pi[u], ri[u], wi[u]);
