@@ -23,6 +23,44 @@ namespace k {
 
 // ------------------------------------------------------------ reference mode
 
+// v - sum over the CSR entries [k0, k1) of anb[k] * xv(col[k]), in entry order
+// (the reference rows' arithmetic).  The entries are taken in chunks whose
+// index and value loads are issued together, then their neighbour loads, so a
+// row waits for two round trips per chunk instead of two per entry; a chunk
+// of N slots holding cnt entries repeats its last entry in the other slots
+// (same address) and does not sum them.
+template <int N, typename XV>
+__device__ __forceinline__ double csr_chunk(const int32_t *__restrict__ col,
+                                            const double *__restrict__ anb, const XV &xv,
+                                            int32_t k0, int32_t cnt, double v) {
+  int32_t c[N];
+  double a[N], xj[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    const int32_t e = k0 + (u < cnt ? u : cnt - 1);
+    c[u] = col[e];
+    a[u] = anb[e];
+  }
+#pragma unroll
+  for (int u = 0; u < N; ++u) xj[u] = xv(c[u]);
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    const double t = v - a[u] * xj[u];
+    v = u < cnt ? t : v;
+  }
+  return v;
+}
+
+template <typename XV>
+__device__ __forceinline__ double csr_row_sum(const int32_t *__restrict__ col,
+                                              const double *__restrict__ anb, const XV &xv,
+                                              int32_t k0, int32_t k1, double v) {
+  if (k1 <= k0) return v;
+  if (k1 - k0 <= 8) return csr_chunk<8>(col, anb, xv, k0, k1 - k0, v);
+  for (; k0 + 16 < k1; k0 += 16) v = csr_chunk<16>(col, anb, xv, k0, 16, v);
+  return csr_chunk<16>(col, anb, xv, k0, k1 - k0, v);
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_residual_ref(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
                const double *__restrict__ ac, const double *__restrict__ anb,
@@ -34,8 +72,8 @@ k_residual_ref(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__rest
   const int32_t i = blockIdx.x * kBlock + threadIdx.x;
   double a = 0.0;
   if (i < n) {
-    double s = bc[i] - ac[i] * x[i];
-    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) s = s - anb[k] * x[col[k]];
+    const double s = csr_row_sum(col, anb, [x](int32_t j) { return x[j]; }, ptr[i], ptr[i + 1],
+                                 bc[i] - ac[i] * x[i]);
     r[i] = s;
     a = fabs(s);
   }
@@ -208,8 +246,7 @@ __device__ __forceinline__ void gs_row(int32_t i, const int32_t *__restrict__ pt
                                        const double *__restrict__ ac,
                                        const double *__restrict__ anb, const double *b,
                                        double *x) {
-  double v = b[i];
-  for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) v = v - anb[k] * x[col[k]];
+  const double v = csr_row_sum(col, anb, [x](int32_t j) { return x[j]; }, ptr[i], ptr[i + 1], b[i]);
   x[i] = v / ac[i];
 }
 
@@ -259,8 +296,8 @@ k_sor_lds(const int32_t *__restrict__ fr, const int32_t *__restrict__ flp, int32
     for (int32_t q = 0; q < nlev; ++q) {
       for (int32_t t = lp[q] + threadIdx.x; t < lp[q + 1]; t += kSweepBlock) {
         const int32_t i = rows[t];
-        double v = b[i];
-        for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) v = v - anb[k] * xs[col[k]];
+        const double v =
+            csr_row_sum(col, anb, [](int32_t j) { return xs[j]; }, ptr[i], ptr[i + 1], b[i]);
         xs[i] = v / ac[i];
       }
       __syncthreads();
