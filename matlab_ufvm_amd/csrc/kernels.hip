@@ -61,6 +61,111 @@ __device__ __forceinline__ double csr_row_sum(const int32_t *__restrict__ col,
   return csr_chunk<16>(col, anb, xv, k0, k1 - k0, v);
 }
 
+// The DILU rows' sums (cfdFactorizeILU.m / cfdSolveILU.m arithmetic, in list
+// order) in chunks of kDiluChunk list entries: every index load of a chunk,
+// then every value load, then the chunk's terms in order (slots past cnt
+// repeat the last entry and are not summed).
+constexpr int kDiluChunk = 8;
+
+// v - sum over the lower-neighbour list [e0, e1) of anb[lpji[e]] * (dinv[i] *
+// rv(i)), i = lnb[e] (the forward substitution)
+template <typename V, typename RV>
+__device__ __forceinline__ V dilu_lower_sum(const int32_t *__restrict__ lnb,
+                                            const int32_t *__restrict__ lpji,
+                                            const V *__restrict__ anb, const V *__restrict__ dinv,
+                                            const RV &rv, int32_t e0, int32_t e1, V v) {
+  constexpr int N = kDiluChunk;
+  for (; e0 < e1; e0 += N) {
+    const int32_t cnt = e1 - e0 < N ? e1 - e0 : N;
+    int32_t ii[N], pj[N];
+    V d[N], r[N], a[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+      const int32_t e = e0 + (u < cnt ? u : cnt - 1);
+      ii[u] = lnb[e];
+      pj[u] = lpji[e];
+    }
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+      d[u] = dinv[ii[u]];
+      r[u] = rv(ii[u]);
+      a[u] = anb[pj[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+      const V mat1 = d[u] * r[u];
+      const V t = v - a[u] * mat1;
+      v = u < cnt ? t : v;
+    }
+  }
+  return v;
+}
+
+// d - sum over the lower-neighbour list [e0, e1), skipping neighbour
+// skip_row, of (anb[lpji[e]] * dv(i)) * anb[lpij[e]], i = lnb[e] (the DILU
+// diagonal of cfdFactorizeILU.m)
+template <typename DV>
+__device__ __forceinline__ double dilu_factor_sum(const int32_t *__restrict__ lnb,
+                                                  const int32_t *__restrict__ lpji,
+                                                  const int32_t *__restrict__ lpij,
+                                                  const double *__restrict__ anb, const DV &dv,
+                                                  int32_t skip_row, int32_t e0, int32_t e1,
+                                                  double d) {
+  constexpr int N = kDiluChunk;
+  for (; e0 < e1; e0 += N) {
+    const int32_t cnt = e1 - e0 < N ? e1 - e0 : N;
+    int32_t ii[N], pj[N], pi[N];
+    double dj[N], aji[N], aij[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+      const int32_t e = e0 + (u < cnt ? u : cnt - 1);
+      ii[u] = lnb[e];
+      pj[u] = lpji[e];
+      pi[u] = lpij[e];
+    }
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+      dj[u] = dv(ii[u]);
+      aji[u] = anb[pj[u]];
+      aij[u] = anb[pi[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+      const double t = d - (aji[u] * dj[u]) * aij[u];
+      d = (u < cnt && ii[u] != skip_row) ? t : d;
+    }
+  }
+  return d;
+}
+
+// v - sum over the CSR entries [k0, k1) with column j > i of anb[k] * rv(j)
+// (the backward substitution)
+template <typename V, typename RV>
+__device__ __forceinline__ V dilu_upper_sum(const int32_t *__restrict__ col,
+                                            const V *__restrict__ anb, const RV &rv, int32_t i,
+                                            int32_t k0, int32_t k1, V v) {
+  constexpr int N = kDiluChunk;
+  for (; k0 < k1; k0 += N) {
+    const int32_t cnt = k1 - k0 < N ? k1 - k0 : N;
+    int32_t c[N];
+    V a[N], r[N];
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+      const int32_t e = k0 + (u < cnt ? u : cnt - 1);
+      c[u] = col[e];
+      a[u] = anb[e];
+    }
+#pragma unroll
+    for (int u = 0; u < N; ++u) r[u] = rv(c[u]);
+#pragma unroll
+    for (int u = 0; u < N; ++u) {
+      const V t = v - a[u] * r[u];
+      v = (u < cnt && c[u] > i) ? t : v;
+    }
+  }
+  return v;
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_residual_ref(int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
                const double *__restrict__ ac, const double *__restrict__ anb,
@@ -333,13 +438,8 @@ k_dilu_lds(const int32_t *__restrict__ fr, const int32_t *__restrict__ flp, int3
   for (int32_t q = 0; q < nf; ++q) {
     for (int32_t t = flp[q] + threadIdx.x; t < flp[q + 1]; t += kSweepBlock) {
       const int32_t j = fr[t];
-      double v = rs[j];
-      for (int32_t e = lptr[j]; e < lptr[j + 1]; ++e) {
-        const int32_t i = lnb[e];
-        const double mat1 = dinv[i] * rs[i];
-        v = v - anb[lpji[e]] * mat1;
-      }
-      rs[j] = v;
+      rs[j] = dilu_lower_sum(lnb, lpji, anb, dinv, [](int32_t i) { return rs[i]; }, lptr[j],
+                             lptr[j + 1], rs[j]);
     }
     __syncthreads();
   }
@@ -347,12 +447,8 @@ k_dilu_lds(const int32_t *__restrict__ fr, const int32_t *__restrict__ flp, int3
     for (int32_t t = blp[q] + threadIdx.x; t < blp[q + 1]; t += kSweepBlock) {
       const int32_t i = br[t];
       double v = rs[i];
-      if (i < n - 1) {
-        for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
-          const int32_t j = col[k];
-          if (j > i) v = v - anb[k] * rs[j];
-        }
-      }
+      if (i < n - 1)
+        v = dilu_upper_sum(col, anb, [](int32_t j) { return rs[j]; }, i, ptr[i], ptr[i + 1], v);
       const double z = dinv[i] * v;
       rs[i] = z;
       x[i] = x[i] + z;
@@ -382,12 +478,8 @@ k_dilu_factor_lds(const int32_t *__restrict__ rows, const int32_t *__restrict__ 
   for (int32_t q = 0; q < nlev; ++q) {
     for (int32_t t = lp[q] + threadIdx.x; t < lp[q + 1]; t += kSweepBlock) {
       const int32_t j = rows[t];
-      double d = ac[j];
-      for (int32_t e = lptr[j]; e < lptr[j + 1]; ++e) {
-        const int32_t i = lnb[e];
-        if (i == skip_row) continue;
-        d = d - (anb[lpji[e]] * ds[i]) * anb[lpij[e]];
-      }
+      const double d = dilu_factor_sum(lnb, lpji, lpij, anb, [](int32_t i) { return ds[i]; },
+                                       skip_row, lptr[j], lptr[j + 1], ac[j]);
       ds[j] = 1.0 / d;
     }
     __syncthreads();
@@ -428,12 +520,8 @@ __device__ __forceinline__ void dilu_factor_row(int32_t j, const double *__restr
                                                 const int32_t *__restrict__ lpji,
                                                 const int32_t *__restrict__ lpij,
                                                 int32_t skip_row, double *dinv) {
-  double d = ac[j];
-  for (int32_t e = lptr[j]; e < lptr[j + 1]; ++e) {
-    const int32_t i = lnb[e];
-    if (i == skip_row) continue;
-    d = d - (anb[lpji[e]] * dinv[i]) * anb[lpij[e]];
-  }
+  const double d = dilu_factor_sum(lnb, lpji, lpij, anb, [dinv](int32_t i) { return dinv[i]; },
+                                   skip_row, lptr[j], lptr[j + 1], ac[j]);
   dinv[j] = 1.0 / d;
 }
 
@@ -521,13 +609,8 @@ __device__ __forceinline__ void dilu_fwd_row(int32_t j, const V *__restrict__ an
                                              const int32_t *__restrict__ lnb,
                                              const int32_t *__restrict__ lpji,
                                              const V *__restrict__ dinv, V *rc) {
-  V v = rc[j];
-  for (int32_t e = lptr[j]; e < lptr[j + 1]; ++e) {
-    const int32_t i = lnb[e];
-    const V mat1 = dinv[i] * rc[i];
-    v = v - anb[lpji[e]] * mat1;
-  }
-  rc[j] = v;
+  rc[j] = dilu_lower_sum(lnb, lpji, anb, dinv, [rc](int32_t i) { return rc[i]; }, lptr[j],
+                         lptr[j + 1], rc[j]);
 }
 
 template <typename V>
@@ -582,12 +665,7 @@ __device__ __forceinline__ void dilu_bwd_row(int32_t i, int32_t n,
                                              const V *__restrict__ anb,
                                              const V *__restrict__ dinv, V *rc, V *x) {
   V v = rc[i];
-  if (i < n - 1) {
-    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
-      const int32_t j = col[k];
-      if (j > i) v = v - anb[k] * rc[j];
-    }
-  }
+  if (i < n - 1) v = dilu_upper_sum(col, anb, [rc](int32_t j) { return rc[j]; }, i, ptr[i], ptr[i + 1], v);
   const V z = dinv[i] * v;
   rc[i] = z;
   x[i] = x[i] + z;
