@@ -93,6 +93,9 @@ void ensure_schedule(ufvm_ctx *ctx, RefLevel &L) {
     for (size_t q = 0; q + 1 < S->level_ptr.size(); ++q)
       widest = std::max(widest, S->level_ptr[q + 1] - S->level_ptr[q]);
   L.one_wg = widest <= kOneWgWidest;
+  L.widest = widest;
+  L.max_row = 0;
+  for (int32_t i = 0; i < L.n; ++i) L.max_row = std::max(L.max_row, L.g.ptr[i + 1] - L.g.ptr[i]);
   L.low_ok = lower_lists(L.g, L.low);
   L.lptr.upload(L.low.ptr, ctx->stream);
   L.lnb.upload(L.low.nb, ctx->stream);
@@ -101,7 +104,20 @@ void ensure_schedule(ufvm_ctx *ctx, RefLevel &L) {
   L.sched = true;
 }
 
+// LDS a workgroup may allocate
+constexpr size_t kLdsBytes = 65536;
+
 void sor_sweep(ufvm_ctx *ctx, RefLevel &L, const double *stop = nullptr) {
+  // levels of at most kPipeBlock rows whose unknowns do not fit in LDS: the
+  // pipelined sweep (for those that fit, the LDS sweep below was as fast)
+  if (L.widest <= k::kPipeBlock && !(L.one_wg && L.n <= k::kLdsRows) &&
+      (size_t)(std::max(nlev(L.fwd), nlev(L.bwd)) + 1) * sizeof(int32_t) <= kLdsBytes) {
+    k::gs_sweep_pipe(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.ptr.p, L.col.p,
+                     L.ac.p, L.anb.p, L.bc.p, L.dphi.p, L.max_row, stop);
+    k::gs_sweep_pipe(ctx->stream, L.bwd_rows.p, L.bwd_lp.p, nlev(L.bwd), L.ptr.p, L.col.p,
+                     L.ac.p, L.anb.p, L.bc.p, L.dphi.p, L.max_row, stop);
+    return;
+  }
   if (L.one_wg && L.n <= k::kLdsRows) {
     k::sor_lds(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.bwd_rows.p, L.bwd_lp.p,
                nlev(L.bwd), L.n, L.ptr.p, L.col.p, L.ac.p, L.anb.p, L.bc.p, L.dphi.p, stop);

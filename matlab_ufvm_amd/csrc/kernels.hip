@@ -496,6 +496,150 @@ void dilu_factor_lds(hipStream_t s, const int32_t *rows, const int32_t *lp, int3
                      rows, lp, nlev, n, ac, anb, lptr, lnb, lpji, lpij, skip_row, dinv);
 }
 
+// ---- pipelined one-workgroup sweeps.  In a level-scheduled sweep only the
+// neighbour values depend on the previous levels; the row of a level, its
+// entry range, right-hand side, diagonal and entries do not.  When no level
+// holds more rows than the kPipeBlock threads, thread t owns row t of every
+// level, and while level q computes, the loads of the levels after it are in
+// flight: level q+3's row, level q+2's range / b / ac, level q+1's first
+// kPipeEntries columns and values.  A level then waits for one round trip
+// (its neighbour values) and the barrier instead of five.  Level pointers
+// are read from LDS.  Row arithmetic as gs_row (the same terms in the same
+// order; entries past the prefetched ones are summed after, in order).
+
+// the first kPipeEntries entries of row i (entries [k0, k1)); slots past the
+// row repeat its last entry; rows without entries (or no row) read slot 0
+template <int NE>
+struct PipeEntries {
+  int32_t c[NE];
+  double a[NE];
+  __device__ __forceinline__ void load(const int32_t *__restrict__ col,
+                                       const double *__restrict__ anb, int32_t i, int32_t k0,
+                                       int32_t k1) {
+    const int32_t m = k1 - k0 < NE ? k1 - k0 : NE;
+#pragma unroll
+    for (int u = 0; u < NE; ++u) {
+      if (i >= 0 && m > 0) {
+        const int32_t e = k0 + (u < m ? u : m - 1);
+        c[u] = col[e];
+        a[u] = anb[e];
+      } else {
+        c[u] = 0;
+        a[u] = 0.0;
+      }
+    }
+  }
+};
+
+// one level-scheduled Gauss-Seidel sweep over the levels lps[q] .. lps[q+1]
+// (LDS) of rows; xr(j) reads x_j, xw(i, v) writes x_i; NE entries of each row
+// are prefetched (8 when no row is longer, else 16)
+template <int NE, typename XR, typename XW>
+__device__ __forceinline__ void gs_sweep_pipe(const int32_t *__restrict__ rows, const int32_t *lps,
+                                              int32_t nlev, const int32_t *__restrict__ ptr,
+                                              const int32_t *__restrict__ col,
+                                              const double *__restrict__ ac,
+                                              const double *__restrict__ anb,
+                                              const double *__restrict__ b, const XR &xr,
+                                              const XW &xw) {
+  const int32_t tid = threadIdx.x;
+  auto row_at = [&](int32_t q) -> int32_t {
+    if (q >= nlev) return -1;
+    const int32_t t = lps[q] + tid;
+    return t < lps[q + 1] ? rows[t] : -1;
+  };
+  // stage A: level q+2's row; stage B: level q+1's row, range, b, ac;
+  // stage C: level q's row, range, b, ac and first entries
+  int32_t ia = row_at(2);
+  int32_t ib = row_at(1), kb0 = 0, kb1 = 0;
+  double bb = 0.0, db = 1.0;
+  if (ib >= 0) {
+    kb0 = ptr[ib];
+    kb1 = ptr[ib + 1];
+    bb = b[ib];
+    db = ac[ib];
+  }
+  int32_t ic = row_at(0), kc0 = 0, kc1 = 0;
+  double bc = 0.0, dc = 1.0;
+  if (ic >= 0) {
+    kc0 = ptr[ic];
+    kc1 = ptr[ic + 1];
+    bc = b[ic];
+    dc = ac[ic];
+  }
+  PipeEntries<NE> ec;
+  ec.load(col, anb, ic, kc0, kc1);
+  for (int32_t q = 0; q < nlev; ++q) {
+    // loads of the later levels (constant data)
+    const int32_t in = row_at(q + 3);
+    int32_t ka0 = 0, ka1 = 0;
+    double ba = 0.0, da = 1.0;
+    if (ia >= 0) {
+      ka0 = ptr[ia];
+      ka1 = ptr[ia + 1];
+      ba = b[ia];
+      da = ac[ia];
+    }
+    PipeEntries<NE> eb;
+    eb.load(col, anb, ib, kb0, kb1);
+    // level q: neighbour values, then the row's new value
+    if (ic >= 0) {
+      const int32_t len = kc1 - kc0, m = len < NE ? len : NE;
+      double xj[NE];
+#pragma unroll
+      for (int u = 0; u < NE; ++u) xj[u] = xr(ec.c[u]);
+      double v = bc;
+#pragma unroll
+      for (int u = 0; u < NE; ++u) {
+        const double t = v - ec.a[u] * xj[u];
+        v = u < m ? t : v;
+      }
+      if (len > NE) v = csr_row_sum(col, anb, xr, kc0 + NE, kc1, v);
+      xw(ic, v / dc);
+    }
+    __syncthreads();
+    ic = ib;
+    kc0 = kb0;
+    kc1 = kb1;
+    bc = bb;
+    dc = db;
+    ec = eb;
+    ib = ia;
+    kb0 = ka0;
+    kb1 = ka1;
+    bb = ba;
+    db = da;
+    ia = in;
+  }
+}
+
+template <int NE>
+__global__ void __launch_bounds__(kPipeBlock)
+k_gs_sweep_pipe(const int32_t *__restrict__ rows, const int32_t *__restrict__ lp, int32_t nlev,
+                const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+                const double *__restrict__ ac, const double *__restrict__ anb,
+                const double *__restrict__ b, double *x, const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;  // stopped by the device-side test
+  extern __shared__ int32_t lps[];
+  for (int32_t q = threadIdx.x; q <= nlev; q += kPipeBlock) lps[q] = lp[q];
+  __syncthreads();
+  gs_sweep_pipe<NE>(rows, lps, nlev, ptr, col, ac, anb, b, [x](int32_t j) { return x[j]; },
+                    [x](int32_t i, double v) { x[i] = v; });
+}
+
+void gs_sweep_pipe(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
+                   const int32_t *ptr, const int32_t *col, const double *ac, const double *anb,
+                   const double *b, double *x, int32_t max_row, const double *stop) {
+  if (nlev <= 0) return;
+  const size_t lds = (size_t)(nlev + 1) * sizeof(int32_t);
+  if (max_row <= 8)
+    hipLaunchKernelGGL(k_gs_sweep_pipe<8>, dim3(1), dim3(kPipeBlock), lds, s, rows, lp, nlev, ptr,
+                       col, ac, anb, b, x, stop);
+  else
+    hipLaunchKernelGGL(k_gs_sweep_pipe<16>, dim3(1), dim3(kPipeBlock), lds, s, rows, lp, nlev,
+                       ptr, col, ac, anb, b, x, stop);
+}
+
 void gs_sweep_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
                     const int32_t *ptr, const int32_t *col, const double *ac, const double *anb,
                     const double *b, double *x, const double *stop) {

@@ -117,6 +117,8 @@ struct RefLevel {
   // instead of one per level)
   DBuf<int32_t> fwd_lp, bwd_lp;
   bool one_wg = false;
+  int32_t widest = 0;             // rows of the widest level of both schedules
+  int32_t max_row = 0;            // the most off-diagonal entries of a row
   bool low_ok = false;
   LowerLists low;
   DBuf<int32_t> lptr, lnb, lpji, lpij;

@@ -103,6 +103,42 @@ def test_solve_algebraic_system_bitexact(ctx, case, smoother):
     ctx.system_destroy(sid)
 
 
+@pytest.mark.parametrize("grid", ["2d100", "3d48"])
+def test_pipelined_sweeps_beyond_lds(ctx, grid):
+    # Levels with more unknowns than LDS holds and no lexicographic level wider
+    # than 256 rows run their SOR sweeps in the pipelined one-workgroup
+    # kernel: the 100 x 100 lexicographic grid itself (10,000 rows, levels are
+    # the anti-diagonals, rows of at most 8 entries), and the first coarse
+    # level of a jittered, renumbered 48^3 hex mesh (16,949 rows, levels of at
+    # most 97 rows, rows of up to 23 entries: the 16-entry prefetch plus the
+    # rest of the row).  The sweeps keep the oracle's bits, alone and inside
+    # V-cycles.
+    if grid == "2d100":
+        m = cavity_mesh(100, nz=1, jitter=0.1, seed=64, renumber_seed=None)
+    else:
+        m = cavity_mesh(48, jitter=0.15, seed=67, renumber_seed=68)
+    ptr, col, ac, anb = poisson_system(m, dirichlet=("movingWall",))
+    b = gaussian_rhs(m, seed=65)
+    print(f"\n[pipelined sweeps] n={m.n_cells}", flush=True)
+    sid = make(ctx, ptr, col, ac, anb, b)
+    x0 = np.random.default_rng(66).standard_normal(m.n_cells) * 1e-3
+    ctx.set_dphi(sid, 0, x0)
+    ctx.solve_algebraic_system(sid, 0, "SOR", 3, rrf=0.0)
+    S = oracle.System(ptr, col, ac, anb, bc=b, dphi=x0)
+    S.solve_algebraic_system(0, "SOR", 3, rrf=0.0)
+    assert np.array_equal(ctx.get_dphi(sid, 0), S.level(0)["dphi"])
+    ctx.set_dphi(sid, 0, np.zeros(m.n_cells))
+    r0, r1, nc = ctx.apply_amg(sid, "V-Cycle", max_cycles=4, termination=1e-12,
+                               max_coarse_levels=10, pre_sweep=2, post_sweep=2, smoother="SOR",
+                               rebuild=1)
+    S2 = oracle.System(ptr, col, ac, anb, bc=b)
+    q0, q1, qc = S2.apply_amg("V-Cycle", max_cycles=4, termination=1e-12, max_coarse_levels=10,
+                              pre_sweep=2, post_sweep=2, smoother="SOR")
+    assert nc == qc
+    assert np.array_equal(ctx.get_dphi(sid, 0), S2.level(0)["dphi"])
+    ctx.system_destroy(sid)
+
+
 @pytest.mark.parametrize("smoother", ["SOR", "ILU"])
 def test_lds_sweeps_near_their_size_limit(ctx, smoother):
     # 76 x 76 = 5,776 cells: every lexicographic level is narrow (one
