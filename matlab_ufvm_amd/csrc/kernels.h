@@ -343,6 +343,7 @@ enum GOp : int32_t {
 template <typename V>
 struct GArgs {
   int32_t n, r0, aux, nb;
+  int32_t wide;                    // G_RESID / G_JACOBI: rows loaded in one round
   const int32_t *off, *child, *parent;
   SellCols cols;
   const V *val, *ac, *b, *x, *xc;

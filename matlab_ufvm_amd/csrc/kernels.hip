@@ -2355,25 +2355,29 @@ __global__ void __launch_bounds__(kBlock) k_group_row(const GArgs<V> *__restrict
     const int32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.n) return;
     if (i < a.aux) a.out[i] = V(0);
+    else if (a.cols.d16 && a.wide)
+      residual_row<V, true, true>(i, a.off, a.cols, a.val, a.ac, a.b, a.x, a.out);
     else if (a.cols.d16) residual_row<V, true>(i, a.off, a.cols, a.val, a.ac, a.b, a.x, a.out);
+    else if (a.wide) residual_row<V, false, true>(i, a.off, a.cols, a.val, a.ac, a.b, a.x, a.out);
     else residual_row<V, false>(i, a.off, a.cols, a.val, a.ac, a.b, a.x, a.out);
   } else if (OP == G_JACOBI) {
     const int32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= a.n) return;
+    // the member's rows, PRO (prolongation fused) and column form, wide or in runs
+    auto row = [&](auto pro, auto pk, auto wide) {
+      constexpr bool PRO = decltype(pro)::value, PK = decltype(pk)::value;
+      jacobi_row<V, PRO, PK, decltype(wide)::value>(i, a.off, a.cols, a.val, a.ac, a.b, a.x,
+                                                    PRO ? a.parent : nullptr,
+                                                    PRO ? a.xc : nullptr, a.out, a.omega);
+    };
+    using T = std::true_type;
+    using F = std::false_type;
     if (a.parent) {
-      if (a.cols.d16)
-        jacobi_row<V, true, true>(i, a.off, a.cols, a.val, a.ac, a.b, a.x, a.parent, a.xc, a.out,
-                                  a.omega);
-      else
-        jacobi_row<V, true, false>(i, a.off, a.cols, a.val, a.ac, a.b, a.x, a.parent, a.xc,
-                                   a.out, a.omega);
+      if (a.cols.d16) a.wide ? row(T{}, T{}, T{}) : row(T{}, T{}, F{});
+      else a.wide ? row(T{}, F{}, T{}) : row(T{}, F{}, F{});
     } else {
-      if (a.cols.d16)
-        jacobi_row<V, false, true>(i, a.off, a.cols, a.val, a.ac, a.b, a.x, nullptr, nullptr,
-                                   a.out, a.omega);
-      else
-        jacobi_row<V, false, false>(i, a.off, a.cols, a.val, a.ac, a.b, a.x, nullptr, nullptr,
-                                    a.out, a.omega);
+      if (a.cols.d16) a.wide ? row(F{}, T{}, T{}) : row(F{}, T{}, F{});
+      else a.wide ? row(F{}, F{}, T{}) : row(F{}, F{}, F{});
     }
   } else if (OP == G_RESTRICT) {
     const int32_t p = blockIdx.x * kBlock + threadIdx.x;

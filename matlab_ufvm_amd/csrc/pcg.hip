@@ -1307,6 +1307,7 @@ struct GroupRec {
     r.b = P::b(LW);
     r.x = x;
     r.out = P::r(LW);
+    r.wide = L.wide;
     add({0, l, 2, 0, 0}, k::G_RESID, r);
     A t = blank();
     t.n = C.n;
@@ -1354,6 +1355,7 @@ struct GroupRec {
       a.xc = xc;
       a.out = outp;
       a.omega = omega;
+      a.wide = L.wide;
       add(key, k::G_JACOBI, a);
     };
     auto prolong = [&](int32_t r0, const V *xc, V *xp) {
