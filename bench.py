@@ -150,6 +150,7 @@ def trace_summary(d, regex):
     import collections
     pat = re.compile(regex)
     dur = collections.defaultdict(list)
+    spans = []
     for f in glob.glob(os.path.join(d, "**", "*kernel_trace.csv"), recursive=True):
         with open(f, newline="") as fh:
             for raw in csv.DictReader(fh):
@@ -158,8 +159,25 @@ def trace_summary(d, regex):
                 if not pat.search(name):
                     continue
                 grid = int(row.get("grid_size_x", row.get("grid_size", 0)) or 0)
-                dur[(name, grid)].append((int(row["end_timestamp"]) -
-                                          int(row["start_timestamp"])) * 1e-3)
+                t0, t1 = int(row["start_timestamp"]), int(row["end_timestamp"])
+                dur[(name, grid)].append((t1 - t0) * 1e-3)
+                spans.append((t0, t1))
+    # overlap: the sum of the matching kernels' durations against the length of
+    # the union of their intervals (> 1: kernels ran concurrently)
+    if spans:
+        spans.sort()
+        union, (cur0, cur1) = 0, spans[0]
+        for a, b in spans[1:]:
+            if a > cur1:
+                union += cur1 - cur0
+                cur0, cur1 = a, b
+            else:
+                cur1 = max(cur1, b)
+        union += cur1 - cur0
+        total = sum(b - a for a, b in spans)
+        print(f"# {len(spans)} dispatches: kernel time {total * 1e-6:.2f} ms, busy (union of "
+              f"intervals) {union * 1e-6:.2f} ms, mean concurrency {total / max(union, 1):.2f}, "
+              f"span {(spans[-1][1] - spans[0][0]) * 1e-6:.2f} ms")
     print(f"{'kernel':60s} {'grid':>9s} {'dispatches':>10s} {'avg us':>9s} {'min us':>9s}"
           f" {'total ms':>9s}")
     rows = sorted(dur.items(), key=lambda kv: -sum(kv[1]))
