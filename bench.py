@@ -810,7 +810,7 @@ def main():
         return
 
     results = {}
-    for prec in ([32] if args.no_fp64 else [32, 64]):
+    for prec in ([32] if args.no_fp64 else [32, 64, 16]):
         t = time.time()
         ctx.pcg_setup(sid, precision=prec, pre_sweep=args.pre, post_sweep=args.post,
                       max_coarse_cells=args.max_coarse, tol=args.tol, max_iter=500,
@@ -958,6 +958,16 @@ def main():
                            "p99_ms": round(float(np.percentile(r64["lat"], 99)) * 1e3, 3),
                            "field_rel_l2_err": ferr(r64["x"])}
         out["speedup_fp32_vs_fp64_pcg"] = round(value / v64, 3)
+        if 16 in results:
+            # the finest level's off-diagonals in FP16 (divided by their row's
+            # diagonal), arithmetic in FP32: the byte lever on the ratio
+            r16 = results[16]
+            v16 = args.steps / r16["total"]
+            out["fp16_storage_pcg"] = {
+                "solves_per_s": round(v16, 3), "iterations": int(r16["its"][0]),
+                "p50_ms": round(float(np.median(r16["lat"])) * 1e3, 3),
+                "field_rel_l2_err": ferr(r16["x"]), "speedup_vs_fp64_pcg": round(v16 / v64, 3),
+                "form": "FP32 V-cycle; finest-level off-diagonals stored as FP16 a_ij / a_ii"}
         # the target's measure: FP32-preconditioned field vs the FP64 run at the
         # same residual tolerance (shift-invariant, p'(0) = 0 as above)
         d32, d64 = shift(r32["x"]), shift(r64["x"])

@@ -164,7 +164,10 @@ int ufvm_apply_amg(ufvm_ctx *ctx, int32_t sys, const ufvm_amg_cfg *cfg, double *
  * GS below it, and an FP64 dense solve on the first level with at most
  * max_coarse_cells cells. */
 typedef struct {
-  int32_t precision;         /* 32: FP32 V-cycle, 64: FP64 V-cycle */
+  int32_t precision;         /* 32: FP32 V-cycle, 64: FP64 V-cycle, 16: FP32 V-cycle whose
+                                finest-level off-diagonals are stored in FP16, each
+                                divided by its row's diagonal (multicolour GS finest
+                                smoother only; no batched or grouped solves) */
   int32_t pre_sweep;         /* smoothing sweeps before restriction */
   int32_t post_sweep;        /* smoothing sweeps after prolongation */
   int32_t coarse_smoother;   /* levels below the first: 0 block-local multicolour GS,

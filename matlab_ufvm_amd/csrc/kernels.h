@@ -202,6 +202,18 @@ template <typename V>
 void jacobi_sell(hipStream_t s, int32_t n, const int32_t *off, const SellCols &cols,
                  const V *val, const V *ac, const V *b, const V *x, const int32_t *parent, const V *xc, V *out,
                  V omega, bool wide = false);
+// The finest level with FP16 off-diagonals scaled by their row's diagonal
+// (s_ij = a_ij / a_ii; FP32 arithmetic): colour pass x_i = b_i / a_ii -
+// sum s_ij x_j, residual r_i = b_i + a_ii (-x_i - sum s_ij x_j); scale_half
+// fills the FP16 values from the FP64 copy (row by row over the chunks)
+void mcgs_sell_h(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const SellCols &cols,
+                 const _Float16 *val, const float *ac, const float *b, float *x,
+                 int32_t zero_above);
+void residual_sell_h(hipStream_t s, int32_t n, int32_t zero_rows, const int32_t *off,
+                     const SellCols &cols, const _Float16 *val, const float *ac, const float *b,
+                     const float *x, float *r);
+void scale_half(hipStream_t s, int32_t n, const int32_t *off, const double *val, const double *ac,
+                _Float16 *out16);
 // block-local multicolour GS over blocks of at most kBRows rows (see kernels.hip)
 constexpr int kBRows = 1024;
 template <typename V>

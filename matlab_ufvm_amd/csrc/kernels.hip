@@ -1124,6 +1124,130 @@ void mcgs_sell(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const 
   });
 }
 
+// ---- finest level with FP16 off-diagonals scaled by their row's diagonal
+// (s_ij = a_ij / a_ii, |s_ij| <= 1 on diagonally dominant rows), arithmetic
+// in FP32: a Gauss-Seidel row is x_i = b_i / a_ii - sum_j s_ij x_j, a
+// residual r_i = b_i + a_ii (-x_i - sum_j s_ij x_j).  Rows in runs of
+// kSellRun as sell_row (same padding: value 0).
+
+template <int N, bool ZA, bool NT, typename CS>
+__device__ __forceinline__ float hsell_run(const CS &col, const _Float16 *__restrict__ val,
+                                           const float *x, int32_t e0, float v,
+                                           int32_t zero_above) {
+  int32_t j[N];
+  float a[N], xj[N];
+#pragma unroll
+  for (int u = 0; u < N; ++u) {
+    j[u] = NT ? col.nt(e0 + u * 64) : col[e0 + u * 64];
+    a[u] = (float)(NT ? __builtin_nontemporal_load(val + e0 + u * 64) : val[e0 + u * 64]);
+  }
+#pragma unroll
+  for (int u = 0; u < N; ++u) xj[u] = x[j[u]];
+  if (ZA) {
+#pragma unroll
+    for (int u = 0; u < N; ++u) asm volatile("" : "+v"(xj[u]));
+#pragma unroll
+    for (int u = 0; u < N; ++u) xj[u] = j[u] < zero_above ? xj[u] : 0.0f;
+  }
+#pragma unroll
+  for (int u = 0; u < N; ++u) v = v - a[u] * xj[u];
+  return v;
+}
+
+// v - sum over the row's w entries from slot base of s x
+template <bool ZA, bool NT, typename CS>
+__device__ __forceinline__ float hsell_row(const CS &col, const _Float16 *__restrict__ val,
+                                           const float *x, int32_t base, int32_t w, float v,
+                                           int32_t zero_above) {
+  int32_t k = 0;
+  for (; k + kSellRun <= w; k += kSellRun)
+    v = hsell_run<kSellRun, ZA, NT>(col, val, x, base + k * 64, v, zero_above);
+  const int32_t e0 = base + k * 64;
+  switch (w - k) {
+    case 1: return hsell_run<1, ZA, NT>(col, val, x, e0, v, zero_above);
+    case 2: return hsell_run<2, ZA, NT>(col, val, x, e0, v, zero_above);
+    case 3: return hsell_run<3, ZA, NT>(col, val, x, e0, v, zero_above);
+    case 4: return hsell_run<4, ZA, NT>(col, val, x, e0, v, zero_above);
+    case 5: return hsell_run<5, ZA, NT>(col, val, x, e0, v, zero_above);
+    case 6: return hsell_run<6, ZA, NT>(col, val, x, e0, v, zero_above);
+    case 7: return hsell_run<7, ZA, NT>(col, val, x, e0, v, zero_above);
+    default: return v;
+  }
+}
+
+template <bool PK>
+__global__ void __launch_bounds__(kBlock)
+k_mcgs_sell_h(int32_t r0, int32_t r1, const int32_t *__restrict__ off, SellCols cols,
+              const _Float16 *__restrict__ val, const float *__restrict__ ac,
+              const float *__restrict__ b, float *x, int32_t zero_above) {
+  const int32_t i = (r0 & ~63) + xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+  if (i < r0 || i >= r1) return;
+  const int32_t q = i >> 6;
+  const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
+  const ColSrc<PK> col(cols, q);
+  const float v0 = b[i] / ac[i];
+  x[i] = (zero_above <= r1) ? hsell_row<true, false>(col, val, x, base, w, v0, zero_above)
+                            : hsell_row<false, false>(col, val, x, base, w, v0, zero_above);
+}
+
+void mcgs_sell_h(hipStream_t s, int32_t r0, int32_t r1, const int32_t *off, const SellCols &cols,
+                 const _Float16 *val, const float *ac, const float *b, float *x,
+                 int32_t zero_above) {
+  if (r1 <= r0) return;
+  const int32_t a = r0 & ~63;
+  with_cols(cols, [&](auto pk) {
+    hipLaunchKernelGGL((k_mcgs_sell_h<decltype(pk)::value>), dim3(nblocks(r1 - a)), dim3(kBlock),
+                       0, s, r0, r1, off, cols, val, ac, b, x, zero_above);
+  });
+}
+
+template <bool PK>
+__global__ void __launch_bounds__(kBlock)
+k_residual_sell_h(int32_t n, int32_t zero_rows, const int32_t *__restrict__ off, SellCols cols,
+                  const _Float16 *__restrict__ val, const float *__restrict__ ac,
+                  const float *__restrict__ b, const float *__restrict__ x,
+                  float *__restrict__ r) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  if (i < zero_rows) {
+    r[i] = 0.0f;
+    return;
+  }
+  const int32_t q = i >> 6;
+  const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
+  const ColSrc<PK> col(cols, q);
+  r[i] = b[i] + ac[i] * hsell_row<false, true>(col, val, x, base, w, -x[i], 0);
+}
+
+void residual_sell_h(hipStream_t s, int32_t n, int32_t zero_rows, const int32_t *off,
+                     const SellCols &cols, const _Float16 *val, const float *ac, const float *b,
+                     const float *x, float *r) {
+  if (n <= 0) return;
+  with_cols(cols, [&](auto pk) {
+    hipLaunchKernelGGL((k_residual_sell_h<decltype(pk)::value>), dim3(nblocks(n)), dim3(kBlock),
+                       0, s, n, zero_rows, off, cols, val, ac, b, x, r);
+  });
+}
+
+// out16[slot] = val[slot] / ac[row] rounded to FP16, over every slot of the
+// n rows' sliced-ELL chunks (padding slots hold 0)
+__global__ void __launch_bounds__(kBlock)
+k_scale_half(int32_t n, const int32_t *__restrict__ off, const double *__restrict__ val,
+             const double *__restrict__ ac, _Float16 *__restrict__ out16) {
+  const int32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int32_t q = i >> 6;
+  const int32_t base = off[q] + (i & 63), w = (off[q + 1] - off[q]) >> 6;
+  const double d = ac[i];
+  for (int32_t k = 0; k < w; ++k) out16[base + k * 64] = (_Float16)(val[base + k * 64] / d);
+}
+
+void scale_half(hipStream_t s, int32_t n, const int32_t *off, const double *val, const double *ac,
+                _Float16 *out16) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_scale_half, dim3(nblocks(n)), dim3(kBlock), 0, s, n, off, val, ac, out16);
+}
+
 // ---- DILU in colour order on the sliced-ELL copy of a level whose colour
 // classes are contiguous row ranges (earlier colours = lower rows).  Forward
 // pass over the colour rows [r0, r1):

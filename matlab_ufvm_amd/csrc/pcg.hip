@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <type_traits>
 #include <cstring>
 #include <map>
 
@@ -188,6 +189,7 @@ void build_fast_level(ufvm_ctx *ctx, System &S, int l, int kind) {
     FL.sell_src.upload(ssrc, s);
     FL.sval64.alloc(tot);
     FL.sval32.alloc(tot);
+    if (kind == 0 && l == 0 && S.pcg.cfg.precision == 16) FL.sval16.alloc(tot);
     if (kind == 6) {
       FL.fsrc.upload(fsrc, s);
       FL.sf64.alloc(tot);
@@ -215,6 +217,7 @@ void refresh_level(ufvm_ctx *ctx, System &S, int l) {
     const int64_t slots = FL.h_soff.back();
     k::gather_vals<double>(s, slots, FL.sell_src.p, R.anb.p, FL.sval64.p);
     k::gather_vals<float>(s, slots, FL.sell_src.p, R.anb.p, FL.sval32.p);
+    if (FL.sval16.p) k::scale_half(s, n, FL.soff.p, FL.sval64.p, FL.ac64.p, FL.sval16.p);
   }
   if (FL.kind == 4) {
     // the reference's DILU factor (cfdFactorizeILU.m:10-47, row N-1 quirk
@@ -330,6 +333,13 @@ struct Prec<double> {
 constexpr int32_t kFusedProlongRows = 100000;
 constexpr int32_t kTailRows = 40000;   // profiling: levels from here down are "tail"
 
+// the FP16 row-scaled finest-level values of an FP32 V-cycle (precision 16),
+// or null
+template <typename V>
+const _Float16 *half_of(const FastLevel &L) {
+  return std::is_same<V, float>::value ? L.sval16.p : nullptr;
+}
+
 // One preconditioner V-cycle of system S on workspace W and stream s.
 template <typename V>
 struct Cycle {
@@ -353,11 +363,19 @@ struct Cycle {
     ProfScope ps(ctx, s, "mcgs0", 0.0);
     auto pass = [&](int c, int32_t zero_above) {
       const int32_t r0 = L.col_off[c], r1 = L.col_off[c + 1];
+      const bool h = half_of<V>(L) != nullptr;
       if (ctx->prof_on) {
         // algorithmic bytes: the colour's entries (col + value), its ac, b,
         // x write, and one read of every other unknown
         const double R = r1 - r0, Z = L.h_ptr[r1] - L.h_ptr[r0];
-        ps.add((index_bytes(L) + w) * Z + 3.0 * w * R + w * (L.n - R));
+        ps.add((index_bytes(L) + (h ? 2.0 : w)) * Z + 3.0 * w * R + w * (L.n - R));
+      }
+      if constexpr (std::is_same<V, float>::value) {
+        if (h) {
+          k::mcgs_sell_h(s, r0, r1, L.soff.p, sell_cols(L), L.sval16.p, P::ac(L), P::b(LW),
+                         P::x(LW), zero_above);
+          return;
+        }
       }
       k::mcgs_sell<V>(s, r0, r1, L.soff.p, sell_cols(L), P::sval(L), P::ac(L), P::b(LW), P::x(LW),
                       zero_above);
@@ -424,6 +442,19 @@ struct Cycle {
     return (l == 0 || c.coarse_post_sweep < 0) ? c.post_sweep : c.coarse_post_sweep;
   }
 
+  // precision 16's finest level: its residual from the FP16 values (false on
+  // every other level)
+  bool residual_half(const FastLevel &L, LevelWork &LW, int32_t zero_rows, const V *x) {
+    if constexpr (std::is_same<V, float>::value) {
+      if (L.sval16.p) {
+        k::residual_sell_h(s, L.n, zero_rows, L.soff.p, sell_cols(L), L.sval16.p, P::ac(L),
+                           P::b(LW), x, P::r(LW));
+        return true;
+      }
+    }
+    return false;
+  }
+
   // residual of level l into r, then restriction to level l+1 (fused with the
   // first Jacobi sweep there when that level is Jacobi-smoothed)
   void down(const FastLevel &L, LevelWork &LW, int l, const V *x, int32_t zero_rows) {
@@ -436,7 +467,7 @@ struct Cycle {
     const bool jac = C.kind == 3 && pre_of(l + 1) > 0;
     if (L.kind == 1)
       k::residual_fast<V>(s, L.n, L.ptr.p, L.col.p, P::ac(L), P::anb(L), P::b(LW), x, P::r(LW));
-    else
+    else if (!residual_half(L, LW, zero_rows, x))
       k::residual_sell<V>(s, L.n, zero_rows, L.soff.p, sell_cols(L), P::sval(L), P::ac(L), P::b(LW),
                           x, P::r(LW), L.wide);
     if (jac)
@@ -545,7 +576,7 @@ constexpr int kRingPer = 6;
 
 size_t work_bytes(const System &S, int k) {
   const PcgState &P = S.pcg;
-  const size_t w = P.cfg.precision == 32 ? 4 : 8;
+  const size_t w = P.cfg.precision != 64 ? 4 : 8;
   size_t bytes = 0;
   for (int32_t l = 0; l < P.nlev; ++l) bytes += 4 * w * k * (size_t)batch_ld(P.lev[l]->n, k);
   const size_t n0 = S.lev[0]->n;
@@ -581,7 +612,7 @@ std::unique_ptr<SolveWork> make_work(const System &S, hipStream_t s, int k) {
   W->n0 = n0;
   W->nrhs = k;
   W->ld0 = batch_ld(n0, k);
-  if (S.pcg.cfg.precision == 32) alloc_levels<float>(S, *W, s);
+  if (S.pcg.cfg.precision != 64) alloc_levels<float>(S, *W, s);
   else alloc_levels<double>(S, *W, s);
   for (DBuf<double> *v : {&W->x, &W->r, &W->p, &W->w}) {
     v->alloc((size_t)W->ld0 * k);
@@ -630,7 +661,10 @@ static void build_coarsest(ufvm_ctx *ctx, System &S) {
 
 void pcg_setup(ufvm_ctx *ctx, System &S, const ufvm_pcg_cfg &cfg) {
   if (!S.hier_valid) throw Error(UFVM_ERR_STATE, "no AMG hierarchy: call ufvm_amg_agglomerate");
-  if (cfg.precision != 32 && cfg.precision != 64) throw Error(UFVM_ERR_ARG, "precision must be 32 or 64");
+  if (cfg.precision != 16 && cfg.precision != 32 && cfg.precision != 64)
+    throw Error(UFVM_ERR_ARG, "precision must be 16, 32 or 64");
+  if (cfg.precision == 16 && cfg.smoother != 0)
+    throw Error(UFVM_ERR_ARG, "precision 16 needs the multicolour Gauss-Seidel finest smoother");
   if (cfg.max_coarse_cells < 1 || cfg.max_coarse_cells > 4096)
     throw Error(UFVM_ERR_ARG, "max_coarse_cells must be in [1, 4096]");
   if (cfg.coarse_smoother != 0 && cfg.coarse_smoother != 1)
@@ -761,7 +795,7 @@ void FcgRun::start(const double *b, double *x) {
   ahead_ = true;
   prev2_ = -1.0;
   nb_ = thr_ = rr_ = 0.0;
-  if (S_->pcg.cfg.precision == 32) start_t<float>(b);
+  if (S_->pcg.cfg.precision != 64) start_t<float>(b);
   else start_t<double>(b);
   phase_ = 1;
 }
@@ -801,7 +835,7 @@ void FcgRun::issue_t(int k) {
 }
 
 void FcgRun::issue(int k) {
-  const bool f32 = S_->pcg.cfg.precision == 32;
+  const bool f32 = S_->pcg.cfg.precision != 64;
   if (!graphs_) {
     if (f32) issue_t<float>(k);
     else issue_t<double>(k);
@@ -975,7 +1009,7 @@ struct CycleB {
 
 bool batch_supported(const System &S) {
   const PcgState &P = S.pcg;
-  if (!P.ready) return false;
+  if (!P.ready || P.cfg.precision == 16) return false;
   if (P.nlev > 1 && P.lev[0]->kind != 0) return false;
   for (int32_t l = 1; l + 1 < P.nlev; ++l)
     if (P.lev[l]->kind != 3) return false;
@@ -986,6 +1020,8 @@ FcgBatch::FcgBatch(ufvm_ctx *ctx, System &S, SolveWork &W, hipStream_t s)
     : ctx_(ctx), S_(&S), W_(&W), s_(s), k_(W.nrhs), nb_(k_), thr_(k_), rr_(k_), its_(k_),
       done_(k_) {
   const PcgState &P = S.pcg;
+  if (P.cfg.precision == 16)
+    throw Error(UFVM_ERR_UNSUPPORTED, "batched solves need precision 32 or 64");
   for (int32_t l = 1; l + 1 < P.nlev; ++l)
     if (P.lev[l]->kind != 3)
       throw Error(UFVM_ERR_UNSUPPORTED, "batched solves need coarse_smoother 1 (Jacobi)");
@@ -1039,7 +1075,7 @@ void FcgBatch::start(const double *B, double *X) {
   it_ = issued_ = 0;
   if (B != W_->bref.p)
     UFVM_HIP(hipMemcpyAsync(W_->bref.p, B, sizeof(double) * W_->n0 * k_, hipMemcpyDefault, s_));
-  if (P.cfg.precision == 32) start_t<float>();
+  if (P.cfg.precision != 64) start_t<float>();
   else start_t<double>();
   phase_ = 1;
 }
@@ -1116,7 +1152,7 @@ bool FcgBatch::poll(bool block) {
     }
     // phase 2: one iteration queued beyond the awaited one
     while (issued_ < cfg.max_iter && issued_ < it_ + 2) {
-      if (S_->pcg.cfg.precision == 32) issue_t<float>(issued_);
+      if (S_->pcg.cfg.precision != 64) issue_t<float>(issued_);
       else issue_t<double>(issued_);
       ++issued_;
     }
@@ -1196,7 +1232,7 @@ void pcg_apply_vcycle(ufvm_ctx *ctx, System &S, const double *r, double *z) {
   hipStream_t s = ctx->stream;
   const int32_t n = W.n0;
   UFVM_HIP(hipMemcpyAsync(W.bref.p, r, n * sizeof(double), hipMemcpyHostToDevice, s));
-  if (S.pcg.cfg.precision == 32) apply_vcycle_t<float>(ctx, S, W);
+  if (S.pcg.cfg.precision != 64) apply_vcycle_t<float>(ctx, S, W);
   else apply_vcycle_t<double>(ctx, S, W);
   UFVM_HIP(hipMemcpyAsync(z, W.xref.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
   UFVM_HIP(hipStreamSynchronize(s));

@@ -176,6 +176,9 @@ struct FastLevel {
   DBuf<int32_t> sell_src;         // reference CSR position of each slot (-1: padding)
   DBuf<float> sval32;
   DBuf<double> sval64;
+  // precision 16, finest level: the off-diagonals divided by their row's
+  // diagonal, FP16 (the colour passes and residual of k::mcgs_sell_h)
+  DBuf<_Float16> sval16;
   // kinds 4, 5: DILU diagonal inverse (factored in FP64 at set-up / refresh)
   DBuf<float> dinv32;
   DBuf<double> dinv64;

@@ -264,6 +264,35 @@ def test_pcg_solve_matches_direct(ctx, case, precision, tol, err, cs, sm):
     ctx.system_destroy(sid)
 
 
+def test_pcg_fp16_storage(ctx, case):
+    # precision 16: the FP32 V-cycle with the finest level's off-diagonals
+    # stored in FP16, divided by their row's diagonal.  The outer FP64 FCG
+    # still reaches the tolerance and the direct solution, in about the FP32
+    # form's iterations; batched solves and the other finest smoothers are
+    # refused.
+    from matlab_ufvm_amd._lib import UfvmError
+    name, m, ptr, col, ac, anb, b = case
+    sid = make(ctx, ptr, col, ac, anb, b)
+    ctx.agglomerate(sid, 10)
+    A = to_scipy(ptr, col, ac, anb)
+    xs = spl.spsolve(A.tocsc(), b)
+    its = {}
+    for prec in (32, 16):
+        ctx.pcg_setup(sid, precision=prec, max_coarse_cells=200, tol=1e-10, max_iter=300)
+        it, rr = ctx.pcg_solve(sid)
+        x = ctx.get_dphi(sid, 0)
+        assert rr <= 1e-10, (prec, it, rr)
+        assert np.linalg.norm(x - xs) <= 1e-8 * np.linalg.norm(xs), (prec, it, rr)
+        its[prec] = it
+    print(f"\n[fp16 storage] {name}: iterations FP32 {its[32]}, FP16 storage {its[16]}", flush=True)
+    assert its[16] <= its[32] + 3
+    with pytest.raises(UfvmError):
+        ctx.pcg_solve_batch(sid, np.stack([b, b]))
+    with pytest.raises(UfvmError):
+        ctx.pcg_setup(sid, precision=16, max_coarse_cells=200, smoother="MCILU")
+    ctx.system_destroy(sid)
+
+
 def triangulated_grid(k, seed):
     """k x k grid graph with one diagonal per square (every square two
     triangles): symmetric negative off-diagonals, rows numbered by a seeded
