@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <tuple>
 
 #include "internal.h"
 #include "kernels.h"
@@ -96,6 +97,25 @@ void ensure_schedule(ufvm_ctx *ctx, RefLevel &L) {
   L.widest = widest;
   L.max_row = 0;
   for (int32_t i = 0; i < L.n; ++i) L.max_row = std::max(L.max_row, L.g.ptr[i + 1] - L.g.ptr[i]);
+  if (widest <= k::kPipeBlock) {
+    const int32_t n = L.n, w = std::max(L.max_row, 1);
+    L.pipe_w = w;
+    std::vector<int32_t> ecol((size_t)n * w), elen(n);
+    for (auto [S, C, N] : {std::tuple{&L.fwd, &L.fwd_ecol, &L.fwd_elen},
+                           std::tuple{&L.bwd, &L.bwd_ecol, &L.bwd_elen}}) {
+      for (int32_t s = 0; s < n; ++s) {
+        const int32_t i = S->rows[s], k0 = L.g.ptr[i], len = L.g.ptr[i + 1] - k0;
+        elen[s] = len;
+        for (int32_t u = 0; u < w; ++u) ecol[(size_t)u * n + s] = u < len ? L.g.col[k0 + u] : i;
+      }
+      C->upload(ecol, ctx->stream);
+      N->upload(elen, ctx->stream);
+    }
+    L.fwd_eanb.alloc((size_t)n * w);
+    L.bwd_eanb.alloc((size_t)n * w);
+    L.fwd_eac.alloc(n);
+    L.bwd_eac.alloc(n);
+  }
   L.low_ok = lower_lists(L.g, L.low);
   L.lptr.upload(L.low.ptr, ctx->stream);
   L.lnb.upload(L.low.nb, ctx->stream);
@@ -111,11 +131,18 @@ void sor_sweep(ufvm_ctx *ctx, RefLevel &L, const double *stop = nullptr) {
   // levels of at most kPipeBlock rows whose unknowns do not fit in LDS: the
   // pipelined sweep (for those that fit, the LDS sweep below was as fast)
   if (L.widest <= k::kPipeBlock && !(L.one_wg && L.n <= k::kLdsRows) &&
-      (size_t)(std::max(nlev(L.fwd), nlev(L.bwd)) + 1) * sizeof(int32_t) <= kLdsBytes) {
-    k::gs_sweep_pipe(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.ptr.p, L.col.p,
-                     L.ac.p, L.anb.p, L.bc.p, L.dphi.p, L.max_row, stop);
-    k::gs_sweep_pipe(ctx->stream, L.bwd_rows.p, L.bwd_lp.p, nlev(L.bwd), L.ptr.p, L.col.p,
-                     L.ac.p, L.anb.p, L.bc.p, L.dphi.p, L.max_row, stop);
+      k::gs_sweep_pipe_lds(std::max(nlev(L.fwd), nlev(L.bwd))) <= kLdsBytes) {
+    // the ELL values from the current coefficients (a few microseconds
+    // against milliseconds of sweeps), then the two sweeps
+    const int32_t n = L.n, w = L.pipe_w;
+    k::pipe_values(ctx->stream, L.fwd_rows.p, n, w, L.ptr.p, L.anb.p, L.ac.p, L.fwd_eanb.p,
+                   L.fwd_eac.p);
+    k::pipe_values(ctx->stream, L.bwd_rows.p, n, w, L.ptr.p, L.anb.p, L.ac.p, L.bwd_eanb.p,
+                   L.bwd_eac.p);
+    k::gs_sweep_pipe(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), n, w, L.fwd_ecol.p,
+                     L.fwd_elen.p, L.fwd_eanb.p, L.fwd_eac.p, L.bc.p, L.dphi.p, stop);
+    k::gs_sweep_pipe(ctx->stream, L.bwd_rows.p, L.bwd_lp.p, nlev(L.bwd), n, w, L.bwd_ecol.p,
+                     L.bwd_elen.p, L.bwd_eanb.p, L.bwd_eac.p, L.bc.p, L.dphi.p, stop);
     return;
   }
   if (L.one_wg && L.n <= k::kLdsRows) {

@@ -100,13 +100,18 @@ void gs_sweep_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32
                     const int32_t *ptr, const int32_t *col, const double *ac, const double *anb,
                     const double *b, double *x, const double *stop = nullptr);
 // The same sweep pipelined (see kernels.hip) for schedules whose levels hold
-// at most kPipeBlock rows, with the level pointers in LDS ((nlev + 1) ints);
-// max_row: the most entries of a row (8 or fewer: 8 entries prefetched per
-// row, else 16).  Same arithmetic.
+// at most kPipeBlock rows, over a schedule-ordered ELL copy of the level:
+// slot u of position s at u * n + s (w slots; ecol static, past a row's
+// length the row itself; elen the row lengths), eanb / eac refreshed from
+// the CSR values by pipe_values before the sweeps; the level pointers in LDS
+// (gs_sweep_pipe_lds bytes).  Same arithmetic.
 constexpr int kPipeBlock = 256;
-void gs_sweep_pipe(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
-                   const int32_t *ptr, const int32_t *col, const double *ac, const double *anb,
-                   const double *b, double *x, int32_t max_row, const double *stop = nullptr);
+size_t gs_sweep_pipe_lds(int32_t nlev);
+void pipe_values(hipStream_t s, const int32_t *rows, int32_t n, int32_t w, const int32_t *ptr,
+                 const double *anb, const double *ac, double *eanb, double *eac);
+void gs_sweep_pipe(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev, int32_t n,
+                   int32_t w, const int32_t *ecol, const int32_t *elen, const double *eanb,
+                   const double *eac, const double *b, double *x, const double *stop = nullptr);
 // the same with the level's unknowns (x, rc, dinv) in LDS: n <= kLdsRows
 // (48 KB of doubles, inside the 64 KB any workgroup may allocate)
 constexpr int32_t kLdsRows = 6144;

@@ -496,34 +496,49 @@ void dilu_factor_lds(hipStream_t s, const int32_t *rows, const int32_t *lp, int3
                      rows, lp, nlev, n, ac, anb, lptr, lnb, lpji, lpij, skip_row, dinv);
 }
 
-// ---- pipelined one-workgroup sweeps.  In a level-scheduled sweep only the
-// neighbour values depend on the previous levels; the row of a level, its
-// entry range, right-hand side, diagonal and entries do not.  When no level
-// holds more rows than the kPipeBlock threads, thread t owns row t of every
-// level, and while level q computes, the loads of the levels after it are in
-// flight: level q+3's row, level q+2's range / b / ac, level q+1's first
-// kPipeEntries columns and values.  A level then waits for one round trip
-// (its neighbour values) and the barrier instead of five.  Level pointers
-// are read from LDS.  Row arithmetic as gs_row (the same terms in the same
-// order; entries past the prefetched ones are summed after, in order).
+// ---- pipelined one-workgroup sweeps over schedule-ordered ELL copies.  The
+// sweep runs in one workgroup (one CU), so its cost per level is set by how
+// many cache lines that CU's loads touch and by one round trip: a CSR row's
+// entries sit at the row's own offsets, so the lanes of a level (rows of
+// arbitrary index) each touch their own lines for every entry's column and
+// value.  Here slot u of schedule position s sits at u * n + s: the lanes of
+// a level (consecutive positions) read consecutive words, and only the
+// neighbour values x[c] and the right-hand side b[i] are gathered.  Positions
+// past a row's length hold the row itself and weight 0 (never summed).
+// Thread t owns position lps[q] + t of level q; level q+1's slots, length,
+// diagonal and right-hand side and level q+2's row are in flight while level
+// q computes.  Row arithmetic as gs_row (the same terms in the same order).
 
-// the first kPipeEntries entries of row i (entries [k0, k1)); slots past the
-// row repeat its last entry; rows without entries (or no row) read slot 0
 template <int NE>
-struct PipeEntries {
+struct PipeRow {
+  int32_t i, len;
+  double b, d;
   int32_t c[NE];
   double a[NE];
-  __device__ __forceinline__ void load(const int32_t *__restrict__ col,
-                                       const double *__restrict__ anb, int32_t i, int32_t k0,
-                                       int32_t k1) {
-    const int32_t m = k1 - k0 < NE ? k1 - k0 : NE;
+  // position s (< 0: none) holding row i; slots past w read slot w - 1
+  __device__ __forceinline__ void load(int32_t s, int32_t row, int32_t n, int32_t w,
+                                       const int32_t *__restrict__ ecol,
+                                       const int32_t *__restrict__ elen,
+                                       const double *__restrict__ eanb,
+                                       const double *__restrict__ eac,
+                                       const double *__restrict__ bv) {
+    i = row;
+    if (s >= 0) {
+      len = elen[s];
+      d = eac[s];
+      b = bv[row];
 #pragma unroll
-    for (int u = 0; u < NE; ++u) {
-      if (i >= 0 && m > 0) {
-        const int32_t e = k0 + (u < m ? u : m - 1);
-        c[u] = col[e];
-        a[u] = anb[e];
-      } else {
+      for (int u = 0; u < NE; ++u) {
+        const size_t e = (size_t)(u < w ? u : w - 1) * n + s;
+        c[u] = ecol[e];
+        a[u] = eanb[e];
+      }
+    } else {
+      len = 0;
+      d = 1.0;
+      b = 0.0;
+#pragma unroll
+      for (int u = 0; u < NE; ++u) {
         c[u] = 0;
         a[u] = 0.0;
       }
@@ -531,113 +546,97 @@ struct PipeEntries {
   }
 };
 
-// one level-scheduled Gauss-Seidel sweep over the levels lps[q] .. lps[q+1]
-// (LDS) of rows; xr(j) reads x_j, xw(i, v) writes x_i; NE entries of each row
-// are prefetched (8 when no row is longer, else 16)
-template <int NE, typename XR, typename XW>
-__device__ __forceinline__ void gs_sweep_pipe(const int32_t *__restrict__ rows, const int32_t *lps,
-                                              int32_t nlev, const int32_t *__restrict__ ptr,
-                                              const int32_t *__restrict__ col,
-                                              const double *__restrict__ ac,
-                                              const double *__restrict__ anb,
-                                              const double *__restrict__ b, const XR &xr,
-                                              const XW &xw) {
-  const int32_t tid = threadIdx.x;
-  auto row_at = [&](int32_t q) -> int32_t {
-    if (q >= nlev) return -1;
-    const int32_t t = lps[q] + tid;
-    return t < lps[q + 1] ? rows[t] : -1;
-  };
-  // stage A: level q+2's row; stage B: level q+1's row, range, b, ac;
-  // stage C: level q's row, range, b, ac and first entries
-  int32_t ia = row_at(2);
-  int32_t ib = row_at(1), kb0 = 0, kb1 = 0;
-  double bb = 0.0, db = 1.0;
-  if (ib >= 0) {
-    kb0 = ptr[ib];
-    kb1 = ptr[ib + 1];
-    bb = b[ib];
-    db = ac[ib];
-  }
-  int32_t ic = row_at(0), kc0 = 0, kc1 = 0;
-  double bc = 0.0, dc = 1.0;
-  if (ic >= 0) {
-    kc0 = ptr[ic];
-    kc1 = ptr[ic + 1];
-    bc = b[ic];
-    dc = ac[ic];
-  }
-  PipeEntries<NE> ec;
-  ec.load(col, anb, ic, kc0, kc1);
-  for (int32_t q = 0; q < nlev; ++q) {
-    // loads of the later levels (constant data)
-    const int32_t in = row_at(q + 3);
-    int32_t ka0 = 0, ka1 = 0;
-    double ba = 0.0, da = 1.0;
-    if (ia >= 0) {
-      ka0 = ptr[ia];
-      ka1 = ptr[ia + 1];
-      ba = b[ia];
-      da = ac[ia];
-    }
-    PipeEntries<NE> eb;
-    eb.load(col, anb, ib, kb0, kb1);
-    // level q: neighbour values, then the row's new value
-    if (ic >= 0) {
-      const int32_t len = kc1 - kc0, m = len < NE ? len : NE;
-      double xj[NE];
-#pragma unroll
-      for (int u = 0; u < NE; ++u) xj[u] = xr(ec.c[u]);
-      double v = bc;
-#pragma unroll
-      for (int u = 0; u < NE; ++u) {
-        const double t = v - ec.a[u] * xj[u];
-        v = u < m ? t : v;
-      }
-      if (len > NE) v = csr_row_sum(col, anb, xr, kc0 + NE, kc1, v);
-      xw(ic, v / dc);
-    }
-    __syncthreads();
-    ic = ib;
-    kc0 = kb0;
-    kc1 = kb1;
-    bc = bb;
-    dc = db;
-    ec = eb;
-    ib = ia;
-    kb0 = ka0;
-    kb1 = ka1;
-    bb = ba;
-    db = da;
-    ia = in;
-  }
-}
-
 template <int NE>
 __global__ void __launch_bounds__(kPipeBlock)
 k_gs_sweep_pipe(const int32_t *__restrict__ rows, const int32_t *__restrict__ lp, int32_t nlev,
-                const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
-                const double *__restrict__ ac, const double *__restrict__ anb,
-                const double *__restrict__ b, double *x, const double *__restrict__ stop) {
+                int32_t n, int32_t w, const int32_t *__restrict__ ecol,
+                const int32_t *__restrict__ elen, const double *__restrict__ eanb,
+                const double *__restrict__ eac, const double *__restrict__ b, double *x,
+                const double *__restrict__ stop) {
   if (stop && *stop != 0.0) return;  // stopped by the device-side test
   extern __shared__ int32_t lps[];
   for (int32_t q = threadIdx.x; q <= nlev; q += kPipeBlock) lps[q] = lp[q];
   __syncthreads();
-  gs_sweep_pipe<NE>(rows, lps, nlev, ptr, col, ac, anb, b, [x](int32_t j) { return x[j]; },
-                    [x](int32_t i, double v) { x[i] = v; });
+  const int32_t tid = threadIdx.x;
+  auto pos_at = [&](int32_t q) -> int32_t {
+    if (q >= nlev) return -1;
+    const int32_t t = lps[q] + tid;
+    return t < lps[q + 1] ? t : -1;
+  };
+  int32_t s0 = pos_at(0), s1 = pos_at(1);
+  PipeRow<NE> cur;
+  cur.load(s0, s0 >= 0 ? rows[s0] : -1, n, w, ecol, elen, eanb, eac, b);
+  int32_t i1 = s1 >= 0 ? rows[s1] : -1;
+  for (int32_t q = 0; q < nlev; ++q) {
+    // in flight: level q+1's slots, level q+2's row
+    const int32_t s2 = pos_at(q + 2);
+    const int32_t i2 = s2 >= 0 ? rows[s2] : -1;
+    PipeRow<NE> nxt;
+    nxt.load(s1, i1, n, w, ecol, elen, eanb, eac, b);
+    // level q (threads without a row compute on column 0 and store nothing)
+    {
+      double xj[NE];
+#pragma unroll
+      for (int u = 0; u < NE; ++u) xj[u] = x[cur.c[u]];
+      double v = cur.b;
+#pragma unroll
+      for (int u = 0; u < NE; ++u) {
+        const double t = v - cur.a[u] * xj[u];
+        v = u < cur.len ? t : v;
+      }
+      for (int32_t u = NE; u < cur.len; ++u) {
+        const size_t e = (size_t)u * n + s0;
+        v = v - eanb[e] * x[ecol[e]];
+      }
+      if (s0 >= 0) x[cur.i] = v / cur.d;
+    }
+    __syncthreads();
+    cur = nxt;
+    s0 = s1;
+    s1 = s2;
+    i1 = i2;
+  }
 }
 
-void gs_sweep_pipe(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
-                   const int32_t *ptr, const int32_t *col, const double *ac, const double *anb,
-                   const double *b, double *x, int32_t max_row, const double *stop) {
+// eanb / eac for one schedule: slot u of position s from row rows[s]'s
+// entry u (0 past the row's length), the diagonal of that row
+__global__ void k_pipe_values(const int32_t *__restrict__ rows, int32_t n, int32_t w,
+                              const int32_t *__restrict__ ptr, const double *__restrict__ anb,
+                              const double *__restrict__ ac, double *__restrict__ eanb,
+                              double *__restrict__ eac) {
+  const int32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int32_t i = rows[s], k0 = ptr[i], len = ptr[i + 1] - k0;
+  eac[s] = ac[i];
+  for (int32_t u = 0; u < w; ++u) eanb[(size_t)u * n + s] = u < len ? anb[k0 + u] : 0.0;
+}
+
+void pipe_values(hipStream_t s, const int32_t *rows, int32_t n, int32_t w, const int32_t *ptr,
+                 const double *anb, const double *ac, double *eanb, double *eac) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_pipe_values, dim3((n + 255) / 256), dim3(256), 0, s, rows, n, w, ptr, anb,
+                     ac, eanb, eac);
+}
+
+size_t gs_sweep_pipe_lds(int32_t nlev) { return (size_t)(nlev + 1) * sizeof(int32_t); }
+
+void gs_sweep_pipe(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev, int32_t n,
+                   int32_t w, const int32_t *ecol, const int32_t *elen, const double *eanb,
+                   const double *eac, const double *b, double *x, const double *stop) {
   if (nlev <= 0) return;
-  const size_t lds = (size_t)(nlev + 1) * sizeof(int32_t);
-  if (max_row <= 8)
-    hipLaunchKernelGGL(k_gs_sweep_pipe<8>, dim3(1), dim3(kPipeBlock), lds, s, rows, lp, nlev, ptr,
-                       col, ac, anb, b, x, stop);
+  const size_t lds = gs_sweep_pipe_lds(nlev);
+#define UFVM_PIPE(NE)                                                                           \
+  hipLaunchKernelGGL(k_gs_sweep_pipe<NE>, dim3(1), dim3(kPipeBlock), lds, s, rows, lp, nlev, n, \
+                     w, ecol, elen, eanb, eac, b, x, stop)
+  if (w <= 8)
+    UFVM_PIPE(8);
+  else if (w <= 16)
+    UFVM_PIPE(16);
+  else if (w <= 24)
+    UFVM_PIPE(24);
   else
-    hipLaunchKernelGGL(k_gs_sweep_pipe<16>, dim3(1), dim3(kPipeBlock), lds, s, rows, lp, nlev,
-                       ptr, col, ac, anb, b, x, stop);
+    UFVM_PIPE(32);
+#undef UFVM_PIPE
 }
 
 void gs_sweep_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,

@@ -119,6 +119,13 @@ struct RefLevel {
   bool one_wg = false;
   int32_t widest = 0;             // rows of the widest level of both schedules
   int32_t max_row = 0;            // the most off-diagonal entries of a row
+  // for the pipelined sweep (widest <= kPipeBlock): per direction, the level
+  // in schedule order as ELL (pipe_w slots; slot u of position s at
+  // u * n + s): columns and row lengths built with the schedule, values and
+  // diagonals refreshed before each use
+  int32_t pipe_w = 0;
+  DBuf<int32_t> fwd_ecol, bwd_ecol, fwd_elen, bwd_elen;
+  DBuf<double> fwd_eanb, bwd_eanb, fwd_eac, bwd_eac;
   bool low_ok = false;
   LowerLists low;
   DBuf<int32_t> lptr, lnb, lpji, lpij;

@@ -110,9 +110,8 @@ def test_pipelined_sweeps_beyond_lds(ctx, grid):
     # kernel: the 100 x 100 lexicographic grid itself (10,000 rows, levels are
     # the anti-diagonals, rows of at most 8 entries), and the first coarse
     # level of a jittered, renumbered 48^3 hex mesh (16,949 rows, levels of at
-    # most 97 rows, rows of up to 23 entries: the 16-entry prefetch plus the
-    # rest of the row).  The sweeps keep the oracle's bits, alone and inside
-    # V-cycles.
+    # most 97 rows, rows of up to 23 entries: the 24-entry prefetch).  The
+    # sweeps keep the oracle's bits, alone and inside V-cycles.
     if grid == "2d100":
         m = cavity_mesh(100, nz=1, jitter=0.1, seed=64, renumber_seed=None)
     else:
@@ -136,6 +135,38 @@ def test_pipelined_sweeps_beyond_lds(ctx, grid):
                               pre_sweep=2, post_sweep=2, smoother="SOR")
     assert nc == qc
     assert np.array_equal(ctx.get_dphi(sid, 0), S2.level(0)["dphi"])
+    ctx.system_destroy(sid)
+
+
+@pytest.mark.parametrize("n_off", [14, 22])
+def test_pipelined_sweeps_long_rows(ctx, n_off):
+    # A banded M-matrix of 9,000 rows whose neighbours sit at n_off random
+    # distances in [40, 400) on either side: lexicographic levels of at most 64 rows,
+    # rows of up to 28 entries (the 32-entry prefetch) or 44 (the prefetch
+    # plus the rest of the row).  SOR sweeps keep the oracle's bits.
+    n = 9000
+    rng = np.random.default_rng(100 + n_off)
+    offs = np.sort(rng.choice(np.arange(40, 400), n_off, replace=False))
+    rows, cols = [], []
+    for d in offs:
+        i = np.arange(n - d)
+        rows += [i, i + d]
+        cols += [i + d, i]
+    A = sp.csr_matrix((-rng.uniform(0.1, 1.0, sum(len(r) for r in rows)),
+                       (np.concatenate(rows), np.concatenate(cols))), shape=(n, n))
+    A = (A + A.T) * 0.5
+    A.sort_indices()
+    ptr, col, anb = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data
+    assert np.diff(ptr).max() == 2 * n_off
+    ac = -np.asarray(A.sum(axis=1)).ravel() * 1.05
+    b = rng.standard_normal(n)
+    sid = make(ctx, ptr, col, ac, anb, b)
+    x0 = rng.standard_normal(n) * 1e-3
+    ctx.set_dphi(sid, 0, x0)
+    ctx.solve_algebraic_system(sid, 0, "SOR", 3, rrf=0.0)
+    S = oracle.System(ptr, col, ac, anb, bc=b, dphi=x0)
+    S.solve_algebraic_system(0, "SOR", 3, rrf=0.0)
+    assert np.array_equal(ctx.get_dphi(sid, 0), S.level(0)["dphi"])
     ctx.system_destroy(sid)
 
 
