@@ -106,7 +106,9 @@ void ensure_schedule(ufvm_ctx *ctx, RefLevel &L) {
       for (int32_t s = 0; s < n; ++s) {
         const int32_t i = S->rows[s], k0 = L.g.ptr[i], len = L.g.ptr[i + 1] - k0;
         elen[s] = len;
-        for (int32_t u = 0; u < w; ++u) ecol[(size_t)u * n + s] = u < len ? L.g.col[k0 + u] : i;
+        // past the row's length: column 0 for every lane (one cache line per
+        // gather instruction; the value is never summed)
+        for (int32_t u = 0; u < w; ++u) ecol[(size_t)u * n + s] = u < len ? L.g.col[k0 + u] : 0;
       }
       C->upload(ecol, ctx->stream);
       N->upload(elen, ctx->stream);

@@ -102,7 +102,7 @@ void gs_sweep_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32
 // The same sweep pipelined (see kernels.hip) for schedules whose levels hold
 // at most kPipeBlock rows, over a schedule-ordered ELL copy of the level:
 // slot u of position s at u * n + s (w slots; ecol static, past a row's
-// length the row itself; elen the row lengths), eanb / eac refreshed from
+// length column 0; elen the row lengths), eanb / eac refreshed from
 // the CSR values by pipe_values before the sweeps; the level pointers in LDS
 // (gs_sweep_pipe_lds bytes).  Same arithmetic.
 constexpr int kPipeBlock = 256;

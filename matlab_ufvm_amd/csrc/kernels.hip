@@ -504,7 +504,7 @@ void dilu_factor_lds(hipStream_t s, const int32_t *rows, const int32_t *lp, int3
 // value.  Here slot u of schedule position s sits at u * n + s: the lanes of
 // a level (consecutive positions) read consecutive words, and only the
 // neighbour values x[c] and the right-hand side b[i] are gathered.  Positions
-// past a row's length hold the row itself and weight 0 (never summed).
+// past a row's length hold column 0 and weight 0 (never summed).
 // Thread t owns position lps[q] + t of level q; level q+1's slots, length,
 // diagonal and right-hand side and level q+2's row are in flight while level
 // q computes.  Row arithmetic as gs_row (the same terms in the same order).
