@@ -123,6 +123,54 @@ void ensure_schedule(ufvm_ctx *ctx, RefLevel &L) {
   L.lnb.upload(L.low.nb, ctx->stream);
   L.lpji.upload(L.low.pos_ji, ctx->stream);
   L.lpij.upload(L.low.pos_ij, ctx->stream);
+  if (widest <= k::kPipeBlock && L.low_ok) {
+    // DILU ELL copies: forward over the lower lists in forward-schedule
+    // order, backward over the entries with column above the row in
+    // backward-schedule order
+    const int32_t n = L.n;
+    auto build = [&](const Schedule &S, bool upper, int32_t &w_out, DBuf<int32_t> &C,
+                     DBuf<int32_t> &N, DBuf<int32_t> &E, DBuf<double> &V) {
+      std::vector<int32_t> len(n);
+      int32_t w = 1;
+      for (int32_t s = 0; s < n; ++s) {
+        const int32_t i = S.rows[s];
+        int32_t m = 0;
+        if (upper) {
+          for (int32_t k = L.g.ptr[i]; k < L.g.ptr[i + 1]; ++k) m += L.g.col[k] > i;
+        } else {
+          m = L.low.ptr[i + 1] - L.low.ptr[i];
+        }
+        len[s] = m;
+        w = std::max(w, m);
+      }
+      std::vector<int32_t> col((size_t)n * w, 0), src((size_t)n * w, -1);
+      for (int32_t s = 0; s < n; ++s) {
+        const int32_t i = S.rows[s];
+        int32_t u = 0;
+        if (upper) {
+          for (int32_t k = L.g.ptr[i]; k < L.g.ptr[i + 1]; ++k)
+            if (L.g.col[k] > i) {
+              col[(size_t)u * n + s] = L.g.col[k];
+              src[(size_t)u * n + s] = k;
+              ++u;
+            }
+        } else {
+          for (int32_t e = L.low.ptr[i]; e < L.low.ptr[i + 1]; ++e, ++u) {
+            col[(size_t)u * n + s] = L.low.nb[e];
+            src[(size_t)u * n + s] = L.low.pos_ji[e];
+          }
+        }
+      }
+      w_out = w;
+      C.upload(col, ctx->stream);
+      N.upload(len, ctx->stream);
+      E.upload(src, ctx->stream);
+      V.alloc((size_t)n * w);
+    };
+    build(L.fwd, false, L.dpipe_wf, L.dfwd_ecol, L.dfwd_elen, L.dfwd_esrc, L.dfwd_eval);
+    build(L.bwd, true, L.dpipe_wb, L.dbwd_ecol, L.dbwd_elen, L.dbwd_esrc, L.dbwd_eval);
+    L.dilu_y.alloc(n);
+  }
   L.sched = true;
 }
 
@@ -271,6 +319,22 @@ void dilu_sweep(ufvm_ctx *ctx, RefLevel &L, const double *stop = nullptr) {
   ensure_partial(ctx, L.n, 1);
   k::residual_ref(ctx->stream, L.n, L.ptr.p, L.col.p, L.ac.p, L.anb.p, L.bc.p, L.dphi.p, L.rc.p,
                   ctx->partial.p, stop);
+  // levels too large for LDS whose schedule levels hold at most kPipeBlock
+  // rows: the pipelined substitutions over the ELL copies (values from the
+  // current coefficients first)
+  if (L.widest <= k::kPipeBlock && L.low_ok && !(L.one_wg && L.n <= k::kLdsRows) &&
+      k::gs_sweep_pipe_lds(std::max(nlev(L.fwd), nlev(L.bwd))) <= kLdsBytes) {
+    const int32_t n = L.n;
+    k::pipe_gather(ctx->stream, (int64_t)n * L.dpipe_wf, L.dfwd_esrc.p, L.anb.p, L.dfwd_eval.p);
+    k::pipe_gather(ctx->stream, (int64_t)n * L.dpipe_wb, L.dbwd_esrc.p, L.anb.p, L.dbwd_eval.p);
+    k::dilu_pipe(ctx->stream, false, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), n, L.dpipe_wf,
+                 L.dfwd_ecol.p, L.dfwd_elen.p, L.dfwd_eval.p, L.dinv.p, L.rc.p, L.dilu_y.p,
+                 L.dphi.p, stop);
+    k::dilu_pipe(ctx->stream, true, L.bwd_rows.p, L.bwd_lp.p, nlev(L.bwd), n, L.dpipe_wb,
+                 L.dbwd_ecol.p, L.dbwd_elen.p, L.dbwd_eval.p, L.dinv.p, L.rc.p, L.dilu_y.p,
+                 L.dphi.p, stop);
+    return;
+  }
   if (L.one_wg && L.n <= k::kLdsRows) {
     k::dilu_lds(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.bwd_rows.p, L.bwd_lp.p,
                 nlev(L.bwd), L.n, L.ptr.p, L.col.p, L.anb.p, L.lptr.p, L.lnb.p, L.lpji.p,

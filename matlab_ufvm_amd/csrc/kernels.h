@@ -112,6 +112,16 @@ void pipe_values(hipStream_t s, const int32_t *rows, int32_t n, int32_t w, const
 void gs_sweep_pipe(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev, int32_t n,
                    int32_t w, const int32_t *ecol, const int32_t *elen, const double *eanb,
                    const double *eac, const double *b, double *x, const double *stop = nullptr);
+// The DILU forward / backward substitutions pipelined the same way, over ELL
+// copies in each schedule's order (forward: the lower-neighbour lists;
+// backward: the entries with column above the row); eval = anb[esrc]
+// refreshed by pipe_gather; y (n values) holds dinv_i * rc_i of the forward
+// sweep.  Same arithmetic as dilu_fwd_row / dilu_bwd_row.
+void pipe_gather(hipStream_t s, int64_t m, const int32_t *esrc, const double *anb, double *eval);
+void dilu_pipe(hipStream_t s, bool backward, const int32_t *rows, const int32_t *lp, int32_t nlev,
+               int32_t n, int32_t w, const int32_t *ecol, const int32_t *elen, const double *eval,
+               const double *dinv, double *rc, double *y, double *x,
+               const double *stop = nullptr);
 // the same with the level's unknowns (x, rc, dinv) in LDS: n <= kLdsRows
 // (48 KB of doubles, inside the 64 KB any workgroup may allocate)
 constexpr int32_t kLdsRows = 6144;

@@ -639,6 +639,154 @@ void gs_sweep_pipe(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_
 #undef UFVM_PIPE
 }
 
+// The DILU substitutions (dilu_fwd_row / dilu_bwd_row arithmetic) in the same
+// pipelined form, over ELL copies in each direction's schedule order: the
+// forward copy holds a row's lower-neighbour list (columns lnb, values
+// anb[lpji]), the backward copy its CSR entries with column above the row
+// (cconn order).  esrc names each slot's anb entry (-1 past the row's
+// length: column 0, value 0).  The forward sweep stores y_i = dinv_i * rc_i
+// beside rc_i, so a lower neighbour's term takes one gather; the product is
+// the one the row function forms.
+template <int NE, bool BWD>
+struct DiluPipeRow {
+  int32_t i, len;
+  double v0, d, x0;
+  int32_t c[NE];
+  double a[NE];
+  __device__ __forceinline__ void load(int32_t s, int32_t row, int32_t n, int32_t w,
+                                       const int32_t *__restrict__ ecol,
+                                       const int32_t *__restrict__ elen,
+                                       const double *__restrict__ eval,
+                                       const double *__restrict__ dinv, const double *rc,
+                                       const double *x) {
+    i = row;
+    if (s >= 0) {
+      len = elen[s];
+      v0 = rc[row];
+      d = dinv[row];
+      x0 = BWD ? x[row] : 0.0;
+#pragma unroll
+      for (int u = 0; u < NE; ++u) {
+        const size_t e = (size_t)(u < w ? u : w - 1) * n + s;
+        c[u] = ecol[e];
+        a[u] = eval[e];
+      }
+    } else {
+      len = 0;
+      v0 = 0.0;
+      d = 0.0;
+      x0 = 0.0;
+#pragma unroll
+      for (int u = 0; u < NE; ++u) {
+        c[u] = 0;
+        a[u] = 0.0;
+      }
+    }
+  }
+};
+
+// forward (BWD false): rc_j -= a_ji y_i over the lower list, y_j = dinv_j rc_j;
+// backward: rc_i -= a_ij rc_j over the upper entries, z = dinv_i rc_i,
+// rc_i = z, x_i += z
+template <int NE, bool BWD>
+__global__ void __launch_bounds__(kPipeBlock)
+k_dilu_pipe(const int32_t *__restrict__ rows, const int32_t *__restrict__ lp, int32_t nlev,
+            int32_t n, int32_t w, const int32_t *__restrict__ ecol,
+            const int32_t *__restrict__ elen, const double *__restrict__ eval,
+            const double *__restrict__ dinv, double *rc, double *y, double *x,
+            const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;  // stopped by the device-side test
+  extern __shared__ int32_t lps[];
+  for (int32_t q = threadIdx.x; q <= nlev; q += kPipeBlock) lps[q] = lp[q];
+  __syncthreads();
+  const int32_t tid = threadIdx.x;
+  auto pos_at = [&](int32_t q) -> int32_t {
+    if (q >= nlev) return -1;
+    const int32_t t = lps[q] + tid;
+    return t < lps[q + 1] ? t : -1;
+  };
+  double *g = BWD ? rc : y;  // the gathered neighbour values
+  int32_t s0 = pos_at(0), s1 = pos_at(1);
+  DiluPipeRow<NE, BWD> cur;
+  cur.load(s0, s0 >= 0 ? rows[s0] : -1, n, w, ecol, elen, eval, dinv, rc, x);
+  int32_t i1 = s1 >= 0 ? rows[s1] : -1;
+  for (int32_t q = 0; q < nlev; ++q) {
+    const int32_t s2 = pos_at(q + 2);
+    const int32_t i2 = s2 >= 0 ? rows[s2] : -1;
+    DiluPipeRow<NE, BWD> nxt;
+    nxt.load(s1, i1, n, w, ecol, elen, eval, dinv, rc, x);
+    {
+      double gj[NE];
+#pragma unroll
+      for (int u = 0; u < NE; ++u) gj[u] = g[cur.c[u]];
+      double v = cur.v0;
+#pragma unroll
+      for (int u = 0; u < NE; ++u) {
+        const double t = v - cur.a[u] * gj[u];
+        v = u < cur.len ? t : v;
+      }
+      for (int32_t u = NE; u < cur.len; ++u) {
+        const size_t e = (size_t)u * n + s0;
+        v = v - eval[e] * g[ecol[e]];
+      }
+      if (s0 >= 0) {
+        if (BWD) {
+          const double z = cur.d * v;
+          rc[cur.i] = z;
+          x[cur.i] = cur.x0 + z;
+        } else {
+          rc[cur.i] = v;
+          y[cur.i] = cur.d * v;
+        }
+      }
+    }
+    __syncthreads();
+    cur = nxt;
+    s0 = s1;
+    s1 = s2;
+    i1 = i2;
+  }
+}
+
+// eval[e] = anb[esrc[e]] (0 where esrc is -1)
+__global__ void k_pipe_gather(int64_t m, const int32_t *__restrict__ esrc,
+                              const double *__restrict__ anb, double *__restrict__ eval) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  const int32_t k = esrc[e];
+  eval[e] = k >= 0 ? anb[k] : 0.0;
+}
+
+void pipe_gather(hipStream_t s, int64_t m, const int32_t *esrc, const double *anb, double *eval) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_pipe_gather, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, s, m, esrc,
+                     anb, eval);
+}
+
+void dilu_pipe(hipStream_t s, bool backward, const int32_t *rows, const int32_t *lp, int32_t nlev,
+               int32_t n, int32_t w, const int32_t *ecol, const int32_t *elen, const double *eval,
+               const double *dinv, double *rc, double *y, double *x, const double *stop) {
+  if (nlev <= 0) return;
+  const size_t lds = gs_sweep_pipe_lds(nlev);
+#define UFVM_DPIPE(NE)                                                                            \
+  do {                                                                                            \
+    if (backward)                                                                                 \
+      hipLaunchKernelGGL((k_dilu_pipe<NE, true>), dim3(1), dim3(kPipeBlock), lds, s, rows, lp,   \
+                         nlev, n, w, ecol, elen, eval, dinv, rc, y, x, stop);                     \
+    else                                                                                          \
+      hipLaunchKernelGGL((k_dilu_pipe<NE, false>), dim3(1), dim3(kPipeBlock), lds, s, rows, lp,  \
+                         nlev, n, w, ecol, elen, eval, dinv, rc, y, x, stop);                     \
+  } while (0)
+  if (w <= 8)
+    UFVM_DPIPE(8);
+  else if (w <= 16)
+    UFVM_DPIPE(16);
+  else
+    UFVM_DPIPE(24);
+#undef UFVM_DPIPE
+}
+
+
 void gs_sweep_block(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
                     const int32_t *ptr, const int32_t *col, const double *ac, const double *anb,
                     const double *b, double *x, const double *stop) {

@@ -126,6 +126,12 @@ struct RefLevel {
   int32_t pipe_w = 0;
   DBuf<int32_t> fwd_ecol, bwd_ecol, fwd_elen, bwd_elen;
   DBuf<double> fwd_eanb, bwd_eanb, fwd_eac, bwd_eac;
+  // the same for the DILU substitutions (when the lower lists exist):
+  // forward copy of the lower lists, backward copy of the upper entries,
+  // each slot's anb index (-1: padding), and y = dinv * rc of the forward
+  int32_t dpipe_wf = 0, dpipe_wb = 0;
+  DBuf<int32_t> dfwd_ecol, dfwd_elen, dfwd_esrc, dbwd_ecol, dbwd_elen, dbwd_esrc;
+  DBuf<double> dfwd_eval, dbwd_eval, dilu_y;
   bool low_ok = false;
   LowerLists low;
   DBuf<int32_t> lptr, lnb, lpji, lpij;

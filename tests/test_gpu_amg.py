@@ -103,15 +103,17 @@ def test_solve_algebraic_system_bitexact(ctx, case, smoother):
     ctx.system_destroy(sid)
 
 
+@pytest.mark.parametrize("smoother", ["SOR", "ILU"])
 @pytest.mark.parametrize("grid", ["2d100", "3d48"])
-def test_pipelined_sweeps_beyond_lds(ctx, grid):
+def test_pipelined_sweeps_beyond_lds(ctx, grid, smoother):
     # Levels with more unknowns than LDS holds and no lexicographic level wider
     # than 256 rows run their SOR sweeps in the pipelined one-workgroup
     # kernel: the 100 x 100 lexicographic grid itself (10,000 rows, levels are
     # the anti-diagonals, rows of at most 8 entries), and the first coarse
     # level of a jittered, renumbered 48^3 hex mesh (16,949 rows, levels of at
     # most 97 rows, rows of up to 23 entries: the 24-entry prefetch).  The
-    # sweeps keep the oracle's bits, alone and inside V-cycles.
+    # SOR sweeps and the DILU substitutions keep the oracle's bits, alone and
+    # inside V-cycles.
     if grid == "2d100":
         m = cavity_mesh(100, nz=1, jitter=0.1, seed=64, renumber_seed=None)
     else:
@@ -122,28 +124,31 @@ def test_pipelined_sweeps_beyond_lds(ctx, grid):
     sid = make(ctx, ptr, col, ac, anb, b)
     x0 = np.random.default_rng(66).standard_normal(m.n_cells) * 1e-3
     ctx.set_dphi(sid, 0, x0)
-    ctx.solve_algebraic_system(sid, 0, "SOR", 3, rrf=0.0)
+    ctx.solve_algebraic_system(sid, 0, smoother, 3, rrf=0.0)
     S = oracle.System(ptr, col, ac, anb, bc=b, dphi=x0)
-    S.solve_algebraic_system(0, "SOR", 3, rrf=0.0)
+    S.solve_algebraic_system(0, smoother, 3, rrf=0.0)
     assert np.array_equal(ctx.get_dphi(sid, 0), S.level(0)["dphi"])
     ctx.set_dphi(sid, 0, np.zeros(m.n_cells))
     r0, r1, nc = ctx.apply_amg(sid, "V-Cycle", max_cycles=4, termination=1e-12,
-                               max_coarse_levels=10, pre_sweep=2, post_sweep=2, smoother="SOR",
-                               rebuild=1)
+                               max_coarse_levels=10, pre_sweep=2, post_sweep=2,
+                               smoother=smoother, rebuild=1)
     S2 = oracle.System(ptr, col, ac, anb, bc=b)
     q0, q1, qc = S2.apply_amg("V-Cycle", max_cycles=4, termination=1e-12, max_coarse_levels=10,
-                              pre_sweep=2, post_sweep=2, smoother="SOR")
+                              pre_sweep=2, post_sweep=2, smoother=smoother)
     assert nc == qc
     assert np.array_equal(ctx.get_dphi(sid, 0), S2.level(0)["dphi"])
     ctx.system_destroy(sid)
 
 
-@pytest.mark.parametrize("n_off", [14, 22])
-def test_pipelined_sweeps_long_rows(ctx, n_off):
+@pytest.mark.parametrize("smoother", ["SOR", "ILU"])
+@pytest.mark.parametrize("n_off", [14, 22, 30])
+def test_pipelined_sweeps_long_rows(ctx, n_off, smoother):
     # A banded M-matrix of 9,000 rows whose neighbours sit at n_off random
     # distances in [40, 400) on either side: lexicographic levels of at most 64 rows,
-    # rows of up to 28 entries (the 32-entry prefetch) or 44 (the prefetch
-    # plus the rest of the row).  SOR sweeps keep the oracle's bits.
+    # rows of 28, 44 or 60 entries (SOR: the 32-entry prefetch, then with the
+    # rest of the row) and lower / upper lists of 14, 22 or 30 (DILU: the 16-
+    # and 24-entry prefetches, then with the rest).  The sweeps keep the
+    # oracle's bits.
     n = 9000
     rng = np.random.default_rng(100 + n_off)
     offs = np.sort(rng.choice(np.arange(40, 400), n_off, replace=False))
@@ -163,9 +168,9 @@ def test_pipelined_sweeps_long_rows(ctx, n_off):
     sid = make(ctx, ptr, col, ac, anb, b)
     x0 = rng.standard_normal(n) * 1e-3
     ctx.set_dphi(sid, 0, x0)
-    ctx.solve_algebraic_system(sid, 0, "SOR", 3, rrf=0.0)
+    ctx.solve_algebraic_system(sid, 0, smoother, 3, rrf=0.0)
     S = oracle.System(ptr, col, ac, anb, bc=b, dphi=x0)
-    S.solve_algebraic_system(0, "SOR", 3, rrf=0.0)
+    S.solve_algebraic_system(0, smoother, 3, rrf=0.0)
     assert np.array_equal(ctx.get_dphi(sid, 0), S.level(0)["dphi"])
     ctx.system_destroy(sid)
 
