@@ -170,6 +170,36 @@ void ensure_schedule(ufvm_ctx *ctx, RefLevel &L) {
     build(L.fwd, false, L.dpipe_wf, L.dfwd_ecol, L.dfwd_elen, L.dfwd_esrc, L.dfwd_eval);
     build(L.bwd, true, L.dpipe_wb, L.dbwd_ecol, L.dbwd_elen, L.dbwd_esrc, L.dbwd_eval);
     L.dilu_y.alloc(n);
+    // the factor's copy (cfdFactorizeILU.m:21: the neighbour n-2 is skipped)
+    const int32_t skip = n - 2;
+    std::vector<int32_t> len(n);
+    int32_t w = 1;
+    for (int32_t s = 0; s < n; ++s) {
+      const int32_t j = L.fwd.rows[s];
+      int32_t m = 0;
+      for (int32_t e = L.low.ptr[j]; e < L.low.ptr[j + 1]; ++e) m += L.low.nb[e] != skip;
+      len[s] = m;
+      w = std::max(w, m);
+    }
+    std::vector<int32_t> col((size_t)n * w, 0), sji((size_t)n * w, -1), sij((size_t)n * w, -1);
+    for (int32_t s = 0; s < n; ++s) {
+      const int32_t j = L.fwd.rows[s];
+      int32_t u = 0;
+      for (int32_t e = L.low.ptr[j]; e < L.low.ptr[j + 1]; ++e)
+        if (L.low.nb[e] != skip) {
+          col[(size_t)u * n + s] = L.low.nb[e];
+          sji[(size_t)u * n + s] = L.low.pos_ji[e];
+          sij[(size_t)u * n + s] = L.low.pos_ij[e];
+          ++u;
+        }
+    }
+    L.fpipe_w = w;
+    L.fac_ecol.upload(col, ctx->stream);
+    L.fac_elen.upload(len, ctx->stream);
+    L.fac_sji.upload(sji, ctx->stream);
+    L.fac_sij.upload(sij, ctx->stream);
+    L.fac_eji.alloc((size_t)n * w);
+    L.fac_eij.alloc((size_t)n * w);
   }
   L.sched = true;
 }
@@ -300,6 +330,15 @@ void dilu_factor(ufvm_ctx *ctx, RefLevel &L) {
   if (!L.low_ok) throw Error(UFVM_ERR_NUMERIC, "ILU: the index for i in j is not found");
   const auto &F = L.fwd.level_ptr;
   const int32_t skip = L.n - 2;  // cfdFactorizeILU.m:21 skips 1-based row N-1
+  if (L.widest <= k::kPipeBlock && !(L.one_wg && L.n <= k::kLdsRows) &&
+      k::gs_sweep_pipe_lds(nlev(L.fwd)) <= kLdsBytes) {
+    const int64_t m = (int64_t)L.n * L.fpipe_w;
+    k::pipe_gather(ctx->stream, m, L.fac_sji.p, L.anb.p, L.fac_eji.p);
+    k::pipe_gather(ctx->stream, m, L.fac_sij.p, L.anb.p, L.fac_eij.p);
+    k::dilu_factor_pipe(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.n, L.fpipe_w,
+                        L.fac_ecol.p, L.fac_elen.p, L.fac_eji.p, L.fac_eij.p, L.ac.p, L.dinv.p);
+    return;
+  }
   if (L.one_wg && L.n <= k::kLdsRows) {
     k::dilu_factor_lds(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.n, L.ac.p, L.anb.p,
                        L.lptr.p, L.lnb.p, L.lpji.p, L.lpij.p, skip, L.dinv.p);

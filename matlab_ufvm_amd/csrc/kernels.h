@@ -122,6 +122,12 @@ void dilu_pipe(hipStream_t s, bool backward, const int32_t *rows, const int32_t 
                int32_t n, int32_t w, const int32_t *ecol, const int32_t *elen, const double *eval,
                const double *dinv, double *rc, double *y, double *x,
                const double *stop = nullptr);
+// The DILU factor pipelined the same way over an ELL copy of the lower lists
+// (forward-schedule order, the skipped neighbour left out); eji / eij the
+// slots' a_ji / a_ij.  Same arithmetic as dilu_factor_row.
+void dilu_factor_pipe(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
+                      int32_t n, int32_t w, const int32_t *ecol, const int32_t *elen,
+                      const double *eji, const double *eij, const double *ac, double *dinv);
 // the same with the level's unknowns (x, rc, dinv) in LDS: n <= kLdsRows
 // (48 KB of doubles, inside the 64 KB any workgroup may allocate)
 constexpr int32_t kLdsRows = 6144;

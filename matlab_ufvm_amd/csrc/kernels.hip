@@ -748,6 +748,110 @@ k_dilu_pipe(const int32_t *__restrict__ rows, const int32_t *__restrict__ lp, in
   }
 }
 
+// The DILU factor (dilu_factor_row arithmetic) pipelined the same way over an
+// ELL copy of the lower-neighbour lists in forward-schedule order, without
+// the skipped neighbour: slot values a_ji (eji) and a_ij (eij), the
+// neighbour's dinv gathered.
+template <int NE>
+struct FactorPipeRow {
+  int32_t i, len;
+  double d0;
+  int32_t c[NE];
+  double aji[NE], aij[NE];
+  __device__ __forceinline__ void load(int32_t s, int32_t row, int32_t n, int32_t w,
+                                       const int32_t *__restrict__ ecol,
+                                       const int32_t *__restrict__ elen,
+                                       const double *__restrict__ eji,
+                                       const double *__restrict__ eij,
+                                       const double *__restrict__ ac) {
+    i = row;
+    if (s >= 0) {
+      len = elen[s];
+      d0 = ac[row];
+#pragma unroll
+      for (int u = 0; u < NE; ++u) {
+        const size_t e = (size_t)(u < w ? u : w - 1) * n + s;
+        c[u] = ecol[e];
+        aji[u] = eji[e];
+        aij[u] = eij[e];
+      }
+    } else {
+      len = 0;
+      d0 = 1.0;
+#pragma unroll
+      for (int u = 0; u < NE; ++u) {
+        c[u] = 0;
+        aji[u] = 0.0;
+        aij[u] = 0.0;
+      }
+    }
+  }
+};
+
+template <int NE>
+__global__ void __launch_bounds__(kPipeBlock)
+k_dilu_factor_pipe(const int32_t *__restrict__ rows, const int32_t *__restrict__ lp, int32_t nlev,
+                   int32_t n, int32_t w, const int32_t *__restrict__ ecol,
+                   const int32_t *__restrict__ elen, const double *__restrict__ eji,
+                   const double *__restrict__ eij, const double *__restrict__ ac, double *dinv) {
+  extern __shared__ int32_t lps[];
+  for (int32_t q = threadIdx.x; q <= nlev; q += kPipeBlock) lps[q] = lp[q];
+  __syncthreads();
+  const int32_t tid = threadIdx.x;
+  auto pos_at = [&](int32_t q) -> int32_t {
+    if (q >= nlev) return -1;
+    const int32_t t = lps[q] + tid;
+    return t < lps[q + 1] ? t : -1;
+  };
+  int32_t s0 = pos_at(0), s1 = pos_at(1);
+  FactorPipeRow<NE> cur;
+  cur.load(s0, s0 >= 0 ? rows[s0] : -1, n, w, ecol, elen, eji, eij, ac);
+  int32_t i1 = s1 >= 0 ? rows[s1] : -1;
+  for (int32_t q = 0; q < nlev; ++q) {
+    const int32_t s2 = pos_at(q + 2);
+    const int32_t i2 = s2 >= 0 ? rows[s2] : -1;
+    FactorPipeRow<NE> nxt;
+    nxt.load(s1, i1, n, w, ecol, elen, eji, eij, ac);
+    {
+      double dj[NE];
+#pragma unroll
+      for (int u = 0; u < NE; ++u) dj[u] = dinv[cur.c[u]];
+      double d = cur.d0;
+#pragma unroll
+      for (int u = 0; u < NE; ++u) {
+        const double t = d - (cur.aji[u] * dj[u]) * cur.aij[u];
+        d = u < cur.len ? t : d;
+      }
+      for (int32_t u = NE; u < cur.len; ++u) {
+        const size_t e = (size_t)u * n + s0;
+        d = d - (eji[e] * dinv[ecol[e]]) * eij[e];
+      }
+      if (s0 >= 0) dinv[cur.i] = 1.0 / d;
+    }
+    __syncthreads();
+    cur = nxt;
+    s0 = s1;
+    s1 = s2;
+    i1 = i2;
+  }
+}
+
+void dilu_factor_pipe(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
+                      int32_t n, int32_t w, const int32_t *ecol, const int32_t *elen,
+                      const double *eji, const double *eij, const double *ac, double *dinv) {
+  if (nlev <= 0) return;
+  const size_t lds = gs_sweep_pipe_lds(nlev);
+  if (w <= 8)
+    hipLaunchKernelGGL(k_dilu_factor_pipe<8>, dim3(1), dim3(kPipeBlock), lds, s, rows, lp, nlev,
+                       n, w, ecol, elen, eji, eij, ac, dinv);
+  else if (w <= 16)
+    hipLaunchKernelGGL(k_dilu_factor_pipe<16>, dim3(1), dim3(kPipeBlock), lds, s, rows, lp, nlev,
+                       n, w, ecol, elen, eji, eij, ac, dinv);
+  else
+    hipLaunchKernelGGL(k_dilu_factor_pipe<24>, dim3(1), dim3(kPipeBlock), lds, s, rows, lp, nlev,
+                       n, w, ecol, elen, eji, eij, ac, dinv);
+}
+
 // eval[e] = anb[esrc[e]] (0 where esrc is -1)
 __global__ void k_pipe_gather(int64_t m, const int32_t *__restrict__ esrc,
                               const double *__restrict__ anb, double *__restrict__ eval) {

@@ -132,6 +132,11 @@ struct RefLevel {
   int32_t dpipe_wf = 0, dpipe_wb = 0;
   DBuf<int32_t> dfwd_ecol, dfwd_elen, dfwd_esrc, dbwd_ecol, dbwd_elen, dbwd_esrc;
   DBuf<double> dfwd_eval, dbwd_eval, dilu_y;
+  // and for the DILU factor: the lower lists without the skipped neighbour,
+  // with both value positions (a_ji, a_ij)
+  int32_t fpipe_w = 0;
+  DBuf<int32_t> fac_ecol, fac_elen, fac_sji, fac_sij;
+  DBuf<double> fac_eji, fac_eij;
   bool low_ok = false;
   LowerLists low;
   DBuf<int32_t> lptr, lnb, lpji, lpij;
