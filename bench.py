@@ -599,20 +599,20 @@ def survey_bytes_per_iteration(levels, nlev, pre, post, cpre, cpost):
     return total + b_pass(n0, z0, 8) + 7 * 8 * n0 + 2 * 12 * n0
 
 
-def plain_oracle_check(ptr, col, ac, anb, b, ncyc, dphi_gpu):
-    """The oracle's cfdApplyAMG (V-cycle, SOR 2/2, FP64) on the same system for
-    the same number of cycles from dphi = 0: the GPU's plain drop-in must give
-    its dphi bit for bit, stall included (checker only, untimed)."""
+def plain_oracle_check(ptr, col, ac, anb, b, ncyc, dphi_gpu, smoother):
+    """The oracle's cfdApplyAMG (V-cycle, smoother 2/2, FP64) on the same system
+    for the same number of cycles from dphi = 0: the GPU's plain drop-in must
+    give its dphi bit for bit, stall included (checker only, untimed)."""
     import oracle   # checker
     t = time.time()
     S = oracle.System(ptr, col, ac, anb, bc=b)
     r0, r1, nc = S.apply_amg("V-Cycle", max_cycles=ncyc, termination=1e-8, max_coarse_levels=10,
-                             pre_sweep=2, post_sweep=2, smoother="SOR")
+                             pre_sweep=2, post_sweep=2, smoother=smoother)
     x = S.level(0)["dphi"]
     out = {"cycles": int(nc - 1), "l1_residual_ratio": r1 / r0,
            "dphi_bit_identical_to_gpu": bool(np.array_equal(x, dphi_gpu)),
            "seconds": round(time.time() - t, 1)}
-    log(f"plain fp64 AMG on the oracle: {out}")
+    log(f"plain fp64 AMG ({smoother}) on the oracle: {out}")
     return out
 
 
@@ -862,28 +862,33 @@ def main():
     # not solved, so the constant mode drifts every cycle), so a bounded number
     # of cycles is timed and the residual ratio and field error it reaches are
     # reported beside the per-cycle time.
-    plain = None
+    # Both smoothers: SOR, and ILU (the reference's DILU, its default
+    # smootherType, cfdSetupSolutionMethods.m:6).
+    plain = {}
+    plain_dphi = {}
     if args.plain_cycles > 0:
         zero = np.zeros(m.n_cells)
-        def plain_run(ncyc):
+        def plain_run(ncyc, smoother):
             ctx.set_dphi(sid, 0, zero)
             ctx.synchronize()
             s = time.perf_counter()
             r0, r1, nc = ctx.apply_amg(sid, "V-Cycle", max_cycles=ncyc, termination=1e-8,
                                        max_coarse_levels=10, pre_sweep=2, post_sweep=2,
-                                       smoother="SOR", rebuild=0)
+                                       smoother=smoother, rebuild=0)
             ctx.synchronize()
             return time.perf_counter() - s, nc - 1, r1 / r0
-        plain_run(1)                          # level schedules built on first use
-        tp, ncyc, ratio = plain_run(args.plain_cycles)
-        plain_dphi = ctx.get_dphi(sid, 0)
-        plain = {"ms_per_cycle": round(tp / ncyc * 1e3, 3), "cycles": int(ncyc),
-                 "l1_residual_ratio": ratio,
-                 "field_rel_l2_err": ferr(ctx.get_dphi(sid, 0)),
-                 "reached_1e-8": bool(ratio <= 1e-8),
-                 "method": "cfdApplyAMG V-cycle, SOR 2/2, FP64, bit-exact reference arithmetic"}
-        log(f"plain fp64 AMG: {plain['ms_per_cycle']:.2f} ms/cycle, {ncyc} cycles, L1 ratio "
-            f"{ratio:.2e}, field err {plain['field_rel_l2_err']:.2e}")
+        for sm in ("SOR", "ILU"):
+            plain_run(1, sm)                  # level schedules built on first use
+            tp, ncyc, ratio = plain_run(args.plain_cycles, sm)
+            plain_dphi[sm] = ctx.get_dphi(sid, 0)
+            plain[sm] = {"ms_per_cycle": round(tp / ncyc * 1e3, 3), "cycles": int(ncyc),
+                         "l1_residual_ratio": ratio,
+                         "field_rel_l2_err": ferr(plain_dphi[sm]),
+                         "reached_1e-8": bool(ratio <= 1e-8),
+                         "method": f"cfdApplyAMG V-cycle, {sm} 2/2, FP64, bit-exact reference "
+                                   "arithmetic"}
+            log(f"plain fp64 AMG ({sm}): {plain[sm]['ms_per_cycle']:.2f} ms/cycle, {ncyc} "
+                f"cycles, L1 ratio {ratio:.2e}, field err {plain[sm]['field_rel_l2_err']:.2e}")
 
     r32 = results[32]
     ms0, n0, by0 = r32["prof"]
@@ -973,13 +978,14 @@ def main():
         d32, d64 = shift(r32["x"]), shift(r64["x"])
         out["field_rel_l2_err_vs_fp64_same_tol"] = float(np.linalg.norm(d32 - d64) /
                                                          np.linalg.norm(d64))
-    if plain is not None:
-        # the plain path never reaches 1e-8 on this system; any solve of it
-        # would take at least one cycle, so one cycle's time over the FP32
-        # solve's bounds the solves/s ratio from below
-        plain["speedup_fp32_pcg_lower_bound"] = round(plain["ms_per_cycle"] /
-                                                      out["latency_ms"]["p50"], 2)
-        out["plain_fp64_amg"] = plain
+    for sm, key in (("SOR", "plain_fp64_amg"), ("ILU", "plain_fp64_amg_ilu")):
+        if sm in plain:
+            # the plain path never reaches 1e-8 on this system; any solve of
+            # it would take at least one cycle, so one cycle's time over the
+            # FP32 solve's bounds the solves/s ratio from below
+            plain[sm]["speedup_fp32_pcg_lower_bound"] = round(plain[sm]["ms_per_cycle"] /
+                                                              out["latency_ms"]["p50"], 2)
+            out[key] = plain[sm]
     if args.simple_leg_steps > 0 and args.workload == "pcorr-k5":
         out["simple_precision_form"] = simple_leg(args, ctx, m, args.simple_leg_steps,
                                                   "pcg-frozen", "ILU")
@@ -989,9 +995,9 @@ def main():
                                  args.coarse_pre, args.coarse_post)
         cb["field_rel_l2_err_vs_gpu_tight"] = ferr(x_cpu)
         out["cpu_baseline"] = cb
-        if plain is not None:
-            out["plain_fp64_amg"]["oracle"] = plain_oracle_check(ptr, col, ac, anb, b,
-                                                                plain["cycles"], plain_dphi)
+        for sm in plain:
+            plain[sm]["oracle"] = plain_oracle_check(ptr, col, ac, anb, b, plain[sm]["cycles"],
+                                                     plain_dphi[sm], sm)
     else:
         out["cpu_baseline"] = None
     ctx.close()
