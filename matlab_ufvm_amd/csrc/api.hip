@@ -207,6 +207,21 @@ void ensure_schedule(ufvm_ctx *ctx, RefLevel &L) {
 // LDS a workgroup may allocate
 constexpr size_t kLdsBytes = 65536;
 
+// the device's LDS limit per workgroup (the current device's, read once)
+static size_t max_lds_bytes() {
+  static const size_t v = [] {
+    int dev = 0, m = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&m, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+      m = (int)kLdsBytes;
+    return (size_t)m;
+  }();
+  return v;
+}
+
+// the widest schedule level the one-wave staged sweep takes (two passes)
+constexpr int32_t kStagedWidest = 128;
+
 void sor_sweep(ufvm_ctx *ctx, RefLevel &L, const double *stop = nullptr) {
   // levels of at most kPipeBlock rows whose unknowns do not fit in LDS: the
   // pipelined sweep (for those that fit, the LDS sweep below was as fast)
@@ -225,6 +240,14 @@ void sor_sweep(ufvm_ctx *ctx, RefLevel &L, const double *stop = nullptr) {
                      L.bwd_elen.p, L.bwd_eanb.p, L.bwd_eac.p, L.bc.p, L.dphi.p, stop);
     return;
   }
+  // small levels with narrow schedule levels and short rows: the whole level
+  // in LDS, one wave sweeping (falls through if the launch is refused)
+  if (L.one_wg && L.widest <= kStagedWidest && L.max_row <= 16 && L.n < 65536 &&
+      k::sor_staged_lds(L.n, L.max_row, nlev(L.fwd), nlev(L.bwd)) <= max_lds_bytes() &&
+      k::sor_staged(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.bwd_rows.p,
+                    L.bwd_lp.p, nlev(L.bwd), L.n, L.max_row, L.ptr.p, L.col.p, L.ac.p, L.anb.p,
+                    L.bc.p, L.dphi.p, stop))
+    return;
   if (L.one_wg && L.n <= k::kLdsRows) {
     k::sor_lds(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.bwd_rows.p, L.bwd_lp.p,
                nlev(L.bwd), L.n, L.ptr.p, L.col.p, L.ac.p, L.anb.p, L.bc.p, L.dphi.p, stop);

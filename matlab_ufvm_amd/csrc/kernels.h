@@ -128,6 +128,15 @@ void dilu_pipe(hipStream_t s, bool backward, const int32_t *rows, const int32_t 
 void dilu_factor_pipe(hipStream_t s, const int32_t *rows, const int32_t *lp, int32_t nlev,
                       int32_t n, int32_t w, const int32_t *ecol, const int32_t *elen,
                       const double *eji, const double *eij, const double *ac, double *dinv);
+// The SOR forward and backward sweep of a small level held whole in LDS and
+// run by one wave (see kernels.hip): rows of at most w <= 16 entries, n below
+// 65,536, sor_staged_lds bytes of LDS.  Returns false if the launch failed
+// (then nothing ran).  Same arithmetic.
+size_t sor_staged_lds(int32_t n, int32_t w, int32_t nf, int32_t nbk);
+bool sor_staged(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf,
+                const int32_t *br, const int32_t *blp, int32_t nbk, int32_t n, int32_t w,
+                const int32_t *ptr, const int32_t *col, const double *ac, const double *anb,
+                const double *b, double *x, const double *stop = nullptr);
 // the same with the level's unknowns (x, rc, dinv) in LDS: n <= kLdsRows
 // (48 KB of doubles, inside the 64 KB any workgroup may allocate)
 constexpr int32_t kLdsRows = 6144;

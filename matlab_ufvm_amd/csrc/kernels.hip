@@ -421,6 +421,94 @@ void sor_lds(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf, c
                      nf, br, blp, nbk, n, ptr, col, ac, anb, b, x, stop);
 }
 
+// The forward then backward sweep (SOR) of a small level held whole in LDS:
+// the workgroup copies the unknowns, right-hand side, diagonal, the rows as
+// ELL (W slots, slot u of row i at u * n + i, 16-bit columns, lengths) and
+// both schedules into LDS, then one wave runs every level.  No level needs a
+// workgroup barrier (a wave's LDS accesses complete in order) and no global
+// load is left on the level chain.  Row arithmetic as gs_row.
+template <int W>
+__global__ void __launch_bounds__(kSweepBlock)
+k_sor_staged(const int32_t *__restrict__ fr, const int32_t *__restrict__ flp, int32_t nf,
+             const int32_t *__restrict__ br, const int32_t *__restrict__ blp, int32_t nbk,
+             int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+             const double *__restrict__ ac, const double *__restrict__ anb,
+             const double *__restrict__ b, double *__restrict__ x,
+             const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;  // stopped by the device-side test
+  extern __shared__ double st_lds[];
+  double *xs = st_lds, *bs = xs + n, *as = bs + n, *an = as + n;
+  int32_t *ln = reinterpret_cast<int32_t *>(an + (size_t)W * n);
+  int32_t *frs = ln + n, *brs = frs + n, *flps = brs + n, *blps = flps + nf + 1;
+  uint16_t *cs = reinterpret_cast<uint16_t *>(blps + nbk + 1);
+  for (int32_t i = threadIdx.x; i < n; i += kSweepBlock) {
+    xs[i] = x[i];
+    bs[i] = b[i];
+    as[i] = ac[i];
+    const int32_t k0 = ptr[i], len = ptr[i + 1] - k0;
+    ln[i] = len;
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      const int32_t k = k0 + (u < len ? u : 0);
+      cs[u * n + i] = (uint16_t)(u < len ? col[k] : i);
+      an[u * n + i] = u < len ? anb[k] : 0.0;
+    }
+    frs[i] = fr[i];
+    brs[i] = br[i];
+  }
+  for (int32_t q = threadIdx.x; q <= nf; q += kSweepBlock) flps[q] = flp[q];
+  for (int32_t q = threadIdx.x; q <= nbk; q += kSweepBlock) blps[q] = blp[q];
+  __syncthreads();
+  if (threadIdx.x >= 64) return;
+  const int32_t lane = threadIdx.x;
+  auto sweep = [&](const int32_t *rows, const int32_t *lps, int32_t nlev) {
+    for (int32_t q = 0; q < nlev; ++q) {
+      for (int32_t t = lps[q] + lane; t < lps[q + 1]; t += 64) {
+        const int32_t i = rows[t], len = ln[i];
+        double v = bs[i];
+#pragma unroll
+        for (int u = 0; u < W; ++u) {
+          const double tv = v - an[u * n + i] * xs[cs[u * n + i]];
+          v = u < len ? tv : v;
+        }
+        xs[i] = v / as[i];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  };
+  sweep(frs, flps, nf);
+  sweep(brs, blps, nbk);
+  for (int32_t i = lane; i < n; i += 64) x[i] = xs[i];
+}
+
+size_t sor_staged_lds(int32_t n, int32_t w, int32_t nf, int32_t nbk) {
+  const int32_t ws = w <= 4 ? 4 : w <= 8 ? 8 : 16;
+  return (size_t)n * (3 * sizeof(double) + ws * (sizeof(double) + sizeof(uint16_t)) +
+                      3 * sizeof(int32_t)) +
+         (size_t)(nf + nbk + 2) * sizeof(int32_t);
+}
+
+bool sor_staged(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf,
+                const int32_t *br, const int32_t *blp, int32_t nbk, int32_t n, int32_t w,
+                const int32_t *ptr, const int32_t *col, const double *ac, const double *anb,
+                const double *b, double *x, const double *stop) {
+  if (n <= 0) return true;
+  const size_t lds = sor_staged_lds(n, w, nf, nbk);
+#define UFVM_STAGED(W)                                                                          \
+  hipLaunchKernelGGL(k_sor_staged<W>, dim3(1), dim3(kSweepBlock), lds, s, fr, flp, nf, br, blp, \
+                     nbk, n, ptr, col, ac, anb, b, x, stop)
+  if (w <= 4)
+    UFVM_STAGED(4);
+  else if (w <= 8)
+    UFVM_STAGED(8);
+  else
+    UFVM_STAGED(16);
+#undef UFVM_STAGED
+  return hipGetLastError() == hipSuccess;
+}
+
 // DILU forward and backward substitution (cfdSolveILU's, as dilu_fwd_row /
 // dilu_bwd_row) of a level whose rc fits in LDS; x += z in memory
 __global__ void __launch_bounds__(kSweepBlock)
