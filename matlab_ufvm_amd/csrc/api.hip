@@ -429,6 +429,15 @@ static void solve_alg_device(ufvm_ctx *ctx, RefLevel &L, int smoother, int n_ite
   hipStream_t s = ctx->stream;
   double *st = ctx->alg.p;  // allocated and cleared by apply_amg, the only caller
   if (!st) throw Error(UFVM_ERR_STATE, "device-side smoothing test outside an AMG solve");
+  // SOR on a small level: the whole call in one launch with the level in LDS
+  // (falls through if the launch is refused)
+  if (smoother == UFVM_SMOOTHER_SOR && L.one_wg && L.widest <= kStagedWidest &&
+      L.max_row <= 16 && L.n < 65536 &&
+      k::sor_smooth_staged_lds(L.n, L.max_row, nlev(L.fwd), nlev(L.bwd)) <= max_lds_bytes() &&
+      k::sor_smooth_staged(s, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.bwd_rows.p, L.bwd_lp.p,
+                           nlev(L.bwd), L.n, L.max_row, L.ptr.p, L.col.p, L.ac.p, L.anb.p,
+                           L.bc.p, L.dphi.p, L.r.p, n_iter, rrf, st))
+    return;
   auto residual = [&](const double *stop) -> int32_t {
     if (chunks) {
       ensure_partial(ctx, (int32_t)std::max<int64_t>(L.cs_slots, 1), 1);

@@ -133,6 +133,16 @@ void dilu_factor_pipe(hipStream_t s, const int32_t *rows, const int32_t *lp, int
 // 65,536, sor_staged_lds bytes of LDS.  Returns false if the launch failed
 // (then nothing ran).  Same arithmetic.
 size_t sor_staged_lds(int32_t n, int32_t w, int32_t nf, int32_t nbk);
+// A whole SOR smoothing call with the device-side stopping test (initial
+// residual and test, up to n_iter sweeps each with its residual and test; st
+// and r as the separate kernels leave them) on such a level, in one launch
+// of sor_smooth_staged_lds bytes of LDS.  Returns false if the launch failed.
+size_t sor_smooth_staged_lds(int32_t n, int32_t w, int32_t nf, int32_t nbk);
+bool sor_smooth_staged(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf,
+                       const int32_t *br, const int32_t *blp, int32_t nbk, int32_t n, int32_t w,
+                       const int32_t *ptr, const int32_t *col, const double *ac,
+                       const double *anb, const double *b, double *x, double *r, int32_t n_iter,
+                       double rrf, double *st);
 bool sor_staged(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf,
                 const int32_t *br, const int32_t *blp, int32_t nbk, int32_t n, int32_t w,
                 const int32_t *ptr, const int32_t *col, const double *ac, const double *anb,

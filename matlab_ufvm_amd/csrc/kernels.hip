@@ -427,6 +427,96 @@ void sor_lds(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf, c
 // both schedules into LDS, then one wave runs every level.  No level needs a
 // workgroup barrier (a wave's LDS accesses complete in order) and no global
 // load is left on the level chain.  Row arithmetic as gs_row.
+// doubles of LDS a staged level takes (sor_staged_lds bytes, rounded up)
+__host__ __device__ inline size_t sor_staged_doubles(int32_t n, int32_t w, int32_t nf,
+                                                     int32_t nbk) {
+  const int32_t ws = w <= 4 ? 4 : w <= 8 ? 8 : 16;
+  const size_t bytes = (size_t)n * (3 * sizeof(double) + ws * (sizeof(double) + sizeof(uint16_t)) +
+                                    3 * sizeof(int32_t)) +
+                       (size_t)(nf + nbk + 2) * sizeof(int32_t);
+  return (bytes + sizeof(double) - 1) / sizeof(double);
+}
+
+template <int W>
+struct StagedLevel {
+  double *xs, *bs, *as, *an;
+  int32_t *ln, *frs, *brs, *flps, *blps;
+  uint16_t *cs;
+  int32_t n, nf, nbk;
+  // carve the LDS (sor_staged_lds bytes) and copy the level in (every thread)
+  __device__ void stage(double *lds, const int32_t *__restrict__ fr,
+                        const int32_t *__restrict__ flp, int32_t nf_,
+                        const int32_t *__restrict__ br, const int32_t *__restrict__ blp,
+                        int32_t nbk_, int32_t n_, const int32_t *__restrict__ ptr,
+                        const int32_t *__restrict__ col, const double *__restrict__ ac,
+                        const double *__restrict__ anb, const double *__restrict__ b,
+                        const double *__restrict__ x) {
+    n = n_;
+    nf = nf_;
+    nbk = nbk_;
+    xs = lds;
+    bs = xs + n;
+    as = bs + n;
+    an = as + n;
+    ln = reinterpret_cast<int32_t *>(an + (size_t)W * n);
+    frs = ln + n;
+    brs = frs + n;
+    flps = brs + n;
+    blps = flps + nf + 1;
+    cs = reinterpret_cast<uint16_t *>(blps + nbk + 1);
+    for (int32_t i = threadIdx.x; i < n; i += blockDim.x) {
+      xs[i] = x[i];
+      bs[i] = b[i];
+      as[i] = ac[i];
+      const int32_t k0 = ptr[i], len = ptr[i + 1] - k0;
+      ln[i] = len;
+#pragma unroll
+      for (int u = 0; u < W; ++u) {
+        const int32_t k = k0 + (u < len ? u : 0);
+        cs[u * n + i] = (uint16_t)(u < len ? col[k] : i);
+        an[u * n + i] = u < len ? anb[k] : 0.0;
+      }
+      frs[i] = fr[i];
+      brs[i] = br[i];
+    }
+    for (int32_t q = threadIdx.x; q <= nf; q += blockDim.x) flps[q] = flp[q];
+    for (int32_t q = threadIdx.x; q <= nbk; q += blockDim.x) blps[q] = blp[q];
+  }
+  // forward then backward sweep by the calling wave (lane 0..63)
+  __device__ void sweep(int32_t lane) {
+    auto one = [&](const int32_t *rows, const int32_t *lps, int32_t nlev) {
+      for (int32_t q = 0; q < nlev; ++q) {
+        for (int32_t t = lps[q] + lane; t < lps[q + 1]; t += 64) {
+          const int32_t i = rows[t], len = ln[i];
+          double v = bs[i];
+#pragma unroll
+          for (int u = 0; u < W; ++u) {
+            const double tv = v - an[u * n + i] * xs[cs[u * n + i]];
+            v = u < len ? tv : v;
+          }
+          xs[i] = v / as[i];
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      }
+    };
+    one(frs, flps, nf);
+    one(brs, blps, nbk);
+  }
+  // row i's residual, k_residual_ref's arithmetic
+  __device__ double residual(int32_t i) const {
+    double v = bs[i] - as[i] * xs[i];
+    const int32_t len = ln[i];
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      const double tv = v - an[u * n + i] * xs[cs[u * n + i]];
+      v = u < len ? tv : v;
+    }
+    return v;
+  }
+};
+
 template <int W>
 __global__ void __launch_bounds__(kSweepBlock)
 k_sor_staged(const int32_t *__restrict__ fr, const int32_t *__restrict__ flp, int32_t nf,
@@ -437,50 +527,110 @@ k_sor_staged(const int32_t *__restrict__ fr, const int32_t *__restrict__ flp, in
              const double *__restrict__ stop) {
   if (stop && *stop != 0.0) return;  // stopped by the device-side test
   extern __shared__ double st_lds[];
-  double *xs = st_lds, *bs = xs + n, *as = bs + n, *an = as + n;
-  int32_t *ln = reinterpret_cast<int32_t *>(an + (size_t)W * n);
-  int32_t *frs = ln + n, *brs = frs + n, *flps = brs + n, *blps = flps + nf + 1;
-  uint16_t *cs = reinterpret_cast<uint16_t *>(blps + nbk + 1);
-  for (int32_t i = threadIdx.x; i < n; i += kSweepBlock) {
-    xs[i] = x[i];
-    bs[i] = b[i];
-    as[i] = ac[i];
-    const int32_t k0 = ptr[i], len = ptr[i + 1] - k0;
-    ln[i] = len;
-#pragma unroll
-    for (int u = 0; u < W; ++u) {
-      const int32_t k = k0 + (u < len ? u : 0);
-      cs[u * n + i] = (uint16_t)(u < len ? col[k] : i);
-      an[u * n + i] = u < len ? anb[k] : 0.0;
-    }
-    frs[i] = fr[i];
-    brs[i] = br[i];
-  }
-  for (int32_t q = threadIdx.x; q <= nf; q += kSweepBlock) flps[q] = flp[q];
-  for (int32_t q = threadIdx.x; q <= nbk; q += kSweepBlock) blps[q] = blp[q];
+  StagedLevel<W> L;
+  L.stage(st_lds, fr, flp, nf, br, blp, nbk, n, ptr, col, ac, anb, b, x);
   __syncthreads();
   if (threadIdx.x >= 64) return;
-  const int32_t lane = threadIdx.x;
-  auto sweep = [&](const int32_t *rows, const int32_t *lps, int32_t nlev) {
-    for (int32_t q = 0; q < nlev; ++q) {
-      for (int32_t t = lps[q] + lane; t < lps[q + 1]; t += 64) {
-        const int32_t i = rows[t], len = ln[i];
-        double v = bs[i];
-#pragma unroll
-        for (int u = 0; u < W; ++u) {
-          const double tv = v - an[u * n + i] * xs[cs[u * n + i]];
-          v = u < len ? tv : v;
-        }
-        xs[i] = v / as[i];
+  L.sweep(threadIdx.x);
+  for (int32_t i = threadIdx.x; i < n; i += 64) x[i] = L.xs[i];
+}
+
+// A whole smoothing call with the device-side stopping test (the initial
+// residual and test, then up to n_iter sweeps, each followed by its residual
+// and test) on a staged level, in one launch.  The mean |r| is summed as
+// k_residual_ref's blocks and k_alg_test sum it (per kBlock rows: wave sums
+// added in wave order; then the block sums, strided over kBlock threads, as
+// wave sums added in order), so st[] gets the values those kernels give.
+// r holds the last residual computed; st as k_alg_test leaves it.
+template <int W>
+__global__ void __launch_bounds__(kSweepBlock)
+k_sor_smooth_staged(const int32_t *__restrict__ fr, const int32_t *__restrict__ flp, int32_t nf,
+                    const int32_t *__restrict__ br, const int32_t *__restrict__ blp, int32_t nbk,
+                    int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+                    const double *__restrict__ ac, const double *__restrict__ anb,
+                    const double *__restrict__ b, double *__restrict__ x, double *__restrict__ r,
+                    int32_t n_iter, double rrf, double *st) {
+  extern __shared__ double st_lds[];
+  StagedLevel<W> L;
+  L.stage(st_lds, fr, flp, nf, br, blp, nbk, n, ptr, col, ac, anb, b, x);
+  const int32_t np = (n + kBlock - 1) / kBlock;
+  double *wsum = st_lds + sor_staged_doubles(n, W, nf, nbk);  // 4 per block of kBlock rows
+  double *wv = wsum + 4 * np;                                  // kBlock / kWave
+  __syncthreads();
+  const int32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  constexpr int kWpb = kBlock / 64;          // waves per residual block
+  constexpr int kVb = kSweepBlock / kBlock;  // residual blocks per pass
+  auto mean_abs_residual = [&]() -> double {
+    for (int32_t p0 = 0; p0 < np; p0 += kVb) {
+      const int32_t vb = p0 + wid / kWpb, w = wid % kWpb, i = vb * kBlock + w * 64 + lane;
+      double a = 0.0;
+      if (vb < np && i < n) {
+        const double v = L.residual(i);
+        r[i] = v;
+        a = fabs(v);
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      a = wave_sum(a);
+      if (lane == 0 && vb < np) wsum[vb * kWpb + w] = a;
     }
+    __syncthreads();
+    if (threadIdx.x < kBlock) {
+      double v = 0.0;
+      for (int32_t blk = threadIdx.x; blk < np; blk += kBlock) {
+        double p = 0;
+        for (int w = 0; w < kWpb; ++w) p += wsum[blk * kWpb + w];
+        v += p;
+      }
+      v = wave_sum(v);
+      if (lane == 0) wv[wid] = v;
+    }
+    __syncthreads();
+    double t = 0;
+    for (int w = 0; w < kBlock / 64; ++w) t += wv[w];
+    __syncthreads();
+    return t / n;
   };
-  sweep(frs, flps, nf);
-  sweep(brs, blps, nbk);
-  for (int32_t i = lane; i < n; i += 64) x[i] = xs[i];
+  double m = mean_abs_residual();
+  const double m0 = m;
+  bool stopped = !(m > rrf * m);
+  const bool nan = isnan(m);
+  for (int32_t it = 1; it <= n_iter && !stopped; ++it) {
+    if (threadIdx.x < 64) L.sweep(lane);
+    __syncthreads();
+    m = mean_abs_residual();
+    if (!(m > rrf * m0)) stopped = true;
+  }
+  for (int32_t i = threadIdx.x; i < n; i += kSweepBlock) x[i] = L.xs[i];
+  if (threadIdx.x == 0) {
+    st[0] = m0;
+    st[1] = m;
+    st[2] = stopped ? 1.0 : 0.0;
+    if (nan) st[3] = 1.0;
+  }
+}
+
+size_t sor_smooth_staged_lds(int32_t n, int32_t w, int32_t nf, int32_t nbk) {
+  const int32_t np = (n + kBlock - 1) / kBlock;
+  return (sor_staged_doubles(n, w, nf, nbk) + 4 * (size_t)np + kBlock / 64) * sizeof(double);
+}
+
+bool sor_smooth_staged(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf,
+                       const int32_t *br, const int32_t *blp, int32_t nbk, int32_t n, int32_t w,
+                       const int32_t *ptr, const int32_t *col, const double *ac,
+                       const double *anb, const double *b, double *x, double *r, int32_t n_iter,
+                       double rrf, double *st) {
+  if (n <= 0) return false;
+  const size_t lds = sor_smooth_staged_lds(n, w, nf, nbk);
+#define UFVM_SMOOTH(W)                                                                      \
+  hipLaunchKernelGGL(k_sor_smooth_staged<W>, dim3(1), dim3(kSweepBlock), lds, s, fr, flp, nf, \
+                     br, blp, nbk, n, ptr, col, ac, anb, b, x, r, n_iter, rrf, st)
+  if (w <= 4)
+    UFVM_SMOOTH(4);
+  else if (w <= 8)
+    UFVM_SMOOTH(8);
+  else
+    UFVM_SMOOTH(16);
+#undef UFVM_SMOOTH
+  return hipGetLastError() == hipSuccess;
 }
 
 size_t sor_staged_lds(int32_t n, int32_t w, int32_t nf, int32_t nbk) {
