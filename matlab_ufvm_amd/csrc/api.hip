@@ -123,6 +123,13 @@ void ensure_schedule(ufvm_ctx *ctx, RefLevel &L) {
   L.lnb.upload(L.low.nb, ctx->stream);
   L.lpji.upload(L.low.pos_ji, ctx->stream);
   L.lpij.upload(L.low.pos_ij, ctx->stream);
+  L.max_tri = 0;
+  if (L.low_ok)
+    for (int32_t i = 0; i < L.n; ++i) {
+      int32_t up = 0;
+      for (int32_t k = L.g.ptr[i]; k < L.g.ptr[i + 1]; ++k) up += L.g.col[k] > i;
+      L.max_tri = std::max({L.max_tri, up, L.low.ptr[i + 1] - L.low.ptr[i]});
+    }
   if (widest <= k::kPipeBlock && L.low_ok) {
     // DILU ELL copies: forward over the lower lists in forward-schedule
     // order, backward over the entries with column above the row in
@@ -381,6 +388,14 @@ void dilu_sweep(ufvm_ctx *ctx, RefLevel &L, const double *stop = nullptr) {
   ensure_partial(ctx, L.n, 1);
   k::residual_ref(ctx->stream, L.n, L.ptr.p, L.col.p, L.ac.p, L.anb.p, L.bc.p, L.dphi.p, L.rc.p,
                   ctx->partial.p, stop);
+  // small levels with narrow schedule levels: the whole level in LDS, one
+  // wave substituting (falls through if the launch is refused)
+  if (L.low_ok && L.one_wg && L.widest <= kStagedWidest && L.max_tri <= 16 && L.n < 65536 &&
+      k::dilu_staged_lds(L.n, L.max_tri, nlev(L.fwd), nlev(L.bwd)) <= max_lds_bytes() &&
+      k::dilu_staged(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.bwd_rows.p,
+                     L.bwd_lp.p, nlev(L.bwd), L.n, L.max_tri, L.ptr.p, L.col.p, L.anb.p,
+                     L.lptr.p, L.lnb.p, L.lpji.p, L.dinv.p, L.rc.p, L.dphi.p, stop))
+    return;
   // levels too large for LDS whose schedule levels hold at most kPipeBlock
   // rows: the pipelined substitutions over the ELL copies (values from the
   // current coefficients first)

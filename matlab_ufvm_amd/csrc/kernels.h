@@ -147,6 +147,16 @@ bool sor_staged(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf
                 const int32_t *br, const int32_t *blp, int32_t nbk, int32_t n, int32_t w,
                 const int32_t *ptr, const int32_t *col, const double *ac, const double *anb,
                 const double *b, double *x, const double *stop = nullptr);
+// The DILU substitutions of a small level held whole in LDS and run by one
+// wave (see kernels.hip): lower lists and entries above the diagonal of at
+// most w <= 16 per row, n below 65,536, dilu_staged_lds bytes of LDS.
+// Returns false if the launch failed (then nothing ran).  Same arithmetic.
+size_t dilu_staged_lds(int32_t n, int32_t w, int32_t nf, int32_t nbk);
+bool dilu_staged(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf,
+                 const int32_t *br, const int32_t *blp, int32_t nbk, int32_t n, int32_t w,
+                 const int32_t *ptr, const int32_t *col, const double *anb, const int32_t *lptr,
+                 const int32_t *lnb, const int32_t *lpji, const double *dinv, double *rc,
+                 double *x, const double *stop = nullptr);
 // the same with the level's unknowns (x, rc, dinv) in LDS: n <= kLdsRows
 // (48 KB of doubles, inside the 64 KB any workgroup may allocate)
 constexpr int32_t kLdsRows = 6144;

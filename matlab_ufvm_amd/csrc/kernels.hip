@@ -659,6 +659,131 @@ bool sor_staged(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf
   return hipGetLastError() == hipSuccess;
 }
 
+// The DILU forward and backward substitutions of a small level held whole in
+// LDS and run by one wave, as k_sor_staged: rc, dinv, the lower lists (W
+// slots: 16-bit neighbour, a_ji) and the entries above the diagonal (16-bit
+// column, a_ij, cconn order) are copied in, the substitutions run with no
+// workgroup barrier, and rc = z, x += z are written at the end.  Row
+// arithmetic as dilu_fwd_row / dilu_bwd_row.
+__host__ __device__ inline int dilu_staged_slots(int32_t w) {
+  return w <= 2 ? 2 : w <= 4 ? 4 : w <= 8 ? 8 : 16;
+}
+
+size_t dilu_staged_lds(int32_t n, int32_t w, int32_t nf, int32_t nbk) {
+  const int ws = dilu_staged_slots(w);
+  return (size_t)n * (2 * sizeof(double) + 2 * ws * (sizeof(double) + sizeof(uint16_t)) +
+                      4 * sizeof(int32_t)) +
+         (size_t)(nf + nbk + 2) * sizeof(int32_t);
+}
+
+template <int W>
+__global__ void __launch_bounds__(kSweepBlock)
+k_dilu_staged(const int32_t *__restrict__ fr, const int32_t *__restrict__ flp, int32_t nf,
+              const int32_t *__restrict__ br, const int32_t *__restrict__ blp, int32_t nbk,
+              int32_t n, const int32_t *__restrict__ ptr, const int32_t *__restrict__ col,
+              const double *__restrict__ anb, const int32_t *__restrict__ lptr,
+              const int32_t *__restrict__ lnb, const int32_t *__restrict__ lpji,
+              const double *__restrict__ dinv, double *__restrict__ rc, double *__restrict__ x,
+              const double *__restrict__ stop) {
+  if (stop && *stop != 0.0) return;  // stopped by the device-side test
+  extern __shared__ double dl_lds[];
+  double *rs = dl_lds, *dv = rs + n, *la = dv + n, *ua = la + (size_t)W * n;
+  int32_t *ll = reinterpret_cast<int32_t *>(ua + (size_t)W * n);
+  int32_t *ul = ll + n, *frs = ul + n, *brs = frs + n, *flps = brs + n, *blps = flps + nf + 1;
+  uint16_t *lc = reinterpret_cast<uint16_t *>(blps + nbk + 1), *uc = lc + (size_t)W * n;
+  for (int32_t i = threadIdx.x; i < n; i += kSweepBlock) {
+    rs[i] = rc[i];
+    dv[i] = dinv[i];
+    const int32_t e0 = lptr[i], m = lptr[i + 1] - e0;
+    ll[i] = m;
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      const int32_t e = e0 + (u < m ? u : 0);
+      lc[u * n + i] = (uint16_t)(u < m ? lnb[e] : i);
+      la[u * n + i] = u < m ? anb[lpji[e]] : 0.0;
+    }
+    int32_t c = 0;
+    for (int32_t k = ptr[i]; k < ptr[i + 1]; ++k) {
+      const int32_t j = col[k];
+      if (j > i) {
+        uc[c * n + i] = (uint16_t)j;
+        ua[c * n + i] = anb[k];
+        ++c;
+      }
+    }
+    ul[i] = c;
+    for (int32_t u = c; u < W; ++u) {
+      uc[u * n + i] = (uint16_t)i;
+      ua[u * n + i] = 0.0;
+    }
+    frs[i] = fr[i];
+    brs[i] = br[i];
+  }
+  for (int32_t q = threadIdx.x; q <= nf; q += kSweepBlock) flps[q] = flp[q];
+  for (int32_t q = threadIdx.x; q <= nbk; q += kSweepBlock) blps[q] = blp[q];
+  __syncthreads();
+  if (threadIdx.x >= 64) return;
+  const int32_t lane = threadIdx.x;
+  auto wave_step = [] {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  };
+  for (int32_t q = 0; q < nf; ++q) {
+    for (int32_t t = flps[q] + lane; t < flps[q + 1]; t += 64) {
+      const int32_t j = frs[t], m = ll[j];
+      double v = rs[j];
+#pragma unroll
+      for (int u = 0; u < W; ++u) {
+        const int32_t i = lc[u * n + j];
+        const double mat1 = dv[i] * rs[i];
+        const double tv = v - la[u * n + j] * mat1;
+        v = u < m ? tv : v;
+      }
+      rs[j] = v;
+    }
+    wave_step();
+  }
+  for (int32_t q = 0; q < nbk; ++q) {
+    for (int32_t t = blps[q] + lane; t < blps[q + 1]; t += 64) {
+      const int32_t i = brs[t], m = ul[i];
+      double v = rs[i];
+#pragma unroll
+      for (int u = 0; u < W; ++u) {
+        const double tv = v - ua[u * n + i] * rs[uc[u * n + i]];
+        v = u < m ? tv : v;
+      }
+      rs[i] = dv[i] * v;
+    }
+    wave_step();
+  }
+  for (int32_t i = lane; i < n; i += 64) {
+    const double z = rs[i];
+    rc[i] = z;
+    x[i] = x[i] + z;
+  }
+}
+
+bool dilu_staged(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf,
+                 const int32_t *br, const int32_t *blp, int32_t nbk, int32_t n, int32_t w,
+                 const int32_t *ptr, const int32_t *col, const double *anb, const int32_t *lptr,
+                 const int32_t *lnb, const int32_t *lpji, const double *dinv, double *rc,
+                 double *x, const double *stop) {
+  if (n <= 0) return true;
+  const size_t lds = dilu_staged_lds(n, w, nf, nbk);
+#define UFVM_DSTAGED(W)                                                                          \
+  hipLaunchKernelGGL(k_dilu_staged<W>, dim3(1), dim3(kSweepBlock), lds, s, fr, flp, nf, br, blp, \
+                     nbk, n, ptr, col, anb, lptr, lnb, lpji, dinv, rc, x, stop)
+  switch (dilu_staged_slots(w)) {
+    case 2: UFVM_DSTAGED(2); break;
+    case 4: UFVM_DSTAGED(4); break;
+    case 8: UFVM_DSTAGED(8); break;
+    default: UFVM_DSTAGED(16); break;
+  }
+#undef UFVM_DSTAGED
+  return hipGetLastError() == hipSuccess;
+}
+
 // DILU forward and backward substitution (cfdSolveILU's, as dilu_fwd_row /
 // dilu_bwd_row) of a level whose rc fits in LDS; x += z in memory
 __global__ void __launch_bounds__(kSweepBlock)

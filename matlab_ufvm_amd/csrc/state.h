@@ -119,6 +119,7 @@ struct RefLevel {
   bool one_wg = false;
   int32_t widest = 0;             // rows of the widest level of both schedules
   int32_t max_row = 0;            // the most off-diagonal entries of a row
+  int32_t max_tri = 0;            // the most lower neighbours or upper entries of a row
   // for the pipelined sweep (widest <= kPipeBlock): per direction, the level
   // in schedule order as ELL (pipe_w slots; slot u of position s at
   // u * n + s): columns and row lengths built with the schedule, values and
