@@ -360,6 +360,12 @@ void dilu_factor(ufvm_ctx *ctx, RefLevel &L) {
   if (!L.low_ok) throw Error(UFVM_ERR_NUMERIC, "ILU: the index for i in j is not found");
   const auto &F = L.fwd.level_ptr;
   const int32_t skip = L.n - 2;  // cfdFactorizeILU.m:21 skips 1-based row N-1
+  if (L.one_wg && L.widest <= kStagedWidest && L.max_tri <= 16 && L.n < 65536 &&
+      k::dilu_factor_staged_lds(L.n, L.max_tri, nlev(L.fwd)) <= max_lds_bytes() &&
+      k::dilu_factor_staged(ctx->stream, L.fwd_rows.p, L.fwd_lp.p, nlev(L.fwd), L.n, L.max_tri,
+                            L.ac.p, L.anb.p, L.lptr.p, L.lnb.p, L.lpji.p, L.lpij.p, skip,
+                            L.dinv.p))
+    return;
   if (L.widest <= k::kPipeBlock && !(L.one_wg && L.n <= k::kLdsRows) &&
       k::gs_sweep_pipe_lds(nlev(L.fwd)) <= kLdsBytes) {
     const int64_t m = (int64_t)L.n * L.fpipe_w;

@@ -157,6 +157,14 @@ bool dilu_staged(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t n
                  const int32_t *ptr, const int32_t *col, const double *anb, const int32_t *lptr,
                  const int32_t *lnb, const int32_t *lpji, const double *dinv, double *rc,
                  double *x, const double *stop = nullptr);
+// The DILU factor of such a level in LDS, one wave (dilu_factor_row's
+// arithmetic; dilu_factor_staged_lds bytes).  Returns false if the launch
+// failed (then nothing ran).
+size_t dilu_factor_staged_lds(int32_t n, int32_t w, int32_t nf);
+bool dilu_factor_staged(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf,
+                        int32_t n, int32_t w, const double *ac, const double *anb,
+                        const int32_t *lptr, const int32_t *lnb, const int32_t *lpji,
+                        const int32_t *lpij, int32_t skip_row, double *dinv);
 // the same with the level's unknowns (x, rc, dinv) in LDS: n <= kLdsRows
 // (48 KB of doubles, inside the 64 KB any workgroup may allocate)
 constexpr int32_t kLdsRows = 6144;

@@ -764,6 +764,90 @@ k_dilu_staged(const int32_t *__restrict__ fr, const int32_t *__restrict__ flp, i
   }
 }
 
+// The DILU factor of a small level held whole in LDS and run by one wave:
+// the diagonal and each row's lower list without the skipped neighbour (W
+// slots: 16-bit neighbour, a_ji, a_ij) are copied in, dinv is built level by
+// level in LDS and written at the end.  Row arithmetic as dilu_factor_row.
+size_t dilu_factor_staged_lds(int32_t n, int32_t w, int32_t nf) {
+  const int ws = dilu_staged_slots(w);
+  return (size_t)n * (2 * sizeof(double) + ws * (2 * sizeof(double) + sizeof(uint16_t)) +
+                      2 * sizeof(int32_t)) +
+         (size_t)(nf + 1) * sizeof(int32_t);
+}
+
+template <int W>
+__global__ void __launch_bounds__(kSweepBlock)
+k_dilu_factor_staged(const int32_t *__restrict__ fr, const int32_t *__restrict__ flp, int32_t nf,
+                     int32_t n, const double *__restrict__ ac, const double *__restrict__ anb,
+                     const int32_t *__restrict__ lptr, const int32_t *__restrict__ lnb,
+                     const int32_t *__restrict__ lpji, const int32_t *__restrict__ lpij,
+                     int32_t skip_row, double *__restrict__ dinv) {
+  extern __shared__ double df_lds[];
+  double *dv = df_lds, *d0 = dv + n, *aji = d0 + n, *aij = aji + (size_t)W * n;
+  int32_t *ll = reinterpret_cast<int32_t *>(aij + (size_t)W * n);
+  int32_t *frs = ll + n, *flps = frs + n;
+  uint16_t *lc = reinterpret_cast<uint16_t *>(flps + nf + 1);
+  for (int32_t j = threadIdx.x; j < n; j += kSweepBlock) {
+    d0[j] = ac[j];
+    int32_t c = 0;
+    for (int32_t e = lptr[j]; e < lptr[j + 1]; ++e) {
+      const int32_t i = lnb[e];
+      if (i != skip_row) {
+        lc[c * n + j] = (uint16_t)i;
+        aji[c * n + j] = anb[lpji[e]];
+        aij[c * n + j] = anb[lpij[e]];
+        ++c;
+      }
+    }
+    ll[j] = c;
+    for (int32_t u = c; u < W; ++u) {
+      lc[u * n + j] = (uint16_t)j;
+      aji[u * n + j] = 0.0;
+      aij[u * n + j] = 0.0;
+    }
+    frs[j] = fr[j];
+  }
+  for (int32_t q = threadIdx.x; q <= nf; q += kSweepBlock) flps[q] = flp[q];
+  __syncthreads();
+  if (threadIdx.x >= 64) return;
+  const int32_t lane = threadIdx.x;
+  for (int32_t q = 0; q < nf; ++q) {
+    for (int32_t t = flps[q] + lane; t < flps[q + 1]; t += 64) {
+      const int32_t j = frs[t], m = ll[j];
+      double d = d0[j];
+#pragma unroll
+      for (int u = 0; u < W; ++u) {
+        const double tv = d - (aji[u * n + j] * dv[lc[u * n + j]]) * aij[u * n + j];
+        d = u < m ? tv : d;
+      }
+      dv[j] = 1.0 / d;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  for (int32_t j = lane; j < n; j += 64) dinv[j] = dv[j];
+}
+
+bool dilu_factor_staged(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf,
+                        int32_t n, int32_t w, const double *ac, const double *anb,
+                        const int32_t *lptr, const int32_t *lnb, const int32_t *lpji,
+                        const int32_t *lpij, int32_t skip_row, double *dinv) {
+  if (n <= 0) return true;
+  const size_t lds = dilu_factor_staged_lds(n, w, nf);
+#define UFVM_FSTAGED(W)                                                                           \
+  hipLaunchKernelGGL(k_dilu_factor_staged<W>, dim3(1), dim3(kSweepBlock), lds, s, fr, flp, nf, n, \
+                     ac, anb, lptr, lnb, lpji, lpij, skip_row, dinv)
+  switch (dilu_staged_slots(w)) {
+    case 2: UFVM_FSTAGED(2); break;
+    case 4: UFVM_FSTAGED(4); break;
+    case 8: UFVM_FSTAGED(8); break;
+    default: UFVM_FSTAGED(16); break;
+  }
+#undef UFVM_FSTAGED
+  return hipGetLastError() == hipSuccess;
+}
+
 bool dilu_staged(hipStream_t s, const int32_t *fr, const int32_t *flp, int32_t nf,
                  const int32_t *br, const int32_t *blp, int32_t nbk, int32_t n, int32_t w,
                  const int32_t *ptr, const int32_t *col, const double *anb, const int32_t *lptr,
